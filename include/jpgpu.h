@@ -1,0 +1,159 @@
+/*
+ * jpgpu.h - C ABI of the MI355X (gfx950) implementation of j_presto md_run's
+ * zero-dipole (ZD) + Lennard-Jones nonbonded path and the SHAKE / leap-frog
+ * step that consumes its forces.
+ *
+ * Conventions (all entry points):
+ *   - Arrays are host pointers in md_run's column-major layout: cord(3,N) is
+ *     C double[N][3] (xyz interleaved), grad(3,N,nfrg) is double[nfrg][N][3].
+ *   - Atom indices in descriptors and outputs are 1-based (Fortran), converted
+ *     at the boundary.
+ *   - Return value: md_run's `ier` code for the routine the entry point
+ *     replaces (0 ok; 28 cell overflow, 51 list overflow, 99 excluded pair not
+ *     in the list, 1 SHAKE not converged, -2 SHAKE singular).  Values <= -100
+ *     are runtime errors (-100 - hipError_t, or JP_EINVAL / JP_ENOMEM).
+ *   - A context owns every device buffer and one HIP stream; the caller owns
+ *     every host buffer.  One context per host thread; calls on one context
+ *     are not re-entrant.
+ *   - Units are md_run's: Angstrom, fs, amu, kcal/mol.
+ */
+#ifndef JPGPU_H
+#define JPGPU_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define JP_OK 0
+#define JP_IER_CELL 28        /* celpar.f90:183-204 */
+#define JP_IER_LIST 51        /* cutoff.f90:30-46 */
+#define JP_IER_EXCL 99        /* cutoff.f90:90-96 */
+#define JP_IER_SHAKE_CONV 1   /* shake.f90:286-319 */
+#define JP_IER_SHAKE_SING (-2)/* shake.f90:266-284 */
+#define JP_EINVAL (-101)
+#define JP_ENOMEM (-102)
+#define JP_ENODEV (-103)
+#define JP_EHIP_BASE (-1000)  /* -1000 - hipError_t */
+
+#define JP_PREC_FP64 0        /* plain FP64 pair arithmetic, FP64 accumulation */
+#define JP_PREC_FP32_FIXED 1  /* FP32 pair arithmetic, 64-bit fixed-point accumulation */
+
+/*
+ * System description: the module state md_run builds in its INPUT phase and
+ * in incel (inptpl.f90 types/LJ, inpcls.f90 chgmod/itab14/n_matrix,
+ * setshk.f90 SHAKE groups, inpmd.f90 box and keywords, incel.f90 ZD
+ * constants and the excess-pair list).
+ */
+typedef struct jp_sys_desc {
+    int32_t natom;             /* ixnatm */
+    int32_t ntyp;              /* iyntyp: number of LJ atom types */
+    int32_t nfrag;             /* cluster classes (max icls) */
+    int32_t nfrg;              /* energy classes (max n_matrix) */
+    const double *x;           /* cord(3,N) initial coordinates */
+    const double *mass;        /* fxmass(N) */
+    const double *charge;      /* fxchrg(N), elementary charges */
+    const int32_t *atype;      /* ixatyp(N), 1-based */
+    const double *rmin2;       /* fyvwrd(ntyp): AMBER Rmin/2 */
+    const double *eps;         /* fyvwme(ntyp) */
+    const int32_t *icls;       /* icls(N), 1-based cluster class */
+    const int32_t *n_matrix;   /* n_matrix(nfrag,nfrag), column-major, 1-based classes */
+    int32_t nexcl;             /* unique excluded pairs (itab14 / 2) */
+    const int32_t *excl;       /* [nexcl][2] 1-based */
+    int32_t nexcess;           /* ZD excess pairs: bonds, angle 1-3, 1-4 (iytnbf=1) */
+    const int32_t *excess;     /* [nexcess][2] 1-based, TPL order (incel.f90:665-688) */
+    int32_t nshake;            /* SHAKE groups (iugshk) */
+    const int32_t *shake_natom;/* [nshake] 2 or 3 atoms */
+    const int32_t *shake_atoms;/* [nshake][4] 1-based, unused slots 0 */
+    const double *shake_dist2; /* [nshake][6] fudshk: squared targets, pair order 1-2,2-3,1-3,... */
+    double shake_tol;          /* fustol (COVSHK), md_run default 1e-6 */
+    int32_t shake_maxit;       /* iuslop (LIMSHK), md_run default 1000 */
+    int32_t cell_level;        /* CELLVL (nlev); 0 -> md_run default 3 */
+    double celwal[6];          /* XMINCL XMAXCL YMINCL YMAXCL ZMINCL ZMAXCL */
+    double cutoff;             /* CUTLEN (fycutl) */
+    double diel;               /* DIEVAL (fydiel) */
+    double dt;                 /* TIMEST (fs) */
+    int32_t lmd;               /* 1: CLSTMETHOD=LMD, grad = l^2 g1 + l g2 + g3 (verlet.f90:67-91) */
+    double lambda;             /* LAMBDA */
+} jp_sys_desc;
+
+/* Per-monitoring-step record of jp_run_nve (monmd.f90 / md.f90:618-628). */
+typedef struct jp_energy_log {
+    int32_t step;              /* idloop */
+    double etot;               /* K + Epot (Epot lambda-scaled under LMD) */
+    double epot;               /* potential energy */
+    double ekin;               /* K from (v(t-dt/2)+v(t+dt/2))/2 (caltmp.f90) */
+    double eelec[3];           /* ELE15 per energy class (enevcc(9,:)) */
+    double evdw[3];            /* VDW15 per energy class (enevcc(8,:)) */
+} jp_energy_log;
+
+typedef struct jp_ctx jp_ctx;
+
+/* incel + inpcls + setshk: build a context from a descriptor; uploads the
+ * topology and the initial coordinates.  (incel.f90:2-689) */
+int jp_ctx_create(const jp_sys_desc *desc, jp_ctx **out);
+int jp_ctx_destroy(jp_ctx *ctx);
+
+/* Host <-> device coordinates and velocities (cord(3,N), vel(3,N)). Either
+ * pointer may be NULL. */
+int jp_set_state(jp_ctx *ctx, const double *cord, const double *vel);
+int jp_get_state(jp_ctx *ctx, double *cord, double *vel);
+
+/* celprp: bin atoms into cells and build the pair list at the given
+ * coordinates (NULL: the context's device coordinates).  (celprp.f90:2-175,
+ * celpar.f90:2-244, cutoff.f90:2-505) */
+int jp_build_list(jp_ctx *ctx, const double *cord);
+
+/* Parity export of the last list: reference cell (mx,my,mz) of every atom
+ * (celpar.f90:117-122 numbering), and the full (DOUB) neighbour list as CSR
+ * over original atom order with ascending 1-based partner indices.  Pass
+ * idx=NULL to query *npairs (sum of list lengths) only. */
+int jp_get_list(jp_ctx *ctx, int32_t *cell_of_atom, int64_t *npairs, int32_t *ptr, int32_t *idx,
+                int64_t cap);
+
+/* cmmdo -> enear_ZD_*_PB plus the ZD self and excess terms: accumulates the
+ * nonbonded gradient into grad(3,N,nfrg) and writes the per-class energies
+ * (cmmdo.f90:2-78, enear.f90:1-593).  cord NULL: use device coordinates;
+ * grad NULL: keep the gradient on the device. */
+int jp_nonbond(jp_ctx *ctx, const double *cord, double *grad, double *eelec, double *evdw,
+               double *ehyd, int precision);
+
+/* shake_PB: constrain cord (x(t+dt)) against cord_old (x(t)) in place.
+ * (shake.f90:2-755) */
+int jp_shake(jp_ctx *ctx, double *cord, const double *cord_old);
+
+/* runmd/verlet for METHOD=MICR: nsteps leap-frog steps on the device state
+ * (forces, kick, drift, SHAKE, v=(x-x_old)/dt), list rebuilt every `update`
+ * steps, energies recorded every `nmon` steps into log[nsteps/nmon].
+ * (md.f90:519-694, verlet.f90:2-226) */
+int jp_run_nve(jp_ctx *ctx, int32_t nsteps, int32_t update, int32_t nmon, int32_t precision,
+               jp_energy_log *log);
+
+/* Seconds spent in the last jp_run_nve, measured with HIP events on the
+ * context stream: total, pair kernel, list build, SHAKE + integration. */
+int jp_last_timing(jp_ctx *ctx, double *t_total, double *t_pair, double *t_list, double *t_step);
+
+/* Number of unique pairs with r <= cutoff at the device coordinates
+ * (pair-interactions/s numerator). */
+int jp_count_pairs(jp_ctx *ctx, int64_t *npairs);
+
+/*
+ * Batched ensemble (replaces j_presto_mpi / j_presto_master: N independent
+ * md_run jobs, j_presto_mpi:27-44): every system of the batch shares each
+ * kernel launch.
+ */
+typedef struct jp_batch jp_batch;
+int jp_batch_create(const jp_sys_desc *descs, int32_t nsys, jp_batch **out);
+int jp_batch_run_nve(jp_batch *b, int32_t nsteps, int32_t update, int32_t nmon, int32_t precision,
+                     jp_energy_log *log /* [nsys][nsteps/nmon] */);
+int jp_batch_get_state(jp_batch *b, int32_t isys, double *cord, double *vel);
+int jp_batch_last_timing(jp_batch *b, double *t_total);
+int jp_batch_destroy(jp_batch *b);
+
+/* Library version and device name (for logs). */
+const char *jp_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

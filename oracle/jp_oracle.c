@@ -1,0 +1,911 @@
+/*
+ * jp_oracle.c - CPU FP64 restatement of j_presto md_run's ZD nonbonded path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker for the GPU library:
+ * only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load
+ * it.  The product (j_presto_amd/, libjpgpu.so) never calls it.
+ *
+ * It restates, in plain C with IEEE FP64 and no FMA contraction (build with
+ * -ffp-contract=off), the reference algorithm in the order md_run runs it with
+ * PARALV= DOUB, N_SIMD= 1 and one OpenMP thread, so that forces match the
+ * compiled reference bit for bit:
+ *   setup        incel.f90:29-57, 326-508 (cell grid, periodic neighbour-cell
+ *                table with mxdelay padding), 627-689 (ZD constants, dself,
+ *                excess pairs); inpcls.f90:213-223 (chgmod);
+ *                inptpl.f90:397-419 (AMBER C6/C12); input.f90:58-69 (zero_vdW)
+ *   celpar_PB    celpar.f90:93-128 (wrap + bin)
+ *   celprp       celprp.f90:47-104 (EL / EL+vdW split, inum block order)
+ *   cutatm_PB    cutoff.f90:334-497 (full "DOUB" list)
+ *   cutoff_PB    cutoff.f90:50-96 (1-4 exclusion swap-delete, ier=99)
+ *   enear_ZD_doub_PB  enear.f90:60-326 (i_vec = 1), 518-571 (excess, self)
+ *   shake        shake.f90:112-327 (3-atom Newton), 687-755 (2-atom exact)
+ *   verlet       verlet.f90:144-215 (MICR leap-frog)
+ *   caltmp       caltmp.f90:18-27
+ *
+ * Parity pins (tests/): the compiled reference md_run (golden vectors under
+ * tests/golden/), brute-force O(N^2) pair sets, closed forms of the ZD and LJ
+ * pair terms, finite differences and SHAKE residuals.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../include/jpgpu.h"
+
+#define EEKCAL 332.06378 /* PHYCNS.f90:30 */
+
+static double anint(double x) { return round(x); } /* Fortran ANINT: half away from zero */
+
+typedef struct {
+    int n, ntyp, nfrag, nfrg;
+    double *chgmod, *mass, *imass, *charge;
+    int *atype, *icls; /* 0-based */
+    unsigned char *zero_vdw;
+    double *c6, *c12; /* [ntyp][ntyp] */
+    int *nmat;        /* [nfrag][nfrag] (a,b) -> class 0-based */
+    /* itab14: symmetric sorted CSR, 0-based */
+    int *x14p, *x14i, nntab14;
+    int nexcess, *excess; /* [nexcess][2] 0-based, TPL order */
+    int *exc_cls;
+    double *dself; /* [nfrg] */
+    /* box and grid */
+    double fxcell[3], fxcbou[3], invcel[3], celwal[6];
+    double rc, irlim2, rlim, rlim2, siz, rr, org[3];
+    double zcore, bcoeff, fcoeff, ccoef;
+    int nlev, nv, idelay;
+    int m1[3], m2[3], mdel[3];
+    int ipmax;
+    /* per-box-cell neighbour table: cells in linear grid numbering */
+    int nbox;
+    int *nnr, **inr; /* indexed by linear grid cell */
+    /* SHAKE */
+    int nshk, *shk_n, *shk_a;
+    double *shk_d2, shk_tol;
+    int shk_maxit;
+    /* lists (built by jo_celprp) */
+    int *cell3;   /* [n][3] reference (mx,my,mz) */
+    int *inum;    /* orig -> sorted position */
+    int *order;   /* sorted position -> orig */
+    /* per (atom, class): EL list then vdW list, stored as CSR over n*nfrg */
+    int64_t *lptr; /* [n*nfrg+1] */
+    int *nel, *nvd; /* counts after exclusion removal */
+    int *elcap;     /* EL part size before removal: vdW part starts there */
+    int *lidx;
+    int have_list;
+    double dt;
+    int lmd;
+    double lambda;
+} jo_ctx;
+
+/* ------------------------------------------------------------------ setup */
+
+static void grid_setup(jo_ctx *c, const double *x) {
+    for (int k = 0; k < 3; ++k) {
+        c->fxcell[k] = c->celwal[2 * k + 1] - c->celwal[2 * k];      /* inpmd.f90:578-580 */
+        c->fxcbou[k] = (c->celwal[2 * k] + c->celwal[2 * k + 1]) * 0.5; /* :581-583 */
+        c->invcel[k] = 1.0 / c->fxcell[k];                            /* :764-768 */
+    }
+    c->rlim = c->rc + 1.5; /* incel.f90:31 */
+    c->irlim2 = 1.0 / (c->rc * c->rc);
+    double cminsiz = c->rlim * 0.5;
+    c->rlim2 = c->rlim * c->rlim;
+    /* incel.f90:42-52: raise nlev if the coordinate extent needs it */
+    double ext = 0.0;
+    for (int k = 0; k < 3; ++k) {
+        double lo = x[k], hi = x[k];
+        for (int i = 1; i < c->n; ++i) {
+            double v = x[3 * i + k];
+            if (v < lo) lo = v;
+            if (v > hi) hi = v;
+        }
+        if (hi - lo > ext) ext = hi - lo;
+    }
+    if ((double)(1 << (c->nlev + 1)) * cminsiz < ext)
+        c->nlev = (int)ceil(log(ext / cminsiz) / log(2.0) - 1.0);
+    c->nv = 4 << (c->nlev - 1);             /* incel.f90:55 */
+    c->siz = cminsiz;                       /* :56, level 1 */
+    c->idelay = (int)ceil(c->rlim / c->siz); /* :57 */
+    c->rr = 1.0 / c->siz;                   /* celpar.f90:108 */
+    double r = -0.5 * (c->nv + 1);          /* mk_c, incel.f90:1258-1259 */
+    double zz = c->siz * 0.5;               /* celpar.f90:93-96 */
+    for (int k = 0; k < 3; ++k) {
+        double c11 = c->siz * (1.0 + r) + c->fxcbou[k];
+        c->org[k] = c11 - zz;
+    }
+    /* ZD constants, incel.f90:655-658 */
+    double rt = 1.0 / c->rc;
+    c->zcore = 1.5 * rt;
+    c->fcoeff = rt * rt * rt;
+    c->bcoeff = 0.5 * c->fcoeff;
+    /* box cell range and edge padding, incel.f90:334-358 */
+    int ic = (int)anint(c->nv * 0.5);
+    for (int k = 0; k < 3; ++k) {
+        double h = 0.5 * c->fxcell[k] / c->siz;
+        c->m1[k] = (int)ceil((double)ic - h);
+        c->m2[k] = (int)ceil((double)ic + h);
+        c->mdel[k] = (ceil(h) - h > 0.5) ? 1 : 2;
+    }
+    c->ipmax = (int)ceil((c->siz * c->siz * c->siz) * 0.27) * 2; /* incel.f90:146, DOUB */
+}
+
+/* Periodic neighbour-cell table for the full list (incel.f90:456-508). */
+static int nbr_setup(jo_ctx *c) {
+    int nv = c->nv, m = c->idelay;
+    int iix = c->m2[0] - c->m1[0] + 1, iiy = c->m2[1] - c->m1[1] + 1, iiz = c->m2[2] - c->m1[2] + 1;
+    float rix = 1.0f / (float)iix, riy = 1.0f / (float)iiy, riz = 1.0f / (float)iiz; /* real(4) */
+    int ncell = nv * nv * nv;
+    c->nnr = (int *)calloc(ncell, sizeof(int));
+    c->inr = (int **)calloc(ncell, sizeof(int *));
+    int nb = iix * iiy * iiz;
+    unsigned char *mark = (unsigned char *)calloc(nb, 1);
+    for (int iz = c->m1[2]; iz <= c->m2[2]; ++iz)
+        for (int iy = c->m1[1]; iy <= c->m2[1]; ++iy)
+            for (int ix = c->m1[0]; ix <= c->m2[0]; ++ix) {
+                int jz1 = iz - m, jz2 = iz + m, jy1 = iy - m, jy2 = iy + m, jx1 = ix - m, jx2 = ix + m;
+                if (jz1 <= c->m1[2]) jz1 -= c->mdel[2];
+                if (jz2 >= c->m2[2]) jz2 += c->mdel[2];
+                if (jy1 <= c->m1[1]) jy1 -= c->mdel[1];
+                if (jy2 >= c->m2[1]) jy2 += c->mdel[1];
+                if (jx1 <= c->m1[0]) jx1 -= c->mdel[0];
+                if (jx2 >= c->m2[0]) jx2 += c->mdel[0];
+                memset(mark, 0, nb);
+                for (int i = jz1; i <= jz2; ++i) {
+                    int ii = i - iiz * (int)floorf((float)(i - c->m1[2]) * riz);
+                    for (int j = jy1; j <= jy2; ++j) {
+                        int jj = j - iiy * (int)floorf((float)(j - c->m1[1]) * riy);
+                        for (int k = jx1; k <= jx2; ++k) {
+                            int kk = k - iix * (int)floorf((float)(k - c->m1[0]) * rix);
+                            if (kk < c->m1[0] || kk > c->m2[0] || jj < c->m1[1] || jj > c->m2[1] ||
+                                ii < c->m1[2] || ii > c->m2[2]) {
+                                free(mark);
+                                return -1; /* the reference writes out of bounds here */
+                            }
+                            mark[((ii - c->m1[2]) * iiy + (jj - c->m1[1])) * iix + (kk - c->m1[0])] = 1;
+                        }
+                    }
+                }
+                mark[((iz - c->m1[2]) * iiy + (iy - c->m1[1])) * iix + (ix - c->m1[0])] = 0;
+                int cnt = 0;
+                for (int q = 0; q < nb; ++q) cnt += mark[q];
+                int lin = ((iz - 1) * nv + (iy - 1)) * nv + (ix - 1);
+                c->nnr[lin] = cnt;
+                c->inr[lin] = (int *)malloc(sizeof(int) * (cnt ? cnt : 1));
+                int w = 0;
+                for (int i = c->m1[2]; i <= c->m2[2]; ++i) /* (k,j,i) order of incel.f90:494-503 */
+                    for (int j = c->m1[1]; j <= c->m2[1]; ++j)
+                        for (int k = c->m1[0]; k <= c->m2[0]; ++k)
+                            if (mark[((i - c->m1[2]) * iiy + (j - c->m1[1])) * iix + (k - c->m1[0])])
+                                c->inr[lin][w++] = ((i - 1) * nv + (j - 1)) * nv + (k - 1);
+            }
+    free(mark);
+    return 0;
+}
+
+jo_ctx *jo_create(const jp_sys_desc *d) {
+    jo_ctx *c = (jo_ctx *)calloc(1, sizeof(jo_ctx));
+    int n = d->natom, nt = d->ntyp;
+    c->n = n;
+    c->ntyp = nt;
+    c->nfrag = d->nfrag;
+    c->nfrg = d->nfrg;
+    c->rc = d->cutoff;
+    c->dt = d->dt;
+    c->lmd = d->lmd;
+    c->lambda = d->lambda;
+    c->nlev = d->cell_level > 0 ? d->cell_level : 3;
+    memcpy(c->celwal, d->celwal, sizeof(c->celwal));
+    c->chgmod = (double *)malloc(sizeof(double) * n);
+    c->charge = (double *)malloc(sizeof(double) * n);
+    c->mass = (double *)malloc(sizeof(double) * n);
+    c->imass = (double *)malloc(sizeof(double) * n);
+    c->atype = (int *)malloc(sizeof(int) * n);
+    c->icls = (int *)malloc(sizeof(int) * n);
+    c->zero_vdw = (unsigned char *)malloc(n);
+    /* inpcls.f90:215-223 */
+    c->ccoef = EEKCAL / d->diel;
+    double fac = sqrt(c->ccoef);
+    for (int i = 0; i < n; ++i) {
+        c->charge[i] = d->charge[i];
+        c->chgmod[i] = fac * d->charge[i];
+        c->mass[i] = d->mass[i];
+        c->imass[i] = 1.0 / d->mass[i]; /* md.f90:64-73 */
+        c->atype[i] = d->atype[i] - 1;
+        c->icls[i] = d->icls[i] - 1;
+    }
+    /* AMBER combination, inptpl.f90:397-419 */
+    c->c6 = (double *)calloc((size_t)nt * nt, sizeof(double));
+    c->c12 = (double *)calloc((size_t)nt * nt, sizeof(double));
+    for (int i = 0; i < nt; ++i)
+        for (int j = i; j < nt; ++j) {
+            double rtmp = sqrt(d->eps[i] * d->eps[j]);
+            double r2 = d->rmin2[i] + d->rmin2[j];
+            r2 = r2 * r2;
+            r2 = r2 * r2 * r2;
+            double a = rtmp * r2 * 2.0, b = rtmp * r2 * r2;
+            c->c6[i * nt + j] = c->c6[j * nt + i] = a;
+            c->c12[i * nt + j] = c->c12[j * nt + i] = b;
+        }
+    /* input.f90:58-69 */
+    for (int i = 0; i < n; ++i) {
+        int t = c->atype[i], z = 1;
+        for (int j = 0; j < nt; ++j)
+            if (fabs(c->c6[j * nt + t]) >= 1e-16 || fabs(c->c12[j * nt + t]) >= 1e-16) z = 0;
+        c->zero_vdw[i] = (unsigned char)z;
+    }
+    c->nmat = (int *)malloc(sizeof(int) * d->nfrag * d->nfrag);
+    for (int a = 0; a < d->nfrag; ++a)
+        for (int b = 0; b < d->nfrag; ++b) c->nmat[a * d->nfrag + b] = d->n_matrix[b * d->nfrag + a] - 1;
+    /* itab14: symmetric, ascending (inpcls.f90:262-295) */
+    c->x14p = (int *)calloc(n + 1, sizeof(int));
+    for (int e = 0; e < d->nexcl; ++e) {
+        c->x14p[d->excl[2 * e] - 1 + 1]++;
+        c->x14p[d->excl[2 * e + 1] - 1 + 1]++;
+    }
+    for (int i = 0; i < n; ++i) c->x14p[i + 1] += c->x14p[i];
+    c->x14i = (int *)malloc(sizeof(int) * (c->x14p[n] ? c->x14p[n] : 1));
+    int *fill = (int *)calloc(n, sizeof(int));
+    for (int e = 0; e < d->nexcl; ++e) {
+        int a = d->excl[2 * e] - 1, b = d->excl[2 * e + 1] - 1;
+        c->x14i[c->x14p[a] + fill[a]++] = b;
+        c->x14i[c->x14p[b] + fill[b]++] = a;
+    }
+    free(fill);
+    for (int i = 0; i < n; ++i) { /* insertion sort, inpcls.f90:279-294 */
+        int *v = c->x14i + c->x14p[i], m = c->x14p[i + 1] - c->x14p[i];
+        for (int j = 1; j < m; ++j) {
+            int t = v[j], k = j - 1;
+            while (k >= 0 && v[k] > t) { v[k + 1] = v[k]; --k; }
+            v[k + 1] = t;
+        }
+    }
+    c->nntab14 = d->nexcl;
+    /* excess pairs and dself, incel.f90:629-688 */
+    c->nexcess = d->nexcess;
+    c->excess = (int *)malloc(sizeof(int) * 2 * (d->nexcess ? d->nexcess : 1));
+    c->exc_cls = (int *)malloc(sizeof(int) * (d->nexcess ? d->nexcess : 1));
+    double *dwork = (double *)calloc((size_t)d->nfrg * n, sizeof(double));
+    for (int i = 0; i < n; ++i) {
+        int j = c->nmat[c->icls[i] * c->nfrag + c->icls[i]];
+        dwork[(size_t)i * d->nfrg + j] += d->charge[i];
+    }
+    for (int e = 0; e < d->nexcess; ++e) {
+        int a = d->excess[2 * e] - 1, b = d->excess[2 * e + 1] - 1;
+        c->excess[2 * e] = a;
+        c->excess[2 * e + 1] = b;
+        int j = c->nmat[c->icls[a] * c->nfrag + c->icls[b]];
+        c->exc_cls[e] = j;
+        dwork[(size_t)a * d->nfrg + j] += d->charge[b];
+        dwork[(size_t)b * d->nfrg + j] += d->charge[a];
+    }
+    c->dself = (double *)malloc(sizeof(double) * d->nfrg);
+    grid_setup(c, d->x);
+    for (int k = 0; k < d->nfrg; ++k) {
+        double s = 0.0;
+        for (int i = 0; i < n; ++i) s = s + d->charge[i] * dwork[(size_t)i * d->nfrg + k];
+        c->dself[k] = -c->zcore * s * 0.5 * c->ccoef;
+    }
+    free(dwork);
+    nbr_setup(c);
+    /* SHAKE groups (setshk.f90 modshk order: 2-, then 3-atom groups) */
+    c->nshk = d->nshake;
+    c->shk_n = (int *)malloc(sizeof(int) * (d->nshake ? d->nshake : 1));
+    c->shk_a = (int *)malloc(sizeof(int) * 4 * (d->nshake ? d->nshake : 1));
+    c->shk_d2 = (double *)malloc(sizeof(double) * 6 * (d->nshake ? d->nshake : 1));
+    for (int g = 0; g < d->nshake; ++g) {
+        c->shk_n[g] = d->shake_natom[g];
+        for (int k = 0; k < 4; ++k) c->shk_a[4 * g + k] = d->shake_atoms[4 * g + k] - 1;
+        for (int k = 0; k < 6; ++k) c->shk_d2[6 * g + k] = d->shake_dist2[6 * g + k];
+    }
+    c->shk_tol = d->shake_tol > 0 ? d->shake_tol : 1e-6;
+    c->shk_maxit = d->shake_maxit > 0 ? d->shake_maxit : 1000;
+    c->cell3 = (int *)malloc(sizeof(int) * 3 * n);
+    c->inum = (int *)malloc(sizeof(int) * n);
+    c->order = (int *)malloc(sizeof(int) * n);
+    return c;
+}
+
+void jo_destroy(jo_ctx *c) {
+    if (!c) return;
+    int ncell = c->nv * c->nv * c->nv;
+    for (int i = 0; i < ncell; ++i) free(c->inr[i]);
+    free(c->inr); free(c->nnr);
+    free(c->chgmod); free(c->charge); free(c->mass); free(c->imass); free(c->atype); free(c->icls);
+    free(c->zero_vdw); free(c->c6); free(c->c12); free(c->nmat); free(c->x14p); free(c->x14i);
+    free(c->excess); free(c->exc_cls); free(c->dself); free(c->shk_n); free(c->shk_a); free(c->shk_d2);
+    free(c->cell3); free(c->inum); free(c->order); free(c->lptr); free(c->nel); free(c->nvd);
+    free(c->lidx); free(c->elcap);
+    free(c);
+}
+
+/* Constants for the tests. */
+void jo_constants(const jo_ctx *c, double *out /* [16] */) {
+    out[0] = c->zcore; out[1] = c->bcoeff; out[2] = c->fcoeff; out[3] = c->irlim2;
+    out[4] = c->rlim; out[5] = c->rlim2; out[6] = c->siz; out[7] = c->org[0];
+    out[8] = c->org[1]; out[9] = c->org[2]; out[10] = c->nv; out[11] = c->nlev;
+    out[12] = c->m1[0]; out[13] = c->m2[0]; out[14] = c->mdel[0]; out[15] = c->ccoef;
+}
+void jo_dself(const jo_ctx *c, double *out) { memcpy(out, c->dself, sizeof(double) * c->nfrg); }
+void jo_chgmod(const jo_ctx *c, double *out) { memcpy(out, c->chgmod, sizeof(double) * c->n); }
+void jo_ljtable(const jo_ctx *c, double *c6, double *c12) {
+    memcpy(c6, c->c6, sizeof(double) * c->ntyp * c->ntyp);
+    memcpy(c12, c->c12, sizeof(double) * c->ntyp * c->ntyp);
+}
+
+/* ------------------------------------------------------------ celprp */
+
+/* celpar_PB binning (celpar.f90:115-127): cell (mx,my,mz), 1-based. */
+static void cell_of(const jo_ctx *c, const double *p, int *m) {
+    double g[3];
+    g[0] = p[0] - c->fxcell[0] * floor((p[0] - c->celwal[0]) * c->invcel[0]);
+    g[1] = p[1] - c->fxcell[1] * floor((p[1] - c->celwal[2]) * c->invcel[1]);
+    g[2] = p[2] - c->fxcell[2] * floor((p[2] - c->celwal[4]) * c->invcel[2]);
+    for (int k = 0; k < 3; ++k) {
+        g[k] = g[k] - c->org[k];
+        m[k] = (int)(g[k] * c->rr) + 1;
+    }
+}
+
+/* FP64 minimum-image distance test of cutoff.f90:464-469 / 478-483. */
+static inline double d2_minimg(const jo_ctx *c, const double *a, const double *b) {
+    double d0 = a[0] - b[0], d1 = a[1] - b[1], d2 = a[2] - b[2];
+    d0 = d0 - c->fxcell[0] * anint(d0 * c->invcel[0]);
+    d1 = d1 - c->fxcell[1] * anint(d1 * c->invcel[1]);
+    d2 = d2 - c->fxcell[2] * anint(d2 * c->invcel[2]);
+    return d0 * d0 + d1 * d1 + d2 * d2;
+}
+
+typedef struct {
+    int *start; /* [ncell*nfrag + 1] */
+    int *cnt;   /* [ncell*nfrag] */
+    int *nel;   /* [ncell*nfrag] */
+    int *atoms; /* ipcl after the EL/vdW split */
+} cellbin;
+
+/* One pass of the cutatm_PB DOUB traversal.  With lidx == NULL only counts. */
+static void traverse(const jo_ctx *c, const double *x, const cellbin *cb, int64_t *cntEL, int64_t *cntVD,
+                     const int64_t *lptr, const int64_t *elbase, int *lidx) {
+    int nv = c->nv, nf = c->nfrag, nfrg = c->nfrg;
+#define APPEND(list_is_el, iat, n_, jat)                                                        \
+    do {                                                                                        \
+        size_t key = (size_t)(iat) * nfrg + (n_);                                               \
+        if (list_is_el) {                                                                       \
+            if (lidx) lidx[lptr[key] + cntEL[key]] = (jat);                                     \
+            cntEL[key]++;                                                                       \
+        } else {                                                                                \
+            if (lidx) lidx[lptr[key] + elbase[key] + cntVD[key]] = (jat);                        \
+            cntVD[key]++;                                                                       \
+        }                                                                                       \
+    } while (0)
+    for (int lin = 0; lin < nv * nv * nv; ++lin) {
+        const int *num = cb->cnt + (size_t)lin * nf;
+        int any = 0;
+        for (int i = 0; i < nf; ++i) any |= num[i];
+        if (!any) continue;
+        for (int i = 0; i < nf; ++i) {
+            if (num[i] == 0) continue;
+            int nel = cb->nel[(size_t)lin * nf + i];
+            const int *ip = cb->atoms + cb->start[(size_t)lin * nf + i];
+            for (int ii = 0; ii < num[i]; ++ii) {
+                int iat1 = ip[ii];
+                int n = c->nmat[i * nf + i];
+                if (ii < nel) {
+                    for (int jj = 0; jj < ii; ++jj) APPEND(1, iat1, n, ip[jj]);
+                    for (int jj = ii + 1; jj < num[i]; ++jj) APPEND(1, iat1, n, ip[jj]);
+                } else {
+                    for (int jj = 0; jj < nel; ++jj) APPEND(1, iat1, n, ip[jj]);
+                    for (int jj = nel; jj < ii; ++jj) APPEND(0, iat1, n, ip[jj]);
+                    for (int jj = ii + 1; jj < num[i]; ++jj) APPEND(0, iat1, n, ip[jj]);
+                }
+                for (int j = 0; j < nf; ++j) {
+                    if (j == i) continue;
+                    int nn = c->nmat[j * nf + i];
+                    const int *jp = cb->atoms + cb->start[(size_t)lin * nf + j];
+                    int border = (ii < nel) ? num[j] : cb->nel[(size_t)lin * nf + j];
+                    for (int jj = 0; jj < border; ++jj) APPEND(1, iat1, nn, jp[jj]);
+                    for (int jj = border; jj < num[j]; ++jj) APPEND(0, iat1, nn, jp[jj]);
+                }
+                const double *tt = x + 3 * (size_t)iat1;
+                for (int ice = 0; ice < c->nnr[lin]; ++ice) {
+                    int jl = c->inr[lin][ice];
+                    const int *num2 = cb->cnt + (size_t)jl * nf;
+                    int any2 = 0;
+                    for (int j = 0; j < nf; ++j) any2 |= num2[j];
+                    if (!any2) continue;
+                    for (int j = 0; j < nf; ++j) {
+                        int nn = c->nmat[j * nf + i];
+                        const int *jp = cb->atoms + cb->start[(size_t)jl * nf + j];
+                        int border = (ii < nel) ? num2[j] : cb->nel[(size_t)jl * nf + j];
+                        for (int kk = 0; kk < border; ++kk)
+                            if (d2_minimg(c, tt, x + 3 * (size_t)jp[kk]) <= c->rlim2) APPEND(1, iat1, nn, jp[kk]);
+                        for (int kk = border; kk < num2[j]; ++kk)
+                            if (d2_minimg(c, tt, x + 3 * (size_t)jp[kk]) <= c->rlim2) APPEND(0, iat1, nn, jp[kk]);
+                    }
+                }
+            }
+        }
+    }
+#undef APPEND
+}
+
+/* celprp + cutoff_PB.  Returns md_run's ier. */
+int jo_celprp(jo_ctx *c, const double *x) {
+    int n = c->n, nv = c->nv, nf = c->nfrag, nfrg = c->nfrg;
+    int ncell = nv * nv * nv;
+    cellbin cb;
+    cb.cnt = (int *)calloc((size_t)ncell * nf, sizeof(int));
+    cb.nel = (int *)calloc((size_t)ncell * nf, sizeof(int));
+    cb.start = (int *)calloc((size_t)ncell * nf + 1, sizeof(int));
+    cb.atoms = (int *)malloc(sizeof(int) * n);
+    int *lin_of = (int *)malloc(sizeof(int) * n);
+    int ier = 0;
+    for (int j = 0; j < n; ++j) {
+        int m[3];
+        cell_of(c, x + 3 * (size_t)j, m);
+        c->cell3[3 * j] = m[0]; c->cell3[3 * j + 1] = m[1]; c->cell3[3 * j + 2] = m[2];
+        if (m[0] < 1 || m[0] > nv || m[1] < 1 || m[1] > nv || m[2] < 1 || m[2] > nv) { ier = 28; lin_of[j] = -1; continue; }
+        lin_of[j] = ((m[2] - 1) * nv + (m[1] - 1)) * nv + (m[0] - 1);
+        cb.cnt[(size_t)lin_of[j] * nf + c->icls[j]]++;
+    }
+    if (ier) goto out;
+    for (size_t k = 0; k < (size_t)ncell * nf; ++k) {
+        cb.start[k + 1] = cb.start[k] + cb.cnt[k];
+        if (cb.cnt[k] > c->ipmax) ier = 28; /* celpar.f90:182-192 */
+    }
+    if (ier) goto out;
+    {
+        /* ipcl in ascending atom order, then split EL | EL+vdW (celprp.f90:52-80) */
+        int *w = (int *)calloc((size_t)ncell * nf, sizeof(int));
+        for (int j = 0; j < n; ++j) if (c->zero_vdw[j]) {
+            size_t k = (size_t)lin_of[j] * nf + c->icls[j];
+            cb.atoms[cb.start[k] + w[k]++] = j;
+        }
+        for (size_t k = 0; k < (size_t)ncell * nf; ++k) cb.nel[k] = w[k];
+        for (int j = 0; j < n; ++j) if (!c->zero_vdw[j]) {
+            size_t k = (size_t)lin_of[j] * nf + c->icls[j];
+            cb.atoms[cb.start[k] + w[k]++] = j;
+        }
+        free(w);
+    }
+    {
+        /* inum in 2x2x2 block order (celprp.f90:83-104) */
+        int ii = 0;
+        for (int iz = 0; iz < nv; iz += 2)
+            for (int iy = 0; iy < nv; iy += 2)
+                for (int ix = 0; ix < nv; ix += 2)
+                    for (int jz = iz; jz <= iz + 1; ++jz)
+                        for (int jy = iy; jy <= iy + 1; ++jy)
+                            for (int jx = ix; jx <= ix + 1; ++jx)
+                                for (int i = 0; i < nf; ++i) {
+                                    size_t k = ((size_t)(jz * nv + jy) * nv + jx) * nf + i;
+                                    for (int j = 0; j < cb.cnt[k]; ++j) {
+                                        int a = cb.atoms[cb.start[k] + j];
+                                        c->inum[a] = ii;
+                                        c->order[ii] = a;
+                                        ++ii;
+                                    }
+                                }
+    }
+    {
+        size_t nl = (size_t)n * nfrg;
+        int64_t *ce = (int64_t *)calloc(nl, sizeof(int64_t));
+        int64_t *cv = (int64_t *)calloc(nl, sizeof(int64_t));
+        traverse(c, x, &cb, ce, cv, NULL, NULL, NULL);
+        free(c->lptr); free(c->lidx); free(c->nel); free(c->nvd);
+        c->lptr = (int64_t *)malloc(sizeof(int64_t) * (nl + 1));
+        c->lptr[0] = 0;
+        int64_t n15mx = (int64_t)ceil(0.15 * (4.0 / 3.0 * 3.14159265358979 * c->rc * c->rc * c->rc)) * 2;
+        for (size_t k = 0; k < nl; ++k) {
+            c->lptr[k + 1] = c->lptr[k] + ce[k] + cv[k];
+            if (ce[k] > n15mx || cv[k] > n15mx) ier = 51; /* cutoff.f90:30-46 */
+        }
+        c->lidx = (int *)malloc(sizeof(int) * (c->lptr[nl] ? c->lptr[nl] : 1));
+        int64_t *eb = (int64_t *)malloc(sizeof(int64_t) * nl);
+        memcpy(eb, ce, nl * sizeof(int64_t));
+        memset(ce, 0, nl * sizeof(int64_t));
+        memset(cv, 0, nl * sizeof(int64_t));
+        traverse(c, x, &cb, ce, cv, c->lptr, eb, c->lidx);
+        free(eb);
+        c->nel = (int *)malloc(sizeof(int) * nl);
+        c->nvd = (int *)malloc(sizeof(int) * nl);
+        free(c->elcap);
+        c->elcap = (int *)malloc(sizeof(int) * nl);
+        for (size_t k = 0; k < nl; ++k) { c->nel[k] = c->elcap[k] = (int)ce[k]; c->nvd[k] = (int)cv[k]; }
+        free(ce); free(cv);
+    }
+    {
+        /* exclusion swap-delete (cutoff.f90:50-96); vdW entries follow the
+         * EL entries in lidx, so "last" is taken within each part */
+        int64_t nchk = 0;
+        for (int a = 0; a < n; ++a) {
+            for (int e = c->x14p[a]; e < c->x14p[a + 1]; ++e) {
+                int b = c->x14i[e];
+                int kk = c->nmat[c->icls[a] * nf + c->icls[b]];
+                size_t key = (size_t)a * nfrg + kk;
+                int *L = c->lidx + c->lptr[key];
+                if (c->zero_vdw[a] || c->zero_vdw[b]) {
+                    for (int j = 0; j < c->nel[key]; ++j)
+                        if (L[j] == b) { L[j] = L[c->nel[key] - 1]; c->nel[key]--; nchk++; break; }
+                } else {
+                    int *V = L + c->elcap[key];
+                    for (int j = 0; j < c->nvd[key]; ++j)
+                        if (V[j] == b) { V[j] = V[c->nvd[key] - 1]; c->nvd[key]--; nchk++; break; }
+                }
+            }
+        }
+        if (nchk / 2 != c->nntab14 && ier == 0) ier = 99;
+    }
+    c->have_list = 1;
+out:
+    free(cb.cnt); free(cb.nel); free(cb.start); free(cb.atoms); free(lin_of);
+    return ier;
+}
+
+/* Decoded full list (ascending 1-based partners per atom, classes merged)
+ * and the reference cell of each atom; same contract as jp_get_list. */
+static int cmp_int(const void *a, const void *b) { int x = *(const int *)a, y = *(const int *)b; return (x > y) - (x < y); }
+
+int jo_get_list(const jo_ctx *c, int32_t *cell3, int64_t *npairs, int32_t *ptr, int32_t *idx, int64_t cap) {
+    if (!c->have_list) return JP_EINVAL;
+    int n = c->n, nfrg = c->nfrg;
+    if (cell3) for (int i = 0; i < 3 * n; ++i) cell3[i] = c->cell3[i];
+    int64_t tot = 0;
+    for (int i = 0; i < n; ++i)
+        for (int k = 0; k < nfrg; ++k) tot += c->nel[(size_t)i * nfrg + k] + c->nvd[(size_t)i * nfrg + k];
+    *npairs = tot;
+    if (!idx) return 0;
+    if (tot > cap) return JP_EINVAL;
+    int64_t w = 0;
+    for (int i = 0; i < n; ++i) {
+        ptr[i] = (int32_t)w;
+        int64_t s = w;
+        for (int k = 0; k < nfrg; ++k) {
+            size_t key = (size_t)i * nfrg + k;
+            const int *L = c->lidx + c->lptr[key];
+            for (int j = 0; j < c->nel[key]; ++j) idx[w++] = L[j] + 1;
+            for (int j = 0; j < c->nvd[key]; ++j) idx[w++] = L[c->elcap[key] + j] + 1;
+        }
+        qsort(idx + s, (size_t)(w - s), sizeof(int32_t), cmp_int);
+    }
+    ptr[n] = (int32_t)w;
+    return 0;
+}
+
+/* Sorted order (inum) used by the energy summation. */
+int jo_get_order(const jo_ctx *c, int32_t *order) {
+    if (!c->have_list) return JP_EINVAL;
+    for (int i = 0; i < c->n; ++i) order[i] = c->order[i] + 1;
+    return 0;
+}
+
+/*
+ * cmmdo -> enear_ZD_doub_PB (enear.f90:60-326, 518-571) with i_vec = 1.
+ * grad[nfrg][n][3] is ACCUMULATED (dnergy zeroes it); eelec/evdw[nfrg] are set.
+ * Per atom: one accumulator over the EL list then the vdW list (gwxv in the
+ * reference); energies: one running sum per class over atoms in inum order.
+ */
+int jo_nonbond(const jo_ctx *c, const double *x, double *grad, double *eelec, double *evdw) {
+    if (!c->have_list) return JP_EINVAL;
+    int n = c->n, nfrg = c->nfrg, nt = c->ntyp;
+    double *Tg = (double *)calloc((size_t)nfrg * n * 3, sizeof(double));
+    double *pe = (double *)malloc(sizeof(double) * (size_t)n * nfrg * 2);
+    const double zc = c->zcore, bc = c->bcoeff, fc = c->fcoeff, ir2 = c->irlim2;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int ia = 0; ia < n; ++ia) {
+        const double *xi = x + 3 * (size_t)ia;
+        double chgi = c->chgmod[ia];
+        int it1 = c->atype[ia];
+        for (int k = 0; k < nfrg; ++k) {
+            size_t key = (size_t)ia * nfrg + k;
+            const int *L = c->lidx + c->lptr[key];
+            double gx = 0.0, gy = 0.0, gz = 0.0, es = 0.0, vs = 0.0;
+            for (int j = 0; j < c->nel[key]; ++j) {
+                int kk = L[j];
+                const double *xj = x + 3 * (size_t)kk;
+                double dx = xi[0] - xj[0], dy = xi[1] - xj[1], dz = xi[2] - xj[2];
+                dx = dx - c->fxcell[0] * anint(dx * c->invcel[0]);
+                dy = dy - c->fxcell[1] * anint(dy * c->invcel[1]);
+                dz = dz - c->fxcell[2] * anint(dz * c->invcel[2]);
+                double dd2 = dx * dx + dy * dy + dz * dz;
+                double dd2i = 1.0 / dd2;
+                double co = copysign(0.5, dd2i - ir2) + 0.5;
+                double ddi = sqrt(dd2i);
+                double ee = chgi * c->chgmod[kk] * co;
+                es = es + ee * (ddi + bc * dd2 - zc);
+                double cc = ee * (dd2i * ddi - fc);
+                gx = gx + dx * cc; gy = gy + dy * cc; gz = gz + dz * cc;
+            }
+            for (int j = 0; j < c->nvd[key]; ++j) {
+                int kk = L[c->elcap[key] + j];
+                const double *xj = x + 3 * (size_t)kk;
+                double dx = xi[0] - xj[0], dy = xi[1] - xj[1], dz = xi[2] - xj[2];
+                dx = dx - c->fxcell[0] * anint(dx * c->invcel[0]);
+                dy = dy - c->fxcell[1] * anint(dy * c->invcel[1]);
+                dz = dz - c->fxcell[2] * anint(dz * c->invcel[2]);
+                int it2 = c->atype[kk];
+                double v6 = c->c6[it2 * nt + it1], v12 = c->c12[it2 * nt + it1];
+                double dd2 = dx * dx + dy * dy + dz * dz;
+                double dd2i = 1.0 / dd2;
+                double co = copysign(0.5, dd2i - ir2) + 0.5;
+                double ddi = sqrt(dd2i);
+                double dd6i = dd2i * dd2i * dd2i * co;
+                double ee = chgi * c->chgmod[kk] * co;
+                es = es + ee * (ddi + bc * dd2 - zc);
+                double v1 = v6 * dd6i;
+                double v2 = v12 * dd6i * dd6i;
+                vs = vs - v1 + v2;
+                double cc = ee * (dd2i * ddi - fc) + (-6.0 * v1 + 12.0 * v2) * dd2i;
+                gx = gx + dx * cc; gy = gy + dy * cc; gz = gz + dz * cc;
+            }
+            double *t = Tg + ((size_t)k * n + ia) * 3;
+            t[0] = 0.0 - (0.0 + gx); t[1] = 0.0 - (0.0 + gy); t[2] = 0.0 - (0.0 + gz);
+            pe[key * 2] = es;
+            pe[key * 2 + 1] = vs;
+        }
+    }
+    /* energies: the reference keeps one running sum per class over all
+     * (ii, j) in sorted order; replay that order from the per-atom terms is
+     * not bit-identical, so sum the per-pair terms again in that order */
+    for (int k = 0; k < nfrg; ++k) {
+        double es = 0.0, vs = 0.0;
+        for (int p = 0; p < n; ++p) {
+            int ia = c->order[p];
+            const double *xi = x + 3 * (size_t)ia;
+            double chgi = c->chgmod[ia];
+            int it1 = c->atype[ia];
+            size_t key = (size_t)ia * nfrg + k;
+            const int *L = c->lidx + c->lptr[key];
+            for (int j = 0; j < c->nel[key]; ++j) {
+                int kk = L[j];
+                const double *xj = x + 3 * (size_t)kk;
+                double dx = xi[0] - xj[0], dy = xi[1] - xj[1], dz = xi[2] - xj[2];
+                dx = dx - c->fxcell[0] * anint(dx * c->invcel[0]);
+                dy = dy - c->fxcell[1] * anint(dy * c->invcel[1]);
+                dz = dz - c->fxcell[2] * anint(dz * c->invcel[2]);
+                double dd2 = dx * dx + dy * dy + dz * dz;
+                double dd2i = 1.0 / dd2;
+                double co = copysign(0.5, dd2i - ir2) + 0.5;
+                double ddi = sqrt(dd2i);
+                double ee = chgi * c->chgmod[kk] * co;
+                es = es + ee * (ddi + bc * dd2 - zc);
+            }
+            for (int j = 0; j < c->nvd[key]; ++j) {
+                int kk = L[c->elcap[key] + j];
+                const double *xj = x + 3 * (size_t)kk;
+                double dx = xi[0] - xj[0], dy = xi[1] - xj[1], dz = xi[2] - xj[2];
+                dx = dx - c->fxcell[0] * anint(dx * c->invcel[0]);
+                dy = dy - c->fxcell[1] * anint(dy * c->invcel[1]);
+                dz = dz - c->fxcell[2] * anint(dz * c->invcel[2]);
+                int it2 = c->atype[kk];
+                double v6 = c->c6[it2 * nt + it1], v12 = c->c12[it2 * nt + it1];
+                double dd2 = dx * dx + dy * dy + dz * dz;
+                double dd2i = 1.0 / dd2;
+                double co = copysign(0.5, dd2i - ir2) + 0.5;
+                double ddi = sqrt(dd2i);
+                double dd6i = dd2i * dd2i * dd2i * co;
+                double ee = chgi * c->chgmod[kk] * co;
+                es = es + ee * (ddi + bc * dd2 - zc);
+                double v1 = v6 * dd6i;
+                double v2 = v12 * dd6i * dd6i;
+                vs = vs - v1 + v2;
+            }
+        }
+        eelec[k] = (0.0 + es);
+        evdw[k] = (0.0 + vs);
+    }
+    free(pe);
+    /* excess (enear.f90:518-553), TPL order = one colour (nomp == 1) */
+    double *Eexc = (double *)calloc(nfrg, sizeof(double));
+    for (int e = 0; e < c->nexcess; ++e) {
+        int a = c->excess[2 * e], b = c->excess[2 * e + 1], k = c->exc_cls[e];
+        double dx = x[3 * a] - x[3 * b], dy = x[3 * a + 1] - x[3 * b + 1], dz = x[3 * a + 2] - x[3 * b + 2];
+        dx = dx - c->fxcell[0] * anint(dx * c->invcel[0]);
+        dy = dy - c->fxcell[1] * anint(dy * c->invcel[1]);
+        dz = dz - c->fxcell[2] * anint(dz * c->invcel[2]);
+        double dd2 = dx * dx + dy * dy + dz * dz;
+        double cc = c->chgmod[a] * c->chgmod[b];
+        Eexc[k] = Eexc[k] + cc * bc * dd2;
+        cc = cc * fc;
+        dx = dx * cc; dy = dy * cc; dz = dz * cc;
+        double *ga = grad + ((size_t)k * n + a) * 3, *gb = grad + ((size_t)k * n + b) * 3;
+        ga[0] = ga[0] + dx; ga[1] = ga[1] + dy; ga[2] = ga[2] + dz;
+        gb[0] = gb[0] - dx; gb[1] = gb[1] - dy; gb[2] = gb[2] - dz;
+    }
+    /* scatter back and finish (enear.f90:557-571) */
+    for (size_t q = 0; q < (size_t)nfrg * n * 3; ++q) grad[q] = grad[q] + Tg[q];
+    for (int k = 0; k < nfrg; ++k) {
+        eelec[k] = eelec[k] * 0.5;
+        evdw[k] = evdw[k] * 0.5;
+        eelec[k] = eelec[k] + c->dself[k] + Eexc[k];
+    }
+    free(Eexc);
+    free(Tg);
+    return 0;
+}
+
+/* ------------------------------------------------------------------ SHAKE */
+
+static inline void mimg(const jo_ctx *c, double *d) {
+    for (int k = 0; k < 3; ++k) d[k] = d[k] - c->fxcell[k] * anint(d[k] * c->invcel[k]);
+}
+
+/* shak2e_PB, shake.f90:687-755 */
+static void shak2e(const jo_ctx *c, int g, double *r1, const double *r2) {
+    int ia = c->shk_a[4 * g], ja = c->shk_a[4 * g + 1];
+    double im1 = 1.0 / c->mass[ia], im2 = 1.0 / c->mass[ja];
+    double im3 = im1 + im2, im4 = im3 * im3;
+    double d1[3], d2[3];
+    for (int k = 0; k < 3; ++k) { d1[k] = r1[3 * ja + k] - r1[3 * ia + k]; d2[k] = r2[3 * ja + k] - r2[3 * ia + k]; }
+    mimg(c, d1); mimg(c, d2);
+    double d1d1 = d1[0] * d1[0] + d1[1] * d1[1] + d1[2] * d1[2];
+    double d1d2 = d1[0] * d2[0] + d1[1] * d2[1] + d1[2] * d2[2];
+    double d2d2 = d2[0] * d2[0] + d2[1] * d2[1] + d2[2] * d2[2];
+    double A = im4 * d2d2, B = im3 * d1d2, C = d1d1 - c->shk_d2[6 * g];
+    double G = (B - sqrt(B * B - C * A)) / A;
+    for (int k = 0; k < 3; ++k) {
+        r1[3 * ia + k] = r1[3 * ia + k] + (im1 * G * d2[k]);
+        r1[3 * ja + k] = r1[3 * ja + k] - (im2 * G * d2[k]);
+    }
+}
+
+/* shake3_PB, shake.f90:112-327.  Returns 0, 1 (not converged), -2 (singular). */
+static int shake3(const jo_ctx *c, int g, double *r1, const double *r2) {
+    const int *a = c->shk_a + 4 * g;
+    double im[3] = {c->imass[a[0]], c->imass[a[1]], c->imass[a[2]]};
+    double co[3][3];
+    co[0][0] = im[0] + im[1]; co[0][1] = -im[1]; co[0][2] = -im[0];
+    co[1][0] = -im[1]; co[1][1] = im[1] + im[2]; co[1][2] = -im[2];
+    co[2][0] = -im[0]; co[2][1] = -im[2]; co[2][2] = im[2] + im[0];
+    const double *T = c->shk_d2 + 6 * g;
+    double TT[3] = {c->shk_tol * T[0], c->shk_tol * T[1], c->shk_tol * T[2]};
+    double G[6] = {0, 0, 0, 0, 0, 0};
+    double d1[3][3], d2[3][3]; /* [constraint][xyz] */
+    for (int k = 0; k < 3; ++k) {
+        d1[0][k] = r1[3 * a[0] + k] - r1[3 * a[1] + k];
+        d1[1][k] = r1[3 * a[1] + k] - r1[3 * a[2] + k];
+        d1[2][k] = r1[3 * a[2] + k] - r1[3 * a[0] + k];
+        d2[0][k] = r2[3 * a[0] + k] - r2[3 * a[1] + k];
+        d2[1][k] = r2[3 * a[1] + k] - r2[3 * a[2] + k];
+        d2[2][k] = r2[3 * a[2] + k] - r2[3 * a[0] + k];
+    }
+    for (int q = 0; q < 3; ++q) { mimg(c, d1[q]); mimg(c, d2[q]); }
+    for (int it = 0; it < c->shk_maxit; ++it) {
+        double B[3];
+        for (int q = 0; q < 3; ++q) B[q] = T[q] - (d1[q][0] * d1[q][0] + d1[q][1] * d1[q][1] + d1[q][2] * d1[q][2]);
+        if (fabs(B[0]) < TT[0] && fabs(B[1]) < TT[1] && fabs(B[2]) < TT[2]) {
+            for (int k = 0; k < 3; ++k) {
+                r1[3 * a[0] + k] = r1[3 * a[0] + k] + im[0] * (G[3] * d2[0][k] - G[5] * d2[2][k]);
+                r1[3 * a[1] + k] = r1[3 * a[1] + k] + im[1] * (-G[3] * d2[0][k] + G[4] * d2[1][k]);
+                r1[3 * a[2] + k] = r1[3 * a[2] + k] + im[2] * (-G[4] * d2[1][k] + G[5] * d2[2][k]);
+            }
+            return 0;
+        }
+        double gf[3][3]; /* gradf(icon, ig) */
+        for (int ig = 0; ig < 3; ++ig)
+            for (int q = 0; q < 3; ++q)
+                gf[q][ig] = 2.0 * co[q][ig] * (d1[q][0] * d2[ig][0] + d1[q][1] * d2[ig][1] + d1[q][2] * d2[ig][2]);
+        double deta = (gf[0][0] * gf[1][1] * gf[2][2]) + (gf[0][1] * gf[1][2] * gf[2][0]) +
+                      (gf[0][2] * gf[1][0] * gf[2][1]) - (gf[0][0] * gf[1][2] * gf[2][1]) -
+                      (gf[0][1] * gf[1][0] * gf[2][2]) - (gf[0][2] * gf[1][1] * gf[2][0]);
+        if (deta == 0.0) return -2;
+        deta = 1.0 / deta;
+        double db[3][3];
+        db[0][0] = gf[1][1] * gf[2][2] - gf[1][2] * gf[2][1];
+        db[0][1] = gf[2][1] * gf[0][2] - gf[2][2] * gf[0][1];
+        db[0][2] = gf[0][1] * gf[1][2] - gf[0][2] * gf[1][1];
+        db[1][0] = gf[1][2] * gf[2][0] - gf[1][0] * gf[2][2];
+        db[1][1] = gf[2][2] * gf[0][0] - gf[2][0] * gf[0][2];
+        db[1][2] = gf[0][2] * gf[1][0] - gf[0][0] * gf[1][2];
+        db[2][0] = gf[1][0] * gf[2][1] - gf[1][1] * gf[2][0];
+        db[2][1] = gf[2][0] * gf[0][1] - gf[2][1] * gf[0][0];
+        db[2][2] = gf[0][0] * gf[1][1] - gf[0][1] * gf[1][0];
+        for (int q = 0; q < 3; ++q) G[q] = deta * (db[q][0] * B[0] + db[q][1] * B[1] + db[q][2] * B[2]);
+        for (int q = 0; q < 3; ++q) G[3 + q] = G[3 + q] + G[q];
+        for (int q = 0; q < 3; ++q)
+            for (int k = 0; k < 3; ++k)
+                d1[q][k] = d1[q][k] + co[q][0] * d2[0][k] * G[0] + co[q][1] * d2[1][k] * G[1] + co[q][2] * d2[2][k] * G[2];
+    }
+    return 1;
+}
+
+/* shake_PB (shake.f90:2-59): 2-atom groups, then 3-atom groups. */
+int jo_shake(const jo_ctx *c, double *r1, const double *r2) {
+    int ier = 0;
+    for (int g = 0; g < c->nshk; ++g)
+        if (c->shk_n[g] == 2) shak2e(c, g, r1, r2);
+    for (int g = 0; g < c->nshk; ++g)
+        if (c->shk_n[g] == 3) {
+            int e = shake3(c, g, r1, r2);
+            if (e == -2) return -2;
+            if (e == 1) ier = 1;
+        }
+    for (int g = 0; g < c->nshk; ++g)
+        if (c->shk_n[g] > 3) return JP_EINVAL;
+    return ier;
+}
+
+/* ------------------------------------------------------------ integrator */
+
+/* verlet.f90:144-215 (METHOD= MICR): gtot is the total gradient (already
+ * lambda-combined); v is v(t-dt/2) on entry, v(t+dt/2) on exit. */
+int jo_verlet_step(const jo_ctx *c, double *x, double *v, const double *gtot, double *xold) {
+    int n = c->n;
+    double rtmp = -c->dt * 4.184e-4; /* md.f90:56 fdcvcv */
+    memcpy(xold, x, sizeof(double) * 3 * n);
+    for (int i = 0; i < n; ++i)
+        for (int k = 0; k < 3; ++k) v[3 * i + k] = v[3 * i + k] + rtmp * gtot[3 * i + k] / c->mass[i];
+    for (int i = 0; i < n; ++i)
+        for (int k = 0; k < 3; ++k) x[3 * i + k] = x[3 * i + k] + c->dt * v[3 * i + k];
+    if (c->nshk) {
+        int ier = jo_shake(c, x, xold);
+        if (ier) return ier;
+        double r = 1.0 / c->dt;
+        for (int i = 0; i < n; ++i)
+            for (int k = 0; k < 3; ++k) v[3 * i + k] = (x[3 * i + k] - xold[3 * i + k]) * r;
+    }
+    return 0;
+}
+
+/* grad3 = lambda^2 g1 + lambda g2 + g3 (verlet.f90:67-91, nfrg = 3) or the
+ * single class. */
+void jo_combine_grad(const jo_ctx *c, const double *grad, double *gtot) {
+    int n = c->n;
+    if (c->nfrg == 1) { memcpy(gtot, grad, sizeof(double) * 3 * n); return; }
+    double l = c->lambda, l2 = l * l;
+    for (size_t q = 0; q < (size_t)3 * n; ++q)
+        gtot[q] = grad[q] * l2 + grad[(size_t)3 * n + q] * l + grad[(size_t)6 * n + q];
+}
+
+/* caltmp.f90:18-27 kinetic energy (kcal/mol) of velocities v. */
+double jo_ekin(const jo_ctx *c, const double *v) {
+    double ek = 0.0;
+    for (int i = 0; i < c->n; ++i)
+        ek = ek + c->mass[i] * (v[3 * i] * v[3 * i] + v[3 * i + 1] * v[3 * i + 1] + v[3 * i + 2] * v[3 * i + 2]);
+    ek = ek * 1.0e7;
+    return ek / 4.184 * 0.0005;
+}
+
+/* NVE run on the CPU (the port baseline): md_run's loop order, list rebuilt
+ * after every `update`-th step.  Returns ier. */
+int jo_run_nve(jo_ctx *c, double *x, double *v, int nsteps, int update) {
+    int n = c->n, nfrg = c->nfrg;
+    double *grad = (double *)malloc(sizeof(double) * 3 * (size_t)n * nfrg);
+    double *gt = (double *)malloc(sizeof(double) * 3 * (size_t)n);
+    double *xo = (double *)malloc(sizeof(double) * 3 * (size_t)n);
+    double ee[3], ev[3];
+    int ier = c->have_list ? 0 : jo_celprp(c, x);
+    for (int s = 1; s <= nsteps && !ier; ++s) {
+        memset(grad, 0, sizeof(double) * 3 * (size_t)n * nfrg);
+        jo_nonbond(c, x, grad, ee, ev);
+        jo_combine_grad(c, grad, gt);
+        ier = jo_verlet_step(c, x, v, gt, xo);
+        if (!ier && update > 0 && s % update == 0) ier = jo_celprp(c, x);
+    }
+    free(grad); free(gt); free(xo);
+    return ier;
+}
+
+/* ------------------------------------------------- brute-force pair set */
+
+/* {j != i : FP64 minimum-image d^2 <= rlim^2} minus excluded pairs, as CSR
+ * with ascending 1-based partners (the mathematical set the cell list must
+ * produce). */
+int64_t jo_list_brute(const jo_ctx *c, const double *x, int32_t *ptr, int32_t *idx, int64_t cap) {
+    int n = c->n;
+    int64_t w = 0;
+    for (int i = 0; i < n; ++i) {
+        if (ptr) ptr[i] = (int32_t)w;
+        int e = c->x14p[i], ee = c->x14p[i + 1];
+        for (int j = 0; j < n; ++j) {
+            if (j == i) continue;
+            if (d2_minimg(c, x + 3 * (size_t)i, x + 3 * (size_t)j) <= c->rlim2) {
+                while (e < ee && c->x14i[e] < j) ++e;
+                if (e < ee && c->x14i[e] == j) continue;
+                if (idx && w < cap) idx[w] = j + 1;
+                ++w;
+            }
+        }
+    }
+    if (ptr) ptr[n] = (int32_t)w;
+    return w;
+}

@@ -1,0 +1,156 @@
+"""ctypes wrapper of the CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: used by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py as the checker; never by the product path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from j_presto_amd.abi import DescHolder
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+_lib = None
+
+i32p = C.POINTER(C.c_int32)
+i64p = C.POINTER(C.c_int64)
+f64p = C.POINTER(C.c_double)
+
+
+def build() -> str:
+    subprocess.run(["make", "-s", "-C", HERE, "liboracle.so"], check=True)
+    return LIB
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = C.CDLL(LIB)
+        L.jo_create.restype = C.c_void_p
+        L.jo_create.argtypes = [C.c_void_p]
+        L.jo_destroy.argtypes = [C.c_void_p]
+        L.jo_celprp.argtypes = [C.c_void_p, f64p]
+        L.jo_get_list.argtypes = [C.c_void_p, i32p, i64p, i32p, i32p, C.c_int64]
+        L.jo_get_order.argtypes = [C.c_void_p, i32p]
+        L.jo_nonbond.argtypes = [C.c_void_p, f64p, f64p, f64p, f64p]
+        L.jo_shake.argtypes = [C.c_void_p, f64p, f64p]
+        L.jo_verlet_step.argtypes = [C.c_void_p, f64p, f64p, f64p, f64p]
+        L.jo_combine_grad.argtypes = [C.c_void_p, f64p, f64p]
+        L.jo_ekin.argtypes = [C.c_void_p, f64p]
+        L.jo_ekin.restype = C.c_double
+        L.jo_run_nve.argtypes = [C.c_void_p, f64p, f64p, C.c_int, C.c_int]
+        L.jo_list_brute.argtypes = [C.c_void_p, f64p, i32p, i32p, C.c_int64]
+        L.jo_list_brute.restype = C.c_int64
+        L.jo_constants.argtypes = [C.c_void_p, f64p]
+        L.jo_dself.argtypes = [C.c_void_p, f64p]
+        L.jo_chgmod.argtypes = [C.c_void_p, f64p]
+        L.jo_ljtable.argtypes = [C.c_void_p, f64p, f64p]
+        _lib = L
+    return _lib
+
+
+def _p(a, t=f64p):
+    return a.ctypes.data_as(t)
+
+
+class Oracle:
+    """FP64 CPU restatement of md_run's ZD path for one system."""
+
+    def __init__(self, system, dt: float = 2.0, **kw):
+        self.h = DescHolder(system, dt=dt, **kw)
+        self.n = self.h.n
+        self.nfrg = self.h.nfrg
+        self.ntyp = len(system.types)
+        self.c = lib().jo_create(C.byref(self.h.desc))
+
+    def __del__(self):
+        try:
+            if self.c:
+                lib().jo_destroy(self.c)
+                self.c = None
+        except Exception:
+            pass
+
+    def celprp(self, x) -> int:
+        x = np.ascontiguousarray(x, np.float64)
+        return lib().jo_celprp(self.c, _p(x))
+
+    def get_list(self):
+        npairs = C.c_int64()
+        lib().jo_get_list(self.c, None, C.byref(npairs), None, None, 0)
+        cell = np.zeros((self.n, 3), np.int32)
+        ptr = np.zeros(self.n + 1, np.int32)
+        idx = np.zeros(max(npairs.value, 1), np.int32)
+        r = lib().jo_get_list(self.c, _p(cell, i32p), C.byref(npairs), _p(ptr, i32p), _p(idx, i32p), len(idx))
+        assert r == 0
+        return cell, ptr, idx[:npairs.value]
+
+    def order(self):
+        o = np.zeros(self.n, np.int32)
+        lib().jo_get_order(self.c, _p(o, i32p))
+        return o
+
+    def nonbond(self, x, grad=None):
+        x = np.ascontiguousarray(x, np.float64)
+        g = np.zeros((self.nfrg, self.n, 3)) if grad is None else grad
+        ee = np.zeros(self.nfrg)
+        ev = np.zeros(self.nfrg)
+        r = lib().jo_nonbond(self.c, _p(x), _p(g), _p(ee), _p(ev))
+        assert r == 0
+        return g, ee, ev
+
+    def shake(self, x_new, x_old) -> int:
+        return lib().jo_shake(self.c, _p(x_new), _p(np.ascontiguousarray(x_old)))
+
+    def combine(self, grad):
+        gt = np.zeros((self.n, 3))
+        lib().jo_combine_grad(self.c, _p(np.ascontiguousarray(grad)), _p(gt))
+        return gt
+
+    def verlet_step(self, x, v, gtot) -> int:
+        xo = np.zeros_like(x)
+        return lib().jo_verlet_step(self.c, _p(x), _p(v), _p(np.ascontiguousarray(gtot)), _p(xo))
+
+    def ekin(self, v) -> float:
+        return lib().jo_ekin(self.c, _p(np.ascontiguousarray(v)))
+
+    def run_nve(self, x, v, nsteps: int, update: int = 10) -> int:
+        return lib().jo_run_nve(self.c, _p(x), _p(v), nsteps, update)
+
+    def list_brute(self, x):
+        x = np.ascontiguousarray(x, np.float64)
+        n = lib().jo_list_brute(self.c, _p(x), None, None, 0)
+        ptr = np.zeros(self.n + 1, np.int32)
+        idx = np.zeros(max(n, 1), np.int32)
+        lib().jo_list_brute(self.c, _p(x), _p(ptr, i32p), _p(idx, i32p), len(idx))
+        return ptr, idx[:n]
+
+    def constants(self):
+        o = np.zeros(16)
+        lib().jo_constants(self.c, _p(o))
+        keys = ["zcore", "bcoeff", "fcoeff", "irlim2", "rlim", "rlim2", "siz", "org_x", "org_y", "org_z",
+                "nv", "nlev", "mx1", "mx2", "mxdelay", "ccoef"]
+        return dict(zip(keys, o))
+
+    def dself(self):
+        o = np.zeros(self.nfrg)
+        lib().jo_dself(self.c, _p(o))
+        return o
+
+    def chgmod(self):
+        o = np.zeros(self.n)
+        lib().jo_chgmod(self.c, _p(o))
+        return o
+
+    def ljtable(self):
+        c6 = np.zeros((self.ntyp, self.ntyp))
+        c12 = np.zeros((self.ntyp, self.ntyp))
+        lib().jo_ljtable(self.c, _p(c6), _p(c12))
+        return c6, c12
