@@ -1,0 +1,172 @@
+"""Benchmark: ns/day and pair-interactions/s of the ZD + LJ MD step on the
+100k-atom water box (33,333 waters + 60 Na/Cl pairs, 100.0 A, 12 A cutoff,
+SHAKE, list rebuilt every 10 steps, dt = 2 fs) on one MI355X.
+
+Prints one JSON line.  The timed region is K full MD steps on the device
+state (pair forces, ZD excess, kick/drift, SHAKE, velocity update, list
+rebuild every 10 steps, energies every 100 steps); jp_run_nve returns after a
+stream synchronize, and the host clock brackets it.  Beside it: the plain
+FP64 GPU path (same protocol, fewer steps) and the reference md_run on the
+host cores (bounded sample).
+
+usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import shutil
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP32_TFLOPS = 157.3      # MI355X FP32 vector (packed FMA counted)
+PEAK_HBM_GBS = 8000.0
+FLOP_EL, FLOP_LJEL = 42, 54   # algorithmic flop per pair (FMA = 2, rsqrt = 1)
+DT_FS = 2.0
+UPDATE = 10
+NMON = 100
+
+
+def ns_per_day(steps_per_s: float) -> float:
+    return steps_per_s * DT_FS * 0.0864      # md.f90:305-306
+
+
+def run_gpu(system, precision, steps, warmup):
+    from j_presto_amd import engine
+    md = engine.MdSystem(system, dt=DT_FS, precision=precision)
+    ier = md.celprp()
+    assert ier == 0, ier
+    if warmup:
+        _, ier = md.runmd(warmup, update=UPDATE)
+        assert ier == 0, ier
+    md.celprp()
+    t0 = time.perf_counter()
+    log, ier = md.runmd(steps, update=UPDATE, nmon=NMON)
+    t1 = time.perf_counter()
+    assert ier == 0, ier
+    tm = md.timing()
+    n, nlj, nlisted = md.count_pairs(detail=True)
+    md.close()
+    return dict(wall=t1 - t0, timing=tm, pairs=n, pairs_lj=nlj, listed=nlisted, log=log)
+
+
+def cpu_reference(system, nsteps: int):
+    """The reference md_run (oracle/_ref/md_run, built from j_presto's own
+    sources) on this host: PARALV= DOUB, N_SIMD= 64, all cores."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "md_run")
+    if not os.path.exists(exe):
+        return None
+    from j_presto_amd import refio
+    cores = len(os.sched_getaffinity(0))
+    tmp = tempfile.mkdtemp(prefix="jpcpu_")
+    try:
+        files = refio.write_inputs(tmp, system)
+        refio.write_restart(os.path.join(tmp, "in.rst"), system.x, system.v)
+        refio.write_md_inp(os.path.join(tmp, "md.inp"), files, system, {
+            "METHOD": "MICR", "RESTAR": "YES", "NAMERI": "in.rst", "NAMERO": "out.rst",
+            "LOOPLI": str(nsteps), "UPDATE": str(UPDATE), "TIMEST": str(DT_FS), "OUTLOG": "100000",
+            "OUTCLE": "100000", "N_SIMD": "64"})
+        env = dict(os.environ, OMP_NUM_THREADS=str(cores), OMP_STACKSIZE="512M")
+        with open(os.path.join(tmp, "md.inp")) as fin, open(os.path.join(tmp, "md.log"), "w") as fout:
+            subprocess.run(["bash", "-c", f"ulimit -s unlimited; exec {exe}"], stdin=fin, stdout=fout,
+                           stderr=subprocess.STDOUT, cwd=tmp, env=env, check=True, timeout=900)
+        log = open(os.path.join(tmp, "md.log")).read()
+        speed = float(re.search(r"SPEED \(ns/day\)\s*:\s*([0-9.Ee+-]+)", log).group(1))
+        loop = float(re.search(r"MD LOOP\s*:\s*([0-9.Ee+-]+)", log).group(1))
+        return dict(value=speed, unit="ns/day", cores=cores, kind="reference",
+                    sample=f"md_run (flang -O2 build of the reference sources), {nsteps} MICR steps of this "
+                           f"workload, PARALV=DOUB N_SIMD=64, {cores} OpenMP threads, MD loop {loop:.2f} s")
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
+def traffic_from_profile():
+    """HBM bytes per pair-kernel launch from the committed rocprofv3 PMC
+    summary (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated), if present."""
+    p = os.path.join(ROOT, "profiles", "pair_kernel_water100k_traffic.json")
+    if os.path.exists(p):
+        d = json.load(open(p))
+        return d.get("bytes_per_launch")
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--fp64-steps", type=int, default=0)
+    ap.add_argument("--cpu-steps", type=int, default=20)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus})",
+              file=sys.stderr)
+        sys.exit(2)
+    rank = int(os.environ.get("RANK", "0"))
+    from j_presto_amd import engine, systems
+    s = systems.workload(2)
+    r32 = run_gpu(s, engine.FP32, args.steps, args.warmup)
+    n64 = args.fp64_steps or max(args.steps // 4, 100)
+    r64 = run_gpu(s, engine.FP64, n64, min(args.warmup, 50))
+    sps = args.steps / r32["wall"]
+    value = ns_per_day(sps)
+    v64 = ns_per_day(n64 / r64["wall"])
+    # pair kernel roofline
+    n_el = r32["pairs"] - r32["pairs_lj"]
+    flop = n_el * FLOP_EL + r32["pairs_lj"] * FLOP_LJEL
+    t_pair = r32["timing"]["pair"] / args.steps
+    achieved = flop / t_pair / 1e12
+    # list build: 64 B per atom + 4 B per unique listed pair, per rebuild
+    nreb = max(1, (args.steps - 1) // UPDATE)
+    t_list = r32["timing"]["list"] / nreb
+    list_bytes = 64 * s.n + 4 * r32["listed"]
+    et = r32["log"]["etot"]
+    out = {
+        "metric": "ns/day + pair-interactions/s, 100k-atom ZD box; FP32/fixed-pt vs FP64 GPU",
+        "value": round(value, 3),
+        "unit": "ns/day",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * r32["wall"] / args.steps, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": systems.WORKLOADS[2], "natom": s.n, "cutoff_A": 12.0, "dt_fs": DT_FS,
+                   "list_update": UPDATE, "energy_every": NMON, "precision": "FP32 pair math, int64 fixed-point "
+                   "accumulation, FP64 SHAKE/integration"},
+        "pair_interactions_per_s": round(r32["pairs"] * sps, 1),
+        "pairs_within_cutoff": r32["pairs"],
+        "fp64_gpu_ns_per_day": round(v64, 3),
+        "speedup_vs_fp64_gpu": round(value / v64, 3),
+        "nve_etot_span_kcal_mol": float(et.max() - et.min()) if len(et) else None,
+        "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 5),
+                     "traffic": traffic_from_profile(),
+                     "kernel": "k_pair_f32<false>", "avg_launch_us": round(t_pair * 1e6, 3),
+                     "flop_per_launch": flop},
+        "list_roofline": {"bound": "hbm", "achieved": round(list_bytes / t_list / 1e9, 2), "peak": PEAK_HBM_GBS,
+                          "unit": "GB/s", "frac": round(list_bytes / t_list / 1e9 / PEAK_HBM_GBS, 5),
+                          "bytes_per_rebuild": list_bytes, "avg_rebuild_us": round(t_list * 1e6, 2)},
+        "step_breakdown_ms": {"pair": round(1e3 * t_pair, 4), "list_per_step": round(1e3 * r32["timing"]["list"] / args.steps, 4),
+                              "rest": round(1e3 * r32["timing"]["step"] / args.steps, 4)},
+        "cpu_baseline": None,
+    }
+    if rank == 0 and not args.no_cpu:
+        cb = cpu_reference(s, args.cpu_steps)
+        out["cpu_baseline"] = cb
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

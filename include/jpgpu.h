@@ -133,9 +133,11 @@ int jp_run_nve(jp_ctx *ctx, int32_t nsteps, int32_t update, int32_t nmon, int32_
  * context stream: total, pair kernel, list build, SHAKE + integration. */
 int jp_last_timing(jp_ctx *ctx, double *t_total, double *t_pair, double *t_list, double *t_step);
 
-/* Number of unique pairs with r <= cutoff at the device coordinates
- * (pair-interactions/s numerator). */
-int jp_count_pairs(jp_ctx *ctx, int64_t *npairs);
+/* Unique pairs with r <= cutoff at the device coordinates (the
+ * pair-interactions/s numerator), those of them where both atoms carry LJ,
+ * and the unique pairs in the list (r <= cutoff+1.5 at the last rebuild).
+ * nlj and nlisted may be NULL. */
+int jp_count_pairs(jp_ctx *ctx, int64_t *npairs, int64_t *nlj, int64_t *nlisted);
 
 /*
  * Batched ensemble (replaces j_presto_mpi / j_presto_master: N independent
@@ -146,6 +148,7 @@ typedef struct jp_batch jp_batch;
 int jp_batch_create(const jp_sys_desc *descs, int32_t nsys, jp_batch **out);
 int jp_batch_run_nve(jp_batch *b, int32_t nsteps, int32_t update, int32_t nmon, int32_t precision,
                      jp_energy_log *log /* [nsys][nsteps/nmon] */);
+int jp_batch_set_state(jp_batch *b, int32_t isys, const double *cord, const double *vel);
 int jp_batch_get_state(jp_batch *b, int32_t isys, double *cord, double *vel);
 int jp_batch_last_timing(jp_batch *b, double *t_total);
 int jp_batch_destroy(jp_batch *b);

@@ -1,0 +1,933 @@
+// libjpgpu: C ABI (include/jpgpu.h) over the gfx950 kernels in jp_kernels.hpp.
+//
+// One Engine drives one or many systems ("batch"); a jp_ctx is an Engine of
+// one system, a jp_batch an Engine of many.  All systems of a batch share every
+// kernel launch: per-atom, per-slot and per-group arrays are concatenated and
+// each work item finds its system's parameters (SysP) through an index array.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/jpgpu.h"
+#include "jp_kernels.hpp"
+
+#define CK(x)                                                          \
+    do {                                                               \
+        hipError_t e_ = (x);                                           \
+        if (e_ != hipSuccess) {                                        \
+            fprintf(stderr, "jpgpu: %s failed: %s (%s:%d)\n", #x,      \
+                    hipGetErrorString(e_), __FILE__, __LINE__);        \
+            return JP_EHIP_BASE - (int)e_;                             \
+        }                                                              \
+    } while (0)
+
+namespace {
+
+template <typename T>
+struct DBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    int alloc(size_t m) {
+        if (m <= n && p) return 0;
+        if (p) hipFree(p);
+        p = nullptr;
+        n = 0;
+        if (m == 0) m = 1;
+        if (hipMalloc(&p, m * sizeof(T)) != hipSuccess) return JP_ENOMEM;
+        n = m;
+        return 0;
+    }
+    void release() {
+        if (p) hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+#define AL(buf, m)                          \
+    do {                                    \
+        int r_ = (buf).alloc(m);            \
+        if (r_) return r_;                  \
+    } while (0)
+
+inline unsigned grid1(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
+
+uint64_t morton3(uint32_t x, uint32_t y, uint32_t z) {
+    auto spread = [](uint64_t v) {
+        v &= 0x1fffff;
+        v = (v | v << 32) & 0x1f00000000ffffULL;
+        v = (v | v << 16) & 0x1f0000ff0000ffULL;
+        v = (v | v << 8) & 0x100f00f00f00f00fULL;
+        v = (v | v << 4) & 0x10c30c30c30c30c3ULL;
+        v = (v | v << 2) & 0x1249249249249249ULL;
+        return v;
+    };
+    return spread(x) | spread(y) << 1 | spread(z) << 2;
+}
+
+struct HostSys {
+    int n = 0, ntyp = 0, nfrag = 1, nfrg = 1, lmd = 0;
+    double lambda = 0.0, dself[3] = {0, 0, 0};
+    int nexcl = 0;
+    int n15mx = 0;
+};
+
+struct Engine {
+    int nsys = 0, natom = 0, nslot = 0, ngrp = 0, ncell = 0;
+    int n2 = 0, n3 = 0;  // SHAKE groups
+    int prec_built = -1;
+    bool have_list = false;
+    double shake_tol = 1e-6;
+    int shake_maxit = 1000;
+    std::vector<HostSys> hs;
+    std::vector<SysP> sp;
+    std::vector<int> atom_sys_h, slot_sys_h, grp_sys_h, gcnt_h, cell_sys_h;
+    std::vector<double> mass_h;
+    hipStream_t st = nullptr;
+    hipEvent_t ev[8];
+    double t_total = 0, t_pair = 0, t_list = 0, t_step = 0;
+    int n_pair = 0, n_list = 0;
+    long long list_entries = 0;
+
+    // device arrays
+    DBuf<SysP> d_sp;
+    DBuf<int> d_atom_sys, d_slot_sys, d_grp_sys, d_gcnt, d_cell_sys, d_crank;
+    DBuf<double> d_x, d_v, d_xold, d_vback, d_mass, d_imass, d_chg, d_grad, d_eexc, d_ekin;
+    DBuf<int> d_tflag, d_mol, d_xptr, d_xidx, d_eptr;
+    DBuf<int2> d_eidx;
+    DBuf<double> d_efac;
+    DBuf<double2> d_lj64;
+    DBuf<float2> d_lj32;
+    DBuf<int> d_img, d_cellid, d_refcell, d_cnt, d_start, d_fill, d_tmp, d_inum, d_perm, d_cslot;
+    DBuf<JAtom> d_ys;
+    DBuf<double> d_qd, d_bb;
+    DBuf<double4> d_xr;
+    DBuf<int4> d_imgs;
+    DBuf<int> d_lcount, d_loff, d_nrem, d_nb, d_err;
+    DBuf<Entry> d_ent;
+    DBuf<double> d_f64, d_ep64;
+    DBuf<long long> d_ffix, d_epfix;
+    DBuf<int> d_s2sys, d_s3sys;
+    DBuf<int4> d_s2a, d_s3a;
+    DBuf<double> d_s2d, d_s3d;
+    DBuf<unsigned long long> d_u64;
+
+    ~Engine() {
+        if (st) {
+            hipStreamSynchronize(st);
+            for (auto &e : ev) hipEventDestroy(e);
+            hipStreamDestroy(st);
+        }
+        d_sp.release(); d_atom_sys.release(); d_slot_sys.release(); d_grp_sys.release(); d_gcnt.release();
+        d_cell_sys.release(); d_crank.release(); d_x.release(); d_v.release(); d_xold.release(); d_vback.release();
+        d_mass.release(); d_imass.release(); d_chg.release(); d_grad.release(); d_eexc.release(); d_ekin.release();
+        d_tflag.release(); d_mol.release(); d_xptr.release(); d_xidx.release(); d_eptr.release(); d_eidx.release();
+        d_efac.release(); d_lj64.release(); d_lj32.release(); d_img.release(); d_cellid.release();
+        d_refcell.release(); d_cnt.release(); d_start.release(); d_fill.release(); d_tmp.release(); d_inum.release();
+        d_perm.release(); d_cslot.release(); d_ys.release(); d_qd.release(); d_bb.release(); d_xr.release();
+        d_imgs.release(); d_lcount.release(); d_loff.release(); d_nrem.release(); d_nb.release(); d_err.release();
+        d_ent.release(); d_f64.release(); d_ep64.release(); d_ffix.release(); d_epfix.release();
+        d_s2sys.release(); d_s3sys.release(); d_s2a.release(); d_s3a.release(); d_s2d.release(); d_s3d.release();
+        d_u64.release();
+    }
+
+    int init(const jp_sys_desc *descs, int ns);
+    int upload_sp() {
+        CK(hipMemcpyAsync(d_sp.p, sp.data(), sizeof(SysP) * nsys, hipMemcpyHostToDevice, st));
+        return 0;
+    }
+    int build_list();
+    int refresh_sorted();
+    int pair(int prec, bool energy);
+    int set_factors(int onehot_class);  // -1: run factors (lambda), k: one-hot class k
+    int nonbond(const double *cord, double *grad, double *eelec, double *evdw, int prec);
+    int energies(int prec, std::vector<double> &eel, std::vector<double> &evd, std::vector<double> &eex);
+    int shake(int &ier_out);
+    int run_nve(int nsteps, int update, int nmon, int prec, jp_energy_log *log);
+    int get_list(int32_t *cell3, int64_t *npairs, int32_t *ptr, int32_t *idx, int64_t cap);
+    int count_pairs(int64_t *np, int64_t *nlj, int64_t *nlist);
+};
+
+int Engine::init(const jp_sys_desc *descs, int ns) {
+    if (ns <= 0) return JP_EINVAL;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return JP_ENODEV;
+    CK(hipStreamCreate(&st));
+    for (auto &e : ev) CK(hipEventCreate(&e));
+    nsys = ns;
+    hs.resize(ns);
+    sp.resize(ns);
+    // host-side concatenation
+    std::vector<double> x, chg, mass, imass;
+    std::vector<int> tflag, mol, xptr{0}, xidx, eptr{0};
+    std::vector<int2> eidx;
+    std::vector<double2> lj64;
+    std::vector<float2> lj32;
+    std::vector<int> s2sys, s3sys;
+    std::vector<int4> s2a, s3a;
+    std::vector<double> s2d, s3d;
+    std::vector<int> crank;
+    int atom0 = 0, slot0 = 0, cell0 = 0, lj0 = 0;
+    shake_tol = descs[0].shake_tol > 0 ? descs[0].shake_tol : 1e-6;
+    shake_maxit = descs[0].shake_maxit > 0 ? descs[0].shake_maxit : 1000;
+    for (int s = 0; s < ns; ++s) {
+        const jp_sys_desc &d = descs[s];
+        HostSys &h = hs[s];
+        SysP &P = sp[s];
+        memset(&P, 0, sizeof(P));
+        const int n = d.natom, nt = d.ntyp;
+        if (n <= 0 || nt <= 0 || nt > 64 || d.nfrag < 1 || d.nfrag > 2 || d.nfrg < 1 || d.nfrg > 3) return JP_EINVAL;
+        h.n = n; h.ntyp = nt; h.nfrag = d.nfrag; h.nfrg = d.nfrg; h.lmd = d.lmd; h.lambda = d.lambda;
+        h.nexcl = d.nexcl;
+        // box (inpmd.f90:578-583, 764-768) and cell grids
+        double rc = d.cutoff, rlim = rc + 1.5;
+        for (int k = 0; k < 3; ++k) {
+            P.L[k] = d.celwal[2 * k + 1] - d.celwal[2 * k];
+            P.invL[k] = 1.0 / P.L[k];
+            P.wall[k] = d.celwal[2 * k];
+            if (!(P.L[k] >= 2.0 * rc)) return JP_EINVAL;  // inpmd.f90:694-698
+            int nc = std::max(1, (int)std::floor(P.L[k] / (rlim * 0.5)));
+            P.nc[k] = nc;
+            P.csz[k] = P.L[k] / nc;
+            P.icsz[k] = nc / P.L[k];
+        }
+        // md_run's cell numbering (incel.f90:29-57, mk_c, celpar.f90:93-108)
+        {
+            int nlev = d.cell_level > 0 ? d.cell_level : 3;
+            double cminsiz = rlim * 0.5, ext = 0.0;
+            for (int k = 0; k < 3; ++k) {
+                double lo = d.x[k], hi = d.x[k];
+                for (int i = 1; i < n; ++i) { lo = std::min(lo, d.x[3 * i + k]); hi = std::max(hi, d.x[3 * i + k]); }
+                ext = std::max(ext, hi - lo);
+            }
+            if ((double)(1 << (nlev + 1)) * cminsiz < ext) nlev = (int)std::ceil(std::log(ext / cminsiz) / std::log(2.0) - 1.0);
+            int nv = 4 << (nlev - 1);
+            double siz = cminsiz, r = -0.5 * (nv + 1), zz = siz * 0.5;
+            for (int k = 0; k < 3; ++k) {
+                double fxcbou = (d.celwal[2 * k] + d.celwal[2 * k + 1]) * 0.5;
+                double c11 = siz * (1.0 + r) + fxcbou;
+                P.rorg[k] = c11 - zz;
+            }
+            P.rr = 1.0 / siz;
+        }
+        P.rlim2 = rlim * rlim;
+        P.rc2 = rc * rc;
+        P.irlim2 = 1.0 / (rc * rc);
+        double rt = 1.0 / rc;
+        P.zcore = 1.5 * rt;
+        P.fcoeff = rt * rt * rt;
+        P.bcoeff = 0.5 * P.fcoeff;
+        P.dt = d.dt;
+        P.atom0 = atom0; P.natom = n;
+        int npad = (n + 63) / 64 * 64;
+        P.slot0 = slot0; P.nslot = npad;
+        P.grp0 = slot0 / 64; P.ngrp = npad / 64;
+        P.cell0 = cell0; P.ncell = P.nc[0] * P.nc[1] * P.nc[2];
+        P.lj0 = lj0; P.ntyp = nt;
+        h.n15mx = (int)std::ceil(0.15 * (4.0 / 3.0 * 3.14159265358979 * rc * rc * rc)) * 2;
+        // energy class of each cluster pair (n_matrix is column-major, 1-based)
+        for (int a = 0; a < 2; ++a)
+            for (int b = 0; b < 2; ++b) {
+                int aa = std::min(a, d.nfrag - 1), bb = std::min(b, d.nfrag - 1);
+                P.cls[a * 2 + b] = d.n_matrix[bb * d.nfrag + aa] - 1;
+                P.fac[a * 2 + b] = 1.0;
+            }
+        // Morton order of the box cells
+        {
+            int nx = P.nc[0], ny = P.nc[1], nz = P.nc[2];
+            std::vector<std::pair<uint64_t, int>> key;
+            key.reserve(P.ncell);
+            for (int cz = 0; cz < nz; ++cz)
+                for (int cy = 0; cy < ny; ++cy)
+                    for (int cx = 0; cx < nx; ++cx) key.push_back({morton3(cx, cy, cz), (cz * ny + cy) * nx + cx});
+            std::sort(key.begin(), key.end());
+            std::vector<int> rk(P.ncell);
+            for (int r = 0; r < P.ncell; ++r) rk[key[r].second] = cell0 + r;
+            crank.insert(crank.end(), rk.begin(), rk.end());
+            for (int r = 0; r < P.ncell; ++r) cell_sys_h.push_back(s);
+        }
+        // LJ table, AMBER combination (inptpl.f90:397-419)
+        std::vector<double> c6((size_t)nt * nt), c12((size_t)nt * nt);
+        for (int i = 0; i < nt; ++i)
+            for (int j = i; j < nt; ++j) {
+                double rtmp = std::sqrt(d.eps[i] * d.eps[j]);
+                double r2 = d.rmin2[i] + d.rmin2[j];
+                r2 = r2 * r2;
+                r2 = r2 * r2 * r2;
+                double a6 = rtmp * r2 * 2.0, a12 = rtmp * r2 * r2;
+                c6[i * nt + j] = c6[j * nt + i] = a6;
+                c12[i * nt + j] = c12[j * nt + i] = a12;
+            }
+        for (int k = 0; k < nt * nt; ++k) {
+            lj64.push_back(make_double2(c6[k], c12[k]));
+            lj32.push_back(make_float2((float)c6[k], (float)c12[k]));
+        }
+        std::vector<char> zt(nt);
+        for (int t = 0; t < nt; ++t) {   // input.f90:58-69
+            bool z = true;
+            for (int j = 0; j < nt; ++j)
+                if (std::fabs(c6[j * nt + t]) >= 1e-16 || std::fabs(c12[j * nt + t]) >= 1e-16) z = false;
+            zt[t] = z;
+        }
+        // atoms
+        double ccoef = 332.06378 / d.diel, fac = std::sqrt(ccoef);   // inpcls.f90:215-223
+        for (int i = 0; i < n; ++i) {
+            for (int k = 0; k < 3; ++k) x.push_back(d.x[3 * i + k]);
+            chg.push_back(fac * d.charge[i]);
+            mass.push_back(d.mass[i]);
+            imass.push_back(1.0 / d.mass[i]);
+            int t = d.atype[i] - 1, c = d.icls[i] - 1;
+            if (t < 0 || t >= nt || c < 0 || c >= d.nfrag) return JP_EINVAL;
+            tflag.push_back(t | ((zt[t] ? 0 : 1) << 12) | (c << 13));
+            atom_sys_h.push_back(s);
+        }
+        // molecules = connected components of the excluded-pair graph
+        std::vector<int> par(n);
+        std::iota(par.begin(), par.end(), 0);
+        auto find = [&](int a) { while (par[a] != a) { par[a] = par[par[a]]; a = par[a]; } return a; };
+        std::vector<std::vector<int>> ex(n);
+        for (int e = 0; e < d.nexcl; ++e) {
+            int a = d.excl[2 * e] - 1, b = d.excl[2 * e + 1] - 1;
+            if (a < 0 || b < 0 || a >= n || b >= n) return JP_EINVAL;
+            ex[a].push_back(b + atom0);
+            ex[b].push_back(a + atom0);
+            par[find(a)] = find(b);
+        }
+        for (int i = 0; i < n; ++i) {
+            mol.push_back(find(i) + atom0);
+            std::sort(ex[i].begin(), ex[i].end());
+            xidx.insert(xidx.end(), ex[i].begin(), ex[i].end());
+            xptr.push_back((int)xidx.size());
+        }
+        // excess pairs (incel.f90:629-688): per-atom partner lists in pair order
+        std::vector<std::vector<int2>> exs(n);
+        std::vector<double> dwork((size_t)n * 3, 0.0);
+        auto clsof = [&](int a, int b) { return d.n_matrix[(d.icls[b] - 1) * d.nfrag + (d.icls[a] - 1)] - 1; };
+        for (int i = 0; i < n; ++i) dwork[(size_t)i * 3 + clsof(i, i)] += d.charge[i];
+        for (int e = 0; e < d.nexcess; ++e) {
+            int a = d.excess[2 * e] - 1, b = d.excess[2 * e + 1] - 1;
+            if (a < 0 || b < 0 || a >= n || b >= n) return JP_EINVAL;
+            int k = clsof(a, b);
+            exs[a].push_back(make_int2(b + atom0, k));
+            exs[b].push_back(make_int2(a + atom0, k));
+            dwork[(size_t)a * 3 + k] += d.charge[b];
+            dwork[(size_t)b * 3 + k] += d.charge[a];
+        }
+        for (int i = 0; i < n; ++i) {
+            eidx.insert(eidx.end(), exs[i].begin(), exs[i].end());
+            eptr.push_back((int)eidx.size());
+        }
+        for (int k = 0; k < d.nfrg; ++k) {
+            double sum = 0.0;
+            for (int i = 0; i < n; ++i) sum = sum + d.charge[i] * dwork[(size_t)i * 3 + k];
+            h.dself[k] = -P.zcore * sum * 0.5 * ccoef;
+        }
+        // SHAKE groups
+        for (int g = 0; g < d.nshake; ++g) {
+            int na = d.shake_natom[g];
+            const int32_t *aa = d.shake_atoms + 4 * g;
+            int4 q = make_int4(aa[0] - 1 + atom0, aa[1] - 1 + atom0, na > 2 ? aa[2] - 1 + atom0 : -1, -1);
+            for (int k = 0; k < na; ++k)
+                if (aa[k] < 1 || aa[k] > n) return JP_EINVAL;
+            if (na == 2) {
+                s2a.push_back(q); s2sys.push_back(s);
+                for (int k = 0; k < 6; ++k) s2d.push_back(d.shake_dist2[6 * g + k]);
+            } else if (na == 3) {
+                s3a.push_back(q); s3sys.push_back(s);
+                for (int k = 0; k < 6; ++k) s3d.push_back(d.shake_dist2[6 * g + k]);
+            } else {
+                return JP_EINVAL;  // 4-atom groups (shake4) are not supported
+            }
+        }
+        for (int i = 0; i < npad; ++i) slot_sys_h.push_back(s);
+        for (int g = 0; g < npad / 64; ++g) {
+            grp_sys_h.push_back(s);
+            gcnt_h.push_back(std::min(64, n - g * 64));
+        }
+        atom0 += n;
+        slot0 += npad;
+        cell0 += P.ncell;
+        lj0 += nt * nt;
+    }
+    natom = atom0; nslot = slot0; ngrp = slot0 / 64; ncell = cell0;
+    n2 = (int)s2a.size(); n3 = (int)s3a.size();
+    mass_h = mass;
+    set_factors(-1);
+    // upload
+    AL(d_sp, ns); AL(d_atom_sys, natom); AL(d_slot_sys, nslot); AL(d_grp_sys, ngrp); AL(d_gcnt, ngrp);
+    AL(d_cell_sys, ncell); AL(d_crank, ncell);
+    AL(d_x, 3 * (size_t)natom); AL(d_v, 3 * (size_t)natom); AL(d_xold, 3 * (size_t)natom);
+    AL(d_vback, 3 * (size_t)natom); AL(d_mass, natom); AL(d_imass, natom); AL(d_chg, natom);
+    AL(d_grad, 3 * (size_t)natom); AL(d_eexc, 3 * (size_t)natom); AL(d_ekin, natom);
+    AL(d_tflag, natom); AL(d_mol, natom); AL(d_xptr, natom + 1); AL(d_xidx, xidx.size());
+    AL(d_eptr, natom + 1); AL(d_eidx, eidx.size()); AL(d_efac, 3 * (size_t)ns);
+    AL(d_lj64, lj64.size()); AL(d_lj32, lj32.size());
+    AL(d_img, 3 * (size_t)natom); AL(d_cellid, natom); AL(d_refcell, 3 * (size_t)natom);
+    AL(d_cnt, ncell + 1); AL(d_start, ncell + 1); AL(d_fill, ncell); AL(d_tmp, natom);
+    AL(d_inum, natom); AL(d_perm, nslot); AL(d_cslot, ncell);
+    AL(d_ys, nslot); AL(d_qd, nslot); AL(d_bb, 6 * (size_t)ngrp); AL(d_xr, nslot); AL(d_imgs, nslot);
+    AL(d_lcount, ngrp + 1); AL(d_loff, ngrp + 1); AL(d_nrem, ns); AL(d_nb, natom); AL(d_err, 1);
+    AL(d_f64, 3 * (size_t)nslot); AL(d_ffix, 3 * (size_t)nslot);
+    AL(d_ep64, 6 * (size_t)ngrp); AL(d_epfix, 6 * (size_t)ngrp);
+    AL(d_s2sys, n2); AL(d_s3sys, n3); AL(d_s2a, n2); AL(d_s3a, n3); AL(d_s2d, 6 * (size_t)n2); AL(d_s3d, 6 * (size_t)n3);
+    AL(d_u64, 3);
+    auto H2D = [&](void *dst, const void *src, size_t bytes) -> int {
+        if (bytes) CK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st));
+        return 0;
+    };
+    int r = 0;
+    r |= upload_sp();
+    r |= H2D(d_atom_sys.p, atom_sys_h.data(), 4 * atom_sys_h.size());
+    r |= H2D(d_slot_sys.p, slot_sys_h.data(), 4 * slot_sys_h.size());
+    r |= H2D(d_grp_sys.p, grp_sys_h.data(), 4 * grp_sys_h.size());
+    r |= H2D(d_gcnt.p, gcnt_h.data(), 4 * gcnt_h.size());
+    r |= H2D(d_cell_sys.p, cell_sys_h.data(), 4 * cell_sys_h.size());
+    r |= H2D(d_crank.p, crank.data(), 4 * crank.size());
+    r |= H2D(d_x.p, x.data(), 8 * x.size());
+    r |= H2D(d_mass.p, mass.data(), 8 * mass.size());
+    r |= H2D(d_imass.p, imass.data(), 8 * imass.size());
+    r |= H2D(d_chg.p, chg.data(), 8 * chg.size());
+    r |= H2D(d_tflag.p, tflag.data(), 4 * tflag.size());
+    r |= H2D(d_mol.p, mol.data(), 4 * mol.size());
+    r |= H2D(d_xptr.p, xptr.data(), 4 * xptr.size());
+    r |= H2D(d_xidx.p, xidx.data(), 4 * xidx.size());
+    r |= H2D(d_eptr.p, eptr.data(), 4 * eptr.size());
+    r |= H2D(d_eidx.p, eidx.data(), 8 * eidx.size());
+    r |= H2D(d_lj64.p, lj64.data(), 16 * lj64.size());
+    r |= H2D(d_lj32.p, lj32.data(), 8 * lj32.size());
+    r |= H2D(d_s2sys.p, s2sys.data(), 4 * s2sys.size());
+    r |= H2D(d_s3sys.p, s3sys.data(), 4 * s3sys.size());
+    r |= H2D(d_s2a.p, s2a.data(), 16 * s2a.size());
+    r |= H2D(d_s3a.p, s3a.data(), 16 * s3a.size());
+    r |= H2D(d_s2d.p, s2d.data(), 8 * s2d.size());
+    r |= H2D(d_s3d.p, s3d.data(), 8 * s3d.size());
+    if (r) return r;
+    CK(hipMemsetAsync(d_v.p, 0, 8 * 3 * (size_t)natom, st));
+    CK(hipStreamSynchronize(st));
+    return 0;
+}
+
+// Class factors: onehot < 0 -> the run's factors (LMD: lambda^2, lambda, 1 on
+// classes 1..3; verlet.f90:67-91), else 1 on class `onehot` and 0 elsewhere.
+int Engine::set_factors(int onehot) {
+    std::vector<double> ef(3 * (size_t)nsys, 0.0);
+    for (int s = 0; s < nsys; ++s) {
+        double f[3];
+        for (int k = 0; k < 3; ++k) {
+            if (onehot >= 0) f[k] = (k == onehot) ? 1.0 : 0.0;
+            else if (hs[s].lmd && hs[s].nfrg == 3) f[k] = (k == 0) ? hs[s].lambda * hs[s].lambda : (k == 1 ? hs[s].lambda : 1.0);
+            else f[k] = 1.0;
+        }
+        for (int q = 0; q < 4; ++q) sp[s].fac[q] = f[sp[s].cls[q]];
+        for (int k = 0; k < 3; ++k) ef[3 * s + k] = f[k];
+    }
+    if (!d_sp.p) return 0;
+    int r = upload_sp();
+    if (r) return r;
+    CK(hipMemcpyAsync(d_efac.p, ef.data(), 8 * ef.size(), hipMemcpyHostToDevice, st));
+    return 0;
+}
+
+// celprp: bin, sort, build the j-lists (two passes: count, scan, fill).
+int Engine::build_list() {
+    CK(hipEventRecord(ev[2], st));
+    CK(hipMemsetAsync(d_cnt.p, 0, 4 * (size_t)(ncell + 1), st));
+    CK(hipMemsetAsync(d_err.p, 0, 4, st));
+    hipLaunchKernelGGL(k_bin, grid1(natom, 256), 256, 0, st, natom, d_atom_sys.p, d_sp.p, d_x.p, d_crank.p, d_img.p,
+                       d_cellid.p, d_refcell.p, d_cnt.p, d_err.p);
+    hipLaunchKernelGGL(k_scan, 1, 1024, 0, st, ncell, d_cnt.p, d_start.p);
+    CK(hipMemsetAsync(d_fill.p, 0, 4 * (size_t)ncell, st));
+    hipLaunchKernelGGL(k_place, grid1(natom, 256), 256, 0, st, natom, d_cellid.p, d_start.p, d_fill.p, d_tmp.p);
+    CK(hipMemsetAsync(d_perm.p, 0xff, 4 * (size_t)nslot, st));
+    hipLaunchKernelGGL(k_rank, grid1(natom, 256), 256, 0, st, natom, d_atom_sys.p, d_sp.p, d_cellid.p, d_start.p,
+                       d_cnt.p, d_tmp.p, d_inum.p, d_perm.p);
+    hipLaunchKernelGGL(k_cslot, grid1(ncell, 256), 256, 0, st, ncell, d_cell_sys.p, d_sp.p, d_start.p, d_cslot.p);
+    hipLaunchKernelGGL(k_fill_sorted, grid1(nslot, 256), 256, 0, st, nslot, d_slot_sys.p, d_sp.p, d_perm.p, d_x.p,
+                       d_img.p, d_chg.p, d_tflag.p, d_ys.p, d_qd.p, d_xr.p, d_imgs.p);
+    hipLaunchKernelGGL(k_bbox, ngrp, 64, 0, st, ngrp, d_gcnt.p, d_ys.p, d_bb.p);
+    CK(hipMemsetAsync(d_nrem.p, 0, 4 * (size_t)nsys, st));
+    unsigned nb = (unsigned)((ngrp + 7) / 8 * 8);
+    hipLaunchKernelGGL(k_list<false>, nb, 64, 0, st, ngrp, d_grp_sys.p, d_gcnt.p, d_sp.p, d_bb.p, d_crank.p, d_cslot.p,
+                       d_cnt.p, d_xr.p, d_imgs.p, d_mol.p, d_xptr.p, d_xidx.p, d_lcount.p, d_loff.p, d_ent.p, 0,
+                       d_nrem.p, d_nb.p, d_err.p);
+    hipLaunchKernelGGL(k_scan, 1, 1024, 0, st, ngrp, d_lcount.p, d_loff.p);
+    int total = 0;
+    CK(hipMemcpyAsync(&total, d_loff.p + ngrp, 4, hipMemcpyDeviceToHost, st));
+    CK(hipStreamSynchronize(st));
+    if (total < 0) return JP_IER_LIST;
+    if ((size_t)total > d_ent.n) AL(d_ent, (size_t)(total * 1.25) + 1024);
+    list_entries = total;
+    hipLaunchKernelGGL(k_list<true>, nb, 64, 0, st, ngrp, d_grp_sys.p, d_gcnt.p, d_sp.p, d_bb.p, d_crank.p, d_cslot.p,
+                       d_cnt.p, d_xr.p, d_imgs.p, d_mol.p, d_xptr.p, d_xidx.p, d_lcount.p, d_loff.p, d_ent.p,
+                       (int)d_ent.n, d_nrem.p, d_nb.p, d_err.p);
+    CK(hipGetLastError());
+    CK(hipEventRecord(ev[3], st));
+    std::vector<int> nrem(nsys);
+    int err = 0;
+    CK(hipMemcpyAsync(nrem.data(), d_nrem.p, 4 * (size_t)nsys, hipMemcpyDeviceToHost, st));
+    CK(hipMemcpyAsync(&err, d_err.p, 4, hipMemcpyDeviceToHost, st));
+    std::vector<int> nbc(natom);
+    CK(hipMemcpyAsync(nbc.data(), d_nb.p, 4 * (size_t)natom, hipMemcpyDeviceToHost, st));
+    CK(hipStreamSynchronize(st));
+    float ms = 0;
+    hipEventElapsedTime(&ms, ev[2], ev[3]);
+    t_list += ms * 1e-3;
+    n_list++;
+    have_list = true;
+    if (err & 1) return JP_IER_CELL;
+    if (err & 2) return JP_EINVAL;
+    for (int s = 0; s < nsys; ++s) {
+        if (nrem[s] != 2 * hs[s].nexcl) return JP_IER_EXCL;      // cutoff.f90:90-96
+        for (int i = 0; i < hs[s].n; ++i)
+            if (nbc[sp[s].atom0 + i] > 2 * hs[s].n15mx) return JP_IER_LIST;  // cutoff.f90:30-46
+    }
+    return 0;
+}
+
+int Engine::refresh_sorted() {
+    hipLaunchKernelGGL(k_post, grid1(natom, 256), 256, 0, st, natom, d_atom_sys.p, d_sp.p, d_inum.p, d_img.p, d_x.p,
+                       d_xold.p, d_v.p, d_ys.p, 0);
+    if (prec_built >= 0) {
+        // qd does not change between rebuilds; nothing else to refresh
+    }
+    CK(hipGetLastError());
+    return 0;
+}
+
+int Engine::pair(int prec, bool energy) {
+    unsigned nb = (unsigned)((ngrp + 7) / 8 * 8);
+    int maxt = 0;
+    for (auto &h : hs) maxt = std::max(maxt, h.ntyp);
+    if (prec == JP_PREC_FP64) {
+        size_t lds = sizeof(double2) * maxt * maxt;
+        if (energy)
+            hipLaunchKernelGGL(k_pair_f64<true>, nb, 64, lds, st, ngrp, d_grp_sys.p, d_gcnt.p, d_sp.p, d_ys.p, d_qd.p,
+                               d_lj64.p, d_loff.p, d_lcount.p, d_ent.p, d_f64.p, d_ep64.p);
+        else
+            hipLaunchKernelGGL(k_pair_f64<false>, nb, 64, lds, st, ngrp, d_grp_sys.p, d_gcnt.p, d_sp.p, d_ys.p, d_qd.p,
+                               d_lj64.p, d_loff.p, d_lcount.p, d_ent.p, d_f64.p, d_ep64.p);
+    } else {
+        size_t lds = sizeof(float2) * maxt * maxt;
+        if (energy)
+            hipLaunchKernelGGL(k_pair_f32<true>, nb, 64, lds, st, ngrp, d_grp_sys.p, d_gcnt.p, d_sp.p, d_ys.p,
+                               d_lj32.p, d_loff.p, d_lcount.p, d_ent.p, d_ffix.p, d_epfix.p);
+        else
+            hipLaunchKernelGGL(k_pair_f32<false>, nb, 64, lds, st, ngrp, d_grp_sys.p, d_gcnt.p, d_sp.p, d_ys.p,
+                               d_lj32.p, d_loff.p, d_lcount.p, d_ent.p, d_ffix.p, d_epfix.p);
+    }
+    CK(hipGetLastError());
+    return 0;
+}
+
+// Per-system, per-class energies after a pair(prec, true) call.
+int Engine::energies(int prec, std::vector<double> &eel, std::vector<double> &evd, std::vector<double> &eex) {
+    eel.assign(3 * (size_t)nsys, 0.0);
+    evd.assign(3 * (size_t)nsys, 0.0);
+    eex.assign(3 * (size_t)nsys, 0.0);
+    hipLaunchKernelGGL(k_excess_energy, grid1(natom, 256), 256, 0, st, natom, d_atom_sys.p, d_sp.p, d_x.p, d_chg.p,
+                       d_eptr.p, d_eidx.p, d_eexc.p);
+    std::vector<double> ex(3 * (size_t)natom);
+    CK(hipMemcpyAsync(ex.data(), d_eexc.p, 8 * ex.size(), hipMemcpyDeviceToHost, st));
+    if (prec == JP_PREC_FP64) {
+        std::vector<double> ep(6 * (size_t)ngrp);
+        CK(hipMemcpyAsync(ep.data(), d_ep64.p, 8 * ep.size(), hipMemcpyDeviceToHost, st));
+        CK(hipStreamSynchronize(st));
+        for (int g = 0; g < ngrp; ++g) {
+            int s = grp_sys_h[g];
+            for (int k = 0; k < 3; ++k) { eel[3 * s + k] += ep[6 * g + k]; evd[3 * s + k] += ep[6 * g + 3 + k]; }
+        }
+    } else {
+        std::vector<long long> ep(6 * (size_t)ngrp);
+        CK(hipMemcpyAsync(ep.data(), d_epfix.p, 8 * ep.size(), hipMemcpyDeviceToHost, st));
+        CK(hipStreamSynchronize(st));
+        std::vector<long long> se(3 * (size_t)nsys, 0), sv(3 * (size_t)nsys, 0);
+        for (int g = 0; g < ngrp; ++g) {
+            int s = grp_sys_h[g];
+            for (int k = 0; k < 3; ++k) { se[3 * s + k] += ep[6 * g + k]; sv[3 * s + k] += ep[6 * g + 3 + k]; }
+        }
+        for (size_t q = 0; q < se.size(); ++q) { eel[q] = se[q] / JP_ESCALE; evd[q] = sv[q] / JP_ESCALE; }
+    }
+    for (int s = 0; s < nsys; ++s) {
+        for (int i = 0; i < hs[s].n; ++i) {
+            size_t a = sp[s].atom0 + i;
+            for (int k = 0; k < 3; ++k) eex[3 * s + k] += ex[3 * a + k];
+        }
+        for (int k = 0; k < 3; ++k) {
+            eel[3 * s + k] = eel[3 * s + k] * 0.5 + hs[s].dself[k] + eex[3 * s + k];   // enear.f90:568-571
+            evd[3 * s + k] = evd[3 * s + k] * 0.5;
+        }
+    }
+    return 0;
+}
+
+int Engine::nonbond(const double *cord, double *grad, double *eelec, double *evdw, int prec) {
+    int r;
+    if (cord) CK(hipMemcpyAsync(d_x.p, cord, 24 * (size_t)natom, hipMemcpyHostToDevice, st));
+    if (!have_list) {
+        if ((r = build_list())) return r;
+    } else if ((r = refresh_sorted())) {
+        return r;
+    }
+    int nfrg = hs[0].nfrg;
+    std::vector<double> eel, evd, eex;
+    std::vector<double> g(3 * (size_t)natom);
+    for (int k = 0; k < nfrg; ++k) {
+        if ((r = set_factors(nfrg == 1 ? -1 : k))) return r;
+        if ((r = pair(prec, k == 0))) return r;
+        if (k == 0 && (r = energies(prec, eel, evd, eex))) return r;
+        if (prec == JP_PREC_FP64)
+            hipLaunchKernelGGL(k_kick<0>, grid1(natom, 256), 256, 0, st, natom, d_atom_sys.p, d_sp.p, d_inum.p, d_f64.p,
+                               d_ffix.p, d_chg.p, d_eptr.p, d_eidx.p, d_efac.p, d_mass.p, d_x.p, d_v.p, d_xold.p,
+                               (double *)nullptr, d_grad.p, 0);
+        else
+            hipLaunchKernelGGL(k_kick<1>, grid1(natom, 256), 256, 0, st, natom, d_atom_sys.p, d_sp.p, d_inum.p, d_f64.p,
+                               d_ffix.p, d_chg.p, d_eptr.p, d_eidx.p, d_efac.p, d_mass.p, d_x.p, d_v.p, d_xold.p,
+                               (double *)nullptr, d_grad.p, 0);
+        CK(hipGetLastError());
+        if (grad) {
+            CK(hipMemcpyAsync(g.data(), d_grad.p, 8 * g.size(), hipMemcpyDeviceToHost, st));
+            CK(hipStreamSynchronize(st));
+            double *gk = grad + (size_t)k * 3 * natom;
+            for (size_t q = 0; q < g.size(); ++q) gk[q] += g[q];
+        }
+    }
+    if ((r = set_factors(-1))) return r;
+    for (int k = 0; k < nfrg; ++k) {
+        if (eelec) eelec[k] = eel[k];
+        if (evdw) evdw[k] = evd[k];
+    }
+    CK(hipStreamSynchronize(st));
+    return 0;
+}
+
+int Engine::shake(int &ier_out) {
+    ier_out = 0;
+    if (n2) hipLaunchKernelGGL(k_shake2, grid1(n2, 128), 128, 0, st, n2, d_s2sys.p, d_sp.p, d_s2a.p, d_s2d.p, d_mass.p,
+                               d_x.p, d_xold.p);
+    if (n3) hipLaunchKernelGGL(k_shake3, grid1(n3, 128), 128, 0, st, n3, d_s3sys.p, d_sp.p, d_s3a.p, d_s3d.p, d_imass.p,
+                               d_x.p, d_xold.p, shake_tol, shake_maxit, d_err.p);
+    CK(hipGetLastError());
+    return 0;
+}
+
+int Engine::run_nve(int nsteps, int update, int nmon, int prec, jp_energy_log *log) {
+    int r;
+    if (nsteps < 0 || update <= 0) return JP_EINVAL;
+    if (!have_list && (r = build_list())) return r;
+    if ((r = set_factors(-1))) return r;
+    if ((r = refresh_sorted())) return r;
+    CK(hipMemsetAsync(d_err.p, 0, 4, st));
+    const bool shake_on = (n2 + n3) > 0;
+    t_pair = 0; t_step = 0; n_pair = 0;
+    double tl0 = t_list;
+    int nl0 = n_list;
+    std::vector<hipEvent_t> pe;
+    pe.reserve(2 * (size_t)nsteps);
+    CK(hipEventRecord(ev[0], st));
+    int nlog = 0;
+    for (int step = 1; step <= nsteps; ++step) {
+        const bool mon = nmon > 0 && step % nmon == 0;
+        hipEvent_t a, b;
+        CK(hipEventCreate(&a));
+        CK(hipEventCreate(&b));
+        CK(hipEventRecord(a, st));
+        if ((r = pair(prec, mon))) return r;
+        CK(hipEventRecord(b, st));
+        pe.push_back(a);
+        pe.push_back(b);
+        std::vector<double> eel, evd, eex;
+        if (mon && (r = energies(prec, eel, evd, eex))) return r;
+        if (prec == JP_PREC_FP64)
+            hipLaunchKernelGGL(k_kick<0>, grid1(natom, 256), 256, 0, st, natom, d_atom_sys.p, d_sp.p, d_inum.p, d_f64.p,
+                               d_ffix.p, d_chg.p, d_eptr.p, d_eidx.p, d_efac.p, d_mass.p, d_x.p, d_v.p, d_xold.p,
+                               mon ? d_vback.p : (double *)nullptr, (double *)nullptr, 1);
+        else
+            hipLaunchKernelGGL(k_kick<1>, grid1(natom, 256), 256, 0, st, natom, d_atom_sys.p, d_sp.p, d_inum.p, d_f64.p,
+                               d_ffix.p, d_chg.p, d_eptr.p, d_eidx.p, d_efac.p, d_mass.p, d_x.p, d_v.p, d_xold.p,
+                               mon ? d_vback.p : (double *)nullptr, (double *)nullptr, 1);
+        int ier = 0;
+        if ((r = shake(ier))) return r;
+        hipLaunchKernelGGL(k_post, grid1(natom, 256), 256, 0, st, natom, d_atom_sys.p, d_sp.p, d_inum.p, d_img.p, d_x.p,
+                           d_xold.p, d_v.p, d_ys.p, shake_on ? 1 : 0);
+        CK(hipGetLastError());
+        if (mon) {
+            hipLaunchKernelGGL(k_ekin, grid1(natom, 256), 256, 0, st, natom, d_mass.p, d_v.p, d_vback.p, d_ekin.p);
+            std::vector<double> ek(natom);
+            int err = 0;
+            CK(hipMemcpyAsync(ek.data(), d_ekin.p, 8 * (size_t)natom, hipMemcpyDeviceToHost, st));
+            CK(hipMemcpyAsync(&err, d_err.p, 4, hipMemcpyDeviceToHost, st));
+            CK(hipStreamSynchronize(st));
+            if (err & 8) return JP_IER_SHAKE_SING;
+            if (err & 4) return JP_IER_SHAKE_CONV;
+            for (int s = 0; s < nsys; ++s) {
+                double e = 0.0;
+                for (int i = 0; i < hs[s].n; ++i) e = e + ek[sp[s].atom0 + i];
+                e = e * 1.0e7 / 4.184 * 0.0005;   // caltmp.f90:24-27
+                jp_energy_log &L = log[(size_t)s * (nsteps / nmon) + nlog];
+                L.step = step;
+                L.ekin = e;
+                double ep = 0.0;
+                double f[3];
+                for (int k = 0; k < 3; ++k) {
+                    f[k] = (hs[s].lmd && hs[s].nfrg == 3) ? (k == 0 ? hs[s].lambda * hs[s].lambda : (k == 1 ? hs[s].lambda : 1.0)) : 1.0;
+                    L.eelec[k] = k < hs[s].nfrg ? eel[3 * s + k] : 0.0;
+                    L.evdw[k] = k < hs[s].nfrg ? evd[3 * s + k] : 0.0;
+                    if (k < hs[s].nfrg) ep += f[k] * (eel[3 * s + k] + evd[3 * s + k]);
+                }
+                L.epot = ep;
+                L.etot = ep + e;
+            }
+            nlog++;
+        }
+        if (step % update == 0 && step < nsteps) {
+            if ((r = build_list())) return r;
+        }
+    }
+    CK(hipEventRecord(ev[1], st));
+    int err = 0;
+    CK(hipMemcpyAsync(&err, d_err.p, 4, hipMemcpyDeviceToHost, st));
+    CK(hipStreamSynchronize(st));
+    float ms = 0;
+    CK(hipEventElapsedTime(&ms, ev[0], ev[1]));
+    t_total = ms * 1e-3;
+    for (size_t q = 0; q < pe.size(); q += 2) {
+        float m2 = 0;
+        hipEventElapsedTime(&m2, pe[q], pe[q + 1]);
+        t_pair += m2 * 1e-3;
+        hipEventDestroy(pe[q]);
+        hipEventDestroy(pe[q + 1]);
+    }
+    n_pair = nsteps;
+    double tl = t_list - tl0;
+    t_step = t_total - t_pair - tl;
+    t_list = tl;
+    (void)nl0;
+    if (err & 8) return JP_IER_SHAKE_SING;
+    if (err & 4) return JP_IER_SHAKE_CONV;
+    return 0;
+}
+
+int Engine::get_list(int32_t *cell3, int64_t *npairs, int32_t *ptr, int32_t *idx, int64_t cap) {
+    if (!have_list) return JP_EINVAL;
+    CK(hipStreamSynchronize(st));
+    std::vector<int> lcount(ngrp), loff(ngrp + 1), perm(nslot);
+    std::vector<Entry> ent(list_entries);
+    CK(hipMemcpy(lcount.data(), d_lcount.p, 4 * (size_t)ngrp, hipMemcpyDeviceToHost));
+    CK(hipMemcpy(loff.data(), d_loff.p, 4 * (size_t)(ngrp + 1), hipMemcpyDeviceToHost));
+    CK(hipMemcpy(perm.data(), d_perm.p, 4 * (size_t)nslot, hipMemcpyDeviceToHost));
+    if (list_entries) CK(hipMemcpy(ent.data(), d_ent.p, sizeof(Entry) * list_entries, hipMemcpyDeviceToHost));
+    if (cell3) CK(hipMemcpy(cell3, d_refcell.p, 12 * (size_t)natom, hipMemcpyDeviceToHost));
+    std::vector<std::vector<int>> nb(natom);
+    for (int g = 0; g < ngrp; ++g)
+        for (int e = loff[g]; e < loff[g] + lcount[g]; ++e) {
+            int jsl = (int)(ent[e].js & JP_SLOT_MASK);
+            int aj = perm[jsl];
+            uint64_t m = ent[e].m;
+            while (m) {
+                int l = __builtin_ctzll(m);
+                m &= m - 1;
+                int ai = perm[g * 64 + l];
+                if (ai >= 0 && aj >= 0) nb[ai].push_back(aj);
+            }
+        }
+    int64_t tot = 0;
+    for (auto &v : nb) tot += (int64_t)v.size();
+    *npairs = tot;
+    if (!idx) return 0;
+    if (tot > cap) return JP_EINVAL;
+    int64_t w = 0;
+    for (int i = 0; i < natom; ++i) {
+        ptr[i] = (int32_t)w;
+        std::sort(nb[i].begin(), nb[i].end());
+        int base = sp[atom_sys_h[i]].atom0;
+        for (int j : nb[i]) idx[w++] = j - base + 1;
+    }
+    ptr[natom] = (int32_t)w;
+    return 0;
+}
+
+int Engine::count_pairs(int64_t *np, int64_t *nlj, int64_t *nlist) {
+    if (!have_list) return JP_EINVAL;
+    CK(hipMemsetAsync(d_u64.p, 0, 24, st));
+    hipLaunchKernelGGL(k_count_pairs, ngrp, 64, 0, st, ngrp, d_grp_sys.p, d_gcnt.p, d_sp.p, d_ys.p, d_loff.p,
+                       d_lcount.p, d_ent.p, d_u64.p);
+    unsigned long long c[3] = {0, 0, 0};
+    CK(hipMemcpyAsync(c, d_u64.p, 24, hipMemcpyDeviceToHost, st));
+    CK(hipStreamSynchronize(st));
+    if (np) *np = (int64_t)c[0];
+    if (nlj) *nlj = (int64_t)c[1];
+    if (nlist) *nlist = (int64_t)c[2];
+    return 0;
+}
+
+}  // namespace
+
+struct jp_ctx {
+    Engine e;
+};
+struct jp_batch {
+    Engine e;
+};
+
+extern "C" {
+
+const char *jp_version(void) { return "jpgpu 0.1 (gfx950)"; }
+
+int jp_ctx_create(const jp_sys_desc *desc, jp_ctx **out) {
+    if (!desc || !out) return JP_EINVAL;
+    jp_ctx *c = new jp_ctx();
+    int r = c->e.init(desc, 1);
+    if (r) {
+        delete c;
+        *out = nullptr;
+        return r;
+    }
+    *out = c;
+    return 0;
+}
+
+int jp_ctx_destroy(jp_ctx *ctx) {
+    delete ctx;
+    return 0;
+}
+
+int jp_set_state(jp_ctx *ctx, const double *cord, const double *vel) {
+    if (!ctx) return JP_EINVAL;
+    Engine &E = ctx->e;
+    if (cord) CK(hipMemcpyAsync(E.d_x.p, cord, 24 * (size_t)E.natom, hipMemcpyHostToDevice, E.st));
+    if (vel) CK(hipMemcpyAsync(E.d_v.p, vel, 24 * (size_t)E.natom, hipMemcpyHostToDevice, E.st));
+    if (cord && E.have_list) {
+        int r = E.refresh_sorted();
+        if (r) return r;
+    }
+    CK(hipStreamSynchronize(E.st));
+    return 0;
+}
+
+int jp_get_state(jp_ctx *ctx, double *cord, double *vel) {
+    if (!ctx) return JP_EINVAL;
+    Engine &E = ctx->e;
+    if (cord) CK(hipMemcpyAsync(cord, E.d_x.p, 24 * (size_t)E.natom, hipMemcpyDeviceToHost, E.st));
+    if (vel) CK(hipMemcpyAsync(vel, E.d_v.p, 24 * (size_t)E.natom, hipMemcpyDeviceToHost, E.st));
+    CK(hipStreamSynchronize(E.st));
+    return 0;
+}
+
+int jp_build_list(jp_ctx *ctx, const double *cord) {
+    if (!ctx) return JP_EINVAL;
+    Engine &E = ctx->e;
+    if (cord) CK(hipMemcpyAsync(E.d_x.p, cord, 24 * (size_t)E.natom, hipMemcpyHostToDevice, E.st));
+    return E.build_list();
+}
+
+int jp_get_list(jp_ctx *ctx, int32_t *cell_of_atom, int64_t *npairs, int32_t *ptr, int32_t *idx, int64_t cap) {
+    if (!ctx || !npairs) return JP_EINVAL;
+    return ctx->e.get_list(cell_of_atom, npairs, ptr, idx, cap);
+}
+
+int jp_nonbond(jp_ctx *ctx, const double *cord, double *grad, double *eelec, double *evdw, double *ehyd, int precision) {
+    if (!ctx || (precision != JP_PREC_FP64 && precision != JP_PREC_FP32_FIXED)) return JP_EINVAL;
+    int r = ctx->e.nonbond(cord, grad, eelec, evdw, precision);
+    if (ehyd)
+        for (int k = 0; k < ctx->e.hs[0].nfrg; ++k) ehyd[k] = 0.0;   // no 12-10 H-bond pairs (iyppid != 2)
+    return r;
+}
+
+int jp_shake(jp_ctx *ctx, double *cord, const double *cord_old) {
+    if (!ctx || !cord || !cord_old) return JP_EINVAL;
+    Engine &E = ctx->e;
+    CK(hipMemcpyAsync(E.d_x.p, cord, 24 * (size_t)E.natom, hipMemcpyHostToDevice, E.st));
+    CK(hipMemcpyAsync(E.d_xold.p, cord_old, 24 * (size_t)E.natom, hipMemcpyHostToDevice, E.st));
+    CK(hipMemsetAsync(E.d_err.p, 0, 4, E.st));
+    int ier = 0, r = E.shake(ier);
+    if (r) return r;
+    int err = 0;
+    CK(hipMemcpyAsync(cord, E.d_x.p, 24 * (size_t)E.natom, hipMemcpyDeviceToHost, E.st));
+    CK(hipMemcpyAsync(&err, E.d_err.p, 4, hipMemcpyDeviceToHost, E.st));
+    CK(hipStreamSynchronize(E.st));
+    if (err & 8) return JP_IER_SHAKE_SING;
+    if (err & 4) return JP_IER_SHAKE_CONV;
+    return 0;
+}
+
+int jp_run_nve(jp_ctx *ctx, int32_t nsteps, int32_t update, int32_t nmon, int32_t precision, jp_energy_log *log) {
+    if (!ctx || (precision != JP_PREC_FP64 && precision != JP_PREC_FP32_FIXED)) return JP_EINVAL;
+    if (nmon > 0 && !log) return JP_EINVAL;
+    return ctx->e.run_nve(nsteps, update, nmon, precision, log);
+}
+
+int jp_last_timing(jp_ctx *ctx, double *t_total, double *t_pair, double *t_list, double *t_step) {
+    if (!ctx) return JP_EINVAL;
+    if (t_total) *t_total = ctx->e.t_total;
+    if (t_pair) *t_pair = ctx->e.t_pair;
+    if (t_list) *t_list = ctx->e.t_list;
+    if (t_step) *t_step = ctx->e.t_step;
+    return 0;
+}
+
+int jp_count_pairs(jp_ctx *ctx, int64_t *npairs, int64_t *nlj, int64_t *nlisted) {
+    if (!ctx || !npairs) return JP_EINVAL;
+    return ctx->e.count_pairs(npairs, nlj, nlisted);
+}
+
+int jp_batch_create(const jp_sys_desc *descs, int32_t nsys, jp_batch **out) {
+    if (!descs || !out || nsys <= 0) return JP_EINVAL;
+    jp_batch *b = new jp_batch();
+    int r = b->e.init(descs, nsys);
+    if (r) {
+        delete b;
+        *out = nullptr;
+        return r;
+    }
+    *out = b;
+    return 0;
+}
+
+int jp_batch_run_nve(jp_batch *b, int32_t nsteps, int32_t update, int32_t nmon, int32_t precision, jp_energy_log *log) {
+    if (!b || (precision != JP_PREC_FP64 && precision != JP_PREC_FP32_FIXED)) return JP_EINVAL;
+    if (nmon > 0 && !log) return JP_EINVAL;
+    return b->e.run_nve(nsteps, update, nmon, precision, log);
+}
+
+int jp_batch_get_state(jp_batch *b, int32_t isys, double *cord, double *vel) {
+    if (!b || isys < 0 || isys >= b->e.nsys) return JP_EINVAL;
+    Engine &E = b->e;
+    size_t off = 3 * (size_t)E.sp[isys].atom0, n = 3 * (size_t)E.sp[isys].natom;
+    if (cord) CK(hipMemcpyAsync(cord, E.d_x.p + off, 8 * n, hipMemcpyDeviceToHost, E.st));
+    if (vel) CK(hipMemcpyAsync(vel, E.d_v.p + off, 8 * n, hipMemcpyDeviceToHost, E.st));
+    CK(hipStreamSynchronize(E.st));
+    return 0;
+}
+
+int jp_batch_set_state(jp_batch *b, int32_t isys, const double *cord, const double *vel) {
+    if (!b || isys < 0 || isys >= b->e.nsys) return JP_EINVAL;
+    Engine &E = b->e;
+    size_t off = 3 * (size_t)E.sp[isys].atom0, n = 3 * (size_t)E.sp[isys].natom;
+    if (cord) CK(hipMemcpyAsync(E.d_x.p + off, cord, 8 * n, hipMemcpyHostToDevice, E.st));
+    if (vel) CK(hipMemcpyAsync(E.d_v.p + off, vel, 8 * n, hipMemcpyHostToDevice, E.st));
+    if (cord && E.have_list) {
+        int r = E.refresh_sorted();
+        if (r) return r;
+    }
+    CK(hipStreamSynchronize(E.st));
+    return 0;
+}
+
+int jp_batch_last_timing(jp_batch *b, double *t_total) {
+    if (!b) return JP_EINVAL;
+    if (t_total) *t_total = b->e.t_total;
+    return 0;
+}
+
+int jp_batch_destroy(jp_batch *b) {
+    delete b;
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,198 @@
+"""Host mirror of md_run's call surface for the ZD path, over libjpgpu.
+
+Same routine names, argument meaning and error behaviour as the reference
+(ier codes are returned, not raised):
+
+  MdSystem.celprp()          celprp.f90:2      (bin + pair list; ier 0/28/51/99)
+  MdSystem.cmmdo()           cmmdo.f90:2       (ZD+LJ pair terms, self, excess)
+  MdSystem.dnergy()          dnergy.f90:2      (nonbonded energy surface:
+                                                 eneval(15), enevcc(15,nfrg), grad)
+  MdSystem.shake(r1, r2)     shake.f90:2       (ier 0/1/-2)
+  MdSystem.runmd(n, ...)     md.f90:472 + verlet.f90:2 (METHOD= MICR)
+  Ensemble                   j_presto_mpi:27-44 (N independent runs) as one
+                             batched launch
+
+Energy slots follow main.f90:157-159: 1 TOTAL, 8 VDW15, 9 ELE15, 10 HYD.
+Runtime (HIP) failures raise native.JpError.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+from . import native
+from .abi import DescHolder, EnergyLog, SysDesc
+from .systems import System
+
+FP64 = 0
+FP32 = 1
+MAXENE = 15
+
+
+def _f64(a):
+    return a.ctypes.data_as(native.f64p) if a is not None else None
+
+
+class MdSystem:
+    """One system on the GPU (a jp_ctx)."""
+
+    def __init__(self, system: System, dt: float = 2.0, precision: int = FP64, **desc_kw):
+        self.lib = native.load()
+        self.system = system
+        self.h = DescHolder(system, dt=dt, **desc_kw)
+        self.n = self.h.n
+        self.nfrg = self.h.nfrg
+        self.precision = precision
+        p = C.c_void_p()
+        native.check("jp_ctx_create", self.lib.jp_ctx_create(C.byref(self.h.desc), C.byref(p)))
+        self.ctx = p
+        v = np.ascontiguousarray(system.v, np.float64)
+        self.set_state(None, v)
+
+    def close(self):
+        if getattr(self, "ctx", None):
+            self.lib.jp_ctx_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- state ---------------------------------------------------------
+    def set_state(self, cord=None, vel=None):
+        cord = None if cord is None else np.ascontiguousarray(cord, np.float64)
+        vel = None if vel is None else np.ascontiguousarray(vel, np.float64)
+        native.check("jp_set_state", self.lib.jp_set_state(self.ctx, _f64(cord), _f64(vel)))
+
+    def get_state(self):
+        x = np.zeros((self.n, 3))
+        v = np.zeros((self.n, 3))
+        native.check("jp_get_state", self.lib.jp_get_state(self.ctx, _f64(x), _f64(v)))
+        return x, v
+
+    # -- reference routines --------------------------------------------
+    def celprp(self, cord=None) -> int:
+        cord = None if cord is None else np.ascontiguousarray(cord, np.float64)
+        return native.check("jp_build_list", self.lib.jp_build_list(self.ctx, _f64(cord)))
+
+    def get_list(self):
+        """(cell (N,3) md_run numbering, ptr (N+1), idx 1-based ascending)."""
+        n = C.c_int64()
+        native.check("jp_get_list", self.lib.jp_get_list(self.ctx, None, C.byref(n), None, None, 0))
+        cell = np.zeros((self.n, 3), np.int32)
+        ptr = np.zeros(self.n + 1, np.int32)
+        idx = np.zeros(max(n.value, 1), np.int32)
+        native.check("jp_get_list", self.lib.jp_get_list(
+            self.ctx, cell.ctypes.data_as(native.i32p), C.byref(n), ptr.ctypes.data_as(native.i32p),
+            idx.ctypes.data_as(native.i32p), len(idx)))
+        return cell, ptr, idx[:n.value]
+
+    def cmmdo(self, cord=None, grad=None, precision: Optional[int] = None):
+        """Returns (grad(nfrg,N,3) accumulated, eelecn, evancn, ehydcn, ier)."""
+        prec = self.precision if precision is None else precision
+        cord = None if cord is None else np.ascontiguousarray(cord, np.float64)
+        g = np.zeros((self.nfrg, self.n, 3)) if grad is None else grad
+        ee, ev, eh = np.zeros(self.nfrg), np.zeros(self.nfrg), np.zeros(self.nfrg)
+        ier = native.check("jp_nonbond", self.lib.jp_nonbond(self.ctx, _f64(cord), _f64(g), _f64(ee), _f64(ev),
+                                                              _f64(eh), prec))
+        return g, ee, ev, eh, ier
+
+    def dnergy(self, cord=None, precision: Optional[int] = None):
+        """Nonbonded part of dnergy: (eneval(15), enevcc(15,nfrg), grad, ier)
+        with slot 8 VDW15, 9 ELE15, 10 HYD and slot 1 their sum
+        (dnergy.f90:77-79; bonded terms are not part of this path)."""
+        g, ee, ev, eh, ier = self.cmmdo(cord, precision=precision)
+        enevcc = np.zeros((MAXENE, self.nfrg))
+        enevcc[8] = ee
+        enevcc[7] = ev
+        enevcc[9] = eh
+        enevcc[0] = enevcc[1:].sum(0)
+        return enevcc.sum(1), enevcc, g, ier
+
+    def shake(self, r1, r2) -> int:
+        """Constrain r1 (x(t+dt)) in place against r2 (x(t))."""
+        assert r1.flags.c_contiguous and r1.dtype == np.float64
+        r2 = np.ascontiguousarray(r2, np.float64)
+        return native.check("jp_shake", self.lib.jp_shake(self.ctx, _f64(r1), _f64(r2)))
+
+    def runmd(self, nsteps: int, update: int = 10, nmon: int = 0, precision: Optional[int] = None):
+        """METHOD= MICR loop on the device state.  Returns (log, ier)."""
+        prec = self.precision if precision is None else precision
+        nl = nsteps // nmon if nmon > 0 else 0
+        log = (EnergyLog * max(nl, 1))()
+        ier = native.check("jp_run_nve", self.lib.jp_run_nve(self.ctx, nsteps, update, nmon, prec, log))
+        return log_to_array(log, nl), ier
+
+    def timing(self):
+        t = [C.c_double() for _ in range(4)]
+        self.lib.jp_last_timing(self.ctx, *[C.byref(x) for x in t])
+        return dict(total=t[0].value, pair=t[1].value, list=t[2].value, step=t[3].value)
+
+    def count_pairs(self, detail: bool = False):
+        """Unique pairs with r <= rc; with detail, (all, both-LJ, listed)."""
+        n, nlj, nl = C.c_int64(), C.c_int64(), C.c_int64()
+        native.check("jp_count_pairs", self.lib.jp_count_pairs(self.ctx, C.byref(n), C.byref(nlj), C.byref(nl)))
+        return (n.value, nlj.value, nl.value) if detail else n.value
+
+
+def log_to_array(log, nl: int) -> np.ndarray:
+    dt = np.dtype([("step", "i4"), ("etot", "f8"), ("epot", "f8"), ("ekin", "f8"),
+                   ("eelec", "f8", 3), ("evdw", "f8", 3)])
+    out = np.zeros(nl, dt)
+    for i in range(nl):
+        L = log[i]
+        out[i] = (L.step, L.etot, L.epot, L.ekin, tuple(L.eelec), tuple(L.evdw))
+    return out
+
+
+class Ensemble:
+    """Many independent systems of uneven size advanced by one set of
+    batched launches (replaces one md_run process per system)."""
+
+    def __init__(self, systems: Sequence[System], dt: float = 2.0, precision: int = FP32, **desc_kw):
+        self.lib = native.load()
+        self.holders = [DescHolder(s, dt=dt, **desc_kw) for s in systems]
+        arr = (SysDesc * len(systems))(*[h.desc for h in self.holders])
+        self._arr = arr
+        self.nsys = len(systems)
+        self.precision = precision
+        p = C.c_void_p()
+        native.check("jp_batch_create", self.lib.jp_batch_create(arr, self.nsys, C.byref(p)))
+        self.b = p
+        for i, s in enumerate(systems):
+            v = np.ascontiguousarray(s.v, np.float64)
+            native.check("jp_batch_set_state", self.lib.jp_batch_set_state(self.b, i, None, _f64(v)))
+
+    def close(self):
+        if getattr(self, "b", None):
+            self.lib.jp_batch_destroy(self.b)
+            self.b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def runmd(self, nsteps: int, update: int = 10, nmon: int = 0, precision: Optional[int] = None):
+        prec = self.precision if precision is None else precision
+        nl = nsteps // nmon if nmon > 0 else 0
+        log = (EnergyLog * max(nl * self.nsys, 1))()
+        ier = native.check("jp_batch_run_nve", self.lib.jp_batch_run_nve(self.b, nsteps, update, nmon, prec, log))
+        full = log_to_array(log, nl * self.nsys)
+        return full.reshape(self.nsys, nl) if nl else full, ier
+
+    def get_state(self, i: int):
+        n = self.holders[i].n
+        x, v = np.zeros((n, 3)), np.zeros((n, 3))
+        native.check("jp_batch_get_state", self.lib.jp_batch_get_state(self.b, i, _f64(x), _f64(v)))
+        return x, v
+
+    def timing(self) -> float:
+        t = C.c_double()
+        self.lib.jp_batch_last_timing(self.b, C.byref(t))
+        return t.value

@@ -179,7 +179,8 @@ def write_md_inp(path: str, files: Dict[str, str], s: System, md: Dict[str, str]
     if files.get("SETSHK"):
         L.append(f"      SETSHK= {files['SETSHK']}")
     if s.cluster_method == "LMD":
-        L.append("      CLUSTA= CHAIN:*")
+        na = int((s.cluster == 1).sum())
+        L.append(f"      CLUSTA= 1<{na}")
         L.append("      GEPSME= LMD")
     L += ["QUIT", "", "EXE> MD"]
     box = s.celwal

@@ -1,0 +1,127 @@
+"""Writes the golden fixtures under tests/golden/ by running the compiled
+reference md_run (oracle/_ref/md_run, built from /root/reference by
+oracle/build_ref.sh) on synthetic systems.
+
+For every system two md_run jobs run on one OpenMP thread with PARALV= DOUB
+and N_SIMD= 1 (a fixed summation order):
+  force job  RESTAR= YES, METHOD= CANO, FDEBUG= MAKE: md_run evaluates dnergy
+             at the restart coordinates and dumps epot(15) and grad(3,N)
+             (verlethe.f90:378-384) -- the FP64 gradient and energies at
+             exactly the stored coordinates;
+  traj job   RESTAR= YES, METHOD= MICR, LOOPLI= K: one restart step plus K
+             leap-frog steps with SHAKE; the output restart holds x and v in
+             FP64 and NAMETO the potential energy of every step.
+A fixture stores the generator recipe, the input x and v, and the outputs.
+The reference itself never leaves this container.
+
+Usage: python tests/golden/make_golden.py [name ...]
+"""
+from __future__ import annotations
+
+import hashlib
+import json
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+
+from j_presto_amd import refio, systems  # noqa: E402
+
+MD_RUN = os.path.join(ROOT, "oracle", "_ref", "md_run")
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+RECIPES = {
+    # small box just above 2*cutoff (box < 2*rlim: periodic images inside the list radius)
+    "water_1500": ("water_box", dict(nwat=500, L=24.8, seed=7), dict(traj=20)),
+    # the 6k-atom water workload itself
+    "water_6k": ("workload", dict(i=0), dict(traj=20)),
+    # chain with 16 LJ types, 2-atom SHAKE groups, chain exclusions
+    "chain_3k": ("chain_in_water", dict(n_chain=200, L=30.0, seed=11), dict(traj=20)),
+    # LMD: three energy classes, lambda-scaled gradient
+    "chain_lmd": ("chain_in_water", dict(n_chain=160, L=28.0, seed=12, lmd=0.7), dict(traj=10)),
+    # ions with their own LJ types
+    "ions_3k": ("water_box", dict(nwat=900, L=30.0, seed=13, n_ion_pairs=10), dict(traj=0)),
+    # box edge a multiple of rlim = 13.5 A (md_run's phantom-cell case)
+    "water_54": ("water_box", dict(nwat=5200, L=54.0, seed=14), dict(traj=0)),
+}
+
+
+def build_system(kind: str, args: dict) -> systems.System:
+    if kind == "workload":
+        return systems.workload(args["i"])
+    return getattr(systems, kind)(**args)
+
+
+def fingerprint(s: systems.System) -> str:
+    h = hashlib.sha256()
+    for a in (s.x, s.v, s.charge, s.mass, s.atype.astype(np.int64)):
+        h.update(np.ascontiguousarray(a).tobytes())
+    h.update(json.dumps([(t.rmin2, t.eps) for t in s.types]).encode())
+    return h.hexdigest()
+
+
+def run_md(workdir: str, s: systems.System, md: dict, shake: bool = True) -> str:
+    files = refio.write_inputs(workdir, s, shake=shake)
+    refio.write_md_inp(os.path.join(workdir, "md.inp"), files, s, md)
+    env = dict(os.environ, OMP_NUM_THREADS="1")
+    with open(os.path.join(workdir, "md.inp")) as fin, open(os.path.join(workdir, "md.log"), "w") as fout:
+        subprocess.run([MD_RUN], stdin=fin, stdout=fout, stderr=subprocess.STDOUT, cwd=workdir, env=env,
+                       check=True, timeout=3600)
+    return open(os.path.join(workdir, "md.log")).read()
+
+
+def make(name: str) -> str:
+    kind, args, opt = RECIPES[name]
+    s = build_system(kind, args)
+    out = dict(recipe=json.dumps(dict(kind=kind, args=args)), fingerprint=fingerprint(s),
+               x=s.x, v=s.v, box=s.box)
+    tmp = tempfile.mkdtemp(prefix="jpgold_")
+    try:
+        refio.write_restart(os.path.join(tmp, "in.rst"), s.x, s.v, lam=s.lam)
+        log = run_md(tmp, s, {"METHOD": "CANO", "RESTAR": "YES", "NAMERI": "in.rst", "NAMERO": "out.rst",
+                              "LOOPLI": "1", "FDEBUG": "MAKE", "SETTEM": "300.0"})
+        dump = refio.read_force_dump(os.path.join(tmp, "1stp_force.bin"), s.n)
+        out["force_grad"] = dump["grad"]
+        out["force_epot"] = dump["epot"]
+        ntraj = opt.get("traj", 0)
+        if ntraj:
+            t2 = os.path.join(tmp, "traj")
+            os.makedirs(t2)
+            refio.write_restart(os.path.join(t2, "in.rst"), s.x, s.v, lam=s.lam)
+            log = run_md(t2, s, {"METHOD": "MICR", "RESTAR": "YES", "NAMERI": "in.rst", "NAMERO": "out.rst",
+                                 "LOOPLI": str(ntraj), "UPDATE": "10", "TIMEST": "2.0",
+                                 "NAMETO": "ene.dat", "OUTLOG": "100000",
+                                 # flang evaluates both operands of `idmoce.gt.0 .and. mod(idloop,idmoce)`
+                                 # (md.f90:604): a positive OUTCLE keeps mod() away from zero
+                                 "OUTCLE": "100000"})
+            r = refio.read_restart(os.path.join(t2, "out.rst"))
+            out["traj_nsteps"] = np.int64(ntraj)
+            out["traj_x"] = r["x"]
+            out["traj_v"] = r["v"]
+            out["traj_loop"] = np.int64(r["loop"])
+            ene = np.loadtxt(os.path.join(t2, "ene.dat"), ndmin=2)
+            out["traj_epot"] = ene
+        path = os.path.join(HERE, f"{name}.npz")
+        np.savez_compressed(path, **out)
+        return path
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
+def main(argv):
+    if not os.path.exists(MD_RUN):
+        subprocess.run(["bash", os.path.join(ROOT, "oracle", "build_ref.sh")], check=True)
+    names = argv or list(RECIPES)
+    for n in names:
+        p = make(n)
+        print(n, "->", os.path.relpath(p, ROOT), os.path.getsize(p), "bytes", flush=True)
+
+
+if __name__ == "__main__":
+    main(sys.argv[1:])

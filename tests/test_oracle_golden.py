@@ -1,0 +1,67 @@
+"""The CPU oracle against the compiled reference md_run (golden fixtures
+made by tests/golden/make_golden.py): bit-exact forces, energies, pair-list
+sets and NVE trajectories."""
+import numpy as np
+import pytest
+
+from conftest import GOLDEN_NAMES
+from oracle import Oracle
+
+
+@pytest.mark.parametrize("name", GOLDEN_NAMES)
+def test_forces_and_energies_bit_exact(golden, name):
+    s, f = golden(name)
+    o = Oracle(s)
+    assert o.celprp(s.x) == 0
+    g, ee, ev = o.nonbond(s.x)
+    gt = o.combine(g)
+    assert np.array_equal(gt, f["force_grad"]), np.abs(gt - f["force_grad"]).max()
+    # dnergy slots: 9 ELE15, 8 VDW15 (main.f90:157-159), unscaled class sums
+    assert ee.sum() == f["force_epot"][8]
+    assert ev.sum() == f["force_epot"][7]
+
+
+@pytest.mark.parametrize("name", GOLDEN_NAMES)
+def test_cell_list_is_bruteforce_set(golden, name):
+    s, f = golden(name)
+    o = Oracle(s)
+    assert o.celprp(s.x) == 0
+    cell, ptr, idx = o.get_list()
+    bp, bi = o.list_brute(s.x)
+    assert np.array_equal(ptr, bp) and np.array_equal(idx, bi)
+
+
+@pytest.mark.parametrize("name", [n for n in GOLDEN_NAMES if n not in ("ions_3k", "water_54")])
+def test_nve_trajectory_bit_exact(golden, name):
+    """md_run METHOD= MICR from a restart: K steps of force, kick, drift,
+    SHAKE, v=(x-x0)/dt, list rebuilt after every 10th step (md.f90:519-694;
+    inirst restores x and v after its check step, md.f90:1241-1256)."""
+    s, f = golden(name)
+    o = Oracle(s)
+    x, v = s.x.copy(), s.v.copy()
+    assert o.celprp(x) == 0
+    for idloop in range(1, int(f["traj_nsteps"]) + 1):
+        g, ee, ev = o.nonbond(x)
+        assert o.verlet_step(x, v, o.combine(g)) == 0
+        if idloop % 10 == 0:
+            assert o.celprp(x) == 0
+    assert np.array_equal(x, f["traj_x"])
+    assert np.array_equal(v, f["traj_v"])
+
+
+def test_potential_energy_series(golden):
+    """NAMETO prints E_pot of every step with 8 significant digits."""
+    s, f = golden("water_1500")
+    o = Oracle(s)
+    x, v = s.x.copy(), s.v.copy()
+    o.celprp(x)
+    ep = []
+    for idloop in range(1, int(f["traj_nsteps"]) + 1):
+        g, ee, ev = o.nonbond(x)
+        ep.append(ee.sum() + ev.sum())
+        o.verlet_step(x, v, o.combine(g))
+        if idloop % 10 == 0:
+            o.celprp(x)
+    ref = f["traj_epot"][:, 0]
+    # E15.7 keeps 7 significant digits: compare the same decimal rounding
+    assert [float(f"{e:.6e}") for e in ep] == list(ref)
