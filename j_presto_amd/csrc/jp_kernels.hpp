@@ -43,7 +43,7 @@ struct SysP {
     int slot0, nslot;                // padded slot range
     int grp0, ngrp;                  // i-group range
     int lj0, ntyp;                   // LJ table offset / size
-    int exc0;                        // unused padding
+    int small;                       // some edge < 2*rlim: minimum image per pair
     double dt;
 };
 
@@ -412,6 +412,14 @@ k_pair_f64(int ngrp, const int *__restrict__ grp_sys, const int *__restrict__ gc
         const int tj = aj.t & 0xfff, cj = (aj.t >> 13) & 1;
         const bool vdwj = (aj.t >> 12) & 1;
         double dx = xs - aj.x, dy = ysh - aj.y, dz = zs - aj.z;
+        if (P.small) {
+            // edge < 2*rlim: the image chosen at the rebuild may stop being
+            // the minimum one before the next rebuild (cutoff.f90 tests anint
+            // images; enear.f90:189-191 re-takes them every step)
+            dx = dx - P.L[0] * anint_d(dx * P.invL[0]);
+            dy = dy - P.L[1] * anint_d(dy * P.invL[1]);
+            dz = dz - P.L[2] * anint_d(dz * P.invL[2]);
+        }
         double dd2 = dx * dx + dy * dy + dz * dz;
         const bool in = (m >> l) & 1;
         dd2 = in ? dd2 : 1.0;
@@ -504,7 +512,13 @@ k_pair_f32(int ngrp, const int *__restrict__ grp_sys, const int *__restrict__ gc
         }
         const JAtom aj = ys[jsl];
         const int tj = aj.t & 0xfff, cj = (aj.t >> 13) & 1;
-        const float dx = (float)(xs - aj.x), dy = (float)(ysh - aj.y), dz = (float)(zs - aj.z);
+        double ddx = xs - aj.x, ddy = ysh - aj.y, ddz = zs - aj.z;
+        if (P.small) {
+            ddx = ddx - P.L[0] * rint(ddx * P.invL[0]);
+            ddy = ddy - P.L[1] * rint(ddy * P.invL[1]);
+            ddz = ddz - P.L[2] * rint(ddz * P.invL[2]);
+        }
+        const float dx = (float)ddx, dy = (float)ddy, dz = (float)ddz;
         float r2 = dx * dx + dy * dy + dz * dz;
         const bool in = ((m >> l) & 1) && (r2 <= rc2);
         r2 = in ? r2 : 1.0f;
@@ -819,6 +833,11 @@ k_count_pairs(int ngrp, const int *__restrict__ grp_sys, const int *__restrict__
         shift_of(E[e].js >> JP_SLOT_BITS, sx, sy, sz);
         const JAtom aj = ys[jsl];
         double dx = ai.x - P.L[0] * sx - aj.x, dy = ai.y - P.L[1] * sy - aj.y, dz = ai.z - P.L[2] * sz - aj.z;
+        if (P.small) {
+            dx = dx - P.L[0] * rint(dx * P.invL[0]);
+            dy = dy - P.L[1] * rint(dy * P.invL[1]);
+            dz = dz - P.L[2] * rint(dz * P.invL[2]);
+        }
         bool lis = ((E[e].m >> l) & 1) && (slot < jsl);
         bool in = lis && (dx * dx + dy * dy + dz * dz <= P.rc2);
         c += in;

@@ -93,6 +93,7 @@ struct Engine {
     hipEvent_t ev[8];
     double t_total = 0, t_pair = 0, t_list = 0, t_step = 0;
     int n_pair = 0, n_list = 0;
+    long long idloop = 0;          // md.f90 idloop: steps done, kept across run calls
     long long list_entries = 0;
 
     // device arrays
@@ -218,6 +219,7 @@ int Engine::init(const jp_sys_desc *descs, int ns) {
         }
         P.rlim2 = rlim * rlim;
         P.rc2 = rc * rc;
+        P.small = (P.L[0] < 2.0 * rlim || P.L[1] < 2.0 * rlim || P.L[2] < 2.0 * rlim) ? 1 : 0;
         P.irlim2 = 1.0 / (rc * rc);
         double rt = 1.0 / rc;
         P.zcore = 1.5 * rt;
@@ -685,7 +687,8 @@ int Engine::run_nve(int nsteps, int update, int nmon, int prec, jp_energy_log *l
             }
             nlog++;
         }
-        if (step % update == 0 && step < nsteps) {
+        ++idloop;
+        if (idloop % update == 0) {     // md.f90:655-660, after the step
             if ((r = build_list())) return r;
         }
     }

@@ -139,6 +139,23 @@ class MdSystem:
         return (n.value, nlj.value, nl.value) if detail else n.value
 
 
+def equilibrate(md: "MdSystem", nsteps: int, temp: float = 300.0, every: int = 10, update: int = 10):
+    """Set-up helper (not part of the reference path): leap-frog steps with
+    the velocities rescaled to `temp` every `every` steps, to start NVE runs
+    from an equilibrated state."""
+    from .systems import GASCST
+    m = md.h.mass[:, None]
+    ndf = md.system.ndf() - 3
+    for _ in range(max(1, nsteps // every)):
+        _, ier = md.runmd(every, update=update)
+        if ier:
+            return ier
+        x, v = md.get_state()
+        t = (m * v * v).sum() * 1e7 / (GASCST * ndf)
+        md.set_state(None, v * np.sqrt(temp / t))
+    return 0
+
+
 def log_to_array(log, nl: int) -> np.ndarray:
     dt = np.dtype([("step", "i4"), ("etot", "f8"), ("epot", "f8"), ("ekin", "f8"),
                    ("eelec", "f8", 3), ("evdw", "f8", 3)])
