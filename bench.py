@@ -38,9 +38,23 @@ def ns_per_day(steps_per_s: float) -> float:
     return steps_per_s * DT_FS * 0.0864      # md.f90:305-306
 
 
-def run_gpu(system, precision, steps, warmup):
+def equilibrated(system, nsteps: int):
+    """Velocity-rescaled leap-frog from the lattice start (set-up only), so
+    that the timed NVE segment starts from an equilibrated liquid."""
+    from j_presto_amd import engine
+    md = engine.MdSystem(system, dt=DT_FS, precision=engine.FP32)
+    assert md.celprp() == 0
+    assert engine.equilibrate(md, nsteps) == 0
+    x, v = md.get_state()
+    md.close()
+    return x, v
+
+
+def run_gpu(system, precision, steps, warmup, state=None):
     from j_presto_amd import engine
     md = engine.MdSystem(system, dt=DT_FS, precision=precision)
+    if state is not None:
+        md.set_state(*state)
     ier = md.celprp()
     assert ier == 0, ier
     if warmup:
@@ -79,7 +93,7 @@ def cpu_reference(system, nsteps: int):
                            stderr=subprocess.STDOUT, cwd=tmp, env=env, check=True, timeout=900)
         log = open(os.path.join(tmp, "md.log")).read()
         speed = float(re.search(r"SPEED \(ns/day\)\s*:\s*([0-9.Ee+-]+)", log).group(1))
-        loop = float(re.search(r"MD LOOP\s*:\s*([0-9.Ee+-]+)", log).group(1))
+        loop = float(re.search(r"^\s+MD LOOP\s+:\s+([0-9.Ee+-]+)", log, re.M).group(1))
         return dict(value=speed, unit="ns/day", cores=cores, kind="reference",
                     sample=f"md_run (flang -O2 build of the reference sources), {nsteps} MICR steps of this "
                            f"workload, PARALV=DOUB N_SIMD=64, {cores} OpenMP threads, MD loop {loop:.2f} s")
@@ -105,6 +119,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--fp64-steps", type=int, default=0)
     ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--equil", type=int, default=1000, help="velocity-rescaled set-up steps before timing")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus})",
@@ -113,9 +128,10 @@ def main():
     rank = int(os.environ.get("RANK", "0"))
     from j_presto_amd import engine, systems
     s = systems.workload(2)
-    r32 = run_gpu(s, engine.FP32, args.steps, args.warmup)
+    state = equilibrated(s, args.equil)
+    r32 = run_gpu(s, engine.FP32, args.steps, args.warmup, state)
     n64 = args.fp64_steps or max(args.steps // 4, 100)
-    r64 = run_gpu(s, engine.FP64, n64, min(args.warmup, 50))
+    r64 = run_gpu(s, engine.FP64, n64, min(args.warmup, 50), state)
     sps = args.steps / r32["wall"]
     value = ns_per_day(sps)
     v64 = ns_per_day(n64 / r64["wall"])
@@ -125,7 +141,7 @@ def main():
     t_pair = r32["timing"]["pair"] / args.steps
     achieved = flop / t_pair / 1e12
     # list build: 64 B per atom + 4 B per unique listed pair, per rebuild
-    nreb = max(1, (args.steps - 1) // UPDATE)
+    nreb = max(1, args.steps // UPDATE)
     t_list = r32["timing"]["list"] / nreb
     list_bytes = 64 * s.n + 4 * r32["listed"]
     et = r32["log"]["etot"]
@@ -150,6 +166,8 @@ def main():
         "fp64_gpu_ns_per_day": round(v64, 3),
         "speedup_vs_fp64_gpu": round(value / v64, 3),
         "nve_etot_span_kcal_mol": float(et.max() - et.min()) if len(et) else None,
+        "nve_etot_slope_kcal_mol_per_ns": (float(__import__("numpy").polyfit(r32["log"]["step"] * DT_FS * 1e-6, et, 1)[0])
+                                           if len(et) > 2 else None),
         "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 5),
                      "traffic": traffic_from_profile(),

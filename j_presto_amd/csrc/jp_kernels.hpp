@@ -244,17 +244,28 @@ __global__ void __launch_bounds__(64) k_bbox(int ngrp, const int *__restrict__ g
 // rebuild: the pair list (cutoff.f90 cutatm_PB + cutoff_PB semantics)
 // --------------------------------------------------------------------------
 //
-// One wavefront per i-group.  Candidate j atoms: every atom of every cell of
-// the unwrapped cell range around the group's bounding box grown by rlim,
-// each with the periodic shift of that unwrapped cell.  Lane l sets its bit
-// for (j, shift) iff
+// One block of WL wavefronts per i-group; wave w takes every WL-th candidate
+// cell.  Candidate j atoms: every atom of every cell of the unwrapped cell
+// range around the group's bounding box grown by rlim, each with the periodic
+// shift of that unwrapped cell.  Lane l sets its bit for (j, shift) iff
 //   (a) md_run's FP64 test passes on raw coordinates: d = xi - xj,
 //       d -= L*anint(d/L), dx*dx+dy*dy+dz*dz <= rlim^2 (cutoff.f90:464-469),
 //   (b) the shift is that pair's minimum image (each pair appears once),
 //   (c) j != i and (i,j) is not an excluded pair (cutoff.f90:50-96).
-// FILL=false counts entries per group; FILL=true writes them.
+// A cell's atoms are fetched with one coalesced load (lane k holds atom k)
+// and broadcast with readlane.  FILL=false counts entries per (group, wave);
+// FILL=true writes them, so a group's entries are contiguous.
+#define JP_WL 4
+
+__device__ __forceinline__ double bcast_d(double v, int k) {
+    int lo = __builtin_amdgcn_readlane(__double2loint(v), k);
+    int hi = __builtin_amdgcn_readlane(__double2hiint(v), k);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ int bcast_i(int v, int k) { return __builtin_amdgcn_readlane(v, k); }
+
 template <bool FILL>
-__global__ void __launch_bounds__(64)
+__global__ void __launch_bounds__(64 * JP_WL)
 k_list(int ngrp, const int *__restrict__ grp_sys, const int *__restrict__ gcnt, const SysP *__restrict__ sp,
        const double *__restrict__ bb, const int *__restrict__ crank, const int *__restrict__ cslot,
        const int *__restrict__ ccnt, const double4 *__restrict__ xr, const int4 *__restrict__ imgs,
@@ -262,9 +273,11 @@ k_list(int ngrp, const int *__restrict__ grp_sys, const int *__restrict__ gcnt, 
        int *__restrict__ lcount, const int *__restrict__ loff, Entry *__restrict__ ent, int lcap,
        int *__restrict__ nremoved, int *__restrict__ nbcount, int *__restrict__ err) {
 #pragma clang fp contract(off)
+    __shared__ int s_nb[JP_WL][64];
+    __shared__ int s_nrem[JP_WL];
     int g = xcd_remap(blockIdx.x, gridDim.x);
     if (g >= ngrp) return;
-    const int l = threadIdx.x;
+    const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
     const SysP &P = sp[grp_sys[g]];
     const bool act = l < gcnt[g];
     const int slot = g * 64 + l;
@@ -276,87 +289,113 @@ k_list(int ngrp, const int *__restrict__ grp_sys, const int *__restrict__ gcnt, 
     const double rl = sqrt(P.rlim2) * (1.0 + 1e-12) + 1e-9;  // conservative pruning radius
     double lo[3], hi[3];
     int ulo[3], uhi[3];
+    bool bad = false;
     for (int k = 0; k < 3; ++k) {
         lo[k] = bb[6 * g + k];
         hi[k] = bb[6 * g + 3 + k];
         double a = floor((lo[k] - rl - P.wall[k]) * P.icsz[k]);
         double b = floor((hi[k] + rl - P.wall[k]) * P.icsz[k]);
         // non-finite or runaway coordinates: report and do nothing (bounded loops)
-        if (!(a >= -4.0 * P.nc[k] - 64.0) || !(b <= 5.0 * P.nc[k] + 64.0) || !(b - a <= 3.0 * P.nc[k] + 16.0)) {
-            if (l == 0) {
-                atomicOr(err, 1);
-                if (!FILL) lcount[g] = 0;
-            }
-            return;
-        }
-        ulo[k] = (int)a;
-        uhi[k] = (int)b;
+        if (!(a >= -4.0 * P.nc[k] - 64.0) || !(b <= 5.0 * P.nc[k] + 64.0) || !(b - a <= 3.0 * P.nc[k] + 16.0))
+            bad = true;
+        ulo[k] = bad ? 0 : (int)a;
+        uhi[k] = bad ? -1 : (int)b;
     }
-    int pos = FILL ? loff[g] : 0;
+    if (bad) {
+        if (threadIdx.x == 0) {
+            atomicOr(err, 1);
+            if (!FILL)
+                for (int q = 0; q < JP_WL; ++q) lcount[g * JP_WL + q] = 0;
+        }
+        return;
+    }
+    const int nx = uhi[0] - ulo[0] + 1, ny = uhi[1] - ulo[1] + 1, nz = uhi[2] - ulo[2] + 1;
+    const int ncand = nx * ny * nz;
+    int pos = FILL ? loff[g * JP_WL + w] : 0;
     int nrem = 0, nb = 0;
-    for (int uz = ulo[2]; uz <= uhi[2]; ++uz) {
+    for (int q = w; q < ncand; q += JP_WL) {
+        const int ux = ulo[0] + q % nx, uy = ulo[1] + (q / nx) % ny, uz = ulo[2] + q / (nx * ny);
+        double gx = fmax(0.0, fmax(P.wall[0] + ux * P.csz[0] - hi[0], lo[0] - (P.wall[0] + (ux + 1) * P.csz[0])));
+        double gy = fmax(0.0, fmax(P.wall[1] + uy * P.csz[1] - hi[1], lo[1] - (P.wall[1] + (uy + 1) * P.csz[1])));
         double gz = fmax(0.0, fmax(P.wall[2] + uz * P.csz[2] - hi[2], lo[2] - (P.wall[2] + (uz + 1) * P.csz[2])));
+        if (gx * gx + gy * gy + gz * gz > rl * rl) continue;
+        int sx = (ux >= 0) ? ux / P.nc[0] : -((-ux + P.nc[0] - 1) / P.nc[0]);
+        int sy = (uy >= 0) ? uy / P.nc[1] : -((-uy + P.nc[1] - 1) / P.nc[1]);
         int sz = (uz >= 0) ? uz / P.nc[2] : -((-uz + P.nc[2] - 1) / P.nc[2]);
-        int cz = uz - sz * P.nc[2];
-        for (int uy = ulo[1]; uy <= uhi[1]; ++uy) {
-            double gy = fmax(0.0, fmax(P.wall[1] + uy * P.csz[1] - hi[1], lo[1] - (P.wall[1] + (uy + 1) * P.csz[1])));
-            if (gz * gz + gy * gy > rl * rl) continue;
-            int sy = (uy >= 0) ? uy / P.nc[1] : -((-uy + P.nc[1] - 1) / P.nc[1]);
-            int cy = uy - sy * P.nc[1];
-            for (int ux = ulo[0]; ux <= uhi[0]; ++ux) {
-                double gx = fmax(0.0, fmax(P.wall[0] + ux * P.csz[0] - hi[0], lo[0] - (P.wall[0] + (ux + 1) * P.csz[0])));
-                if (gz * gz + gy * gy + gx * gx > rl * rl) continue;
-                int sx = (ux >= 0) ? ux / P.nc[0] : -((-ux + P.nc[0] - 1) / P.nc[0]);
-                int cx = ux - sx * P.nc[0];
-                int cell = crank[P.cell0 + (cz * P.nc[1] + cy) * P.nc[0] + cx];
-                int j0 = cslot[cell], nj = ccnt[cell];
-                uint32_t code = (uint32_t)((sx + 1) * 9 + (sy + 1) * 3 + (sz + 1));
-                for (int jj = 0; jj < nj; ++jj) {
-                    const int js = j0 + jj;
-                    const double4 xj = xr[js];
-                    const int4 ij = imgs[js];
-                    double d0 = xi.x - xj.x, d1 = xi.y - xj.y, d2 = xi.z - xj.z;
-                    double n0 = anint_d(d0 * P.invL[0]), n1 = anint_d(d1 * P.invL[1]), n2 = anint_d(d2 * P.invL[2]);
-                    d0 = d0 - P.L[0] * n0;
-                    d1 = d1 - P.L[1] * n1;
-                    d2 = d2 - P.L[2] * n2;
-                    double dd = d0 * d0 + d1 * d1 + d2 * d2;
-                    bool in = act && (js != slot) && (dd <= P.rlim2);
-                    in = in && ((int)n0 - ii.x + ij.x == sx) && ((int)n1 - ii.y + ij.y == sy) &&
-                         ((int)n2 - ii.z + ij.z == sz);
-                    if (in && mol[ij.w] == moli) {
-                        for (int e = xb; e < xe; ++e)
-                            if (xidx[e] == ij.w) { in = false; ++nrem; break; }
+        const int cx = ux - sx * P.nc[0], cy = uy - sy * P.nc[1], cz = uz - sz * P.nc[2];
+        const int cell = crank[P.cell0 + (cz * P.nc[1] + cy) * P.nc[0] + cx];
+        const int j0 = cslot[cell], nj = ccnt[cell];
+        const uint32_t code = (uint32_t)((sx + 1) * 9 + (sy + 1) * 3 + (sz + 1));
+        for (int base = 0; base < nj; base += 64) {
+            const int m = min(64, nj - base);
+            double4 cx4 = make_double4(0, 0, 0, 0);
+            int4 ci4 = make_int4(0, 0, 0, 0);
+            int cmol = -3;
+            if (l < m) {
+                cx4 = xr[j0 + base + l];
+                ci4 = imgs[j0 + base + l];
+                cmol = mol[ci4.w];
+            }
+            for (int k = 0; k < m; ++k) {
+                const int js = j0 + base + k;
+                const double xjx = bcast_d(cx4.x, k), xjy = bcast_d(cx4.y, k), xjz = bcast_d(cx4.z, k);
+                const int ijx = bcast_i(ci4.x, k), ijy = bcast_i(ci4.y, k), ijz = bcast_i(ci4.z, k);
+                const int aj = bcast_i(ci4.w, k), molj = bcast_i(cmol, k);
+                double d0 = xi.x - xjx, d1 = xi.y - xjy, d2 = xi.z - xjz;
+                double n0 = anint_d(d0 * P.invL[0]), n1 = anint_d(d1 * P.invL[1]), n2 = anint_d(d2 * P.invL[2]);
+                d0 = d0 - P.L[0] * n0;
+                d1 = d1 - P.L[1] * n1;
+                d2 = d2 - P.L[2] * n2;
+                double dd = d0 * d0 + d1 * d1 + d2 * d2;
+                bool in = act && (js != slot) && (dd <= P.rlim2);
+                in = in && ((int)n0 - ii.x + ijx == sx) && ((int)n1 - ii.y + ijy == sy) &&
+                     ((int)n2 - ii.z + ijz == sz);
+                if (in && molj == moli) {
+                    for (int e = xb; e < xe; ++e)
+                        if (xidx[e] == aj) { in = false; ++nrem; break; }
+                }
+                const uint64_t mk = __ballot(in);
+                if (mk) {
+                    if (FILL && l == 0 && pos < lcap) {
+                        Entry E;
+                        E.js = (uint32_t)js | (code << JP_SLOT_BITS);
+                        E.pad = 0;
+                        E.m = mk;
+                        ent[pos] = E;
                     }
-                    uint64_t m = __ballot(in);
-                    if (m) {
-                        if (FILL && l == 0 && pos < lcap) {
-                            Entry E;
-                            E.js = (uint32_t)js | (code << JP_SLOT_BITS);
-                            E.pad = 0;
-                            E.m = m;
-                            ent[pos] = E;
-                        }
-                        ++pos;
-                        nb += in ? 1 : 0;
-                    }
+                    ++pos;
+                    nb += in ? 1 : 0;
                 }
             }
         }
     }
     if (!FILL) {
-        if (l == 0) lcount[g] = pos;
+        if (l == 0) lcount[g * JP_WL + w] = pos;
         int t = wave_sum(nrem);
-        if (l == 0 && t) atomicAdd(&nremoved[grp_sys[g]], t);
-        if (act) nbcount[ai] = nb;
+        s_nb[w][l] = nb;
+        if (l == 0) s_nrem[w] = t;
+        __syncthreads();
+        if (w == 0) {
+            int tot = 0, nbt = 0;
+            for (int q = 0; q < JP_WL; ++q) { tot += s_nrem[q]; nbt += s_nb[q][l]; }
+            if (l == 0 && tot) atomicAdd(&nremoved[grp_sys[g]], tot);
+            if (act) nbcount[ai] = nbt;
+        }
     } else {
-        if (l == 0 && pos != loff[g] + lcount[g]) atomicOr(err, 2);
+        if (l == 0 && pos != loff[g * JP_WL + w] + lcount[g * JP_WL + w]) atomicOr(err, 2);
     }
 }
 
 // --------------------------------------------------------------------------
 // per step: nonbonded pair kernels (enear_ZD_doub_PB semantics)
 // --------------------------------------------------------------------------
+//
+// One block of WP wavefronts per i-group; lane l of every wave owns i atom
+// slot 64g+l.  The block stages 64*WP list entries and their j atoms in LDS
+// (coalesced entry loads, gathered 32-byte j loads, all in flight at once);
+// wave w then runs over its 64 of them, every lane computing its pair with
+// the j atom read by LDS broadcast.  The WP partial forces of an atom are
+// combined in a fixed order at the end.
 
 __device__ __forceinline__ void shift_of(uint32_t code, int &sx, int &sy, int &sz) {
     sx = (int)(code / 9) - 1;
@@ -365,22 +404,23 @@ __device__ __forceinline__ void shift_of(uint32_t code, int &sx, int &sy, int &s
 }
 
 // Plain FP64 path: md_run's per-pair arithmetic (enear.f90:182-258 ZD lines)
-// in FP64 with IEEE division and square root, no FMA contraction; each lane
-// owns one i atom and accumulates its gradient over its j in list order.
-template <bool ENERGY>
-__global__ void __launch_bounds__(64)
+// in FP64 with IEEE division and square root, no FMA contraction.
+template <bool ENERGY, int WP>
+__global__ void __launch_bounds__(64 * WP)
 k_pair_f64(int ngrp, const int *__restrict__ grp_sys, const int *__restrict__ gcnt, const SysP *__restrict__ sp,
            const JAtom *__restrict__ ys, const double *__restrict__ qd, const double2 *__restrict__ lj64,
-           const int *__restrict__ loff, const int *__restrict__ lcount, const Entry *__restrict__ ent,
-           double *__restrict__ fout, double *__restrict__ epart) {
+           const int *__restrict__ loff, const Entry *__restrict__ ent, double *__restrict__ fout,
+           double *__restrict__ epart) {
 #pragma clang fp contract(off)
     extern __shared__ double2 ljs[];
+    __shared__ Entry se[64 * WP];
+    __shared__ JAtom sj[64 * WP];
+    __shared__ double sq[64 * WP];
     int g = xcd_remap(blockIdx.x, gridDim.x);
     if (g >= ngrp) return;
     const SysP &P = sp[grp_sys[g]];
-    const int l = threadIdx.x;
-    for (int k = l; k < P.ntyp * P.ntyp; k += 64) ljs[k] = lj64[P.lj0 + k];
-    __syncthreads();
+    const int tid = threadIdx.x, l = tid & 63, w = tid >> 6;
+    for (int k = tid; k < P.ntyp * P.ntyp; k += 64 * WP) ljs[k] = lj64[P.lj0 + k];
     const int slot = g * 64 + l;
     const bool act = l < gcnt[g];
     const JAtom ai = ys[slot];
@@ -391,74 +431,105 @@ k_pair_f64(int ngrp, const int *__restrict__ grp_sys, const int *__restrict__ gc
     double gx = 0.0, gy = 0.0, gz = 0.0;
     double ee[3] = {0.0, 0.0, 0.0}, ev[3] = {0.0, 0.0, 0.0};
     double xs = ai.x, ysh = ai.y, zs = ai.z;
-    uint32_t cur = 13;  // no shift
-    const Entry *E = ent + loff[g];
-    const int ne = lcount[g];
-    for (int e = 0; e < ne; ++e) {
-        const uint32_t js = E[e].js;
-        const uint64_t m = E[e].m;
-        const int jsl = (int)(js & JP_SLOT_MASK);
-        const uint32_t code = js >> JP_SLOT_BITS;
-        if (code != cur) {
-            int sx, sy, sz;
-            shift_of(code, sx, sy, sz);
-            xs = ai.x - P.L[0] * (double)sx;
-            ysh = ai.y - P.L[1] * (double)sy;
-            zs = ai.z - P.L[2] * (double)sz;
-            cur = code;
+    uint32_t cur = 13;
+    const int e0 = loff[g * JP_WL], ne = loff[(g + 1) * JP_WL] - e0;
+    for (int base = 0; base < ne; base += 64 * WP) {
+        __syncthreads();
+        if (base + tid < ne) {
+            const Entry E = ent[e0 + base + tid];
+            const int jsl = (int)(E.js & JP_SLOT_MASK);
+            se[tid] = E;
+            sj[tid] = ys[jsl];
+            sq[tid] = qd[jsl];
         }
-        const JAtom aj = ys[jsl];
-        const double qj = qd[jsl];
-        const int tj = aj.t & 0xfff, cj = (aj.t >> 13) & 1;
-        const bool vdwj = (aj.t >> 12) & 1;
-        double dx = xs - aj.x, dy = ysh - aj.y, dz = zs - aj.z;
-        if (P.small) {
-            // edge < 2*rlim: the image chosen at the rebuild may stop being
-            // the minimum one before the next rebuild (cutoff.f90 tests anint
-            // images; enear.f90:189-191 re-takes them every step)
-            dx = dx - P.L[0] * anint_d(dx * P.invL[0]);
-            dy = dy - P.L[1] * anint_d(dy * P.invL[1]);
-            dz = dz - P.L[2] * anint_d(dz * P.invL[2]);
-        }
-        double dd2 = dx * dx + dy * dy + dz * dz;
-        const bool in = (m >> l) & 1;
-        dd2 = in ? dd2 : 1.0;
-        double dd2i = 1.0 / dd2;
-        double co = (dd2i - P.irlim2 >= 0.0) ? 1.0 : 0.0;
-        co = in ? co : 0.0;
-        double ddi = sqrt(dd2i);
-        double eel = qi * qj * co;
-        double cc = eel * (dd2i * ddi - P.fcoeff);
-        double vv = 0.0;
-        if (vdwj) {
-            const double2 c = ljs[ti * P.ntyp + tj];
-            double dd6i = dd2i * dd2i * dd2i * co;
-            double v1 = c.x * dd6i;
-            double v2 = c.y * dd6i * dd6i;
-            vv = v2 - v1;
-            cc = cc + (-6.0 * v1 + 12.0 * v2) * dd2i;
-        }
-        const double fc = cj ? f1 : f0;
-        cc = cc * fc;
-        gx = gx + dx * cc;
-        gy = gy + dy * cc;
-        gz = gz + dz * cc;
-        if (ENERGY) {
-            double el = eel * (ddi + P.bcoeff * dd2 - P.zcore);
-            const int k = cj ? k1 : k0;
-            ee[0] += (k == 0) ? el : 0.0; ee[1] += (k == 1) ? el : 0.0; ee[2] += (k == 2) ? el : 0.0;
-            ev[0] += (k == 0) ? vv : 0.0; ev[1] += (k == 1) ? vv : 0.0; ev[2] += (k == 2) ? vv : 0.0;
+        __syncthreads();
+        const int hi = min(64 * (w + 1), ne - base);
+        for (int q = 64 * w; q < hi; ++q) {
+            const uint32_t js = se[q].js;
+            const uint64_t m = se[q].m;
+            const uint32_t code = js >> JP_SLOT_BITS;
+            if (code != cur) {
+                int sx, sy, sz;
+                shift_of(code, sx, sy, sz);
+                xs = ai.x - P.L[0] * (double)sx;
+                ysh = ai.y - P.L[1] * (double)sy;
+                zs = ai.z - P.L[2] * (double)sz;
+                cur = code;
+            }
+            const JAtom aj = sj[q];
+            const double qj = sq[q];
+            const int tj = aj.t & 0xfff, cj = (aj.t >> 13) & 1;
+            const bool vdwj = (aj.t >> 12) & 1;
+            double dx = xs - aj.x, dy = ysh - aj.y, dz = zs - aj.z;
+            if (P.small) {
+                // edge < 2*rlim: the image chosen at the rebuild may stop being
+                // the minimum one before the next rebuild (cutoff.f90 tests anint
+                // images; enear.f90:189-191 re-takes them every step)
+                dx = dx - P.L[0] * anint_d(dx * P.invL[0]);
+                dy = dy - P.L[1] * anint_d(dy * P.invL[1]);
+                dz = dz - P.L[2] * anint_d(dz * P.invL[2]);
+            }
+            double dd2 = dx * dx + dy * dy + dz * dz;
+            const bool in = (m >> l) & 1;
+            dd2 = in ? dd2 : 1.0;
+            double dd2i = 1.0 / dd2;
+            double co = (dd2i - P.irlim2 >= 0.0) ? 1.0 : 0.0;
+            co = in ? co : 0.0;
+            double ddi = sqrt(dd2i);
+            double eel = qi * qj * co;
+            double cc = eel * (dd2i * ddi - P.fcoeff);
+            double vv = 0.0;
+            if (vdwj) {
+                const double2 c = ljs[ti * P.ntyp + tj];
+                double dd6i = dd2i * dd2i * dd2i * co;
+                double v1 = c.x * dd6i;
+                double v2 = c.y * dd6i * dd6i;
+                vv = v2 - v1;
+                cc = cc + (-6.0 * v1 + 12.0 * v2) * dd2i;
+            }
+            const double fc = cj ? f1 : f0;
+            cc = cc * fc;
+            gx = gx + dx * cc;
+            gy = gy + dy * cc;
+            gz = gz + dz * cc;
+            if (ENERGY) {
+                double el = eel * (ddi + P.bcoeff * dd2 - P.zcore);
+                const int k = cj ? k1 : k0;
+                ee[0] += (k == 0) ? el : 0.0; ee[1] += (k == 1) ? el : 0.0; ee[2] += (k == 2) ? el : 0.0;
+                ev[0] += (k == 0) ? vv : 0.0; ev[1] += (k == 1) ? vv : 0.0; ev[2] += (k == 2) ? vv : 0.0;
+            }
         }
     }
-    if (act) {
-        fout[3 * (size_t)slot] = 0.0 - gx;
-        fout[3 * (size_t)slot + 1] = 0.0 - gy;
-        fout[3 * (size_t)slot + 2] = 0.0 - gz;
+    // combine the WP partial sums of each atom in wave order
+    __syncthreads();
+    double *red = (double *)sj;   // 64*WP*32 B >= WP*64*3*8 B
+    red[(w * 64 + l) * 3] = gx;
+    red[(w * 64 + l) * 3 + 1] = gy;
+    red[(w * 64 + l) * 3 + 2] = gz;
+    __syncthreads();
+    if (w == 0 && act) {
+        double sx = 0.0, sy = 0.0, sz = 0.0;
+        for (int q = 0; q < WP; ++q) {
+            sx = sx + red[(q * 64 + l) * 3];
+            sy = sy + red[(q * 64 + l) * 3 + 1];
+            sz = sz + red[(q * 64 + l) * 3 + 2];
+        }
+        fout[3 * (size_t)slot] = 0.0 - sx;
+        fout[3 * (size_t)slot + 1] = 0.0 - sy;
+        fout[3 * (size_t)slot + 2] = 0.0 - sz;
     }
     if (ENERGY) {
+        __syncthreads();
+        double *re = (double *)se;   // 64*WP*16 B >= WP*6*8 B
         for (int k = 0; k < 3; ++k) {
             double a = wave_sum(act ? ee[k] : 0.0), b = wave_sum(act ? ev[k] : 0.0);
-            if (l == 0) { epart[6 * (size_t)g + k] = a; epart[6 * (size_t)g + 3 + k] = b; }
+            if (l == 0) { re[w * 6 + k] = a; re[w * 6 + 3 + k] = b; }
+        }
+        __syncthreads();
+        if (tid < 6) {
+            double t = 0.0;
+            for (int q = 0; q < WP; ++q) t = t + re[q * 6 + tid];
+            epart[6 * (size_t)g + tid] = t;
         }
     }
 }
@@ -466,19 +537,19 @@ k_pair_f64(int ngrp, const int *__restrict__ grp_sys, const int *__restrict__ gc
 // FP32 path: coordinate differences in FP64 (y is up to ~200 A), everything
 // after them in FP32; per-lane FP32 partial sums are flushed every JP_FLUSH j
 // into 64-bit fixed point (deterministic, independent of summation order).
-template <bool ENERGY>
-__global__ void __launch_bounds__(64)
+template <bool ENERGY, int WP>
+__global__ void __launch_bounds__(64 * WP)
 k_pair_f32(int ngrp, const int *__restrict__ grp_sys, const int *__restrict__ gcnt, const SysP *__restrict__ sp,
            const JAtom *__restrict__ ys, const float2 *__restrict__ lj32, const int *__restrict__ loff,
-           const int *__restrict__ lcount, const Entry *__restrict__ ent, long long *__restrict__ fout,
-           long long *__restrict__ epart) {
+           const Entry *__restrict__ ent, long long *__restrict__ fout, long long *__restrict__ epart) {
     extern __shared__ float2 ljf[];
+    __shared__ Entry se[64 * WP];
+    __shared__ JAtom sj[64 * WP];
     int g = xcd_remap(blockIdx.x, gridDim.x);
     if (g >= ngrp) return;
     const SysP &P = sp[grp_sys[g]];
-    const int l = threadIdx.x;
-    for (int k = l; k < P.ntyp * P.ntyp; k += 64) ljf[k] = lj32[P.lj0 + k];
-    __syncthreads();
+    const int tid = threadIdx.x, l = tid & 63, w = tid >> 6;
+    for (int k = tid; k < P.ntyp * P.ntyp; k += 64 * WP) ljf[k] = lj32[P.lj0 + k];
     const int slot = g * 64 + l;
     const bool act = l < gcnt[g];
     const JAtom ai = ys[slot];
@@ -487,89 +558,117 @@ k_pair_f32(int ngrp, const int *__restrict__ grp_sys, const int *__restrict__ gc
     const float f0 = (float)P.fac[ci * 2 + 0], f1 = (float)P.fac[ci * 2 + 1];
     const int k0 = P.cls[ci * 2 + 0], k1 = P.cls[ci * 2 + 1];
     const float rc2 = (float)P.rc2, fcf = (float)P.fcoeff, bcf = (float)P.bcoeff, zcf = (float)P.zcore;
-    const float *ljrow = (const float *)(ljf + ti * P.ntyp);
     float fx = 0.f, fy = 0.f, fz = 0.f;
     long long ax = 0, ay = 0, az = 0;
     float ee[3] = {0.f, 0.f, 0.f}, ev[3] = {0.f, 0.f, 0.f};
     long long eacc[6] = {0, 0, 0, 0, 0, 0};
     double xs = ai.x, ysh = ai.y, zs = ai.z;
     uint32_t cur = 13;
-    const Entry *E = ent + loff[g];
-    const int ne = lcount[g];
     int nf = 0;
-    for (int e = 0; e < ne; ++e) {
-        const uint32_t js = E[e].js;
-        const uint64_t m = E[e].m;
-        const int jsl = (int)(js & JP_SLOT_MASK);
-        const uint32_t code = js >> JP_SLOT_BITS;
-        if (code != cur) {
-            int sx, sy, sz;
-            shift_of(code, sx, sy, sz);
-            xs = ai.x - P.L[0] * (double)sx;
-            ysh = ai.y - P.L[1] * (double)sy;
-            zs = ai.z - P.L[2] * (double)sz;
-            cur = code;
+    const int e0 = loff[g * JP_WL], ne = loff[(g + 1) * JP_WL] - e0;
+    for (int base = 0; base < ne; base += 64 * WP) {
+        __syncthreads();
+        if (base + tid < ne) {
+            const Entry E = ent[e0 + base + tid];
+            se[tid] = E;
+            sj[tid] = ys[E.js & JP_SLOT_MASK];
         }
-        const JAtom aj = ys[jsl];
-        const int tj = aj.t & 0xfff, cj = (aj.t >> 13) & 1;
-        double ddx = xs - aj.x, ddy = ysh - aj.y, ddz = zs - aj.z;
-        if (P.small) {
-            ddx = ddx - P.L[0] * rint(ddx * P.invL[0]);
-            ddy = ddy - P.L[1] * rint(ddy * P.invL[1]);
-            ddz = ddz - P.L[2] * rint(ddz * P.invL[2]);
-        }
-        const float dx = (float)ddx, dy = (float)ddy, dz = (float)ddz;
-        float r2 = dx * dx + dy * dy + dz * dz;
-        const bool in = ((m >> l) & 1) && (r2 <= rc2);
-        r2 = in ? r2 : 1.0f;
-        const float ri = __builtin_amdgcn_rsqf(r2);
-        const float ri2 = ri * ri;
-        const float qq = qi * aj.q;
-        float c = qq * (ri * ri2 - fcf);
-        float vv = 0.f;
-        if ((aj.t >> 12) & 1) {
-            const float c6 = ljrow[2 * tj], c12 = ljrow[2 * tj + 1];
-            const float ri6 = ri2 * ri2 * ri2;
-            const float v1 = c6 * ri6, v2 = c12 * ri6 * ri6;
-            c += (12.f * v2 - 6.f * v1) * ri2;
-            vv = v2 - v1;
-        }
-        c = in ? c * (cj ? f1 : f0) : 0.f;
-        fx += c * dx;
-        fy += c * dy;
-        fz += c * dz;
-        if (ENERGY) {
-            float el = in ? qq * (ri + bcf * r2 - zcf) : 0.f;
-            vv = in ? vv : 0.f;
-            const int k = cj ? k1 : k0;
-            ee[0] += (k == 0) ? el : 0.f; ee[1] += (k == 1) ? el : 0.f; ee[2] += (k == 2) ? el : 0.f;
-            ev[0] += (k == 0) ? vv : 0.f; ev[1] += (k == 1) ? vv : 0.f; ev[2] += (k == 2) ? vv : 0.f;
-        }
-        if (++nf == JP_FLUSH) {
-            nf = 0;
-            ax += f2fix(fx, JP_FSCALE); ay += f2fix(fy, JP_FSCALE); az += f2fix(fz, JP_FSCALE);
-            fx = fy = fz = 0.f;
-            if (ENERGY)
-                for (int k = 0; k < 3; ++k) {
-                    eacc[k] += f2fix(ee[k], JP_ESCALE); eacc[3 + k] += f2fix(ev[k], JP_ESCALE);
-                    ee[k] = ev[k] = 0.f;
-                }
+        __syncthreads();
+        const float *ljrow = (const float *)(ljf + ti * P.ntyp);
+        const int hi = min(64 * (w + 1), ne - base);
+        for (int q = 64 * w; q < hi; ++q) {
+            const uint32_t js = se[q].js;
+            const uint64_t m = se[q].m;
+            const uint32_t code = js >> JP_SLOT_BITS;
+            if (code != cur) {
+                int sx, sy, sz;
+                shift_of(code, sx, sy, sz);
+                xs = ai.x - P.L[0] * (double)sx;
+                ysh = ai.y - P.L[1] * (double)sy;
+                zs = ai.z - P.L[2] * (double)sz;
+                cur = code;
+            }
+            const JAtom aj = sj[q];
+            const int tj = aj.t & 0xfff, cj = (aj.t >> 13) & 1;
+            double ddx = xs - aj.x, ddy = ysh - aj.y, ddz = zs - aj.z;
+            if (P.small) {
+                ddx = ddx - P.L[0] * rint(ddx * P.invL[0]);
+                ddy = ddy - P.L[1] * rint(ddy * P.invL[1]);
+                ddz = ddz - P.L[2] * rint(ddz * P.invL[2]);
+            }
+            const float dx = (float)ddx, dy = (float)ddy, dz = (float)ddz;
+            float r2 = dx * dx + dy * dy + dz * dz;
+            const bool in = ((m >> l) & 1) && (r2 <= rc2);
+            r2 = in ? r2 : 1.0f;
+            const float ri = __builtin_amdgcn_rsqf(r2);
+            const float ri2 = ri * ri;
+            const float qq = qi * aj.q;
+            float c = qq * (ri * ri2 - fcf);
+            float vv = 0.f;
+            if ((aj.t >> 12) & 1) {
+                const float c6 = ljrow[2 * tj], c12 = ljrow[2 * tj + 1];
+                const float ri6 = ri2 * ri2 * ri2;
+                const float v1 = c6 * ri6, v2 = c12 * ri6 * ri6;
+                c += (12.f * v2 - 6.f * v1) * ri2;
+                vv = v2 - v1;
+            }
+            c = in ? c * (cj ? f1 : f0) : 0.f;
+            fx += c * dx;
+            fy += c * dy;
+            fz += c * dz;
+            if (ENERGY) {
+                float el = in ? qq * (ri + bcf * r2 - zcf) : 0.f;
+                vv = in ? vv : 0.f;
+                const int k = cj ? k1 : k0;
+                ee[0] += (k == 0) ? el : 0.f; ee[1] += (k == 1) ? el : 0.f; ee[2] += (k == 2) ? el : 0.f;
+                ev[0] += (k == 0) ? vv : 0.f; ev[1] += (k == 1) ? vv : 0.f; ev[2] += (k == 2) ? vv : 0.f;
+            }
+            if (++nf == JP_FLUSH) {
+                nf = 0;
+                ax += f2fix(fx, JP_FSCALE); ay += f2fix(fy, JP_FSCALE); az += f2fix(fz, JP_FSCALE);
+                fx = fy = fz = 0.f;
+                if (ENERGY)
+                    for (int k = 0; k < 3; ++k) {
+                        eacc[k] += f2fix(ee[k], JP_ESCALE); eacc[3 + k] += f2fix(ev[k], JP_ESCALE);
+                        ee[k] = ev[k] = 0.f;
+                    }
+            }
         }
     }
     ax += f2fix(fx, JP_FSCALE); ay += f2fix(fy, JP_FSCALE); az += f2fix(fz, JP_FSCALE);
-    if (act) {
-        fout[3 * (size_t)slot] = -ax;
-        fout[3 * (size_t)slot + 1] = -ay;
-        fout[3 * (size_t)slot + 2] = -az;
+    __syncthreads();
+    long long *red = (long long *)sj;   // 64*WP*32 B >= WP*64*3*8 B
+    red[(w * 64 + l) * 3] = ax;
+    red[(w * 64 + l) * 3 + 1] = ay;
+    red[(w * 64 + l) * 3 + 2] = az;
+    __syncthreads();
+    if (w == 0 && act) {
+        long long sx = 0, sy = 0, sz = 0;
+        for (int q = 0; q < WP; ++q) {
+            sx += red[(q * 64 + l) * 3];
+            sy += red[(q * 64 + l) * 3 + 1];
+            sz += red[(q * 64 + l) * 3 + 2];
+        }
+        fout[3 * (size_t)slot] = -sx;
+        fout[3 * (size_t)slot + 1] = -sy;
+        fout[3 * (size_t)slot + 2] = -sz;
     }
     if (ENERGY) {
         for (int k = 0; k < 3; ++k) {
             eacc[k] += f2fix(ee[k], JP_ESCALE);
             eacc[3 + k] += f2fix(ev[k], JP_ESCALE);
         }
+        __syncthreads();
+        long long *re = (long long *)se;
         for (int k = 0; k < 6; ++k) {
             long long t = wave_sum(act ? eacc[k] : 0ll);
-            if (l == 0) epart[6 * (size_t)g + k] = t;
+            if (l == 0) re[w * 6 + k] = t;
+        }
+        __syncthreads();
+        if (tid < 6) {
+            long long t = 0;
+            for (int q = 0; q < WP; ++q) t += re[q * 6 + tid];
+            epart[6 * (size_t)g + tid] = t;
         }
     }
 }
@@ -818,16 +917,17 @@ __global__ void k_ekin(int natom, const double *__restrict__ mass, const double 
 // pairs (r <= rlim at the last rebuild, each once).
 __global__ void __launch_bounds__(64)
 k_count_pairs(int ngrp, const int *__restrict__ grp_sys, const int *__restrict__ gcnt, const SysP *__restrict__ sp,
-              const JAtom *__restrict__ ys, const int *__restrict__ loff, const int *__restrict__ lcount,
+              const JAtom *__restrict__ ys, const int *__restrict__ loff,
               const Entry *__restrict__ ent, unsigned long long *__restrict__ out) {
     int g = blockIdx.x;
     if (g >= ngrp) return;
     const SysP &P = sp[grp_sys[g]];
     const int l = threadIdx.x, slot = g * 64 + l;
     const JAtom ai = ys[slot];
-    const Entry *E = ent + loff[g];
+    const Entry *E = ent + loff[g * JP_WL];
+    const int ne = loff[(g + 1) * JP_WL] - loff[g * JP_WL];
     unsigned long long c = 0, clj = 0, cl = 0;
-    for (int e = 0; e < lcount[g]; ++e) {
+    for (int e = 0; e < ne; ++e) {
         const int jsl = (int)(E[e].js & JP_SLOT_MASK);
         int sx, sy, sz;
         shift_of(E[e].js >> JP_SLOT_BITS, sx, sy, sz);

@@ -374,7 +374,7 @@ int Engine::init(const jp_sys_desc *descs, int ns) {
     AL(d_cnt, ncell + 1); AL(d_start, ncell + 1); AL(d_fill, ncell); AL(d_tmp, natom);
     AL(d_inum, natom); AL(d_perm, nslot); AL(d_cslot, ncell);
     AL(d_ys, nslot); AL(d_qd, nslot); AL(d_bb, 6 * (size_t)ngrp); AL(d_xr, nslot); AL(d_imgs, nslot);
-    AL(d_lcount, ngrp + 1); AL(d_loff, ngrp + 1); AL(d_nrem, ns); AL(d_nb, natom); AL(d_err, 1);
+    AL(d_lcount, (size_t)ngrp * JP_WL + 1); AL(d_loff, (size_t)ngrp * JP_WL + 1); AL(d_nrem, ns); AL(d_nb, natom); AL(d_err, 1);
     AL(d_f64, 3 * (size_t)nslot); AL(d_ffix, 3 * (size_t)nslot);
     AL(d_ep64, 6 * (size_t)ngrp); AL(d_epfix, 6 * (size_t)ngrp);
     AL(d_s2sys, n2); AL(d_s3sys, n3); AL(d_s2a, n2); AL(d_s3a, n3); AL(d_s2d, 6 * (size_t)n2); AL(d_s3d, 6 * (size_t)n3);
@@ -455,17 +455,17 @@ int Engine::build_list() {
     hipLaunchKernelGGL(k_bbox, ngrp, 64, 0, st, ngrp, d_gcnt.p, d_ys.p, d_bb.p);
     CK(hipMemsetAsync(d_nrem.p, 0, 4 * (size_t)nsys, st));
     unsigned nb = (unsigned)((ngrp + 7) / 8 * 8);
-    hipLaunchKernelGGL(k_list<false>, nb, 64, 0, st, ngrp, d_grp_sys.p, d_gcnt.p, d_sp.p, d_bb.p, d_crank.p, d_cslot.p,
+    hipLaunchKernelGGL(k_list<false>, nb, 64 * JP_WL, 0, st, ngrp, d_grp_sys.p, d_gcnt.p, d_sp.p, d_bb.p, d_crank.p, d_cslot.p,
                        d_cnt.p, d_xr.p, d_imgs.p, d_mol.p, d_xptr.p, d_xidx.p, d_lcount.p, d_loff.p, d_ent.p, 0,
                        d_nrem.p, d_nb.p, d_err.p);
-    hipLaunchKernelGGL(k_scan, 1, 1024, 0, st, ngrp, d_lcount.p, d_loff.p);
+    hipLaunchKernelGGL(k_scan, 1, 1024, 0, st, ngrp * JP_WL, d_lcount.p, d_loff.p);
     int total = 0;
-    CK(hipMemcpyAsync(&total, d_loff.p + ngrp, 4, hipMemcpyDeviceToHost, st));
+    CK(hipMemcpyAsync(&total, d_loff.p + (size_t)ngrp * JP_WL, 4, hipMemcpyDeviceToHost, st));
     CK(hipStreamSynchronize(st));
     if (total < 0) return JP_IER_LIST;
     if ((size_t)total > d_ent.n) AL(d_ent, (size_t)(total * 1.25) + 1024);
     list_entries = total;
-    hipLaunchKernelGGL(k_list<true>, nb, 64, 0, st, ngrp, d_grp_sys.p, d_gcnt.p, d_sp.p, d_bb.p, d_crank.p, d_cslot.p,
+    hipLaunchKernelGGL(k_list<true>, nb, 64 * JP_WL, 0, st, ngrp, d_grp_sys.p, d_gcnt.p, d_sp.p, d_bb.p, d_crank.p, d_cslot.p,
                        d_cnt.p, d_xr.p, d_imgs.p, d_mol.p, d_xptr.p, d_xidx.p, d_lcount.p, d_loff.p, d_ent.p,
                        (int)d_ent.n, d_nrem.p, d_nb.p, d_err.p);
     CK(hipGetLastError());
@@ -506,22 +506,23 @@ int Engine::pair(int prec, bool energy) {
     unsigned nb = (unsigned)((ngrp + 7) / 8 * 8);
     int maxt = 0;
     for (auto &h : hs) maxt = std::max(maxt, h.ntyp);
+    constexpr int WP = 4;
     if (prec == JP_PREC_FP64) {
         size_t lds = sizeof(double2) * maxt * maxt;
         if (energy)
-            hipLaunchKernelGGL(k_pair_f64<true>, nb, 64, lds, st, ngrp, d_grp_sys.p, d_gcnt.p, d_sp.p, d_ys.p, d_qd.p,
-                               d_lj64.p, d_loff.p, d_lcount.p, d_ent.p, d_f64.p, d_ep64.p);
+            hipLaunchKernelGGL((k_pair_f64<true, WP>), nb, 64 * WP, lds, st, ngrp, d_grp_sys.p, d_gcnt.p, d_sp.p,
+                               d_ys.p, d_qd.p, d_lj64.p, d_loff.p, d_ent.p, d_f64.p, d_ep64.p);
         else
-            hipLaunchKernelGGL(k_pair_f64<false>, nb, 64, lds, st, ngrp, d_grp_sys.p, d_gcnt.p, d_sp.p, d_ys.p, d_qd.p,
-                               d_lj64.p, d_loff.p, d_lcount.p, d_ent.p, d_f64.p, d_ep64.p);
+            hipLaunchKernelGGL((k_pair_f64<false, WP>), nb, 64 * WP, lds, st, ngrp, d_grp_sys.p, d_gcnt.p, d_sp.p,
+                               d_ys.p, d_qd.p, d_lj64.p, d_loff.p, d_ent.p, d_f64.p, d_ep64.p);
     } else {
         size_t lds = sizeof(float2) * maxt * maxt;
         if (energy)
-            hipLaunchKernelGGL(k_pair_f32<true>, nb, 64, lds, st, ngrp, d_grp_sys.p, d_gcnt.p, d_sp.p, d_ys.p,
-                               d_lj32.p, d_loff.p, d_lcount.p, d_ent.p, d_ffix.p, d_epfix.p);
+            hipLaunchKernelGGL((k_pair_f32<true, WP>), nb, 64 * WP, lds, st, ngrp, d_grp_sys.p, d_gcnt.p, d_sp.p,
+                               d_ys.p, d_lj32.p, d_loff.p, d_ent.p, d_ffix.p, d_epfix.p);
         else
-            hipLaunchKernelGGL(k_pair_f32<false>, nb, 64, lds, st, ngrp, d_grp_sys.p, d_gcnt.p, d_sp.p, d_ys.p,
-                               d_lj32.p, d_loff.p, d_lcount.p, d_ent.p, d_ffix.p, d_epfix.p);
+            hipLaunchKernelGGL((k_pair_f32<false, WP>), nb, 64 * WP, lds, st, ngrp, d_grp_sys.p, d_gcnt.p, d_sp.p,
+                               d_ys.p, d_lj32.p, d_loff.p, d_ent.p, d_ffix.p, d_epfix.p);
     }
     CK(hipGetLastError());
     return 0;
@@ -719,16 +720,15 @@ int Engine::run_nve(int nsteps, int update, int nmon, int prec, jp_energy_log *l
 int Engine::get_list(int32_t *cell3, int64_t *npairs, int32_t *ptr, int32_t *idx, int64_t cap) {
     if (!have_list) return JP_EINVAL;
     CK(hipStreamSynchronize(st));
-    std::vector<int> lcount(ngrp), loff(ngrp + 1), perm(nslot);
+    std::vector<int> loff((size_t)ngrp * JP_WL + 1), perm(nslot);
     std::vector<Entry> ent(list_entries);
-    CK(hipMemcpy(lcount.data(), d_lcount.p, 4 * (size_t)ngrp, hipMemcpyDeviceToHost));
-    CK(hipMemcpy(loff.data(), d_loff.p, 4 * (size_t)(ngrp + 1), hipMemcpyDeviceToHost));
+    CK(hipMemcpy(loff.data(), d_loff.p, 4 * loff.size(), hipMemcpyDeviceToHost));
     CK(hipMemcpy(perm.data(), d_perm.p, 4 * (size_t)nslot, hipMemcpyDeviceToHost));
     if (list_entries) CK(hipMemcpy(ent.data(), d_ent.p, sizeof(Entry) * list_entries, hipMemcpyDeviceToHost));
     if (cell3) CK(hipMemcpy(cell3, d_refcell.p, 12 * (size_t)natom, hipMemcpyDeviceToHost));
     std::vector<std::vector<int>> nb(natom);
     for (int g = 0; g < ngrp; ++g)
-        for (int e = loff[g]; e < loff[g] + lcount[g]; ++e) {
+        for (int e = loff[(size_t)g * JP_WL]; e < loff[(size_t)(g + 1) * JP_WL]; ++e) {
             int jsl = (int)(ent[e].js & JP_SLOT_MASK);
             int aj = perm[jsl];
             uint64_t m = ent[e].m;
@@ -759,7 +759,7 @@ int Engine::count_pairs(int64_t *np, int64_t *nlj, int64_t *nlist) {
     if (!have_list) return JP_EINVAL;
     CK(hipMemsetAsync(d_u64.p, 0, 24, st));
     hipLaunchKernelGGL(k_count_pairs, ngrp, 64, 0, st, ngrp, d_grp_sys.p, d_gcnt.p, d_sp.p, d_ys.p, d_loff.p,
-                       d_lcount.p, d_ent.p, d_u64.p);
+                       d_ent.p, d_u64.p);
     unsigned long long c[3] = {0, 0, 0};
     CK(hipMemcpyAsync(c, d_u64.p, 24, hipMemcpyDeviceToHost, st));
     CK(hipStreamSynchronize(st));
