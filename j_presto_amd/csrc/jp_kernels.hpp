@@ -43,6 +43,7 @@ struct SysP {
     int slot0, nslot;                // padded slot range
     int grp0, ngrp;                  // i-group range
     int lj0, ntyp;                   // LJ table offset / size
+    int hbits;                       // nc = 2^hbits cells per edge
     int small;                       // some edge < 2*rlim: minimum image per pair
     double dt;
 };
@@ -86,6 +87,38 @@ __device__ __forceinline__ T wave_sum(T v) {
 
 __device__ __forceinline__ long long f2fix(double v, double scale) { return __double2ll_rn(v * scale); }
 
+// Index of cell (x,y,z) along the 3-D Hilbert curve of a 2^bits grid
+// (J. Skilling, "Programming the Hilbert curve", AIP Conf. Proc. 707, 2004:
+// axes -> transpose, then bit interleave).  Consecutive indices are face
+// neighbours and the curve is nested: index(2c+s) >> 3 == index(c) one level
+// up, so a cell's 2x2x2 sub-cells are visited together.
+__host__ __device__ inline uint32_t hilbert3(uint32_t x, uint32_t y, uint32_t z, int bits) {
+    uint32_t X[3] = {x, y, z};
+    const uint32_t M = 1u << (bits - 1);
+    for (uint32_t Q = M; Q > 1; Q >>= 1) {
+        const uint32_t P = Q - 1;
+        for (int i = 0; i < 3; ++i) {
+            if (X[i] & Q) {
+                X[0] ^= P;
+            } else {
+                const uint32_t t = (X[0] ^ X[i]) & P;
+                X[0] ^= t;
+                X[i] ^= t;
+            }
+        }
+    }
+    X[1] ^= X[0];
+    X[2] ^= X[1];
+    uint32_t t = 0;
+    for (uint32_t Q = M; Q > 1; Q >>= 1)
+        if (X[2] & Q) t ^= Q - 1;
+    X[0] ^= t; X[1] ^= t; X[2] ^= t;
+    uint32_t h = 0;
+    for (int b = bits - 1; b >= 0; --b)
+        for (int i = 0; i < 3; ++i) h = (h << 1) | ((X[i] >> b) & 1u);
+    return h;
+}
+
 // --------------------------------------------------------------------------
 // rebuild: binning
 // --------------------------------------------------------------------------
@@ -93,8 +126,8 @@ __device__ __forceinline__ long long f2fix(double v, double scale) { return __do
 // Wrap into the box and bin (celpar.f90:115-127 for md_run's cell numbering;
 // the same wrapped coordinate g is binned on the box-aligned grid).
 __global__ void k_bin(int natom, const int *__restrict__ atom_sys, const SysP *__restrict__ sp,
-                      const double *__restrict__ x, const int *__restrict__ crank, int *__restrict__ img,
-                      int *__restrict__ cellid, int *__restrict__ refcell, int *__restrict__ cnt,
+                      const double *__restrict__ x, int *__restrict__ img, int *__restrict__ cellid,
+                      int *__restrict__ subkey, int *__restrict__ refcell, int *__restrict__ cnt,
                       int *__restrict__ err) {
 #pragma clang fp contract(off)
     int a = blockIdx.x * blockDim.x + threadIdx.x;
@@ -108,12 +141,15 @@ __global__ void k_bin(int natom, const int *__restrict__ atom_sys, const SysP *_
         if (!(fabs(p) < 1e15)) { atomicOr(err, 1); f = 0; g = P.wall[k]; }
         img[3 * a + k] = (int)f;
         refcell[3 * a + k] = (int)((g - P.rorg[k]) * P.rr) + 1;
-        int ck = (int)floor((g - P.wall[k]) * P.icsz[k]);
-        ck = ck < 0 ? 0 : (ck >= P.nc[k] ? P.nc[k] - 1 : ck);
+        int ck = (int)floor((g - P.wall[k]) * P.icsz[k] * 2.0);   // half-cell resolution
+        ck = ck < 0 ? 0 : (ck >= 2 * P.nc[k] ? 2 * P.nc[k] - 1 : ck);
         c[k] = ck;
     }
-    int cell = crank[P.cell0 + (c[2] * P.nc[1] + c[1]) * P.nc[0] + c[0]];
+    // cells in Hilbert order; inside a cell, atoms ordered by sub-cell
+    const uint32_t h = hilbert3((uint32_t)c[0], (uint32_t)c[1], (uint32_t)c[2], P.hbits + 1);
+    int cell = P.cell0 + (int)(h >> 3);
     cellid[a] = cell;
+    subkey[a] = (int)(h & 7u);
     atomicAdd(&cnt[cell], 1);
 }
 
@@ -165,14 +201,19 @@ __global__ void k_place(int natom, const int *__restrict__ cellid, const int *__
     tmp[start[c] + p] = a;
 }
 
-// Deterministic order inside a cell: rank by atom index.
+// Deterministic order inside a cell: rank by (sub-cell, atom index).
 __global__ void k_rank(int natom, const int *__restrict__ atom_sys, const SysP *__restrict__ sp,
-                       const int *__restrict__ cellid, const int *__restrict__ start, const int *__restrict__ cnt,
-                       const int *__restrict__ tmp, int *__restrict__ inum, int *__restrict__ perm) {
+                       const int *__restrict__ cellid, const int *__restrict__ subkey, const int *__restrict__ start,
+                       const int *__restrict__ cnt, const int *__restrict__ tmp, int *__restrict__ inum,
+                       int *__restrict__ perm) {
     int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= natom) return;
     int c = cellid[a], b = start[c], n = cnt[c], r = 0;
-    for (int k = 0; k < n; ++k) r += tmp[b + k] < a;
+    const long long ka = ((long long)subkey[a] << 32) | a;
+    for (int k = 0; k < n; ++k) {
+        const int o = tmp[b + k];
+        r += (((long long)subkey[o] << 32) | o) < ka;
+    }
     const SysP &P = sp[atom_sys[a]];
     int slot = b + r - P.atom0 + P.slot0;
     inum[a] = slot;
@@ -244,18 +285,81 @@ __global__ void __launch_bounds__(64) k_bbox(int ngrp, const int *__restrict__ g
 // rebuild: the pair list (cutoff.f90 cutatm_PB + cutoff_PB semantics)
 // --------------------------------------------------------------------------
 //
-// One block of WL wavefronts per i-group; wave w takes every WL-th candidate
-// cell.  Candidate j atoms: every atom of every cell of the unwrapped cell
-// range around the group's bounding box grown by rlim, each with the periodic
-// shift of that unwrapped cell.  Lane l sets its bit for (j, shift) iff
+// One block of JP_WL wavefronts per i-group; wave w takes every JP_WL-th
+// candidate cell.  Candidate j atoms: every atom of every cell of the
+// unwrapped cell range around the group's bounding box grown by rlim, each
+// with the periodic shift of that unwrapped cell.  Lane l sets its bit for
+// (j, shift) iff
 //   (a) md_run's FP64 test passes on raw coordinates: d = xi - xj,
 //       d -= L*anint(d/L), dx*dx+dy*dy+dz*dz <= rlim^2 (cutoff.f90:464-469),
 //   (b) the shift is that pair's minimum image (each pair appears once),
 //   (c) j != i and (i,j) is not an excluded pair (cutoff.f90:50-96).
-// A cell's atoms are fetched with one coalesced load (lane k holds atom k)
-// and broadcast with readlane.  FILL=false counts entries per (group, wave);
-// FILL=true writes them, so a group's entries are contiguous.
+// Pipeline: k_list_cap (entries can not exceed candidates) -> scan ->
+// k_list (one pass; a cell chunk is loaded coalesced, every lane tests its
+// own j against the group box first, only survivors get the 64-lane test)
+// -> scan -> k_list_compact.
 #define JP_WL 4
+
+struct CandRange {
+    int ulo[3], n[3], ncand;
+    double lo[3], hi[3], rl;
+    bool bad;
+};
+
+__device__ __forceinline__ CandRange cand_range(const SysP &P, const double *__restrict__ bb, int g) {
+    CandRange c;
+    c.rl = sqrt(P.rlim2) * (1.0 + 1e-12) + 1e-9;  // conservative pruning radius
+    c.bad = false;
+    for (int k = 0; k < 3; ++k) {
+        c.lo[k] = bb[6 * g + k];
+        c.hi[k] = bb[6 * g + 3 + k];
+        double a = floor((c.lo[k] - c.rl - P.wall[k]) * P.icsz[k]);
+        double b = floor((c.hi[k] + c.rl - P.wall[k]) * P.icsz[k]);
+        // non-finite or runaway coordinates: report and do nothing (bounded loops)
+        if (!(a >= -4.0 * P.nc[k] - 64.0) || !(b <= 5.0 * P.nc[k] + 64.0) || !(b - a <= 3.0 * P.nc[k] + 16.0))
+            c.bad = true;
+        c.ulo[k] = c.bad ? 0 : (int)a;
+        c.n[k] = c.bad ? 0 : (int)(b - a) + 1;
+    }
+    c.ncand = c.bad ? 0 : c.n[0] * c.n[1] * c.n[2];
+    return c;
+}
+
+// Candidate q -> (cell, shift code) or -1 when the unwrapped cell is farther
+// than rlim from the group box.
+__device__ __forceinline__ int cand_cell(const SysP &P, const CandRange &c, const int *__restrict__ crank, int q,
+                                         int &sx, int &sy, int &sz) {
+    const int ux = c.ulo[0] + q % c.n[0], uy = c.ulo[1] + (q / c.n[0]) % c.n[1], uz = c.ulo[2] + q / (c.n[0] * c.n[1]);
+    double gx = fmax(0.0, fmax(P.wall[0] + ux * P.csz[0] - c.hi[0], c.lo[0] - (P.wall[0] + (ux + 1) * P.csz[0])));
+    double gy = fmax(0.0, fmax(P.wall[1] + uy * P.csz[1] - c.hi[1], c.lo[1] - (P.wall[1] + (uy + 1) * P.csz[1])));
+    double gz = fmax(0.0, fmax(P.wall[2] + uz * P.csz[2] - c.hi[2], c.lo[2] - (P.wall[2] + (uz + 1) * P.csz[2])));
+    if (gx * gx + gy * gy + gz * gz > c.rl * c.rl) return -1;
+    sx = (ux >= 0) ? ux / P.nc[0] : -((-ux + P.nc[0] - 1) / P.nc[0]);
+    sy = (uy >= 0) ? uy / P.nc[1] : -((-uy + P.nc[1] - 1) / P.nc[1]);
+    sz = (uz >= 0) ? uz / P.nc[2] : -((-uz + P.nc[2] - 1) / P.nc[2]);
+    const int cx = ux - sx * P.nc[0], cy = uy - sy * P.nc[1], cz = uz - sz * P.nc[2];
+    return crank[P.cell0 + (cz * P.nc[1] + cy) * P.nc[0] + cx];
+}
+
+// Upper bound of each (group, wave)'s entries: its candidate atoms.
+__global__ void __launch_bounds__(64)
+k_list_cap(int ngrp, const int *__restrict__ grp_sys, const SysP *__restrict__ sp, const double *__restrict__ bb,
+           const int *__restrict__ crank, const int *__restrict__ ccnt, int *__restrict__ cap,
+           int *__restrict__ err) {
+    const int gw = blockIdx.x * 64 + threadIdx.x;   // one thread per (group, wave)
+    if (gw >= ngrp * JP_WL) return;
+    const int g = gw / JP_WL, w = gw % JP_WL;
+    const SysP &P = sp[grp_sys[g]];
+    const CandRange c = cand_range(P, bb, g);
+    if (c.bad) atomicOr(err, 1);
+    int n = 0;
+    for (int q = w; q < c.ncand; q += JP_WL) {
+        int sx, sy, sz;
+        int cell = cand_cell(P, c, crank, q, sx, sy, sz);
+        if (cell >= 0) n += ccnt[cell];
+    }
+    cap[gw] = n;
+}
 
 __device__ __forceinline__ double bcast_d(double v, int k) {
     int lo = __builtin_amdgcn_readlane(__double2loint(v), k);
@@ -264,14 +368,13 @@ __device__ __forceinline__ double bcast_d(double v, int k) {
 }
 __device__ __forceinline__ int bcast_i(int v, int k) { return __builtin_amdgcn_readlane(v, k); }
 
-template <bool FILL>
 __global__ void __launch_bounds__(64 * JP_WL)
 k_list(int ngrp, const int *__restrict__ grp_sys, const int *__restrict__ gcnt, const SysP *__restrict__ sp,
        const double *__restrict__ bb, const int *__restrict__ crank, const int *__restrict__ cslot,
        const int *__restrict__ ccnt, const double4 *__restrict__ xr, const int4 *__restrict__ imgs,
        const int *__restrict__ mol, const int *__restrict__ xptr, const int *__restrict__ xidx,
-       int *__restrict__ lcount, const int *__restrict__ loff, Entry *__restrict__ ent, int lcap,
-       int *__restrict__ nremoved, int *__restrict__ nbcount, int *__restrict__ err) {
+       const int *__restrict__ coff, Entry *__restrict__ tmp, int *__restrict__ lcount,
+       int *__restrict__ nremoved, int *__restrict__ nbcount) {
 #pragma clang fp contract(off)
     __shared__ int s_nb[JP_WL][64];
     __shared__ int s_nrem[JP_WL];
@@ -286,57 +389,40 @@ k_list(int ngrp, const int *__restrict__ grp_sys, const int *__restrict__ gcnt, 
     const int ai = act ? ii.w : -1;
     const int moli = act ? mol[ai] : -2;
     const int xb = act ? xptr[ai] : 0, xe = act ? xptr[ai + 1] : 0;
-    const double rl = sqrt(P.rlim2) * (1.0 + 1e-12) + 1e-9;  // conservative pruning radius
-    double lo[3], hi[3];
-    int ulo[3], uhi[3];
-    bool bad = false;
-    for (int k = 0; k < 3; ++k) {
-        lo[k] = bb[6 * g + k];
-        hi[k] = bb[6 * g + 3 + k];
-        double a = floor((lo[k] - rl - P.wall[k]) * P.icsz[k]);
-        double b = floor((hi[k] + rl - P.wall[k]) * P.icsz[k]);
-        // non-finite or runaway coordinates: report and do nothing (bounded loops)
-        if (!(a >= -4.0 * P.nc[k] - 64.0) || !(b <= 5.0 * P.nc[k] + 64.0) || !(b - a <= 3.0 * P.nc[k] + 16.0))
-            bad = true;
-        ulo[k] = bad ? 0 : (int)a;
-        uhi[k] = bad ? -1 : (int)b;
-    }
-    if (bad) {
-        if (threadIdx.x == 0) {
-            atomicOr(err, 1);
-            if (!FILL)
-                for (int q = 0; q < JP_WL; ++q) lcount[g * JP_WL + q] = 0;
-        }
-        return;
-    }
-    const int nx = uhi[0] - ulo[0] + 1, ny = uhi[1] - ulo[1] + 1, nz = uhi[2] - ulo[2] + 1;
-    const int ncand = nx * ny * nz;
-    int pos = FILL ? loff[g * JP_WL + w] : 0;
+    const CandRange c = cand_range(P, bb, g);
+    const double rl2 = c.rl * c.rl;
+    const int pos0 = coff[g * JP_WL + w];
+    int pos = pos0;
     int nrem = 0, nb = 0;
-    for (int q = w; q < ncand; q += JP_WL) {
-        const int ux = ulo[0] + q % nx, uy = ulo[1] + (q / nx) % ny, uz = ulo[2] + q / (nx * ny);
-        double gx = fmax(0.0, fmax(P.wall[0] + ux * P.csz[0] - hi[0], lo[0] - (P.wall[0] + (ux + 1) * P.csz[0])));
-        double gy = fmax(0.0, fmax(P.wall[1] + uy * P.csz[1] - hi[1], lo[1] - (P.wall[1] + (uy + 1) * P.csz[1])));
-        double gz = fmax(0.0, fmax(P.wall[2] + uz * P.csz[2] - hi[2], lo[2] - (P.wall[2] + (uz + 1) * P.csz[2])));
-        if (gx * gx + gy * gy + gz * gz > rl * rl) continue;
-        int sx = (ux >= 0) ? ux / P.nc[0] : -((-ux + P.nc[0] - 1) / P.nc[0]);
-        int sy = (uy >= 0) ? uy / P.nc[1] : -((-uy + P.nc[1] - 1) / P.nc[1]);
-        int sz = (uz >= 0) ? uz / P.nc[2] : -((-uz + P.nc[2] - 1) / P.nc[2]);
-        const int cx = ux - sx * P.nc[0], cy = uy - sy * P.nc[1], cz = uz - sz * P.nc[2];
-        const int cell = crank[P.cell0 + (cz * P.nc[1] + cy) * P.nc[0] + cx];
+    for (int q = w; q < c.ncand; q += JP_WL) {
+        int sx, sy, sz;
+        const int cell = cand_cell(P, c, crank, q, sx, sy, sz);
+        if (cell < 0) continue;
         const int j0 = cslot[cell], nj = ccnt[cell];
         const uint32_t code = (uint32_t)((sx + 1) * 9 + (sy + 1) * 3 + (sz + 1));
+        const double shx = P.L[0] * sx, shy = P.L[1] * sy, shz = P.L[2] * sz;
         for (int base = 0; base < nj; base += 64) {
             const int m = min(64, nj - base);
             double4 cx4 = make_double4(0, 0, 0, 0);
             int4 ci4 = make_int4(0, 0, 0, 0);
             int cmol = -3;
+            bool near = false;
             if (l < m) {
                 cx4 = xr[j0 + base + l];
                 ci4 = imgs[j0 + base + l];
                 cmol = mol[ci4.w];
+                // lane-parallel pre-test: this j (shifted image) against the group box
+                double y0 = cx4.x - P.L[0] * ci4.x + shx, y1 = cx4.y - P.L[1] * ci4.y + shy,
+                       y2 = cx4.z - P.L[2] * ci4.z + shz;
+                double t0 = fmax(0.0, fmax(c.lo[0] - y0, y0 - c.hi[0]));
+                double t1 = fmax(0.0, fmax(c.lo[1] - y1, y1 - c.hi[1]));
+                double t2 = fmax(0.0, fmax(c.lo[2] - y2, y2 - c.hi[2]));
+                near = t0 * t0 + t1 * t1 + t2 * t2 <= rl2 * (1.0 + 1e-9) + 1e-9;
             }
-            for (int k = 0; k < m; ++k) {
+            uint64_t todo = __ballot(near);
+            while (todo) {
+                const int k = __builtin_ctzll(todo);
+                todo &= todo - 1;
                 const int js = j0 + base + k;
                 const double xjx = bcast_d(cx4.x, k), xjy = bcast_d(cx4.y, k), xjz = bcast_d(cx4.z, k);
                 const int ijx = bcast_i(ci4.x, k), ijy = bcast_i(ci4.y, k), ijz = bcast_i(ci4.z, k);
@@ -356,12 +442,12 @@ k_list(int ngrp, const int *__restrict__ grp_sys, const int *__restrict__ gcnt, 
                 }
                 const uint64_t mk = __ballot(in);
                 if (mk) {
-                    if (FILL && l == 0 && pos < lcap) {
+                    if (l == 0) {
                         Entry E;
                         E.js = (uint32_t)js | (code << JP_SLOT_BITS);
                         E.pad = 0;
                         E.m = mk;
-                        ent[pos] = E;
+                        tmp[pos] = E;
                     }
                     ++pos;
                     nb += in ? 1 : 0;
@@ -369,21 +455,27 @@ k_list(int ngrp, const int *__restrict__ grp_sys, const int *__restrict__ gcnt, 
             }
         }
     }
-    if (!FILL) {
-        if (l == 0) lcount[g * JP_WL + w] = pos;
-        int t = wave_sum(nrem);
-        s_nb[w][l] = nb;
-        if (l == 0) s_nrem[w] = t;
-        __syncthreads();
-        if (w == 0) {
-            int tot = 0, nbt = 0;
-            for (int q = 0; q < JP_WL; ++q) { tot += s_nrem[q]; nbt += s_nb[q][l]; }
-            if (l == 0 && tot) atomicAdd(&nremoved[grp_sys[g]], tot);
-            if (act) nbcount[ai] = nbt;
-        }
-    } else {
-        if (l == 0 && pos != loff[g * JP_WL + w] + lcount[g * JP_WL + w]) atomicOr(err, 2);
+    if (l == 0) lcount[g * JP_WL + w] = pos - pos0;
+    int t = wave_sum(nrem);
+    s_nb[w][l] = nb;
+    if (l == 0) s_nrem[w] = t;
+    __syncthreads();
+    if (w == 0) {
+        int tot = 0, nbt = 0;
+        for (int q = 0; q < JP_WL; ++q) { tot += s_nrem[q]; nbt += s_nb[q][l]; }
+        if (l == 0 && tot) atomicAdd(&nremoved[grp_sys[g]], tot);
+        if (act) nbcount[ai] = nbt;
     }
+}
+
+// tmp (per (group,wave) capacity-sized segments) -> contiguous entries.
+__global__ void __launch_bounds__(256)
+k_list_compact(int ngw, const int *__restrict__ coff, const int *__restrict__ lcount, const int *__restrict__ loff,
+               const Entry *__restrict__ tmp, Entry *__restrict__ ent) {
+    const int gw = blockIdx.x;
+    if (gw >= ngw) return;
+    const int n = lcount[gw], s = coff[gw], d = loff[gw];
+    for (int k = threadIdx.x; k < n; k += blockDim.x) ent[d + k] = tmp[s + k];
 }
 
 // --------------------------------------------------------------------------

@@ -58,19 +58,6 @@ struct DBuf {
 
 inline unsigned grid1(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
 
-uint64_t morton3(uint32_t x, uint32_t y, uint32_t z) {
-    auto spread = [](uint64_t v) {
-        v &= 0x1fffff;
-        v = (v | v << 32) & 0x1f00000000ffffULL;
-        v = (v | v << 16) & 0x1f0000ff0000ffULL;
-        v = (v | v << 8) & 0x100f00f00f00f00fULL;
-        v = (v | v << 4) & 0x10c30c30c30c30c3ULL;
-        v = (v | v << 2) & 0x1249249249249249ULL;
-        return v;
-    };
-    return spread(x) | spread(y) << 1 | spread(z) << 2;
-}
-
 struct HostSys {
     int n = 0, ntyp = 0, nfrag = 1, nfrg = 1, lmd = 0;
     double lambda = 0.0, dself[3] = {0, 0, 0};
@@ -105,13 +92,15 @@ struct Engine {
     DBuf<double> d_efac;
     DBuf<double2> d_lj64;
     DBuf<float2> d_lj32;
+    DBuf<int> d_subkey;
     DBuf<int> d_img, d_cellid, d_refcell, d_cnt, d_start, d_fill, d_tmp, d_inum, d_perm, d_cslot;
     DBuf<JAtom> d_ys;
     DBuf<double> d_qd, d_bb;
     DBuf<double4> d_xr;
     DBuf<int4> d_imgs;
     DBuf<int> d_lcount, d_loff, d_nrem, d_nb, d_err;
-    DBuf<Entry> d_ent;
+    DBuf<Entry> d_ent, d_etmp;
+    DBuf<int> d_cap, d_coff;
     DBuf<double> d_f64, d_ep64;
     DBuf<long long> d_ffix, d_epfix;
     DBuf<int> d_s2sys, d_s3sys;
@@ -130,10 +119,10 @@ struct Engine {
         d_mass.release(); d_imass.release(); d_chg.release(); d_grad.release(); d_eexc.release(); d_ekin.release();
         d_tflag.release(); d_mol.release(); d_xptr.release(); d_xidx.release(); d_eptr.release(); d_eidx.release();
         d_efac.release(); d_lj64.release(); d_lj32.release(); d_img.release(); d_cellid.release();
-        d_refcell.release(); d_cnt.release(); d_start.release(); d_fill.release(); d_tmp.release(); d_inum.release();
+        d_refcell.release(); d_subkey.release(); d_cnt.release(); d_start.release(); d_fill.release(); d_tmp.release(); d_inum.release();
         d_perm.release(); d_cslot.release(); d_ys.release(); d_qd.release(); d_bb.release(); d_xr.release();
         d_imgs.release(); d_lcount.release(); d_loff.release(); d_nrem.release(); d_nb.release(); d_err.release();
-        d_ent.release(); d_f64.release(); d_ep64.release(); d_ffix.release(); d_epfix.release();
+        d_ent.release(); d_etmp.release(); d_cap.release(); d_coff.release(); d_f64.release(); d_ep64.release(); d_ffix.release(); d_epfix.release();
         d_s2sys.release(); d_s3sys.release(); d_s2a.release(); d_s3a.release(); d_s2d.release(); d_s3d.release();
         d_u64.release();
     }
@@ -193,10 +182,17 @@ int Engine::init(const jp_sys_desc *descs, int ns) {
             P.invL[k] = 1.0 / P.L[k];
             P.wall[k] = d.celwal[2 * k];
             if (!(P.L[k] >= 2.0 * rc)) return JP_EINVAL;  // inpmd.f90:694-698
-            int nc = std::max(1, (int)std::floor(P.L[k] / (rlim * 0.5)));
-            P.nc[k] = nc;
-            P.csz[k] = P.L[k] / nc;
-            P.icsz[k] = nc / P.L[k];
+        }
+        // power-of-2 box-aligned grid, cells of 4.5-9 A, in Hilbert order
+        {
+            double lmin = std::min(P.L[0], std::min(P.L[1], P.L[2]));
+            int hb = std::max(1, (int)std::floor(std::log2(lmin / 4.5)));
+            P.hbits = hb;
+            for (int k = 0; k < 3; ++k) {
+                P.nc[k] = 1 << hb;
+                P.csz[k] = P.L[k] / P.nc[k];
+                P.icsz[k] = P.nc[k] / P.L[k];
+            }
         }
         // md_run's cell numbering (incel.f90:29-57, mk_c, celpar.f90:93-108)
         {
@@ -240,18 +236,12 @@ int Engine::init(const jp_sys_desc *descs, int ns) {
                 P.cls[a * 2 + b] = d.n_matrix[bb * d.nfrag + aa] - 1;
                 P.fac[a * 2 + b] = 1.0;
             }
-        // Morton order of the box cells
+        // Hilbert rank of every box cell (the same curve k_bin bins on)
         {
             int nx = P.nc[0], ny = P.nc[1], nz = P.nc[2];
-            std::vector<std::pair<uint64_t, int>> key;
-            key.reserve(P.ncell);
             for (int cz = 0; cz < nz; ++cz)
                 for (int cy = 0; cy < ny; ++cy)
-                    for (int cx = 0; cx < nx; ++cx) key.push_back({morton3(cx, cy, cz), (cz * ny + cy) * nx + cx});
-            std::sort(key.begin(), key.end());
-            std::vector<int> rk(P.ncell);
-            for (int r = 0; r < P.ncell; ++r) rk[key[r].second] = cell0 + r;
-            crank.insert(crank.end(), rk.begin(), rk.end());
+                    for (int cx = 0; cx < nx; ++cx) crank.push_back(cell0 + (int)hilbert3(cx, cy, cz, P.hbits));
             for (int r = 0; r < P.ncell; ++r) cell_sys_h.push_back(s);
         }
         // LJ table, AMBER combination (inptpl.f90:397-419)
@@ -370,11 +360,12 @@ int Engine::init(const jp_sys_desc *descs, int ns) {
     AL(d_tflag, natom); AL(d_mol, natom); AL(d_xptr, natom + 1); AL(d_xidx, xidx.size());
     AL(d_eptr, natom + 1); AL(d_eidx, eidx.size()); AL(d_efac, 3 * (size_t)ns);
     AL(d_lj64, lj64.size()); AL(d_lj32, lj32.size());
-    AL(d_img, 3 * (size_t)natom); AL(d_cellid, natom); AL(d_refcell, 3 * (size_t)natom);
+    AL(d_img, 3 * (size_t)natom); AL(d_cellid, natom); AL(d_subkey, natom); AL(d_refcell, 3 * (size_t)natom);
     AL(d_cnt, ncell + 1); AL(d_start, ncell + 1); AL(d_fill, ncell); AL(d_tmp, natom);
     AL(d_inum, natom); AL(d_perm, nslot); AL(d_cslot, ncell);
     AL(d_ys, nslot); AL(d_qd, nslot); AL(d_bb, 6 * (size_t)ngrp); AL(d_xr, nslot); AL(d_imgs, nslot);
-    AL(d_lcount, (size_t)ngrp * JP_WL + 1); AL(d_loff, (size_t)ngrp * JP_WL + 1); AL(d_nrem, ns); AL(d_nb, natom); AL(d_err, 1);
+    AL(d_lcount, (size_t)ngrp * JP_WL + 1); AL(d_loff, (size_t)ngrp * JP_WL + 1);
+    AL(d_cap, (size_t)ngrp * JP_WL + 1); AL(d_coff, (size_t)ngrp * JP_WL + 1); AL(d_nrem, ns); AL(d_nb, natom); AL(d_err, 1);
     AL(d_f64, 3 * (size_t)nslot); AL(d_ffix, 3 * (size_t)nslot);
     AL(d_ep64, 6 * (size_t)ngrp); AL(d_epfix, 6 * (size_t)ngrp);
     AL(d_s2sys, n2); AL(d_s3sys, n3); AL(d_s2a, n2); AL(d_s3a, n3); AL(d_s2d, 6 * (size_t)n2); AL(d_s3d, 6 * (size_t)n3);
@@ -441,33 +432,40 @@ int Engine::build_list() {
     CK(hipEventRecord(ev[2], st));
     CK(hipMemsetAsync(d_cnt.p, 0, 4 * (size_t)(ncell + 1), st));
     CK(hipMemsetAsync(d_err.p, 0, 4, st));
-    hipLaunchKernelGGL(k_bin, grid1(natom, 256), 256, 0, st, natom, d_atom_sys.p, d_sp.p, d_x.p, d_crank.p, d_img.p,
-                       d_cellid.p, d_refcell.p, d_cnt.p, d_err.p);
+    hipLaunchKernelGGL(k_bin, grid1(natom, 256), 256, 0, st, natom, d_atom_sys.p, d_sp.p, d_x.p, d_img.p,
+                       d_cellid.p, d_subkey.p, d_refcell.p, d_cnt.p, d_err.p);
     hipLaunchKernelGGL(k_scan, 1, 1024, 0, st, ncell, d_cnt.p, d_start.p);
     CK(hipMemsetAsync(d_fill.p, 0, 4 * (size_t)ncell, st));
     hipLaunchKernelGGL(k_place, grid1(natom, 256), 256, 0, st, natom, d_cellid.p, d_start.p, d_fill.p, d_tmp.p);
     CK(hipMemsetAsync(d_perm.p, 0xff, 4 * (size_t)nslot, st));
-    hipLaunchKernelGGL(k_rank, grid1(natom, 256), 256, 0, st, natom, d_atom_sys.p, d_sp.p, d_cellid.p, d_start.p,
-                       d_cnt.p, d_tmp.p, d_inum.p, d_perm.p);
+    hipLaunchKernelGGL(k_rank, grid1(natom, 256), 256, 0, st, natom, d_atom_sys.p, d_sp.p, d_cellid.p, d_subkey.p,
+                       d_start.p, d_cnt.p, d_tmp.p, d_inum.p, d_perm.p);
     hipLaunchKernelGGL(k_cslot, grid1(ncell, 256), 256, 0, st, ncell, d_cell_sys.p, d_sp.p, d_start.p, d_cslot.p);
     hipLaunchKernelGGL(k_fill_sorted, grid1(nslot, 256), 256, 0, st, nslot, d_slot_sys.p, d_sp.p, d_perm.p, d_x.p,
                        d_img.p, d_chg.p, d_tflag.p, d_ys.p, d_qd.p, d_xr.p, d_imgs.p);
     hipLaunchKernelGGL(k_bbox, ngrp, 64, 0, st, ngrp, d_gcnt.p, d_ys.p, d_bb.p);
     CK(hipMemsetAsync(d_nrem.p, 0, 4 * (size_t)nsys, st));
+    const int ngw = ngrp * JP_WL;
+    hipLaunchKernelGGL(k_list_cap, grid1(ngw, 64), 64, 0, st, ngrp, d_grp_sys.p, d_sp.p, d_bb.p, d_crank.p, d_cnt.p,
+                       d_cap.p, d_err.p);
+    hipLaunchKernelGGL(k_scan, 1, 1024, 0, st, ngw, d_cap.p, d_coff.p);
+    int captot = 0;
+    CK(hipMemcpyAsync(&captot, d_coff.p + ngw, 4, hipMemcpyDeviceToHost, st));
+    CK(hipStreamSynchronize(st));
+    if (captot < 0) return JP_IER_LIST;
+    if ((size_t)captot > d_etmp.n) AL(d_etmp, (size_t)(captot * 1.2) + 1024);
     unsigned nb = (unsigned)((ngrp + 7) / 8 * 8);
-    hipLaunchKernelGGL(k_list<false>, nb, 64 * JP_WL, 0, st, ngrp, d_grp_sys.p, d_gcnt.p, d_sp.p, d_bb.p, d_crank.p, d_cslot.p,
-                       d_cnt.p, d_xr.p, d_imgs.p, d_mol.p, d_xptr.p, d_xidx.p, d_lcount.p, d_loff.p, d_ent.p, 0,
-                       d_nrem.p, d_nb.p, d_err.p);
-    hipLaunchKernelGGL(k_scan, 1, 1024, 0, st, ngrp * JP_WL, d_lcount.p, d_loff.p);
+    hipLaunchKernelGGL(k_list, nb, 64 * JP_WL, 0, st, ngrp, d_grp_sys.p, d_gcnt.p, d_sp.p, d_bb.p, d_crank.p,
+                       d_cslot.p, d_cnt.p, d_xr.p, d_imgs.p, d_mol.p, d_xptr.p, d_xidx.p, d_coff.p, d_etmp.p,
+                       d_lcount.p, d_nrem.p, d_nb.p);
+    hipLaunchKernelGGL(k_scan, 1, 1024, 0, st, ngw, d_lcount.p, d_loff.p);
     int total = 0;
-    CK(hipMemcpyAsync(&total, d_loff.p + (size_t)ngrp * JP_WL, 4, hipMemcpyDeviceToHost, st));
+    CK(hipMemcpyAsync(&total, d_loff.p + ngw, 4, hipMemcpyDeviceToHost, st));
     CK(hipStreamSynchronize(st));
     if (total < 0) return JP_IER_LIST;
     if ((size_t)total > d_ent.n) AL(d_ent, (size_t)(total * 1.25) + 1024);
     list_entries = total;
-    hipLaunchKernelGGL(k_list<true>, nb, 64 * JP_WL, 0, st, ngrp, d_grp_sys.p, d_gcnt.p, d_sp.p, d_bb.p, d_crank.p, d_cslot.p,
-                       d_cnt.p, d_xr.p, d_imgs.p, d_mol.p, d_xptr.p, d_xidx.p, d_lcount.p, d_loff.p, d_ent.p,
-                       (int)d_ent.n, d_nrem.p, d_nb.p, d_err.p);
+    hipLaunchKernelGGL(k_list_compact, ngw, 256, 0, st, ngw, d_coff.p, d_lcount.p, d_loff.p, d_etmp.p, d_ent.p);
     CK(hipGetLastError());
     CK(hipEventRecord(ev[3], st));
     std::vector<int> nrem(nsys);
