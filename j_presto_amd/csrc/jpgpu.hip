@@ -15,7 +15,7 @@
 #include <vector>
 
 #include "../../include/jpgpu.h"
-#include "jp_kernels.hpp"
+#include "jp_pair.hpp"
 
 #define CK(x)                                                          \
     do {                                                               \
@@ -66,7 +66,7 @@ struct HostSys {
 };
 
 struct Engine {
-    int nsys = 0, natom = 0, nslot = 0, ngrp = 0, ncell = 0;
+    int nsys = 0, natom = 0, nslot = 0, nclu = 0, ncell = 0;
     int n2 = 0, n3 = 0;  // SHAKE groups
     int prec_built = -1;
     bool have_list = false;
@@ -74,18 +74,18 @@ struct Engine {
     int shake_maxit = 1000;
     std::vector<HostSys> hs;
     std::vector<SysP> sp;
-    std::vector<int> atom_sys_h, slot_sys_h, grp_sys_h, gcnt_h, cell_sys_h;
+    std::vector<int> atom_sys_h, slot_sys_h, cell_sys_h, clu_sys_h;
     std::vector<double> mass_h;
     hipStream_t st = nullptr;
     hipEvent_t ev[8];
     double t_total = 0, t_pair = 0, t_list = 0, t_step = 0;
     int n_pair = 0, n_list = 0;
     long long idloop = 0;          // md.f90 idloop: steps done, kept across run calls
-    long long list_entries = 0;
+    long long list_cap = 0;
 
     // device arrays
     DBuf<SysP> d_sp;
-    DBuf<int> d_atom_sys, d_slot_sys, d_grp_sys, d_gcnt, d_cell_sys, d_crank;
+    DBuf<int> d_atom_sys, d_slot_sys, d_cell_sys, d_crank, d_ccoord, d_clu_sys;
     DBuf<double> d_x, d_v, d_xold, d_vback, d_mass, d_imass, d_chg, d_grad, d_eexc, d_ekin;
     DBuf<int> d_tflag, d_mol, d_xptr, d_xidx, d_eptr;
     DBuf<int2> d_eidx;
@@ -95,14 +95,16 @@ struct Engine {
     DBuf<int> d_subkey;
     DBuf<int> d_img, d_cellid, d_refcell, d_cnt, d_start, d_fill, d_tmp, d_inum, d_perm, d_cslot;
     DBuf<JAtom> d_ys;
-    DBuf<double> d_qd, d_bb;
-    DBuf<double4> d_xr;
+    DBuf<double> d_qd, d_cbb;
+    DBuf<double4> d_xr, d_ccen;
     DBuf<int4> d_imgs;
-    DBuf<int> d_lcount, d_loff, d_nrem, d_nb, d_err;
-    DBuf<Entry> d_ent, d_etmp;
+    DBuf<int> d_smol, d_nrem, d_nb, d_err;
+    DBuf<unsigned> d_cellext, d_sysext;
+    DBuf<int2> d_lcnt;
+    DBuf<uint2> d_ent;
     DBuf<int> d_cap, d_coff;
-    DBuf<double> d_f64, d_ep64;
-    DBuf<long long> d_ffix, d_epfix;
+    DBuf<double> d_gslot, d_ep64;
+    DBuf<long long> d_epfix;
     DBuf<int> d_s2sys, d_s3sys;
     DBuf<int4> d_s2a, d_s3a;
     DBuf<double> d_s2d, d_s3d;
@@ -114,15 +116,16 @@ struct Engine {
             for (auto &e : ev) hipEventDestroy(e);
             hipStreamDestroy(st);
         }
-        d_sp.release(); d_atom_sys.release(); d_slot_sys.release(); d_grp_sys.release(); d_gcnt.release();
-        d_cell_sys.release(); d_crank.release(); d_x.release(); d_v.release(); d_xold.release(); d_vback.release();
+        d_sp.release(); d_atom_sys.release(); d_slot_sys.release();
+        d_cell_sys.release(); d_crank.release(); d_ccoord.release(); d_clu_sys.release(); d_x.release(); d_v.release(); d_xold.release(); d_vback.release();
         d_mass.release(); d_imass.release(); d_chg.release(); d_grad.release(); d_eexc.release(); d_ekin.release();
         d_tflag.release(); d_mol.release(); d_xptr.release(); d_xidx.release(); d_eptr.release(); d_eidx.release();
         d_efac.release(); d_lj64.release(); d_lj32.release(); d_img.release(); d_cellid.release();
         d_refcell.release(); d_subkey.release(); d_cnt.release(); d_start.release(); d_fill.release(); d_tmp.release(); d_inum.release();
-        d_perm.release(); d_cslot.release(); d_ys.release(); d_qd.release(); d_bb.release(); d_xr.release();
-        d_imgs.release(); d_lcount.release(); d_loff.release(); d_nrem.release(); d_nb.release(); d_err.release();
-        d_ent.release(); d_etmp.release(); d_cap.release(); d_coff.release(); d_f64.release(); d_ep64.release(); d_ffix.release(); d_epfix.release();
+        d_perm.release(); d_cslot.release(); d_ys.release(); d_qd.release(); d_cbb.release(); d_xr.release();
+        d_ccen.release(); d_imgs.release(); d_smol.release(); d_nrem.release(); d_nb.release(); d_err.release();
+        d_cellext.release(); d_sysext.release(); d_lcnt.release();
+        d_ent.release(); d_cap.release(); d_coff.release(); d_gslot.release(); d_ep64.release(); d_epfix.release();
         d_s2sys.release(); d_s3sys.release(); d_s2a.release(); d_s3a.release(); d_s2d.release(); d_s3d.release();
         d_u64.release();
     }
@@ -134,6 +137,11 @@ struct Engine {
     }
     int build_list();
     int refresh_sorted();
+    ExcessArgs excess_args() const {
+        ExcessArgs a;
+        a.perm = d_perm.p; a.x = d_x.p; a.chg = d_chg.p; a.eptr = d_eptr.p; a.eidx = d_eidx.p; a.efac = d_efac.p;
+        return a;
+    }
     int pair(int prec, bool energy);
     int set_factors(int onehot_class);  // -1: run factors (lambda), k: one-hot class k
     int nonbond(const double *cord, double *grad, double *eelec, double *evdw, int prec);
@@ -162,7 +170,7 @@ int Engine::init(const jp_sys_desc *descs, int ns) {
     std::vector<int> s2sys, s3sys;
     std::vector<int4> s2a, s3a;
     std::vector<double> s2d, s3d;
-    std::vector<int> crank;
+    std::vector<int> crank, ccoord;
     int atom0 = 0, slot0 = 0, cell0 = 0, lj0 = 0;
     shake_tol = descs[0].shake_tol > 0 ? descs[0].shake_tol : 1e-6;
     shake_maxit = descs[0].shake_maxit > 0 ? descs[0].shake_maxit : 1000;
@@ -186,7 +194,7 @@ int Engine::init(const jp_sys_desc *descs, int ns) {
         // power-of-2 box-aligned grid, cells of 4.5-9 A, in Hilbert order
         {
             double lmin = std::min(P.L[0], std::min(P.L[1], P.L[2]));
-            int hb = std::max(1, (int)std::floor(std::log2(lmin / 4.5)));
+            int hb = std::min(10, std::max(1, (int)std::floor(std::log2(lmin / 4.5))));
             P.hbits = hb;
             for (int k = 0; k < 3; ++k) {
                 P.nc[k] = 1 << hb;
@@ -225,7 +233,6 @@ int Engine::init(const jp_sys_desc *descs, int ns) {
         P.atom0 = atom0; P.natom = n;
         int npad = (n + 63) / 64 * 64;
         P.slot0 = slot0; P.nslot = npad;
-        P.grp0 = slot0 / 64; P.ngrp = npad / 64;
         P.cell0 = cell0; P.ncell = P.nc[0] * P.nc[1] * P.nc[2];
         P.lj0 = lj0; P.ntyp = nt;
         h.n15mx = (int)std::ceil(0.15 * (4.0 / 3.0 * 3.14159265358979 * rc * rc * rc)) * 2;
@@ -239,9 +246,14 @@ int Engine::init(const jp_sys_desc *descs, int ns) {
         // Hilbert rank of every box cell (the same curve k_bin bins on)
         {
             int nx = P.nc[0], ny = P.nc[1], nz = P.nc[2];
+            ccoord.resize((size_t)cell0 + P.ncell);
             for (int cz = 0; cz < nz; ++cz)
                 for (int cy = 0; cy < ny; ++cy)
-                    for (int cx = 0; cx < nx; ++cx) crank.push_back(cell0 + (int)hilbert3(cx, cy, cz, P.hbits));
+                    for (int cx = 0; cx < nx; ++cx) {
+                        const int h = (int)hilbert3(cx, cy, cz, P.hbits);
+                        crank.push_back(cell0 + h);
+                        ccoord[(size_t)cell0 + h] = cx | (cy << 10) | (cz << 20);
+                    }
             for (int r = 0; r < P.ncell; ++r) cell_sys_h.push_back(s);
         }
         // LJ table, AMBER combination (inptpl.f90:397-419)
@@ -338,22 +350,19 @@ int Engine::init(const jp_sys_desc *descs, int ns) {
             }
         }
         for (int i = 0; i < npad; ++i) slot_sys_h.push_back(s);
-        for (int g = 0; g < npad / 64; ++g) {
-            grp_sys_h.push_back(s);
-            gcnt_h.push_back(std::min(64, n - g * 64));
-        }
+        for (int i = 0; i < npad / JP_CL; ++i) clu_sys_h.push_back(s);
         atom0 += n;
         slot0 += npad;
         cell0 += P.ncell;
         lj0 += nt * nt;
     }
-    natom = atom0; nslot = slot0; ngrp = slot0 / 64; ncell = cell0;
+    natom = atom0; nslot = slot0; nclu = slot0 / JP_CL; ncell = cell0;
     n2 = (int)s2a.size(); n3 = (int)s3a.size();
     mass_h = mass;
     set_factors(-1);
     // upload
-    AL(d_sp, ns); AL(d_atom_sys, natom); AL(d_slot_sys, nslot); AL(d_grp_sys, ngrp); AL(d_gcnt, ngrp);
-    AL(d_cell_sys, ncell); AL(d_crank, ncell);
+    AL(d_sp, ns); AL(d_atom_sys, natom); AL(d_slot_sys, nslot);
+    AL(d_cell_sys, ncell); AL(d_crank, ncell); AL(d_ccoord, ncell); AL(d_clu_sys, nclu);
     AL(d_x, 3 * (size_t)natom); AL(d_v, 3 * (size_t)natom); AL(d_xold, 3 * (size_t)natom);
     AL(d_vback, 3 * (size_t)natom); AL(d_mass, natom); AL(d_imass, natom); AL(d_chg, natom);
     AL(d_grad, 3 * (size_t)natom); AL(d_eexc, 3 * (size_t)natom); AL(d_ekin, natom);
@@ -363,11 +372,11 @@ int Engine::init(const jp_sys_desc *descs, int ns) {
     AL(d_img, 3 * (size_t)natom); AL(d_cellid, natom); AL(d_subkey, natom); AL(d_refcell, 3 * (size_t)natom);
     AL(d_cnt, ncell + 1); AL(d_start, ncell + 1); AL(d_fill, ncell); AL(d_tmp, natom);
     AL(d_inum, natom); AL(d_perm, nslot); AL(d_cslot, ncell);
-    AL(d_ys, nslot); AL(d_qd, nslot); AL(d_bb, 6 * (size_t)ngrp); AL(d_xr, nslot); AL(d_imgs, nslot);
-    AL(d_lcount, (size_t)ngrp * JP_WL + 1); AL(d_loff, (size_t)ngrp * JP_WL + 1);
-    AL(d_cap, (size_t)ngrp * JP_WL + 1); AL(d_coff, (size_t)ngrp * JP_WL + 1); AL(d_nrem, ns); AL(d_nb, natom); AL(d_err, 1);
-    AL(d_f64, 3 * (size_t)nslot); AL(d_ffix, 3 * (size_t)nslot);
-    AL(d_ep64, 6 * (size_t)ngrp); AL(d_epfix, 6 * (size_t)ngrp);
+    AL(d_ys, nslot); AL(d_qd, nslot); AL(d_xr, nslot); AL(d_imgs, nslot); AL(d_smol, nslot);
+    AL(d_cbb, 6 * (size_t)nclu); AL(d_ccen, nclu); AL(d_lcnt, nclu); AL(d_cellext, ncell); AL(d_sysext, ns);
+    AL(d_cap, (size_t)nclu + 1); AL(d_coff, (size_t)nclu + 1); AL(d_nrem, ns); AL(d_nb, natom); AL(d_err, 1);
+    AL(d_gslot, 3 * (size_t)nslot);
+    AL(d_ep64, 6 * (size_t)nclu); AL(d_epfix, 6 * (size_t)nclu);
     AL(d_s2sys, n2); AL(d_s3sys, n3); AL(d_s2a, n2); AL(d_s3a, n3); AL(d_s2d, 6 * (size_t)n2); AL(d_s3d, 6 * (size_t)n3);
     AL(d_u64, 3);
     auto H2D = [&](void *dst, const void *src, size_t bytes) -> int {
@@ -378,10 +387,10 @@ int Engine::init(const jp_sys_desc *descs, int ns) {
     r |= upload_sp();
     r |= H2D(d_atom_sys.p, atom_sys_h.data(), 4 * atom_sys_h.size());
     r |= H2D(d_slot_sys.p, slot_sys_h.data(), 4 * slot_sys_h.size());
-    r |= H2D(d_grp_sys.p, grp_sys_h.data(), 4 * grp_sys_h.size());
-    r |= H2D(d_gcnt.p, gcnt_h.data(), 4 * gcnt_h.size());
     r |= H2D(d_cell_sys.p, cell_sys_h.data(), 4 * cell_sys_h.size());
     r |= H2D(d_crank.p, crank.data(), 4 * crank.size());
+    r |= H2D(d_ccoord.p, ccoord.data(), 4 * ccoord.size());
+    r |= H2D(d_clu_sys.p, clu_sys_h.data(), 4 * clu_sys_h.size());
     r |= H2D(d_x.p, x.data(), 8 * x.size());
     r |= H2D(d_mass.p, mass.data(), 8 * mass.size());
     r |= H2D(d_imass.p, imass.data(), 8 * imass.size());
@@ -442,30 +451,25 @@ int Engine::build_list() {
                        d_start.p, d_cnt.p, d_tmp.p, d_inum.p, d_perm.p);
     hipLaunchKernelGGL(k_cslot, grid1(ncell, 256), 256, 0, st, ncell, d_cell_sys.p, d_sp.p, d_start.p, d_cslot.p);
     hipLaunchKernelGGL(k_fill_sorted, grid1(nslot, 256), 256, 0, st, nslot, d_slot_sys.p, d_sp.p, d_perm.p, d_x.p,
-                       d_img.p, d_chg.p, d_tflag.p, d_ys.p, d_qd.p, d_xr.p, d_imgs.p);
-    hipLaunchKernelGGL(k_bbox, ngrp, 64, 0, st, ngrp, d_gcnt.p, d_ys.p, d_bb.p);
+                       d_img.p, d_chg.p, d_tflag.p, d_mol.p, d_ys.p, d_qd.p, d_xr.p, d_imgs.p, d_smol.p);
+    CK(hipMemsetAsync(d_cellext.p, 0, 4 * (size_t)ncell, st));
+    CK(hipMemsetAsync(d_sysext.p, 0, 4 * (size_t)nsys, st));
+    hipLaunchKernelGGL(k_cbox, grid1(nclu, 256), 256, 0, st, nclu, d_clu_sys.p, d_sp.p, d_perm.p, d_cellid.p,
+                       d_ccoord.p, d_ys.p, d_cbb.p, d_ccen.p, d_cellext.p, d_sysext.p);
     CK(hipMemsetAsync(d_nrem.p, 0, 4 * (size_t)nsys, st));
-    const int ngw = ngrp * JP_WL;
-    hipLaunchKernelGGL(k_list_cap, grid1(ngw, 64), 64, 0, st, ngrp, d_grp_sys.p, d_sp.p, d_bb.p, d_crank.p, d_cnt.p,
-                       d_cap.p, d_err.p);
-    hipLaunchKernelGGL(k_scan, 1, 1024, 0, st, ngw, d_cap.p, d_coff.p);
+    const unsigned nbl = grid1(nclu, JP_LW);
+    hipLaunchKernelGGL(k_list_cap8, nbl, 64 * JP_LW, 0, st, nclu, d_clu_sys.p, d_sp.p, d_cbb.p, d_crank.p, d_cslot.p,
+                       d_cnt.p, d_cellext.p, d_sysext.p, d_cap.p, d_err.p);
+    hipLaunchKernelGGL(k_scan, 1, 1024, 0, st, nclu, d_cap.p, d_coff.p);
     int captot = 0;
-    CK(hipMemcpyAsync(&captot, d_coff.p + ngw, 4, hipMemcpyDeviceToHost, st));
+    CK(hipMemcpyAsync(&captot, d_coff.p + nclu, 4, hipMemcpyDeviceToHost, st));
     CK(hipStreamSynchronize(st));
     if (captot < 0) return JP_IER_LIST;
-    if ((size_t)captot > d_etmp.n) AL(d_etmp, (size_t)(captot * 1.2) + 1024);
-    unsigned nb = (unsigned)((ngrp + 7) / 8 * 8);
-    hipLaunchKernelGGL(k_list, nb, 64 * JP_WL, 0, st, ngrp, d_grp_sys.p, d_gcnt.p, d_sp.p, d_bb.p, d_crank.p,
-                       d_cslot.p, d_cnt.p, d_xr.p, d_imgs.p, d_mol.p, d_xptr.p, d_xidx.p, d_coff.p, d_etmp.p,
-                       d_lcount.p, d_nrem.p, d_nb.p);
-    hipLaunchKernelGGL(k_scan, 1, 1024, 0, st, ngw, d_lcount.p, d_loff.p);
-    int total = 0;
-    CK(hipMemcpyAsync(&total, d_loff.p + ngw, 4, hipMemcpyDeviceToHost, st));
-    CK(hipStreamSynchronize(st));
-    if (total < 0) return JP_IER_LIST;
-    if ((size_t)total > d_ent.n) AL(d_ent, (size_t)(total * 1.25) + 1024);
-    list_entries = total;
-    hipLaunchKernelGGL(k_list_compact, ngw, 256, 0, st, ngw, d_coff.p, d_lcount.p, d_loff.p, d_etmp.p, d_ent.p);
+    if ((size_t)captot > d_ent.n) AL(d_ent, (size_t)(captot * 1.2) + 1024);
+    list_cap = captot;
+    hipLaunchKernelGGL(k_list8, nbl, 64 * JP_LW, 0, st, nclu, d_clu_sys.p, d_sp.p, d_cbb.p, d_ccen.p, d_crank.p,
+                       d_cslot.p, d_cnt.p, d_cellext.p, d_sysext.p, d_ys.p, d_xr.p, d_imgs.p, d_smol.p, d_xptr.p,
+                       d_xidx.p, d_coff.p, d_ent.p, d_lcnt.p, d_nrem.p, d_nb.p, d_err.p);
     CK(hipGetLastError());
     CK(hipEventRecord(ev[3], st));
     std::vector<int> nrem(nsys);
@@ -482,6 +486,7 @@ int Engine::build_list() {
     have_list = true;
     if (err & 1) return JP_IER_CELL;
     if (err & 2) return JP_EINVAL;
+    if (err & JP_ERR_LISTCAP) return JP_IER_LIST;
     for (int s = 0; s < nsys; ++s) {
         if (nrem[s] != 2 * hs[s].nexcl) return JP_IER_EXCL;      // cutoff.f90:90-96
         for (int i = 0; i < hs[s].n; ++i)
@@ -500,30 +505,44 @@ int Engine::refresh_sorted() {
     return 0;
 }
 
+template <bool E, bool M, bool S>
+static void launch_pair(int prec, unsigned nb, size_t lds32, size_t lds64, hipStream_t st, Engine &g);
+
 int Engine::pair(int prec, bool energy) {
-    unsigned nb = (unsigned)((ngrp + 7) / 8 * 8);
+    const unsigned nb = (unsigned)((nclu + 7) / 8 * 8);
     int maxt = 0;
-    for (auto &h : hs) maxt = std::max(maxt, h.ntyp);
-    constexpr int WP = 4;
-    if (prec == JP_PREC_FP64) {
-        size_t lds = sizeof(double2) * maxt * maxt;
-        if (energy)
-            hipLaunchKernelGGL((k_pair_f64<true, WP>), nb, 64 * WP, lds, st, ngrp, d_grp_sys.p, d_gcnt.p, d_sp.p,
-                               d_ys.p, d_qd.p, d_lj64.p, d_loff.p, d_ent.p, d_f64.p, d_ep64.p);
-        else
-            hipLaunchKernelGGL((k_pair_f64<false, WP>), nb, 64 * WP, lds, st, ngrp, d_grp_sys.p, d_gcnt.p, d_sp.p,
-                               d_ys.p, d_qd.p, d_lj64.p, d_loff.p, d_ent.p, d_f64.p, d_ep64.p);
-    } else {
-        size_t lds = sizeof(float2) * maxt * maxt;
-        if (energy)
-            hipLaunchKernelGGL((k_pair_f32<true, WP>), nb, 64 * WP, lds, st, ngrp, d_grp_sys.p, d_gcnt.p, d_sp.p,
-                               d_ys.p, d_lj32.p, d_loff.p, d_ent.p, d_ffix.p, d_epfix.p);
-        else
-            hipLaunchKernelGGL((k_pair_f32<false, WP>), nb, 64 * WP, lds, st, ngrp, d_grp_sys.p, d_gcnt.p, d_sp.p,
-                               d_ys.p, d_lj32.p, d_loff.p, d_ent.p, d_ffix.p, d_epfix.p);
+    bool lmd = false, small = false;
+    for (int s = 0; s < nsys; ++s) {
+        maxt = std::max(maxt, hs[s].ntyp);
+        small = small || sp[s].small;
+        for (int q = 0; q < 4; ++q) lmd = lmd || sp[s].fac[q] != 1.0 || sp[s].cls[q] != sp[s].cls[0];
+    }
+    const size_t l32 = sizeof(float2) * maxt * maxt, l64 = sizeof(double2) * maxt * maxt;
+    const int key = (energy ? 4 : 0) | (lmd ? 2 : 0) | (small ? 1 : 0);
+    switch (key) {
+        case 0: launch_pair<false, false, false>(prec, nb, l32, l64, st, *this); break;
+        case 1: launch_pair<false, false, true>(prec, nb, l32, l64, st, *this); break;
+        case 2: launch_pair<false, true, false>(prec, nb, l32, l64, st, *this); break;
+        case 3: launch_pair<false, true, true>(prec, nb, l32, l64, st, *this); break;
+        case 4: launch_pair<true, false, false>(prec, nb, l32, l64, st, *this); break;
+        case 5: launch_pair<true, false, true>(prec, nb, l32, l64, st, *this); break;
+        case 6: launch_pair<true, true, false>(prec, nb, l32, l64, st, *this); break;
+        default: launch_pair<true, true, true>(prec, nb, l32, l64, st, *this); break;
     }
     CK(hipGetLastError());
     return 0;
+}
+
+template <bool E, bool M, bool S>
+static void launch_pair(int prec, unsigned nb, size_t lds32, size_t lds64, hipStream_t st, Engine &g) {
+    if (prec == JP_PREC_FP64)
+        hipLaunchKernelGGL((k_pair_f64<E, M, S>), nb, 64 * JP_PW, lds64, st, g.nclu, g.d_clu_sys.p, g.d_sp.p,
+                           g.d_ccen.p, g.d_ys.p, g.d_qd.p, g.d_lj64.p, g.d_coff.p, g.d_lcnt.p, g.d_ent.p, g.excess_args(),
+                           g.d_gslot.p, g.d_ep64.p);
+    else
+        hipLaunchKernelGGL((k_pair_f32<E, M, S>), nb, 64 * JP_PW, lds32, st, g.nclu, g.d_clu_sys.p, g.d_sp.p,
+                           g.d_ccen.p, g.d_ys.p, g.d_lj32.p, g.d_coff.p, g.d_lcnt.p, g.d_ent.p, g.excess_args(),
+                           g.d_gslot.p, g.d_epfix.p);
 }
 
 // Per-system, per-class energies after a pair(prec, true) call.
@@ -536,21 +555,21 @@ int Engine::energies(int prec, std::vector<double> &eel, std::vector<double> &ev
     std::vector<double> ex(3 * (size_t)natom);
     CK(hipMemcpyAsync(ex.data(), d_eexc.p, 8 * ex.size(), hipMemcpyDeviceToHost, st));
     if (prec == JP_PREC_FP64) {
-        std::vector<double> ep(6 * (size_t)ngrp);
+        std::vector<double> ep(6 * (size_t)nclu);
         CK(hipMemcpyAsync(ep.data(), d_ep64.p, 8 * ep.size(), hipMemcpyDeviceToHost, st));
         CK(hipStreamSynchronize(st));
-        for (int g = 0; g < ngrp; ++g) {
-            int s = grp_sys_h[g];
-            for (int k = 0; k < 3; ++k) { eel[3 * s + k] += ep[6 * g + k]; evd[3 * s + k] += ep[6 * g + 3 + k]; }
+        for (int c = 0; c < nclu; ++c) {
+            int s = clu_sys_h[c];
+            for (int k = 0; k < 3; ++k) { eel[3 * s + k] += ep[6 * c + k]; evd[3 * s + k] += ep[6 * c + 3 + k]; }
         }
     } else {
-        std::vector<long long> ep(6 * (size_t)ngrp);
+        std::vector<long long> ep(6 * (size_t)nclu);
         CK(hipMemcpyAsync(ep.data(), d_epfix.p, 8 * ep.size(), hipMemcpyDeviceToHost, st));
         CK(hipStreamSynchronize(st));
         std::vector<long long> se(3 * (size_t)nsys, 0), sv(3 * (size_t)nsys, 0);
-        for (int g = 0; g < ngrp; ++g) {
-            int s = grp_sys_h[g];
-            for (int k = 0; k < 3; ++k) { se[3 * s + k] += ep[6 * g + k]; sv[3 * s + k] += ep[6 * g + 3 + k]; }
+        for (int c = 0; c < nclu; ++c) {
+            int s = clu_sys_h[c];
+            for (int k = 0; k < 3; ++k) { se[3 * s + k] += ep[6 * c + k]; sv[3 * s + k] += ep[6 * c + 3 + k]; }
         }
         for (size_t q = 0; q < se.size(); ++q) { eel[q] = se[q] / JP_ESCALE; evd[q] = sv[q] / JP_ESCALE; }
     }
@@ -582,14 +601,8 @@ int Engine::nonbond(const double *cord, double *grad, double *eelec, double *evd
         if ((r = set_factors(nfrg == 1 ? -1 : k))) return r;
         if ((r = pair(prec, k == 0))) return r;
         if (k == 0 && (r = energies(prec, eel, evd, eex))) return r;
-        if (prec == JP_PREC_FP64)
-            hipLaunchKernelGGL(k_kick<0>, grid1(natom, 256), 256, 0, st, natom, d_atom_sys.p, d_sp.p, d_inum.p, d_f64.p,
-                               d_ffix.p, d_chg.p, d_eptr.p, d_eidx.p, d_efac.p, d_mass.p, d_x.p, d_v.p, d_xold.p,
-                               (double *)nullptr, d_grad.p, 0);
-        else
-            hipLaunchKernelGGL(k_kick<1>, grid1(natom, 256), 256, 0, st, natom, d_atom_sys.p, d_sp.p, d_inum.p, d_f64.p,
-                               d_ffix.p, d_chg.p, d_eptr.p, d_eidx.p, d_efac.p, d_mass.p, d_x.p, d_v.p, d_xold.p,
-                               (double *)nullptr, d_grad.p, 0);
+        hipLaunchKernelGGL(k_kick, grid1(natom, 256), 256, 0, st, natom, d_atom_sys.p, d_sp.p, d_inum.p, d_gslot.p,
+                           d_mass.p, d_x.p, d_v.p, d_xold.p, (double *)nullptr, d_grad.p, 0);
         CK(hipGetLastError());
         if (grad) {
             CK(hipMemcpyAsync(g.data(), d_grad.p, 8 * g.size(), hipMemcpyDeviceToHost, st));
@@ -644,14 +657,8 @@ int Engine::run_nve(int nsteps, int update, int nmon, int prec, jp_energy_log *l
         pe.push_back(b);
         std::vector<double> eel, evd, eex;
         if (mon && (r = energies(prec, eel, evd, eex))) return r;
-        if (prec == JP_PREC_FP64)
-            hipLaunchKernelGGL(k_kick<0>, grid1(natom, 256), 256, 0, st, natom, d_atom_sys.p, d_sp.p, d_inum.p, d_f64.p,
-                               d_ffix.p, d_chg.p, d_eptr.p, d_eidx.p, d_efac.p, d_mass.p, d_x.p, d_v.p, d_xold.p,
-                               mon ? d_vback.p : (double *)nullptr, (double *)nullptr, 1);
-        else
-            hipLaunchKernelGGL(k_kick<1>, grid1(natom, 256), 256, 0, st, natom, d_atom_sys.p, d_sp.p, d_inum.p, d_f64.p,
-                               d_ffix.p, d_chg.p, d_eptr.p, d_eidx.p, d_efac.p, d_mass.p, d_x.p, d_v.p, d_xold.p,
-                               mon ? d_vback.p : (double *)nullptr, (double *)nullptr, 1);
+        hipLaunchKernelGGL(k_kick, grid1(natom, 256), 256, 0, st, natom, d_atom_sys.p, d_sp.p, d_inum.p, d_gslot.p,
+                           d_mass.p, d_x.p, d_v.p, d_xold.p, mon ? d_vback.p : (double *)nullptr, (double *)nullptr, 1);
         int ier = 0;
         if ((r = shake(ier))) return r;
         hipLaunchKernelGGL(k_post, grid1(natom, 256), 256, 0, st, natom, d_atom_sys.p, d_sp.p, d_inum.p, d_img.p, d_x.p,
@@ -718,25 +725,27 @@ int Engine::run_nve(int nsteps, int update, int nmon, int prec, jp_energy_log *l
 int Engine::get_list(int32_t *cell3, int64_t *npairs, int32_t *ptr, int32_t *idx, int64_t cap) {
     if (!have_list) return JP_EINVAL;
     CK(hipStreamSynchronize(st));
-    std::vector<int> loff((size_t)ngrp * JP_WL + 1), perm(nslot);
-    std::vector<Entry> ent(list_entries);
-    CK(hipMemcpy(loff.data(), d_loff.p, 4 * loff.size(), hipMemcpyDeviceToHost));
+    std::vector<int> coff((size_t)nclu + 1), perm(nslot);
+    std::vector<int2> lc(nclu);
+    std::vector<uint2> ent((size_t)std::max<long long>(list_cap, 1));
+    CK(hipMemcpy(coff.data(), d_coff.p, 4 * coff.size(), hipMemcpyDeviceToHost));
+    CK(hipMemcpy(lc.data(), d_lcnt.p, 8 * lc.size(), hipMemcpyDeviceToHost));
     CK(hipMemcpy(perm.data(), d_perm.p, 4 * (size_t)nslot, hipMemcpyDeviceToHost));
-    if (list_entries) CK(hipMemcpy(ent.data(), d_ent.p, sizeof(Entry) * list_entries, hipMemcpyDeviceToHost));
+    if (list_cap) CK(hipMemcpy(ent.data(), d_ent.p, sizeof(uint2) * list_cap, hipMemcpyDeviceToHost));
     if (cell3) CK(hipMemcpy(cell3, d_refcell.p, 12 * (size_t)natom, hipMemcpyDeviceToHost));
     std::vector<std::vector<int>> nb(natom);
-    for (int g = 0; g < ngrp; ++g)
-        for (int e = loff[(size_t)g * JP_WL]; e < loff[(size_t)(g + 1) * JP_WL]; ++e) {
-            int jsl = (int)(ent[e].js & JP_SLOT_MASK);
-            int aj = perm[jsl];
-            uint64_t m = ent[e].m;
-            while (m) {
-                int l = __builtin_ctzll(m);
-                m &= m - 1;
-                int ai = perm[g * 64 + l];
+    for (int c = 0; c < nclu; ++c) {
+        const int e0 = coff[c], cp = coff[c + 1] - e0;
+        for (int e = 0; e < lc[c].x + lc[c].y; ++e) {
+            const uint2 E = ent[e < lc[c].x ? e0 + e : e0 + cp - lc[c].y + (e - lc[c].x)];
+            const int aj = perm[E.x & JP_SLOT_MASK];
+            for (int k = 0; k < JP_CL; ++k) {
+                if (!((E.y >> k) & 1u)) continue;
+                const int ai = perm[c * JP_CL + k];
                 if (ai >= 0 && aj >= 0) nb[ai].push_back(aj);
             }
         }
+    }
     int64_t tot = 0;
     for (auto &v : nb) tot += (int64_t)v.size();
     *npairs = tot;
@@ -756,7 +765,7 @@ int Engine::get_list(int32_t *cell3, int64_t *npairs, int32_t *ptr, int32_t *idx
 int Engine::count_pairs(int64_t *np, int64_t *nlj, int64_t *nlist) {
     if (!have_list) return JP_EINVAL;
     CK(hipMemsetAsync(d_u64.p, 0, 24, st));
-    hipLaunchKernelGGL(k_count_pairs, ngrp, 64, 0, st, ngrp, d_grp_sys.p, d_gcnt.p, d_sp.p, d_ys.p, d_loff.p,
+    hipLaunchKernelGGL(k_count_pairs, nclu, 64, 0, st, nclu, d_clu_sys.p, d_sp.p, d_ys.p, d_coff.p, d_lcnt.p,
                        d_ent.p, d_u64.p);
     unsigned long long c[3] = {0, 0, 0};
     CK(hipMemcpyAsync(c, d_u64.p, 24, hipMemcpyDeviceToHost, st));
