@@ -13,7 +13,8 @@
 //                   q (FP32 chgmod), packed type/class/vdW flags} -- 32 B
 //     qd[slot]      chgmod in FP64 (plain FP64 path)
 //     xr[slot]      raw coordinates at the last rebuild (list test)
-//     smol[slot]    molecule id (exclusion pre-test)
+//     lp[slot]      list-test record: FP32 coordinates about the box centre,
+//                   bits of w: molecule id (0-29), has-vdW (30); -1 = padding
 //   per i-cluster   box, centre, and a j-list of uint2 entries {j slot | image
 //                   shift, 8-bit mask} (jp_list.hpp).
 #pragma once
@@ -42,7 +43,10 @@ struct SysP {
     int lj0, ntyp;                   // LJ table offset / size
     int hbits;                       // nc = 2^hbits cells per edge
     int small;                       // some edge < 2*rlim: minimum image per pair
+    int pad_;
     double dt;
+    double ctr[3];                   // box centre: origin of the FP32 list-test frame
+    double band;                     // A^2: FP32 list-test results within band of rlim^2 take the FP64 test
 };
 
 struct JAtom {          // 32 B
@@ -226,7 +230,7 @@ __global__ void k_fill_sorted(int nslot, const int *__restrict__ slot_sys, const
                               const int *__restrict__ img, const double *__restrict__ chg,
                               const int *__restrict__ tflag, const int *__restrict__ mol, JAtom *__restrict__ ys,
                               double *__restrict__ qd, double4 *__restrict__ xr, int4 *__restrict__ imgs,
-                              int *__restrict__ smol) {
+                              float4 *__restrict__ lp) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nslot) return;
     int a = perm[s];
@@ -239,7 +243,7 @@ __global__ void k_fill_sorted(int nslot, const int *__restrict__ slot_sys, const
         qd[s] = 0.0;
         xr[s] = make_double4(j.x, j.y, j.z, 0.0);
         imgs[s] = make_int4(0, 0, 0, -1);
-        smol[s] = -3;
+        lp[s] = make_float4(-1.0e6f, -1.0e6f, -1.0e6f, __int_as_float(-1));
         return;
     }
     double p0 = x[3 * a], p1 = x[3 * a + 1], p2 = x[3 * a + 2];
@@ -253,7 +257,8 @@ __global__ void k_fill_sorted(int nslot, const int *__restrict__ slot_sys, const
     qd[s] = chg[a];
     xr[s] = make_double4(p0, p1, p2, 0.0);
     imgs[s] = make_int4(i0, i1, i2, a);
-    smol[s] = mol[a];
+    const int meta = (mol[a] & 0x3fffffff) | (((tflag[a] >> 12) & 1) << 30);
+    lp[s] = make_float4((float)(j.x - P.ctr[0]), (float)(j.y - P.ctr[1]), (float)(j.z - P.ctr[2]), __int_as_float(meta));
 }
 
 // --------------------------------------------------------------------------

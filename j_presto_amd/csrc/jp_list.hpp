@@ -9,31 +9,33 @@
 // code gives the periodic shift s (code = (sx+1)*9+(sy+1)*3+(sz+1)); the
 // pair kernels use j at y_j + s*L.
 //
-// Entries whose j carries LJ are stored from the front of the cluster's
-// segment [coff[C], coff[C+1]), the others from its back, so that every chunk
-// the pair kernels stage is uniformly with or without the LJ term.
+// Every i-cluster owns a segment of ccap entries at ent[C*ccap]; entries whose
+// j carries LJ are stored from its front, the others from its back, so that
+// every chunk the pair kernels stage is uniformly with or without the LJ term.
+// A segment that would overflow is reported (the host grows ccap and builds
+// again).
 //
 // Candidates: every j-cluster whose home cell (the cell of its first atom) is
 // within rlim + 2*ext of C's box, ext bounding how far a cluster's atoms
-// stick out of its home cell, then every j-cluster whose own box (shifted by
-// s*L) is within rlim of C's box.  Pipeline: k_cbox -> k_list_cap8 (count
-// candidate j-clusters) -> scan -> k_list8 (one wavefront per i-cluster; the
-// 8 i atoms sit in scalar registers, each lane tests one candidate j atom).
+// stick out of their home cell, then every j-cluster whose own box (shifted by
+// s*L) is within rlim of C's box.  Pipeline: k_cbox -> k_list8 (one wavefront
+// per i-cluster; the 8 i atoms sit in scalar registers, each lane tests one
+// candidate j atom).
 //
-// The per-pair test runs in FP32 on coordinates relative to C's box centre.
-// Its error is bounded by 2^-24 * (|r_i| + |r_j| + |d|) per component, i.e.
-// below 3e-4 A^2 on d^2 for any cluster smaller than 100 A; pairs with
-// |d^2 - rlim^2| <= JP_BAND (0.01 A^2) are decided by md_run's FP64 test on
-// raw coordinates, so the list is the reference's bit for bit.  In boxes with
-// an edge below 2*rlim every pair takes the FP64 test, which also checks that
-// s is the minimum image.
+// The per-pair test runs in FP32 on coordinates about the box centre, taken
+// relative to C's centre.  With E the bound on the per-component error (a few
+// ulp of 1.5 L), the FP32 d^2 is within 2*rlim*sqrt(3)*E + 4e-7*rlim^2 of the
+// FP64 one; pairs with |d^2 - rlim^2| <= band (4x that bound, at least
+// 0.01 A^2, SysP::band) are decided by md_run's FP64 test on raw coordinates,
+// so the list is the reference's bit for bit.  In boxes with an edge below
+// 2*rlim every pair takes the FP64 test, which also checks that s is the
+// minimum image.
 #pragma once
 #include "jp_kernels.hpp"
 
 #define JP_CL 8          // atoms per i-cluster
 #define JP_LW 4          // i-clusters (wavefronts) per block in the list kernels
 #define JP_QCAP 128      // per-wave queue of candidate j-clusters (power of 2)
-#define JP_BAND 0.01     // A^2: FP32 results this close to rlim^2 go to the FP64 test
 #define JP_ERR_LISTCAP 16
 
 // Orders LDS accesses of one wavefront (lanes write, other lanes read).
@@ -53,42 +55,66 @@ __host__ __device__ __forceinline__ void shift_of(uint32_t code, int &sx, int &s
     sz = (int)(code % 3) - 1;
 }
 
-// Box and centre of every cluster (wrapped coordinates at this rebuild), and
-// how far it sticks out of its home cell: maximum per cell and per system
-// (non-negative floats compare like their bit patterns).
-__global__ void k_cbox(int nclu, const int *__restrict__ clu_sys, const SysP *__restrict__ sp,
-                       const int *__restrict__ perm, const int *__restrict__ cellid, const int *__restrict__ ccoord,
-                       const JAtom *__restrict__ ys, double *__restrict__ cbb, double4 *__restrict__ ccen,
-                       unsigned *__restrict__ cellext, unsigned *__restrict__ sysext) {
+// Box and centres of every cluster (wrapped coordinates at this rebuild: FP64
+// box for the candidate search and the pair kernels' frame, FP32 centre for
+// the list test), and how far it sticks out of its home cell: maximum per cell
+// and per system (non-negative floats compare like their bit patterns).
+__global__ void __launch_bounds__(256)
+k_cbox(int nclu, const int *__restrict__ clu_sys, const SysP *__restrict__ sp, const int *__restrict__ perm,
+       const int *__restrict__ cellid, const int *__restrict__ ccoord, const JAtom *__restrict__ ys,
+       const float4 *__restrict__ lp, double *__restrict__ cbb, double4 *__restrict__ ccen,
+       float4 *__restrict__ ccf, unsigned *__restrict__ cellext, unsigned *__restrict__ sysext) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= nclu) return;
-    const int s = clu_sys[c];
-    const SysP &P = sp[s];
-    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
-    int n = 0;
-    for (int k = 0; k < JP_CL; ++k) {
-        if (perm[c * JP_CL + k] < 0) continue;
-        const JAtom a = ys[c * JP_CL + k];
-        lo[0] = fmin(lo[0], a.x); hi[0] = fmax(hi[0], a.x);
-        lo[1] = fmin(lo[1], a.y); hi[1] = fmax(hi[1], a.y);
-        lo[2] = fmin(lo[2], a.z); hi[2] = fmax(hi[2], a.z);
-        ++n;
+    const bool act = c < nclu;
+    const int s = act ? clu_sys[c] : -1;
+    float ef = 0.0f;
+    if (act) {
+        const SysP &P = sp[s];
+        double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+        float flo[3] = {1e30f, 1e30f, 1e30f}, fhi[3] = {-1e30f, -1e30f, -1e30f};
+        int n = 0;
+        for (int k = 0; k < JP_CL; ++k) {
+            if (perm[c * JP_CL + k] < 0) continue;
+            const JAtom a = ys[c * JP_CL + k];
+            const float4 q = lp[c * JP_CL + k];
+            lo[0] = fmin(lo[0], a.x); hi[0] = fmax(hi[0], a.x);
+            lo[1] = fmin(lo[1], a.y); hi[1] = fmax(hi[1], a.y);
+            lo[2] = fmin(lo[2], a.z); hi[2] = fmax(hi[2], a.z);
+            flo[0] = fminf(flo[0], q.x); fhi[0] = fmaxf(fhi[0], q.x);
+            flo[1] = fminf(flo[1], q.y); fhi[1] = fmaxf(fhi[1], q.y);
+            flo[2] = fminf(flo[2], q.z); fhi[2] = fmaxf(fhi[2], q.z);
+            ++n;
+        }
+        for (int k = 0; k < 3; ++k) { cbb[6 * c + k] = lo[k]; cbb[6 * c + 3 + k] = hi[k]; }
+        if (n == 0) {
+            ccen[c] = make_double4(0.0, 0.0, 0.0, 0.0);
+            ccf[c] = make_float4(0.f, 0.f, 0.f, 0.f);
+        } else {
+            ccen[c] = make_double4(0.5 * (lo[0] + hi[0]), 0.5 * (lo[1] + hi[1]), 0.5 * (lo[2] + hi[2]), 0.0);
+            ccf[c] = make_float4(0.5f * (flo[0] + fhi[0]), 0.5f * (flo[1] + fhi[1]), 0.5f * (flo[2] + fhi[2]), 0.f);
+            // slots are real atoms first, padding last: a non-empty cluster's first slot is real
+            const int cell = cellid[perm[c * JP_CL]];
+            const int cc = ccoord[cell];
+            const int cx[3] = {cc & 1023, (cc >> 10) & 1023, (cc >> 20) & 1023};
+            double e = 0.0;
+            for (int k = 0; k < 3; ++k) {
+                const double clo = P.wall[k] + cx[k] * P.csz[k], chi = P.wall[k] + (cx[k] + 1) * P.csz[k];
+                e = fmax(e, fmax(clo - lo[k], hi[k] - chi));
+            }
+            ef = (float)e * 1.0001f + 1e-5f;
+            atomicMax(cellext + cell, __float_as_uint(ef));
+        }
     }
-    for (int k = 0; k < 3; ++k) { cbb[6 * c + k] = lo[k]; cbb[6 * c + 3 + k] = hi[k]; }
-    if (n == 0) { ccen[c] = make_double4(0.0, 0.0, 0.0, 0.0); return; }
-    ccen[c] = make_double4(0.5 * (lo[0] + hi[0]), 0.5 * (lo[1] + hi[1]), 0.5 * (lo[2] + hi[2]), 0.0);
-    // slots are real atoms first, padding last: a non-empty cluster's first slot is real
-    const int cell = cellid[perm[c * JP_CL]];
-    const int cc = ccoord[cell];
-    const int cx[3] = {cc & 1023, (cc >> 10) & 1023, (cc >> 20) & 1023};
-    double e = 0.0;
-    for (int k = 0; k < 3; ++k) {
-        const double clo = P.wall[k] + cx[k] * P.csz[k], chi = P.wall[k] + (cx[k] + 1) * P.csz[k];
-        e = fmax(e, fmax(clo - lo[k], hi[k] - chi));
+    // per-system maximum: one atomic per wavefront when it holds one system
+    const int s0 = __builtin_amdgcn_readfirstlane(s);
+    if (__ballot(act && s != s0) == 0) {
+        float m = ef;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+        if ((threadIdx.x & 63) == 0 && s0 >= 0) atomicMax(sysext + s0, __float_as_uint(m));
+    } else if (act) {
+        atomicMax(sysext + s, __float_as_uint(ef));
     }
-    const float ef = (float)e * 1.0001f + 1e-5f;
-    atomicMax(cellext + cell, __float_as_uint(ef));
-    atomicMax(sysext + s, __float_as_uint(ef));
 }
 
 struct CluRange {
@@ -122,27 +148,37 @@ __device__ __forceinline__ CluRange clu_range(const SysP &P, const double *__res
 
 __device__ __forceinline__ int floordiv(int a, int b) { return a >= 0 ? a / b : -((-a + b - 1) / b); }
 
-// Candidate q -> global cell id and image shift, or -1 when the unwrapped cell
-// (grown by 2*its ext) is farther than rlim from the cluster box.  Shifts of
-// two or more boxes are never within rlim (every edge is >= 2*cutoff).
-__device__ __forceinline__ int clu_cell(const SysP &P, const CluRange &r, const int *__restrict__ crank,
-                                        const unsigned *__restrict__ cellext, int q, uint32_t &code, double &shx,
-                                        double &shy, double &shz) {
+// Per box-grid position: {first slot, atoms, ext bits, 0} of the cell there
+// (one load per candidate cell in k_list8).
+__global__ void k_crec(int ncell, const int *__restrict__ crank, const int *__restrict__ cslot,
+                       const int *__restrict__ ccnt, const unsigned *__restrict__ cellext, int4 *__restrict__ crec) {
+    const int gi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gi >= ncell) return;
+    const int c = crank[gi];
+    crec[gi] = make_int4(cslot[c], ccnt[c], (int)cellext[c], 0);
+}
+
+// Candidate q -> its j-clusters [k0, k1) and image shift; k1 == k0 when the
+// unwrapped cell (grown by 2*its ext) is farther than rlim from the cluster
+// box.  Shifts of two or more boxes are never within rlim (every edge is
+// >= 2*cutoff).
+__device__ __forceinline__ void clu_cell(const SysP &P, const CluRange &r, const int4 *__restrict__ crec, int q,
+                                         uint32_t &code, int &k0, int &k1) {
+    k0 = k1 = 0;
     const int ux = r.ulo[0] + q % r.n[0], uy = r.ulo[1] + (q / r.n[0]) % r.n[1], uz = r.ulo[2] + q / (r.n[0] * r.n[1]);
     const int sx = floordiv(ux, P.nc[0]), sy = floordiv(uy, P.nc[1]), sz = floordiv(uz, P.nc[2]);
-    if (sx < -1 || sx > 1 || sy < -1 || sy > 1 || sz < -1 || sz > 1) return -1;
+    if (sx < -1 || sx > 1 || sy < -1 || sy > 1 || sz < -1 || sz > 1) return;
     const int cx = ux - sx * P.nc[0], cy = uy - sy * P.nc[1], cz = uz - sz * P.nc[2];
-    const int cell = crank[P.cell0 + (cz * P.nc[1] + cy) * P.nc[0] + cx];
-    const double m = r.rl + 2.0 * (double)__uint_as_float(cellext[cell]);
+    const int4 cr = crec[P.cell0 + (cz * P.nc[1] + cy) * P.nc[0] + cx];
+    const double m = r.rl + 2.0 * (double)__int_as_float(cr.z);
     const double gx = fmax(0.0, fmax(P.wall[0] + ux * P.csz[0] - r.hi[0], r.lo[0] - (P.wall[0] + (ux + 1) * P.csz[0])));
     const double gy = fmax(0.0, fmax(P.wall[1] + uy * P.csz[1] - r.hi[1], r.lo[1] - (P.wall[1] + (uy + 1) * P.csz[1])));
     const double gz = fmax(0.0, fmax(P.wall[2] + uz * P.csz[2] - r.hi[2], r.lo[2] - (P.wall[2] + (uz + 1) * P.csz[2])));
-    if (gx * gx + gy * gy + gz * gz > m * m) return -1;
+    if (gx * gx + gy * gy + gz * gz > m * m) return;
     code = (uint32_t)((sx + 1) * 9 + (sy + 1) * 3 + (sz + 1));
-    shx = P.L[0] * sx;
-    shy = P.L[1] * sy;
-    shz = P.L[2] * sz;
-    return cell;
+    // j-clusters whose first slot lies in this cell
+    k0 = (cr.x + JP_CL - 1) / JP_CL;
+    k1 = (cr.x + cr.y + JP_CL - 1) / JP_CL;
 }
 
 // Box of j-cluster jc, shifted by (shx,shy,shz), within rl of the i-cluster box?
@@ -153,14 +189,6 @@ __device__ __forceinline__ bool clu_near(const double *__restrict__ cbb, int jc,
     const double gy = fmax(0.0, fmax(b[1] + shy - r.hi[1], r.lo[1] - (b[4] + shy)));
     const double gz = fmax(0.0, fmax(b[2] + shz - r.hi[2], r.lo[2] - (b[5] + shz)));
     return gx * gx + gy * gy + gz * gz <= r.rl * r.rl;
-}
-
-// j-clusters whose first slot lies in `cell`
-__device__ __forceinline__ void cell_clusters(const int *__restrict__ cslot, const int *__restrict__ ccnt, int cell,
-                                              int &k0, int &k1) {
-    const int a = cslot[cell], n = ccnt[cell];
-    k0 = (a + JP_CL - 1) / JP_CL;
-    k1 = (a + n + JP_CL - 1) / JP_CL;
 }
 
 // md_run's list test on raw coordinates (cutoff.f90:464-469) plus: s is the
@@ -178,35 +206,6 @@ __device__ __forceinline__ bool exact_in(const SysP &P, const double4 &a, const 
            ((int)n2 - ia.z + ib.z == sz);
 }
 
-// Upper bound of each i-cluster's entries: 8 per candidate j-cluster.
-__global__ void __launch_bounds__(64 * JP_LW)
-k_list_cap8(int nclu, const int *__restrict__ clu_sys, const SysP *__restrict__ sp, const double *__restrict__ cbb,
-            const int *__restrict__ crank, const int *__restrict__ cslot, const int *__restrict__ ccnt,
-            const unsigned *__restrict__ cellext, const unsigned *__restrict__ sysext, int *__restrict__ cap,
-            int *__restrict__ err) {
-    const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int ci = blockIdx.x * JP_LW + w;
-    if (ci >= nclu) return;
-    const int s = clu_sys[ci];
-    const SysP &P = sp[s];
-    const CluRange r = clu_range(P, cbb, ci, __uint_as_float(sysext[s]));
-    if (r.bad && l == 0) atomicOr(err, 1);
-    int cnt = 0;
-    for (int qb = 0; qb < r.ncand; qb += 64) {
-        const int q = qb + l;
-        if (q >= r.ncand) continue;
-        uint32_t code = 13;
-        double shx = 0.0, shy = 0.0, shz = 0.0;
-        const int cell = clu_cell(P, r, crank, cellext, q, code, shx, shy, shz);
-        if (cell < 0) continue;
-        int k0, k1;
-        cell_clusters(cslot, ccnt, cell, k0, k1);
-        for (int jc = k0; jc < k1; ++jc) cnt += clu_near(cbb, jc, shx, shy, shz, r) ? 1 : 0;
-    }
-    cnt = wave_sum(cnt);
-    if (l == 0) cap[ci] = cnt * JP_CL;
-}
-
 // One wavefront per i-cluster.  Candidate j-clusters go through a queue in
 // LDS; every 8 of them are tested as one batch (lane = one j atom against the
 // 8 i atoms).  Per i atom the listed count (nbcount) and per system the number
@@ -214,24 +213,24 @@ k_list_cap8(int nclu, const int *__restrict__ clu_sys, const SysP *__restrict__ 
 // checks.
 __global__ void __launch_bounds__(64 * JP_LW)
 k_list8(int nclu, const int *__restrict__ clu_sys, const SysP *__restrict__ sp, const double *__restrict__ cbb,
-        const double4 *__restrict__ ccen, const int *__restrict__ crank, const int *__restrict__ cslot,
-        const int *__restrict__ ccnt, const unsigned *__restrict__ cellext, const unsigned *__restrict__ sysext,
-        const JAtom *__restrict__ ys, const double4 *__restrict__ xr, const int4 *__restrict__ imgs,
-        const int *__restrict__ smol, const int *__restrict__ xptr, const int *__restrict__ xidx,
-        const int *__restrict__ coff, uint2 *__restrict__ ent, int2 *__restrict__ lcnt, int *__restrict__ nremoved,
-        int *__restrict__ nbcount, int *__restrict__ err) {
-    __shared__ uint32_t s_q[JP_LW][JP_QCAP];
+        const float4 *__restrict__ ccf, const int4 *__restrict__ crec, const unsigned *__restrict__ sysext,
+        const float4 *__restrict__ lp, const double4 *__restrict__ xr, const int4 *__restrict__ imgs,
+        const int *__restrict__ xptr, const int *__restrict__ xidx, int ccap, uint2 *__restrict__ ent,
+        int2 *__restrict__ lcnt, int *__restrict__ nremoved, int *__restrict__ nbcount, int *__restrict__ err,
+        int *__restrict__ lmax) {
+    __shared__ uint32_t s_c[JP_LW][JP_QCAP];   // candidate j-clusters of passing cells
+    __shared__ uint32_t s_q[JP_LW][JP_QCAP];   // candidates whose box is within rlim
     __shared__ double4 s_xr[JP_LW][JP_CL];
     __shared__ int4 s_im[JP_LW][JP_CL];
-    const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int l = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // wave-uniform
     const int ci = blockIdx.x * JP_LW + w;
     if (ci >= nclu) return;   // whole wavefront; no block-wide barrier below
     const int s = clu_sys[ci];
     const SysP &P = sp[s];
     const CluRange r = clu_range(P, cbb, ci, __uint_as_float(sysext[s]));
-    const double4 cen = ccen[ci];
+    const float4 cf = ccf[ci];
 
-    // the 8 i atoms: FP32 relative coordinates, molecule, atom index
+    // the 8 i atoms: FP32 coordinates relative to the cluster centre, molecule, atom index
     float fx = 1e15f, fy = 1e15f, fz = 1e15f;
     int mi = -2, ai = -1;
     if (l < JP_CL) {
@@ -239,11 +238,11 @@ k_list8(int nclu, const int *__restrict__ clu_sys, const SysP *__restrict__ sp, 
         const int4 im = imgs[sl];
         ai = im.w;
         if (ai >= 0) {
-            const JAtom a = ys[sl];
-            fx = (float)(a.x - cen.x);
-            fy = (float)(a.y - cen.y);
-            fz = (float)(a.z - cen.z);
-            mi = smol[sl];
+            const float4 q = lp[sl];
+            fx = q.x - cf.x;
+            fy = q.y - cf.y;
+            fz = q.z - cf.z;
+            mi = __float_as_int(q.w) & 0x3fffffff;
         }
         s_xr[w][l] = xr[sl];
         s_im[w][l] = im;
@@ -262,8 +261,10 @@ k_list8(int nclu, const int *__restrict__ clu_sys, const SysP *__restrict__ sp, 
     int nbv = 0;
     wave_lds_sync();
 
-    const int e0 = coff[ci], cap = coff[ci + 1] - e0;
-    const float rlo = (float)(P.rlim2 - JP_BAND), rhi = (float)(P.rlim2 + JP_BAND);
+    const size_t e0 = (size_t)ci * ccap;
+    const int cap = ccap;
+    const float rlo = (float)(P.rlim2 - P.band), rhi = (float)(P.rlim2 + P.band);
+    const float Lx = (float)P.L[0], Ly = (float)P.L[1], Lz = (float)P.L[2];
     const bool small = P.small != 0;
     int nA = 0, nB = 0, nrem = 0, qh = 0, qt = 0;
     bool ovf = false;
@@ -279,19 +280,17 @@ k_list8(int nclu, const int *__restrict__ clu_sys, const SysP *__restrict__ sp, 
         int sx, sy, sz;
         shift_of(code, sx, sy, sz);
         uint32_t m = 0, amb = 0;
-        int aj = -1, molj = -3, tj = 0;
-        int4 ij = make_int4(0, 0, 0, -1);
+        int meta = -1;
+        float4 q = make_float4(0.f, 0.f, 0.f, 0.f);
         if (qv) {
-            ij = imgs[js];
-            aj = ij.w;
+            q = lp[js];
+            meta = __float_as_int(q.w);
         }
-        if (aj >= 0) {
-            const JAtom b = ys[js];
-            tj = b.t;
-            molj = smol[js];
-            const float rx = (float)((b.x + P.L[0] * sx) - cen.x);
-            const float ry = (float)((b.y + P.L[1] * sy) - cen.y);
-            const float rz = (float)((b.z + P.L[2] * sz) - cen.z);
+        const int molj = meta & 0x3fffffff;
+        if (meta >= 0) {
+            const float rx = (q.x - cf.x) + Lx * (float)sx;
+            const float ry = (q.y - cf.y) + Ly * (float)sy;
+            const float rz = (q.z - cf.z) + Lz * (float)sz;
 #pragma unroll
             for (int k = 0; k < JP_CL; ++k) {
                 const float dx = xi[k] - rx, dy = yi[k] - ry, dz = zi[k] - rz;
@@ -306,6 +305,7 @@ k_list8(int nclu, const int *__restrict__ clu_sys, const SysP *__restrict__ sp, 
         if (__ballot(amb != 0)) {
             if (amb) {
                 const double4 xj = xr[js];
+                const int4 ij = imgs[js];
 #pragma unroll 1
                 for (int k = 0; k < JP_CL; ++k)
                     if ((amb >> k) & 1u)
@@ -318,6 +318,7 @@ k_list8(int nclu, const int *__restrict__ clu_sys, const SysP *__restrict__ sp, 
         for (int k = 0; k < JP_CL; ++k) same |= (molj == moli[k] ? 1u : 0u) << k;
         same &= m;
         if (__ballot(same != 0)) {
+            const int aj = same ? imgs[js].w : -1;
             for (int k = 0; k < JP_CL; ++k) {
                 if (__ballot((same >> k) & 1u) == 0) continue;
                 const int a = __builtin_amdgcn_readlane(ai, k);
@@ -333,7 +334,7 @@ k_list8(int nclu, const int *__restrict__ clu_sys, const SysP *__restrict__ sp, 
             const int c = __popcll(__ballot((m >> k) & 1u));
             nbv += (l == k) ? c : 0;
         }
-        const bool has = m != 0, lj = (tj >> 12) & 1;
+        const bool has = m != 0, lj = (meta >> 30) & 1;
         const uint64_t bA = __ballot(has && lj), bB = __ballot(has && !lj);
         const int na = __popcll(bA), nb = __popcll(bB);
         if (nA + nB + na + nb <= cap) {
@@ -347,44 +348,59 @@ k_list8(int nclu, const int *__restrict__ clu_sys, const SysP *__restrict__ sp, 
         }
     };
 
-    // one loop, one drain site: drain while 8 are queued (or the rest at the
-    // end), else append the next round of candidate j-clusters
-    int qb = -64, t = 0, k0 = 0, k1 = 0;
-    uint32_t code = 13;
-    double shx = 0.0, shy = 0.0, shz = 0.0;
-    bool done = false;
+    // One loop, one site per phase, highest priority first: drain 8 queued
+    // j-clusters; box-test 64 candidates (independent loads); append one
+    // round of the current cell batch's j-clusters (LDS only); test the next
+    // 64 candidate cells.  Bounds: the queue holds <= 7 + 64 and the
+    // candidate ring <= 63 + 64 entries when written, both < JP_QCAP.
+    int ch = 0, ct = 0, qb = -64, rnd = 0, k0 = 0, k1 = 0;
+    uint32_t ccode = 13;
+    bool cells_done = false;
     for (;;) {
-        const int nq = qt - qh;
-        if (nq >= 8 || (done && nq > 0)) {
+        const int nq = qt - qh, nc = ct - ch;
+        if (nq >= 8 || (cells_done && nc == 0 && nq > 0)) {
             wave_lds_sync();
             drain(min(nq, 8));
             qh += min(nq, 8);
             wave_lds_sync();
             continue;
         }
-        if (done) break;
-        if (__ballot(k0 + t < k1) == 0) {   // next 64 candidate cells
-            qb += 64;
-            if (qb >= r.ncand) { done = true; continue; }
-            const int q = qb + l;
-            k0 = k1 = 0;
-            t = 0;
-            if (q < r.ncand) {
-                const int cell = clu_cell(P, r, crank, cellext, q, code, shx, shy, shz);
-                if (cell >= 0) cell_clusters(cslot, ccnt, cell, k0, k1);
+        if (nc >= 64 || (cells_done && nc > 0)) {
+            const int mc = min(nc, 64);
+            bool ok = false;
+            uint32_t ce = 0;
+            wave_lds_sync();
+            if (l < mc) {
+                ce = s_c[w][(ch + l) & (JP_QCAP - 1)];
+                int sx, sy, sz;
+                shift_of(ce >> JP_SLOT_BITS, sx, sy, sz);
+                ok = clu_near(cbb, (int)(ce & JP_SLOT_MASK), P.L[0] * sx, P.L[1] * sy, P.L[2] * sz, r);
             }
+            const uint64_t b = __ballot(ok);
+            if (ok) s_q[w][(qt + lane_rank(b)) & (JP_QCAP - 1)] = ce;
+            qt += __popcll(b);
+            ch += mc;
             continue;
         }
-        const int jc = k0 + t;
-        const bool ok = (jc < k1) && clu_near(cbb, jc, shx, shy, shz, r);
-        const uint64_t b = __ballot(ok);
-        if (ok) s_q[w][(qt + lane_rank(b)) & (JP_QCAP - 1)] = (uint32_t)jc | (code << JP_SLOT_BITS);
-        qt += __popcll(b);
-        ++t;
+        if (cells_done) break;
+        if (__ballot(rnd < k1 - k0) == 0) {   // next 64 candidate cells
+            qb += 64;
+            if (qb >= r.ncand) { cells_done = true; continue; }
+            rnd = 0;
+            k0 = k1 = 0;
+            if (qb + l < r.ncand) clu_cell(P, r, crec, qb + l, ccode, k0, k1);
+            continue;
+        }
+        const bool p = rnd < k1 - k0;
+        const uint64_t b = __ballot(p);
+        if (p) s_c[w][(ct + lane_rank(b)) & (JP_QCAP - 1)] = (uint32_t)(k0 + rnd) | (ccode << JP_SLOT_BITS);
+        ct += __popcll(b);
+        ++rnd;
     }
     if (l == 0) {
         lcnt[ci] = make_int2(nA, nB);
         if (ovf) atomicOr(err, JP_ERR_LISTCAP);
+        atomicMax(lmax, nA + nB);
     }
     if (l < JP_CL && ai >= 0) nbcount[ai] = nbv;
     const int nr = wave_sum(nrem);

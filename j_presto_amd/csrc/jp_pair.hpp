@@ -34,13 +34,14 @@ struct ExcessArgs {
 };
 
 struct ChunkSpan {
-    int base, n;
+    size_t base;
+    int n;
     bool lj;
 };
 
 // Chunk ch of cluster c: entries [base, base+n) of the LJ part (front of the
 // segment) or of the non-LJ part (back of the segment).
-__device__ __forceinline__ ChunkSpan chunk_span(int ch, int e0, int cap, int2 nn) {
+__device__ __forceinline__ ChunkSpan chunk_span(int ch, size_t e0, int cap, int2 nn) {
     const int chA = (nn.x + 63) >> 6;
     ChunkSpan s;
     s.lj = ch < chA;
@@ -56,7 +57,7 @@ __device__ __forceinline__ ChunkSpan chunk_span(int ch, int e0, int cap, int2 nn
 template <bool ENERGY, bool LMD, bool SMALL>
 __global__ void __launch_bounds__(64 * JP_PW)
 k_pair_f32(int nclu, const int *__restrict__ clu_sys, const SysP *__restrict__ sp, const double4 *__restrict__ ccen,
-           const JAtom *__restrict__ ys, const float2 *__restrict__ lj32, const int *__restrict__ coff,
+           const JAtom *__restrict__ ys, const float2 *__restrict__ lj32, int ccap,
            const int2 *__restrict__ lcnt, const uint2 *__restrict__ ent, const ExcessArgs ex,
            double *__restrict__ gslot, long long *__restrict__ epart) {
     extern __shared__ float2 ljf[];
@@ -68,7 +69,7 @@ k_pair_f32(int nclu, const int *__restrict__ clu_sys, const SysP *__restrict__ s
     const int c = xcd_remap(blockIdx.x, gridDim.x);
     if (c >= nclu) return;   // whole block
     const SysP &P = sp[clu_sys[c]];
-    const int tid = threadIdx.x, l = tid & 63, w = tid >> 6, ia = l & 7, jsub = l >> 3;
+    const int tid = threadIdx.x, l = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6), ia = l & 7, jsub = l >> 3;
     const double4 cen = ccen[c];
     for (int k = tid; k < P.ntyp * P.ntyp; k += 64 * JP_PW) ljf[k] = lj32[P.lj0 + k];
     if (tid < 27) {
@@ -91,7 +92,8 @@ k_pair_f32(int nclu, const int *__restrict__ clu_sys, const SysP *__restrict__ s
     const float rc2 = (float)P.rc2, fcf = (float)P.fcoeff, bcf = (float)P.bcoeff, zcf = (float)P.zcore;
     const float Lx = (float)P.L[0], Ly = (float)P.L[1], Lz = (float)P.L[2];
     const float iLx = (float)P.invL[0], iLy = (float)P.invL[1], iLz = (float)P.invL[2];
-    const int e0 = coff[c], cap = coff[c + 1] - e0;
+    const size_t e0 = (size_t)c * ccap;
+    const int cap = ccap;
     const int2 nn = lcnt[c];
     const int nch = ((nn.x + 63) >> 6) + ((nn.y + 63) >> 6);
     __syncthreads();
@@ -207,7 +209,7 @@ template <bool ENERGY, bool LMD, bool SMALL>
 __global__ void __launch_bounds__(64 * JP_PW)
 k_pair_f64(int nclu, const int *__restrict__ clu_sys, const SysP *__restrict__ sp, const double4 *__restrict__ ccen,
            const JAtom *__restrict__ ys, const double *__restrict__ qd, const double2 *__restrict__ lj64,
-           const int *__restrict__ coff, const int2 *__restrict__ lcnt, const uint2 *__restrict__ ent,
+           int ccap, const int2 *__restrict__ lcnt, const uint2 *__restrict__ ent,
            const ExcessArgs ex, double *__restrict__ gslot, double *__restrict__ epart) {
 #pragma clang fp contract(off)
     extern __shared__ double2 ljs[];
@@ -219,7 +221,7 @@ k_pair_f64(int nclu, const int *__restrict__ clu_sys, const SysP *__restrict__ s
     const int c = xcd_remap(blockIdx.x, gridDim.x);
     if (c >= nclu) return;
     const SysP &P = sp[clu_sys[c]];
-    const int tid = threadIdx.x, l = tid & 63, w = tid >> 6, ia = l & 7, jsub = l >> 3;
+    const int tid = threadIdx.x, l = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6), ia = l & 7, jsub = l >> 3;
     const double4 cen = ccen[c];
     for (int k = tid; k < P.ntyp * P.ntyp; k += 64 * JP_PW) ljs[k] = lj64[P.lj0 + k];
     if (tid < 27) {
@@ -240,7 +242,8 @@ k_pair_f64(int nclu, const int *__restrict__ clu_sys, const SysP *__restrict__ s
         f0 = P.fac[cli * 2]; f1 = P.fac[cli * 2 + 1];
         k0 = P.cls[cli * 2]; k1 = P.cls[cli * 2 + 1];
     }
-    const int e0 = coff[c], cap = coff[c + 1] - e0;
+    const size_t e0 = (size_t)c * ccap;
+    const int cap = ccap;
     const int2 nn = lcnt[c];
     const int nch = ((nn.x + 63) >> 6) + ((nn.y + 63) >> 6);
     __syncthreads();
@@ -340,14 +343,15 @@ k_pair_f64(int nclu, const int *__restrict__ clu_sys, const SysP *__restrict__ s
 // i-cluster.
 __global__ void __launch_bounds__(64)
 k_count_pairs(int nclu, const int *__restrict__ clu_sys, const SysP *__restrict__ sp, const JAtom *__restrict__ ys,
-              const int *__restrict__ coff, const int2 *__restrict__ lcnt, const uint2 *__restrict__ ent,
+              int ccap, const int2 *__restrict__ lcnt, const uint2 *__restrict__ ent,
               unsigned long long *__restrict__ out) {
     const int c = blockIdx.x;
     if (c >= nclu) return;
     const SysP &P = sp[clu_sys[c]];
     const int l = threadIdx.x, ia = l & 7, jsub = l >> 3, islot = c * JP_CL + ia;
     const JAtom ai = ys[islot];
-    const int e0 = coff[c], cap = coff[c + 1] - e0;
+    const size_t e0 = (size_t)c * ccap;
+    const int cap = ccap;
     const int2 nn = lcnt[c];
     unsigned long long n = 0, nlj = 0, nl = 0;
     for (int e = jsub; e < nn.x + nn.y; e += 8) {

@@ -81,7 +81,8 @@ struct Engine {
     double t_total = 0, t_pair = 0, t_list = 0, t_step = 0;
     int n_pair = 0, n_list = 0;
     long long idloop = 0;          // md.f90 idloop: steps done, kept across run calls
-    long long list_cap = 0;
+    int ccap = 0;                   // entries per i-cluster segment of the current list
+    int ccap_want = 2048;           // segment size for the next rebuild (from the last one's maximum)
 
     // device arrays
     DBuf<SysP> d_sp;
@@ -97,12 +98,13 @@ struct Engine {
     DBuf<JAtom> d_ys;
     DBuf<double> d_qd, d_cbb;
     DBuf<double4> d_xr, d_ccen;
+    DBuf<float4> d_lp, d_ccf;
     DBuf<int4> d_imgs;
-    DBuf<int> d_smol, d_nrem, d_nb, d_err;
+    DBuf<int> d_nrem, d_nb, d_err, d_lmax;
     DBuf<unsigned> d_cellext, d_sysext;
     DBuf<int2> d_lcnt;
+    DBuf<int4> d_crec;
     DBuf<uint2> d_ent;
-    DBuf<int> d_cap, d_coff;
     DBuf<double> d_gslot, d_ep64;
     DBuf<long long> d_epfix;
     DBuf<int> d_s2sys, d_s3sys;
@@ -123,9 +125,10 @@ struct Engine {
         d_efac.release(); d_lj64.release(); d_lj32.release(); d_img.release(); d_cellid.release();
         d_refcell.release(); d_subkey.release(); d_cnt.release(); d_start.release(); d_fill.release(); d_tmp.release(); d_inum.release();
         d_perm.release(); d_cslot.release(); d_ys.release(); d_qd.release(); d_cbb.release(); d_xr.release();
-        d_ccen.release(); d_imgs.release(); d_smol.release(); d_nrem.release(); d_nb.release(); d_err.release();
-        d_cellext.release(); d_sysext.release(); d_lcnt.release();
-        d_ent.release(); d_cap.release(); d_coff.release(); d_gslot.release(); d_ep64.release(); d_epfix.release();
+        d_ccen.release(); d_imgs.release(); d_lp.release(); d_ccf.release(); d_nrem.release(); d_nb.release();
+        d_err.release(); d_lmax.release();
+        d_cellext.release(); d_sysext.release(); d_lcnt.release(); d_crec.release();
+        d_ent.release(); d_gslot.release(); d_ep64.release(); d_epfix.release();
         d_s2sys.release(); d_s3sys.release(); d_s2a.release(); d_s3a.release(); d_s2d.release(); d_s3d.release();
         d_u64.release();
     }
@@ -222,6 +225,14 @@ int Engine::init(const jp_sys_desc *descs, int ns) {
             P.rr = 1.0 / siz;
         }
         P.rlim2 = rlim * rlim;
+        {
+            // FP32 list-test band (jp_list.hpp): E = 3 ulp of 1.5 * the longest edge
+            const double lmax = std::max(P.L[0], std::max(P.L[1], P.L[2]));
+            const double ulp = std::ldexp(1.0, (int)std::floor(std::log2(1.5 * lmax)) - 23);
+            const double E = 3.0 * ulp;
+            P.band = std::max(0.01, 4.0 * (2.0 * rlim * std::sqrt(3.0) * E + 4e-7 * rlim * rlim));
+            for (int k = 0; k < 3; ++k) P.ctr[k] = P.wall[k] + 0.5 * P.L[k];
+        }
         P.rc2 = rc * rc;
         P.small = (P.L[0] < 2.0 * rlim || P.L[1] < 2.0 * rlim || P.L[2] < 2.0 * rlim) ? 1 : 0;
         P.irlim2 = 1.0 / (rc * rc);
@@ -372,9 +383,9 @@ int Engine::init(const jp_sys_desc *descs, int ns) {
     AL(d_img, 3 * (size_t)natom); AL(d_cellid, natom); AL(d_subkey, natom); AL(d_refcell, 3 * (size_t)natom);
     AL(d_cnt, ncell + 1); AL(d_start, ncell + 1); AL(d_fill, ncell); AL(d_tmp, natom);
     AL(d_inum, natom); AL(d_perm, nslot); AL(d_cslot, ncell);
-    AL(d_ys, nslot); AL(d_qd, nslot); AL(d_xr, nslot); AL(d_imgs, nslot); AL(d_smol, nslot);
-    AL(d_cbb, 6 * (size_t)nclu); AL(d_ccen, nclu); AL(d_lcnt, nclu); AL(d_cellext, ncell); AL(d_sysext, ns);
-    AL(d_cap, (size_t)nclu + 1); AL(d_coff, (size_t)nclu + 1); AL(d_nrem, ns); AL(d_nb, natom); AL(d_err, 1);
+    AL(d_ys, nslot); AL(d_qd, nslot); AL(d_xr, nslot); AL(d_imgs, nslot); AL(d_lp, nslot);
+    AL(d_cbb, 6 * (size_t)nclu); AL(d_ccen, nclu); AL(d_ccf, nclu); AL(d_lcnt, nclu); AL(d_cellext, ncell);
+    AL(d_sysext, ns); AL(d_crec, ncell); AL(d_nrem, ns); AL(d_nb, natom); AL(d_err, 1); AL(d_lmax, 1);
     AL(d_gslot, 3 * (size_t)nslot);
     AL(d_ep64, 6 * (size_t)nclu); AL(d_epfix, 6 * (size_t)nclu);
     AL(d_s2sys, n2); AL(d_s3sys, n3); AL(d_s2a, n2); AL(d_s3a, n3); AL(d_s2d, 6 * (size_t)n2); AL(d_s3d, 6 * (size_t)n3);
@@ -451,25 +462,35 @@ int Engine::build_list() {
                        d_start.p, d_cnt.p, d_tmp.p, d_inum.p, d_perm.p);
     hipLaunchKernelGGL(k_cslot, grid1(ncell, 256), 256, 0, st, ncell, d_cell_sys.p, d_sp.p, d_start.p, d_cslot.p);
     hipLaunchKernelGGL(k_fill_sorted, grid1(nslot, 256), 256, 0, st, nslot, d_slot_sys.p, d_sp.p, d_perm.p, d_x.p,
-                       d_img.p, d_chg.p, d_tflag.p, d_mol.p, d_ys.p, d_qd.p, d_xr.p, d_imgs.p, d_smol.p);
+                       d_img.p, d_chg.p, d_tflag.p, d_mol.p, d_ys.p, d_qd.p, d_xr.p, d_imgs.p, d_lp.p);
     CK(hipMemsetAsync(d_cellext.p, 0, 4 * (size_t)ncell, st));
     CK(hipMemsetAsync(d_sysext.p, 0, 4 * (size_t)nsys, st));
     hipLaunchKernelGGL(k_cbox, grid1(nclu, 256), 256, 0, st, nclu, d_clu_sys.p, d_sp.p, d_perm.p, d_cellid.p,
-                       d_ccoord.p, d_ys.p, d_cbb.p, d_ccen.p, d_cellext.p, d_sysext.p);
-    CK(hipMemsetAsync(d_nrem.p, 0, 4 * (size_t)nsys, st));
-    const unsigned nbl = grid1(nclu, JP_LW);
-    hipLaunchKernelGGL(k_list_cap8, nbl, 64 * JP_LW, 0, st, nclu, d_clu_sys.p, d_sp.p, d_cbb.p, d_crank.p, d_cslot.p,
-                       d_cnt.p, d_cellext.p, d_sysext.p, d_cap.p, d_err.p);
-    hipLaunchKernelGGL(k_scan, 1, 1024, 0, st, nclu, d_cap.p, d_coff.p);
-    int captot = 0;
-    CK(hipMemcpyAsync(&captot, d_coff.p + nclu, 4, hipMemcpyDeviceToHost, st));
-    CK(hipStreamSynchronize(st));
-    if (captot < 0) return JP_IER_LIST;
-    if ((size_t)captot > d_ent.n) AL(d_ent, (size_t)(captot * 1.2) + 1024);
-    list_cap = captot;
-    hipLaunchKernelGGL(k_list8, nbl, 64 * JP_LW, 0, st, nclu, d_clu_sys.p, d_sp.p, d_cbb.p, d_ccen.p, d_crank.p,
-                       d_cslot.p, d_cnt.p, d_cellext.p, d_sysext.p, d_ys.p, d_xr.p, d_imgs.p, d_smol.p, d_xptr.p,
-                       d_xidx.p, d_coff.p, d_ent.p, d_lcnt.p, d_nrem.p, d_nb.p, d_err.p);
+                       d_ccoord.p, d_ys.p, d_lp.p, d_cbb.p, d_ccen.p, d_ccf.p, d_cellext.p, d_sysext.p);
+    hipLaunchKernelGGL(k_crec, grid1(ncell, 256), 256, 0, st, ncell, d_crank.p, d_cslot.p, d_cnt.p, d_cellext.p,
+                       d_crec.p);
+    // segments of ccap entries per i-cluster; on overflow grow ccap and build again
+    ccap = ccap_want;
+    for (int attempt = 0;; ++attempt) {
+        AL(d_ent, (size_t)nclu * ccap);
+        CK(hipMemsetAsync(d_nrem.p, 0, 4 * (size_t)nsys, st));
+        CK(hipMemsetAsync(d_lmax.p, 0, 4, st));
+        hipLaunchKernelGGL(k_list8, grid1(nclu, JP_LW), 64 * JP_LW, 0, st, nclu, d_clu_sys.p, d_sp.p, d_cbb.p, d_ccf.p,
+                           d_crec.p, d_sysext.p, d_lp.p, d_xr.p, d_imgs.p, d_xptr.p,
+                           d_xidx.p, ccap, d_ent.p, d_lcnt.p, d_nrem.p, d_nb.p, d_err.p, d_lmax.p);
+        int st_err = 0, lmax = 0;
+        CK(hipMemcpyAsync(&st_err, d_err.p, 4, hipMemcpyDeviceToHost, st));
+        CK(hipMemcpyAsync(&lmax, d_lmax.p, 4, hipMemcpyDeviceToHost, st));
+        CK(hipStreamSynchronize(st));
+        if (!(st_err & JP_ERR_LISTCAP)) {
+            // keep ~1/4 headroom for the next rebuilds
+            ccap_want = std::max(256, (lmax + lmax / 4 + 63) / 64 * 64);
+            break;
+        }
+        if (attempt >= 6 || ccap >= (1 << 20)) return JP_IER_LIST;
+        ccap *= 2;
+        CK(hipMemsetAsync(d_err.p, 0, 4, st));
+    }
     CK(hipGetLastError());
     CK(hipEventRecord(ev[3], st));
     std::vector<int> nrem(nsys);
@@ -537,11 +558,11 @@ template <bool E, bool M, bool S>
 static void launch_pair(int prec, unsigned nb, size_t lds32, size_t lds64, hipStream_t st, Engine &g) {
     if (prec == JP_PREC_FP64)
         hipLaunchKernelGGL((k_pair_f64<E, M, S>), nb, 64 * JP_PW, lds64, st, g.nclu, g.d_clu_sys.p, g.d_sp.p,
-                           g.d_ccen.p, g.d_ys.p, g.d_qd.p, g.d_lj64.p, g.d_coff.p, g.d_lcnt.p, g.d_ent.p, g.excess_args(),
+                           g.d_ccen.p, g.d_ys.p, g.d_qd.p, g.d_lj64.p, g.ccap, g.d_lcnt.p, g.d_ent.p, g.excess_args(),
                            g.d_gslot.p, g.d_ep64.p);
     else
         hipLaunchKernelGGL((k_pair_f32<E, M, S>), nb, 64 * JP_PW, lds32, st, g.nclu, g.d_clu_sys.p, g.d_sp.p,
-                           g.d_ccen.p, g.d_ys.p, g.d_lj32.p, g.d_coff.p, g.d_lcnt.p, g.d_ent.p, g.excess_args(),
+                           g.d_ccen.p, g.d_ys.p, g.d_lj32.p, g.ccap, g.d_lcnt.p, g.d_ent.p, g.excess_args(),
                            g.d_gslot.p, g.d_epfix.p);
 }
 
@@ -725,17 +746,17 @@ int Engine::run_nve(int nsteps, int update, int nmon, int prec, jp_energy_log *l
 int Engine::get_list(int32_t *cell3, int64_t *npairs, int32_t *ptr, int32_t *idx, int64_t cap) {
     if (!have_list) return JP_EINVAL;
     CK(hipStreamSynchronize(st));
-    std::vector<int> coff((size_t)nclu + 1), perm(nslot);
+    std::vector<int> perm(nslot);
     std::vector<int2> lc(nclu);
-    std::vector<uint2> ent((size_t)std::max<long long>(list_cap, 1));
-    CK(hipMemcpy(coff.data(), d_coff.p, 4 * coff.size(), hipMemcpyDeviceToHost));
+    std::vector<uint2> ent((size_t)nclu * ccap);
     CK(hipMemcpy(lc.data(), d_lcnt.p, 8 * lc.size(), hipMemcpyDeviceToHost));
     CK(hipMemcpy(perm.data(), d_perm.p, 4 * (size_t)nslot, hipMemcpyDeviceToHost));
-    if (list_cap) CK(hipMemcpy(ent.data(), d_ent.p, sizeof(uint2) * list_cap, hipMemcpyDeviceToHost));
+    CK(hipMemcpy(ent.data(), d_ent.p, sizeof(uint2) * ent.size(), hipMemcpyDeviceToHost));
     if (cell3) CK(hipMemcpy(cell3, d_refcell.p, 12 * (size_t)natom, hipMemcpyDeviceToHost));
     std::vector<std::vector<int>> nb(natom);
     for (int c = 0; c < nclu; ++c) {
-        const int e0 = coff[c], cp = coff[c + 1] - e0;
+        const size_t e0 = (size_t)c * ccap;
+        const int cp = ccap;
         for (int e = 0; e < lc[c].x + lc[c].y; ++e) {
             const uint2 E = ent[e < lc[c].x ? e0 + e : e0 + cp - lc[c].y + (e - lc[c].x)];
             const int aj = perm[E.x & JP_SLOT_MASK];
@@ -765,7 +786,7 @@ int Engine::get_list(int32_t *cell3, int64_t *npairs, int32_t *ptr, int32_t *idx
 int Engine::count_pairs(int64_t *np, int64_t *nlj, int64_t *nlist) {
     if (!have_list) return JP_EINVAL;
     CK(hipMemsetAsync(d_u64.p, 0, 24, st));
-    hipLaunchKernelGGL(k_count_pairs, nclu, 64, 0, st, nclu, d_clu_sys.p, d_sp.p, d_ys.p, d_coff.p, d_lcnt.p,
+    hipLaunchKernelGGL(k_count_pairs, nclu, 64, 0, st, nclu, d_clu_sys.p, d_sp.p, d_ys.p, ccap, d_lcnt.p,
                        d_ent.p, d_u64.p);
     unsigned long long c[3] = {0, 0, 0};
     CK(hipMemcpyAsync(c, d_u64.p, 24, hipMemcpyDeviceToHost, st));
