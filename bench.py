@@ -111,8 +111,23 @@ def traffic_from_profile():
     return None
 
 
+def calibrate_copy(nbytes: int = 1 << 30, reps: int = 5):
+    """Plain device-to-device copies of a known size: the reference point
+    for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes."""
+    import torch
+    src = torch.empty(nbytes, dtype=torch.uint8, device="cuda").fill_(1)
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": nbytes, "reps": reps}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run 1 GiB device copies (profiler traffic calibration)")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=200)
@@ -121,6 +136,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--equil", type=int, default=1000, help="velocity-rescaled set-up steps before timing")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus})",
               file=sys.stderr)
