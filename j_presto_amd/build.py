@@ -1,4 +1,5 @@
-"""In-tree build of libjpgpu.so for gfx950 (hipcc)."""
+"""In-tree builds: libjpgpu.so for gfx950 (hipcc) and the Fortran host
+program fortran/jp_host (flang, ISO_C_BINDING module fortran/jpgpu.f90)."""
 from __future__ import annotations
 
 import os
@@ -11,7 +12,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
-    # packed FP32 VALU ops stay off (GPU box notes)
+    # no packed FP32 VALU instructions in device code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-value", "-Wno-unused-result",
 ]
@@ -28,5 +29,27 @@ def build_library(verbose: bool = False) -> str:
     return OUT
 
 
+FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
+FDIR = os.path.join(HERE, "fortran")
+FHOST = os.path.join(FDIR, "jp_host")
+
+
+def build_fortran(verbose: bool = False) -> str:
+    """Fortran module + host program, linked against the in-tree library
+    (runpath $ORIGIN/.., so the pair travels together)."""
+    mdir = os.path.join(FDIR, "_build")
+    os.makedirs(mdir, exist_ok=True)
+    cmd = [FLANG, "-O2", "-J", mdir, os.path.join(FDIR, "jpgpu.f90"), os.path.join(FDIR, "jp_host.f90"),
+           "-L" + HERE, "-ljpgpu", "-Wl,-rpath,$ORIGIN/..", "-o", FHOST + ".tmp"]
+    r = subprocess.run(cmd, capture_output=True, text=True, cwd=mdir)
+    if r.returncode != 0:
+        raise RuntimeError("flang failed:\n" + r.stderr[-4000:])
+    os.replace(FHOST + ".tmp", FHOST)
+    if verbose:
+        print(" ".join(cmd))
+    return FHOST
+
+
 if __name__ == "__main__":
     print(build_library(verbose=True))
+    print(build_fortran(verbose=True))

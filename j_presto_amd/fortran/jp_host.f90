@@ -1,0 +1,147 @@
+! Fortran host of libjpgpu.  Reads a system file, then does through the C ABI
+! what md_run does through its own routines:
+!   celprp (jp_build_list), dnergy's nonbonded part (grad zeroed, then
+!   jp_nonbond accumulates; FP64 and FP32/fixed-point), and the METHOD= MICR
+!   MD loop (jp_run_nve), and writes every result to an output file.
+!
+! usage: jp_host <system file> <output file> <nsteps> <update> <nmon> <precision 0|1>
+!        jp_host --layout   (byte sizes and offsets of the bind(C) types; no GPU use)
+!
+! System file (Fortran unformatted sequential, one array per record):
+!   natom ntyp nfrag nfrg nexcl nexcess nshake shake_maxit cell_level lmd   (int32)
+!   shake_tol cutoff diel dt lambda celwal(6)                               (real*8)
+!   cord(3,N) vel(3,N) mass(N) charge(N) atype(N) icls(N) rmin2(ntyp) eps(ntyp)
+!   n_matrix(nfrag,nfrag) excl(2,nexcl) excess(2,nexcess)
+!   shake_natom(nshake) shake_atoms(4,nshake) shake_dist2(6,nshake)
+! Output file (same record format):
+!   ier(6): create, build_list, nonbond FP64, nonbond FP32, run_nve, get_state
+!   grad64(3,N,nfrg) eelec64(nfrg) evdw64(nfrg) grad32(3,N,nfrg) eelec32(nfrg) evdw32(nfrg)
+!   cord(3,N) vel(3,N) after the run, nlog, then per record: step etot epot ekin
+program jp_host
+  use iso_c_binding
+  use jpgpu
+  implicit none
+  character(len=1024) :: fin, fout, arg
+  integer(c_int32_t) :: hdr(10), nsteps, update, nmon, prec
+  real(c_double) :: rhdr(11)
+  integer :: n, ntyp, nfrag, nfrg, nexcl, nexcess, nshake, nlog, k, u
+  integer(c_int) :: ier(6)
+  real(c_double), allocatable, target :: cord(:,:), vel(:,:), mass(:), charge(:), rmin2(:), eps(:)
+  real(c_double), allocatable, target :: sdist(:,:), grad64(:,:,:), grad32(:,:,:)
+  real(c_double), target :: eel64(3), evd64(3), ehy64(3), eel32(3), evd32(3), ehy32(3)
+  integer(c_int32_t), allocatable, target :: atype(:), icls(:), nmat(:,:), excl(:,:), excess(:,:)
+  integer(c_int32_t), allocatable, target :: snat(:), satm(:,:)
+  type(jp_energy_log), allocatable, target :: elog(:)
+  type(jp_sys_desc), target :: desc
+  type(jp_energy_log), target :: rec1
+  type(c_ptr) :: ctx
+
+  call get_command_argument(1, arg)
+  if (trim(arg) == '--layout') then
+     write(*,'(a,i0)') 'sizeof jp_sys_desc ', c_sizeof(desc)
+     write(*,'(a,i0)') 'offset excl ', off(c_loc(desc%excl))
+     write(*,'(a,i0)') 'offset shake_tol ', off(c_loc(desc%shake_tol))
+     write(*,'(a,i0)') 'offset celwal ', off(c_loc(desc%celwal))
+     write(*,'(a,i0)') 'offset lmd ', off(c_loc(desc%lmd))
+     write(*,'(a,i0)') 'offset lambda ', off(c_loc(desc%lambda))
+     write(*,'(a,i0)') 'sizeof jp_energy_log ', c_sizeof(rec1)
+     stop
+  end if
+  if (command_argument_count() < 6) then
+     write(*,'(a)') 'usage: jp_host <system file> <output file> <nsteps> <update> <nmon> <precision 0|1>'
+     stop 2
+  end if
+  call get_command_argument(1, fin)
+  call get_command_argument(2, fout)
+  call get_command_argument(3, arg); read(arg, *) nsteps
+  call get_command_argument(4, arg); read(arg, *) update
+  call get_command_argument(5, arg); read(arg, *) nmon
+  call get_command_argument(6, arg); read(arg, *) prec
+
+  open(newunit=u, file=trim(fin), form='unformatted', access='sequential', status='old', action='read')
+  read(u) hdr
+  read(u) rhdr
+  n = hdr(1); ntyp = hdr(2); nfrag = hdr(3); nfrg = hdr(4)
+  nexcl = hdr(5); nexcess = hdr(6); nshake = hdr(7)
+  allocate(cord(3,n), vel(3,n), mass(n), charge(n), atype(n), icls(n), rmin2(ntyp), eps(ntyp))
+  allocate(nmat(nfrag,nfrag), excl(2,max(nexcl,1)), excess(2,max(nexcess,1)))
+  allocate(snat(max(nshake,1)), satm(4,max(nshake,1)), sdist(6,max(nshake,1)))
+  read(u) cord
+  read(u) vel
+  read(u) mass
+  read(u) charge
+  read(u) atype
+  read(u) icls
+  read(u) rmin2
+  read(u) eps
+  read(u) nmat
+  read(u) excl(:, 1:nexcl)
+  read(u) excess(:, 1:nexcess)
+  read(u) snat(1:nshake)
+  read(u) satm(:, 1:nshake)
+  read(u) sdist(:, 1:nshake)
+  close(u)
+
+  desc%natom = n; desc%ntyp = ntyp; desc%nfrag = nfrag; desc%nfrg = nfrg
+  desc%x = c_loc(cord); desc%mass = c_loc(mass); desc%charge = c_loc(charge)
+  desc%atype = c_loc(atype); desc%rmin2 = c_loc(rmin2); desc%eps = c_loc(eps)
+  desc%icls = c_loc(icls); desc%n_matrix = c_loc(nmat)
+  desc%nexcl = nexcl; desc%excl = c_loc(excl)
+  desc%nexcess = nexcess; desc%excess = c_loc(excess)
+  desc%nshake = nshake; desc%shake_natom = c_loc(snat)
+  desc%shake_atoms = c_loc(satm); desc%shake_dist2 = c_loc(sdist)
+  desc%shake_maxit = hdr(8); desc%cell_level = hdr(9); desc%lmd = hdr(10)
+  desc%shake_tol = rhdr(1); desc%cutoff = rhdr(2); desc%diel = rhdr(3); desc%dt = rhdr(4)
+  desc%lambda = rhdr(5); desc%celwal = rhdr(6:11)
+
+  write(*,'(2a)') 'jp_host: ', jp_version()
+  ier = 0
+  ier(1) = jp_ctx_create(desc, ctx)
+  if (ier(1) /= JP_OK) then
+     write(*,'(a,i0)') 'jp_ctx_create: ier = ', ier(1)
+     stop 1
+  end if
+  ier(1) = jp_set_state(ctx, c_null_ptr, c_loc(vel))
+  ier(2) = jp_build_list(ctx, c_null_ptr)                 ! celprp
+
+  ! dnergy: grad zeroed, the nonbonded part accumulated (dnergy.f90:45-55, cmmdo)
+  allocate(grad64(3,n,nfrg), grad32(3,n,nfrg))
+  grad64 = 0.0d0; eel64 = 0.0d0; evd64 = 0.0d0; ehy64 = 0.0d0
+  ier(3) = jp_nonbond(ctx, c_null_ptr, c_loc(grad64), c_loc(eel64), c_loc(evd64), c_loc(ehy64), JP_PREC_FP64)
+  grad32 = 0.0d0; eel32 = 0.0d0; evd32 = 0.0d0; ehy32 = 0.0d0
+  ier(4) = jp_nonbond(ctx, c_null_ptr, c_loc(grad32), c_loc(eel32), c_loc(evd32), c_loc(ehy32), JP_PREC_FP32_FIXED)
+
+  ! METHOD= MICR loop
+  nlog = 0
+  if (nmon > 0) nlog = nsteps / nmon
+  allocate(elog(max(nlog,1)))
+  ier(5) = jp_run_nve(ctx, nsteps, update, nmon, prec, c_loc(elog))
+  ier(6) = jp_get_state(ctx, c_loc(cord), c_loc(vel))
+  write(*,'(a,6(1x,i0))') 'jp_host: ier =', ier
+  if (nlog > 0) write(*,'(a,i0,a,f16.6)') 'jp_host: step ', elog(nlog)%step, ' etot ', elog(nlog)%etot
+
+  open(newunit=u, file=trim(fout), form='unformatted', access='sequential', status='replace', action='write')
+  write(u) ier
+  write(u) grad64
+  write(u) eel64(1:nfrg)
+  write(u) evd64(1:nfrg)
+  write(u) grad32
+  write(u) eel32(1:nfrg)
+  write(u) evd32(1:nfrg)
+  write(u) cord
+  write(u) vel
+  write(u) nlog
+  do k = 1, nlog
+     write(u) elog(k)%step, elog(k)%etot, elog(k)%epot, elog(k)%ekin
+  end do
+  close(u)
+  ier(1) = jp_ctx_destroy(ctx)
+
+contains
+
+  integer function off(p)
+    type(c_ptr), intent(in) :: p
+    off = int(transfer(p, 0_c_intptr_t) - transfer(c_loc(desc), 0_c_intptr_t))
+  end function off
+
+end program jp_host

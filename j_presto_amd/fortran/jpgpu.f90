@@ -1,0 +1,189 @@
+! ISO_C_BINDING interface of libjpgpu (include/jpgpu.h): the MI355X path for
+! md_run's ZD + LJ nonbonded forces, the pair list that feeds them, and the
+! SHAKE / leap-frog step.
+!
+! Every routine returns md_run's ier code for the routine it replaces
+! (0 ok; 28 cell overflow, 51 list overflow, 99 excluded pair missing from
+! the list, 1 SHAKE not converged, -2 SHAKE singular); values <= -100 are
+! runtime errors.  Arrays are passed in md_run's own layout: cord(3,N),
+! vel(3,N), grad(3,N,nfrg), 1-based atom indices.  The reference routines
+! each entry point stands in for:
+!   jp_ctx_create   incel.f90:2 + inpcls.f90 + setshk.f90 (set-up)
+!   jp_build_list   celprp.f90:2 (celpar_PB, cutatm_PB, cutoff_PB)
+!   jp_nonbond      cmmdo.f90:2 -> enear_ZD_*_PB (enear.f90:1-593)
+!   jp_shake        shake.f90:2 (shake_PB)
+!   jp_run_nve      md.f90:519-694 + verlet.f90:2 (METHOD= MICR loop)
+!   jp_batch_*      j_presto_mpi:27-44 (independent runs, one launch)
+module jpgpu
+  use iso_c_binding
+  implicit none
+  private :: c_strlen
+
+  integer(c_int), parameter :: JP_OK = 0
+  integer(c_int), parameter :: JP_IER_CELL = 28, JP_IER_LIST = 51, JP_IER_EXCL = 99
+  integer(c_int), parameter :: JP_IER_SHAKE_CONV = 1, JP_IER_SHAKE_SING = -2
+  integer(c_int), parameter :: JP_PREC_FP64 = 0, JP_PREC_FP32_FIXED = 1
+
+  ! System description: the module state md_run builds in its input phase.
+  ! Same field order and C layout as jp_sys_desc in include/jpgpu.h.
+  type, bind(C) :: jp_sys_desc
+     integer(c_int32_t) :: natom = 0, ntyp = 0, nfrag = 1, nfrg = 1
+     type(c_ptr) :: x = c_null_ptr, mass = c_null_ptr, charge = c_null_ptr
+     type(c_ptr) :: atype = c_null_ptr, rmin2 = c_null_ptr, eps = c_null_ptr
+     type(c_ptr) :: icls = c_null_ptr, n_matrix = c_null_ptr
+     integer(c_int32_t) :: nexcl = 0
+     type(c_ptr) :: excl = c_null_ptr
+     integer(c_int32_t) :: nexcess = 0
+     type(c_ptr) :: excess = c_null_ptr
+     integer(c_int32_t) :: nshake = 0
+     type(c_ptr) :: shake_natom = c_null_ptr, shake_atoms = c_null_ptr, shake_dist2 = c_null_ptr
+     real(c_double) :: shake_tol = 1.0d-6
+     integer(c_int32_t) :: shake_maxit = 1000
+     integer(c_int32_t) :: cell_level = 0
+     real(c_double) :: celwal(6) = 0.0d0
+     real(c_double) :: cutoff = 12.0d0, diel = 1.0d0, dt = 2.0d0
+     integer(c_int32_t) :: lmd = 0
+     real(c_double) :: lambda = 0.0d0
+  end type jp_sys_desc
+
+  ! One monitoring record of jp_run_nve (md.f90:618-628).
+  type, bind(C) :: jp_energy_log
+     integer(c_int32_t) :: step
+     real(c_double) :: etot, epot, ekin
+     real(c_double) :: eelec(3), evdw(3)
+  end type jp_energy_log
+
+  interface
+     integer(c_int) function jp_ctx_create(desc, ctx) bind(C, name='jp_ctx_create')
+       import :: c_int, c_ptr, jp_sys_desc
+       type(jp_sys_desc), intent(in) :: desc
+       type(c_ptr), intent(out) :: ctx
+     end function jp_ctx_create
+
+     integer(c_int) function jp_ctx_destroy(ctx) bind(C, name='jp_ctx_destroy')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+     end function jp_ctx_destroy
+
+     ! cord / vel: c_loc of real(8) (3,N) arrays, or c_null_ptr
+     integer(c_int) function jp_set_state(ctx, cord, vel) bind(C, name='jp_set_state')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx, cord, vel
+     end function jp_set_state
+
+     integer(c_int) function jp_get_state(ctx, cord, vel) bind(C, name='jp_get_state')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx, cord, vel
+     end function jp_get_state
+
+     integer(c_int) function jp_build_list(ctx, cord) bind(C, name='jp_build_list')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx, cord
+     end function jp_build_list
+
+     integer(c_int) function jp_get_list(ctx, cell_of_atom, npairs, ptr, idx, cap) bind(C, name='jp_get_list')
+       import :: c_int, c_ptr, c_int64_t
+       type(c_ptr), value :: ctx, cell_of_atom
+       integer(c_int64_t), intent(out) :: npairs
+       type(c_ptr), value :: ptr, idx
+       integer(c_int64_t), value :: cap
+     end function jp_get_list
+
+     ! grad is accumulated (md_run zeroes it in dnergy before cmmdo)
+     integer(c_int) function jp_nonbond(ctx, cord, grad, eelec, evdw, ehyd, precision) bind(C, name='jp_nonbond')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx, cord, grad, eelec, evdw, ehyd
+       integer(c_int), value :: precision
+     end function jp_nonbond
+
+     integer(c_int) function jp_shake(ctx, cord, cord_old) bind(C, name='jp_shake')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx, cord, cord_old
+     end function jp_shake
+
+     integer(c_int) function jp_run_nve(ctx, nsteps, update, nmon, precision, log) bind(C, name='jp_run_nve')
+       import :: c_int, c_int32_t, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: nsteps, update, nmon, precision
+       type(c_ptr), value :: log
+     end function jp_run_nve
+
+     integer(c_int) function jp_last_timing(ctx, t_total, t_pair, t_list, t_step) bind(C, name='jp_last_timing')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out) :: t_total, t_pair, t_list, t_step
+     end function jp_last_timing
+
+     integer(c_int) function jp_count_pairs(ctx, npairs, nlj, nlisted) bind(C, name='jp_count_pairs')
+       import :: c_int, c_ptr, c_int64_t
+       type(c_ptr), value :: ctx
+       integer(c_int64_t), intent(out) :: npairs, nlj, nlisted
+     end function jp_count_pairs
+
+     integer(c_int) function jp_batch_create(descs, nsys, b) bind(C, name='jp_batch_create')
+       import :: c_int, c_int32_t, c_ptr, jp_sys_desc
+       type(jp_sys_desc), intent(in) :: descs(*)
+       integer(c_int32_t), value :: nsys
+       type(c_ptr), intent(out) :: b
+     end function jp_batch_create
+
+     integer(c_int) function jp_batch_run_nve(b, nsteps, update, nmon, precision, log) bind(C, name='jp_batch_run_nve')
+       import :: c_int, c_int32_t, c_ptr
+       type(c_ptr), value :: b
+       integer(c_int32_t), value :: nsteps, update, nmon, precision
+       type(c_ptr), value :: log
+     end function jp_batch_run_nve
+
+     integer(c_int) function jp_batch_set_state(b, isys, cord, vel) bind(C, name='jp_batch_set_state')
+       import :: c_int, c_int32_t, c_ptr
+       type(c_ptr), value :: b
+       integer(c_int32_t), value :: isys
+       type(c_ptr), value :: cord, vel
+     end function jp_batch_set_state
+
+     integer(c_int) function jp_batch_get_state(b, isys, cord, vel) bind(C, name='jp_batch_get_state')
+       import :: c_int, c_int32_t, c_ptr
+       type(c_ptr), value :: b
+       integer(c_int32_t), value :: isys
+       type(c_ptr), value :: cord, vel
+     end function jp_batch_get_state
+
+     integer(c_int) function jp_batch_last_timing(b, t_total) bind(C, name='jp_batch_last_timing')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: b
+       real(c_double), intent(out) :: t_total
+     end function jp_batch_last_timing
+
+     integer(c_int) function jp_batch_destroy(b) bind(C, name='jp_batch_destroy')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: b
+     end function jp_batch_destroy
+
+     type(c_ptr) function jp_version_c() bind(C, name='jp_version')
+       import :: c_ptr
+     end function jp_version_c
+
+     integer(c_size_t) function c_strlen(s) bind(C, name='strlen')
+       import :: c_ptr, c_size_t
+       type(c_ptr), value :: s
+     end function c_strlen
+  end interface
+
+contains
+
+  ! Library version string as a Fortran character value.
+  function jp_version() result(v)
+    character(len=:), allocatable :: v
+    type(c_ptr) :: p
+    character(kind=c_char), pointer :: s(:)
+    integer :: i, n
+    p = jp_version_c()
+    n = int(c_strlen(p))
+    call c_f_pointer(p, s, [n])
+    allocate(character(len=n) :: v)
+    do i = 1, n
+       v(i:i) = s(i)
+    end do
+  end function jp_version
+
+end module jpgpu

@@ -1,0 +1,95 @@
+"""The Fortran host (j_presto_amd/fortran/jp_host, ISO_C_BINDING module
+jpgpu.f90) drives the C ABI the way md_run's routines would.
+
+CPU: the bind(C) types have the C layout of include/jpgpu.h (the ctypes
+mirror is checked against the header by test_abi); the program runs.
+GPU: one system file through jp_host; forces against the golden fixtures of
+the compiled reference, and the MD run bit-identical to the same run driven
+from Python through the same library."""
+import os
+import subprocess
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from conftest import rel_rms
+from j_presto_amd import engine, refio
+from j_presto_amd.abi import EnergyLog, SysDesc
+from j_presto_amd.build import FHOST
+
+
+def _host():
+    if not os.path.exists(FHOST):
+        pytest.fail(f"{FHOST} is missing: run __graft_entry__.build()")
+    return FHOST
+
+
+def test_bindc_layout_matches_c():
+    out = subprocess.run([_host(), "--layout"], capture_output=True, text=True, timeout=60).stdout
+    got = {}
+    for line in out.splitlines():
+        parts = line.split()
+        if len(parts) >= 3 and parts[0] in ("sizeof", "offset"):
+            got[(parts[0], parts[1])] = int(parts[2])
+    assert got[("sizeof", "jp_sys_desc")] == C.sizeof(SysDesc)
+    assert got[("sizeof", "jp_energy_log")] == C.sizeof(EnergyLog)
+    for field in ("excl", "shake_tol", "celwal", "lmd"):
+        assert got[("offset", field)] == getattr(SysDesc, field).offset, field
+    assert got[("offset", "lambda")] == SysDesc.lambda_.offset
+
+
+def test_host_usage_exit_code():
+    r = subprocess.run([_host()], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "usage" in r.stdout
+
+
+def test_system_file_roundtrip(tmp_path):
+    """write_jpsys writes the records jp_host reads, in its order."""
+    from j_presto_amd import systems
+    s = systems.chain_in_water(48, 24.6, 22, lmd=0.8)
+    p = str(tmp_path / "sys.bin")
+    refio.write_jpsys(p, s)
+    r = refio.read_records(p)
+    assert len(r) == 16
+    hdr = np.frombuffer(r[0], np.int32)
+    n, ntyp, nfrag, nfrg, nexcl, nexcess, nshake = hdr[:7]
+    assert n == s.n and nfrag == 2 and nfrg == 3 and hdr[9] == 1
+    assert len(r[2]) == 24 * n and np.array_equal(np.frombuffer(r[2], np.float64).reshape(n, 3), s.x)
+    assert len(r[10]) == 4 * nfrag * nfrag
+    assert len(r[11]) == 8 * nexcl and len(r[12]) == 8 * nexcess
+    assert len(r[13]) == 4 * nshake and len(r[14]) == 16 * nshake and len(r[15]) == 48 * nshake
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["water_1500", "chain_lmd"])
+def test_fortran_host_run(golden, tmp_path, name):
+    s, f = golden(name)
+    print(f"case {name} n={s.n}", flush=True)
+    sysf, outf = str(tmp_path / "sys.bin"), str(tmp_path / "out.bin")
+    refio.write_jpsys(sysf, s)
+    nsteps, update, nmon = 20, 10, 10
+    r = subprocess.run([_host(), sysf, outf, str(nsteps), str(update), str(nmon), "0"],
+                       capture_output=True, text=True, timeout=240)
+    print(r.stdout, flush=True)
+    assert r.returncode == 0, r.stderr
+    nfrg = 3 if s.cluster_method.startswith("LMD") else 1
+    o = refio.read_jphost(outf, s.n, nfrg)
+    assert np.all(o["ier"] == 0), o["ier"]
+
+    def comb(g):
+        return g[0] * s.lam ** 2 + g[1] * s.lam + g[2] if nfrg == 3 else g[0]
+    assert rel_rms(comb(o["grad64"]), f["force_grad"]) < 1e-12
+    assert rel_rms(comb(o["grad32"]), f["force_grad"]) < 1e-5
+    # the same run driven from Python through the same library: identical bits
+    md = engine.MdSystem(s, precision=engine.FP64)
+    assert md.celprp() == 0
+    md.cmmdo()
+    md.cmmdo(precision=engine.FP32)
+    log, ier = md.runmd(nsteps, update=update, nmon=nmon)
+    assert ier == 0
+    x, v = md.get_state()
+    md.close()
+    assert np.array_equal(x, o["x"]) and np.array_equal(v, o["v"])
+    assert [L[0] for L in o["log"]] == list(log["step"])
+    assert np.array_equal(np.array([L[1] for L in o["log"]]), log["etot"])
