@@ -130,7 +130,11 @@ int jp_run_nve(jp_ctx *ctx, int32_t nsteps, int32_t update, int32_t nmon, int32_
                jp_energy_log *log);
 
 /* Seconds spent in the last jp_run_nve, measured with HIP events on the
- * context stream: total, pair kernel, list build, SHAKE + integration. */
+ * context stream: total, pair kernel, list build, the rest (SHAKE,
+ * integration, launch gaps).  The pair time is the mean over the pair
+ * launches issued from the host (energy steps, or every step with
+ * JP_OPT_GRAPHS = 0) times the number of steps; steps replayed from a HIP
+ * graph are not timed one by one. */
 int jp_last_timing(jp_ctx *ctx, double *t_total, double *t_pair, double *t_list, double *t_step);
 
 /* Unique pairs with r <= cutoff at the device coordinates (the
@@ -152,6 +156,14 @@ int jp_batch_set_state(jp_batch *b, int32_t isys, const double *cord, const doub
 int jp_batch_get_state(jp_batch *b, int32_t isys, double *cord, double *vel);
 int jp_batch_last_timing(jp_batch *b, double *t_total);
 int jp_batch_destroy(jp_batch *b);
+
+/* Run options (no counterpart in md_run).  JP_OPT_GRAPHS: 1 (default) runs
+ * the plain steps between list rebuilds and energy steps of jp_run_nve /
+ * jp_batch_run_nve as captured HIP graphs; 0 launches every kernel from the
+ * host.  Results are identical either way. */
+#define JP_OPT_GRAPHS 1
+int jp_set_option(jp_ctx *ctx, int32_t option, int32_t value);
+int jp_batch_set_option(jp_batch *b, int32_t option, int32_t value);
 
 /* Library version and device name (for logs). */
 const char *jp_version(void);

@@ -43,7 +43,7 @@ struct SysP {
     int lj0, ntyp;                   // LJ table offset / size
     int hbits;                       // nc = 2^hbits cells per edge
     int small;                       // some edge < 2*rlim: minimum image per pair
-    int pad_;
+    int nblim;                       // listed partners allowed per atom (2*n15mx, cutoff.f90:30-46)
     double dt;
     double ctr[3];                   // box centre: origin of the FP32 list-test frame
     double band;                     // A^2: FP32 list-test results within band of rlim^2 take the FP64 test
@@ -118,6 +118,19 @@ __host__ __device__ inline uint32_t hilbert3(uint32_t x, uint32_t y, uint32_t z,
 // --------------------------------------------------------------------------
 // rebuild: binning
 // --------------------------------------------------------------------------
+
+// Start of a rebuild: clear the per-cell counters and extents, the per-system
+// extents, the list status words, and mark every slot as padding.
+__global__ void k_list_reset(int ncell, int *__restrict__ cnt, int *__restrict__ fill, unsigned *__restrict__ cellext,
+                             int nsys, unsigned *__restrict__ sysext, int nstat, int *__restrict__ lst, int nslot,
+                             int *__restrict__ perm) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i <= ncell) cnt[i] = 0;
+    if (i < ncell) { fill[i] = 0; cellext[i] = 0u; }
+    if (i < nsys) sysext[i] = 0u;
+    if (i < nstat) lst[i] = 0;
+    if (i < nslot) perm[i] = -1;
+}
 
 // Wrap into the box and bin (celpar.f90:115-127 for md_run's cell numbering;
 // the same wrapped coordinate g is binned on the box-aligned grid).

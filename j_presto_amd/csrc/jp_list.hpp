@@ -36,7 +36,11 @@
 #define JP_CL 8          // atoms per i-cluster
 #define JP_LW 4          // i-clusters (wavefronts) per block in the list kernels
 #define JP_QCAP 128      // per-wave queue of candidate j-clusters (power of 2)
-#define JP_ERR_LISTCAP 16
+// list status words lst[]: [0] error bits, [1] longest list above the high-water
+// mark, [2 + s] excluded pairs met within rlim by system s
+#define JP_ERR_CELL 1
+#define JP_ERR_LISTCAP 16     // a segment overflowed: grow ccap and build again
+#define JP_ERR_NBLIM 32       // an atom has more listed partners than md_run allows (ier 51)
 
 // Orders LDS accesses of one wavefront (lanes write, other lanes read).
 __device__ __forceinline__ void wave_lds_sync() {
@@ -215,9 +219,8 @@ __global__ void __launch_bounds__(64 * JP_LW)
 k_list8(int nclu, const int *__restrict__ clu_sys, const SysP *__restrict__ sp, const double *__restrict__ cbb,
         const float4 *__restrict__ ccf, const int4 *__restrict__ crec, const unsigned *__restrict__ sysext,
         const float4 *__restrict__ lp, const double4 *__restrict__ xr, const int4 *__restrict__ imgs,
-        const int *__restrict__ xptr, const int *__restrict__ xidx, int ccap, uint2 *__restrict__ ent,
-        int2 *__restrict__ lcnt, int *__restrict__ nremoved, int *__restrict__ nbcount, int *__restrict__ err,
-        int *__restrict__ lmax) {
+        const int *__restrict__ xptr, const int *__restrict__ xidx, int ccap, int hiwater, uint2 *__restrict__ ent,
+        int2 *__restrict__ lcnt, int *__restrict__ lst) {
     __shared__ uint32_t s_c[JP_LW][JP_QCAP];   // candidate j-clusters of passing cells
     __shared__ uint32_t s_q[JP_LW][JP_QCAP];   // candidates whose box is within rlim
     __shared__ double4 s_xr[JP_LW][JP_CL];
@@ -399,10 +402,10 @@ k_list8(int nclu, const int *__restrict__ clu_sys, const SysP *__restrict__ sp, 
     }
     if (l == 0) {
         lcnt[ci] = make_int2(nA, nB);
-        if (ovf) atomicOr(err, JP_ERR_LISTCAP);
-        atomicMax(lmax, nA + nB);
+        if (ovf) atomicOr(lst, JP_ERR_LISTCAP);
+        if (nA + nB > hiwater) atomicMax(lst + 1, nA + nB);
     }
-    if (l < JP_CL && ai >= 0) nbcount[ai] = nbv;
+    if (l < JP_CL && ai >= 0 && nbv > P.nblim) atomicOr(lst, JP_ERR_NBLIM);
     const int nr = wave_sum(nrem);
-    if (l == 0 && nr) atomicAdd(nremoved + s, nr);
+    if (l == 0 && nr) atomicAdd(lst + 2 + s, nr);
 }

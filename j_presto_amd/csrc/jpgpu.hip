@@ -79,7 +79,7 @@ struct Engine {
     hipStream_t st = nullptr;
     hipEvent_t ev[8];
     double t_total = 0, t_pair = 0, t_list = 0, t_step = 0;
-    int n_pair = 0, n_list = 0;
+    int n_pair = 0, n_list = 0, n_pair_steps = 0;   // timed pair launches, steps of the last run
     long long idloop = 0;          // md.f90 idloop: steps done, kept across run calls
     int ccap = 0;                   // entries per i-cluster segment of the current list
     int ccap_want = 2048;           // segment size for the next rebuild (from the last one's maximum)
@@ -100,7 +100,7 @@ struct Engine {
     DBuf<double4> d_xr, d_ccen;
     DBuf<float4> d_lp, d_ccf;
     DBuf<int4> d_imgs;
-    DBuf<int> d_nrem, d_nb, d_err, d_lmax;
+    DBuf<int> d_err, d_lst;   // SHAKE error bits; list status words (jp_list.hpp)
     DBuf<unsigned> d_cellext, d_sysext;
     DBuf<int2> d_lcnt;
     DBuf<int4> d_crec;
@@ -113,6 +113,7 @@ struct Engine {
     DBuf<unsigned long long> d_u64;
 
     ~Engine() {
+        drop_graphs();
         if (st) {
             hipStreamSynchronize(st);
             for (auto &e : ev) hipEventDestroy(e);
@@ -125,8 +126,7 @@ struct Engine {
         d_efac.release(); d_lj64.release(); d_lj32.release(); d_img.release(); d_cellid.release();
         d_refcell.release(); d_subkey.release(); d_cnt.release(); d_start.release(); d_fill.release(); d_tmp.release(); d_inum.release();
         d_perm.release(); d_cslot.release(); d_ys.release(); d_qd.release(); d_cbb.release(); d_xr.release();
-        d_ccen.release(); d_imgs.release(); d_lp.release(); d_ccf.release(); d_nrem.release(); d_nb.release();
-        d_err.release(); d_lmax.release();
+        d_ccen.release(); d_imgs.release(); d_lp.release(); d_ccf.release(); d_err.release(); d_lst.release();
         d_cellext.release(); d_sysext.release(); d_lcnt.release(); d_crec.release();
         d_ent.release(); d_gslot.release(); d_ep64.release(); d_epfix.release();
         d_s2sys.release(); d_s3sys.release(); d_s2a.release(); d_s3a.release(); d_s2d.release(); d_s3d.release();
@@ -146,11 +146,28 @@ struct Engine {
         return a;
     }
     int pair(int prec, bool energy);
+    int pair_variant() const;
     int set_factors(int onehot_class);  // -1: run factors (lambda), k: one-hot class k
     int nonbond(const double *cord, double *grad, double *eelec, double *evdw, int prec);
     int energies(int prec, std::vector<double> &eel, std::vector<double> &evd, std::vector<double> &eex);
     int shake(int &ier_out);
     int run_nve(int nsteps, int update, int nmon, int prec, jp_energy_log *log);
+    int integrate(int prec, bool mon);   // kick + SHAKE + v update of one step (after the forces)
+    int mon_record(int step, int nsteps, int nmon, int nlog, const std::vector<double> &eel,
+                   const std::vector<double> &evd, jp_energy_log *log);
+
+    // Steps between list rebuilds and energy steps run as captured HIP graphs:
+    // nsteps x (pair, kick, SHAKE, post), cached per shape.  Pair-kernel
+    // times come from host-launched steps only (t_pair / n_pair).
+    struct StepGraph {
+        hipGraphExec_t exec = nullptr;
+        int nsteps = 0, prec = -1, ccap = -1, variant = -1;
+        const void *ent = nullptr;
+    };
+    std::vector<StepGraph> graphs;
+    int use_graphs = 1;
+    StepGraph *step_graph(int n, int prec);
+    void drop_graphs();
     int get_list(int32_t *cell3, int64_t *npairs, int32_t *ptr, int32_t *idx, int64_t cap);
     int count_pairs(int64_t *np, int64_t *nlj, int64_t *nlist);
 };
@@ -247,6 +264,7 @@ int Engine::init(const jp_sys_desc *descs, int ns) {
         P.cell0 = cell0; P.ncell = P.nc[0] * P.nc[1] * P.nc[2];
         P.lj0 = lj0; P.ntyp = nt;
         h.n15mx = (int)std::ceil(0.15 * (4.0 / 3.0 * 3.14159265358979 * rc * rc * rc)) * 2;
+        P.nblim = 2 * h.n15mx;
         // energy class of each cluster pair (n_matrix is column-major, 1-based)
         for (int a = 0; a < 2; ++a)
             for (int b = 0; b < 2; ++b) {
@@ -385,7 +403,7 @@ int Engine::init(const jp_sys_desc *descs, int ns) {
     AL(d_inum, natom); AL(d_perm, nslot); AL(d_cslot, ncell);
     AL(d_ys, nslot); AL(d_qd, nslot); AL(d_xr, nslot); AL(d_imgs, nslot); AL(d_lp, nslot);
     AL(d_cbb, 6 * (size_t)nclu); AL(d_ccen, nclu); AL(d_ccf, nclu); AL(d_lcnt, nclu); AL(d_cellext, ncell);
-    AL(d_sysext, ns); AL(d_crec, ncell); AL(d_nrem, ns); AL(d_nb, natom); AL(d_err, 1); AL(d_lmax, 1);
+    AL(d_sysext, ns); AL(d_crec, ncell); AL(d_err, 1); AL(d_lst, 2 + (size_t)ns);
     AL(d_gslot, 3 * (size_t)nslot);
     AL(d_ep64, 6 * (size_t)nclu); AL(d_epfix, 6 * (size_t)nclu);
     AL(d_s2sys, n2); AL(d_s3sys, n3); AL(d_s2a, n2); AL(d_s3a, n3); AL(d_s2d, 6 * (size_t)n2); AL(d_s3d, 6 * (size_t)n3);
@@ -447,72 +465,58 @@ int Engine::set_factors(int onehot) {
     return 0;
 }
 
-// celprp: bin, sort, build the j-lists (two passes: count, scan, fill).
+// celprp: bin and sort the atoms, then build the j-lists into segments of
+// ccap entries per i-cluster.  One status read-back per rebuild.
 int Engine::build_list() {
     CK(hipEventRecord(ev[2], st));
-    CK(hipMemsetAsync(d_cnt.p, 0, 4 * (size_t)(ncell + 1), st));
-    CK(hipMemsetAsync(d_err.p, 0, 4, st));
+    const int nstat = 2 + nsys;
+    const int nrs = std::max(std::max(ncell + 1, nslot), std::max(nsys, nstat));
+    hipLaunchKernelGGL(k_list_reset, grid1(nrs, 256), 256, 0, st, ncell, d_cnt.p, d_fill.p, d_cellext.p, nsys,
+                       d_sysext.p, nstat, d_lst.p, nslot, d_perm.p);
     hipLaunchKernelGGL(k_bin, grid1(natom, 256), 256, 0, st, natom, d_atom_sys.p, d_sp.p, d_x.p, d_img.p,
-                       d_cellid.p, d_subkey.p, d_refcell.p, d_cnt.p, d_err.p);
+                       d_cellid.p, d_subkey.p, d_refcell.p, d_cnt.p, d_lst.p);
     hipLaunchKernelGGL(k_scan, 1, 1024, 0, st, ncell, d_cnt.p, d_start.p);
-    CK(hipMemsetAsync(d_fill.p, 0, 4 * (size_t)ncell, st));
     hipLaunchKernelGGL(k_place, grid1(natom, 256), 256, 0, st, natom, d_cellid.p, d_start.p, d_fill.p, d_tmp.p);
-    CK(hipMemsetAsync(d_perm.p, 0xff, 4 * (size_t)nslot, st));
     hipLaunchKernelGGL(k_rank, grid1(natom, 256), 256, 0, st, natom, d_atom_sys.p, d_sp.p, d_cellid.p, d_subkey.p,
                        d_start.p, d_cnt.p, d_tmp.p, d_inum.p, d_perm.p);
     hipLaunchKernelGGL(k_cslot, grid1(ncell, 256), 256, 0, st, ncell, d_cell_sys.p, d_sp.p, d_start.p, d_cslot.p);
     hipLaunchKernelGGL(k_fill_sorted, grid1(nslot, 256), 256, 0, st, nslot, d_slot_sys.p, d_sp.p, d_perm.p, d_x.p,
                        d_img.p, d_chg.p, d_tflag.p, d_mol.p, d_ys.p, d_qd.p, d_xr.p, d_imgs.p, d_lp.p);
-    CK(hipMemsetAsync(d_cellext.p, 0, 4 * (size_t)ncell, st));
-    CK(hipMemsetAsync(d_sysext.p, 0, 4 * (size_t)nsys, st));
     hipLaunchKernelGGL(k_cbox, grid1(nclu, 256), 256, 0, st, nclu, d_clu_sys.p, d_sp.p, d_perm.p, d_cellid.p,
                        d_ccoord.p, d_ys.p, d_lp.p, d_cbb.p, d_ccen.p, d_ccf.p, d_cellext.p, d_sysext.p);
     hipLaunchKernelGGL(k_crec, grid1(ncell, 256), 256, 0, st, ncell, d_crank.p, d_cslot.p, d_cnt.p, d_cellext.p,
                        d_crec.p);
-    // segments of ccap entries per i-cluster; on overflow grow ccap and build again
-    ccap = ccap_want;
+    // Segments of ccap entries per i-cluster.  ccap only grows (lists above
+    // 7/8 of it report their length), so captured step graphs stay valid.
+    std::vector<int> lst(nstat);
+    int cell_err = 0;   // k_bin's flag survives a retry
     for (int attempt = 0;; ++attempt) {
+        ccap = std::max(ccap, ccap_want);
         AL(d_ent, (size_t)nclu * ccap);
-        CK(hipMemsetAsync(d_nrem.p, 0, 4 * (size_t)nsys, st));
-        CK(hipMemsetAsync(d_lmax.p, 0, 4, st));
         hipLaunchKernelGGL(k_list8, grid1(nclu, JP_LW), 64 * JP_LW, 0, st, nclu, d_clu_sys.p, d_sp.p, d_cbb.p, d_ccf.p,
-                           d_crec.p, d_sysext.p, d_lp.p, d_xr.p, d_imgs.p, d_xptr.p,
-                           d_xidx.p, ccap, d_ent.p, d_lcnt.p, d_nrem.p, d_nb.p, d_err.p, d_lmax.p);
-        int st_err = 0, lmax = 0;
-        CK(hipMemcpyAsync(&st_err, d_err.p, 4, hipMemcpyDeviceToHost, st));
-        CK(hipMemcpyAsync(&lmax, d_lmax.p, 4, hipMemcpyDeviceToHost, st));
+                           d_crec.p, d_sysext.p, d_lp.p, d_xr.p, d_imgs.p, d_xptr.p, d_xidx.p, ccap, ccap - ccap / 8,
+                           d_ent.p, d_lcnt.p, d_lst.p);
+        CK(hipGetLastError());
+        CK(hipMemcpyAsync(lst.data(), d_lst.p, 4 * (size_t)nstat, hipMemcpyDeviceToHost, st));
         CK(hipStreamSynchronize(st));
-        if (!(st_err & JP_ERR_LISTCAP)) {
-            // keep ~1/4 headroom for the next rebuilds
-            ccap_want = std::max(256, (lmax + lmax / 4 + 63) / 64 * 64);
-            break;
-        }
+        if (lst[1] > 0) ccap_want = std::max(ccap_want, (lst[1] + lst[1] / 4 + 255) / 256 * 256);
+        cell_err |= lst[0] & JP_ERR_CELL;
+        if (!(lst[0] & JP_ERR_LISTCAP)) break;
         if (attempt >= 6 || ccap >= (1 << 20)) return JP_IER_LIST;
-        ccap *= 2;
-        CK(hipMemsetAsync(d_err.p, 0, 4, st));
+        ccap_want = std::max(ccap_want, 2 * ccap);
+        CK(hipMemsetAsync(d_lst.p, 0, 4 * (size_t)nstat, st));
     }
-    CK(hipGetLastError());
     CK(hipEventRecord(ev[3], st));
-    std::vector<int> nrem(nsys);
-    int err = 0;
-    CK(hipMemcpyAsync(nrem.data(), d_nrem.p, 4 * (size_t)nsys, hipMemcpyDeviceToHost, st));
-    CK(hipMemcpyAsync(&err, d_err.p, 4, hipMemcpyDeviceToHost, st));
-    std::vector<int> nbc(natom);
-    CK(hipMemcpyAsync(nbc.data(), d_nb.p, 4 * (size_t)natom, hipMemcpyDeviceToHost, st));
-    CK(hipStreamSynchronize(st));
+    CK(hipEventSynchronize(ev[3]));
     float ms = 0;
     hipEventElapsedTime(&ms, ev[2], ev[3]);
     t_list += ms * 1e-3;
     n_list++;
     have_list = true;
-    if (err & 1) return JP_IER_CELL;
-    if (err & 2) return JP_EINVAL;
-    if (err & JP_ERR_LISTCAP) return JP_IER_LIST;
-    for (int s = 0; s < nsys; ++s) {
-        if (nrem[s] != 2 * hs[s].nexcl) return JP_IER_EXCL;      // cutoff.f90:90-96
-        for (int i = 0; i < hs[s].n; ++i)
-            if (nbc[sp[s].atom0 + i] > 2 * hs[s].n15mx) return JP_IER_LIST;  // cutoff.f90:30-46
-    }
+    if ((lst[0] | cell_err) & JP_ERR_CELL) return JP_IER_CELL;
+    if (lst[0] & JP_ERR_NBLIM) return JP_IER_LIST;            // cutoff.f90:30-46
+    for (int s = 0; s < nsys; ++s)
+        if (lst[2 + s] != 2 * hs[s].nexcl) return JP_IER_EXCL;  // cutoff.f90:90-96
     return 0;
 }
 
@@ -529,17 +533,23 @@ int Engine::refresh_sorted() {
 template <bool E, bool M, bool S>
 static void launch_pair(int prec, unsigned nb, size_t lds32, size_t lds64, hipStream_t st, Engine &g);
 
-int Engine::pair(int prec, bool energy) {
-    const unsigned nb = (unsigned)((nclu + 7) / 8 * 8);
-    int maxt = 0;
+// Pair-kernel variant: bit 1 class factors differ from 1 (LMD passes), bit 0
+// some box edge < 2*rlim (per-pair minimum image).
+int Engine::pair_variant() const {
     bool lmd = false, small = false;
     for (int s = 0; s < nsys; ++s) {
-        maxt = std::max(maxt, hs[s].ntyp);
         small = small || sp[s].small;
         for (int q = 0; q < 4; ++q) lmd = lmd || sp[s].fac[q] != 1.0 || sp[s].cls[q] != sp[s].cls[0];
     }
+    return (lmd ? 2 : 0) | (small ? 1 : 0);
+}
+
+int Engine::pair(int prec, bool energy) {
+    const unsigned nb = (unsigned)((nclu + 7) / 8 * 8);
+    int maxt = 0;
+    for (int s = 0; s < nsys; ++s) maxt = std::max(maxt, hs[s].ntyp);
     const size_t l32 = sizeof(float2) * maxt * maxt, l64 = sizeof(double2) * maxt * maxt;
-    const int key = (energy ? 4 : 0) | (lmd ? 2 : 0) | (small ? 1 : 0);
+    const int key = (energy ? 4 : 0) | pair_variant();
     switch (key) {
         case 0: launch_pair<false, false, false>(prec, nb, l32, l64, st, *this); break;
         case 1: launch_pair<false, false, true>(prec, nb, l32, l64, st, *this); break;
@@ -651,6 +661,99 @@ int Engine::shake(int &ier_out) {
     return 0;
 }
 
+int Engine::integrate(int prec, bool mon) {
+    (void)prec;
+    hipLaunchKernelGGL(k_kick, grid1(natom, 256), 256, 0, st, natom, d_atom_sys.p, d_sp.p, d_inum.p, d_gslot.p,
+                       d_mass.p, d_x.p, d_v.p, d_xold.p, mon ? d_vback.p : (double *)nullptr, (double *)nullptr, 1);
+    int ier = 0, r;
+    if ((r = shake(ier))) return r;
+    hipLaunchKernelGGL(k_post, grid1(natom, 256), 256, 0, st, natom, d_atom_sys.p, d_sp.p, d_inum.p, d_img.p, d_x.p,
+                       d_xold.p, d_v.p, d_ys.p, (n2 + n3) > 0 ? 1 : 0);
+    CK(hipGetLastError());
+    return 0;
+}
+
+void Engine::drop_graphs() {
+    for (auto &g : graphs)
+        if (g.exec) hipGraphExecDestroy(g.exec);
+    graphs.clear();
+}
+
+// Graph of n plain steps for the current list segment size and buffers.
+Engine::StepGraph *Engine::step_graph(int n, int prec) {
+    const int variant = pair_variant();
+    for (auto &g : graphs)
+        if (g.exec && g.nsteps == n && g.prec == prec && g.ccap == ccap && g.ent == d_ent.p && g.variant == variant)
+            return &g;
+    // the list buffers changed: stale graphs go
+    for (auto it = graphs.begin(); it != graphs.end();) {
+        if (it->ccap != ccap || it->ent != d_ent.p) {
+            if (it->exec) hipGraphExecDestroy(it->exec);
+            it = graphs.erase(it);
+        } else {
+            ++it;
+        }
+    }
+    StepGraph g;
+    g.nsteps = n; g.prec = prec; g.ccap = ccap; g.ent = d_ent.p; g.variant = variant;
+    hipGraph_t graph = nullptr;
+    if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) return nullptr;
+    int r = 0;
+    for (int k = 0; k < n && !r; ++k) {
+        r = pair(prec, false);
+        if (!r) r = integrate(prec, false);
+    }
+    const hipError_t ec = hipStreamEndCapture(st, &graph);
+    if (r || ec != hipSuccess || !graph) {
+        if (graph) hipGraphDestroy(graph);
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    const hipError_t ei = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+    hipGraphDestroy(graph);
+    if (ei != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    graphs.push_back(g);
+    return &graphs.back();
+}
+
+int Engine::mon_record(int step, int nsteps, int nmon, int nlog, const std::vector<double> &eel,
+                       const std::vector<double> &evd, jp_energy_log *log) {
+    hipLaunchKernelGGL(k_ekin, grid1(natom, 256), 256, 0, st, natom, d_mass.p, d_v.p, d_vback.p, d_ekin.p);
+    std::vector<double> ek(natom);
+    int err = 0;
+    CK(hipMemcpyAsync(ek.data(), d_ekin.p, 8 * (size_t)natom, hipMemcpyDeviceToHost, st));
+    CK(hipMemcpyAsync(&err, d_err.p, 4, hipMemcpyDeviceToHost, st));
+    CK(hipStreamSynchronize(st));
+    if (err & 8) return JP_IER_SHAKE_SING;
+    if (err & 4) return JP_IER_SHAKE_CONV;
+    for (int s = 0; s < nsys; ++s) {
+        double e = 0.0;
+        for (int i = 0; i < hs[s].n; ++i) e = e + ek[sp[s].atom0 + i];
+        e = e * 1.0e7 / 4.184 * 0.0005;   // caltmp.f90:24-27
+        jp_energy_log &L = log[(size_t)s * (nsteps / nmon) + nlog];
+        L.step = step;
+        L.ekin = e;
+        double ep = 0.0;
+        for (int k = 0; k < 3; ++k) {
+            const double f = (hs[s].lmd && hs[s].nfrg == 3) ? (k == 0 ? hs[s].lambda * hs[s].lambda : (k == 1 ? hs[s].lambda : 1.0)) : 1.0;
+            L.eelec[k] = k < hs[s].nfrg ? eel[3 * s + k] : 0.0;
+            L.evdw[k] = k < hs[s].nfrg ? evd[3 * s + k] : 0.0;
+            if (k < hs[s].nfrg) ep += f * (eel[3 * s + k] + evd[3 * s + k]);
+        }
+        L.epot = ep;
+        L.etot = ep + e;
+    }
+    return 0;
+}
+
+// METHOD= MICR loop (md.f90:519-694): per step the forces (pair kernel with
+// the excess term), kick and drift, SHAKE, v = (x - x_old)/dt; every
+// `update` steps a rebuild after the step (md.f90:655-660); energies every
+// `nmon` steps.  Runs of plain steps between those points go through cached
+// HIP graphs.
 int Engine::run_nve(int nsteps, int update, int nmon, int prec, jp_energy_log *log) {
     int r;
     if (nsteps < 0 || update <= 0) return JP_EINVAL;
@@ -658,16 +761,35 @@ int Engine::run_nve(int nsteps, int update, int nmon, int prec, jp_energy_log *l
     if ((r = set_factors(-1))) return r;
     if ((r = refresh_sorted())) return r;
     CK(hipMemsetAsync(d_err.p, 0, 4, st));
-    const bool shake_on = (n2 + n3) > 0;
-    t_pair = 0; t_step = 0; n_pair = 0;
-    double tl0 = t_list;
-    int nl0 = n_list;
+    t_pair = 0; t_step = 0; n_pair = 0; n_pair_steps = nsteps;
+    const double tl0 = t_list;
     std::vector<hipEvent_t> pe;
-    pe.reserve(2 * (size_t)nsteps);
     CK(hipEventRecord(ev[0], st));
     int nlog = 0;
-    for (int step = 1; step <= nsteps; ++step) {
+    int step = 1;
+    while (step <= nsteps) {
         const bool mon = nmon > 0 && step % nmon == 0;
+        // plain steps up to the next rebuild, energy step or the end
+        int nplain = 0;
+        if (!mon && use_graphs) {
+            const int to_rebuild = (int)(update - idloop % update);
+            int lim = std::min(nsteps - step + 1, to_rebuild);
+            if (nmon > 0) lim = std::min(lim, nmon - step % nmon);   // stop before the energy step
+            nplain = lim;
+        }
+        if (nplain >= 2) {
+            StepGraph *g = step_graph(nplain, prec);
+            if (g) {
+                CK(hipGraphLaunch(g->exec, st));
+                step += nplain;
+                idloop += nplain;
+                if (idloop % update == 0) {
+                    if ((r = build_list())) return r;
+                }
+                continue;
+            }
+            use_graphs = 0;   // capture unavailable: plain launches from here on
+        }
         hipEvent_t a, b;
         CK(hipEventCreate(&a));
         CK(hipEventCreate(&b));
@@ -676,45 +798,16 @@ int Engine::run_nve(int nsteps, int update, int nmon, int prec, jp_energy_log *l
         CK(hipEventRecord(b, st));
         pe.push_back(a);
         pe.push_back(b);
+        ++n_pair;
         std::vector<double> eel, evd, eex;
         if (mon && (r = energies(prec, eel, evd, eex))) return r;
-        hipLaunchKernelGGL(k_kick, grid1(natom, 256), 256, 0, st, natom, d_atom_sys.p, d_sp.p, d_inum.p, d_gslot.p,
-                           d_mass.p, d_x.p, d_v.p, d_xold.p, mon ? d_vback.p : (double *)nullptr, (double *)nullptr, 1);
-        int ier = 0;
-        if ((r = shake(ier))) return r;
-        hipLaunchKernelGGL(k_post, grid1(natom, 256), 256, 0, st, natom, d_atom_sys.p, d_sp.p, d_inum.p, d_img.p, d_x.p,
-                           d_xold.p, d_v.p, d_ys.p, shake_on ? 1 : 0);
-        CK(hipGetLastError());
+        if ((r = integrate(prec, mon))) return r;
         if (mon) {
-            hipLaunchKernelGGL(k_ekin, grid1(natom, 256), 256, 0, st, natom, d_mass.p, d_v.p, d_vback.p, d_ekin.p);
-            std::vector<double> ek(natom);
-            int err = 0;
-            CK(hipMemcpyAsync(ek.data(), d_ekin.p, 8 * (size_t)natom, hipMemcpyDeviceToHost, st));
-            CK(hipMemcpyAsync(&err, d_err.p, 4, hipMemcpyDeviceToHost, st));
-            CK(hipStreamSynchronize(st));
-            if (err & 8) return JP_IER_SHAKE_SING;
-            if (err & 4) return JP_IER_SHAKE_CONV;
-            for (int s = 0; s < nsys; ++s) {
-                double e = 0.0;
-                for (int i = 0; i < hs[s].n; ++i) e = e + ek[sp[s].atom0 + i];
-                e = e * 1.0e7 / 4.184 * 0.0005;   // caltmp.f90:24-27
-                jp_energy_log &L = log[(size_t)s * (nsteps / nmon) + nlog];
-                L.step = step;
-                L.ekin = e;
-                double ep = 0.0;
-                double f[3];
-                for (int k = 0; k < 3; ++k) {
-                    f[k] = (hs[s].lmd && hs[s].nfrg == 3) ? (k == 0 ? hs[s].lambda * hs[s].lambda : (k == 1 ? hs[s].lambda : 1.0)) : 1.0;
-                    L.eelec[k] = k < hs[s].nfrg ? eel[3 * s + k] : 0.0;
-                    L.evdw[k] = k < hs[s].nfrg ? evd[3 * s + k] : 0.0;
-                    if (k < hs[s].nfrg) ep += f[k] * (eel[3 * s + k] + evd[3 * s + k]);
-                }
-                L.epot = ep;
-                L.etot = ep + e;
-            }
+            if ((r = mon_record(step, nsteps, nmon, nlog, eel, evd, log))) return r;
             nlog++;
         }
         ++idloop;
+        ++step;
         if (idloop % update == 0) {     // md.f90:655-660, after the step
             if ((r = build_list())) return r;
         }
@@ -733,11 +826,9 @@ int Engine::run_nve(int nsteps, int update, int nmon, int prec, jp_energy_log *l
         hipEventDestroy(pe[q]);
         hipEventDestroy(pe[q + 1]);
     }
-    n_pair = nsteps;
-    double tl = t_list - tl0;
-    t_step = t_total - t_pair - tl;
+    const double tl = t_list - tl0;
+    t_step = t_total - (n_pair ? t_pair / n_pair * nsteps : 0.0) - tl;
     t_list = tl;
-    (void)nl0;
     if (err & 8) return JP_IER_SHAKE_SING;
     if (err & 4) return JP_IER_SHAKE_CONV;
     return 0;
@@ -896,7 +987,8 @@ int jp_run_nve(jp_ctx *ctx, int32_t nsteps, int32_t update, int32_t nmon, int32_
 int jp_last_timing(jp_ctx *ctx, double *t_total, double *t_pair, double *t_list, double *t_step) {
     if (!ctx) return JP_EINVAL;
     if (t_total) *t_total = ctx->e.t_total;
-    if (t_pair) *t_pair = ctx->e.t_pair;
+    // mean pair-kernel time over the host-launched steps, times all steps
+    if (t_pair) *t_pair = ctx->e.n_pair ? ctx->e.t_pair / ctx->e.n_pair * ctx->e.n_pair_steps : 0.0;
     if (t_list) *t_list = ctx->e.t_list;
     if (t_step) *t_step = ctx->e.t_step;
     return 0;
@@ -954,6 +1046,25 @@ int jp_batch_last_timing(jp_batch *b, double *t_total) {
     if (!b) return JP_EINVAL;
     if (t_total) *t_total = b->e.t_total;
     return 0;
+}
+
+static int set_option(Engine &E, int32_t option, int32_t value) {
+    if (option == JP_OPT_GRAPHS) {
+        E.use_graphs = value ? 1 : 0;
+        if (!value) E.drop_graphs();
+        return 0;
+    }
+    return JP_EINVAL;
+}
+
+int jp_set_option(jp_ctx *ctx, int32_t option, int32_t value) {
+    if (!ctx) return JP_EINVAL;
+    return set_option(ctx->e, option, value);
+}
+
+int jp_batch_set_option(jp_batch *b, int32_t option, int32_t value) {
+    if (!b) return JP_EINVAL;
+    return set_option(b->e, option, value);
 }
 
 int jp_batch_destroy(jp_batch *b) {

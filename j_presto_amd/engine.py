@@ -127,6 +127,11 @@ class MdSystem:
         ier = native.check("jp_run_nve", self.lib.jp_run_nve(self.ctx, nsteps, update, nmon, prec, log))
         return log_to_array(log, nl), ier
 
+    def set_graphs(self, on: bool):
+        """Plain steps of runmd as captured HIP graphs (default) or launched
+        one kernel at a time; the results are the same."""
+        native.check("jp_set_option", self.lib.jp_set_option(self.ctx, native.JP_OPT_GRAPHS, int(bool(on))))
+
     def timing(self):
         t = [C.c_double() for _ in range(4)]
         self.lib.jp_last_timing(self.ctx, *[C.byref(x) for x in t])
@@ -202,6 +207,9 @@ class Ensemble:
         ier = native.check("jp_batch_run_nve", self.lib.jp_batch_run_nve(self.b, nsteps, update, nmon, prec, log))
         full = log_to_array(log, nl * self.nsys)
         return full.reshape(self.nsys, nl) if nl else full, ier
+
+    def set_graphs(self, on: bool):
+        native.check("jp_batch_set_option", self.lib.jp_batch_set_option(self.b, native.JP_OPT_GRAPHS, int(bool(on))))
 
     def get_state(self, i: int):
         n = self.holders[i].n

@@ -38,7 +38,10 @@ SIGNATURES = {
     "jp_batch_get_state": (C.c_int, [vp, C.c_int32, f64p, f64p]),
     "jp_batch_last_timing": (C.c_int, [vp, f64p]),
     "jp_batch_destroy": (C.c_int, [vp]),
+    "jp_set_option": (C.c_int, [vp, C.c_int32, C.c_int32]),
+    "jp_batch_set_option": (C.c_int, [vp, C.c_int32, C.c_int32]),
 }
+JP_OPT_GRAPHS = 1
 
 _lib = None
 

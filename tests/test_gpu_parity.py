@@ -163,3 +163,45 @@ def test_count_pairs_matches_bruteforce():
     ok = r2[iu] <= 144.0
     n = int(ok.sum()) - sum(1 for (i, j) in excl if r2[i, j] <= 144.0)
     assert md.count_pairs() == n
+
+
+@pytest.mark.parametrize("prec", [engine.FP64, engine.FP32])
+def test_graph_steps_equal_eager(prec):
+    """Plain steps replayed from captured HIP graphs (several replays, with
+    rebuilds and energy steps in between) give the same bits as kernels
+    launched one by one."""
+    s = systems.water_box(700, 28.0, 23, n_ion_pairs=2)
+    print(f"case graphs prec={prec}", flush=True)
+    out = []
+    for graphs in (False, True):
+        md = engine.MdSystem(s, precision=prec)
+        md.set_graphs(graphs)
+        logs = []
+        for nst in (25, 13, 40):
+            log, ier = md.runmd(nst, update=10, nmon=5)
+            assert ier == 0
+            logs.append(log["etot"].copy())
+        out.append((md.get_state(), logs))
+        md.close()
+    (x0, v0), l0 = out[0]
+    (x1, v1), l1 = out[1]
+    assert np.array_equal(x0, x1) and np.array_equal(v0, v1)
+    for a, b in zip(l0, l1):
+        assert np.array_equal(a, b)
+
+
+def test_batch_graphs_equal_single_eager():
+    syss = [systems.water_box(200, 24.2, 21), systems.water_box(700, 28.0, 23, n_ion_pairs=2)]
+    ens = engine.Ensemble(syss, precision=engine.FP32)
+    _, ier = ens.runmd(30, update=10, nmon=0)
+    assert ier == 0
+    for i, s in enumerate(syss):
+        md = engine.MdSystem(s, precision=engine.FP32)
+        md.set_graphs(False)
+        _, ier = md.runmd(30, update=10, nmon=0)
+        assert ier == 0
+        x1, v1 = md.get_state()
+        xb, vb = ens.get_state(i)
+        assert np.array_equal(x1, xb) and np.array_equal(v1, vb), i
+        md.close()
+    ens.close()
