@@ -13,6 +13,9 @@ state.  Beside it: the plain FP64 GPU path (same protocol, fewer steps) and
 the reference md_run on the host cores (bounded sample).
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+       python bench.py --workload {0,1,4}   other single-system workloads (same protocol)
+       python bench.py --ensemble           64 uneven systems: one batch vs 64 sequential runs
+       python bench.py --drift N            N-step NVE energy drift, FP32/fixed point vs FP64
 """
 from __future__ import annotations
 
@@ -133,8 +136,82 @@ def calibrate_copy(nbytes: int = 1 << 30, reps: int = 5):
     print(json.dumps({"copy_bytes": nbytes, "reps": reps}), flush=True)
 
 
+def drift_fit(log, natom):
+    """Least-squares slope of E_tot(t) in kcal/mol/ns, per 1,000 atoms too,
+    and the span and RMS fluctuation about the line."""
+    import numpy as np
+    t_ns = log["step"] * DT_FS * 1e-6
+    e = log["etot"]
+    a, b = np.polyfit(t_ns, e, 1)
+    res = e - (a * t_ns + b)
+    return {"slope_kcal_mol_ns": float(a), "slope_per_1000_atoms": float(a * 1000.0 / natom),
+            "span_kcal_mol": float(e.max() - e.min()), "rms_fluct_kcal_mol": float(res.std()),
+            "etot_first": float(e[0]), "etot_last": float(e[-1]), "samples": int(len(e))}
+
+
+def run_drift(nsteps: int, equil: int):
+    from j_presto_amd import engine, systems
+    s = systems.workload(2)
+    state = equilibrated(s, equil)
+    out = {"mode": "nve-drift", "workload": systems.WORKLOADS[2], "natom": s.n, "steps": nsteps,
+           "dt_fs": DT_FS, "energy_every": NMON, "list_update": UPDATE, "data": "synthetic"}
+    for name, prec in (("fp32_fixed", engine.FP32), ("fp64", engine.FP64)):
+        md = engine.MdSystem(s, dt=DT_FS, precision=prec)
+        md.set_state(*state)
+        assert md.celprp() == 0
+        t0 = time.perf_counter()
+        log, ier = md.runmd(nsteps, update=UPDATE, nmon=NMON)
+        wall = time.perf_counter() - t0
+        assert ier == 0, ier
+        md.close()
+        out[name] = drift_fit(log, s.n)
+        out[name]["ns_per_day"] = round(ns_per_day(nsteps / wall), 3)
+        print(f"# {name}: {out[name]}", file=sys.stderr, flush=True)
+    print(json.dumps(out), flush=True)
+
+
+def run_ensemble(steps: int, warmup: int):
+    """64 uneven chain-in-water systems (3k-30k atoms, each with its own
+    lambda): one batch sharing every launch against 64 separate runs."""
+    from j_presto_amd import engine, systems
+    syss = systems.ensemble(64)
+    natom = sum(s.n for s in syss)
+    ens = engine.Ensemble(syss, dt=DT_FS, precision=engine.FP32)
+    _, ier = ens.runmd(warmup, update=UPDATE)
+    assert ier == 0, ier
+    t0 = time.perf_counter()
+    _, ier = ens.runmd(steps, update=UPDATE)
+    wb = time.perf_counter() - t0
+    assert ier == 0, ier
+    ens.close()
+    ws = 0.0
+    for s in syss:
+        md = engine.MdSystem(s, dt=DT_FS, precision=engine.FP32)
+        _, ier = md.runmd(warmup, update=UPDATE)
+        assert ier == 0, ier
+        t0 = time.perf_counter()
+        _, ier = md.runmd(steps, update=UPDATE)
+        ws += time.perf_counter() - t0
+        assert ier == 0, ier
+        md.close()
+    agg_b = ns_per_day(len(syss) * steps / wb)
+    agg_s = ns_per_day(len(syss) * steps / ws)
+    print(json.dumps({
+        "metric": "aggregate ns/day, 64-system ensemble", "value": round(agg_b, 3), "unit": "ns/day",
+        "n_gpus": 1, "steps": steps, "warmup": warmup, "higher_is_better": True, "dtype": "fp32",
+        "data": "synthetic", "config": {"workload": systems.WORKLOADS[3], "nsys": len(syss), "natom_total": natom,
+                                        "sizes_min_max": [min(s.n for s in syss), max(s.n for s in syss)],
+                                        "precision": "FP32 pair math, int64 fixed-point accumulation"},
+        "batched_wall_s": round(wb, 4), "sequential_wall_s": round(ws, 4),
+        "sequential_ns_per_day": round(agg_s, 3), "batch_speedup": round(ws / wb, 3),
+        "note": "nonbonded + SHAKE only: the chains' bonded terms are not part of this path"}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", type=int, default=2, choices=[0, 1, 2, 4])
+    ap.add_argument("--ensemble", action="store_true")
+    ap.add_argument("--drift", type=int, default=0, help="run an N-step NVE drift comparison instead")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run 1 GiB device copies (profiler traffic calibration)")
     ap.add_argument("--gpus", type=int, default=1)
@@ -152,10 +229,18 @@ def main():
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus})",
               file=sys.stderr)
         sys.exit(2)
+    if args.drift:
+        run_drift(args.drift, args.equil)
+        return
+    if args.ensemble:
+        run_ensemble(args.steps, args.warmup)
+        return
     rank = int(os.environ.get("RANK", "0"))
     from j_presto_amd import engine, systems
-    s = systems.workload(2)
-    state = equilibrated(s, args.equil)
+    s = systems.workload(args.workload)
+    # the chain workload has no bonded terms on this path: time it from the
+    # generated state (its separated chain would end in md_run's ier 99)
+    state = equilibrated(s, args.equil) if args.workload != 1 else None
     r32 = run_gpu(s, engine.FP32, args.steps, args.warmup, state)
     n64 = args.fp64_steps or max(args.steps // 4, 100)
     r64 = run_gpu(s, engine.FP64, n64, min(args.warmup, 50), state)
@@ -173,7 +258,8 @@ def main():
     list_bytes = 64 * s.n + 4 * r32["listed"]
     et = r32["log"]["etot"]
     out = {
-        "metric": "ns/day + pair-interactions/s, 100k-atom ZD box; FP32/fixed-pt vs FP64 GPU",
+        "metric": ("ns/day + pair-interactions/s, 100k-atom ZD box; FP32/fixed-pt vs FP64 GPU" if args.workload == 2
+                   else f"ns/day + pair-interactions/s, {systems.WORKLOADS[args.workload]}"),
         "value": round(value, 3),
         "unit": "ns/day",
         "n_gpus": 1,
@@ -185,7 +271,7 @@ def main():
         "vs_baseline": None,
         "dtype": "fp32",
         "data": "synthetic",
-        "config": {"workload": systems.WORKLOADS[2], "natom": s.n, "cutoff_A": 12.0, "dt_fs": DT_FS,
+        "config": {"workload": systems.WORKLOADS[args.workload], "natom": s.n, "cutoff_A": 12.0, "dt_fs": DT_FS,
                    "list_update": UPDATE, "energy_every": NMON, "precision": "FP32 pair math, int64 fixed-point "
                    "accumulation, FP64 SHAKE/integration"},
         "pair_interactions_per_s": round(r32["pairs"] * sps, 1),
@@ -207,7 +293,7 @@ def main():
                               "rest": round(1e3 * (r32["wall"] - r32["timing"]["list"]) / args.steps - 1e3 * t_pair, 4)},
         "cpu_baseline": None,
     }
-    if rank == 0 and not args.no_cpu:
+    if rank == 0 and not args.no_cpu and args.workload == 2:
         cb = cpu_reference(s, args.cpu_steps)
         out["cpu_baseline"] = cb
     print(json.dumps(out), flush=True)

@@ -75,6 +75,20 @@ typedef struct jp_sys_desc {
     double dt;                 /* TIMEST (fs) */
     int32_t lmd;               /* 1: CLSTMETHOD=LMD, grad = l^2 g1 + l g2 + g3 (verlet.f90:67-91) */
     double lambda;             /* LAMBDA */
+    /* Bonded terms of dnergv_PB (dnergv.f90:62-193): the bonds and angles left
+     * after setshk removed the SHAKE-constrained ones (setshk.f90 reabnd /
+     * reaang), i.e. md_run's iypbnd(:,1:nbnd) and iypang(:,1:nang) in its own
+     * order.  nbond = nangle = 0 (water) means no bonded terms. */
+    int32_t nbond;             /* nbnd = iynbnd - iyndbn */
+    const int32_t *bond_atoms; /* [nbond][2] iypbnd, 1-based */
+    const double *bond_fc;     /* fyfbnd: E = fc (r - r0)^2, kcal/mol/A^2 */
+    const double *bond_r0;     /* fyqbnd, A */
+    const int32_t *bond_cls;   /* bndcls, 1-based energy class; NULL: max(n_matrix) (SCALED_TERM default) */
+    int32_t nangle;            /* nang = iynang - iyndag */
+    const int32_t *angle_atoms;/* [nangle][3] iypang, 1-based */
+    const double *angle_fc;    /* fyfang: E = fc (theta - theta0)^2, kcal/mol/rad^2 */
+    const double *angle_theta0;/* fyqang, radians */
+    const int32_t *angle_cls;  /* angcls, 1-based; NULL: max(n_matrix) */
 } jp_sys_desc;
 
 /* Per-monitoring-step record of jp_run_nve (monmd.f90 / md.f90:618-628). */
@@ -85,6 +99,8 @@ typedef struct jp_energy_log {
     double ekin;               /* K from (v(t-dt/2)+v(t+dt/2))/2 (caltmp.f90) */
     double eelec[3];           /* ELE15 per energy class (enevcc(9,:)) */
     double evdw[3];            /* VDW15 per energy class (enevcc(8,:)) */
+    double ebond[3];           /* BOND per energy class (enevcc(2,:)) */
+    double eangle[3];          /* ANGLE per energy class (enevcc(3,:)) */
 } jp_energy_log;
 
 typedef struct jp_ctx jp_ctx;
@@ -118,12 +134,17 @@ int jp_get_list(jp_ctx *ctx, int32_t *cell_of_atom, int64_t *npairs, int32_t *pt
 int jp_nonbond(jp_ctx *ctx, const double *cord, double *grad, double *eelec, double *evdw,
                double *ehyd, int precision);
 
+/* dnebnd_PB + dneang_PB: harmonic bond and angle terms.  Accumulates into
+ * grad(3,N,nfrg) and writes the per-class energies (dnergv.f90:62-193).
+ * cord NULL: use device coordinates; grad NULL: energies only. */
+int jp_bonded(jp_ctx *ctx, const double *cord, double *grad, double *ebond, double *eangle);
+
 /* shake_PB: constrain cord (x(t+dt)) against cord_old (x(t)) in place.
  * (shake.f90:2-755) */
 int jp_shake(jp_ctx *ctx, double *cord, const double *cord_old);
 
 /* runmd/verlet for METHOD=MICR: nsteps leap-frog steps on the device state
- * (forces, kick, drift, SHAKE, v=(x-x_old)/dt), list rebuilt every `update`
+ * (pair forces, bonded terms, kick, drift, SHAKE, v=(x-x_old)/dt), list rebuilt every `update`
  * steps, energies recorded every `nmon` steps into log[nsteps/nmon].
  * (md.f90:519-694, verlet.f90:2-226) */
 int jp_run_nve(jp_ctx *ctx, int32_t nsteps, int32_t update, int32_t nmon, int32_t precision,

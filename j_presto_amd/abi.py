@@ -26,12 +26,16 @@ class SysDesc(C.Structure):
         ("shake_tol", C.c_double), ("shake_maxit", C.c_int32), ("cell_level", C.c_int32),
         ("celwal", C.c_double * 6), ("cutoff", C.c_double), ("diel", C.c_double), ("dt", C.c_double),
         ("lmd", C.c_int32), ("lambda_", C.c_double),
+        ("nbond", C.c_int32), ("bond_atoms", i32p), ("bond_fc", f64p), ("bond_r0", f64p), ("bond_cls", i32p),
+        ("nangle", C.c_int32), ("angle_atoms", i32p), ("angle_fc", f64p), ("angle_theta0", f64p),
+        ("angle_cls", i32p),
     ]
 
 
 class EnergyLog(C.Structure):
     _fields_ = [("step", C.c_int32), ("etot", C.c_double), ("epot", C.c_double), ("ekin", C.c_double),
-                ("eelec", C.c_double * 3), ("evdw", C.c_double * 3)]
+                ("eelec", C.c_double * 3), ("evdw", C.c_double * 3), ("ebond", C.c_double * 3),
+                ("eangle", C.c_double * 3)]
 
 
 def n_matrix(nfrag: int, method: str) -> np.ndarray:
@@ -52,6 +56,57 @@ def n_matrix(nfrag: int, method: str) -> np.ndarray:
     elif nfrag > 2:
         raise ValueError("nfrag > 2 not supported")
     return np.asfortranarray(m)
+
+
+# pairs of a SHAKE group's constraints, in the reference's order (setshk.f90 parlis)
+_PARLIS = [(0, 1), (1, 2), (2, 0), (3, 0), (2, 3), (1, 3)]
+
+
+def shake_removed_terms(bonds: np.ndarray, angles: np.ndarray, groups) -> tuple:
+    """setshk.f90 reabnd / reaang: flags of the bonds and angles md_run drops
+    because a SHAKE constraint holds their (1-2 or 1-3) distance.  The search
+    follows the reference's loops: per group size, a forward scan from the
+    position after the last match, abandoned once the term's atoms pass the
+    constraint's.  `bonds` (nb,2) and `angles` (na,3) are 0-based in TPL
+    order; `groups` is shake_groups() (sizes ascending).  Returns boolean
+    arrays, True = removed."""
+    off_b = np.zeros(len(bonds), bool)
+    off_a = np.zeros(len(angles), bool)
+
+    def passed(i1, i2, j1, j2):
+        return (i1 < j1 and i2 < j2) or (i1 < j2 and i2 < j1)
+
+    for size in (2, 3, 4):
+        grp = [a for a, _ in groups if len(a) == size]
+        nst = 0
+        for atoms in grp:
+            for p, q in _PARLIS[:size * (size - 1) // 2]:
+                i1, i2 = atoms[p], atoms[q]
+                for ib in range(nst, len(bonds)):
+                    j1, j2 = int(bonds[ib, 0]), int(bonds[ib, 1])
+                    if (i1 == j1 and i2 == j2) or (i1 == j2 and i2 == j1):
+                        off_b[ib] = True
+                        # 1-based: nst = ibnd + 1 (2-, 4-atom), max(ibnd - 1, 1) (3-atom)
+                        nst = ib + 1 if size != 3 else max(ib, 1) - 1
+                        break
+                    if passed(i1, i2, j1, j2):
+                        break
+    for size in (3, 4):
+        grp = [a for a, _ in groups if len(a) == size]
+        nst = 0
+        for atoms in grp:
+            for p, q in _PARLIS[:size * (size - 1) // 2]:
+                i1, i2 = atoms[p], atoms[q]
+                for ia in range(nst, len(angles)):
+                    j1, j2 = int(angles[ia, 0]), int(angles[ia, 2])
+                    if (i1 == j1 and i2 == j2) or (i1 == j2 and i2 == j1):
+                        off_a[ia] = True
+                        # 1-based: nst = max(iang - 1, 1) (3-atom), max(iang - 2, 1) (4-atom)
+                        nst = max(ia, 1) - 1 if size == 3 else max(ia - 1, 1) - 1
+                        break
+                    if passed(i1, i2, j1, j2):
+                        break
+    return off_b, off_a
 
 
 class DescHolder:
@@ -109,6 +164,34 @@ class DescHolder:
         d.dt = dt
         d.lmd = 1 if s.cluster_method.startswith("LMD") else 0
         d.lambda_ = s.lam
+        # bonded terms md_run keeps after setshk (reabnd / reaang), TPL order
+        groups_all = s.shake_groups()
+        bonds, angles = s.bonds(), s.angles()
+        bfc, br0 = s.bond_params()
+        afc, ath = s.angle_params()
+        off_b, off_a = shake_removed_terms(bonds, angles, groups_all)
+        keep_b = ~off_b
+        keep_a = ~off_a
+        self.bond_atoms = np.ascontiguousarray(bonds[keep_b] + 1, np.int32).reshape(-1, 2)
+        self.bond_fc = np.ascontiguousarray(bfc[keep_b])
+        self.bond_r0 = np.ascontiguousarray(br0[keep_b])
+        self.angle_atoms = np.ascontiguousarray(angles[keep_a] + 1, np.int32).reshape(-1, 3)
+        self.angle_fc = np.ascontiguousarray(afc[keep_a])
+        self.angle_theta0 = np.ascontiguousarray(ath[keep_a] * (np.pi / 180.0))   # inptpl.f90 rdangl: * rad
+        # a term with fc == 0 contributes nothing: leave such systems without bonded work
+        if not np.any(self.bond_fc):
+            self.bond_atoms, self.bond_fc, self.bond_r0 = self.bond_atoms[:0], self.bond_fc[:0], self.bond_r0[:0]
+        if not np.any(self.angle_fc):
+            self.angle_atoms, self.angle_fc = self.angle_atoms[:0], self.angle_fc[:0]
+            self.angle_theta0 = self.angle_theta0[:0]
+        d.nbond = len(self.bond_fc)
+        d.bond_atoms = self.bond_atoms.ctypes.data_as(i32p)
+        d.bond_fc = self.bond_fc.ctypes.data_as(f64p)
+        d.bond_r0 = self.bond_r0.ctypes.data_as(f64p)
+        d.nangle = len(self.angle_fc)
+        d.angle_atoms = self.angle_atoms.ctypes.data_as(i32p)
+        d.angle_fc = self.angle_fc.ctypes.data_as(f64p)
+        d.angle_theta0 = self.angle_theta0.ctypes.data_as(f64p)
         self.desc = d
         self.nfrg = nfrg
         self.n = n

@@ -16,6 +16,7 @@
 
 #include "../../include/jpgpu.h"
 #include "jp_pair.hpp"
+#include "jp_bonded.hpp"
 
 #define CK(x)                                                          \
     do {                                                               \
@@ -111,6 +112,13 @@ struct Engine {
     DBuf<int4> d_s2a, d_s3a;
     DBuf<double> d_s2d, d_s3d;
     DBuf<unsigned long long> d_u64;
+    // bonded terms (jp_bonded.hpp): bonds, angles, per-atom gather lists
+    int nbt = 0, nat = 0, nbat = 0;
+    std::vector<int> tsys_h, tcls_h;   // system and class of every term (bonds, then angles)
+    DBuf<int> d_bat, d_bptr, d_bref;
+    DBuf<int4> d_bnd, d_ang;
+    DBuf<double2> d_bndp, d_angp;
+    DBuf<double> d_eterm;
 
     ~Engine() {
         drop_graphs();
@@ -131,6 +139,8 @@ struct Engine {
         d_ent.release(); d_gslot.release(); d_ep64.release(); d_epfix.release();
         d_s2sys.release(); d_s3sys.release(); d_s2a.release(); d_s3a.release(); d_s2d.release(); d_s3d.release();
         d_u64.release();
+        d_bat.release(); d_bptr.release(); d_bref.release(); d_bnd.release(); d_ang.release();
+        d_bndp.release(); d_angp.release(); d_eterm.release();
     }
 
     int init(const jp_sys_desc *descs, int ns);
@@ -147,6 +157,15 @@ struct Engine {
     }
     int pair(int prec, bool energy);
     int pair_variant() const;
+    BondedArgs bonded_args() const {
+        BondedArgs b;
+        b.nbat = nbat; b.bat = d_bat.p; b.bptr = d_bptr.p; b.bref = d_bref.p;
+        b.bnd = d_bnd.p; b.ang = d_ang.p; b.bndp = d_bndp.p; b.angp = d_angp.p;
+        return b;
+    }
+    int forces(int prec, bool energy);   // pair kernel, then the bonded terms into the same slots
+    int bonded_energies(std::vector<double> &eb, std::vector<double> &ea);
+    int bonded(const double *cord, double *grad, double *ebond, double *eangle);
     int set_factors(int onehot_class);  // -1: run factors (lambda), k: one-hot class k
     int nonbond(const double *cord, double *grad, double *eelec, double *evdw, int prec);
     int energies(int prec, std::vector<double> &eel, std::vector<double> &evd, std::vector<double> &eex);
@@ -154,7 +173,8 @@ struct Engine {
     int run_nve(int nsteps, int update, int nmon, int prec, jp_energy_log *log);
     int integrate(int prec, bool mon);   // kick + SHAKE + v update of one step (after the forces)
     int mon_record(int step, int nsteps, int nmon, int nlog, const std::vector<double> &eel,
-                   const std::vector<double> &evd, jp_energy_log *log);
+                   const std::vector<double> &evd, const std::vector<double> &eb, const std::vector<double> &ea,
+                   jp_energy_log *log);
 
     // Steps between list rebuilds and energy steps run as captured HIP graphs:
     // nsteps x (pair, kick, SHAKE, post), cached per shape.  Pair-kernel
@@ -191,6 +211,10 @@ int Engine::init(const jp_sys_desc *descs, int ns) {
     std::vector<int4> s2a, s3a;
     std::vector<double> s2d, s3d;
     std::vector<int> crank, ccoord;
+    std::vector<int4> bnd, ang;
+    std::vector<double2> bndp, angp;
+    std::vector<std::vector<int>> bref_of;   // per global atom: term codes, angles after bonds
+    std::vector<int> asys, acls;             // angles' system / class until they follow the bonds
     int atom0 = 0, slot0 = 0, cell0 = 0, lj0 = 0;
     shake_tol = descs[0].shake_tol > 0 ? descs[0].shake_tol : 1e-6;
     shake_maxit = descs[0].shake_maxit > 0 ? descs[0].shake_maxit : 1000;
@@ -378,6 +402,41 @@ int Engine::init(const jp_sys_desc *descs, int ns) {
                 return JP_EINVAL;  // 4-atom groups (shake4) are not supported
             }
         }
+        // bonded terms (dnergv.f90:62-193); class default max(n_matrix) (incel.f90:952-969)
+        {
+            int cmax = 0;
+            for (int q = 0; q < d.nfrag * d.nfrag; ++q) cmax = std::max(cmax, d.n_matrix[q] - 1);
+            if (d.nbond < 0 || d.nangle < 0) return JP_EINVAL;
+            if ((d.nbond && (!d.bond_atoms || !d.bond_fc || !d.bond_r0)) ||
+                (d.nangle && (!d.angle_atoms || !d.angle_fc || !d.angle_theta0)))
+                return JP_EINVAL;
+            bref_of.resize((size_t)atom0 + n);
+            for (int b = 0; b < d.nbond; ++b) {
+                const int i = d.bond_atoms[2 * b] - 1, j = d.bond_atoms[2 * b + 1] - 1;
+                const int c = d.bond_cls ? d.bond_cls[b] - 1 : cmax;
+                if (i < 0 || j < 0 || i >= n || j >= n || i == j || c < 0 || c >= d.nfrg) return JP_EINVAL;
+                const int t = (int)bnd.size();
+                bnd.push_back(make_int4(i + atom0, j + atom0, c, 0));
+                bndp.push_back(make_double2(d.bond_fc[b], d.bond_r0[b]));
+                tsys_h.push_back(s); tcls_h.push_back(c);
+                bref_of[i + atom0].push_back(bref_code(t, false, 0));
+                bref_of[j + atom0].push_back(bref_code(t, false, 1));
+            }
+            for (int g = 0; g < d.nangle; ++g) {
+                const int i = d.angle_atoms[3 * g] - 1, j = d.angle_atoms[3 * g + 1] - 1, k = d.angle_atoms[3 * g + 2] - 1;
+                const int c = d.angle_cls ? d.angle_cls[g] - 1 : cmax;
+                if (i < 0 || j < 0 || k < 0 || i >= n || j >= n || k >= n || i == j || j == k || i == k || c < 0 ||
+                    c >= d.nfrg)
+                    return JP_EINVAL;
+                const int t = (int)ang.size();
+                ang.push_back(make_int4(i + atom0, j + atom0, k + atom0, c));
+                angp.push_back(make_double2(d.angle_fc[g], d.angle_theta0[g]));
+                asys.push_back(s); acls.push_back(c);
+                bref_of[i + atom0].push_back(bref_code(t, true, 0));
+                bref_of[j + atom0].push_back(bref_code(t, true, 1));
+                bref_of[k + atom0].push_back(bref_code(t, true, 2));
+            }
+        }
         for (int i = 0; i < npad; ++i) slot_sys_h.push_back(s);
         for (int i = 0; i < npad / JP_CL; ++i) clu_sys_h.push_back(s);
         atom0 += n;
@@ -386,6 +445,19 @@ int Engine::init(const jp_sys_desc *descs, int ns) {
         lj0 += nt * nt;
     }
     natom = atom0; nslot = slot0; nclu = slot0 / JP_CL; ncell = cell0;
+    // bonded gather lists: atoms with terms, their codes (bonds, then angles)
+    nbt = (int)bnd.size(); nat = (int)ang.size();
+    tsys_h.insert(tsys_h.end(), asys.begin(), asys.end());
+    tcls_h.insert(tcls_h.end(), acls.begin(), acls.end());
+    std::vector<int> bat, bptr{0}, bref;
+    bref_of.resize(natom);
+    for (int a = 0; a < natom; ++a) {
+        if (bref_of[a].empty()) continue;
+        bat.push_back(a);
+        bref.insert(bref.end(), bref_of[a].begin(), bref_of[a].end());
+        bptr.push_back((int)bref.size());
+    }
+    nbat = (int)bat.size();
     n2 = (int)s2a.size(); n3 = (int)s3a.size();
     mass_h = mass;
     set_factors(-1);
@@ -408,6 +480,9 @@ int Engine::init(const jp_sys_desc *descs, int ns) {
     AL(d_ep64, 6 * (size_t)nclu); AL(d_epfix, 6 * (size_t)nclu);
     AL(d_s2sys, n2); AL(d_s3sys, n3); AL(d_s2a, n2); AL(d_s3a, n3); AL(d_s2d, 6 * (size_t)n2); AL(d_s3d, 6 * (size_t)n3);
     AL(d_u64, 3);
+    AL(d_bat, bat.size()); AL(d_bptr, bptr.size()); AL(d_bref, bref.size());
+    AL(d_bnd, bnd.size()); AL(d_bndp, bndp.size()); AL(d_ang, ang.size()); AL(d_angp, angp.size());
+    AL(d_eterm, (size_t)nbt + nat);
     auto H2D = [&](void *dst, const void *src, size_t bytes) -> int {
         if (bytes) CK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st));
         return 0;
@@ -438,6 +513,13 @@ int Engine::init(const jp_sys_desc *descs, int ns) {
     r |= H2D(d_s3a.p, s3a.data(), 16 * s3a.size());
     r |= H2D(d_s2d.p, s2d.data(), 8 * s2d.size());
     r |= H2D(d_s3d.p, s3d.data(), 8 * s3d.size());
+    r |= H2D(d_bat.p, bat.data(), 4 * bat.size());
+    r |= H2D(d_bptr.p, bptr.data(), 4 * bptr.size());
+    r |= H2D(d_bref.p, bref.data(), 4 * bref.size());
+    r |= H2D(d_bnd.p, bnd.data(), 16 * bnd.size());
+    r |= H2D(d_bndp.p, bndp.data(), 16 * bndp.size());
+    r |= H2D(d_ang.p, ang.data(), 16 * ang.size());
+    r |= H2D(d_angp.p, angp.data(), 16 * angp.size());
     if (r) return r;
     CK(hipMemsetAsync(d_v.p, 0, 8 * 3 * (size_t)natom, st));
     CK(hipStreamSynchronize(st));
@@ -561,6 +643,66 @@ int Engine::pair(int prec, bool energy) {
         default: launch_pair<true, true, true>(prec, nb, l32, l64, st, *this); break;
     }
     CK(hipGetLastError());
+    return 0;
+}
+
+int Engine::forces(int prec, bool energy) {
+    int r = pair(prec, energy);
+    if (r || !nbat) return r;
+    hipLaunchKernelGGL(k_bonded, grid1(nbat, 128), 128, 0, st, bonded_args(), d_atom_sys.p, d_sp.p, d_x.p, d_efac.p,
+                       d_inum.p, d_gslot.p, (double *)nullptr);
+    CK(hipGetLastError());
+    return 0;
+}
+
+// Per-system, per-class bond and angle energies at the device coordinates,
+// summed in term order (dnebnd_PB / dneang_PB).
+int Engine::bonded_energies(std::vector<double> &eb, std::vector<double> &ea) {
+    eb.assign(3 * (size_t)nsys, 0.0);
+    ea.assign(3 * (size_t)nsys, 0.0);
+    const int nt = nbt + nat;
+    if (!nt) return 0;
+    hipLaunchKernelGGL(k_bonded_energy, grid1(nt, 256), 256, 0, st, nbt, nat, bonded_args(), d_atom_sys.p, d_sp.p,
+                       d_x.p, d_eterm.p);
+    CK(hipGetLastError());
+    std::vector<double> e(nt);
+    CK(hipMemcpyAsync(e.data(), d_eterm.p, 8 * (size_t)nt, hipMemcpyDeviceToHost, st));
+    CK(hipStreamSynchronize(st));
+    for (int q = 0; q < nt; ++q) {
+        double &dst = (q < nbt ? eb : ea)[3 * (size_t)tsys_h[q] + tcls_h[q]];
+        dst = dst + e[q];
+    }
+    return 0;
+}
+
+// dnebnd_PB + dneang_PB for jp_bonded: per-class gradient accumulated into
+// grad(3,N,nfrg), per-class energies.
+int Engine::bonded(const double *cord, double *grad, double *ebond, double *eangle) {
+    int r;
+    if (cord) CK(hipMemcpyAsync(d_x.p, cord, 24 * (size_t)natom, hipMemcpyHostToDevice, st));
+    const int nfrg = hs[0].nfrg;
+    if (grad && nbat) {
+        std::vector<double> g(3 * (size_t)natom);
+        for (int k = 0; k < nfrg; ++k) {
+            if ((r = set_factors(k))) return r;
+            CK(hipMemsetAsync(d_grad.p, 0, 8 * g.size(), st));
+            hipLaunchKernelGGL(k_bonded, grid1(nbat, 128), 128, 0, st, bonded_args(), d_atom_sys.p, d_sp.p, d_x.p,
+                               d_efac.p, d_inum.p, (double *)nullptr, d_grad.p);
+            CK(hipGetLastError());
+            CK(hipMemcpyAsync(g.data(), d_grad.p, 8 * g.size(), hipMemcpyDeviceToHost, st));
+            CK(hipStreamSynchronize(st));
+            double *gk = grad + (size_t)k * 3 * natom;
+            for (size_t q = 0; q < g.size(); ++q) gk[q] += g[q];
+        }
+        if ((r = set_factors(-1))) return r;
+    }
+    std::vector<double> eb, ea;
+    if ((r = bonded_energies(eb, ea))) return r;
+    for (int k = 0; k < nfrg; ++k) {
+        if (ebond) ebond[k] = eb[k];
+        if (eangle) eangle[k] = ea[k];
+    }
+    CK(hipStreamSynchronize(st));
     return 0;
 }
 
@@ -700,7 +842,7 @@ Engine::StepGraph *Engine::step_graph(int n, int prec) {
     if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) return nullptr;
     int r = 0;
     for (int k = 0; k < n && !r; ++k) {
-        r = pair(prec, false);
+        r = forces(prec, false);
         if (!r) r = integrate(prec, false);
     }
     const hipError_t ec = hipStreamEndCapture(st, &graph);
@@ -720,7 +862,8 @@ Engine::StepGraph *Engine::step_graph(int n, int prec) {
 }
 
 int Engine::mon_record(int step, int nsteps, int nmon, int nlog, const std::vector<double> &eel,
-                       const std::vector<double> &evd, jp_energy_log *log) {
+                       const std::vector<double> &evd, const std::vector<double> &eb, const std::vector<double> &ea,
+                       jp_energy_log *log) {
     hipLaunchKernelGGL(k_ekin, grid1(natom, 256), 256, 0, st, natom, d_mass.p, d_v.p, d_vback.p, d_ekin.p);
     std::vector<double> ek(natom);
     int err = 0;
@@ -741,7 +884,9 @@ int Engine::mon_record(int step, int nsteps, int nmon, int nlog, const std::vect
             const double f = (hs[s].lmd && hs[s].nfrg == 3) ? (k == 0 ? hs[s].lambda * hs[s].lambda : (k == 1 ? hs[s].lambda : 1.0)) : 1.0;
             L.eelec[k] = k < hs[s].nfrg ? eel[3 * s + k] : 0.0;
             L.evdw[k] = k < hs[s].nfrg ? evd[3 * s + k] : 0.0;
-            if (k < hs[s].nfrg) ep += f * (eel[3 * s + k] + evd[3 * s + k]);
+            L.ebond[k] = k < hs[s].nfrg ? eb[3 * s + k] : 0.0;
+            L.eangle[k] = k < hs[s].nfrg ? ea[3 * s + k] : 0.0;
+            if (k < hs[s].nfrg) ep += f * (eb[3 * s + k] + ea[3 * s + k] + eel[3 * s + k] + evd[3 * s + k]);
         }
         L.epot = ep;
         L.etot = ep + e;
@@ -796,14 +941,20 @@ int Engine::run_nve(int nsteps, int update, int nmon, int prec, jp_energy_log *l
         CK(hipEventRecord(a, st));
         if ((r = pair(prec, mon))) return r;
         CK(hipEventRecord(b, st));
+        if (nbat) {
+            hipLaunchKernelGGL(k_bonded, grid1(nbat, 128), 128, 0, st, bonded_args(), d_atom_sys.p, d_sp.p, d_x.p,
+                               d_efac.p, d_inum.p, d_gslot.p, (double *)nullptr);
+            CK(hipGetLastError());
+        }
         pe.push_back(a);
         pe.push_back(b);
         ++n_pair;
-        std::vector<double> eel, evd, eex;
+        std::vector<double> eel, evd, eex, eb, ea;
         if (mon && (r = energies(prec, eel, evd, eex))) return r;
+        if (mon && (r = bonded_energies(eb, ea))) return r;
         if ((r = integrate(prec, mon))) return r;
         if (mon) {
-            if ((r = mon_record(step, nsteps, nmon, nlog, eel, evd, log))) return r;
+            if ((r = mon_record(step, nsteps, nmon, nlog, eel, evd, eb, ea, log))) return r;
             nlog++;
         }
         ++idloop;
@@ -959,6 +1110,11 @@ int jp_nonbond(jp_ctx *ctx, const double *cord, double *grad, double *eelec, dou
     if (ehyd)
         for (int k = 0; k < ctx->e.hs[0].nfrg; ++k) ehyd[k] = 0.0;   // no 12-10 H-bond pairs (iyppid != 2)
     return r;
+}
+
+int jp_bonded(jp_ctx *ctx, const double *cord, double *grad, double *ebond, double *eangle) {
+    if (!ctx) return JP_EINVAL;
+    return ctx->e.bonded(cord, grad, ebond, eangle);
 }
 
 int jp_shake(jp_ctx *ctx, double *cord, const double *cord_old) {

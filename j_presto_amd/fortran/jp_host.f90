@@ -1,34 +1,39 @@
 ! Fortran host of libjpgpu.  Reads a system file, then does through the C ABI
 ! what md_run does through its own routines:
-!   celprp (jp_build_list), dnergy's nonbonded part (grad zeroed, then
-!   jp_nonbond accumulates; FP64 and FP32/fixed-point), and the METHOD= MICR
-!   MD loop (jp_run_nve), and writes every result to an output file.
+!   celprp (jp_build_list), dnergy (grad zeroed, then jp_bonded and
+!   jp_nonbond accumulate; FP64 and FP32/fixed-point pair arithmetic), and the
+!   METHOD= MICR MD loop (jp_run_nve), and writes every result to an output file.
 !
 ! usage: jp_host <system file> <output file> <nsteps> <update> <nmon> <precision 0|1>
 !        jp_host --layout   (byte sizes and offsets of the bind(C) types; no GPU use)
 !
 ! System file (Fortran unformatted sequential, one array per record):
-!   natom ntyp nfrag nfrg nexcl nexcess nshake shake_maxit cell_level lmd   (int32)
+!   natom ntyp nfrag nfrg nexcl nexcess nshake shake_maxit cell_level lmd nbond nangle (int32)
 !   shake_tol cutoff diel dt lambda celwal(6)                               (real*8)
 !   cord(3,N) vel(3,N) mass(N) charge(N) atype(N) icls(N) rmin2(ntyp) eps(ntyp)
 !   n_matrix(nfrag,nfrag) excl(2,nexcl) excess(2,nexcess)
 !   shake_natom(nshake) shake_atoms(4,nshake) shake_dist2(6,nshake)
+!   bond_atoms(2,nbond) bond_fc(nbond) bond_r0(nbond)
+!   angle_atoms(3,nangle) angle_fc(nangle) angle_theta0(nangle)
 ! Output file (same record format):
-!   ier(6): create, build_list, nonbond FP64, nonbond FP32, run_nve, get_state
+!   ier(8): create, build_list, nonbond FP64, nonbond FP32, run_nve, get_state, bonded (x2)
 !   grad64(3,N,nfrg) eelec64(nfrg) evdw64(nfrg) grad32(3,N,nfrg) eelec32(nfrg) evdw32(nfrg)
+!   ebond(nfrg) eangle(nfrg)
 !   cord(3,N) vel(3,N) after the run, nlog, then per record: step etot epot ekin
 program jp_host
   use iso_c_binding
   use jpgpu
   implicit none
   character(len=1024) :: fin, fout, arg
-  integer(c_int32_t) :: hdr(10), nsteps, update, nmon, prec
+  integer(c_int32_t) :: hdr(12), nsteps, update, nmon, prec
   real(c_double) :: rhdr(11)
-  integer :: n, ntyp, nfrag, nfrg, nexcl, nexcess, nshake, nlog, k, u
-  integer(c_int) :: ier(6)
+  integer :: n, ntyp, nfrag, nfrg, nexcl, nexcess, nshake, nbond, nangle, nlog, k, u
+  integer(c_int) :: ier(8)
   real(c_double), allocatable, target :: cord(:,:), vel(:,:), mass(:), charge(:), rmin2(:), eps(:)
   real(c_double), allocatable, target :: sdist(:,:), grad64(:,:,:), grad32(:,:,:)
-  real(c_double), target :: eel64(3), evd64(3), ehy64(3), eel32(3), evd32(3), ehy32(3)
+  real(c_double), target :: eel64(3), evd64(3), ehy64(3), eel32(3), evd32(3), ehy32(3), ebnd(3), eang(3)
+  real(c_double), allocatable, target :: bfc(:), br0(:), afc(:), ath(:)
+  integer(c_int32_t), allocatable, target :: batm(:,:), aatm(:,:)
   integer(c_int32_t), allocatable, target :: atype(:), icls(:), nmat(:,:), excl(:,:), excess(:,:)
   integer(c_int32_t), allocatable, target :: snat(:), satm(:,:)
   type(jp_energy_log), allocatable, target :: elog(:)
@@ -44,6 +49,8 @@ program jp_host
      write(*,'(a,i0)') 'offset celwal ', off(c_loc(desc%celwal))
      write(*,'(a,i0)') 'offset lmd ', off(c_loc(desc%lmd))
      write(*,'(a,i0)') 'offset lambda ', off(c_loc(desc%lambda))
+     write(*,'(a,i0)') 'offset nbond ', off(c_loc(desc%nbond))
+     write(*,'(a,i0)') 'offset angle_cls ', off(c_loc(desc%angle_cls))
      write(*,'(a,i0)') 'sizeof jp_energy_log ', c_sizeof(rec1)
      stop
   end if
@@ -62,10 +69,12 @@ program jp_host
   read(u) hdr
   read(u) rhdr
   n = hdr(1); ntyp = hdr(2); nfrag = hdr(3); nfrg = hdr(4)
-  nexcl = hdr(5); nexcess = hdr(6); nshake = hdr(7)
+  nexcl = hdr(5); nexcess = hdr(6); nshake = hdr(7); nbond = hdr(11); nangle = hdr(12)
   allocate(cord(3,n), vel(3,n), mass(n), charge(n), atype(n), icls(n), rmin2(ntyp), eps(ntyp))
   allocate(nmat(nfrag,nfrag), excl(2,max(nexcl,1)), excess(2,max(nexcess,1)))
   allocate(snat(max(nshake,1)), satm(4,max(nshake,1)), sdist(6,max(nshake,1)))
+  allocate(batm(2,max(nbond,1)), bfc(max(nbond,1)), br0(max(nbond,1)))
+  allocate(aatm(3,max(nangle,1)), afc(max(nangle,1)), ath(max(nangle,1)))
   read(u) cord
   read(u) vel
   read(u) mass
@@ -80,6 +89,12 @@ program jp_host
   read(u) snat(1:nshake)
   read(u) satm(:, 1:nshake)
   read(u) sdist(:, 1:nshake)
+  read(u) batm(:, 1:nbond)
+  read(u) bfc(1:nbond)
+  read(u) br0(1:nbond)
+  read(u) aatm(:, 1:nangle)
+  read(u) afc(1:nangle)
+  read(u) ath(1:nangle)
   close(u)
 
   desc%natom = n; desc%ntyp = ntyp; desc%nfrag = nfrag; desc%nfrg = nfrg
@@ -93,6 +108,9 @@ program jp_host
   desc%shake_maxit = hdr(8); desc%cell_level = hdr(9); desc%lmd = hdr(10)
   desc%shake_tol = rhdr(1); desc%cutoff = rhdr(2); desc%diel = rhdr(3); desc%dt = rhdr(4)
   desc%lambda = rhdr(5); desc%celwal = rhdr(6:11)
+  desc%nbond = nbond; desc%bond_atoms = c_loc(batm); desc%bond_fc = c_loc(bfc); desc%bond_r0 = c_loc(br0)
+  desc%nangle = nangle; desc%angle_atoms = c_loc(aatm); desc%angle_fc = c_loc(afc)
+  desc%angle_theta0 = c_loc(ath)
 
   write(*,'(2a)') 'jp_host: ', jp_version()
   ier = 0
@@ -104,11 +122,13 @@ program jp_host
   ier(1) = jp_set_state(ctx, c_null_ptr, c_loc(vel))
   ier(2) = jp_build_list(ctx, c_null_ptr)                 ! celprp
 
-  ! dnergy: grad zeroed, the nonbonded part accumulated (dnergy.f90:45-55, cmmdo)
+  ! dnergy: grad zeroed, then bonds and angles, then cmmdo (dnergy.f90:45-55, dnergv.f90)
   allocate(grad64(3,n,nfrg), grad32(3,n,nfrg))
-  grad64 = 0.0d0; eel64 = 0.0d0; evd64 = 0.0d0; ehy64 = 0.0d0
+  grad64 = 0.0d0; eel64 = 0.0d0; evd64 = 0.0d0; ehy64 = 0.0d0; ebnd = 0.0d0; eang = 0.0d0
+  ier(7) = jp_bonded(ctx, c_null_ptr, c_loc(grad64), c_loc(ebnd), c_loc(eang))
   ier(3) = jp_nonbond(ctx, c_null_ptr, c_loc(grad64), c_loc(eel64), c_loc(evd64), c_loc(ehy64), JP_PREC_FP64)
   grad32 = 0.0d0; eel32 = 0.0d0; evd32 = 0.0d0; ehy32 = 0.0d0
+  ier(8) = jp_bonded(ctx, c_null_ptr, c_loc(grad32), c_null_ptr, c_null_ptr)
   ier(4) = jp_nonbond(ctx, c_null_ptr, c_loc(grad32), c_loc(eel32), c_loc(evd32), c_loc(ehy32), JP_PREC_FP32_FIXED)
 
   ! METHOD= MICR loop
@@ -117,7 +137,7 @@ program jp_host
   allocate(elog(max(nlog,1)))
   ier(5) = jp_run_nve(ctx, nsteps, update, nmon, prec, c_loc(elog))
   ier(6) = jp_get_state(ctx, c_loc(cord), c_loc(vel))
-  write(*,'(a,6(1x,i0))') 'jp_host: ier =', ier
+  write(*,'(a,8(1x,i0))') 'jp_host: ier =', ier
   if (nlog > 0) write(*,'(a,i0,a,f16.6)') 'jp_host: step ', elog(nlog)%step, ' etot ', elog(nlog)%etot
 
   open(newunit=u, file=trim(fout), form='unformatted', access='sequential', status='replace', action='write')
@@ -128,6 +148,8 @@ program jp_host
   write(u) grad32
   write(u) eel32(1:nfrg)
   write(u) evd32(1:nfrg)
+  write(u) ebnd(1:nfrg)
+  write(u) eang(1:nfrg)
   write(u) cord
   write(u) vel
   write(u) nlog

@@ -11,6 +11,7 @@
 !   jp_ctx_create   incel.f90:2 + inpcls.f90 + setshk.f90 (set-up)
 !   jp_build_list   celprp.f90:2 (celpar_PB, cutatm_PB, cutoff_PB)
 !   jp_nonbond      cmmdo.f90:2 -> enear_ZD_*_PB (enear.f90:1-593)
+!   jp_bonded       dnergv.f90:62-193 (dnebnd_PB, dneang_PB)
 !   jp_shake        shake.f90:2 (shake_PB)
 !   jp_run_nve      md.f90:519-694 + verlet.f90:2 (METHOD= MICR loop)
 !   jp_batch_*      j_presto_mpi:27-44 (independent runs, one launch)
@@ -45,6 +46,13 @@ module jpgpu
      real(c_double) :: cutoff = 12.0d0, diel = 1.0d0, dt = 2.0d0
      integer(c_int32_t) :: lmd = 0
      real(c_double) :: lambda = 0.0d0
+     ! bonds and angles left after setshk (iypbnd(:,1:nbnd), iypang(:,1:nang))
+     integer(c_int32_t) :: nbond = 0
+     type(c_ptr) :: bond_atoms = c_null_ptr, bond_fc = c_null_ptr, bond_r0 = c_null_ptr
+     type(c_ptr) :: bond_cls = c_null_ptr
+     integer(c_int32_t) :: nangle = 0
+     type(c_ptr) :: angle_atoms = c_null_ptr, angle_fc = c_null_ptr, angle_theta0 = c_null_ptr
+     type(c_ptr) :: angle_cls = c_null_ptr
   end type jp_sys_desc
 
   ! One monitoring record of jp_run_nve (md.f90:618-628).
@@ -52,6 +60,7 @@ module jpgpu
      integer(c_int32_t) :: step
      real(c_double) :: etot, epot, ekin
      real(c_double) :: eelec(3), evdw(3)
+     real(c_double) :: ebond(3), eangle(3)
   end type jp_energy_log
 
   interface
@@ -96,6 +105,12 @@ module jpgpu
        type(c_ptr), value :: ctx, cord, grad, eelec, evdw, ehyd
        integer(c_int), value :: precision
      end function jp_nonbond
+
+     ! dnebnd_PB + dneang_PB: grad accumulated, per-class energies
+     integer(c_int) function jp_bonded(ctx, cord, grad, ebond, eangle) bind(C, name='jp_bonded')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx, cord, grad, ebond, eangle
+     end function jp_bonded
 
      integer(c_int) function jp_shake(ctx, cord, cord_old) bind(C, name='jp_shake')
        import :: c_int, c_ptr

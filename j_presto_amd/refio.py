@@ -219,15 +219,16 @@ def write_jpsys(path: str, s: System, dt: float = 2.0, cell_level: int = 3, shak
     h = DescHolder(s, dt=dt, cell_level=cell_level, shake_tol=shake_tol, shake_maxit=shake_maxit)
     d = h.desc
     ng = d.nshake
-    hdr = np.array([d.natom, d.ntyp, d.nfrag, d.nfrg, d.nexcl, d.nexcess, ng, shake_maxit, cell_level, d.lmd],
-                   np.int32)
+    hdr = np.array([d.natom, d.ntyp, d.nfrag, d.nfrg, d.nexcl, d.nexcess, ng, shake_maxit, cell_level, d.lmd,
+                    d.nbond, d.nangle], np.int32)
     rhdr = np.array([shake_tol, d.cutoff, d.diel, dt, d.lambda_] + [float(w) for w in d.celwal], np.float64)
     i32 = lambda a: np.ascontiguousarray(a, np.int32).tobytes()
     f64 = lambda a: np.ascontiguousarray(a, np.float64).tobytes()
     write_records(path, [
         hdr.tobytes(), rhdr.tobytes(), f64(h.x), f64(s.v), f64(h.mass), f64(h.charge), i32(h.atype), i32(h.icls),
         f64(h.rmin2), f64(h.eps), i32(h.nmat), i32(h.excl), i32(h.excess), i32(h.shk_n[:ng]), i32(h.shk_a[:ng]),
-        f64(h.shk_d[:ng]),
+        f64(h.shk_d[:ng]), i32(h.bond_atoms), f64(h.bond_fc), f64(h.bond_r0), i32(h.angle_atoms),
+        f64(h.angle_fc), f64(h.angle_theta0),
     ])
 
 
@@ -239,8 +240,9 @@ def read_jphost(path: str, n: int, nfrg: int) -> Dict[str, object]:
         "ier": np.frombuffer(r[0], np.int32).copy(),
         "grad64": f(r[1]).reshape(nfrg, n, 3).copy(), "eelec64": f(r[2]).copy(), "evdw64": f(r[3]).copy(),
         "grad32": f(r[4]).reshape(nfrg, n, 3).copy(), "eelec32": f(r[5]).copy(), "evdw32": f(r[6]).copy(),
-        "x": f(r[7]).reshape(n, 3).copy(), "v": f(r[8]).reshape(n, 3).copy(),
+        "ebond": f(r[7]).copy(), "eangle": f(r[8]).copy(),
+        "x": f(r[9]).reshape(n, 3).copy(), "v": f(r[10]).reshape(n, 3).copy(),
     }
-    nlog = struct.unpack("<i", r[9])[0]
-    out["log"] = [struct.unpack("<iddd", r[10 + k]) for k in range(nlog)]
+    nlog = struct.unpack("<i", r[11])[0]
+    out["log"] = [struct.unpack("<iddd", r[12 + k]) for k in range(nlog)]
     return out

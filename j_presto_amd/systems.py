@@ -144,6 +144,18 @@ class System:
         out = [m.angles - 1 + off for m, off in self._offsets() if len(m.angles)]
         return np.concatenate(out).astype(np.int64) if out else np.zeros((0, 3), np.int64)
 
+    def bond_params(self):
+        """(fc, r0) of every bond, in bonds() order."""
+        fc = [np.asarray(m.bond_fc, np.float64) for m, _ in self._offsets() if len(m.bonds)]
+        r0 = [np.asarray(m.bond_r0, np.float64) for m, _ in self._offsets() if len(m.bonds)]
+        return (np.concatenate(fc), np.concatenate(r0)) if fc else (np.zeros(0), np.zeros(0))
+
+    def angle_params(self):
+        """(fc, theta0 in degrees) of every angle, in angles() order."""
+        fc = [np.asarray(m.angle_fc, np.float64) for m, _ in self._offsets() if len(m.angles)]
+        th = [np.asarray(m.angle_deg, np.float64) for m, _ in self._offsets() if len(m.angles)]
+        return (np.concatenate(fc), np.concatenate(th)) if fc else (np.zeros(0), np.zeros(0))
+
     def excl14(self) -> np.ndarray:
         """Unique excluded pairs (i<j, 0-based) from the ATOM partner lists
         (the reference's itab14, inpcls.f90:229-295)."""
@@ -423,12 +435,13 @@ def _saw_chain(rng, n: int, L: float, bond: float = 1.5, excl: float = 3.0,
 
 def chain_template(rng, n: int, type0: int, ntypes: int, cluster: int = 1,
                    bond_fc: float = 0.0, angle_fc: float = 0.0, pos: Optional[np.ndarray] = None,
-                   L: float = 0.0) -> Tuple[MolTemplate, List[AtomType]]:
+                   L: float = 0.0, bond_r0: Optional[float] = None,
+                   angle_deg: Optional[float] = None) -> Tuple[MolTemplate, List[AtomType]]:
     """Linear chain: bonds i-(i+1), angles i-(i+1)-(i+2); exclusions are the
     1-2 and 1-3 pairs; every 4th atom (index 3, 7, ...) is 'H' constrained to
-    its predecessor.  Bond/angle force constants default to 0 so that the
-    chain's forces are the nonbonded ones only (bonded terms are not part of
-    this path)."""
+    its predecessor.  Harmonic bond/angle force constants default to 0 (no
+    bonded forces); their equilibrium values are the generated geometry's own
+    unless `bond_r0` / `angle_deg` fix them."""
     q = rng.normal(0.0, 0.3, size=n)
     q -= q.mean()
     sig = rng.uniform(1.9, 3.8, size=ntypes)
@@ -462,6 +475,10 @@ def chain_template(rng, n: int, type0: int, ntypes: int, cluster: int = 1,
     else:
         r0 = np.full(n - 1, 1.5)
         th0 = np.full(n - 2, 109.5)
+    if bond_r0 is not None:
+        r0 = np.full(n - 1, float(bond_r0))
+    if angle_deg is not None:
+        th0 = np.full(n - 2, float(angle_deg))
     shake = [((i, i + 1), (1.5,)) for i in range(3, n + 1, 4) if i <= n and i >= 1 and i + 1 <= n]
     shake = [((a, b), d) for (a, b), d in shake if is_h[b - 1]]
     mol = MolTemplate(
@@ -473,8 +490,10 @@ def chain_template(rng, n: int, type0: int, ntypes: int, cluster: int = 1,
 
 
 def chain_in_water(n_chain: int, L: float, seed: int, n_chain_types: int = 16, temp: float = 300.0,
-                   lmd: Optional[float] = None, name: str = "chain") -> System:
-    """A config[1]-style system: SAW chain in a periodic water box; waters
+                   lmd: Optional[float] = None, name: str = "chain", bond_fc: float = 0.0,
+                   angle_fc: float = 0.0, bond_r0: Optional[float] = None,
+                   angle_deg: Optional[float] = None) -> System:
+    """Solvated chain: SAW chain in a periodic water box; waters
     with any atom within 2.6 A of a chain atom are removed.  With `lmd`, the
     chain is cluster class 1 and water class 2 under CLSTMETHOD=LMD (three
     energy classes: chain-chain lambda^2, chain-water lambda, water-water 1)."""
@@ -498,7 +517,8 @@ def chain_in_water(n_chain: int, L: float, seed: int, n_chain_types: int = 16, t
     lm = lmd is not None
     wt = water_types()
     chain, ctypes = chain_template(rng, n_chain, type0=len(wt) + 1, ntypes=n_chain_types,
-                                   cluster=1, pos=cpos, L=L)
+                                   cluster=1, bond_fc=bond_fc, angle_fc=angle_fc, pos=cpos, L=L,
+                                   bond_r0=bond_r0, angle_deg=angle_deg)
     mols = [chain, water_template(nwat, cluster=2 if lm else 1)]
     xx = np.mod(np.concatenate([cpos, pos_w]), L)
     s = System(name=name, mols=mols, types=wt + ctypes, box=Lv, x=xx, v=np.zeros_like(xx),

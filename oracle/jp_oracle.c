@@ -18,6 +18,8 @@
  *   cutatm_PB    cutoff.f90:334-497 (full "DOUB" list)
  *   cutoff_PB    cutoff.f90:50-96 (1-4 exclusion swap-delete, ier=99)
  *   enear_ZD_doub_PB  enear.f90:60-326 (i_vec = 1), 518-571 (excess, self)
+ *   dnebnd_PB    dnergv.f90:62-114 (harmonic bonds)
+ *   dneang_PB    dnergv.f90:117-193 (harmonic angles)
  *   shake        shake.f90:112-327 (3-atom Newton), 687-755 (2-atom exact)
  *   verlet       verlet.f90:144-215 (MICR leap-frog)
  *   caltmp       caltmp.f90:18-27
@@ -76,6 +78,11 @@ typedef struct {
     double dt;
     int lmd;
     double lambda;
+    /* bonded terms left after setshk: 0-based atoms, 0-based class */
+    int nbond, *bnd, *bcls;
+    double *bfc, *br0;
+    int nangle, *ang, *acls;
+    double *afc, *ath0;
 } jo_ctx;
 
 /* ------------------------------------------------------------------ setup */
@@ -302,6 +309,33 @@ jo_ctx *jo_create(const jp_sys_desc *d) {
     c->cell3 = (int *)malloc(sizeof(int) * 3 * n);
     c->inum = (int *)malloc(sizeof(int) * n);
     c->order = (int *)malloc(sizeof(int) * n);
+    /* bonds and angles (dnergv.f90); class max(n_matrix) unless given (incel.f90:952-969) */
+    int cmax = 0;
+    for (int q = 0; q < d->nfrag * d->nfrag; ++q)
+        if (d->n_matrix[q] - 1 > cmax) cmax = d->n_matrix[q] - 1;
+    c->nbond = d->nbond;
+    c->bnd = (int *)malloc(sizeof(int) * 2 * (d->nbond ? d->nbond : 1));
+    c->bcls = (int *)malloc(sizeof(int) * (d->nbond ? d->nbond : 1));
+    c->bfc = (double *)malloc(sizeof(double) * (d->nbond ? d->nbond : 1));
+    c->br0 = (double *)malloc(sizeof(double) * (d->nbond ? d->nbond : 1));
+    for (int b = 0; b < d->nbond; ++b) {
+        c->bnd[2 * b] = d->bond_atoms[2 * b] - 1;
+        c->bnd[2 * b + 1] = d->bond_atoms[2 * b + 1] - 1;
+        c->bcls[b] = d->bond_cls ? d->bond_cls[b] - 1 : cmax;
+        c->bfc[b] = d->bond_fc[b];
+        c->br0[b] = d->bond_r0[b];
+    }
+    c->nangle = d->nangle;
+    c->ang = (int *)malloc(sizeof(int) * 3 * (d->nangle ? d->nangle : 1));
+    c->acls = (int *)malloc(sizeof(int) * (d->nangle ? d->nangle : 1));
+    c->afc = (double *)malloc(sizeof(double) * (d->nangle ? d->nangle : 1));
+    c->ath0 = (double *)malloc(sizeof(double) * (d->nangle ? d->nangle : 1));
+    for (int a = 0; a < d->nangle; ++a) {
+        for (int k = 0; k < 3; ++k) c->ang[3 * a + k] = d->angle_atoms[3 * a + k] - 1;
+        c->acls[a] = d->angle_cls ? d->angle_cls[a] - 1 : cmax;
+        c->afc[a] = d->angle_fc[a];
+        c->ath0[a] = d->angle_theta0[a];
+    }
     return c;
 }
 
@@ -315,6 +349,8 @@ void jo_destroy(jo_ctx *c) {
     free(c->excess); free(c->exc_cls); free(c->dself); free(c->shk_n); free(c->shk_a); free(c->shk_d2);
     free(c->cell3); free(c->inum); free(c->order); free(c->lptr); free(c->nel); free(c->nvd);
     free(c->lidx); free(c->elcap);
+    free(c->bnd); free(c->bcls); free(c->bfc); free(c->br0);
+    free(c->ang); free(c->acls); free(c->afc); free(c->ath0);
     free(c);
 }
 
@@ -729,6 +765,70 @@ static inline void mimg(const jo_ctx *c, double *d) {
     for (int k = 0; k < 3; ++k) d[k] = d[k] - c->fxcell[k] * anint(d[k] * c->invcel[k]);
 }
 
+/* ---------------------------------------------------------------- bonded */
+
+/* dnebnd_PB (dnergv.f90:62-114) then dneang_PB (dnergv.f90:117-193), one
+ * thread: grad(3,N,nfrg) accumulated in term order, energies per class
+ * (ebond/eangle zeroed here). */
+int jo_bonded(const jo_ctx *c, const double *x, double *grad, double *ebond, double *eangle) {
+    const size_t n3 = (size_t)3 * c->n;
+    for (int k = 0; k < c->nfrg; ++k) { ebond[k] = 0.0; eangle[k] = 0.0; }
+    for (int b = 0; b < c->nbond; ++b) {
+        int i = c->bnd[2 * b], j = c->bnd[2 * b + 1], n = c->bcls[b];
+        double d[3];
+        for (int k = 0; k < 3; ++k) d[k] = x[3 * i + k] - x[3 * j + k];
+        mimg(c, d);
+        double r12 = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        double dif = r12 - c->br0[b];
+        double coef = c->bfc[b] * dif;
+        ebond[n] = ebond[n] + (coef * dif);
+        coef = 2.0 * coef / r12;
+        double *g = grad + n * n3;
+        for (int k = 0; k < 3; ++k) {
+            double w = coef * d[k];
+            g[3 * i + k] = g[3 * i + k] + w;
+            g[3 * j + k] = g[3 * j + k] - w;
+        }
+    }
+    for (int a = 0; a < c->nangle; ++a) {
+        const int *t = c->ang + 3 * a;
+        int n = c->acls[a];
+        double d12[3], d32[3], w[6];
+        for (int k = 0; k < 3; ++k) {
+            d12[k] = x[3 * t[0] + k] - x[3 * t[1] + k];
+            d32[k] = x[3 * t[2] + k] - x[3 * t[1] + k];
+        }
+        mimg(c, d12);
+        mimg(c, d32);
+        double r12 = d12[0] * d12[0] + d12[1] * d12[1] + d12[2] * d12[2];
+        double r32 = d32[0] * d32[0] + d32[1] * d32[1] + d32[2] * d32[2];
+        double r12r32 = sqrt(1.0 / (r12 * r32));
+        double cost = d12[0] * d32[0] + d12[1] * d32[1] + d12[2] * d32[2];
+        r12 = 1.0 / r12;
+        r32 = 1.0 / r32;
+        cost = fmin(cost * r12r32, 1.0);
+        cost = fmax(cost, -1.0);
+        double theta = acos(cost);
+        double dif = theta - c->ath0[a];
+        double coef = c->afc[a] * dif;
+        eangle[n] = eangle[n] + coef * dif;
+        double sint = fmax(sin(theta), 1.0e-10); /* PHYCNS.f90:34 eps */
+        coef = -2.0 * coef / sint;
+        double rtmp = cost * r12;
+        for (int k = 0; k < 3; ++k) w[k] = d32[k] * r12r32 - d12[k] * rtmp;
+        rtmp = cost * r32;
+        for (int k = 0; k < 3; ++k) w[3 + k] = d12[k] * r12r32 - d32[k] * rtmp;
+        for (int k = 0; k < 6; ++k) w[k] = coef * w[k];
+        double *g = grad + n * n3;
+        for (int k = 0; k < 3; ++k) {
+            g[3 * t[0] + k] = g[3 * t[0] + k] + w[k];
+            g[3 * t[1] + k] = g[3 * t[1] + k] - (w[k] + w[3 + k]);
+            g[3 * t[2] + k] = g[3 * t[2] + k] + w[3 + k];
+        }
+    }
+    return 0;
+}
+
 /* shak2e_PB, shake.f90:687-755 */
 static void shak2e(const jo_ctx *c, int g, double *r1, const double *r2) {
     int ia = c->shk_a[4 * g], ja = c->shk_a[4 * g + 1];
@@ -872,10 +972,11 @@ int jo_run_nve(jo_ctx *c, double *x, double *v, int nsteps, int update) {
     double *grad = (double *)malloc(sizeof(double) * 3 * (size_t)n * nfrg);
     double *gt = (double *)malloc(sizeof(double) * 3 * (size_t)n);
     double *xo = (double *)malloc(sizeof(double) * 3 * (size_t)n);
-    double ee[3], ev[3];
+    double ee[3], ev[3], eb[3], ea[3];
     int ier = c->have_list ? 0 : jo_celprp(c, x);
     for (int s = 1; s <= nsteps && !ier; ++s) {
         memset(grad, 0, sizeof(double) * 3 * (size_t)n * nfrg);
+        jo_bonded(c, x, grad, eb, ea); /* dnergy order: bonded terms, then cmmdo */
         jo_nonbond(c, x, grad, ee, ev);
         jo_combine_grad(c, grad, gt);
         ier = jo_verlet_step(c, x, v, gt, xo);

@@ -45,6 +45,12 @@ RECIPES = {
     "chain_3k": ("chain_in_water", dict(n_chain=200, L=30.0, seed=11), dict(traj=20)),
     # LMD: three energy classes, lambda-scaled gradient
     "chain_lmd": ("chain_in_water", dict(n_chain=160, L=28.0, seed=12, lmd=0.7), dict(traj=10)),
+    # chain with harmonic bonds and angles (dnebnd_PB / dneang_PB), strained off their minima
+    "chain_bonded": ("chain_in_water", dict(n_chain=200, L=30.0, seed=15, bond_fc=300.0, angle_fc=50.0,
+                                            bond_r0=1.53, angle_deg=112.0), dict(traj=20)),
+    # LMD with bonded terms: bonds and angles in the unscaled class (SCALED_TERM default)
+    "chain_lmd_bonded": ("chain_in_water", dict(n_chain=120, L=26.0, seed=16, lmd=0.7, bond_fc=300.0,
+                                                angle_fc=50.0, bond_r0=1.53, angle_deg=112.0), dict(traj=10)),
     # ions with their own LJ types
     "ions_3k": ("water_box", dict(nwat=900, L=30.0, seed=13, n_ion_pairs=10), dict(traj=0)),
     # box edge a multiple of rlim = 13.5 A (md_run's phantom-cell case)

@@ -25,8 +25,25 @@ def test_library_exports_every_symbol():
     assert lib.jp_version().startswith(b"jpgpu")
 
 
-def test_descriptor_layout_matches_header():
+def test_descriptor_layout_matches_header(tmp_path):
+    """Every field of the ctypes mirrors sits at the offset gcc gives it in
+    include/jpgpu.h."""
+    import subprocess
     from j_presto_amd.abi import SysDesc, EnergyLog
-    # natom..nfrg (4 ints), 8 pointers, nexcl + ptr, nexcess + ptr, nshake + 3 ptrs, ...
-    assert SysDesc.celwal.offset % 8 == 0
-    assert ctypes.sizeof(EnergyLog) == 8 + 3 * 8 + 6 * 8
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "jpgpu.h"', 'int main(void) {']
+    for cname, ct in (("jp_sys_desc", SysDesc), ("jp_energy_log", EnergyLog)):
+        lines.append(f'printf("{cname} sizeof %zu\\n", sizeof({cname}));')
+        for f, _ in ct._fields_:
+            cf = "lambda" if f == "lambda_" else f
+            lines.append(f'printf("{cname} {f} %zu\\n", offsetof({cname}, {cf}));')
+    lines += ["return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = str(tmp_path / "layout")
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", exe], check=True)
+    out = subprocess.run([exe], capture_output=True, text=True, check=True).stdout
+    got = {(a, b): int(c) for a, b, c in (ln.split() for ln in out.splitlines())}
+    for cname, ct in (("jp_sys_desc", SysDesc), ("jp_energy_log", EnergyLog)):
+        assert got[(cname, "sizeof")] == ctypes.sizeof(ct), cname
+        for f, _ in ct._fields_:
+            assert got[(cname, f)] == getattr(ct, f).offset, (cname, f)

@@ -34,7 +34,7 @@ def test_bindc_layout_matches_c():
             got[(parts[0], parts[1])] = int(parts[2])
     assert got[("sizeof", "jp_sys_desc")] == C.sizeof(SysDesc)
     assert got[("sizeof", "jp_energy_log")] == C.sizeof(EnergyLog)
-    for field in ("excl", "shake_tol", "celwal", "lmd"):
+    for field in ("excl", "shake_tol", "celwal", "lmd", "nbond", "angle_cls"):
         assert got[("offset", field)] == getattr(SysDesc, field).offset, field
     assert got[("offset", "lambda")] == SysDesc.lambda_.offset
 
@@ -51,14 +51,33 @@ def test_system_file_roundtrip(tmp_path):
     p = str(tmp_path / "sys.bin")
     refio.write_jpsys(p, s)
     r = refio.read_records(p)
-    assert len(r) == 16
+    assert len(r) == 22
     hdr = np.frombuffer(r[0], np.int32)
+    assert len(hdr) == 12
     n, ntyp, nfrag, nfrg, nexcl, nexcess, nshake = hdr[:7]
     assert n == s.n and nfrag == 2 and nfrg == 3 and hdr[9] == 1
     assert len(r[2]) == 24 * n and np.array_equal(np.frombuffer(r[2], np.float64).reshape(n, 3), s.x)
     assert len(r[10]) == 4 * nfrag * nfrag
     assert len(r[11]) == 8 * nexcl and len(r[12]) == 8 * nexcess
     assert len(r[13]) == 4 * nshake and len(r[14]) == 16 * nshake and len(r[15]) == 48 * nshake
+    nbond, nangle = hdr[10:12]
+    assert len(r[16]) == 8 * nbond and len(r[17]) == 8 * nbond and len(r[18]) == 8 * nbond
+    assert len(r[19]) == 12 * nangle and len(r[20]) == 8 * nangle and len(r[21]) == 8 * nangle
+
+
+def test_system_file_bonded_records(tmp_path):
+    """Bonds and angles md_run keeps after setshk travel in the system file."""
+    from j_presto_amd import systems
+    from j_presto_amd.abi import DescHolder
+    s = systems.chain_in_water(40, 24.6, 23, bond_fc=300.0, angle_fc=50.0)
+    h = DescHolder(s)
+    p = str(tmp_path / "sys.bin")
+    refio.write_jpsys(p, s)
+    r = refio.read_records(p)
+    hdr = np.frombuffer(r[0], np.int32)
+    assert hdr[10] == h.desc.nbond == 29 and hdr[11] == h.desc.nangle == 38   # 39 bonds - 10 X-H, 38 angles
+    assert np.array_equal(np.frombuffer(r[16], np.int32).reshape(-1, 2), h.bond_atoms)
+    assert np.array_equal(np.frombuffer(r[21], np.float64), h.angle_theta0)
 
 
 @pytest.mark.gpu

@@ -13,12 +13,16 @@ def test_forces_and_energies_bit_exact(golden, name):
     s, f = golden(name)
     o = Oracle(s)
     assert o.celprp(s.x) == 0
-    g, ee, ev = o.nonbond(s.x)
+    # dnergy order: bonds and angles, then cmmdo, into one zeroed gradient
+    g, eb, ea = o.bonded(s.x)
+    g, ee, ev = o.nonbond(s.x, g)
     gt = o.combine(g)
     assert np.array_equal(gt, f["force_grad"]), np.abs(gt - f["force_grad"]).max()
-    # dnergy slots: 9 ELE15, 8 VDW15 (main.f90:157-159), unscaled class sums
+    # dnergy slots: 2 BOND, 3 ANGLE, 9 ELE15, 8 VDW15 (main.f90:157-159), unscaled class sums
     assert ee.sum() == f["force_epot"][8]
     assert ev.sum() == f["force_epot"][7]
+    assert eb.sum() == f["force_epot"][1]
+    assert ea.sum() == f["force_epot"][2]
 
 
 @pytest.mark.parametrize("name", GOLDEN_NAMES)
@@ -41,7 +45,8 @@ def test_nve_trajectory_bit_exact(golden, name):
     x, v = s.x.copy(), s.v.copy()
     assert o.celprp(x) == 0
     for idloop in range(1, int(f["traj_nsteps"]) + 1):
-        g, ee, ev = o.nonbond(x)
+        g, eb, ea = o.bonded(x)
+        g, ee, ev = o.nonbond(x, g)
         assert o.verlet_step(x, v, o.combine(g)) == 0
         if idloop % 10 == 0:
             assert o.celprp(x) == 0
