@@ -174,6 +174,21 @@ def equilibrate(md: "MdSystem", nsteps: int, temp: float = 300.0, every: int = 1
     return 0
 
 
+def equilibrate_ensemble(ens: "Ensemble", nsteps: int, temp: float = 300.0, every: int = 10, update: int = 10):
+    """equilibrate() for every system of a batch: batched leap-frog steps,
+    each system's velocities rescaled to `temp` every `every` steps."""
+    from .systems import GASCST
+    for _ in range(max(1, nsteps // every)):
+        _, ier = ens.runmd(every, update=update)
+        if ier:
+            return ier
+        for i, h in enumerate(ens.holders):
+            _, v = ens.get_state(i)
+            t = (h.mass[:, None] * v * v).sum() * 1e7 / (GASCST * (h.system.ndf() - 3))
+            ens.set_state(i, None, v * np.sqrt(temp / t))
+    return 0
+
+
 def log_to_array(log, nl: int) -> np.ndarray:
     dt = np.dtype([("step", "i4"), ("etot", "f8"), ("epot", "f8"), ("ekin", "f8"),
                    ("eelec", "f8", 3), ("evdw", "f8", 3), ("ebond", "f8", 3), ("eangle", "f8", 3)])
@@ -229,6 +244,11 @@ class Ensemble:
         x, v = np.zeros((n, 3)), np.zeros((n, 3))
         native.check("jp_batch_get_state", self.lib.jp_batch_get_state(self.b, i, _f64(x), _f64(v)))
         return x, v
+
+    def set_state(self, i: int, cord=None, vel=None):
+        cord = None if cord is None else np.ascontiguousarray(cord, np.float64)
+        vel = None if vel is None else np.ascontiguousarray(vel, np.float64)
+        native.check("jp_batch_set_state", self.lib.jp_batch_set_state(self.b, i, _f64(cord), _f64(vel)))
 
     def timing(self) -> float:
         t = C.c_double()

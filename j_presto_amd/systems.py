@@ -377,9 +377,12 @@ def water_box(nwat: int, L: float, seed: int, n_ion_pairs: int = 0, temp: float 
 
 
 def _saw_chain(rng, n: int, L: float, bond: float = 1.5, excl: float = 3.0,
-               min_angle_deg: float = 100.0) -> np.ndarray:
+               min_angle_deg: float = 100.0, confine: Optional[float] = None) -> np.ndarray:
     """Self-avoiding random walk in a periodic box: atom k keeps >= excl from
-    every atom k' <= k-3 (minimum image) and bond angles >= min_angle."""
+    every atom k' <= k-3 (minimum image) and bond angles >= min_angle.  With
+    `confine`, every atom stays within that radius of the box centre (a
+    compact, globule-like chain); the radius grows by 2% after every 20
+    backtracks so that a trapped walk always gets out."""
     cell = excl
     nc = max(1, int(L // cell))
     cs = L / nc
@@ -405,6 +408,8 @@ def _saw_chain(rng, n: int, L: float, bond: float = 1.5, excl: float = 3.0,
         return True
 
     cosmax = math.cos(math.radians(min_angle_deg))
+    grid.setdefault(key(pos[0]), []).append(0)   # the first atom excludes like every other
+    nback = 0
     k = 1
     while k < n:
         placed = False
@@ -417,10 +422,15 @@ def _saw_chain(rng, n: int, L: float, bond: float = 1.5, excl: float = 3.0,
                 if -(u @ prev) > cosmax:      # angle(k-2,k-1,k) must exceed min_angle
                     continue
             p = pos[k - 1] + bond * u
+            if confine is not None and np.sum((p - L / 2) ** 2) > confine * confine:
+                continue
             if ok(p, k):
                 placed = True
                 break
         if not placed:                         # backtrack a few atoms
+            nback += 1
+            if confine is not None and nback % 20 == 0:
+                confine *= 1.02
             back = min(k - 1, 5)
             for j in range(k - back, k):
                 grid[key(pos[j])].remove(j)
@@ -429,7 +439,6 @@ def _saw_chain(rng, n: int, L: float, bond: float = 1.5, excl: float = 3.0,
         pos[k] = p
         grid.setdefault(key(p), []).append(k)
         k += 1
-    grid.setdefault(key(pos[0]), []).append(0)
     return pos
 
 
@@ -492,14 +501,18 @@ def chain_template(rng, n: int, type0: int, ntypes: int, cluster: int = 1,
 def chain_in_water(n_chain: int, L: float, seed: int, n_chain_types: int = 16, temp: float = 300.0,
                    lmd: Optional[float] = None, name: str = "chain", bond_fc: float = 0.0,
                    angle_fc: float = 0.0, bond_r0: Optional[float] = None,
-                   angle_deg: Optional[float] = None) -> System:
+                   angle_deg: Optional[float] = None, compact: bool = False) -> System:
     """Solvated chain: SAW chain in a periodic water box; waters
     with any atom within 2.6 A of a chain atom are removed.  With `lmd`, the
     chain is cluster class 1 and water class 2 under CLSTMETHOD=LMD (three
     energy classes: chain-chain lambda^2, chain-water lambda, water-water 1)."""
     rng = np.random.Generator(np.random.PCG64(seed))
     Lv = np.array([L, L, L], np.float64)
-    cpos = _saw_chain(rng, n_chain, L)
+    if compact:
+        # globule of ~23 A^3 per chain atom (non-neighbours >= 2.6 A apart)
+        cpos = _saw_chain(rng, n_chain, L, excl=2.6, confine=(n_chain * 23.0 * 3 / (4 * math.pi)) ** (1 / 3))
+    else:
+        cpos = _saw_chain(rng, n_chain, L)
     # bond lengths of the walk are exactly 1.5 A, so SHAKE targets are met
     nsite = int(round(WATER_PER_A3 * L ** 3))
     sites = _lattice_sites(nsite, L)
@@ -531,11 +544,15 @@ def chain_in_water(n_chain: int, L: float, seed: int, n_chain_types: int = 16, t
 # the benchmark workloads
 # ---------------------------------------------------------------------------
 
+# harmonic constants of the benchmark chains (kcal/mol/A^2, kcal/mol/rad^2),
+# AMBER-like C-C values; equilibrium values are the generated geometry's
+CHAIN_BOND_FC, CHAIN_ANGLE_FC = 300.0, 50.0
+
 WORKLOADS = {
     0: "water-6k: 2,000 TIP3P waters, 39.1 A box",
-    1: "chain-24k: 2,500-atom chain in water, 64 A box",
+    1: "chain-24k: 2,500-atom chain (harmonic bonds/angles) in water, 64 A box",
     2: "water-100k: 33,333 waters + 60 Na/Cl pairs, 100.0 A box",
-    3: "ensemble-64: 64 chain-in-water systems, 3k-30k atoms each",
+    3: "ensemble-64: 64 chain-in-water systems (harmonic bonds/angles), 3k-30k atoms each",
     4: "water-1M: 333,333 waters, 216 A box",
 }
 
@@ -545,7 +562,8 @@ def workload(i: int) -> System:
     if i == 0:
         return water_box(2000, 39.1, seed, name="water-6k")
     if i == 1:
-        return chain_in_water(2500, 64.0, seed, name="chain-24k")
+        return chain_in_water(2500, 64.0, seed, name="chain-24k", bond_fc=CHAIN_BOND_FC, angle_fc=CHAIN_ANGLE_FC,
+                              compact=True)
     if i == 2:
         L = (33333 / WATER_PER_A3) ** (1.0 / 3.0)
         return water_box(33333, round(L, 3), seed, n_ion_pairs=60, name="water-100k")
@@ -566,5 +584,6 @@ def ensemble(nsys: int = 64, seed: int = 1003, lo: int = 3000, hi: int = 30000) 
         L = (ns / (3 * WATER_PER_A3)) ** (1.0 / 3.0)
         L = max(L, 24.5)
         nch = max(8, int(round(0.104 * ns)) // 4 * 4)
-        out.append(chain_in_water(nch, round(L, 3), seed * 1000 + k, lmd=float(lam), name=f"ens{k:02d}"))
+        out.append(chain_in_water(nch, round(L, 3), seed * 1000 + k, lmd=float(lam), name=f"ens{k:02d}",
+                                  bond_fc=CHAIN_BOND_FC, angle_fc=CHAIN_ANGLE_FC, compact=True))
     return out
