@@ -509,8 +509,8 @@ def chain_in_water(n_chain: int, L: float, seed: int, n_chain_types: int = 16, t
     rng = np.random.Generator(np.random.PCG64(seed))
     Lv = np.array([L, L, L], np.float64)
     if compact:
-        # globule of ~23 A^3 per chain atom (non-neighbours >= 2.6 A apart)
-        cpos = _saw_chain(rng, n_chain, L, excl=2.6, confine=(n_chain * 23.0 * 3 / (4 * math.pi)) ** (1 / 3))
+        # globule of ~23 A^3 per chain atom to start with (non-neighbours >= 3.0 A apart)
+        cpos = _saw_chain(rng, n_chain, L, excl=3.0, confine=(n_chain * 23.0 * 3 / (4 * math.pi)) ** (1 / 3))
     else:
         cpos = _saw_chain(rng, n_chain, L)
     # bond lengths of the walk are exactly 1.5 A, so SHAKE targets are met
