@@ -80,9 +80,10 @@ struct BondedArgs {
 // (efac[3 * system + class]).  gout == nullptr: added to the atom's slot of
 // gslot (after the pair kernel wrote it); else stored to gout in original
 // order (gout pre-zeroed by the caller).
-__global__ void k_bonded(BondedArgs B, const int *__restrict__ atom_sys, const SysP *__restrict__ sp,
-                         const double *__restrict__ x, const double *__restrict__ efac, const int *__restrict__ inum,
-                         double *__restrict__ gslot, double *__restrict__ gout) {
+__global__ void __launch_bounds__(128)
+    k_bonded(BondedArgs B, const int *__restrict__ atom_sys, const SysP *__restrict__ sp, const double *__restrict__ x,
+             const double *__restrict__ efac, const int *__restrict__ inum, double *__restrict__ gslot,
+             double *__restrict__ gout) {
 #pragma clang fp contract(off)
     const int u = blockIdx.x * blockDim.x + threadIdx.x;
     if (u >= B.nbat) return;

@@ -364,60 +364,50 @@ __device__ __forceinline__ void mimg3(const SysP &P, double *d) {
 }
 
 // shak2e_PB, shake.f90:687-755 (independent 2-atom groups, exact solution)
-__global__ void k_shake2(int ng, const int *__restrict__ gsys, const SysP *__restrict__ sp,
-                         const int4 *__restrict__ ga, const double *__restrict__ gd, const double *__restrict__ mass,
-                         double *__restrict__ r1, const double *__restrict__ r2) {
+// shak2e_PB (shake.f90:687-755) on registers: r1 = the pair at t+dt (in/out),
+// r2 = at t.  Exact solution of the single constraint.
+__device__ __forceinline__ void shake2_regs(const SysP &P, double im1, double im2, double d2target,
+                                            double (*r1)[3], const double (*r2)[3]) {
 #pragma clang fp contract(off)
-    int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= ng) return;
-    const SysP &P = sp[gsys[g]];
-    const int ia = ga[g].x, ja = ga[g].y;
-    const double im1 = 1.0 / mass[ia], im2 = 1.0 / mass[ja];
     const double im3 = im1 + im2, im4 = im3 * im3;
     double d1[3], d2[3];
     for (int k = 0; k < 3; ++k) {
-        d1[k] = r1[3 * ja + k] - r1[3 * ia + k];
-        d2[k] = r2[3 * ja + k] - r2[3 * ia + k];
+        d1[k] = r1[1][k] - r1[0][k];
+        d2[k] = r2[1][k] - r2[0][k];
     }
     mimg3(P, d1);
     mimg3(P, d2);
     double d1d1 = d1[0] * d1[0] + d1[1] * d1[1] + d1[2] * d1[2];
     double d1d2 = d1[0] * d2[0] + d1[1] * d2[1] + d1[2] * d2[2];
     double d2d2 = d2[0] * d2[0] + d2[1] * d2[1] + d2[2] * d2[2];
-    double A = im4 * d2d2, B = im3 * d1d2, C = d1d1 - gd[6 * g];
+    double A = im4 * d2d2, B = im3 * d1d2, C = d1d1 - d2target;
     double G = (B - sqrt(B * B - C * A)) / A;
     for (int k = 0; k < 3; ++k) {
-        r1[3 * ia + k] = r1[3 * ia + k] + (im1 * G * d2[k]);
-        r1[3 * ja + k] = r1[3 * ja + k] - (im2 * G * d2[k]);
+        r1[0][k] = r1[0][k] + (im1 * G * d2[k]);
+        r1[1][k] = r1[1][k] - (im2 * G * d2[k]);
     }
 }
 
-// shake3_PB, shake.f90:112-327 (3-atom groups, Newton with exact 3x3 inverse)
-__global__ void k_shake3(int ng, const int *__restrict__ gsys, const SysP *__restrict__ sp,
-                         const int4 *__restrict__ ga, const double *__restrict__ gd,
-                         const double *__restrict__ imass, double *__restrict__ r1, const double *__restrict__ r2,
-                         double tol, int maxit, int *__restrict__ err) {
+// shake3_PB (shake.f90:113-327) on registers: constraints (1,2), (2,3), (3,1),
+// Newton iterations on the 3x3 system through its adjugate.  Returns 0, 4
+// (not converged after maxit) or 8 (singular).
+__device__ __forceinline__ int shake3_regs(const SysP &P, const double *im, const double *T, double tol, int maxit,
+                                           double (*r1)[3], const double (*r2)[3]) {
 #pragma clang fp contract(off)
-    int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= ng) return;
-    const SysP &P = sp[gsys[g]];
-    const int a[3] = {ga[g].x, ga[g].y, ga[g].z};
-    const double im[3] = {imass[a[0]], imass[a[1]], imass[a[2]]};
     double co[3][3];
     co[0][0] = im[0] + im[1]; co[0][1] = -im[1]; co[0][2] = -im[0];
     co[1][0] = -im[1]; co[1][1] = im[1] + im[2]; co[1][2] = -im[2];
     co[2][0] = -im[0]; co[2][1] = -im[2]; co[2][2] = im[2] + im[0];
-    const double T[3] = {gd[6 * g], gd[6 * g + 1], gd[6 * g + 2]};
     const double TT[3] = {tol * T[0], tol * T[1], tol * T[2]};
     double G[6] = {0, 0, 0, 0, 0, 0};
     double d1[3][3], d2[3][3];
     for (int k = 0; k < 3; ++k) {
-        d1[0][k] = r1[3 * a[0] + k] - r1[3 * a[1] + k];
-        d1[1][k] = r1[3 * a[1] + k] - r1[3 * a[2] + k];
-        d1[2][k] = r1[3 * a[2] + k] - r1[3 * a[0] + k];
-        d2[0][k] = r2[3 * a[0] + k] - r2[3 * a[1] + k];
-        d2[1][k] = r2[3 * a[1] + k] - r2[3 * a[2] + k];
-        d2[2][k] = r2[3 * a[2] + k] - r2[3 * a[0] + k];
+        d1[0][k] = r1[0][k] - r1[1][k];
+        d1[1][k] = r1[1][k] - r1[2][k];
+        d1[2][k] = r1[2][k] - r1[0][k];
+        d2[0][k] = r2[0][k] - r2[1][k];
+        d2[1][k] = r2[1][k] - r2[2][k];
+        d2[2][k] = r2[2][k] - r2[0][k];
     }
     for (int q = 0; q < 3; ++q) { mimg3(P, d1[q]); mimg3(P, d2[q]); }
     for (int it = 0; it < maxit; ++it) {
@@ -425,11 +415,11 @@ __global__ void k_shake3(int ng, const int *__restrict__ gsys, const SysP *__res
         for (int q = 0; q < 3; ++q) B[q] = T[q] - (d1[q][0] * d1[q][0] + d1[q][1] * d1[q][1] + d1[q][2] * d1[q][2]);
         if (fabs(B[0]) < TT[0] && fabs(B[1]) < TT[1] && fabs(B[2]) < TT[2]) {
             for (int k = 0; k < 3; ++k) {
-                r1[3 * a[0] + k] = r1[3 * a[0] + k] + im[0] * (G[3] * d2[0][k] - G[5] * d2[2][k]);
-                r1[3 * a[1] + k] = r1[3 * a[1] + k] + im[1] * (-G[3] * d2[0][k] + G[4] * d2[1][k]);
-                r1[3 * a[2] + k] = r1[3 * a[2] + k] + im[2] * (-G[4] * d2[1][k] + G[5] * d2[2][k]);
+                r1[0][k] = r1[0][k] + im[0] * (G[3] * d2[0][k] - G[5] * d2[2][k]);
+                r1[1][k] = r1[1][k] + im[1] * (-G[3] * d2[0][k] + G[4] * d2[1][k]);
+                r1[2][k] = r1[2][k] + im[2] * (-G[4] * d2[1][k] + G[5] * d2[2][k]);
             }
-            return;
+            return 0;
         }
         double gf[3][3];
         for (int ig = 0; ig < 3; ++ig)
@@ -438,7 +428,7 @@ __global__ void k_shake3(int ng, const int *__restrict__ gsys, const SysP *__res
         double deta = (gf[0][0] * gf[1][1] * gf[2][2]) + (gf[0][1] * gf[1][2] * gf[2][0]) +
                       (gf[0][2] * gf[1][0] * gf[2][1]) - (gf[0][0] * gf[1][2] * gf[2][1]) -
                       (gf[0][1] * gf[1][0] * gf[2][2]) - (gf[0][2] * gf[1][1] * gf[2][0]);
-        if (deta == 0.0) { atomicOr(err, 8); return; }
+        if (deta == 0.0) return 8;
         deta = 1.0 / deta;
         double db[3][3];
         db[0][0] = gf[1][1] * gf[2][2] - gf[1][2] * gf[2][1];
@@ -458,11 +448,106 @@ __global__ void k_shake3(int ng, const int *__restrict__ gsys, const SysP *__res
                 d1[q][k] = d1[q][k] + co[q][0] * d2[0][k] * G[0] + co[q][1] * d2[1][k] * G[1] +
                            co[q][2] * d2[2][k] * G[2];
     }
-    atomicOr(err, 4);  // not converged
+    return 4;
 }
 
-// v = (x - xold)/dt after SHAKE (verlet.f90:203-211) and the per-step refresh
-// of the sorted pair-kernel coordinates (enear.f90:72-76 re-sorts every call).
+__global__ void __launch_bounds__(128) k_shake2(int ng, const int *__restrict__ gsys, const SysP *__restrict__ sp,
+                         const int4 *__restrict__ ga, const double *__restrict__ gd, const double *__restrict__ mass,
+                         double *__restrict__ r1, const double *__restrict__ r2) {
+    int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ng) return;
+    const SysP &P = sp[gsys[g]];
+    const int a[2] = {ga[g].x, ga[g].y};
+    double x1[2][3], x2[2][3];
+    for (int q = 0; q < 2; ++q)
+        for (int k = 0; k < 3; ++k) { x1[q][k] = r1[3 * a[q] + k]; x2[q][k] = r2[3 * a[q] + k]; }
+    shake2_regs(P, 1.0 / mass[a[0]], 1.0 / mass[a[1]], gd[6 * g], x1, x2);
+    for (int q = 0; q < 2; ++q)
+        for (int k = 0; k < 3; ++k) r1[3 * a[q] + k] = x1[q][k];
+}
+
+__global__ void __launch_bounds__(128) k_shake3(int ng, const int *__restrict__ gsys, const SysP *__restrict__ sp,
+                         const int4 *__restrict__ ga, const double *__restrict__ gd,
+                         const double *__restrict__ imass, double *__restrict__ r1, const double *__restrict__ r2,
+                         double tol, int maxit, int *__restrict__ err) {
+    int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ng) return;
+    const SysP &P = sp[gsys[g]];
+    const int a[3] = {ga[g].x, ga[g].y, ga[g].z};
+    const double im[3] = {imass[a[0]], imass[a[1]], imass[a[2]]};
+    const double T[3] = {gd[6 * g], gd[6 * g + 1], gd[6 * g + 2]};
+    double x1[3][3], x2[3][3];
+    for (int q = 0; q < 3; ++q)
+        for (int k = 0; k < 3; ++k) { x1[q][k] = r1[3 * a[q] + k]; x2[q][k] = r2[3 * a[q] + k]; }
+    const int e = shake3_regs(P, im, T, tol, maxit, x1, x2);
+    if (e) { atomicOr(err, e); return; }
+    for (int q = 0; q < 3; ++q)
+        for (int k = 0; k < 3; ++k) r1[3 * a[q] + k] = x1[q][k];
+}
+
+// One MD step after the forces, fused per "unit" (a 3-atom SHAKE group, a
+// 2-atom group or a free atom): leap-frog kick and drift (verlet.f90:144-190),
+// SHAKE on the unit's atoms in registers, v = (x - x_old)/dt when the run
+// has constraints (verlet.f90:196-215), and the refreshed sorted coordinates.
+// The same operations in the same order as k_kick -> k_shake* -> k_post.
+// unit = {a0, a1, a2, kind | param << 2}, kind 1/2/3 = atoms in the unit.
+__global__ void __launch_bounds__(128) k_step(int nunit, const int4 *__restrict__ unit, const int *__restrict__ atom_sys,
+                       const SysP *__restrict__ sp, const int *__restrict__ inum, const int *__restrict__ img,
+                       const double *__restrict__ gslot, const double *__restrict__ mass,
+                       const double *__restrict__ imass, const double *__restrict__ s2d,
+                       const double *__restrict__ s3d, double tol, int maxit, int shake_on, double *__restrict__ x,
+                       double *__restrict__ v, double *__restrict__ vback, JAtom *__restrict__ ys,
+                       int *__restrict__ err) {
+#pragma clang fp contract(off)
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= nunit) return;
+    const int4 U = unit[u];
+    const int kind = U.w & 3, prm = U.w >> 2;
+    const int a[3] = {U.x, U.y, U.z};
+    const SysP &P = sp[atom_sys[a[0]]];
+    const double rt = -P.dt * 4.184e-4;   // md.f90:56 fdcvcv
+    double xn[3][3], xo[3][3], vv[3][3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {   // fixed trip count: the arrays stay in registers
+        if (q >= kind) break;
+        const int b = a[q];
+        const size_t sl = inum[b];
+        const double m = mass[b];
+        for (int k = 0; k < 3; ++k) {
+            const double g = gslot[3 * sl + k];
+            double vk = v[3 * b + k];
+            if (vback) vback[3 * b + k] = vk;
+            vk = vk + rt * g / m;
+            const double p = x[3 * b + k];
+            xo[q][k] = p;
+            xn[q][k] = p + P.dt * vk;
+            vv[q][k] = vk;
+        }
+    }
+    if (kind == 3) {
+        const double im[3] = {imass[a[0]], imass[a[1]], imass[a[2]]};
+        const double T[3] = {s3d[6 * prm], s3d[6 * prm + 1], s3d[6 * prm + 2]};
+        const int e = shake3_regs(P, im, T, tol, maxit, xn, xo);
+        if (e) atomicOr(err, e);
+    } else if (kind == 2) {
+        shake2_regs(P, 1.0 / mass[a[0]], 1.0 / mass[a[1]], s2d[6 * prm], xn, xo);
+    }
+    const double r = 1.0 / P.dt;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        if (q >= kind) break;
+        const int b = a[q];
+        JAtom *j = ys + inum[b];
+        for (int k = 0; k < 3; ++k) {
+            x[3 * b + k] = xn[q][k];
+            v[3 * b + k] = shake_on ? (xn[q][k] - xo[q][k]) * r : vv[q][k];
+        }
+        j->x = xn[q][0] - P.L[0] * (double)img[3 * b];
+        j->y = xn[q][1] - P.L[1] * (double)img[3 * b + 1];
+        j->z = xn[q][2] - P.L[2] * (double)img[3 * b + 2];
+    }
+}
+
 __global__ void k_post(int natom, const int *__restrict__ atom_sys, const SysP *__restrict__ sp,
                        const int *__restrict__ inum, const int *__restrict__ img, const double *__restrict__ x,
                        const double *__restrict__ xold, double *__restrict__ v, JAtom *__restrict__ ys,

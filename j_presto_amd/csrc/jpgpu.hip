@@ -114,6 +114,8 @@ struct Engine {
     DBuf<unsigned long long> d_u64;
     // bonded terms (jp_bonded.hpp): bonds, angles, per-atom gather lists
     int nbt = 0, nat = 0, nbat = 0;
+    int nunit = 0;             // k_step units: SHAKE groups and free atoms
+    DBuf<int4> d_unit;
     std::vector<int> tsys_h, tcls_h;   // system and class of every term (bonds, then angles)
     DBuf<int> d_bat, d_bptr, d_bref;
     DBuf<int4> d_bnd, d_ang;
@@ -140,7 +142,7 @@ struct Engine {
         d_s2sys.release(); d_s3sys.release(); d_s2a.release(); d_s3a.release(); d_s2d.release(); d_s3d.release();
         d_u64.release();
         d_bat.release(); d_bptr.release(); d_bref.release(); d_bnd.release(); d_ang.release();
-        d_bndp.release(); d_angp.release(); d_eterm.release();
+        d_bndp.release(); d_angp.release(); d_eterm.release(); d_unit.release();
     }
 
     int init(const jp_sys_desc *descs, int ns);
@@ -458,6 +460,30 @@ int Engine::init(const jp_sys_desc *descs, int ns) {
         bptr.push_back((int)bref.size());
     }
     nbat = (int)bat.size();
+    // k_step units in atom order: each SHAKE group once (at its first atom), free atoms alone
+    std::vector<int4> units;
+    {
+        std::vector<int> owner(natom, -1);   // unit code per group atom
+        std::vector<int4> gu;
+        for (int g = 0; g < (int)s2a.size(); ++g) gu.push_back(make_int4(s2a[g].x, s2a[g].y, -1, 2 | (g << 2)));
+        for (int g = 0; g < (int)s3a.size(); ++g) gu.push_back(make_int4(s3a[g].x, s3a[g].y, s3a[g].z, 3 | (g << 2)));
+        for (int q = 0; q < (int)gu.size(); ++q) {
+            const int m[3] = {gu[q].x, gu[q].y, gu[q].z};
+            for (int k = 0; k < (gu[q].w & 3); ++k) {
+                if (owner[m[k]] >= 0) return JP_EINVAL;   // an atom in two SHAKE groups (shake.f90:71-73)
+                owner[m[k]] = q;
+            }
+        }
+        for (int a = 0; a < natom; ++a) {
+            if (owner[a] < 0) {
+                units.push_back(make_int4(a, -1, -1, 1));
+            } else {
+                const int4 q = gu[owner[a]];
+                if (std::min(q.x, std::min(q.y, (q.w & 3) == 3 ? q.z : q.y)) == a) units.push_back(q);
+            }
+        }
+        nunit = (int)units.size();
+    }
     n2 = (int)s2a.size(); n3 = (int)s3a.size();
     mass_h = mass;
     set_factors(-1);
@@ -483,6 +509,7 @@ int Engine::init(const jp_sys_desc *descs, int ns) {
     AL(d_bat, bat.size()); AL(d_bptr, bptr.size()); AL(d_bref, bref.size());
     AL(d_bnd, bnd.size()); AL(d_bndp, bndp.size()); AL(d_ang, ang.size()); AL(d_angp, angp.size());
     AL(d_eterm, (size_t)nbt + nat);
+    AL(d_unit, units.size());
     auto H2D = [&](void *dst, const void *src, size_t bytes) -> int {
         if (bytes) CK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st));
         return 0;
@@ -520,6 +547,7 @@ int Engine::init(const jp_sys_desc *descs, int ns) {
     r |= H2D(d_bndp.p, bndp.data(), 16 * bndp.size());
     r |= H2D(d_ang.p, ang.data(), 16 * ang.size());
     r |= H2D(d_angp.p, angp.data(), 16 * angp.size());
+    r |= H2D(d_unit.p, units.data(), 16 * units.size());
     if (r) return r;
     CK(hipMemsetAsync(d_v.p, 0, 8 * 3 * (size_t)natom, st));
     CK(hipStreamSynchronize(st));
@@ -805,12 +833,9 @@ int Engine::shake(int &ier_out) {
 
 int Engine::integrate(int prec, bool mon) {
     (void)prec;
-    hipLaunchKernelGGL(k_kick, grid1(natom, 256), 256, 0, st, natom, d_atom_sys.p, d_sp.p, d_inum.p, d_gslot.p,
-                       d_mass.p, d_x.p, d_v.p, d_xold.p, mon ? d_vback.p : (double *)nullptr, (double *)nullptr, 1);
-    int ier = 0, r;
-    if ((r = shake(ier))) return r;
-    hipLaunchKernelGGL(k_post, grid1(natom, 256), 256, 0, st, natom, d_atom_sys.p, d_sp.p, d_inum.p, d_img.p, d_x.p,
-                       d_xold.p, d_v.p, d_ys.p, (n2 + n3) > 0 ? 1 : 0);
+    hipLaunchKernelGGL(k_step, grid1(nunit, 128), 128, 0, st, nunit, d_unit.p, d_atom_sys.p, d_sp.p, d_inum.p,
+                       d_img.p, d_gslot.p, d_mass.p, d_imass.p, d_s2d.p, d_s3d.p, shake_tol, shake_maxit,
+                       (n2 + n3) > 0 ? 1 : 0, d_x.p, d_v.p, mon ? d_vback.p : (double *)nullptr, d_ys.p, d_err.p);
     CK(hipGetLastError());
     return 0;
 }
