@@ -261,12 +261,26 @@ k_list8(int nclu, const int *__restrict__ clu_sys, const SysP *__restrict__ sp, 
         moli[k] = __builtin_amdgcn_readlane(mi, k);
     }
     const uint32_t ivalid = (uint32_t)__ballot(l < JP_CL && ai >= 0);
-    int nbv = 0;
+    // listed partners per i atom: per-lane byte counters (atom k in byte k&3 of
+    // cnt[k>>2]), folded into nbv (lane k holds atom k's count) every 255 drains
+    int nbv = 0, ndr = 0;
+    uint32_t cnt0 = 0, cnt1 = 0;
+    auto fold = [&]() {
+#pragma unroll
+        for (int k = 0; k < JP_CL; ++k) {
+            const int c = wave_sum((int)(((k < 4 ? cnt0 : cnt1) >> (8 * (k & 3))) & 255u));
+            nbv += (l == k) ? c : 0;
+        }
+        cnt0 = cnt1 = 0;
+        ndr = 0;
+    };
     wave_lds_sync();
 
     const size_t e0 = (size_t)ci * ccap;
     const int cap = ccap;
     const float rlo = (float)(P.rlim2 - P.band), rhi = (float)(P.rlim2 + P.band);
+    // any d2 in [rlo, rhi] has |d2 - rlim2| <= band + rounding: a looser trigger
+    const float rl2f = (float)P.rlim2, trig = (float)P.band + 0.01f;
     const float Lx = (float)P.L[0], Ly = (float)P.L[1], Lz = (float)P.L[2];
     const bool small = P.small != 0;
     int nA = 0, nB = 0, nrem = 0, qh = 0, qt = 0;
@@ -290,22 +304,37 @@ k_list8(int nclu, const int *__restrict__ clu_sys, const SysP *__restrict__ sp, 
             meta = __float_as_int(q.w);
         }
         const int molj = meta & 0x3fffffff;
+        const float rx = (q.x - cf.x) + Lx * (float)sx;
+        const float ry = (q.y - cf.y) + Ly * (float)sy;
+        const float rz = (q.z - cf.z) + Lz * (float)sz;
+        float amin = 3.0e38f;
         if (meta >= 0) {
-            const float rx = (q.x - cf.x) + Lx * (float)sx;
-            const float ry = (q.y - cf.y) + Ly * (float)sy;
-            const float rz = (q.z - cf.z) + Lz * (float)sz;
+            // bit k = [d2 < rlo]: the sign of d2 - rlo shifted in, atom 7 first
 #pragma unroll
-            for (int k = 0; k < JP_CL; ++k) {
+            for (int k = JP_CL - 1; k >= 0; --k) {
                 const float dx = xi[k] - rx, dy = yi[k] - ry, dz = zi[k] - rz;
                 const float d2 = dx * dx + dy * dy + dz * dz;
-                m |= (d2 < rlo ? 1u : 0u) << k;
-                amb |= ((d2 >= rlo) && (d2 <= rhi) ? 1u : 0u) << k;
+                m = __builtin_amdgcn_alignbit(m, __float_as_uint(d2 - rlo), 31);
+                amin = fminf(amin, fabsf(d2 - rl2f));
             }
-            if (small) { m = 0; amb = 0xffu; }
             m &= ivalid;
-            amb &= ivalid;
         }
-        if (__ballot(amb != 0)) {
+        // pairs near rlim (rare) or in a small box: md_run's FP64 test decides
+        if (__ballot(meta >= 0 && (small || amin <= trig))) {
+            if (meta >= 0) {
+                if (small) {
+                    m = 0;
+                    amb = 0xffu;
+                } else {
+#pragma unroll
+                    for (int k = 0; k < JP_CL; ++k) {
+                        const float dx = xi[k] - rx, dy = yi[k] - ry, dz = zi[k] - rz;
+                        const float d2 = dx * dx + dy * dy + dz * dz;
+                        amb |= ((d2 >= rlo) && (d2 <= rhi) ? 1u : 0u) << k;
+                    }
+                }
+                amb &= ivalid;
+            }
             if (amb) {
                 const double4 xj = xr[js];
                 const int4 ij = imgs[js];
@@ -316,9 +345,11 @@ k_list8(int nclu, const int *__restrict__ clu_sys, const SysP *__restrict__ sp, 
             }
         }
         if (jc == ci) m &= ~(1u << ja);   // j == i
+        // bit k = [molj == moli[k]]: (molj ^ moli[k]) - 1 is negative only for
+        // equal ids (ids < 2^30)
         uint32_t same = 0;
 #pragma unroll
-        for (int k = 0; k < JP_CL; ++k) same |= (molj == moli[k] ? 1u : 0u) << k;
+        for (int k = JP_CL - 1; k >= 0; --k) same = __builtin_amdgcn_alignbit(same, (uint32_t)(molj ^ moli[k]) - 1u, 31);
         same &= m;
         if (__ballot(same != 0)) {
             const int aj = same ? imgs[js].w : -1;
@@ -332,11 +363,10 @@ k_list8(int nclu, const int *__restrict__ clu_sys, const SysP *__restrict__ sp, 
                 }
             }
         }
-#pragma unroll
-        for (int k = 0; k < JP_CL; ++k) {
-            const int c = __popcll(__ballot((m >> k) & 1u));
-            nbv += (l == k) ? c : 0;
-        }
+        // bits 0-3 and 4-7 of m spread to bytes: n * 0x204081 puts bit b at 7b + b
+        cnt0 += ((m & 15u) * 0x204081u) & 0x01010101u;
+        cnt1 += ((m >> 4) * 0x204081u) & 0x01010101u;
+        if (++ndr == 255) fold();
         const bool has = m != 0, lj = (meta >> 30) & 1;
         const uint64_t bA = __ballot(has && lj), bB = __ballot(has && !lj);
         const int na = __popcll(bA), nb = __popcll(bB);
@@ -405,6 +435,7 @@ k_list8(int nclu, const int *__restrict__ clu_sys, const SysP *__restrict__ sp, 
         if (ovf) atomicOr(lst, JP_ERR_LISTCAP);
         if (nA + nB > hiwater) atomicMax(lst + 1, nA + nB);
     }
+    fold();
     if (l < JP_CL && ai >= 0 && nbv > P.nblim) atomicOr(lst, JP_ERR_NBLIM);
     const int nr = wave_sum(nrem);
     if (l == 0 && nr) atomicAdd(lst + 2 + s, nr);

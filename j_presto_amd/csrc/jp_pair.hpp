@@ -99,10 +99,15 @@ k_pair_f32(int nclu, const int *__restrict__ clu_sys, const SysP *__restrict__ s
     __syncthreads();
 
     float fx = 0.f, fy = 0.f, fz = 0.f;
+    float hx = 0.f, hy = 0.f, hz = 0.f;   // chunks without LJ (force steps): sum of q_j c d, times q_i at the flush
     long long ax = 0, ay = 0, az = 0;
     float ee[3] = {0.f, 0.f, 0.f}, ev[3] = {0.f, 0.f, 0.f};
     long long eacc[6] = {0, 0, 0, 0, 0, 0};
     auto flush = [&]() {
+        if (!ENERGY) {
+            fx += qi * hx; fy += qi * hy; fz += qi * hz;
+            hx = hy = hz = 0.f;
+        }
         ax += fix_f(fx, JP_FSCALE); ay += fix_f(fy, JP_FSCALE); az += fix_f(fz, JP_FSCALE);
         fx = fy = fz = 0.f;
         if (ENERGY)
@@ -122,11 +127,21 @@ k_pair_f32(int nclu, const int *__restrict__ clu_sys, const SysP *__restrict__ s
                 dy = dy - Ly * rintf(dy * iLy);
                 dz = dz - Lz * rintf(dz * iLz);
             }
-            float r2 = dx * dx + dy * dy + dz * dz;
+            const float r2 = dx * dx + dy * dy + dz * dz;
             const bool in = ((mt >> ia) & 1) && (r2 <= rc2);
-            r2 = in ? r2 : 1.0f;
+            // pairs not in the list or beyond rc (r2 may be 0 there) are removed by
+            // the selects below, never by arithmetic
             const float ri = __builtin_amdgcn_rsqf(r2);
             const float ri2 = ri * ri;
+            if (!LJ && !ENERGY) {
+                float cq = pj.w * (ri * ri2 - fcf);
+                if (LMD) cq *= ((mt >> 20) & 1) ? f1 : f0;
+                cq = in ? cq : 0.f;
+                hx += cq * dx;
+                hy += cq * dy;
+                hz += cq * dz;
+                continue;
+            }
             const float qq = qi * pj.w;
             float cf = qq * (ri * ri2 - fcf);
             float vv = 0.f;
