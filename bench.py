@@ -70,13 +70,17 @@ def equilibrated(system, nsteps: int):
     return state
 
 
-def run_gpu(system, precision, steps, warmup, state=None, pair_steps=200):
-    from j_presto_amd import engine
+def run_gpu(system, precision, steps, warmup, state=None, pair_steps=200, autotune=True):
+    from j_presto_amd import engine, native
     md = engine.MdSystem(system, dt=DT_FS, precision=precision)
     if state is not None:
         md.set_state(*state)
     ier = md.celprp()
     assert ier == 0, ier
+    if autotune:   # launch shapes timed by a zero-step run, before the warm-up
+        md.set_option(native.JP_OPT_AUTOTUNE, 1)
+        _, ier = md.runmd(0, update=UPDATE)
+        assert ier == 0, ier
     if warmup:
         _, ier = md.runmd(warmup, update=UPDATE)
         assert ier == 0, ier
@@ -93,8 +97,10 @@ def run_gpu(system, precision, steps, warmup, state=None, pair_steps=200):
     _, ier = md.runmd(pair_steps, update=UPDATE)
     assert ier == 0, ier
     tp = md.timing()["pair"] / pair_steps
+    shapes = {"pair_waves": md.get_option(native.JP_OPT_PAIR_WAVES),
+              "list_waves": md.get_option(native.JP_OPT_LIST_WAVES), "autotuned": bool(autotune)}
     md.close()
-    return dict(wall=t1 - t0, timing=tm, pairs=n, pairs_lj=nlj, listed=nlisted, log=log, t_pair=tp)
+    return dict(wall=t1 - t0, timing=tm, pairs=n, pairs_lj=nlj, listed=nlisted, log=log, t_pair=tp, shapes=shapes)
 
 
 def cpu_reference(system, nsteps: int):
@@ -201,6 +207,12 @@ def run_ensemble(steps: int, warmup: int, equil: int):
         states = [ens.get_state(i) for i in range(len(syss))]
         if dt != DT_FS:
             ens.close()
+    from j_presto_amd import native
+    ens.set_option(native.JP_OPT_AUTOTUNE, 1)
+    _, ier = ens.runmd(0, update=UPDATE)
+    assert ier == 0, ier
+    shapes = {"pair_waves": ens.get_option(native.JP_OPT_PAIR_WAVES),
+              "list_waves": ens.get_option(native.JP_OPT_LIST_WAVES), "autotuned": True}
     _, ier = ens.runmd(warmup, update=UPDATE)
     assert ier == 0, ier
     t0 = time.perf_counter()
@@ -213,6 +225,9 @@ def run_ensemble(steps: int, warmup: int, equil: int):
         md = engine.MdSystem(s, dt=DT_FS, precision=engine.FP32)
         md.set_state(*st)
         assert md.celprp() == 0
+        md.set_option(native.JP_OPT_AUTOTUNE, 1)   # each sequential run tuned for its own size
+        _, ier = md.runmd(0, update=UPDATE)
+        assert ier == 0, ier
         _, ier = md.runmd(warmup, update=UPDATE)
         assert ier == 0, ier
         t0 = time.perf_counter()
@@ -229,7 +244,7 @@ def run_ensemble(steps: int, warmup: int, equil: int):
                                         "sizes_min_max": [min(s.n for s in syss), max(s.n for s in syss)],
                                         "precision": "FP32 pair math, int64 fixed-point accumulation"},
         "batched_wall_s": round(wb, 4), "sequential_wall_s": round(ws, 4),
-        "sequential_ns_per_day": round(agg_s, 3), "batch_speedup": round(ws / wb, 3),
+        "sequential_ns_per_day": round(agg_s, 3), "batch_speedup": round(ws / wb, 3), "launch_shapes": shapes,
         "dt_fs": DT_FS, "equilibration_steps": equil}), flush=True)
 
 
@@ -300,7 +315,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": systems.WORKLOADS[args.workload], "natom": s.n, "cutoff_A": 12.0, "dt_fs": DT_FS,
                    "list_update": UPDATE, "energy_every": NMON, "precision": "FP32 pair math, int64 fixed-point "
-                   "accumulation, FP64 SHAKE/integration"},
+                   "accumulation, FP64 SHAKE/integration", "launch_shapes": r32["shapes"]},
         "pair_interactions_per_s": round(r32["pairs"] * sps, 1),
         "pairs_within_cutoff": r32["pairs"],
         "fp64_gpu_ns_per_day": round(v64, 3),

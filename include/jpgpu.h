@@ -178,13 +178,33 @@ int jp_batch_get_state(jp_batch *b, int32_t isys, double *cord, double *vel);
 int jp_batch_last_timing(jp_batch *b, double *t_total);
 int jp_batch_destroy(jp_batch *b);
 
-/* Run options (no counterpart in md_run).  JP_OPT_GRAPHS: 1 (default) runs
- * the plain steps between list rebuilds and energy steps of jp_run_nve /
- * jp_batch_run_nve as captured HIP graphs; 0 launches every kernel from the
- * host.  Results are identical either way. */
+/* Run options (no counterpart in md_run, except that the autotuner plays the
+ * role of SIMD_check's runtime choice of i_vec, SIMD_check.f90:2-42).
+ *   JP_OPT_GRAPHS      1 (default): the plain steps between list rebuilds and
+ *                      energy steps of jp_run_nve / jp_batch_run_nve run as
+ *                      captured HIP graphs; 0: every kernel launched from the
+ *                      host.  Results are identical either way.
+ *   JP_OPT_AUTOTUNE    1: the next run times the list kernel at 1/2/4/8
+ *                      wavefronts per block and the FP32 pair kernel at 2/4/8
+ *                      wavefronts per i-cluster and keeps the fastest; 0
+ *                      (default): fixed shapes.
+ *   JP_OPT_PAIR_WAVES  wavefronts per i-cluster in the FP32 force-step pair kernel (2, 4 =
+ *                      default, 8; energy steps and the FP64 path use 4).
+ *                      It regroups FP32 partial sums: FP32/fixed-point results are
+ *                      deterministic for a given value and agree within the FP32
+ *                      tolerance across values.
+ *   JP_OPT_LIST_WAVES  i-clusters per block in the list kernel (1, 2, 4 = default, 8);
+ *                      the list does not depend on it.
+ * Setting a shape turns the autotuner off; jp_get_option reads the current
+ * (possibly tuned) values. */
 #define JP_OPT_GRAPHS 1
+#define JP_OPT_AUTOTUNE 2
+#define JP_OPT_PAIR_WAVES 3
+#define JP_OPT_LIST_WAVES 4
 int jp_set_option(jp_ctx *ctx, int32_t option, int32_t value);
 int jp_batch_set_option(jp_batch *b, int32_t option, int32_t value);
+int jp_get_option(jp_ctx *ctx, int32_t option, int32_t *value);
+int jp_batch_get_option(jp_batch *b, int32_t option, int32_t *value);
 
 /* Library version and device name (for logs). */
 const char *jp_version(void);

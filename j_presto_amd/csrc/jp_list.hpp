@@ -34,7 +34,7 @@
 #include "jp_kernels.hpp"
 
 #define JP_CL 8          // atoms per i-cluster
-#define JP_LW 4          // i-clusters (wavefronts) per block in the list kernels
+#define JP_LW 4          // default i-clusters (wavefronts) per block of k_list8 (1, 2, 4 or 8; autotuned)
 #define JP_QCAP 128      // per-wave queue of candidate j-clusters (power of 2)
 // list status words lst[]: [0] error bits, [1] longest list above the high-water
 // mark, [2 + s] excluded pairs met within rlim by system s
@@ -215,18 +215,19 @@ __device__ __forceinline__ bool exact_in(const SysP &P, const double4 &a, const 
 // 8 i atoms).  Per i atom the listed count (nbcount) and per system the number
 // of excluded pairs met within rlim (nremoved) are reported for md_run's
 // checks.
-__global__ void __launch_bounds__(64 * JP_LW)
+template <int LW>
+__global__ void __launch_bounds__(64 * LW)
 k_list8(int nclu, const int *__restrict__ clu_sys, const SysP *__restrict__ sp, const double *__restrict__ cbb,
         const float4 *__restrict__ ccf, const int4 *__restrict__ crec, const unsigned *__restrict__ sysext,
         const float4 *__restrict__ lp, const double4 *__restrict__ xr, const int4 *__restrict__ imgs,
         const int *__restrict__ xptr, const int *__restrict__ xidx, int ccap, int hiwater, uint2 *__restrict__ ent,
         int2 *__restrict__ lcnt, int *__restrict__ lst) {
-    __shared__ uint32_t s_c[JP_LW][JP_QCAP];   // candidate j-clusters of passing cells
-    __shared__ uint32_t s_q[JP_LW][JP_QCAP];   // candidates whose box is within rlim
-    __shared__ double4 s_xr[JP_LW][JP_CL];
-    __shared__ int4 s_im[JP_LW][JP_CL];
+    __shared__ uint32_t s_c[LW][JP_QCAP];   // candidate j-clusters of passing cells
+    __shared__ uint32_t s_q[LW][JP_QCAP];   // candidates whose box is within rlim
+    __shared__ double4 s_xr[LW][JP_CL];
+    __shared__ int4 s_im[LW][JP_CL];
     const int l = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // wave-uniform
-    const int ci = blockIdx.x * JP_LW + w;
+    const int ci = blockIdx.x * LW + w;
     if (ci >= nclu) return;   // whole wavefront; no block-wide barrier below
     const int s = clu_sys[ci];
     const SysP &P = sp[s];

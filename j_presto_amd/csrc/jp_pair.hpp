@@ -1,20 +1,20 @@
 // Nonbonded ZD + LJ pair kernels on the 8-atom i-cluster lists
 // (enear_ZD_doub_PB semantics: full list, each atom owns its own sum).
 //
-// One block of JP_PW wavefronts per i-cluster.  The cluster's entries are cut
-// into chunks of 64; wave w takes chunks w, w+JP_PW, ...  A chunk is staged
+// One block of PW wavefronts per i-cluster.  The cluster's entries are cut
+// into chunks of 64; wave w takes chunks w, w+PW, ...  A chunk is staged
 // in the wave's own LDS slice (one entry and its j atom per lane, as
 // coordinates relative to the cluster centre with the image shift applied),
 // then 8 iterations run over it: lane l computes the pair (i = l & 7,
 // j = entry 8*it + (l >> 3)).  Chunks come from the LJ part or the non-LJ part
 // of the list, so the LJ term is uniform per chunk.  At the end the 8 lanes
-// of each i atom and the JP_PW waves are combined in a fixed order.
+// of each i atom and the PW waves are combined in a fixed order.
 #pragma once
 #include <type_traits>
 
 #include "jp_list.hpp"
 
-#define JP_PW 4      // wavefronts per i-cluster in the pair kernels
+#define JP_PW 4      // default wavefronts per i-cluster in the pair kernels (2, 4 or 8; autotuned)
 #define JP_FLUSH_CH 4   // FP32 partial sums go to fixed point every 4 chunks (<= 32 terms)
 
 __device__ __forceinline__ long long fix_f(float v, double scale) { return __double2ll_rn((double)v * scale); }
@@ -54,24 +54,24 @@ __device__ __forceinline__ ChunkSpan chunk_span(int ch, size_t e0, int cap, int2
 // FP32 path: pair arithmetic in FP32 (coordinates relative to the cluster
 // centre, |r| < ~20 A), per-lane partial sums flushed to 64-bit fixed point
 // every <= 32 pairs: deterministic and independent of the launch shape.
-template <bool ENERGY, bool LMD, bool SMALL>
-__global__ void __launch_bounds__(64 * JP_PW)
+template <bool ENERGY, bool LMD, bool SMALL, int PW>
+__global__ void __launch_bounds__(64 * PW)
 k_pair_f32(int nclu, const int *__restrict__ clu_sys, const SysP *__restrict__ sp, const double4 *__restrict__ ccen,
            const JAtom *__restrict__ ys, const float2 *__restrict__ lj32, int ccap,
            const int2 *__restrict__ lcnt, const uint2 *__restrict__ ent, const ExcessArgs ex,
            double *__restrict__ gslot, long long *__restrict__ epart) {
     extern __shared__ float2 ljf[];
     __shared__ double s_sh[27][3];
-    __shared__ float4 s_pos[JP_PW][64];
-    __shared__ int s_meta[JP_PW][64];
-    __shared__ long long s_red[JP_PW][JP_CL][3];
-    __shared__ long long s_ered[JP_PW][6];
+    __shared__ float4 s_pos[PW][64];
+    __shared__ int s_meta[PW][64];
+    __shared__ long long s_red[PW][JP_CL][3];
+    __shared__ long long s_ered[PW][6];
     const int c = xcd_remap(blockIdx.x, gridDim.x);
     if (c >= nclu) return;   // whole block
     const SysP &P = sp[clu_sys[c]];
     const int tid = threadIdx.x, l = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6), ia = l & 7, jsub = l >> 3;
     const double4 cen = ccen[c];
-    for (int k = tid; k < P.ntyp * P.ntyp; k += 64 * JP_PW) ljf[k] = lj32[P.lj0 + k];
+    for (int k = tid; k < P.ntyp * P.ntyp; k += 64 * PW) ljf[k] = lj32[P.lj0 + k];
     if (tid < 27) {
         int sx, sy, sz;
         shift_of((uint32_t)tid, sx, sy, sz);
@@ -167,7 +167,7 @@ k_pair_f32(int nclu, const int *__restrict__ clu_sys, const SysP *__restrict__ s
         }
     };
     int nfl = 0;
-    for (int ch = w; ch < nch; ch += JP_PW) {
+    for (int ch = w; ch < nch; ch += PW) {
         const ChunkSpan sp_ = chunk_span(ch, e0, cap, nn);
         wave_lds_sync();
         if (l < sp_.n) {
@@ -205,7 +205,7 @@ k_pair_f32(int nclu, const int *__restrict__ clu_sys, const SysP *__restrict__ s
     if (tid < JP_CL * 3) {
         const int i = tid / 3, k = tid % 3;
         long long t = 0;
-        for (int q = 0; q < JP_PW; ++q) t += s_red[q][i][k];
+        for (int q = 0; q < PW; ++q) t += s_red[q][i][k];
         const size_t sl = (size_t)c * JP_CL + i;
         {
 #pragma clang fp contract(off)
@@ -213,15 +213,15 @@ k_pair_f32(int nclu, const int *__restrict__ clu_sys, const SysP *__restrict__ s
         }
     } else if (ENERGY && tid >= 32 && tid < 38) {
         long long t = 0;
-        for (int q = 0; q < JP_PW; ++q) t += s_ered[q][tid - 32];
+        for (int q = 0; q < PW; ++q) t += s_ered[q][tid - 32];
         epart[6 * (size_t)c + tid - 32] = t;
     }
 }
 
 // Plain FP64 path: md_run's per-pair arithmetic (enear.f90:182-258 ZD lines)
 // in FP64 with IEEE division and square root and no FMA contraction.
-template <bool ENERGY, bool LMD, bool SMALL>
-__global__ void __launch_bounds__(64 * JP_PW)
+template <bool ENERGY, bool LMD, bool SMALL, int PW>
+__global__ void __launch_bounds__(64 * PW)
 k_pair_f64(int nclu, const int *__restrict__ clu_sys, const SysP *__restrict__ sp, const double4 *__restrict__ ccen,
            const JAtom *__restrict__ ys, const double *__restrict__ qd, const double2 *__restrict__ lj64,
            int ccap, const int2 *__restrict__ lcnt, const uint2 *__restrict__ ent,
@@ -229,16 +229,16 @@ k_pair_f64(int nclu, const int *__restrict__ clu_sys, const SysP *__restrict__ s
 #pragma clang fp contract(off)
     extern __shared__ double2 ljs[];
     __shared__ double s_sh[27][3];
-    __shared__ double4 s_pos[JP_PW][64];
-    __shared__ int s_meta[JP_PW][64];
-    __shared__ double s_red[JP_PW][JP_CL][3];
-    __shared__ double s_ered[JP_PW][6];
+    __shared__ double4 s_pos[PW][64];
+    __shared__ int s_meta[PW][64];
+    __shared__ double s_red[PW][JP_CL][3];
+    __shared__ double s_ered[PW][6];
     const int c = xcd_remap(blockIdx.x, gridDim.x);
     if (c >= nclu) return;
     const SysP &P = sp[clu_sys[c]];
     const int tid = threadIdx.x, l = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6), ia = l & 7, jsub = l >> 3;
     const double4 cen = ccen[c];
-    for (int k = tid; k < P.ntyp * P.ntyp; k += 64 * JP_PW) ljs[k] = lj64[P.lj0 + k];
+    for (int k = tid; k < P.ntyp * P.ntyp; k += 64 * PW) ljs[k] = lj64[P.lj0 + k];
     if (tid < 27) {
         int sx, sy, sz;
         shift_of((uint32_t)tid, sx, sy, sz);
@@ -307,7 +307,7 @@ k_pair_f64(int nclu, const int *__restrict__ clu_sys, const SysP *__restrict__ s
             }
         }
     };
-    for (int ch = w; ch < nch; ch += JP_PW) {
+    for (int ch = w; ch < nch; ch += PW) {
         const ChunkSpan sp_ = chunk_span(ch, e0, cap, nn);
         wave_lds_sync();
         if (l < sp_.n) {
@@ -342,12 +342,12 @@ k_pair_f64(int nclu, const int *__restrict__ clu_sys, const SysP *__restrict__ s
     if (tid < JP_CL * 3) {
         const int i = tid / 3, k = tid % 3;
         double t = 0.0;
-        for (int q = 0; q < JP_PW; ++q) t = t + s_red[q][i][k];
+        for (int q = 0; q < PW; ++q) t = t + s_red[q][i][k];
         const size_t sl = (size_t)c * JP_CL + i;
         gslot[3 * sl + k] = ex.grad(P, clu_sys[c], sl, k) + (0.0 - t);
     } else if (ENERGY && tid >= 32 && tid < 38) {
         double t = 0.0;
-        for (int q = 0; q < JP_PW; ++q) t = t + s_ered[q][tid - 32];
+        for (int q = 0; q < PW; ++q) t = t + s_ered[q][tid - 32];
         epart[6 * (size_t)c + tid - 32] = t;
     }
 }

@@ -188,11 +188,17 @@ struct Engine {
     };
     std::vector<StepGraph> graphs;
     int use_graphs = 1;
+    // launch shapes: wavefronts per i-cluster (pair kernels) and per block
+    // (k_list8); autotune = 1 times the candidates at the next jp_run_nve
+    int pw = JP_PW, lw = JP_LW, autotune = 0, tuned = 0;
+    int tune(int prec);
     StepGraph *step_graph(int n, int prec);
     void drop_graphs();
     int get_list(int32_t *cell3, int64_t *npairs, int32_t *ptr, int32_t *idx, int64_t cap);
     int count_pairs(int64_t *np, int64_t *nlj, int64_t *nlist);
 };
+
+static void launch_list8(Engine &g, int hiwater);
 
 int Engine::init(const jp_sys_desc *descs, int ns) {
     if (ns <= 0) return JP_EINVAL;
@@ -603,9 +609,7 @@ int Engine::build_list() {
     for (int attempt = 0;; ++attempt) {
         ccap = std::max(ccap, ccap_want);
         AL(d_ent, (size_t)nclu * ccap);
-        hipLaunchKernelGGL(k_list8, grid1(nclu, JP_LW), 64 * JP_LW, 0, st, nclu, d_clu_sys.p, d_sp.p, d_cbb.p, d_ccf.p,
-                           d_crec.p, d_sysext.p, d_lp.p, d_xr.p, d_imgs.p, d_xptr.p, d_xidx.p, ccap, ccap - ccap / 8,
-                           d_ent.p, d_lcnt.p, d_lst.p);
+        launch_list8(*this, ccap - ccap / 8);
         CK(hipGetLastError());
         CK(hipMemcpyAsync(lst.data(), d_lst.p, 4 * (size_t)nstat, hipMemcpyDeviceToHost, st));
         CK(hipStreamSynchronize(st));
@@ -651,7 +655,7 @@ int Engine::pair_variant() const {
         small = small || sp[s].small;
         for (int q = 0; q < 4; ++q) lmd = lmd || sp[s].fac[q] != 1.0 || sp[s].cls[q] != sp[s].cls[0];
     }
-    return (lmd ? 2 : 0) | (small ? 1 : 0);
+    return (lmd ? 2 : 0) | (small ? 1 : 0) | (pw << 4);
 }
 
 int Engine::pair(int prec, bool energy) {
@@ -659,7 +663,7 @@ int Engine::pair(int prec, bool energy) {
     int maxt = 0;
     for (int s = 0; s < nsys; ++s) maxt = std::max(maxt, hs[s].ntyp);
     const size_t l32 = sizeof(float2) * maxt * maxt, l64 = sizeof(double2) * maxt * maxt;
-    const int key = (energy ? 4 : 0) | pair_variant();
+    const int key = (energy ? 4 : 0) | (pair_variant() & 3);
     switch (key) {
         case 0: launch_pair<false, false, false>(prec, nb, l32, l64, st, *this); break;
         case 1: launch_pair<false, false, true>(prec, nb, l32, l64, st, *this); break;
@@ -734,16 +738,100 @@ int Engine::bonded(const double *cord, double *grad, double *ebond, double *eang
     return 0;
 }
 
-template <bool E, bool M, bool S>
-static void launch_pair(int prec, unsigned nb, size_t lds32, size_t lds64, hipStream_t st, Engine &g) {
+template <bool E, bool M, bool S, int PW>
+static void launch_pair_f32_only(unsigned nb, size_t lds32, hipStream_t st, Engine &g) {
+    hipLaunchKernelGGL((k_pair_f32<E, M, S, PW>), nb, 64 * PW, lds32, st, g.nclu, g.d_clu_sys.p, g.d_sp.p, g.d_ccen.p,
+                       g.d_ys.p, g.d_lj32.p, g.ccap, g.d_lcnt.p, g.d_ent.p, g.excess_args(), g.d_gslot.p, g.d_epfix.p);
+}
+
+template <bool E, bool M, bool S, int PW>
+static void launch_pair_pw(int prec, unsigned nb, size_t lds32, size_t lds64, hipStream_t st, Engine &g) {
     if (prec == JP_PREC_FP64)
-        hipLaunchKernelGGL((k_pair_f64<E, M, S>), nb, 64 * JP_PW, lds64, st, g.nclu, g.d_clu_sys.p, g.d_sp.p,
+        hipLaunchKernelGGL((k_pair_f64<E, M, S, PW>), nb, 64 * PW, lds64, st, g.nclu, g.d_clu_sys.p, g.d_sp.p,
                            g.d_ccen.p, g.d_ys.p, g.d_qd.p, g.d_lj64.p, g.ccap, g.d_lcnt.p, g.d_ent.p, g.excess_args(),
                            g.d_gslot.p, g.d_ep64.p);
     else
-        hipLaunchKernelGGL((k_pair_f32<E, M, S>), nb, 64 * JP_PW, lds32, st, g.nclu, g.d_clu_sys.p, g.d_sp.p,
+        hipLaunchKernelGGL((k_pair_f32<E, M, S, PW>), nb, 64 * PW, lds32, st, g.nclu, g.d_clu_sys.p, g.d_sp.p,
                            g.d_ccen.p, g.d_ys.p, g.d_lj32.p, g.ccap, g.d_lcnt.p, g.d_ent.p, g.excess_args(),
                            g.d_gslot.p, g.d_epfix.p);
+}
+
+// The tuned wave count applies to the FP32 force-step kernels; energy steps
+// and the plain FP64 path keep JP_PW.
+template <bool E, bool M, bool S>
+static void launch_pair(int prec, unsigned nb, size_t lds32, size_t lds64, hipStream_t st, Engine &g) {
+    if constexpr (E) {
+        launch_pair_pw<E, M, S, JP_PW>(prec, nb, lds32, lds64, st, g);
+    } else {
+        if (prec == JP_PREC_FP64 || g.pw == JP_PW) launch_pair_pw<E, M, S, JP_PW>(prec, nb, lds32, lds64, st, g);
+        else if (g.pw == 2) launch_pair_f32_only<E, M, S, 2>(nb, lds32, st, g);
+        else launch_pair_f32_only<E, M, S, 8>(nb, lds32, st, g);
+    }
+}
+
+template <int LW>
+static void launch_list8_lw(Engine &g, int hiwater) {
+    hipLaunchKernelGGL(k_list8<LW>, grid1(g.nclu, LW), 64 * LW, 0, g.st, g.nclu, g.d_clu_sys.p, g.d_sp.p, g.d_cbb.p,
+                       g.d_ccf.p, g.d_crec.p, g.d_sysext.p, g.d_lp.p, g.d_xr.p, g.d_imgs.p, g.d_xptr.p, g.d_xidx.p,
+                       g.ccap, hiwater, g.d_ent.p, g.d_lcnt.p, g.d_lst.p);
+}
+
+static void launch_list8(Engine &g, int hiwater) {
+    switch (g.lw) {
+        case 1: launch_list8_lw<1>(g, hiwater); break;
+        case 2: launch_list8_lw<2>(g, hiwater); break;
+        case 8: launch_list8_lw<8>(g, hiwater); break;
+        default: launch_list8_lw<4>(g, hiwater); break;
+    }
+}
+
+// Autotuner: times k_list8 at 1, 2, 4 and 8 wavefronts per block and the
+// FP32 force-step pair kernel at 2, 4 and 8 wavefronts per i-cluster on the
+// current list (median of 3 launches each) and keeps the fastest.  The list
+// is rebuilt afterwards (the timing launches add to its status counters).
+// The list does not depend on lw; pw regroups the FP32 partial sums
+// (results stay deterministic for a given pw, within the FP32 tolerance
+// across pw values).
+int Engine::tune(int prec) {
+    int r;
+    if (!have_list && (r = build_list())) return r;
+    hipEvent_t a, b;
+    CK(hipEventCreate(&a));
+    CK(hipEventCreate(&b));
+    auto timed = [&](auto launch) -> float {
+        float t[3];
+        for (int k = 0; k < 3; ++k) {
+            hipEventRecord(a, st);
+            launch();
+            hipEventRecord(b, st);
+            hipEventSynchronize(b);
+            hipEventElapsedTime(&t[k], a, b);
+        }
+        std::sort(t, t + 3);
+        return t[1];
+    };
+    float best = 1e30f;
+    int bl = lw;
+    for (int c : {1, 2, 4, 8}) {
+        lw = c;
+        const float ms = timed([&] { launch_list8(*this, ccap - ccap / 8); });
+        if (ms < best) { best = ms; bl = c; }
+    }
+    lw = bl;
+    best = 1e30f;
+    int bp = pw;
+    for (int c : {2, 4, 8}) {
+        if (prec == JP_PREC_FP64 && c != JP_PW) continue;
+        pw = c;
+        const float ms = timed([&] { pair(prec, false); });
+        if (ms < best) { best = ms; bp = c; }
+    }
+    pw = bp;
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    CK(hipGetLastError());
+    tuned = 1;
+    return build_list();
 }
 
 // Per-system, per-class energies after a pair(prec, true) call.
@@ -930,6 +1018,7 @@ int Engine::run_nve(int nsteps, int update, int nmon, int prec, jp_energy_log *l
     if (!have_list && (r = build_list())) return r;
     if ((r = set_factors(-1))) return r;
     if ((r = refresh_sorted())) return r;
+    if (autotune && !tuned && (r = tune(prec))) return r;
     CK(hipMemsetAsync(d_err.p, 0, 4, st));
     t_pair = 0; t_step = 0; n_pair = 0; n_pair_steps = nsteps;
     const double tl0 = t_list;
@@ -1230,12 +1319,49 @@ int jp_batch_last_timing(jp_batch *b, double *t_total) {
 }
 
 static int set_option(Engine &E, int32_t option, int32_t value) {
-    if (option == JP_OPT_GRAPHS) {
-        E.use_graphs = value ? 1 : 0;
-        if (!value) E.drop_graphs();
-        return 0;
+    switch (option) {
+        case JP_OPT_GRAPHS:
+            E.use_graphs = value ? 1 : 0;
+            if (!value) E.drop_graphs();
+            return 0;
+        case JP_OPT_AUTOTUNE:
+            E.autotune = value ? 1 : 0;
+            E.tuned = 0;
+            return 0;
+        case JP_OPT_PAIR_WAVES:
+            if (value != 2 && value != 4 && value != 8) return JP_EINVAL;
+            E.pw = value;
+            E.autotune = 0;
+            return 0;
+        case JP_OPT_LIST_WAVES:
+            if (value != 1 && value != 2 && value != 4 && value != 8) return JP_EINVAL;
+            E.lw = value;
+            E.autotune = 0;
+            return 0;
+        default:
+            return JP_EINVAL;
     }
-    return JP_EINVAL;
+}
+
+static int get_option(const Engine &E, int32_t option, int32_t *value) {
+    if (!value) return JP_EINVAL;
+    switch (option) {
+        case JP_OPT_GRAPHS: *value = E.use_graphs; return 0;
+        case JP_OPT_AUTOTUNE: *value = E.autotune; return 0;
+        case JP_OPT_PAIR_WAVES: *value = E.pw; return 0;
+        case JP_OPT_LIST_WAVES: *value = E.lw; return 0;
+        default: return JP_EINVAL;
+    }
+}
+
+int jp_get_option(jp_ctx *ctx, int32_t option, int32_t *value) {
+    if (!ctx) return JP_EINVAL;
+    return get_option(ctx->e, option, value);
+}
+
+int jp_batch_get_option(jp_batch *b, int32_t option, int32_t *value) {
+    if (!b) return JP_EINVAL;
+    return get_option(b->e, option, value);
 }
 
 int jp_set_option(jp_ctx *ctx, int32_t option, int32_t value) {

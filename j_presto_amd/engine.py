@@ -145,6 +145,14 @@ class MdSystem:
         one kernel at a time; the results are the same."""
         native.check("jp_set_option", self.lib.jp_set_option(self.ctx, native.JP_OPT_GRAPHS, int(bool(on))))
 
+    def set_option(self, option: int, value: int):
+        native.check("jp_set_option", self.lib.jp_set_option(self.ctx, option, int(value)))
+
+    def get_option(self, option: int) -> int:
+        v = C.c_int32()
+        native.check("jp_get_option", self.lib.jp_get_option(self.ctx, option, C.byref(v)))
+        return v.value
+
     def timing(self):
         t = [C.c_double() for _ in range(4)]
         self.lib.jp_last_timing(self.ctx, *[C.byref(x) for x in t])
@@ -238,6 +246,14 @@ class Ensemble:
 
     def set_graphs(self, on: bool):
         native.check("jp_batch_set_option", self.lib.jp_batch_set_option(self.b, native.JP_OPT_GRAPHS, int(bool(on))))
+
+    def set_option(self, option: int, value: int):
+        native.check("jp_batch_set_option", self.lib.jp_batch_set_option(self.b, option, int(value)))
+
+    def get_option(self, option: int) -> int:
+        v = C.c_int32()
+        native.check("jp_batch_get_option", self.lib.jp_batch_get_option(self.b, option, C.byref(v)))
+        return v.value
 
     def get_state(self, i: int):
         n = self.holders[i].n

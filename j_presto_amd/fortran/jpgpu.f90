@@ -24,7 +24,8 @@ module jpgpu
   integer(c_int), parameter :: JP_IER_CELL = 28, JP_IER_LIST = 51, JP_IER_EXCL = 99
   integer(c_int), parameter :: JP_IER_SHAKE_CONV = 1, JP_IER_SHAKE_SING = -2
   integer(c_int), parameter :: JP_PREC_FP64 = 0, JP_PREC_FP32_FIXED = 1
-  integer(c_int32_t), parameter :: JP_OPT_GRAPHS = 1
+  integer(c_int32_t), parameter :: JP_OPT_GRAPHS = 1, JP_OPT_AUTOTUNE = 2
+  integer(c_int32_t), parameter :: JP_OPT_PAIR_WAVES = 3, JP_OPT_LIST_WAVES = 4
 
   ! System description: the module state md_run builds in its input phase.
   ! Same field order and C layout as jp_sys_desc in include/jpgpu.h.
@@ -186,6 +187,20 @@ module jpgpu
        type(c_ptr), value :: b
        integer(c_int32_t), value :: option, value
      end function jp_batch_set_option
+
+     integer(c_int) function jp_get_option(ctx, option, value) bind(C, name='jp_get_option')
+       import :: c_int, c_int32_t, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: option
+       integer(c_int32_t), intent(out) :: value
+     end function jp_get_option
+
+     integer(c_int) function jp_batch_get_option(b, option, value) bind(C, name='jp_batch_get_option')
+       import :: c_int, c_int32_t, c_ptr
+       type(c_ptr), value :: b
+       integer(c_int32_t), value :: option
+       integer(c_int32_t), intent(out) :: value
+     end function jp_batch_get_option
 
      type(c_ptr) function jp_version_c() bind(C, name='jp_version')
        import :: c_ptr

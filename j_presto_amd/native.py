@@ -41,8 +41,10 @@ SIGNATURES = {
     "jp_batch_destroy": (C.c_int, [vp]),
     "jp_set_option": (C.c_int, [vp, C.c_int32, C.c_int32]),
     "jp_batch_set_option": (C.c_int, [vp, C.c_int32, C.c_int32]),
+    "jp_get_option": (C.c_int, [vp, C.c_int32, i32p]),
+    "jp_batch_get_option": (C.c_int, [vp, C.c_int32, i32p]),
 }
-JP_OPT_GRAPHS = 1
+JP_OPT_GRAPHS, JP_OPT_AUTOTUNE, JP_OPT_PAIR_WAVES, JP_OPT_LIST_WAVES = 1, 2, 3, 4
 
 _lib = None
 

@@ -237,3 +237,44 @@ def test_batch_graphs_equal_single_eager():
         assert np.array_equal(x1, xb) and np.array_equal(v1, vb), i
         md.close()
     ens.close()
+
+
+def test_launch_shapes_and_autotuner():
+    """The list does not depend on the list kernel's wavefronts per block;
+    the FP32 forces at 2/4/8 wavefronts per i-cluster agree within the FP32
+    bar and each setting is reproducible; the autotuner picks valid shapes."""
+    from j_presto_amd import native
+    s = systems.water_box(700, 28.0, 23, n_ion_pairs=2)
+    print("case launch shapes", flush=True)
+    ref = None
+    lists = []
+    for lw in (1, 2, 4, 8):
+        md = engine.MdSystem(s, precision=engine.FP32)
+        md.set_option(native.JP_OPT_LIST_WAVES, lw)
+        assert md.celprp() == 0
+        lists.append(md.get_list())
+        md.close()
+    for a in lists[1:]:
+        assert all(np.array_equal(x, y) for x, y in zip(lists[0], a))
+    runs = {}
+    for pw in (2, 4, 8):
+        out = []
+        for rep in range(2):
+            md = engine.MdSystem(s, precision=engine.FP32)
+            md.set_option(native.JP_OPT_PAIR_WAVES, pw)
+            _, ier = md.runmd(20, update=10)
+            assert ier == 0
+            out.append(md.get_state())
+            md.close()
+        assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1]), pw
+        runs[pw] = out[0]
+    for pw in (2, 8):
+        assert np.abs(runs[pw][0] - runs[4][0]).max() < 1e-5
+    md = engine.MdSystem(s, precision=engine.FP32)
+    md.set_option(native.JP_OPT_AUTOTUNE, 1)
+    _, ier = md.runmd(20, update=10)
+    assert ier == 0
+    assert md.get_option(native.JP_OPT_PAIR_WAVES) in (2, 4, 8)
+    assert md.get_option(native.JP_OPT_LIST_WAVES) in (1, 2, 4, 8)
+    assert np.abs(md.get_state()[0] - runs[4][0]).max() < 1e-5
+    md.close()
