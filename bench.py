@@ -326,7 +326,8 @@ def main():
         "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 5),
                      "traffic": traffic_from_profile() if args.workload == 2 else None,
-                     "kernel": "k_pair_f32<false>", "avg_launch_us": round(t_pair * 1e6, 3),
+                     "kernel": f"k_pair_f32<false, false, false, {r32['shapes']['pair_waves']}>",
+                     "avg_launch_us": round(t_pair * 1e6, 3),
                      "flop_per_launch": flop},
         "list_roofline": {"bound": "hbm", "achieved": round(list_bytes / t_list / 1e9, 2), "peak": PEAK_HBM_GBS,
                           "unit": "GB/s", "frac": round(list_bytes / t_list / 1e9 / PEAK_HBM_GBS, 5),

@@ -787,7 +787,8 @@ static void launch_list8(Engine &g, int hiwater) {
 
 // Autotuner: times k_list8 at 1, 2, 4 and 8 wavefronts per block and the
 // FP32 force-step pair kernel at 2, 4 and 8 wavefronts per i-cluster on the
-// current list (median of 3 launches each) and keeps the fastest.  The list
+// current list (median of 7 launches each, candidates interleaved so that
+// clock drift hits all of them alike) and keeps the fastest.  The list
 // is rebuilt afterwards (the timing launches add to its status counters).
 // The list does not depend on lw; pw regroups the FP32 partial sums
 // (results stay deterministic for a given pw, within the FP32 tolerance
@@ -798,35 +799,34 @@ int Engine::tune(int prec) {
     hipEvent_t a, b;
     CK(hipEventCreate(&a));
     CK(hipEventCreate(&b));
-    auto timed = [&](auto launch) -> float {
-        float t[3];
-        for (int k = 0; k < 3; ++k) {
-            hipEventRecord(a, st);
-            launch();
-            hipEventRecord(b, st);
-            hipEventSynchronize(b);
-            hipEventElapsedTime(&t[k], a, b);
-        }
-        std::sort(t, t + 3);
-        return t[1];
+    constexpr int NREP = 7;
+    auto once = [&](auto launch) -> float {
+        float ms = 0.f;
+        hipEventRecord(a, st);
+        launch();
+        hipEventRecord(b, st);
+        hipEventSynchronize(b);
+        hipEventElapsedTime(&ms, a, b);
+        return ms;
     };
-    float best = 1e30f;
-    int bl = lw;
-    for (int c : {1, 2, 4, 8}) {
-        lw = c;
-        const float ms = timed([&] { launch_list8(*this, ccap - ccap / 8); });
-        if (ms < best) { best = ms; bl = c; }
-    }
-    lw = bl;
-    best = 1e30f;
-    int bp = pw;
-    for (int c : {2, 4, 8}) {
-        if (prec == JP_PREC_FP64 && c != JP_PW) continue;
-        pw = c;
-        const float ms = timed([&] { pair(prec, false); });
-        if (ms < best) { best = ms; bp = c; }
-    }
-    pw = bp;
+    auto pick = [&](const std::vector<int> &cand, int &knob, auto launch) {
+        std::vector<std::vector<float>> t(cand.size());
+        for (int rep = 0; rep < NREP; ++rep)
+            for (size_t q = 0; q < cand.size(); ++q) {
+                knob = cand[q];
+                t[q].push_back(once(launch));
+            }
+        float best = 1e30f;
+        int bk = cand[0];
+        for (size_t q = 0; q < cand.size(); ++q) {
+            std::sort(t[q].begin(), t[q].end());
+            if (t[q][NREP / 2] < best) { best = t[q][NREP / 2]; bk = cand[q]; }
+        }
+        knob = bk;
+    };
+    pick({1, 2, 4, 8}, lw, [&] { launch_list8(*this, ccap - ccap / 8); });
+    if (prec == JP_PREC_FP64) pw = JP_PW;
+    else pick({2, 4, 8}, pw, [&] { pair(prec, false); });
     hipEventDestroy(a);
     hipEventDestroy(b);
     CK(hipGetLastError());
