@@ -169,24 +169,43 @@ def drift_fit(log, natom):
             "etot_first": float(e[0]), "etot_last": float(e[-1]), "samples": int(len(e))}
 
 
-def run_drift(nsteps: int, equil: int):
+def run_drift(nsteps: int, equil: int, reps: int = 1):
+    """N-step NVE runs in both precisions from the same equilibrated state;
+    with reps > 1, further start states 1,000 rescaled steps apart, and the
+    mean and spread of the slopes."""
+    import numpy as np
     from j_presto_amd import engine, systems
     s = systems.workload(2)
     state = equilibrated(s, equil)
     out = {"mode": "nve-drift", "workload": systems.WORKLOADS[2], "natom": s.n, "steps": nsteps,
-           "dt_fs": DT_FS, "energy_every": NMON, "list_update": UPDATE, "data": "synthetic"}
-    for name, prec in (("fp32_fixed", engine.FP32), ("fp64", engine.FP64)):
-        md = engine.MdSystem(s, dt=DT_FS, precision=prec)
-        md.set_state(*state)
-        assert md.celprp() == 0
-        t0 = time.perf_counter()
-        log, ier = md.runmd(nsteps, update=UPDATE, nmon=NMON)
-        wall = time.perf_counter() - t0
-        assert ier == 0, ier
-        md.close()
-        out[name] = drift_fit(log, s.n)
-        out[name]["ns_per_day"] = round(ns_per_day(nsteps / wall), 3)
-        print(f"# {name}: {out[name]}", file=sys.stderr, flush=True)
+           "dt_fs": DT_FS, "energy_every": NMON, "list_update": UPDATE, "data": "synthetic", "reps": reps}
+    slopes = {"fp32_fixed": [], "fp64": []}
+    for rep in range(reps):
+        if rep:
+            md = engine.MdSystem(s, dt=DT_FS, precision=engine.FP32)
+            md.set_state(*state)
+            assert md.celprp() == 0
+            assert engine.equilibrate(md, 1000) == 0
+            state = md.get_state()
+            md.close()
+        for name, prec in (("fp32_fixed", engine.FP32), ("fp64", engine.FP64)):
+            md = engine.MdSystem(s, dt=DT_FS, precision=prec)
+            md.set_state(*state)
+            assert md.celprp() == 0
+            t0 = time.perf_counter()
+            log, ier = md.runmd(nsteps, update=UPDATE, nmon=NMON)
+            wall = time.perf_counter() - t0
+            assert ier == 0, ier
+            md.close()
+            fit = drift_fit(log, s.n)
+            fit["ns_per_day"] = round(ns_per_day(nsteps / wall), 3)
+            slopes[name].append(fit["slope_per_1000_atoms"])
+            if rep == 0:
+                out[name] = fit
+            print(f"# rep {rep} {name}: {fit}", file=sys.stderr, flush=True)
+    if reps > 1:
+        out["slope_per_1000_atoms"] = {k: {"mean": float(np.mean(v)), "std": float(np.std(v, ddof=1)),
+                                           "all": [round(x, 4) for x in v]} for k, v in slopes.items()}
     print(json.dumps(out), flush=True)
 
 
@@ -254,6 +273,7 @@ def main():
     ap.add_argument("--workload", type=int, default=2, choices=[0, 1, 2, 4])
     ap.add_argument("--ensemble", action="store_true")
     ap.add_argument("--drift", type=int, default=0, help="run an N-step NVE drift comparison instead")
+    ap.add_argument("--drift-reps", type=int, default=1, help="independent start states for --drift")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run 1 GiB device copies (profiler traffic calibration)")
     ap.add_argument("--gpus", type=int, default=1)
@@ -274,7 +294,7 @@ def main():
     if args.ensemble or args.workload == 1:
         DT_FS = DT_CHAIN_FS
     if args.drift:
-        run_drift(args.drift, args.equil)
+        run_drift(args.drift, args.equil, args.drift_reps)
         return
     if args.ensemble:
         run_ensemble(args.steps, args.warmup, args.equil)
