@@ -571,6 +571,42 @@ __global__ void k_post(int natom, const int *__restrict__ atom_sys, const SysP *
 
 // Kinetic energy terms m|v|^2 with v = (vback + v)/2 (md.f90:618-628); summed
 // per system on the host in atom order.
+// Per-segment sums in a fixed order (deterministic run to run), two passes.
+// k_seg_part: block (j, s) adds a[i * ncomp + k] over i = beg[s] + j*256 + t,
+// + JP_SEG_NB*256, ... (thread t, in sequence), then a fixed tree over its 256
+// threads -> part[(s * JP_SEG_NB + j) * ncomp + k].  k_seg_fin: the JP_SEG_NB
+// partials of segment s in order -> out[s * ncomp + k].  ncomp <= 8.
+#define JP_SEG_NB 64
+template <typename T>
+__global__ void __launch_bounds__(256) k_seg_part(const int *__restrict__ beg, const int *__restrict__ end, int ncomp,
+                                                  const T *__restrict__ a, T *__restrict__ part) {
+#pragma clang fp contract(off)
+    __shared__ T s[8][256];
+    const int j = blockIdx.x, sg = blockIdx.y, t = threadIdx.x;
+    T acc[8];
+    for (int k = 0; k < 8; ++k) acc[k] = T(0);
+    for (int i = beg[sg] + j * 256 + t; i < end[sg]; i += JP_SEG_NB * 256)
+        for (int k = 0; k < ncomp; ++k) acc[k] = acc[k] + a[(size_t)i * ncomp + k];
+    for (int k = 0; k < ncomp; ++k) s[k][t] = acc[k];
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (t < h)
+            for (int k = 0; k < ncomp; ++k) s[k][t] = s[k][t] + s[k][t + h];
+        __syncthreads();
+    }
+    if (t < ncomp) part[((size_t)sg * JP_SEG_NB + j) * ncomp + t] = s[t][0];
+}
+
+template <typename T>
+__global__ void k_seg_fin(int ncomp, const T *__restrict__ part, T *__restrict__ out) {
+#pragma clang fp contract(off)
+    const int sg = blockIdx.x, t = threadIdx.x;
+    if (t >= ncomp) return;
+    T acc = T(0);
+    for (int j = 0; j < JP_SEG_NB; ++j) acc = acc + part[((size_t)sg * JP_SEG_NB + j) * ncomp + t];
+    out[(size_t)sg * ncomp + t] = acc;
+}
+
 __global__ void k_ekin(int natom, const double *__restrict__ mass, const double *__restrict__ v,
                        const double *__restrict__ vback, double *__restrict__ out) {
 #pragma clang fp contract(off)

@@ -116,6 +116,24 @@ struct Engine {
     int nbt = 0, nat = 0, nbat = 0;
     int nunit = 0;             // k_step units: SHAKE groups and free atoms
     DBuf<int4> d_unit;
+    // energy sums on the device: per-system atom and cluster ranges, results,
+    // and a pinned host copy of them
+    DBuf<int> d_abeg, d_aend, d_cbeg, d_cend;
+    DBuf<double> d_rsum;        // [nsys][6] pair (FP64), [nsys][3] excess, [nsys] kinetic
+    DBuf<long long> d_rfix;     // [nsys][6] pair (fixed point)
+    DBuf<double> d_part;        // k_seg_part partials
+    DBuf<long long> d_partf;
+    // per-system sums of a[i * ncomp + k] over segments [beg, end) into out
+    template <typename T>
+    int seg_sum(const int *beg, const int *end, int ncomp, const T *a, T *part, T *out) {
+        hipLaunchKernelGGL(k_seg_part<T>, dim3(JP_SEG_NB, nsys), 256, 0, st, beg, end, ncomp, a, part);
+        hipLaunchKernelGGL(k_seg_fin<T>, nsys, 64, 0, st, ncomp, (const T *)part, out);
+        CK(hipGetLastError());
+        return 0;
+    }
+    double *h_rsum = nullptr;
+    long long *h_rfix = nullptr;
+    int *h_err = nullptr;
     std::vector<int> tsys_h, tcls_h;   // system and class of every term (bonds, then angles)
     DBuf<int> d_bat, d_bptr, d_bref;
     DBuf<int4> d_bnd, d_ang;
@@ -143,6 +161,11 @@ struct Engine {
         d_u64.release();
         d_bat.release(); d_bptr.release(); d_bref.release(); d_bnd.release(); d_ang.release();
         d_bndp.release(); d_angp.release(); d_eterm.release(); d_unit.release();
+        d_abeg.release(); d_aend.release(); d_cbeg.release(); d_cend.release(); d_rsum.release(); d_rfix.release();
+        d_part.release(); d_partf.release();
+        if (h_rsum) hipHostFree(h_rsum);
+        if (h_rfix) hipHostFree(h_rfix);
+        if (h_err) hipHostFree(h_err);
     }
 
     int init(const jp_sys_desc *descs, int ns);
@@ -516,6 +539,17 @@ int Engine::init(const jp_sys_desc *descs, int ns) {
     AL(d_bnd, bnd.size()); AL(d_bndp, bndp.size()); AL(d_ang, ang.size()); AL(d_angp, angp.size());
     AL(d_eterm, (size_t)nbt + nat);
     AL(d_unit, units.size());
+    AL(d_abeg, ns); AL(d_aend, ns); AL(d_cbeg, ns); AL(d_cend, ns);
+    AL(d_rsum, 10 * (size_t)ns); AL(d_rfix, 6 * (size_t)ns);
+    AL(d_part, 8 * (size_t)JP_SEG_NB * ns); AL(d_partf, 8 * (size_t)JP_SEG_NB * ns);
+    CK(hipHostMalloc((void **)&h_rsum, sizeof(double) * 10 * (size_t)ns, hipHostMallocDefault));
+    CK(hipHostMalloc((void **)&h_rfix, sizeof(long long) * 6 * (size_t)ns, hipHostMallocDefault));
+    CK(hipHostMalloc((void **)&h_err, sizeof(int), hipHostMallocDefault));
+    std::vector<int> abeg(ns), aend(ns), cbeg(ns), cend(ns);
+    for (int s = 0; s < ns; ++s) {
+        abeg[s] = sp[s].atom0; aend[s] = sp[s].atom0 + sp[s].natom;
+        cbeg[s] = sp[s].slot0 / JP_CL; cend[s] = (sp[s].slot0 + sp[s].nslot) / JP_CL;
+    }
     auto H2D = [&](void *dst, const void *src, size_t bytes) -> int {
         if (bytes) CK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st));
         return 0;
@@ -554,6 +588,10 @@ int Engine::init(const jp_sys_desc *descs, int ns) {
     r |= H2D(d_ang.p, ang.data(), 16 * ang.size());
     r |= H2D(d_angp.p, angp.data(), 16 * angp.size());
     r |= H2D(d_unit.p, units.data(), 16 * units.size());
+    r |= H2D(d_abeg.p, abeg.data(), 4 * abeg.size());
+    r |= H2D(d_aend.p, aend.data(), 4 * aend.size());
+    r |= H2D(d_cbeg.p, cbeg.data(), 4 * cbeg.size());
+    r |= H2D(d_cend.p, cend.data(), 4 * cend.size());
     if (r) return r;
     CK(hipMemsetAsync(d_v.p, 0, 8 * 3 * (size_t)natom, st));
     CK(hipStreamSynchronize(st));
@@ -839,34 +877,31 @@ int Engine::energies(int prec, std::vector<double> &eel, std::vector<double> &ev
     eel.assign(3 * (size_t)nsys, 0.0);
     evd.assign(3 * (size_t)nsys, 0.0);
     eex.assign(3 * (size_t)nsys, 0.0);
+    // per-atom excess energies and per-cluster pair energies, summed per
+    // system on the device in a fixed order; one small read-back
     hipLaunchKernelGGL(k_excess_energy, grid1(natom, 256), 256, 0, st, natom, d_atom_sys.p, d_sp.p, d_x.p, d_chg.p,
                        d_eptr.p, d_eidx.p, d_eexc.p);
-    std::vector<double> ex(3 * (size_t)natom);
-    CK(hipMemcpyAsync(ex.data(), d_eexc.p, 8 * ex.size(), hipMemcpyDeviceToHost, st));
-    if (prec == JP_PREC_FP64) {
-        std::vector<double> ep(6 * (size_t)nclu);
-        CK(hipMemcpyAsync(ep.data(), d_ep64.p, 8 * ep.size(), hipMemcpyDeviceToHost, st));
-        CK(hipStreamSynchronize(st));
-        for (int c = 0; c < nclu; ++c) {
-            int s = clu_sys_h[c];
-            for (int k = 0; k < 3; ++k) { eel[3 * s + k] += ep[6 * c + k]; evd[3 * s + k] += ep[6 * c + 3 + k]; }
+    int r;
+    if ((r = seg_sum<double>(d_abeg.p, d_aend.p, 3, d_eexc.p, d_part.p, d_rsum.p + 6 * (size_t)nsys))) return r;
+    if (prec == JP_PREC_FP64) r = seg_sum<double>(d_cbeg.p, d_cend.p, 6, d_ep64.p, d_part.p, d_rsum.p);
+    else r = seg_sum<long long>(d_cbeg.p, d_cend.p, 6, d_epfix.p, d_partf.p, d_rfix.p);
+    if (r) return r;
+    CK(hipMemcpyAsync(h_rsum, d_rsum.p, sizeof(double) * 9 * (size_t)nsys, hipMemcpyDeviceToHost, st));
+    if (prec != JP_PREC_FP64)
+        CK(hipMemcpyAsync(h_rfix, d_rfix.p, sizeof(long long) * 6 * (size_t)nsys, hipMemcpyDeviceToHost, st));
+    CK(hipStreamSynchronize(st));
+    for (int s = 0; s < nsys; ++s)
+        for (int k = 0; k < 3; ++k) {
+            if (prec == JP_PREC_FP64) {
+                eel[3 * s + k] = h_rsum[6 * s + k];
+                evd[3 * s + k] = h_rsum[6 * s + 3 + k];
+            } else {
+                eel[3 * s + k] = h_rfix[6 * s + k] / JP_ESCALE;
+                evd[3 * s + k] = h_rfix[6 * s + 3 + k] / JP_ESCALE;
+            }
+            eex[3 * s + k] = h_rsum[6 * (size_t)nsys + 3 * s + k];
         }
-    } else {
-        std::vector<long long> ep(6 * (size_t)nclu);
-        CK(hipMemcpyAsync(ep.data(), d_epfix.p, 8 * ep.size(), hipMemcpyDeviceToHost, st));
-        CK(hipStreamSynchronize(st));
-        std::vector<long long> se(3 * (size_t)nsys, 0), sv(3 * (size_t)nsys, 0);
-        for (int c = 0; c < nclu; ++c) {
-            int s = clu_sys_h[c];
-            for (int k = 0; k < 3; ++k) { se[3 * s + k] += ep[6 * c + k]; sv[3 * s + k] += ep[6 * c + 3 + k]; }
-        }
-        for (size_t q = 0; q < se.size(); ++q) { eel[q] = se[q] / JP_ESCALE; evd[q] = sv[q] / JP_ESCALE; }
-    }
     for (int s = 0; s < nsys; ++s) {
-        for (int i = 0; i < hs[s].n; ++i) {
-            size_t a = sp[s].atom0 + i;
-            for (int k = 0; k < 3; ++k) eex[3 * s + k] += ex[3 * a + k];
-        }
         for (int k = 0; k < 3; ++k) {
             eel[3 * s + k] = eel[3 * s + k] * 0.5 + hs[s].dself[k] + eex[3 * s + k];   // enear.f90:568-571
             evd[3 * s + k] = evd[3 * s + k] * 0.5;
@@ -978,16 +1013,16 @@ int Engine::mon_record(int step, int nsteps, int nmon, int nlog, const std::vect
                        const std::vector<double> &evd, const std::vector<double> &eb, const std::vector<double> &ea,
                        jp_energy_log *log) {
     hipLaunchKernelGGL(k_ekin, grid1(natom, 256), 256, 0, st, natom, d_mass.p, d_v.p, d_vback.p, d_ekin.p);
-    std::vector<double> ek(natom);
-    int err = 0;
-    CK(hipMemcpyAsync(ek.data(), d_ekin.p, 8 * (size_t)natom, hipMemcpyDeviceToHost, st));
-    CK(hipMemcpyAsync(&err, d_err.p, 4, hipMemcpyDeviceToHost, st));
+    if (int r = seg_sum<double>(d_abeg.p, d_aend.p, 1, d_ekin.p, d_part.p, d_rsum.p + 9 * (size_t)nsys)) return r;
+    CK(hipMemcpyAsync(h_rsum + 9 * (size_t)nsys, d_rsum.p + 9 * (size_t)nsys, sizeof(double) * nsys,
+                      hipMemcpyDeviceToHost, st));
+    CK(hipMemcpyAsync(h_err, d_err.p, 4, hipMemcpyDeviceToHost, st));
     CK(hipStreamSynchronize(st));
+    const int err = *h_err;
     if (err & 8) return JP_IER_SHAKE_SING;
     if (err & 4) return JP_IER_SHAKE_CONV;
     for (int s = 0; s < nsys; ++s) {
-        double e = 0.0;
-        for (int i = 0; i < hs[s].n; ++i) e = e + ek[sp[s].atom0 + i];
+        double e = h_rsum[9 * (size_t)nsys + s];
         e = e * 1.0e7 / 4.184 * 0.0005;   // caltmp.f90:24-27
         jp_energy_log &L = log[(size_t)s * (nsteps / nmon) + nlog];
         L.step = step;
