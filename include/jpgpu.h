@@ -195,12 +195,17 @@ int jp_batch_destroy(jp_batch *b);
  *                      tolerance across values.
  *   JP_OPT_LIST_WAVES  i-clusters per block in the list kernel (1, 2, 4 = default, 8);
  *                      the list does not depend on it.
+ *   JP_OPT_LIST_SEGMENT  list entries per i-cluster segment at the next rebuild
+ *                      only (set, not read); the library grows segments by itself
+ *                      and rebuilds when one overflows, so this only exercises that
+ *                      path (results do not depend on it).
  * Setting a shape turns the autotuner off; jp_get_option reads the current
  * (possibly tuned) values. */
 #define JP_OPT_GRAPHS 1
 #define JP_OPT_AUTOTUNE 2
 #define JP_OPT_PAIR_WAVES 3
 #define JP_OPT_LIST_WAVES 4
+#define JP_OPT_LIST_SEGMENT 5
 int jp_set_option(jp_ctx *ctx, int32_t option, int32_t value);
 int jp_batch_set_option(jp_batch *b, int32_t option, int32_t value);
 int jp_get_option(jp_ctx *ctx, int32_t option, int32_t *value);

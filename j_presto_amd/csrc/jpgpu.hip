@@ -59,6 +59,10 @@ struct DBuf {
 
 inline unsigned grid1(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
 
+// finish_list(false): the deferred rebuild failed or overflowed; rebuild from
+// the coordinates it was meant for (internal, never returned to callers)
+#define JP_LIST_REDO (-50)
+
 struct HostSys {
     int n = 0, ntyp = 0, nfrag = 1, nfrg = 1, lmd = 0;
     double lambda = 0.0, dself[3] = {0, 0, 0};
@@ -84,6 +88,7 @@ struct Engine {
     long long idloop = 0;          // md.f90 idloop: steps done, kept across run calls
     int ccap = 0;                   // entries per i-cluster segment of the current list
     int ccap_want = 2048;           // segment size for the next rebuild (from the last one's maximum)
+    int ccap_force = 0;             // JP_OPT_LIST_SEGMENT: exact segment size of the next rebuild
 
     // device arrays
     DBuf<SysP> d_sp;
@@ -166,6 +171,8 @@ struct Engine {
         if (h_rsum) hipHostFree(h_rsum);
         if (h_rfix) hipHostFree(h_rfix);
         if (h_err) hipHostFree(h_err);
+        if (h_lst) hipHostFree(h_lst);
+        d_xbak.release(); d_vbak.release(); d_errbak.release();
     }
 
     int init(const jp_sys_desc *descs, int ns);
@@ -173,7 +180,16 @@ struct Engine {
         CK(hipMemcpyAsync(d_sp.p, sp.data(), sizeof(SysP) * nsys, hipMemcpyHostToDevice, st));
         return 0;
     }
-    int build_list();
+    // celprp.  defer = true: enqueue the rebuild and its status read-back and
+    // return; finish_list() later reads the status (retry = false: an
+    // overflow or error is returned as JP_LIST_REDO for the caller to rebuild
+    // from restored coordinates).
+    int build_list(bool defer = false);
+    int finish_list(bool retry);
+    int cell_err = 0;          // k_bin's flag survives a retry
+    int *h_lst = nullptr;      // pinned list status words
+    DBuf<double> d_xbak, d_vbak;   // state before a step segment that overlaps a list check
+    DBuf<int> d_errbak;
     int refresh_sorted();
     ExcessArgs excess_args() const {
         ExcessArgs a;
@@ -545,6 +561,8 @@ int Engine::init(const jp_sys_desc *descs, int ns) {
     CK(hipHostMalloc((void **)&h_rsum, sizeof(double) * 10 * (size_t)ns, hipHostMallocDefault));
     CK(hipHostMalloc((void **)&h_rfix, sizeof(long long) * 6 * (size_t)ns, hipHostMallocDefault));
     CK(hipHostMalloc((void **)&h_err, sizeof(int), hipHostMallocDefault));
+    CK(hipHostMalloc((void **)&h_lst, sizeof(int) * (2 + (size_t)ns), hipHostMallocDefault));
+    AL(d_xbak, 3 * (size_t)natom); AL(d_vbak, 3 * (size_t)natom); AL(d_errbak, 1);
     std::vector<int> abeg(ns), aend(ns), cbeg(ns), cend(ns);
     for (int s = 0; s < ns; ++s) {
         abeg[s] = sp[s].atom0; aend[s] = sp[s].atom0 + sp[s].natom;
@@ -621,7 +639,7 @@ int Engine::set_factors(int onehot) {
 
 // celprp: bin and sort the atoms, then build the j-lists into segments of
 // ccap entries per i-cluster.  One status read-back per rebuild.
-int Engine::build_list() {
+int Engine::build_list(bool defer) {
     CK(hipEventRecord(ev[2], st));
     const int nstat = 2 + nsys;
     const int nrs = std::max(std::max(ncell + 1, nslot), std::max(nsys, nstat));
@@ -642,34 +660,50 @@ int Engine::build_list() {
                        d_crec.p);
     // Segments of ccap entries per i-cluster.  ccap only grows (lists above
     // 7/8 of it report their length), so captured step graphs stay valid.
-    std::vector<int> lst(nstat);
-    int cell_err = 0;   // k_bin's flag survives a retry
+    cell_err = 0;
+    ccap = ccap_force > 0 ? ccap_force : std::max(ccap, ccap_want);
+    ccap_force = 0;
+    AL(d_ent, (size_t)nclu * ccap);
+    launch_list8(*this, ccap - ccap / 8);
+    CK(hipGetLastError());
+    CK(hipMemcpyAsync(h_lst, d_lst.p, 4 * (size_t)nstat, hipMemcpyDeviceToHost, st));
+    CK(hipEventRecord(ev[3], st));
+    if (defer) return 0;
+    return finish_list(true);
+}
+
+int Engine::finish_list(bool retry) {
+    const int nstat = 2 + nsys;
+    CK(hipEventSynchronize(ev[3]));
     for (int attempt = 0;; ++attempt) {
+        if (h_lst[1] > 0) ccap_want = std::max(ccap_want, (h_lst[1] + h_lst[1] / 4 + 255) / 256 * 256);
+        cell_err |= h_lst[0] & JP_ERR_CELL;
+        if (!(h_lst[0] & JP_ERR_LISTCAP)) break;
+        if (!retry) return JP_LIST_REDO;
+        if (attempt >= 6 || ccap >= (1 << 20)) return JP_IER_LIST;
+        ccap_want = std::max(ccap_want, 2 * ccap);
         ccap = std::max(ccap, ccap_want);
+        CK(hipMemsetAsync(d_lst.p, 0, 4 * (size_t)nstat, st));
         AL(d_ent, (size_t)nclu * ccap);
         launch_list8(*this, ccap - ccap / 8);
         CK(hipGetLastError());
-        CK(hipMemcpyAsync(lst.data(), d_lst.p, 4 * (size_t)nstat, hipMemcpyDeviceToHost, st));
-        CK(hipStreamSynchronize(st));
-        if (lst[1] > 0) ccap_want = std::max(ccap_want, (lst[1] + lst[1] / 4 + 255) / 256 * 256);
-        cell_err |= lst[0] & JP_ERR_CELL;
-        if (!(lst[0] & JP_ERR_LISTCAP)) break;
-        if (attempt >= 6 || ccap >= (1 << 20)) return JP_IER_LIST;
-        ccap_want = std::max(ccap_want, 2 * ccap);
-        CK(hipMemsetAsync(d_lst.p, 0, 4 * (size_t)nstat, st));
+        CK(hipMemcpyAsync(h_lst, d_lst.p, 4 * (size_t)nstat, hipMemcpyDeviceToHost, st));
+        CK(hipEventRecord(ev[3], st));
+        CK(hipEventSynchronize(ev[3]));
     }
-    CK(hipEventRecord(ev[3], st));
-    CK(hipEventSynchronize(ev[3]));
     float ms = 0;
     hipEventElapsedTime(&ms, ev[2], ev[3]);
     t_list += ms * 1e-3;
     n_list++;
     have_list = true;
-    if ((lst[0] | cell_err) & JP_ERR_CELL) return JP_IER_CELL;
-    if (lst[0] & JP_ERR_NBLIM) return JP_IER_LIST;            // cutoff.f90:30-46
-    for (int s = 0; s < nsys; ++s)
-        if (lst[2 + s] != 2 * hs[s].nexcl) return JP_IER_EXCL;  // cutoff.f90:90-96
-    return 0;
+    int ier = 0;
+    if ((h_lst[0] | cell_err) & JP_ERR_CELL) ier = JP_IER_CELL;
+    else if (h_lst[0] & JP_ERR_NBLIM) ier = JP_IER_LIST;            // cutoff.f90:30-46
+    else
+        for (int s = 0; s < nsys; ++s)
+            if (h_lst[2 + s] != 2 * hs[s].nexcl) ier = JP_IER_EXCL;  // cutoff.f90:90-96
+    if (ier && !retry) return JP_LIST_REDO;
+    return ier;
 }
 
 int Engine::refresh_sorted() {
@@ -1061,6 +1095,11 @@ int Engine::run_nve(int nsteps, int update, int nmon, int prec, jp_energy_log *l
     CK(hipEventRecord(ev[0], st));
     int nlog = 0;
     int step = 1;
+    // A rebuild after a graph segment is checked while the next graph segment
+    // runs: x, v and the SHAKE flags are saved first, and a failed or
+    // overflowed rebuild restores them, rebuilds synchronously and replays
+    // the segment (or returns md_run's ier from the state it refers to).
+    bool pending = false;
     while (step <= nsteps) {
         const bool mon = nmon > 0 && step % nmon == 0;
         // plain steps up to the next rebuild, energy step or the end
@@ -1074,15 +1113,38 @@ int Engine::run_nve(int nsteps, int update, int nmon, int prec, jp_energy_log *l
         if (nplain >= 2) {
             StepGraph *g = step_graph(nplain, prec);
             if (g) {
+                if (pending) {
+                    CK(hipMemcpyAsync(d_xbak.p, d_x.p, 24 * (size_t)natom, hipMemcpyDeviceToDevice, st));
+                    CK(hipMemcpyAsync(d_vbak.p, d_v.p, 24 * (size_t)natom, hipMemcpyDeviceToDevice, st));
+                    CK(hipMemcpyAsync(d_errbak.p, d_err.p, 4, hipMemcpyDeviceToDevice, st));
+                }
                 CK(hipGraphLaunch(g->exec, st));
+                if (pending) {
+                    pending = false;
+                    r = finish_list(false);
+                    if (r == JP_LIST_REDO) {
+                        CK(hipStreamSynchronize(st));
+                        CK(hipMemcpyAsync(d_x.p, d_xbak.p, 24 * (size_t)natom, hipMemcpyDeviceToDevice, st));
+                        CK(hipMemcpyAsync(d_v.p, d_vbak.p, 24 * (size_t)natom, hipMemcpyDeviceToDevice, st));
+                        CK(hipMemcpyAsync(d_err.p, d_errbak.p, 4, hipMemcpyDeviceToDevice, st));
+                        if ((r = build_list())) return r;
+                        continue;   // same segment again, on the new list
+                    }
+                    if (r) return r;
+                }
                 step += nplain;
                 idloop += nplain;
                 if (idloop % update == 0) {
-                    if ((r = build_list())) return r;
+                    if ((r = build_list(true))) return r;
+                    pending = true;
                 }
                 continue;
             }
             use_graphs = 0;   // capture unavailable: plain launches from here on
+        }
+        if (pending) {   // nothing ran after the rebuild: check it in place
+            pending = false;
+            if ((r = finish_list(true))) return r;
         }
         hipEvent_t a, b;
         CK(hipEventCreate(&a));
@@ -1112,6 +1174,7 @@ int Engine::run_nve(int nsteps, int update, int nmon, int prec, jp_energy_log *l
             if ((r = build_list())) return r;
         }
     }
+    if (pending && (r = finish_list(true))) return r;
     CK(hipEventRecord(ev[1], st));
     int err = 0;
     CK(hipMemcpyAsync(&err, d_err.p, 4, hipMemcpyDeviceToHost, st));
@@ -1372,6 +1435,10 @@ static int set_option(Engine &E, int32_t option, int32_t value) {
             if (value != 1 && value != 2 && value != 4 && value != 8) return JP_EINVAL;
             E.lw = value;
             E.autotune = 0;
+            return 0;
+        case JP_OPT_LIST_SEGMENT:
+            if (value < 8) return JP_EINVAL;
+            E.ccap_force = value;
             return 0;
         default:
             return JP_EINVAL;

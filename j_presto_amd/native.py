@@ -44,7 +44,7 @@ SIGNATURES = {
     "jp_get_option": (C.c_int, [vp, C.c_int32, i32p]),
     "jp_batch_get_option": (C.c_int, [vp, C.c_int32, i32p]),
 }
-JP_OPT_GRAPHS, JP_OPT_AUTOTUNE, JP_OPT_PAIR_WAVES, JP_OPT_LIST_WAVES = 1, 2, 3, 4
+JP_OPT_GRAPHS, JP_OPT_AUTOTUNE, JP_OPT_PAIR_WAVES, JP_OPT_LIST_WAVES, JP_OPT_LIST_SEGMENT = 1, 2, 3, 4, 5
 
 _lib = None
 

@@ -278,3 +278,28 @@ def test_launch_shapes_and_autotuner():
     assert md.get_option(native.JP_OPT_LIST_WAVES) in (1, 2, 4, 8)
     assert np.abs(md.get_state()[0] - runs[4][0]).max() < 1e-5
     md.close()
+
+
+@pytest.mark.parametrize("prec", [engine.FP64, engine.FP32])
+def test_list_overflow_during_overlapped_check(prec):
+    """A rebuild whose status is read while the next step segment already
+    runs: a segment overflow there restores the state, rebuilds and replays
+    the segment; the trajectory is the same bits as without the overflow."""
+    from j_presto_amd import native
+    s = systems.water_box(700, 28.0, 23, n_ion_pairs=2)
+    print(f"case overflow prec={prec}", flush=True)
+    out = []
+    for force in (False, True):
+        md = engine.MdSystem(s, precision=prec)
+        _, ier = md.runmd(5, update=10)
+        assert ier == 0
+        if force:   # the rebuild after step 10 gets segments far too small
+            md.set_option(native.JP_OPT_LIST_SEGMENT, 64)
+        log, ier = md.runmd(35, update=10, nmon=5)
+        assert ier == 0
+        out.append((md.get_state(), log["etot"].copy()))
+        md.close()
+    (x0, v0), e0 = out[0]
+    (x1, v1), e1 = out[1]
+    assert np.array_equal(x0, x1) and np.array_equal(v0, v1)
+    assert np.array_equal(e0, e1)
