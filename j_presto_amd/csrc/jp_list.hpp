@@ -146,11 +146,11 @@ __device__ __forceinline__ CluRange clu_range(const SysP &P, const double *__res
         r.ulo[k] = (r.bad || empty) ? 0 : (int)a;
         r.n[k] = (r.bad || empty) ? 0 : (int)(b - a) + 1;
     }
+    // candidate indices stay below 2^22 (clu_cell's FP32 division)
+    if (!empty && (double)r.n[0] * r.n[1] * r.n[2] >= 4194304.0) r.bad = true;
     r.ncand = (r.bad || empty) ? 0 : r.n[0] * r.n[1] * r.n[2];
     return r;
 }
-
-__device__ __forceinline__ int floordiv(int a, int b) { return a >= 0 ? a / b : -((-a + b - 1) / b); }
 
 // Per box-grid position: {first slot, atoms, ext bits, 0} of the cell there
 // (one load per candidate cell in k_list8).
@@ -166,11 +166,23 @@ __global__ void k_crec(int ncell, const int *__restrict__ crank, const int *__re
 // unwrapped cell (grown by 2*its ext) is farther than rlim from the cluster
 // box.  Shifts of two or more boxes are never within rlim (every edge is
 // >= 2*cutoff).
+// q / n for 0 <= q < 2^22, 1 <= n < 2^10: FP32 reciprocal estimate, then one
+// correction each way (exact).
+__device__ __forceinline__ int udiv_small(int q, int n, float rn) {
+    int d = (int)((float)q * rn);
+    d -= (d * n > q) ? 1 : 0;
+    d += ((d + 1) * n <= q) ? 1 : 0;
+    return d;
+}
+
 __device__ __forceinline__ void clu_cell(const SysP &P, const CluRange &r, const int4 *__restrict__ crec, int q,
                                          uint32_t &code, int &k0, int &k1) {
     k0 = k1 = 0;
-    const int ux = r.ulo[0] + q % r.n[0], uy = r.ulo[1] + (q / r.n[0]) % r.n[1], uz = r.ulo[2] + q / (r.n[0] * r.n[1]);
-    const int sx = floordiv(ux, P.nc[0]), sy = floordiv(uy, P.nc[1]), sz = floordiv(uz, P.nc[2]);
+    const int q1 = udiv_small(q, r.n[0], 1.0f / (float)r.n[0]);
+    const int q2 = udiv_small(q1, r.n[1], 1.0f / (float)r.n[1]);
+    const int ux = r.ulo[0] + (q - q1 * r.n[0]), uy = r.ulo[1] + (q1 - q2 * r.n[1]), uz = r.ulo[2] + q2;
+    // nc = 2^hbits per edge: floor division is an arithmetic shift
+    const int sx = ux >> P.hbits, sy = uy >> P.hbits, sz = uz >> P.hbits;
     if (sx < -1 || sx > 1 || sy < -1 || sy > 1 || sz < -1 || sz > 1) return;
     const int cx = ux - sx * P.nc[0], cy = uy - sy * P.nc[1], cz = uz - sz * P.nc[2];
     const int4 cr = crec[P.cell0 + (cz * P.nc[1] + cy) * P.nc[0] + cx];
@@ -226,12 +238,15 @@ k_list8(int nclu, const int *__restrict__ clu_sys, const SysP *__restrict__ sp, 
     __shared__ uint32_t s_q[LW][JP_QCAP];   // candidates whose box is within rlim
     __shared__ double4 s_xr[LW][JP_CL];
     __shared__ int4 s_im[LW][JP_CL];
+    __shared__ float4 s_off[LW][27];        // per image code: L*s - cluster centre (FP32)
+    __shared__ int s_mol[LW][JP_CL];
     const int l = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // wave-uniform
     const int ci = blockIdx.x * LW + w;
     if (ci >= nclu) return;   // whole wavefront; no block-wide barrier below
     const int s = clu_sys[ci];
     const SysP &P = sp[s];
     const CluRange r = clu_range(P, cbb, ci, __uint_as_float(sysext[s]));
+    if (r.bad && l == 0) atomicOr(lst, JP_ERR_CELL);   // runaway coordinates: md_run's ier 28
     const float4 cf = ccf[ci];
 
     // the 8 i atoms: FP32 coordinates relative to the cluster centre, molecule, atom index
@@ -250,16 +265,21 @@ k_list8(int nclu, const int *__restrict__ clu_sys, const SysP *__restrict__ sp, 
         }
         s_xr[w][l] = xr[sl];
         s_im[w][l] = im;
+        s_mol[w][l] = mi;
+    }
+    if (l < 27) {
+        int sx, sy, sz;
+        shift_of((uint32_t)l, sx, sy, sz);
+        s_off[w][l] = make_float4((float)P.L[0] * (float)sx - cf.x, (float)P.L[1] * (float)sy - cf.y,
+                                  (float)P.L[2] * (float)sz - cf.z, 0.f);
     }
     // uniform copies (scalar registers); lane k keeps atom k's index and count
     float xi[JP_CL], yi[JP_CL], zi[JP_CL];
-    int moli[JP_CL];
 #pragma unroll
     for (int k = 0; k < JP_CL; ++k) {
         xi[k] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(fx), k));
         yi[k] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(fy), k));
         zi[k] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(fz), k));
-        moli[k] = __builtin_amdgcn_readlane(mi, k);
     }
     const uint32_t ivalid = (uint32_t)__ballot(l < JP_CL && ai >= 0);
     // listed partners per i atom: per-lane byte counters (atom k in byte k&3 of
@@ -282,7 +302,6 @@ k_list8(int nclu, const int *__restrict__ clu_sys, const SysP *__restrict__ sp, 
     const float rlo = (float)(P.rlim2 - P.band), rhi = (float)(P.rlim2 + P.band);
     // any d2 in [rlo, rhi] has |d2 - rlim2| <= band + rounding: a looser trigger
     const float rl2f = (float)P.rlim2, trig = (float)P.band + 0.01f;
-    const float Lx = (float)P.L[0], Ly = (float)P.L[1], Lz = (float)P.L[2];
     const bool small = P.small != 0;
     int nA = 0, nB = 0, nrem = 0, qh = 0, qt = 0;
     bool ovf = false;
@@ -295,8 +314,6 @@ k_list8(int nclu, const int *__restrict__ clu_sys, const SysP *__restrict__ sp, 
         const int jc = (int)(qe & JP_SLOT_MASK);
         const uint32_t code = qe >> JP_SLOT_BITS;
         const int js = jc * JP_CL + ja;
-        int sx, sy, sz;
-        shift_of(code, sx, sy, sz);
         uint32_t m = 0, amb = 0;
         int meta = -1;
         float4 q = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -305,9 +322,10 @@ k_list8(int nclu, const int *__restrict__ clu_sys, const SysP *__restrict__ sp, 
             meta = __float_as_int(q.w);
         }
         const int molj = meta & 0x3fffffff;
-        const float rx = (q.x - cf.x) + Lx * (float)sx;
-        const float ry = (q.y - cf.y) + Ly * (float)sy;
-        const float rz = (q.z - cf.z) + Lz * (float)sz;
+        // j relative to the cluster centre, image applied: 3 roundings like
+        // (q - cf) + L*s, within the error bound of the band
+        const float4 off = s_off[w][code];
+        const float rx = q.x + off.x, ry = q.y + off.y, rz = q.z + off.z;
         float amin = 3.0e38f;
         if (meta >= 0) {
             // bit k = [d2 < rlo]: the sign of d2 - rlo shifted in, atom 7 first
@@ -337,6 +355,8 @@ k_list8(int nclu, const int *__restrict__ clu_sys, const SysP *__restrict__ sp, 
                 amb &= ivalid;
             }
             if (amb) {
+                int sx, sy, sz;
+                shift_of(code, sx, sy, sz);
                 const double4 xj = xr[js];
                 const int4 ij = imgs[js];
 #pragma unroll 1
@@ -350,7 +370,8 @@ k_list8(int nclu, const int *__restrict__ clu_sys, const SysP *__restrict__ sp, 
         // equal ids (ids < 2^30)
         uint32_t same = 0;
 #pragma unroll
-        for (int k = JP_CL - 1; k >= 0; --k) same = __builtin_amdgcn_alignbit(same, (uint32_t)(molj ^ moli[k]) - 1u, 31);
+        for (int k = JP_CL - 1; k >= 0; --k)
+            same = __builtin_amdgcn_alignbit(same, (uint32_t)(molj ^ s_mol[w][k]) - 1u, 31);
         same &= m;
         if (__ballot(same != 0)) {
             const int aj = same ? imgs[js].w : -1;
