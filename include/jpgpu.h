@@ -89,6 +89,29 @@ typedef struct jp_sys_desc {
     const double *angle_fc;    /* fyfang: E = fc (theta - theta0)^2, kcal/mol/rad^2 */
     const double *angle_theta0;/* fyqang, radians */
     const int32_t *angle_cls;  /* angcls, 1-based; NULL: max(n_matrix) */
+    /* Torsions and improper torsions of dnei14_PB / dnetor_PB (dnergv.f90:197-429),
+     * as incel leaves them (force constants below epsilon dropped,
+     * incel.f90:993-1061): periodicity (1-6), phase in radians, division number. */
+    int32_t ntors;             /* ntor */
+    const int32_t *tors_atoms; /* [ntors][4] iyptor, 1-based */
+    const double *tors_fc;     /* fyftor: E = fc (1 + cos(n phi + phase)) / div */
+    const double *tors_rot;    /* fytrot (n) */
+    const double *tors_phase;  /* fytphs */
+    const int32_t *tors_div;   /* iytdiv */
+    const int32_t *tors_cls;   /* torcls, 1-based; NULL: n_matrix of the two middle atoms (SCALED_TERM default) */
+    int32_t nimp;              /* nimp: same layout (iypimp, fyfimp, fyirot, fyiphs, iyidiv) */
+    const int32_t *imp_atoms;
+    const double *imp_fc;
+    const double *imp_rot;
+    const double *imp_phase;
+    const int32_t *imp_div;
+    const int32_t *imp_cls;    /* impcls, 1-based; NULL: max(n_matrix) */
+    /* 1-4 pairs of dne14_PB (dnergv.f90:433-516, incel.f90:975-984, 1084-1087) */
+    int32_t n14;               /* n14int */
+    const int32_t *p14_atoms;  /* [n14][2] iyp14, 1-based */
+    const double *p14_vdw;     /* fytvws14: 1-4 LJ scale */
+    const double *p14_elec;    /* fytess14 after incel: scale * ccoef * q_i * q_j */
+    const int32_t *p14_cls;    /* i14cls, 1-based; NULL: n_matrix of the pair */
 } jp_sys_desc;
 
 /* Per-monitoring-step record of jp_run_nve (monmd.f90 / md.f90:618-628). */
@@ -101,6 +124,10 @@ typedef struct jp_energy_log {
     double evdw[3];            /* VDW15 per energy class (enevcc(8,:)) */
     double ebond[3];           /* BOND per energy class (enevcc(2,:)) */
     double eangle[3];          /* ANGLE per energy class (enevcc(3,:)) */
+    double etors[3];           /* TORSION (enevcc(4,:)) */
+    double eimp[3];            /* IMPROPER (enevcc(5,:)) */
+    double evdw14[3];          /* 1-4 VDW (enevcc(6,:)) */
+    double eelec14[3];         /* 1-4 ELEC (enevcc(7,:)) */
 } jp_energy_log;
 
 typedef struct jp_ctx jp_ctx;
@@ -134,10 +161,12 @@ int jp_get_list(jp_ctx *ctx, int32_t *cell_of_atom, int64_t *npairs, int32_t *pt
 int jp_nonbond(jp_ctx *ctx, const double *cord, double *grad, double *eelec, double *evdw,
                double *ehyd, int precision);
 
-/* dnebnd_PB + dneang_PB: harmonic bond and angle terms.  Accumulates into
- * grad(3,N,nfrg) and writes the per-class energies (dnergv.f90:62-193).
- * cord NULL: use device coordinates; grad NULL: energies only. */
-int jp_bonded(jp_ctx *ctx, const double *cord, double *grad, double *ebond, double *eangle);
+/* dnergv_PB's bonded part: dnebnd_PB, dneang_PB and dnei14_PB (torsions,
+ * impropers, 1-4 LJ and electrostatics), dnergv.f90:29-516.  Accumulates into
+ * grad(3,N,nfrg) and writes eterm(6,nfrg) = enevcc(2:7,1:nfrg) (bond, angle,
+ * torsion, improper, 1-4 vdW, 1-4 elec).  cord NULL: device coordinates;
+ * grad NULL: energies only; eterm may be NULL. */
+int jp_bonded(jp_ctx *ctx, const double *cord, double *grad, double *eterm);
 
 /* shake_PB: constrain cord (x(t+dt)) against cord_old (x(t)) in place.
  * (shake.f90:2-755) */

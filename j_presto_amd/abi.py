@@ -29,13 +29,19 @@ class SysDesc(C.Structure):
         ("nbond", C.c_int32), ("bond_atoms", i32p), ("bond_fc", f64p), ("bond_r0", f64p), ("bond_cls", i32p),
         ("nangle", C.c_int32), ("angle_atoms", i32p), ("angle_fc", f64p), ("angle_theta0", f64p),
         ("angle_cls", i32p),
+        ("ntors", C.c_int32), ("tors_atoms", i32p), ("tors_fc", f64p), ("tors_rot", f64p), ("tors_phase", f64p),
+        ("tors_div", i32p), ("tors_cls", i32p),
+        ("nimp", C.c_int32), ("imp_atoms", i32p), ("imp_fc", f64p), ("imp_rot", f64p), ("imp_phase", f64p),
+        ("imp_div", i32p), ("imp_cls", i32p),
+        ("n14", C.c_int32), ("p14_atoms", i32p), ("p14_vdw", f64p), ("p14_elec", f64p), ("p14_cls", i32p),
     ]
 
 
 class EnergyLog(C.Structure):
     _fields_ = [("step", C.c_int32), ("etot", C.c_double), ("epot", C.c_double), ("ekin", C.c_double),
                 ("eelec", C.c_double * 3), ("evdw", C.c_double * 3), ("ebond", C.c_double * 3),
-                ("eangle", C.c_double * 3)]
+                ("eangle", C.c_double * 3), ("etors", C.c_double * 3), ("eimp", C.c_double * 3),
+                ("evdw14", C.c_double * 3), ("eelec14", C.c_double * 3)]
 
 
 def n_matrix(nfrag: int, method: str) -> np.ndarray:
@@ -192,6 +198,37 @@ class DescHolder:
         d.angle_atoms = self.angle_atoms.ctypes.data_as(i32p)
         d.angle_fc = self.angle_fc.ctypes.data_as(f64p)
         d.angle_theta0 = self.angle_theta0.ctypes.data_as(f64p)
+        # torsions incel keeps (|fc| >= epsilon, incel.f90:993-1009) and the 1-4
+        # pairs of every torsion with the 1-4 flag (incel.f90:975-984); AMBER 1-4
+        # scales min over the two types (inptpl.f90 mknopm), times ccoef q_i q_j
+        tors = s.torsions()
+        tfc, tdiv, trot, tph, tnbf = s.torsion_params()
+        keep_t = np.abs(tfc) >= np.finfo(np.float64).eps
+        self.tors_atoms = np.ascontiguousarray(tors[keep_t] + 1, np.int32).reshape(-1, 4)
+        self.tors_fc = np.ascontiguousarray(tfc[keep_t], np.float64)
+        self.tors_rot = np.ascontiguousarray(trot[keep_t], np.float64)
+        self.tors_phase = np.ascontiguousarray(tph[keep_t] * (np.pi / 180.0), np.float64)
+        self.tors_div = np.ascontiguousarray(tdiv[keep_t], np.int32)
+        sel14 = tnbf == 1
+        self.p14_atoms = np.ascontiguousarray(tors[sel14][:, [0, 3]] + 1, np.int32).reshape(-1, 2)
+        sv = np.array([t_.scale14_vdw for t_ in s.types]); se = np.array([t_.scale14_elec for t_ in s.types])
+        at = s.atype - 1
+        a1, a4 = tors[sel14][:, 0], tors[sel14][:, 3]
+        self.p14_vdw = np.ascontiguousarray(np.minimum(sv[at[a1]], sv[at[a4]]) * tnbf[sel14], np.float64)
+        ess = np.minimum(se[at[a1]], se[at[a4]]) * tnbf[sel14]
+        ccoef = 332.06378 / s.diel
+        q = s.charge
+        self.p14_elec = np.ascontiguousarray(ess * ccoef * q[a1] * q[a4], np.float64)
+        d.ntors = len(self.tors_fc)
+        d.tors_atoms = self.tors_atoms.ctypes.data_as(i32p)
+        d.tors_fc = self.tors_fc.ctypes.data_as(f64p)
+        d.tors_rot = self.tors_rot.ctypes.data_as(f64p)
+        d.tors_phase = self.tors_phase.ctypes.data_as(f64p)
+        d.tors_div = self.tors_div.ctypes.data_as(i32p)
+        d.n14 = len(self.p14_vdw)
+        d.p14_atoms = self.p14_atoms.ctypes.data_as(i32p)
+        d.p14_vdw = self.p14_vdw.ctypes.data_as(f64p)
+        d.p14_elec = self.p14_elec.ctypes.data_as(f64p)
         self.desc = d
         self.nfrg = nfrg
         self.n = n

@@ -117,8 +117,9 @@ struct Engine {
     DBuf<int4> d_s2a, d_s3a;
     DBuf<double> d_s2d, d_s3d;
     DBuf<unsigned long long> d_u64;
-    // bonded terms (jp_bonded.hpp): bonds, angles, per-atom gather lists
-    int nbt = 0, nat = 0, nbat = 0;
+    // bonded terms (jp_bonded.hpp): bonds, angles, torsions (+ impropers),
+    // 1-4 pairs, per-atom gather lists
+    int nbt = 0, nat = 0, ntt = 0, n14 = 0, nbat = 0;
     int nunit = 0;             // k_step units: SHAKE groups and free atoms
     DBuf<int4> d_unit;
     // energy sums on the device: per-system atom and cluster ranges, results,
@@ -139,10 +140,13 @@ struct Engine {
     double *h_rsum = nullptr;
     long long *h_rfix = nullptr;
     int *h_err = nullptr;
-    std::vector<int> tsys_h, tcls_h;   // system and class of every term (bonds, then angles)
-    DBuf<int> d_bat, d_bptr, d_bref;
-    DBuf<int4> d_bnd, d_ang;
-    DBuf<double2> d_bndp, d_angp;
+    // per energy entry of k_bonded_energy: system, class, energy slot (0..5 =
+    // bond, angle, torsion, improper, 1-4 vdW, 1-4 elec)
+    std::vector<int> esys_h, ecls_h, eslot_h;
+    DBuf<int> d_bat, d_bptr, d_bref, d_torc;
+    DBuf<int4> d_bnd, d_ang, d_tor, d_p14;
+    DBuf<double2> d_bndp, d_angp, d_p14p;
+    DBuf<double4> d_torp;
     DBuf<double> d_eterm;
 
     ~Engine() {
@@ -166,6 +170,7 @@ struct Engine {
         d_u64.release();
         d_bat.release(); d_bptr.release(); d_bref.release(); d_bnd.release(); d_ang.release();
         d_bndp.release(); d_angp.release(); d_eterm.release(); d_unit.release();
+        d_torc.release(); d_tor.release(); d_p14.release(); d_p14p.release(); d_torp.release();
         d_abeg.release(); d_aend.release(); d_cbeg.release(); d_cend.release(); d_rsum.release(); d_rfix.release();
         d_part.release(); d_partf.release();
         if (h_rsum) hipHostFree(h_rsum);
@@ -202,11 +207,13 @@ struct Engine {
         BondedArgs b;
         b.nbat = nbat; b.bat = d_bat.p; b.bptr = d_bptr.p; b.bref = d_bref.p;
         b.bnd = d_bnd.p; b.ang = d_ang.p; b.bndp = d_bndp.p; b.angp = d_angp.p;
+        b.tor = d_tor.p; b.torp = d_torp.p; b.torc = d_torc.p; b.p14 = d_p14.p; b.p14p = d_p14p.p;
+        b.tflag = d_tflag.p; b.lj64 = d_lj64.p;
         return b;
     }
     int forces(int prec, bool energy);   // pair kernel, then the bonded terms into the same slots
-    int bonded_energies(std::vector<double> &eb, std::vector<double> &ea);
-    int bonded(const double *cord, double *grad, double *ebond, double *eangle);
+    int bonded_energies(std::vector<double> &et);   // [nsys][3 classes][6 slots]
+    int bonded(const double *cord, double *grad, double *eterm);
     int set_factors(int onehot_class);  // -1: run factors (lambda), k: one-hot class k
     int nonbond(const double *cord, double *grad, double *eelec, double *evdw, int prec);
     int energies(int prec, std::vector<double> &eel, std::vector<double> &evd, std::vector<double> &eex);
@@ -214,8 +221,7 @@ struct Engine {
     int run_nve(int nsteps, int update, int nmon, int prec, jp_energy_log *log);
     int integrate(int prec, bool mon);   // kick + SHAKE + v update of one step (after the forces)
     int mon_record(int step, int nsteps, int nmon, int nlog, const std::vector<double> &eel,
-                   const std::vector<double> &evd, const std::vector<double> &eb, const std::vector<double> &ea,
-                   jp_energy_log *log);
+                   const std::vector<double> &evd, const std::vector<double> &et, jp_energy_log *log);
 
     // Steps between list rebuilds and energy steps run as captured HIP graphs:
     // nsteps x (pair, kick, SHAKE, post), cached per shape.  Pair-kernel
@@ -258,10 +264,13 @@ int Engine::init(const jp_sys_desc *descs, int ns) {
     std::vector<int4> s2a, s3a;
     std::vector<double> s2d, s3d;
     std::vector<int> crank, ccoord;
-    std::vector<int4> bnd, ang;
-    std::vector<double2> bndp, angp;
-    std::vector<std::vector<int>> bref_of;   // per global atom: term codes, angles after bonds
-    std::vector<int> asys, acls;             // angles' system / class until they follow the bonds
+    std::vector<int4> bnd, ang, tor, p14;
+    std::vector<double2> bndp, angp, p14p;
+    std::vector<double4> torp;
+    std::vector<int> torc;
+    std::vector<std::vector<int>> bref_of;   // per global atom: term codes in kind order
+    // energy-entry metadata per kind (system, class, slot), concatenated after the loop
+    std::vector<int> em_sys[4], em_cls[4], em_slot[4];
     int atom0 = 0, slot0 = 0, cell0 = 0, lj0 = 0;
     shake_tol = descs[0].shake_tol > 0 ? descs[0].shake_tol : 1e-6;
     shake_maxit = descs[0].shake_maxit > 0 ? descs[0].shake_maxit : 1000;
@@ -465,9 +474,9 @@ int Engine::init(const jp_sys_desc *descs, int ns) {
                 const int t = (int)bnd.size();
                 bnd.push_back(make_int4(i + atom0, j + atom0, c, 0));
                 bndp.push_back(make_double2(d.bond_fc[b], d.bond_r0[b]));
-                tsys_h.push_back(s); tcls_h.push_back(c);
-                bref_of[i + atom0].push_back(bref_code(t, false, 0));
-                bref_of[j + atom0].push_back(bref_code(t, false, 1));
+                em_sys[0].push_back(s); em_cls[0].push_back(c); em_slot[0].push_back(0);
+                bref_of[i + atom0].push_back(bref_code(t, JP_BK_BOND, 0));
+                bref_of[j + atom0].push_back(bref_code(t, JP_BK_BOND, 1));
             }
             for (int g = 0; g < d.nangle; ++g) {
                 const int i = d.angle_atoms[3 * g] - 1, j = d.angle_atoms[3 * g + 1] - 1, k = d.angle_atoms[3 * g + 2] - 1;
@@ -478,10 +487,54 @@ int Engine::init(const jp_sys_desc *descs, int ns) {
                 const int t = (int)ang.size();
                 ang.push_back(make_int4(i + atom0, j + atom0, k + atom0, c));
                 angp.push_back(make_double2(d.angle_fc[g], d.angle_theta0[g]));
-                asys.push_back(s); acls.push_back(c);
-                bref_of[i + atom0].push_back(bref_code(t, true, 0));
-                bref_of[j + atom0].push_back(bref_code(t, true, 1));
-                bref_of[k + atom0].push_back(bref_code(t, true, 2));
+                em_sys[1].push_back(s); em_cls[1].push_back(c); em_slot[1].push_back(1);
+                bref_of[i + atom0].push_back(bref_code(t, JP_BK_ANGLE, 0));
+                bref_of[j + atom0].push_back(bref_code(t, JP_BK_ANGLE, 1));
+                bref_of[k + atom0].push_back(bref_code(t, JP_BK_ANGLE, 2));
+            }
+            // torsions, then impropers (dnei14_PB order); class defaults: torsions
+            // by their middle atoms (SCALED_TERM default), impropers max(n_matrix)
+            if (d.ntors < 0 || d.nimp < 0 || d.n14 < 0) return JP_EINVAL;
+            if ((d.ntors && (!d.tors_atoms || !d.tors_fc || !d.tors_rot || !d.tors_phase || !d.tors_div)) ||
+                (d.nimp && (!d.imp_atoms || !d.imp_fc || !d.imp_rot || !d.imp_phase || !d.imp_div)) ||
+                (d.n14 && (!d.p14_atoms || !d.p14_vdw || !d.p14_elec)))
+                return JP_EINVAL;
+            for (int q = 0; q < d.ntors + d.nimp; ++q) {
+                const bool imp = q >= d.ntors;
+                const int u = imp ? q - d.ntors : q;
+                const int32_t *at = imp ? d.imp_atoms + 4 * u : d.tors_atoms + 4 * u;
+                int a4[4];
+                for (int k = 0; k < 4; ++k) {
+                    a4[k] = at[k] - 1;
+                    if (a4[k] < 0 || a4[k] >= n) return JP_EINVAL;
+                }
+                const double rot = imp ? d.imp_rot[u] : d.tors_rot[u];
+                const int divn = imp ? d.imp_div[u] : d.tors_div[u];
+                const double nr = std::round(rot);
+                if (!(nr >= 1.0 && nr <= 6.0) || divn == 0) return JP_EINVAL;   // dnetor_PB handles n = 1..6
+                const int32_t *cl = imp ? d.imp_cls : d.tors_cls;
+                const int c = cl ? cl[u] - 1
+                                 : (imp ? cmax : d.n_matrix[(d.icls[a4[2]] - 1) * d.nfrag + (d.icls[a4[1]] - 1)] - 1);
+                if (c < 0 || c >= d.nfrg) return JP_EINVAL;
+                const int t = (int)tor.size();
+                tor.push_back(make_int4(a4[0] + atom0, a4[1] + atom0, a4[2] + atom0, a4[3] + atom0));
+                torp.push_back(make_double4(imp ? d.imp_fc[u] : d.tors_fc[u], rot, imp ? d.imp_phase[u] : d.tors_phase[u],
+                                            (double)divn));
+                torc.push_back(c | (imp ? 256 : 0));
+                em_sys[2].push_back(s); em_cls[2].push_back(c); em_slot[2].push_back(imp ? 3 : 2);
+                for (int k = 0; k < 4; ++k) bref_of[a4[k] + atom0].push_back(bref_code(t, JP_BK_TORS, k));
+            }
+            for (int q = 0; q < d.n14; ++q) {
+                const int i = d.p14_atoms[2 * q] - 1, j = d.p14_atoms[2 * q + 1] - 1;
+                if (i < 0 || j < 0 || i >= n || j >= n || i == j) return JP_EINVAL;
+                const int c = d.p14_cls ? d.p14_cls[q] - 1 : d.n_matrix[(d.icls[j] - 1) * d.nfrag + (d.icls[i] - 1)] - 1;
+                if (c < 0 || c >= d.nfrg) return JP_EINVAL;
+                const int t = (int)p14.size();
+                p14.push_back(make_int4(i + atom0, j + atom0, c, 0));
+                p14p.push_back(make_double2(d.p14_vdw[q], d.p14_elec[q]));
+                for (int e = 0; e < 2; ++e) { em_sys[3].push_back(s); em_cls[3].push_back(c); em_slot[3].push_back(4 + e); }
+                bref_of[i + atom0].push_back(bref_code(t, JP_BK_P14, 0));
+                bref_of[j + atom0].push_back(bref_code(t, JP_BK_P14, 1));
             }
         }
         for (int i = 0; i < npad; ++i) slot_sys_h.push_back(s);
@@ -492,10 +545,14 @@ int Engine::init(const jp_sys_desc *descs, int ns) {
         lj0 += nt * nt;
     }
     natom = atom0; nslot = slot0; nclu = slot0 / JP_CL; ncell = cell0;
-    // bonded gather lists: atoms with terms, their codes (bonds, then angles)
-    nbt = (int)bnd.size(); nat = (int)ang.size();
-    tsys_h.insert(tsys_h.end(), asys.begin(), asys.end());
-    tcls_h.insert(tcls_h.end(), acls.begin(), acls.end());
+    // bonded gather lists: atoms with terms and their codes; energy entries in
+    // k_bonded_energy order (bonds, angles, torsions, 1-4 pairs x 2)
+    nbt = (int)bnd.size(); nat = (int)ang.size(); ntt = (int)tor.size(); n14 = (int)p14.size();
+    for (int k = 0; k < 4; ++k) {
+        esys_h.insert(esys_h.end(), em_sys[k].begin(), em_sys[k].end());
+        ecls_h.insert(ecls_h.end(), em_cls[k].begin(), em_cls[k].end());
+        eslot_h.insert(eslot_h.end(), em_slot[k].begin(), em_slot[k].end());
+    }
     std::vector<int> bat, bptr{0}, bref;
     bref_of.resize(natom);
     for (int a = 0; a < natom; ++a) {
@@ -553,7 +610,8 @@ int Engine::init(const jp_sys_desc *descs, int ns) {
     AL(d_u64, 3);
     AL(d_bat, bat.size()); AL(d_bptr, bptr.size()); AL(d_bref, bref.size());
     AL(d_bnd, bnd.size()); AL(d_bndp, bndp.size()); AL(d_ang, ang.size()); AL(d_angp, angp.size());
-    AL(d_eterm, (size_t)nbt + nat);
+    AL(d_eterm, (size_t)nbt + nat + ntt + 2 * (size_t)n14);
+    AL(d_tor, tor.size()); AL(d_torp, torp.size()); AL(d_torc, torc.size()); AL(d_p14, p14.size()); AL(d_p14p, p14p.size());
     AL(d_unit, units.size());
     AL(d_abeg, ns); AL(d_aend, ns); AL(d_cbeg, ns); AL(d_cend, ns);
     AL(d_rsum, 10 * (size_t)ns); AL(d_rfix, 6 * (size_t)ns);
@@ -605,6 +663,11 @@ int Engine::init(const jp_sys_desc *descs, int ns) {
     r |= H2D(d_bndp.p, bndp.data(), 16 * bndp.size());
     r |= H2D(d_ang.p, ang.data(), 16 * ang.size());
     r |= H2D(d_angp.p, angp.data(), 16 * angp.size());
+    r |= H2D(d_tor.p, tor.data(), 16 * tor.size());
+    r |= H2D(d_torp.p, torp.data(), 32 * torp.size());
+    r |= H2D(d_torc.p, torc.data(), 4 * torc.size());
+    r |= H2D(d_p14.p, p14.data(), 16 * p14.size());
+    r |= H2D(d_p14p.p, p14p.data(), 16 * p14p.size());
     r |= H2D(d_unit.p, units.data(), 16 * units.size());
     r |= H2D(d_abeg.p, abeg.data(), 4 * abeg.size());
     r |= H2D(d_aend.p, aend.data(), 4 * aend.size());
@@ -759,29 +822,28 @@ int Engine::forces(int prec, bool energy) {
     return 0;
 }
 
-// Per-system, per-class bond and angle energies at the device coordinates,
-// summed in term order (dnebnd_PB / dneang_PB).
-int Engine::bonded_energies(std::vector<double> &eb, std::vector<double> &ea) {
-    eb.assign(3 * (size_t)nsys, 0.0);
-    ea.assign(3 * (size_t)nsys, 0.0);
-    const int nt = nbt + nat;
-    if (!nt) return 0;
-    hipLaunchKernelGGL(k_bonded_energy, grid1(nt, 256), 256, 0, st, nbt, nat, bonded_args(), d_atom_sys.p, d_sp.p,
-                       d_x.p, d_eterm.p);
+// Per-system, per-class bonded energies at the device coordinates, summed
+// per energy slot in term order (dnergv_PB's routines): et[(s*3 + class)*6 + slot].
+int Engine::bonded_energies(std::vector<double> &et) {
+    et.assign(18 * (size_t)nsys, 0.0);
+    const int ne = nbt + nat + ntt + 2 * n14;
+    if (!ne) return 0;
+    hipLaunchKernelGGL(k_bonded_energy, grid1(ne, 256), 256, 0, st, nbt, nat, ntt, n14, bonded_args(), d_atom_sys.p,
+                       d_sp.p, d_x.p, d_eterm.p);
     CK(hipGetLastError());
-    std::vector<double> e(nt);
-    CK(hipMemcpyAsync(e.data(), d_eterm.p, 8 * (size_t)nt, hipMemcpyDeviceToHost, st));
+    std::vector<double> e(ne);
+    CK(hipMemcpyAsync(e.data(), d_eterm.p, 8 * (size_t)ne, hipMemcpyDeviceToHost, st));
     CK(hipStreamSynchronize(st));
-    for (int q = 0; q < nt; ++q) {
-        double &dst = (q < nbt ? eb : ea)[3 * (size_t)tsys_h[q] + tcls_h[q]];
+    for (int q = 0; q < ne; ++q) {
+        double &dst = et[(3 * (size_t)esys_h[q] + ecls_h[q]) * 6 + eslot_h[q]];
         dst = dst + e[q];
     }
     return 0;
 }
 
-// dnebnd_PB + dneang_PB for jp_bonded: per-class gradient accumulated into
-// grad(3,N,nfrg), per-class energies.
-int Engine::bonded(const double *cord, double *grad, double *ebond, double *eangle) {
+// dnergv_PB's bonded part for jp_bonded: per-class gradient accumulated into
+// grad(3,N,nfrg), per-class energies eterm(6,nfrg).
+int Engine::bonded(const double *cord, double *grad, double *eterm) {
     int r;
     if (cord) CK(hipMemcpyAsync(d_x.p, cord, 24 * (size_t)natom, hipMemcpyHostToDevice, st));
     const int nfrg = hs[0].nfrg;
@@ -800,12 +862,11 @@ int Engine::bonded(const double *cord, double *grad, double *ebond, double *eang
         }
         if ((r = set_factors(-1))) return r;
     }
-    std::vector<double> eb, ea;
-    if ((r = bonded_energies(eb, ea))) return r;
-    for (int k = 0; k < nfrg; ++k) {
-        if (ebond) ebond[k] = eb[k];
-        if (eangle) eangle[k] = ea[k];
-    }
+    std::vector<double> et;
+    if ((r = bonded_energies(et))) return r;
+    if (eterm)
+        for (int k = 0; k < nfrg; ++k)
+            for (int q = 0; q < 6; ++q) eterm[6 * k + q] = et[6 * k + q];
     CK(hipStreamSynchronize(st));
     return 0;
 }
@@ -1044,8 +1105,7 @@ Engine::StepGraph *Engine::step_graph(int n, int prec) {
 }
 
 int Engine::mon_record(int step, int nsteps, int nmon, int nlog, const std::vector<double> &eel,
-                       const std::vector<double> &evd, const std::vector<double> &eb, const std::vector<double> &ea,
-                       jp_energy_log *log) {
+                       const std::vector<double> &evd, const std::vector<double> &et, jp_energy_log *log) {
     hipLaunchKernelGGL(k_ekin, grid1(natom, 256), 256, 0, st, natom, d_mass.p, d_v.p, d_vback.p, d_ekin.p);
     if (int r = seg_sum<double>(d_abeg.p, d_aend.p, 1, d_ekin.p, d_part.p, d_rsum.p + 9 * (size_t)nsys)) return r;
     CK(hipMemcpyAsync(h_rsum + 9 * (size_t)nsys, d_rsum.p + 9 * (size_t)nsys, sizeof(double) * nsys,
@@ -1066,9 +1126,15 @@ int Engine::mon_record(int step, int nsteps, int nmon, int nlog, const std::vect
             const double f = (hs[s].lmd && hs[s].nfrg == 3) ? (k == 0 ? hs[s].lambda * hs[s].lambda : (k == 1 ? hs[s].lambda : 1.0)) : 1.0;
             L.eelec[k] = k < hs[s].nfrg ? eel[3 * s + k] : 0.0;
             L.evdw[k] = k < hs[s].nfrg ? evd[3 * s + k] : 0.0;
-            L.ebond[k] = k < hs[s].nfrg ? eb[3 * s + k] : 0.0;
-            L.eangle[k] = k < hs[s].nfrg ? ea[3 * s + k] : 0.0;
-            if (k < hs[s].nfrg) ep += f * (eb[3 * s + k] + ea[3 * s + k] + eel[3 * s + k] + evd[3 * s + k]);
+            const double *b = et.data() + (3 * (size_t)s + k) * 6;   // bond, angle, torsion, improper, 1-4 vdW/elec
+            const bool on = k < hs[s].nfrg;
+            L.ebond[k] = on ? b[0] : 0.0;
+            L.eangle[k] = on ? b[1] : 0.0;
+            L.etors[k] = on ? b[2] : 0.0;
+            L.eimp[k] = on ? b[3] : 0.0;
+            L.evdw14[k] = on ? b[4] : 0.0;
+            L.eelec14[k] = on ? b[5] : 0.0;
+            if (on) ep += f * (b[0] + b[1] + b[2] + b[3] + b[4] + b[5] + evd[3 * s + k] + eel[3 * s + k]);
         }
         L.epot = ep;
         L.etot = ep + e;
@@ -1160,12 +1226,12 @@ int Engine::run_nve(int nsteps, int update, int nmon, int prec, jp_energy_log *l
         pe.push_back(a);
         pe.push_back(b);
         ++n_pair;
-        std::vector<double> eel, evd, eex, eb, ea;
+        std::vector<double> eel, evd, eex, et;
         if (mon && (r = energies(prec, eel, evd, eex))) return r;
-        if (mon && (r = bonded_energies(eb, ea))) return r;
+        if (mon && (r = bonded_energies(et))) return r;
         if ((r = integrate(prec, mon))) return r;
         if (mon) {
-            if ((r = mon_record(step, nsteps, nmon, nlog, eel, evd, eb, ea, log))) return r;
+            if ((r = mon_record(step, nsteps, nmon, nlog, eel, evd, et, log))) return r;
             nlog++;
         }
         ++idloop;
@@ -1324,9 +1390,9 @@ int jp_nonbond(jp_ctx *ctx, const double *cord, double *grad, double *eelec, dou
     return r;
 }
 
-int jp_bonded(jp_ctx *ctx, const double *cord, double *grad, double *ebond, double *eangle) {
+int jp_bonded(jp_ctx *ctx, const double *cord, double *grad, double *eterm) {
     if (!ctx) return JP_EINVAL;
-    return ctx->e.bonded(cord, grad, ebond, eangle);
+    return ctx->e.bonded(cord, grad, eterm);
 }
 
 int jp_shake(jp_ctx *ctx, double *cord, const double *cord_old) {

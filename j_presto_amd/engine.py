@@ -102,24 +102,25 @@ class MdSystem:
         return g, ee, ev, eh, ier
 
     def bonded(self, cord=None, grad=None):
-        """dnebnd_PB + dneang_PB: (grad(nfrg,N,3) accumulated, ebond, eangle, ier)."""
+        """dnergv_PB's bonded part: (grad(nfrg,N,3) accumulated, eterm(nfrg,6)
+        with columns bond, angle, torsion, improper, 1-4 vdW, 1-4 elec, ier)."""
         cord = None if cord is None else np.ascontiguousarray(cord, np.float64)
         g = np.zeros((self.nfrg, self.n, 3)) if grad is None else grad
-        eb, ea = np.zeros(self.nfrg), np.zeros(self.nfrg)
-        ier = native.check("jp_bonded", self.lib.jp_bonded(self.ctx, _f64(cord), _f64(g), _f64(eb), _f64(ea)))
-        return g, eb, ea, ier
+        et = np.zeros((self.nfrg, 6))
+        ier = native.check("jp_bonded", self.lib.jp_bonded(self.ctx, _f64(cord), _f64(g), _f64(et)))
+        return g, et, ier
 
     def dnergy(self, cord=None, precision: Optional[int] = None):
         """dnergy on this path: (eneval(15), enevcc(15,nfrg), grad, ier) with
-        slot 2 BOND, 3 ANGLE, 8 VDW15, 9 ELE15, 10 HYD and slot 1 their sum
-        (dnergy.f90:45-79: grad zeroed, bonds and angles, then cmmdo)."""
-        g, eb, ea, ier = self.bonded(cord)
+        slots 2-7 BOND, ANGLE, TORSION, IMPROPER, 1-4 VDW, 1-4 ELEC, 8 VDW15,
+        9 ELE15, 10 HYD and slot 1 their sum (dnergy.f90:45-79: grad zeroed,
+        the bonded terms, then cmmdo)."""
+        g, et, ier = self.bonded(cord)
         if ier:
             return None, None, g, ier
         g, ee, ev, eh, ier = self.cmmdo(None, grad=g, precision=precision)
         enevcc = np.zeros((MAXENE, self.nfrg))
-        enevcc[1] = eb
-        enevcc[2] = ea
+        enevcc[1:7] = et.T
         enevcc[8] = ee
         enevcc[7] = ev
         enevcc[9] = eh
@@ -199,11 +200,13 @@ def equilibrate_ensemble(ens: "Ensemble", nsteps: int, temp: float = 300.0, ever
 
 def log_to_array(log, nl: int) -> np.ndarray:
     dt = np.dtype([("step", "i4"), ("etot", "f8"), ("epot", "f8"), ("ekin", "f8"),
-                   ("eelec", "f8", 3), ("evdw", "f8", 3), ("ebond", "f8", 3), ("eangle", "f8", 3)])
+                   ("eelec", "f8", 3), ("evdw", "f8", 3), ("ebond", "f8", 3), ("eangle", "f8", 3),
+                   ("etors", "f8", 3), ("eimp", "f8", 3), ("evdw14", "f8", 3), ("eelec14", "f8", 3)])
     out = np.zeros(nl, dt)
     for i in range(nl):
         L = log[i]
-        out[i] = (L.step, L.etot, L.epot, L.ekin, tuple(L.eelec), tuple(L.evdw), tuple(L.ebond), tuple(L.eangle))
+        out[i] = (L.step, L.etot, L.epot, L.ekin, tuple(L.eelec), tuple(L.evdw), tuple(L.ebond), tuple(L.eangle),
+                  tuple(L.etors), tuple(L.eimp), tuple(L.evdw14), tuple(L.eelec14))
     return out
 
 

@@ -8,32 +8,34 @@
 !        jp_host --layout   (byte sizes and offsets of the bind(C) types; no GPU use)
 !
 ! System file (Fortran unformatted sequential, one array per record):
-!   natom ntyp nfrag nfrg nexcl nexcess nshake shake_maxit cell_level lmd nbond nangle (int32)
+!   natom ntyp nfrag nfrg nexcl nexcess nshake shake_maxit cell_level lmd nbond nangle ntors n14 (int32)
 !   shake_tol cutoff diel dt lambda celwal(6)                               (real*8)
 !   cord(3,N) vel(3,N) mass(N) charge(N) atype(N) icls(N) rmin2(ntyp) eps(ntyp)
 !   n_matrix(nfrag,nfrag) excl(2,nexcl) excess(2,nexcess)
 !   shake_natom(nshake) shake_atoms(4,nshake) shake_dist2(6,nshake)
 !   bond_atoms(2,nbond) bond_fc(nbond) bond_r0(nbond)
 !   angle_atoms(3,nangle) angle_fc(nangle) angle_theta0(nangle)
+!   tors_atoms(4,ntors) tors_fc tors_rot tors_phase(ntors) tors_div(ntors)
+!   p14_atoms(2,n14) p14_vdw(n14) p14_elec(n14)
 ! Output file (same record format):
 !   ier(8): create, build_list, nonbond FP64, nonbond FP32, run_nve, get_state, bonded (x2)
 !   grad64(3,N,nfrg) eelec64(nfrg) evdw64(nfrg) grad32(3,N,nfrg) eelec32(nfrg) evdw32(nfrg)
-!   ebond(nfrg) eangle(nfrg)
+!   eterm(6,nfrg) = enevcc(2:7,:) (bond, angle, torsion, improper, 1-4 vdW, 1-4 elec)
 !   cord(3,N) vel(3,N) after the run, nlog, then per record: step etot epot ekin
 program jp_host
   use iso_c_binding
   use jpgpu
   implicit none
   character(len=1024) :: fin, fout, arg
-  integer(c_int32_t) :: hdr(12), nsteps, update, nmon, prec
+  integer(c_int32_t) :: hdr(14), nsteps, update, nmon, prec
   real(c_double) :: rhdr(11)
-  integer :: n, ntyp, nfrag, nfrg, nexcl, nexcess, nshake, nbond, nangle, nlog, k, u
+  integer :: n, ntyp, nfrag, nfrg, nexcl, nexcess, nshake, nbond, nangle, ntors, n14, nlog, k, u
   integer(c_int) :: ier(8)
   real(c_double), allocatable, target :: cord(:,:), vel(:,:), mass(:), charge(:), rmin2(:), eps(:)
   real(c_double), allocatable, target :: sdist(:,:), grad64(:,:,:), grad32(:,:,:)
-  real(c_double), target :: eel64(3), evd64(3), ehy64(3), eel32(3), evd32(3), ehy32(3), ebnd(3), eang(3)
-  real(c_double), allocatable, target :: bfc(:), br0(:), afc(:), ath(:)
-  integer(c_int32_t), allocatable, target :: batm(:,:), aatm(:,:)
+  real(c_double), target :: eel64(3), evd64(3), ehy64(3), eel32(3), evd32(3), ehy32(3), eterm(6,3)
+  real(c_double), allocatable, target :: bfc(:), br0(:), afc(:), ath(:), tfc(:), trot(:), tph(:), pv(:), pe(:)
+  integer(c_int32_t), allocatable, target :: batm(:,:), aatm(:,:), tatm(:,:), tdiv(:), patm(:,:)
   integer(c_int32_t), allocatable, target :: atype(:), icls(:), nmat(:,:), excl(:,:), excess(:,:)
   integer(c_int32_t), allocatable, target :: snat(:), satm(:,:)
   type(jp_energy_log), allocatable, target :: elog(:)
@@ -70,11 +72,14 @@ program jp_host
   read(u) rhdr
   n = hdr(1); ntyp = hdr(2); nfrag = hdr(3); nfrg = hdr(4)
   nexcl = hdr(5); nexcess = hdr(6); nshake = hdr(7); nbond = hdr(11); nangle = hdr(12)
+  ntors = hdr(13); n14 = hdr(14)
   allocate(cord(3,n), vel(3,n), mass(n), charge(n), atype(n), icls(n), rmin2(ntyp), eps(ntyp))
   allocate(nmat(nfrag,nfrag), excl(2,max(nexcl,1)), excess(2,max(nexcess,1)))
   allocate(snat(max(nshake,1)), satm(4,max(nshake,1)), sdist(6,max(nshake,1)))
   allocate(batm(2,max(nbond,1)), bfc(max(nbond,1)), br0(max(nbond,1)))
   allocate(aatm(3,max(nangle,1)), afc(max(nangle,1)), ath(max(nangle,1)))
+  allocate(tatm(4,max(ntors,1)), tfc(max(ntors,1)), trot(max(ntors,1)), tph(max(ntors,1)), tdiv(max(ntors,1)))
+  allocate(patm(2,max(n14,1)), pv(max(n14,1)), pe(max(n14,1)))
   read(u) cord
   read(u) vel
   read(u) mass
@@ -95,6 +100,14 @@ program jp_host
   read(u) aatm(:, 1:nangle)
   read(u) afc(1:nangle)
   read(u) ath(1:nangle)
+  read(u) tatm(:, 1:ntors)
+  read(u) tfc(1:ntors)
+  read(u) trot(1:ntors)
+  read(u) tph(1:ntors)
+  read(u) tdiv(1:ntors)
+  read(u) patm(:, 1:n14)
+  read(u) pv(1:n14)
+  read(u) pe(1:n14)
   close(u)
 
   desc%natom = n; desc%ntyp = ntyp; desc%nfrag = nfrag; desc%nfrg = nfrg
@@ -111,6 +124,9 @@ program jp_host
   desc%nbond = nbond; desc%bond_atoms = c_loc(batm); desc%bond_fc = c_loc(bfc); desc%bond_r0 = c_loc(br0)
   desc%nangle = nangle; desc%angle_atoms = c_loc(aatm); desc%angle_fc = c_loc(afc)
   desc%angle_theta0 = c_loc(ath)
+  desc%ntors = ntors; desc%tors_atoms = c_loc(tatm); desc%tors_fc = c_loc(tfc); desc%tors_rot = c_loc(trot)
+  desc%tors_phase = c_loc(tph); desc%tors_div = c_loc(tdiv)
+  desc%n14 = n14; desc%p14_atoms = c_loc(patm); desc%p14_vdw = c_loc(pv); desc%p14_elec = c_loc(pe)
 
   write(*,'(2a)') 'jp_host: ', jp_version()
   ier = 0
@@ -124,11 +140,11 @@ program jp_host
 
   ! dnergy: grad zeroed, then bonds and angles, then cmmdo (dnergy.f90:45-55, dnergv.f90)
   allocate(grad64(3,n,nfrg), grad32(3,n,nfrg))
-  grad64 = 0.0d0; eel64 = 0.0d0; evd64 = 0.0d0; ehy64 = 0.0d0; ebnd = 0.0d0; eang = 0.0d0
-  ier(7) = jp_bonded(ctx, c_null_ptr, c_loc(grad64), c_loc(ebnd), c_loc(eang))
+  grad64 = 0.0d0; eel64 = 0.0d0; evd64 = 0.0d0; ehy64 = 0.0d0; eterm = 0.0d0
+  ier(7) = jp_bonded(ctx, c_null_ptr, c_loc(grad64), c_loc(eterm))
   ier(3) = jp_nonbond(ctx, c_null_ptr, c_loc(grad64), c_loc(eel64), c_loc(evd64), c_loc(ehy64), JP_PREC_FP64)
   grad32 = 0.0d0; eel32 = 0.0d0; evd32 = 0.0d0; ehy32 = 0.0d0
-  ier(8) = jp_bonded(ctx, c_null_ptr, c_loc(grad32), c_null_ptr, c_null_ptr)
+  ier(8) = jp_bonded(ctx, c_null_ptr, c_loc(grad32), c_null_ptr)
   ier(4) = jp_nonbond(ctx, c_null_ptr, c_loc(grad32), c_loc(eel32), c_loc(evd32), c_loc(ehy32), JP_PREC_FP32_FIXED)
 
   ! METHOD= MICR loop
@@ -148,8 +164,7 @@ program jp_host
   write(u) grad32
   write(u) eel32(1:nfrg)
   write(u) evd32(1:nfrg)
-  write(u) ebnd(1:nfrg)
-  write(u) eang(1:nfrg)
+  write(u) eterm(:, 1:nfrg)
   write(u) cord
   write(u) vel
   write(u) nlog

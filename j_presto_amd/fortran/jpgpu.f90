@@ -11,7 +11,7 @@
 !   jp_ctx_create   incel.f90:2 + inpcls.f90 + setshk.f90 (set-up)
 !   jp_build_list   celprp.f90:2 (celpar_PB, cutatm_PB, cutoff_PB)
 !   jp_nonbond      cmmdo.f90:2 -> enear_ZD_*_PB (enear.f90:1-593)
-!   jp_bonded       dnergv.f90:62-193 (dnebnd_PB, dneang_PB)
+!   jp_bonded       dnergv.f90:29-516 (dnebnd_PB, dneang_PB, dnei14_PB)
 !   jp_shake        shake.f90:2 (shake_PB)
 !   jp_run_nve      md.f90:519-694 + verlet.f90:2 (METHOD= MICR loop)
 !   jp_batch_*      j_presto_mpi:27-44 (independent runs, one launch)
@@ -54,6 +54,16 @@ module jpgpu
      integer(c_int32_t) :: nangle = 0
      type(c_ptr) :: angle_atoms = c_null_ptr, angle_fc = c_null_ptr, angle_theta0 = c_null_ptr
      type(c_ptr) :: angle_cls = c_null_ptr
+     ! torsions and impropers as incel leaves them, 1-4 pairs (iyp14, fytvws14, fytess14, i14cls)
+     integer(c_int32_t) :: ntors = 0
+     type(c_ptr) :: tors_atoms = c_null_ptr, tors_fc = c_null_ptr, tors_rot = c_null_ptr
+     type(c_ptr) :: tors_phase = c_null_ptr, tors_div = c_null_ptr, tors_cls = c_null_ptr
+     integer(c_int32_t) :: nimp = 0
+     type(c_ptr) :: imp_atoms = c_null_ptr, imp_fc = c_null_ptr, imp_rot = c_null_ptr
+     type(c_ptr) :: imp_phase = c_null_ptr, imp_div = c_null_ptr, imp_cls = c_null_ptr
+     integer(c_int32_t) :: n14 = 0
+     type(c_ptr) :: p14_atoms = c_null_ptr, p14_vdw = c_null_ptr, p14_elec = c_null_ptr
+     type(c_ptr) :: p14_cls = c_null_ptr
   end type jp_sys_desc
 
   ! One monitoring record of jp_run_nve (md.f90:618-628).
@@ -61,7 +71,7 @@ module jpgpu
      integer(c_int32_t) :: step
      real(c_double) :: etot, epot, ekin
      real(c_double) :: eelec(3), evdw(3)
-     real(c_double) :: ebond(3), eangle(3)
+     real(c_double) :: ebond(3), eangle(3), etors(3), eimp(3), evdw14(3), eelec14(3)
   end type jp_energy_log
 
   interface
@@ -107,10 +117,10 @@ module jpgpu
        integer(c_int), value :: precision
      end function jp_nonbond
 
-     ! dnebnd_PB + dneang_PB: grad accumulated, per-class energies
-     integer(c_int) function jp_bonded(ctx, cord, grad, ebond, eangle) bind(C, name='jp_bonded')
+     ! dnergv_PB's bonded part: grad accumulated, eterm(6,nfrg) = enevcc(2:7,:)
+     integer(c_int) function jp_bonded(ctx, cord, grad, eterm) bind(C, name='jp_bonded')
        import :: c_int, c_ptr
-       type(c_ptr), value :: ctx, cord, grad, ebond, eangle
+       type(c_ptr), value :: ctx, cord, grad, eterm
      end function jp_bonded
 
      integer(c_int) function jp_shake(ctx, cord, cord_old) bind(C, name='jp_shake')

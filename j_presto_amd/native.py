@@ -28,7 +28,7 @@ SIGNATURES = {
     "jp_build_list": (C.c_int, [vp, f64p]),
     "jp_get_list": (C.c_int, [vp, i32p, i64p, i32p, i32p, C.c_int64]),
     "jp_nonbond": (C.c_int, [vp, f64p, f64p, f64p, f64p, f64p, C.c_int]),
-    "jp_bonded": (C.c_int, [vp, f64p, f64p, f64p, f64p]),
+    "jp_bonded": (C.c_int, [vp, f64p, f64p, f64p]),
     "jp_shake": (C.c_int, [vp, f64p, f64p]),
     "jp_run_nve": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.POINTER(EnergyLog)]),
     "jp_last_timing": (C.c_int, [vp, f64p, f64p, f64p, f64p]),

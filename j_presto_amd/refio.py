@@ -127,7 +127,8 @@ def write_tpl(path: str, s: System) -> None:
     maxatm = s.n
     maxbnd = sum(len(m.bonds) * m.ncopy for m in s.mols)
     maxang = sum(len(m.angles) * m.ncopy for m in s.mols)
-    L = [f"; {maxatm} {max(maxbnd, 1)} {max(maxang, 1)} 1 1 {len(s.types)}", "",
+    maxtor = sum(len(m.torsions) * m.ncopy for m in s.mols)
+    L = [f"; {maxatm} {max(maxbnd, 1)} {max(maxang, 1)} {max(maxtor, 1)} 1 {len(s.types)}", "",
          "TPL> TITLE", f"{s.name}", "", "TPL> MOLECULES"]
     for m in s.mols:
         L.append(f"{m.name:<40}{m.ncopy:>12}")
@@ -151,9 +152,17 @@ def write_tpl(path: str, s: System) -> None:
             L += ["", "TPL> ANGLES", m.name]
             for (i, j, k), fc, th in zip(m.angles, m.angle_fc, m.angle_deg):
                 L.append(f"{i} {j} {k} {_fmt(fc)} {_fmt(th)}")
+    for m in s.mols:
+        if len(m.torsions):   # inptpl.f90 rdtors: i j k l fc div rot phase(deg) 1-4 flag
+            L += ["", "TPL> TORSIONS", m.name]
+            for (i, j, k, l), fc, dv, rot, ph, nbf in zip(m.torsions, m.tors_fc, m.tors_div, m.tors_rot,
+                                                          m.tors_phase_deg, m.tors_nbf):
+                L.append(f"{i} {j} {k} {l} {_fmt(fc)} {int(dv)} {_fmt(rot)} {_fmt(ph)} {int(nbf)}")
     L += ["", "TPL> FUNCTIONS", "1 4 LENNARD-JONES-AMBER", "", "TPL> NONBONDS"]
     for t, at in enumerate(s.types):
-        L.append(f"{t + 1} 0 1 {_fmt(at.rmin2)} {_fmt(at.eps)} 0.5 0.8333333333333334 ; {at.name}")
+        # inptpl.f90 NONB record: type, ..., Rmin/2, eps, 1-4 elec scale (fy14se), 1-4 vdW scale (fy14sv)
+        L.append(f"{t + 1} 0 1 {_fmt(at.rmin2)} {_fmt(at.eps)} {_fmt(at.scale14_elec)} {_fmt(at.scale14_vdw)}"
+                 f" ; {at.name}")
     L.append("")
     with open(path, "w") as f:
         f.write("\n".join(L) + "\n")
@@ -220,7 +229,7 @@ def write_jpsys(path: str, s: System, dt: float = 2.0, cell_level: int = 3, shak
     d = h.desc
     ng = d.nshake
     hdr = np.array([d.natom, d.ntyp, d.nfrag, d.nfrg, d.nexcl, d.nexcess, ng, shake_maxit, cell_level, d.lmd,
-                    d.nbond, d.nangle], np.int32)
+                    d.nbond, d.nangle, d.ntors, d.n14], np.int32)
     rhdr = np.array([shake_tol, d.cutoff, d.diel, dt, d.lambda_] + [float(w) for w in d.celwal], np.float64)
     i32 = lambda a: np.ascontiguousarray(a, np.int32).tobytes()
     f64 = lambda a: np.ascontiguousarray(a, np.float64).tobytes()
@@ -228,7 +237,8 @@ def write_jpsys(path: str, s: System, dt: float = 2.0, cell_level: int = 3, shak
         hdr.tobytes(), rhdr.tobytes(), f64(h.x), f64(s.v), f64(h.mass), f64(h.charge), i32(h.atype), i32(h.icls),
         f64(h.rmin2), f64(h.eps), i32(h.nmat), i32(h.excl), i32(h.excess), i32(h.shk_n[:ng]), i32(h.shk_a[:ng]),
         f64(h.shk_d[:ng]), i32(h.bond_atoms), f64(h.bond_fc), f64(h.bond_r0), i32(h.angle_atoms),
-        f64(h.angle_fc), f64(h.angle_theta0),
+        f64(h.angle_fc), f64(h.angle_theta0), i32(h.tors_atoms), f64(h.tors_fc), f64(h.tors_rot),
+        f64(h.tors_phase), i32(h.tors_div), i32(h.p14_atoms), f64(h.p14_vdw), f64(h.p14_elec),
     ])
 
 
@@ -240,9 +250,9 @@ def read_jphost(path: str, n: int, nfrg: int) -> Dict[str, object]:
         "ier": np.frombuffer(r[0], np.int32).copy(),
         "grad64": f(r[1]).reshape(nfrg, n, 3).copy(), "eelec64": f(r[2]).copy(), "evdw64": f(r[3]).copy(),
         "grad32": f(r[4]).reshape(nfrg, n, 3).copy(), "eelec32": f(r[5]).copy(), "evdw32": f(r[6]).copy(),
-        "ebond": f(r[7]).copy(), "eangle": f(r[8]).copy(),
-        "x": f(r[9]).reshape(n, 3).copy(), "v": f(r[10]).reshape(n, 3).copy(),
+        "eterm": f(r[7]).reshape(nfrg, 6).copy(),
+        "x": f(r[8]).reshape(n, 3).copy(), "v": f(r[9]).reshape(n, 3).copy(),
     }
-    nlog = struct.unpack("<i", r[11])[0]
-    out["log"] = [struct.unpack("<iddd", r[12 + k]) for k in range(nlog)]
+    nlog = struct.unpack("<i", r[10])[0]
+    out["log"] = [struct.unpack("<iddd", r[11 + k]) for k in range(nlog)]
     return out

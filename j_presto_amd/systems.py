@@ -67,6 +67,14 @@ class MolTemplate:
     angle_deg: np.ndarray
     shake: List[Tuple[Tuple[int, ...], Tuple[float, ...]]]  # local 1-based atoms, distances
     cluster: int = 1             # cluster class of every atom of this molecule
+    # TPL> TORSIONS records: atoms (nt,4) local 1-based, force constant, division
+    # number, periodicity, phase (deg), 1-4 flag (iytnbf)
+    torsions: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros((0, 4), int))
+    tors_fc: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros(0))
+    tors_div: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros(0, int))
+    tors_rot: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros(0))
+    tors_phase_deg: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros(0))
+    tors_nbf: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros(0, int))
 
     @property
     def natom(self) -> int:
@@ -78,6 +86,8 @@ class AtomType:
     name: str
     rmin2: float     # AMBER Rmin/2 (TPL NONB column 4, fyvwrd)
     eps: float       # well depth (fyvwme)
+    scale14_vdw: float = 0.5                  # fy14sv (TPL NONB column 7)
+    scale14_elec: float = 0.8333333333333334  # fy14se (TPL NONB column 6)
 
 
 @dataclasses.dataclass
@@ -144,6 +154,18 @@ class System:
         out = [m.angles - 1 + off for m, off in self._offsets() if len(m.angles)]
         return np.concatenate(out).astype(np.int64) if out else np.zeros((0, 3), np.int64)
 
+    def torsions(self) -> np.ndarray:
+        out = [m.torsions - 1 + off for m, off in self._offsets() if len(m.torsions)]
+        return np.concatenate(out).astype(np.int64) if out else np.zeros((0, 4), np.int64)
+
+    def torsion_params(self):
+        """(fc, div, rot, phase_deg, nbf) of every torsion, in torsions() order."""
+        keys = ("tors_fc", "tors_div", "tors_rot", "tors_phase_deg", "tors_nbf")
+        parts = [[np.asarray(getattr(m, k)) for m, _ in self._offsets() if len(m.torsions)] for k in keys]
+        if not parts[0]:
+            return np.zeros(0), np.zeros(0, int), np.zeros(0), np.zeros(0), np.zeros(0, int)
+        return tuple(np.concatenate(p) for p in parts)
+
     def bond_params(self):
         """(fc, r0) of every bond, in bonds() order."""
         fc = [np.asarray(m.bond_fc, np.float64) for m, _ in self._offsets() if len(m.bonds)]
@@ -171,13 +193,18 @@ class System:
         return np.unique(e, axis=0)
 
     def excess_pairs(self) -> np.ndarray:
-        """ZD excess pairs: every bond, then every angle's 1-3 pair (no
-        torsions here), in TPL order (reference incel.f90:665-688)."""
+        """ZD excess pairs: every bond, every angle's 1-3 pair, then the 1-4
+        pair of every torsion with the 1-4 flag, in TPL order (reference
+        incel.f90:665-688)."""
         b = self.bonds()
         a = self.angles()
         parts = [b[:, :2]] if len(b) else []
         if len(a):
             parts.append(a[:, [0, 2]])
+        tr = self.torsions()
+        if len(tr):
+            nbf = self.torsion_params()[4]
+            parts.append(tr[nbf == 1][:, [0, 3]])
         return np.concatenate(parts).astype(np.int64) if parts else np.zeros((0, 2), np.int64)
 
     def shake_groups(self):
@@ -445,12 +472,16 @@ def _saw_chain(rng, n: int, L: float, bond: float = 1.5, excl: float = 3.0,
 def chain_template(rng, n: int, type0: int, ntypes: int, cluster: int = 1,
                    bond_fc: float = 0.0, angle_fc: float = 0.0, pos: Optional[np.ndarray] = None,
                    L: float = 0.0, bond_r0: Optional[float] = None,
-                   angle_deg: Optional[float] = None) -> Tuple[MolTemplate, List[AtomType]]:
+                   angle_deg: Optional[float] = None,
+                   torsion_fc: float = 0.0) -> Tuple[MolTemplate, List[AtomType]]:
     """Linear chain: bonds i-(i+1), angles i-(i+1)-(i+2); exclusions are the
     1-2 and 1-3 pairs; every 4th atom (index 3, 7, ...) is 'H' constrained to
     its predecessor.  Harmonic bond/angle force constants default to 0 (no
     bonded forces); their equilibrium values are the generated geometry's own
-    unless `bond_r0` / `angle_deg` fix them."""
+    unless `bond_r0` / `angle_deg` fix them.  With torsion_fc > 0 every
+    i..i+3 is a torsion (periodicity 1-3, phase 0 or 180 deg, force constant
+    torsion_fc * U[0.5, 1.5], 1-4 flag set), and the 1-4 pairs join the
+    exclusions."""
     q = rng.normal(0.0, 0.3, size=n)
     q -= q.mean()
     sig = rng.uniform(1.9, 3.8, size=ntypes)
@@ -470,6 +501,9 @@ def chain_template(rng, n: int, type0: int, ntypes: int, cluster: int = 1,
         if i + 2 < n:
             p.append(2)
             n1x[i, 1] = 1
+        if torsion_fc > 0 and i + 3 < n:
+            p.append(3)
+            n1x[i, 2] = 1
         partners.append(p)
     bonds = np.stack([np.arange(1, n), np.arange(2, n + 1)], axis=1)
     angles = np.stack([np.arange(1, n - 1), np.arange(2, n), np.arange(3, n + 1)], axis=1)
@@ -490,18 +524,25 @@ def chain_template(rng, n: int, type0: int, ntypes: int, cluster: int = 1,
         th0 = np.full(n - 2, float(angle_deg))
     shake = [((i, i + 1), (1.5,)) for i in range(3, n + 1, 4) if i <= n and i >= 1 and i + 1 <= n]
     shake = [((a, b), d) for (a, b), d in shake if is_h[b - 1]]
+    tors = dict()
+    if torsion_fc > 0 and n >= 4:
+        nt = n - 3
+        tors = dict(torsions=np.stack([np.arange(1, nt + 1) + k for k in range(4)], axis=1),
+                    tors_fc=torsion_fc * rng.uniform(0.5, 1.5, size=nt), tors_div=np.ones(nt, int),
+                    tors_rot=rng.integers(1, 4, size=nt).astype(np.float64),
+                    tors_phase_deg=np.where(rng.random(nt) < 0.5, 0.0, 180.0), tors_nbf=np.ones(nt, int))
     mol = MolTemplate(
         name="CHAIN", ncopy=1, aname=aname, atname=["HC" if h else "CT" for h in is_h],
         itype=itype, rname=["RES"] * n, resnum=np.arange(1, n + 1), mass=mass, charge=q,
         n1x=n1x, partners=partners, bonds=bonds, bond_fc=np.full(n - 1, bond_fc), bond_r0=r0,
-        angles=angles, angle_fc=np.full(n - 2, angle_fc), angle_deg=th0, shake=shake, cluster=cluster)
+        angles=angles, angle_fc=np.full(n - 2, angle_fc), angle_deg=th0, shake=shake, cluster=cluster, **tors)
     return mol, types
 
 
 def chain_in_water(n_chain: int, L: float, seed: int, n_chain_types: int = 16, temp: float = 300.0,
                    lmd: Optional[float] = None, name: str = "chain", bond_fc: float = 0.0,
                    angle_fc: float = 0.0, bond_r0: Optional[float] = None,
-                   angle_deg: Optional[float] = None, compact: bool = False) -> System:
+                   angle_deg: Optional[float] = None, compact: bool = False, torsion_fc: float = 0.0) -> System:
     """Solvated chain: SAW chain in a periodic water box; waters
     with any atom within 2.6 A of a chain atom are removed.  With `lmd`, the
     chain is cluster class 1 and water class 2 under CLSTMETHOD=LMD (three
@@ -531,7 +572,7 @@ def chain_in_water(n_chain: int, L: float, seed: int, n_chain_types: int = 16, t
     wt = water_types()
     chain, ctypes = chain_template(rng, n_chain, type0=len(wt) + 1, ntypes=n_chain_types,
                                    cluster=1, bond_fc=bond_fc, angle_fc=angle_fc, pos=cpos, L=L,
-                                   bond_r0=bond_r0, angle_deg=angle_deg)
+                                   bond_r0=bond_r0, angle_deg=angle_deg, torsion_fc=torsion_fc)
     mols = [chain, water_template(nwat, cluster=2 if lm else 1)]
     xx = np.mod(np.concatenate([cpos, pos_w]), L)
     s = System(name=name, mols=mols, types=wt + ctypes, box=Lv, x=xx, v=np.zeros_like(xx),

@@ -20,6 +20,8 @@
  *   enear_ZD_doub_PB  enear.f90:60-326 (i_vec = 1), 518-571 (excess, self)
  *   dnebnd_PB    dnergv.f90:62-114 (harmonic bonds)
  *   dneang_PB    dnergv.f90:117-193 (harmonic angles)
+ *   dnetor_PB    dnergv.f90:253-429 (torsions and impropers)
+ *   dne14_PB     dnergv.f90:433-516 (1-4 LJ and electrostatics)
  *   shake        shake.f90:112-327 (3-atom Newton), 687-755 (2-atom exact)
  *   verlet       verlet.f90:144-215 (MICR leap-frog)
  *   caltmp       caltmp.f90:18-27
@@ -83,6 +85,11 @@ typedef struct {
     double *bfc, *br0;
     int nangle, *ang, *acls;
     double *afc, *ath0;
+    /* torsions [0, ntors) then impropers [ntors, ntors + nimp) */
+    int ntors, nimp, *tor, *tdiv, *tcls;
+    double *tfc, *trot, *tphs;
+    int n14, *p14, *p14cls;
+    double *p14v, *p14e;
 } jo_ctx;
 
 /* ------------------------------------------------------------------ setup */
@@ -336,6 +343,41 @@ jo_ctx *jo_create(const jp_sys_desc *d) {
         c->afc[a] = d->angle_fc[a];
         c->ath0[a] = d->angle_theta0[a];
     }
+    /* torsions: class of the middle atoms unless given (incel.f90:1012-1019);
+     * impropers: max(n_matrix) unless given (incel.f90:1063-1068) */
+    c->ntors = d->ntors;
+    c->nimp = d->nimp;
+    const int ntt = d->ntors + d->nimp + 1;
+    c->tor = (int *)malloc(sizeof(int) * 4 * ntt);
+    c->tdiv = (int *)malloc(sizeof(int) * ntt);
+    c->tcls = (int *)malloc(sizeof(int) * ntt);
+    c->tfc = (double *)malloc(sizeof(double) * ntt);
+    c->trot = (double *)malloc(sizeof(double) * ntt);
+    c->tphs = (double *)malloc(sizeof(double) * ntt);
+    for (int q = 0; q < d->ntors + d->nimp; ++q) {
+        const int imp = q >= d->ntors, u = imp ? q - d->ntors : q;
+        const int32_t *at = imp ? d->imp_atoms + 4 * u : d->tors_atoms + 4 * u;
+        for (int k = 0; k < 4; ++k) c->tor[4 * q + k] = at[k] - 1;
+        c->tfc[q] = imp ? d->imp_fc[u] : d->tors_fc[u];
+        c->trot[q] = imp ? d->imp_rot[u] : d->tors_rot[u];
+        c->tphs[q] = imp ? d->imp_phase[u] : d->tors_phase[u];
+        c->tdiv[q] = imp ? d->imp_div[u] : d->tors_div[u];
+        const int32_t *cl = imp ? d->imp_cls : d->tors_cls;
+        c->tcls[q] = cl ? cl[u] - 1 : (imp ? cmax : c->nmat[c->icls[at[1] - 1] * c->nfrag + c->icls[at[2] - 1]]);
+    }
+    c->n14 = d->n14;
+    c->p14 = (int *)malloc(sizeof(int) * 2 * (d->n14 + 1));
+    c->p14cls = (int *)malloc(sizeof(int) * (d->n14 + 1));
+    c->p14v = (double *)malloc(sizeof(double) * (d->n14 + 1));
+    c->p14e = (double *)malloc(sizeof(double) * (d->n14 + 1));
+    for (int q = 0; q < d->n14; ++q) {
+        const int a = d->p14_atoms[2 * q] - 1, b = d->p14_atoms[2 * q + 1] - 1;
+        c->p14[2 * q] = a;
+        c->p14[2 * q + 1] = b;
+        c->p14v[q] = d->p14_vdw[q];
+        c->p14e[q] = d->p14_elec[q];
+        c->p14cls[q] = d->p14_cls ? d->p14_cls[q] - 1 : c->nmat[c->icls[a] * c->nfrag + c->icls[b]];
+    }
     return c;
 }
 
@@ -351,6 +393,8 @@ void jo_destroy(jo_ctx *c) {
     free(c->lidx); free(c->elcap);
     free(c->bnd); free(c->bcls); free(c->bfc); free(c->br0);
     free(c->ang); free(c->acls); free(c->afc); free(c->ath0);
+    free(c->tor); free(c->tdiv); free(c->tcls); free(c->tfc); free(c->trot); free(c->tphs);
+    free(c->p14); free(c->p14cls); free(c->p14v); free(c->p14e);
     free(c);
 }
 
@@ -767,12 +811,15 @@ static inline void mimg(const jo_ctx *c, double *d) {
 
 /* ---------------------------------------------------------------- bonded */
 
-/* dnebnd_PB (dnergv.f90:62-114) then dneang_PB (dnergv.f90:117-193), one
- * thread: grad(3,N,nfrg) accumulated in term order, energies per class
- * (ebond/eangle zeroed here). */
-int jo_bonded(const jo_ctx *c, const double *x, double *grad, double *ebond, double *eangle) {
+/* dnergv_PB's bonded part in md_run's order, one thread: dnebnd_PB
+ * (dnergv.f90:62-114), dneang_PB (:117-193), then dnei14_PB (:197-249):
+ * dnetor_PB on torsions and on impropers, dne14_PB.  grad(3,N,nfrg)
+ * accumulated in term order; eterm[6 * class + k] (zeroed here) for k =
+ * bond, angle, torsion, improper, 1-4 vdW, 1-4 elec. */
+int jo_bonded(const jo_ctx *c, const double *x, double *grad, double *eterm) {
     const size_t n3 = (size_t)3 * c->n;
-    for (int k = 0; k < c->nfrg; ++k) { ebond[k] = 0.0; eangle[k] = 0.0; }
+    double ebond[3] = {0, 0, 0}, eangle[3] = {0, 0, 0};
+    for (int k = 0; k < 6 * c->nfrg; ++k) eterm[k] = 0.0;
     for (int b = 0; b < c->nbond; ++b) {
         int i = c->bnd[2 * b], j = c->bnd[2 * b + 1], n = c->bcls[b];
         double d[3];
@@ -824,6 +871,106 @@ int jo_bonded(const jo_ctx *c, const double *x, double *grad, double *ebond, dou
             g[3 * t[0] + k] = g[3 * t[0] + k] + w[k];
             g[3 * t[1] + k] = g[3 * t[1] + k] - (w[k] + w[3 + k]);
             g[3 * t[2] + k] = g[3 * t[2] + k] + w[3 + k];
+        }
+    }
+    for (int k = 0; k < c->nfrg; ++k) { eterm[6 * k] = ebond[k]; eterm[6 * k + 1] = eangle[k]; }
+    /* dnetor_PB: torsions (energy slot 2), then impropers (slot 3) */
+    const double pi = 3.141592653589793; /* PHYCNS.f90:11 */
+    for (int q = 0; q < c->ntors + c->nimp; ++q) {
+        const int *a = c->tor + 4 * q;
+        const int n = c->tcls[q], slot = q < c->ntors ? 2 : 3;
+        double d21[3], d32[3], d43[3], p12[3], p23[3], p123[3], work[9];
+        for (int k = 0; k < 3; ++k) {
+            d21[k] = x[3 * a[1] + k] - x[3 * a[0] + k];
+            d32[k] = x[3 * a[2] + k] - x[3 * a[1] + k];
+            d43[k] = x[3 * a[3] + k] - x[3 * a[2] + k];
+        }
+        mimg(c, d21); mimg(c, d32); mimg(c, d43);
+        p12[0] = d21[1] * d32[2] - d32[1] * d21[2];
+        p12[1] = d21[2] * d32[0] - d32[2] * d21[0];
+        p12[2] = d21[0] * d32[1] - d32[0] * d21[1];
+        p23[0] = d32[1] * d43[2] - d43[1] * d32[2];
+        p23[1] = d32[2] * d43[0] - d43[2] * d32[0];
+        p23[2] = d32[0] * d43[1] - d43[0] * d32[1];
+        double r12 = p12[0] * p12[0] + p12[1] * p12[1] + p12[2] * p12[2];
+        double r23 = p23[0] * p23[0] + p23[1] * p23[1] + p23[2] * p23[2];
+        double r12r23 = sqrt(r12 * r23);
+        if (!(r12r23 > 0.0)) continue; /* bond angle 0 or pi: no torsion term */
+        r12r23 = 1.0 / r12r23;
+        const double s1223 = p12[0] * p23[0] + p12[1] * p23[1] + p12[2] * p23[2];
+        double cosp = s1223 * r12r23;
+        if (cosp > 1.0) cosp = 1.0;
+        else if (cosp < -1.0) cosp = -1.0;
+        p123[0] = p12[1] * p23[2] - p23[1] * p12[2];
+        p123[1] = p12[2] * p23[0] - p23[2] * p12[0];
+        p123[2] = p12[0] * p23[1] - p23[0] * p12[1];
+        const double s32123 = d32[0] * p123[0] + d32[1] * p123[1] + d32[2] * p123[2];
+        double phi = acos(cosp);
+        if (s32123 < 0.0) phi = 2.0 * pi - phi;
+        const double phimod = c->trot[q] * phi + c->tphs[q];
+        const double div = 1.0 / (double)c->tdiv[q];
+        eterm[6 * n + slot] = eterm[6 * n + slot] + c->tfc[q] * (1.0 + cos(phimod)) * div;
+        const double sinp = sin(phi);
+        double wrk0 = 0.0;
+        switch ((int)anint(c->trot[q])) {
+            case 1: wrk0 = 1.0; break;
+            case 2: wrk0 = 2.0 * cosp; break;
+            case 3: wrk0 = -4.0 * sinp * sinp + 3.0; break;
+            case 4: wrk0 = (-8.0 * sinp * sinp + 4.0) * cosp; break;
+            case 5: wrk0 = (16.0 * sinp * sinp - 20.0) * sinp * sinp + 5.0; break;
+            case 6: wrk0 = cosp * (6.0 + 32.0 * cosp * cosp * (cosp * cosp - 1.0)); break;
+            default: return JP_EINVAL;
+        }
+        wrk0 = cos(c->tphs[q]) * c->trot[q] * c->tfc[q] * div * wrk0 * r12r23;
+        const double wrk1 = d21[0] + d32[0], wrk2 = d21[1] + d32[1], wrk3 = d21[2] + d32[2];
+        r12 = s1223 / r12;
+        r23 = s1223 / r23;
+        work[0] = d32[2] * p23[1] - d32[1] * p23[2] - r12 * (d32[2] * p12[1] - d32[1] * p12[2]);
+        work[1] = d32[0] * p23[2] - d32[2] * p23[0] - r12 * (d32[0] * p12[2] - d32[2] * p12[0]);
+        work[2] = d32[1] * p23[0] - d32[0] * p23[1] - r12 * (d32[1] * p12[0] - d32[0] * p12[1]);
+        work[3] = -wrk3 * p23[1] + d43[2] * p12[1] + wrk2 * p23[2] - d43[1] * p12[2] -
+                  r12 * (-wrk3 * p12[1] + wrk2 * p12[2]) - r23 * (d43[2] * p23[1] - d43[1] * p23[2]);
+        work[4] = wrk3 * p23[0] - d43[2] * p12[0] - wrk1 * p23[2] + d43[0] * p12[2] -
+                  r12 * (wrk3 * p12[0] - wrk1 * p12[2]) - r23 * (d43[0] * p23[2] - d43[2] * p23[0]);
+        work[5] = -wrk2 * p23[0] + d43[1] * p12[0] + wrk1 * p23[1] - d43[0] * p12[1] -
+                  r12 * (-wrk2 * p12[0] + wrk1 * p12[1]) - r23 * (d43[1] * p23[0] - d43[0] * p23[1]);
+        work[6] = d32[2] * p12[1] - d32[1] * p12[2] - r23 * (d32[2] * p23[1] - d32[1] * p23[2]);
+        work[7] = d32[0] * p12[2] - d32[2] * p12[0] - r23 * (d32[0] * p23[2] - d32[2] * p23[0]);
+        work[8] = d32[1] * p12[0] - d32[0] * p12[1] - r23 * (d32[1] * p23[0] - d32[0] * p23[1]);
+        for (int k = 0; k < 9; ++k) work[k] = work[k] * wrk0;
+        double *g = grad + n * n3;
+        for (int k = 0; k < 3; ++k) {
+            g[3 * a[0] + k] = g[3 * a[0] + k] + work[k];
+            g[3 * a[1] + k] = g[3 * a[1] + k] + work[3 + k];
+            g[3 * a[2] + k] = g[3 * a[2] + k] - (work[k] + work[3 + k] + work[6 + k]);
+            g[3 * a[3] + k] = g[3 * a[3] + k] + work[6 + k];
+        }
+    }
+    /* dne14_PB: 1-4 LJ (slot 4) and electrostatics (slot 5) */
+    for (int q = 0; q < c->n14; ++q) {
+        const int i1 = c->p14[2 * q], i2 = c->p14[2 * q + 1], n = c->p14cls[q];
+        double d14[3], work[3];
+        for (int k = 0; k < 3; ++k) d14[k] = x[3 * i1 + k] - x[3 * i2 + k];
+        mimg(c, d14);
+        double r14 = d14[0] * d14[0] + d14[1] * d14[1] + d14[2] * d14[2];
+        r14 = 1.0 / r14;
+        const double r141 = sqrt(r14);
+        const double r146 = r14 * r14 * r14;
+        const double r1412 = r146 * r146;
+        const int ti = c->atype[i1], tj = c->atype[i2];
+        const double co6 = c->c6[ti * c->ntyp + tj] * r146;
+        const double co12 = c->c12[ti * c->ntyp + tj] * r1412;
+        eterm[6 * n + 4] = eterm[6 * n + 4] + c->p14v[q] * (-co6 + co12);
+        double coef = r14 * c->p14v[q] * (-2.0 * co12 + co6) * 6.0;
+        for (int k = 0; k < 3; ++k) work[k] = coef * d14[k];
+        coef = c->p14e[q] * r141;
+        eterm[6 * n + 5] = eterm[6 * n + 5] + coef;
+        coef = coef * (-r14);
+        for (int k = 0; k < 3; ++k) work[k] = work[k] + coef * d14[k];
+        double *g = grad + n * n3;
+        for (int k = 0; k < 3; ++k) {
+            g[3 * i1 + k] = g[3 * i1 + k] + work[k];
+            g[3 * i2 + k] = g[3 * i2 + k] - work[k];
         }
     }
     return 0;
@@ -972,11 +1119,11 @@ int jo_run_nve(jo_ctx *c, double *x, double *v, int nsteps, int update) {
     double *grad = (double *)malloc(sizeof(double) * 3 * (size_t)n * nfrg);
     double *gt = (double *)malloc(sizeof(double) * 3 * (size_t)n);
     double *xo = (double *)malloc(sizeof(double) * 3 * (size_t)n);
-    double ee[3], ev[3], eb[3], ea[3];
+    double ee[3], ev[3], eterm[18];
     int ier = c->have_list ? 0 : jo_celprp(c, x);
     for (int s = 1; s <= nsteps && !ier; ++s) {
         memset(grad, 0, sizeof(double) * 3 * (size_t)n * nfrg);
-        jo_bonded(c, x, grad, eb, ea); /* dnergy order: bonded terms, then cmmdo */
+        jo_bonded(c, x, grad, eterm); /* dnergy order: bonded terms, then cmmdo */
         jo_nonbond(c, x, grad, ee, ev);
         jo_combine_grad(c, grad, gt);
         ier = jo_verlet_step(c, x, v, gt, xo);

@@ -41,7 +41,7 @@ def lib():
         L.jo_get_order.argtypes = [C.c_void_p, i32p]
         L.jo_nonbond.argtypes = [C.c_void_p, f64p, f64p, f64p, f64p]
         L.jo_shake.argtypes = [C.c_void_p, f64p, f64p]
-        L.jo_bonded.argtypes = [C.c_void_p, f64p, f64p, f64p, f64p]
+        L.jo_bonded.argtypes = [C.c_void_p, f64p, f64p, f64p]
         L.jo_verlet_step.argtypes = [C.c_void_p, f64p, f64p, f64p, f64p]
         L.jo_combine_grad.argtypes = [C.c_void_p, f64p, f64p]
         L.jo_ekin.argtypes = [C.c_void_p, f64p]
@@ -108,13 +108,14 @@ class Oracle:
         return g, ee, ev
 
     def bonded(self, x, grad=None):
-        """dnebnd_PB + dneang_PB: (grad(nfrg,N,3) accumulated, ebond, eangle)."""
+        """dnergv_PB's bonded part: (grad(nfrg,N,3) accumulated, eterm(nfrg,6))
+        with eterm columns bond, angle, torsion, improper, 1-4 vdW, 1-4 elec."""
         x = np.ascontiguousarray(x, np.float64)
         g = np.zeros((self.nfrg, self.n, 3)) if grad is None else grad
-        eb = np.zeros(self.nfrg)
-        ea = np.zeros(self.nfrg)
-        lib().jo_bonded(self.c, _p(x), _p(g), _p(eb), _p(ea))
-        return g, eb, ea
+        et = np.zeros((self.nfrg, 6))
+        r = lib().jo_bonded(self.c, _p(x), _p(g), _p(et))
+        assert r == 0
+        return g, et
 
     def shake(self, x_new, x_old) -> int:
         return lib().jo_shake(self.c, _p(x_new), _p(np.ascontiguousarray(x_old)))

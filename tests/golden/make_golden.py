@@ -51,6 +51,13 @@ RECIPES = {
     # LMD with bonded terms: bonds and angles in the unscaled class (SCALED_TERM default)
     "chain_lmd_bonded": ("chain_in_water", dict(n_chain=120, L=26.0, seed=16, lmd=0.7, bond_fc=300.0,
                                                 angle_fc=50.0, bond_r0=1.53, angle_deg=112.0), dict(traj=10)),
+    # chain with torsions and 1-4 terms on top of bonds and angles (dnei14_PB)
+    "chain_tors": ("chain_in_water", dict(n_chain=200, L=30.0, seed=17, bond_fc=300.0, angle_fc=50.0,
+                                          bond_r0=1.53, angle_deg=112.0, torsion_fc=1.0), dict(traj=20)),
+    # LMD with torsions: torsion classes from the middle atoms, 1-4 classes from the pair
+    "chain_lmd_tors": ("chain_in_water", dict(n_chain=120, L=26.0, seed=18, lmd=0.7, bond_fc=300.0,
+                                              angle_fc=50.0, bond_r0=1.53, angle_deg=112.0, torsion_fc=1.0),
+                       dict(traj=10)),
     # ions with their own LJ types
     "ions_3k": ("water_box", dict(nwat=900, L=30.0, seed=13, n_ion_pairs=10), dict(traj=0)),
     # box edge a multiple of rlim = 13.5 A (md_run's phantom-cell case)

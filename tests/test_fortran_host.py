@@ -51,9 +51,9 @@ def test_system_file_roundtrip(tmp_path):
     p = str(tmp_path / "sys.bin")
     refio.write_jpsys(p, s)
     r = refio.read_records(p)
-    assert len(r) == 22
+    assert len(r) == 30
     hdr = np.frombuffer(r[0], np.int32)
-    assert len(hdr) == 12
+    assert len(hdr) == 14
     n, ntyp, nfrag, nfrg, nexcl, nexcess, nshake = hdr[:7]
     assert n == s.n and nfrag == 2 and nfrg == 3 and hdr[9] == 1
     assert len(r[2]) == 24 * n and np.array_equal(np.frombuffer(r[2], np.float64).reshape(n, 3), s.x)

@@ -45,7 +45,7 @@ def test_forces_energies_vs_reference(golden, name, prec, tol):
     print(f"case {name} prec={prec}", flush=True)
     md = engine.MdSystem(s, precision=prec)
     # dnergy: bonds and angles (FP64 in both modes), then cmmdo, into one gradient
-    g, eb, ea, ier = md.bonded(cord=s.x)
+    g, et, ier = md.bonded(cord=s.x)
     assert ier == 0
     g, ee, ev, eh, ier = md.cmmdo(grad=g)
     assert ier == 0
@@ -54,8 +54,9 @@ def test_forces_energies_vs_reference(golden, name, prec, tol):
     etol = 1e-11 if prec == engine.FP64 else 1e-5
     assert abs(ee.sum() - f["force_epot"][8]) <= etol * abs(f["force_epot"][8])
     assert abs(ev.sum() - f["force_epot"][7]) <= etol * abs(f["force_epot"][7])
-    assert abs(eb.sum() - f["force_epot"][1]) <= 1e-11 * max(abs(f["force_epot"][1]), 1.0)
-    assert abs(ea.sum() - f["force_epot"][2]) <= 1e-11 * max(abs(f["force_epot"][2]), 1.0)
+    for k in range(6):   # bond, angle, torsion, improper, 1-4 vdW, 1-4 elec: FP64 in both modes
+        ref = f["force_epot"][1 + k]
+        assert abs(et[:, k].sum() - ref) <= 1e-11 * max(abs(ref), 1.0), k
     md.close()
 
 
@@ -73,10 +74,10 @@ def test_per_class_gradients_lmd(golden):
         assert abs(ev[k] - evo[k]) < 1e-11 * abs(evo).max()
 
 
-@pytest.mark.parametrize("name", ["chain_bonded", "chain_lmd_bonded"])
+@pytest.mark.parametrize("name", ["chain_bonded", "chain_lmd_bonded", "chain_tors", "chain_lmd_tors"])
 def test_bonded_per_class_vs_oracle(golden, name):
     """jp_bonded returns grad(3,N,nfrg) and energies per class like
-    dnebnd_PB / dneang_PB, also at displaced coordinates."""
+    dnergv_PB's bonded routines, also at displaced coordinates."""
     s, f = golden(name)
     print(f"case bonded {name}", flush=True)
     md = engine.MdSystem(s, precision=engine.FP64)
@@ -84,15 +85,15 @@ def test_bonded_per_class_vs_oracle(golden, name):
     rng = np.random.default_rng(3)
     for x in (s.x, s.x + rng.normal(scale=0.05, size=s.x.shape)):
         x = np.ascontiguousarray(x)
-        g, eb, ea, ier = md.bonded(cord=x)
+        g, et, ier = md.bonded(cord=x)
         assert ier == 0
-        go, ebo, eao = o.bonded(x)
+        go, eto = o.bonded(x)
         for k in range(g.shape[0]):
             if np.abs(go[k]).max() == 0.0:
                 assert np.abs(g[k]).max() == 0.0
                 continue
             assert rel_rms(g[k], go[k]) < 1e-13, k
-        assert np.allclose(eb, ebo, rtol=1e-13, atol=0) and np.allclose(ea, eao, rtol=1e-13, atol=0)
+        assert np.allclose(et, eto, rtol=1e-13, atol=1e-12)
     md.close()
 
 
@@ -111,7 +112,7 @@ def test_shake_bit_exact_vs_oracle():
 
 
 @pytest.mark.parametrize("name", ["water_1500", "water_6k", "chain_3k", "chain_lmd", "chain_bonded",
-                                  "chain_lmd_bonded"])
+                                  "chain_lmd_bonded", "chain_tors", "chain_lmd_tors"])
 def test_nve_trajectory_vs_reference(golden, name):
     s, f = golden(name)
     print(f"case {name}", flush=True)
@@ -140,7 +141,7 @@ def test_fp32_trajectory_close():
 def test_batch_equals_single_runs():
     syss = [systems.water_box(200, 24.2, 21), systems.chain_in_water(48, 24.6, 22, lmd=0.8),
             systems.water_box(700, 28.0, 23, n_ion_pairs=2),
-            systems.chain_in_water(64, 24.8, 24, lmd=0.6, bond_fc=300.0, angle_fc=50.0)]
+            systems.chain_in_water(64, 24.8, 24, lmd=0.6, bond_fc=300.0, angle_fc=50.0, torsion_fc=1.0)]
     for prec in (engine.FP64, engine.FP32):
         ens = engine.Ensemble(syss, precision=prec)
         log, ier = ens.runmd(20, update=10, nmon=10)
@@ -152,7 +153,8 @@ def test_batch_equals_single_runs():
             xb, vb = ens.get_state(i)
             assert np.array_equal(x1, xb) and np.array_equal(v1, vb), (prec, i)
             assert np.allclose(l1["etot"], log[i]["etot"], rtol=0, atol=1e-9)
-            assert np.array_equal(l1["ebond"], log[i]["ebond"]) and np.array_equal(l1["eangle"], log[i]["eangle"])
+            for key in ("ebond", "eangle", "etors", "evdw14", "eelec14"):
+                assert np.array_equal(l1[key], log[i][key]), key
             md.close()
         ens.close()
 
@@ -201,7 +203,8 @@ def test_graph_steps_equal_eager(prec, bonded):
     """Plain steps replayed from captured HIP graphs (several replays, with
     rebuilds and energy steps in between) give the same bits as kernels
     launched one by one."""
-    s = (systems.chain_in_water(64, 24.8, 24, bond_fc=300.0, angle_fc=50.0, bond_r0=1.53, angle_deg=112.0)
+    s = (systems.chain_in_water(64, 24.8, 24, bond_fc=300.0, angle_fc=50.0, bond_r0=1.53, angle_deg=112.0,
+                                torsion_fc=1.0)
          if bonded else systems.water_box(700, 28.0, 23, n_ion_pairs=2))
     print(f"case graphs prec={prec} bonded={bonded}", flush=True)
     out = []

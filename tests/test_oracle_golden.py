@@ -14,15 +14,16 @@ def test_forces_and_energies_bit_exact(golden, name):
     o = Oracle(s)
     assert o.celprp(s.x) == 0
     # dnergy order: bonds and angles, then cmmdo, into one zeroed gradient
-    g, eb, ea = o.bonded(s.x)
+    g, et = o.bonded(s.x)
     g, ee, ev = o.nonbond(s.x, g)
     gt = o.combine(g)
     assert np.array_equal(gt, f["force_grad"]), np.abs(gt - f["force_grad"]).max()
-    # dnergy slots: 2 BOND, 3 ANGLE, 9 ELE15, 8 VDW15 (main.f90:157-159), unscaled class sums
+    # dnergy slots (main.f90:157-159): 2 BOND, 3 ANGLE, 4 TORSION, 5 IMPROPER, 6 1-4 VDW,
+    # 7 1-4 ELEC, 8 VDW15, 9 ELE15; unscaled class sums
     assert ee.sum() == f["force_epot"][8]
     assert ev.sum() == f["force_epot"][7]
-    assert eb.sum() == f["force_epot"][1]
-    assert ea.sum() == f["force_epot"][2]
+    for k in range(6):
+        assert et[:, k].sum() == f["force_epot"][1 + k], k
 
 
 @pytest.mark.parametrize("name", GOLDEN_NAMES)
@@ -45,7 +46,7 @@ def test_nve_trajectory_bit_exact(golden, name):
     x, v = s.x.copy(), s.v.copy()
     assert o.celprp(x) == 0
     for idloop in range(1, int(f["traj_nsteps"]) + 1):
-        g, eb, ea = o.bonded(x)
+        g, et = o.bonded(x)
         g, ee, ev = o.nonbond(x, g)
         assert o.verlet_step(x, v, o.combine(g)) == 0
         if idloop % 10 == 0:
