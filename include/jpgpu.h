@@ -225,9 +225,9 @@ int jp_batch_destroy(jp_batch *b);
  *   JP_OPT_LIST_WAVES  i-clusters per block in the list kernel (1, 2, 4 = default, 8);
  *                      the list does not depend on it.
  *   JP_OPT_LIST_SEGMENT  list entries per i-cluster segment at the next rebuild
- *                      only (set, not read); the library grows segments by itself
- *                      and rebuilds when one overflows, so this only exercises that
- *                      path (results do not depend on it).
+ *                      only (a multiple of 64; set, not read); the library grows
+ *                      segments by itself and rebuilds when one overflows, so this
+ *                      only exercises that path (results do not depend on it).
  * Setting a shape turns the autotuner off; jp_get_option reads the current
  * (possibly tuned) values. */
 #define JP_OPT_GRAPHS 1

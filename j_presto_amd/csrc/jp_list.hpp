@@ -12,8 +12,10 @@
 // Every i-cluster owns a segment of ccap entries at ent[C*ccap]; entries whose
 // j carries LJ are stored from its front, the others from its back, so that
 // every chunk the pair kernels stage is uniformly with or without the LJ term.
-// A segment that would overflow is reported (the host grows ccap and builds
-// again).
+// Each part is kept to whole 64-entry chunks (rounded-up sizes within ccap, a
+// multiple of 64), and mb[] holds the mask of every entry as a byte at the
+// same index, zero on the padding of the partial chunks.  A segment that would
+// overflow is reported (the host grows ccap and builds again).
 //
 // Candidates: every j-cluster whose home cell (the cell of its first atom) is
 // within rlim + 2*ext of C's box, ext bounding how far a cluster's atoms
@@ -36,6 +38,8 @@
 #define JP_CL 8          // atoms per i-cluster
 #define JP_LW 4          // default i-clusters (wavefronts) per block of k_list8 (1, 2, 4 or 8; autotuned)
 #define JP_QCAP 128      // per-wave queue of candidate j-clusters (power of 2)
+
+__host__ __device__ __forceinline__ int round64(int n) { return (n + 63) & ~63; }
 // list status words lst[]: [0] error bits, [1] longest list above the high-water
 // mark, [2 + s] excluded pairs met within rlim by system s
 #define JP_ERR_CELL 1
@@ -233,7 +237,7 @@ k_list8(int nclu, const int *__restrict__ clu_sys, const SysP *__restrict__ sp, 
         const float4 *__restrict__ ccf, const int4 *__restrict__ crec, const unsigned *__restrict__ sysext,
         const float4 *__restrict__ lp, const double4 *__restrict__ xr, const int4 *__restrict__ imgs,
         const int *__restrict__ xptr, const int *__restrict__ xidx, int ccap, int hiwater, uint2 *__restrict__ ent,
-        int2 *__restrict__ lcnt, int *__restrict__ lst) {
+        uint8_t *__restrict__ mb, int2 *__restrict__ lcnt, int *__restrict__ lst) {
     __shared__ uint32_t s_c[LW][JP_QCAP];   // candidate j-clusters of passing cells
     __shared__ uint32_t s_q[LW][JP_QCAP];   // candidates whose box is within rlim
     __shared__ double4 s_xr[LW][JP_CL];
@@ -392,10 +396,13 @@ k_list8(int nclu, const int *__restrict__ clu_sys, const SysP *__restrict__ sp, 
         const bool has = m != 0, lj = (meta >> 30) & 1;
         const uint64_t bA = __ballot(has && lj), bB = __ballot(has && !lj);
         const int na = __popcll(bA), nb = __popcll(bB);
-        if (nA + nB + na + nb <= cap) {
+        if (round64(nA + na) + round64(nB + nb) <= cap) {
             const uint2 E = make_uint2((uint32_t)js | (code << JP_SLOT_BITS), m);
-            if (has && lj) ent[e0 + nA + lane_rank(bA)] = E;
-            if (has && !lj) ent[e0 + cap - 1 - (nB + lane_rank(bB))] = E;
+            const size_t pos = lj ? e0 + nA + lane_rank(bA) : e0 + cap - 1 - (nB + lane_rank(bB));
+            if (has) {
+                ent[pos] = E;
+                mb[pos] = (uint8_t)m;
+            }
             nA += na;
             nB += nb;
         } else {
@@ -452,10 +459,15 @@ k_list8(int nclu, const int *__restrict__ clu_sys, const SysP *__restrict__ sp, 
         ct += __popcll(b);
         ++rnd;
     }
+    // zero masks on the padding of the two partial 64-entry chunks (the
+    // FP32 pair kernel reads whole chunks of mb); the parts never share a chunk
+    if (nA + l < round64(nA)) mb[e0 + nA + l] = 0;
+    if (nB + l < round64(nB)) mb[e0 + cap - 1 - (nB + l)] = 0;
     if (l == 0) {
         lcnt[ci] = make_int2(nA, nB);
         if (ovf) atomicOr(lst, JP_ERR_LISTCAP);
-        if (nA + nB > hiwater) atomicMax(lst + 1, nA + nB);
+        const int used = round64(nA) + round64(nB);
+        if (used > hiwater) atomicMax(lst + 1, used);
     }
     fold();
     if (l < JP_CL && ai >= 0 && nbv > P.nblim) atomicOr(lst, JP_ERR_NBLIM);

@@ -9,6 +9,13 @@
 // j = entry 8*it + (l >> 3)).  Chunks come from the LJ part or the non-LJ part
 // of the list, so the LJ term is uniform per chunk.  At the end the 8 lanes
 // of each i atom and the PW waves are combined in a fixed order.
+//
+// FP32 chunks start at multiples of 64 entries: LJ chunks from the front of
+// the segment, non-LJ chunks from its end.  The list kernel keeps a byte per
+// entry (the 8-bit pair mask) in mb[], with the padding of the two partial
+// chunks zeroed, so the masks of one iteration are one 64-bit word of mb
+// (bit 8*jsub + ia = pair (ia, entry 8*it + jsub)): a scalar load that selects
+// lanes directly, without a per-lane bit test.
 #pragma once
 #include <type_traits>
 
@@ -39,6 +46,40 @@ struct ChunkSpan {
     bool lj;
 };
 
+// Chunk ch of cluster c in the aligned layout: lanes [lo, hi) of the 64
+// entries at base hold entries of the LJ part (front of the segment) or of
+// the non-LJ part (back of the segment); base is a multiple of 64 when cap is.
+struct ChunkAl {
+    size_t base;
+    int lo, hi;
+    bool lj;
+};
+
+__device__ __forceinline__ ChunkAl chunk_al(int ch, size_t e0, int cap, int2 nn) {
+    const int chA = (nn.x + 63) >> 6;
+    ChunkAl s;
+    s.lj = ch < chA;
+    if (s.lj) {
+        s.base = e0 + 64 * ch;
+        s.lo = 0;
+        s.hi = min(64, nn.x - 64 * ch);
+    } else {
+        const int k = ch - chA;
+        s.base = e0 + cap - 64 * (k + 1);
+        s.lo = max(0, 64 * (k + 1) - nn.y);
+        s.hi = 64;
+    }
+    return s;
+}
+
+// v where bit (lane) of the wave mask m is set, else 0 (one v_cndmask_b32 on
+// an SGPR-pair mask).
+__device__ __forceinline__ float lane_select(float v, uint64_t m) {
+    float r;
+    asm("v_cndmask_b32 %0, 0, %1, %2" : "=v"(r) : "v"(v), "s"(m));
+    return r;
+}
+
 // Chunk ch of cluster c: entries [base, base+n) of the LJ part (front of the
 // segment) or of the non-LJ part (back of the segment).
 __device__ __forceinline__ ChunkSpan chunk_span(int ch, size_t e0, int cap, int2 nn) {
@@ -58,8 +99,8 @@ template <bool ENERGY, bool LMD, bool SMALL, int PW>
 __global__ void __launch_bounds__(64 * PW)
 k_pair_f32(int nclu, const int *__restrict__ clu_sys, const SysP *__restrict__ sp, const double4 *__restrict__ ccen,
            const JAtom *__restrict__ ys, const float2 *__restrict__ lj32, int ccap,
-           const int2 *__restrict__ lcnt, const uint2 *__restrict__ ent, const ExcessArgs ex,
-           double *__restrict__ gslot, long long *__restrict__ epart) {
+           const int2 *__restrict__ lcnt, const uint2 *__restrict__ ent, const uint64_t *__restrict__ mb,
+           const ExcessArgs ex, double *__restrict__ gslot, long long *__restrict__ epart) {
     extern __shared__ float2 ljf[];
     __shared__ double s_sh[27][3];
     __shared__ float4 s_pos[PW][64];
@@ -71,7 +112,11 @@ k_pair_f32(int nclu, const int *__restrict__ clu_sys, const SysP *__restrict__ s
     const SysP &P = sp[clu_sys[c]];
     const int tid = threadIdx.x, l = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6), ia = l & 7, jsub = l >> 3;
     const double4 cen = ccen[c];
-    for (int k = tid; k < P.ntyp * P.ntyp; k += 64 * PW) ljf[k] = lj32[P.lj0 + k];
+    // force steps use (6 A, 12 B) so that the LJ force factor is (12 B/r^6 - 6 A) / r^8
+    for (int k = tid; k < P.ntyp * P.ntyp; k += 64 * PW) {
+        const float2 ab = lj32[P.lj0 + k];
+        ljf[k] = ENERGY ? ab : make_float2(6.f * ab.x, 12.f * ab.y);
+    }
     if (tid < 27) {
         int sx, sy, sz;
         shift_of((uint32_t)tid, sx, sy, sz);
@@ -116,9 +161,55 @@ k_pair_f32(int nclu, const int *__restrict__ clu_sys, const SysP *__restrict__ s
                 ee[k] = ev[k] = 0.f;
             }
     };
-    auto run = [&](auto ljtag, int nit) {
+    // force steps: pair masks from mb (scalar), no per-lane bit test; s_meta
+    // holds the byte offset 8*type_j of j's LJ column (| class << 20 with LMD)
+    const char *ljrow = reinterpret_cast<const char *>(ljf + trow);
+    auto pair_m = [&](auto ljtag, uint64_t M, int e) {
         constexpr bool LJ = decltype(ljtag)::value;
-        for (int it = 0; it < nit; ++it) {
+        const float4 pj = s_pos[w][e];
+        float dx = xi - pj.x, dy = yi - pj.y, dz = zi - pj.z;
+        if (SMALL) {
+            dx = dx - Lx * rintf(dx * iLx);
+            dy = dy - Ly * rintf(dy * iLy);
+            dz = dz - Lz * rintf(dz * iLz);
+        }
+        const float r2 = dx * dx + dy * dy + dz * dz;
+        // lanes outside M (r2 may be 0 there) are removed by the select,
+        // never by arithmetic
+        const float ri = __builtin_amdgcn_rsqf(r2);
+        const float ri2 = ri * ri;
+        const uint64_t in = __builtin_amdgcn_ballot_w64(r2 <= rc2) & M;
+        if (!LJ) {
+            float cq = pj.w * (ri * ri2 - fcf);
+            if (LMD) cq *= ((s_meta[w][e] >> 20) & 1) ? f1 : f0;
+            cq = lane_select(cq, in);
+            hx += cq * dx;
+            hy += cq * dy;
+            hz += cq * dz;
+        } else {
+            const int mt = s_meta[w][e];
+            const float2 q6 = *reinterpret_cast<const float2 *>(ljrow + (LMD ? (mt & 0xfffff) : mt));
+            const float ri6 = ri2 * ri2 * ri2;
+            const float lj = (q6.y * ri6 - q6.x) * ri6;
+            float cf = (qi * pj.w) * (ri * ri2 - fcf) + lj * ri2;
+            if (LMD) cf *= ((mt >> 20) & 1) ? f1 : f0;
+            cf = lane_select(cf, in);
+            fx += cf * dx;
+            fy += cf * dy;
+            fz += cf * dz;
+        }
+    };
+    auto run_m = [&](auto ljtag, const uint64_t *mrow, int it0, int it1) {
+        if (it0 == 0 && it1 == 8) {   // full chunk
+#pragma unroll
+            for (int it = 0; it < 8; ++it) pair_m(ljtag, mrow[it], it * 8 + jsub);
+        } else {
+            for (int it = it0; it < it1; ++it) pair_m(ljtag, mrow[it], it * 8 + jsub);
+        }
+    };
+    auto run = [&](auto ljtag, int it0, int it1) {
+        constexpr bool LJ = decltype(ljtag)::value;
+        for (int it = it0; it < it1; ++it) {
             const float4 pj = s_pos[w][it * 8 + jsub];
             const int mt = s_meta[w][it * 8 + jsub];
             float dx = xi - pj.x, dy = yi - pj.y, dz = zi - pj.z;
@@ -133,15 +224,6 @@ k_pair_f32(int nclu, const int *__restrict__ clu_sys, const SysP *__restrict__ s
             // the selects below, never by arithmetic
             const float ri = __builtin_amdgcn_rsqf(r2);
             const float ri2 = ri * ri;
-            if (!LJ && !ENERGY) {
-                float cq = pj.w * (ri * ri2 - fcf);
-                if (LMD) cq *= ((mt >> 20) & 1) ? f1 : f0;
-                cq = in ? cq : 0.f;
-                hx += cq * dx;
-                hy += cq * dy;
-                hz += cq * dz;
-                continue;
-            }
             const float qq = qi * pj.w;
             float cf = qq * (ri * ri2 - fcf);
             float vv = 0.f;
@@ -166,27 +248,55 @@ k_pair_f32(int nclu, const int *__restrict__ clu_sys, const SysP *__restrict__ s
             }
         }
     };
+    // Software pipeline over this wave's chunks: while a chunk computes, the
+    // j atoms of the next chunk and the entries of the one after are in flight.
+    // The loads are unconditional (lanes without an entry read entry e0 and
+    // j slot 0, both allocated, and ignore them), so that no wait is placed
+    // where they are issued.
+    auto entry_of = [&](int ch, uint2 &E) -> bool {
+        bool ok = false;
+        size_t idx = e0;
+        if (ch < nch) {
+            const ChunkAl s = chunk_al(ch, e0, cap, nn);
+            ok = l >= s.lo && l < s.hi;
+            if (ok) idx = s.base + l;
+        }
+        E = ent[idx];
+        return ok;
+    };
+    uint2 Ec, En;
+    bool okc = entry_of(w, Ec), okn = entry_of(w + PW, En);
+    JAtom bc = ys[okc ? (Ec.x & JP_SLOT_MASK) : 0u];
     int nfl = 0;
     for (int ch = w; ch < nch; ch += PW) {
-        const ChunkSpan sp_ = chunk_span(ch, e0, cap, nn);
+        const ChunkAl sp_ = chunk_al(ch, e0, cap, nn);
         wave_lds_sync();
-        if (l < sp_.n) {
-            const uint2 E = ent[sp_.base + l];
-            const int js = (int)(E.x & JP_SLOT_MASK);
-            const int code = (int)(E.x >> JP_SLOT_BITS);
-            const JAtom b = ys[js];
-            s_pos[w][l] = make_float4((float)(b.x + s_sh[code][0]), (float)(b.y + s_sh[code][1]),
-                                      (float)(b.z + s_sh[code][2]), b.q);
-            s_meta[w][l] = (int)(E.y & 0xffu) | ((b.t & 0xfff) << 8) | (((b.t >> 13) & 1) << 20);
+        if (okc) {
+            const int code = (int)(Ec.x >> JP_SLOT_BITS);
+            s_pos[w][l] = make_float4((float)(bc.x + s_sh[code][0]), (float)(bc.y + s_sh[code][1]),
+                                      (float)(bc.z + s_sh[code][2]), bc.q);
+            s_meta[w][l] = ENERGY ? (int)(Ec.y & 0xffu) | ((bc.t & 0xfff) << 8) | (((bc.t >> 13) & 1) << 20)
+                                  : ((bc.t & 0xfff) << 3) | (((bc.t >> 13) & 1) << 20);
         } else {
             s_pos[w][l] = make_float4(0.f, 0.f, 0.f, 0.f);
             s_meta[w][l] = 0;
         }
         wave_lds_sync();
-        const int nit = (sp_.n + 7) >> 3;
-        if (sp_.lj) run(std::true_type{}, nit);
-        else run(std::false_type{}, nit);
+        const JAtom bn = ys[okn ? (En.x & JP_SLOT_MASK) : 0u];
+        uint2 Enn;
+        const bool oknn = entry_of(ch + 2 * PW, Enn);
+        const int it0 = sp_.lo >> 3, it1 = (sp_.hi + 7) >> 3;
+        if (ENERGY) {
+            if (sp_.lj) run(std::true_type{}, it0, it1);
+            else run(std::false_type{}, it0, it1);
+        } else {
+            const uint64_t *mrow = mb + (sp_.base >> 3);
+            if (sp_.lj) run_m(std::true_type{}, mrow, it0, it1);
+            else run_m(std::false_type{}, mrow, it0, it1);
+        }
         if (++nfl == JP_FLUSH_CH) { nfl = 0; flush(); }
+        Ec = En; okc = okn; bc = bn;
+        En = Enn; okn = oknn;
     }
     flush();
 #pragma unroll

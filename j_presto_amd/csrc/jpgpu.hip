@@ -111,6 +111,7 @@ struct Engine {
     DBuf<int2> d_lcnt;
     DBuf<int4> d_crec;
     DBuf<uint2> d_ent;
+    DBuf<uint64_t> d_mb;            // pair mask of every entry, one byte each (8 per word)
     DBuf<double> d_gslot, d_ep64;
     DBuf<long long> d_epfix;
     DBuf<int> d_s2sys, d_s3sys;
@@ -165,7 +166,7 @@ struct Engine {
         d_perm.release(); d_cslot.release(); d_ys.release(); d_qd.release(); d_cbb.release(); d_xr.release();
         d_ccen.release(); d_imgs.release(); d_lp.release(); d_ccf.release(); d_err.release(); d_lst.release();
         d_cellext.release(); d_sysext.release(); d_lcnt.release(); d_crec.release();
-        d_ent.release(); d_gslot.release(); d_ep64.release(); d_epfix.release();
+        d_ent.release(); d_mb.release(); d_gslot.release(); d_ep64.release(); d_epfix.release();
         d_s2sys.release(); d_s3sys.release(); d_s2a.release(); d_s3a.release(); d_s2d.release(); d_s3d.release();
         d_u64.release();
         d_bat.release(); d_bptr.release(); d_bref.release(); d_bnd.release(); d_ang.release();
@@ -727,6 +728,7 @@ int Engine::build_list(bool defer) {
     ccap = ccap_force > 0 ? ccap_force : std::max(ccap, ccap_want);
     ccap_force = 0;
     AL(d_ent, (size_t)nclu * ccap);
+    AL(d_mb, (size_t)nclu * ccap / 8);
     launch_list8(*this, ccap - ccap / 8);
     CK(hipGetLastError());
     CK(hipMemcpyAsync(h_lst, d_lst.p, 4 * (size_t)nstat, hipMemcpyDeviceToHost, st));
@@ -748,6 +750,7 @@ int Engine::finish_list(bool retry) {
         ccap = std::max(ccap, ccap_want);
         CK(hipMemsetAsync(d_lst.p, 0, 4 * (size_t)nstat, st));
         AL(d_ent, (size_t)nclu * ccap);
+        AL(d_mb, (size_t)nclu * ccap / 8);
         launch_list8(*this, ccap - ccap / 8);
         CK(hipGetLastError());
         CK(hipMemcpyAsync(h_lst, d_lst.p, 4 * (size_t)nstat, hipMemcpyDeviceToHost, st));
@@ -874,7 +877,8 @@ int Engine::bonded(const double *cord, double *grad, double *eterm) {
 template <bool E, bool M, bool S, int PW>
 static void launch_pair_f32_only(unsigned nb, size_t lds32, hipStream_t st, Engine &g) {
     hipLaunchKernelGGL((k_pair_f32<E, M, S, PW>), nb, 64 * PW, lds32, st, g.nclu, g.d_clu_sys.p, g.d_sp.p, g.d_ccen.p,
-                       g.d_ys.p, g.d_lj32.p, g.ccap, g.d_lcnt.p, g.d_ent.p, g.excess_args(), g.d_gslot.p, g.d_epfix.p);
+                       g.d_ys.p, g.d_lj32.p, g.ccap, g.d_lcnt.p, g.d_ent.p, g.d_mb.p, g.excess_args(), g.d_gslot.p,
+                       g.d_epfix.p);
 }
 
 template <bool E, bool M, bool S, int PW>
@@ -885,7 +889,7 @@ static void launch_pair_pw(int prec, unsigned nb, size_t lds32, size_t lds64, hi
                            g.d_gslot.p, g.d_ep64.p);
     else
         hipLaunchKernelGGL((k_pair_f32<E, M, S, PW>), nb, 64 * PW, lds32, st, g.nclu, g.d_clu_sys.p, g.d_sp.p,
-                           g.d_ccen.p, g.d_ys.p, g.d_lj32.p, g.ccap, g.d_lcnt.p, g.d_ent.p, g.excess_args(),
+                           g.d_ccen.p, g.d_ys.p, g.d_lj32.p, g.ccap, g.d_lcnt.p, g.d_ent.p, g.d_mb.p, g.excess_args(),
                            g.d_gslot.p, g.d_epfix.p);
 }
 
@@ -906,7 +910,7 @@ template <int LW>
 static void launch_list8_lw(Engine &g, int hiwater) {
     hipLaunchKernelGGL(k_list8<LW>, grid1(g.nclu, LW), 64 * LW, 0, g.st, g.nclu, g.d_clu_sys.p, g.d_sp.p, g.d_cbb.p,
                        g.d_ccf.p, g.d_crec.p, g.d_sysext.p, g.d_lp.p, g.d_xr.p, g.d_imgs.p, g.d_xptr.p, g.d_xidx.p,
-                       g.ccap, hiwater, g.d_ent.p, g.d_lcnt.p, g.d_lst.p);
+                       g.ccap, hiwater, g.d_ent.p, reinterpret_cast<uint8_t *>(g.d_mb.p), g.d_lcnt.p, g.d_lst.p);
 }
 
 static void launch_list8(Engine &g, int hiwater) {
@@ -1503,7 +1507,7 @@ static int set_option(Engine &E, int32_t option, int32_t value) {
             E.autotune = 0;
             return 0;
         case JP_OPT_LIST_SEGMENT:
-            if (value < 8) return JP_EINVAL;
+            if (value < 64 || value % 64) return JP_EINVAL;   // whole 64-entry chunks
             E.ccap_force = value;
             return 0;
         default:
