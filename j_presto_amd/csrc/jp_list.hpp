@@ -311,20 +311,14 @@ k_list8(int nclu, const int *__restrict__ clu_sys, const SysP *__restrict__ sp, 
     bool ovf = false;
     const int jsub = l >> 3, ja = l & 7;
 
-    // test j-clusters s_q[qh .. qh+nq) (nq <= 8): lane = atom ja of entry jsub
-    auto drain = [&](int nq) {
-        const bool qv = jsub < nq;
-        const uint32_t qe = qv ? s_q[w][(qh + jsub) & (JP_QCAP - 1)] : 0u;
+    // test one queued j-cluster per 8 lanes (lane = atom ja of queue entry qe,
+    // valid if qv; q = its lp record, loaded by the caller)
+    auto test = [&](bool qv, uint32_t qe, float4 q) __attribute__((always_inline)) {
         const int jc = (int)(qe & JP_SLOT_MASK);
         const uint32_t code = qe >> JP_SLOT_BITS;
         const int js = jc * JP_CL + ja;
         uint32_t m = 0, amb = 0;
-        int meta = -1;
-        float4 q = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (qv) {
-            q = lp[js];
-            meta = __float_as_int(q.w);
-        }
+        const int meta = qv ? __float_as_int(q.w) : -1;
         const int molj = meta & 0x3fffffff;
         // j relative to the cluster centre, image applied: 3 roundings like
         // (q - cf) + L*s, within the error bound of the band
@@ -409,21 +403,33 @@ k_list8(int nclu, const int *__restrict__ clu_sys, const SysP *__restrict__ sp, 
             ovf = true;
         }
     };
+    // test j-clusters s_q[qh .. qh+nq) (nq <= 16) as two batches of 8, both
+    // loaded before either is tested (unconditional loads: slot 0 for lanes
+    // without an entry, so that no wait lands where they are issued)
+    auto drain = [&](int nq) {
+        const bool qv0 = jsub < nq, qv1 = 8 + jsub < nq;
+        const uint32_t qe0 = qv0 ? s_q[w][(qh + jsub) & (JP_QCAP - 1)] : 0u;
+        const uint32_t qe1 = qv1 ? s_q[w][(qh + 8 + jsub) & (JP_QCAP - 1)] : 0u;
+        const float4 q0 = lp[qv0 ? (int)(qe0 & JP_SLOT_MASK) * JP_CL + ja : 0];
+        const float4 q1 = lp[qv1 ? (int)(qe1 & JP_SLOT_MASK) * JP_CL + ja : 0];
+        test(qv0, qe0, q0);
+        if (nq > 8) test(qv1, qe1, q1);
+    };
 
-    // One loop, one site per phase, highest priority first: drain 8 queued
+    // One loop, one site per phase, highest priority first: drain 16 queued
     // j-clusters; box-test 64 candidates (independent loads); append one
     // round of the current cell batch's j-clusters (LDS only); test the next
-    // 64 candidate cells.  Bounds: the queue holds <= 7 + 64 and the
+    // 64 candidate cells.  Bounds: the queue holds <= 15 + 64 and the
     // candidate ring <= 63 + 64 entries when written, both < JP_QCAP.
     int ch = 0, ct = 0, qb = -64, rnd = 0, k0 = 0, k1 = 0;
     uint32_t ccode = 13;
     bool cells_done = false;
     for (;;) {
         const int nq = qt - qh, nc = ct - ch;
-        if (nq >= 8 || (cells_done && nc == 0 && nq > 0)) {
+        if (nq >= 16 || (cells_done && nc == 0 && nq > 0)) {
             wave_lds_sync();
-            drain(min(nq, 8));
-            qh += min(nq, 8);
+            drain(min(nq, 16));
+            qh += min(nq, 16);
             wave_lds_sync();
             continue;
         }
