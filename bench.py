@@ -7,9 +7,9 @@ state (pair forces, ZD excess, kick/drift, SHAKE, velocity update, list
 rebuild every 10 steps, energies every 100 steps; the plain steps between
 rebuilds replayed from captured HIP graphs); jp_run_nve returns after a
 stream synchronize, and the host clock brackets it.  The pair kernel's
-duration for the roofline comes from HIP events around each of 200 pair
-launches of a host-launched segment that follows, on the same stream and
-state.  Beside it: the plain FP64 GPU path (same protocol, fewer steps) and
+duration for the roofline comes from two HIP events around 200 force-step
+pair launches back to back (jp_time_pair) that follow, on the same stream
+and state.  Beside it: the plain FP64 GPU path (same protocol, fewer steps) and
 the reference md_run on the host cores (bounded sample).
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
@@ -91,12 +91,8 @@ def run_gpu(system, precision, steps, warmup, state=None, pair_steps=200, autotu
     assert ier == 0, ier
     tm = md.timing()
     n, nlj, nlisted = md.count_pairs(detail=True)
-    # pair-kernel duration: HIP events around every pair launch of a segment
-    # launched from the host (graph replays are not timed per kernel)
-    md.set_graphs(False)
-    _, ier = md.runmd(pair_steps, update=UPDATE)
-    assert ier == 0, ier
-    tp = md.timing()["pair"] / pair_steps
+    # pair-kernel duration: HIP events around pair_steps launches back to back
+    tp = md.time_pair(pair_steps)
     shapes = {"pair_waves": md.get_option(native.JP_OPT_PAIR_WAVES),
               "list_waves": md.get_option(native.JP_OPT_LIST_WAVES), "autotuned": bool(autotune)}
     md.close()

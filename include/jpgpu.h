@@ -187,6 +187,13 @@ int jp_run_nve(jp_ctx *ctx, int32_t nsteps, int32_t update, int32_t nmon, int32_
  * graph are not timed one by one. */
 int jp_last_timing(jp_ctx *ctx, double *t_total, double *t_pair, double *t_list, double *t_step);
 
+/* Mean duration in seconds of the force-step pair kernel (the kernel of every
+ * step without energies) over reps launches back to back on the current list
+ * and coordinates, measured with two HIP events around the launches.  A
+ * measurement helper with no counterpart in the reference; it leaves the
+ * positions and velocities alone. */
+int jp_time_pair(jp_ctx *ctx, int32_t precision, int32_t reps, double *seconds);
+
 /* Unique pairs with r <= cutoff at the device coordinates (the
  * pair-interactions/s numerator), those of them where both atoms carry LJ,
  * and the unique pairs in the list (r <= cutoff+1.5 at the last rebuild).

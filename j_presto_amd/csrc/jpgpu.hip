@@ -242,6 +242,7 @@ struct Engine {
     void drop_graphs();
     int get_list(int32_t *cell3, int64_t *npairs, int32_t *ptr, int32_t *idx, int64_t cap);
     int count_pairs(int64_t *np, int64_t *nlj, int64_t *nlist);
+    int time_pair(int prec, int reps, double &sec);
 };
 
 static void launch_list8(Engine &g, int hiwater);
@@ -787,6 +788,25 @@ static void launch_pair(int prec, unsigned nb, size_t lds32, size_t lds64, hipSt
 
 // Pair-kernel variant: bit 1 class factors differ from 1 (LMD passes), bit 0
 // some box edge < 2*rlim (per-pair minimum image).
+// reps force-step pair launches back to back between two events (one warm-up
+// launch first); the pair kernel only rewrites the gradient slots.
+int Engine::time_pair(int prec, int reps, double &sec) {
+    int r;
+    if (!have_list && (r = build_list())) return r;
+    if ((r = set_factors(-1))) return r;
+    if ((r = refresh_sorted())) return r;
+    if ((r = pair(prec, false))) return r;
+    CK(hipEventRecord(ev[0], st));
+    for (int k = 0; k < reps; ++k)
+        if ((r = pair(prec, false))) return r;
+    CK(hipEventRecord(ev[1], st));
+    CK(hipEventSynchronize(ev[1]));
+    float ms = 0;
+    CK(hipEventElapsedTime(&ms, ev[0], ev[1]));
+    sec = ms * 1e-3 / reps;
+    return 0;
+}
+
 int Engine::pair_variant() const {
     bool lmd = false, small = false;
     for (int s = 0; s < nsys; ++s) {
@@ -1430,6 +1450,12 @@ int jp_last_timing(jp_ctx *ctx, double *t_total, double *t_pair, double *t_list,
     if (t_list) *t_list = ctx->e.t_list;
     if (t_step) *t_step = ctx->e.t_step;
     return 0;
+}
+
+int jp_time_pair(jp_ctx *ctx, int32_t precision, int32_t reps, double *seconds) {
+    if (!ctx || !seconds || reps <= 0 || (precision != JP_PREC_FP64 && precision != JP_PREC_FP32_FIXED))
+        return JP_EINVAL;
+    return ctx->e.time_pair(precision, reps, *seconds);
 }
 
 int jp_count_pairs(jp_ctx *ctx, int64_t *npairs, int64_t *nlj, int64_t *nlisted) {

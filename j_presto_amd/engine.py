@@ -159,6 +159,12 @@ class MdSystem:
         self.lib.jp_last_timing(self.ctx, *[C.byref(x) for x in t])
         return dict(total=t[0].value, pair=t[1].value, list=t[2].value, step=t[3].value)
 
+    def time_pair(self, reps: int = 200) -> float:
+        """Mean force-step pair-kernel duration (s) over reps back-to-back launches."""
+        t = C.c_double()
+        native.check("jp_time_pair", self.lib.jp_time_pair(self.ctx, self.precision, reps, C.byref(t)))
+        return t.value
+
     def count_pairs(self, detail: bool = False):
         """Unique pairs with r <= rc; with detail, (all, both-LJ, listed)."""
         n, nlj, nl = C.c_int64(), C.c_int64(), C.c_int64()

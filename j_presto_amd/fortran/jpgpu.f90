@@ -147,6 +147,13 @@ module jpgpu
        integer(c_int64_t), intent(out) :: npairs, nlj, nlisted
      end function jp_count_pairs
 
+     integer(c_int) function jp_time_pair(ctx, precision, reps, seconds) bind(C, name='jp_time_pair')
+       import :: c_int, c_int32_t, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: precision, reps
+       real(c_double), intent(out) :: seconds
+     end function jp_time_pair
+
      integer(c_int) function jp_batch_create(descs, nsys, b) bind(C, name='jp_batch_create')
        import :: c_int, c_int32_t, c_ptr, jp_sys_desc
        type(jp_sys_desc), intent(in) :: descs(*)

@@ -135,6 +135,14 @@ class System:
     def nfrag(self) -> int:
         return int(self.cluster.max())
 
+    def temperature_groups(self) -> np.ndarray:
+        """0-based temperature group of every atom for TEMPCO= YES (md.f90:
+        136-152): ixamol2 = (molecule kind - 1) * nfrag + cluster, renumbered
+        without the empty values."""
+        kind = np.concatenate([np.full(m.natom * m.ncopy, k + 1, np.int64) for k, m in enumerate(self.mols)])
+        g = (kind - 1) * self.nfrag + self.cluster
+        return np.unique(g, return_inverse=True)[1].astype(np.int32)
+
     @property
     def celwal(self) -> np.ndarray:
         return np.array([0.0, self.box[0], 0.0, self.box[1], 0.0, self.box[2]])

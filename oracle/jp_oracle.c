@@ -24,6 +24,7 @@
  *   dne14_PB     dnergv.f90:433-516 (1-4 LJ and electrostatics)
  *   shake        shake.f90:112-327 (3-atom Newton), 687-755 (2-atom exact)
  *   verlet       verlet.f90:144-215 (MICR leap-frog)
+ *   verlethe     verlethe.f90:79-757 (CANO velocity scaling + SHAKE loop)
  *   caltmp       caltmp.f90:18-27
  *
  * Parity pins (tests/): the compiled reference md_run (golden vectors under
@@ -1091,6 +1092,106 @@ int jo_verlet_step(const jo_ctx *c, double *x, double *v, const double *gtot, do
             for (int k = 0; k < 3; ++k) v[3 * i + k] = (x[3 * i + k] - xold[3 * i + k]) * r;
     }
     return 0;
+}
+
+/* caltmp.f90:25-43 (one group) and caltmp2 (caltmp.f90:82-105, groups):
+ * T[g] = (sum over the group's atoms, in atom order, of m |w|^2) * 1e7 /
+ * (dof[g] * gascst). */
+static void group_temps(const jo_ctx *c, const double *w, int ngrp, const int *grp, const double *dof, double *T) {
+    for (int g = 0; g < ngrp; ++g) T[g] = 0.0;
+    for (int i = 0; i < c->n; ++i) {
+        const int g = grp ? grp[i] : 0;
+        T[g] = T[g] + c->mass[i] * (w[3 * i] * w[3 * i] + w[3 * i + 1] * w[3 * i + 1] + w[3 * i + 2] * w[3 * i + 2]);
+    }
+    for (int g = 0; g < ngrp; ++g) T[g] = T[g] * 1.0e7 / (dof[g] * 8.31451); /* PHYCNS.f90:21 gascst */
+}
+
+/* verlethe.f90:79-757 for METHOD= CANO (idfvsl = 3) with constant lambda,
+ * STOPCE= NO and HEATLO= 0: itmeth 1 (grp == NULL, one temperature) or 2
+ * (grp[i] = 0-based temperature group of atom i, md.f90:136-172).  gtot is
+ * the lambda-combined gradient; v is v(t-dt/2) on entry, v(t+dt/2) on exit;
+ * vback receives v(t-dt/2) and xold x(t).  Velocity scaling toward temps
+ * (chi = sqrt(temps/T) on the unconstrained half-step velocities), then up
+ * to icslop rounds of SHAKE + temperature (exit when every group is within
+ * fcstol*temps).  A SHAKE failure returns its ier (md_run would halve dt
+ * and retry, verlethe.f90:546-573).  Expressions keep md_run's operand order. */
+int jo_cano_step(const jo_ctx *c, double *x, double *v, const double *gtot, double *xold, double *vback,
+                 double temps, int icslop, double fcstol, int ngrp, const int *grp, const double *dof) {
+    const int n = c->n;
+    const double dt = c->dt, conv = 4.184e-4; /* md.f90:56 fdcvcv */
+    double *im = (double *)malloc(sizeof(double) * n);
+    double *w = (double *)malloc(sizeof(double) * 3 * (size_t)n);
+    double *fc = (double *)calloc(3 * (size_t)n, sizeof(double));
+    double *xb = (double *)malloc(sizeof(double) * 3 * (size_t)n);
+    double T[64];
+    int ier = 0;
+    if (ngrp > 64) { free(im); free(w); free(fc); free(xb); return JP_EINVAL; }
+    for (int i = 0; i < n; ++i) im[i] = 1.0 / c->mass[i]; /* md.f90:68 ifxmass */
+    memcpy(xold, x, sizeof(double) * 3 * n);
+    memcpy(vback, v, sizeof(double) * 3 * n);
+    double r = 0.5 * dt * conv;
+    for (int i = 0; i < n; ++i)
+        for (int k = 0; k < 3; ++k) w[3 * i + k] = v[3 * i + k] - r * gtot[3 * i + k] * im[i];
+    if (c->nshk) group_temps(c, w, ngrp, grp, dof, T);
+    else for (int g = 0; g < ngrp; ++g) T[g] = temps;
+    for (int i = 0; i < n; ++i) {
+        const double itc = 1.0 / T[grp ? grp[i] : 0];
+        const double chi = sqrt(temps * itc);
+        const double rt = chi * dt * conv, rt2 = 2.0 * chi - 1.0;
+        for (int k = 0; k < 3; ++k) v[3 * i + k] = rt2 * v[3 * i + k] - rt * gtot[3 * i + k] * im[i];
+    }
+    for (int i = 0; i < 3 * n; ++i) x[i] = x[i] + dt * v[i];
+    if (c->nshk) {
+        for (int it = 1; it <= icslop; ++it) {
+            memcpy(xb, x, sizeof(double) * 3 * n);
+            ier = jo_shake(c, x, xold);
+            if (ier) break;
+            r = 1.0 / dt;
+            for (int i = 0; i < 3 * n; ++i) {
+                v[i] = (x[i] - xold[i]) * r;
+                w[i] = (vback[i] + v[i]) * 0.5;
+            }
+            group_temps(c, w, ngrp, grp, dof, T);
+            double tmin = T[0], tmax = T[0];
+            for (int g = 1; g < ngrp; ++g) { tmin = fmin(tmin, T[g]); tmax = fmax(tmax, T[g]); }
+            if (fmax(fabs(tmin - temps), fabs(tmax - temps)) < fcstol * temps) break;
+            if (it == icslop) break;
+            r = 1.0 / (dt * dt);
+            const double r2 = 0.5 * dt;
+            for (int i = 0; i < n; ++i)
+                for (int k = 0; k < 3; ++k) {
+                    const int q = 3 * i + k;
+                    fc[q] = fc[q] + r * (x[q] - xb[q]);
+                    w[q] = r2 * (-conv * gtot[q] * im[i] + fc[q]) + vback[q];
+                }
+            group_temps(c, w, ngrp, grp, dof, T);
+            for (int i = 0; i < n; ++i) {
+                const double itc = 1.0 / T[grp ? grp[i] : 0];
+                const double chi = sqrt(temps * itc);
+                const double rt = chi * dt, rt2 = 2.0 * chi - 1.0;
+                for (int k = 0; k < 3; ++k) {
+                    const int q = 3 * i + k;
+                    v[q] = (-conv * gtot[q] * im[i] + fc[q]) * rt + vback[q] * rt2;
+                }
+            }
+            for (int i = 0; i < 3 * n; ++i) x[i] = xold[i] + dt * v[i];
+        }
+    }
+    free(im); free(w); free(fc); free(xb);
+    return ier;
+}
+
+/* Degrees of freedom per temperature group (md.f90:114-118 for one group,
+ * 172-183 per group): 3 per atom minus the constraints of every SHAKE group
+ * ((h+1)h/2 for h atoms besides the first), no centre-of-mass term. */
+void jo_group_dof(const jo_ctx *c, int ngrp, const int *grp, double *dof) {
+    for (int g = 0; g < ngrp; ++g) dof[g] = 0.0;
+    for (int i = 0; i < c->n; ++i) dof[grp ? grp[i] : 0] += 3.0;
+    for (int g = 0; g < c->nshk; ++g) {
+        const int h = c->shk_n[g] - 1;
+        const int a = c->shk_a[4 * g];
+        dof[grp ? grp[a] : 0] -= (double)((h + 1) * h / 2);
+    }
 }
 
 /* grad3 = lambda^2 g1 + lambda g2 + g3 (verlet.f90:67-91, nfrg = 3) or the

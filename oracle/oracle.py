@@ -44,6 +44,9 @@ def lib():
         L.jo_bonded.argtypes = [C.c_void_p, f64p, f64p, f64p]
         L.jo_verlet_step.argtypes = [C.c_void_p, f64p, f64p, f64p, f64p]
         L.jo_combine_grad.argtypes = [C.c_void_p, f64p, f64p]
+        L.jo_cano_step.argtypes = [C.c_void_p, f64p, f64p, f64p, f64p, f64p, C.c_double, C.c_int, C.c_double,
+                                   C.c_int, i32p, f64p]
+        L.jo_group_dof.argtypes = [C.c_void_p, C.c_int, i32p, f64p]
         L.jo_ekin.argtypes = [C.c_void_p, f64p]
         L.jo_ekin.restype = C.c_double
         L.jo_run_nve.argtypes = [C.c_void_p, f64p, f64p, C.c_int, C.c_int]
@@ -128,6 +131,25 @@ class Oracle:
     def verlet_step(self, x, v, gtot) -> int:
         xo = np.zeros_like(x)
         return lib().jo_verlet_step(self.c, _p(x), _p(v), _p(np.ascontiguousarray(gtot)), _p(xo))
+
+    def group_dof(self, grp=None):
+        """Degrees of freedom per temperature group (grp: 0-based per atom, or
+        None for one group)."""
+        ng = 1 if grp is None else int(grp.max()) + 1
+        dof = np.zeros(ng)
+        g = None if grp is None else np.ascontiguousarray(grp, np.int32)
+        lib().jo_group_dof(self.c, ng, None if g is None else _p(g, i32p), _p(dof))
+        return dof
+
+    def cano_step(self, x, v, gtot, temps: float, icslop: int = 3, fcstol: float = 1e-3, grp=None):
+        """One METHOD= CANO step in place; returns (ier, vback)."""
+        xo = np.zeros_like(x)
+        vb = np.zeros_like(v)
+        dof = self.group_dof(grp)
+        g = None if grp is None else np.ascontiguousarray(grp, np.int32)
+        ier = lib().jo_cano_step(self.c, _p(x), _p(v), _p(np.ascontiguousarray(gtot)), _p(xo), _p(vb), temps,
+                                 icslop, fcstol, len(dof), None if g is None else _p(g, i32p), _p(dof))
+        return ier, vb
 
     def ekin(self, v) -> float:
         return lib().jo_ekin(self.c, _p(np.ascontiguousarray(v)))

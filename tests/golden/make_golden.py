@@ -11,10 +11,13 @@ and N_SIMD= 1 (a fixed summation order):
   traj job   RESTAR= YES, METHOD= MICR, LOOPLI= K: one restart step plus K
              leap-frog steps with SHAKE; the output restart holds x and v in
              FP64 and NAMETO the potential energy of every step.
+  cano job   (some systems) RESTAR= YES, METHOD= CANO, SETTEM= 300,
+             TEMPCO= NO or YES, LOOPLI= K: K velocity-scaled steps
+             (verlethe.f90); the output restart holds x and v.
 A fixture stores the generator recipe, the input x and v, and the outputs.
 The reference itself never leaves this container.
 
-Usage: python tests/golden/make_golden.py [name ...]
+Usage: python tests/golden/make_golden.py [--cano-only] [name ...]
 """
 from __future__ import annotations
 
@@ -38,13 +41,13 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 
 RECIPES = {
     # small box just above 2*cutoff (box < 2*rlim: periodic images inside the list radius)
-    "water_1500": ("water_box", dict(nwat=500, L=24.8, seed=7), dict(traj=20)),
+    "water_1500": ("water_box", dict(nwat=500, L=24.8, seed=7), dict(traj=20, cano=("NO", 20))),
     # the 6k-atom water workload itself
     "water_6k": ("workload", dict(i=0), dict(traj=20)),
     # chain with 16 LJ types, 2-atom SHAKE groups, chain exclusions
-    "chain_3k": ("chain_in_water", dict(n_chain=200, L=30.0, seed=11), dict(traj=20)),
+    "chain_3k": ("chain_in_water", dict(n_chain=200, L=30.0, seed=11), dict(traj=20, cano=("YES", 20))),
     # LMD: three energy classes, lambda-scaled gradient
-    "chain_lmd": ("chain_in_water", dict(n_chain=160, L=28.0, seed=12, lmd=0.7), dict(traj=10)),
+    "chain_lmd": ("chain_in_water", dict(n_chain=160, L=28.0, seed=12, lmd=0.7), dict(traj=10, cano=("YES", 10))),
     # chain with harmonic bonds and angles (dnebnd_PB / dneang_PB), strained off their minima
     "chain_bonded": ("chain_in_water", dict(n_chain=200, L=30.0, seed=15, bond_fc=300.0, angle_fc=50.0,
                                             bond_r0=1.53, angle_deg=112.0), dict(traj=20)),
@@ -120,6 +123,7 @@ def make(name: str) -> str:
             out["traj_loop"] = np.int64(r["loop"])
             ene = np.loadtxt(os.path.join(t2, "ene.dat"), ndmin=2)
             out["traj_epot"] = ene
+        out.update(cano_job(s, opt, tmp))
         path = os.path.join(HERE, f"{name}.npz")
         np.savez_compressed(path, **out)
         return path
@@ -127,12 +131,45 @@ def make(name: str) -> str:
         shutil.rmtree(tmp, ignore_errors=True)
 
 
+def cano_job(s: systems.System, opt: dict, tmp: str) -> dict:
+    if "cano" not in opt:
+        return {}
+    tempco, ncano = opt["cano"]
+    t3 = os.path.join(tmp, "cano")
+    os.makedirs(t3)
+    refio.write_restart(os.path.join(t3, "in.rst"), s.x, s.v, lam=s.lam)
+    run_md(t3, s, {"METHOD": "CANO", "RESTAR": "YES", "NAMERI": "in.rst", "NAMERO": "out.rst",
+                   "LOOPLI": str(ncano), "UPDATE": "10", "TIMEST": "2.0", "SETTEM": "300.0",
+                   "TEMPCO": tempco, "OUTLOG": "100000", "OUTCLE": "100000"})
+    r = refio.read_restart(os.path.join(t3, "out.rst"))
+    return dict(cano_tempco=np.array(tempco), cano_nsteps=np.int64(ncano), cano_x=r["x"], cano_v=r["v"])
+
+
+def add_cano(name: str) -> str:
+    """Adds the cano job's outputs to an existing fixture (same recipe)."""
+    kind, args, opt = RECIPES[name]
+    s = build_system(kind, args)
+    path = os.path.join(HERE, f"{name}.npz")
+    with np.load(path, allow_pickle=False) as z:
+        out = {k: z[k] for k in z.files}
+    assert str(out["fingerprint"]) == fingerprint(s)
+    tmp = tempfile.mkdtemp(prefix="jpgold_")
+    try:
+        out.update(cano_job(s, opt, tmp))
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    np.savez_compressed(path, **out)
+    return path
+
+
 def main(argv):
     if not os.path.exists(MD_RUN):
         subprocess.run(["bash", os.path.join(ROOT, "oracle", "build_ref.sh")], check=True)
+    cano_only = "--cano-only" in argv
+    argv = [a for a in argv if a != "--cano-only"]
     names = argv or list(RECIPES)
     for n in names:
-        p = make(n)
+        p = add_cano(n) if cano_only else make(n)
         print(n, "->", os.path.relpath(p, ROOT), os.path.getsize(p), "bytes", flush=True)
 
 

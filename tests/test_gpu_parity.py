@@ -184,6 +184,22 @@ def test_ier99_excluded_pair_out_of_list():
     assert md.celprp(x) == 99
 
 
+@pytest.mark.parametrize("prec", [engine.FP32, engine.FP64])
+def test_time_pair_leaves_state_and_forces(prec):
+    """jp_time_pair: a positive mean duration; positions, velocities and the
+    forces of the next evaluation are those without it."""
+    s = systems.water_box(500, 24.8, 7)
+    md = engine.MdSystem(s, precision=prec)
+    md.celprp()
+    g0 = md.dnergy()[2]
+    x0, v0 = md.get_state()
+    t = md.time_pair(5)
+    assert 0.0 < t < 1.0
+    x1, v1 = md.get_state()
+    assert np.array_equal(x0, x1) and np.array_equal(v0, v1)
+    assert np.array_equal(md.dnergy()[2], g0)
+
+
 def test_count_pairs_matches_bruteforce():
     s = systems.water_box(500, 24.8, 7)
     md = engine.MdSystem(s)

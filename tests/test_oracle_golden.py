@@ -71,3 +71,44 @@ def test_potential_energy_series(golden):
     ref = f["traj_epot"][:, 0]
     # E15.7 keeps 7 significant digits: compare the same decimal rounding
     assert [float(f"{e:.6e}") for e in ep] == list(ref)
+
+
+@pytest.mark.parametrize("name", ["water_1500", "chain_3k", "chain_lmd"])
+def test_cano_trajectory_bit_exact(golden, name):
+    """md_run METHOD= CANO (verlethe.f90) from a restart, TEMPCO= NO (one
+    temperature) or YES (one per molecule kind and cluster): velocity scaling
+    to 300 K, then the SHAKE/temperature rounds, list rebuilt after every
+    10th step."""
+    s, f = golden(name)
+    o = Oracle(s)
+    grp = None if str(f["cano_tempco"]) == "NO" else s.temperature_groups()
+    x, v = s.x.copy(), s.v.copy()
+    assert o.celprp(x) == 0
+    for idloop in range(1, int(f["cano_nsteps"]) + 1):
+        g, et = o.bonded(x)
+        g, ee, ev = o.nonbond(x, g)
+        ier, vb = o.cano_step(x, v, o.combine(g), 300.0, 3, 1e-3, grp)
+        assert ier == 0
+        if idloop % 10 == 0:
+            assert o.celprp(x) == 0
+    assert np.array_equal(x, f["cano_x"])
+    assert np.array_equal(v, f["cano_v"])
+
+
+def test_cano_groups_and_dof():
+    """Temperature groups and their degrees of freedom in closed form: rigid
+    water has 6 per molecule; a chain in water splits into the chain's kind
+    and the water's kind (TEMPCO= YES), each with 3 per atom minus its own
+    SHAKE constraints (one per X-H group on the chain, three per water)."""
+    from j_presto_amd import systems
+    s = systems.water_box(60, 14.0, 3)
+    o = Oracle(s)
+    assert o.group_dof()[0] == 6 * 60
+    assert s.temperature_groups().max() == 0
+    c = systems.chain_in_water(n_chain=40, L=20.0, seed=5)
+    grp = c.temperature_groups()
+    nch = int((grp == 0).sum())
+    nw = (c.n - nch) // 3
+    n_xh = sum(1 for atoms, _ in c.shake_groups() if len(atoms) == 2)
+    dof = Oracle(c).group_dof(grp)
+    assert list(dof) == [3.0 * nch - n_xh, 6.0 * nw]
