@@ -16,6 +16,7 @@ usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
        python bench.py --workload {0,1,4}   other single-system workloads (same protocol)
        python bench.py --ensemble           64 uneven systems: one batch vs 64 sequential runs
        python bench.py --drift N            N-step NVE energy drift, FP32/fixed point vs FP64
+       python bench.py --method CANO        the same runs with METHOD= CANO (velocity scaling, 300 K)
 """
 from __future__ import annotations
 
@@ -70,11 +71,13 @@ def equilibrated(system, nsteps: int):
     return state
 
 
-def run_gpu(system, precision, steps, warmup, state=None, pair_steps=200, autotune=True):
+def run_gpu(system, precision, steps, warmup, state=None, pair_steps=200, autotune=True, cano=False):
     from j_presto_amd import engine, native
     md = engine.MdSystem(system, dt=DT_FS, precision=precision)
     if state is not None:
         md.set_state(*state)
+    if cano:   # METHOD= CANO, SETTEM= 300 K, TEMPCO= NO
+        md.set_canonical(300.0)
     ier = md.celprp()
     assert ier == 0, ier
     if autotune:   # launch shapes timed by a zero-step run, before the warm-up
@@ -279,6 +282,8 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=0)
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--equil", type=int, default=1000, help="velocity-rescaled set-up steps before timing")
+    ap.add_argument("--method", default="MICR", choices=["MICR", "CANO"],
+                    help="md_run METHOD of the timed runs (CANO: 300 K, TEMPCO= NO)")
     args = ap.parse_args()
     if args.calibrate_copy:
         calibrate_copy()
@@ -299,9 +304,10 @@ def main():
     from j_presto_amd import engine, systems
     s = systems.workload(args.workload)
     state = equilibrated(s, args.equil)
-    r32 = run_gpu(s, engine.FP32, args.steps, args.warmup, state)
+    cano = args.method == "CANO"
+    r32 = run_gpu(s, engine.FP32, args.steps, args.warmup, state, cano=cano)
     n64 = args.fp64_steps or max(args.steps // 4, 100)
-    r64 = run_gpu(s, engine.FP64, n64, min(args.warmup, 50), state)
+    r64 = run_gpu(s, engine.FP64, n64, min(args.warmup, 50), state, cano=cano)
     sps = args.steps / r32["wall"]
     value = ns_per_day(sps)
     v64 = ns_per_day(n64 / r64["wall"])
@@ -317,7 +323,8 @@ def main():
     et = r32["log"]["etot"]
     out = {
         "metric": ("ns/day + pair-interactions/s, 100k-atom ZD box; FP32/fixed-pt vs FP64 GPU" if args.workload == 2
-                   else f"ns/day + pair-interactions/s, {systems.WORKLOADS[args.workload]}"),
+                   else f"ns/day + pair-interactions/s, {systems.WORKLOADS[args.workload]}")
+                  + ("; METHOD= CANO" if cano else ""),
         "value": round(value, 3),
         "unit": "ns/day",
         "n_gpus": 1,
@@ -331,7 +338,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": systems.WORKLOADS[args.workload], "natom": s.n, "cutoff_A": 12.0, "dt_fs": DT_FS,
                    "list_update": UPDATE, "energy_every": NMON, "precision": "FP32 pair math, int64 fixed-point "
-                   "accumulation, FP64 SHAKE/integration", "launch_shapes": r32["shapes"]},
+                   "accumulation, FP64 SHAKE/integration", "launch_shapes": r32["shapes"],
+                   "method": args.method},
         "pair_interactions_per_s": round(r32["pairs"] * sps, 1),
         "pairs_within_cutoff": r32["pairs"],
         "fp64_gpu_ns_per_day": round(v64, 3),
@@ -352,7 +360,7 @@ def main():
                               "rest": round(1e3 * (r32["wall"] - r32["timing"]["list"]) / args.steps - 1e3 * t_pair, 4)},
         "cpu_baseline": None,
     }
-    if rank == 0 and not args.no_cpu and args.workload == 2:
+    if rank == 0 and not args.no_cpu and args.workload == 2 and not cano:
         cb = cpu_reference(s, args.cpu_steps)
         out["cpu_baseline"] = cb
     print(json.dumps(out), flush=True)

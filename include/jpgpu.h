@@ -175,7 +175,8 @@ int jp_shake(jp_ctx *ctx, double *cord, const double *cord_old);
 /* runmd/verlet for METHOD=MICR: nsteps leap-frog steps on the device state
  * (pair forces, bonded terms, kick, drift, SHAKE, v=(x-x_old)/dt), list rebuilt every `update`
  * steps, energies recorded every `nmon` steps into log[nsteps/nmon].
- * (md.f90:519-694, verlet.f90:2-226) */
+ * (md.f90:519-694, verlet.f90:2-226).  After jp_set_canonical the step is
+ * verlethe's METHOD=CANO step instead (verlethe.f90:79-757). */
 int jp_run_nve(jp_ctx *ctx, int32_t nsteps, int32_t update, int32_t nmon, int32_t precision,
                jp_energy_log *log);
 
@@ -186,6 +187,22 @@ int jp_run_nve(jp_ctx *ctx, int32_t nsteps, int32_t update, int32_t nmon, int32_
  * JP_OPT_GRAPHS = 0) times the number of steps; steps replayed from a HIP
  * graph are not timed one by one. */
 int jp_last_timing(jp_ctx *ctx, double *t_total, double *t_pair, double *t_list, double *t_step);
+
+/* Canonical MD (md_run METHOD= CANO, verlethe.f90:79-757 with constant
+ * lambda, STOPCE= NO and HEATLO= 0) for the following jp_run_nve calls:
+ * every step scales the unconstrained half-step velocities by
+ * chi = sqrt(temps / T) (caltmp.f90), then runs up to icslop rounds of SHAKE
+ * and temperature (md_run's defaults 3 and fcstol 1e-3, main.f90:166-167)
+ * until every temperature group is within fcstol * temps.  group: NULL for
+ * one temperature per system (TEMPCO= NO), or per atom a positive label of
+ * its temperature group (TEMPCO= YES: md_run's ixamol2, md.f90:136-152);
+ * in a batch, labels are per system and the array covers all atoms in order.
+ * Degrees of freedom per group: 3 per atom minus the constraints of the SHAKE
+ * groups whose first atom is in it.  on = 0 returns to METHOD= MICR.  A SHAKE
+ * failure ends the run with its ier (md_run halves dt and retries,
+ * verlethe.f90:546-573).  Returns JP_EINVAL for temps <= 0, icslop < 1, a
+ * label < 1 or a group without degrees of freedom. */
+int jp_set_canonical(jp_ctx *ctx, int32_t on, double temps, int32_t icslop, double fcstol, const int32_t *group);
 
 /* Mean duration in seconds of the force-step pair kernel (the kernel of every
  * step without energies) over reps launches back to back on the current list
@@ -246,6 +263,11 @@ int jp_set_option(jp_ctx *ctx, int32_t option, int32_t value);
 int jp_batch_set_option(jp_batch *b, int32_t option, int32_t value);
 int jp_get_option(jp_ctx *ctx, int32_t option, int32_t *value);
 int jp_batch_get_option(jp_batch *b, int32_t option, int32_t *value);
+
+/* jp_set_canonical for every system of a batch (group: labels for all atoms
+ * of the batch in order, or NULL). */
+int jp_batch_set_canonical(jp_batch *b, int32_t on, double temps, int32_t icslop, double fcstol,
+                           const int32_t *group);
 
 /* Library version and device name (for logs). */
 const char *jp_version(void);

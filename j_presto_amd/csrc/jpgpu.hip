@@ -17,6 +17,7 @@
 #include "../../include/jpgpu.h"
 #include "jp_pair.hpp"
 #include "jp_bonded.hpp"
+#include "jp_cano.hpp"
 
 #define CK(x)                                                          \
     do {                                                               \
@@ -123,6 +124,17 @@ struct Engine {
     int nbt = 0, nat = 0, ntt = 0, n14 = 0, nbat = 0;
     int nunit = 0;             // k_step units: SHAKE groups and free atoms
     DBuf<int4> d_unit;
+    std::vector<int4> units_h;
+    // METHOD= CANO (jp_set_canonical): temperature groups and the state of the
+    // SHAKE/temperature rounds
+    bool cano = false;
+    int cano_icslop = 3, ngrp = 0;
+    double cano_fcstol = 1e-3;
+    DBuf<int> d_gid, d_gpos, d_gbeg, d_gend, d_sg0, d_sg1, d_sfirst, d_shkon, d_cdone;
+    DBuf<double> d_gdof, d_ctemps, d_fcc, d_xcb, d_kp, d_gk, d_gpart;
+    int set_canonical(int on, double temps, int icslop, double fcstol, const int *group);
+    CanoArgs cano_args();
+    int cano_step();
     // energy sums on the device: per-system atom and cluster ranges, results,
     // and a pinned host copy of them
     DBuf<int> d_abeg, d_aend, d_cbeg, d_cend;
@@ -132,9 +144,10 @@ struct Engine {
     DBuf<long long> d_partf;
     // per-system sums of a[i * ncomp + k] over segments [beg, end) into out
     template <typename T>
-    int seg_sum(const int *beg, const int *end, int ncomp, const T *a, T *part, T *out) {
-        hipLaunchKernelGGL(k_seg_part<T>, dim3(JP_SEG_NB, nsys), 256, 0, st, beg, end, ncomp, a, part);
-        hipLaunchKernelGGL(k_seg_fin<T>, nsys, 64, 0, st, ncomp, (const T *)part, out);
+    int seg_sum(const int *beg, const int *end, int ncomp, const T *a, T *part, T *out, int nseg = -1) {
+        if (nseg < 0) nseg = nsys;
+        hipLaunchKernelGGL(k_seg_part<T>, dim3(JP_SEG_NB, nseg), 256, 0, st, beg, end, ncomp, a, part);
+        hipLaunchKernelGGL(k_seg_fin<T>, nseg, 64, 0, st, ncomp, (const T *)part, out);
         CK(hipGetLastError());
         return 0;
     }
@@ -171,6 +184,9 @@ struct Engine {
         d_u64.release();
         d_bat.release(); d_bptr.release(); d_bref.release(); d_bnd.release(); d_ang.release();
         d_bndp.release(); d_angp.release(); d_eterm.release(); d_unit.release();
+        d_gid.release(); d_gpos.release(); d_gbeg.release(); d_gend.release(); d_sg0.release(); d_sg1.release();
+        d_sfirst.release(); d_shkon.release(); d_cdone.release(); d_gdof.release(); d_ctemps.release();
+        d_fcc.release(); d_xcb.release(); d_kp.release(); d_gk.release(); d_gpart.release();
         d_torc.release(); d_tor.release(); d_p14.release(); d_p14p.release(); d_torp.release();
         d_abeg.release(); d_aend.release(); d_cbeg.release(); d_cend.release(); d_rsum.release(); d_rfix.release();
         d_part.release(); d_partf.release();
@@ -587,6 +603,7 @@ int Engine::init(const jp_sys_desc *descs, int ns) {
             }
         }
         nunit = (int)units.size();
+        units_h = units;
     }
     n2 = (int)s2a.size(); n3 = (int)s3a.size();
     mass_h = mass;
@@ -1075,9 +1092,120 @@ int Engine::shake(int &ier_out) {
 
 int Engine::integrate(int prec, bool mon) {
     (void)prec;
+    if (cano) return cano_step();
     hipLaunchKernelGGL(k_step, grid1(nunit, 128), 128, 0, st, nunit, d_unit.p, d_atom_sys.p, d_sp.p, d_inum.p,
                        d_img.p, d_gslot.p, d_mass.p, d_imass.p, d_s2d.p, d_s3d.p, shake_tol, shake_maxit,
                        (n2 + n3) > 0 ? 1 : 0, d_x.p, d_v.p, mon ? d_vback.p : (double *)nullptr, d_ys.p, d_err.p);
+    CK(hipGetLastError());
+    return 0;
+}
+
+// METHOD= CANO: temperature groups (one per system, or per (system, label)
+// with labels such as md_run's ixamol2), their degrees of freedom (3 per atom
+// minus the SHAKE constraints of the groups whose first atom is theirs,
+// md.f90:114-118, 165-172) and the round buffers.
+int Engine::set_canonical(int on, double temps, int icslop, double fcstol, const int *group) {
+    drop_graphs();
+    if (!on) {
+        cano = false;
+        return 0;
+    }
+    if (!(temps > 0.0) || icslop < 1 || !(fcstol >= 0.0)) return JP_EINVAL;
+    std::vector<long long> key(natom);
+    for (int a = 0; a < natom; ++a) {
+        const int lab = group ? group[a] : 1;
+        if (lab < 1) return JP_EINVAL;
+        key[a] = ((long long)atom_sys_h[a] << 32) | lab;
+    }
+    std::vector<long long> keys(key);
+    std::sort(keys.begin(), keys.end());
+    keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+    ngrp = (int)keys.size();
+    std::vector<int> gid(natom), gcount(ngrp, 0), gbeg(ngrp), gend(ngrp), gpos(natom);
+    std::vector<double> dof(ngrp, 0.0);
+    for (int a = 0; a < natom; ++a) {
+        gid[a] = (int)(std::lower_bound(keys.begin(), keys.end(), key[a]) - keys.begin());
+        gcount[gid[a]]++;
+        dof[gid[a]] += 3.0;
+    }
+    for (int g = 0, o = 0; g < ngrp; ++g) { gbeg[g] = o; o += gcount[g]; gend[g] = o; }
+    std::vector<int> fill(gbeg);
+    for (int a = 0; a < natom; ++a) gpos[a] = fill[gid[a]]++;
+    std::vector<int> shkon(nsys, 0), sg0(nsys, ngrp), sg1(nsys, 0), sfirst(nsys);
+    for (const int4 &U : units_h) {
+        const int kind = U.w & 3;
+        if (kind < 2) continue;
+        dof[gid[U.x]] -= (double)(kind * (kind - 1) / 2);
+        shkon[atom_sys_h[U.x]] = 1;
+    }
+    for (int g = 0; g < ngrp; ++g) {
+        if (!(dof[g] > 0.0)) return JP_EINVAL;   // md.f90:124-129
+        const int s = (int)(keys[g] >> 32);
+        sg0[s] = std::min(sg0[s], g);
+        sg1[s] = std::max(sg1[s], g + 1);
+    }
+    for (int s = 0; s < nsys; ++s) sfirst[s] = sp[s].atom0;
+    std::vector<double> tset(nsys, temps);
+    AL(d_gid, natom); AL(d_gpos, natom); AL(d_gbeg, ngrp); AL(d_gend, ngrp); AL(d_gdof, ngrp);
+    AL(d_sg0, nsys); AL(d_sg1, nsys); AL(d_sfirst, nsys); AL(d_shkon, nsys); AL(d_cdone, nsys); AL(d_ctemps, nsys);
+    AL(d_fcc, 3 * (size_t)natom); AL(d_xcb, 3 * (size_t)natom); AL(d_kp, natom); AL(d_gk, ngrp);
+    AL(d_gpart, (size_t)ngrp * JP_SEG_NB);
+    auto H2D = [&](void *dst, const void *src, size_t bytes) -> int {
+        if (bytes) CK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st));
+        return 0;
+    };
+    int r = 0;
+    r |= H2D(d_gid.p, gid.data(), 4 * (size_t)natom);
+    r |= H2D(d_gpos.p, gpos.data(), 4 * (size_t)natom);
+    r |= H2D(d_gbeg.p, gbeg.data(), 4 * (size_t)ngrp);
+    r |= H2D(d_gend.p, gend.data(), 4 * (size_t)ngrp);
+    r |= H2D(d_gdof.p, dof.data(), 8 * (size_t)ngrp);
+    r |= H2D(d_sg0.p, sg0.data(), 4 * (size_t)nsys);
+    r |= H2D(d_sg1.p, sg1.data(), 4 * (size_t)nsys);
+    r |= H2D(d_sfirst.p, sfirst.data(), 4 * (size_t)nsys);
+    r |= H2D(d_shkon.p, shkon.data(), 4 * (size_t)nsys);
+    r |= H2D(d_ctemps.p, tset.data(), 8 * (size_t)nsys);
+    if (r) return r;
+    CK(hipStreamSynchronize(st));   // the host vectors end with this call
+    cano_icslop = icslop;
+    cano_fcstol = fcstol;
+    cano = true;
+    return 0;
+}
+
+CanoArgs Engine::cano_args() {
+    CanoArgs A;
+    A.natom = natom; A.nunit = nunit; A.icslop = cano_icslop; A.fcstol = cano_fcstol;
+    A.atom_sys = d_atom_sys.p; A.inum = d_inum.p; A.img = d_img.p;
+    A.gid = d_gid.p; A.gpos = d_gpos.p; A.sg0 = d_sg0.p; A.sg1 = d_sg1.p; A.sfirst = d_sfirst.p;
+    A.shk_on = d_shkon.p; A.gdof = d_gdof.p; A.temps = d_ctemps.p;
+    A.mass = d_mass.p; A.imass = d_imass.p; A.gslot = d_gslot.p;
+    A.x = d_x.p; A.v = d_v.p; A.xold = d_xold.p; A.vback = d_vback.p; A.fcc = d_fcc.p; A.xcb = d_xcb.p;
+    A.kp = d_kp.p; A.gk = d_gk.p; A.done = d_cdone.p;
+    return A;
+}
+
+// One METHOD= CANO step after the forces (jp_cano.hpp); the rounds of a
+// system that has ended are skipped on the device, the last round's check
+// always ends them.
+int Engine::cano_step() {
+    const CanoArgs A = cano_args();
+    const unsigned ga = grid1(natom, 256);
+    hipLaunchKernelGGL(k_cano_begin, ga, 256, 0, st, A, d_sp.p);
+    if (int r = seg_sum<double>(d_gbeg.p, d_gend.p, 1, d_kp.p, d_gpart.p, d_gk.p, ngrp)) return r;
+    hipLaunchKernelGGL(k_cano_drift, ga, 256, 0, st, A, d_sp.p);
+    if (n2 + n3 > 0) {
+        for (int round = 1; round <= cano_icslop; ++round) {
+            hipLaunchKernelGGL(k_cano_shake, grid1(nunit, 128), 128, 0, st, A, d_sp.p, d_unit.p, d_s2d.p, d_s3d.p,
+                               shake_tol, shake_maxit, d_err.p);
+            if (int r = seg_sum<double>(d_gbeg.p, d_gend.p, 1, d_kp.p, d_gpart.p, d_gk.p, ngrp)) return r;
+            hipLaunchKernelGGL(k_cano_check, ga, 256, 0, st, A, d_sp.p, round);
+            if (round == cano_icslop) break;
+            if (int r = seg_sum<double>(d_gbeg.p, d_gend.p, 1, d_kp.p, d_gpart.p, d_gk.p, ngrp)) return r;
+            hipLaunchKernelGGL(k_cano_update, ga, 256, 0, st, A, d_sp.p);
+        }
+    }
+    hipLaunchKernelGGL(k_cano_end, ga, 256, 0, st, A, d_sp.p, d_ys.p);
     CK(hipGetLastError());
     return 0;
 }
@@ -1570,6 +1698,17 @@ int jp_set_option(jp_ctx *ctx, int32_t option, int32_t value) {
 int jp_batch_set_option(jp_batch *b, int32_t option, int32_t value) {
     if (!b) return JP_EINVAL;
     return set_option(b->e, option, value);
+}
+
+int jp_set_canonical(jp_ctx *ctx, int32_t on, double temps, int32_t icslop, double fcstol, const int32_t *group) {
+    if (!ctx) return JP_EINVAL;
+    return ctx->e.set_canonical(on, temps, icslop, fcstol, group);
+}
+
+int jp_batch_set_canonical(jp_batch *b, int32_t on, double temps, int32_t icslop, double fcstol,
+                           const int32_t *group) {
+    if (!b) return JP_EINVAL;
+    return b->e.set_canonical(on, temps, icslop, fcstol, group);
 }
 
 int jp_batch_destroy(jp_batch *b) {

@@ -35,6 +35,10 @@ def _f64(a):
     return a.ctypes.data_as(native.f64p) if a is not None else None
 
 
+def _i32(a):
+    return a.ctypes.data_as(native.i32p) if a is not None else None
+
+
 class MdSystem:
     """One system on the GPU (a jp_ctx)."""
 
@@ -134,7 +138,8 @@ class MdSystem:
         return native.check("jp_shake", self.lib.jp_shake(self.ctx, _f64(r1), _f64(r2)))
 
     def runmd(self, nsteps: int, update: int = 10, nmon: int = 0, precision: Optional[int] = None):
-        """METHOD= MICR loop on the device state.  Returns (log, ier)."""
+        """md_run's MD loop on the device state (METHOD= MICR, or CANO after
+        set_canonical).  Returns (log, ier)."""
         prec = self.precision if precision is None else precision
         nl = nsteps // nmon if nmon > 0 else 0
         log = (EnergyLog * max(nl, 1))()
@@ -148,6 +153,19 @@ class MdSystem:
 
     def set_option(self, option: int, value: int):
         native.check("jp_set_option", self.lib.jp_set_option(self.ctx, option, int(value)))
+
+    def set_canonical(self, temps: Optional[float] = 300.0, icslop: int = 3, fcstol: float = 1e-3,
+                      tempco: bool = False):
+        """METHOD= CANO for the following runmd calls (SETTEM = temps; TEMPCO=
+        YES with tempco: one temperature per molecule kind and cluster);
+        temps=None returns to METHOD= MICR."""
+        self._grp = None
+        if tempco and temps is not None:
+            self._grp = np.ascontiguousarray(self.system.temperature_groups() + 1, np.int32)
+        on = 0 if temps is None else 1
+        native.check("jp_set_canonical", self.lib.jp_set_canonical(
+            self.ctx, on, float(temps or 0.0), int(icslop), float(fcstol),
+            None if self._grp is None else _i32(self._grp)))
 
     def get_option(self, option: int) -> int:
         v = C.c_int32()
@@ -225,6 +243,7 @@ class Ensemble:
         self.holders = [DescHolder(s, dt=dt, **desc_kw) for s in systems]
         arr = (SysDesc * len(systems))(*[h.desc for h in self.holders])
         self._arr = arr
+        self.systems = list(systems)
         self.nsys = len(systems)
         self.precision = precision
         p = C.c_void_p()
@@ -258,6 +277,18 @@ class Ensemble:
 
     def set_option(self, option: int, value: int):
         native.check("jp_batch_set_option", self.lib.jp_batch_set_option(self.b, option, int(value)))
+
+    def set_canonical(self, temps: Optional[float] = 300.0, icslop: int = 3, fcstol: float = 1e-3,
+                      tempco: bool = False):
+        """METHOD= CANO for every system of the batch (see MdSystem.set_canonical)."""
+        self._grp = None
+        if tempco and temps is not None:
+            self._grp = np.ascontiguousarray(
+                np.concatenate([s.temperature_groups() + 1 for s in self.systems]), np.int32)
+        on = 0 if temps is None else 1
+        native.check("jp_batch_set_canonical", self.lib.jp_batch_set_canonical(
+            self.b, on, float(temps or 0.0), int(icslop), float(fcstol),
+            None if self._grp is None else _i32(self._grp)))
 
     def get_option(self, option: int) -> int:
         v = C.c_int32()

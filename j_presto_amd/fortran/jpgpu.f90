@@ -147,6 +147,22 @@ module jpgpu
        integer(c_int64_t), intent(out) :: npairs, nlj, nlisted
      end function jp_count_pairs
 
+     integer(c_int) function jp_set_canonical(ctx, on, temps, icslop, fcstol, group) &
+          bind(C, name='jp_set_canonical')
+       import :: c_int, c_int32_t, c_ptr, c_double
+       type(c_ptr), value :: ctx, group
+       integer(c_int32_t), value :: on, icslop
+       real(c_double), value :: temps, fcstol
+     end function jp_set_canonical
+
+     integer(c_int) function jp_batch_set_canonical(b, on, temps, icslop, fcstol, group) &
+          bind(C, name='jp_batch_set_canonical')
+       import :: c_int, c_int32_t, c_ptr, c_double
+       type(c_ptr), value :: b, group
+       integer(c_int32_t), value :: on, icslop
+       real(c_double), value :: temps, fcstol
+     end function jp_batch_set_canonical
+
      integer(c_int) function jp_time_pair(ctx, precision, reps, seconds) bind(C, name='jp_time_pair')
        import :: c_int, c_int32_t, c_ptr, c_double
        type(c_ptr), value :: ctx

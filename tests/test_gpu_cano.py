@@ -1,0 +1,119 @@
+"""METHOD= CANO on the GPU (jp_set_canonical) against md_run's canonical
+trajectories (golden fixtures) and the oracle.
+
+Bars: FP64 trajectories within 1e-9 (A, A/fs) of md_run's after 10-20 steps
+(the group kinetic-energy sums run in a fixed tree order, md_run's serially,
+so the last bits differ); runs deterministic, graph replays equal to host
+launches and batch members equal to single runs bit for bit."""
+import numpy as np
+import pytest
+
+from j_presto_amd import engine, systems
+from oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+CANO_NAMES = ["water_1500", "chain_3k", "chain_lmd"]
+
+
+@pytest.mark.parametrize("name", CANO_NAMES)
+def test_cano_trajectory_vs_reference(golden, name):
+    s, f = golden(name)
+    print(f"case {name}", flush=True)
+    md = engine.MdSystem(s, precision=engine.FP64)
+    md.set_state(s.x, s.v)
+    md.set_canonical(300.0, tempco=str(f["cano_tempco"]) == "YES")
+    assert md.celprp() == 0
+    _, ier = md.runmd(int(f["cano_nsteps"]), update=10)
+    assert ier == 0
+    x, v = md.get_state()
+    assert np.abs(x - f["cano_x"]).max() < 1e-9
+    assert np.abs(v - f["cano_v"]).max() < 1e-9
+    md.close()
+
+
+@pytest.mark.parametrize("tempco", [False, True])
+def test_cano_vs_oracle_fp32_close(tempco):
+    """FP32/fixed-point pair forces in the canonical step: within 1e-5 A of
+    the oracle's FP64 canonical trajectory after 20 steps."""
+    s = systems.chain_in_water(48, 24.6, 31)
+    grp = s.temperature_groups() if tempco else None
+    o = Oracle(s)
+    x, v = s.x.copy(), s.v.copy()
+    assert o.celprp(x) == 0
+    for idloop in range(1, 21):
+        g, _ = o.bonded(x)
+        g, _, _ = o.nonbond(x, g)
+        assert o.cano_step(x, v, o.combine(g), 300.0, 3, 1e-3, grp)[0] == 0
+        if idloop % 10 == 0:
+            assert o.celprp(x) == 0
+    md = engine.MdSystem(s, precision=engine.FP32)
+    md.set_canonical(300.0, tempco=tempco)
+    _, ier = md.runmd(20, update=10)
+    assert ier == 0
+    xg, vg = md.get_state()
+    assert np.abs(xg - x).max() < 1e-5
+    md.close()
+
+
+def test_cano_graphs_equal_eager_and_batch():
+    """Graph replays equal host launches, and batch members equal single
+    runs, bit for bit (energy steps included)."""
+    syss = [systems.water_box(200, 24.2, 41), systems.chain_in_water(48, 24.6, 42, lmd=0.8)]
+    ens = engine.Ensemble(syss, precision=engine.FP32)
+    ens.set_canonical(310.0, tempco=True)
+    log, ier = ens.runmd(25, update=10, nmon=5)
+    assert ier == 0
+    for i, s in enumerate(syss):
+        res = []
+        for graphs in (True, False):
+            md = engine.MdSystem(s, precision=engine.FP32)
+            md.set_graphs(graphs)
+            md.set_canonical(310.0, tempco=True)
+            l1, ier1 = md.runmd(25, update=10, nmon=5)
+            assert ier1 == 0
+            res.append(md.get_state() + (l1["etot"],))
+            md.close()
+        (xa, va, ea), (xb, vb, eb) = res
+        assert np.array_equal(xa, xb) and np.array_equal(va, vb)
+        assert np.array_equal(ea, eb)
+        xe, ve = ens.get_state(i)
+        assert np.array_equal(xa, xe) and np.array_equal(va, ve), i
+        assert np.array_equal(ea, log[i]["etot"])
+    ens.close()
+
+
+def test_cano_reaches_target_temperature():
+    """Water started at 300 K and coupled to 360 K: after 100 steps the
+    temperature of 1/2 (v- + v+) is within 2 % of the target, while the same
+    run in METHOD= MICR stays near its start."""
+    s = systems.water_box(500, 24.8, 7)
+    dof = Oracle(s).group_dof()[0]
+    temps = {}
+    for cano in (True, False):
+        md = engine.MdSystem(s, precision=engine.FP64)
+        if cano:
+            md.set_canonical(360.0)
+        log, ier = md.runmd(100, update=10, nmon=100)
+        assert ier == 0
+        # caltmp.f90: T = 2 Ek / (dof R) with Ek in kcal/mol
+        temps[cano] = 2.0 * log["ekin"][-1] * 4184.0 / (dof * 8.31451)
+        md.close()
+    assert abs(temps[True] - 360.0) < 0.02 * 360.0
+    assert abs(temps[False] - 360.0) > 0.05 * 360.0
+
+
+def test_cano_off_returns_to_micr():
+    s = systems.water_box(200, 24.2, 43)
+    a = engine.MdSystem(s, precision=engine.FP64)
+    a.set_canonical(300.0)
+    a.set_canonical(None)
+    b = engine.MdSystem(s, precision=engine.FP64)
+    for md in (a, b):
+        _, ier = md.runmd(12, update=10)
+        assert ier == 0
+    assert all(np.array_equal(p, q) for p, q in zip(a.get_state(), b.get_state()))
+    with pytest.raises(Exception):
+        a.set_canonical(-1.0)
+    a.close()
+    b.close()
