@@ -131,7 +131,7 @@ struct Engine {
     int cano_icslop = 3, ngrp = 0;
     double cano_fcstol = 1e-3;
     DBuf<int> d_gid, d_gpos, d_gbeg, d_gend, d_sg0, d_sg1, d_sfirst, d_shkon, d_cdone;
-    DBuf<double> d_gdof, d_ctemps, d_fcc, d_xcb, d_kp, d_gk, d_gpart;
+    DBuf<double> d_gdof, d_ctemps, d_fcc, d_xcb, d_kp, d_gpart;
     int set_canonical(int on, double temps, int icslop, double fcstol, const int *group);
     CanoArgs cano_args();
     int cano_step();
@@ -186,7 +186,7 @@ struct Engine {
         d_bndp.release(); d_angp.release(); d_eterm.release(); d_unit.release();
         d_gid.release(); d_gpos.release(); d_gbeg.release(); d_gend.release(); d_sg0.release(); d_sg1.release();
         d_sfirst.release(); d_shkon.release(); d_cdone.release(); d_gdof.release(); d_ctemps.release();
-        d_fcc.release(); d_xcb.release(); d_kp.release(); d_gk.release(); d_gpart.release();
+        d_fcc.release(); d_xcb.release(); d_kp.release(); d_gpart.release();
         d_torc.release(); d_tor.release(); d_p14.release(); d_p14p.release(); d_torp.release();
         d_abeg.release(); d_aend.release(); d_cbeg.release(); d_cend.release(); d_rsum.release(); d_rfix.release();
         d_part.release(); d_partf.release();
@@ -1145,10 +1145,17 @@ int Engine::set_canonical(int on, double temps, int icslop, double fcstol, const
         sg1[s] = std::max(sg1[s], g + 1);
     }
     for (int s = 0; s < nsys; ++s) sfirst[s] = sp[s].atom0;
+    // every block of the canonical kernels (256 atoms, 128 units) reads the
+    // temperatures of the groups of its systems into JP_CANO_MAXG slots
+    auto span_ok = [&](int a0, int a1) { return sg1[atom_sys_h[a1]] - sg0[atom_sys_h[a0]] <= JP_CANO_MAXG; };
+    for (int a0 = 0; a0 < natom; a0 += 256)
+        if (!span_ok(a0, std::min(a0 + 256, natom) - 1)) return JP_EINVAL;
+    for (int u0 = 0; u0 < nunit; u0 += 128)
+        if (!span_ok(units_h[u0].x, units_h[std::min(u0 + 128, nunit) - 1].x)) return JP_EINVAL;
     std::vector<double> tset(nsys, temps);
     AL(d_gid, natom); AL(d_gpos, natom); AL(d_gbeg, ngrp); AL(d_gend, ngrp); AL(d_gdof, ngrp);
     AL(d_sg0, nsys); AL(d_sg1, nsys); AL(d_sfirst, nsys); AL(d_shkon, nsys); AL(d_cdone, nsys); AL(d_ctemps, nsys);
-    AL(d_fcc, 3 * (size_t)natom); AL(d_xcb, 3 * (size_t)natom); AL(d_kp, natom); AL(d_gk, ngrp);
+    AL(d_fcc, 3 * (size_t)natom); AL(d_xcb, 3 * (size_t)natom); AL(d_kp, natom);
     AL(d_gpart, (size_t)ngrp * JP_SEG_NB);
     auto H2D = [&](void *dst, const void *src, size_t bytes) -> int {
         if (bytes) CK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st));
@@ -1181,31 +1188,32 @@ CanoArgs Engine::cano_args() {
     A.shk_on = d_shkon.p; A.gdof = d_gdof.p; A.temps = d_ctemps.p;
     A.mass = d_mass.p; A.imass = d_imass.p; A.gslot = d_gslot.p;
     A.x = d_x.p; A.v = d_v.p; A.xold = d_xold.p; A.vback = d_vback.p; A.fcc = d_fcc.p; A.xcb = d_xcb.p;
-    A.kp = d_kp.p; A.gk = d_gk.p; A.done = d_cdone.p;
+    A.kp = d_kp.p; A.part = d_gpart.p; A.done = d_cdone.p;
     return A;
 }
 
-// One METHOD= CANO step after the forces (jp_cano.hpp); the rounds of a
-// system that has ended are skipped on the device, the last round's check
-// always ends them.
+// One METHOD= CANO step after the forces (jp_cano.hpp): 3 + 3 * icslop - 1
+// launches; the rounds of a system that has ended are skipped on the device,
+// the last round's check always ends them.
 int Engine::cano_step() {
     const CanoArgs A = cano_args();
     const unsigned ga = grid1(natom, 256);
+    auto part = [&]() {
+        hipLaunchKernelGGL(k_seg_part<double>, dim3(JP_SEG_NB, ngrp), 256, 0, st, d_gbeg.p, d_gend.p, 1,
+                           (const double *)d_kp.p, d_gpart.p);
+    };
     hipLaunchKernelGGL(k_cano_begin, ga, 256, 0, st, A, d_sp.p);
-    if (int r = seg_sum<double>(d_gbeg.p, d_gend.p, 1, d_kp.p, d_gpart.p, d_gk.p, ngrp)) return r;
-    hipLaunchKernelGGL(k_cano_drift, ga, 256, 0, st, A, d_sp.p);
+    part();
+    hipLaunchKernelGGL(k_cano_drift, ga, 256, 0, st, A, d_sp.p, d_ys.p);
     if (n2 + n3 > 0) {
         for (int round = 1; round <= cano_icslop; ++round) {
             hipLaunchKernelGGL(k_cano_shake, grid1(nunit, 128), 128, 0, st, A, d_sp.p, d_unit.p, d_s2d.p, d_s3d.p,
-                               shake_tol, shake_maxit, d_err.p);
-            if (int r = seg_sum<double>(d_gbeg.p, d_gend.p, 1, d_kp.p, d_gpart.p, d_gk.p, ngrp)) return r;
+                               shake_tol, shake_maxit, round > 1 ? 1 : 0, d_ys.p, d_err.p);
+            part();
             hipLaunchKernelGGL(k_cano_check, ga, 256, 0, st, A, d_sp.p, round);
-            if (round == cano_icslop) break;
-            if (int r = seg_sum<double>(d_gbeg.p, d_gend.p, 1, d_kp.p, d_gpart.p, d_gk.p, ngrp)) return r;
-            hipLaunchKernelGGL(k_cano_update, ga, 256, 0, st, A, d_sp.p);
+            if (round < cano_icslop) part();
         }
     }
-    hipLaunchKernelGGL(k_cano_end, ga, 256, 0, st, A, d_sp.p, d_ys.p);
     CK(hipGetLastError());
     return 0;
 }
