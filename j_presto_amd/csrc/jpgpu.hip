@@ -128,10 +128,11 @@ struct Engine {
     // METHOD= CANO (jp_set_canonical): temperature groups and the state of the
     // SHAKE/temperature rounds
     bool cano = false;
-    int cano_icslop = 3, ngrp = 0;
+    int cano_icslop = 3, ngrp = 0, nab = 0, nub = 0;
     double cano_fcstol = 1e-3;
-    DBuf<int> d_gid, d_gpos, d_gbeg, d_gend, d_sg0, d_sg1, d_sfirst, d_shkon, d_cdone;
-    DBuf<double> d_gdof, d_ctemps, d_fcc, d_xcb, d_kp, d_gpart;
+    DBuf<int> d_gid, d_gsys, d_sg0, d_sg1, d_ab0, d_ab1, d_ub0, d_ub1, d_sfirst, d_shkon, d_cdone;
+    DBuf<int2> d_ablk, d_ublk;
+    DBuf<double> d_gdof, d_ctemps, d_fcc, d_xcb, d_pa, d_pu;
     int set_canonical(int on, double temps, int icslop, double fcstol, const int *group);
     CanoArgs cano_args();
     int cano_step();
@@ -184,9 +185,9 @@ struct Engine {
         d_u64.release();
         d_bat.release(); d_bptr.release(); d_bref.release(); d_bnd.release(); d_ang.release();
         d_bndp.release(); d_angp.release(); d_eterm.release(); d_unit.release();
-        d_gid.release(); d_gpos.release(); d_gbeg.release(); d_gend.release(); d_sg0.release(); d_sg1.release();
-        d_sfirst.release(); d_shkon.release(); d_cdone.release(); d_gdof.release(); d_ctemps.release();
-        d_fcc.release(); d_xcb.release(); d_kp.release(); d_gpart.release();
+        d_gid.release(); d_gsys.release(); d_sg0.release(); d_sg1.release(); d_ab0.release(); d_ab1.release();
+        d_ub0.release(); d_ub1.release(); d_ablk.release(); d_ublk.release(); d_sfirst.release(); d_shkon.release(); d_cdone.release();
+        d_gdof.release(); d_ctemps.release(); d_fcc.release(); d_xcb.release(); d_pa.release(); d_pu.release();
         d_torc.release(); d_tor.release(); d_p14.release(); d_p14p.release(); d_torp.release();
         d_abeg.release(); d_aend.release(); d_cbeg.release(); d_cend.release(); d_rsum.release(); d_rfix.release();
         d_part.release(); d_partf.release();
@@ -1106,10 +1107,8 @@ int Engine::integrate(int prec, bool mon) {
 // md.f90:114-118, 165-172) and the round buffers.
 int Engine::set_canonical(int on, double temps, int icslop, double fcstol, const int *group) {
     drop_graphs();
-    if (!on) {
-        cano = false;
-        return 0;
-    }
+    cano = false;   // METHOD= MICR unless the new set-up completes
+    if (!on) return 0;
     if (!(temps > 0.0) || icslop < 1 || !(fcstol >= 0.0)) return JP_EINVAL;
     std::vector<long long> key(natom);
     for (int a = 0; a < natom; ++a) {
@@ -1121,17 +1120,31 @@ int Engine::set_canonical(int on, double temps, int icslop, double fcstol, const
     std::sort(keys.begin(), keys.end());
     keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
     ngrp = (int)keys.size();
-    std::vector<int> gid(natom), gcount(ngrp, 0), gbeg(ngrp), gend(ngrp), gpos(natom);
+    std::vector<int> gid(natom), gsys(ngrp);
     std::vector<double> dof(ngrp, 0.0);
     for (int a = 0; a < natom; ++a) {
         gid[a] = (int)(std::lower_bound(keys.begin(), keys.end(), key[a]) - keys.begin());
-        gcount[gid[a]]++;
         dof[gid[a]] += 3.0;
     }
-    for (int g = 0, o = 0; g < ngrp; ++g) { gbeg[g] = o; o += gcount[g]; gend[g] = o; }
-    std::vector<int> fill(gbeg);
-    for (int a = 0; a < natom; ++a) gpos[a] = fill[gid[a]]++;
+    for (int g = 0; g < ngrp; ++g) gsys[g] = (int)(keys[g] >> 32);
     std::vector<int> shkon(nsys, 0), sg0(nsys, ngrp), sg1(nsys, 0), sfirst(nsys);
+    // blocks of the canonical kernels: up to 256 atoms or 128 units of one
+    // system, counted from the system's first (units follow the atom order)
+    std::vector<int2> ablk, ublk;
+    std::vector<int> ab0(nsys), ab1(nsys), ub0(nsys), ub1(nsys), ubeg(nsys + 1, nunit);
+    for (int u = nunit - 1; u >= 0; --u) ubeg[atom_sys_h[units_h[u].x]] = u;
+    for (int s = nsys - 1; s >= 0; --s) ubeg[s] = std::min(ubeg[s], ubeg[s + 1]);
+    for (int s = 0; s < nsys; ++s) {
+        ab0[s] = (int)ablk.size();
+        for (int a = sp[s].atom0; a < sp[s].atom0 + hs[s].n; a += 256)
+            ablk.push_back(make_int2(a, std::min(a + 256, sp[s].atom0 + hs[s].n)));
+        ab1[s] = (int)ablk.size();
+        ub0[s] = (int)ublk.size();
+        for (int u = ubeg[s]; u < ubeg[s + 1]; u += 128) ublk.push_back(make_int2(u, std::min(u + 128, ubeg[s + 1])));
+        ub1[s] = (int)ublk.size();
+    }
+    nab = (int)ablk.size();
+    nub = (int)ublk.size();
     for (const int4 &U : units_h) {
         const int kind = U.w & 3;
         if (kind < 2) continue;
@@ -1145,27 +1158,28 @@ int Engine::set_canonical(int on, double temps, int icslop, double fcstol, const
         sg1[s] = std::max(sg1[s], g + 1);
     }
     for (int s = 0; s < nsys; ++s) sfirst[s] = sp[s].atom0;
-    // every block of the canonical kernels (256 atoms, 128 units) reads the
-    // temperatures of the groups of its systems into JP_CANO_MAXG slots
-    auto span_ok = [&](int a0, int a1) { return sg1[atom_sys_h[a1]] - sg0[atom_sys_h[a0]] <= JP_CANO_MAXG; };
-    for (int a0 = 0; a0 < natom; a0 += 256)
-        if (!span_ok(a0, std::min(a0 + 256, natom) - 1)) return JP_EINVAL;
-    for (int u0 = 0; u0 < nunit; u0 += 128)
-        if (!span_ok(units_h[u0].x, units_h[std::min(u0 + 128, nunit) - 1].x)) return JP_EINVAL;
+    // a block reads its system's group temperatures into JP_CANO_MAXG slots
+    for (int s = 0; s < nsys; ++s)
+        if (sg1[s] - sg0[s] > JP_CANO_MAXG) return JP_EINVAL;
     std::vector<double> tset(nsys, temps);
-    AL(d_gid, natom); AL(d_gpos, natom); AL(d_gbeg, ngrp); AL(d_gend, ngrp); AL(d_gdof, ngrp);
-    AL(d_sg0, nsys); AL(d_sg1, nsys); AL(d_sfirst, nsys); AL(d_shkon, nsys); AL(d_cdone, nsys); AL(d_ctemps, nsys);
-    AL(d_fcc, 3 * (size_t)natom); AL(d_xcb, 3 * (size_t)natom); AL(d_kp, natom);
-    AL(d_gpart, (size_t)ngrp * JP_SEG_NB);
+    AL(d_gid, natom); AL(d_gsys, ngrp); AL(d_gdof, ngrp);
+    AL(d_sg0, nsys); AL(d_sg1, nsys); AL(d_ab0, nsys); AL(d_ab1, nsys); AL(d_ub0, nsys); AL(d_ub1, nsys);
+    AL(d_sfirst, nsys); AL(d_shkon, nsys); AL(d_cdone, nsys); AL(d_ctemps, nsys);
+    AL(d_fcc, 3 * (size_t)natom); AL(d_xcb, 3 * (size_t)natom);
+    AL(d_pa, (size_t)ngrp * nab); AL(d_pu, (size_t)ngrp * nub); AL(d_ablk, nab); AL(d_ublk, nub);
     auto H2D = [&](void *dst, const void *src, size_t bytes) -> int {
         if (bytes) CK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st));
         return 0;
     };
     int r = 0;
     r |= H2D(d_gid.p, gid.data(), 4 * (size_t)natom);
-    r |= H2D(d_gpos.p, gpos.data(), 4 * (size_t)natom);
-    r |= H2D(d_gbeg.p, gbeg.data(), 4 * (size_t)ngrp);
-    r |= H2D(d_gend.p, gend.data(), 4 * (size_t)ngrp);
+    r |= H2D(d_gsys.p, gsys.data(), 4 * (size_t)ngrp);
+    r |= H2D(d_ab0.p, ab0.data(), 4 * (size_t)nsys);
+    r |= H2D(d_ab1.p, ab1.data(), 4 * (size_t)nsys);
+    r |= H2D(d_ub0.p, ub0.data(), 4 * (size_t)nsys);
+    r |= H2D(d_ub1.p, ub1.data(), 4 * (size_t)nsys);
+    r |= H2D(d_ablk.p, ablk.data(), 8 * (size_t)nab);
+    r |= H2D(d_ublk.p, ublk.data(), 8 * (size_t)nub);
     r |= H2D(d_gdof.p, dof.data(), 8 * (size_t)ngrp);
     r |= H2D(d_sg0.p, sg0.data(), 4 * (size_t)nsys);
     r |= H2D(d_sg1.p, sg1.data(), 4 * (size_t)nsys);
@@ -1182,36 +1196,31 @@ int Engine::set_canonical(int on, double temps, int icslop, double fcstol, const
 
 CanoArgs Engine::cano_args() {
     CanoArgs A;
-    A.natom = natom; A.nunit = nunit; A.icslop = cano_icslop; A.fcstol = cano_fcstol;
+    A.natom = natom; A.nunit = nunit; A.icslop = cano_icslop; A.nab = nab; A.nub = nub; A.fcstol = cano_fcstol;
     A.atom_sys = d_atom_sys.p; A.inum = d_inum.p; A.img = d_img.p;
-    A.gid = d_gid.p; A.gpos = d_gpos.p; A.sg0 = d_sg0.p; A.sg1 = d_sg1.p; A.sfirst = d_sfirst.p;
+    A.gid = d_gid.p; A.gsys = d_gsys.p; A.sg0 = d_sg0.p; A.sg1 = d_sg1.p;
+    A.ab0 = d_ab0.p; A.ab1 = d_ab1.p; A.ub0 = d_ub0.p; A.ub1 = d_ub1.p; A.sfirst = d_sfirst.p;
+    A.ablk = d_ablk.p; A.ublk = d_ublk.p;
     A.shk_on = d_shkon.p; A.gdof = d_gdof.p; A.temps = d_ctemps.p;
     A.mass = d_mass.p; A.imass = d_imass.p; A.gslot = d_gslot.p;
     A.x = d_x.p; A.v = d_v.p; A.xold = d_xold.p; A.vback = d_vback.p; A.fcc = d_fcc.p; A.xcb = d_xcb.p;
-    A.kp = d_kp.p; A.part = d_gpart.p; A.done = d_cdone.p;
+    A.pa = d_pa.p; A.pu = d_pu.p; A.done = d_cdone.p;
     return A;
 }
 
-// One METHOD= CANO step after the forces (jp_cano.hpp): 3 + 3 * icslop - 1
+// One METHOD= CANO step after the forces (jp_cano.hpp): 2 + 2 * icslop
 // launches; the rounds of a system that has ended are skipped on the device,
 // the last round's check always ends them.
 int Engine::cano_step() {
     const CanoArgs A = cano_args();
-    const unsigned ga = grid1(natom, 256);
-    auto part = [&]() {
-        hipLaunchKernelGGL(k_seg_part<double>, dim3(JP_SEG_NB, ngrp), 256, 0, st, d_gbeg.p, d_gend.p, 1,
-                           (const double *)d_kp.p, d_gpart.p);
-    };
+    const unsigned ga = (unsigned)nab;
     hipLaunchKernelGGL(k_cano_begin, ga, 256, 0, st, A, d_sp.p);
-    part();
     hipLaunchKernelGGL(k_cano_drift, ga, 256, 0, st, A, d_sp.p, d_ys.p);
     if (n2 + n3 > 0) {
         for (int round = 1; round <= cano_icslop; ++round) {
-            hipLaunchKernelGGL(k_cano_shake, grid1(nunit, 128), 128, 0, st, A, d_sp.p, d_unit.p, d_s2d.p, d_s3d.p,
+            hipLaunchKernelGGL(k_cano_shake, (unsigned)nub, 128, 0, st, A, d_sp.p, d_unit.p, d_s2d.p, d_s3d.p,
                                shake_tol, shake_maxit, round > 1 ? 1 : 0, d_ys.p, d_err.p);
-            part();
             hipLaunchKernelGGL(k_cano_check, ga, 256, 0, st, A, d_sp.p, round);
-            if (round < cano_icslop) part();
         }
     }
     CK(hipGetLastError());
