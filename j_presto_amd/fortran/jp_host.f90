@@ -2,9 +2,10 @@
 ! what md_run does through its own routines:
 !   celprp (jp_build_list), dnergy (grad zeroed, then jp_bonded and
 !   jp_nonbond accumulate; FP64 and FP32/fixed-point pair arithmetic), and the
-!   METHOD= MICR MD loop (jp_run_nve), and writes every result to an output file.
+!   MD loop (jp_run_nve; METHOD= MICR, or METHOD= CANO at SETTEM = settem K with
+!   TEMPCO= NO through jp_set_canonical), and writes every result to an output file.
 !
-! usage: jp_host <system file> <output file> <nsteps> <update> <nmon> <precision 0|1>
+! usage: jp_host <system file> <output file> <nsteps> <update> <nmon> <precision 0|1> [settem]
 !        jp_host --layout   (byte sizes and offsets of the bind(C) types; no GPU use)
 !
 ! System file (Fortran unformatted sequential, one array per record):
@@ -18,7 +19,8 @@
 !   tors_atoms(4,ntors) tors_fc tors_rot tors_phase(ntors) tors_div(ntors)
 !   p14_atoms(2,n14) p14_vdw(n14) p14_elec(n14)
 ! Output file (same record format):
-!   ier(8): create, build_list, nonbond FP64, nonbond FP32, run_nve, get_state, bonded (x2)
+!   ier(9): create, build_list, nonbond FP64, nonbond FP32, run_nve, get_state, bonded (x2),
+!           set_canonical
 !   grad64(3,N,nfrg) eelec64(nfrg) evdw64(nfrg) grad32(3,N,nfrg) eelec32(nfrg) evdw32(nfrg)
 !   eterm(6,nfrg) = enevcc(2:7,:) (bond, angle, torsion, improper, 1-4 vdW, 1-4 elec)
 !   cord(3,N) vel(3,N) after the run, nlog, then per record: step etot epot ekin
@@ -30,7 +32,8 @@ program jp_host
   integer(c_int32_t) :: hdr(14), nsteps, update, nmon, prec
   real(c_double) :: rhdr(11)
   integer :: n, ntyp, nfrag, nfrg, nexcl, nexcess, nshake, nbond, nangle, ntors, n14, nlog, k, u
-  integer(c_int) :: ier(8)
+  integer(c_int) :: ier(9)
+  real(c_double) :: settem
   real(c_double), allocatable, target :: cord(:,:), vel(:,:), mass(:), charge(:), rmin2(:), eps(:)
   real(c_double), allocatable, target :: sdist(:,:), grad64(:,:,:), grad32(:,:,:)
   real(c_double), target :: eel64(3), evd64(3), ehy64(3), eel32(3), evd32(3), ehy32(3), eterm(6,3)
@@ -66,6 +69,10 @@ program jp_host
   call get_command_argument(4, arg); read(arg, *) update
   call get_command_argument(5, arg); read(arg, *) nmon
   call get_command_argument(6, arg); read(arg, *) prec
+  settem = 0.0d0
+  if (command_argument_count() >= 7) then
+     call get_command_argument(7, arg); read(arg, *) settem
+  end if
 
   open(newunit=u, file=trim(fin), form='unformatted', access='sequential', status='old', action='read')
   read(u) hdr
@@ -147,13 +154,14 @@ program jp_host
   ier(8) = jp_bonded(ctx, c_null_ptr, c_loc(grad32), c_null_ptr)
   ier(4) = jp_nonbond(ctx, c_null_ptr, c_loc(grad32), c_loc(eel32), c_loc(evd32), c_loc(ehy32), JP_PREC_FP32_FIXED)
 
-  ! METHOD= MICR loop
+  ! the MD loop: METHOD= MICR, or METHOD= CANO (md.f90:546-552, verlethe)
+  if (settem > 0.0d0) ier(9) = jp_set_canonical(ctx, 1_c_int32_t, settem, 3_c_int32_t, 1.0d-3, c_null_ptr)
   nlog = 0
   if (nmon > 0) nlog = nsteps / nmon
   allocate(elog(max(nlog,1)))
   ier(5) = jp_run_nve(ctx, nsteps, update, nmon, prec, c_loc(elog))
   ier(6) = jp_get_state(ctx, c_loc(cord), c_loc(vel))
-  write(*,'(a,8(1x,i0))') 'jp_host: ier =', ier
+  write(*,'(a,9(1x,i0))') 'jp_host: ier =', ier
   if (nlog > 0) write(*,'(a,i0,a,f16.6)') 'jp_host: step ', elog(nlog)%step, ' etot ', elog(nlog)%etot
 
   open(newunit=u, file=trim(fout), form='unformatted', access='sequential', status='replace', action='write')

@@ -112,3 +112,32 @@ def test_fortran_host_run(golden, tmp_path, name):
     assert np.array_equal(x, o["x"]) and np.array_equal(v, o["v"])
     assert [L[0] for L in o["log"]] == list(log["step"])
     assert np.array_equal(np.array([L[1] for L in o["log"]]), log["etot"])
+
+
+@pytest.mark.gpu
+def test_fortran_host_canonical(golden, tmp_path):
+    """METHOD= CANO through jp_set_canonical from Fortran: md_run's canonical
+    trajectory within 1e-9, and the same bits as the Python-driven run."""
+    s, f = golden("water_1500")
+    print(f"case water_1500 cano n={s.n}", flush=True)
+    sysf, outf = str(tmp_path / "sys.bin"), str(tmp_path / "out.bin")
+    refio.write_jpsys(sysf, s)
+    nsteps = int(f["cano_nsteps"])
+    r = subprocess.run([_host(), sysf, outf, str(nsteps), "10", "0", "0", "300.0"],
+                       capture_output=True, text=True, timeout=240)
+    print(r.stdout, flush=True)
+    assert r.returncode == 0, r.stderr
+    o = refio.read_jphost(outf, s.n, 1)
+    assert len(o["ier"]) == 9 and np.all(o["ier"] == 0), o["ier"]
+    assert np.abs(o["x"] - f["cano_x"]).max() < 1e-9
+    assert np.abs(o["v"] - f["cano_v"]).max() < 1e-9
+    md = engine.MdSystem(s, precision=engine.FP64)
+    assert md.celprp() == 0
+    md.cmmdo()
+    md.cmmdo(precision=engine.FP32)
+    md.set_canonical(300.0)
+    _, ier = md.runmd(nsteps, update=10)
+    assert ier == 0
+    x, v = md.get_state()
+    md.close()
+    assert np.array_equal(x, o["x"]) and np.array_equal(v, o["v"])
