@@ -6,6 +6,15 @@
 !   TEMPCO= NO through jp_set_canonical), and writes every result to an output file.
 !
 ! usage: jp_host <system file> <output file> <nsteps> <update> <nmon> <precision 0|1> [settem]
+!        jp_host --mdinp <md.inp> <system file> [precision 0|1]
+!                 md_run's own control file: the EXE> MD keywords LOOPLI, UPDATE,
+!                 TIMEST, METHOD (MICR, CANO; required, since md_run's defaults
+!                 CANO with TEMPCO= YES need molecule kinds the system file does
+!                 not carry), SETTEM, TEMPCO (NO), OUTLOG, RESTAR, NAMERI,
+!                 NAMERO (inpmd.f90); x and v from the NAMERI
+!                 restart, the final state written to NAMERO in md_run's
+!                 restart format (outrst.f90:56-61); the topology from the
+!                 system file
 !        jp_host --layout   (byte sizes and offsets of the bind(C) types; no GPU use)
 !
 ! System file (Fortran unformatted sequential, one array per record):
@@ -33,7 +42,12 @@ program jp_host
   real(c_double) :: rhdr(11)
   integer :: n, ntyp, nfrag, nfrg, nexcl, nexcess, nshake, nbond, nangle, ntors, n14, nlog, k, u
   integer(c_int) :: ier(9)
-  real(c_double) :: settem
+  real(c_double) :: settem, timest
+  character(len=1024) :: mdinp, rstin, rstout
+  logical :: mdmode, restar, tempco
+  character(len=80) :: title
+  integer(c_int32_t) :: rn, rnvar, rloop
+  real(c_double) :: rtime, ret, rek, rep, rlam, rlamv
   real(c_double), allocatable, target :: cord(:,:), vel(:,:), mass(:), charge(:), rmin2(:), eps(:)
   real(c_double), allocatable, target :: sdist(:,:), grad64(:,:,:), grad32(:,:,:)
   real(c_double), target :: eel64(3), evd64(3), ehy64(3), eel32(3), evd32(3), ehy32(3), eterm(6,3)
@@ -59,19 +73,34 @@ program jp_host
      write(*,'(a,i0)') 'sizeof jp_energy_log ', c_sizeof(rec1)
      stop
   end if
-  if (command_argument_count() < 6) then
-     write(*,'(a)') 'usage: jp_host <system file> <output file> <nsteps> <update> <nmon> <precision 0|1>'
-     stop 2
-  end if
-  call get_command_argument(1, fin)
-  call get_command_argument(2, fout)
-  call get_command_argument(3, arg); read(arg, *) nsteps
-  call get_command_argument(4, arg); read(arg, *) update
-  call get_command_argument(5, arg); read(arg, *) nmon
-  call get_command_argument(6, arg); read(arg, *) prec
+  mdmode = trim(arg) == '--mdinp'
   settem = 0.0d0
-  if (command_argument_count() >= 7) then
-     call get_command_argument(7, arg); read(arg, *) settem
+  if (mdmode) then
+     if (command_argument_count() < 3) then
+        write(*,'(a)') 'usage: jp_host --mdinp <md.inp> <system file> [precision 0|1]'
+        stop 2
+     end if
+     call get_command_argument(2, mdinp)
+     call get_command_argument(3, fin)
+     prec = 0
+     if (command_argument_count() >= 4) then
+        call get_command_argument(4, arg); read(arg, *) prec
+     end if
+     call read_mdinp(trim(mdinp))
+  else
+     if (command_argument_count() < 6) then
+        write(*,'(a)') 'usage: jp_host <system file> <output file> <nsteps> <update> <nmon> <precision 0|1>'
+        stop 2
+     end if
+     call get_command_argument(1, fin)
+     call get_command_argument(2, fout)
+     call get_command_argument(3, arg); read(arg, *) nsteps
+     call get_command_argument(4, arg); read(arg, *) update
+     call get_command_argument(5, arg); read(arg, *) nmon
+     call get_command_argument(6, arg); read(arg, *) prec
+     if (command_argument_count() >= 7) then
+        call get_command_argument(7, arg); read(arg, *) settem
+     end if
   end if
 
   open(newunit=u, file=trim(fin), form='unformatted', access='sequential', status='old', action='read')
@@ -135,6 +164,11 @@ program jp_host
   desc%tors_phase = c_loc(tph); desc%tors_div = c_loc(tdiv)
   desc%n14 = n14; desc%p14_atoms = c_loc(patm); desc%p14_vdw = c_loc(pv); desc%p14_elec = c_loc(pe)
 
+  if (mdmode) then
+     desc%dt = timest
+     if (restar) call read_restart(trim(rstin))
+  end if
+
   write(*,'(2a)') 'jp_host: ', jp_version()
   ier = 0
   ier(1) = jp_ctx_create(desc, ctx)
@@ -144,6 +178,10 @@ program jp_host
   end if
   ier(1) = jp_set_state(ctx, c_null_ptr, c_loc(vel))
   ier(2) = jp_build_list(ctx, c_null_ptr)                 ! celprp
+  if (mdmode) then
+     call run_mdinp()
+     stop
+  end if
 
   ! dnergy: grad zeroed, then bonds and angles, then cmmdo (dnergy.f90:45-55, dnergv.f90)
   allocate(grad64(3,n,nfrg), grad32(3,n,nfrg))
@@ -183,6 +221,126 @@ program jp_host
   ier(1) = jp_ctx_destroy(ctx)
 
 contains
+
+  ! EXE> MD section of md_run's control file (inpmd.f90 keywords and
+  ! defaults; METHOD must be given, TEMPCO= YES is refused)
+  subroutine read_mdinp(path)
+    character(len=*), intent(in) :: path
+    character(len=256) :: line, rest, key, val
+    integer :: uu, ios, p, q
+    logical :: inmd, have_method
+    nsteps = 0; update = 10; nmon = 0; timest = 2.0d0; settem = -300.0d0   ! negative: METHOD= MICR
+    restar = .false.; tempco = .true.; rstin = ' '; rstout = ' '
+    have_method = .false.
+    inmd = .false.
+    open(newunit=uu, file=path, status='old', action='read')
+    do
+       read(uu, '(a)', iostat=ios) line
+       if (ios /= 0) exit
+       if (index(line, 'EXE>') > 0) then
+          inmd = index(line, 'EXE> MD') > 0
+          cycle
+       end if
+       if (.not. inmd) cycle
+       if (index(line, 'QUIT') > 0) then
+          inmd = .false.
+          cycle
+       end if
+       rest = line
+       do
+          p = index(rest, '=')
+          if (p < 2) exit
+          key = adjustl(rest(max(1, p - 6):p - 1))
+          rest = adjustl(rest(p + 1:))
+          q = index(rest, ' ')
+          if (q == 0) q = len_trim(rest) + 1
+          val = rest(1:q - 1)
+          rest = rest(q:)
+          select case (trim(key))
+          case ('LOOPLI'); read(val, *) nsteps
+          case ('UPDATE'); read(val, *) update
+          case ('TIMEST'); read(val, *) timest
+          case ('OUTLOG'); read(val, *) nmon
+          case ('SETTEM'); read(val, *) rep; settem = sign(rep, settem)
+          case ('RESTAR'); restar = val(1:3) == 'YES'
+          case ('NAMERI'); rstin = val
+          case ('NAMERO'); rstout = val
+          case ('TEMPCO'); tempco = val(1:3) == 'YES'
+          case ('METHOD')
+             have_method = .true.
+             if (val(1:4) == 'CANO') then
+                settem = abs(settem)
+             else if (val(1:4) == 'MICR') then
+                settem = -abs(settem)
+             else
+                write(*,'(2a)') 'jp_host: METHOD not supported: ', trim(val)
+                stop 3
+             end if
+          end select
+       end do
+    end do
+    close(uu)
+    if (.not. have_method) then
+       write(*,'(a)') 'jp_host: METHOD= MICR or CANO must be given in EXE> MD'
+       stop 3
+    end if
+    if (nmon <= 0 .or. nmon > nsteps) nmon = 0
+    if (settem > 0.0d0 .and. tempco) then
+       write(*,'(a)') 'jp_host: TEMPCO= YES needs the molecule kinds (use jp_set_canonical with groups)'
+       stop 3
+    end if
+  end subroutine read_mdinp
+
+  ! md_run restart file (outrst.f90:56-61): title, natom nvar, loop time
+  ! etot ekin epot, cord(3,natom), vel(3,nvar), lambda lambda_v
+  subroutine read_restart(path)
+    character(len=*), intent(in) :: path
+    integer :: uu
+    open(newunit=uu, file=path, form='unformatted', access='sequential', status='old', action='read')
+    read(uu) title
+    read(uu) rn, rnvar
+    if (rn /= n .or. rnvar /= n) then
+       write(*,'(a,2(1x,i0))') 'jp_host: restart atoms / moving atoms differ from the system:', rn, rnvar
+       stop 3
+    end if
+    read(uu) rloop, rtime, ret, rek, rep
+    read(uu) cord
+    read(uu) vel
+    read(uu) rlam, rlamv
+    close(uu)
+  end subroutine read_restart
+
+  subroutine run_mdinp()
+    integer :: uu, nl
+    if (.not. restar) then
+       rloop = 0; rtime = 0.0d0
+    end if
+    if (settem > 0.0d0) ier(9) = jp_set_canonical(ctx, 1_c_int32_t, settem, 3_c_int32_t, 1.0d-3, c_null_ptr)
+    nl = 0
+    if (nmon > 0) nl = nsteps / nmon
+    allocate(elog(max(nl, 1)))
+    ier(5) = jp_run_nve(ctx, nsteps, update, nmon, prec, c_loc(elog))
+    ier(6) = jp_get_state(ctx, c_loc(cord), c_loc(vel))
+    write(*,'(a,9(1x,i0))') 'jp_host: ier =', ier
+    ret = 0.0d0; rek = 0.0d0; rep = 0.0d0
+    if (nl > 0) then
+       ret = elog(nl)%etot; rek = elog(nl)%ekin; rep = elog(nl)%epot
+    end if
+    if (len_trim(rstout) > 0) then
+       title = 'jp_host'
+       open(newunit=uu, file=trim(rstout), form='unformatted', access='sequential', status='replace', &
+            action='write')
+       write(uu) title
+       write(uu) int(n, c_int32_t), int(n, c_int32_t)
+       write(uu) rloop + nsteps, rtime + timest * nsteps, ret, rek, rep
+       write(uu) cord
+       write(uu) vel
+       write(uu) desc%lambda, 0.0d0
+       close(uu)
+    end if
+    if (any(ier /= 0)) stop 1
+    ier(1) = jp_ctx_destroy(ctx)
+  end subroutine run_mdinp
 
   integer function off(p)
     type(c_ptr), intent(in) :: p

@@ -141,3 +141,45 @@ def test_fortran_host_canonical(golden, tmp_path):
     x, v = md.get_state()
     md.close()
     assert np.array_equal(x, o["x"]) and np.array_equal(v, o["v"])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("method", ["MICR", "CANO"])
+def test_fortran_host_mdinp(golden, tmp_path, method):
+    """jp_host --mdinp: md_run's own md.inp (EXE> MD keywords) and restart
+    file in, md_run's restart file out; the final state matches md_run's run
+    of the same md.inp within 1e-9."""
+    s, f = golden("water_1500")
+    print(f"case water_1500 mdinp {method}", flush=True)
+    d = str(tmp_path)
+    refio.write_jpsys(os.path.join(d, "sys.bin"), s)
+    refio.write_restart(os.path.join(d, "in.rst"), s.x, s.v, lam=s.lam)
+    files = refio.write_inputs(d, s)
+    md = {"METHOD": method, "RESTAR": "YES", "NAMERI": "in.rst", "NAMERO": "out.rst", "UPDATE": "10",
+          "TIMEST": "2.0", "OUTLOG": "100000", "OUTCLE": "100000"}
+    if method == "MICR":
+        md["LOOPLI"] = str(int(f["traj_nsteps"]))
+        want_x, want_v = f["traj_x"], f["traj_v"]
+    else:
+        md.update({"LOOPLI": str(int(f["cano_nsteps"])), "SETTEM": "300.0", "TEMPCO": "NO"})
+        want_x, want_v = f["cano_x"], f["cano_v"]
+    refio.write_md_inp(os.path.join(d, "md.inp"), files, s, md)
+    r = subprocess.run([_host(), "--mdinp", "md.inp", "sys.bin"], capture_output=True, text=True, timeout=240,
+                       cwd=d)
+    print(r.stdout, flush=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    o = refio.read_restart(os.path.join(d, "out.rst"))
+    assert o["n"] == s.n and o["loop"] == int(md["LOOPLI"])
+    assert np.abs(o["x"] - want_x).max() < 1e-9
+    assert np.abs(o["v"] - want_v).max() < 1e-9
+
+
+def test_host_mdinp_needs_method(tmp_path):
+    from j_presto_amd import systems
+    s = systems.water_box(60, 14.0, 3)
+    d = str(tmp_path)
+    refio.write_jpsys(os.path.join(d, "sys.bin"), s)
+    files = refio.write_inputs(d, s)
+    refio.write_md_inp(os.path.join(d, "md.inp"), files, s, {"LOOPLI": "5"})
+    r = subprocess.run([_host(), "--mdinp", "md.inp", "sys.bin"], capture_output=True, text=True, timeout=60, cwd=d)
+    assert r.returncode == 3 and "METHOD" in r.stdout
