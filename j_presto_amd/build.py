@@ -1,5 +1,6 @@
 """In-tree builds: libjpgpu.so for gfx950 (hipcc) and the Fortran host
-program fortran/jp_host (flang, ISO_C_BINDING module fortran/jpgpu.f90)."""
+program fortran/jp_host (flang: ISO_C_BINDING module fortran/jpgpu.f90 and
+the md_run input readers fortran/jp_input.f90)."""
 from __future__ import annotations
 
 import os
@@ -39,7 +40,8 @@ def build_fortran(verbose: bool = False) -> str:
     (runpath $ORIGIN/.., so the pair travels together)."""
     mdir = os.path.join(FDIR, "_build")
     os.makedirs(mdir, exist_ok=True)
-    cmd = [FLANG, "-O2", "-J", mdir, os.path.join(FDIR, "jpgpu.f90"), os.path.join(FDIR, "jp_host.f90"),
+    cmd = [FLANG, "-O2", "-J", mdir, os.path.join(FDIR, "jpgpu.f90"), os.path.join(FDIR, "jp_input.f90"),
+           os.path.join(FDIR, "jp_host.f90"),
            "-L" + HERE, "-ljpgpu", "-Wl,-rpath,$ORIGIN/..", "-o", FHOST + ".tmp"]
     r = subprocess.run(cmd, capture_output=True, text=True, cwd=mdir)
     if r.returncode != 0:

@@ -6,15 +6,23 @@
 !   TEMPCO= NO through jp_set_canonical), and writes every result to an output file.
 !
 ! usage: jp_host <system file> <output file> <nsteps> <update> <nmon> <precision 0|1> [settem]
-!        jp_host --mdinp <md.inp> <system file> [precision 0|1]
-!                 md_run's own control file: the EXE> MD keywords LOOPLI, UPDATE,
-!                 TIMEST, METHOD (MICR, CANO; required, since md_run's defaults
-!                 CANO with TEMPCO= YES need molecule kinds the system file does
-!                 not carry), SETTEM, TEMPCO (NO), OUTLOG, RESTAR, NAMERI,
-!                 NAMERO (inpmd.f90); x and v from the NAMERI
-!                 restart, the final state written to NAMERO in md_run's
-!                 restart format (outrst.f90:56-61); the topology from the
-!                 system file
+!        jp_host --mdinp <md.inp> [system file] [precision 0|1]
+!                 md_run's own control file.  EXE> INPUT: TOPOLO (TPL),
+!                 COORDI with CRDFMT= BINA, SETSHK, and CLUSTA= a<b (atoms
+!                 a..b in cluster 1, the rest in cluster 2) with GEPSME= LMD,
+!                 read by jp_input.f90.  EXE> MD: LOOPLI, UPDATE, TIMEST,
+!                 METHOD (MICR or CANO; required), SETTEM, TEMPCO, OUTLOG,
+!                 RESTAR, NAMERI, NAMERO, CUTLEN, DIEVAL, XMINCL..ZMAXCL,
+!                 COVSHK, LIMSHK (inpmd.f90; defaults of main.f90:150-185).
+!                 x, v and lambda from the NAMERI restart, the final state
+!                 written to NAMERO in md_run's restart format
+!                 (outrst.f90:56-61).  Given a system file, the topology and
+!                 the run parameters other than TIMEST come from it instead
+!                 (and TEMPCO= YES is refused: it carries no molecule kinds).
+!        jp_host --tplsys <md.inp> <system file>
+!                 the system md.inp's TPL/SHK/BINA inputs describe, written as
+!                 a system file (x and v from the restart with RESTAR= YES);
+!                 no GPU use
 !        jp_host --layout   (byte sizes and offsets of the bind(C) types; no GPU use)
 !
 ! System file (Fortran unformatted sequential, one array per record):
@@ -36,6 +44,7 @@
 program jp_host
   use iso_c_binding
   use jpgpu
+  use jp_input
   implicit none
   character(len=1024) :: fin, fout, arg
   integer(c_int32_t) :: hdr(14), nsteps, update, nmon, prec
@@ -44,7 +53,14 @@ program jp_host
   integer(c_int) :: ier(9)
   real(c_double) :: settem, timest
   character(len=1024) :: mdinp, rstin, rstout
-  logical :: mdmode, restar, tempco
+  logical :: mdmode, restar, tempco, tplsys, fromtpl
+  character(len=1024) :: tplf, crdf, shkf, crdfmt
+  integer :: cla, clb
+  logical :: lmdin
+  real(c_double) :: cutlen, dieval, box(6), covshk
+  integer(c_int32_t) :: limshk
+  integer(c_int32_t), allocatable, target :: grp(:)
+  type(top_t) :: top
   character(len=80) :: title
   integer(c_int32_t) :: rn, rnvar, rloop
   real(c_double) :: rtime, ret, rek, rep, rlam, rlamv
@@ -73,18 +89,27 @@ program jp_host
      write(*,'(a,i0)') 'sizeof jp_energy_log ', c_sizeof(rec1)
      stop
   end if
-  mdmode = trim(arg) == '--mdinp'
+  tplsys = trim(arg) == '--tplsys'
+  mdmode = trim(arg) == '--mdinp' .or. tplsys
   settem = 0.0d0
+  fin = ' '; fout = ' '
   if (mdmode) then
-     if (command_argument_count() < 3) then
-        write(*,'(a)') 'usage: jp_host --mdinp <md.inp> <system file> [precision 0|1]'
-        stop 2
-     end if
      call get_command_argument(2, mdinp)
-     call get_command_argument(3, fin)
      prec = 0
-     if (command_argument_count() >= 4) then
-        call get_command_argument(4, arg); read(arg, *) prec
+     do k = 3, command_argument_count()
+        call get_command_argument(k, arg)
+        if (tplsys) then
+           if (k == 3) fout = arg
+        else if (trim(arg) == '0' .or. trim(arg) == '1') then
+           read(arg, *) prec
+        else
+           fin = arg
+        end if
+     end do
+     if (len_trim(mdinp) == 0 .or. (tplsys .and. len_trim(fout) == 0)) then
+        write(*,'(a)') 'usage: jp_host --mdinp <md.inp> [system file] [precision 0|1]'
+        write(*,'(a)') '       jp_host --tplsys <md.inp> <system file>'
+        stop 2
      end if
      call read_mdinp(trim(mdinp))
   else
@@ -103,48 +128,12 @@ program jp_host
      end if
   end if
 
-  open(newunit=u, file=trim(fin), form='unformatted', access='sequential', status='old', action='read')
-  read(u) hdr
-  read(u) rhdr
-  n = hdr(1); ntyp = hdr(2); nfrag = hdr(3); nfrg = hdr(4)
-  nexcl = hdr(5); nexcess = hdr(6); nshake = hdr(7); nbond = hdr(11); nangle = hdr(12)
-  ntors = hdr(13); n14 = hdr(14)
-  allocate(cord(3,n), vel(3,n), mass(n), charge(n), atype(n), icls(n), rmin2(ntyp), eps(ntyp))
-  allocate(nmat(nfrag,nfrag), excl(2,max(nexcl,1)), excess(2,max(nexcess,1)))
-  allocate(snat(max(nshake,1)), satm(4,max(nshake,1)), sdist(6,max(nshake,1)))
-  allocate(batm(2,max(nbond,1)), bfc(max(nbond,1)), br0(max(nbond,1)))
-  allocate(aatm(3,max(nangle,1)), afc(max(nangle,1)), ath(max(nangle,1)))
-  allocate(tatm(4,max(ntors,1)), tfc(max(ntors,1)), trot(max(ntors,1)), tph(max(ntors,1)), tdiv(max(ntors,1)))
-  allocate(patm(2,max(n14,1)), pv(max(n14,1)), pe(max(n14,1)))
-  read(u) cord
-  read(u) vel
-  read(u) mass
-  read(u) charge
-  read(u) atype
-  read(u) icls
-  read(u) rmin2
-  read(u) eps
-  read(u) nmat
-  read(u) excl(:, 1:nexcl)
-  read(u) excess(:, 1:nexcess)
-  read(u) snat(1:nshake)
-  read(u) satm(:, 1:nshake)
-  read(u) sdist(:, 1:nshake)
-  read(u) batm(:, 1:nbond)
-  read(u) bfc(1:nbond)
-  read(u) br0(1:nbond)
-  read(u) aatm(:, 1:nangle)
-  read(u) afc(1:nangle)
-  read(u) ath(1:nangle)
-  read(u) tatm(:, 1:ntors)
-  read(u) tfc(1:ntors)
-  read(u) trot(1:ntors)
-  read(u) tph(1:ntors)
-  read(u) tdiv(1:ntors)
-  read(u) patm(:, 1:n14)
-  read(u) pv(1:n14)
-  read(u) pe(1:n14)
-  close(u)
+  fromtpl = len_trim(fin) == 0
+  if (fromtpl) then
+     call load_inputs()
+  else
+     call read_system_file()
+  end if
 
   desc%natom = n; desc%ntyp = ntyp; desc%nfrag = nfrag; desc%nfrg = nfrg
   desc%x = c_loc(cord); desc%mass = c_loc(mass); desc%charge = c_loc(charge)
@@ -166,7 +155,14 @@ program jp_host
 
   if (mdmode) then
      desc%dt = timest
-     if (restar) call read_restart(trim(rstin))
+     if (restar) then
+        call read_restart(trim(rstin))
+        if (fromtpl) desc%lambda = rlam
+     end if
+     if (tplsys) then
+        call write_system_file()
+        stop
+     end if
   end if
 
   write(*,'(2a)') 'jp_host: ', jp_version()
@@ -222,28 +218,81 @@ program jp_host
 
 contains
 
-  ! EXE> MD section of md_run's control file (inpmd.f90 keywords and
-  ! defaults; METHOD must be given, TEMPCO= YES is refused)
+  ! the system file (record layout above)
+  subroutine read_system_file()
+    open(newunit=u, file=trim(fin), form='unformatted', access='sequential', status='old', action='read')
+    read(u) hdr
+    read(u) rhdr
+    n = hdr(1); ntyp = hdr(2); nfrag = hdr(3); nfrg = hdr(4)
+    nexcl = hdr(5); nexcess = hdr(6); nshake = hdr(7); nbond = hdr(11); nangle = hdr(12)
+    ntors = hdr(13); n14 = hdr(14)
+      call alloc_arrays()
+    read(u) cord
+    read(u) vel
+    read(u) mass
+    read(u) charge
+    read(u) atype
+    read(u) icls
+    read(u) rmin2
+    read(u) eps
+    read(u) nmat
+    read(u) excl(:, 1:nexcl)
+    read(u) excess(:, 1:nexcess)
+    read(u) snat(1:nshake)
+    read(u) satm(:, 1:nshake)
+    read(u) sdist(:, 1:nshake)
+    read(u) batm(:, 1:nbond)
+    read(u) bfc(1:nbond)
+    read(u) br0(1:nbond)
+    read(u) aatm(:, 1:nangle)
+    read(u) afc(1:nangle)
+    read(u) ath(1:nangle)
+    read(u) tatm(:, 1:ntors)
+    read(u) tfc(1:ntors)
+    read(u) trot(1:ntors)
+    read(u) tph(1:ntors)
+    read(u) tdiv(1:ntors)
+    read(u) patm(:, 1:n14)
+    read(u) pv(1:n14)
+    read(u) pe(1:n14)
+    close(u)
+  end subroutine read_system_file
+
+  subroutine alloc_arrays()
+    allocate(cord(3,n), vel(3,n), mass(n), charge(n), atype(n), icls(n), rmin2(ntyp), eps(ntyp))
+    allocate(nmat(nfrag,nfrag), excl(2,max(nexcl,1)), excess(2,max(nexcess,1)))
+    allocate(snat(max(nshake,1)), satm(4,max(nshake,1)), sdist(6,max(nshake,1)))
+    allocate(batm(2,max(nbond,1)), bfc(max(nbond,1)), br0(max(nbond,1)))
+    allocate(aatm(3,max(nangle,1)), afc(max(nangle,1)), ath(max(nangle,1)))
+    allocate(tatm(4,max(ntors,1)), tfc(max(ntors,1)), trot(max(ntors,1)), tph(max(ntors,1)), tdiv(max(ntors,1)))
+    allocate(patm(2,max(n14,1)), pv(max(n14,1)), pe(max(n14,1)))
+  end subroutine alloc_arrays
+
+  ! EXE> INPUT and EXE> MD sections of md_run's control file (inpmd.f90
+  ! keywords, main.f90:150-185 defaults; METHOD must be given)
   subroutine read_mdinp(path)
     character(len=*), intent(in) :: path
     character(len=256) :: line, rest, key, val
     integer :: uu, ios, p, q
-    logical :: inmd, have_method
+    logical :: inmd, inin, have_method
     nsteps = 0; update = 10; nmon = 0; timest = 2.0d0; settem = -300.0d0   ! negative: METHOD= MICR
     restar = .false.; tempco = .true.; rstin = ' '; rstout = ' '
+    tplf = ' '; crdf = ' '; shkf = ' '; crdfmt = 'FORM'; cla = 0; clb = 0; lmdin = .false.
+    cutlen = 12.0d0; dieval = 1.0d0; box = 0.0d0; covshk = 1.0d-6; limshk = 1000
     have_method = .false.
-    inmd = .false.
+    inmd = .false.; inin = .false.
     open(newunit=uu, file=path, status='old', action='read')
     do
        read(uu, '(a)', iostat=ios) line
        if (ios /= 0) exit
        if (index(line, 'EXE>') > 0) then
           inmd = index(line, 'EXE> MD') > 0
+          inin = index(line, 'EXE> INPUT') > 0
           cycle
        end if
-       if (.not. inmd) cycle
+       if (.not. (inmd .or. inin)) cycle
        if (index(line, 'QUIT') > 0) then
-          inmd = .false.
+          inmd = .false.; inin = .false.
           cycle
        end if
        rest = line
@@ -256,6 +305,24 @@ contains
           if (q == 0) q = len_trim(rest) + 1
           val = rest(1:q - 1)
           rest = rest(q:)
+          if (inin) then
+             select case (trim(key))
+             case ('TOPOLO'); tplf = val
+             case ('COORDI'); crdf = val
+             case ('CRDFMT'); crdfmt = val
+             case ('SETSHK'); shkf = val
+             case ('GEPSME'); lmdin = val(1:3) == 'LMD'
+             case ('CLUSTA')
+                q = index(val, '<')
+                if (q < 2) then
+                   write(*,'(2a)') 'jp_host: CLUSTA must read a<b: ', trim(val)
+                   stop 3
+                end if
+                read(val(1:q - 1), *) cla
+                read(val(q + 1:), *) clb
+             end select
+             cycle
+          end if
           select case (trim(key))
           case ('LOOPLI'); read(val, *) nsteps
           case ('UPDATE'); read(val, *) update
@@ -266,6 +333,16 @@ contains
           case ('NAMERI'); rstin = val
           case ('NAMERO'); rstout = val
           case ('TEMPCO'); tempco = val(1:3) == 'YES'
+          case ('CUTLEN'); read(val, *) cutlen
+          case ('DIEVAL'); read(val, *) dieval
+          case ('COVSHK'); read(val, *) covshk
+          case ('LIMSHK'); read(val, *) limshk
+          case ('XMINCL'); read(val, *) box(1)
+          case ('XMAXCL'); read(val, *) box(2)
+          case ('YMINCL'); read(val, *) box(3)
+          case ('YMAXCL'); read(val, *) box(4)
+          case ('ZMINCL'); read(val, *) box(5)
+          case ('ZMAXCL'); read(val, *) box(6)
           case ('METHOD')
              have_method = .true.
              if (val(1:4) == 'CANO') then
@@ -285,8 +362,8 @@ contains
        stop 3
     end if
     if (nmon <= 0 .or. nmon > nsteps) nmon = 0
-    if (settem > 0.0d0 .and. tempco) then
-       write(*,'(a)') 'jp_host: TEMPCO= YES needs the molecule kinds (use jp_set_canonical with groups)'
+    if (settem > 0.0d0 .and. tempco .and. len_trim(fin) > 0) then
+       write(*,'(a)') 'jp_host: TEMPCO= YES needs the molecule kinds of the TPL (a system file has none)'
        stop 3
     end if
   end subroutine read_mdinp
@@ -315,7 +392,14 @@ contains
     if (.not. restar) then
        rloop = 0; rtime = 0.0d0
     end if
-    if (settem > 0.0d0) ier(9) = jp_set_canonical(ctx, 1_c_int32_t, settem, 3_c_int32_t, 1.0d-3, c_null_ptr)
+    if (settem > 0.0d0) then
+       if (tempco .and. fromtpl) then
+          call temperature_groups()
+          ier(9) = jp_set_canonical(ctx, 1_c_int32_t, settem, 3_c_int32_t, 1.0d-3, c_loc(grp))
+       else
+          ier(9) = jp_set_canonical(ctx, 1_c_int32_t, settem, 3_c_int32_t, 1.0d-3, c_null_ptr)
+       end if
+    end if
     nl = 0
     if (nmon > 0) nl = nsteps / nmon
     allocate(elog(max(nl, 1)))
@@ -341,6 +425,131 @@ contains
     if (any(ier /= 0)) stop 1
     ier(1) = jp_ctx_destroy(ctx)
   end subroutine run_mdinp
+
+  ! the system of md.inp's EXE> INPUT files (TPL, SHK, BINA coordinates) and
+  ! EXE> MD parameters, in the arrays a system file fills
+  subroutine load_inputs()
+    integer :: ie
+    if (len_trim(tplf) == 0 .or. len_trim(crdf) == 0) then
+       write(*,'(a)') 'jp_host: without a system file EXE> INPUT must give TOPOLO and COORDI'
+       stop 3
+    end if
+    if (crdfmt(1:4) /= 'BINA') then
+       write(*,'(2a)') 'jp_host: CRDFMT not supported (BINA only): ', trim(crdfmt)
+       stop 3
+    end if
+    if (lmdin .neqv. clb > 0) then
+       write(*,'(a)') 'jp_host: GEPSME= LMD and CLUSTA= a<b go together (other cluster methods not supported)'
+       stop 3
+    end if
+    if (box(2) <= box(1) .or. box(4) <= box(3) .or. box(6) <= box(5)) then
+       write(*,'(a)') 'jp_host: EXE> MD must give the periodic box XMINCL..ZMAXCL'
+       stop 3
+    end if
+    call read_topology(trim(tplf), trim(shkf), dieval, top, ie)
+    if (ie /= 0) then
+       write(*,'(a,i0)') 'jp_host: cannot read TOPOLO / SETSHK, ier = ', ie
+       stop 3
+    end if
+    n = top%n; ntyp = top%ntyp; nexcl = top%nexcl; nexcess = top%nexcess; nshake = top%nshake
+    nbond = top%nbond; nangle = top%nangle; ntors = top%ntors; n14 = top%n14
+    nfrag = 1; nfrg = 1
+    if (lmdin) then
+       if (cla < 1 .or. clb > n .or. cla > clb) then
+          write(*,'(a,2(1x,i0))') 'jp_host: CLUSTA atoms out of range:', cla, clb
+          stop 3
+       end if
+       nfrag = 2; nfrg = 3
+    end if
+    call alloc_arrays()
+    call read_crd_bina(trim(crdf), n, cord, ie)
+    if (ie /= 0) then
+       write(*,'(a,i0)') 'jp_host: cannot read COORDI (BINA, same atom count as the TPL), ier = ', ie
+       stop 3
+    end if
+    vel = 0.0d0
+    mass = top%mass; charge = top%chg; atype = top%atype; rmin2 = top%rmin2; eps = top%eps
+    icls = 1; nmat = 1
+    if (lmdin) then
+       ! cluster 1 = atoms cla..clb, cluster 2 = the rest; LMD classes (inpcls.f90:297-324)
+       icls = 2; icls(cla:clb) = 1
+       nmat = reshape([1, 2, 2, 3], [2, 2])
+    end if
+    excl(:, 1:nexcl) = top%excl(:, 1:nexcl); excess(:, 1:nexcess) = top%excess(:, 1:nexcess)
+    snat(1:nshake) = top%snat(1:nshake); satm(:, 1:nshake) = top%satm(:, 1:nshake)
+    sdist(:, 1:nshake) = top%sdist(:, 1:nshake)
+    batm(:, 1:nbond) = top%batm(:, 1:nbond); bfc(1:nbond) = top%bfc(1:nbond); br0(1:nbond) = top%br0(1:nbond)
+    aatm(:, 1:nangle) = top%aatm(:, 1:nangle); afc(1:nangle) = top%afc(1:nangle)
+    ath(1:nangle) = top%ath(1:nangle)
+    tatm(:, 1:ntors) = top%tatm(:, 1:ntors); tfc(1:ntors) = top%tfc(1:ntors); trot(1:ntors) = top%trot(1:ntors)
+    tph(1:ntors) = top%tph(1:ntors); tdiv(1:ntors) = top%tdiv(1:ntors)
+    patm(:, 1:n14) = top%patm(:, 1:n14); pv(1:n14) = top%p14v(1:n14); pe(1:n14) = top%p14e(1:n14)
+    hdr = [int(n, c_int32_t), int(ntyp, c_int32_t), int(nfrag, c_int32_t), int(nfrg, c_int32_t), &
+           int(nexcl, c_int32_t), int(nexcess, c_int32_t), int(nshake, c_int32_t), limshk, 3_c_int32_t, &
+           merge(1_c_int32_t, 0_c_int32_t, lmdin), int(nbond, c_int32_t), int(nangle, c_int32_t), &
+           int(ntors, c_int32_t), int(n14, c_int32_t)]
+    rhdr(1) = covshk; rhdr(2) = cutlen; rhdr(3) = dieval; rhdr(4) = timest; rhdr(5) = 0.0d0
+    rhdr(6:11) = box
+  end subroutine load_inputs
+
+  ! TEMPCO= YES groups (md.f90:136-152): ixamol2 = (molecule kind - 1) *
+  ! nfrag + cluster, numbered 1.. without the values no atom takes
+  subroutine temperature_groups()
+    integer, allocatable :: lab(:)
+    integer :: a, g, ng
+    allocate(grp(n), lab(maxval(top%kind) * nfrag))
+    lab = 0
+    do a = 1, n
+       lab((top%kind(a) - 1) * nfrag + icls(a)) = 1
+    end do
+    ng = 0
+    do g = 1, size(lab)
+       if (lab(g) == 0) cycle
+       ng = ng + 1
+       lab(g) = ng
+    end do
+    do a = 1, n
+       grp(a) = lab((top%kind(a) - 1) * nfrag + icls(a))
+    end do
+  end subroutine temperature_groups
+
+  ! the system file of the arrays and parameters in use (x and v included)
+  subroutine write_system_file()
+    integer :: uw
+    rhdr(4) = desc%dt; rhdr(5) = desc%lambda
+    open(newunit=uw, file=trim(fout), form='unformatted', access='sequential', status='replace', action='write')
+    write(uw) hdr
+    write(uw) rhdr
+    write(uw) cord
+    write(uw) vel
+    write(uw) mass
+    write(uw) charge
+    write(uw) atype
+    write(uw) icls
+    write(uw) rmin2
+    write(uw) eps
+    write(uw) nmat
+    write(uw) excl(:, 1:nexcl)
+    write(uw) excess(:, 1:nexcess)
+    write(uw) snat(1:nshake)
+    write(uw) satm(:, 1:nshake)
+    write(uw) sdist(:, 1:nshake)
+    write(uw) batm(:, 1:nbond)
+    write(uw) bfc(1:nbond)
+    write(uw) br0(1:nbond)
+    write(uw) aatm(:, 1:nangle)
+    write(uw) afc(1:nangle)
+    write(uw) ath(1:nangle)
+    write(uw) tatm(:, 1:ntors)
+    write(uw) tfc(1:ntors)
+    write(uw) trot(1:ntors)
+    write(uw) tph(1:ntors)
+    write(uw) tdiv(1:ntors)
+    write(uw) patm(:, 1:n14)
+    write(uw) pv(1:n14)
+    write(uw) pe(1:n14)
+    close(uw)
+  end subroutine write_system_file
 
   integer function off(p)
     type(c_ptr), intent(in) :: p

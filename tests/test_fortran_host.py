@@ -183,3 +183,90 @@ def test_host_mdinp_needs_method(tmp_path):
     refio.write_md_inp(os.path.join(d, "md.inp"), files, s, {"LOOPLI": "5"})
     r = subprocess.run([_host(), "--mdinp", "md.inp", "sys.bin"], capture_output=True, text=True, timeout=60, cwd=d)
     assert r.returncode == 3 and "METHOD" in r.stdout
+
+
+_RECORDS = ("hdr rhdr x v mass charge atype icls rmin2 eps nmat excl excess snat satm sdist batm bfc br0 aatm "
+            "afc ath tatm tfc trot tph tdiv patm pv pe").split()
+
+
+def _tpl_case(d, s, md):
+    refio.write_restart(os.path.join(d, "in.rst"), s.x, s.v, lam=s.lam)
+    files = refio.write_inputs(d, s)
+    base = {"RESTAR": "YES", "NAMERI": "in.rst", "NAMERO": "out.rst", "UPDATE": "10", "TIMEST": "2.0",
+            "OUTLOG": "100000", "OUTCLE": "100000"}
+    base.update(md)
+    refio.write_md_inp(os.path.join(d, "md.inp"), files, s, base)
+
+
+@pytest.mark.parametrize("name", ["water_1500", "chain_3k", "chain_lmd", "chain_lmd_bonded", "chain_tors",
+                                  "chain_lmd_tors", "ions_3k"])
+def test_tplsys_equals_system_file(golden, tmp_path, name):
+    """jp_host --tplsys reads md.inp's TPL, SHK and BINA files and the
+    restart (jp_input.f90) and writes the system file: every record equals,
+    byte for byte, the one written from the same System through the Python
+    descriptor (whose forces and trajectories match md_run's golden
+    fixtures): atoms, types, clusters, excluded and excess pairs, SHAKE
+    groups, the bonds and angles setshk keeps, torsions and 1-4 pairs."""
+    s, _ = golden(name)
+    d = str(tmp_path)
+    refio.write_jpsys(os.path.join(d, "want.bin"), s)
+    _tpl_case(d, s, {"METHOD": "MICR", "LOOPLI": "5"})
+    r = subprocess.run([_host(), "--tplsys", "md.inp", "got.bin"], capture_output=True, text=True, timeout=120,
+                       cwd=d)
+    assert r.returncode == 0, r.stdout + r.stderr
+    want = refio.read_records(os.path.join(d, "want.bin"))
+    got = refio.read_records(os.path.join(d, "got.bin"))
+    assert len(got) == len(want) == len(_RECORDS)
+    for lab, p, q in zip(_RECORDS, want, got):
+        assert p == q, lab
+
+
+def test_tplsys_input_errors(tmp_path):
+    """Missing or inconsistent EXE> INPUT data ends jp_host with status 3
+    and a message, before any GPU use."""
+    from j_presto_amd import systems
+    s = systems.chain_in_water(48, 24.6, 22, lmd=0.8)
+    d = str(tmp_path)
+    _tpl_case(d, s, {"METHOD": "MICR", "LOOPLI": "5"})
+    inp = open(os.path.join(d, "md.inp")).read()
+    cases = {
+        "TOPOLO": inp.replace("TOPOLO= sys.tpl", "TOPOLE= sys.tpl"),
+        "CLUSTA": inp.replace("GEPSME= LMD", "GEPSME= AND"),
+        "CRDFMT": inp.replace("CRDFMT= BINA", "CRDFMT= FORM"),
+        "XMINCL..ZMAXCL": "\n".join(l for l in inp.splitlines() if "ZMAXCL" not in l),
+        "COORDI": inp.replace("sys.crd", "none.crd"),
+    }
+    for what, text in cases.items():
+        with open(os.path.join(d, "bad.inp"), "w") as f:
+            f.write(text)
+        r = subprocess.run([_host(), "--tplsys", "bad.inp", "got.bin"], capture_output=True, text=True,
+                           timeout=60, cwd=d)
+        assert r.returncode == 3 and what in r.stdout, (what, r.stdout)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,method", [("water_1500", "MICR"), ("chain_lmd", "MICR"), ("chain_3k", "CANO"),
+                                         ("chain_lmd", "CANO")])
+def test_fortran_host_mdinp_tpl(golden, tmp_path, name, method):
+    """jp_host --mdinp with md_run's own inputs only (md.inp, TPL, SHK, BINA,
+    restart; no system file): the final state written to NAMERO matches
+    md_run's run of the same md.inp within 1e-9 (CANO here with the
+    fixtures' TEMPCO, YES for both: groups from the TPL's molecule kinds)."""
+    s, f = golden(name)
+    print(f"case {name} mdinp-tpl {method}", flush=True)
+    d = str(tmp_path)
+    if method == "MICR":
+        md = {"METHOD": "MICR", "LOOPLI": str(int(f["traj_nsteps"]))}
+        want_x, want_v = f["traj_x"], f["traj_v"]
+    else:
+        md = {"METHOD": "CANO", "LOOPLI": str(int(f["cano_nsteps"])), "SETTEM": "300.0",
+              "TEMPCO": str(f["cano_tempco"])}
+        want_x, want_v = f["cano_x"], f["cano_v"]
+    _tpl_case(d, s, md)
+    r = subprocess.run([_host(), "--mdinp", "md.inp"], capture_output=True, text=True, timeout=240, cwd=d)
+    print(r.stdout, flush=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    o = refio.read_restart(os.path.join(d, "out.rst"))
+    assert o["n"] == s.n and o["loop"] == int(md["LOOPLI"])
+    assert np.abs(o["x"] - want_x).max() < 1e-9
+    assert np.abs(o["v"] - want_v).max() < 1e-9
