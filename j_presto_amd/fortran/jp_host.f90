@@ -12,8 +12,10 @@
 !                 a..b in cluster 1, the rest in cluster 2) with GEPSME= LMD,
 !                 read by jp_input.f90.  EXE> MD: LOOPLI, UPDATE, TIMEST,
 !                 METHOD (MICR or CANO; required), SETTEM, TEMPCO, OUTLOG,
-!                 RESTAR, NAMERI, NAMERO, CUTLEN, DIEVAL, XMINCL..ZMAXCL,
+!                 RESTAR, NAMERI, NAMERO, NAMETO, CUTLEN, DIEVAL, XMINCL..ZMAXCL,
 !                 COVSHK, LIMSHK (inpmd.f90; defaults of main.f90:150-185).
+!                 NAMETO receives the potential energy of every step per
+!                 energy class in md_run's format (md.f90:578-601).
 !                 x, v and lambda from the NAMERI restart, the final state
 !                 written to NAMERO in md_run's restart format
 !                 (outrst.f90:56-61).  Given a system file, the topology and
@@ -52,7 +54,7 @@ program jp_host
   integer :: n, ntyp, nfrag, nfrg, nexcl, nexcess, nshake, nbond, nangle, ntors, n14, nlog, k, u
   integer(c_int) :: ier(9)
   real(c_double) :: settem, timest
-  character(len=1024) :: mdinp, rstin, rstout
+  character(len=1024) :: mdinp, rstin, rstout, etof
   logical :: mdmode, restar, tempco, tplsys, fromtpl
   character(len=1024) :: tplf, crdf, shkf, crdfmt
   integer :: cla, clb
@@ -276,7 +278,7 @@ contains
     integer :: uu, ios, p, q
     logical :: inmd, inin, have_method
     nsteps = 0; update = 10; nmon = 0; timest = 2.0d0; settem = -300.0d0   ! negative: METHOD= MICR
-    restar = .false.; tempco = .true.; rstin = ' '; rstout = ' '
+    restar = .false.; tempco = .true.; rstin = ' '; rstout = ' '; etof = ' '
     tplf = ' '; crdf = ' '; shkf = ' '; crdfmt = 'FORM'; cla = 0; clb = 0; lmdin = .false.
     cutlen = 12.0d0; dieval = 1.0d0; box = 0.0d0; covshk = 1.0d-6; limshk = 1000
     have_method = .false.
@@ -332,6 +334,7 @@ contains
           case ('RESTAR'); restar = val(1:3) == 'YES'
           case ('NAMERI'); rstin = val
           case ('NAMERO'); rstout = val
+          case ('NAMETO'); etof = val
           case ('TEMPCO'); tempco = val(1:3) == 'YES'
           case ('CUTLEN'); read(val, *) cutlen
           case ('DIEVAL'); read(val, *) dieval
@@ -388,7 +391,8 @@ contains
   end subroutine read_restart
 
   subroutine run_mdinp()
-    integer :: uu, nl
+    integer :: uu, nl, nm, kk
+    real(c_double) :: epc(3)
     if (.not. restar) then
        rloop = 0; rtime = 0.0d0
     end if
@@ -400,12 +404,38 @@ contains
           ier(9) = jp_set_canonical(ctx, 1_c_int32_t, settem, 3_c_int32_t, 1.0d-3, c_null_ptr)
        end if
     end if
+    ! NAMETO needs the energies of every step
+    nm = nmon
+    if (len_trim(etof) > 0) nm = 1
     nl = 0
-    if (nmon > 0) nl = nsteps / nmon
+    if (nm > 0) nl = nsteps / nm
     allocate(elog(max(nl, 1)))
-    ier(5) = jp_run_nve(ctx, nsteps, update, nmon, prec, c_loc(elog))
+    ier(5) = jp_run_nve(ctx, nsteps, update, nm, prec, c_loc(elog))
     ier(6) = jp_get_state(ctx, c_loc(cord), c_loc(vel))
     write(*,'(a,9(1x,i0))') 'jp_host: ier =', ier
+    if (len_trim(etof) > 0 .and. ier(5) == 0) then
+       ! md.f90:578-601: epotcc(1,k), the unscaled potential energy of class k
+       open(newunit=uu, file=trim(etof), status='replace', action='write')
+       do kk = 1, nl
+          epc = elog(kk)%ebond + elog(kk)%eangle + elog(kk)%etors + elog(kk)%eimp + elog(kk)%evdw14 &
+                + elog(kk)%eelec14 + elog(kk)%evdw + elog(kk)%eelec
+          if (desc%lmd /= 0) then
+             write(uu, '(2(e15.7,x))', advance='no') desc%lambda, epc(1)
+             if (nfrg == 1) then
+                write(uu, *)
+             else
+                write(uu, '(2(e15.7,x))') epc(2), sum(epc(1:nfrg))
+             end if
+          else if (nfrg == 1) then
+             write(uu, '(e15.7)') epc(1)
+          else if (nfrg == 2) then
+             write(uu, '(2(e15.7,x))') epc(1), sum(epc(1:2))
+          else
+             write(uu, '(3(e15.7,x))') epc(1), epc(2), sum(epc(1:3))
+          end if
+       end do
+       close(uu)
+    end if
     ret = 0.0d0; rek = 0.0d0; rep = 0.0d0
     if (nl > 0) then
        ret = elog(nl)%etot; rek = elog(nl)%ekin; rep = elog(nl)%epot

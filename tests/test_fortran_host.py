@@ -251,12 +251,15 @@ def test_fortran_host_mdinp_tpl(golden, tmp_path, name, method):
     """jp_host --mdinp with md_run's own inputs only (md.inp, TPL, SHK, BINA,
     restart; no system file): the final state written to NAMERO matches
     md_run's run of the same md.inp within 1e-9 (CANO here with the
-    fixtures' TEMPCO, YES for both: groups from the TPL's molecule kinds)."""
+    fixtures' TEMPCO, YES for both: groups from the TPL's molecule kinds).
+    MICR also writes NAMETO, the potential energy of every step per energy
+    class (and lambda under LMD) in E15.7: equal to md_run's file within one
+    unit of its last printed digit."""
     s, f = golden(name)
     print(f"case {name} mdinp-tpl {method}", flush=True)
     d = str(tmp_path)
     if method == "MICR":
-        md = {"METHOD": "MICR", "LOOPLI": str(int(f["traj_nsteps"]))}
+        md = {"METHOD": "MICR", "LOOPLI": str(int(f["traj_nsteps"])), "NAMETO": "ene.dat"}
         want_x, want_v = f["traj_x"], f["traj_v"]
     else:
         md = {"METHOD": "CANO", "LOOPLI": str(int(f["cano_nsteps"])), "SETTEM": "300.0",
@@ -270,3 +273,10 @@ def test_fortran_host_mdinp_tpl(golden, tmp_path, name, method):
     assert o["n"] == s.n and o["loop"] == int(md["LOOPLI"])
     assert np.abs(o["x"] - want_x).max() < 1e-9
     assert np.abs(o["v"] - want_v).max() < 1e-9
+    if method == "MICR":
+        got = np.loadtxt(os.path.join(d, "ene.dat"), ndmin=2)
+        ref = f["traj_epot"]
+        assert got.shape == ref.shape
+        # E15.7: 7 significant digits, one unit of the last is 1e-7 of the leading power of ten
+        ulp = 10.0 ** (np.floor(np.log10(np.maximum(np.abs(ref), 1e-300))) - 6)
+        assert np.all(np.abs(got - ref) <= ulp * 1.0001)
