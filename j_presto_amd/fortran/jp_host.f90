@@ -8,7 +8,7 @@
 ! usage: jp_host <system file> <output file> <nsteps> <update> <nmon> <precision 0|1> [settem]
 !        jp_host --mdinp <md.inp> [system file] [precision 0|1]
 !                 md_run's own control file.  EXE> INPUT: TOPOLO (TPL),
-!                 COORDI with CRDFMT= BINA, SETSHK, and CLUSTA= a<b (atoms
+!                 COORDI with CRDFMT= PDB (default) or BINA, SETSHK, and CLUSTA= a<b (atoms
 !                 a..b in cluster 1, the rest in cluster 2) with GEPSME= LMD,
 !                 read by jp_input.f90.  EXE> MD: LOOPLI, UPDATE, TIMEST,
 !                 METHOD (MICR or CANO; required), SETTEM, TEMPCO, OUTLOG,
@@ -279,7 +279,7 @@ contains
     logical :: inmd, inin, have_method
     nsteps = 0; update = 10; nmon = 0; timest = 2.0d0; settem = -300.0d0   ! negative: METHOD= MICR
     restar = .false.; tempco = .true.; rstin = ' '; rstout = ' '; etof = ' '
-    tplf = ' '; crdf = ' '; shkf = ' '; crdfmt = 'FORM'; cla = 0; clb = 0; lmdin = .false.
+    tplf = ' '; crdf = ' '; shkf = ' '; crdfmt = 'PDB'; cla = 0; clb = 0; lmdin = .false.
     cutlen = 12.0d0; dieval = 1.0d0; box = 0.0d0; covshk = 1.0d-6; limshk = 1000
     have_method = .false.
     inmd = .false.; inin = .false.
@@ -464,8 +464,8 @@ contains
        write(*,'(a)') 'jp_host: without a system file EXE> INPUT must give TOPOLO and COORDI'
        stop 3
     end if
-    if (crdfmt(1:4) /= 'BINA') then
-       write(*,'(2a)') 'jp_host: CRDFMT not supported (BINA only): ', trim(crdfmt)
+    if (crdfmt(1:4) /= 'BINA' .and. crdfmt(1:3) /= 'PDB') then
+       write(*,'(2a)') 'jp_host: CRDFMT not supported (PDB or BINA): ', trim(crdfmt)
        stop 3
     end if
     if (lmdin .neqv. clb > 0) then
@@ -492,9 +492,13 @@ contains
        nfrag = 2; nfrg = 3
     end if
     call alloc_arrays()
-    call read_crd_bina(trim(crdf), n, cord, ie)
+    if (crdfmt(1:4) == 'BINA') then
+       call read_crd_bina(trim(crdf), n, cord, ie)
+    else
+       call read_crd_pdb(trim(crdf), n, cord, ie)
+    end if
     if (ie /= 0) then
-       write(*,'(a,i0)') 'jp_host: cannot read COORDI (BINA, same atom count as the TPL), ier = ', ie
+       write(*,'(a,i0)') 'jp_host: cannot read COORDI (same atom count as the TPL), ier = ', ie
        stop 3
     end if
     vel = 0.0d0

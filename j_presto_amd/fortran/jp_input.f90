@@ -2,7 +2,7 @@
 ! TPL topology (inptpl.f90: TITLE, MOLECULES, ATOMS, BONDS, ANGLES, TORSIONS,
 ! FUNCTIONS, NONBONDS records; ';' comments, '->' continuations), the SHAKE
 ! file (setshk.f90: SHAKE> SHAKE groups with their distances) and BINA
-! coordinates (inpcrb.f90).  read_topology expands the molecules by their copy
+! (inpcrb.f90) or PDB (inppdb.f90) coordinates.  read_topology expands the molecules by their copy
 ! counts and derives the tables jp_sys_desc takes, as md_run's input phase
 ! does: excluded pairs from the ATOMS partner lists (inpcls.f90:229-295), ZD
 ! excess pairs (incel.f90:665-688), SHAKE groups by size (setshk.f90 modshk),
@@ -13,7 +13,7 @@ module jp_input
   use iso_c_binding
   implicit none
   private
-  public :: read_topology, read_crd_bina, top_t
+  public :: read_topology, read_crd_bina, read_crd_pdb, top_t
 
   integer, parameter :: maxtok = 512, toklen = 64
 
@@ -672,5 +672,37 @@ contains
     end do
     close(u)
   end subroutine read_crd_bina
+
+  ! PDB coordinates (inppdb.f90): every ATOM / HETATM record, x y z as
+  ! 3f8.3 after column 30.  ier: 0, 1 (file), 2 (atom count), 3 (format)
+  subroutine read_crd_pdb(path, n, x, ier)
+    character(len=*), intent(in) :: path
+    integer, intent(in) :: n
+    real(8), intent(out) :: x(3, n)
+    integer, intent(out) :: ier
+    character(len=80) :: line
+    integer :: u, ios, m
+    ier = 0
+    open(newunit=u, file=path, status='old', action='read', iostat=ios)
+    if (ios /= 0) then
+       ier = 1
+       return
+    end if
+    m = 0
+    do
+       read(u, '(a80)', iostat=ios) line
+       if (ios /= 0) exit
+       if (line(1:4) /= 'ATOM' .and. line(1:4) /= 'HETA') cycle
+       m = m + 1
+       if (m > n) exit
+       read(line, '(30x,3f8.3)', iostat=ios) x(1:3, m)
+       if (ios /= 0) then
+          ier = 3
+          exit
+       end if
+    end do
+    close(u)
+    if (ier == 0 .and. m /= n) ier = 2
+  end subroutine read_crd_pdb
 
 end module jp_input

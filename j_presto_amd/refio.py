@@ -9,6 +9,7 @@ Formats (reference file:line):
         (setshk.f90:120-400; pair order 1-2, 2-3, 1-3, 1-4, 3-4, 2-4 from
         parlis at setshk.f90:541-542 and mkshkl.f90:674-690)
   .crd  binary coordinates, CRDFMT=BINA (inpcrb.f90:30-40)
+  .pdb  CRDFMT=PDB: ATOM records, x y z as 3f8.3 after column 30 (inppdb.f90)
   restart  Fortran unformatted: title; natm,nvar; loop,time,Et,Ek,Ep;
         cord(3,N); vel(3,nvar); lambda,lambda_v (outrst.f90:56-61)
   1stp_force.bin  FDEBUG=MAKE dump: epot(15); lambda_f; grad(3,N)
@@ -91,6 +92,19 @@ def write_crd(path: str, x: np.ndarray) -> None:
         struct.pack("<i", len(x)),
         x[:, 0].tobytes(), x[:, 1].tobytes(), x[:, 2].tobytes(),
     ])
+
+
+def write_pdb(path: str, x: np.ndarray) -> None:
+    """PDB coordinates as md_run reads them (inppdb.f90: ATOM records, x y z
+    in columns 31-54 as 3f8.3)."""
+    x = np.asarray(x, np.float64)
+    L = []
+    for i, (a, b, c) in enumerate(x):
+        L.append(f"ATOM  {(i + 1) % 100000:>5d} X    UNK A{(i // 3 + 1) % 10000:>4d}    {a:8.3f}{b:8.3f}{c:8.3f}"
+                 f"{1.0:6.2f}{0.0:6.2f}")
+    L.append("END")
+    with open(path, "w") as f:
+        f.write("\n".join(L) + "\n")
 
 
 def read_force_dump(path: str, n: int) -> Dict[str, np.ndarray]:

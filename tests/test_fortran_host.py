@@ -221,6 +221,40 @@ def test_tplsys_equals_system_file(golden, tmp_path, name):
         assert p == q, lab
 
 
+def test_tplsys_pdb_coordinates(tmp_path):
+    """CRDFMT= PDB without a restart: x is what 3f8.3 reads from columns
+    31-54 of every ATOM record (inppdb.f90), v is zero; the rest equals the
+    BINA case."""
+    from j_presto_amd import systems
+    s = systems.chain_in_water(48, 24.6, 22, lmd=0.8)
+    d = str(tmp_path)
+    _tpl_case(d, s, {"METHOD": "MICR", "LOOPLI": "5"})
+    refio.write_pdb(os.path.join(d, "sys.pdb"), s.x)
+    inp = open(os.path.join(d, "md.inp")).read()
+    r = subprocess.run([_host(), "--tplsys", "md.inp", "bina.bin"], capture_output=True, text=True, timeout=60,
+                       cwd=d)
+    assert r.returncode == 0, r.stdout
+    inp = inp.replace("COORDI= sys.crd", "COORDI= sys.pdb").replace("CRDFMT= BINA", "CRDFMT= PDB")
+    inp = inp.replace("RESTAR= YES", "RESTAR= NO")
+    with open(os.path.join(d, "pdb.inp"), "w") as f:
+        f.write(inp)
+    r = subprocess.run([_host(), "--tplsys", "pdb.inp", "pdb.bin"], capture_output=True, text=True, timeout=60,
+                       cwd=d)
+    assert r.returncode == 0, r.stdout
+    a = refio.read_records(os.path.join(d, "bina.bin"))
+    b = refio.read_records(os.path.join(d, "pdb.bin"))
+    want_x = np.array([[float(f"{c:8.3f}") for c in row] for row in s.x])
+    assert np.array_equal(np.frombuffer(b[2], np.float64).reshape(-1, 3), want_x)
+    assert not np.frombuffer(b[3], np.float64).any()
+    assert np.abs(want_x - s.x).max() > 0      # the PDB rounding is visible
+    for lab, p, q in zip(_RECORDS, a, b):
+        if lab not in ("x", "v", "rhdr"):
+            assert p == q, lab
+    # rhdr differs only in lambda (restart vs none)
+    ra, rb = np.frombuffer(a[1], np.float64), np.frombuffer(b[1], np.float64)
+    assert np.array_equal(np.delete(ra, 4), np.delete(rb, 4)) and rb[4] == 0.0
+
+
 def test_tplsys_input_errors(tmp_path):
     """Missing or inconsistent EXE> INPUT data ends jp_host with status 3
     and a message, before any GPU use."""
@@ -233,6 +267,7 @@ def test_tplsys_input_errors(tmp_path):
         "TOPOLO": inp.replace("TOPOLO= sys.tpl", "TOPOLE= sys.tpl"),
         "CLUSTA": inp.replace("GEPSME= LMD", "GEPSME= AND"),
         "CRDFMT": inp.replace("CRDFMT= BINA", "CRDFMT= FORM"),
+        "same atom count": inp.replace("CRDFMT= BINA", "CRDFMT= PDB"),
         "XMINCL..ZMAXCL": "\n".join(l for l in inp.splitlines() if "ZMAXCL" not in l),
         "COORDI": inp.replace("sys.crd", "none.crd"),
     }
