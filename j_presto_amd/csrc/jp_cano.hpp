@@ -3,16 +3,19 @@
 //
 // One step after the forces:
 //   begin   v_b = v, x_o = x, f_c = 0, w = v - (dt/2) c g / m  -> m|w|^2
-//   drift   chi = sqrt(T_set / T_g(w)) (1 without SHAKE), v = (2 chi - 1) v
-//           - chi dt c g / m, x += dt v
-//   icslop rounds, each skipped once the system is done:
-//     shake   (rounds after the first: chi from w, v = (-c g / m + f_c)
-//             chi dt + v_b (2 chi - 1), x = x_o + dt v), then x_b = x,
-//             SHAKE(x; x_o), v = (x - x_o)/dt, w = (v_b + v)/2
-//     check   every group within fcstol T_set, or the last round: done;
-//             else f_c += (x - x_b)/dt^2, w = (dt/2)(-c g / m + f_c) + v_b
-// The sorted coordinates follow x in the shake kernel (in the drift kernel
-// for a system without SHAKE groups).
+//   round 1 drift: chi = sqrt(T_set / T_g(w)) (1 without SHAKE),
+//           v = (2 chi - 1) v - chi dt c g / m, x += dt v; then as below
+//   round r > 1, skipped once the system is done: the end of round r - 1
+//           (every group within fcstol T_set: done), else
+//           f_c += (x - x_b)/dt^2, chi from w' below, v = (-c g / m + f_c)
+//           chi dt + v_b (2 chi - 1), x = x_o + dt v; then as below
+//   every round: x_b = x, SHAKE(x; x_o), v = (x - x_o)/dt, w = (v_b + v)/2
+//           -> m|w|^2 for the end-of-round test, and the trial velocity of
+//           the next round w' = (dt/2)(-c g / m + f_c + (x - x_b)/dt^2) + v_b
+//           -> m|w'|^2, so that the next round needs no kernel of its own
+//           to reduce them
+// Each round is one launch (k_cano_shake); a system without SHAKE groups
+// takes only round 1's drift.  The sorted coordinates follow x.
 // T_g = (sum over group g of m|w|^2) 1e7 / (dof_g gascst) (caltmp.f90), with
 // one group per system (TEMPCO= NO) or per molecule kind and cluster
 // (TEMPCO= YES, caltmp2).  c = 4.184e-4 (md.f90:56), 1/m is md_run's ifxmass.
@@ -34,6 +37,7 @@
 
 struct CanoArgs {
     int natom, nunit, icslop, nab, nub;   // nab, nub: atom blocks, unit blocks
+    int ngrp;                  // temperature groups of all systems
     const int2 *ablk, *ublk;   // per block: [first, end) atoms (<= 256) or units (<= 128) of one system
     double fcstol;
     const int *atom_sys, *inum, *img;
@@ -47,7 +51,9 @@ struct CanoArgs {
     const double *gdof, *temps;
     const double *mass, *imass, *gslot;
     double *x, *v, *xold, *vback, *fcc, *xcb;
-    double *pa, *pu;           // partials of m|w|^2: [group][atom block], [group][unit block]
+    double *pa;                // partials of m|w|^2 of begin: [group][atom block]
+    double *pu, *pn;           // of w and w' per round, two buffers by round parity:
+                               // [round & 1][group][unit block]
     int *done;                 // per system: the SHAKE/temperature rounds have ended
 };
 
@@ -100,6 +106,32 @@ __device__ __forceinline__ void block_temps(const CanoArgs &A, const double *__r
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) p = p + __shfl_xor(p, o, 64);
         if (l == 0) sT[g - g0] = p * 1.0e7 / (A.gdof[g] * JP_GASCST);
+    }
+    __syncthreads();
+}
+
+// block_temps of two partial arrays in one pass: P into sT, P2 into sT2.
+__device__ __forceinline__ void block_temps2(const CanoArgs &A, const double *__restrict__ P,
+                                             const double *__restrict__ P2, int nblk, const int *__restrict__ bl0,
+                                             const int *__restrict__ bl1, int g0, int g1, double *sT, double *sT2) {
+#pragma clang fp contract(off)
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63, nw = blockDim.x >> 6;
+    for (int g = g0 + w; g < g1; g += nw) {
+        const int s = A.gsys[g];
+        double p = 0.0, p2 = 0.0;
+        for (int b = bl0[s] + l; b < bl1[s]; b += 64) {
+            p = p + P[(size_t)g * nblk + b];
+            p2 = p2 + P2[(size_t)g * nblk + b];
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            p = p + __shfl_xor(p, o, 64);
+            p2 = p2 + __shfl_xor(p2, o, 64);
+        }
+        if (l == 0) {
+            sT[g - g0] = p * 1.0e7 / (A.gdof[g] * JP_GASCST);
+            sT2[g - g0] = p2 * 1.0e7 / (A.gdof[g] * JP_GASCST);
+        }
     }
     __syncthreads();
 }
@@ -172,27 +204,55 @@ __global__ void __launch_bounds__(256) k_cano_drift(CanoArgs A, const SysP *__re
     if (!A.shk_on[s]) cano_sorted(P, A, a, xa, ys);
 }
 
-// One SHAKE round per unit (SHAKE group or free atom), as k_step's; rounds
-// after the first first apply the previous round's velocity update.
+// One round per unit (SHAKE group or free atom): round 1 first applies the
+// drift, later rounds the end-of-round test of the previous one and its
+// velocity update (verlethe.f90:396-404, 601-611, 646-657); then SHAKE as in
+// k_step.  A block holds units of one system, so the decisions below are
+// uniform over the block, and every block of a system reaches the same one
+// from the same fixed-order sums.
 __global__ void __launch_bounds__(128) k_cano_shake(CanoArgs A, const SysP *__restrict__ sp,
                                                     const int4 *__restrict__ unit, const double *__restrict__ s2d,
                                                     const double *__restrict__ s3d, double tol, int maxit,
-                                                    int update, JAtom *__restrict__ ys, int *__restrict__ err) {
+                                                    int round, JAtom *__restrict__ ys, int *__restrict__ err) {
 #pragma clang fp contract(off)
-    __shared__ double sT[JP_CANO_MAXG];
+    __shared__ double sT[JP_CANO_MAXG], sU[JP_CANO_MAXG];
+    __shared__ int s_run;
     const int2 B = A.ublk[blockIdx.x];
     const int u = B.x + threadIdx.x;
-    // a unit's atoms share its system
     int g0, g1;
     block_groups(A, unit[B.x].x, g0, g1);
-    if (update) block_temps(A, A.pa, A.nab, A.ab0, A.ab1, g0, g1, sT);
-    double val[3] = {0.0, 0.0, 0.0};
+    const int s = A.atom_sys[unit[B.x].x];
+    const size_t half = (size_t)A.ngrp * A.nub;
+    double *pu_out = A.pu + (size_t)(round & 1) * half, *pn_out = A.pn + (size_t)(round & 1) * half;
+    const double *pu_in = A.pu + (size_t)((round - 1) & 1) * half;
+    const double *pn_in = A.pn + (size_t)((round - 1) & 1) * half;
+    const double ts = A.temps[s];
+    if (round == 1) {
+        block_temps(A, A.pa, A.nab, A.ab0, A.ab1, g0, g1, sT);
+    } else {
+        if (threadIdx.x == 0) s_run = !A.done[s];
+        __syncthreads();
+        if (!s_run) return;   // uniform over the block
+        // sU: temperatures of w (end-of-round test), sT: of w' (the update)
+        block_temps2(A, pu_in, pn_in, A.nub, A.ub0, A.ub1, g0, g1, sU, sT);
+        double tmin = 0.0, tmax = 0.0;
+        for (int g = g0; g < g1; ++g) {
+            const double T = sU[g - g0];
+            tmin = (g == g0) ? T : fmin(tmin, T);
+            tmax = (g == g0) ? T : fmax(tmax, T);
+        }
+        if (fmax(fabs(tmin - ts), fabs(tmax - ts)) < A.fcstol * ts) {
+            if (threadIdx.x == 0 && blockIdx.x == A.ub0[s]) A.done[s] = 1;
+            return;
+        }
+    }
+    double val[3] = {0.0, 0.0, 0.0}, valn[3] = {0.0, 0.0, 0.0};
     int gq[3] = {-1, -1, -1};
     const int4 U = unit[min(u, B.y - 1)];
     const int kind = U.w & 3, prm = U.w >> 2;
     const int a[3] = {U.x, U.y, U.z};
-    const int s = A.atom_sys[a[0]];
-    if (u < B.y && !A.done[s]) {
+    const bool shk = A.shk_on[s] != 0;
+    if (u < B.y) {
         const SysP &P = sp[s];
         const double dt = P.dt;
         double xn[3][3], xo[3][3];
@@ -200,91 +260,76 @@ __global__ void __launch_bounds__(128) k_cano_shake(CanoArgs A, const SysP *__re
         for (int q = 0; q < 3; ++q) {
             if (q >= kind) break;
             const int b = a[q];
-            if (update) {   // verlethe.f90:646-657 of the previous round
-                const double itc = 1.0 / sT[A.gid[b] - g0];
-                const double chi = sqrt(A.temps[s] * itc);
-                const double rt = chi * dt, rt2 = 2.0 * chi - 1.0, im = A.imass[b];
-                const size_t sl = (size_t)A.inum[b];
+            const double im = A.imass[b];
+            const size_t sl = (size_t)A.inum[b];
+            if (round == 1) {   // verlethe.f90:396-404, 509-516
+                const double T = shk ? sT[A.gid[b] - g0] : ts;
+                const double itc = 1.0 / T;
+                const double chi = sqrt(ts * itc);
+                const double rt = chi * dt * JP_FDCVCV, rt2 = 2.0 * chi - 1.0;
                 for (int k = 0; k < 3; ++k) {
                     const int qq = 3 * b + k;
-                    const double vk = (-JP_FDCVCV * A.gslot[3 * sl + k] * im + A.fcc[qq]) * rt + A.vback[qq] * rt2;
+                    const double vk = rt2 * A.v[qq] - rt * A.gslot[3 * sl + k] * im;
+                    A.v[qq] = vk;
+                    A.x[qq] = A.x[qq] + dt * vk;
+                }
+            } else {   // verlethe.f90:611-657: constraint force, then the update
+                const double itc = 1.0 / sT[A.gid[b] - g0];
+                const double chi = sqrt(ts * itc);
+                const double rt = chi * dt, rt2 = 2.0 * chi - 1.0, r = 1.0 / (dt * dt);
+                for (int k = 0; k < 3; ++k) {
+                    const int qq = 3 * b + k;
+                    const double f = A.fcc[qq] + r * (A.x[qq] - A.xcb[qq]);
+                    A.fcc[qq] = f;
+                    const double vk = (-JP_FDCVCV * A.gslot[3 * sl + k] * im + f) * rt + A.vback[qq] * rt2;
                     A.v[qq] = vk;
                     A.x[qq] = A.xold[qq] + dt * vk;
                 }
             }
             for (int k = 0; k < 3; ++k) {
                 const double p = A.x[3 * b + k];
-                A.xcb[3 * b + k] = p;
                 xn[q][k] = p;
                 xo[q][k] = A.xold[3 * b + k];
+                if (shk) A.xcb[3 * b + k] = p;
             }
         }
-        if (kind == 3) {
-            const double im[3] = {A.imass[a[0]], A.imass[a[1]], A.imass[a[2]]};
-            const double T[3] = {s3d[6 * prm], s3d[6 * prm + 1], s3d[6 * prm + 2]};
-            const int e = shake3_regs(P, im, T, tol, maxit, xn, xo);
-            if (e) atomicOr(err, e);
-        } else if (kind == 2) {
-            shake2_regs(P, 1.0 / A.mass[a[0]], 1.0 / A.mass[a[1]], s2d[6 * prm], xn, xo);
+        if (shk) {
+            if (kind == 3) {
+                const double im[3] = {A.imass[a[0]], A.imass[a[1]], A.imass[a[2]]};
+                const double T[3] = {s3d[6 * prm], s3d[6 * prm + 1], s3d[6 * prm + 2]};
+                const int e = shake3_regs(P, im, T, tol, maxit, xn, xo);
+                if (e) atomicOr(err, e);
+            } else if (kind == 2) {
+                shake2_regs(P, 1.0 / A.mass[a[0]], 1.0 / A.mass[a[1]], s2d[6 * prm], xn, xo);
+            }
         }
-        const double r = 1.0 / dt;
+        const double r = 1.0 / dt, r2 = 1.0 / (dt * dt), r3 = 0.5 * dt;
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
             if (q >= kind) break;
             const int b = a[q];
-            double w[3];
-            for (int k = 0; k < 3; ++k) {
-                A.x[3 * b + k] = xn[q][k];
-                const double vk = (xn[q][k] - xo[q][k]) * r;
-                A.v[3 * b + k] = vk;
-                w[k] = (A.vback[3 * b + k] + vk) * 0.5;
+            if (shk) {
+                const double im = A.imass[b];
+                const size_t sl = (size_t)A.inum[b];
+                double w[3], wn[3];
+                for (int k = 0; k < 3; ++k) {
+                    const int qq = 3 * b + k;
+                    A.x[qq] = xn[q][k];
+                    const double vk = (xn[q][k] - xo[q][k]) * r;
+                    A.v[qq] = vk;
+                    w[k] = (A.vback[qq] + vk) * 0.5;
+                    const double f = A.fcc[qq] + r2 * (xn[q][k] - A.xcb[qq]);
+                    wn[k] = r3 * (-JP_FDCVCV * A.gslot[3 * sl + k] * im + f) + A.vback[qq];
+                }
+                val[q] = cano_mw2(A.mass[b], w[0], w[1], w[2]);
+                valn[q] = cano_mw2(A.mass[b], wn[0], wn[1], wn[2]);
+                gq[q] = A.gid[b];
             }
-            val[q] = cano_mw2(A.mass[b], w[0], w[1], w[2]);
-            gq[q] = A.gid[b];
             cano_sorted(P, A, b, xn[q], ys);
         }
     }
-    block_partials(val, gq, g0, g1, A.pu, A.nub);
-}
-
-// End of round `round` (1-based): every group of the system within fcstol of
-// its target, or the last round, ends the rounds (verlethe.f90:601-611);
-// otherwise the constraint force grows and w is the next trial velocity.
-__global__ void __launch_bounds__(256) k_cano_check(CanoArgs A, const SysP *__restrict__ sp, int round) {
-#pragma clang fp contract(off)
-    __shared__ double sT[JP_CANO_MAXG];
-    const int2 B = A.ablk[blockIdx.x];
-    const int a = B.x + threadIdx.x;
-    int g0, g1;
-    block_groups(A, B.x, g0, g1);
-    block_temps(A, A.pu, A.nub, A.ub0, A.ub1, g0, g1, sT);
-    double val[1] = {0.0};
-    int gq[1] = {-1};
-    const int s = A.atom_sys[B.x];
-    if (a < B.y && !A.done[s]) {
-        const double ts = A.temps[s];
-        double tmin = 0.0, tmax = 0.0;
-        for (int g = A.sg0[s]; g < A.sg1[s]; ++g) {
-            const double T = sT[g - g0];
-            tmin = (g == A.sg0[s]) ? T : fmin(tmin, T);
-            tmax = (g == A.sg0[s]) ? T : fmax(tmax, T);
-        }
-        if (fmax(fabs(tmin - ts), fabs(tmax - ts)) < A.fcstol * ts || round == A.icslop) {
-            if (a == A.sfirst[s]) A.done[s] = 1;
-        } else {
-            const double dt = sp[s].dt, im = A.imass[a];
-            const double r = 1.0 / (dt * dt), r2 = 0.5 * dt;
-            const size_t sl = (size_t)A.inum[a];
-            double w[3];
-            for (int k = 0; k < 3; ++k) {
-                const int q = 3 * a + k;
-                const double f = A.fcc[q] + r * (A.x[q] - A.xcb[q]);
-                A.fcc[q] = f;
-                w[k] = r2 * (-JP_FDCVCV * A.gslot[3 * sl + k] * im + f) + A.vback[q];
-            }
-            val[0] = cano_mw2(A.mass[a], w[0], w[1], w[2]);
-            gq[0] = A.gid[a];
-        }
+    if (shk && round < A.icslop) {
+        block_partials(val, gq, g0, g1, pu_out, A.nub);
+        block_partials(valn, gq, g0, g1, pn_out, A.nub);
     }
-    if (round < A.icslop) block_partials(val, gq, g0, g1, A.pa, A.nab);
 }

@@ -132,7 +132,7 @@ struct Engine {
     double cano_fcstol = 1e-3;
     DBuf<int> d_gid, d_gsys, d_sg0, d_sg1, d_ab0, d_ab1, d_ub0, d_ub1, d_sfirst, d_shkon, d_cdone;
     DBuf<int2> d_ablk, d_ublk;
-    DBuf<double> d_gdof, d_ctemps, d_fcc, d_xcb, d_pa, d_pu;
+    DBuf<double> d_gdof, d_ctemps, d_fcc, d_xcb, d_pa, d_pu, d_pn;
     int set_canonical(int on, double temps, int icslop, double fcstol, const int *group);
     CanoArgs cano_args();
     int cano_step();
@@ -187,7 +187,7 @@ struct Engine {
         d_bndp.release(); d_angp.release(); d_eterm.release(); d_unit.release();
         d_gid.release(); d_gsys.release(); d_sg0.release(); d_sg1.release(); d_ab0.release(); d_ab1.release();
         d_ub0.release(); d_ub1.release(); d_ablk.release(); d_ublk.release(); d_sfirst.release(); d_shkon.release(); d_cdone.release();
-        d_gdof.release(); d_ctemps.release(); d_fcc.release(); d_xcb.release(); d_pa.release(); d_pu.release();
+        d_gdof.release(); d_ctemps.release(); d_fcc.release(); d_xcb.release(); d_pa.release(); d_pu.release(); d_pn.release();
         d_torc.release(); d_tor.release(); d_p14.release(); d_p14p.release(); d_torp.release();
         d_abeg.release(); d_aend.release(); d_cbeg.release(); d_cend.release(); d_rsum.release(); d_rfix.release();
         d_part.release(); d_partf.release();
@@ -1166,7 +1166,7 @@ int Engine::set_canonical(int on, double temps, int icslop, double fcstol, const
     AL(d_sg0, nsys); AL(d_sg1, nsys); AL(d_ab0, nsys); AL(d_ab1, nsys); AL(d_ub0, nsys); AL(d_ub1, nsys);
     AL(d_sfirst, nsys); AL(d_shkon, nsys); AL(d_cdone, nsys); AL(d_ctemps, nsys);
     AL(d_fcc, 3 * (size_t)natom); AL(d_xcb, 3 * (size_t)natom);
-    AL(d_pa, (size_t)ngrp * nab); AL(d_pu, (size_t)ngrp * nub); AL(d_ablk, nab); AL(d_ublk, nub);
+    AL(d_pa, (size_t)ngrp * nab); AL(d_pu, 2 * (size_t)ngrp * nub); AL(d_pn, 2 * (size_t)ngrp * nub); AL(d_ablk, nab); AL(d_ublk, nub);
     auto H2D = [&](void *dst, const void *src, size_t bytes) -> int {
         if (bytes) CK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st));
         return 0;
@@ -1196,7 +1196,8 @@ int Engine::set_canonical(int on, double temps, int icslop, double fcstol, const
 
 CanoArgs Engine::cano_args() {
     CanoArgs A;
-    A.natom = natom; A.nunit = nunit; A.icslop = cano_icslop; A.nab = nab; A.nub = nub; A.fcstol = cano_fcstol;
+    A.natom = natom; A.nunit = nunit; A.icslop = cano_icslop; A.nab = nab; A.nub = nub; A.ngrp = ngrp;
+    A.fcstol = cano_fcstol;
     A.atom_sys = d_atom_sys.p; A.inum = d_inum.p; A.img = d_img.p;
     A.gid = d_gid.p; A.gsys = d_gsys.p; A.sg0 = d_sg0.p; A.sg1 = d_sg1.p;
     A.ab0 = d_ab0.p; A.ab1 = d_ab1.p; A.ub0 = d_ub0.p; A.ub1 = d_ub1.p; A.sfirst = d_sfirst.p;
@@ -1204,24 +1205,23 @@ CanoArgs Engine::cano_args() {
     A.shk_on = d_shkon.p; A.gdof = d_gdof.p; A.temps = d_ctemps.p;
     A.mass = d_mass.p; A.imass = d_imass.p; A.gslot = d_gslot.p;
     A.x = d_x.p; A.v = d_v.p; A.xold = d_xold.p; A.vback = d_vback.p; A.fcc = d_fcc.p; A.xcb = d_xcb.p;
-    A.pa = d_pa.p; A.pu = d_pu.p; A.done = d_cdone.p;
+    A.pa = d_pa.p; A.pu = d_pu.p; A.pn = d_pn.p; A.done = d_cdone.p;
     return A;
 }
 
-// One METHOD= CANO step after the forces (jp_cano.hpp): 2 + 2 * icslop
-// launches; the rounds of a system that has ended are skipped on the device,
-// the last round's check always ends them.
+// One METHOD= CANO step after the forces (jp_cano.hpp): 1 + icslop launches
+// (2 without SHAKE groups); the rounds of a system that has ended are skipped
+// on the device, and there is none after the last.
 int Engine::cano_step() {
     const CanoArgs A = cano_args();
     const unsigned ga = (unsigned)nab;
     hipLaunchKernelGGL(k_cano_begin, ga, 256, 0, st, A, d_sp.p);
-    hipLaunchKernelGGL(k_cano_drift, ga, 256, 0, st, A, d_sp.p, d_ys.p);
     if (n2 + n3 > 0) {
-        for (int round = 1; round <= cano_icslop; ++round) {
+        for (int round = 1; round <= cano_icslop; ++round)
             hipLaunchKernelGGL(k_cano_shake, (unsigned)nub, 128, 0, st, A, d_sp.p, d_unit.p, d_s2d.p, d_s3d.p,
-                               shake_tol, shake_maxit, round > 1 ? 1 : 0, d_ys.p, d_err.p);
-            hipLaunchKernelGGL(k_cano_check, ga, 256, 0, st, A, d_sp.p, round);
-        }
+                               shake_tol, shake_maxit, round, d_ys.p, d_err.p);
+    } else {
+        hipLaunchKernelGGL(k_cano_drift, ga, 256, 0, st, A, d_sp.p, d_ys.p);
     }
     CK(hipGetLastError());
     return 0;

@@ -117,3 +117,42 @@ def test_cano_off_returns_to_micr():
         a.set_canonical(-1.0)
     a.close()
     b.close()
+
+
+def _no_shake(s):
+    for m in s.mols:
+        m.shake = []
+    return s
+
+
+def test_cano_without_shake_groups():
+    """A system without SHAKE groups takes chi = 1 (verlethe.f90:396-404)
+    and no rounds: FP64 against the oracle within 1e-9 after 10 steps, alone
+    (begin + drift kernels) and in a batch with a SHAKE system (the drift
+    inside the first round's kernel), the batch member equal to the single
+    run bit for bit."""
+    s = _no_shake(systems.water_box(200, 24.2, 44))
+    assert not s.shake_groups()
+    o = Oracle(s)
+    x, v = s.x.copy(), s.v.copy()
+    assert o.celprp(x) == 0
+    for idloop in range(1, 11):
+        g, _ = o.bonded(x)
+        g, _, _ = o.nonbond(x, g)
+        assert o.cano_step(x, v, o.combine(g), 300.0, 3, 1e-3, None)[0] == 0
+        if idloop % 10 == 0:
+            assert o.celprp(x) == 0
+    md = engine.MdSystem(s, precision=engine.FP64)
+    md.set_canonical(300.0)
+    _, ier = md.runmd(10, update=10)
+    assert ier == 0
+    xs, vs = md.get_state()
+    md.close()
+    assert np.abs(xs - x).max() < 1e-9 and np.abs(vs - v).max() < 1e-9
+    ens = engine.Ensemble([systems.water_box(200, 24.2, 45), s], precision=engine.FP64)
+    ens.set_canonical(300.0)
+    _, ier = ens.runmd(10, update=10)
+    assert ier == 0
+    xe, ve = ens.get_state(1)
+    ens.close()
+    assert np.array_equal(xe, xs) and np.array_equal(ve, vs)
