@@ -227,6 +227,31 @@ __global__ void __launch_bounds__(128) k_cano_shake(CanoArgs A, const SysP *__re
     const double *pu_in = A.pu + (size_t)((round - 1) & 1) * half;
     const double *pn_in = A.pn + (size_t)((round - 1) & 1) * half;
     const double ts = A.temps[s];
+    // the unit's atom data, loaded before the group sums so that both
+    // memory round trips overlap: x, x_o, g, f_c, v_b, and v (round 1) or
+    // x_b (later rounds)
+    const int4 U = unit[min(u, B.y - 1)];
+    const int kind = U.w & 3, prm = U.w >> 2;
+    const int a[3] = {U.x, U.y, U.z};
+    const bool live = u < B.y;
+    double ax[3][3], axo[3][3], ag[3][3], afc[3][3], avb[3][3], avx[3][3], aim[3];
+    int agid[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        const int b = (live && q < kind) ? a[q] : a[0];
+        const size_t sl = (size_t)A.inum[b];
+        aim[q] = A.imass[b];
+        agid[q] = A.gid[b];
+        for (int k = 0; k < 3; ++k) {
+            const int qq = 3 * b + k;
+            ax[q][k] = A.x[qq];
+            axo[q][k] = A.xold[qq];
+            ag[q][k] = A.gslot[3 * sl + k];
+            afc[q][k] = A.fcc[qq];
+            avb[q][k] = A.vback[qq];
+            avx[q][k] = round == 1 ? A.v[qq] : A.xcb[qq];
+        }
+    }
     if (round == 1) {
         block_temps(A, A.pa, A.nab, A.ab0, A.ab1, g0, g1, sT);
     } else {
@@ -248,56 +273,48 @@ __global__ void __launch_bounds__(128) k_cano_shake(CanoArgs A, const SysP *__re
     }
     double val[3] = {0.0, 0.0, 0.0}, valn[3] = {0.0, 0.0, 0.0};
     int gq[3] = {-1, -1, -1};
-    const int4 U = unit[min(u, B.y - 1)];
-    const int kind = U.w & 3, prm = U.w >> 2;
-    const int a[3] = {U.x, U.y, U.z};
     const bool shk = A.shk_on[s] != 0;
-    if (u < B.y) {
+    if (live) {
         const SysP &P = sp[s];
         const double dt = P.dt;
-        double xn[3][3], xo[3][3];
+        double xn[3][3], xo[3][3], xb[3][3], vd[3][3];
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
             if (q >= kind) break;
-            const int b = a[q];
-            const double im = A.imass[b];
-            const size_t sl = (size_t)A.inum[b];
+            const double im = aim[q];
             if (round == 1) {   // verlethe.f90:396-404, 509-516
-                const double T = shk ? sT[A.gid[b] - g0] : ts;
+                const double T = shk ? sT[agid[q] - g0] : ts;
                 const double itc = 1.0 / T;
                 const double chi = sqrt(ts * itc);
                 const double rt = chi * dt * JP_FDCVCV, rt2 = 2.0 * chi - 1.0;
                 for (int k = 0; k < 3; ++k) {
-                    const int qq = 3 * b + k;
-                    const double vk = rt2 * A.v[qq] - rt * A.gslot[3 * sl + k] * im;
-                    A.v[qq] = vk;
-                    A.x[qq] = A.x[qq] + dt * vk;
+                    const double vk = rt2 * avx[q][k] - rt * ag[q][k] * im;
+                    vd[q][k] = vk;
+                    xn[q][k] = ax[q][k] + dt * vk;
                 }
             } else {   // verlethe.f90:611-657: constraint force, then the update
-                const double itc = 1.0 / sT[A.gid[b] - g0];
+                const double itc = 1.0 / sT[agid[q] - g0];
                 const double chi = sqrt(ts * itc);
                 const double rt = chi * dt, rt2 = 2.0 * chi - 1.0, r = 1.0 / (dt * dt);
                 for (int k = 0; k < 3; ++k) {
-                    const int qq = 3 * b + k;
-                    const double f = A.fcc[qq] + r * (A.x[qq] - A.xcb[qq]);
-                    A.fcc[qq] = f;
-                    const double vk = (-JP_FDCVCV * A.gslot[3 * sl + k] * im + f) * rt + A.vback[qq] * rt2;
-                    A.v[qq] = vk;
-                    A.x[qq] = A.xold[qq] + dt * vk;
+                    const double f = afc[q][k] + r * (ax[q][k] - avx[q][k]);
+                    afc[q][k] = f;
+                    A.fcc[3 * a[q] + k] = f;
+                    const double vk = (-JP_FDCVCV * ag[q][k] * im + f) * rt + avb[q][k] * rt2;
+                    vd[q][k] = vk;
+                    xn[q][k] = axo[q][k] + dt * vk;
                 }
             }
             for (int k = 0; k < 3; ++k) {
-                const double p = A.x[3 * b + k];
-                xn[q][k] = p;
-                xo[q][k] = A.xold[3 * b + k];
-                if (shk) A.xcb[3 * b + k] = p;
+                xb[q][k] = xn[q][k];
+                xo[q][k] = axo[q][k];
+                if (shk) A.xcb[3 * a[q] + k] = xn[q][k];
             }
         }
         if (shk) {
             if (kind == 3) {
-                const double im[3] = {A.imass[a[0]], A.imass[a[1]], A.imass[a[2]]};
                 const double T[3] = {s3d[6 * prm], s3d[6 * prm + 1], s3d[6 * prm + 2]};
-                const int e = shake3_regs(P, im, T, tol, maxit, xn, xo);
+                const int e = shake3_regs(P, aim, T, tol, maxit, xn, xo);
                 if (e) atomicOr(err, e);
             } else if (kind == 2) {
                 shake2_regs(P, 1.0 / A.mass[a[0]], 1.0 / A.mass[a[1]], s2d[6 * prm], xn, xo);
@@ -309,21 +326,24 @@ __global__ void __launch_bounds__(128) k_cano_shake(CanoArgs A, const SysP *__re
             if (q >= kind) break;
             const int b = a[q];
             if (shk) {
-                const double im = A.imass[b];
-                const size_t sl = (size_t)A.inum[b];
                 double w[3], wn[3];
                 for (int k = 0; k < 3; ++k) {
                     const int qq = 3 * b + k;
                     A.x[qq] = xn[q][k];
                     const double vk = (xn[q][k] - xo[q][k]) * r;
                     A.v[qq] = vk;
-                    w[k] = (A.vback[qq] + vk) * 0.5;
-                    const double f = A.fcc[qq] + r2 * (xn[q][k] - A.xcb[qq]);
-                    wn[k] = r3 * (-JP_FDCVCV * A.gslot[3 * sl + k] * im + f) + A.vback[qq];
+                    w[k] = (avb[q][k] + vk) * 0.5;
+                    const double f = afc[q][k] + r2 * (xn[q][k] - xb[q][k]);
+                    wn[k] = r3 * (-JP_FDCVCV * ag[q][k] * aim[q] + f) + avb[q][k];
                 }
                 val[q] = cano_mw2(A.mass[b], w[0], w[1], w[2]);
                 valn[q] = cano_mw2(A.mass[b], wn[0], wn[1], wn[2]);
-                gq[q] = A.gid[b];
+                gq[q] = agid[q];
+            } else {
+                for (int k = 0; k < 3; ++k) {
+                    A.x[3 * b + k] = xn[q][k];
+                    A.v[3 * b + k] = vd[q][k];
+                }
             }
             cano_sorted(P, A, b, xn[q], ys);
         }
