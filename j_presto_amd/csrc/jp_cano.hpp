@@ -234,14 +234,17 @@ __global__ void __launch_bounds__(128) k_cano_shake(CanoArgs A, const SysP *__re
     const int kind = U.w & 3, prm = U.w >> 2;
     const int a[3] = {U.x, U.y, U.z};
     const bool live = u < B.y;
-    double ax[3][3], axo[3][3], ag[3][3], afc[3][3], avb[3][3], avx[3][3], aim[3];
-    int agid[3];
+    double ax[3][3], axo[3][3], ag[3][3], afc[3][3], avb[3][3], avx[3][3], aim[3], am[3];
+    int agid[3], asl[3], aimg[3][3];
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
         const int b = (live && q < kind) ? a[q] : a[0];
         const size_t sl = (size_t)A.inum[b];
+        asl[q] = (int)sl;
         aim[q] = A.imass[b];
+        am[q] = A.mass[b];
         agid[q] = A.gid[b];
+        for (int k = 0; k < 3; ++k) aimg[q][k] = A.img[3 * b + k];
         for (int k = 0; k < 3; ++k) {
             const int qq = 3 * b + k;
             ax[q][k] = A.x[qq];
@@ -270,6 +273,12 @@ __global__ void __launch_bounds__(128) k_cano_shake(CanoArgs A, const SysP *__re
             if (threadIdx.x == 0 && blockIdx.x == A.ub0[s]) A.done[s] = 1;
             return;
         }
+    }
+    double T3[3] = {0.0, 0.0, 0.0};
+    if (kind == 3) {
+        T3[0] = s3d[6 * prm]; T3[1] = s3d[6 * prm + 1]; T3[2] = s3d[6 * prm + 2];
+    } else if (kind == 2) {
+        T3[0] = s2d[6 * prm];
     }
     double val[3] = {0.0, 0.0, 0.0}, valn[3] = {0.0, 0.0, 0.0};
     int gq[3] = {-1, -1, -1};
@@ -313,11 +322,10 @@ __global__ void __launch_bounds__(128) k_cano_shake(CanoArgs A, const SysP *__re
         }
         if (shk) {
             if (kind == 3) {
-                const double T[3] = {s3d[6 * prm], s3d[6 * prm + 1], s3d[6 * prm + 2]};
-                const int e = shake3_regs(P, aim, T, tol, maxit, xn, xo);
+                const int e = shake3_regs(P, aim, T3, tol, maxit, xn, xo);
                 if (e) atomicOr(err, e);
             } else if (kind == 2) {
-                shake2_regs(P, 1.0 / A.mass[a[0]], 1.0 / A.mass[a[1]], s2d[6 * prm], xn, xo);
+                shake2_regs(P, 1.0 / am[0], 1.0 / am[1], T3[0], xn, xo);
             }
         }
         const double r = 1.0 / dt, r2 = 1.0 / (dt * dt), r3 = 0.5 * dt;
@@ -336,8 +344,8 @@ __global__ void __launch_bounds__(128) k_cano_shake(CanoArgs A, const SysP *__re
                     const double f = afc[q][k] + r2 * (xn[q][k] - xb[q][k]);
                     wn[k] = r3 * (-JP_FDCVCV * ag[q][k] * aim[q] + f) + avb[q][k];
                 }
-                val[q] = cano_mw2(A.mass[b], w[0], w[1], w[2]);
-                valn[q] = cano_mw2(A.mass[b], wn[0], wn[1], wn[2]);
+                val[q] = cano_mw2(am[q], w[0], w[1], w[2]);
+                valn[q] = cano_mw2(am[q], wn[0], wn[1], wn[2]);
                 gq[q] = agid[q];
             } else {
                 for (int k = 0; k < 3; ++k) {
@@ -345,7 +353,10 @@ __global__ void __launch_bounds__(128) k_cano_shake(CanoArgs A, const SysP *__re
                     A.v[3 * b + k] = vd[q][k];
                 }
             }
-            cano_sorted(P, A, b, xn[q], ys);
+            JAtom *j = ys + asl[q];   // as cano_sorted, from the preloaded slot and image
+            j->x = xn[q][0] - P.L[0] * (double)aimg[q][0];
+            j->y = xn[q][1] - P.L[1] * (double)aimg[q][1];
+            j->z = xn[q][2] - P.L[2] * (double)aimg[q][2];
         }
     }
     if (shk && round < A.icslop) {

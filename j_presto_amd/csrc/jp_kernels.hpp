@@ -507,12 +507,29 @@ __global__ void __launch_bounds__(128) k_step(int nunit, const int4 *__restrict_
     const SysP &P = sp[atom_sys[a[0]]];
     const double rt = -P.dt * 4.184e-4;   // md.f90:56 fdcvcv
     double xn[3][3], xo[3][3], vv[3][3];
+    // what SHAKE and the sorted copy need, loaded up front so that these
+    // loads overlap the first ones instead of following the SHAKE loop
+    int sl3[3] = {0, 0, 0}, im3[3][3];
+    double m3[3] = {1.0, 1.0, 1.0}, iw[3] = {0.0, 0.0, 0.0}, T3[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        const int b = q < kind ? a[q] : a[0];
+        sl3[q] = inum[b];
+        m3[q] = mass[b];
+        for (int k = 0; k < 3; ++k) im3[q][k] = img[3 * b + k];
+    }
+    if (kind == 3) {
+        iw[0] = imass[a[0]]; iw[1] = imass[a[1]]; iw[2] = imass[a[2]];
+        T3[0] = s3d[6 * prm]; T3[1] = s3d[6 * prm + 1]; T3[2] = s3d[6 * prm + 2];
+    } else if (kind == 2) {
+        T3[0] = s2d[6 * prm];
+    }
 #pragma unroll
     for (int q = 0; q < 3; ++q) {   // fixed trip count: the arrays stay in registers
         if (q >= kind) break;
         const int b = a[q];
-        const size_t sl = inum[b];
-        const double m = mass[b];
+        const size_t sl = sl3[q];
+        const double m = m3[q];
         for (int k = 0; k < 3; ++k) {
             const double g = gslot[3 * sl + k];
             double vk = v[3 * b + k];
@@ -525,26 +542,24 @@ __global__ void __launch_bounds__(128) k_step(int nunit, const int4 *__restrict_
         }
     }
     if (kind == 3) {
-        const double im[3] = {imass[a[0]], imass[a[1]], imass[a[2]]};
-        const double T[3] = {s3d[6 * prm], s3d[6 * prm + 1], s3d[6 * prm + 2]};
-        const int e = shake3_regs(P, im, T, tol, maxit, xn, xo);
+        const int e = shake3_regs(P, iw, T3, tol, maxit, xn, xo);
         if (e) atomicOr(err, e);
     } else if (kind == 2) {
-        shake2_regs(P, 1.0 / mass[a[0]], 1.0 / mass[a[1]], s2d[6 * prm], xn, xo);
+        shake2_regs(P, 1.0 / m3[0], 1.0 / m3[1], T3[0], xn, xo);
     }
     const double r = 1.0 / P.dt;
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
         if (q >= kind) break;
         const int b = a[q];
-        JAtom *j = ys + inum[b];
+        JAtom *j = ys + sl3[q];
         for (int k = 0; k < 3; ++k) {
             x[3 * b + k] = xn[q][k];
             v[3 * b + k] = shake_on ? (xn[q][k] - xo[q][k]) * r : vv[q][k];
         }
-        j->x = xn[q][0] - P.L[0] * (double)img[3 * b];
-        j->y = xn[q][1] - P.L[1] * (double)img[3 * b + 1];
-        j->z = xn[q][2] - P.L[2] * (double)img[3 * b + 2];
+        j->x = xn[q][0] - P.L[0] * (double)im3[q][0];
+        j->y = xn[q][1] - P.L[1] * (double)im3[q][1];
+        j->z = xn[q][2] - P.L[2] * (double)im3[q][2];
     }
 }
 
