@@ -307,6 +307,7 @@ contains
           if (q == 0) q = len_trim(rest) + 1
           val = rest(1:q - 1)
           rest = rest(q:)
+          ios = 0
           if (inin) then
              select case (trim(key))
              case ('TOPOLO'); tplf = val
@@ -320,32 +321,33 @@ contains
                    write(*,'(2a)') 'jp_host: CLUSTA must read a<b: ', trim(val)
                    stop 3
                 end if
-                read(val(1:q - 1), *) cla
-                read(val(q + 1:), *) clb
+                read(val(1:q - 1), *, iostat=ios) cla
+                read(val(q + 1:), *, iostat=ios) clb
              end select
+             if (ios /= 0) call bad_value(key, val)
              cycle
           end if
           select case (trim(key))
-          case ('LOOPLI'); read(val, *) nsteps
-          case ('UPDATE'); read(val, *) update
-          case ('TIMEST'); read(val, *) timest
-          case ('OUTLOG'); read(val, *) nmon
-          case ('SETTEM'); read(val, *) rep; settem = sign(rep, settem)
+          case ('LOOPLI'); read(val, *, iostat=ios) nsteps
+          case ('UPDATE'); read(val, *, iostat=ios) update
+          case ('TIMEST'); read(val, *, iostat=ios) timest
+          case ('OUTLOG'); read(val, *, iostat=ios) nmon
+          case ('SETTEM'); read(val, *, iostat=ios) rep; settem = sign(rep, settem)
           case ('RESTAR'); restar = val(1:3) == 'YES'
           case ('NAMERI'); rstin = val
           case ('NAMERO'); rstout = val
           case ('NAMETO'); etof = val
           case ('TEMPCO'); tempco = val(1:3) == 'YES'
-          case ('CUTLEN'); read(val, *) cutlen
-          case ('DIEVAL'); read(val, *) dieval
-          case ('COVSHK'); read(val, *) covshk
-          case ('LIMSHK'); read(val, *) limshk
-          case ('XMINCL'); read(val, *) box(1)
-          case ('XMAXCL'); read(val, *) box(2)
-          case ('YMINCL'); read(val, *) box(3)
-          case ('YMAXCL'); read(val, *) box(4)
-          case ('ZMINCL'); read(val, *) box(5)
-          case ('ZMAXCL'); read(val, *) box(6)
+          case ('CUTLEN'); read(val, *, iostat=ios) cutlen
+          case ('DIEVAL'); read(val, *, iostat=ios) dieval
+          case ('COVSHK'); read(val, *, iostat=ios) covshk
+          case ('LIMSHK'); read(val, *, iostat=ios) limshk
+          case ('XMINCL'); read(val, *, iostat=ios) box(1)
+          case ('XMAXCL'); read(val, *, iostat=ios) box(2)
+          case ('YMINCL'); read(val, *, iostat=ios) box(3)
+          case ('YMAXCL'); read(val, *, iostat=ios) box(4)
+          case ('ZMINCL'); read(val, *, iostat=ios) box(5)
+          case ('ZMAXCL'); read(val, *, iostat=ios) box(6)
           case ('METHOD')
              have_method = .true.
              if (val(1:4) == 'CANO') then
@@ -357,6 +359,7 @@ contains
                 stop 3
              end if
           end select
+          if (ios /= 0) call bad_value(key, val)
        end do
     end do
     close(uu)
@@ -370,6 +373,12 @@ contains
        stop 3
     end if
   end subroutine read_mdinp
+
+  subroutine bad_value(key, val)
+    character(len=*), intent(in) :: key, val
+    write(*,'(4a)') 'jp_host: cannot read the value of ', trim(key), ': ', trim(val)
+    stop 3
+  end subroutine bad_value
 
   ! md_run restart file (outrst.f90:56-61): title, natom nvar, loop time
   ! etot ekin epot, cord(3,natom), vel(3,nvar), lambda lambda_v

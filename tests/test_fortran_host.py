@@ -268,6 +268,8 @@ def test_tplsys_input_errors(tmp_path):
         "CLUSTA": inp.replace("GEPSME= LMD", "GEPSME= AND"),
         "CRDFMT": inp.replace("CRDFMT= BINA", "CRDFMT= FORM"),
         "same atom count": inp.replace("CRDFMT= BINA", "CRDFMT= PDB"),
+        "value of LOOPLI": inp.replace("LOOPLI= 5", "LOOPLI= five"),
+        "value of CUTLEN": inp.replace("CUTLEN= 12.0", "CUTLEN= twelve"),
         "XMINCL..ZMAXCL": "\n".join(l for l in inp.splitlines() if "ZMAXCL" not in l),
         "COORDI": inp.replace("sys.crd", "none.crd"),
     }
