@@ -11,7 +11,8 @@
  *   - Return value: md_run's `ier` code for the routine the entry point
  *     replaces (0 ok; 28 cell overflow, 51 list overflow, 99 excluded pair not
  *     in the list, 1 SHAKE not converged, -2 SHAKE singular).  Values <= -100
- *     are runtime errors (-100 - hipError_t, or JP_EINVAL / JP_ENOMEM).
+ *     are runtime errors: JP_EINVAL / JP_ENOMEM / JP_ENODEV (-101..-103), or
+ *     JP_EHIP_BASE - hipError_t = -1000 - hipError_t for a failed HIP call.
  *   - A context owns every device buffer and one HIP stream; the caller owns
  *     every host buffer.  One context per host thread; calls on one context
  *     are not re-entrant.
@@ -173,12 +174,23 @@ int jp_bonded(jp_ctx *ctx, const double *cord, double *grad, double *eterm);
 int jp_shake(jp_ctx *ctx, double *cord, const double *cord_old);
 
 /* runmd/verlet for METHOD=MICR: nsteps leap-frog steps on the device state
- * (pair forces, bonded terms, kick, drift, SHAKE, v=(x-x_old)/dt), list rebuilt every `update`
- * steps, energies recorded every `nmon` steps into log[nsteps/nmon].
+ * (pair forces, bonded terms, kick, drift, SHAKE, v=(x-x_old)/dt).  The
+ * context counts steps like md_run's idloop (from 0, or from jp_set_step's
+ * value, e.g. a restart's loop number): the list is rebuilt after every step
+ * whose idloop is a multiple of `update` (md.f90:655-660), and energies are
+ * recorded at every step whose idloop is a multiple of `nmon` (OUTLOG,
+ * md.f90:604-633).  log has room for nsteps/nmon + 1 records (nmon > 0); the
+ * recorded steps come first, in order, and the unused records get step = -1.
  * (md.f90:519-694, verlet.f90:2-226).  After jp_set_canonical the step is
  * verlethe's METHOD=CANO step instead (verlethe.f90:79-757). */
 int jp_run_nve(jp_ctx *ctx, int32_t nsteps, int32_t update, int32_t nmon, int32_t precision,
                jp_energy_log *log);
+
+/* md_run's step counter idloop: inirst sets it from the restart file's loop
+ * number (inprst call, md.f90:1132), so that list rebuilds and OUTLOG steps of a
+ * continued run fall where md_run's do.  A new context starts at 0. */
+int jp_set_step(jp_ctx *ctx, int64_t idloop);
+int jp_get_step(jp_ctx *ctx, int64_t *idloop);
 
 /* Seconds spent in the last jp_run_nve, measured with HIP events on the
  * context stream: total, pair kernel, list build, the rest (SHAKE,
@@ -225,7 +237,8 @@ int jp_count_pairs(jp_ctx *ctx, int64_t *npairs, int64_t *nlj, int64_t *nlisted)
 typedef struct jp_batch jp_batch;
 int jp_batch_create(const jp_sys_desc *descs, int32_t nsys, jp_batch **out);
 int jp_batch_run_nve(jp_batch *b, int32_t nsteps, int32_t update, int32_t nmon, int32_t precision,
-                     jp_energy_log *log /* [nsys][nsteps/nmon] */);
+                     jp_energy_log *log /* [nsys][nsteps/nmon + 1] */);
+int jp_batch_set_step(jp_batch *b, int64_t idloop);   /* jp_set_step for the batch's shared counter */
 int jp_batch_set_state(jp_batch *b, int32_t isys, const double *cord, const double *vel);
 int jp_batch_get_state(jp_batch *b, int32_t isys, double *cord, double *vel);
 int jp_batch_last_timing(jp_batch *b, double *t_total);

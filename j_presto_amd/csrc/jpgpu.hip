@@ -83,7 +83,8 @@ struct Engine {
     std::vector<int> atom_sys_h, slot_sys_h, cell_sys_h, clu_sys_h;
     std::vector<double> mass_h;
     hipStream_t st = nullptr;
-    hipEvent_t ev[8];
+    hipEvent_t ev[8];               // [0,1] run, [2,3] list build, [4,5] host-launched pair kernels
+    bool pe_pending = false;        // ev[4,5] bracket a pair launch not yet added to t_pair
     double t_total = 0, t_pair = 0, t_list = 0, t_step = 0;
     int n_pair = 0, n_list = 0, n_pair_steps = 0;   // timed pair launches, steps of the last run
     long long idloop = 0;          // md.f90 idloop: steps done, kept across run calls
@@ -1286,8 +1287,9 @@ int Engine::mon_record(int step, int nsteps, int nmon, int nlog, const std::vect
     if (err & 4) return JP_IER_SHAKE_CONV;
     for (int s = 0; s < nsys; ++s) {
         double e = h_rsum[9 * (size_t)nsys + s];
-        e = e * 1.0e7 / 4.184 * 0.0005;   // caltmp.f90:24-27
-        jp_energy_log &L = log[(size_t)s * (nsteps / nmon) + nlog];
+        // caltmp.f90:24-43; its factor 0.0005 is a single-precision literal
+        e = e * 1.0e7 / 4.184 * (double)0.0005f;
+        jp_energy_log &L = log[(size_t)s * (nsteps / nmon + 1) + nlog];
         L.step = step;
         L.ekin = e;
         double ep = 0.0;
@@ -1325,8 +1327,11 @@ int Engine::run_nve(int nsteps, int update, int nmon, int prec, jp_energy_log *l
     if (autotune && !tuned && (r = tune(prec))) return r;
     CK(hipMemsetAsync(d_err.p, 0, 4, st));
     t_pair = 0; t_step = 0; n_pair = 0; n_pair_steps = nsteps;
+    pe_pending = false;
     const double tl0 = t_list;
-    std::vector<hipEvent_t> pe;
+    // records of the monitoring steps first, the rest marked unused
+    if (nmon > 0 && log)
+        for (size_t q = 0; q < (size_t)nsys * (nsteps / nmon + 1); ++q) log[q].step = -1;
     CK(hipEventRecord(ev[0], st));
     int nlog = 0;
     int step = 1;
@@ -1336,13 +1341,15 @@ int Engine::run_nve(int nsteps, int update, int nmon, int prec, jp_energy_log *l
     // the segment (or returns md_run's ier from the state it refers to).
     bool pending = false;
     while (step <= nsteps) {
-        const bool mon = nmon > 0 && step % nmon == 0;
+        // md_run monitors at idloop multiples of OUTLOG, idloop counting from
+        // the restart's loop number (md.f90:604-633)
+        const bool mon = nmon > 0 && (idloop + 1) % nmon == 0;
         // plain steps up to the next rebuild, energy step or the end
         int nplain = 0;
         if (!mon && use_graphs) {
             const int to_rebuild = (int)(update - idloop % update);
             int lim = std::min(nsteps - step + 1, to_rebuild);
-            if (nmon > 0) lim = std::min(lim, nmon - step % nmon);   // stop before the energy step
+            if (nmon > 0) lim = std::min(lim, (int)(nmon - (idloop + 1) % nmon));   // stop before the energy step
             nplain = lim;
         }
         if (nplain >= 2) {
@@ -1381,26 +1388,32 @@ int Engine::run_nve(int nsteps, int update, int nmon, int prec, jp_energy_log *l
             pending = false;
             if ((r = finish_list(true))) return r;
         }
-        hipEvent_t a, b;
-        CK(hipEventCreate(&a));
-        CK(hipEventCreate(&b));
-        CK(hipEventRecord(a, st));
+        // the pair launch of a host-launched step is timed with one pair of
+        // events; energy steps synchronise anyway, so their time is added at once,
+        // other steps are timed when the next one would reuse the events
+        if (pe_pending) {
+            CK(hipEventSynchronize(ev[5]));
+            float m2 = 0;
+            CK(hipEventElapsedTime(&m2, ev[4], ev[5]));
+            t_pair += m2 * 1e-3;
+            pe_pending = false;
+        }
+        CK(hipEventRecord(ev[4], st));
         if ((r = pair(prec, mon))) return r;
-        CK(hipEventRecord(b, st));
+        CK(hipEventRecord(ev[5], st));
+        pe_pending = true;
         if (nbat) {
             hipLaunchKernelGGL(k_bonded, grid1(nbat, 128), 128, 0, st, bonded_args(), d_atom_sys.p, d_sp.p, d_x.p,
                                d_efac.p, d_inum.p, d_gslot.p, (double *)nullptr);
             CK(hipGetLastError());
         }
-        pe.push_back(a);
-        pe.push_back(b);
         ++n_pair;
         std::vector<double> eel, evd, eex, et;
         if (mon && (r = energies(prec, eel, evd, eex))) return r;
         if (mon && (r = bonded_energies(et))) return r;
         if ((r = integrate(prec, mon))) return r;
         if (mon) {
-            if ((r = mon_record(step, nsteps, nmon, nlog, eel, evd, et, log))) return r;
+            if ((r = mon_record((int)(idloop + 1), nsteps, nmon, nlog, eel, evd, et, log))) return r;
             nlog++;
         }
         ++idloop;
@@ -1417,12 +1430,11 @@ int Engine::run_nve(int nsteps, int update, int nmon, int prec, jp_energy_log *l
     float ms = 0;
     CK(hipEventElapsedTime(&ms, ev[0], ev[1]));
     t_total = ms * 1e-3;
-    for (size_t q = 0; q < pe.size(); q += 2) {
+    if (pe_pending) {
         float m2 = 0;
-        hipEventElapsedTime(&m2, pe[q], pe[q + 1]);
+        CK(hipEventElapsedTime(&m2, ev[4], ev[5]));
         t_pair += m2 * 1e-3;
-        hipEventDestroy(pe[q]);
-        hipEventDestroy(pe[q + 1]);
+        pe_pending = false;
     }
     const double tl = t_list - tl0;
     t_step = t_total - (n_pair ? t_pair / n_pair * nsteps : 0.0) - tl;
@@ -1601,6 +1613,24 @@ int jp_time_pair(jp_ctx *ctx, int32_t precision, int32_t reps, double *seconds) 
     if (!ctx || !seconds || reps <= 0 || (precision != JP_PREC_FP64 && precision != JP_PREC_FP32_FIXED))
         return JP_EINVAL;
     return ctx->e.time_pair(precision, reps, *seconds);
+}
+
+int jp_set_step(jp_ctx *ctx, int64_t idloop) {
+    if (!ctx || idloop < 0) return JP_EINVAL;
+    ctx->e.idloop = idloop;
+    return 0;
+}
+
+int jp_get_step(jp_ctx *ctx, int64_t *idloop) {
+    if (!ctx || !idloop) return JP_EINVAL;
+    *idloop = ctx->e.idloop;
+    return 0;
+}
+
+int jp_batch_set_step(jp_batch *b, int64_t idloop) {
+    if (!b || idloop < 0) return JP_EINVAL;
+    b->e.idloop = idloop;
+    return 0;
 }
 
 int jp_count_pairs(jp_ctx *ctx, int64_t *npairs, int64_t *nlj, int64_t *nlisted) {

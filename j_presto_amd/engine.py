@@ -141,10 +141,21 @@ class MdSystem:
         """md_run's MD loop on the device state (METHOD= MICR, or CANO after
         set_canonical).  Returns (log, ier)."""
         prec = self.precision if precision is None else precision
-        nl = nsteps // nmon if nmon > 0 else 0
+        nl = nsteps // nmon + 1 if nmon > 0 else 0
         log = (EnergyLog * max(nl, 1))()
         ier = native.check("jp_run_nve", self.lib.jp_run_nve(self.ctx, nsteps, update, nmon, prec, log))
-        return log_to_array(log, nl), ier
+        out = log_to_array(log, nl)
+        return out[out["step"] >= 0], ier
+
+    def set_step(self, idloop: int):
+        """md_run's step counter (a restart's loop number); list rebuilds and
+        monitoring steps follow it."""
+        native.check("jp_set_step", self.lib.jp_set_step(self.ctx, int(idloop)))
+
+    def get_step(self) -> int:
+        v = C.c_int64()
+        native.check("jp_get_step", self.lib.jp_get_step(self.ctx, C.byref(v)))
+        return v.value
 
     def set_graphs(self, on: bool):
         """Plain steps of runmd as captured HIP graphs (default) or launched
@@ -266,11 +277,18 @@ class Ensemble:
 
     def runmd(self, nsteps: int, update: int = 10, nmon: int = 0, precision: Optional[int] = None):
         prec = self.precision if precision is None else precision
-        nl = nsteps // nmon if nmon > 0 else 0
+        nl = nsteps // nmon + 1 if nmon > 0 else 0
         log = (EnergyLog * max(nl * self.nsys, 1))()
         ier = native.check("jp_batch_run_nve", self.lib.jp_batch_run_nve(self.b, nsteps, update, nmon, prec, log))
         full = log_to_array(log, nl * self.nsys)
-        return full.reshape(self.nsys, nl) if nl else full, ier
+        if not nl:
+            return full, ier
+        full = full.reshape(self.nsys, nl)
+        used = full[0]["step"] >= 0        # every system records the same steps
+        return full[:, used], ier
+
+    def set_step(self, idloop: int):
+        native.check("jp_batch_set_step", self.lib.jp_batch_set_step(self.b, int(idloop)))
 
     def set_graphs(self, on: bool):
         native.check("jp_batch_set_option", self.lib.jp_batch_set_option(self.b, native.JP_OPT_GRAPHS, int(bool(on))))

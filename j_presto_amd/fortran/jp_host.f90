@@ -194,7 +194,7 @@ program jp_host
   if (settem > 0.0d0) ier(9) = jp_set_canonical(ctx, 1_c_int32_t, settem, 3_c_int32_t, 1.0d-3, c_null_ptr)
   nlog = 0
   if (nmon > 0) nlog = nsteps / nmon
-  allocate(elog(max(nlog,1)))
+  allocate(elog(nlog + 1))
   ier(5) = jp_run_nve(ctx, nsteps, update, nmon, prec, c_loc(elog))
   ier(6) = jp_get_state(ctx, c_loc(cord), c_loc(vel))
   write(*,'(a,9(1x,i0))') 'jp_host: ier =', ier
@@ -322,7 +322,7 @@ contains
                    stop 3
                 end if
                 read(val(1:q - 1), *, iostat=ios) cla
-                read(val(q + 1:), *, iostat=ios) clb
+                if (ios == 0) read(val(q + 1:), *, iostat=ios) clb
              end select
              if (ios /= 0) call bad_value(key, val)
              cycle
@@ -417,9 +417,19 @@ contains
     nm = nmon
     if (len_trim(etof) > 0) nm = 1
     nl = 0
-    if (nm > 0) nl = nsteps / nm
+    if (nm > 0) nl = nsteps / nm + 1
     allocate(elog(max(nl, 1)))
+    ! md_run's idloop continues from the restart's loop number (inirst, md.f90:1132)
+    if (jp_set_step(ctx, int(rloop, c_int64_t)) /= JP_OK) then
+       write(*,'(a,i0)') 'jp_host: jp_set_step failed for loop ', rloop
+       stop 1
+    end if
     ier(5) = jp_run_nve(ctx, nsteps, update, nm, prec, c_loc(elog))
+    ! the records of the monitoring steps come first; the rest have step = -1
+    do while (nl > 0)
+       if (elog(nl)%step >= 0) exit
+       nl = nl - 1
+    end do
     ier(6) = jp_get_state(ctx, c_loc(cord), c_loc(vel))
     write(*,'(a,9(1x,i0))') 'jp_host: ier =', ier
     if (len_trim(etof) > 0 .and. ier(5) == 0) then

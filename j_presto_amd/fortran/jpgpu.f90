@@ -5,7 +5,8 @@
 ! Every routine returns md_run's ier code for the routine it replaces
 ! (0 ok; 28 cell overflow, 51 list overflow, 99 excluded pair missing from
 ! the list, 1 SHAKE not converged, -2 SHAKE singular); values <= -100 are
-! runtime errors.  Arrays are passed in md_run's own layout: cord(3,N),
+! runtime errors (-101..-103: JP_EINVAL, JP_ENOMEM, JP_ENODEV; -1000 - e for
+! a HIP call that failed with hipError_t e).  Arrays are passed in md_run's own layout: cord(3,N),
 ! vel(3,N), grad(3,N,nfrg), 1-based atom indices.  The reference routines
 ! each entry point stands in for:
 !   jp_ctx_create   incel.f90:2 + inpcls.f90 + setshk.f90 (set-up)
@@ -140,6 +141,19 @@ module jpgpu
        type(c_ptr), value :: ctx
        real(c_double), intent(out) :: t_total, t_pair, t_list, t_step
      end function jp_last_timing
+
+     ! md_run's step counter idloop (inirst sets it from the restart, md.f90:1132)
+     integer(c_int) function jp_set_step(ctx, idloop) bind(C, name='jp_set_step')
+       import :: c_int, c_ptr, c_int64_t
+       type(c_ptr), value :: ctx
+       integer(c_int64_t), value :: idloop
+     end function jp_set_step
+
+     integer(c_int) function jp_get_step(ctx, idloop) bind(C, name='jp_get_step')
+       import :: c_int, c_ptr, c_int64_t
+       type(c_ptr), value :: ctx
+       integer(c_int64_t), intent(out) :: idloop
+     end function jp_get_step
 
      integer(c_int) function jp_count_pairs(ctx, npairs, nlj, nlisted) bind(C, name='jp_count_pairs')
        import :: c_int, c_ptr, c_int64_t

@@ -43,6 +43,10 @@ Q_O, Q_H = -0.834, 0.417
 NA_RMIN2, NA_EPS, NA_MASS = 1.212, 0.3526418, 22.98977
 CL_RMIN2, CL_EPS, CL_MASS = 2.513, 0.0355910, 35.453
 WATER_PER_A3 = 0.997 / 18.01528 * 6.02214076e23 / 1e24   # 0.997 g/cm^3
+BULK_WATER_PER_A3 = 2000.0 / 39.1 ** 3                     # the 6k-atom water box: 1.0007 g/cm^3
+# water volume a compact-chain atom displaces in the "bulk" solvent mode
+# (measured on chains of 300-3,000 atoms: 26.3-29.2 A^3), used to size boxes
+CHAIN_ATOM_A3 = 28.0
 
 
 @dataclasses.dataclass
@@ -550,11 +554,21 @@ def chain_template(rng, n: int, type0: int, ntypes: int, cluster: int = 1,
 def chain_in_water(n_chain: int, L: float, seed: int, n_chain_types: int = 16, temp: float = 300.0,
                    lmd: Optional[float] = None, name: str = "chain", bond_fc: float = 0.0,
                    angle_fc: float = 0.0, bond_r0: Optional[float] = None,
-                   angle_deg: Optional[float] = None, compact: bool = False, torsion_fc: float = 0.0) -> System:
-    """Solvated chain: SAW chain in a periodic water box; waters
-    with any atom within 2.6 A of a chain atom are removed.  With `lmd`, the
+                   angle_deg: Optional[float] = None, compact: bool = False, torsion_fc: float = 0.0,
+                   solvent: str = "lattice", n_target: Optional[int] = None) -> System:
+    """Solvated chain: SAW chain in a periodic water box.  With `lmd`, the
     chain is cluster class 1 and water class 2 under CLSTMETHOD=LMD (three
-    energy classes: chain-chain lambda^2, chain-water lambda, water-water 1)."""
+    energy classes: chain-chain lambda^2, chain-water lambda, water-water 1).
+
+    solvent="lattice": every site of the smallest cubic lattice with at least
+    0.997 g/cm^3 worth of sites; waters with any atom within 2.6 A of a chain
+    atom are removed (O sites within 3.96 A never get a water).
+    solvent="bulk": the water of the 6k-atom water box (2,000 waters in a
+    39.1 A box, 1.0007 g/cm^3) on a random subset of lattice sites; a water
+    is removed when its O is within 2.5 A or an H within 1.8 A of a chain
+    atom (O sites within 2.8 A never get a water).  With n_target, waters
+    drawn at random are removed until the system has n_target atoms (or the
+    nearest count above it that whole waters allow)."""
     rng = np.random.Generator(np.random.PCG64(seed))
     Lv = np.array([L, L, L], np.float64)
     if compact:
@@ -563,18 +577,32 @@ def chain_in_water(n_chain: int, L: float, seed: int, n_chain_types: int = 16, t
     else:
         cpos = _saw_chain(rng, n_chain, L)
     # bond lengths of the walk are exactly 1.5 A, so SHAKE targets are met
-    nsite = int(round(WATER_PER_A3 * L ** 3))
-    sites = _lattice_sites(nsite, L)
-    sites = sites + rng.uniform(-0.2, 0.2, size=sites.shape)
     from scipy.spatial import cKDTree
     ctree = cKDTree(np.mod(cpos, L), boxsize=Lv)
-    near = ctree.query_ball_point(np.mod(sites, L), r=2.6 + R_OH + 0.4)
+    if solvent == "lattice":
+        nsite = int(round(WATER_PER_A3 * L ** 3))
+        sites = _lattice_sites(nsite, L)
+        pre, o_gap, h_gap = 2.6 + R_OH + 0.4, 2.6, 2.6
+    elif solvent == "bulk":
+        nsite = int(round(BULK_WATER_PER_A3 * L ** 3))
+        sites = _lattice_sites(nsite, L)
+        sites = sites[np.sort(rng.permutation(len(sites))[:nsite])]
+        pre, o_gap, h_gap = 2.8, 2.5, 1.8
+    else:
+        raise ValueError(f"unknown solvent mode {solvent!r}")
+    sites = sites + rng.uniform(-0.2, 0.2, size=sites.shape)
+    near = ctree.query_ball_point(np.mod(sites, L), r=pre)
     keep = np.array([len(nb) == 0 for nb in near])
     pos_w = _place_waters(rng, sites[keep], Lv, fixed=cpos)
-    # drop waters with any atom within 2.6 A of the chain
     d, _ = ctree.query(np.mod(pos_w, L))
-    bad = (d.reshape(-1, 3) < 2.6).any(1)
-    pos_w = pos_w.reshape(-1, 3, 3)[~bad].reshape(-1, 3)
+    d = d.reshape(-1, 3)
+    bad = (d[:, 0] < o_gap) | (d[:, 1:] < h_gap).any(1)
+    pos_w = pos_w.reshape(-1, 3, 3)[~bad]
+    if n_target is not None:
+        nw_want = max(0, -(-(int(n_target) - n_chain) // 3))
+        if len(pos_w) > nw_want:
+            pos_w = pos_w[np.sort(rng.permutation(len(pos_w))[:nw_want])]
+    pos_w = pos_w.reshape(-1, 3)
     nwat = len(pos_w) // 3
     lm = lmd is not None
     wt = water_types()
@@ -599,7 +627,7 @@ CHAIN_BOND_FC, CHAIN_ANGLE_FC = 300.0, 50.0
 
 WORKLOADS = {
     0: "water-6k: 2,000 TIP3P waters, 39.1 A box",
-    1: "chain-24k: 2,500-atom chain (harmonic bonds/angles) in water, 64 A box",
+    1: "chain-22k: 2,500-atom chain (harmonic bonds/angles) in water, 64 A box (22,297 atoms)",
     2: "water-100k: 33,333 waters + 60 Na/Cl pairs, 100.0 A box",
     3: "ensemble-64: 64 chain-in-water systems (harmonic bonds/angles), 3k-30k atoms each",
     4: "water-1M: 333,333 waters, 216 A box",
@@ -611,8 +639,8 @@ def workload(i: int) -> System:
     if i == 0:
         return water_box(2000, 39.1, seed, name="water-6k")
     if i == 1:
-        return chain_in_water(2500, 64.0, seed, name="chain-24k", bond_fc=CHAIN_BOND_FC, angle_fc=CHAIN_ANGLE_FC,
-                              compact=True)
+        return chain_in_water(2500, 64.0, seed, name="chain-22k", bond_fc=CHAIN_BOND_FC, angle_fc=CHAIN_ANGLE_FC,
+                              compact=True, solvent="bulk")
     if i == 2:
         L = (33333 / WATER_PER_A3) ** (1.0 / 3.0)
         return water_box(33333, round(L, 3), seed, n_ion_pairs=60, name="water-100k")
@@ -621,18 +649,40 @@ def workload(i: int) -> System:
     raise ValueError("workload 3 is an ensemble; use ensemble()")
 
 
-def ensemble(nsys: int = 64, seed: int = 1003, lo: int = 3000, hi: int = 30000) -> List[System]:
-    """Uneven ensemble: sizes log-uniform in [lo, hi] atoms; each member a
-    chain-in-water system (chain = 10.4% of atoms) with its own lambda in
-    [0.5, 1.0] on the chain's interactions."""
+def ensemble_sizes(nsys: int = 64, seed: int = 1003, lo: int = 3000, hi: int = 30000):
+    """Target atom counts (log-uniform in [lo, hi]) and lambdas (U[0.5, 1.0])
+    of the ensemble members."""
     rng = np.random.Generator(np.random.PCG64(seed))
     sizes = np.round(np.exp(rng.uniform(math.log(lo), math.log(hi), size=nsys))).astype(int)
     lams = rng.uniform(0.5, 1.0, size=nsys)
-    out = []
-    for k, (ns, lam) in enumerate(zip(sizes, lams)):
-        L = (ns / (3 * WATER_PER_A3)) ** (1.0 / 3.0)
-        L = max(L, 24.5)
-        nch = max(8, int(round(0.104 * ns)) // 4 * 4)
-        out.append(chain_in_water(nch, round(L, 3), seed * 1000 + k, lmd=float(lam), name=f"ens{k:02d}",
-                                  bond_fc=CHAIN_BOND_FC, angle_fc=CHAIN_ANGLE_FC, compact=True))
-    return out
+    return sizes, lams
+
+
+def _ensemble_member(args) -> System:
+    k, ns, lam, seed = args
+    nch = max(8, int(round(0.104 * ns)) // 4 * 4)
+    # box for ns atoms: the waters at bulk density plus the chain's displaced volume
+    # (slightly overestimated; the surplus waters are removed at random)
+    L = ((ns - nch) / (3 * BULK_WATER_PER_A3) + nch * CHAIN_ATOM_A3) ** (1.0 / 3.0)
+    L = max(L, 24.5)
+    return chain_in_water(nch, round(L, 3), seed * 1000 + k, lmd=float(lam), name=f"ens{k:02d}",
+                          bond_fc=CHAIN_BOND_FC, angle_fc=CHAIN_ANGLE_FC, compact=True,
+                          solvent="bulk", n_target=int(ns))
+
+
+def ensemble(nsys: int = 64, seed: int = 1003, lo: int = 3000, hi: int = 30000, members=None,
+             workers: int = 8) -> List[System]:
+    """Uneven ensemble: sizes log-uniform in [lo, hi] atoms; each member a
+    chain-in-water system (chain = 10.4% of atoms) with its own lambda in
+    [0.5, 1.0] on the chain's interactions.  `members`: indices of the members
+    to build (default all).  Members are independent (own seeds), so they are
+    built in `workers` processes with the same result as serially."""
+    sizes, lams = ensemble_sizes(nsys, seed, lo, hi)
+    idx = list(range(nsys)) if members is None else list(members)
+    jobs = [(k, int(sizes[k]), float(lams[k]), seed) for k in idx]
+    if workers <= 1 or len(jobs) <= 1:
+        return [_ensemble_member(j) for j in jobs]
+    import concurrent.futures as cf
+    import multiprocessing as mp
+    with cf.ProcessPoolExecutor(max_workers=min(workers, len(jobs)), mp_context=mp.get_context("fork")) as ex:
+        return list(ex.map(_ensemble_member, jobs))

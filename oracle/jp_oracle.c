@@ -1210,7 +1210,9 @@ double jo_ekin(const jo_ctx *c, const double *v) {
     for (int i = 0; i < c->n; ++i)
         ek = ek + c->mass[i] * (v[3 * i] * v[3 * i] + v[3 * i + 1] * v[3 * i + 1] + v[3 * i + 2] * v[3 * i + 2]);
     ek = ek * 1.0e7;
-    return ek / 4.184 * 0.0005;
+    /* caltmp.f90:43 writes the factor as the default-real literal 0.0005:
+     * single precision, 5.00000023748725e-4 after promotion */
+    return ek / 4.184 * (double)0.0005f;
 }
 
 /* NVE run on the CPU (the port baseline): md_run's loop order, list rebuilt

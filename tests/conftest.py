@@ -45,3 +45,32 @@ def rel_rms(a, b):
     a = np.asarray(a, np.float64)
     b = np.asarray(b, np.float64)
     return float(np.sqrt(((a - b) ** 2).sum() / (b ** 2).sum()))
+
+
+def oracle_monitor(o, s, x, v, nsteps, every=5, cano=None, scaled=False):
+    """Runs the oracle's md_run loop (MICR, or CANO with cano=(temps, grp))
+    and returns rows (step, E_tot, E_pot, K) at every `every`-th step:
+    K = caltmp of (v(t-dt/2)+v(t+dt/2))/2 (md.f90:618-628), E_pot md_run's
+    fdepot(1), the unscaled sum of the class energies (dnergy.f90:78-79;
+    with scaled=True the lambda-scaled sum the LMD dynamics conserve,
+    verlet.f90:67-91), E_tot = K + E_pot (md.f90:805, monmd.f90:123)."""
+    lam = s.lam
+    fac = [lam * lam, lam, 1.0] if (o.nfrg == 3 and scaled) else [1.0] * o.nfrg
+    rows = []
+    assert o.celprp(x) == 0
+    for idloop in range(1, nsteps + 1):
+        g, et = o.bonded(x)
+        g, ee, ev = o.nonbond(x, g)
+        ep = sum(f * (ee[k] + ev[k] + et[k].sum()) for k, f in enumerate(fac))
+        if cano is None:
+            vb = v.copy()
+            assert o.verlet_step(x, v, o.combine(g)) == 0
+        else:
+            ier, vb = o.cano_step(x, v, o.combine(g), cano[0], 3, 1e-3, cano[1])
+            assert ier == 0
+        if idloop % every == 0:
+            ek = o.ekin(0.5 * (vb + v))
+            rows.append((idloop, ek + ep, ep, ek))
+        if idloop % 10 == 0:
+            assert o.celprp(x) == 0
+    return np.array(rows)

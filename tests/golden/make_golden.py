@@ -14,10 +14,15 @@ and N_SIMD= 1 (a fixed summation order):
   cano job   (some systems) RESTAR= YES, METHOD= CANO, SETTEM= 300,
              TEMPCO= NO or YES, LOOPLI= K: K velocity-scaled steps
              (verlethe.f90); the output restart holds x and v.
+  mon jobs   (some systems) the traj and cano jobs again with OUTLOG= 5:
+             md_run's monitoring blocks (monmd.f90 outtlg) print, every 5th
+             step, the loop number, E_tot, E_pot, T and the kinetic energy
+             caltmp computes from (v(t-dt/2)+v(t+dt/2))/2 (md.f90:618-628),
+             each to 8 significant digits (T: 7 decimals).
 A fixture stores the generator recipe, the input x and v, and the outputs.
 The reference itself never leaves this container.
 
-Usage: python tests/golden/make_golden.py [--cano-only] [name ...]
+Usage: python tests/golden/make_golden.py [--cano-only | --mon-only] [name ...]
 """
 from __future__ import annotations
 
@@ -41,13 +46,14 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 
 RECIPES = {
     # small box just above 2*cutoff (box < 2*rlim: periodic images inside the list radius)
-    "water_1500": ("water_box", dict(nwat=500, L=24.8, seed=7), dict(traj=20, cano=("NO", 20))),
+    "water_1500": ("water_box", dict(nwat=500, L=24.8, seed=7), dict(traj=20, cano=("NO", 20), mon=True)),
     # the 6k-atom water workload itself
     "water_6k": ("workload", dict(i=0), dict(traj=20)),
     # chain with 16 LJ types, 2-atom SHAKE groups, chain exclusions
     "chain_3k": ("chain_in_water", dict(n_chain=200, L=30.0, seed=11), dict(traj=20, cano=("YES", 20))),
     # LMD: three energy classes, lambda-scaled gradient
-    "chain_lmd": ("chain_in_water", dict(n_chain=160, L=28.0, seed=12, lmd=0.7), dict(traj=10, cano=("YES", 10))),
+    "chain_lmd": ("chain_in_water", dict(n_chain=160, L=28.0, seed=12, lmd=0.7),
+                  dict(traj=10, cano=("YES", 10), mon=True)),
     # chain with harmonic bonds and angles (dnebnd_PB / dneang_PB), strained off their minima
     "chain_bonded": ("chain_in_water", dict(n_chain=200, L=30.0, seed=15, bond_fc=300.0, angle_fc=50.0,
                                             bond_r0=1.53, angle_deg=112.0), dict(traj=20)),
@@ -124,6 +130,8 @@ def make(name: str) -> str:
             ene = np.loadtxt(os.path.join(t2, "ene.dat"), ndmin=2)
             out["traj_epot"] = ene
         out.update(cano_job(s, opt, tmp))
+        if opt.get("mon"):
+            out.update(mon_jobs(s, opt, tmp))
         path = os.path.join(HERE, f"{name}.npz")
         np.savez_compressed(path, **out)
         return path
@@ -143,6 +151,69 @@ def cano_job(s: systems.System, opt: dict, tmp: str) -> dict:
                    "TEMPCO": tempco, "OUTLOG": "100000", "OUTCLE": "100000"})
     r = refio.read_restart(os.path.join(t3, "out.rst"))
     return dict(cano_tempco=np.array(tempco), cano_nsteps=np.int64(ncano), cano_x=r["x"], cano_v=r["v"])
+
+
+MON_KEYS = ("MD LOOP NUMBER", "TOTAL      (KCAL/MOL)", "POTENTIAL  (KCAL/MOL)", "TEMPERATURE (K)",
+            "KINETIC    (KCAL/MOL)")
+
+
+def parse_monitor(log: str) -> np.ndarray:
+    """(step, E_tot, E_pot, T, K) of every monitoring block md_run printed
+    after its loop started (outtlg's format 810: 'NAME : value' fields, two
+    per line); the restart's check block before the loop is skipped."""
+    body = log[log.index("MOLECULAR DYNAMICS LOOP START"):]
+    rows, cur = [], {}
+    for line in body.splitlines():
+        for key in MON_KEYS:
+            k = line.find(key)
+            if k < 0:
+                continue
+            rest = line[k + len(key):].lstrip()
+            if not rest.startswith(":"):
+                continue
+            cur[key] = float(rest[1:].split()[0])
+        if all(k in cur for k in MON_KEYS):
+            rows.append([cur[k] for k in MON_KEYS])
+            cur = {}
+    return np.array(rows, np.float64)
+
+
+def mon_jobs(s: systems.System, opt: dict, tmp: str) -> dict:
+    """The traj job (and the cano job) again with OUTLOG= 5; returns the
+    parsed monitoring rows."""
+    out = {}
+    common = {"RESTAR": "YES", "NAMERI": "in.rst", "NAMERO": "out.rst", "UPDATE": "10", "TIMEST": "2.0",
+              "OUTLOG": "5", "OUTCLE": "100000"}
+    d = os.path.join(tmp, "mon")
+    os.makedirs(d)
+    refio.write_restart(os.path.join(d, "in.rst"), s.x, s.v, lam=s.lam)
+    log = run_md(d, s, dict(common, METHOD="MICR", LOOPLI=str(opt["traj"])))
+    out["mon_micr"] = parse_monitor(log)
+    if "cano" in opt:
+        tempco, ncano = opt["cano"]
+        d = os.path.join(tmp, "cmon")
+        os.makedirs(d)
+        refio.write_restart(os.path.join(d, "in.rst"), s.x, s.v, lam=s.lam)
+        log = run_md(d, s, dict(common, METHOD="CANO", LOOPLI=str(ncano), SETTEM="300.0", TEMPCO=tempco))
+        out["mon_cano"] = parse_monitor(log)
+    return out
+
+
+def add_mon(name: str) -> str:
+    """Adds the monitoring jobs' outputs to an existing fixture (same recipe)."""
+    kind, args, opt = RECIPES[name]
+    s = build_system(kind, args)
+    path = os.path.join(HERE, f"{name}.npz")
+    with np.load(path, allow_pickle=False) as z:
+        out = {k: z[k] for k in z.files}
+    assert str(out["fingerprint"]) == fingerprint(s)
+    tmp = tempfile.mkdtemp(prefix="jpgold_")
+    try:
+        out.update(mon_jobs(s, opt, tmp))
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    np.savez_compressed(path, **out)
+    return path
 
 
 def add_cano(name: str) -> str:
@@ -166,10 +237,11 @@ def main(argv):
     if not os.path.exists(MD_RUN):
         subprocess.run(["bash", os.path.join(ROOT, "oracle", "build_ref.sh")], check=True)
     cano_only = "--cano-only" in argv
-    argv = [a for a in argv if a != "--cano-only"]
-    names = argv or list(RECIPES)
+    mon_only = "--mon-only" in argv
+    argv = [a for a in argv if a not in ("--cano-only", "--mon-only")]
+    names = argv or [n for n in RECIPES if not mon_only or RECIPES[n][2].get("mon")]
     for n in names:
-        p = add_cano(n) if cano_only else make(n)
+        p = add_cano(n) if cano_only else (add_mon(n) if mon_only else make(n))
         print(n, "->", os.path.relpath(p, ROOT), os.path.getsize(p), "bytes", flush=True)
 
 

@@ -322,3 +322,46 @@ def test_list_overflow_during_overlapped_check(prec):
     (x1, v1), e1 = out[1]
     assert np.array_equal(x0, x1) and np.array_equal(v0, v1)
     assert np.array_equal(e0, e1)
+
+
+@pytest.mark.parametrize("name,method", [("water_1500", "micr"), ("chain_lmd", "micr"),
+                                         ("water_1500", "cano"), ("chain_lmd", "cano")])
+def test_energy_log_vs_oracle_and_reference(golden, name, method):
+    """jp_run_nve's monitored energies (caltmp K from (v(t-dt/2)+v(t+dt/2))/2,
+    E_pot, E_tot; md.f90:618-628) against the oracle within 1e-12 relative,
+    and md_run's printed KINETIC / POTENTIAL / TOTAL (monmd.f90 outtlg, 7
+    significant digits) within one unit of the last printed digit.  md_run
+    prints the unscaled class sum as POTENTIAL (dnergy.f90:78-79); the log's
+    epot/etot carry the lambda-scaled sum under LMD and its per-class terms
+    give the unscaled one."""
+    from conftest import oracle_monitor
+    s, f = golden(name)
+    print(f"case energy log {name} {method}", flush=True)
+    cano = method == "cano"
+    nsteps = int(f["cano_nsteps"] if cano else f["traj_nsteps"])
+    grp = None
+    if cano and str(f["cano_tempco"]) == "YES":
+        grp = s.temperature_groups()
+    md = engine.MdSystem(s, precision=engine.FP64)
+    md.set_state(s.x, s.v)
+    if cano:
+        md.set_canonical(300.0, 3, 1e-3, tempco=grp is not None)
+    assert md.celprp() == 0
+    log, ier = md.runmd(nsteps, update=10, nmon=5)
+    assert ier == 0
+    md.close()
+    o = Oracle(s)
+    c = (300.0, grp) if cano else None
+    sc = oracle_monitor(o, s, s.x.copy(), s.v.copy(), nsteps, 5, c, scaled=True)
+    un = oracle_monitor(Oracle(s), s, s.x.copy(), s.v.copy(), nsteps, 5, c, scaled=False)
+    assert np.array_equal(log["step"], sc[:, 0].astype(np.int32))
+    for key, col in (("etot", 1), ("epot", 2), ("ekin", 3)):
+        assert np.allclose(log[key], sc[:, col], rtol=1e-12, atol=1e-12 * np.abs(sc[:, 2]).max()), key
+    terms = ("eelec", "evdw", "ebond", "eangle", "etors", "eimp", "evdw14", "eelec14")
+    ep_un = sum(log[k].sum(axis=1) for k in terms)
+    assert np.allclose(ep_un, un[:, 2], rtol=1e-12, atol=1e-12 * np.abs(un[:, 2]).max())
+    mon = f[f"mon_{method}"]
+    mon = mon[mon[:, 0] <= nsteps]
+    for val, col in ((log["ekin"], 4), (ep_un, 2), (log["ekin"] + ep_un, 1)):
+        ulp = 10.0 ** (np.floor(np.log10(np.abs(mon[:, col]))) - 6)   # one unit of the 7th digit
+        assert np.all(np.abs(val - mon[:, col]) <= 1.0001 * ulp), (col, val, mon[:, col])

@@ -4,7 +4,7 @@ sets and NVE trajectories."""
 import numpy as np
 import pytest
 
-from conftest import GOLDEN_NAMES
+from conftest import GOLDEN_NAMES, oracle_monitor
 from oracle import Oracle
 
 
@@ -112,3 +112,45 @@ def test_cano_groups_and_dof():
     n_xh = sum(1 for atoms, _ in c.shake_groups() if len(atoms) == 2)
     dof = Oracle(c).group_dof(grp)
     assert list(dof) == [3.0 * nch - n_xh, 6.0 * nw]
+
+
+def printed(a):
+    """md_run's e15.7 rounding: 7 significant digits."""
+    return [float(f"{e:.6e}") for e in np.ravel(a)]
+
+
+@pytest.mark.parametrize("name,method", [("water_1500", "micr"), ("chain_lmd", "micr"),
+                                         ("water_1500", "cano"), ("chain_lmd", "cano")])
+def test_monitored_energies_vs_reference(golden, name, method):
+    """caltmp's kinetic energy, E_pot and E_tot at md_run's OUTLOG steps
+    (monmd.f90 outtlg) equal the oracle's to every printed digit, and the
+    printed temperature is K over md_run's degrees of freedom (3 per atom
+    minus one per SHAKE constraint, md.f90:114-118)."""
+    s, f = golden(name)
+    mon = f[f"mon_{method}"]
+    o = Oracle(s)
+    cano = None
+    if method == "cano":
+        cano = (300.0, None if str(f["cano_tempco"]) == "NO" else s.temperature_groups())
+    nsteps = int(f["traj_nsteps"] if method == "micr" else f["cano_nsteps"])
+    rows = oracle_monitor(o, s, s.x.copy(), s.v.copy(), nsteps, 5, cano)
+    ref = mon[mon[:, 0] <= nsteps]
+    assert np.array_equal(rows[:, 0], ref[:, 0])
+    assert printed(rows[:, 3]) == list(ref[:, 4])      # KINETIC
+    assert printed(rows[:, 2]) == list(ref[:, 2])      # POTENTIAL
+    assert printed(rows[:, 1]) == list(ref[:, 1])      # TOTAL
+    # TEMPERATURE (f15.7) = sum m v^2 / (R ndf) in md_run's units (caltmp.f90:40-43,
+    # whose kinetic-energy factor 0.0005 is a single-precision literal)
+    temp = rows[:, 3] * 4.184 / float(np.float32(0.0005)) / (8.31451 * s.ndf())
+    assert np.allclose(temp, ref[:, 3], rtol=0, atol=1e-6)
+
+
+def test_oracle_run_nve_equals_reference_trajectory(golden):
+    """Oracle.run_nve (the oracle's own MICR loop: dnergy, verlet, rebuild
+    after every `update`-th step) reproduces md_run's trajectory bit for bit."""
+    for name in ("water_1500", "chain_bonded"):
+        s, f = golden(name)
+        o = Oracle(s)
+        x, v = s.x.copy(), s.v.copy()
+        assert o.run_nve(x, v, int(f["traj_nsteps"]), 10) == 0
+        assert np.array_equal(x, f["traj_x"]) and np.array_equal(v, f["traj_v"]), name

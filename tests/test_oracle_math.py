@@ -174,3 +174,25 @@ def test_exclusion_out_of_range_is_ier99():
     x[1] += np.array([12.0, 12.0, 0.0])   # pull H1 of water 0 ~17 A from its O
     o = Oracle(s)
     assert o.celprp(x) == 99
+
+
+def test_ekin_closed_form():
+    """caltmp's kinetic energy is (1/2) sum m v^2 in kcal/mol: with m in g/mol
+    and v in A/fs (1 A/fs = 1e5 m/s), SI arithmetic gives the same value up
+    to caltmp's factor 0.0005, a single-precision literal (caltmp.f90:43):
+    K = (1/2) sum m v^2 * float32(0.0005) / 0.0005."""
+    from j_presto_amd import systems
+    s = systems.water_box(8, 12.5, 3)
+    o = Oracle(s)
+    rng = np.random.default_rng(4)
+    v = rng.normal(scale=0.01, size=(s.n, 3))
+    m_kg = s.mass * 1e-3                       # kg/mol
+    v_si = v * 1e5                             # m/s
+    lit = float(np.float32(0.0005)) / 0.0005   # 1 + 4.75e-8
+    k_si = 0.5 * (m_kg[:, None] * v_si * v_si).sum() / 4184.0   # kcal/mol
+    assert abs(o.ekin(v) - k_si * lit) <= 1e-12 * k_si
+    assert abs(o.ekin(v) / k_si - 1.0) < 5e-8
+    # one atom moving alone
+    v1 = np.zeros_like(v)
+    v1[4] = [3e-3, 0.0, 0.0]
+    assert abs(o.ekin(v1) - 0.5 * s.mass[4] * 1e-3 * (300.0 ** 2) / 4184.0 * lit) < 1e-15
