@@ -265,6 +265,18 @@ int jp_batch_destroy(jp_batch *b);
  *                      only (a multiple of 64; set, not read); the library grows
  *                      segments by itself and rebuilds when one overflows, so this
  *                      only exercises that path (results do not depend on it).
+ *   JP_OPT_CELL_TABLE  1 (default): the pair list holds the pairs md_run's own
+ *                      periodic neighbour-cell table lets it test (incel.f90:
+ *                      456-508), bit for bit like cutatm_PB.  That table wraps
+ *                      a cell range that can be up to two cells longer than the
+ *                      box edge, and for some edges (40.75-47 A, 54.25-60.5 A,
+ *                      ..., 94.5-99.75 A at a 12 A cutoff) md_run never tests
+ *                      pairs across the periodic boundary that lie inside the
+ *                      cutoff.  0: every pair within rlim by the minimum image
+ *                      (the complete list).  Where md_run's table is undefined
+ *                      (its cell range leaves 1..nv, e.g. a 216 A box at
+ *                      CELLVL 3) the list is the complete one either way.  A
+ *                      change takes effect at the next list build.
  * Setting a shape turns the autotuner off; jp_get_option reads the current
  * (possibly tuned) values. */
 #define JP_OPT_GRAPHS 1
@@ -272,6 +284,7 @@ int jp_batch_destroy(jp_batch *b);
 #define JP_OPT_PAIR_WAVES 3
 #define JP_OPT_LIST_WAVES 4
 #define JP_OPT_LIST_SEGMENT 5
+#define JP_OPT_CELL_TABLE 6
 int jp_set_option(jp_ctx *ctx, int32_t option, int32_t value);
 int jp_batch_set_option(jp_batch *b, int32_t option, int32_t value);
 int jp_get_option(jp_ctx *ctx, int32_t option, int32_t *value);

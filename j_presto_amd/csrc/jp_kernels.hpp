@@ -47,6 +47,8 @@ struct SysP {
     double dt;
     double ctr[3];                   // box centre: origin of the FP32 list-test frame
     double band;                     // A^2: FP32 list-test results within band of rlim^2 take the FP64 test
+    int ctab;                        // bit a: md_run's neighbour-cell table drops pairs within rlim on axis a
+    int ctab_sym;                    // 1: those drops are symmetric in i and j (a half list can hold them)
 };
 
 struct JAtom {          // 32 B
@@ -243,12 +245,15 @@ __global__ void k_fill_sorted(int nslot, const int *__restrict__ slot_sys, const
                               const int *__restrict__ img, const double *__restrict__ chg,
                               const int *__restrict__ tflag, const int *__restrict__ mol, JAtom *__restrict__ ys,
                               double *__restrict__ qd, double4 *__restrict__ xr, int4 *__restrict__ imgs,
-                              float4 *__restrict__ lp) {
+                              float4 *__restrict__ lp, const int *__restrict__ refcell, int *__restrict__ mcell) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nslot) return;
     int a = perm[s];
     const SysP &P = sp[slot_sys[s]];
     JAtom j;
+    // md_run cell (celpar.f90:117-122) packed 8 bits per axis, for its neighbour-cell table
+    mcell[s] = a < 0 ? 0
+                     : (refcell[3 * a] & 0xff) | ((refcell[3 * a + 1] & 0xff) << 8) | ((refcell[3 * a + 2] & 0xff) << 16);
     if (a < 0) {
         j.x = P.wall[0] - 1.0e4; j.y = P.wall[1] - 1.0e4; j.z = P.wall[2] - 1.0e4;
         j.q = 0.f; j.t = 0;

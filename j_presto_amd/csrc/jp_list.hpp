@@ -236,7 +236,8 @@ __global__ void __launch_bounds__(64 * LW)
 k_list8(int nclu, const int *__restrict__ clu_sys, const SysP *__restrict__ sp, const double *__restrict__ cbb,
         const float4 *__restrict__ ccf, const int4 *__restrict__ crec, const unsigned *__restrict__ sysext,
         const float4 *__restrict__ lp, const double4 *__restrict__ xr, const int4 *__restrict__ imgs,
-        const int *__restrict__ xptr, const int *__restrict__ xidx, int ccap, int hiwater, uint2 *__restrict__ ent,
+        const int *__restrict__ xptr, const int *__restrict__ xidx, const int *__restrict__ mcell,
+        const uint64_t *__restrict__ wmask, int ccap, int hiwater, uint2 *__restrict__ ent,
         uint8_t *__restrict__ mb, int2 *__restrict__ lcnt, int *__restrict__ lst) {
     __shared__ uint32_t s_c[LW][JP_QCAP];   // candidate j-clusters of passing cells
     __shared__ uint32_t s_q[LW][JP_QCAP];   // candidates whose box is within rlim
@@ -244,11 +245,19 @@ k_list8(int nclu, const int *__restrict__ clu_sys, const SysP *__restrict__ sp, 
     __shared__ int4 s_im[LW][JP_CL];
     __shared__ float4 s_off[LW][27];        // per image code: L*s - cluster centre (FP32)
     __shared__ int s_mol[LW][JP_CL];
+    __shared__ uint64_t s_w[LW][3][JP_CL];  // md_run's cell window of each i atom per axis
     const int l = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // wave-uniform
     const int ci = blockIdx.x * LW + w;
     if (ci >= nclu) return;   // whole wavefront; no block-wide barrier below
     const int s = clu_sys[ci];
     const SysP &P = sp[s];
+    // axes on which md_run's neighbour-cell table drops pairs within rlim
+    const int ctab = wmask ? P.ctab : 0;
+    if (ctab && l < 3 * JP_CL) {
+        const int k = l & 7, a = l >> 3;
+        const int c = (mcell[ci * JP_CL + k] >> (8 * a)) & 0xff;
+        s_w[w][a][k] = c >= 1 ? wmask[(3 * (size_t)s + a) * 64 + c - 1] : ~0ull;
+    }
     const CluRange r = clu_range(P, cbb, ci, __uint_as_float(sysext[s]));
     if (r.bad && l == 0) atomicOr(lst, JP_ERR_CELL);   // runaway coordinates: md_run's ier 28
     const float4 cf = ccf[ci];
@@ -364,6 +373,28 @@ k_list8(int nclu, const int *__restrict__ clu_sys, const SysP *__restrict__ sp, 
             }
         }
         if (jc == ci) m &= ~(1u << ja);   // j == i
+        // md_run tests only the pairs its neighbour-cell table holds
+        // (incel.f90:456-508): without an image shift a pair within rlim is
+        // always in it; with one, j's md_run cell must lie in the i atom's
+        // window on every shifted axis where the table drops pairs
+        if (ctab && __ballot(meta >= 0 && code != 13u)) {
+            if (meta >= 0 && code != 13u) {
+                int sh[3];
+                shift_of(code, sh[0], sh[1], sh[2]);
+                const int cj = mcell[js];
+                uint32_t adm = 0xffu;
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    if (!((ctab >> a) & 1) || sh[a] == 0) continue;
+                    const int c = ((cj >> (8 * a)) & 0xff) - 1;
+                    uint32_t b = 0;
+#pragma unroll
+                    for (int k = 0; k < JP_CL; ++k) b |= (uint32_t)((s_w[w][a][k] >> c) & 1ull) << k;
+                    adm &= b;
+                }
+                m &= adm;
+            }
+        }
         // bit k = [molj == moli[k]]: (molj ^ moli[k]) - 1 is negative only for
         // equal ids (ids < 2^30)
         uint32_t same = 0;

@@ -229,9 +229,13 @@ k_pair_f32(int nclu, const int *__restrict__ clu_sys, const SysP *__restrict__ s
             float vv = 0.f;
             if (LJ) {
                 const float2 q6 = ljf[trow + ((mt >> 8) & 0xfff)];
-                const float ri6 = ri2 * ri2 * ri2;
+                // 1/r^2 correctly rounded (IEEE division) rather than the rsq
+                // estimate squared: r^-12 would carry twelve times its error, and the
+                // near-cancelling LJ energy sums it over every pair of the system
+                const float rr2 = ENERGY ? 1.0f / r2 : ri2;
+                const float ri6 = rr2 * rr2 * rr2;
                 const float v1 = q6.x * ri6, v2 = q6.y * ri6 * ri6;
-                cf += (12.f * v2 - 6.f * v1) * ri2;
+                cf += (12.f * v2 - 6.f * v1) * rr2;
                 vv = v2 - v1;
             }
             if (LMD) cf *= ((mt >> 20) & 1) ? f1 : f0;

@@ -64,6 +64,77 @@ inline unsigned grid1(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b)
 // the coordinates it was meant for (internal, never returned to callers)
 #define JP_LIST_REDO (-50)
 
+// md_run's periodic neighbour-cell table (incel.f90:334-358 and 456-508).
+// The list of cell (ix,iy,iz) holds every atom of the cells whose index on
+// each axis lies in that axis's window of the cell's own index: i -/+ idelay,
+// widened by m?delay where it reaches the box-cell range [m1, m2], wrapped
+// into the range modulo iix = m2 - m1 + 1 with a single-precision reciprocal
+// (floor(real(k - m1) * (1.0/iix))).  So the table is a product of per-axis
+// windows, stored as one 64-bit mask per axis and cell index.  The range
+// holds iix cells of rlim/2 although the box edge can be up to two cells
+// shorter, so cells facing each other across the periodic boundary can be
+// more than the window apart and their pairs are never tested: md_run lists
+// no pair between them, even inside the cutoff (box edges of 40.75-47 A,
+// 94.5-99.75 A, ...).  P.ctab marks the axes where that drops a pair within
+// rlim (per-axis gap between the cells' slabs <= rlim), P.ctab_sym whether the
+// drops are symmetric.  A window that leaves [m1, m2] after wrapping, or a
+// range outside [1, nv] (the reference writes out of bounds there), leaves
+// the table undefined: then no pair is dropped.
+static void md_cell_windows(SysP &P, int nv, double siz, double rlim, std::vector<uint64_t> &wm) {
+    const size_t base = wm.size();
+    wm.resize(base + 3 * 64, 0);
+    P.ctab = 0;
+    P.ctab_sym = 1;
+    if (nv > 64) return;
+    const int m = (int)std::ceil(rlim / siz);              // idelay (incel.f90:57)
+    const int ic = (int)std::lround(nv * 0.5);             // anint(nv(1)*0.5d0)
+    int ctab = 0, sym = 1;
+    for (int a = 0; a < 3; ++a) {
+        const double h = 0.5 * P.L[a] / siz;
+        const int m1 = (int)std::ceil(ic - h), m2 = (int)std::ceil(ic + h);
+        const int mdel = (std::ceil(h) - h > 0.5) ? 1 : 2;
+        if (m1 < 1 || m2 > nv) return;
+        const int ii = m2 - m1 + 1;
+        const float ri = 1.0f / (float)ii;
+        uint64_t *W = wm.data() + base + 64 * a;
+        for (int c = m1; c <= m2; ++c) {
+            int j1 = c - m, j2 = c + m;
+            if (j1 <= m1) j1 -= mdel;
+            if (j2 >= m2) j2 += mdel;
+            for (int k = j1; k <= j2; ++k) {
+                const int kk = k - ii * (int)std::floor((float)(k - m1) * ri);
+                if (kk < m1 || kk > m2) return;
+                W[c - 1] |= 1ull << (kk - 1);
+            }
+        }
+        // slab of cell c on this axis, clipped to the box
+        auto slab = [&](int c, double &lo, double &hi) {
+            lo = std::max(P.rorg[a] + (c - 1) * siz, P.wall[a]);
+            hi = std::min(P.rorg[a] + c * siz, P.wall[a] + P.L[a]);
+        };
+        for (int c = m1; c <= m2; ++c)
+            for (int d = m1; d <= m2; ++d) {
+                const bool cd = (W[c - 1] >> (d - 1)) & 1, dc = (W[d - 1] >> (c - 1)) & 1;
+                if (cd && dc) continue;
+                double alo, ahi, blo, bhi;
+                slab(c, alo, ahi);
+                slab(d, blo, bhi);
+                if (!(alo < ahi) || !(blo < bhi)) continue;   // a cell no atom can be in
+                double gap = 1e300;
+                for (int t = -1; t <= 1; ++t) {
+                    const double sh = t * P.L[a];
+                    gap = std::min(gap, std::max(0.0, std::max(blo + sh - ahi, alo - (bhi + sh))));
+                }
+                if (gap <= rlim * (1.0 + 1e-12)) {
+                    ctab |= 1 << a;
+                    if (cd != dc) sym = 0;
+                }
+            }
+    }
+    P.ctab = ctab;
+    P.ctab_sym = sym;
+}
+
 struct HostSys {
     int n = 0, ntyp = 0, nfrag = 1, nfrg = 1, lmd = 0;
     double lambda = 0.0, dself[3] = {0, 0, 0};
@@ -106,6 +177,10 @@ struct Engine {
     DBuf<JAtom> d_ys;
     DBuf<double> d_qd, d_cbb;
     DBuf<double4> d_xr, d_ccen;
+    DBuf<int> d_mcell;              // md_run cell of every slot, 8 bits per axis
+    DBuf<uint64_t> d_wmask;         // md_run's neighbour-cell windows: [system][axis][cell - 1], bit c - 1
+    std::vector<uint64_t> wmask_h;
+    int cell_table = 1;             // JP_OPT_CELL_TABLE: lists follow md_run's neighbour-cell table
     DBuf<float4> d_lp, d_ccf;
     DBuf<int4> d_imgs;
     DBuf<int> d_err, d_lst;   // SHAKE error bits; list status words (jp_list.hpp)
@@ -340,6 +415,7 @@ int Engine::init(const jp_sys_desc *descs, int ns) {
                 P.rorg[k] = c11 - zz;
             }
             P.rr = 1.0 / siz;
+            md_cell_windows(P, nv, siz, rlim, wmask_h);
         }
         P.rlim2 = rlim * rlim;
         {
@@ -622,7 +698,8 @@ int Engine::init(const jp_sys_desc *descs, int ns) {
     AL(d_img, 3 * (size_t)natom); AL(d_cellid, natom); AL(d_subkey, natom); AL(d_refcell, 3 * (size_t)natom);
     AL(d_cnt, ncell + 1); AL(d_start, ncell + 1); AL(d_fill, ncell); AL(d_tmp, natom);
     AL(d_inum, natom); AL(d_perm, nslot); AL(d_cslot, ncell);
-    AL(d_ys, nslot); AL(d_qd, nslot); AL(d_xr, nslot); AL(d_imgs, nslot); AL(d_lp, nslot);
+    AL(d_ys, nslot); AL(d_qd, nslot); AL(d_xr, nslot); AL(d_imgs, nslot); AL(d_lp, nslot); AL(d_mcell, nslot);
+    AL(d_wmask, wmask_h.size());
     AL(d_cbb, 6 * (size_t)nclu); AL(d_ccen, nclu); AL(d_ccf, nclu); AL(d_lcnt, nclu); AL(d_cellext, ncell);
     AL(d_sysext, ns); AL(d_crec, ncell); AL(d_err, 1); AL(d_lst, 2 + (size_t)ns);
     AL(d_gslot, 3 * (size_t)nslot);
@@ -694,6 +771,7 @@ int Engine::init(const jp_sys_desc *descs, int ns) {
     r |= H2D(d_aend.p, aend.data(), 4 * aend.size());
     r |= H2D(d_cbeg.p, cbeg.data(), 4 * cbeg.size());
     r |= H2D(d_cend.p, cend.data(), 4 * cend.size());
+    r |= H2D(d_wmask.p, wmask_h.data(), 8 * wmask_h.size());
     if (r) return r;
     CK(hipMemsetAsync(d_v.p, 0, 8 * 3 * (size_t)natom, st));
     CK(hipStreamSynchronize(st));
@@ -737,7 +815,8 @@ int Engine::build_list(bool defer) {
                        d_start.p, d_cnt.p, d_tmp.p, d_inum.p, d_perm.p);
     hipLaunchKernelGGL(k_cslot, grid1(ncell, 256), 256, 0, st, ncell, d_cell_sys.p, d_sp.p, d_start.p, d_cslot.p);
     hipLaunchKernelGGL(k_fill_sorted, grid1(nslot, 256), 256, 0, st, nslot, d_slot_sys.p, d_sp.p, d_perm.p, d_x.p,
-                       d_img.p, d_chg.p, d_tflag.p, d_mol.p, d_ys.p, d_qd.p, d_xr.p, d_imgs.p, d_lp.p);
+                       d_img.p, d_chg.p, d_tflag.p, d_mol.p, d_ys.p, d_qd.p, d_xr.p, d_imgs.p, d_lp.p,
+                       d_refcell.p, d_mcell.p);
     hipLaunchKernelGGL(k_cbox, grid1(nclu, 256), 256, 0, st, nclu, d_clu_sys.p, d_sp.p, d_perm.p, d_cellid.p,
                        d_ccoord.p, d_ys.p, d_lp.p, d_cbb.p, d_ccen.p, d_ccf.p, d_cellext.p, d_sysext.p);
     hipLaunchKernelGGL(k_crec, grid1(ncell, 256), 256, 0, st, ncell, d_crank.p, d_cslot.p, d_cnt.p, d_cellext.p,
@@ -949,7 +1028,8 @@ template <int LW>
 static void launch_list8_lw(Engine &g, int hiwater) {
     hipLaunchKernelGGL(k_list8<LW>, grid1(g.nclu, LW), 64 * LW, 0, g.st, g.nclu, g.d_clu_sys.p, g.d_sp.p, g.d_cbb.p,
                        g.d_ccf.p, g.d_crec.p, g.d_sysext.p, g.d_lp.p, g.d_xr.p, g.d_imgs.p, g.d_xptr.p, g.d_xidx.p,
-                       g.ccap, hiwater, g.d_ent.p, reinterpret_cast<uint8_t *>(g.d_mb.p), g.d_lcnt.p, g.d_lst.p);
+                       g.d_mcell.p, g.cell_table ? g.d_wmask.p : nullptr, g.ccap, hiwater, g.d_ent.p,
+                       reinterpret_cast<uint8_t *>(g.d_mb.p), g.d_lcnt.p, g.d_lst.p);
 }
 
 static void launch_list8(Engine &g, int hiwater) {
@@ -1711,6 +1791,10 @@ static int set_option(Engine &E, int32_t option, int32_t value) {
             if (value < 64 || value % 64) return JP_EINVAL;   // whole 64-entry chunks
             E.ccap_force = value;
             return 0;
+        case JP_OPT_CELL_TABLE:
+            E.cell_table = value ? 1 : 0;
+            E.have_list = false;   // the next run or call builds the list again
+            return 0;
         default:
             return JP_EINVAL;
     }
@@ -1723,6 +1807,7 @@ static int get_option(const Engine &E, int32_t option, int32_t *value) {
         case JP_OPT_AUTOTUNE: *value = E.autotune; return 0;
         case JP_OPT_PAIR_WAVES: *value = E.pw; return 0;
         case JP_OPT_LIST_WAVES: *value = E.lw; return 0;
+        case JP_OPT_CELL_TABLE: *value = E.cell_table; return 0;
         default: return JP_EINVAL;
     }
 }

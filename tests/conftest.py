@@ -12,7 +12,7 @@ for p in (ROOT, os.path.join(ROOT, "oracle"), os.path.join(ROOT, "tests", "golde
 
 GOLDEN = os.path.join(ROOT, "tests", "golden")
 GOLDEN_NAMES = ["water_1500", "water_6k", "chain_3k", "chain_lmd", "ions_3k", "water_54", "chain_bonded",
-                "chain_lmd_bonded", "chain_tors", "chain_lmd_tors"]
+                "chain_lmd_bonded", "chain_tors", "chain_lmd_tors", "water_42"]
 
 
 def pytest_configure(config):
@@ -74,3 +74,49 @@ def oracle_monitor(o, s, x, v, nsteps, every=5, cano=None, scaled=False):
         if idloop % 10 == 0:
             assert o.celprp(x) == 0
     return np.array(rows)
+
+
+def md_cell_windows(L, nv, rc=12.0):
+    """md_run's periodic neighbour-cell table per axis (incel.f90:334-358,
+    456-508), restated from the reference for the tests: {cell: set of cells
+    in its window} for each edge in L; None where md_run writes out of bounds."""
+    import math
+    rlim = rc + 1.5
+    siz = 0.5 * rlim
+    m = math.ceil(rlim / siz)
+    ic = int(math.floor(nv * 0.5 + 0.5))
+    out = []
+    for edge in L:
+        h = 0.5 * edge / siz
+        m1, m2 = math.ceil(ic - h), math.ceil(ic + h)
+        mdel = 1 if math.ceil(h) - h > 0.5 else 2
+        if m1 < 1 or m2 > nv:
+            return None
+        ii = m2 - m1 + 1
+        ri = np.float32(1.0) / np.float32(ii)
+        win = {}
+        for c in range(m1, m2 + 1):
+            lo, hi = c - m, c + m
+            if lo <= m1:
+                lo -= mdel
+            if hi >= m2:
+                hi += mdel
+            w = set()
+            for k in range(lo, hi + 1):
+                kk = k - ii * int(math.floor(np.float32(k - m1) * ri))
+                if kk < m1 or kk > m2:
+                    return None
+                w.add(kk)
+            win[c] = w
+        out.append(win)
+    return out
+
+
+def admissible(windows, ci, cj):
+    """Pairs (i, j) whose md_run cells are in md_run's table: cj's index in
+    ci's window on every axis (ci, cj: (n, 3) 1-based cells)."""
+    ok = np.ones(len(ci), bool)
+    for a in range(3):
+        w = windows[a]
+        ok &= np.array([int(b) in w[int(c)] for c, b in zip(ci[:, a], cj[:, a])], bool)
+    return ok

@@ -71,6 +71,9 @@ RECIPES = {
     "ions_3k": ("water_box", dict(nwat=900, L=30.0, seed=13, n_ion_pairs=10), dict(traj=0)),
     # box edge a multiple of rlim = 13.5 A (md_run's phantom-cell case)
     "water_54": ("water_box", dict(nwat=5200, L=54.0, seed=14), dict(traj=0)),
+    # 42 A edge: md_run's periodic cell table (8 cells of 6.75 A for 42 A) never tests
+    # pairs between some cells facing each other across the boundary, inside the cutoff
+    "water_42": ("water_box", dict(nwat=2469, L=42.0, seed=19), dict(traj=10)),
 }
 
 
@@ -91,10 +94,11 @@ def fingerprint(s: systems.System) -> str:
 def run_md(workdir: str, s: systems.System, md: dict, shake: bool = True) -> str:
     files = refio.write_inputs(workdir, s, shake=shake)
     refio.write_md_inp(os.path.join(workdir, "md.inp"), files, s, md)
-    env = dict(os.environ, OMP_NUM_THREADS="1")
+    env = dict(os.environ, OMP_NUM_THREADS="1", OMP_STACKSIZE="512M")
+    # md_run keeps per-atom automatic arrays on the stack: lift the limit for large systems
     with open(os.path.join(workdir, "md.inp")) as fin, open(os.path.join(workdir, "md.log"), "w") as fout:
-        subprocess.run([MD_RUN], stdin=fin, stdout=fout, stderr=subprocess.STDOUT, cwd=workdir, env=env,
-                       check=True, timeout=3600)
+        subprocess.run(["bash", "-c", f"ulimit -s unlimited; exec {MD_RUN}"], stdin=fin, stdout=fout,
+                       stderr=subprocess.STDOUT, cwd=workdir, env=env, check=True, timeout=3600)
     return open(os.path.join(workdir, "md.log")).read()
 
 

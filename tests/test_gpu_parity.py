@@ -112,7 +112,7 @@ def test_shake_bit_exact_vs_oracle():
 
 
 @pytest.mark.parametrize("name", ["water_1500", "water_6k", "chain_3k", "chain_lmd", "chain_bonded",
-                                  "chain_lmd_bonded", "chain_tors", "chain_lmd_tors"])
+                                  "chain_lmd_bonded", "chain_tors", "chain_lmd_tors", "water_42"])
 def test_nve_trajectory_vs_reference(golden, name):
     s, f = golden(name)
     print(f"case {name}", flush=True)
@@ -365,3 +365,25 @@ def test_energy_log_vs_oracle_and_reference(golden, name, method):
     for val, col in ((log["ekin"], 4), (ep_un, 2), (log["ekin"] + ep_un, 1)):
         ulp = 10.0 ** (np.floor(np.log10(np.abs(mon[:, col]))) - 6)   # one unit of the 7th digit
         assert np.all(np.abs(val - mon[:, col]) <= 1.0001 * ulp), (col, val, mon[:, col])
+
+
+@pytest.mark.parametrize("prec", [engine.FP64, engine.FP32])
+def test_complete_list_option(golden, prec):
+    """JP_OPT_CELL_TABLE = 0: on the 42 A box, where md_run's cell table
+    drops pairs inside the cutoff, the list is the complete minimum-image set
+    (the oracle's brute force) and the forces include the dropped pairs."""
+    from j_presto_amd import native
+    s, f = golden("water_42")
+    print(f"case complete list prec={prec}", flush=True)
+    o = Oracle(s)
+    bp, bi = o.list_brute(s.x)
+    md = engine.MdSystem(s, precision=prec)
+    assert md.get_option(native.JP_OPT_CELL_TABLE) == 1
+    md.set_option(native.JP_OPT_CELL_TABLE, 0)
+    assert md.celprp() == 0
+    cell, ptr, idx = md.get_list()
+    assert np.array_equal(ptr, bp) and np.array_equal(idx, bi)
+    g, ee, ev, eh, ier = md.cmmdo(cord=s.x)
+    md.close()
+    # the dropped pairs inside the cutoff change the forces by far more than round-off
+    assert rel_rms(g[0], f["force_grad"]) > 1e-4
