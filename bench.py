@@ -88,10 +88,15 @@ def run_gpu(system, precision, steps, warmup, state=None, pair_steps=200, autotu
         _, ier = md.runmd(warmup, update=UPDATE)
         assert ier == 0, ier
     md.celprp()
+    # the timed run starts at a step count that is a multiple of the list and
+    # energy intervals, and every step graph it replays is captured beforehand
+    md.set_step(0)
+    md.prepare_graphs(steps, update=UPDATE, nmon=NMON)
     t0 = time.perf_counter()
     log, ier = md.runmd(steps, update=UPDATE, nmon=NMON)
     t1 = time.perf_counter()
     assert ier == 0, ier
+    captures = md.get_option(native.JP_OPT_GRAPH_CAPTURES)
     tm = md.timing()
     n, nlj, nlisted = md.count_pairs(detail=True)
     # pair-kernel duration: HIP events around pair_steps launches back to back
@@ -99,7 +104,47 @@ def run_gpu(system, precision, steps, warmup, state=None, pair_steps=200, autotu
     shapes = {"pair_waves": md.get_option(native.JP_OPT_PAIR_WAVES),
               "list_waves": md.get_option(native.JP_OPT_LIST_WAVES), "autotuned": bool(autotune)}
     md.close()
-    return dict(wall=t1 - t0, timing=tm, pairs=n, pairs_lj=nlj, listed=nlisted, log=log, t_pair=tp, shapes=shapes)
+    return dict(wall=t1 - t0, timing=tm, pairs=n, pairs_lj=nlj, listed=nlisted, log=log, t_pair=tp, shapes=shapes,
+                graph_captures_timed=captures)
+
+
+def accuracy(system, state):
+    """FP32/fixed-point forces and energies against the plain FP64 GPU path
+    (itself within 1e-12 of the FP64 oracle) at the benchmark's start state:
+    relative RMS of the 3N force components, relative ELE15 and VDW15 error."""
+    import numpy as np
+    from j_presto_amd import engine
+    out = {}
+    res = {}
+    for name, prec in (("fp64", engine.FP64), ("fp32", engine.FP32)):
+        md = engine.MdSystem(system, dt=DT_FS, precision=prec)
+        md.set_state(*state)
+        assert md.celprp() == 0
+        g, et, ier = md.bonded()
+        assert ier == 0
+        g, ee, ev, eh, ier = md.cmmdo(grad=g)
+        assert ier == 0
+        md.close()
+        res[name] = (g, ee.sum(), ev.sum())
+    g64, e64, v64 = res["fp64"]
+    g32, e32, v32 = res["fp32"]
+    out["force_rel_rms_fp32_vs_fp64"] = float(np.sqrt(((g32 - g64) ** 2).sum() / (g64 ** 2).sum()))
+    out["ele15_rel_err"] = float(abs(e32 - e64) / abs(e64))
+    out["vdw15_rel_err"] = float(abs(v32 - v64) / abs(v64))
+    out["ele15_fp64"] = float(e64)
+    out["vdw15_fp64"] = float(v64)
+    return out
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def cpu_reference(system, nsteps: int):
@@ -125,7 +170,7 @@ def cpu_reference(system, nsteps: int):
         log = open(os.path.join(tmp, "md.log")).read()
         speed = float(re.search(r"SPEED \(ns/day\)\s*:\s*([0-9.Ee+-]+)", log).group(1))
         loop = float(re.search(r"^\s+MD LOOP\s+:\s+([0-9.Ee+-]+)", log, re.M).group(1))
-        return dict(value=speed, unit="ns/day", cores=cores, kind="reference",
+        return dict(value=speed, unit="ns/day", cores=cores, kind="reference", cpu_model=cpu_model(),
                     sample=f"md_run (flang -O2 build of the reference sources), {nsteps} MICR steps of this "
                            f"workload, PARALV=DOUB N_SIMD=64, {cores} OpenMP threads, MD loop {loop:.2f} s")
     finally:
@@ -358,6 +403,8 @@ def main():
                           "bytes_per_rebuild": list_bytes, "avg_rebuild_us": round(t_list * 1e6, 2)},
         "step_breakdown_ms": {"pair": round(1e3 * t_pair, 4), "list_per_step": round(1e3 * r32["timing"]["list"] / args.steps, 4),
                               "rest": round(1e3 * (r32["wall"] - r32["timing"]["list"]) / args.steps - 1e3 * t_pair, 4)},
+        "graph_captures_timed": r32["graph_captures_timed"],
+        "accuracy_vs_fp64": accuracy(s, state),
         "cpu_baseline": None,
     }
     if rank == 0 and not args.no_cpu and args.workload == 2 and not cano:

@@ -186,6 +186,13 @@ int jp_shake(jp_ctx *ctx, double *cord, const double *cord_old);
 int jp_run_nve(jp_ctx *ctx, int32_t nsteps, int32_t update, int32_t nmon, int32_t precision,
                jp_energy_log *log);
 
+/* Captures, ahead of time, the HIP graphs of plain steps that a following
+ * jp_run_nve with the same arguments will replay from the current step
+ * counter, so that the run itself captures none (unless a list outgrows its
+ * segments).  No step is run.  A preparation helper with no counterpart in
+ * the reference. */
+int jp_prepare_graphs(jp_ctx *ctx, int32_t nsteps, int32_t update, int32_t nmon, int32_t precision);
+
 /* md_run's step counter idloop: inirst sets it from the restart file's loop
  * number (inprst call, md.f90:1132), so that list rebuilds and OUTLOG steps of a
  * continued run fall where md_run's do.  A new context starts at 0. */
@@ -239,6 +246,7 @@ int jp_batch_create(const jp_sys_desc *descs, int32_t nsys, jp_batch **out);
 int jp_batch_run_nve(jp_batch *b, int32_t nsteps, int32_t update, int32_t nmon, int32_t precision,
                      jp_energy_log *log /* [nsys][nsteps/nmon + 1] */);
 int jp_batch_set_step(jp_batch *b, int64_t idloop);   /* jp_set_step for the batch's shared counter */
+int jp_batch_prepare_graphs(jp_batch *b, int32_t nsteps, int32_t update, int32_t nmon, int32_t precision);
 int jp_batch_set_state(jp_batch *b, int32_t isys, const double *cord, const double *vel);
 int jp_batch_get_state(jp_batch *b, int32_t isys, double *cord, double *vel);
 int jp_batch_last_timing(jp_batch *b, double *t_total);
@@ -285,6 +293,7 @@ int jp_batch_destroy(jp_batch *b);
 #define JP_OPT_LIST_WAVES 4
 #define JP_OPT_LIST_SEGMENT 5
 #define JP_OPT_CELL_TABLE 6
+#define JP_OPT_GRAPH_CAPTURES 7   /* read only: step graphs captured during the last run */
 int jp_set_option(jp_ctx *ctx, int32_t option, int32_t value);
 int jp_batch_set_option(jp_batch *b, int32_t option, int32_t value);
 int jp_get_option(jp_ctx *ctx, int32_t option, int32_t *value);

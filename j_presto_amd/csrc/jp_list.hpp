@@ -232,7 +232,7 @@ __device__ __forceinline__ bool exact_in(const SysP &P, const double4 &a, const 
 // of excluded pairs met within rlim (nremoved) are reported for md_run's
 // checks.
 template <int LW>
-__global__ void __launch_bounds__(64 * LW)
+__global__ void __launch_bounds__(64 * LW) __attribute__((amdgpu_waves_per_eu(6)))
 k_list8(int nclu, const int *__restrict__ clu_sys, const SysP *__restrict__ sp, const double *__restrict__ cbb,
         const float4 *__restrict__ ccf, const int4 *__restrict__ crec, const unsigned *__restrict__ sysext,
         const float4 *__restrict__ lp, const double4 *__restrict__ xr, const int4 *__restrict__ imgs,
@@ -251,12 +251,42 @@ k_list8(int nclu, const int *__restrict__ clu_sys, const SysP *__restrict__ sp, 
     if (ci >= nclu) return;   // whole wavefront; no block-wide barrier below
     const int s = clu_sys[ci];
     const SysP &P = sp[s];
-    // axes on which md_run's neighbour-cell table drops pairs within rlim
+    // Axes on which md_run's neighbour-cell table drops pairs within rlim.
+    // Per such axis the i atoms' md_run cells usually take one or two
+    // neighbouring values lo and lo+1: then their windows are two uniform
+    // masks and hm (the atoms in cell lo+1) picks between them; a cluster
+    // spread wider takes the windows of its 8 atoms from LDS.
     const int ctab = wmask ? P.ctab : 0;
-    if (ctab && l < 3 * JP_CL) {
-        const int k = l & 7, a = l >> 3;
-        const int c = (mcell[ci * JP_CL + k] >> (8 * a)) & 0xff;
-        s_w[w][a][k] = c >= 1 ? wmask[(3 * (size_t)s + a) * 64 + c - 1] : ~0ull;
+    __shared__ uint64_t s_wlo[LW][3], s_whi[LW][3];
+    __shared__ uint32_t s_hm[LW][3];
+    bool wide = false;
+    if (ctab) {
+        const int mc = l < JP_CL ? mcell[ci * JP_CL + l] : 0;
+        const bool valid = l < JP_CL && imgs[ci * JP_CL + l].w >= 0;
+        if (l < 3 * JP_CL) {
+            const int k = l & 7, a = l >> 3;
+            const int c = (mcell[ci * JP_CL + k] >> (8 * a)) & 0xff;
+            s_w[w][a][k] = c >= 1 ? wmask[(3 * (size_t)s + a) * 64 + c - 1] : ~0ull;
+        }
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            if (!((ctab >> a) & 1)) continue;
+            int c = valid ? (mc >> (8 * a)) & 0xff : 255, d = valid ? (mc >> (8 * a)) & 0xff : 0;
+#pragma unroll
+            for (int o = 1; o < JP_CL; o <<= 1) {
+                c = min(c, __shfl_xor(c, o, 64));
+                d = max(d, __shfl_xor(d, o, 64));
+            }
+            const int lo = __builtin_amdgcn_readfirstlane(c), hi = __builtin_amdgcn_readfirstlane(d);
+            if (lo > hi) continue;   // no atom
+            if (hi - lo > 1) wide = true;
+            const uint32_t h = (uint32_t)__ballot(valid && ((mc >> (8 * a)) & 0xff) == hi && hi != lo) & 0xffu;
+            if (l == 0) {
+                s_hm[w][a] = h;
+                s_wlo[w][a] = wmask[(3 * (size_t)s + a) * 64 + lo - 1];
+                s_whi[w][a] = wmask[(3 * (size_t)s + a) * 64 + hi - 1];
+            }
+        }
     }
     const CluRange r = clu_range(P, cbb, ci, __uint_as_float(sysext[s]));
     if (r.bad && l == 0) atomicOr(lst, JP_ERR_CELL);   // runaway coordinates: md_run's ier 28
@@ -322,7 +352,7 @@ k_list8(int nclu, const int *__restrict__ clu_sys, const SysP *__restrict__ sp, 
 
     // test one queued j-cluster per 8 lanes (lane = atom ja of queue entry qe,
     // valid if qv; q = its lp record, loaded by the caller)
-    auto test = [&](bool qv, uint32_t qe, float4 q) __attribute__((always_inline)) {
+    auto test = [&](bool qv, uint32_t qe, float4 q, int cj) __attribute__((always_inline)) {
         const int jc = (int)(qe & JP_SLOT_MASK);
         const uint32_t code = qe >> JP_SLOT_BITS;
         const int js = jc * JP_CL + ja;
@@ -377,19 +407,24 @@ k_list8(int nclu, const int *__restrict__ clu_sys, const SysP *__restrict__ sp, 
         // (incel.f90:456-508): without an image shift a pair within rlim is
         // always in it; with one, j's md_run cell must lie in the i atom's
         // window on every shifted axis where the table drops pairs
-        if (ctab && __ballot(meta >= 0 && code != 13u)) {
-            if (meta >= 0 && code != 13u) {
+        if (ctab && __ballot(m != 0 && code != 13u)) {
+            if (m != 0 && code != 13u) {
                 int sh[3];
                 shift_of(code, sh[0], sh[1], sh[2]);
-                const int cj = mcell[js];
                 uint32_t adm = 0xffu;
 #pragma unroll
                 for (int a = 0; a < 3; ++a) {
                     if (!((ctab >> a) & 1) || sh[a] == 0) continue;
                     const int c = ((cj >> (8 * a)) & 0xff) - 1;
-                    uint32_t b = 0;
+                    uint32_t b;
+                    if (!wide) {
+                        const uint32_t h = s_hm[w][a];
+                        b = (((s_wlo[w][a] >> c) & 1ull) ? (~h & 0xffu) : 0u) | (((s_whi[w][a] >> c) & 1ull) ? h : 0u);
+                    } else {
+                        b = 0;
 #pragma unroll
-                    for (int k = 0; k < JP_CL; ++k) b |= (uint32_t)((s_w[w][a][k] >> c) & 1ull) << k;
+                        for (int k = 0; k < JP_CL; ++k) b |= (uint32_t)((s_w[w][a][k] >> c) & 1ull) << k;
+                    }
                     adm &= b;
                 }
                 m &= adm;
@@ -441,10 +476,14 @@ k_list8(int nclu, const int *__restrict__ clu_sys, const SysP *__restrict__ sp, 
         const bool qv0 = jsub < nq, qv1 = 8 + jsub < nq;
         const uint32_t qe0 = qv0 ? s_q[w][(qh + jsub) & (JP_QCAP - 1)] : 0u;
         const uint32_t qe1 = qv1 ? s_q[w][(qh + 8 + jsub) & (JP_QCAP - 1)] : 0u;
-        const float4 q0 = lp[qv0 ? (int)(qe0 & JP_SLOT_MASK) * JP_CL + ja : 0];
-        const float4 q1 = lp[qv1 ? (int)(qe1 & JP_SLOT_MASK) * JP_CL + ja : 0];
-        test(qv0, qe0, q0);
-        if (nq > 8) test(qv1, qe1, q1);
+        const int j0 = qv0 ? (int)(qe0 & JP_SLOT_MASK) * JP_CL + ja : 0;
+        const int j1 = qv1 ? (int)(qe1 & JP_SLOT_MASK) * JP_CL + ja : 0;
+        const float4 q0 = lp[j0];
+        const float4 q1 = lp[j1];
+        // md_run cells of the j atoms, only where its cell table drops pairs
+        const int c0 = ctab ? mcell[j0] : 0, c1 = ctab ? mcell[j1] : 0;
+        test(qv0, qe0, q0, c0);
+        if (nq > 8) test(qv1, qe1, q1, c1);
     };
 
     // One loop, one site per phase, highest priority first: drain 16 queued

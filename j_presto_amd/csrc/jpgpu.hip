@@ -156,6 +156,8 @@ struct Engine {
     hipStream_t st = nullptr;
     hipEvent_t ev[8];               // [0,1] run, [2,3] list build, [4,5] host-launched pair kernels
     bool pe_pending = false;        // ev[4,5] bracket a pair launch not yet added to t_pair
+    int n_captures = 0;             // step graphs captured so far
+    int captures_last_run = 0;      // of them, during the last run_nve
     double t_total = 0, t_pair = 0, t_list = 0, t_step = 0;
     int n_pair = 0, n_list = 0, n_pair_steps = 0;   // timed pair launches, steps of the last run
     long long idloop = 0;          // md.f90 idloop: steps done, kept across run calls
@@ -313,6 +315,7 @@ struct Engine {
     int energies(int prec, std::vector<double> &eel, std::vector<double> &evd, std::vector<double> &eex);
     int shake(int &ier_out);
     int run_nve(int nsteps, int update, int nmon, int prec, jp_energy_log *log);
+    int prepare_graphs(int nsteps, int update, int nmon, int prec);
     int integrate(int prec, bool mon);   // kick + SHAKE + v update of one step (after the forces)
     int mon_record(int step, int nsteps, int nmon, int nlog, const std::vector<double> &eel,
                    const std::vector<double> &evd, const std::vector<double> &et, jp_energy_log *log);
@@ -1331,6 +1334,7 @@ Engine::StepGraph *Engine::step_graph(int n, int prec) {
     }
     StepGraph g;
     g.nsteps = n; g.prec = prec; g.ccap = ccap; g.ent = d_ent.p; g.variant = variant;
+    ++n_captures;
     hipGraph_t graph = nullptr;
     if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) return nullptr;
     int r = 0;
@@ -1408,6 +1412,7 @@ int Engine::run_nve(int nsteps, int update, int nmon, int prec, jp_energy_log *l
     CK(hipMemsetAsync(d_err.p, 0, 4, st));
     t_pair = 0; t_step = 0; n_pair = 0; n_pair_steps = nsteps;
     pe_pending = false;
+    const int cap0 = n_captures;
     const double tl0 = t_list;
     // records of the monitoring steps first, the rest marked unused
     if (nmon > 0 && log)
@@ -1519,8 +1524,39 @@ int Engine::run_nve(int nsteps, int update, int nmon, int prec, jp_energy_log *l
     const double tl = t_list - tl0;
     t_step = t_total - (n_pair ? t_pair / n_pair * nsteps : 0.0) - tl;
     t_list = tl;
+    captures_last_run = n_captures - cap0;
     if (err & 8) return JP_IER_SHAKE_SING;
     if (err & 4) return JP_IER_SHAKE_CONV;
+    return 0;
+}
+
+// Captures the step graphs a following run_nve(nsteps, update, nmon, prec)
+// from the current step counter will replay: the same segment schedule,
+// walked without running a step.
+int Engine::prepare_graphs(int nsteps, int update, int nmon, int prec) {
+    int r;
+    if (nsteps < 0 || update <= 0) return JP_EINVAL;
+    if (!use_graphs) return 0;
+    if (!have_list && (r = build_list())) return r;
+    if ((r = set_factors(-1))) return r;
+    long long id = idloop;
+    int step = 1;
+    while (step <= nsteps) {
+        const bool mon = nmon > 0 && (id + 1) % nmon == 0;
+        int nplain = 0;
+        if (!mon) {
+            nplain = std::min(nsteps - step + 1, (int)(update - id % update));
+            if (nmon > 0) nplain = std::min(nplain, (int)(nmon - (id + 1) % nmon));
+        }
+        if (nplain >= 2) {
+            if (!step_graph(nplain, prec)) return 0;   // capture unavailable: the run launches kernels itself
+            step += nplain;
+            id += nplain;
+        } else {
+            ++step;
+            ++id;
+        }
+    }
     return 0;
 }
 
@@ -1695,6 +1731,16 @@ int jp_time_pair(jp_ctx *ctx, int32_t precision, int32_t reps, double *seconds) 
     return ctx->e.time_pair(precision, reps, *seconds);
 }
 
+int jp_prepare_graphs(jp_ctx *ctx, int32_t nsteps, int32_t update, int32_t nmon, int32_t precision) {
+    if (!ctx || (precision != JP_PREC_FP64 && precision != JP_PREC_FP32_FIXED)) return JP_EINVAL;
+    return ctx->e.prepare_graphs(nsteps, update, nmon, precision);
+}
+
+int jp_batch_prepare_graphs(jp_batch *b, int32_t nsteps, int32_t update, int32_t nmon, int32_t precision) {
+    if (!b || (precision != JP_PREC_FP64 && precision != JP_PREC_FP32_FIXED)) return JP_EINVAL;
+    return b->e.prepare_graphs(nsteps, update, nmon, precision);
+}
+
 int jp_set_step(jp_ctx *ctx, int64_t idloop) {
     if (!ctx || idloop < 0) return JP_EINVAL;
     ctx->e.idloop = idloop;
@@ -1808,6 +1854,7 @@ static int get_option(const Engine &E, int32_t option, int32_t *value) {
         case JP_OPT_PAIR_WAVES: *value = E.pw; return 0;
         case JP_OPT_LIST_WAVES: *value = E.lw; return 0;
         case JP_OPT_CELL_TABLE: *value = E.cell_table; return 0;
+        case JP_OPT_GRAPH_CAPTURES: *value = E.captures_last_run; return 0;
         default: return JP_EINVAL;
     }
 }

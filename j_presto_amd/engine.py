@@ -147,6 +147,11 @@ class MdSystem:
         out = log_to_array(log, nl)
         return out[out["step"] >= 0], ier
 
+    def prepare_graphs(self, nsteps: int, update: int = 10, nmon: int = 0, precision: Optional[int] = None):
+        """Capture the step graphs runmd(nsteps, update, nmon) will replay."""
+        prec = self.precision if precision is None else precision
+        native.check("jp_prepare_graphs", self.lib.jp_prepare_graphs(self.ctx, nsteps, update, nmon, prec))
+
     def set_step(self, idloop: int):
         """md_run's step counter (a restart's loop number); list rebuilds and
         monitoring steps follow it."""
@@ -289,6 +294,10 @@ class Ensemble:
 
     def set_step(self, idloop: int):
         native.check("jp_batch_set_step", self.lib.jp_batch_set_step(self.b, int(idloop)))
+
+    def prepare_graphs(self, nsteps: int, update: int = 10, nmon: int = 0, precision: Optional[int] = None):
+        prec = self.precision if precision is None else precision
+        native.check("jp_batch_prepare_graphs", self.lib.jp_batch_prepare_graphs(self.b, nsteps, update, nmon, prec))
 
     def set_graphs(self, on: bool):
         native.check("jp_batch_set_option", self.lib.jp_batch_set_option(self.b, native.JP_OPT_GRAPHS, int(bool(on))))

@@ -34,6 +34,8 @@ SIGNATURES = {
     "jp_last_timing": (C.c_int, [vp, f64p, f64p, f64p, f64p]),
     "jp_count_pairs": (C.c_int, [vp, i64p, i64p, i64p]),
     "jp_set_step": (C.c_int, [vp, C.c_int64]),
+    "jp_prepare_graphs": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
+    "jp_batch_prepare_graphs": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
     "jp_get_step": (C.c_int, [vp, i64p]),
     "jp_batch_set_step": (C.c_int, [vp, C.c_int64]),
     "jp_time_pair": (C.c_int, [vp, C.c_int32, C.c_int32, f64p]),
@@ -51,7 +53,7 @@ SIGNATURES = {
     "jp_batch_get_option": (C.c_int, [vp, C.c_int32, i32p]),
 }
 JP_OPT_GRAPHS, JP_OPT_AUTOTUNE, JP_OPT_PAIR_WAVES, JP_OPT_LIST_WAVES, JP_OPT_LIST_SEGMENT = 1, 2, 3, 4, 5
-JP_OPT_CELL_TABLE = 6
+JP_OPT_CELL_TABLE, JP_OPT_GRAPH_CAPTURES = 6, 7
 
 _lib = None
 

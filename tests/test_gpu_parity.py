@@ -387,3 +387,32 @@ def test_complete_list_option(golden, prec):
     md.close()
     # the dropped pairs inside the cutoff change the forces by far more than round-off
     assert rel_rms(g[0], f["force_grad"]) > 1e-4
+
+
+def test_prepared_graphs_are_replayed():
+    """jp_prepare_graphs captures every step graph of the following run ahead
+    of time: the run captures none and gives the same bits as one without it."""
+    from j_presto_amd import native
+    s = systems.water_box(700, 28.0, 23, n_ion_pairs=2)
+    print("case prepared graphs", flush=True)
+    out = []
+    for prep in (False, True):
+        md = engine.MdSystem(s, precision=engine.FP32)
+        _, ier = md.runmd(7, update=10)
+        assert ier == 0
+        # a rebuild here takes the segment size the first lists asked for, so
+        # that no list grows during the run (a grown list needs new graphs)
+        assert md.celprp() == 0
+        md.set_step(0)
+        if prep:
+            md.prepare_graphs(60, update=10, nmon=25)
+        log, ier = md.runmd(60, update=10, nmon=25)
+        assert ier == 0
+        assert list(log["step"]) == [25, 50]
+        caps = md.get_option(native.JP_OPT_GRAPH_CAPTURES)
+        print(f"prepared={prep}: {caps} graphs captured in the run", flush=True)
+        assert (caps == 0) if prep else (caps > 0)
+        out.append((md.get_state(), log["etot"].copy()))
+        md.close()
+    assert np.array_equal(out[0][0][0], out[1][0][0]) and np.array_equal(out[0][0][1], out[1][0][1])
+    assert np.array_equal(out[0][1], out[1][1])
