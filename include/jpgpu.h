@@ -236,6 +236,16 @@ int jp_time_pair(jp_ctx *ctx, int32_t precision, int32_t reps, double *seconds);
  * nlj and nlisted may be NULL. */
 int jp_count_pairs(jp_ctx *ctx, int64_t *npairs, int64_t *nlj, int64_t *nlisted);
 
+/* jp_count_pairs for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * No counterpart in the reference.  It calls no allocating routine: its
+ * per-cluster partial counts live in a fixed scratch array the library holds
+ * on every GPU it runs on (6 MB, up to 262,144 i-clusters of 8 atoms, i.e.
+ * about 2M atoms); a larger system returns JP_EINVAL.  Results equal
+ * jp_count_pairs's.  The scratch is shared by every call on one GPU: calls
+ * that overlap on one GPU (several host threads, several contexts) race. */
+int jp_count_pairs_noalloc(jp_ctx *ctx, int64_t *npairs, int64_t *nlj, int64_t *nlisted);
+
 /*
  * Batched ensemble (replaces j_presto_mpi / j_presto_master: N independent
  * md_run jobs, j_presto_mpi:27-44): every system of the batch shares each

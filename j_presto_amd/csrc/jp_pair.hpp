@@ -466,14 +466,32 @@ k_pair_f64(int nclu, const int *__restrict__ clu_sys, const SysP *__restrict__ s
     }
 }
 
+// Sums of the per-cluster counts, in cluster order, by one block:
+// tot[k] = sum_c part[3c + k].
+__global__ void __launch_bounds__(256) k_sum3(int nclu, const unsigned long long *__restrict__ part,
+                                              unsigned long long *__restrict__ tot) {
+    __shared__ unsigned long long s[3][256];
+    unsigned long long a[3] = {0, 0, 0};
+    for (int c = threadIdx.x; c < nclu; c += 256)
+        for (int k = 0; k < 3; ++k) a[k] += part[3 * (size_t)c + k];
+    for (int k = 0; k < 3; ++k) s[k][threadIdx.x] = a[k];
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o)
+            for (int k = 0; k < 3; ++k) s[k][threadIdx.x] += s[k][threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x < 3) tot[threadIdx.x] = s[threadIdx.x][0];
+}
+
 // Unique pairs with r <= rc (each pair once) at the current coordinates:
 // out[0] all pairs, out[1] pairs where both atoms carry LJ, out[2] listed
 // pairs (r <= rlim at the last rebuild, each once).  One wavefront per
 // i-cluster.
-__global__ void __launch_bounds__(64)
-k_count_pairs(int nclu, const int *__restrict__ clu_sys, const SysP *__restrict__ sp, const JAtom *__restrict__ ys,
-              int ccap, const int2 *__restrict__ lcnt, const uint2 *__restrict__ ent,
-              unsigned long long *__restrict__ out) {
+template <bool PART>
+__device__ __forceinline__ void count_pairs_body(int nclu, const int *__restrict__ clu_sys, const SysP *__restrict__ sp,
+                                                 const JAtom *__restrict__ ys, int ccap, const int2 *__restrict__ lcnt,
+                                                 const uint2 *__restrict__ ent, unsigned long long *__restrict__ out) {
     const int c = blockIdx.x;
     if (c >= nclu) return;
     const SysP &P = sp[clu_sys[c]];
@@ -505,8 +523,30 @@ k_count_pairs(int nclu, const int *__restrict__ clu_sys, const SysP *__restrict_
     nlj = wave_sum(nlj);
     nl = wave_sum(nl);
     if (l == 0) {
-        atomicAdd(out, n);
-        atomicAdd(out + 1, nlj);
-        atomicAdd(out + 2, nl);
+        if (PART) {
+            out[3 * (size_t)c] = n;
+            out[3 * (size_t)c + 1] = nlj;
+            out[3 * (size_t)c + 2] = nl;
+        } else {
+            atomicAdd(out, n);
+            atomicAdd(out + 1, nlj);
+            atomicAdd(out + 2, nl);
+        }
     }
+}
+
+__global__ void __launch_bounds__(64)
+k_count_pairs(int nclu, const int *__restrict__ clu_sys, const SysP *__restrict__ sp, const JAtom *__restrict__ ys,
+              int ccap, const int2 *__restrict__ lcnt, const uint2 *__restrict__ ent,
+              unsigned long long *__restrict__ out) {
+    count_pairs_body<false>(nclu, clu_sys, sp, ys, ccap, lcnt, ent, out);
+}
+
+// Per-cluster counts of the pairs k_count_pairs counts: out[3c + 0..2] for
+// cluster c (plain stores, no atomics).
+__global__ void __launch_bounds__(64)
+k_count_pairs_part(int nclu, const int *__restrict__ clu_sys, const SysP *__restrict__ sp, const JAtom *__restrict__ ys,
+                   int ccap, const int2 *__restrict__ lcnt, const uint2 *__restrict__ ent,
+                   unsigned long long *__restrict__ out) {
+    count_pairs_body<true>(nclu, clu_sys, sp, ys, ccap, lcnt, ent, out);
 }

@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -63,6 +64,33 @@ inline unsigned grid1(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b)
 // finish_list(false): the deferred rebuild failed or overflowed; rebuild from
 // the coordinates it was meant for (internal, never returned to callers)
 #define JP_LIST_REDO (-50)
+
+// Scratch of the entry point that allocates nothing (jp_count_pairs_noalloc):
+// three per-cluster pair counts for up to JP_NOALLOC_MAX_CLU i-clusters (2M
+// atoms) and the three totals, 6 MB held by every GPU the library runs on
+// until the process ends.  Host code reaches it only through the address
+// hipGetSymbolAddress returns on each device (noalloc_scratch).
+#define JP_NOALLOC_MAX_CLU (1 << 18)
+static __device__ __align__(256) unsigned long long scratch[3 * (size_t)JP_NOALLOC_MAX_CLU + 4];
+static std::mutex noalloc_mu;
+static unsigned long long *noalloc_ptr[64];   // per device
+
+// The device address of scratch on the current device (looked up once per
+// device), or nullptr.
+static unsigned long long *noalloc_scratch() {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
+    std::lock_guard<std::mutex> g(noalloc_mu);
+    if (!noalloc_ptr[dev]) {
+        void *p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess) {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+        noalloc_ptr[dev] = static_cast<unsigned long long *>(p);
+    }
+    return noalloc_ptr[dev];
+}
 
 // md_run's periodic neighbour-cell table (incel.f90:334-358 and 456-508).
 // The list of cell (ix,iy,iz) holds every atom of the cells whose index on
@@ -338,6 +366,7 @@ struct Engine {
     void drop_graphs();
     int get_list(int32_t *cell3, int64_t *npairs, int32_t *ptr, int32_t *idx, int64_t cap);
     int count_pairs(int64_t *np, int64_t *nlj, int64_t *nlist);
+    int count_pairs_noalloc(int64_t *np, int64_t *nlj, int64_t *nlist);
     int time_pair(int prec, int reps, double &sec);
 };
 
@@ -349,6 +378,7 @@ int Engine::init(const jp_sys_desc *descs, int ns) {
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return JP_ENODEV;
     CK(hipStreamCreate(&st));
     for (auto &e : ev) CK(hipEventCreate(&e));
+    if (!noalloc_scratch()) return JP_ENODEV;   // set-up of jp_count_pairs_noalloc's scratch on this device
     nsys = ns;
     hs.resize(ns);
     sp.resize(ns);
@@ -1600,6 +1630,27 @@ int Engine::get_list(int32_t *cell3, int64_t *npairs, int32_t *ptr, int32_t *idx
     return 0;
 }
 
+// jp_count_pairs without allocation: per-cluster counts into the static
+// scratch, summed by one block in cluster order.
+int Engine::count_pairs_noalloc(int64_t *np, int64_t *nlj, int64_t *nlist) {
+    if (!have_list) return JP_EINVAL;
+    if (nclu > JP_NOALLOC_MAX_CLU) return JP_EINVAL;   // beyond the scratch
+    unsigned long long *sc = noalloc_scratch();
+    if (!sc) return JP_ENODEV;
+    unsigned long long *tot = sc + 3 * (size_t)JP_NOALLOC_MAX_CLU;
+    hipLaunchKernelGGL(k_count_pairs_part, nclu, 64, 0, st, nclu, d_clu_sys.p, d_sp.p, d_ys.p, ccap, d_lcnt.p,
+                       d_ent.p, sc);
+    hipLaunchKernelGGL(k_sum3, 1, 256, 0, st, nclu, (const unsigned long long *)sc, tot);
+    CK(hipGetLastError());
+    unsigned long long c[3] = {0, 0, 0};
+    CK(hipMemcpyAsync(c, tot, 24, hipMemcpyDeviceToHost, st));
+    CK(hipStreamSynchronize(st));
+    if (np) *np = (int64_t)c[0];
+    if (nlj) *nlj = (int64_t)c[1];
+    if (nlist) *nlist = (int64_t)c[2];
+    return 0;
+}
+
 int Engine::count_pairs(int64_t *np, int64_t *nlj, int64_t *nlist) {
     if (!have_list) return JP_EINVAL;
     CK(hipMemsetAsync(d_u64.p, 0, 24, st));
@@ -1757,6 +1808,11 @@ int jp_batch_set_step(jp_batch *b, int64_t idloop) {
     if (!b || idloop < 0) return JP_EINVAL;
     b->e.idloop = idloop;
     return 0;
+}
+
+int jp_count_pairs_noalloc(jp_ctx *ctx, int64_t *npairs, int64_t *nlj, int64_t *nlisted) {
+    if (!ctx || !npairs) return JP_EINVAL;
+    return ctx->e.count_pairs_noalloc(npairs, nlj, nlisted);
 }
 
 int jp_count_pairs(jp_ctx *ctx, int64_t *npairs, int64_t *nlj, int64_t *nlisted) {

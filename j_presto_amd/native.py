@@ -33,6 +33,7 @@ SIGNATURES = {
     "jp_run_nve": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.POINTER(EnergyLog)]),
     "jp_last_timing": (C.c_int, [vp, f64p, f64p, f64p, f64p]),
     "jp_count_pairs": (C.c_int, [vp, i64p, i64p, i64p]),
+    "jp_count_pairs_noalloc": (C.c_int, [vp, i64p, i64p, i64p]),
     "jp_set_step": (C.c_int, [vp, C.c_int64]),
     "jp_prepare_graphs": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
     "jp_batch_prepare_graphs": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
@@ -54,6 +55,7 @@ SIGNATURES = {
 }
 JP_OPT_GRAPHS, JP_OPT_AUTOTUNE, JP_OPT_PAIR_WAVES, JP_OPT_LIST_WAVES, JP_OPT_LIST_SEGMENT = 1, 2, 3, 4, 5
 JP_OPT_CELL_TABLE, JP_OPT_GRAPH_CAPTURES = 6, 7
+JP_EINVAL, JP_ENOMEM, JP_ENODEV = -101, -102, -103
 
 _lib = None
 
