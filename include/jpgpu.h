@@ -272,8 +272,22 @@ int jp_batch_destroy(jp_batch *b);
  *                      wavefronts per block and the FP32 pair kernel at 2/4/8
  *                      wavefronts per i-cluster and keeps the fastest; 0
  *                      (default): fixed shapes.
- *   JP_OPT_PAIR_WAVES  wavefronts per i-cluster in the FP32 force-step pair kernel (2, 4 =
- *                      default, 8; energy steps and the FP64 path use 4).
+ *   JP_OPT_HALF_LIST   1: the FP32/fixed-point path runs on a half list (each unique
+ *                      pair once, its force applied to both atoms, the j side as
+ *                      one 64-bit fixed-point atomic add per entry and component);
+ *                      0 (default): md_run's full list, each atom summing its own
+ *                      pairs.  Results agree within the FP32 tolerance.  On MI355X
+ *                      the half list is the slower one: its 3 atomics per list
+ *                      entry cost more than the pair work they save.  A system
+ *                      whose md_run cell table drops pairs asymmetrically keeps the
+ *                      full list; the FP64 path always uses it; jp_get_list exports
+ *                      the full list either way.  A run or call whose precision
+ *                      needs the other kind of list builds it at the current
+ *                      coordinates.
+ *   JP_OPT_PAIR_WAVES  wavefronts per i-cluster in the FP32 force-step pair kernel:
+ *                      full list 2, 4 (default), 8 (1 counts as 2); half list 1,
+ *                      2 (default), 4, 8; energy steps and the FP64 path keep
+ *                      their defaults.
  *                      It regroups FP32 partial sums: FP32/fixed-point results are
  *                      deterministic for a given value and agree within the FP32
  *                      tolerance across values.
@@ -304,6 +318,7 @@ int jp_batch_destroy(jp_batch *b);
 #define JP_OPT_LIST_SEGMENT 5
 #define JP_OPT_CELL_TABLE 6
 #define JP_OPT_GRAPH_CAPTURES 7   /* read only: step graphs captured during the last run */
+#define JP_OPT_HALF_LIST 8
 int jp_set_option(jp_ctx *ctx, int32_t option, int32_t value);
 int jp_batch_set_option(jp_batch *b, int32_t option, int32_t value);
 int jp_get_option(jp_ctx *ctx, int32_t option, int32_t *value);

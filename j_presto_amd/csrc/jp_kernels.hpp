@@ -125,13 +125,22 @@ __host__ __device__ inline uint32_t hilbert3(uint32_t x, uint32_t y, uint32_t z,
 // extents, the list status words, and mark every slot as padding.
 __global__ void k_list_reset(int ncell, int *__restrict__ cnt, int *__restrict__ fill, unsigned *__restrict__ cellext,
                              int nsys, unsigned *__restrict__ sysext, int nstat, int *__restrict__ lst, int nslot,
-                             int *__restrict__ perm) {
+                             int *__restrict__ perm, int *__restrict__ jcnt) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i <= ncell) cnt[i] = 0;
     if (i < ncell) { fill[i] = 0; cellext[i] = 0u; }
     if (i < nsys) sysext[i] = 0u;
     if (i < nstat) lst[i] = 0;
-    if (i < nslot) perm[i] = -1;
+    if (i < nslot) { perm[i] = -1; jcnt[i] = 0; }
+}
+
+// Half lists: an atom's listed partners are those of its own cluster's
+// entries (icnt) and those it has as j in earlier clusters' entries (jcnt);
+// md_run's limit applies to their sum (cutoff.f90:30-46).
+__global__ void k_nbcheck(int nslot, const int *__restrict__ slot_sys, const SysP *__restrict__ sp,
+                          const int *__restrict__ icnt, const int *__restrict__ jcnt, int *__restrict__ lst) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nslot && icnt[i] + jcnt[i] > sp[slot_sys[i]].nblim) atomicOr(lst, 32);   // JP_ERR_NBLIM
 }
 
 // Wrap into the box and bin (celpar.f90:115-127 for md_run's cell numbering;

@@ -231,14 +231,18 @@ __device__ __forceinline__ bool exact_in(const SysP &P, const double4 &a, const 
 // 8 i atoms).  Per i atom the listed count (nbcount) and per system the number
 // of excluded pairs met within rlim (nremoved) are reported for md_run's
 // checks.
-template <int LW>
+// HALF: the half list -- only j-clusters after the i-cluster, and inside
+// the i-cluster only the pairs (ia < ja): each unique pair once.  Per-atom
+// counts then go to icnt (own entries) and jcnt (as a j atom) for k_nbcheck.
+template <int LW, bool HALF>
 __global__ void __launch_bounds__(64 * LW) __attribute__((amdgpu_waves_per_eu(6)))
 k_list8(int nclu, const int *__restrict__ clu_sys, const SysP *__restrict__ sp, const double *__restrict__ cbb,
         const float4 *__restrict__ ccf, const int4 *__restrict__ crec, const unsigned *__restrict__ sysext,
         const float4 *__restrict__ lp, const double4 *__restrict__ xr, const int4 *__restrict__ imgs,
         const int *__restrict__ xptr, const int *__restrict__ xidx, const int *__restrict__ mcell,
         const uint64_t *__restrict__ wmask, int ccap, int hiwater, uint2 *__restrict__ ent,
-        uint8_t *__restrict__ mb, int2 *__restrict__ lcnt, int *__restrict__ lst) {
+        uint8_t *__restrict__ mb, int2 *__restrict__ lcnt, int *__restrict__ lst, int *__restrict__ icnt,
+        int *__restrict__ jcnt) {
     __shared__ uint32_t s_c[LW][JP_QCAP];   // candidate j-clusters of passing cells
     __shared__ uint32_t s_q[LW][JP_QCAP];   // candidates whose box is within rlim
     __shared__ double4 s_xr[LW][JP_CL];
@@ -402,7 +406,7 @@ k_list8(int nclu, const int *__restrict__ clu_sys, const SysP *__restrict__ sp, 
                         if (exact_in(P, s_xr[w][k], s_im[w][k], xj, ij, sx, sy, sz)) m |= 1u << k;
             }
         }
-        if (jc == ci) m &= ~(1u << ja);   // j == i
+        if (jc == ci) m &= HALF ? (1u << ja) - 1u : ~(1u << ja);   // j == i; half: ia < ja only
         // md_run tests only the pairs its neighbour-cell table holds
         // (incel.f90:456-508): without an image shift a pair within rlim is
         // always in it; with one, j's md_run cell must lie in the i atom's
@@ -459,6 +463,7 @@ k_list8(int nclu, const int *__restrict__ clu_sys, const SysP *__restrict__ sp, 
         if (round64(nA + na) + round64(nB + nb) <= cap) {
             const uint2 E = make_uint2((uint32_t)js | (code << JP_SLOT_BITS), m);
             const size_t pos = lj ? e0 + nA + lane_rank(bA) : e0 + cap - 1 - (nB + lane_rank(bB));
+            if (HALF && has) atomicAdd(jcnt + js, __popc(m));
             if (has) {
                 ent[pos] = E;
                 mb[pos] = (uint8_t)m;
@@ -512,7 +517,8 @@ k_list8(int nclu, const int *__restrict__ clu_sys, const SysP *__restrict__ sp, 
                 ce = s_c[w][(ch + l) & (JP_QCAP - 1)];
                 int sx, sy, sz;
                 shift_of(ce >> JP_SLOT_BITS, sx, sy, sz);
-                ok = clu_near(cbb, (int)(ce & JP_SLOT_MASK), P.L[0] * sx, P.L[1] * sy, P.L[2] * sz, r);
+                ok = (!HALF || (int)(ce & JP_SLOT_MASK) >= ci) &&
+                     clu_near(cbb, (int)(ce & JP_SLOT_MASK), P.L[0] * sx, P.L[1] * sy, P.L[2] * sz, r);
             }
             const uint64_t b = __ballot(ok);
             if (ok) s_q[w][(qt + lane_rank(b)) & (JP_QCAP - 1)] = ce;
@@ -546,7 +552,11 @@ k_list8(int nclu, const int *__restrict__ clu_sys, const SysP *__restrict__ sp, 
         if (used > hiwater) atomicMax(lst + 1, used);
     }
     fold();
-    if (l < JP_CL && ai >= 0 && nbv > P.nblim) atomicOr(lst, JP_ERR_NBLIM);
+    if (HALF) {
+        if (l < JP_CL) icnt[ci * JP_CL + l] = ai >= 0 ? nbv : 0;
+    } else if (l < JP_CL && ai >= 0 && nbv > P.nblim) {
+        atomicOr(lst, JP_ERR_NBLIM);
+    }
     const int nr = wave_sum(nrem);
     if (l == 0 && nr) atomicAdd(lst + 2 + s, nr);
 }

@@ -17,6 +17,7 @@
 
 #include "../../include/jpgpu.h"
 #include "jp_pair.hpp"
+#include "jp_pair_half.hpp"
 #include "jp_bonded.hpp"
 #include "jp_cano.hpp"
 
@@ -211,6 +212,25 @@ struct Engine {
     DBuf<uint64_t> d_wmask;         // md_run's neighbour-cell windows: [system][axis][cell - 1], bit c - 1
     std::vector<uint64_t> wmask_h;
     int cell_table = 1;             // JP_OPT_CELL_TABLE: lists follow md_run's neighbour-cell table
+    DBuf<unsigned long long> d_facc;  // k_pair_h32's fixed-point gradient accumulators, [slot][4]
+    DBuf<int> d_icnt, d_jcnt;       // half lists: listed partners per slot as i and as j
+    int half_opt = 0;               // JP_OPT_HALF_LIST: FP32 runs on the half list where it can hold md_run's list
+    bool list_half = false;         // the current list is a half list
+    // The FP32 path takes the half list unless md_run's cell table drops pairs
+    // within rlim asymmetrically in some system (a half list holds a pair
+    // either both ways or not at all).
+    bool want_half(int prec) const {
+        if (!half_opt || prec != JP_PREC_FP32_FIXED) return false;
+        for (int s = 0; s < nsys; ++s)
+            if (cell_table && sp[s].ctab && !sp[s].ctab_sym) return false;
+        return true;
+    }
+    // the list the pair kernel of `prec` runs on
+    int ensure_list(int prec) {
+        if (have_list && list_half == want_half(prec)) return 0;
+        list_half = want_half(prec);
+        return build_list();
+    }
     DBuf<float4> d_lp, d_ccf;
     DBuf<int4> d_imgs;
     DBuf<int> d_err, d_lst;   // SHAKE error bits; list status words (jp_list.hpp)
@@ -326,6 +346,8 @@ struct Engine {
         return a;
     }
     int pair(int prec, bool energy);
+    int pair_out(int prec);          // half list: accumulators -> gradient slots (after pair())
+    int facc_zero();
     int pair_variant() const;
     BondedArgs bonded_args() const {
         BondedArgs b;
@@ -361,6 +383,7 @@ struct Engine {
     // launch shapes: wavefronts per i-cluster (pair kernels) and per block
     // (k_list8); autotune = 1 times the candidates at the next jp_run_nve
     int pw = JP_PW, lw = JP_LW, autotune = 0, tuned = 0;
+    int pwh = JP_PWH;               // wavefronts per i-cluster of the half-list force kernel
     int tune(int prec);
     StepGraph *step_graph(int n, int prec);
     void drop_graphs();
@@ -732,6 +755,7 @@ int Engine::init(const jp_sys_desc *descs, int ns) {
     AL(d_cnt, ncell + 1); AL(d_start, ncell + 1); AL(d_fill, ncell); AL(d_tmp, natom);
     AL(d_inum, natom); AL(d_perm, nslot); AL(d_cslot, ncell);
     AL(d_ys, nslot); AL(d_qd, nslot); AL(d_xr, nslot); AL(d_imgs, nslot); AL(d_lp, nslot); AL(d_mcell, nslot);
+    AL(d_facc, 4 * (size_t)nslot); AL(d_icnt, nslot); AL(d_jcnt, nslot);
     AL(d_wmask, wmask_h.size());
     AL(d_cbb, 6 * (size_t)nclu); AL(d_ccen, nclu); AL(d_ccf, nclu); AL(d_lcnt, nclu); AL(d_cellext, ncell);
     AL(d_sysext, ns); AL(d_crec, ncell); AL(d_err, 1); AL(d_lst, 2 + (size_t)ns);
@@ -807,7 +831,9 @@ int Engine::init(const jp_sys_desc *descs, int ns) {
     r |= H2D(d_wmask.p, wmask_h.data(), 8 * wmask_h.size());
     if (r) return r;
     CK(hipMemsetAsync(d_v.p, 0, 8 * 3 * (size_t)natom, st));
+    CK(hipMemsetAsync(d_facc.p, 0, 32 * (size_t)nslot, st));
     CK(hipStreamSynchronize(st));
+    list_half = want_half(JP_PREC_FP32_FIXED);   // jp_build_list's list until a run asks for another
     return 0;
 }
 
@@ -839,7 +865,7 @@ int Engine::build_list(bool defer) {
     const int nstat = 2 + nsys;
     const int nrs = std::max(std::max(ncell + 1, nslot), std::max(nsys, nstat));
     hipLaunchKernelGGL(k_list_reset, grid1(nrs, 256), 256, 0, st, ncell, d_cnt.p, d_fill.p, d_cellext.p, nsys,
-                       d_sysext.p, nstat, d_lst.p, nslot, d_perm.p);
+                       d_sysext.p, nstat, d_lst.p, nslot, d_perm.p, d_jcnt.p);
     hipLaunchKernelGGL(k_bin, grid1(natom, 256), 256, 0, st, natom, d_atom_sys.p, d_sp.p, d_x.p, d_img.p,
                        d_cellid.p, d_subkey.p, d_refcell.p, d_cnt.p, d_lst.p);
     hipLaunchKernelGGL(k_scan, 1, 1024, 0, st, ncell, d_cnt.p, d_start.p);
@@ -881,6 +907,7 @@ int Engine::finish_list(bool retry) {
         ccap_want = std::max(ccap_want, 2 * ccap);
         ccap = std::max(ccap, ccap_want);
         CK(hipMemsetAsync(d_lst.p, 0, 4 * (size_t)nstat, st));
+        CK(hipMemsetAsync(d_jcnt.p, 0, 4 * (size_t)nslot, st));
         AL(d_ent, (size_t)nclu * ccap);
         AL(d_mb, (size_t)nclu * ccap / 8);
         launch_list8(*this, ccap - ccap / 8);
@@ -899,7 +926,7 @@ int Engine::finish_list(bool retry) {
     else if (h_lst[0] & JP_ERR_NBLIM) ier = JP_IER_LIST;            // cutoff.f90:30-46
     else
         for (int s = 0; s < nsys; ++s)
-            if (h_lst[2 + s] != 2 * hs[s].nexcl) ier = JP_IER_EXCL;  // cutoff.f90:90-96
+            if (h_lst[2 + s] != (list_half ? 1 : 2) * hs[s].nexcl) ier = JP_IER_EXCL;  // cutoff.f90:90-96
     if (ier && !retry) return JP_LIST_REDO;
     return ier;
 }
@@ -916,6 +943,8 @@ int Engine::refresh_sorted() {
 
 template <bool E, bool M, bool S>
 static void launch_pair(int prec, unsigned nb, size_t lds32, size_t lds64, hipStream_t st, Engine &g);
+template <bool E, bool M, bool S>
+static void launch_h32(unsigned nb, size_t lds32, hipStream_t st, Engine &g);
 
 // Pair-kernel variant: bit 1 class factors differ from 1 (LMD passes), bit 0
 // some box edge < 2*rlim (per-pair minimum image).
@@ -923,7 +952,7 @@ static void launch_pair(int prec, unsigned nb, size_t lds32, size_t lds64, hipSt
 // launch first); the pair kernel only rewrites the gradient slots.
 int Engine::time_pair(int prec, int reps, double &sec) {
     int r;
-    if (!have_list && (r = build_list())) return r;
+    if ((r = ensure_list(prec))) return r;
     if ((r = set_factors(-1))) return r;
     if ((r = refresh_sorted())) return r;
     if ((r = pair(prec, false))) return r;
@@ -931,6 +960,7 @@ int Engine::time_pair(int prec, int reps, double &sec) {
     for (int k = 0; k < reps; ++k)
         if ((r = pair(prec, false))) return r;
     CK(hipEventRecord(ev[1], st));
+    if ((r = facc_zero())) return r;   // the timing launches' sums are not used
     CK(hipEventSynchronize(ev[1]));
     float ms = 0;
     CK(hipEventElapsedTime(&ms, ev[0], ev[1]));
@@ -944,7 +974,7 @@ int Engine::pair_variant() const {
         small = small || sp[s].small;
         for (int q = 0; q < 4; ++q) lmd = lmd || sp[s].fac[q] != 1.0 || sp[s].cls[q] != sp[s].cls[0];
     }
-    return (lmd ? 2 : 0) | (small ? 1 : 0) | (pw << 4);
+    return (lmd ? 2 : 0) | (small ? 1 : 0) | (pw << 4) | (pwh << 8) | (list_half ? 1 << 12 : 0);
 }
 
 int Engine::pair(int prec, bool energy) {
@@ -953,6 +983,20 @@ int Engine::pair(int prec, bool energy) {
     for (int s = 0; s < nsys; ++s) maxt = std::max(maxt, hs[s].ntyp);
     const size_t l32 = sizeof(float2) * maxt * maxt, l64 = sizeof(double2) * maxt * maxt;
     const int key = (energy ? 4 : 0) | (pair_variant() & 3);
+    if (prec == JP_PREC_FP32_FIXED && list_half) {
+        switch (key) {
+            case 0: launch_h32<false, false, false>(nb, l32, st, *this); break;
+            case 1: launch_h32<false, false, true>(nb, l32, st, *this); break;
+            case 2: launch_h32<false, true, false>(nb, l32, st, *this); break;
+            case 3: launch_h32<false, true, true>(nb, l32, st, *this); break;
+            case 4: launch_h32<true, false, false>(nb, l32, st, *this); break;
+            case 5: launch_h32<true, false, true>(nb, l32, st, *this); break;
+            case 6: launch_h32<true, true, false>(nb, l32, st, *this); break;
+            default: launch_h32<true, true, true>(nb, l32, st, *this); break;
+        }
+        CK(hipGetLastError());
+        return 0;
+    }
     switch (key) {
         case 0: launch_pair<false, false, false>(prec, nb, l32, l64, st, *this); break;
         case 1: launch_pair<false, false, true>(prec, nb, l32, l64, st, *this); break;
@@ -967,8 +1011,23 @@ int Engine::pair(int prec, bool energy) {
     return 0;
 }
 
+int Engine::pair_out(int prec) {
+    if (prec != JP_PREC_FP32_FIXED || !list_half) return 0;
+    hipLaunchKernelGGL(k_facc_out, grid1(nslot, 256), 256, 0, st, nslot, d_slot_sys.p, d_sp.p, excess_args(), d_facc.p,
+                       d_gslot.p);
+    CK(hipGetLastError());
+    return 0;
+}
+
+int Engine::facc_zero() {
+    hipLaunchKernelGGL(k_facc_zero, grid1(nslot, 256), 256, 0, st, nslot, d_facc.p);
+    CK(hipGetLastError());
+    return 0;
+}
+
 int Engine::forces(int prec, bool energy) {
     int r = pair(prec, energy);
+    if (!r) r = pair_out(prec);
     if (r || !nbat) return r;
     hipLaunchKernelGGL(k_bonded, grid1(nbat, 128), 128, 0, st, bonded_args(), d_atom_sys.p, d_sp.p, d_x.p, d_efac.p,
                        d_inum.p, d_gslot.p, (double *)nullptr);
@@ -1026,6 +1085,24 @@ int Engine::bonded(const double *cord, double *grad, double *eterm) {
 }
 
 template <bool E, bool M, bool S, int PW>
+static void launch_h32_pw(unsigned nb, size_t lds32, hipStream_t st, Engine &g) {
+    hipLaunchKernelGGL((k_pair_h32<E, M, S, PW>), nb, 64 * PW, lds32, st, g.nclu, g.d_clu_sys.p, g.d_sp.p, g.d_ccen.p,
+                       g.d_ys.p, g.d_lj32.p, g.ccap, g.d_lcnt.p, g.d_ent.p, g.d_facc.p, g.d_epfix.p);
+}
+
+// Half-list FP32 kernel: force steps at the tuned wave count, energy steps
+// at JP_PWH.
+template <bool E, bool M, bool S>
+static void launch_h32(unsigned nb, size_t lds32, hipStream_t st, Engine &g) {
+    switch (E ? JP_PWH : g.pwh) {
+        case 1: launch_h32_pw<E, M, S, 1>(nb, lds32, st, g); break;
+        case 4: launch_h32_pw<E, M, S, 4>(nb, lds32, st, g); break;
+        case 8: launch_h32_pw<E, M, S, 8>(nb, lds32, st, g); break;
+        default: launch_h32_pw<E, M, S, 2>(nb, lds32, st, g); break;
+    }
+}
+
+template <bool E, bool M, bool S, int PW>
 static void launch_pair_f32_only(unsigned nb, size_t lds32, hipStream_t st, Engine &g) {
     hipLaunchKernelGGL((k_pair_f32<E, M, S, PW>), nb, 64 * PW, lds32, st, g.nclu, g.d_clu_sys.p, g.d_sp.p, g.d_ccen.p,
                        g.d_ys.p, g.d_lj32.p, g.ccap, g.d_lcnt.p, g.d_ent.p, g.d_mb.p, g.excess_args(), g.d_gslot.p,
@@ -1057,21 +1134,30 @@ static void launch_pair(int prec, unsigned nb, size_t lds32, size_t lds64, hipSt
     }
 }
 
-template <int LW>
+template <int LW, bool HALF>
 static void launch_list8_lw(Engine &g, int hiwater) {
-    hipLaunchKernelGGL(k_list8<LW>, grid1(g.nclu, LW), 64 * LW, 0, g.st, g.nclu, g.d_clu_sys.p, g.d_sp.p, g.d_cbb.p,
-                       g.d_ccf.p, g.d_crec.p, g.d_sysext.p, g.d_lp.p, g.d_xr.p, g.d_imgs.p, g.d_xptr.p, g.d_xidx.p,
-                       g.d_mcell.p, g.cell_table ? g.d_wmask.p : nullptr, g.ccap, hiwater, g.d_ent.p,
-                       reinterpret_cast<uint8_t *>(g.d_mb.p), g.d_lcnt.p, g.d_lst.p);
+    hipLaunchKernelGGL((k_list8<LW, HALF>), grid1(g.nclu, LW), 64 * LW, 0, g.st, g.nclu, g.d_clu_sys.p, g.d_sp.p,
+                       g.d_cbb.p, g.d_ccf.p, g.d_crec.p, g.d_sysext.p, g.d_lp.p, g.d_xr.p, g.d_imgs.p, g.d_xptr.p,
+                       g.d_xidx.p, g.d_mcell.p, g.cell_table ? g.d_wmask.p : nullptr, g.ccap, hiwater, g.d_ent.p,
+                       reinterpret_cast<uint8_t *>(g.d_mb.p), g.d_lcnt.p, g.d_lst.p, g.d_icnt.p, g.d_jcnt.p);
+    if (HALF)
+        hipLaunchKernelGGL(k_nbcheck, grid1(g.nslot, 256), 256, 0, g.st, g.nslot, g.d_slot_sys.p, g.d_sp.p, g.d_icnt.p,
+                           g.d_jcnt.p, g.d_lst.p);
+}
+
+template <bool HALF>
+static void launch_list8_h(Engine &g, int hiwater) {
+    switch (g.lw) {
+        case 1: launch_list8_lw<1, HALF>(g, hiwater); break;
+        case 2: launch_list8_lw<2, HALF>(g, hiwater); break;
+        case 8: launch_list8_lw<8, HALF>(g, hiwater); break;
+        default: launch_list8_lw<4, HALF>(g, hiwater); break;
+    }
 }
 
 static void launch_list8(Engine &g, int hiwater) {
-    switch (g.lw) {
-        case 1: launch_list8_lw<1>(g, hiwater); break;
-        case 2: launch_list8_lw<2>(g, hiwater); break;
-        case 8: launch_list8_lw<8>(g, hiwater); break;
-        default: launch_list8_lw<4>(g, hiwater); break;
-    }
+    if (g.list_half) launch_list8_h<true>(g, hiwater);
+    else launch_list8_h<false>(g, hiwater);
 }
 
 // Autotuner: times k_list8 at 1, 2, 4 and 8 wavefronts per block and the
@@ -1084,7 +1170,7 @@ static void launch_list8(Engine &g, int hiwater) {
 // across pw values).
 int Engine::tune(int prec) {
     int r;
-    if (!have_list && (r = build_list())) return r;
+    if ((r = ensure_list(prec))) return r;
     hipEvent_t a, b;
     CK(hipEventCreate(&a));
     CK(hipEventCreate(&b));
@@ -1115,10 +1201,13 @@ int Engine::tune(int prec) {
     };
     pick({1, 2, 4, 8}, lw, [&] { launch_list8(*this, ccap - ccap / 8); });
     if (prec == JP_PREC_FP64) pw = JP_PW;
+    else if (list_half) pick({1, 2, 4, 8}, pwh, [&] { pair(prec, false); });
     else pick({2, 4, 8}, pw, [&] { pair(prec, false); });
     hipEventDestroy(a);
     hipEventDestroy(b);
     CK(hipGetLastError());
+    int r2 = facc_zero();
+    if (r2) return r2;
     tuned = 1;
     return build_list();
 }
@@ -1152,10 +1241,12 @@ int Engine::energies(int prec, std::vector<double> &eel, std::vector<double> &ev
             }
             eex[3 * s + k] = h_rsum[6 * (size_t)nsys + 3 * s + k];
         }
+    // the full list counts every pair from both sides (enear.f90:568-571), the half list once
+    const double hf = (prec == JP_PREC_FP32_FIXED && list_half) ? 1.0 : 0.5;
     for (int s = 0; s < nsys; ++s) {
         for (int k = 0; k < 3; ++k) {
-            eel[3 * s + k] = eel[3 * s + k] * 0.5 + hs[s].dself[k] + eex[3 * s + k];   // enear.f90:568-571
-            evd[3 * s + k] = evd[3 * s + k] * 0.5;
+            eel[3 * s + k] = eel[3 * s + k] * hf + hs[s].dself[k] + eex[3 * s + k];
+            evd[3 * s + k] = evd[3 * s + k] * hf;
         }
     }
     return 0;
@@ -1164,7 +1255,8 @@ int Engine::energies(int prec, std::vector<double> &eel, std::vector<double> &ev
 int Engine::nonbond(const double *cord, double *grad, double *eelec, double *evdw, int prec) {
     int r;
     if (cord) CK(hipMemcpyAsync(d_x.p, cord, 24 * (size_t)natom, hipMemcpyHostToDevice, st));
-    if (!have_list) {
+    if (!have_list || list_half != want_half(prec)) {
+        list_half = want_half(prec);
         if ((r = build_list())) return r;
     } else if ((r = refresh_sorted())) {
         return r;
@@ -1175,6 +1267,7 @@ int Engine::nonbond(const double *cord, double *grad, double *eelec, double *evd
     for (int k = 0; k < nfrg; ++k) {
         if ((r = set_factors(nfrg == 1 ? -1 : k))) return r;
         if ((r = pair(prec, k == 0))) return r;
+        if ((r = pair_out(prec))) return r;
         if (k == 0 && (r = energies(prec, eel, evd, eex))) return r;
         hipLaunchKernelGGL(k_kick, grid1(natom, 256), 256, 0, st, natom, d_atom_sys.p, d_sp.p, d_inum.p, d_gslot.p,
                            d_mass.p, d_x.p, d_v.p, d_xold.p, (double *)nullptr, d_grad.p, 0);
@@ -1435,7 +1528,7 @@ int Engine::mon_record(int step, int nsteps, int nmon, int nlog, const std::vect
 int Engine::run_nve(int nsteps, int update, int nmon, int prec, jp_energy_log *log) {
     int r;
     if (nsteps < 0 || update <= 0) return JP_EINVAL;
-    if (!have_list && (r = build_list())) return r;
+    if ((r = ensure_list(prec))) return r;
     if ((r = set_factors(-1))) return r;
     if ((r = refresh_sorted())) return r;
     if (autotune && !tuned && (r = tune(prec))) return r;
@@ -1517,6 +1610,7 @@ int Engine::run_nve(int nsteps, int update, int nmon, int prec, jp_energy_log *l
         if ((r = pair(prec, mon))) return r;
         CK(hipEventRecord(ev[5], st));
         pe_pending = true;
+        if ((r = pair_out(prec))) return r;
         if (nbat) {
             hipLaunchKernelGGL(k_bonded, grid1(nbat, 128), 128, 0, st, bonded_args(), d_atom_sys.p, d_sp.p, d_x.p,
                                d_efac.p, d_inum.p, d_gslot.p, (double *)nullptr);
@@ -1567,7 +1661,7 @@ int Engine::prepare_graphs(int nsteps, int update, int nmon, int prec) {
     int r;
     if (nsteps < 0 || update <= 0) return JP_EINVAL;
     if (!use_graphs) return 0;
-    if (!have_list && (r = build_list())) return r;
+    if ((r = ensure_list(prec))) return r;
     if ((r = set_factors(-1))) return r;
     long long id = idloop;
     int step = 1;
@@ -1610,7 +1704,10 @@ int Engine::get_list(int32_t *cell3, int64_t *npairs, int32_t *ptr, int32_t *idx
             for (int k = 0; k < JP_CL; ++k) {
                 if (!((E.y >> k) & 1u)) continue;
                 const int ai = perm[c * JP_CL + k];
-                if (ai >= 0 && aj >= 0) nb[ai].push_back(aj);
+                if (ai >= 0 && aj >= 0) {
+                    nb[ai].push_back(aj);
+                    if (list_half) nb[aj].push_back(ai);   // each unique pair once: both directions
+                }
             }
         }
     }
@@ -1880,9 +1977,15 @@ static int set_option(Engine &E, int32_t option, int32_t value) {
             E.tuned = 0;
             return 0;
         case JP_OPT_PAIR_WAVES:
-            if (value != 2 && value != 4 && value != 8) return JP_EINVAL;
-            E.pw = value;
+            if (value != 1 && value != 2 && value != 4 && value != 8) return JP_EINVAL;
+            E.pw = std::max(2, value);   // full-list kernel: 2, 4, 8
+            E.pwh = value;               // half-list kernel: 1, 2, 4, 8
             E.autotune = 0;
+            return 0;
+        case JP_OPT_HALF_LIST:
+            E.half_opt = value ? 1 : 0;
+            E.have_list = false;
+            E.list_half = E.want_half(JP_PREC_FP32_FIXED);
             return 0;
         case JP_OPT_LIST_WAVES:
             if (value != 1 && value != 2 && value != 4 && value != 8) return JP_EINVAL;
@@ -1896,6 +1999,7 @@ static int set_option(Engine &E, int32_t option, int32_t value) {
         case JP_OPT_CELL_TABLE:
             E.cell_table = value ? 1 : 0;
             E.have_list = false;   // the next run or call builds the list again
+            E.list_half = E.want_half(JP_PREC_FP32_FIXED);
             return 0;
         default:
             return JP_EINVAL;
@@ -1907,7 +2011,8 @@ static int get_option(const Engine &E, int32_t option, int32_t *value) {
     switch (option) {
         case JP_OPT_GRAPHS: *value = E.use_graphs; return 0;
         case JP_OPT_AUTOTUNE: *value = E.autotune; return 0;
-        case JP_OPT_PAIR_WAVES: *value = E.pw; return 0;
+        case JP_OPT_PAIR_WAVES: *value = E.want_half(JP_PREC_FP32_FIXED) ? E.pwh : E.pw; return 0;
+        case JP_OPT_HALF_LIST: *value = E.half_opt; return 0;
         case JP_OPT_LIST_WAVES: *value = E.lw; return 0;
         case JP_OPT_CELL_TABLE: *value = E.cell_table; return 0;
         case JP_OPT_GRAPH_CAPTURES: *value = E.captures_last_run; return 0;

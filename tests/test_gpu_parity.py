@@ -416,3 +416,47 @@ def test_prepared_graphs_are_replayed():
         md.close()
     assert np.array_equal(out[0][0][0], out[1][0][0]) and np.array_equal(out[0][0][1], out[1][0][1])
     assert np.array_equal(out[0][1], out[1][1])
+
+
+@pytest.mark.parametrize("name", ["water_1500", "chain_lmd", "chain_lmd_tors", "water_42"])
+def test_half_list_option(golden, name):
+    """JP_OPT_HALF_LIST = 1: the FP32 path on the half list (each pair once,
+    j forces as fixed-point atomics) gives the reference's forces and energies
+    within the FP32 bar, exports the same (full) list, and runs batch members
+    bit for bit like single runs."""
+    from j_presto_amd import native
+    s, f = golden(name)
+    print(f"case half list {name}", flush=True)
+    md = engine.MdSystem(s, precision=engine.FP32)
+    md.set_option(native.JP_OPT_HALF_LIST, 1)
+    g, et, ier = md.bonded(cord=s.x)
+    assert ier == 0
+    g, ee, ev, eh, ier = md.cmmdo(cord=s.x, grad=g)
+    assert ier == 0
+    assert rel_rms(combined(s, g), f["force_grad"]) < 1e-5
+    assert abs(ee.sum() - f["force_epot"][8]) <= 1e-5 * abs(f["force_epot"][8])
+    assert abs(ev.sum() - f["force_epot"][7]) <= 1e-5 * abs(f["force_epot"][7])
+    cell, ptr, idx = md.get_list()
+    o = Oracle(s)
+    assert o.celprp(s.x) == 0
+    ocell, optr, oidx = o.get_list()
+    assert np.array_equal(ptr, optr) and np.array_equal(idx, oidx)
+    md.close()
+    # trajectories: single and batched runs equal bit for bit, close to the full list's
+    runs = []
+    for half in (0, 1):
+        m2 = engine.MdSystem(s, precision=engine.FP32)
+        m2.set_option(native.JP_OPT_HALF_LIST, half)
+        log, ier = m2.runmd(20, update=10, nmon=10)
+        assert ier == 0
+        runs.append((m2.get_state(), log["etot"].copy()))
+        m2.close()
+    assert np.abs(runs[0][0][0] - runs[1][0][0]).max() < 1e-5
+    ens = engine.Ensemble([systems.water_box(200, 24.2, 21), s], precision=engine.FP32)
+    ens.set_option(native.JP_OPT_HALF_LIST, 1)
+    log, ier = ens.runmd(20, update=10, nmon=10)
+    assert ier == 0
+    xb, vb = ens.get_state(1)
+    ens.close()
+    assert np.array_equal(xb, runs[1][0][0]) and np.array_equal(vb, runs[1][0][1])
+    assert np.array_equal(log[1]["etot"], runs[1][1])
