@@ -43,7 +43,9 @@ struct SysP {
     int lj0, ntyp;                   // LJ table offset / size
     int hbits;                       // nc = 2^hbits cells per edge
     int small;                       // some edge < 2*rlim: minimum image per pair
-    int nblim;                       // listed partners allowed per atom (2*n15mx, cutoff.f90:30-46)
+    int nblim;                       // n15mx: partners of one table allowed per atom (cutoff.f90:30-46)
+    int mnv, ipmax, mc0;             // md_run's cells per edge, atoms per cell and class (celpar.f90:182-192),
+                                     // offset of this system's per-cell counters
     double dt;
     double ctr[3];                   // box centre: origin of the FP32 list-test frame
     double band;                     // A^2: FP32 list-test results within band of rlim^2 take the FP64 test
@@ -125,8 +127,9 @@ __host__ __device__ inline uint32_t hilbert3(uint32_t x, uint32_t y, uint32_t z,
 // extents, the list status words, and mark every slot as padding.
 __global__ void k_list_reset(int ncell, int *__restrict__ cnt, int *__restrict__ fill, unsigned *__restrict__ cellext,
                              int nsys, unsigned *__restrict__ sysext, int nstat, int *__restrict__ lst, int nslot,
-                             int *__restrict__ perm, int *__restrict__ jcnt) {
+                             int *__restrict__ perm, int *__restrict__ jcnt, int nmc, int *__restrict__ mcnt) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nmc) mcnt[i] = 0;
     if (i <= ncell) cnt[i] = 0;
     if (i < ncell) { fill[i] = 0; cellext[i] = 0u; }
     if (i < nsys) sysext[i] = 0u;
@@ -148,7 +151,7 @@ __global__ void k_nbcheck(int nslot, const int *__restrict__ slot_sys, const Sys
 __global__ void k_bin(int natom, const int *__restrict__ atom_sys, const SysP *__restrict__ sp,
                       const double *__restrict__ x, int *__restrict__ img, int *__restrict__ cellid,
                       int *__restrict__ subkey, int *__restrict__ refcell, int *__restrict__ cnt,
-                      int *__restrict__ err) {
+                      int *__restrict__ err, const int *__restrict__ tflag, int *__restrict__ mcnt) {
 #pragma clang fp contract(off)
     int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= natom) return;
@@ -164,6 +167,18 @@ __global__ void k_bin(int natom, const int *__restrict__ atom_sys, const SysP *_
         int ck = (int)floor((g - P.wall[k]) * P.icsz[k] * 2.0);   // half-cell resolution
         ck = ck < 0 ? 0 : (ck >= 2 * P.nc[k] ? 2 * P.nc[k] - 1 : ck);
         c[k] = ck;
+    }
+    // md_run's checks (celpar.f90:176-204): an atom outside its cell grid,
+    // or more than ipmax atoms of one cluster class in a cell, is ier 28
+    {
+        const int mx = refcell[3 * a], my = refcell[3 * a + 1], mz = refcell[3 * a + 2];
+        if (mx < 1 || mx > P.mnv || my < 1 || my > P.mnv || mz < 1 || mz > P.mnv) {
+            atomicOr(err, 1);
+        } else {
+            const int lin = ((mz - 1) * P.mnv + (my - 1)) * P.mnv + (mx - 1);
+            const int old = atomicAdd(mcnt + P.mc0 + 2 * lin + ((tflag[a] >> 13) & 1), 1);
+            if (old + 1 > P.ipmax) atomicOr(err, 1);
+        }
     }
     // cells in Hilbert order; inside a cell, atoms ordered by sub-cell
     const uint32_t h = hilbert3((uint32_t)c[0], (uint32_t)c[1], (uint32_t)c[2], P.hbits + 1);

@@ -542,6 +542,40 @@ k_count_pairs(int nclu, const int *__restrict__ clu_sys, const SysP *__restrict_
     count_pairs_body<false>(nclu, clu_sys, sp, ys, ccap, lcnt, ent, out);
 }
 
+// md_run's list-size check, exactly (cutoff.f90:25-46): per atom and energy
+// class, the partners in the vdW table (both atoms carry LJ, itb) and in the
+// EL table (the others, itbEL) may not exceed n15mx each.  Runs only when
+// some atom has more than n15mx partners in all (the list kernel's cheap
+// necessary condition).  One wavefront per i-cluster.
+__global__ void __launch_bounds__(64)
+k_nbexact(int nclu, const int *__restrict__ clu_sys, const SysP *__restrict__ sp, const JAtom *__restrict__ ys,
+          int ccap, const int2 *__restrict__ lcnt, const uint2 *__restrict__ ent, int *__restrict__ flag) {
+    __shared__ int s_n[JP_CL][2][3];
+    __shared__ int s_t[JP_CL];
+    const int c = blockIdx.x;
+    if (c >= nclu) return;
+    const SysP &P = sp[clu_sys[c]];
+    const int l = threadIdx.x;
+    if (l < 6 * JP_CL) (&s_n[0][0][0])[l] = 0;
+    if (l < JP_CL) s_t[l] = ys[c * JP_CL + l].t;
+    __syncthreads();
+    const size_t e0 = (size_t)c * ccap;
+    const int2 nn = lcnt[c];
+    for (int e = l; e < nn.x + nn.y; e += 64) {
+        const uint2 E = ent[e < nn.x ? e0 + e : e0 + ccap - nn.y + (e - nn.x)];
+        const int tj = ys[E.x & JP_SLOT_MASK].t;
+        for (int k = 0; k < JP_CL; ++k) {
+            if (!((E.y >> k) & 1u)) continue;
+            const int ti = s_t[k];
+            const int tab = ((ti >> 12) & 1) & ((tj >> 12) & 1);
+            const int cls = P.cls[((ti >> 13) & 1) * 2 + ((tj >> 13) & 1)];
+            atomicAdd(&s_n[k][tab][cls], 1);
+        }
+    }
+    __syncthreads();
+    if (l < 6 * JP_CL && (&s_n[0][0][0])[l] > P.nblim) atomicOr(flag, 1);
+}
+
 // Per-cluster counts of the pairs k_count_pairs counts: out[3c + 0..2] for
 // cluster c (plain stores, no atomics).
 __global__ void __launch_bounds__(64)

@@ -214,6 +214,9 @@ struct Engine {
     int cell_table = 1;             // JP_OPT_CELL_TABLE: lists follow md_run's neighbour-cell table
     DBuf<unsigned long long> d_facc;  // k_pair_h32's fixed-point gradient accumulators, [slot][4]
     DBuf<int> d_icnt, d_jcnt;       // half lists: listed partners per slot as i and as j
+    DBuf<int> d_mcnt;               // atoms per md_run cell and cluster class, all systems
+    DBuf<int> d_nbflag;             // k_nbexact's result
+    int nmcnt = 0;
     int half_opt = 0;               // JP_OPT_HALF_LIST: FP32 runs on the half list where it can hold md_run's list
     bool list_half = false;         // the current list is a half list
     // The FP32 path takes the half list unless md_run's cell table drops pairs
@@ -335,6 +338,7 @@ struct Engine {
     // from restored coordinates).
     int build_list(bool defer = false);
     int finish_list(bool retry);
+    bool nblim_exceeded();
     int cell_err = 0;          // k_bin's flag survives a retry
     int *h_lst = nullptr;      // pinned list status words
     DBuf<double> d_xbak, d_vbak;   // state before a step segment that overlaps a list check
@@ -472,6 +476,10 @@ int Engine::init(const jp_sys_desc *descs, int ns) {
             }
             P.rr = 1.0 / siz;
             md_cell_windows(P, nv, siz, rlim, wmask_h);
+            P.mnv = nv;
+            P.ipmax = (int)std::ceil(siz * siz * siz * 0.27) * 2;   // incel.f90:146 (PARALV= DOUB)
+            P.mc0 = nmcnt;
+            nmcnt += 2 * nv * nv * nv;
         }
         P.rlim2 = rlim * rlim;
         {
@@ -496,7 +504,7 @@ int Engine::init(const jp_sys_desc *descs, int ns) {
         P.cell0 = cell0; P.ncell = P.nc[0] * P.nc[1] * P.nc[2];
         P.lj0 = lj0; P.ntyp = nt;
         h.n15mx = (int)std::ceil(0.15 * (4.0 / 3.0 * 3.14159265358979 * rc * rc * rc)) * 2;
-        P.nblim = 2 * h.n15mx;
+        P.nblim = h.n15mx;
         // energy class of each cluster pair (n_matrix is column-major, 1-based)
         for (int a = 0; a < 2; ++a)
             for (int b = 0; b < 2; ++b) {
@@ -756,6 +764,7 @@ int Engine::init(const jp_sys_desc *descs, int ns) {
     AL(d_inum, natom); AL(d_perm, nslot); AL(d_cslot, ncell);
     AL(d_ys, nslot); AL(d_qd, nslot); AL(d_xr, nslot); AL(d_imgs, nslot); AL(d_lp, nslot); AL(d_mcell, nslot);
     AL(d_facc, 4 * (size_t)nslot); AL(d_icnt, nslot); AL(d_jcnt, nslot);
+    AL(d_mcnt, nmcnt); AL(d_nbflag, 1);
     AL(d_wmask, wmask_h.size());
     AL(d_cbb, 6 * (size_t)nclu); AL(d_ccen, nclu); AL(d_ccf, nclu); AL(d_lcnt, nclu); AL(d_cellext, ncell);
     AL(d_sysext, ns); AL(d_crec, ncell); AL(d_err, 1); AL(d_lst, 2 + (size_t)ns);
@@ -863,11 +872,11 @@ int Engine::set_factors(int onehot) {
 int Engine::build_list(bool defer) {
     CK(hipEventRecord(ev[2], st));
     const int nstat = 2 + nsys;
-    const int nrs = std::max(std::max(ncell + 1, nslot), std::max(nsys, nstat));
+    const int nrs = std::max(std::max(std::max(ncell + 1, nslot), std::max(nsys, nstat)), nmcnt);
     hipLaunchKernelGGL(k_list_reset, grid1(nrs, 256), 256, 0, st, ncell, d_cnt.p, d_fill.p, d_cellext.p, nsys,
-                       d_sysext.p, nstat, d_lst.p, nslot, d_perm.p, d_jcnt.p);
+                       d_sysext.p, nstat, d_lst.p, nslot, d_perm.p, d_jcnt.p, nmcnt, d_mcnt.p);
     hipLaunchKernelGGL(k_bin, grid1(natom, 256), 256, 0, st, natom, d_atom_sys.p, d_sp.p, d_x.p, d_img.p,
-                       d_cellid.p, d_subkey.p, d_refcell.p, d_cnt.p, d_lst.p);
+                       d_cellid.p, d_subkey.p, d_refcell.p, d_cnt.p, d_lst.p, d_tflag.p, d_mcnt.p);
     hipLaunchKernelGGL(k_scan, 1, 1024, 0, st, ncell, d_cnt.p, d_start.p);
     hipLaunchKernelGGL(k_place, grid1(natom, 256), 256, 0, st, natom, d_cellid.p, d_start.p, d_fill.p, d_tmp.p);
     hipLaunchKernelGGL(k_rank, grid1(natom, 256), 256, 0, st, natom, d_atom_sys.p, d_sp.p, d_cellid.p, d_subkey.p,
@@ -923,12 +932,26 @@ int Engine::finish_list(bool retry) {
     have_list = true;
     int ier = 0;
     if ((h_lst[0] | cell_err) & JP_ERR_CELL) ier = JP_IER_CELL;
-    else if (h_lst[0] & JP_ERR_NBLIM) ier = JP_IER_LIST;            // cutoff.f90:30-46
+    else if ((h_lst[0] & JP_ERR_NBLIM) && nblim_exceeded()) ier = JP_IER_LIST;   // cutoff.f90:25-46
     else
         for (int s = 0; s < nsys; ++s)
             if (h_lst[2 + s] != (list_half ? 1 : 2) * hs[s].nexcl) ier = JP_IER_EXCL;  // cutoff.f90:90-96
     if (ier && !retry) return JP_LIST_REDO;
     return ier;
+}
+
+// Some atom has more than n15mx listed partners: md_run's own test counts
+// them per energy class and table (k_nbexact).  A half list keeps no
+// per-atom partner list to count, so there the total decides.
+bool Engine::nblim_exceeded() {
+    if (list_half) return true;
+    if (hipMemsetAsync(d_nbflag.p, 0, 4, st) != hipSuccess) return true;
+    hipLaunchKernelGGL(k_nbexact, nclu, 64, 0, st, nclu, d_clu_sys.p, d_sp.p, d_ys.p, ccap, d_lcnt.p, d_ent.p,
+                       d_nbflag.p);
+    int f = 1;
+    if (hipMemcpyAsync(&f, d_nbflag.p, 4, hipMemcpyDeviceToHost, st) != hipSuccess) return true;
+    if (hipStreamSynchronize(st) != hipSuccess) return true;
+    return f != 0;
 }
 
 int Engine::refresh_sorted() {

@@ -60,7 +60,10 @@ program jp_host
   integer :: cla, clb
   logical :: lmdin
   real(c_double) :: cutlen, dieval, box(6), covshk
-  integer(c_int32_t) :: limshk
+  integer(c_int32_t) :: limshk, cellvl
+  real(c_double) :: cputim
+  logical :: initzero
+  integer(8) :: tstart, trate, tnow          ! wall clock since the program started (md_run's fxcpus)
   integer(c_int32_t), allocatable, target :: grp(:)
   type(top_t) :: top
   character(len=80) :: title
@@ -92,6 +95,7 @@ program jp_host
      stop
   end if
   tplsys = trim(arg) == '--tplsys'
+  call system_clock(tstart, trate)
   mdmode = trim(arg) == '--mdinp' .or. tplsys
   settem = 0.0d0
   fin = ' '; fout = ' '
@@ -276,12 +280,15 @@ contains
     character(len=*), intent(in) :: path
     character(len=256) :: line, rest, key, val
     integer :: uu, ios, p, q
-    logical :: inmd, inin, have_method
+    logical :: inmd, inin, have_method, have_zd, have_pbc
+    integer :: iv
+    real(c_double) :: rv
     nsteps = 0; update = 10; nmon = 0; timest = 2.0d0; settem = -300.0d0   ! negative: METHOD= MICR
     restar = .false.; tempco = .true.; rstin = ' '; rstout = ' '; etof = ' '
     tplf = ' '; crdf = ' '; shkf = ' '; crdfmt = 'PDB'; cla = 0; clb = 0; lmdin = .false.
     cutlen = 12.0d0; dieval = 1.0d0; box = 0.0d0; covshk = 1.0d-6; limshk = 1000
-    have_method = .false.
+    cellvl = 3; cputim = 99999999.0d0; initzero = .false.
+    have_method = .false.; have_zd = .false.; have_pbc = .false.
     inmd = .false.; inin = .false.
     open(newunit=uu, file=path, status='old', action='read')
     do
@@ -309,12 +316,34 @@ contains
           rest = rest(q:)
           ios = 0
           if (inin) then
+             ! EXE> INPUT keywords (inpinp.f90:40-44)
              select case (trim(key))
              case ('TOPOLO'); tplf = val
              case ('COORDI'); crdf = val
              case ('CRDFMT'); crdfmt = val
              case ('SETSHK'); shkf = val
-             case ('GEPSME'); lmdin = val(1:3) == 'LMD'
+             case ('TPLFMT')
+                if (val(1:4) /= 'FORM') call refuse(key, val, 'only the formatted TPL is read')
+             case ('GEPSME')
+                if (trim(val) == 'LMD') then
+                   lmdin = .true.
+                else if (trim(val) /= 'AND') then
+                   call refuse(key, val, 'cluster methods other than AND and LMD are not implemented')
+                end if
+             case ('CELLVL')
+                read(val, *, iostat=ios) cellvl
+                if (ios == 0 .and. (cellvl < 1 .or. cellvl > 6)) call refuse(key, val, 'cell levels 1-6 only')
+             case ('SCALET')
+                if (trim(val) /= '0030') call refuse(key, val, 'only the default class choice 0030 is implemented')
+             case ('LWEIGH', 'LTCNTL')
+                ! lambda mass and temperature control: used by lambda dynamics only
+             case ('REFCOO', 'OUTMON')
+                write(*,'(3a)') 'jp_host: ', trim(key), ' (monitoring only) is not written by this host'
+             case ('SETVAR', 'SETBOU', 'POSITI', 'DISTAN', 'DIHEDR', 'SETORI', 'EXTCMM', 'REPULS', 'CELSIZ', &
+                   'CELRES', 'CLUSTB')
+                call refuse(key, val, 'not implemented by this host')
+             case default
+                call refuse(key, val, 'not an EXE> INPUT keyword of md_run')
              case ('CLUSTA')
                 q = index(val, '<')
                 if (q < 2) then
@@ -327,7 +356,50 @@ contains
              if (ios /= 0) call bad_value(key, val)
              cycle
           end if
+          ! EXE> MD keywords (inpmd.f90:37-50): implemented, accepted as
+          ! no-ops, output-only (warned), or refused when their value would make
+          ! the run differ from md_run's
           select case (trim(key))
+          case ('CAL15M')
+             if (trim(val) /= 'ZD') call refuse(key, val, 'only zero-dipole (ZD) electrostatics is implemented')
+             have_zd = .true.
+          case ('BOUNDA')
+             if (trim(val) /= 'PBC') call refuse(key, val, 'only periodic boundaries (PBC) are implemented')
+             have_pbc = .true.
+          case ('STOPCE')
+             if (trim(val) /= 'NO') call refuse(key, val, 'centre-of-mass motion removal is not implemented')
+          case ('HEATLO')
+             read(val, *, iostat=ios) iv
+             if (ios == 0 .and. iv /= 0) call refuse(key, val, 'the heating loop is not implemented')
+          case ('SHAKEM')
+             if (trim(val) /= 'HBON') call refuse(key, val, 'only SHAKEM= HBON (the SHK file''s groups) is implemented')
+          case ('PSETMP')
+             read(val, *, iostat=ios) rv
+             if (ios == 0 .and. rv > 1.0d-10) call refuse(key, val, 'pseudo-temperature scaling is not implemented')
+          case ('LAMBDA', 'LMDSQU')
+             call refuse(key, val, 'lambda dynamics input is not implemented')
+          case ('LXCELL', 'LYCELL', 'LZCELL', 'SETCEN', 'CENTRX', 'CENTRY', 'CENTRZ')
+             call refuse(key, val, 'give the box as XMINCL..ZMAXCL')
+          case ('CPUTIM'); read(val, *, iostat=ios) cputim
+          case ('INITIA')
+             if (trim(val) /= 'ZERO' .and. trim(val) /= 'SET') call refuse(key, val, 'ZERO or SET')
+             initzero = trim(val) == 'ZERO'
+          case ('MNTRTO')
+             if (trim(val) == 'NO') then
+                etof = ' '
+             else if (trim(val) /= 'ASCI') then
+                call refuse(key, val, 'NAMETO is written as ASCII only')
+             end if
+          case ('PARALV', 'N_SIMD', 'LOGFOR', 'RELAXA', 'TEMPER', 'NSTPCN', 'OUTATM', 'DUMMYL', 'BESTFI', &
+                'RANDOM', 'STARTT', 'CMMUPD', 'RADCAP', 'FORCAP', 'FUNCAP', 'WETPSR', 'WETDSR', 'WETDHR', &
+                'CAPSHP', 'FOACAP', 'RADPRO', 'HTPSLO', 'STAPST', 'ELLIPA', 'ELLIPB', 'ELLIPC', 'ELLBUF', &
+                'CELBUF', 'RADIUS')
+             ! thread layout and SIMD width, log format, methods and boundaries
+             ! this run does not use (CONS relaxation, random initial
+             ! velocities, CMM, CAP, ellipsoid): no effect on a ZD/PBC run
+          case ('NAMECO', 'NAMEVE', 'NAMEEN', 'NAMETR', 'NAMECL', 'NAMEGE', 'OUTCOO', 'OUTVEL', 'OUTENE', &
+                'OUTTRJ', 'OUTCLE', 'MNTRCO', 'MNTRVE', 'MNTREN', 'MNTRTR', 'MNTRCL', 'FDEBUG')
+             write(*,'(3a)') 'jp_host: ', trim(key), ' (trajectory/monitoring output) is not written by this host'
           case ('LOOPLI'); read(val, *, iostat=ios) nsteps
           case ('UPDATE'); read(val, *, iostat=ios) update
           case ('TIMEST'); read(val, *, iostat=ios) timest
@@ -355,9 +427,10 @@ contains
              else if (val(1:4) == 'MICR') then
                 settem = -abs(settem)
              else
-                write(*,'(2a)') 'jp_host: METHOD not supported: ', trim(val)
-                stop 3
+                call refuse(key, val, 'only METHOD= MICR and CANO are implemented')
              end if
+          case default
+             call refuse(key, val, 'not an EXE> MD keyword of md_run')
           end select
           if (ios /= 0) call bad_value(key, val)
        end do
@@ -367,12 +440,31 @@ contains
        write(*,'(a)') 'jp_host: METHOD= MICR or CANO must be given in EXE> MD'
        stop 3
     end if
+    ! md_run's defaults are CMM electrostatics and no boundary (main.f90:149,170)
+    if (.not. have_zd) then
+       write(*,'(a)') 'jp_host: CAL15M= ZD must be given (md_run''s default is CMM, which this host does not run)'
+       stop 3
+    end if
+    if (.not. have_pbc) then
+       write(*,'(a)') 'jp_host: BOUNDA= PBC must be given (md_run''s default is no boundary)'
+       stop 3
+    end if
+    if (.not. restar .and. .not. initzero .and. len_trim(fin) == 0) then
+       write(*,'(a)') 'jp_host: RESTAR= NO needs INITIA= ZERO (md_run''s random initial velocities are not reproduced)'
+       stop 3
+    end if
     if (nmon <= 0 .or. nmon > nsteps) nmon = 0
     if (settem > 0.0d0 .and. tempco .and. len_trim(fin) > 0) then
        write(*,'(a)') 'jp_host: TEMPCO= YES needs the molecule kinds of the TPL (a system file has none)'
        stop 3
     end if
   end subroutine read_mdinp
+
+  subroutine refuse(key, val, why)
+    character(len=*), intent(in) :: key, val, why
+    write(*,'(6a)') 'jp_host: ', trim(key), '= ', trim(val), ': ', why
+    stop 3
+  end subroutine refuse
 
   subroutine bad_value(key, val)
     character(len=*), intent(in) :: key, val
@@ -400,7 +492,7 @@ contains
   end subroutine read_restart
 
   subroutine run_mdinp()
-    integer :: uu, nl, nm, kk
+    integer :: uu, nl, nm, kk, ndone, nseg
     real(c_double) :: epc(3)
     if (.not. restar) then
        rloop = 0; rtime = 0.0d0
@@ -416,20 +508,48 @@ contains
     ! NAMETO needs the energies of every step
     nm = nmon
     if (len_trim(etof) > 0) nm = 1
-    nl = 0
-    if (nm > 0) nl = nsteps / nm + 1
-    allocate(elog(max(nl, 1)))
     ! md_run's idloop continues from the restart's loop number (inirst, md.f90:1132)
     if (jp_set_step(ctx, int(rloop, c_int64_t)) /= JP_OK) then
        write(*,'(a,i0)') 'jp_host: jp_set_step failed for loop ', rloop
        stop 1
     end if
-    ier(5) = jp_run_nve(ctx, nsteps, update, nm, prec, c_loc(elog))
-    ! the records of the monitoring steps come first; the rest have step = -1
-    do while (nl > 0)
-       if (elog(nl)%step >= 0) exit
-       nl = nl - 1
-    end do
+    ndone = nsteps
+    if (cputim < 99999999.0d0) then
+       ! CPUTIM: md_run checks the elapsed time after every list update and,
+       ! past the limit, ends the loop there and writes its restart file
+       ! (md.f90:655-676, finmd): run from list update to list update
+       allocate(elog(max(nsteps / max(nm, 1) + nsteps / update + 2, 1)))
+       nl = 0
+       ndone = 0
+       do while (ndone < nsteps)
+          nseg = min(nsteps - ndone, int(update - mod(int(rloop, 8) + ndone, int(update, 8))))
+          ier(5) = jp_run_nve(ctx, nseg, update, nm, prec, c_loc(elog(nl + 1)))
+          if (ier(5) /= 0) exit
+          ndone = ndone + nseg
+          kk = nl
+          do while (kk < size(elog))
+             if (elog(kk + 1)%step < 0) exit
+             kk = kk + 1
+          end do
+          nl = kk
+          call system_clock(tnow)
+          if (mod(int(rloop, 8) + ndone, int(update, 8)) == 0 .and. ndone < nsteps .and. &
+              real(tnow - tstart, c_double) / real(trate, c_double) >= cputim) then
+             write(*,'(a,i0)') 'jp_host: WARNING> MD CPU TIME IS EXCEEDED at loop ', rloop + ndone
+             exit
+          end if
+       end do
+    else
+       nl = 0
+       if (nm > 0) nl = nsteps / nm + 1
+       allocate(elog(max(nl, 1)))
+       ier(5) = jp_run_nve(ctx, nsteps, update, nm, prec, c_loc(elog))
+       ! the records of the monitoring steps come first; the rest have step = -1
+       do while (nl > 0)
+          if (elog(nl)%step >= 0) exit
+          nl = nl - 1
+       end do
+    end if
     ier(6) = jp_get_state(ctx, c_loc(cord), c_loc(vel))
     write(*,'(a,9(1x,i0))') 'jp_host: ier =', ier
     if (len_trim(etof) > 0 .and. ier(5) == 0) then
@@ -465,7 +585,7 @@ contains
             action='write')
        write(uu) title
        write(uu) int(n, c_int32_t), int(n, c_int32_t)
-       write(uu) rloop + nsteps, rtime + timest * nsteps, ret, rek, rep
+       write(uu) rloop + ndone, rtime + timest * ndone, ret, rek, rep
        write(uu) cord
        write(uu) vel
        write(uu) desc%lambda, 0.0d0
@@ -538,7 +658,7 @@ contains
     tph(1:ntors) = top%tph(1:ntors); tdiv(1:ntors) = top%tdiv(1:ntors)
     patm(:, 1:n14) = top%patm(:, 1:n14); pv(1:n14) = top%p14v(1:n14); pe(1:n14) = top%p14e(1:n14)
     hdr = [int(n, c_int32_t), int(ntyp, c_int32_t), int(nfrag, c_int32_t), int(nfrg, c_int32_t), &
-           int(nexcl, c_int32_t), int(nexcess, c_int32_t), int(nshake, c_int32_t), limshk, 3_c_int32_t, &
+           int(nexcl, c_int32_t), int(nexcess, c_int32_t), int(nshake, c_int32_t), limshk, cellvl, &
            merge(1_c_int32_t, 0_c_int32_t, lmdin), int(nbond, c_int32_t), int(nangle, c_int32_t), &
            int(ntors, c_int32_t), int(n14, c_int32_t)]
     rhdr(1) = covshk; rhdr(2) = cutlen; rhdr(3) = dieval; rhdr(4) = timest; rhdr(5) = 0.0d0

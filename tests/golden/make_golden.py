@@ -22,7 +22,7 @@ and N_SIMD= 1 (a fixed summation order):
 A fixture stores the generator recipe, the input x and v, and the outputs.
 The reference itself never leaves this container.
 
-Usage: python tests/golden/make_golden.py [--cano-only | --mon-only] [name ...]
+Usage: python tests/golden/make_golden.py [--cano-only | --mon-only | --host-only] [name ...]
 """
 from __future__ import annotations
 
@@ -46,7 +46,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 
 RECIPES = {
     # small box just above 2*cutoff (box < 2*rlim: periodic images inside the list radius)
-    "water_1500": ("water_box", dict(nwat=500, L=24.8, seed=7), dict(traj=20, cano=("NO", 20), mon=True)),
+    "water_1500": ("water_box", dict(nwat=500, L=24.8, seed=7),
+                   dict(traj=20, cano=("NO", 20), mon=True, host=True)),
     # the 6k-atom water workload itself
     "water_6k": ("workload", dict(i=0), dict(traj=20)),
     # chain with 16 LJ types, 2-atom SHAKE groups, chain exclusions
@@ -136,6 +137,8 @@ def make(name: str) -> str:
         out.update(cano_job(s, opt, tmp))
         if opt.get("mon"):
             out.update(mon_jobs(s, opt, tmp))
+        if opt.get("host"):
+            out.update(host_jobs(s, tmp))
         path = os.path.join(HERE, f"{name}.npz")
         np.savez_compressed(path, **out)
         return path
@@ -203,6 +206,48 @@ def mon_jobs(s: systems.System, opt: dict, tmp: str) -> dict:
     return out
 
 
+def host_jobs(s: systems.System, tmp: str) -> dict:
+    """Two md_run runs for the md.inp host (jp_host --mdinp):
+      rst25   a restart whose loop number is 25, LOOPLI= 20, UPDATE= 10,
+              OUTLOG= 5: md_run's idloop runs 26..45, so the list is rebuilt
+              after steps 30 and 40 and energies print at 30, 35, 40, 45;
+      cputim  LOOPLI= 30 with CPUTIM= 0.0001 s: md_run stops at its first
+              list update (loop 10) and writes its restart (md.f90:655-676)."""
+    out = {}
+    common = {"METHOD": "MICR", "RESTAR": "YES", "NAMERI": "in.rst", "NAMERO": "out.rst", "UPDATE": "10",
+              "TIMEST": "2.0", "OUTCLE": "100000"}
+    d = os.path.join(tmp, "rst25")
+    os.makedirs(d)
+    refio.write_restart(os.path.join(d, "in.rst"), s.x, s.v, loop=25, lam=s.lam)
+    log = run_md(d, s, dict(common, LOOPLI="20", OUTLOG="5"))
+    r = refio.read_restart(os.path.join(d, "out.rst"))
+    out.update(rst25_x=r["x"], rst25_v=r["v"], rst25_loop=np.int64(r["loop"]), rst25_mon=parse_monitor(log))
+    d = os.path.join(tmp, "cputim")
+    os.makedirs(d)
+    refio.write_restart(os.path.join(d, "in.rst"), s.x, s.v, lam=s.lam)
+    run_md(d, s, dict(common, LOOPLI="30", OUTLOG="100000", CPUTIM="0.0001"))
+    r = refio.read_restart(os.path.join(d, "out.rst"))
+    out.update(cputim_x=r["x"], cputim_v=r["v"], cputim_loop=np.int64(r["loop"]))
+    return out
+
+
+def add_host(name: str) -> str:
+    """Adds the md.inp host jobs' outputs to an existing fixture."""
+    kind, args, opt = RECIPES[name]
+    s = build_system(kind, args)
+    path = os.path.join(HERE, f"{name}.npz")
+    with np.load(path, allow_pickle=False) as z:
+        out = {k: z[k] for k in z.files}
+    assert str(out["fingerprint"]) == fingerprint(s)
+    tmp = tempfile.mkdtemp(prefix="jpgold_")
+    try:
+        out.update(host_jobs(s, tmp))
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    np.savez_compressed(path, **out)
+    return path
+
+
 def add_mon(name: str) -> str:
     """Adds the monitoring jobs' outputs to an existing fixture (same recipe)."""
     kind, args, opt = RECIPES[name]
@@ -242,10 +287,14 @@ def main(argv):
         subprocess.run(["bash", os.path.join(ROOT, "oracle", "build_ref.sh")], check=True)
     cano_only = "--cano-only" in argv
     mon_only = "--mon-only" in argv
-    argv = [a for a in argv if a not in ("--cano-only", "--mon-only")]
+    host_only = "--host-only" in argv
+    argv = [a for a in argv if a not in ("--cano-only", "--mon-only", "--host-only")]
     names = argv or [n for n in RECIPES if not mon_only or RECIPES[n][2].get("mon")]
     for n in names:
-        p = add_cano(n) if cano_only else (add_mon(n) if mon_only else make(n))
+        if host_only:
+            p = add_host(n)
+        else:
+            p = add_cano(n) if cano_only else (add_mon(n) if mon_only else make(n))
         print(n, "->", os.path.relpath(p, ROOT), os.path.getsize(p), "bytes", flush=True)
 
 

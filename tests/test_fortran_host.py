@@ -235,7 +235,8 @@ def test_tplsys_pdb_coordinates(tmp_path):
                        cwd=d)
     assert r.returncode == 0, r.stdout
     inp = inp.replace("COORDI= sys.crd", "COORDI= sys.pdb").replace("CRDFMT= BINA", "CRDFMT= PDB")
-    inp = inp.replace("RESTAR= YES", "RESTAR= NO")
+    # no restart: md_run starts from zero velocities with INITIA= ZERO
+    inp = inp.replace("RESTAR= YES", "RESTAR= NO\n      INITIA= ZERO")
     with open(os.path.join(d, "pdb.inp"), "w") as f:
         f.write(inp)
     r = subprocess.run([_host(), "--tplsys", "pdb.inp", "pdb.bin"], capture_output=True, text=True, timeout=60,
@@ -264,7 +265,7 @@ def test_tplsys_input_errors(tmp_path):
     _tpl_case(d, s, {"METHOD": "MICR", "LOOPLI": "5"})
     inp = open(os.path.join(d, "md.inp")).read()
     cases = {
-        "TOPOLO": inp.replace("TOPOLO= sys.tpl", "TOPOLE= sys.tpl"),
+        "TOPOLE= sys.tpl: not an EXE> INPUT keyword": inp.replace("TOPOLO= sys.tpl", "TOPOLE= sys.tpl"),
         "CLUSTA": inp.replace("GEPSME= LMD", "GEPSME= AND"),
         "CRDFMT": inp.replace("CRDFMT= BINA", "CRDFMT= FORM"),
         "same atom count": inp.replace("CRDFMT= BINA", "CRDFMT= PDB"),
@@ -317,3 +318,91 @@ def test_fortran_host_mdinp_tpl(golden, tmp_path, name, method):
         # E15.7: 7 significant digits, one unit of the last is 1e-7 of the leading power of ten
         ulp = 10.0 ** (np.floor(np.log10(np.maximum(np.abs(ref), 1e-300))) - 6)
         assert np.all(np.abs(got - ref) <= ulp * 1.0001)
+
+
+def test_mdinp_refuses_what_it_does_not_run(tmp_path):
+    """Every md.inp keyword whose value would make the run differ from
+    md_run's and that jp_host does not implement ends it with status 3 and a
+    message naming the keyword, before any GPU use (inpmd.f90:37-50 keywords,
+    main.f90:149-170 defaults: CMM electrostatics, no boundary)."""
+    from j_presto_amd import systems
+    s = systems.water_box(60, 14.0, 3)
+    d = str(tmp_path)
+    _tpl_case(d, s, {"METHOD": "MICR", "LOOPLI": "5"})
+    inp = open(os.path.join(d, "md.inp")).read()
+
+    def md_line(text, key, val):
+        return text.replace("EXE> MD\n", f"EXE> MD\n      {key}= {val}\n")
+
+    cases = {
+        "CAL15M= CMM": inp.replace("CAL15M= ZD", "CAL15M= CMM"),
+        "CAL15M= ZD must be given": "\n".join(l for l in inp.splitlines() if "CAL15M" not in l),
+        "BOUNDA= NO": inp.replace("BOUNDA= PBC", "BOUNDA= NO"),
+        "BOUNDA= ELLI": inp.replace("BOUNDA= PBC", "BOUNDA= ELLI"),
+        "BOUNDA= PBC must be given": "\n".join(l for l in inp.splitlines() if "BOUNDA" not in l),
+        "STOPCE= YES": inp.replace("STOPCE= NO", "STOPCE= BOTH").replace("BOTH", "YES"),
+        "STOPCE= TRAN": inp.replace("STOPCE= NO", "STOPCE= TRAN"),
+        "HEATLO= 1000": md_line(inp, "HEATLO", "1000"),
+        "SHAKEM= ALLB": md_line(inp, "SHAKEM", "ALLB"),
+        "PSETMP= 300.0": md_line(inp, "PSETMP", "300.0"),
+        "LAMBDA= lam.dat": md_line(inp, "LAMBDA", "lam.dat"),
+        "LXCELL= 14.0": md_line(inp, "LXCELL", "14.0"),
+        "METHOD= GE": inp.replace("METHOD= MICR", "METHOD= GE"),
+        "MNTRTO= SING": md_line(inp, "MNTRTO", "SING"),
+        "FOOBAR= 1": md_line(inp, "FOOBAR", "1"),
+        "GEPSME= OR": inp.replace("QUIT\n\nEXE> MD", "      GEPSME= OR\nQUIT\n\nEXE> MD"),
+        "SETVAR= var.dat": inp.replace("QUIT\n\nEXE> MD", "      SETVAR= var.dat\nQUIT\n\nEXE> MD"),
+        "RESTAR= NO needs INITIA= ZERO": inp.replace("RESTAR= YES", "RESTAR= NO"),
+    }
+    for what, text in cases.items():
+        with open(os.path.join(d, "bad.inp"), "w") as f:
+            f.write(text)
+        r = subprocess.run([_host(), "--tplsys", "bad.inp", "got.bin"], capture_output=True, text=True,
+                           timeout=60, cwd=d)
+        assert r.returncode == 3 and what in r.stdout, (what, r.stdout)
+    # keywords with no effect on a ZD/PBC run are accepted; output-only ones are named
+    ok = md_line(md_line(md_line(inp, "N_SIMD", "64"), "RADCAP", "20.0"), "OUTCOO", "10")
+    ok = md_line(md_line(ok, "CPUTIM", "3600.0"), "INITIA", "SET")
+    with open(os.path.join(d, "ok.inp"), "w") as f:
+        f.write(ok)
+    r = subprocess.run([_host(), "--tplsys", "ok.inp", "got.bin"], capture_output=True, text=True, timeout=60, cwd=d)
+    assert r.returncode == 0, r.stdout
+    assert "OUTCOO" in r.stdout
+
+
+@pytest.mark.gpu
+def test_mdinp_restart_keeps_md_runs_step_count(golden, tmp_path):
+    """A restart whose loop number is 25, LOOPLI= 20, UPDATE= 10: md_run's
+    idloop continues from 25 (md.f90:1132), so the list is rebuilt after steps
+    30 and 40; jp_host's final state and loop count match md_run's within 1e-9."""
+    s, f = golden("water_1500")
+    print("case host restart loop 25", flush=True)
+    d = str(tmp_path)
+    _tpl_case(d, s, {"METHOD": "MICR", "LOOPLI": "20", "OUTLOG": "5"})
+    refio.write_restart(os.path.join(d, "in.rst"), s.x, s.v, loop=25, lam=s.lam)
+    r = subprocess.run([_host(), "--mdinp", "md.inp"], capture_output=True, text=True, timeout=240, cwd=d)
+    print(r.stdout, flush=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    o = refio.read_restart(os.path.join(d, "out.rst"))
+    assert o["loop"] == int(f["rst25_loop"]) == 45
+    assert np.abs(o["x"] - f["rst25_x"]).max() < 1e-9
+    assert np.abs(o["v"] - f["rst25_v"]).max() < 1e-9
+
+
+@pytest.mark.gpu
+def test_mdinp_cputim_stops_like_md_run(golden, tmp_path):
+    """CPUTIM: past the limit md_run ends the loop at the next list update
+    and writes its restart there (md.f90:655-676); with 0.0001 s that is loop
+    10 of 30.  jp_host stops at the same step with the same state."""
+    s, f = golden("water_1500")
+    print("case host CPUTIM", flush=True)
+    d = str(tmp_path)
+    _tpl_case(d, s, {"METHOD": "MICR", "LOOPLI": "30", "CPUTIM": "0.0001"})
+    r = subprocess.run([_host(), "--mdinp", "md.inp"], capture_output=True, text=True, timeout=240, cwd=d)
+    print(r.stdout, flush=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "CPU TIME IS EXCEEDED" in r.stdout
+    o = refio.read_restart(os.path.join(d, "out.rst"))
+    assert o["loop"] == int(f["cputim_loop"]) == 10
+    assert np.abs(o["x"] - f["cputim_x"]).max() < 1e-9
+    assert np.abs(o["v"] - f["cputim_v"]).max() < 1e-9
