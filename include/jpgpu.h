@@ -199,6 +199,17 @@ int jp_prepare_graphs(jp_ctx *ctx, int32_t nsteps, int32_t update, int32_t nmon,
 int jp_set_step(jp_ctx *ctx, int64_t idloop);
 int jp_get_step(jp_ctx *ctx, int64_t *idloop);
 
+/* md_run's restart set-up (inirst, md.f90:1104-1263): one step from the
+ * current state, recorded in log[0] (NULL for none; one record per system,
+ * step = the current idloop) as md_run's energy-continuity check, after
+ * which the coordinates are put back and the velocities become the ones the
+ * step's last verlethe substep started from (md.f90:1240-1250 restores vel
+ * from verlethe's velback).  Without a SHAKE retry these are the restart's
+ * own velocities; after one (METHOD= CANO) they are those at the start of
+ * the step's last substep.  The step counter does not advance.  Returns the
+ * step's SHAKE ier, with the state restored all the same. */
+int jp_restart_step(jp_ctx *ctx, int32_t precision, jp_energy_log *log);
+
 /* Seconds spent in the last jp_run_nve, measured with HIP events on the
  * context stream: total, pair kernel, list build, the rest (SHAKE,
  * integration, launch gaps).  The pair time is the mean over the pair
@@ -217,9 +228,12 @@ int jp_last_timing(jp_ctx *ctx, double *t_total, double *t_pair, double *t_list,
  * its temperature group (TEMPCO= YES: md_run's ixamol2, md.f90:136-152);
  * in a batch, labels are per system and the array covers all atoms in order.
  * Degrees of freedom per group: 3 per atom minus the constraints of the SHAKE
- * groups whose first atom is in it.  on = 0 returns to METHOD= MICR.  A SHAKE
- * failure ends the run with its ier (md_run halves dt and retries,
- * verlethe.f90:546-573).  Returns JP_EINVAL for temps <= 0, icslop < 1, a
+ * groups whose first atom is in it.  on = 0 returns to METHOD= MICR.  For a
+ * single system, a step whose SHAKE fails is redone as md_run's verlethe
+ * does (verlethe.f90:552-573, 745-752): undone, then substeps of half the
+ * length (halving again on a failure, doubling back after two successes)
+ * until they add up to dt; a substep below dt/100 ends the run with the
+ * SHAKE ier.  In a batch a SHAKE failure ends the run with its ier.  Returns JP_EINVAL for temps <= 0, icslop < 1, a
  * label < 1 or a group without degrees of freedom. */
 int jp_set_canonical(jp_ctx *ctx, int32_t on, double temps, int32_t icslop, double fcstol, const int32_t *group);
 
@@ -256,6 +270,7 @@ int jp_batch_create(const jp_sys_desc *descs, int32_t nsys, jp_batch **out);
 int jp_batch_run_nve(jp_batch *b, int32_t nsteps, int32_t update, int32_t nmon, int32_t precision,
                      jp_energy_log *log /* [nsys][nsteps/nmon + 1] */);
 int jp_batch_set_step(jp_batch *b, int64_t idloop);   /* jp_set_step for the batch's shared counter */
+int jp_batch_restart_step(jp_batch *b, int32_t precision, jp_energy_log *log /* [nsys] */);
 int jp_batch_prepare_graphs(jp_batch *b, int32_t nsteps, int32_t update, int32_t nmon, int32_t precision);
 int jp_batch_set_state(jp_batch *b, int32_t isys, const double *cord, const double *vel);
 int jp_batch_get_state(jp_batch *b, int32_t isys, double *cord, double *vel);

@@ -265,6 +265,12 @@ struct Engine {
     int set_canonical(int on, double temps, int icslop, double fcstol, const int *group);
     CanoArgs cano_args();
     int cano_step();
+    bool cano_retry() const { return cano && nsys == 1 && (n2 + n3) > 0; }
+    int save_state(double *x, double *v, int *e);
+    int load_state(const double *x, const double *v, const int *e);
+    int shake_failed(bool &failed);
+    int cano_retry_step(int prec);
+    int cano_steps_eager(int prec, int n);
     // energy sums on the device: per-system atom and cluster ranges, results,
     // and a pinned host copy of them
     DBuf<int> d_abeg, d_aend, d_cbeg, d_cend;
@@ -342,6 +348,11 @@ struct Engine {
     int cell_err = 0;          // k_bin's flag survives a retry
     int *h_lst = nullptr;      // pinned list status words
     DBuf<double> d_xbak, d_vbak;   // state before a step segment that overlaps a list check
+    // METHOD= CANO, one system: state before a segment and before a step of
+    // verlethe's SHAKE retry (its dt halving after a SHAKE failure)
+    DBuf<double> d_xs, d_vs, d_xs2, d_vs2;
+    DBuf<int> d_errs;
+    int n_retry = 0;               // steps that needed the retry in the last run
     DBuf<int> d_errbak;
     int refresh_sorted();
     ExcessArgs excess_args() const {
@@ -369,6 +380,7 @@ struct Engine {
     int energies(int prec, std::vector<double> &eel, std::vector<double> &evd, std::vector<double> &eex);
     int shake(int &ier_out);
     int run_nve(int nsteps, int update, int nmon, int prec, jp_energy_log *log);
+    int restart_step(int prec, jp_energy_log *log);
     int prepare_graphs(int nsteps, int update, int nmon, int prec);
     int integrate(int prec, bool mon);   // kick + SHAKE + v update of one step (after the forces)
     int mon_record(int step, int nsteps, int nmon, int nlog, const std::vector<double> &eel,
@@ -1336,6 +1348,10 @@ int Engine::integrate(int prec, bool mon) {
 // minus the SHAKE constraints of the groups whose first atom is theirs,
 // md.f90:114-118, 165-172) and the round buffers.
 int Engine::set_canonical(int on, double temps, int icslop, double fcstol, const int *group) {
+    if (on && nsys == 1) {
+        AL(d_xs, 3 * (size_t)natom); AL(d_vs, 3 * (size_t)natom); AL(d_errs, 1);
+        AL(d_xs2, 3 * (size_t)natom); AL(d_vs2, 3 * (size_t)natom);
+    }
     drop_graphs();
     cano = false;   // METHOD= MICR unless the new set-up completes
     if (!on) return 0;
@@ -1457,6 +1473,136 @@ int Engine::cano_step() {
     return 0;
 }
 
+int Engine::save_state(double *x, double *v, int *e) {
+    CK(hipMemcpyAsync(x, d_x.p, 24 * (size_t)natom, hipMemcpyDeviceToDevice, st));
+    CK(hipMemcpyAsync(v, d_v.p, 24 * (size_t)natom, hipMemcpyDeviceToDevice, st));
+    if (e) CK(hipMemcpyAsync(e, d_err.p, 4, hipMemcpyDeviceToDevice, st));
+    return 0;
+}
+
+int Engine::load_state(const double *x, const double *v, const int *e) {
+    CK(hipMemcpyAsync(d_x.p, x, 24 * (size_t)natom, hipMemcpyDeviceToDevice, st));
+    CK(hipMemcpyAsync(d_v.p, v, 24 * (size_t)natom, hipMemcpyDeviceToDevice, st));
+    if (e) CK(hipMemcpyAsync(d_err.p, e, 4, hipMemcpyDeviceToDevice, st));
+    return refresh_sorted();
+}
+
+int Engine::shake_failed(bool &failed) {
+    CK(hipMemcpyAsync(h_err, d_err.p, 4, hipMemcpyDeviceToHost, st));
+    CK(hipStreamSynchronize(st));
+    failed = (*h_err & 12) != 0;
+    return 0;
+}
+
+// One METHOD= CANO step with verlethe's SHAKE retry (verlethe.f90:552-573,
+// 745-752): a substep whose SHAKE fails is undone (x and v of its start),
+// the step length halves and the forces are evaluated again; after two
+// successful half substeps the length doubles back, and the step ends when
+// substeps adding up to dt are done.  A substep below dt/100 ends the run
+// with the SHAKE ier (md_run stops there).
+int Engine::cano_retry_step(int prec) {
+    int r;
+    const double dt0 = sp[0].dt;
+    double T = dt0;
+    int ishk = 0, jshk = 1;
+    ++n_retry;
+    for (;;) {
+        if ((r = save_state(d_xs2.p, d_vs2.p, nullptr))) break;
+        if (sp[0].dt != T) {
+            sp[0].dt = T;
+            if ((r = upload_sp())) break;
+        }
+        if ((r = forces(prec, false)) || (r = cano_step())) break;
+        bool failed = false;
+        if ((r = shake_failed(failed))) break;
+        if (failed) {
+            CK(hipMemsetAsync(d_err.p, 0, 4, st));
+            if ((r = load_state(d_xs2.p, d_vs2.p, nullptr))) break;
+            if (T < 0.01 * dt0) { r = JP_IER_SHAKE_CONV; break; }
+            jshk *= 2;
+            ishk *= 2;
+            T *= 0.5;
+            continue;
+        }
+        ++ishk;
+        if (ishk == jshk) break;
+        if (ishk % 2 == 0) {
+            jshk /= 2;
+            ishk /= 2;
+            T *= 2.0;
+        }
+    }
+    sp[0].dt = dt0;
+    int r2 = upload_sp();
+    CK(hipStreamSynchronize(st));
+    return r ? r : r2;
+}
+
+// n plain METHOD= CANO steps launched one by one, each checked for a SHAKE
+// failure and, where one occurs, redone with the retry.
+int Engine::cano_steps_eager(int prec, int n) {
+    int r;
+    for (int k = 0; k < n; ++k) {
+        if ((r = save_state(d_xs.p, d_vs.p, d_errs.p))) return r;
+        if ((r = forces(prec, false)) || (r = integrate(prec, false))) return r;
+        bool failed = false;
+        if ((r = shake_failed(failed))) return r;
+        if (failed) {
+            if ((r = load_state(d_xs.p, d_vs.p, d_errs.p))) return r;
+            if ((r = cano_retry_step(prec))) return r;
+        }
+    }
+    return 0;
+}
+
+// md_run's restart set-up (md.f90:1104-1263, inirst): one step from the
+// restart state, whose energies are md_run's energy-continuity check
+// (kinetic energy of the mean of the velocities before and after the step);
+// then the coordinates go back to the restart's and the velocities become
+// those the step's last verlethe substep started from (inirst restores vel
+// from verlethe's velback, md.f90:1240-1250): the restart's own velocities
+// unless a SHAKE retry split the step.  idloop does not advance.
+int Engine::restart_step(int prec, jp_energy_log *log) {
+    int r;
+    if ((r = ensure_list(prec))) return r;
+    if ((r = set_factors(-1))) return r;
+    if ((r = refresh_sorted())) return r;
+    const bool retry = cano_retry();
+    CK(hipMemsetAsync(d_err.p, 0, 4, st));
+    CK(hipMemcpyAsync(d_xbak.p, d_x.p, 24 * (size_t)natom, hipMemcpyDeviceToDevice, st));
+    if ((r = pair(prec, true)) || (r = pair_out(prec))) return r;
+    if (nbat) {
+        hipLaunchKernelGGL(k_bonded, grid1(nbat, 128), 128, 0, st, bonded_args(), d_atom_sys.p, d_sp.p, d_x.p,
+                           d_efac.p, d_inum.p, d_gslot.p, (double *)nullptr);
+        CK(hipGetLastError());
+    }
+    std::vector<double> eel, evd, eex, et;
+    if ((r = energies(prec, eel, evd, eex)) || (r = bonded_energies(et))) return r;
+    if (retry && (r = save_state(d_xs.p, d_vs.p, d_errs.p))) return r;
+    if ((r = integrate(prec, true))) return r;
+    if (retry) {
+        bool failed = false;
+        if ((r = shake_failed(failed))) return r;
+        if (failed) {
+            if ((r = load_state(d_xs.p, d_vs.p, d_errs.p))) return r;
+            if ((r = cano_retry_step(prec))) return r;
+        }
+    }
+    // one record per system: ekin from the mean of v before and after
+    const int rl = log ? mon_record((int)idloop, 0, 1, 0, eel, evd, et, log) : 0;
+    int err = 0;
+    CK(hipMemcpyAsync(&err, d_err.p, 4, hipMemcpyDeviceToHost, st));
+    CK(hipMemcpyAsync(d_x.p, d_xbak.p, 24 * (size_t)natom, hipMemcpyDeviceToDevice, st));
+    CK(hipMemcpyAsync(d_v.p, d_vback.p, 24 * (size_t)natom, hipMemcpyDeviceToDevice, st));
+    CK(hipMemsetAsync(d_err.p, 0, 4, st));
+    CK(hipStreamSynchronize(st));
+    if ((r = refresh_sorted())) return r;
+    if (rl) return rl;
+    if (err & 8) return JP_IER_SHAKE_SING;
+    if (err & 4) return JP_IER_SHAKE_CONV;
+    return 0;
+}
+
 void Engine::drop_graphs() {
     for (auto &g : graphs)
         if (g.exec) hipGraphExecDestroy(g.exec);
@@ -1558,6 +1704,8 @@ int Engine::run_nve(int nsteps, int update, int nmon, int prec, jp_energy_log *l
     CK(hipMemsetAsync(d_err.p, 0, 4, st));
     t_pair = 0; t_step = 0; n_pair = 0; n_pair_steps = nsteps;
     pe_pending = false;
+    const bool retry = cano_retry();
+    n_retry = 0;
     const int cap0 = n_captures;
     const double tl0 = t_list;
     // records of the monitoring steps first, the rest marked unused
@@ -1586,6 +1734,7 @@ int Engine::run_nve(int nsteps, int update, int nmon, int prec, jp_energy_log *l
         if (nplain >= 2) {
             StepGraph *g = step_graph(nplain, prec);
             if (g) {
+                if (retry && (r = save_state(d_xs.p, d_vs.p, d_errs.p))) return r;
                 if (pending) {
                     CK(hipMemcpyAsync(d_xbak.p, d_x.p, 24 * (size_t)natom, hipMemcpyDeviceToDevice, st));
                     CK(hipMemcpyAsync(d_vbak.p, d_v.p, 24 * (size_t)natom, hipMemcpyDeviceToDevice, st));
@@ -1604,6 +1753,14 @@ int Engine::run_nve(int nsteps, int update, int nmon, int prec, jp_energy_log *l
                         continue;   // same segment again, on the new list
                     }
                     if (r) return r;
+                }
+                if (retry) {   // a SHAKE failure in the segment: redo it step by step
+                    bool failed = false;
+                    if ((r = shake_failed(failed))) return r;
+                    if (failed) {
+                        if ((r = load_state(d_xs.p, d_vs.p, d_errs.p))) return r;
+                        if ((r = cano_steps_eager(prec, nplain))) return r;
+                    }
                 }
                 step += nplain;
                 idloop += nplain;
@@ -1643,7 +1800,16 @@ int Engine::run_nve(int nsteps, int update, int nmon, int prec, jp_energy_log *l
         std::vector<double> eel, evd, eex, et;
         if (mon && (r = energies(prec, eel, evd, eex))) return r;
         if (mon && (r = bonded_energies(et))) return r;
+        if (retry && (r = save_state(d_xs.p, d_vs.p, d_errs.p))) return r;
         if ((r = integrate(prec, mon))) return r;
+        if (retry) {
+            bool failed = false;
+            if ((r = shake_failed(failed))) return r;
+            if (failed) {
+                if ((r = load_state(d_xs.p, d_vs.p, d_errs.p))) return r;
+                if ((r = cano_retry_step(prec))) return r;
+            }
+        }
         if (mon) {
             if ((r = mon_record((int)(idloop + 1), nsteps, nmon, nlog, eel, evd, et, log))) return r;
             nlog++;
@@ -1884,6 +2050,16 @@ int jp_run_nve(jp_ctx *ctx, int32_t nsteps, int32_t update, int32_t nmon, int32_
     if (!ctx || (precision != JP_PREC_FP64 && precision != JP_PREC_FP32_FIXED)) return JP_EINVAL;
     if (nmon > 0 && !log) return JP_EINVAL;
     return ctx->e.run_nve(nsteps, update, nmon, precision, log);
+}
+
+int jp_restart_step(jp_ctx *ctx, int32_t precision, jp_energy_log *log) {
+    if (!ctx || (precision != JP_PREC_FP64 && precision != JP_PREC_FP32_FIXED)) return JP_EINVAL;
+    return ctx->e.restart_step(precision, log);
+}
+
+int jp_batch_restart_step(jp_batch *b, int32_t precision, jp_energy_log *log) {
+    if (!b || (precision != JP_PREC_FP64 && precision != JP_PREC_FP32_FIXED)) return JP_EINVAL;
+    return b->e.restart_step(precision, log);
 }
 
 int jp_last_timing(jp_ctx *ctx, double *t_total, double *t_pair, double *t_list, double *t_step) {

@@ -152,6 +152,16 @@ class MdSystem:
         prec = self.precision if precision is None else precision
         native.check("jp_prepare_graphs", self.lib.jp_prepare_graphs(self.ctx, nsteps, update, nmon, prec))
 
+    def restart_step(self, precision: Optional[int] = None):
+        """md_run's restart set-up (inirst, md.f90:1104-1263): one step whose
+        energies are returned as a one-record log, then the coordinates back
+        and the velocities those of the step's last substep start.  Returns
+        (log, ier)."""
+        prec = self.precision if precision is None else precision
+        log = (EnergyLog * 1)()
+        ier = native.check("jp_restart_step", self.lib.jp_restart_step(self.ctx, prec, log))
+        return log_to_array(log, 1), ier
+
     def set_step(self, idloop: int):
         """md_run's step counter (a restart's loop number); list rebuilds and
         monitoring steps follow it."""
@@ -294,6 +304,13 @@ class Ensemble:
 
     def set_step(self, idloop: int):
         native.check("jp_batch_set_step", self.lib.jp_batch_set_step(self.b, int(idloop)))
+
+    def restart_step(self, precision: Optional[int] = None):
+        """MdSystem.restart_step for every system of the batch."""
+        prec = self.precision if precision is None else precision
+        log = (EnergyLog * self.nsys)()
+        ier = native.check("jp_batch_restart_step", self.lib.jp_batch_restart_step(self.b, prec, log))
+        return log_to_array(log, self.nsys), ier
 
     def prepare_graphs(self, nsteps: int, update: int = 10, nmon: int = 0, precision: Optional[int] = None):
         prec = self.precision if precision is None else precision

@@ -39,7 +39,7 @@
 !   p14_atoms(2,n14) p14_vdw(n14) p14_elec(n14)
 ! Output file (same record format):
 !   ier(9): create, build_list, nonbond FP64, nonbond FP32, run_nve, get_state, bonded (x2),
-!           set_canonical
+!           set_canonical (with --mdinp, ier(7) is the restart step's)
 !   grad64(3,N,nfrg) eelec64(nfrg) evdw64(nfrg) grad32(3,N,nfrg) eelec32(nfrg) evdw32(nfrg)
 !   eterm(6,nfrg) = enevcc(2:7,:) (bond, angle, torsion, improper, 1-4 vdW, 1-4 elec)
 !   cord(3,N) vel(3,N) after the run, nlog, then per record: step etot epot ekin
@@ -513,6 +513,10 @@ contains
        write(*,'(a,i0)') 'jp_host: jp_set_step failed for loop ', rloop
        stop 1
     end if
+    ! a restart starts with inirst's step (md.f90:1104-1263), which keeps the
+    ! coordinates and leaves the velocities its last verlethe substep started
+    ! from (the restart's own unless a SHAKE retry split the step)
+    if (restar) ier(7) = jp_restart_step(ctx, prec, c_null_ptr)
     ndone = nsteps
     if (cputim < 99999999.0d0) then
        ! CPUTIM: md_run checks the elapsed time after every list update and,

@@ -156,6 +156,15 @@ module jpgpu
        integer(c_int64_t), intent(out) :: idloop
      end function jp_get_step
 
+     ! md_run's restart set-up step (inirst, md.f90:1104-1263): one step,
+     ! then the coordinates back and the velocities of its last substep start
+     integer(c_int) function jp_restart_step(ctx, precision, log) bind(C, name='jp_restart_step')
+       import :: c_int, c_ptr, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: precision
+       type(c_ptr), value :: log
+     end function jp_restart_step
+
      integer(c_int) function jp_count_pairs(ctx, npairs, nlj, nlisted) bind(C, name='jp_count_pairs')
        import :: c_int, c_ptr, c_int64_t
        type(c_ptr), value :: ctx

@@ -35,10 +35,12 @@ SIGNATURES = {
     "jp_count_pairs": (C.c_int, [vp, i64p, i64p, i64p]),
     "jp_count_pairs_noalloc": (C.c_int, [vp, i64p, i64p, i64p]),
     "jp_set_step": (C.c_int, [vp, C.c_int64]),
+    "jp_restart_step": (C.c_int, [vp, C.c_int32, C.POINTER(EnergyLog)]),
     "jp_prepare_graphs": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
     "jp_batch_prepare_graphs": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
     "jp_get_step": (C.c_int, [vp, i64p]),
     "jp_batch_set_step": (C.c_int, [vp, C.c_int64]),
+    "jp_batch_restart_step": (C.c_int, [vp, C.c_int32, C.POINTER(EnergyLog)]),
     "jp_time_pair": (C.c_int, [vp, C.c_int32, C.c_int32, f64p]),
     "jp_set_canonical": (C.c_int, [vp, C.c_int32, C.c_double, C.c_int32, C.c_double, i32p]),
     "jp_batch_set_canonical": (C.c_int, [vp, C.c_int32, C.c_double, C.c_int32, C.c_double, i32p]),

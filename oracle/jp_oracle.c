@@ -1236,6 +1236,85 @@ int jo_run_nve(jo_ctx *c, double *x, double *v, int nsteps, int update) {
     return ier;
 }
 
+/* One verlethe call for METHOD= CANO with its SHAKE retry: the substep's
+ * start is backed up (verlethe.f90:83-98), the forces evaluated (103), the
+ * step of length T taken; a SHAKE failure restores x and v of the substep
+ * start, counts a provision, halves T and evaluates the forces again
+ * (552-573), unless T is below dt/100, where the run ends with the SHAKE
+ * ier (564-567).  After a successful substep ishk advances; the call ends at
+ * ishk == jshk, and two successes at T double it back (745-752).  vback
+ * receives the velocities the last substep started from. */
+static int cano_verlethe(jo_ctx *c, double *x, double *v, double *vback, double *grad, double *gt, double *xo,
+                         double *xs, double *vs, double temps, int icslop, double fcstol, int ngrp, const int *grp,
+                         const double *dof, int *nprov) {
+    const int n = c->n, nfrg = c->nfrg;
+    const double dt0 = c->dt;
+    double T = dt0, eterm[18], ee[3], ev[3];
+    int ishk = 0, jshk = 1, ier = 0;
+    for (;;) {
+        memcpy(xs, x, sizeof(double) * 3 * n);
+        memcpy(vs, v, sizeof(double) * 3 * n);
+        c->dt = T;
+        memset(grad, 0, sizeof(double) * 3 * (size_t)n * nfrg);
+        jo_bonded(c, x, grad, eterm);
+        jo_nonbond(c, x, grad, ee, ev);
+        jo_combine_grad(c, grad, gt);
+        ier = jo_cano_step(c, x, v, gt, xo, vback, temps, icslop, fcstol, ngrp, grp, dof);
+        if (ier) {
+            memcpy(x, xs, sizeof(double) * 3 * n);
+            memcpy(v, vs, sizeof(double) * 3 * n);
+            if (T < 0.01 * dt0) break;
+            ++*nprov;
+            jshk *= 2;
+            ishk *= 2;
+            T *= 0.5;
+            ier = 0;
+            continue;
+        }
+        ++ishk;
+        if (ishk == jshk) break;
+        if (ishk % 2 == 0) {
+            jshk /= 2;
+            ishk /= 2;
+            T *= 2.0;
+        }
+    }
+    c->dt = dt0;
+    return ier;
+}
+
+/* md_run's METHOD= CANO loop from a restart: with inirst, the restart
+ * set-up first (md.f90:1104-1263: one verlethe call, then the coordinates
+ * back and vel = verlethe's velback, the velocities its last substep started
+ * from), then nsteps verlethe calls with a list rebuild after every step
+ * that is a multiple of update (md.f90:519-694).  nprov counts the SHAKE
+ * provisions (failed substeps) md_run prints. */
+int jo_run_cano(jo_ctx *c, double *x, double *v, int nsteps, int update, int inirst, double temps, int icslop,
+                double fcstol, int ngrp, const int *grp, const double *dof, int *nprov) {
+    const size_t n3 = 3 * (size_t)c->n;
+    double *grad = (double *)malloc(sizeof(double) * n3 * c->nfrg);
+    double *gt = (double *)malloc(sizeof(double) * n3);
+    double *xo = (double *)malloc(sizeof(double) * n3);
+    double *vb = (double *)malloc(sizeof(double) * n3);
+    double *xs = (double *)malloc(sizeof(double) * n3);
+    double *vs = (double *)malloc(sizeof(double) * n3);
+    double *x0 = (double *)malloc(sizeof(double) * n3);
+    *nprov = 0;
+    int ier = c->have_list ? 0 : jo_celprp(c, x);
+    if (!ier && inirst) {
+        memcpy(x0, x, sizeof(double) * n3);
+        ier = cano_verlethe(c, x, v, vb, grad, gt, xo, xs, vs, temps, icslop, fcstol, ngrp, grp, dof, nprov);
+        memcpy(x, x0, sizeof(double) * n3);
+        memcpy(v, vb, sizeof(double) * n3);
+    }
+    for (int s = 1; s <= nsteps && !ier; ++s) {
+        ier = cano_verlethe(c, x, v, vb, grad, gt, xo, xs, vs, temps, icslop, fcstol, ngrp, grp, dof, nprov);
+        if (!ier && update > 0 && s % update == 0) ier = jo_celprp(c, x);
+    }
+    free(grad); free(gt); free(xo); free(vb); free(xs); free(vs); free(x0);
+    return ier;
+}
+
 /* ------------------------------------------------- brute-force pair set */
 
 /* {j != i : FP64 minimum-image d^2 <= rlim^2} minus excluded pairs, as CSR

@@ -50,6 +50,8 @@ def lib():
         L.jo_ekin.argtypes = [C.c_void_p, f64p]
         L.jo_ekin.restype = C.c_double
         L.jo_run_nve.argtypes = [C.c_void_p, f64p, f64p, C.c_int, C.c_int]
+        L.jo_run_cano.argtypes = [C.c_void_p, f64p, f64p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int,
+                                  C.c_double, C.c_int, i32p, f64p, i32p]
         L.jo_list_brute.argtypes = [C.c_void_p, f64p, i32p, i32p, C.c_int64]
         L.jo_list_brute.restype = C.c_int64
         L.jo_constants.argtypes = [C.c_void_p, f64p]
@@ -156,6 +158,17 @@ class Oracle:
 
     def run_nve(self, x, v, nsteps: int, update: int = 10) -> int:
         return lib().jo_run_nve(self.c, _p(x), _p(v), nsteps, update)
+
+    def run_cano(self, x, v, nsteps: int, update: int = 10, temps: float = 300.0, icslop: int = 3,
+                 fcstol: float = 1e-3, grp=None, inirst: bool = True):
+        """md_run's METHOD= CANO loop with verlethe's SHAKE retry, in place,
+        after the restart set-up step when inirst; returns (ier, provisions)."""
+        dof = self.group_dof(grp)
+        g = None if grp is None else np.ascontiguousarray(grp, np.int32)
+        nprov = C.c_int32()
+        ier = lib().jo_run_cano(self.c, _p(x), _p(v), nsteps, update, int(inirst), temps, icslop, fcstol, len(dof),
+                                None if g is None else _p(g, i32p), _p(dof), C.byref(nprov))
+        return ier, nprov.value
 
     def list_brute(self, x):
         x = np.ascontiguousarray(x, np.float64)

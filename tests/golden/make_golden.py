@@ -212,7 +212,10 @@ def host_jobs(s: systems.System, tmp: str) -> dict:
               OUTLOG= 5: md_run's idloop runs 26..45, so the list is rebuilt
               after steps 30 and 40 and energies print at 30, 35, 40, 45;
       cputim  LOOPLI= 30 with CPUTIM= 0.0001 s: md_run stops at its first
-              list update (loop 10) and writes its restart (md.f90:655-676)."""
+              list update (loop 10) and writes its restart (md.f90:655-676);
+      retry   METHOD= CANO with LIMSHK= 3: SHAKE does not converge within 3
+              iterations at dt = 2 fs, so every step takes verlethe's retry
+              (verlethe.f90:552-573: undo, halve dt, redo the forces)."""
     out = {}
     common = {"METHOD": "MICR", "RESTAR": "YES", "NAMERI": "in.rst", "NAMERO": "out.rst", "UPDATE": "10",
               "TIMEST": "2.0", "OUTCLE": "100000"}
@@ -222,6 +225,14 @@ def host_jobs(s: systems.System, tmp: str) -> dict:
     log = run_md(d, s, dict(common, LOOPLI="20", OUTLOG="5"))
     r = refio.read_restart(os.path.join(d, "out.rst"))
     out.update(rst25_x=r["x"], rst25_v=r["v"], rst25_loop=np.int64(r["loop"]), rst25_mon=parse_monitor(log))
+    d = os.path.join(tmp, "retry")
+    os.makedirs(d)
+    refio.write_restart(os.path.join(d, "in.rst"), s.x, s.v, lam=s.lam)
+    log = run_md(d, s, dict(common, METHOD="CANO", SETTEM="300.0", TEMPCO="NO", LOOPLI="20", OUTLOG="100000",
+                            LIMSHK="3"))
+    r = refio.read_restart(os.path.join(d, "out.rst"))
+    out.update(retry_x=r["x"], retry_v=r["v"], retry_loop=np.int64(r["loop"]),
+               retry_provisions=np.int64(log.count("SHAKE provision")))
     d = os.path.join(tmp, "cputim")
     os.makedirs(d)
     refio.write_restart(os.path.join(d, "in.rst"), s.x, s.v, lam=s.lam)

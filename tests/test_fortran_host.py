@@ -406,3 +406,22 @@ def test_mdinp_cputim_stops_like_md_run(golden, tmp_path):
     assert o["loop"] == int(f["cputim_loop"]) == 10
     assert np.abs(o["x"] - f["cputim_x"]).max() < 1e-9
     assert np.abs(o["v"] - f["cputim_v"]).max() < 1e-9
+
+
+@pytest.mark.gpu
+def test_mdinp_cano_shake_retry_like_md_run(golden, tmp_path):
+    """METHOD= CANO with LIMSHK= 3: every step takes verlethe's SHAKE retry,
+    and the restart set-up step (inirst) leaves the velocities of its last
+    substep's start; jp_host's final state matches md_run's within 1e-9."""
+    s, f = golden("water_1500")
+    print("case host CANO SHAKE retry", flush=True)
+    d = str(tmp_path)
+    _tpl_case(d, s, {"METHOD": "CANO", "LOOPLI": str(int(f["retry_loop"])), "SETTEM": "300.0", "TEMPCO": "NO",
+                     "LIMSHK": "3"})
+    r = subprocess.run([_host(), "--mdinp", "md.inp"], capture_output=True, text=True, timeout=240, cwd=d)
+    print(r.stdout, flush=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    o = refio.read_restart(os.path.join(d, "out.rst"))
+    assert o["loop"] == int(f["retry_loop"])
+    assert np.abs(o["x"] - f["retry_x"]).max() < 1e-9
+    assert np.abs(o["v"] - f["retry_v"]).max() < 1e-9

@@ -156,3 +156,57 @@ def test_cano_without_shake_groups():
     xe, ve = ens.get_state(1)
     ens.close()
     assert np.array_equal(xe, xs) and np.array_equal(ve, vs)
+
+
+@pytest.mark.parametrize("graphs", [True, False])
+def test_cano_shake_retry_vs_reference(golden, graphs):
+    """METHOD= CANO with LIMSHK= 3: SHAKE fails at the full step, and
+    verlethe undoes the substep, halves dt and redoes it (verlethe.f90:
+    552-573, 745-752).  md_run's restart set-up step (inirst) runs first: it
+    leaves the velocities of its last substep's start (md.f90:1240-1250).
+    The trajectory matches md_run's within 1e-9, with step graphs (a failed
+    segment is redone step by step) and without."""
+    s, f = golden("water_1500")
+    print(f"case cano retry graphs={graphs}", flush=True)
+    assert int(f["retry_provisions"]) > 0
+    md = engine.MdSystem(s, precision=engine.FP64, shake_maxit=3)
+    md.set_graphs(graphs)
+    md.set_state(s.x, s.v)
+    md.set_canonical(300.0, 3, 1e-3)
+    assert md.celprp() == 0
+    rlog, ier = md.restart_step()
+    assert ier == 0 and rlog["step"][0] == 0
+    x0, _ = md.get_state()
+    assert np.array_equal(x0, s.x)
+    log, ier = md.runmd(int(f["retry_loop"]), update=10)
+    assert ier == 0
+    assert md.get_step() == int(f["retry_loop"])
+    x, v = md.get_state()
+    md.close()
+    print(f"max |dx| {np.abs(x - f['retry_x']).max():.2e}, max |dv| {np.abs(v - f['retry_v']).max():.2e}",
+          flush=True)
+    assert np.abs(x - f["retry_x"]).max() < 1e-9
+    assert np.abs(v - f["retry_v"]).max() < 1e-9
+
+
+def test_restart_step_keeps_the_state_without_a_retry(golden):
+    """Without a SHAKE failure the restart set-up step leaves coordinates,
+    velocities and the step counter as they were, and its record is the
+    energy of the restart state (E_pot of the oracle's forces at x)."""
+    s, f = golden("water_1500")
+    print("case restart step", flush=True)
+    md = engine.MdSystem(s, precision=engine.FP64)
+    md.set_state(s.x, s.v)
+    md.set_step(25)
+    assert md.celprp() == 0
+    rlog, ier = md.restart_step()
+    assert ier == 0 and rlog["step"][0] == 25 and md.get_step() == 25
+    x, v = md.get_state()
+    md.close()
+    assert np.array_equal(x, s.x) and np.array_equal(v, s.v)
+    o = Oracle(s)
+    o.celprp(s.x)
+    g, et = o.bonded(s.x)
+    g, ee, ev = o.nonbond(s.x, g)
+    epot = et.sum() + ee.sum() + ev.sum()
+    assert abs(rlog["epot"][0] - epot) < 1e-9 * max(1.0, abs(epot))

@@ -175,3 +175,25 @@ def test_oracle_run_nve_equals_reference_trajectory(golden):
         x, v = s.x.copy(), s.v.copy()
         assert o.run_nve(x, v, int(f["traj_nsteps"]), 10) == 0
         assert np.array_equal(x, f["traj_x"]) and np.array_equal(v, f["traj_v"]), name
+
+
+def test_cano_shake_retry_equals_reference(golden):
+    """METHOD= CANO with LIMSHK= 3 (SHAKE fails at every full 2 fs step):
+    Oracle.run_cano's verlethe retry (undo, halve dt, redo the forces, double
+    back after two half substeps) and md_run's restart set-up step reproduce
+    md_run's trajectory bit for bit.  md_run printed one provision more than
+    the oracle counts: finmd repeats a step from the final state for its
+    report and discards it (md.f90:905-955).  Without the restart set-up step
+    (inirst's velocities from verlethe's velback) the trajectory differs."""
+    s, f = golden("water_1500")
+    o = Oracle(s, shake_maxit=3)
+    x, v = s.x.copy(), s.v.copy()
+    ier, nprov = o.run_cano(x, v, int(f["retry_loop"]), 10, 300.0, 3, 1e-3)
+    assert ier == 0
+    assert nprov + 1 == int(f["retry_provisions"])
+    assert np.array_equal(x, f["retry_x"]) and np.array_equal(v, f["retry_v"])
+    o = Oracle(s, shake_maxit=3)
+    x, v = s.x.copy(), s.v.copy()
+    ier, nprov = o.run_cano(x, v, int(f["retry_loop"]), 10, 300.0, 3, 1e-3, inirst=False)
+    assert ier == 0 and nprov + 2 == int(f["retry_provisions"])
+    assert np.abs(x - f["retry_x"]).max() > 1e-3
