@@ -87,6 +87,32 @@ __device__ __forceinline__ T wave_sum(T v) {
 
 __device__ __forceinline__ long long f2fix(double v, double scale) { return __double2ll_rn(v * scale); }
 
+// cnt[idx] += 1 for every active lane, with one atomic per distinct idx in
+// the wavefront (neighbouring atoms share cells, and same-address atomics
+// serialize): lanes are grouped by idx first, then every group's first lane
+// adds the group's size, all in one atomic instruction.  Returns the lane's
+// old value: its group's base plus its rank among the group's lanes.
+__device__ __forceinline__ int wave_inc(int *__restrict__ cnt, int idx, bool act) {
+    const int l = (int)__builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+    uint64_t todo = __ballot(act);
+    int lead_of = l, rank = 0, size = 0;
+    while (todo) {
+        const int lead = (int)__builtin_ctzll(todo);
+        const int key = __builtin_amdgcn_readlane(idx, lead);
+        const uint64_t grp = __ballot(act && idx == key) & todo;
+        if ((grp >> l) & 1ull) {
+            lead_of = lead;
+            rank = (int)__popcll(grp & ((1ull << l) - 1ull));
+        }
+        if (l == lead) size = (int)__popcll(grp);
+        todo &= ~grp;
+    }
+    int base = 0;
+    if (act && lead_of == l) base = atomicAdd(cnt + idx, size);
+    base = __shfl(base, lead_of, 64);
+    return base + rank;
+}
+
 // Index of cell (x,y,z) along the 3-D Hilbert curve of a 2^bits grid
 // (J. Skilling, "Programming the Hilbert curve", AIP Conf. Proc. 707, 2004:
 // axes -> transpose, then bit interleave).  Consecutive indices are face
@@ -153,39 +179,43 @@ __global__ void k_bin(int natom, const int *__restrict__ atom_sys, const SysP *_
                       int *__restrict__ subkey, int *__restrict__ refcell, int *__restrict__ cnt,
                       int *__restrict__ err, const int *__restrict__ tflag, int *__restrict__ mcnt) {
 #pragma clang fp contract(off)
-    int a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= natom) return;
-    const SysP &P = sp[atom_sys[a]];
-    int c[3];
-    for (int k = 0; k < 3; ++k) {
-        double p = x[3 * a + k];
-        double f = floor((p - P.wall[k]) * P.invL[k]);
-        double g = p - P.L[k] * f;
-        if (!(fabs(p) < 1e15)) { atomicOr(err, 1); f = 0; g = P.wall[k]; }
-        img[3 * a + k] = (int)f;
-        refcell[3 * a + k] = (int)((g - P.rorg[k]) * P.rr) + 1;
-        int ck = (int)floor((g - P.wall[k]) * P.icsz[k] * 2.0);   // half-cell resolution
-        ck = ck < 0 ? 0 : (ck >= 2 * P.nc[k] ? 2 * P.nc[k] - 1 : ck);
-        c[k] = ck;
-    }
-    // md_run's checks (celpar.f90:176-204): an atom outside its cell grid,
-    // or more than ipmax atoms of one cluster class in a cell, is ier 28
-    {
-        const int mx = refcell[3 * a], my = refcell[3 * a + 1], mz = refcell[3 * a + 2];
-        if (mx < 1 || mx > P.mnv || my < 1 || my > P.mnv || mz < 1 || mz > P.mnv) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool act = a < natom;
+    int cell = 0, mci = 0;
+    bool mok = false;
+    if (act) {
+        const SysP &P = sp[atom_sys[a]];
+        int c[3], m[3];
+        for (int k = 0; k < 3; ++k) {
+            double p = x[3 * a + k];
+            double f = floor((p - P.wall[k]) * P.invL[k]);
+            double g = p - P.L[k] * f;
+            if (!(fabs(p) < 1e15)) { atomicOr(err, 1); f = 0; g = P.wall[k]; }
+            img[3 * a + k] = (int)f;
+            m[k] = (int)((g - P.rorg[k]) * P.rr) + 1;
+            refcell[3 * a + k] = m[k];
+            int ck = (int)floor((g - P.wall[k]) * P.icsz[k] * 2.0);   // half-cell resolution
+            ck = ck < 0 ? 0 : (ck >= 2 * P.nc[k] ? 2 * P.nc[k] - 1 : ck);
+            c[k] = ck;
+        }
+        // md_run's checks (celpar.f90:176-204): an atom outside its cell grid,
+        // or more than ipmax atoms of one cluster class in a cell, is ier 28
+        if (m[0] < 1 || m[0] > P.mnv || m[1] < 1 || m[1] > P.mnv || m[2] < 1 || m[2] > P.mnv) {
             atomicOr(err, 1);
         } else {
-            const int lin = ((mz - 1) * P.mnv + (my - 1)) * P.mnv + (mx - 1);
-            const int old = atomicAdd(mcnt + P.mc0 + 2 * lin + ((tflag[a] >> 13) & 1), 1);
-            if (old + 1 > P.ipmax) atomicOr(err, 1);
+            const int lin = ((m[2] - 1) * P.mnv + (m[1] - 1)) * P.mnv + (m[0] - 1);
+            mci = P.mc0 + 2 * lin + ((tflag[a] >> 13) & 1);
+            mok = true;
         }
+        // cells in Hilbert order; inside a cell, atoms ordered by sub-cell
+        const uint32_t h = hilbert3((uint32_t)c[0], (uint32_t)c[1], (uint32_t)c[2], P.hbits + 1);
+        cell = P.cell0 + (int)(h >> 3);
+        cellid[a] = cell;
+        subkey[a] = (int)(h & 7u);
     }
-    // cells in Hilbert order; inside a cell, atoms ordered by sub-cell
-    const uint32_t h = hilbert3((uint32_t)c[0], (uint32_t)c[1], (uint32_t)c[2], P.hbits + 1);
-    int cell = P.cell0 + (int)(h >> 3);
-    cellid[a] = cell;
-    subkey[a] = (int)(h & 7u);
-    atomicAdd(&cnt[cell], 1);
+    const int old = wave_inc(mcnt, mci, mok);
+    if (mok && old + 1 > sp[atom_sys[a]].ipmax) atomicOr(err, 1);
+    (void)wave_inc(cnt, cell, act);
 }
 
 // Exclusive scan of n ints by one 1024-thread block (wave scans + block
@@ -227,28 +257,29 @@ __global__ void __launch_bounds__(1024) k_scan(int n, const int *__restrict__ in
     if (threadIdx.x == 0) out[n] = carry;
 }
 
-__global__ void k_place(int natom, const int *__restrict__ cellid, const int *__restrict__ start,
-                        int *__restrict__ fill, int *__restrict__ tmp) {
-    int a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= natom) return;
-    int c = cellid[a];
-    int p = atomicAdd(&fill[c], 1);
-    tmp[start[c] + p] = a;
+// Atoms of each cell at [start, start + cnt) as 64-bit keys (sub-cell << 32 |
+// atom), in arrival order.
+__global__ void k_place(int natom, const int *__restrict__ cellid, const int *__restrict__ subkey,
+                        const int *__restrict__ start, int *__restrict__ fill, long long *__restrict__ tmp) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool act = a < natom;
+    const int c = act ? cellid[a] : 0;
+    const int p = wave_inc(fill, c, act);
+    if (act) tmp[start[c] + p] = ((long long)subkey[a] << 32) | a;
 }
 
 // Deterministic order inside a cell: rank by (sub-cell, atom index).
 __global__ void k_rank(int natom, const int *__restrict__ atom_sys, const SysP *__restrict__ sp,
                        const int *__restrict__ cellid, const int *__restrict__ subkey, const int *__restrict__ start,
-                       const int *__restrict__ cnt, const int *__restrict__ tmp, int *__restrict__ inum,
+                       const int *__restrict__ cnt, const long long *__restrict__ tmp, int *__restrict__ inum,
                        int *__restrict__ perm) {
     int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= natom) return;
-    int c = cellid[a], b = start[c], n = cnt[c], r = 0;
+    const int c = cellid[a], b = start[c], n = cnt[c];
     const long long ka = ((long long)subkey[a] << 32) | a;
-    for (int k = 0; k < n; ++k) {
-        const int o = tmp[b + k];
-        r += (((long long)subkey[o] << 32) | o) < ka;
-    }
+    int r = 0;
+#pragma unroll 8
+    for (int k = 0; k < n; ++k) r += tmp[b + k] < ka;
     const SysP &P = sp[atom_sys[a]];
     int slot = b + r - P.atom0 + P.slot0;
     inum[a] = slot;

@@ -66,34 +66,40 @@ __host__ __device__ __forceinline__ void shift_of(uint32_t code, int &sx, int &s
 // Box and centres of every cluster (wrapped coordinates at this rebuild: FP64
 // box for the candidate search and the pair kernels' frame, FP32 centre for
 // the list test), and how far it sticks out of its home cell: maximum per cell
-// and per system (non-negative floats compare like their bit patterns).
+// per cell (non-negative floats compare like their bit patterns; k_crec
+// takes the maximum per system).  One lane per slot, the 8 lanes of a
+// cluster reduced by shuffles.
 __global__ void __launch_bounds__(256)
 k_cbox(int nclu, const int *__restrict__ clu_sys, const SysP *__restrict__ sp, const int *__restrict__ perm,
        const int *__restrict__ cellid, const int *__restrict__ ccoord, const JAtom *__restrict__ ys,
        const float4 *__restrict__ lp, double *__restrict__ cbb, double4 *__restrict__ ccen,
-       float4 *__restrict__ ccf, unsigned *__restrict__ cellext, unsigned *__restrict__ sysext) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+       float4 *__restrict__ ccf, unsigned *__restrict__ cellext) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = t >> 3, k = t & 7, l = threadIdx.x & 63;
     const bool act = c < nclu;
     const int s = act ? clu_sys[c] : -1;
-    float ef = 0.0f;
-    if (act) {
-        const SysP &P = sp[s];
-        double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
-        float flo[3] = {1e30f, 1e30f, 1e30f}, fhi[3] = {-1e30f, -1e30f, -1e30f};
-        int n = 0;
-        for (int k = 0; k < JP_CL; ++k) {
-            if (perm[c * JP_CL + k] < 0) continue;
-            const JAtom a = ys[c * JP_CL + k];
-            const float4 q = lp[c * JP_CL + k];
-            lo[0] = fmin(lo[0], a.x); hi[0] = fmax(hi[0], a.x);
-            lo[1] = fmin(lo[1], a.y); hi[1] = fmax(hi[1], a.y);
-            lo[2] = fmin(lo[2], a.z); hi[2] = fmax(hi[2], a.z);
-            flo[0] = fminf(flo[0], q.x); fhi[0] = fmaxf(fhi[0], q.x);
-            flo[1] = fminf(flo[1], q.y); fhi[1] = fmaxf(fhi[1], q.y);
-            flo[2] = fminf(flo[2], q.z); fhi[2] = fmaxf(fhi[2], q.z);
-            ++n;
+    const bool real = act && perm[t] >= 0;
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    float flo[3] = {1e30f, 1e30f, 1e30f}, fhi[3] = {-1e30f, -1e30f, -1e30f};
+    if (real) {
+        const JAtom a = ys[t];
+        const float4 q = lp[t];
+        lo[0] = hi[0] = a.x; lo[1] = hi[1] = a.y; lo[2] = hi[2] = a.z;
+        flo[0] = fhi[0] = q.x; flo[1] = fhi[1] = q.y; flo[2] = fhi[2] = q.z;
+    }
+#pragma unroll
+    for (int o = 1; o < JP_CL; o <<= 1)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = fmin(lo[d], __shfl_xor(lo[d], o, 64));
+            hi[d] = fmax(hi[d], __shfl_xor(hi[d], o, 64));
+            flo[d] = fminf(flo[d], __shfl_xor(flo[d], o, 64));
+            fhi[d] = fmaxf(fhi[d], __shfl_xor(fhi[d], o, 64));
         }
-        for (int k = 0; k < 3; ++k) { cbb[6 * c + k] = lo[k]; cbb[6 * c + 3 + k] = hi[k]; }
+    const int n = __popcll((__ballot(real) >> (l & ~7)) & 0xffull);
+    if (act && k == 0) {
+        const SysP &P = sp[s];
+        for (int d = 0; d < 3; ++d) { cbb[6 * c + d] = lo[d]; cbb[6 * c + 3 + d] = hi[d]; }
         if (n == 0) {
             ccen[c] = make_double4(0.0, 0.0, 0.0, 0.0);
             ccf[c] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -101,27 +107,16 @@ k_cbox(int nclu, const int *__restrict__ clu_sys, const SysP *__restrict__ sp, c
             ccen[c] = make_double4(0.5 * (lo[0] + hi[0]), 0.5 * (lo[1] + hi[1]), 0.5 * (lo[2] + hi[2]), 0.0);
             ccf[c] = make_float4(0.5f * (flo[0] + fhi[0]), 0.5f * (flo[1] + fhi[1]), 0.5f * (flo[2] + fhi[2]), 0.f);
             // slots are real atoms first, padding last: a non-empty cluster's first slot is real
-            const int cell = cellid[perm[c * JP_CL]];
+            const int cell = cellid[perm[t]];
             const int cc = ccoord[cell];
             const int cx[3] = {cc & 1023, (cc >> 10) & 1023, (cc >> 20) & 1023};
             double e = 0.0;
-            for (int k = 0; k < 3; ++k) {
-                const double clo = P.wall[k] + cx[k] * P.csz[k], chi = P.wall[k] + (cx[k] + 1) * P.csz[k];
-                e = fmax(e, fmax(clo - lo[k], hi[k] - chi));
+            for (int d = 0; d < 3; ++d) {
+                const double clo = P.wall[d] + cx[d] * P.csz[d], chi = P.wall[d] + (cx[d] + 1) * P.csz[d];
+                e = fmax(e, fmax(clo - lo[d], hi[d] - chi));
             }
-            ef = (float)e * 1.0001f + 1e-5f;
-            atomicMax(cellext + cell, __float_as_uint(ef));
+            atomicMax(cellext + cell, __float_as_uint((float)e * 1.0001f + 1e-5f));
         }
-    }
-    // per-system maximum: one atomic per wavefront when it holds one system
-    const int s0 = __builtin_amdgcn_readfirstlane(s);
-    if (__ballot(act && s != s0) == 0) {
-        float m = ef;
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
-        if ((threadIdx.x & 63) == 0 && s0 >= 0) atomicMax(sysext + s0, __float_as_uint(m));
-    } else if (act) {
-        atomicMax(sysext + s, __float_as_uint(ef));
     }
 }
 
@@ -157,13 +152,30 @@ __device__ __forceinline__ CluRange clu_range(const SysP &P, const double *__res
 }
 
 // Per box-grid position: {first slot, atoms, ext bits, 0} of the cell there
-// (one load per candidate cell in k_list8).
+// (one load per candidate cell in k_list8), and the largest extent of each
+// system (one atomic per wavefront when it holds one system).
 __global__ void k_crec(int ncell, const int *__restrict__ crank, const int *__restrict__ cslot,
-                       const int *__restrict__ ccnt, const unsigned *__restrict__ cellext, int4 *__restrict__ crec) {
+                       const int *__restrict__ ccnt, const unsigned *__restrict__ cellext, int4 *__restrict__ crec,
+                       const int *__restrict__ cell_sys, unsigned *__restrict__ sysext) {
     const int gi = blockIdx.x * blockDim.x + threadIdx.x;
-    if (gi >= ncell) return;
-    const int c = crank[gi];
-    crec[gi] = make_int4(cslot[c], ccnt[c], (int)cellext[c], 0);
+    const bool act = gi < ncell;
+    unsigned e = 0u;
+    int s = -1;
+    if (act) {
+        const int c = crank[gi];
+        e = cellext[c];
+        s = cell_sys[c];
+        crec[gi] = make_int4(cslot[c], ccnt[c], (int)e, 0);
+    }
+    const int s0 = __builtin_amdgcn_readfirstlane(s);
+    if (__ballot(act && s != s0) == 0) {
+        unsigned m = e;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) m = max(m, (unsigned)__shfl_xor((int)m, o, 64));
+        if ((threadIdx.x & 63) == 0 && s0 >= 0) atomicMax(sysext + s0, m);
+    } else if (act) {
+        atomicMax(sysext + s, e);
+    }
 }
 
 // Candidate q -> its j-clusters [k0, k1) and image shift; k1 == k0 when the

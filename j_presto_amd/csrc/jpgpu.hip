@@ -204,7 +204,8 @@ struct Engine {
     DBuf<double2> d_lj64;
     DBuf<float2> d_lj32;
     DBuf<int> d_subkey;
-    DBuf<int> d_img, d_cellid, d_refcell, d_cnt, d_start, d_fill, d_tmp, d_inum, d_perm, d_cslot;
+    DBuf<int> d_img, d_cellid, d_refcell, d_cnt, d_start, d_fill, d_inum, d_perm, d_cslot;
+    DBuf<long long> d_tmp;    // per cell, (sub-cell << 32 | atom) keys in arrival order
     DBuf<JAtom> d_ys;
     DBuf<double> d_qd, d_cbb;
     DBuf<double4> d_xr, d_ccen;
@@ -890,17 +891,18 @@ int Engine::build_list(bool defer) {
     hipLaunchKernelGGL(k_bin, grid1(natom, 256), 256, 0, st, natom, d_atom_sys.p, d_sp.p, d_x.p, d_img.p,
                        d_cellid.p, d_subkey.p, d_refcell.p, d_cnt.p, d_lst.p, d_tflag.p, d_mcnt.p);
     hipLaunchKernelGGL(k_scan, 1, 1024, 0, st, ncell, d_cnt.p, d_start.p);
-    hipLaunchKernelGGL(k_place, grid1(natom, 256), 256, 0, st, natom, d_cellid.p, d_start.p, d_fill.p, d_tmp.p);
+    hipLaunchKernelGGL(k_place, grid1(natom, 256), 256, 0, st, natom, d_cellid.p, d_subkey.p, d_start.p, d_fill.p,
+                       d_tmp.p);
     hipLaunchKernelGGL(k_rank, grid1(natom, 256), 256, 0, st, natom, d_atom_sys.p, d_sp.p, d_cellid.p, d_subkey.p,
                        d_start.p, d_cnt.p, d_tmp.p, d_inum.p, d_perm.p);
     hipLaunchKernelGGL(k_cslot, grid1(ncell, 256), 256, 0, st, ncell, d_cell_sys.p, d_sp.p, d_start.p, d_cslot.p);
     hipLaunchKernelGGL(k_fill_sorted, grid1(nslot, 256), 256, 0, st, nslot, d_slot_sys.p, d_sp.p, d_perm.p, d_x.p,
                        d_img.p, d_chg.p, d_tflag.p, d_mol.p, d_ys.p, d_qd.p, d_xr.p, d_imgs.p, d_lp.p,
                        d_refcell.p, d_mcell.p);
-    hipLaunchKernelGGL(k_cbox, grid1(nclu, 256), 256, 0, st, nclu, d_clu_sys.p, d_sp.p, d_perm.p, d_cellid.p,
-                       d_ccoord.p, d_ys.p, d_lp.p, d_cbb.p, d_ccen.p, d_ccf.p, d_cellext.p, d_sysext.p);
+    hipLaunchKernelGGL(k_cbox, grid1(JP_CL * nclu, 256), 256, 0, st, nclu, d_clu_sys.p, d_sp.p, d_perm.p, d_cellid.p,
+                       d_ccoord.p, d_ys.p, d_lp.p, d_cbb.p, d_ccen.p, d_ccf.p, d_cellext.p);
     hipLaunchKernelGGL(k_crec, grid1(ncell, 256), 256, 0, st, ncell, d_crank.p, d_cslot.p, d_cnt.p, d_cellext.p,
-                       d_crec.p);
+                       d_crec.p, d_cell_sys.p, d_sysext.p);
     // Segments of ccap entries per i-cluster.  ccap only grows (lists above
     // 7/8 of it report their length), so captured step graphs stay valid.
     cell_err = 0;
