@@ -1236,6 +1236,68 @@ int jo_run_nve(jo_ctx *c, double *x, double *v, int nsteps, int update) {
     return ier;
 }
 
+/* Multicanonical (METHOD= GE) parameters of inmcmd.f90:20-31: nwindow
+ * windows [low_v, high_v) of a degree-ndeg polynomial c[w*(ndeg+1) + j] in
+ * the reduced energy, the constant weights below celw and above ceup, and
+ * tempce. */
+typedef struct {
+    int nwindow, ndeg;
+    const double *c, *low, *high;
+    double alphalw, alphaup, celw, ceup, tempce;
+} jo_ge;
+
+/* x**j for an integer j >= 1, as llvm.powi (flang's lowering of real**int)
+ * computes it: binary exponentiation, squaring the base and multiplying the
+ * bits set in j from the lowest up. */
+static double powi_d(double a, int b) {
+    double r = 1.0;
+    for (;;) {
+        if (b & 1) r = r * a;
+        b /= 2;
+        if (b == 0) break;
+        a = a * a;
+    }
+    return r;
+}
+
+/* The force weight of multicanonical MD (verlethe.f90:196-222): from the
+ * class-1 potential energy E, dprobl = alphalw below celw, alphaup above
+ * ceup, else the polynomial of the first window whose high_v exceeds the
+ * reduced energy (E - celw)/(ceup - celw), taken about its low_v; the
+ * weight is (gascst/joucal)*1e-3 * tempce * dprobl.  Reduced energies at or
+ * above the last high_v, where md_run reads past its window arrays, take the
+ * last window. */
+double jo_ge_weight(const jo_ge *g, double E) {
+    double dprobl;
+    if (E <= g->celw) {
+        dprobl = g->alphalw;
+    } else if (E >= g->ceup) {
+        dprobl = g->alphaup;
+    } else {
+        double r = (E - g->celw) / (g->ceup - g->celw);
+        int i = 0;
+        while (i < g->nwindow - 1 && !(r < g->high[i])) ++i;
+        r = r - g->low[i];
+        const double *ci = g->c + (size_t)i * (g->ndeg + 1);
+        dprobl = ci[0];
+        for (int j = 1; j <= g->ndeg; ++j) dprobl = dprobl + ci[j] * powi_d(r, j);
+    }
+    double dew = (8.31451 / 4.184) * 1.0e-3; /* PHYCNS.f90:21,25 gascst, joucal */
+    dew = dew * g->tempce * dprobl;
+    return dew;
+}
+
+/* epotcc(1,1): dnergy's sum of the energy slots 2..maxene of class 1
+ * (dnergy.f90:78), bond, angle, torsion, improper, 1-4 vdW, 1-4 elec,
+ * VDW15, ELE15 in slot order; the H-bond and restraint slots are zero. */
+static double class1_epot(const double *eterm, const double *ee, const double *ev) {
+    double e = 0.0;
+    for (int k = 0; k < 6; ++k) e = e + eterm[k];
+    e = e + ev[0];
+    e = e + ee[0];
+    return e;
+}
+
 /* One verlethe call for METHOD= CANO with its SHAKE retry: the substep's
  * start is backed up (verlethe.f90:83-98), the forces evaluated (103), the
  * step of length T taken; a SHAKE failure restores x and v of the substep
@@ -1246,7 +1308,7 @@ int jo_run_nve(jo_ctx *c, double *x, double *v, int nsteps, int update) {
  * receives the velocities the last substep started from. */
 static int cano_verlethe(jo_ctx *c, double *x, double *v, double *vback, double *grad, double *gt, double *xo,
                          double *xs, double *vs, double temps, int icslop, double fcstol, int ngrp, const int *grp,
-                         const double *dof, int *nprov) {
+                         const double *dof, const jo_ge *ge, int *nprov) {
     const int n = c->n, nfrg = c->nfrg;
     const double dt0 = c->dt;
     double T = dt0, eterm[18], ee[3], ev[3];
@@ -1259,6 +1321,10 @@ static int cano_verlethe(jo_ctx *c, double *x, double *v, double *vback, double 
         jo_bonded(c, x, grad, eterm);
         jo_nonbond(c, x, grad, ee, ev);
         jo_combine_grad(c, grad, gt);
+        if (ge) { /* METHOD= GE: grad(:,:,1) scaled by the weight (verlethe.f90:223-233) */
+            const double dew = jo_ge_weight(ge, class1_epot(eterm, ee, ev));
+            for (size_t q = 0; q < 3 * (size_t)n; ++q) gt[q] = gt[q] * dew;
+        }
         ier = jo_cano_step(c, x, v, gt, xo, vback, temps, icslop, fcstol, ngrp, grp, dof);
         if (ier) {
             memcpy(x, xs, sizeof(double) * 3 * n);
@@ -1283,14 +1349,14 @@ static int cano_verlethe(jo_ctx *c, double *x, double *v, double *vback, double 
     return ier;
 }
 
-/* md_run's METHOD= CANO loop from a restart: with inirst, the restart
+/* md_run's METHOD= CANO loop (METHOD= GE with ge, nfrg = 1) from a restart: with inirst, the restart
  * set-up first (md.f90:1104-1263: one verlethe call, then the coordinates
  * back and vel = verlethe's velback, the velocities its last substep started
  * from), then nsteps verlethe calls with a list rebuild after every step
  * that is a multiple of update (md.f90:519-694).  nprov counts the SHAKE
  * provisions (failed substeps) md_run prints. */
 int jo_run_cano(jo_ctx *c, double *x, double *v, int nsteps, int update, int inirst, double temps, int icslop,
-                double fcstol, int ngrp, const int *grp, const double *dof, int *nprov) {
+                double fcstol, int ngrp, const int *grp, const double *dof, const jo_ge *ge, int *nprov) {
     const size_t n3 = 3 * (size_t)c->n;
     double *grad = (double *)malloc(sizeof(double) * n3 * c->nfrg);
     double *gt = (double *)malloc(sizeof(double) * n3);
@@ -1300,15 +1366,19 @@ int jo_run_cano(jo_ctx *c, double *x, double *v, int nsteps, int update, int ini
     double *vs = (double *)malloc(sizeof(double) * n3);
     double *x0 = (double *)malloc(sizeof(double) * n3);
     *nprov = 0;
+    if (ge && (c->nfrg != 1 || ge->nwindow < 1 || ge->ndeg < 0)) {
+        free(grad); free(gt); free(xo); free(vb); free(xs); free(vs); free(x0);
+        return JP_EINVAL;
+    }
     int ier = c->have_list ? 0 : jo_celprp(c, x);
     if (!ier && inirst) {
         memcpy(x0, x, sizeof(double) * n3);
-        ier = cano_verlethe(c, x, v, vb, grad, gt, xo, xs, vs, temps, icslop, fcstol, ngrp, grp, dof, nprov);
+        ier = cano_verlethe(c, x, v, vb, grad, gt, xo, xs, vs, temps, icslop, fcstol, ngrp, grp, dof, ge, nprov);
         memcpy(x, x0, sizeof(double) * n3);
         memcpy(v, vb, sizeof(double) * n3);
     }
     for (int s = 1; s <= nsteps && !ier; ++s) {
-        ier = cano_verlethe(c, x, v, vb, grad, gt, xo, xs, vs, temps, icslop, fcstol, ngrp, grp, dof, nprov);
+        ier = cano_verlethe(c, x, v, vb, grad, gt, xo, xs, vs, temps, icslop, fcstol, ngrp, grp, dof, ge, nprov);
         if (!ier && update > 0 && s % update == 0) ier = jo_celprp(c, x);
     }
     free(grad); free(gt); free(xo); free(vb); free(xs); free(vs); free(x0);

@@ -51,7 +51,9 @@ def lib():
         L.jo_ekin.restype = C.c_double
         L.jo_run_nve.argtypes = [C.c_void_p, f64p, f64p, C.c_int, C.c_int]
         L.jo_run_cano.argtypes = [C.c_void_p, f64p, f64p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int,
-                                  C.c_double, C.c_int, i32p, f64p, i32p]
+                                  C.c_double, C.c_int, i32p, f64p, C.POINTER(GeParams), i32p]
+        L.jo_ge_weight.argtypes = [C.POINTER(GeParams), C.c_double]
+        L.jo_ge_weight.restype = C.c_double
         L.jo_list_brute.argtypes = [C.c_void_p, f64p, i32p, i32p, C.c_int64]
         L.jo_list_brute.restype = C.c_int64
         L.jo_constants.argtypes = [C.c_void_p, f64p]
@@ -64,6 +66,30 @@ def lib():
 
 def _p(a, t=f64p):
     return a.ctypes.data_as(t)
+
+
+class GeParams(C.Structure):
+    """jo_ge: multicanonical parameters (inmcmd.f90:20-31)."""
+    _fields_ = [("nwindow", C.c_int), ("ndeg", C.c_int), ("c", f64p), ("low", f64p), ("high", f64p),
+                ("alphalw", C.c_double), ("alphaup", C.c_double), ("celw", C.c_double), ("ceup", C.c_double),
+                ("tempce", C.c_double)]
+
+
+def ge_params(ge: dict):
+    """(GeParams, arrays it points into) from a dict with c (nwindow, ndeg+1),
+    low, high (nwindow), alphalw, alphaup, celw, ceup, tempce."""
+    c = np.ascontiguousarray(ge["c"], np.float64)
+    low = np.ascontiguousarray(ge["low"], np.float64)
+    high = np.ascontiguousarray(ge["high"], np.float64)
+    p = GeParams(c.shape[0], c.shape[1] - 1, _p(c), _p(low), _p(high), float(ge["alphalw"]), float(ge["alphaup"]),
+                 float(ge["celw"]), float(ge["ceup"]), float(ge["tempce"]))
+    return p, (c, low, high)
+
+
+def ge_weight(ge: dict, e: float) -> float:
+    """md_run's multicanonical force weight at class-1 energy e."""
+    p, keep = ge_params(ge)
+    return lib().jo_ge_weight(C.byref(p), float(e))
 
 
 class Oracle:
@@ -160,14 +186,17 @@ class Oracle:
         return lib().jo_run_nve(self.c, _p(x), _p(v), nsteps, update)
 
     def run_cano(self, x, v, nsteps: int, update: int = 10, temps: float = 300.0, icslop: int = 3,
-                 fcstol: float = 1e-3, grp=None, inirst: bool = True):
-        """md_run's METHOD= CANO loop with verlethe's SHAKE retry, in place,
-        after the restart set-up step when inirst; returns (ier, provisions)."""
+                 fcstol: float = 1e-3, grp=None, inirst: bool = True, ge=None):
+        """md_run's METHOD= CANO loop with verlethe's SHAKE retry (METHOD= GE
+        with the multicanonical parameters ge), in place, after the restart
+        set-up step when inirst; returns (ier, provisions)."""
         dof = self.group_dof(grp)
         g = None if grp is None else np.ascontiguousarray(grp, np.int32)
         nprov = C.c_int32()
+        gp, keep = ge_params(ge) if ge is not None else (None, None)
         ier = lib().jo_run_cano(self.c, _p(x), _p(v), nsteps, update, int(inirst), temps, icslop, fcstol, len(dof),
-                                None if g is None else _p(g, i32p), _p(dof), C.byref(nprov))
+                                None if g is None else _p(g, i32p), _p(dof),
+                                None if gp is None else C.byref(gp), C.byref(nprov))
         return ier, nprov.value
 
     def list_brute(self, x):

@@ -19,10 +19,14 @@ and N_SIMD= 1 (a fixed summation order):
              step, the loop number, E_tot, E_pot, T and the kinetic energy
              caltmp computes from (v(t-dt/2)+v(t+dt/2))/2 (md.f90:618-628),
              each to 8 significant digits (T: 7 decimals).
+  ge jobs    (water_1500, --ge-only) RESTAR= YES, METHOD= GE with a NAMEGE=
+             parameter file (GE_POLY etc. below), SETTEM= 300, TEMPCO= NO:
+             multicanonical steps whose forces carry the weight of the
+             class-1 energy (verlethe.f90:196-233).
 A fixture stores the generator recipe, the input x and v, and the outputs.
 The reference itself never leaves this container.
 
-Usage: python tests/golden/make_golden.py [--cano-only | --mon-only | --host-only] [name ...]
+Usage: python tests/golden/make_golden.py [--cano-only | --mon-only | --host-only | --ge-only] [name ...]
 """
 from __future__ import annotations
 
@@ -242,6 +246,66 @@ def host_jobs(s: systems.System, tmp: str) -> dict:
     return out
 
 
+# Multicanonical (METHOD= GE) parameter sets for water_1500: three windows of
+# cubic polynomials in the reduced energy, weights near 1 at 300 K; the
+# energy range [celw, ceup] puts the run in the windows ("win"), below celw
+# ("lo": alphalw) or above ceup ("hi": alphaup) for its whole length.
+GE_POLY = np.array([[1.72, -0.20, 0.15, -0.05], [1.68, -0.12, 0.10, 0.03], [1.64, 0.05, -0.08, 0.02]])
+GE_LOW, GE_HIGH = np.array([0.0, 0.3, 0.6]), np.array([0.3, 0.6, 1.0])
+GE_RUNS = {"win": (-4500.0, 0.0, 20), "lo": (1000.0, 2000.0, 10), "hi": (-9000.0, -8000.0, 10)}
+GE_ALPHA = (1.70, 1.62)      # alphalw, alphaup
+GE_TEMPCE = 300.0
+
+
+def write_ge_params(path: str, celw: float, ceup: float) -> None:
+    """inmcmd.f90:20-31 layout: nwindow ndeg; per window one coefficient per
+    record, then low_v high_v; alphalw alphaup; celw ceup; tempce."""
+    nw, nd1 = GE_POLY.shape
+    L = [f"{nw} {nd1 - 1}"]
+    for w in range(nw):
+        L += [repr(float(c)) for c in GE_POLY[w]]
+        L.append(f"{float(GE_LOW[w])!r} {float(GE_HIGH[w])!r}")
+    L += [f"{GE_ALPHA[0]!r} {GE_ALPHA[1]!r}", f"{celw!r} {ceup!r}", repr(GE_TEMPCE)]
+    with open(path, "w") as f:
+        f.write("\n".join(L) + "\n")
+
+
+def ge_jobs(s: systems.System, tmp: str) -> dict:
+    """METHOD= GE (idfvsl = 4, verlethe.f90:196-233) runs from the fixture's
+    restart with TEMPCO= NO at 300 K, one per GE_RUNS entry: final x, v."""
+    out = dict(ge_poly=GE_POLY, ge_low=GE_LOW, ge_high=GE_HIGH, ge_alpha=np.array(GE_ALPHA),
+               ge_tempce=np.array(GE_TEMPCE))
+    for tag, (celw, ceup, nsteps) in GE_RUNS.items():
+        d = os.path.join(tmp, "ge_" + tag)
+        os.makedirs(d)
+        refio.write_restart(os.path.join(d, "in.rst"), s.x, s.v, lam=s.lam)
+        write_ge_params(os.path.join(d, "ge.par"), celw, ceup)
+        run_md(d, s, {"METHOD": "GE", "RESTAR": "YES", "NAMERI": "in.rst", "NAMERO": "out.rst", "UPDATE": "10",
+                      "TIMEST": "2.0", "SETTEM": "300.0", "TEMPCO": "NO", "NAMEGE": "ge.par",
+                      "LOOPLI": str(nsteps), "OUTLOG": "100000", "OUTCLE": "100000"})
+        r = refio.read_restart(os.path.join(d, "out.rst"))
+        out.update({f"ge_{tag}_x": r["x"], f"ge_{tag}_v": r["v"], f"ge_{tag}_loop": np.int64(r["loop"]),
+                    f"ge_{tag}_range": np.array([celw, ceup])})
+    return out
+
+
+def add_ge(name: str) -> str:
+    """Adds the METHOD= GE runs to an existing fixture."""
+    kind, args, opt = RECIPES[name]
+    s = build_system(kind, args)
+    path = os.path.join(HERE, f"{name}.npz")
+    with np.load(path, allow_pickle=False) as z:
+        out = {k: z[k] for k in z.files}
+    assert str(out["fingerprint"]) == fingerprint(s)
+    tmp = tempfile.mkdtemp(prefix="jpgold_")
+    try:
+        out.update(ge_jobs(s, tmp))
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    np.savez_compressed(path, **out)
+    return path
+
+
 def add_host(name: str) -> str:
     """Adds the md.inp host jobs' outputs to an existing fixture."""
     kind, args, opt = RECIPES[name]
@@ -299,10 +363,13 @@ def main(argv):
     cano_only = "--cano-only" in argv
     mon_only = "--mon-only" in argv
     host_only = "--host-only" in argv
-    argv = [a for a in argv if a not in ("--cano-only", "--mon-only", "--host-only")]
+    ge_only = "--ge-only" in argv
+    argv = [a for a in argv if a not in ("--cano-only", "--mon-only", "--host-only", "--ge-only")]
     names = argv or [n for n in RECIPES if not mon_only or RECIPES[n][2].get("mon")]
     for n in names:
-        if host_only:
+        if ge_only:
+            p = add_ge(n)
+        elif host_only:
             p = add_host(n)
         else:
             p = add_cano(n) if cano_only else (add_mon(n) if mon_only else make(n))

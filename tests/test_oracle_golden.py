@@ -197,3 +197,33 @@ def test_cano_shake_retry_equals_reference(golden):
     ier, nprov = o.run_cano(x, v, int(f["retry_loop"]), 10, 300.0, 3, 1e-3, inirst=False)
     assert ier == 0 and nprov + 2 == int(f["retry_provisions"])
     assert np.abs(x - f["retry_x"]).max() > 1e-3
+
+
+def ge_params(f, tag):
+    """The multicanonical parameters of the fixture's METHOD= GE run `tag`."""
+    celw, ceup = f[f"ge_{tag}_range"]
+    return dict(c=f["ge_poly"], low=f["ge_low"], high=f["ge_high"], alphalw=float(f["ge_alpha"][0]),
+                alphaup=float(f["ge_alpha"][1]), celw=float(celw), ceup=float(ceup), tempce=float(f["ge_tempce"]))
+
+
+@pytest.mark.parametrize("tag", ["win", "lo", "hi"])
+def test_ge_trajectory_equals_reference(golden, tag):
+    """METHOD= GE (multicanonical, verlethe.f90:196-233): the forces carry
+    the weight of the class-1 energy, from the window polynomials ("win":
+    the run's reduced energy falls from 0.87 to about 0.2 through all three
+    windows), alphalw below celw ("lo") or alphaup above ceup ("hi").
+    Oracle.run_cano with the parameters reproduces md_run bit for bit; the
+    same run with the two constant weights exchanged does not."""
+    s, f = golden("water_1500")
+    ge = ge_params(f, tag)
+    n = int(f[f"ge_{tag}_loop"])
+    o = Oracle(s)
+    x, v = s.x.copy(), s.v.copy()
+    assert o.run_cano(x, v, n, 10, 300.0, 3, 1e-3, ge=ge) == (0, 0)
+    assert np.array_equal(x, f[f"ge_{tag}_x"]) and np.array_equal(v, f[f"ge_{tag}_v"])
+    if tag != "win":
+        bad = dict(ge, alphalw=ge["alphaup"], alphaup=ge["alphalw"])
+        o = Oracle(s)
+        x, v = s.x.copy(), s.v.copy()
+        assert o.run_cano(x, v, n, 10, 300.0, 3, 1e-3, ge=bad) == (0, 0)
+        assert np.abs(x - f[f"ge_{tag}_x"]).max() > 1e-6
