@@ -237,6 +237,22 @@ int jp_last_timing(jp_ctx *ctx, double *t_total, double *t_pair, double *t_list,
  * label < 1 or a group without degrees of freedom. */
 int jp_set_canonical(jp_ctx *ctx, int32_t on, double temps, int32_t icslop, double fcstol, const int32_t *group);
 
+/* Multicanonical MD (md_run METHOD= GE, idfvsl = 4, verlethe.f90:196-233) for
+ * the following jp_run_nve calls, on top of jp_set_canonical's temperature
+ * control: every step's gradient is multiplied by the weight
+ * (gascst/joucal) 1e-3 tempce dprobl of the step's potential energy E
+ * (epotcc(1,1)), with dprobl = alphalw for E <= celw, alphaup for E >= ceup,
+ * else the polynomial c[i*(ndeg+1) + j] (r - low[i])^j, j = 0..ndeg, of the
+ * first window i with r = (E - celw)/(ceup - celw) < high[i] (the last
+ * window beyond it); the parameters of the NAMEGE= file (inmcmd.f90:20-31).
+ * Every step then evaluates energies, and E never leaves the device.  One
+ * system with one energy class (no LMD; ALSD and two-class McMD are not
+ * provided).  on = 0 switches it off.  Returns JP_EINVAL for other systems,
+ * nwindow < 1, ndeg < 0, celw >= ceup or non-finite parameters; jp_run_nve
+ * returns JP_EINVAL while it is on without jp_set_canonical. */
+int jp_set_mcmd(jp_ctx *ctx, int32_t on, int32_t nwindow, int32_t ndeg, const double *c, const double *low,
+                const double *high, double alphalw, double alphaup, double celw, double ceup, double tempce);
+
 /* Mean duration in seconds of the force-step pair kernel (the kernel of every
  * step without energies) over reps launches back to back on the current list
  * and coordinates, measured with two HIP events around the launches.  A

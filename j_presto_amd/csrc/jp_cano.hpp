@@ -50,6 +50,7 @@ struct CanoArgs {
     const int *shk_on;         // system s has SHAKE groups
     const double *gdof, *temps;
     const double *mass, *imass, *gslot;
+    const double *gew;         // METHOD= GE: per system, the weight every gradient read carries (null: 1)
     double *x, *v, *xold, *vback, *fcc, *xcb;
     double *pa;                // partials of m|w|^2 of begin: [group][atom block]
     double *pu, *pn;           // of w and w' per round, two buffers by round parity:
@@ -136,6 +137,14 @@ __device__ __forceinline__ void block_temps2(const CanoArgs &A, const double *__
     __syncthreads();
 }
 
+// Gradient component k of slot sl as md_run uses it: under METHOD= GE
+// scaled in place by the multicanonical weight (verlethe.f90:223-233).
+__device__ __forceinline__ double cano_grad(const CanoArgs &A, int s, size_t sl, int k) {
+#pragma clang fp contract(off)
+    const double g = A.gslot[3 * sl + k];
+    return A.gew ? g * A.gew[s] : g;
+}
+
 __device__ __forceinline__ double cano_mw2(double m, double w0, double w1, double w2) {
 #pragma clang fp contract(off)
     return m * (w0 * w0 + w1 * w1 + w2 * w2);
@@ -166,7 +175,7 @@ __global__ void __launch_bounds__(256) k_cano_begin(CanoArgs A, const SysP *__re
             A.vback[3 * a + k] = vk;
             A.xold[3 * a + k] = A.x[3 * a + k];
             A.fcc[3 * a + k] = 0.0;
-            w[k] = vk - r * A.gslot[3 * sl + k] * im;
+            w[k] = vk - r * cano_grad(A, s, sl, k) * im;
         }
         val[0] = cano_mw2(A.mass[a], w[0], w[1], w[2]);
         gq[0] = A.gid[a];
@@ -196,7 +205,7 @@ __global__ void __launch_bounds__(256) k_cano_drift(CanoArgs A, const SysP *__re
     const size_t sl = (size_t)A.inum[a];
     double xa[3];
     for (int k = 0; k < 3; ++k) {
-        const double vk = rt2 * A.v[3 * a + k] - rt * A.gslot[3 * sl + k] * im;
+        const double vk = rt2 * A.v[3 * a + k] - rt * cano_grad(A, s, sl, k) * im;
         A.v[3 * a + k] = vk;
         xa[k] = A.x[3 * a + k] + dt * vk;
         A.x[3 * a + k] = xa[k];
@@ -249,7 +258,7 @@ __global__ void __launch_bounds__(128) k_cano_shake(CanoArgs A, const SysP *__re
             const int qq = 3 * b + k;
             ax[q][k] = A.x[qq];
             axo[q][k] = A.xold[qq];
-            ag[q][k] = A.gslot[3 * sl + k];
+            ag[q][k] = cano_grad(A, s, sl, k);
             afc[q][k] = A.fcc[qq];
             avb[q][k] = A.vback[qq];
             avx[q][k] = round == 1 ? A.v[qq] : A.xcb[qq];

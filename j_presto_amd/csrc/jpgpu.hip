@@ -20,6 +20,7 @@
 #include "jp_pair_half.hpp"
 #include "jp_bonded.hpp"
 #include "jp_cano.hpp"
+#include "jp_ge.hpp"
 
 #define CK(x)                                                          \
     do {                                                               \
@@ -272,6 +273,14 @@ struct Engine {
     int shake_failed(bool &failed);
     int cano_retry_step(int prec);
     int cano_steps_eager(int prec, int n);
+    // METHOD= GE (jp_set_mcmd): multicanonical weight of every step's forces
+    bool ge = false;
+    int ge_nw = 0, ge_nd = 0;
+    DBuf<double> d_gepar, d_gew;   // parameters; weight and class-1 energy of the step
+    DBuf<int> d_eslot;             // energy slot of every bonded energy term
+    int set_mcmd(int on, int nw, int nd, const double *c, const double *low, const double *high, double alphalw,
+                 double alphaup, double celw, double ceup, double tempce);
+    int ge_weight(int prec);
     // energy sums on the device: per-system atom and cluster ranges, results,
     // and a pinned host copy of them
     DBuf<int> d_abeg, d_aend, d_cbeg, d_cend;
@@ -327,6 +336,7 @@ struct Engine {
         d_torc.release(); d_tor.release(); d_p14.release(); d_p14p.release(); d_torp.release();
         d_abeg.release(); d_aend.release(); d_cbeg.release(); d_cend.release(); d_rsum.release(); d_rfix.release();
         d_part.release(); d_partf.release();
+        d_gepar.release(); d_gew.release(); d_eslot.release();
         if (h_rsum) hipHostFree(h_rsum);
         if (h_rfix) hipHostFree(h_rfix);
         if (h_err) hipHostFree(h_err);
@@ -1063,12 +1073,77 @@ int Engine::facc_zero() {
 }
 
 int Engine::forces(int prec, bool energy) {
-    int r = pair(prec, energy);
+    int r = pair(prec, energy || ge);
     if (!r) r = pair_out(prec);
-    if (r || !nbat) return r;
-    hipLaunchKernelGGL(k_bonded, grid1(nbat, 128), 128, 0, st, bonded_args(), d_atom_sys.p, d_sp.p, d_x.p, d_efac.p,
-                       d_inum.p, d_gslot.p, (double *)nullptr);
+    if (r) return r;
+    if (nbat) {
+        hipLaunchKernelGGL(k_bonded, grid1(nbat, 128), 128, 0, st, bonded_args(), d_atom_sys.p, d_sp.p, d_x.p,
+                           d_efac.p, d_inum.p, d_gslot.p, (double *)nullptr);
+        CK(hipGetLastError());
+    }
+    return ge ? ge_weight(prec) : 0;
+}
+
+// METHOD= GE: the step's class-1 potential energy from device sums (excess
+// energies per atom, pair energies per cluster -- the pair kernel ran in
+// energy mode --, bonded energies per term), then its multicanonical weight
+// into d_gew (jp_ge.hpp).  Launches only: capturable in a step graph.
+int Engine::ge_weight(int prec) {
+    int r;
+    hipLaunchKernelGGL(k_excess_energy, grid1(natom, 256), 256, 0, st, natom, d_atom_sys.p, d_sp.p, d_x.p, d_chg.p,
+                       d_eptr.p, d_eidx.p, d_eexc.p);
+    if ((r = seg_sum<double>(d_abeg.p, d_aend.p, 3, d_eexc.p, d_part.p, d_rsum.p + 6 * (size_t)nsys))) return r;
+    if (prec == JP_PREC_FP64) r = seg_sum<double>(d_cbeg.p, d_cend.p, 6, d_ep64.p, d_part.p, d_rsum.p);
+    else r = seg_sum<long long>(d_cbeg.p, d_cend.p, 6, d_epfix.p, d_partf.p, d_rfix.p);
+    if (r) return r;
+    const int ne = nbt + nat + ntt + 2 * n14;
+    if (ne) {
+        hipLaunchKernelGGL(k_bonded_energy, grid1(ne, 256), 256, 0, st, nbt, nat, ntt, n14, bonded_args(),
+                           d_atom_sys.p, d_sp.p, d_x.p, d_eterm.p);
+        CK(hipGetLastError());
+    }
+    GeArgs G;
+    G.nw = ge_nw; G.nd = ge_nd; G.par = d_gepar.p;
+    G.rsum = d_rsum.p;
+    G.rfix = prec == JP_PREC_FP64 ? nullptr : d_rfix.p;
+    G.nsys = nsys;
+    G.hf = (prec == JP_PREC_FP32_FIXED && list_half) ? 1.0 : 0.5;
+    G.dself = hs[0].dself[0];
+    G.eterm = d_eterm.p; G.eslot = d_eslot.p; G.ne = ne;
+    G.out = d_gew.p;
+    hipLaunchKernelGGL(k_ge_weight, 1, 256, 0, st, G);
     CK(hipGetLastError());
+    return 0;
+}
+
+// METHOD= GE for the following runs (with METHOD= CANO's temperature control
+// set by jp_set_canonical): one system, one energy class.
+int Engine::set_mcmd(int on, int nw, int nd, const double *c, const double *low, const double *high,
+                     double alphalw, double alphaup, double celw, double ceup, double tempce) {
+    drop_graphs();
+    ge = false;
+    if (!on) return 0;
+    if (nsys != 1 || hs[0].nfrg != 1 || hs[0].lmd) return JP_EINVAL;
+    if (nw < 1 || nd < 0 || nw > 4096 || nd > 64 || !c || !low || !high) return JP_EINVAL;
+    if (!(celw < ceup) || !std::isfinite(alphalw) || !std::isfinite(alphaup) || !std::isfinite(tempce))
+        return JP_EINVAL;
+    std::vector<double> par;
+    par.push_back(alphalw); par.push_back(alphaup); par.push_back(celw); par.push_back(ceup); par.push_back(tempce);
+    for (int w = 0; w < nw; ++w) par.push_back(low[w]);
+    for (int w = 0; w < nw; ++w) par.push_back(high[w]);
+    for (size_t q = 0; q < (size_t)nw * (nd + 1); ++q) par.push_back(c[q]);
+    for (double p : par)
+        if (!std::isfinite(p)) return JP_EINVAL;
+    AL(d_gepar, par.size());
+    AL(d_gew, 2);
+    const int ne = nbt + nat + ntt + 2 * n14;
+    AL(d_eslot, std::max(ne, 1));
+    CK(hipMemcpyAsync(d_gepar.p, par.data(), 8 * par.size(), hipMemcpyHostToDevice, st));
+    if (ne) CK(hipMemcpyAsync(d_eslot.p, eslot_h.data(), 4 * (size_t)ne, hipMemcpyHostToDevice, st));
+    CK(hipStreamSynchronize(st));
+    ge_nw = nw;
+    ge_nd = nd;
+    ge = true;
     return 0;
 }
 
@@ -1452,6 +1527,7 @@ CanoArgs Engine::cano_args() {
     A.ablk = d_ablk.p; A.ublk = d_ublk.p;
     A.shk_on = d_shkon.p; A.gdof = d_gdof.p; A.temps = d_ctemps.p;
     A.mass = d_mass.p; A.imass = d_imass.p; A.gslot = d_gslot.p;
+    A.gew = ge ? d_gew.p : nullptr;
     A.x = d_x.p; A.v = d_v.p; A.xold = d_xold.p; A.vback = d_vback.p; A.fcc = d_fcc.p; A.xcb = d_xcb.p;
     A.pa = d_pa.p; A.pu = d_pu.p; A.pn = d_pn.p; A.done = d_cdone.p;
     return A;
@@ -1578,6 +1654,7 @@ int Engine::restart_step(int prec, jp_energy_log *log) {
                            d_efac.p, d_inum.p, d_gslot.p, (double *)nullptr);
         CK(hipGetLastError());
     }
+    if (ge && (r = ge_weight(prec))) return r;
     std::vector<double> eel, evd, eex, et;
     if ((r = energies(prec, eel, evd, eex)) || (r = bonded_energies(et))) return r;
     if (retry && (r = save_state(d_xs.p, d_vs.p, d_errs.p))) return r;
@@ -1698,7 +1775,7 @@ int Engine::mon_record(int step, int nsteps, int nmon, int nlog, const std::vect
 // HIP graphs.
 int Engine::run_nve(int nsteps, int update, int nmon, int prec, jp_energy_log *log) {
     int r;
-    if (nsteps < 0 || update <= 0) return JP_EINVAL;
+    if (nsteps < 0 || update <= 0 || (ge && !cano)) return JP_EINVAL;
     if ((r = ensure_list(prec))) return r;
     if ((r = set_factors(-1))) return r;
     if ((r = refresh_sorted())) return r;
@@ -1789,7 +1866,7 @@ int Engine::run_nve(int nsteps, int update, int nmon, int prec, jp_energy_log *l
             pe_pending = false;
         }
         CK(hipEventRecord(ev[4], st));
-        if ((r = pair(prec, mon))) return r;
+        if ((r = pair(prec, mon || ge))) return r;
         CK(hipEventRecord(ev[5], st));
         pe_pending = true;
         if ((r = pair_out(prec))) return r;
@@ -1798,6 +1875,7 @@ int Engine::run_nve(int nsteps, int update, int nmon, int prec, jp_energy_log *l
                                d_efac.p, d_inum.p, d_gslot.p, (double *)nullptr);
             CK(hipGetLastError());
         }
+        if (ge && (r = ge_weight(prec))) return r;
         ++n_pair;
         std::vector<double> eel, evd, eex, et;
         if (mon && (r = energies(prec, eel, evd, eex))) return r;
@@ -2239,6 +2317,12 @@ int jp_set_option(jp_ctx *ctx, int32_t option, int32_t value) {
 int jp_batch_set_option(jp_batch *b, int32_t option, int32_t value) {
     if (!b) return JP_EINVAL;
     return set_option(b->e, option, value);
+}
+
+int jp_set_mcmd(jp_ctx *ctx, int32_t on, int32_t nwindow, int32_t ndeg, const double *c, const double *low,
+                const double *high, double alphalw, double alphaup, double celw, double ceup, double tempce) {
+    if (!ctx) return JP_EINVAL;
+    return ctx->e.set_mcmd(on, nwindow, ndeg, c, low, high, alphalw, alphaup, celw, ceup, tempce);
 }
 
 int jp_set_canonical(jp_ctx *ctx, int32_t on, double temps, int32_t icslop, double fcstol, const int32_t *group) {

@@ -180,6 +180,22 @@ class MdSystem:
     def set_option(self, option: int, value: int):
         native.check("jp_set_option", self.lib.jp_set_option(self.ctx, option, int(value)))
 
+    def set_mcmd(self, ge: Optional[dict]):
+        """METHOD= GE (multicanonical) on top of set_canonical: ge holds c
+        (nwindow, ndeg+1), low, high (nwindow), alphalw, alphaup, celw, ceup
+        and tempce (the NAMEGE= file); None switches it off."""
+        if ge is None:
+            native.check("jp_set_mcmd", self.lib.jp_set_mcmd(self.ctx, 0, 0, 0, None, None, None,
+                                                             0.0, 0.0, 0.0, 0.0, 0.0))
+            return
+        c = np.ascontiguousarray(ge["c"], np.float64)
+        low = np.ascontiguousarray(ge["low"], np.float64)
+        high = np.ascontiguousarray(ge["high"], np.float64)
+        self._ge = (c, low, high)
+        native.check("jp_set_mcmd", self.lib.jp_set_mcmd(
+            self.ctx, 1, c.shape[0], c.shape[1] - 1, _f64(c), _f64(low), _f64(high), float(ge["alphalw"]),
+            float(ge["alphaup"]), float(ge["celw"]), float(ge["ceup"]), float(ge["tempce"])))
+
     def set_canonical(self, temps: Optional[float] = 300.0, icslop: int = 3, fcstol: float = 1e-3,
                       tempco: bool = False):
         """METHOD= CANO for the following runmd calls (SETTEM = temps; TEMPCO=

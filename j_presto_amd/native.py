@@ -43,6 +43,8 @@ SIGNATURES = {
     "jp_batch_restart_step": (C.c_int, [vp, C.c_int32, C.POINTER(EnergyLog)]),
     "jp_time_pair": (C.c_int, [vp, C.c_int32, C.c_int32, f64p]),
     "jp_set_canonical": (C.c_int, [vp, C.c_int32, C.c_double, C.c_int32, C.c_double, i32p]),
+    "jp_set_mcmd": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_int32, f64p, f64p, f64p, C.c_double, C.c_double,
+                              C.c_double, C.c_double, C.c_double]),
     "jp_batch_set_canonical": (C.c_int, [vp, C.c_int32, C.c_double, C.c_int32, C.c_double, i32p]),
     "jp_batch_create": (C.c_int, [C.POINTER(SysDesc), C.c_int32, C.POINTER(vp)]),
     "jp_batch_run_nve": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.POINTER(EnergyLog)]),

@@ -76,6 +76,13 @@ def oracle_monitor(o, s, x, v, nsteps, every=5, cano=None, scaled=False):
     return np.array(rows)
 
 
+def ge_params(f, tag):
+    """The multicanonical parameters of a fixture's METHOD= GE run `tag`."""
+    celw, ceup = f[f"ge_{tag}_range"]
+    return dict(c=f["ge_poly"], low=f["ge_low"], high=f["ge_high"], alphalw=float(f["ge_alpha"][0]),
+                alphaup=float(f["ge_alpha"][1]), celw=float(celw), ceup=float(ceup), tempce=float(f["ge_tempce"]))
+
+
 def md_cell_windows(L, nv, rc=12.0):
     """md_run's periodic neighbour-cell table per axis (incel.f90:334-358,
     456-508), restated from the reference for the tests: {cell: set of cells

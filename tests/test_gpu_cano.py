@@ -8,7 +8,8 @@ launches and batch members equal to single runs bit for bit."""
 import numpy as np
 import pytest
 
-from j_presto_amd import engine, systems
+from conftest import ge_params
+from j_presto_amd import engine, native, systems
 from oracle import Oracle
 
 pytestmark = pytest.mark.gpu
@@ -210,3 +211,80 @@ def test_restart_step_keeps_the_state_without_a_retry(golden):
     g, ee, ev = o.nonbond(s.x, g)
     epot = et.sum() + ee.sum() + ev.sum()
     assert abs(rlog["epot"][0] - epot) < 1e-9 * max(1.0, abs(epot))
+
+
+@pytest.mark.parametrize("tag,graphs", [("win", True), ("win", False), ("lo", True), ("hi", True)])
+def test_ge_vs_reference(golden, tag, graphs):
+    """METHOD= GE (multicanonical, verlethe.f90:196-233), FP64: every step's
+    forces carry the weight of its potential energy, formed on the device
+    (window polynomials for "win", whose energy crosses all three windows;
+    alphalw for "lo", alphaup for "hi").  After md_run's restart step the
+    trajectory matches md_run's within 1e-9, with step graphs and without."""
+    s, f = golden("water_1500")
+    print(f"case ge {tag} graphs={graphs}", flush=True)
+    md = engine.MdSystem(s, precision=engine.FP64)
+    md.set_graphs(graphs)
+    md.set_state(s.x, s.v)
+    md.set_canonical(300.0, 3, 1e-3)
+    md.set_mcmd(ge_params(f, tag))
+    assert md.celprp() == 0
+    _, ier = md.restart_step()
+    assert ier == 0
+    _, ier = md.runmd(int(f[f"ge_{tag}_loop"]), update=10)
+    assert ier == 0
+    x, v = md.get_state()
+    md.close()
+    print(f"max |dx| {np.abs(x - f[f'ge_{tag}_x']).max():.2e}", flush=True)
+    assert np.abs(x - f[f"ge_{tag}_x"]).max() < 1e-9
+    assert np.abs(v - f[f"ge_{tag}_v"]).max() < 1e-9
+
+
+@pytest.mark.parametrize("prec,tol", [(engine.FP64, 1e-9), (engine.FP32, 1e-5)])
+def test_ge_with_bonded_terms_vs_oracle(prec, tol):
+    """METHOD= GE on a chain in water: the device energy includes the bonded
+    terms per slot.  FP64 within 1e-9 A of Oracle.run_cano with the same
+    parameters after 20 steps (energy window around the start), FP32 within
+    1e-5 A."""
+    s = systems.chain_in_water(48, 24.6, 31)
+    o = Oracle(s)
+    assert o.celprp(s.x) == 0
+    g, et = o.bonded(s.x)
+    g, ee, ev = o.nonbond(s.x, g)
+    e0 = et[0].sum() + ev[0] + ee[0]
+    ge = dict(c=np.array([[1.70, -0.3, 0.2], [1.65, 0.1, -0.1]]), low=np.array([0.0, 0.5]),
+              high=np.array([0.5, 1.0]), alphalw=1.66, alphaup=1.69, celw=e0 - 3000.0, ceup=e0 + 1000.0, tempce=300.0)
+    x, v = s.x.copy(), s.v.copy()
+    o = Oracle(s)
+    assert o.run_cano(x, v, 20, 10, 300.0, 3, 1e-3, ge=ge) == (0, 0)
+    print(f"case ge chain prec={prec}", flush=True)
+    md = engine.MdSystem(s, precision=prec)
+    md.set_canonical(300.0, 3, 1e-3)
+    md.set_mcmd(ge)
+    assert md.celprp() == 0
+    assert md.restart_step()[1] == 0
+    _, ier = md.runmd(20, update=10)
+    assert ier == 0
+    xg, vg = md.get_state()
+    md.close()
+    print(f"max |dx| {np.abs(xg - x).max():.2e}", flush=True)
+    assert np.abs(xg - x).max() < tol
+
+
+def test_ge_requires_one_class_and_canonical(golden):
+    """METHOD= GE takes one system with one energy class and runs only with
+    the canonical temperature control on."""
+    s, f = golden("water_1500")
+    ge = ge_params(f, "win")
+    md = engine.MdSystem(s, precision=engine.FP64)
+    md.set_mcmd(ge)
+    with pytest.raises(native.JpError):
+        md.runmd(2, update=10)
+    md.set_canonical(300.0, 3, 1e-3)
+    assert md.runmd(2, update=10)[1] == 0
+    md.set_mcmd(None)
+    md.close()
+    lmd = systems.chain_in_water(48, 24.6, 31, lmd=0.8)
+    md = engine.MdSystem(lmd, precision=engine.FP64)
+    with pytest.raises(native.JpError):
+        md.set_mcmd(ge)
+    md.close()

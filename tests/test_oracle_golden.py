@@ -4,7 +4,7 @@ sets and NVE trajectories."""
 import numpy as np
 import pytest
 
-from conftest import GOLDEN_NAMES, oracle_monitor
+from conftest import GOLDEN_NAMES, ge_params, oracle_monitor
 from oracle import Oracle
 
 
@@ -197,13 +197,6 @@ def test_cano_shake_retry_equals_reference(golden):
     ier, nprov = o.run_cano(x, v, int(f["retry_loop"]), 10, 300.0, 3, 1e-3, inirst=False)
     assert ier == 0 and nprov + 2 == int(f["retry_provisions"])
     assert np.abs(x - f["retry_x"]).max() > 1e-3
-
-
-def ge_params(f, tag):
-    """The multicanonical parameters of the fixture's METHOD= GE run `tag`."""
-    celw, ceup = f[f"ge_{tag}_range"]
-    return dict(c=f["ge_poly"], low=f["ge_low"], high=f["ge_high"], alphalw=float(f["ge_alpha"][0]),
-                alphaup=float(f["ge_alpha"][1]), celw=float(celw), ceup=float(ceup), tempce=float(f["ge_tempce"]))
 
 
 @pytest.mark.parametrize("tag", ["win", "lo", "hi"])
