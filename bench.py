@@ -102,7 +102,9 @@ def run_gpu(system, precision, steps, warmup, state=None, pair_steps=200, autotu
     # pair-kernel duration: HIP events around pair_steps launches back to back
     tp = md.time_pair(pair_steps)
     shapes = {"pair_waves": md.get_option(native.JP_OPT_PAIR_WAVES),
-              "list_waves": md.get_option(native.JP_OPT_LIST_WAVES), "autotuned": bool(autotune)}
+              "list_waves": md.get_option(native.JP_OPT_LIST_WAVES),
+              "list_batches": md.get_option(native.JP_OPT_LIST_BATCH),
+              "grid_shift": md.get_option(native.JP_OPT_GRID_SHIFT), "autotuned": bool(autotune)}
     md.close()
     return dict(wall=t1 - t0, timing=tm, pairs=n, pairs_lj=nlj, listed=nlisted, log=log, t_pair=tp, shapes=shapes,
                 graph_captures_timed=captures)
@@ -275,7 +277,9 @@ def run_ensemble(steps: int, warmup: int, equil: int):
     _, ier = ens.runmd(0, update=UPDATE)
     assert ier == 0, ier
     shapes = {"pair_waves": ens.get_option(native.JP_OPT_PAIR_WAVES),
-              "list_waves": ens.get_option(native.JP_OPT_LIST_WAVES), "autotuned": True}
+              "list_waves": ens.get_option(native.JP_OPT_LIST_WAVES),
+              "list_batches": ens.get_option(native.JP_OPT_LIST_BATCH),
+              "grid_shift": ens.get_option(native.JP_OPT_GRID_SHIFT), "autotuned": True}
     _, ier = ens.runmd(warmup, update=UPDATE)
     assert ier == 0, ier
     t0 = time.perf_counter()

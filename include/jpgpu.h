@@ -299,10 +299,14 @@ int jp_batch_destroy(jp_batch *b);
  *                      energy steps of jp_run_nve / jp_batch_run_nve run as
  *                      captured HIP graphs; 0: every kernel launched from the
  *                      host.  Results are identical either way.
- *   JP_OPT_AUTOTUNE    1: the next run times the list kernel at 1/2/4/8
- *                      wavefronts per block and the FP32 pair kernel at 2/4/8
- *                      wavefronts per i-cluster and keeps the fastest; 0
- *                      (default): fixed shapes.
+ *   JP_OPT_AUTOTUNE    1: the next run tunes the box-grid cell size (the
+ *                      4.5-9 A grid and the grids with cells half and twice as
+ *                      wide), the list kernel's wavefronts per block (1/2/4/8)
+ *                      and tile width (1/2/4 batches of 8 j-clusters per pass),
+ *                      and the FP32 pair kernel's wavefronts per i-cluster
+ *                      (2/4/8), timing each on the live list, and keeps the
+ *                      grid with the least rebuild time / update + pair time
+ *                      with its fastest shapes; 0 (default): fixed choices.
  *   JP_OPT_HALF_LIST   1: the FP32/fixed-point path runs on a half list (each unique
  *                      pair once, its force applied to both atoms, the j side as
  *                      one 64-bit fixed-point atomic add per entry and component);
@@ -324,6 +328,14 @@ int jp_batch_destroy(jp_batch *b);
  *                      tolerance across values.
  *   JP_OPT_LIST_WAVES  i-clusters per block in the list kernel (1, 2, 4 = default, 8);
  *                      the list does not depend on it.
+ *   JP_OPT_LIST_BATCH  batches of 8 j-clusters the list kernel tests per pass
+ *                      (1, 2 = default, 4); the list does not depend on it.
+ *   JP_OPT_GRID_SHIFT  box-grid cell size: 0 (default) cells of 4.5-9 A, k > 0
+ *                      2^k times narrower, k < 0 2^-k times wider (-3..3; at most
+ *                      512 and at least 2 cells per edge).  The pair set does not
+ *                      depend on it; the slot order does, so it regroups partial
+ *                      sums like JP_OPT_PAIR_WAVES.  The next evaluation rebuilds
+ *                      the list.
  *   JP_OPT_LIST_SEGMENT  list entries per i-cluster segment at the next rebuild
  *                      only (a multiple of 64; set, not read); the library grows
  *                      segments by itself and rebuilds when one overflows, so this
@@ -350,6 +362,8 @@ int jp_batch_destroy(jp_batch *b);
 #define JP_OPT_CELL_TABLE 6
 #define JP_OPT_GRAPH_CAPTURES 7   /* read only: step graphs captured during the last run */
 #define JP_OPT_HALF_LIST 8
+#define JP_OPT_LIST_BATCH 9
+#define JP_OPT_GRID_SHIFT 10
 int jp_set_option(jp_ctx *ctx, int32_t option, int32_t value);
 int jp_batch_set_option(jp_batch *b, int32_t option, int32_t value);
 int jp_get_option(jp_ctx *ctx, int32_t option, int32_t *value);

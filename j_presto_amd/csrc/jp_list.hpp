@@ -38,6 +38,7 @@
 #define JP_CL 8          // atoms per i-cluster
 #define JP_LW 4          // default i-clusters (wavefronts) per block of k_list8 (1, 2, 4 or 8; autotuned)
 #define JP_QCAP 128      // per-wave queue of candidate j-clusters (power of 2)
+#define JP_LNB 2         // default batches of 8 j-clusters drained per pass of k_list8 (1, 2 or 4; autotuned)
 
 __host__ __device__ __forceinline__ int round64(int n) { return (n + 63) & ~63; }
 // list status words lst[]: [0] error bits, [1] longest list above the high-water
@@ -240,14 +241,16 @@ __device__ __forceinline__ bool exact_in(const SysP &P, const double4 &a, const 
 
 // One wavefront per i-cluster.  Candidate j-clusters go through a queue in
 // LDS; every 8 of them are tested as one batch (lane = one j atom against the
-// 8 i atoms).  Per i atom the listed count (nbcount) and per system the number
+// 8 i atoms), NB batches per pass with all of their loads issued before the
+// first test (the tile width: more batches keep more loads in flight and use
+// more registers).  Per i atom the listed count (nbcount) and per system the number
 // of excluded pairs met within rlim (nremoved) are reported for md_run's
 // checks.
 // HALF: the half list -- only j-clusters after the i-cluster, and inside
 // the i-cluster only the pairs (ia < ja): each unique pair once.  Per-atom
 // counts then go to icnt (own entries) and jcnt (as a j atom) for k_nbcheck.
-template <int LW, bool HALF>
-__global__ void __launch_bounds__(64 * LW) __attribute__((amdgpu_waves_per_eu(6)))
+template <int LW, bool HALF, int NB>
+__global__ void __launch_bounds__(64 * LW) __attribute__((amdgpu_waves_per_eu(NB > 2 ? 4 : 6)))
 k_list8(int nclu, const int *__restrict__ clu_sys, const SysP *__restrict__ sp, const double *__restrict__ cbb,
         const float4 *__restrict__ ccf, const int4 *__restrict__ crec, const unsigned *__restrict__ sysext,
         const float4 *__restrict__ lp, const double4 *__restrict__ xr, const int4 *__restrict__ imgs,
@@ -486,37 +489,46 @@ k_list8(int nclu, const int *__restrict__ clu_sys, const SysP *__restrict__ sp, 
             ovf = true;
         }
     };
-    // test j-clusters s_q[qh .. qh+nq) (nq <= 16) as two batches of 8, both
-    // loaded before either is tested (unconditional loads: slot 0 for lanes
+    // test j-clusters s_q[qh .. qh+nq) (nq <= 8 * NB) as NB batches of 8, all
+    // loaded before the first is tested (unconditional loads: slot 0 for lanes
     // without an entry, so that no wait lands where they are issued)
     auto drain = [&](int nq) {
-        const bool qv0 = jsub < nq, qv1 = 8 + jsub < nq;
-        const uint32_t qe0 = qv0 ? s_q[w][(qh + jsub) & (JP_QCAP - 1)] : 0u;
-        const uint32_t qe1 = qv1 ? s_q[w][(qh + 8 + jsub) & (JP_QCAP - 1)] : 0u;
-        const int j0 = qv0 ? (int)(qe0 & JP_SLOT_MASK) * JP_CL + ja : 0;
-        const int j1 = qv1 ? (int)(qe1 & JP_SLOT_MASK) * JP_CL + ja : 0;
-        const float4 q0 = lp[j0];
-        const float4 q1 = lp[j1];
-        // md_run cells of the j atoms, only where its cell table drops pairs
-        const int c0 = ctab ? mcell[j0] : 0, c1 = ctab ? mcell[j1] : 0;
-        test(qv0, qe0, q0, c0);
-        if (nq > 8) test(qv1, qe1, q1, c1);
+        bool qv[NB];
+        uint32_t qe[NB];
+        float4 q[NB];
+        int cj[NB];
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            qv[b] = 8 * b + jsub < nq;
+            qe[b] = qv[b] ? s_q[w][(qh + 8 * b + jsub) & (JP_QCAP - 1)] : 0u;
+        }
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            const int j = qv[b] ? (int)(qe[b] & JP_SLOT_MASK) * JP_CL + ja : 0;
+            q[b] = lp[j];
+            // md_run cells of the j atoms, only where its cell table drops pairs
+            cj[b] = ctab ? mcell[j] : 0;
+        }
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+            if (b == 0 || nq > 8 * b) test(qv[b], qe[b], q[b], cj[b]);
     };
 
-    // One loop, one site per phase, highest priority first: drain 16 queued
-    // j-clusters; box-test 64 candidates (independent loads); append one
-    // round of the current cell batch's j-clusters (LDS only); test the next
-    // 64 candidate cells.  Bounds: the queue holds <= 15 + 64 and the
-    // candidate ring <= 63 + 64 entries when written, both < JP_QCAP.
+    // One loop, one site per phase, highest priority first: drain 8 * NB
+    // queued j-clusters; box-test 64 candidates (independent loads); append
+    // one round of the current cell batch's j-clusters (LDS only); test the
+    // next 64 candidate cells.  Bounds: the queue holds <= 8 * NB - 1 + 64 and
+    // the candidate ring <= 63 + 64 entries when written, both < JP_QCAP.
+    static_assert(8 * NB - 1 + 64 < JP_QCAP, "queue bound");
     int ch = 0, ct = 0, qb = -64, rnd = 0, k0 = 0, k1 = 0;
     uint32_t ccode = 13;
     bool cells_done = false;
     for (;;) {
         const int nq = qt - qh, nc = ct - ch;
-        if (nq >= 16 || (cells_done && nc == 0 && nq > 0)) {
+        if (nq >= 8 * NB || (cells_done && nc == 0 && nq > 0)) {
             wave_lds_sync();
-            drain(min(nq, 16));
-            qh += min(nq, 16);
+            drain(min(nq, 8 * NB));
+            qh += min(nq, 8 * NB);
             wave_lds_sync();
             continue;
         }

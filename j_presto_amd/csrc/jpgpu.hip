@@ -409,9 +409,13 @@ struct Engine {
     int use_graphs = 1;
     // launch shapes: wavefronts per i-cluster (pair kernels) and per block
     // (k_list8); autotune = 1 times the candidates at the next jp_run_nve
-    int pw = JP_PW, lw = JP_LW, autotune = 0, tuned = 0;
+    int pw = JP_PW, lw = JP_LW, lnb = JP_LNB, autotune = 0, tuned = 0;
+    int grid_shift = 0;             // JP_OPT_GRID_SHIFT: box-grid cells 2^shift times finer than 4.5-9 A
+    int grid_tables();
+    int regrid(int shift);
+    float last_build_ms = 0.f;      // device time of the last list rebuild
     int pwh = JP_PWH;               // wavefronts per i-cluster of the half-list force kernel
-    int tune(int prec);
+    int tune(int prec, int update);
     StepGraph *step_graph(int n, int prec);
     void drop_graphs();
     int get_list(int32_t *cell3, int64_t *npairs, int32_t *ptr, int32_t *idx, int64_t cap);
@@ -421,6 +425,22 @@ struct Engine {
 };
 
 static void launch_list8(Engine &g, int hiwater);
+
+// Box grid of one system: 2^hbits cells per edge, box-aligned, binned in
+// Hilbert order.  shift 0 gives cells of 4.5-9 A; each unit of shift halves
+// (positive) or doubles (negative) the cell edge, within 1 <= hbits <= 9 (the
+// 2x2x2 sub-cell keys of k_bin hold 3 * (hbits + 1) bits).
+static void set_grid(SysP &P, int shift) {
+    const double lmin = std::min(P.L[0], std::min(P.L[1], P.L[2]));
+    const int hb0 = std::min(9, std::max(1, (int)std::floor(std::log2(lmin / 4.5))));
+    const int hb = std::min(9, std::max(1, hb0 + shift));
+    P.hbits = hb;
+    for (int k = 0; k < 3; ++k) {
+        P.nc[k] = 1 << hb;
+        P.csz[k] = P.L[k] / P.nc[k];
+        P.icsz[k] = P.nc[k] / P.L[k];
+    }
+}
 
 int Engine::init(const jp_sys_desc *descs, int ns) {
     if (ns <= 0) return JP_EINVAL;
@@ -441,7 +461,6 @@ int Engine::init(const jp_sys_desc *descs, int ns) {
     std::vector<int> s2sys, s3sys;
     std::vector<int4> s2a, s3a;
     std::vector<double> s2d, s3d;
-    std::vector<int> crank, ccoord;
     std::vector<int4> bnd, ang, tor, p14;
     std::vector<double2> bndp, angp, p14p;
     std::vector<double4> torp;
@@ -449,7 +468,7 @@ int Engine::init(const jp_sys_desc *descs, int ns) {
     std::vector<std::vector<int>> bref_of;   // per global atom: term codes in kind order
     // energy-entry metadata per kind (system, class, slot), concatenated after the loop
     std::vector<int> em_sys[4], em_cls[4], em_slot[4];
-    int atom0 = 0, slot0 = 0, cell0 = 0, lj0 = 0;
+    int atom0 = 0, slot0 = 0, lj0 = 0;
     shake_tol = descs[0].shake_tol > 0 ? descs[0].shake_tol : 1e-6;
     shake_maxit = descs[0].shake_maxit > 0 ? descs[0].shake_maxit : 1000;
     for (int s = 0; s < ns; ++s) {
@@ -469,17 +488,7 @@ int Engine::init(const jp_sys_desc *descs, int ns) {
             P.wall[k] = d.celwal[2 * k];
             if (!(P.L[k] >= 2.0 * rc)) return JP_EINVAL;  // inpmd.f90:694-698
         }
-        // power-of-2 box-aligned grid, cells of 4.5-9 A, in Hilbert order
-        {
-            double lmin = std::min(P.L[0], std::min(P.L[1], P.L[2]));
-            int hb = std::min(10, std::max(1, (int)std::floor(std::log2(lmin / 4.5))));
-            P.hbits = hb;
-            for (int k = 0; k < 3; ++k) {
-                P.nc[k] = 1 << hb;
-                P.csz[k] = P.L[k] / P.nc[k];
-                P.icsz[k] = P.nc[k] / P.L[k];
-            }
-        }
+        set_grid(P, grid_shift);
         // md_run's cell numbering (incel.f90:29-57, mk_c, celpar.f90:93-108)
         {
             int nlev = d.cell_level > 0 ? d.cell_level : 3;
@@ -524,7 +533,6 @@ int Engine::init(const jp_sys_desc *descs, int ns) {
         P.atom0 = atom0; P.natom = n;
         int npad = (n + 63) / 64 * 64;
         P.slot0 = slot0; P.nslot = npad;
-        P.cell0 = cell0; P.ncell = P.nc[0] * P.nc[1] * P.nc[2];
         P.lj0 = lj0; P.ntyp = nt;
         h.n15mx = (int)std::ceil(0.15 * (4.0 / 3.0 * 3.14159265358979 * rc * rc * rc)) * 2;
         P.nblim = h.n15mx;
@@ -535,19 +543,6 @@ int Engine::init(const jp_sys_desc *descs, int ns) {
                 P.cls[a * 2 + b] = d.n_matrix[bb * d.nfrag + aa] - 1;
                 P.fac[a * 2 + b] = 1.0;
             }
-        // Hilbert rank of every box cell (the same curve k_bin bins on)
-        {
-            int nx = P.nc[0], ny = P.nc[1], nz = P.nc[2];
-            ccoord.resize((size_t)cell0 + P.ncell);
-            for (int cz = 0; cz < nz; ++cz)
-                for (int cy = 0; cy < ny; ++cy)
-                    for (int cx = 0; cx < nx; ++cx) {
-                        const int h = (int)hilbert3(cx, cy, cz, P.hbits);
-                        crank.push_back(cell0 + h);
-                        ccoord[(size_t)cell0 + h] = cx | (cy << 10) | (cz << 20);
-                    }
-            for (int r = 0; r < P.ncell; ++r) cell_sys_h.push_back(s);
-        }
         // LJ table, AMBER combination (inptpl.f90:397-419)
         std::vector<double> c6((size_t)nt * nt), c12((size_t)nt * nt);
         for (int i = 0; i < nt; ++i)
@@ -724,10 +719,9 @@ int Engine::init(const jp_sys_desc *descs, int ns) {
         for (int i = 0; i < npad / JP_CL; ++i) clu_sys_h.push_back(s);
         atom0 += n;
         slot0 += npad;
-        cell0 += P.ncell;
         lj0 += nt * nt;
     }
-    natom = atom0; nslot = slot0; nclu = slot0 / JP_CL; ncell = cell0;
+    natom = atom0; nslot = slot0; nclu = slot0 / JP_CL;
     // bonded gather lists: atoms with terms and their codes; energy entries in
     // k_bonded_energy order (bonds, angles, torsions, 1-4 pairs x 2)
     nbt = (int)bnd.size(); nat = (int)ang.size(); ntt = (int)tor.size(); n14 = (int)p14.size();
@@ -775,7 +769,7 @@ int Engine::init(const jp_sys_desc *descs, int ns) {
     set_factors(-1);
     // upload
     AL(d_sp, ns); AL(d_atom_sys, natom); AL(d_slot_sys, nslot);
-    AL(d_cell_sys, ncell); AL(d_crank, ncell); AL(d_ccoord, ncell); AL(d_clu_sys, nclu);
+    AL(d_clu_sys, nclu);
     AL(d_x, 3 * (size_t)natom); AL(d_v, 3 * (size_t)natom); AL(d_xold, 3 * (size_t)natom);
     AL(d_vback, 3 * (size_t)natom); AL(d_mass, natom); AL(d_imass, natom); AL(d_chg, natom);
     AL(d_grad, 3 * (size_t)natom); AL(d_eexc, 3 * (size_t)natom); AL(d_ekin, natom);
@@ -783,14 +777,14 @@ int Engine::init(const jp_sys_desc *descs, int ns) {
     AL(d_eptr, natom + 1); AL(d_eidx, eidx.size()); AL(d_efac, 3 * (size_t)ns);
     AL(d_lj64, lj64.size()); AL(d_lj32, lj32.size());
     AL(d_img, 3 * (size_t)natom); AL(d_cellid, natom); AL(d_subkey, natom); AL(d_refcell, 3 * (size_t)natom);
-    AL(d_cnt, ncell + 1); AL(d_start, ncell + 1); AL(d_fill, ncell); AL(d_tmp, natom);
-    AL(d_inum, natom); AL(d_perm, nslot); AL(d_cslot, ncell);
+    AL(d_tmp, natom);
+    AL(d_inum, natom); AL(d_perm, nslot);
     AL(d_ys, nslot); AL(d_qd, nslot); AL(d_xr, nslot); AL(d_imgs, nslot); AL(d_lp, nslot); AL(d_mcell, nslot);
     AL(d_facc, 4 * (size_t)nslot); AL(d_icnt, nslot); AL(d_jcnt, nslot);
     AL(d_mcnt, nmcnt); AL(d_nbflag, 1);
     AL(d_wmask, wmask_h.size());
-    AL(d_cbb, 6 * (size_t)nclu); AL(d_ccen, nclu); AL(d_ccf, nclu); AL(d_lcnt, nclu); AL(d_cellext, ncell);
-    AL(d_sysext, ns); AL(d_crec, ncell); AL(d_err, 1); AL(d_lst, 2 + (size_t)ns);
+    AL(d_cbb, 6 * (size_t)nclu); AL(d_ccen, nclu); AL(d_ccf, nclu); AL(d_lcnt, nclu);
+    AL(d_sysext, ns); AL(d_err, 1); AL(d_lst, 2 + (size_t)ns);
     AL(d_gslot, 3 * (size_t)nslot);
     AL(d_ep64, 6 * (size_t)nclu); AL(d_epfix, 6 * (size_t)nclu);
     AL(d_s2sys, n2); AL(d_s3sys, n3); AL(d_s2a, n2); AL(d_s3a, n3); AL(d_s2d, 6 * (size_t)n2); AL(d_s3d, 6 * (size_t)n3);
@@ -818,12 +812,10 @@ int Engine::init(const jp_sys_desc *descs, int ns) {
         return 0;
     };
     int r = 0;
+    r |= grid_tables();
     r |= upload_sp();
     r |= H2D(d_atom_sys.p, atom_sys_h.data(), 4 * atom_sys_h.size());
     r |= H2D(d_slot_sys.p, slot_sys_h.data(), 4 * slot_sys_h.size());
-    r |= H2D(d_cell_sys.p, cell_sys_h.data(), 4 * cell_sys_h.size());
-    r |= H2D(d_crank.p, crank.data(), 4 * crank.size());
-    r |= H2D(d_ccoord.p, ccoord.data(), 4 * ccoord.size());
     r |= H2D(d_clu_sys.p, clu_sys_h.data(), 4 * clu_sys_h.size());
     r |= H2D(d_x.p, x.data(), 8 * x.size());
     r |= H2D(d_mass.p, mass.data(), 8 * mass.size());
@@ -890,6 +882,55 @@ int Engine::set_factors(int onehot) {
     return 0;
 }
 
+// Cell ranges of the systems' box grids and the per-cell tables: Hilbert
+// rank -> cell, cell -> grid coordinates, cell -> system (the same curve
+// k_bin bins on).  Allocates the per-cell device arrays.
+int Engine::grid_tables() {
+    std::vector<int> crank, ccoord, csys;
+    int cell0 = 0;
+    for (int s = 0; s < nsys; ++s) {
+        SysP &P = sp[s];
+        P.cell0 = cell0;
+        P.ncell = P.nc[0] * P.nc[1] * P.nc[2];
+        ccoord.resize((size_t)cell0 + P.ncell);
+        for (int cz = 0; cz < P.nc[2]; ++cz)
+            for (int cy = 0; cy < P.nc[1]; ++cy)
+                for (int cx = 0; cx < P.nc[0]; ++cx) {
+                    const int h = (int)hilbert3(cx, cy, cz, P.hbits);
+                    crank.push_back(cell0 + h);
+                    ccoord[(size_t)cell0 + h] = cx | (cy << 10) | (cz << 20);
+                }
+        csys.insert(csys.end(), (size_t)P.ncell, s);
+        cell0 += P.ncell;
+    }
+    ncell = cell0;
+    AL(d_cell_sys, ncell); AL(d_crank, ncell); AL(d_ccoord, ncell);
+    AL(d_cnt, ncell + 1); AL(d_start, ncell + 1); AL(d_fill, ncell); AL(d_cslot, ncell);
+    AL(d_cellext, ncell); AL(d_crec, ncell);
+    CK(hipMemcpyAsync(d_cell_sys.p, csys.data(), 4 * csys.size(), hipMemcpyHostToDevice, st));
+    CK(hipMemcpyAsync(d_crank.p, crank.data(), 4 * crank.size(), hipMemcpyHostToDevice, st));
+    CK(hipMemcpyAsync(d_ccoord.p, ccoord.data(), 4 * ccoord.size(), hipMemcpyHostToDevice, st));
+    // the host vectors go out of scope: the copies must be done
+    CK(hipStreamSynchronize(st));
+    cell_sys_h.swap(csys);
+    return 0;
+}
+
+// A different box grid (JP_OPT_GRID_SHIFT, the autotuner): new cell tables,
+// and the next evaluation rebuilds the list on it.  The pair set is the same
+// on every grid; the slot order, and with it the grouping of partial sums,
+// follows the grid.
+int Engine::regrid(int shift) {
+    for (int s = 0; s < nsys; ++s) set_grid(sp[s], shift);
+    grid_shift = shift;
+    int r;
+    if ((r = grid_tables())) return r;
+    if ((r = upload_sp())) return r;
+    drop_graphs();
+    have_list = false;
+    return 0;
+}
+
 // celprp: bin and sort the atoms, then build the j-lists into segments of
 // ccap entries per i-cluster.  One status read-back per rebuild.
 int Engine::build_list(bool defer) {
@@ -951,6 +992,7 @@ int Engine::finish_list(bool retry) {
     }
     float ms = 0;
     hipEventElapsedTime(&ms, ev[2], ev[3]);
+    last_build_ms = ms;
     t_list += ms * 1e-3;
     n_list++;
     have_list = true;
@@ -1246,9 +1288,9 @@ static void launch_pair(int prec, unsigned nb, size_t lds32, size_t lds64, hipSt
     }
 }
 
-template <int LW, bool HALF>
+template <int LW, bool HALF, int NB>
 static void launch_list8_lw(Engine &g, int hiwater) {
-    hipLaunchKernelGGL((k_list8<LW, HALF>), grid1(g.nclu, LW), 64 * LW, 0, g.st, g.nclu, g.d_clu_sys.p, g.d_sp.p,
+    hipLaunchKernelGGL((k_list8<LW, HALF, NB>), grid1(g.nclu, LW), 64 * LW, 0, g.st, g.nclu, g.d_clu_sys.p, g.d_sp.p,
                        g.d_cbb.p, g.d_ccf.p, g.d_crec.p, g.d_sysext.p, g.d_lp.p, g.d_xr.p, g.d_imgs.p, g.d_xptr.p,
                        g.d_xidx.p, g.d_mcell.p, g.cell_table ? g.d_wmask.p : nullptr, g.ccap, hiwater, g.d_ent.p,
                        reinterpret_cast<uint8_t *>(g.d_mb.p), g.d_lcnt.p, g.d_lst.p, g.d_icnt.p, g.d_jcnt.p);
@@ -1257,35 +1299,45 @@ static void launch_list8_lw(Engine &g, int hiwater) {
                            g.d_jcnt.p, g.d_lst.p);
 }
 
-template <bool HALF>
+template <bool HALF, int NB>
 static void launch_list8_h(Engine &g, int hiwater) {
     switch (g.lw) {
-        case 1: launch_list8_lw<1, HALF>(g, hiwater); break;
-        case 2: launch_list8_lw<2, HALF>(g, hiwater); break;
-        case 8: launch_list8_lw<8, HALF>(g, hiwater); break;
-        default: launch_list8_lw<4, HALF>(g, hiwater); break;
+        case 1: launch_list8_lw<1, HALF, NB>(g, hiwater); break;
+        case 2: launch_list8_lw<2, HALF, NB>(g, hiwater); break;
+        case 8: launch_list8_lw<8, HALF, NB>(g, hiwater); break;
+        default: launch_list8_lw<4, HALF, NB>(g, hiwater); break;
     }
 }
 
+// The half list keeps the default batch count.
 static void launch_list8(Engine &g, int hiwater) {
-    if (g.list_half) launch_list8_h<true>(g, hiwater);
-    else launch_list8_h<false>(g, hiwater);
+    if (g.list_half) launch_list8_h<true, JP_LNB>(g, hiwater);
+    else if (g.lnb == 1) launch_list8_h<false, 1>(g, hiwater);
+    else if (g.lnb == 4) launch_list8_h<false, 4>(g, hiwater);
+    else launch_list8_h<false, JP_LNB>(g, hiwater);
 }
 
-// Autotuner: times k_list8 at 1, 2, 4 and 8 wavefronts per block and the
-// FP32 force-step pair kernel at 2, 4 and 8 wavefronts per i-cluster on the
-// current list (median of 7 launches each, candidates interleaved so that
-// clock drift hits all of them alike) and keeps the fastest.  The list
-// is rebuilt afterwards (the timing launches add to its status counters).
-// The list does not depend on lw; pw regroups the FP32 partial sums
-// (results stay deterministic for a given pw, within the FP32 tolerance
-// across pw values).
-int Engine::tune(int prec) {
+// Autotuner (where SIMD_check.f90:2-42 doubles i_vec while the pair routine
+// gets faster).  Per box-grid cell size (the 4.5-9 A grid, and grids with
+// cells half and twice as wide): the list kernel at 1, 2, 4 and 8
+// wavefronts per block and 1, 2 and 4 batches of 8 j-clusters per pass (the
+// tile width), the FP32 force-step pair kernel at 2, 4 and 8 wavefronts per
+// i-cluster, each on the live list (median of 7 launches, candidates
+// interleaved so that clock drift hits all of them alike); then whole
+// rebuilds at the best list shape.  The grid with the least rebuild time /
+// update + pair time wins, with its shapes, and the list is rebuilt on it.
+// The pair set is the same on every grid and shape; the grid and pw regroup
+// the partial sums (deterministic for given choices, within the FP32 bar
+// across them, and FP64 sums within round-off).
+int Engine::tune(int prec, int update) {
     int r;
-    if ((r = ensure_list(prec))) return r;
     hipEvent_t a, b;
     CK(hipEventCreate(&a));
     CK(hipEventCreate(&b));
+    struct EvGuard {
+        hipEvent_t &a, &b;
+        ~EvGuard() { hipEventDestroy(a); hipEventDestroy(b); }
+    } guard{a, b};
     constexpr int NREP = 7;
     auto once = [&](auto launch) -> float {
         float ms = 0.f;
@@ -1296,27 +1348,76 @@ int Engine::tune(int prec) {
         hipEventElapsedTime(&ms, a, b);
         return ms;
     };
-    auto pick = [&](const std::vector<int> &cand, int &knob, auto launch) {
+    auto median = [](std::vector<float> t) {
+        std::sort(t.begin(), t.end());
+        return t[t.size() / 2];
+    };
+    // fastest candidate of a set of knob settings; returns its median time
+    auto pick = [&](const std::vector<std::pair<int, int>> &cand, int &k1, int &k2, auto launch) -> float {
         std::vector<std::vector<float>> t(cand.size());
         for (int rep = 0; rep < NREP; ++rep)
             for (size_t q = 0; q < cand.size(); ++q) {
-                knob = cand[q];
+                k1 = cand[q].first;
+                k2 = cand[q].second;
                 t[q].push_back(once(launch));
             }
         float best = 1e30f;
-        int bk = cand[0];
+        size_t bq = 0;
         for (size_t q = 0; q < cand.size(); ++q) {
-            std::sort(t[q].begin(), t[q].end());
-            if (t[q][NREP / 2] < best) { best = t[q][NREP / 2]; bk = cand[q]; }
+            const float m = median(t[q]);
+            if (m < best) { best = m; bq = q; }
         }
-        knob = bk;
+        k1 = cand[bq].first;
+        k2 = cand[bq].second;
+        return best;
     };
-    pick({1, 2, 4, 8}, lw, [&] { launch_list8(*this, ccap - ccap / 8); });
-    if (prec == JP_PREC_FP64) pw = JP_PW;
-    else if (list_half) pick({1, 2, 4, 8}, pwh, [&] { pair(prec, false); });
-    else pick({2, 4, 8}, pw, [&] { pair(prec, false); });
-    hipEventDestroy(a);
-    hipEventDestroy(b);
+    int dummy = 0;
+    std::vector<std::pair<int, int>> lcand, pcand;
+    for (int nb : {1, 2, 4})
+        for (int w : {1, 2, 4, 8}) lcand.push_back({w, nb});
+    if (prec == JP_PREC_FP64) pcand = {{JP_PW, 0}};
+    else if (list_half) pcand = {{1, 0}, {2, 0}, {4, 0}, {8, 0}};
+    else pcand = {{2, 0}, {4, 0}, {8, 0}};
+    const int shift0 = grid_shift;
+    struct Choice { int shift, lw, lnb, pw; float score; };
+    Choice best{shift0, lw, lnb, pw, 1e30f};
+    std::vector<std::vector<int>> seen;
+    for (int dsh : {0, -1, 1}) {
+        const int shift = shift0 + dsh;
+        // skip a shift that gives every system a grid already timed
+        std::vector<int> hb;
+        for (int s = 0; s < nsys; ++s) {
+            SysP q = sp[s];
+            set_grid(q, shift);
+            hb.push_back(q.hbits);
+        }
+        if (std::find(seen.begin(), seen.end(), hb) != seen.end()) continue;
+        seen.push_back(hb);
+        if (shift != grid_shift && (r = regrid(shift))) return r;
+        if ((r = ensure_list(prec))) return r;
+        pick(lcand, lw, lnb, [&] { launch_list8(*this, ccap - ccap / 8); });
+        int &pk = list_half ? pwh : pw;
+        float tp;
+        if (prec == JP_PREC_FP64) {
+            std::vector<float> t;
+            for (int rep = 0; rep < NREP; ++rep) t.push_back(once([&] { pair(prec, false); }));
+            tp = median(t);
+        } else {
+            tp = pick(pcand, pk, dummy, [&] { pair(prec, false); });
+        }
+        // whole rebuilds at this shape (set-up kernels, list, status read-back)
+        std::vector<float> tb;
+        for (int rep = 0; rep < 3; ++rep) {
+            if ((r = build_list())) return r;
+            tb.push_back(last_build_ms);
+        }
+        const float score = median(tb) / (float)std::max(1, update) + tp;
+        if (score < best.score) best = Choice{shift, lw, lnb, list_half ? pwh : pw, score};
+    }
+    if (best.shift != grid_shift && (r = regrid(best.shift))) return r;
+    lw = best.lw;
+    lnb = best.lnb;
+    if (prec != JP_PREC_FP64) (list_half ? pwh : pw) = best.pw;
     CK(hipGetLastError());
     int r2 = facc_zero();
     if (r2) return r2;
@@ -1779,7 +1880,7 @@ int Engine::run_nve(int nsteps, int update, int nmon, int prec, jp_energy_log *l
     if ((r = ensure_list(prec))) return r;
     if ((r = set_factors(-1))) return r;
     if ((r = refresh_sorted())) return r;
-    if (autotune && !tuned && (r = tune(prec))) return r;
+    if (autotune && !tuned && (r = tune(prec, update))) return r;
     CK(hipMemsetAsync(d_err.p, 0, 4, st));
     t_pair = 0; t_step = 0; n_pair = 0; n_pair_steps = nsteps;
     pe_pending = false;
@@ -2280,6 +2381,15 @@ static int set_option(Engine &E, int32_t option, int32_t value) {
             E.have_list = false;   // the next run or call builds the list again
             E.list_half = E.want_half(JP_PREC_FP32_FIXED);
             return 0;
+        case JP_OPT_GRID_SHIFT:
+            if (value < -3 || value > 3) return JP_EINVAL;
+            E.autotune = 0;
+            return value == E.grid_shift ? 0 : E.regrid(value);
+        case JP_OPT_LIST_BATCH:
+            if (value != 1 && value != 2 && value != 4) return JP_EINVAL;
+            E.lnb = value;
+            E.autotune = 0;
+            return 0;
         default:
             return JP_EINVAL;
     }
@@ -2295,6 +2405,8 @@ static int get_option(const Engine &E, int32_t option, int32_t *value) {
         case JP_OPT_LIST_WAVES: *value = E.lw; return 0;
         case JP_OPT_CELL_TABLE: *value = E.cell_table; return 0;
         case JP_OPT_GRAPH_CAPTURES: *value = E.captures_last_run; return 0;
+        case JP_OPT_GRID_SHIFT: *value = E.grid_shift; return 0;
+        case JP_OPT_LIST_BATCH: *value = E.lnb; return 0;
         default: return JP_EINVAL;
     }
 }

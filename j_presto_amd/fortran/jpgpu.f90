@@ -27,7 +27,8 @@ module jpgpu
   integer(c_int), parameter :: JP_PREC_FP64 = 0, JP_PREC_FP32_FIXED = 1
   integer(c_int32_t), parameter :: JP_OPT_GRAPHS = 1, JP_OPT_AUTOTUNE = 2
   integer(c_int32_t), parameter :: JP_OPT_PAIR_WAVES = 3, JP_OPT_LIST_WAVES = 4, JP_OPT_LIST_SEGMENT = 5
-  integer(c_int32_t), parameter :: JP_OPT_CELL_TABLE = 6
+  integer(c_int32_t), parameter :: JP_OPT_CELL_TABLE = 6, JP_OPT_HALF_LIST = 8, JP_OPT_LIST_BATCH = 9
+  integer(c_int32_t), parameter :: JP_OPT_GRID_SHIFT = 10
 
   ! System description: the module state md_run builds in its input phase.
   ! Same field order and C layout as jp_sys_desc in include/jpgpu.h.

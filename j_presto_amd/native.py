@@ -58,7 +58,7 @@ SIGNATURES = {
     "jp_batch_get_option": (C.c_int, [vp, C.c_int32, i32p]),
 }
 JP_OPT_GRAPHS, JP_OPT_AUTOTUNE, JP_OPT_PAIR_WAVES, JP_OPT_LIST_WAVES, JP_OPT_LIST_SEGMENT = 1, 2, 3, 4, 5
-JP_OPT_CELL_TABLE, JP_OPT_GRAPH_CAPTURES, JP_OPT_HALF_LIST = 6, 7, 8
+JP_OPT_CELL_TABLE, JP_OPT_GRAPH_CAPTURES, JP_OPT_HALF_LIST, JP_OPT_LIST_BATCH, JP_OPT_GRID_SHIFT = 6, 7, 8, 9, 10
 JP_EINVAL, JP_ENOMEM, JP_ENODEV = -101, -102, -103
 
 _lib = None
