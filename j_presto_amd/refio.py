@@ -194,9 +194,11 @@ def write_shk(path: str, s: System) -> None:
         f.write("\n".join(L) + "\n")
 
 
-def write_md_inp(path: str, files: Dict[str, str], s: System, md: Dict[str, str]) -> None:
+def write_md_inp(path: str, files: Dict[str, str], s: System, md: Dict[str, str],
+                 inp: Dict[str, str] = None) -> None:
     """md.inp with an INPUT and an MD phase.  `files` has TOPOLO, COORDI
-    (binary) and optionally SETSHK; `md` is extra EXE> MD keywords."""
+    (binary) and optionally SETSHK; `md` is extra EXE> MD keywords, `inp`
+    extra EXE> INPUT keywords (e.g. LWEIGH, LTCNTL of inpinp.f90)."""
     L = ["EXE> INPUT", f"      TOPOLO= {files['TOPOLO']}", f"      COORDI= {files['COORDI']}",
          "      CRDFMT= BINA"]
     if files.get("SETSHK"):
@@ -205,6 +207,8 @@ def write_md_inp(path: str, files: Dict[str, str], s: System, md: Dict[str, str]
         na = int((s.cluster == 1).sum())
         L.append(f"      CLUSTA= 1<{na}")
         L.append("      GEPSME= LMD")
+    for k, val in (inp or {}).items():
+        L.append(f"      {k}= {val}")
     L += ["QUIT", "", "EXE> MD"]
     box = s.celwal
     base = {"CAL15M": "ZD", "CUTLEN": _fmt(s.cutoff), "BOUNDA": "PBC",

@@ -1094,16 +1094,46 @@ int jo_verlet_step(const jo_ctx *c, double *x, double *v, const double *gtot, do
     return 0;
 }
 
+/* Adaptive lambda square dynamics (METHOD= GE with GEPSME= LMD: idfvsl = 4,
+ * cluster_method LMD, verlethe.f90:235-290): lambda is a dynamical variable
+ * of mass lmass (LWEIGH, or the summed mass of the class-1 atoms,
+ * inpcls.f90:348-357) driven by
+ *   lambda_f = -2 E1 lambda - E2 - dew(lambda),
+ * with E1, E2 the unscaled class-1 and class-2 potential energies and dew
+ * the bias of the NAMEGE= polynomial windows taken in lambda itself plus
+ * the linear walls lower/upper outside [celw, ceup].  tctl is
+ * l_temp_control (LTCNTL, inpinp.f90:178-182): lambda's velocity is scaled
+ * with the temperature control and its degree of freedom counted; lgrp is
+ * the temperature group that takes lambda under TEMPCO= YES (ifg_lambda,
+ * md.f90:221-228), -1 with one group. */
+typedef struct {
+    int nwindow, ndeg;
+    const double *c, *low, *high;
+    double lower, upper, celw, ceup, tempce;
+    double lmass;
+    int tctl, lgrp;
+} jo_alsd;
+
 /* caltmp.f90:25-43 (one group) and caltmp2 (caltmp.f90:82-105, groups):
  * T[g] = (sum over the group's atoms, in atom order, of m |w|^2) * 1e7 /
- * (dof[g] * gascst). */
-static void group_temps(const jo_ctx *c, const double *w, int ngrp, const int *grp, const double *dof, double *T) {
+ * (dof[g] * gascst).  lm lv lv is lambda's kinetic term: caltmp starts ek
+ * with it (whatever l_temp_control says) and its OpenMP reduction adds the
+ * atoms' sum, formed from zero, to that; caltmp2 adds it to group lgrp after
+ * the atoms when tctl. */
+static void group_temps_l(const jo_ctx *c, const double *w, int ngrp, const int *grp, const double *dof, double *T,
+                          double lm, double lv, int tctl, int lgrp) {
     for (int g = 0; g < ngrp; ++g) T[g] = 0.0;
     for (int i = 0; i < c->n; ++i) {
         const int g = grp ? grp[i] : 0;
         T[g] = T[g] + c->mass[i] * (w[3 * i] * w[3 * i] + w[3 * i + 1] * w[3 * i + 1] + w[3 * i + 2] * w[3 * i + 2]);
     }
+    if (!grp) T[0] = lm * lv * lv + T[0];
+    if (grp && tctl && lgrp >= 0) T[lgrp] = T[lgrp] + lm * lv * lv;
     for (int g = 0; g < ngrp; ++g) T[g] = T[g] * 1.0e7 / (dof[g] * 8.31451); /* PHYCNS.f90:21 gascst */
+}
+
+static void group_temps(const jo_ctx *c, const double *w, int ngrp, const int *grp, const double *dof, double *T) {
+    group_temps_l(c, w, ngrp, grp, dof, T, 0.0, 0.0, 0, -1);
 }
 
 /* verlethe.f90:79-757 for METHOD= CANO (idfvsl = 3) with constant lambda,
@@ -1115,8 +1145,21 @@ static void group_temps(const jo_ctx *c, const double *w, int ngrp, const int *g
  * to icslop rounds of SHAKE + temperature (exit when every group is within
  * fcstol*temps).  A SHAKE failure returns its ier (md_run would halve dt
  * and retry, verlethe.f90:546-573).  Expressions keep md_run's operand order. */
+/* al != NULL: ALSD (verlethe.f90:235-290, 375-445, 465-470, 545-546,
+ * 577-595, 742): lambda (c->lambda) and its velocity *lv take the step
+ * beside the atoms under the force lf. */
+static int cano_step_l(jo_ctx *c, double *x, double *v, const double *gtot, double *xold, double *vback,
+                       double temps, int icslop, double fcstol, int ngrp, const int *grp, const double *dof,
+                       const jo_alsd *al, double lf, double *lv);
+
 int jo_cano_step(const jo_ctx *c, double *x, double *v, const double *gtot, double *xold, double *vback,
                  double temps, int icslop, double fcstol, int ngrp, const int *grp, const double *dof) {
+    return cano_step_l((jo_ctx *)c, x, v, gtot, xold, vback, temps, icslop, fcstol, ngrp, grp, dof, NULL, 0.0, NULL);
+}
+
+static int cano_step_l(jo_ctx *c, double *x, double *v, const double *gtot, double *xold, double *vback,
+                       double temps, int icslop, double fcstol, int ngrp, const int *grp, const double *dof,
+                       const jo_alsd *al, double lf, double *lv) {
     const int n = c->n;
     const double dt = c->dt, conv = 4.184e-4; /* md.f90:56 fdcvcv */
     double *im = (double *)malloc(sizeof(double) * n);
@@ -1129,10 +1172,15 @@ int jo_cano_step(const jo_ctx *c, double *x, double *v, const double *gtot, doub
     for (int i = 0; i < n; ++i) im[i] = 1.0 / c->mass[i]; /* md.f90:68 ifxmass */
     memcpy(xold, x, sizeof(double) * 3 * n);
     memcpy(vback, v, sizeof(double) * 3 * n);
+    /* lambda's state at the step start (lmd_back, lmd_vback) and mass */
+    const double lb = c->lambda, lvb = al ? *lv : 0.0;
+    const double lm = al ? al->lmass : 0.0, ilm = al ? 1.0 / al->lmass : 0.0;
+    const int tctl = al ? al->tctl : 0, lgrp = al ? al->lgrp : -1;
     double r = 0.5 * dt * conv;
     for (int i = 0; i < n; ++i)
         for (int k = 0; k < 3; ++k) w[3 * i + k] = v[3 * i + k] - r * gtot[3 * i + k] * im[i];
-    if (c->nshk) group_temps(c, w, ngrp, grp, dof, T);
+    double lvt = al ? lvb + r * lf * ilm : 0.0;
+    if (c->nshk) group_temps_l(c, w, ngrp, grp, dof, T, lm, lvt, tctl, lgrp);
     else for (int g = 0; g < ngrp; ++g) T[g] = temps;
     for (int i = 0; i < n; ++i) {
         const double itc = 1.0 / T[grp ? grp[i] : 0];
@@ -1140,8 +1188,20 @@ int jo_cano_step(const jo_ctx *c, double *x, double *v, const double *gtot, doub
         const double rt = chi * dt * conv, rt2 = 2.0 * chi - 1.0;
         for (int k = 0; k < 3; ++k) v[3 * i + k] = rt2 * v[3 * i + k] - rt * gtot[3 * i + k] * im[i];
     }
+    if (al) {
+        const int g = grp ? lgrp : 0;
+        if (tctl && g >= 0) {
+            const double chi = sqrt(temps * (1.0 / T[g]));
+            const double rt = chi * dt * conv, rt2 = 2.0 * chi - 1.0;
+            *lv = rt2 * *lv + rt * lf * ilm;
+        } else {
+            *lv = *lv + dt * conv * lf * ilm;
+        }
+    }
     for (int i = 0; i < 3 * n; ++i) x[i] = x[i] + dt * v[i];
+    if (al) c->lambda = c->lambda + dt * *lv;
     if (c->nshk) {
+        lvt = (lvb + (al ? *lv : 0.0)) * 0.5;
         for (int it = 1; it <= icslop; ++it) {
             memcpy(xb, x, sizeof(double) * 3 * n);
             ier = jo_shake(c, x, xold);
@@ -1151,7 +1211,7 @@ int jo_cano_step(const jo_ctx *c, double *x, double *v, const double *gtot, doub
                 v[i] = (x[i] - xold[i]) * r;
                 w[i] = (vback[i] + v[i]) * 0.5;
             }
-            group_temps(c, w, ngrp, grp, dof, T);
+            group_temps_l(c, w, ngrp, grp, dof, T, lm, lvt, tctl, lgrp);
             double tmin = T[0], tmax = T[0];
             for (int g = 1; g < ngrp; ++g) { tmin = fmin(tmin, T[g]); tmax = fmax(tmax, T[g]); }
             if (fmax(fabs(tmin - temps), fabs(tmax - temps)) < fcstol * temps) break;
@@ -1164,7 +1224,7 @@ int jo_cano_step(const jo_ctx *c, double *x, double *v, const double *gtot, doub
                     fc[q] = fc[q] + r * (x[q] - xb[q]);
                     w[q] = r2 * (-conv * gtot[q] * im[i] + fc[q]) + vback[q];
                 }
-            group_temps(c, w, ngrp, grp, dof, T);
+            group_temps_l(c, w, ngrp, grp, dof, T, lm, lvt, tctl, lgrp);
             for (int i = 0; i < n; ++i) {
                 const double itc = 1.0 / T[grp ? grp[i] : 0];
                 const double chi = sqrt(temps * itc);
@@ -1175,6 +1235,7 @@ int jo_cano_step(const jo_ctx *c, double *x, double *v, const double *gtot, doub
                 }
             }
             for (int i = 0; i < 3 * n; ++i) x[i] = xold[i] + dt * v[i];
+            if (al) c->lambda = lb + dt * *lv;   /* lambda's velocity is not touched by the rounds */
         }
     }
     free(im); free(w); free(fc); free(xb);
@@ -1287,15 +1348,47 @@ double jo_ge_weight(const jo_ge *g, double E) {
     return dew;
 }
 
-/* epotcc(1,1): dnergy's sum of the energy slots 2..maxene of class 1
+/* epotcc(1,k+1): dnergy's sum of the energy slots 2..maxene of class k
  * (dnergy.f90:78), bond, angle, torsion, improper, 1-4 vdW, 1-4 elec,
  * VDW15, ELE15 in slot order; the H-bond and restraint slots are zero. */
-static double class1_epot(const double *eterm, const double *ee, const double *ev) {
+static double class_epot(const double *eterm, const double *ee, const double *ev, int k) {
     double e = 0.0;
-    for (int k = 0; k < 6; ++k) e = e + eterm[k];
-    e = e + ev[0];
-    e = e + ee[0];
+    for (int m = 0; m < 6; ++m) e = e + eterm[6 * k + m];
+    e = e + ev[k];
+    e = e + ee[k];
     return e;
+}
+
+static double class1_epot(const double *eterm, const double *ee, const double *ev) {
+    return class_epot(eterm, ee, ev, 0);
+}
+
+/* ALSD's bias (verlethe.f90:236-253): the window is the first whose high_v
+ * exceeds lambda (the last one otherwise), its polynomial taken about
+ * low_v; below celw the wall lower*(celw - lambda) is added, above ceup
+ * upper*(lambda - ceup); the result times (gascst/joucal)*1e-3*tempce. */
+double jo_alsd_dew(const jo_alsd *a, double lambda) {
+    int i = a->nwindow - 1;
+    for (int j = 0; j < a->nwindow; ++j)
+        if (lambda < a->high[j]) { i = j; break; }
+    const double rtmp = lambda - a->low[i];
+    const double *ci = a->c + (size_t)i * (a->ndeg + 1);
+    double dprobl = ci[0];
+    for (int j = 1; j <= a->ndeg; ++j) dprobl = dprobl + ci[j] * powi_d(rtmp, j);
+    if (lambda < a->celw) {
+        dprobl = dprobl + a->lower * (a->celw - lambda);
+    } else if (lambda > a->ceup) {
+        dprobl = dprobl + a->upper * (lambda - a->ceup);
+    }
+    return (8.31451 / 4.184) * 1.0e-3 * a->tempce * dprobl; /* PHYCNS.f90:21,25 gascst, joucal */
+}
+
+double jo_lambda(const jo_ctx *c) { return c->lambda; }
+void jo_set_lambda(jo_ctx *c, double lambda) { c->lambda = lambda; }
+
+/* lambda_f = -2 E1 lambda - E2 - dew (verlethe.f90:255-275, LMD). */
+double jo_alsd_force(const jo_alsd *a, double lambda, double e1, double e2) {
+    return -(2.0 * e1 * lambda) - e2 - jo_alsd_dew(a, lambda);
 }
 
 /* One verlethe call for METHOD= CANO with its SHAKE retry: the substep's
@@ -1308,7 +1401,7 @@ static double class1_epot(const double *eterm, const double *ee, const double *e
  * receives the velocities the last substep started from. */
 static int cano_verlethe(jo_ctx *c, double *x, double *v, double *vback, double *grad, double *gt, double *xo,
                          double *xs, double *vs, double temps, int icslop, double fcstol, int ngrp, const int *grp,
-                         const double *dof, const jo_ge *ge, int *nprov) {
+                         const double *dof, const jo_ge *ge, const jo_alsd *al, double *lv, int *nprov) {
     const int n = c->n, nfrg = c->nfrg;
     const double dt0 = c->dt;
     double T = dt0, eterm[18], ee[3], ev[3];
@@ -1316,6 +1409,7 @@ static int cano_verlethe(jo_ctx *c, double *x, double *v, double *vback, double 
     for (;;) {
         memcpy(xs, x, sizeof(double) * 3 * n);
         memcpy(vs, v, sizeof(double) * 3 * n);
+        const double lb = c->lambda, lvb = al ? *lv : 0.0;
         c->dt = T;
         memset(grad, 0, sizeof(double) * 3 * (size_t)n * nfrg);
         jo_bonded(c, x, grad, eterm);
@@ -1325,10 +1419,14 @@ static int cano_verlethe(jo_ctx *c, double *x, double *v, double *vback, double 
             const double dew = jo_ge_weight(ge, class1_epot(eterm, ee, ev));
             for (size_t q = 0; q < 3 * (size_t)n; ++q) gt[q] = gt[q] * dew;
         }
-        ier = jo_cano_step(c, x, v, gt, xo, vback, temps, icslop, fcstol, ngrp, grp, dof);
+        double lf = 0.0;
+        if (al) lf = jo_alsd_force(al, c->lambda, class_epot(eterm, ee, ev, 0), class_epot(eterm, ee, ev, 1));
+        ier = cano_step_l(c, x, v, gt, xo, vback, temps, icslop, fcstol, ngrp, grp, dof, al, lf, lv);
         if (ier) {
             memcpy(x, xs, sizeof(double) * 3 * n);
             memcpy(v, vs, sizeof(double) * 3 * n);
+            c->lambda = lb;
+            if (al) *lv = lvb;
             if (T < 0.01 * dt0) break;
             ++*nprov;
             jshk *= 2;
@@ -1355,8 +1453,34 @@ static int cano_verlethe(jo_ctx *c, double *x, double *v, double *vback, double 
  * from), then nsteps verlethe calls with a list rebuild after every step
  * that is a multiple of update (md.f90:519-694).  nprov counts the SHAKE
  * provisions (failed substeps) md_run prints. */
+static int run_cano_l(jo_ctx *c, double *x, double *v, int nsteps, int update, int inirst, double temps,
+                      int icslop, double fcstol, int ngrp, const int *grp, const double *dof, const jo_ge *ge,
+                      const jo_alsd *al, double *lv, double *lam_log, int *nprov);
+
 int jo_run_cano(jo_ctx *c, double *x, double *v, int nsteps, int update, int inirst, double temps, int icslop,
                 double fcstol, int ngrp, const int *grp, const double *dof, const jo_ge *ge, int *nprov) {
+    return run_cano_l(c, x, v, nsteps, update, inirst, temps, icslop, fcstol, ngrp, grp, dof, ge, NULL, NULL, NULL,
+                      nprov);
+}
+
+/* md_run's METHOD= GE loop under GEPSME= LMD (ALSD) from a restart: as
+ * jo_run_cano, with lambda (c->lambda) and its velocity *lv integrated
+ * beside the atoms; lam_log (nsteps values, or NULL) receives lambda after
+ * every step (the first column of md_run's NAMETO file under LMD,
+ * md.f90:578-583).  Needs three energy classes.  The restart set-up step
+ * puts lambda and its velocity back with the coordinates (md.f90:1150-1151,
+ * 1255-1258). */
+int jo_run_alsd(jo_ctx *c, double *x, double *v, double *lv, int nsteps, int update, int inirst, double temps,
+                int icslop, double fcstol, int ngrp, const int *grp, const double *dof, const jo_alsd *al,
+                double *lam_log, int *nprov) {
+    if (!al || c->nfrg != 3 || al->nwindow < 1 || al->ndeg < 0 || !(al->lmass > 0.0)) return JP_EINVAL;
+    return run_cano_l(c, x, v, nsteps, update, inirst, temps, icslop, fcstol, ngrp, grp, dof, NULL, al, lv, lam_log,
+                      nprov);
+}
+
+static int run_cano_l(jo_ctx *c, double *x, double *v, int nsteps, int update, int inirst, double temps,
+                      int icslop, double fcstol, int ngrp, const int *grp, const double *dof, const jo_ge *ge,
+                      const jo_alsd *al, double *lv, double *lam_log, int *nprov) {
     const size_t n3 = 3 * (size_t)c->n;
     double *grad = (double *)malloc(sizeof(double) * n3 * c->nfrg);
     double *gt = (double *)malloc(sizeof(double) * n3);
@@ -1372,13 +1496,19 @@ int jo_run_cano(jo_ctx *c, double *x, double *v, int nsteps, int update, int ini
     }
     int ier = c->have_list ? 0 : jo_celprp(c, x);
     if (!ier && inirst) {
+        const double l0 = c->lambda, lv0 = al ? *lv : 0.0;
         memcpy(x0, x, sizeof(double) * n3);
-        ier = cano_verlethe(c, x, v, vb, grad, gt, xo, xs, vs, temps, icslop, fcstol, ngrp, grp, dof, ge, nprov);
+        ier = cano_verlethe(c, x, v, vb, grad, gt, xo, xs, vs, temps, icslop, fcstol, ngrp, grp, dof, ge, al, lv,
+                            nprov);
         memcpy(x, x0, sizeof(double) * n3);
         memcpy(v, vb, sizeof(double) * n3);
+        c->lambda = l0;
+        if (al) *lv = lv0;
     }
     for (int s = 1; s <= nsteps && !ier; ++s) {
-        ier = cano_verlethe(c, x, v, vb, grad, gt, xo, xs, vs, temps, icslop, fcstol, ngrp, grp, dof, ge, nprov);
+        ier = cano_verlethe(c, x, v, vb, grad, gt, xo, xs, vs, temps, icslop, fcstol, ngrp, grp, dof, ge, al, lv,
+                            nprov);
+        if (lam_log) lam_log[s - 1] = c->lambda;
         if (!ier && update > 0 && s % update == 0) ier = jo_celprp(c, x);
     }
     free(grad); free(gt); free(xo); free(vb); free(xs); free(vs); free(x0);

@@ -54,6 +54,15 @@ def lib():
                                   C.c_double, C.c_int, i32p, f64p, C.POINTER(GeParams), i32p]
         L.jo_ge_weight.argtypes = [C.POINTER(GeParams), C.c_double]
         L.jo_ge_weight.restype = C.c_double
+        L.jo_alsd_dew.argtypes = [C.POINTER(AlsdParams), C.c_double]
+        L.jo_alsd_dew.restype = C.c_double
+        L.jo_alsd_force.argtypes = [C.POINTER(AlsdParams), C.c_double, C.c_double, C.c_double]
+        L.jo_alsd_force.restype = C.c_double
+        L.jo_run_alsd.argtypes = [C.c_void_p, f64p, f64p, f64p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int,
+                                  C.c_double, C.c_int, i32p, f64p, C.POINTER(AlsdParams), f64p, i32p]
+        L.jo_lambda.argtypes = [C.c_void_p]
+        L.jo_lambda.restype = C.c_double
+        L.jo_set_lambda.argtypes = [C.c_void_p, C.c_double]
         L.jo_list_brute.argtypes = [C.c_void_p, f64p, i32p, i32p, C.c_int64]
         L.jo_list_brute.restype = C.c_int64
         L.jo_constants.argtypes = [C.c_void_p, f64p]
@@ -84,6 +93,32 @@ def ge_params(ge: dict):
     p = GeParams(c.shape[0], c.shape[1] - 1, _p(c), _p(low), _p(high), float(ge["alphalw"]), float(ge["alphaup"]),
                  float(ge["celw"]), float(ge["ceup"]), float(ge["tempce"]))
     return p, (c, low, high)
+
+
+class AlsdParams(C.Structure):
+    """jo_alsd: adaptive lambda square dynamics (inmcmd.f90:20-41 under
+    GEPSME= LMD, LWEIGH, LTCNTL)."""
+    _fields_ = [("nwindow", C.c_int), ("ndeg", C.c_int), ("c", f64p), ("low", f64p), ("high", f64p),
+                ("lower", C.c_double), ("upper", C.c_double), ("celw", C.c_double), ("ceup", C.c_double),
+                ("tempce", C.c_double), ("lmass", C.c_double), ("tctl", C.c_int), ("lgrp", C.c_int)]
+
+
+def alsd_params(al: dict, lgrp: int = -1):
+    """(AlsdParams, arrays it points into) from a dict with c (nwindow,
+    ndeg+1), low, high, lower, upper, celw, ceup, tempce, lmass, tctl."""
+    c = np.ascontiguousarray(al["c"], np.float64)
+    low = np.ascontiguousarray(al["low"], np.float64)
+    high = np.ascontiguousarray(al["high"], np.float64)
+    p = AlsdParams(c.shape[0], c.shape[1] - 1, _p(c), _p(low), _p(high), float(al["lower"]), float(al["upper"]),
+                   float(al["celw"]), float(al["ceup"]), float(al["tempce"]), float(al["lmass"]),
+                   int(bool(al["tctl"])), int(lgrp))
+    return p, (c, low, high)
+
+
+def alsd_dew(al: dict, lam: float) -> float:
+    """ALSD's bias dew at lambda (the polynomial windows plus the walls)."""
+    p, keep = alsd_params(al)
+    return lib().jo_alsd_dew(C.byref(p), float(lam))
 
 
 def ge_weight(ge: dict, e: float) -> float:
@@ -198,6 +233,45 @@ class Oracle:
                                 None if g is None else _p(g, i32p), _p(dof),
                                 None if gp is None else C.byref(gp), C.byref(nprov))
         return ier, nprov.value
+
+    @property
+    def lam(self) -> float:
+        return lib().jo_lambda(self.c)
+
+    @lam.setter
+    def lam(self, value: float):
+        lib().jo_set_lambda(self.c, float(value))
+
+    def alsd_dof(self, grp=None, tctl: bool = True):
+        """Degrees of freedom per temperature group with lambda's (md.f90:
+        114-123 for one group; 221-228 under TEMPCO= YES: the first group with
+        the most degrees of freedom takes it), and that group (-1: one group
+        or no lambda temperature control)."""
+        dof = self.group_dof(grp)
+        lgrp = -1
+        if tctl:
+            if grp is None:
+                dof[0] += 1.0
+            else:
+                lgrp = int(np.argmax(dof))
+                dof[lgrp] += 1.0
+        return dof, lgrp
+
+    def run_alsd(self, x, v, lam_v: float, nsteps: int, alsd: dict, update: int = 10, temps: float = 300.0,
+                 icslop: int = 3, fcstol: float = 1e-3, grp=None, inirst: bool = True):
+        """md_run's ALSD loop (METHOD= GE, GEPSME= LMD) in place from the
+        oracle's lambda and lam_v; returns (ier, lambda, lambda_v, lambda after
+        every step, provisions)."""
+        dof, lgrp = self.alsd_dof(grp, bool(alsd["tctl"]))
+        g = None if grp is None else np.ascontiguousarray(grp, np.int32)
+        ap, keep = alsd_params(alsd, lgrp)
+        lv = C.c_double(float(lam_v))
+        lam_log = np.zeros(max(nsteps, 1))
+        nprov = C.c_int32()
+        ier = lib().jo_run_alsd(self.c, _p(x), _p(v), C.byref(lv), nsteps, update, int(inirst), temps, icslop,
+                                fcstol, len(dof), None if g is None else _p(g, i32p), _p(dof), C.byref(ap),
+                                _p(lam_log), C.byref(nprov))
+        return ier, self.lam, lv.value, lam_log[:nsteps], nprov.value
 
     def list_brute(self, x):
         x = np.ascontiguousarray(x, np.float64)

@@ -127,3 +127,19 @@ def admissible(windows, ci, cj):
         w = windows[a]
         ok &= np.array([int(b) in w[int(c)] for c, b in zip(ci[:, a], cj[:, a])], bool)
     return ok
+
+
+def alsd_params(s, f, tag):
+    """The ALSD parameters of a fixture's run `tag` (make_golden.ALSD_*):
+    lambda's mass is LWEIGH, or md_run's default, the class-1 atoms' masses
+    added in atom order (inpcls.f90:348-357)."""
+    celw, ceup = f[f"alsd_{tag}_range"]
+    tempco, tctl, lw = f[f"alsd_{tag}_opt"]
+    lm = float(lw)
+    if lm == 0.0:
+        for m, c in zip(s.mass, s.cluster):
+            if c == 1:
+                lm += float(m)
+    return dict(c=f["alsd_poly"], low=f["alsd_low"], high=f["alsd_high"], lower=float(f["alsd_walls"][0]),
+                upper=float(f["alsd_walls"][1]), celw=float(celw), ceup=float(ceup),
+                tempce=float(f["alsd_tempce"]), lmass=lm, tctl=bool(tctl)), bool(tempco)

@@ -26,7 +26,12 @@ and N_SIMD= 1 (a fixed summation order):
 A fixture stores the generator recipe, the input x and v, and the outputs.
 The reference itself never leaves this container.
 
-Usage: python tests/golden/make_golden.py [--cano-only | --mon-only | --host-only | --ge-only] [name ...]
+  alsd jobs  (chain_lmd, --alsd-only) RESTAR= YES, METHOD= GE with GEPSME= LMD
+             (adaptive lambda square dynamics, ALSD_* below): lambda moves
+             under -2 E1 lambda - E2 - dew(lambda) beside the atoms.
+
+Usage: python tests/golden/make_golden.py [--cano-only | --mon-only | --host-only | --ge-only | --alsd-only]
+       [name ...]
 """
 from __future__ import annotations
 
@@ -96,9 +101,9 @@ def fingerprint(s: systems.System) -> str:
     return h.hexdigest()
 
 
-def run_md(workdir: str, s: systems.System, md: dict, shake: bool = True) -> str:
+def run_md(workdir: str, s: systems.System, md: dict, shake: bool = True, inp: dict = None) -> str:
     files = refio.write_inputs(workdir, s, shake=shake)
-    refio.write_md_inp(os.path.join(workdir, "md.inp"), files, s, md)
+    refio.write_md_inp(os.path.join(workdir, "md.inp"), files, s, md, inp=inp)
     env = dict(os.environ, OMP_NUM_THREADS="1", OMP_STACKSIZE="512M")
     # md_run keeps per-atom automatic arrays on the stack: lift the limit for large systems
     with open(os.path.join(workdir, "md.inp")) as fin, open(os.path.join(workdir, "md.log"), "w") as fout:
@@ -289,6 +294,82 @@ def ge_jobs(s: systems.System, tmp: str) -> dict:
     return out
 
 
+# Adaptive lambda square dynamics (METHOD= GE under GEPSME= LMD) for
+# chain_lmd: two windows of cubic polynomials in lambda itself (bias dew of
+# several hundred kcal/mol, of the order of -2 E1 lambda - E2 at lambda =
+# 0.7), walls lower/upper outside [celw, ceup].  Runs (tag: celw, ceup,
+# TEMPCO, LTCNTL, LWEIGH or None for md_run's default, the class-1 mass):
+# inside the windows, below celw, above ceup, with and without lambda's
+# temperature control, one and two temperature groups.
+ALSD_POLY = np.array([[1400.0, 300.0, -200.0, 100.0], [1350.0, 400.0, -150.0, 80.0]])
+ALSD_LOW, ALSD_HIGH = np.array([0.0, 0.65]), np.array([0.65, 1.5])
+ALSD_WALLS = (2000.0, 3000.0)   # lower, upper
+ALSD_TEMPCE = 300.0
+ALSD_RUNS = {"a": (0.5, 1.0, "NO", "YES", None), "b": (0.8, 0.95, "YES", "YES", 400.0),
+             "c": (0.4, 0.6, "NO", "NO", None), "d": (0.5, 1.0, "YES", "NO", 250.0)}
+
+
+def write_alsd_params(path: str, celw: float, ceup: float) -> None:
+    """inmcmd.f90:20-41 layout under LMD: the GE records, then lower upper."""
+    nw, nd1 = ALSD_POLY.shape
+    L = [f"{nw} {nd1 - 1}"]
+    for w in range(nw):
+        L += [repr(float(c)) for c in ALSD_POLY[w]]
+        L.append(f"{float(ALSD_LOW[w])!r} {float(ALSD_HIGH[w])!r}")
+    L += ["1.0 1.0", f"{celw!r} {ceup!r}", repr(ALSD_TEMPCE), f"{ALSD_WALLS[0]!r} {ALSD_WALLS[1]!r}"]
+    with open(path, "w") as f:
+        f.write("\n".join(L) + "\n")
+
+
+def alsd_jobs(s: systems.System, tmp: str) -> dict:
+    """ALSD runs (verlethe.f90 idfvsl = 4 under LMD) from the fixture's
+    restart at 300 K, 10 steps each: final x, v, lambda, lambda_v and the
+    NAMETO series (lambda after the step, E1, E2, their sum with the
+    unscaled class 3, md.f90:578-583); run "a" also prints its monitoring
+    blocks every 5 steps (kinetic energy with lambda's term)."""
+    out = dict(alsd_poly=ALSD_POLY, alsd_low=ALSD_LOW, alsd_high=ALSD_HIGH, alsd_walls=np.array(ALSD_WALLS),
+               alsd_tempce=np.array(ALSD_TEMPCE))
+    for tag, (celw, ceup, tempco, ltc, lw) in ALSD_RUNS.items():
+        d = os.path.join(tmp, "alsd_" + tag)
+        os.makedirs(d)
+        refio.write_restart(os.path.join(d, "in.rst"), s.x, s.v, lam=s.lam)
+        write_alsd_params(os.path.join(d, "ge.par"), celw, ceup)
+        inp = {"LTCNTL": ltc}
+        if lw is not None:
+            inp["LWEIGH"] = repr(float(lw))
+        log = run_md(d, s, {"METHOD": "GE", "RESTAR": "YES", "NAMERI": "in.rst", "NAMERO": "out.rst",
+                            "UPDATE": "10", "TIMEST": "2.0", "SETTEM": "300.0", "TEMPCO": tempco,
+                            "NAMEGE": "ge.par", "NAMETO": "ene.dat", "LOOPLI": "10",
+                            "OUTLOG": "5" if tag == "a" else "100000", "OUTCLE": "100000"}, inp=inp)
+        r = refio.read_restart(os.path.join(d, "out.rst"))
+        ene = np.loadtxt(os.path.join(d, "ene.dat"), ndmin=2)
+        out.update({f"alsd_{tag}_x": r["x"], f"alsd_{tag}_v": r["v"], f"alsd_{tag}_lam": np.float64(r["lam"]),
+                    f"alsd_{tag}_lamv": np.float64(r["lam_v"]), f"alsd_{tag}_loop": np.int64(r["loop"]),
+                    f"alsd_{tag}_range": np.array([celw, ceup]), f"alsd_{tag}_ene": ene,
+                    f"alsd_{tag}_opt": np.array([1 if tempco == "YES" else 0, 1 if ltc == "YES" else 0,
+                                                 0.0 if lw is None else lw])})
+        if tag == "a":
+            out["alsd_a_mon"] = parse_monitor(log)
+    return out
+
+
+def add_alsd(name: str) -> str:
+    """Adds the ALSD runs to an existing fixture."""
+    kind, args, opt = RECIPES[name]
+    s = build_system(kind, args)
+    path = os.path.join(HERE, f"{name}.npz")
+    with np.load(path, allow_pickle=False) as z:
+        out = {k: z[k] for k in z.files}
+    assert str(out["fingerprint"]) == fingerprint(s)
+    tmp = tempfile.mkdtemp(prefix="jpgold_")
+    try:
+        out.update(alsd_jobs(s, tmp))
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    np.savez_compressed(path, **out)
+    return path
+
+
 def add_ge(name: str) -> str:
     """Adds the METHOD= GE runs to an existing fixture."""
     kind, args, opt = RECIPES[name]
@@ -364,10 +445,13 @@ def main(argv):
     mon_only = "--mon-only" in argv
     host_only = "--host-only" in argv
     ge_only = "--ge-only" in argv
-    argv = [a for a in argv if a not in ("--cano-only", "--mon-only", "--host-only", "--ge-only")]
+    alsd_only = "--alsd-only" in argv
+    argv = [a for a in argv if a not in ("--cano-only", "--mon-only", "--host-only", "--ge-only", "--alsd-only")]
     names = argv or [n for n in RECIPES if not mon_only or RECIPES[n][2].get("mon")]
     for n in names:
-        if ge_only:
+        if alsd_only:
+            p = add_alsd(n)
+        elif ge_only:
             p = add_ge(n)
         elif host_only:
             p = add_host(n)

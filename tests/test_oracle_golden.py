@@ -220,3 +220,72 @@ def test_ge_trajectory_equals_reference(golden, tag):
         x, v = s.x.copy(), s.v.copy()
         assert o.run_cano(x, v, n, 10, 300.0, 3, 1e-3, ge=bad) == (0, 0)
         assert np.abs(x - f[f"ge_{tag}_x"]).max() > 1e-6
+
+
+@pytest.mark.parametrize("tag", ["a", "b", "c", "d"])
+def test_alsd_trajectory_equals_reference(golden, tag):
+    """Adaptive lambda square dynamics (METHOD= GE under GEPSME= LMD,
+    verlethe.f90:235-290): lambda moves under -2 E1 lambda - E2 - dew beside
+    the atoms, its kinetic term in the temperature.  Runs inside the windows
+    ("a", "d"), below celw ("b": lower wall) and above ceup ("c": upper
+    wall), with lambda's temperature control ("a", "b") and without ("c",
+    "d"), one temperature group ("a", "c") and TEMPCO= YES ("b", "d").
+    Oracle.run_alsd reproduces md_run's x, v, lambda and lambda velocity bit
+    for bit, and lambda after every step to NAMETO's printed digits; the same
+    run with the walls exchanged, or with the temperature-control flag
+    flipped, does not."""
+    from conftest import alsd_params
+    s, f = golden("chain_lmd")
+    al, tempco = alsd_params(s, f, tag)
+    grp = s.temperature_groups() if tempco else None
+    n = int(f[f"alsd_{tag}_loop"])
+    o = Oracle(s)
+    assert o.lam == s.lam
+    x, v = s.x.copy(), s.v.copy()
+    ier, lam, lamv, lam_log, nprov = o.run_alsd(x, v, 0.0, n, al, grp=grp)
+    assert (ier, nprov) == (0, 0)
+    assert np.array_equal(x, f[f"alsd_{tag}_x"]) and np.array_equal(v, f[f"alsd_{tag}_v"])
+    assert lam == float(f[f"alsd_{tag}_lam"]) and lamv == float(f[f"alsd_{tag}_lamv"])
+    ene = f[f"alsd_{tag}_ene"]
+    assert np.allclose(lam_log, ene[:, 0], rtol=1e-7, atol=0)
+    for bad in (dict(al, lower=al["upper"], upper=al["lower"]), dict(al, tctl=not al["tctl"])):
+        if tag in ("a", "d") and bad["lower"] != al["lower"]:
+            continue   # inside the windows the walls do not act
+        o = Oracle(s)
+        x, v = s.x.copy(), s.v.copy()
+        ier, lam2, _, _, _ = o.run_alsd(x, v, 0.0, n, bad, grp=grp)
+        assert ier == 0
+        assert abs(lam2 - lam) > 1e-9 or np.abs(x - f[f"alsd_{tag}_x"]).max() > 1e-9
+
+
+def test_alsd_monitored_temperature(golden):
+    """md_run's monitoring of ALSD run "a" (monmd.f90:112 via caltmp with
+    lambda_vt = (lambda_v before + after the step)/2, md.f90:618-633): the
+    printed kinetic energy and temperature take lambda's term lm lvt^2 beside
+    the atoms' (v(t-dt/2)+v(t+dt/2))/2, and the temperature counts lambda's
+    degree of freedom (md.f90:114-123).  The term is far below K's printed
+    digits; the temperature's seven decimals see both."""
+    from conftest import alsd_params
+    s, f = golden("chain_lmd")
+    al, tempco = alsd_params(s, f, "a")
+    assert al["tctl"]
+    mon = f["alsd_a_mon"]
+    o = Oracle(s)
+    x, v = s.x.copy(), s.v.copy()
+    ier, _, lv, _, _ = o.run_alsd(x, v, 0.0, 4, al)
+    assert ier == 0
+    vb, lvb = v.copy(), lv
+    ier, _, lv, _, _ = o.run_alsd(x, v, lv, 1, al, inirst=False)
+    assert ier == 0
+    w = 0.5 * (vb + v)
+    mw2 = float((s.mass[:, None] * w * w).sum())
+    lvt = 0.5 * (lvb + lv)
+    lke = al["lmass"] * lvt * lvt
+    dof = float(o.group_dof(None)[0])
+    row = mon[mon[:, 0] == 5][0]
+    k = (mw2 + lke) * 1.0e7 / 4.184 * float(np.float32(0.0005))
+    assert abs(k - row[4]) <= 5e-7 * abs(row[4])   # seven significant digits (E15.7)
+    temp = lambda e, n: e * 1.0e7 / (8.31451 * n)
+    assert abs(temp(mw2 + lke, dof + 1.0) - row[3]) <= 6e-8
+    assert abs(temp(mw2, dof + 1.0) - row[3]) > 1e-6
+    assert abs(temp(mw2 + lke, dof) - row[3]) > 1e-3
