@@ -129,6 +129,9 @@ typedef struct jp_energy_log {
     double eimp[3];            /* IMPROPER (enevcc(5,:)) */
     double evdw14[3];          /* 1-4 VDW (enevcc(6,:)) */
     double eelec14[3];         /* 1-4 ELEC (enevcc(7,:)) */
+    double lambda;             /* lambda after the step (LMD: the first column of NAMETO,
+                                  md.f90:578-583; moves under ALSD, jp_set_alsd) */
+    double lambda_v;           /* its velocity after the step (ALSD; 0 otherwise) */
 } jp_energy_log;
 
 typedef struct jp_ctx jp_ctx;
@@ -246,12 +249,45 @@ int jp_set_canonical(jp_ctx *ctx, int32_t on, double temps, int32_t icslop, doub
  * first window i with r = (E - celw)/(ceup - celw) < high[i] (the last
  * window beyond it); the parameters of the NAMEGE= file (inmcmd.f90:20-31).
  * Every step then evaluates energies, and E never leaves the device.  One
- * system with one energy class (no LMD; ALSD and two-class McMD are not
- * provided).  on = 0 switches it off.  Returns JP_EINVAL for other systems,
- * nwindow < 1, ndeg < 0, celw >= ceup or non-finite parameters; jp_run_nve
- * returns JP_EINVAL while it is on without jp_set_canonical. */
+ * system with one energy class (no LMD: ALSD is jp_set_alsd; two-class McMD
+ * is not provided).  on = 0 switches it off; on = 1 switches ALSD off.
+ * Returns JP_EINVAL for other systems, nwindow < 1, ndeg < 0, celw >= ceup
+ * or non-finite parameters; jp_run_nve returns JP_EINVAL while it is on
+ * without jp_set_canonical. */
 int jp_set_mcmd(jp_ctx *ctx, int32_t on, int32_t nwindow, int32_t ndeg, const double *c, const double *low,
                 const double *high, double alphalw, double alphaup, double celw, double ceup, double tempce);
+
+/* Adaptive lambda square dynamics (md_run METHOD= GE under GEPSME= LMD:
+ * idfvsl = 4 with cluster_method LMD, verlethe.f90:235-290) for the
+ * following jp_run_nve calls, on top of jp_set_canonical's temperature
+ * control (call it after jp_set_canonical; a new jp_set_canonical turns it
+ * off).  lambda becomes a dynamical variable of mass lambda_m (LWEIGH;
+ * <= 0: md_run's default, the class-1 atoms' mass added in atom order,
+ * inpcls.f90:348-357) driven by
+ *   lambda_f = -2 E1 lambda - E2 - (gascst/joucal) 1e-3 tempce dprobl,
+ * E1 and E2 the step's unscaled class-1 and class-2 potential energies and
+ * dprobl the polynomial c[i*(ndeg+1) + j] (lambda - low[i])^j of the first
+ * window i with lambda < high[i] (the last one beyond it), plus
+ * lower * (celw - lambda) below celw and upper * (lambda - ceup) above ceup
+ * (the NAMEGE= file of inmcmd.f90:20-41).  temp_control is LTCNTL
+ * (l_temp_control): lambda's velocity is scaled with its temperature group
+ * and its degree of freedom is counted; its kinetic term m_l lv^2 is in the
+ * temperature of the one group always (caltmp), of the group with the most
+ * degrees of freedom under temperature groups only with temp_control
+ * (caltmp2, md.f90:221-228).  lambda and its velocity stay on the device
+ * during a run (jp_get_lambda / jp_set_lambda, restart files:
+ * outrst.f90:61); every step evaluates energies.  Systems need LMD with
+ * three energy classes.  on = 0 switches it off (and jp_set_mcmd).  Returns
+ * JP_EINVAL without METHOD= CANO, for other systems, nwindow < 1, ndeg < 0
+ * or non-finite parameters. */
+int jp_set_alsd(jp_ctx *ctx, int32_t on, int32_t nwindow, int32_t ndeg, const double *c, const double *low,
+                const double *high, double lower, double upper, double celw, double ceup, double tempce,
+                double lambda_m, int32_t temp_control);
+/* lambda and its velocity (COMCMM's lambda, lambda_v; the restart file's
+ * last record, inprst.f90:56-66, outrst.f90:61).  Setting them also sets the
+ * class factors of the following evaluations. */
+int jp_get_lambda(jp_ctx *ctx, double *lambda, double *lambda_v);
+int jp_set_lambda(jp_ctx *ctx, double lambda, double lambda_v);
 
 /* Mean duration in seconds of the force-step pair kernel (the kernel of every
  * step without energies) over reps launches back to back on the current list
@@ -373,6 +409,15 @@ int jp_batch_get_option(jp_batch *b, int32_t option, int32_t *value);
  * of the batch in order, or NULL). */
 int jp_batch_set_canonical(jp_batch *b, int32_t on, double temps, int32_t icslop, double fcstol,
                            const int32_t *group);
+/* jp_set_alsd, jp_get_lambda and jp_set_lambda for the systems of a batch
+ * (one parameter set; lambda, its velocity and, with lambda_m <= 0, its
+ * mass per system). */
+int jp_batch_set_alsd(jp_batch *b, int32_t on, int32_t nwindow, int32_t ndeg, const double *c, const double *low,
+                      const double *high, double lower, double upper, double celw, double ceup, double tempce,
+                      double lambda_m, int32_t temp_control);
+int jp_batch_get_lambda(jp_batch *b, int32_t isys, double *lambda, double *lambda_v);
+int jp_batch_set_lambda(jp_batch *b, int32_t isys, double lambda, double lambda_v);
+
 
 /* Library version and device name (for logs). */
 const char *jp_version(void);

@@ -41,7 +41,8 @@ class EnergyLog(C.Structure):
     _fields_ = [("step", C.c_int32), ("etot", C.c_double), ("epot", C.c_double), ("ekin", C.c_double),
                 ("eelec", C.c_double * 3), ("evdw", C.c_double * 3), ("ebond", C.c_double * 3),
                 ("eangle", C.c_double * 3), ("etors", C.c_double * 3), ("eimp", C.c_double * 3),
-                ("evdw14", C.c_double * 3), ("eelec14", C.c_double * 3)]
+                ("evdw14", C.c_double * 3), ("eelec14", C.c_double * 3), ("lambda_", C.c_double),
+                ("lambda_v", C.c_double)]
 
 
 def n_matrix(nfrag: int, method: str) -> np.ndarray:

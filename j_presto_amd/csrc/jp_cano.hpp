@@ -34,6 +34,7 @@
 #define JP_FDCVCV 4.184e-4  // md.f90:56
 
 #define JP_CANO_MAXG 32   // temperature groups of one system
+#define JP_LS 10          // doubles of ALSD state per system (jp_alsd.hpp)
 
 struct CanoArgs {
     int natom, nunit, icslop, nab, nub;   // nab, nub: atom blocks, unit blocks
@@ -51,6 +52,13 @@ struct CanoArgs {
     const double *gdof, *temps;
     const double *mass, *imass, *gslot;
     const double *gew;         // METHOD= GE: per system, the weight every gradient read carries (null: 1)
+    // ALSD (jp_alsd.hpp; lam null: off): per system the lambda state
+    // [JP_LS], its mass, the group whose temperature takes lambda's kinetic
+    // term (-1: none), and l_temp_control
+    double *lam;
+    const double *lmass;
+    const int *lgrp;
+    int ltc;
     double *x, *v, *xold, *vback, *fcc, *xcb;
     double *pa;                // partials of m|w|^2 of begin: [group][atom block]
     double *pu, *pn;           // of w and w' per round, two buffers by round parity:
@@ -91,13 +99,23 @@ __device__ __forceinline__ void block_partials(const double (&val)[NQ], const in
     }
 }
 
+// ALSD: lambda's kinetic term m_l lv^2 (lv from state slot ls) where group
+// g takes it (caltmp.f90:24, 98-99), else 0.
+__device__ __forceinline__ double lambda_ke(const CanoArgs &A, int s, int g, int ls) {
+#pragma clang fp contract(off)
+    if (!A.lam || A.lgrp[s] != g) return 0.0;
+    const double lv = A.lam[JP_LS * s + ls];
+    return A.lmass[s] * lv * lv;
+}
+
 // Temperatures of groups [g0, g1) into sT (caltmp.f90:39-43) from the
 // partials of the blocks [bl0[s], bl1[s]) of each group's system: lane l adds
 // blocks l, l+64, ... in order, then a butterfly (every block gets the same
-// value).  All threads of the block take part.
+// value), plus lambda's kinetic term (ALSD, velocity in state slot ls).  All
+// threads of the block take part.
 __device__ __forceinline__ void block_temps(const CanoArgs &A, const double *__restrict__ P, int nblk,
                                             const int *__restrict__ bl0, const int *__restrict__ bl1, int g0, int g1,
-                                            double *sT) {
+                                            double *sT, int ls) {
 #pragma clang fp contract(off)
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63, nw = blockDim.x >> 6;
     for (int g = g0 + w; g < g1; g += nw) {
@@ -106,9 +124,32 @@ __device__ __forceinline__ void block_temps(const CanoArgs &A, const double *__r
         for (int b = bl0[s] + l; b < bl1[s]; b += 64) p = p + P[(size_t)g * nblk + b];
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) p = p + __shfl_xor(p, o, 64);
+        p = lambda_ke(A, s, g, ls) + p;
         if (l == 0) sT[g - g0] = p * 1.0e7 / (A.gdof[g] * JP_GASCST);
     }
     __syncthreads();
+}
+
+// ALSD, round 1 of a step (verlethe.f90:438-445, 477-482, 508): lambda's
+// velocity scaled with the temperature T of its group when l_temp_control
+// (else a plain kick), then lambda's drift, and the mean velocity the later
+// rounds' temperatures use.  One thread per system.
+__device__ __forceinline__ void alsd_move(const CanoArgs &A, int s, double T, double ts, double dt) {
+#pragma clang fp contract(off)
+    double *L = A.lam + JP_LS * s;
+    const double lf = L[2], ilm = 1.0 / A.lmass[s];
+    double lv = L[1];
+    if (A.ltc && A.lgrp[s] >= 0) {
+        const double itc = 1.0 / T;
+        const double chi = sqrt(ts * itc);
+        const double rt = chi * dt * JP_FDCVCV, rt2 = 2.0 * chi - 1.0;
+        lv = rt2 * lv + rt * lf * ilm;
+    } else {
+        lv = lv + dt * JP_FDCVCV * lf * ilm;
+    }
+    L[1] = lv;
+    L[0] = L[0] + dt * lv;
+    L[6] = (L[5] + lv) * 0.5;
 }
 
 // block_temps of two partial arrays in one pass: P into sT, P2 into sT2.
@@ -129,6 +170,9 @@ __device__ __forceinline__ void block_temps2(const CanoArgs &A, const double *__
             p = p + __shfl_xor(p, o, 64);
             p2 = p2 + __shfl_xor(p2, o, 64);
         }
+        const double ke = lambda_ke(A, s, g, 6);
+        p = ke + p;
+        p2 = ke + p2;
         if (l == 0) {
             sT[g - g0] = p * 1.0e7 / (A.gdof[g] * JP_GASCST);
             sT2[g - g0] = p2 * 1.0e7 / (A.gdof[g] * JP_GASCST);
@@ -193,12 +237,16 @@ __global__ void __launch_bounds__(256) k_cano_drift(CanoArgs A, const SysP *__re
     const int a = B.x + threadIdx.x;
     int g0, g1;
     block_groups(A, B.x, g0, g1);
-    block_temps(A, A.pa, A.nab, A.ab0, A.ab1, g0, g1, sT);
+    block_temps(A, A.pa, A.nab, A.ab0, A.ab1, g0, g1, sT, 3);
     if (a >= B.y) return;
     const int s = A.atom_sys[a];
     const SysP &P = sp[s];
     const double dt = P.dt, im = A.imass[a], ts = A.temps[s];
     const double T = A.shk_on[s] ? sT[A.gid[a] - g0] : ts;   // verlethe.f90:396-404
+    if (A.lam && a == A.sfirst[s]) {
+        const int lg = A.lgrp[s];
+        alsd_move(A, s, (A.shk_on[s] && lg >= 0) ? sT[lg - g0] : ts, ts, dt);
+    }
     const double itc = 1.0 / T;
     const double chi = sqrt(ts * itc);
     const double rt = chi * dt * JP_FDCVCV, rt2 = 2.0 * chi - 1.0;
@@ -265,7 +313,11 @@ __global__ void __launch_bounds__(128) k_cano_shake(CanoArgs A, const SysP *__re
         }
     }
     if (round == 1) {
-        block_temps(A, A.pa, A.nab, A.ab0, A.ab1, g0, g1, sT);
+        block_temps(A, A.pa, A.nab, A.ab0, A.ab1, g0, g1, sT, 3);
+        if (A.lam && threadIdx.x == 0 && blockIdx.x == A.ub0[s]) {
+            const int lg = A.lgrp[s];
+            alsd_move(A, s, (A.shk_on[s] && lg >= 0) ? sT[lg - g0] : ts, ts, sp[s].dt);
+        }
     } else {
         if (threadIdx.x == 0) s_run = !A.done[s];
         __syncthreads();

@@ -21,6 +21,7 @@
 #include "jp_bonded.hpp"
 #include "jp_cano.hpp"
 #include "jp_ge.hpp"
+#include "jp_alsd.hpp"
 
 #define CK(x)                                                          \
     do {                                                               \
@@ -167,7 +168,7 @@ static void md_cell_windows(SysP &P, int nv, double siz, double rlim, std::vecto
 
 struct HostSys {
     int n = 0, ntyp = 0, nfrag = 1, nfrg = 1, lmd = 0;
-    double lambda = 0.0, dself[3] = {0, 0, 0};
+    double lambda = 0.0, lambda_v = 0.0, dself[3] = {0, 0, 0};
     int nexcl = 0;
     int n15mx = 0;
 };
@@ -182,6 +183,7 @@ struct Engine {
     std::vector<HostSys> hs;
     std::vector<SysP> sp;
     std::vector<int> atom_sys_h, slot_sys_h, cell_sys_h, clu_sys_h;
+    std::vector<int> atom_cls_h;    // cluster (icls - 1) of every atom
     std::vector<double> mass_h;
     hipStream_t st = nullptr;
     hipEvent_t ev[8];               // [0,1] run, [2,3] list build, [4,5] host-launched pair kernels
@@ -281,6 +283,23 @@ struct Engine {
     int set_mcmd(int on, int nw, int nd, const double *c, const double *low, const double *high, double alphalw,
                  double alphaup, double celw, double ceup, double tempce);
     int ge_weight(int prec);
+    // ALSD (jp_set_alsd): lambda as a dynamical variable (jp_alsd.hpp)
+    bool alsd = false;
+    int al_nw = 0, al_nd = 0, al_tctl = 1;
+    DBuf<double> d_alpar, d_lam, d_lamb, d_lamb2, d_lmass, d_dself;   // parameters, state, two backups
+    DBuf<int> d_lgrp, d_esys, d_ecls;
+    std::vector<int> lgrp_h;
+    std::vector<double> lmass_h;
+    bool cano_grouped = false;          // jp_set_canonical got temperature groups (caltmp2)
+    std::vector<double> gdof_h;         // degrees of freedom per group, without lambda's
+    std::vector<int> sg0_h, sg1_h;
+    int set_alsd(int on, int nw, int nd, const double *c, const double *low, const double *high, double lower,
+                 double upper, double celw, double ceup, double tempce, double lmass, int tctl);
+    int alsd_force(int prec);
+    int alsd_fac();
+    DBuf<double> d_lambk;               // lambda's state beside d_xbak (a deferred rebuild's redo)
+    int lam_upload();                   // host lambda and velocity -> device state
+    int lam_download();                 // device state -> host lambda and velocity
     // energy sums on the device: per-system atom and cluster ranges, results,
     // and a pinned host copy of them
     DBuf<int> d_abeg, d_aend, d_cbeg, d_cend;
@@ -337,6 +356,8 @@ struct Engine {
         d_abeg.release(); d_aend.release(); d_cbeg.release(); d_cend.release(); d_rsum.release(); d_rfix.release();
         d_part.release(); d_partf.release();
         d_gepar.release(); d_gew.release(); d_eslot.release();
+        d_alpar.release(); d_lam.release(); d_lamb.release(); d_lamb2.release(); d_lmass.release(); d_dself.release();
+        d_lgrp.release(); d_esys.release(); d_ecls.release(); d_lambk.release();
         if (h_rsum) hipHostFree(h_rsum);
         if (h_rfix) hipHostFree(h_rfix);
         if (h_err) hipHostFree(h_err);
@@ -577,6 +598,7 @@ int Engine::init(const jp_sys_desc *descs, int ns) {
             if (t < 0 || t >= nt || c < 0 || c >= d.nfrag) return JP_EINVAL;
             tflag.push_back(t | ((zt[t] ? 0 : 1) << 12) | (c << 13));
             atom_sys_h.push_back(s);
+            atom_cls_h.push_back(c);
         }
         // molecules = connected components of the excluded-pair graph
         std::vector<int> par(n);
@@ -1114,8 +1136,18 @@ int Engine::facc_zero() {
     return 0;
 }
 
+// ALSD: the class factors of the current lambda, before the forces.
+int Engine::alsd_fac() {
+    if (!alsd) return 0;
+    hipLaunchKernelGGL(k_alsd_fac, grid1(nsys, 64), 64, 0, st, nsys, (const double *)d_lam.p, d_sp.p, d_efac.p);
+    CK(hipGetLastError());
+    return 0;
+}
+
 int Engine::forces(int prec, bool energy) {
-    int r = pair(prec, energy || ge);
+    int r = alsd_fac();
+    if (r) return r;
+    r = pair(prec, energy || ge || alsd);
     if (!r) r = pair_out(prec);
     if (r) return r;
     if (nbat) {
@@ -1123,7 +1155,150 @@ int Engine::forces(int prec, bool energy) {
                            d_efac.p, d_inum.p, d_gslot.p, (double *)nullptr);
         CK(hipGetLastError());
     }
+    if (alsd) return alsd_force(prec);
     return ge ? ge_weight(prec) : 0;
+}
+
+// ALSD: the step's class-1 and class-2 energies from device sums (as
+// ge_weight), then lambda's force and half-step velocity (k_alsd).
+int Engine::alsd_force(int prec) {
+    int r;
+    hipLaunchKernelGGL(k_excess_energy, grid1(natom, 256), 256, 0, st, natom, d_atom_sys.p, d_sp.p, d_x.p, d_chg.p,
+                       d_eptr.p, d_eidx.p, d_eexc.p);
+    if ((r = seg_sum<double>(d_abeg.p, d_aend.p, 3, d_eexc.p, d_part.p, d_rsum.p + 6 * (size_t)nsys))) return r;
+    if (prec == JP_PREC_FP64) r = seg_sum<double>(d_cbeg.p, d_cend.p, 6, d_ep64.p, d_part.p, d_rsum.p);
+    else r = seg_sum<long long>(d_cbeg.p, d_cend.p, 6, d_epfix.p, d_partf.p, d_rfix.p);
+    if (r) return r;
+    const int ne = nbt + nat + ntt + 2 * n14;
+    if (ne) {
+        hipLaunchKernelGGL(k_bonded_energy, grid1(ne, 256), 256, 0, st, nbt, nat, ntt, n14, bonded_args(),
+                           d_atom_sys.p, d_sp.p, d_x.p, d_eterm.p);
+        CK(hipGetLastError());
+    }
+    AlsdArgs A;
+    A.nw = al_nw; A.nd = al_nd; A.par = d_alpar.p;
+    A.rsum = d_rsum.p;
+    A.rfix = prec == JP_PREC_FP64 ? nullptr : d_rfix.p;
+    A.nsys = nsys;
+    A.hf = (prec == JP_PREC_FP32_FIXED && list_half) ? 1.0 : 0.5;
+    A.dself = d_dself.p;
+    A.eterm = d_eterm.p; A.esys = d_esys.p; A.ecls = d_ecls.p; A.eslot = d_eslot.p; A.ne = ne;
+    A.lmass = d_lmass.p; A.lam = d_lam.p; A.sp = d_sp.p;
+    hipLaunchKernelGGL(k_alsd, nsys, 256, 0, st, A);
+    CK(hipGetLastError());
+    return 0;
+}
+
+int Engine::lam_upload() {
+    std::vector<double> L((size_t)JP_LS * nsys, 0.0);
+    for (int s = 0; s < nsys; ++s) {
+        L[(size_t)JP_LS * s] = hs[s].lambda;
+        L[(size_t)JP_LS * s + 1] = hs[s].lambda_v;
+        L[(size_t)JP_LS * s + 4] = hs[s].lambda;
+        L[(size_t)JP_LS * s + 5] = hs[s].lambda_v;
+    }
+    AL(d_lam, L.size());
+    CK(hipMemcpyAsync(d_lam.p, L.data(), 8 * L.size(), hipMemcpyHostToDevice, st));
+    CK(hipStreamSynchronize(st));
+    return 0;
+}
+
+int Engine::lam_download() {
+    std::vector<double> L((size_t)JP_LS * nsys);
+    CK(hipMemcpyAsync(L.data(), d_lam.p, 8 * L.size(), hipMemcpyDeviceToHost, st));
+    CK(hipStreamSynchronize(st));
+    for (int s = 0; s < nsys; ++s) {
+        hs[s].lambda = L[(size_t)JP_LS * s];
+        hs[s].lambda_v = L[(size_t)JP_LS * s + 1];
+    }
+    return 0;
+}
+
+// ALSD for the following runs (after jp_set_canonical; three energy
+// classes, GEPSME= LMD): the parameters of the NAMEGE= file (inmcmd.f90),
+// lambda's mass (lmass <= 0: the class-1 atoms' mass added in atom order,
+// inpcls.f90:348-357) and l_temp_control.  lambda's degree of freedom goes
+// to the one temperature group, or under temperature groups to the first
+// with the most degrees of freedom (md.f90:114-123, 221-228).
+int Engine::set_alsd(int on, int nw, int nd, const double *c, const double *low, const double *high, double lower,
+                     double upper, double celw, double ceup, double tempce, double lmass, int tctl) {
+    drop_graphs();
+    if (alsd) {
+        int r = lam_download();
+        if (r) return r;
+    }
+    alsd = false;
+    if (cano && !gdof_h.empty()) {   // degrees of freedom without lambda's
+        CK(hipMemcpyAsync(d_gdof.p, gdof_h.data(), 8 * gdof_h.size(), hipMemcpyHostToDevice, st));
+        CK(hipStreamSynchronize(st));
+    }
+    if (!on) {
+        set_factors(-1);
+        return 0;
+    }
+    if (!cano) return JP_EINVAL;
+    for (int s = 0; s < nsys; ++s)
+        if (!hs[s].lmd || hs[s].nfrg != 3) return JP_EINVAL;
+    if (nw < 1 || nd < 0 || nw > 4096 || nd > 64 || !c || !low || !high) return JP_EINVAL;
+    std::vector<double> par{lower, upper, celw, ceup, tempce};
+    for (int w = 0; w < nw; ++w) par.push_back(low[w]);
+    for (int w = 0; w < nw; ++w) par.push_back(high[w]);
+    for (size_t q = 0; q < (size_t)nw * (nd + 1); ++q) par.push_back(c[q]);
+    for (double p : par)
+        if (!std::isfinite(p)) return JP_EINVAL;
+    // lambda's mass per system
+    lmass_h.assign(nsys, 0.0);
+    for (int s = 0; s < nsys; ++s) {
+        if (lmass > 0.0) {
+            lmass_h[s] = lmass;
+        } else {
+            double m = 0.0;
+            for (int a = sp[s].atom0; a < sp[s].atom0 + hs[s].n; ++a)
+                if (atom_cls_h[a] == 0) m = m + mass_h[a];
+            lmass_h[s] = m;
+        }
+        if (!(lmass_h[s] > 0.0) || !std::isfinite(lmass_h[s])) return JP_EINVAL;
+    }
+    // lambda's temperature group and degree of freedom
+    std::vector<double> dof = gdof_h;
+    lgrp_h.assign(nsys, -1);
+    for (int s = 0; s < nsys; ++s) {
+        if (!cano_grouped) {
+            lgrp_h[s] = sg0_h[s];   // caltmp always adds lambda's kinetic term
+            if (tctl) dof[sg0_h[s]] += 1.0;
+        } else if (tctl) {
+            int g = sg0_h[s];
+            for (int q = sg0_h[s] + 1; q < sg1_h[s]; ++q)
+                if (dof[q] > dof[g]) g = q;
+            lgrp_h[s] = g;
+            dof[g] += 1.0;
+        }
+    }
+    const int ne = nbt + nat + ntt + 2 * n14;
+    std::vector<double> ds(3 * (size_t)nsys);
+    for (int s = 0; s < nsys; ++s)
+        for (int k = 0; k < 3; ++k) ds[3 * s + k] = hs[s].dself[k];
+    AL(d_alpar, par.size());
+    AL(d_lmass, nsys); AL(d_lgrp, nsys); AL(d_dself, 3 * (size_t)nsys);
+    AL(d_eslot, std::max(ne, 1)); AL(d_esys, std::max(ne, 1)); AL(d_ecls, std::max(ne, 1));
+    AL(d_lam, (size_t)JP_LS * nsys); AL(d_lamb, (size_t)JP_LS * nsys); AL(d_lamb2, (size_t)JP_LS * nsys);
+    CK(hipMemcpyAsync(d_alpar.p, par.data(), 8 * par.size(), hipMemcpyHostToDevice, st));
+    CK(hipMemcpyAsync(d_lmass.p, lmass_h.data(), 8 * (size_t)nsys, hipMemcpyHostToDevice, st));
+    CK(hipMemcpyAsync(d_lgrp.p, lgrp_h.data(), 4 * (size_t)nsys, hipMemcpyHostToDevice, st));
+    CK(hipMemcpyAsync(d_dself.p, ds.data(), 8 * ds.size(), hipMemcpyHostToDevice, st));
+    CK(hipMemcpyAsync(d_gdof.p, dof.data(), 8 * dof.size(), hipMemcpyHostToDevice, st));
+    if (ne) {
+        CK(hipMemcpyAsync(d_eslot.p, eslot_h.data(), 4 * (size_t)ne, hipMemcpyHostToDevice, st));
+        CK(hipMemcpyAsync(d_esys.p, esys_h.data(), 4 * (size_t)ne, hipMemcpyHostToDevice, st));
+        CK(hipMemcpyAsync(d_ecls.p, ecls_h.data(), 4 * (size_t)ne, hipMemcpyHostToDevice, st));
+    }
+    CK(hipStreamSynchronize(st));
+    al_nw = nw;
+    al_nd = nd;
+    al_tctl = tctl ? 1 : 0;
+    ge = false;
+    alsd = true;
+    return lam_upload();
 }
 
 // METHOD= GE: the step's class-1 potential energy from device sums (excess
@@ -1165,6 +1340,10 @@ int Engine::set_mcmd(int on, int nw, int nd, const double *c, const double *low,
     drop_graphs();
     ge = false;
     if (!on) return 0;
+    if (alsd) {
+        int r = set_alsd(0, 0, 0, nullptr, nullptr, nullptr, 0, 0, 0, 0, 0, 0, 0);
+        if (r) return r;
+    }
     if (nsys != 1 || hs[0].nfrg != 1 || hs[0].lmd) return JP_EINVAL;
     if (nw < 1 || nd < 0 || nw > 4096 || nd > 64 || !c || !low || !high) return JP_EINVAL;
     if (!(celw < ceup) || !std::isfinite(alphalw) || !std::isfinite(alphaup) || !std::isfinite(tempce))
@@ -1532,6 +1711,12 @@ int Engine::set_canonical(int on, double temps, int icslop, double fcstol, const
     }
     drop_graphs();
     cano = false;   // METHOD= MICR unless the new set-up completes
+    if (alsd) {     // ALSD needs the canonical set-up it was given with
+        int r = lam_download();
+        if (r) return r;
+        alsd = false;
+        set_factors(-1);
+    }
     if (!on) return 0;
     if (!(temps > 0.0) || icslop < 1 || !(fcstol >= 0.0)) return JP_EINVAL;
     std::vector<long long> key(natom);
@@ -1614,6 +1799,10 @@ int Engine::set_canonical(int on, double temps, int icslop, double fcstol, const
     CK(hipStreamSynchronize(st));   // the host vectors end with this call
     cano_icslop = icslop;
     cano_fcstol = fcstol;
+    cano_grouped = group != nullptr;
+    gdof_h = dof;
+    sg0_h = sg0;
+    sg1_h = sg1;
     cano = true;
     return 0;
 }
@@ -1629,6 +1818,10 @@ CanoArgs Engine::cano_args() {
     A.shk_on = d_shkon.p; A.gdof = d_gdof.p; A.temps = d_ctemps.p;
     A.mass = d_mass.p; A.imass = d_imass.p; A.gslot = d_gslot.p;
     A.gew = ge ? d_gew.p : nullptr;
+    A.lam = alsd ? d_lam.p : nullptr;
+    A.lmass = d_lmass.p;
+    A.lgrp = d_lgrp.p;
+    A.ltc = al_tctl;
     A.x = d_x.p; A.v = d_v.p; A.xold = d_xold.p; A.vback = d_vback.p; A.fcc = d_fcc.p; A.xcb = d_xcb.p;
     A.pa = d_pa.p; A.pu = d_pu.p; A.pn = d_pn.p; A.done = d_cdone.p;
     return A;
@@ -1652,10 +1845,15 @@ int Engine::cano_step() {
     return 0;
 }
 
+// x, v (and the SHAKE status) to or from a backup; under ALSD lambda's
+// state goes along (one backup per x buffer).
 int Engine::save_state(double *x, double *v, int *e) {
     CK(hipMemcpyAsync(x, d_x.p, 24 * (size_t)natom, hipMemcpyDeviceToDevice, st));
     CK(hipMemcpyAsync(v, d_v.p, 24 * (size_t)natom, hipMemcpyDeviceToDevice, st));
     if (e) CK(hipMemcpyAsync(e, d_err.p, 4, hipMemcpyDeviceToDevice, st));
+    if (alsd)
+        CK(hipMemcpyAsync(x == d_xs2.p ? d_lamb2.p : d_lamb.p, d_lam.p, 8 * (size_t)JP_LS * nsys,
+                          hipMemcpyDeviceToDevice, st));
     return 0;
 }
 
@@ -1663,6 +1861,9 @@ int Engine::load_state(const double *x, const double *v, const int *e) {
     CK(hipMemcpyAsync(d_x.p, x, 24 * (size_t)natom, hipMemcpyDeviceToDevice, st));
     CK(hipMemcpyAsync(d_v.p, v, 24 * (size_t)natom, hipMemcpyDeviceToDevice, st));
     if (e) CK(hipMemcpyAsync(d_err.p, e, 4, hipMemcpyDeviceToDevice, st));
+    if (alsd)
+        CK(hipMemcpyAsync(d_lam.p, x == d_xs2.p ? d_lamb2.p : d_lamb.p, 8 * (size_t)JP_LS * nsys,
+                          hipMemcpyDeviceToDevice, st));
     return refresh_sorted();
 }
 
@@ -1747,15 +1948,17 @@ int Engine::restart_step(int prec, jp_energy_log *log) {
     if ((r = set_factors(-1))) return r;
     if ((r = refresh_sorted())) return r;
     const bool retry = cano_retry();
+    if (alsd && (r = lam_upload())) return r;
     CK(hipMemsetAsync(d_err.p, 0, 4, st));
     CK(hipMemcpyAsync(d_xbak.p, d_x.p, 24 * (size_t)natom, hipMemcpyDeviceToDevice, st));
-    if ((r = pair(prec, true)) || (r = pair_out(prec))) return r;
+    if ((r = alsd_fac()) || (r = pair(prec, true)) || (r = pair_out(prec))) return r;
     if (nbat) {
         hipLaunchKernelGGL(k_bonded, grid1(nbat, 128), 128, 0, st, bonded_args(), d_atom_sys.p, d_sp.p, d_x.p,
                            d_efac.p, d_inum.p, d_gslot.p, (double *)nullptr);
         CK(hipGetLastError());
     }
     if (ge && (r = ge_weight(prec))) return r;
+    if (alsd && (r = alsd_force(prec))) return r;
     std::vector<double> eel, evd, eex, et;
     if ((r = energies(prec, eel, evd, eex)) || (r = bonded_energies(et))) return r;
     if (retry && (r = save_state(d_xs.p, d_vs.p, d_errs.p))) return r;
@@ -1777,6 +1980,7 @@ int Engine::restart_step(int prec, jp_energy_log *log) {
     CK(hipMemsetAsync(d_err.p, 0, 4, st));
     CK(hipStreamSynchronize(st));
     if ((r = refresh_sorted())) return r;
+    if (alsd && (r = lam_upload())) return r;   // lambda and its velocity back too (md.f90:1255-1258)
     if (rl) return rl;
     if (err & 8) return JP_IER_SHAKE_SING;
     if (err & 4) return JP_IER_SHAKE_CONV;
@@ -1837,20 +2041,35 @@ int Engine::mon_record(int step, int nsteps, int nmon, int nlog, const std::vect
     CK(hipMemcpyAsync(h_rsum + 9 * (size_t)nsys, d_rsum.p + 9 * (size_t)nsys, sizeof(double) * nsys,
                       hipMemcpyDeviceToHost, st));
     CK(hipMemcpyAsync(h_err, d_err.p, 4, hipMemcpyDeviceToHost, st));
+    std::vector<double> lam;
+    if (alsd) {
+        lam.resize((size_t)JP_LS * nsys);
+        CK(hipMemcpyAsync(lam.data(), d_lam.p, 8 * lam.size(), hipMemcpyDeviceToHost, st));
+    }
     CK(hipStreamSynchronize(st));
     const int err = *h_err;
     if (err & 8) return JP_IER_SHAKE_SING;
     if (err & 4) return JP_IER_SHAKE_CONV;
     for (int s = 0; s < nsys; ++s) {
         double e = h_rsum[9 * (size_t)nsys + s];
+        // ALSD: caltmp starts with lambda's term, lambda_vt the mean of its
+        // velocity before and after the step (md.f90:628-633)
+        const double *ls = alsd ? lam.data() + (size_t)JP_LS * s : nullptr;
+        if (ls) {
+            const double lvt = 0.5 * (ls[5] + ls[1]);
+            e = lmass_h[s] * lvt * lvt + e;
+        }
         // caltmp.f90:24-43; its factor 0.0005 is a single-precision literal
         e = e * 1.0e7 / 4.184 * (double)0.0005f;
         jp_energy_log &L = log[(size_t)s * (nsteps / nmon + 1) + nlog];
         L.step = step;
         L.ekin = e;
+        L.lambda = ls ? ls[0] : hs[s].lambda;
+        L.lambda_v = ls ? ls[1] : 0.0;
+        const double lam_e = ls ? ls[4] : hs[s].lambda;   // the lambda the step's energies were evaluated at
         double ep = 0.0;
         for (int k = 0; k < 3; ++k) {
-            const double f = (hs[s].lmd && hs[s].nfrg == 3) ? (k == 0 ? hs[s].lambda * hs[s].lambda : (k == 1 ? hs[s].lambda : 1.0)) : 1.0;
+            const double f = (hs[s].lmd && hs[s].nfrg == 3) ? (k == 0 ? lam_e * lam_e : (k == 1 ? lam_e : 1.0)) : 1.0;
             L.eelec[k] = k < hs[s].nfrg ? eel[3 * s + k] : 0.0;
             L.evdw[k] = k < hs[s].nfrg ? evd[3 * s + k] : 0.0;
             const double *b = et.data() + (3 * (size_t)s + k) * 6;   // bond, angle, torsion, improper, 1-4 vdW/elec
@@ -1881,6 +2100,7 @@ int Engine::run_nve(int nsteps, int update, int nmon, int prec, jp_energy_log *l
     if ((r = set_factors(-1))) return r;
     if ((r = refresh_sorted())) return r;
     if (autotune && !tuned && (r = tune(prec, update))) return r;
+    if (alsd && (r = lam_upload())) return r;
     CK(hipMemsetAsync(d_err.p, 0, 4, st));
     t_pair = 0; t_step = 0; n_pair = 0; n_pair_steps = nsteps;
     pe_pending = false;
@@ -1919,6 +2139,11 @@ int Engine::run_nve(int nsteps, int update, int nmon, int prec, jp_energy_log *l
                     CK(hipMemcpyAsync(d_xbak.p, d_x.p, 24 * (size_t)natom, hipMemcpyDeviceToDevice, st));
                     CK(hipMemcpyAsync(d_vbak.p, d_v.p, 24 * (size_t)natom, hipMemcpyDeviceToDevice, st));
                     CK(hipMemcpyAsync(d_errbak.p, d_err.p, 4, hipMemcpyDeviceToDevice, st));
+                    if (alsd) {
+                        AL(d_lambk, (size_t)JP_LS * nsys);
+                        CK(hipMemcpyAsync(d_lambk.p, d_lam.p, 8 * (size_t)JP_LS * nsys, hipMemcpyDeviceToDevice,
+                                          st));
+                    }
                 }
                 CK(hipGraphLaunch(g->exec, st));
                 if (pending) {
@@ -1929,6 +2154,9 @@ int Engine::run_nve(int nsteps, int update, int nmon, int prec, jp_energy_log *l
                         CK(hipMemcpyAsync(d_x.p, d_xbak.p, 24 * (size_t)natom, hipMemcpyDeviceToDevice, st));
                         CK(hipMemcpyAsync(d_v.p, d_vbak.p, 24 * (size_t)natom, hipMemcpyDeviceToDevice, st));
                         CK(hipMemcpyAsync(d_err.p, d_errbak.p, 4, hipMemcpyDeviceToDevice, st));
+                        if (alsd)
+                            CK(hipMemcpyAsync(d_lam.p, d_lambk.p, 8 * (size_t)JP_LS * nsys,
+                                              hipMemcpyDeviceToDevice, st));
                         if ((r = build_list())) return r;
                         continue;   // same segment again, on the new list
                     }
@@ -1966,8 +2194,9 @@ int Engine::run_nve(int nsteps, int update, int nmon, int prec, jp_energy_log *l
             t_pair += m2 * 1e-3;
             pe_pending = false;
         }
+        if ((r = alsd_fac())) return r;
         CK(hipEventRecord(ev[4], st));
-        if ((r = pair(prec, mon || ge))) return r;
+        if ((r = pair(prec, mon || ge || alsd))) return r;
         CK(hipEventRecord(ev[5], st));
         pe_pending = true;
         if ((r = pair_out(prec))) return r;
@@ -1977,6 +2206,7 @@ int Engine::run_nve(int nsteps, int update, int nmon, int prec, jp_energy_log *l
             CK(hipGetLastError());
         }
         if (ge && (r = ge_weight(prec))) return r;
+        if (alsd && (r = alsd_force(prec))) return r;
         ++n_pair;
         std::vector<double> eel, evd, eex, et;
         if (mon && (r = energies(prec, eel, evd, eex))) return r;
@@ -2019,6 +2249,7 @@ int Engine::run_nve(int nsteps, int update, int nmon, int prec, jp_energy_log *l
     t_step = t_total - (n_pair ? t_pair / n_pair * nsteps : 0.0) - tl;
     t_list = tl;
     captures_last_run = n_captures - cap0;
+    if (alsd && (r = lam_download())) return r;
     if (err & 8) return JP_IER_SHAKE_SING;
     if (err & 4) return JP_IER_SHAKE_CONV;
     return 0;
@@ -2435,6 +2666,60 @@ int jp_set_mcmd(jp_ctx *ctx, int32_t on, int32_t nwindow, int32_t ndeg, const do
                 const double *high, double alphalw, double alphaup, double celw, double ceup, double tempce) {
     if (!ctx) return JP_EINVAL;
     return ctx->e.set_mcmd(on, nwindow, ndeg, c, low, high, alphalw, alphaup, celw, ceup, tempce);
+}
+
+int jp_set_alsd(jp_ctx *ctx, int32_t on, int32_t nwindow, int32_t ndeg, const double *c, const double *low,
+                const double *high, double lower, double upper, double celw, double ceup, double tempce,
+                double lambda_m, int32_t temp_control) {
+    if (!ctx) return JP_EINVAL;
+    return ctx->e.set_alsd(on, nwindow, ndeg, c, low, high, lower, upper, celw, ceup, tempce, lambda_m, temp_control);
+}
+
+int jp_batch_set_alsd(jp_batch *b, int32_t on, int32_t nwindow, int32_t ndeg, const double *c, const double *low,
+                      const double *high, double lower, double upper, double celw, double ceup, double tempce,
+                      double lambda_m, int32_t temp_control) {
+    if (!b) return JP_EINVAL;
+    return b->e.set_alsd(on, nwindow, ndeg, c, low, high, lower, upper, celw, ceup, tempce, lambda_m, temp_control);
+}
+
+static int get_lambda(Engine &E, int s, double *lambda, double *lambda_v) {
+    if (s < 0 || s >= E.nsys) return JP_EINVAL;
+    if (lambda) *lambda = E.hs[s].lambda;
+    if (lambda_v) *lambda_v = E.hs[s].lambda_v;
+    return 0;
+}
+
+static int set_lambda(Engine &E, int s, double lambda, double lambda_v) {
+    if (s < 0 || s >= E.nsys || !std::isfinite(lambda) || !std::isfinite(lambda_v)) return JP_EINVAL;
+    if (E.alsd) {
+        int r = E.lam_download();   // the other systems' state
+        if (r) return r;
+    }
+    E.hs[s].lambda = lambda;
+    E.hs[s].lambda_v = lambda_v;
+    int r = E.set_factors(-1);
+    if (!r && E.alsd) r = E.lam_upload();
+    return r;
+}
+
+int jp_get_lambda(jp_ctx *ctx, double *lambda, double *lambda_v) {
+    if (!ctx) return JP_EINVAL;
+    return get_lambda(ctx->e, 0, lambda, lambda_v);
+}
+
+int jp_set_lambda(jp_ctx *ctx, double lambda, double lambda_v) {
+    if (!ctx) return JP_EINVAL;
+    return set_lambda(ctx->e, 0, lambda, lambda_v);
+}
+
+int jp_batch_get_lambda(jp_batch *b, int32_t isys, double *lambda, double *lambda_v) {
+    if (!b) return JP_EINVAL;
+    return get_lambda(b->e, isys, lambda, lambda_v);
+}
+
+int jp_batch_set_lambda(jp_batch *b, int32_t isys, double lambda, double lambda_v) {
+    if (!b) return JP_EINVAL;
+    return set_lambda(b->e, isys, lambda, lambda_v);
 }
 
 int jp_set_canonical(jp_ctx *ctx, int32_t on, double temps, int32_t icslop, double fcstol, const int32_t *group) {

@@ -196,6 +196,22 @@ class MdSystem:
             self.ctx, 1, c.shape[0], c.shape[1] - 1, _f64(c), _f64(low), _f64(high), float(ge["alphalw"]),
             float(ge["alphaup"]), float(ge["celw"]), float(ge["ceup"]), float(ge["tempce"])))
 
+    def set_alsd(self, al: Optional[dict]):
+        """ALSD (METHOD= GE under GEPSME= LMD) on top of set_canonical: al
+        holds c (nwindow, ndeg+1), low, high (nwindow), lower, upper, celw,
+        ceup, tempce (the NAMEGE= file), lmass (LWEIGH; <= 0: md_run's
+        default) and tctl (LTCNTL); None switches it off."""
+        self._al = _alsd_call(self.lib.jp_set_alsd, "jp_set_alsd", self.ctx, al)
+
+    def get_lambda(self):
+        """(lambda, its velocity)."""
+        lam, lv = C.c_double(), C.c_double()
+        native.check("jp_get_lambda", self.lib.jp_get_lambda(self.ctx, C.byref(lam), C.byref(lv)))
+        return lam.value, lv.value
+
+    def set_lambda(self, lam: float, lam_v: float = 0.0):
+        native.check("jp_set_lambda", self.lib.jp_set_lambda(self.ctx, float(lam), float(lam_v)))
+
     def set_canonical(self, temps: Optional[float] = 300.0, icslop: int = 3, fcstol: float = 1e-3,
                       tempco: bool = False):
         """METHOD= CANO for the following runmd calls (SETTEM = temps; TEMPCO=
@@ -264,15 +280,29 @@ def equilibrate_ensemble(ens: "Ensemble", nsteps: int, temp: float = 300.0, ever
     return 0
 
 
+def _alsd_call(fn, name, handle, al):
+    if al is None:
+        native.check(name, fn(handle, 0, 0, 0, None, None, None, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0))
+        return None
+    c = np.ascontiguousarray(al["c"], np.float64)
+    low = np.ascontiguousarray(al["low"], np.float64)
+    high = np.ascontiguousarray(al["high"], np.float64)
+    native.check(name, fn(handle, 1, c.shape[0], c.shape[1] - 1, _f64(c), _f64(low), _f64(high),
+                          float(al["lower"]), float(al["upper"]), float(al["celw"]), float(al["ceup"]),
+                          float(al["tempce"]), float(al.get("lmass", 0.0)), int(bool(al.get("tctl", True)))))
+    return (c, low, high)
+
+
 def log_to_array(log, nl: int) -> np.ndarray:
     dt = np.dtype([("step", "i4"), ("etot", "f8"), ("epot", "f8"), ("ekin", "f8"),
                    ("eelec", "f8", 3), ("evdw", "f8", 3), ("ebond", "f8", 3), ("eangle", "f8", 3),
-                   ("etors", "f8", 3), ("eimp", "f8", 3), ("evdw14", "f8", 3), ("eelec14", "f8", 3)])
+                   ("etors", "f8", 3), ("eimp", "f8", 3), ("evdw14", "f8", 3), ("eelec14", "f8", 3),
+                   ("lambda", "f8"), ("lambda_v", "f8")])
     out = np.zeros(nl, dt)
     for i in range(nl):
         L = log[i]
         out[i] = (L.step, L.etot, L.epot, L.ekin, tuple(L.eelec), tuple(L.evdw), tuple(L.ebond), tuple(L.eangle),
-                  tuple(L.etors), tuple(L.eimp), tuple(L.evdw14), tuple(L.eelec14))
+                  tuple(L.etors), tuple(L.eimp), tuple(L.evdw14), tuple(L.eelec14), L.lambda_, L.lambda_v)
     return out
 
 
@@ -337,6 +367,18 @@ class Ensemble:
 
     def set_option(self, option: int, value: int):
         native.check("jp_batch_set_option", self.lib.jp_batch_set_option(self.b, option, int(value)))
+
+    def set_alsd(self, al: Optional[dict]):
+        """ALSD for every system of the batch (see MdSystem.set_alsd)."""
+        self._al = _alsd_call(self.lib.jp_batch_set_alsd, "jp_batch_set_alsd", self.b, al)
+
+    def get_lambda(self, i: int):
+        lam, lv = C.c_double(), C.c_double()
+        native.check("jp_batch_get_lambda", self.lib.jp_batch_get_lambda(self.b, i, C.byref(lam), C.byref(lv)))
+        return lam.value, lv.value
+
+    def set_lambda(self, i: int, lam: float, lam_v: float = 0.0):
+        native.check("jp_batch_set_lambda", self.lib.jp_batch_set_lambda(self.b, i, float(lam), float(lam_v)))
 
     def set_canonical(self, temps: Optional[float] = 300.0, icslop: int = 3, fcstol: float = 1e-3,
                       tempco: bool = False):

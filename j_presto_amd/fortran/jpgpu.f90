@@ -75,6 +75,7 @@ module jpgpu
      real(c_double) :: etot, epot, ekin
      real(c_double) :: eelec(3), evdw(3)
      real(c_double) :: ebond(3), eangle(3), etors(3), eimp(3), evdw14(3), eelec14(3)
+     real(c_double) :: lambda, lambda_v
   end type jp_energy_log
 
   interface

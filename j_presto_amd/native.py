@@ -46,6 +46,14 @@ SIGNATURES = {
     "jp_set_mcmd": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_int32, f64p, f64p, f64p, C.c_double, C.c_double,
                               C.c_double, C.c_double, C.c_double]),
     "jp_batch_set_canonical": (C.c_int, [vp, C.c_int32, C.c_double, C.c_int32, C.c_double, i32p]),
+    "jp_set_alsd": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_int32, f64p, f64p, f64p, C.c_double, C.c_double,
+                              C.c_double, C.c_double, C.c_double, C.c_double, C.c_int32]),
+    "jp_batch_set_alsd": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_int32, f64p, f64p, f64p, C.c_double,
+                                    C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int32]),
+    "jp_get_lambda": (C.c_int, [vp, f64p, f64p]),
+    "jp_set_lambda": (C.c_int, [vp, C.c_double, C.c_double]),
+    "jp_batch_get_lambda": (C.c_int, [vp, C.c_int32, f64p, f64p]),
+    "jp_batch_set_lambda": (C.c_int, [vp, C.c_int32, C.c_double, C.c_double]),
     "jp_batch_create": (C.c_int, [C.POINTER(SysDesc), C.c_int32, C.POINTER(vp)]),
     "jp_batch_run_nve": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.POINTER(EnergyLog)]),
     "jp_batch_set_state": (C.c_int, [vp, C.c_int32, f64p, f64p]),
