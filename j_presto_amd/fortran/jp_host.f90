@@ -11,9 +11,13 @@
 !                 COORDI with CRDFMT= PDB (default) or BINA, SETSHK, and CLUSTA= a<b (atoms
 !                 a..b in cluster 1, the rest in cluster 2) with GEPSME= LMD,
 !                 read by jp_input.f90.  EXE> MD: LOOPLI, UPDATE, TIMEST,
-!                 METHOD (MICR or CANO; required), SETTEM, TEMPCO, OUTLOG,
-!                 RESTAR, NAMERI, NAMERO, NAMETO, CUTLEN, DIEVAL, XMINCL..ZMAXCL,
-!                 COVSHK, LIMSHK (inpmd.f90; defaults of main.f90:150-185).
+!                 METHOD (MICR, CANO or GE; required), SETTEM, TEMPCO, OUTLOG,
+!                 RESTAR, NAMERI, NAMERO, NAMETO, NAMEGE, LAMBDA, LMDSQU, CUTLEN,
+!                 DIEVAL, XMINCL..ZMAXCL, COVSHK, LIMSHK (inpmd.f90; defaults of
+!                 main.f90:150-185).  METHOD= GE reads the NAMEGE= parameter
+!                 file (inmcmd.f90): multicanonical MD (jp_set_mcmd), or under
+!                 GEPSME= LMD adaptive lambda square dynamics (jp_set_alsd, with
+!                 EXE> INPUT's LWEIGH and LTCNTL).
 !                 NAMETO receives the potential energy of every step per
 !                 energy class in md_run's format (md.f90:578-601).
 !                 x, v and lambda from the NAMERI restart, the final state
@@ -39,7 +43,8 @@
 !   p14_atoms(2,n14) p14_vdw(n14) p14_elec(n14)
 ! Output file (same record format):
 !   ier(9): create, build_list, nonbond FP64, nonbond FP32, run_nve, get_state, bonded (x2),
-!           set_canonical (with --mdinp, ier(7) is the restart step's)
+!           set_canonical (with --mdinp, ier(7) is the restart step's and
+!           ier(8) METHOD= GE's set-up: jp_set_mcmd or jp_set_alsd)
 !   grad64(3,N,nfrg) eelec64(nfrg) evdw64(nfrg) grad32(3,N,nfrg) eelec32(nfrg) evdw32(nfrg)
 !   eterm(6,nfrg) = enevcc(2:7,:) (bond, angle, torsion, improper, 1-4 vdW, 1-4 elec)
 !   cord(3,N) vel(3,N) after the run, nlog, then per record: step etot epot ekin
@@ -69,6 +74,14 @@ program jp_host
   character(len=80) :: title
   integer(c_int32_t) :: rn, rnvar, rloop
   real(c_double) :: rtime, ret, rek, rep, rlam, rlamv
+  ! METHOD= GE: the NAMEGE= file (inmcmd.f90:20-41); LAMBDA/LMDSQU; LWEIGH, LTCNTL
+  logical :: gemode
+  character(len=1024) :: gef
+  integer :: ge_nw, ge_nd
+  real(c_double), allocatable, target :: ge_c(:,:), ge_low(:), ge_high(:)
+  real(c_double) :: ge_alw, ge_aup, ge_celw, ge_ceup, ge_tempce, ge_lower, ge_upper
+  real(c_double) :: lweigh, lamkw, lsqkw, lamv0
+  integer(c_int32_t) :: ltcntl
   real(c_double), allocatable, target :: cord(:,:), vel(:,:), mass(:), charge(:), rmin2(:), eps(:)
   real(c_double), allocatable, target :: sdist(:,:), grad64(:,:,:), grad32(:,:,:)
   real(c_double), target :: eel64(3), evd64(3), ehy64(3), eel32(3), evd32(3), ehy32(3), eterm(6,3)
@@ -161,9 +174,18 @@ program jp_host
 
   if (mdmode) then
      desc%dt = timest
+     ! lambda (inpmd.f90:290-297: LAMBDA, else sqrt(LMDSQU); inprst.f90:56-66:
+     ! the restart's unless set, its velocity only under METHOD= GE)
+     if (lmdin .and. lamkw /= 0.0d0) then
+        desc%lambda = lamkw
+     else if (lmdin .and. lsqkw /= 0.0d0) then
+        desc%lambda = sqrt(lsqkw)
+     end if
+     lamv0 = 0.0d0
      if (restar) then
         call read_restart(trim(rstin))
-        if (fromtpl) desc%lambda = rlam
+        if (fromtpl .and. desc%lambda == 0.0d0) desc%lambda = rlam
+        if (gemode) lamv0 = rlamv
      end if
      if (tplsys) then
         call write_system_file()
@@ -288,6 +310,7 @@ contains
     tplf = ' '; crdf = ' '; shkf = ' '; crdfmt = 'PDB'; cla = 0; clb = 0; lmdin = .false.
     cutlen = 12.0d0; dieval = 1.0d0; box = 0.0d0; covshk = 1.0d-6; limshk = 1000
     cellvl = 3; cputim = 99999999.0d0; initzero = .false.
+    gemode = .false.; gef = ' '; lweigh = 0.0d0; ltcntl = 1; lamkw = 0.0d0; lsqkw = 0.0d0
     have_method = .false.; have_zd = .false.; have_pbc = .false.
     inmd = .false.; inin = .false.
     open(newunit=uu, file=path, status='old', action='read')
@@ -335,8 +358,16 @@ contains
                 if (ios == 0 .and. (cellvl < 1 .or. cellvl > 6)) call refuse(key, val, 'cell levels 1-6 only')
              case ('SCALET')
                 if (trim(val) /= '0030') call refuse(key, val, 'only the default class choice 0030 is implemented')
-             case ('LWEIGH', 'LTCNTL')
-                ! lambda mass and temperature control: used by lambda dynamics only
+             case ('LWEIGH')   ! lambda's mass (0: the class-1 atoms' mass), ALSD only
+                read(val, *, iostat=ios) lweigh
+             case ('LTCNTL')   ! lambda's temperature control, ALSD only (inpinp.f90:178-182)
+                if (trim(val) == 'NO') then
+                   ltcntl = 0
+                else if (trim(val) == 'YES') then
+                   ltcntl = 1
+                else
+                   call refuse(key, val, 'NO or YES')
+                end if
              case ('REFCOO', 'OUTMON')
                 write(*,'(3a)') 'jp_host: ', trim(key), ' (monitoring only) is not written by this host'
              case ('SETVAR', 'SETBOU', 'POSITI', 'DISTAN', 'DIHEDR', 'SETORI', 'EXTCMM', 'REPULS', 'CELSIZ', &
@@ -376,8 +407,9 @@ contains
           case ('PSETMP')
              read(val, *, iostat=ios) rv
              if (ios == 0 .and. rv > 1.0d-10) call refuse(key, val, 'pseudo-temperature scaling is not implemented')
-          case ('LAMBDA', 'LMDSQU')
-             call refuse(key, val, 'lambda dynamics input is not implemented')
+          case ('LAMBDA'); read(val, *, iostat=ios) lamkw   ! inpmd.f90:290-297
+          case ('LMDSQU'); read(val, *, iostat=ios) lsqkw
+          case ('NAMEGE'); gef = val
           case ('LXCELL', 'LYCELL', 'LZCELL', 'SETCEN', 'CENTRX', 'CENTRY', 'CENTRZ')
              call refuse(key, val, 'give the box as XMINCL..ZMAXCL')
           case ('CPUTIM'); read(val, *, iostat=ios) cputim
@@ -397,7 +429,7 @@ contains
              ! thread layout and SIMD width, log format, methods and boundaries
              ! this run does not use (CONS relaxation, random initial
              ! velocities, CMM, CAP, ellipsoid): no effect on a ZD/PBC run
-          case ('NAMECO', 'NAMEVE', 'NAMEEN', 'NAMETR', 'NAMECL', 'NAMEGE', 'OUTCOO', 'OUTVEL', 'OUTENE', &
+          case ('NAMECO', 'NAMEVE', 'NAMEEN', 'NAMETR', 'NAMECL', 'OUTCOO', 'OUTVEL', 'OUTENE', &
                 'OUTTRJ', 'OUTCLE', 'MNTRCO', 'MNTRVE', 'MNTREN', 'MNTRTR', 'MNTRCL', 'FDEBUG')
              write(*,'(3a)') 'jp_host: ', trim(key), ' (trajectory/monitoring output) is not written by this host'
           case ('LOOPLI'); read(val, *, iostat=ios) nsteps
@@ -424,10 +456,13 @@ contains
              have_method = .true.
              if (val(1:4) == 'CANO') then
                 settem = abs(settem)
+             else if (val(1:2) == 'GE') then
+                settem = abs(settem)
+                gemode = .true.
              else if (val(1:4) == 'MICR') then
                 settem = -abs(settem)
              else
-                call refuse(key, val, 'only METHOD= MICR and CANO are implemented')
+                call refuse(key, val, 'only METHOD= MICR, CANO and GE are implemented')
              end if
           case default
              call refuse(key, val, 'not an EXE> MD keyword of md_run')
@@ -437,8 +472,15 @@ contains
     end do
     close(uu)
     if (.not. have_method) then
-       write(*,'(a)') 'jp_host: METHOD= MICR or CANO must be given in EXE> MD'
+       write(*,'(a)') 'jp_host: METHOD= MICR, CANO or GE must be given in EXE> MD'
        stop 3
+    end if
+    if (gemode) then
+       if (len_trim(gef) == 0) then
+          write(*,'(a)') 'jp_host: METHOD= GE needs its parameter file (NAMEGE=)'
+          stop 3
+       end if
+       call read_namege(trim(gef))
     end if
     ! md_run's defaults are CMM electrostatics and no boundary (main.f90:149,170)
     if (.not. have_zd) then
@@ -459,6 +501,45 @@ contains
        stop 3
     end if
   end subroutine read_mdinp
+
+  ! inmcmd.f90:20-41: nwindow ndeg; per window ndeg+1 coefficients, one per
+  ! record, then low_v high_v; alphalw alphaup; celw ceup; tempce; under
+  ! GEPSME= LMD also lower upper.  List-directed reads, as md_run's.
+  subroutine read_namege(path)
+    character(len=*), intent(in) :: path
+    integer :: ug, ios, i, j
+    open(newunit=ug, file=path, status='old', action='read', iostat=ios)
+    if (ios /= 0) then
+       write(*,'(2a)') 'jp_host: cannot open the NAMEGE= file ', path
+       stop 3
+    end if
+    read(ug, *, iostat=ios) ge_nw, ge_nd
+    if (ios /= 0 .or. ge_nw < 1 .or. ge_nd < 0) call bad_ge()
+    allocate(ge_c(0:ge_nd, ge_nw), ge_low(ge_nw), ge_high(ge_nw))
+    do i = 1, ge_nw
+       do j = 0, ge_nd
+          read(ug, *, iostat=ios) ge_c(j, i)
+          if (ios /= 0) call bad_ge()
+       end do
+       read(ug, *, iostat=ios) ge_low(i), ge_high(i)
+       if (ios /= 0) call bad_ge()
+    end do
+    read(ug, *, iostat=ios) ge_alw, ge_aup
+    if (ios == 0) read(ug, *, iostat=ios) ge_celw, ge_ceup
+    if (ios == 0) read(ug, *, iostat=ios) ge_tempce
+    if (ios /= 0) call bad_ge()
+    ge_lower = 0.0d0; ge_upper = 0.0d0
+    if (lmdin) then
+       read(ug, *, iostat=ios) ge_lower, ge_upper
+       if (ios /= 0) call bad_ge()
+    end if
+    close(ug)
+  contains
+    subroutine bad_ge()
+      write(*,'(2a)') 'jp_host: READ FILE ERROR IN the NAMEGE= file ', path
+      stop 3
+    end subroutine bad_ge
+  end subroutine read_namege
 
   subroutine refuse(key, val, why)
     character(len=*), intent(in) :: key, val, why
@@ -503,6 +584,21 @@ contains
           ier(9) = jp_set_canonical(ctx, 1_c_int32_t, settem, 3_c_int32_t, 1.0d-3, c_loc(grp))
        else
           ier(9) = jp_set_canonical(ctx, 1_c_int32_t, settem, 3_c_int32_t, 1.0d-3, c_null_ptr)
+       end if
+    end if
+    if (gemode .and. ier(9) == 0) then
+       if (lmdin) then   ! ALSD (verlethe.f90:235-290)
+          ier(8) = jp_set_alsd(ctx, 1_c_int32_t, int(ge_nw, c_int32_t), int(ge_nd, c_int32_t), c_loc(ge_c), &
+                               c_loc(ge_low), c_loc(ge_high), ge_lower, ge_upper, ge_celw, ge_ceup, ge_tempce, &
+                               lweigh, ltcntl)
+          if (ier(8) == 0) ier(8) = jp_set_lambda(ctx, desc%lambda, lamv0)
+       else              ! multicanonical MD (verlethe.f90:196-233)
+          ier(8) = jp_set_mcmd(ctx, 1_c_int32_t, int(ge_nw, c_int32_t), int(ge_nd, c_int32_t), c_loc(ge_c), &
+                               c_loc(ge_low), c_loc(ge_high), ge_alw, ge_aup, ge_celw, ge_ceup, ge_tempce)
+       end if
+       if (ier(8) /= 0) then
+          write(*,'(a,i0)') 'jp_host: the METHOD= GE set-up failed, ier = ', ier(8)
+          stop 1
        end if
     end if
     ! NAMETO needs the energies of every step
@@ -563,7 +659,7 @@ contains
           epc = elog(kk)%ebond + elog(kk)%eangle + elog(kk)%etors + elog(kk)%eimp + elog(kk)%evdw14 &
                 + elog(kk)%eelec14 + elog(kk)%evdw + elog(kk)%eelec
           if (desc%lmd /= 0) then
-             write(uu, '(2(e15.7,x))', advance='no') desc%lambda, epc(1)
+             write(uu, '(2(e15.7,x))', advance='no') elog(kk)%lambda, epc(1)
              if (nfrg == 1) then
                 write(uu, *)
              else
@@ -592,7 +688,9 @@ contains
        write(uu) rloop + ndone, rtime + timest * ndone, ret, rek, rep
        write(uu) cord
        write(uu) vel
-       write(uu) desc%lambda, 0.0d0
+       ier(8) = jp_get_lambda(ctx, rlam, rlamv)
+       if (.not. gemode) rlamv = 0.0d0
+       write(uu) rlam, rlamv
        close(uu)
     end if
     if (any(ier /= 0)) stop 1

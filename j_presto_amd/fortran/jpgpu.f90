@@ -181,6 +181,34 @@ module jpgpu
        real(c_double), value :: temps, fcstol
      end function jp_set_canonical
 
+     integer(c_int) function jp_set_mcmd(ctx, on, nwindow, ndeg, c, low, high, alphalw, alphaup, celw, ceup, &
+          tempce) bind(C, name='jp_set_mcmd')
+       import :: c_int, c_int32_t, c_ptr, c_double
+       type(c_ptr), value :: ctx, c, low, high
+       integer(c_int32_t), value :: on, nwindow, ndeg
+       real(c_double), value :: alphalw, alphaup, celw, ceup, tempce
+     end function jp_set_mcmd
+
+     integer(c_int) function jp_set_alsd(ctx, on, nwindow, ndeg, c, low, high, lower, upper, celw, ceup, &
+          tempce, lambda_m, temp_control) bind(C, name='jp_set_alsd')
+       import :: c_int, c_int32_t, c_ptr, c_double
+       type(c_ptr), value :: ctx, c, low, high
+       integer(c_int32_t), value :: on, nwindow, ndeg, temp_control
+       real(c_double), value :: lower, upper, celw, ceup, tempce, lambda_m
+     end function jp_set_alsd
+
+     integer(c_int) function jp_get_lambda(ctx, lambda, lambda_v) bind(C, name='jp_get_lambda')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out) :: lambda, lambda_v
+     end function jp_get_lambda
+
+     integer(c_int) function jp_set_lambda(ctx, lambda, lambda_v) bind(C, name='jp_set_lambda')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), value :: lambda, lambda_v
+     end function jp_set_lambda
+
      integer(c_int) function jp_batch_set_canonical(b, on, temps, icslop, fcstol, group) &
           bind(C, name='jp_batch_set_canonical')
        import :: c_int, c_int32_t, c_ptr, c_double

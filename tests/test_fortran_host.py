@@ -189,13 +189,13 @@ _RECORDS = ("hdr rhdr x v mass charge atype icls rmin2 eps nmat excl excess snat
             "afc ath tatm tfc trot tph tdiv patm pv pe").split()
 
 
-def _tpl_case(d, s, md):
+def _tpl_case(d, s, md, inp=None):
     refio.write_restart(os.path.join(d, "in.rst"), s.x, s.v, lam=s.lam)
     files = refio.write_inputs(d, s)
     base = {"RESTAR": "YES", "NAMERI": "in.rst", "NAMERO": "out.rst", "UPDATE": "10", "TIMEST": "2.0",
             "OUTLOG": "100000", "OUTCLE": "100000"}
     base.update(md)
-    refio.write_md_inp(os.path.join(d, "md.inp"), files, s, base)
+    refio.write_md_inp(os.path.join(d, "md.inp"), files, s, base, inp=inp)
 
 
 @pytest.mark.parametrize("name", ["water_1500", "chain_3k", "chain_lmd", "chain_lmd_bonded", "chain_tors",
@@ -345,9 +345,11 @@ def test_mdinp_refuses_what_it_does_not_run(tmp_path):
         "HEATLO= 1000": md_line(inp, "HEATLO", "1000"),
         "SHAKEM= ALLB": md_line(inp, "SHAKEM", "ALLB"),
         "PSETMP= 300.0": md_line(inp, "PSETMP", "300.0"),
-        "LAMBDA= lam.dat": md_line(inp, "LAMBDA", "lam.dat"),
+        "cannot read the value of LAMBDA": md_line(inp, "LAMBDA", "lam.dat"),
+        "METHOD= GE needs its parameter file": inp.replace("METHOD= MICR", "METHOD= GE"),
+        "NAMEGE= file": md_line(inp.replace("METHOD= MICR", "METHOD= GE"), "NAMEGE", "missing.par"),
+        "LTCNTL= MAYBE": inp.replace("QUIT\n\nEXE> MD", "      LTCNTL= MAYBE\nQUIT\n\nEXE> MD"),
         "LXCELL= 14.0": md_line(inp, "LXCELL", "14.0"),
-        "METHOD= GE": inp.replace("METHOD= MICR", "METHOD= GE"),
         "MNTRTO= SING": md_line(inp, "MNTRTO", "SING"),
         "FOOBAR= 1": md_line(inp, "FOOBAR", "1"),
         "GEPSME= OR": inp.replace("QUIT\n\nEXE> MD", "      GEPSME= OR\nQUIT\n\nEXE> MD"),
@@ -425,3 +427,59 @@ def test_mdinp_cano_shake_retry_like_md_run(golden, tmp_path):
     assert o["loop"] == int(f["retry_loop"])
     assert np.abs(o["x"] - f["retry_x"]).max() < 1e-9
     assert np.abs(o["v"] - f["retry_v"]).max() < 1e-9
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("tag", ["win", "lo"])
+def test_mdinp_ge_like_md_run(golden, tmp_path, tag):
+    """METHOD= GE with md_run's NAMEGE= file (inmcmd.f90:20-31): the host
+    reads it and runs multicanonical MD (jp_set_mcmd); its final state matches
+    md_run's GE runs within 1e-9."""
+    from make_golden import write_ge_params
+    s, f = golden("water_1500")
+    print(f"case host GE {tag}", flush=True)
+    d = str(tmp_path)
+    celw, ceup = f[f"ge_{tag}_range"]
+    write_ge_params(os.path.join(d, "ge.par"), float(celw), float(ceup))
+    _tpl_case(d, s, {"METHOD": "GE", "LOOPLI": str(int(f[f"ge_{tag}_loop"])), "SETTEM": "300.0", "TEMPCO": "NO",
+                     "NAMEGE": "ge.par"})
+    r = subprocess.run([_host(), "--mdinp", "md.inp"], capture_output=True, text=True, timeout=240, cwd=d)
+    print(r.stdout, flush=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    o = refio.read_restart(os.path.join(d, "out.rst"))
+    assert np.abs(o["x"] - f[f"ge_{tag}_x"]).max() < 1e-9
+    assert np.abs(o["v"] - f[f"ge_{tag}_v"]).max() < 1e-9
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("tag", ["a", "b", "c"])
+def test_mdinp_alsd_like_md_run(golden, tmp_path, tag):
+    """METHOD= GE under GEPSME= LMD (ALSD) from md_run's own inputs: NAMEGE=
+    with its wall line, EXE> INPUT's LWEIGH and LTCNTL, lambda from the
+    restart; the restart the host writes holds md_run's x, v (1e-9), lambda
+    and its velocity (1e-11), and its NAMETO file md_run's lambda, E1 and E2
+    per step to the printed digits."""
+    from make_golden import ALSD_RUNS, write_alsd_params
+    s, f = golden("chain_lmd")
+    print(f"case host ALSD {tag}", flush=True)
+    d = str(tmp_path)
+    celw, ceup, tempco, ltc, lw = ALSD_RUNS[tag]
+    write_alsd_params(os.path.join(d, "ge.par"), celw, ceup)
+    inp = {"LTCNTL": ltc}
+    if lw is not None:
+        inp["LWEIGH"] = repr(float(lw))
+    _tpl_case(d, s, {"METHOD": "GE", "LOOPLI": str(int(f[f"alsd_{tag}_loop"])), "SETTEM": "300.0",
+                     "TEMPCO": tempco, "NAMEGE": "ge.par", "NAMETO": "ene.dat"}, inp=inp)
+    r = subprocess.run([_host(), "--mdinp", "md.inp"], capture_output=True, text=True, timeout=240, cwd=d)
+    print(r.stdout, flush=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    o = refio.read_restart(os.path.join(d, "out.rst"))
+    assert np.abs(o["x"] - f[f"alsd_{tag}_x"]).max() < 1e-9
+    assert np.abs(o["v"] - f[f"alsd_{tag}_v"]).max() < 1e-9
+    assert abs(o["lam"] - float(f[f"alsd_{tag}_lam"])) < 1e-11
+    assert abs(o["lam_v"] - float(f[f"alsd_{tag}_lamv"])) < 1e-11
+    ene = np.loadtxt(os.path.join(d, "ene.dat"), ndmin=2)
+    ref = f[f"alsd_{tag}_ene"]
+    assert ene.shape == ref.shape
+    # E15.7: one unit of the last printed digit is at most 1e-6 relative
+    assert np.allclose(ene, ref, rtol=1.01e-6, atol=0)
