@@ -415,6 +415,51 @@ def water_box(nwat: int, L: float, seed: int, n_ion_pairs: int = 0, temp: float 
     return s
 
 
+def water_droplet(R: float, seed: int, density_L: float = 31.04, temp: float = 300.0,
+                  name: str = "droplet") -> System:
+    """A non-periodic water droplet for the cell multipole (CMM) path: the
+    waters of a water_box at 0.997 g/cm^3 whose oxygen lies within R of the
+    box centre.  The box only places the droplet (its centre is the box
+    centre); the system is not periodic."""
+    L = max(density_L, 2.0 * R + 8.0)
+    nwat = int(round(L ** 3 * 0.997 / 18.015 * 0.6022))
+    w = water_box(nwat, L, seed, temp=temp)
+    c = 0.5 * L
+    ox = w.x[0::3]
+    keep = np.where(((ox - c) ** 2).sum(axis=1) <= R * R)[0]
+    idx = (3 * keep[:, None] + np.arange(3)[None, :]).ravel()
+    s = System(name=name, mols=[water_template(len(keep))], types=water_types(), box=w.box.copy(),
+               x=w.x[idx].copy(), v=np.zeros((len(idx), 3)))
+    rng = np.random.Generator(np.random.PCG64(seed + 7919))
+    s.v = thermal_velocities(rng, s, temp)
+    return s
+
+
+def chain_droplet(n_chain: int, R: float, seed: int, lmd: Optional[float] = None, bond_fc: float = 0.0,
+                  angle_fc: float = 0.0, torsion_fc: float = 0.0, temp: float = 300.0,
+                  name: str = "chain_droplet") -> System:
+    """A non-periodic droplet around a compact chain (cell multipole path):
+    a chain_in_water system with a globular chain at the box centre, of which
+    the chain and the waters whose oxygen lies within R of the centre are
+    kept."""
+    L = 2.0 * R + 10.0
+    w = chain_in_water(n_chain, L, seed, lmd=lmd, bond_fc=bond_fc, angle_fc=angle_fc, torsion_fc=torsion_fc,
+                       compact=True, temp=temp)
+    chain, water = w.mols
+    nc = chain.natom
+    c = 0.5 * L
+    ox = w.x[nc::3]
+    keep = np.where(((ox - c) ** 2).sum(axis=1) <= R * R)[0]
+    widx = nc + (3 * keep[:, None] + np.arange(3)[None, :]).ravel()
+    idx = np.concatenate([np.arange(nc), widx])
+    s = System(name=name, mols=[chain, water_template(len(keep), cluster=water.cluster)], types=w.types,
+               box=w.box.copy(), x=w.x[idx].copy(), v=np.zeros((len(idx), 3)),
+               cluster_method=w.cluster_method, lam=w.lam)
+    rng = np.random.Generator(np.random.PCG64(seed + 104729))
+    s.v = thermal_velocities(rng, s, temp)
+    return s
+
+
 def _saw_chain(rng, n: int, L: float, bond: float = 1.5, excl: float = 3.0,
                min_angle_deg: float = 100.0, confine: Optional[float] = None) -> np.ndarray:
     """Self-avoiding random walk in a periodic box: atom k keeps >= excl from

@@ -91,7 +91,10 @@ typedef struct {
     double *tfc, *trot, *tphs;
     int n14, *p14, *p14cls;
     double *p14v, *p14e;
+    struct jo_cmm *cmm;   /* CAL15M= CMM state (jo_cmm_setup), else NULL */
 } jo_ctx;
+
+static void jo_cmm_free(struct jo_cmm *m);
 
 /* ------------------------------------------------------------------ setup */
 
@@ -396,6 +399,7 @@ void jo_destroy(jo_ctx *c) {
     free(c->ang); free(c->acls); free(c->afc); free(c->ath0);
     free(c->tor); free(c->tdiv); free(c->tcls); free(c->tfc); free(c->trot); free(c->tphs);
     free(c->p14); free(c->p14cls); free(c->p14v); free(c->p14e);
+    jo_cmm_free(c->cmm);
     free(c);
 }
 
@@ -1538,4 +1542,725 @@ int64_t jo_list_brute(const jo_ctx *c, const double *x, int32_t *ptr, int32_t *i
     }
     if (ptr) ptr[n] = (int32_t)w;
     return w;
+}
+
+/* ===================================================== CMM (CAL15M= CMM)
+ *
+ * md_run's cell multipole method for non-periodic systems (BOUNDA= NO,
+ * iy15m = 1, PARALV= DOUB, N_SIMD= 1, atom-based cells, no external CMM):
+ *   incel     incel.f90:29-57 (levels, sizes, idelay), 154-175 (lvdn,
+ *             lvup), 176-300 (second-nearest cells for LJ), 311-313 (mk_c),
+ *             456-478 (nearest cells), 1098-1276 (mk_cm, mk_c)
+ *   celpar_CB celpar.f90:49-160 (cell centres shifted by the coordinate
+ *             mean, binning), 161-196 (ier 28)
+ *   celprp    celprp.f90:47-104 (EL / EL+vdW split, inum), 139-154
+ *   celoth    celoth.f90 (atoms per cell at every level)
+ *   celpol    celpol.f90 (monopole, dipole, quadrupole of every cell, level 1
+ *             from the atoms, higher levels from their 8 children)
+ *   neartb    neartb.f90:155-320 (DOUB near tables), 321-366 (LJ table of
+ *             the second-nearest cells), 368-400 (ier 51), 402-453
+ *             (exclusion swap-delete, ier 99), 596-700 (far-cell lists)
+ *   enear_CMM_doub  enear.f90 #CMM #DOUB (Coulomb and LJ of the near
+ *             tables, LJ of the second-nearest table, i_vec = 1)
+ *   cmmdo     cmmdo.f90:79-156, 164-288 (particle-multipole far field every
+ *             idupcm steps, reused between them)
+ *   md        md.f90:655-684 (celprp every UPDATE, far field every CMMUPD)
+ * Lists hold original atom indices in md_run's per-atom order; atoms are
+ * visited in md_run's sorted order, so every sum runs in md_run's order. */
+
+#define CMM_MAXLEV 5
+
+typedef struct jo_cmm {
+    int nlev, nv[CMM_MAXLEV], idelay, nfrag;
+    double cminsiz, siz[CMM_MAXLEV], rlim, rlim2;
+    double *cg[CMM_MAXLEV];              /* mk_c: unshifted centre of cell i along an axis */
+    double *cc[CMM_MAXLEV][3];           /* shifted by the coordinate mean at the last celprp */
+    int *lvup;                           /* [nv1] parent (0-based) */
+    int *ncm[CMM_MAXLEV], **icm[CMM_MAXLEV];   /* far cells of every cell per level (linear) */
+    int *nnr, **inr, *nnrv, **inrv;      /* level-1 nearest and second-nearest cells */
+    int ipmax, n15mx, n15mxel, nvdw;
+    /* celprp */
+    int *npcl[CMM_MAXLEV];               /* [cell][frag] atoms per cell and fragment */
+    int *pstart, *patom;                 /* level-1 ipcl: atoms of (cell, frag) after the EL split */
+    int *cnel;                           /* [cell][frag] EL-only atoms */
+    double *pot[CMM_MAXLEV];             /* [cell][frag][13] */
+    /* near tables per (atom, class): EL, EL+vdW, LJ of second-nearest cells */
+    int *tel, *tvd, *tvw, nel_cap, nvd_cap, nvw_cap;
+    int *nel, *nvd, *nvw;
+    /* far-cell lists per (fragment k, my-cell index, level, source fragment) */
+    int *nmy, **mycel;
+    int *yptr, *ycel;                    /* CSR over ((k * maxmy + ii) * nlev + l) * nfrag + kk */
+    int maxmy;
+    /* far field state (reused when itwin = 0) */
+    double *gwk;                         /* [nfrg][n][3] */
+    double ecmm[CMM_MAXLEV][3];
+    int have_far;
+} jo_cmm;
+
+static void jo_cmm_free(struct jo_cmm *m) {
+    if (!m) return;
+    int nc1 = m->nv[0] * m->nv[0] * m->nv[0];
+    for (int l = 0; l < m->nlev; ++l) {
+        int ncl = m->nv[l] * m->nv[l] * m->nv[l];
+        free(m->cg[l]);
+        for (int k = 0; k < 3; ++k) free(m->cc[l][k]);
+        if (m->icm[l]) for (int i = 0; i < ncl; ++i) free(m->icm[l][i]);
+        free(m->icm[l]); free(m->ncm[l]); free(m->npcl[l]); free(m->pot[l]);
+    }
+    if (m->inr) for (int i = 0; i < nc1; ++i) free(m->inr[i]);
+    if (m->inrv) for (int i = 0; i < nc1; ++i) free(m->inrv[i]);
+    free(m->inr); free(m->nnr); free(m->inrv); free(m->nnrv); free(m->lvup);
+    free(m->pstart); free(m->patom); free(m->cnel);
+    free(m->tel); free(m->tvd); free(m->tvw); free(m->nel); free(m->nvd); free(m->nvw);
+    if (m->mycel) for (int k = 0; k < m->nfrag; ++k) free(m->mycel[k]);
+    free(m->mycel); free(m->nmy); free(m->yptr); free(m->ycel); free(m->gwk);
+    free(m);
+}
+
+static inline int cmm_lin(int nv, int ix, int iy, int iz) { return (iz * nv + iy) * nv + ix; }
+
+/* incel's CMM set-up: cminsiz (CELSIZ, 6 A by default) and nlev (CELLVL,
+ * 3 by default) raised to cover the coordinate extent (incel.f90:42-52);
+ * rlim = the cutoff (incel.f90:36). */
+int jo_cmm_setup(jo_ctx *c, const double *x, double cminsiz, int nlev) {
+    jo_cmm_free(c->cmm);
+    c->cmm = NULL;
+    if (!(cminsiz > 0.0) || nlev < 1 || nlev > CMM_MAXLEV) return JP_EINVAL;
+    jo_cmm *m = (jo_cmm *)calloc(1, sizeof(jo_cmm));
+    m->nfrag = c->nfrag;
+    m->cminsiz = cminsiz;
+    m->rlim = c->rc;
+    m->rlim2 = m->rlim * m->rlim;
+    double rtmp = 0.0;
+    for (int k = 0; k < 3; ++k) {
+        double lo = x[k], hi = x[k];
+        for (int i = 1; i < c->n; ++i) {
+            if (x[3 * i + k] > hi) hi = x[3 * i + k];
+            if (x[3 * i + k] < lo) lo = x[3 * i + k];
+        }
+        if (k == 0) rtmp = hi - lo;
+        else rtmp = fmax(hi - lo, rtmp);
+    }
+    if ((double)(1 << (nlev + 1)) * cminsiz < rtmp) {
+        nlev = (int)ceil(log(rtmp / cminsiz) / log(2.0) - 1.0);
+        if (nlev > CMM_MAXLEV) { free(m); return JP_EINVAL; }   /* md_run stops */
+    }
+    m->nlev = nlev;
+    for (int i = 0; i < nlev; ++i) m->nv[nlev - 1 - i] = 4 << i;
+    for (int i = 0; i < nlev; ++i) m->siz[i] = cminsiz * (double)(1 << i);
+    m->idelay = (int)ceil(m->rlim / m->siz[0]);
+    const int nv1 = m->nv[0], nc1 = nv1 * nv1 * nv1;
+    /* lvup (incel.f90:162-172): parent of cell i, 1-based i/2 rounded up */
+    m->lvup = (int *)malloc(sizeof(int) * nv1);
+    for (int i = 1; i <= nv1; ++i) m->lvup[i - 1] = (i % 2 == 0 ? i / 2 : i / 2 + 1) - 1;
+    /* mk_c: centres siz (i - (nv+1)/2), 1-based i */
+    for (int l = 0; l < nlev; ++l) {
+        double r = -0.5 * (m->nv[l] + 1);
+        m->cg[l] = (double *)malloc(sizeof(double) * m->nv[l]);
+        for (int i = 1; i <= m->nv[l]; ++i) m->cg[l][i - 1] = m->siz[l] * ((double)i + r);
+        for (int k = 0; k < 3; ++k) m->cc[l][k] = (double *)malloc(sizeof(double) * m->nv[l]);
+    }
+    /* mk_cm: far cells per level (1-based ranges restated 0-based) */
+    for (int l = 0; l < nlev; ++l) {
+        const int mm = m->nv[l], ncl = mm * mm * mm;
+        m->ncm[l] = (int *)calloc(ncl, sizeof(int));
+        m->icm[l] = (int **)calloc(ncl, sizeof(int *));
+        for (int iz = 1; iz <= mm; ++iz)
+            for (int iy = 1; iy <= mm; ++iy)
+                for (int ix = 1; ix <= mm; ++ix) {
+                    const int iz1 = iz - 1 > 1 ? iz - 1 : 1, iz2 = iz + 1 < mm ? iz + 1 : mm;
+                    const int iy1 = iy - 1 > 1 ? iy - 1 : 1, iy2 = iy + 1 < mm ? iy + 1 : mm;
+                    const int ix1 = ix - 1 > 1 ? ix - 1 : 1, ix2 = ix + 1 < mm ? ix + 1 : mm;
+                    int jz1 = 1, jz2 = mm, jy1 = 1, jy2 = mm, jx1 = 1, jx2 = mm;
+                    if (l != nlev - 1) {
+                        const int m1 = m->nv[l + 1];
+                        const int jz = (iz % 2 == 0 ? iz / 2 : iz / 2 + 1), jy = (iy % 2 == 0 ? iy / 2 : iy / 2 + 1),
+                                  jx = (ix % 2 == 0 ? ix / 2 : ix / 2 + 1);
+                        jz1 = (jz - 1 > 1 ? jz - 1 : 1) * 2 - 1; jz2 = (jz + 1 < m1 ? jz + 1 : m1) * 2;
+                        jy1 = (jy - 1 > 1 ? jy - 1 : 1) * 2 - 1; jy2 = (jy + 1 < m1 ? jy + 1 : m1) * 2;
+                        jx1 = (jx - 1 > 1 ? jx - 1 : 1) * 2 - 1; jx2 = (jx + 1 < m1 ? jx + 1 : m1) * 2;
+                    }
+                    const int cell = cmm_lin(mm, ix - 1, iy - 1, iz - 1);
+                    int *lst = (int *)malloc(sizeof(int) * (size_t)(jz2 - jz1 + 1) * (jy2 - jy1 + 1) * (jx2 - jx1 + 1));
+                    int nt = 0;
+                    for (int kz = jz1; kz <= jz2; ++kz)
+                        for (int ky = jy1; ky <= jy2; ++ky)
+                            for (int kx = jx1; kx <= jx2; ++kx) {
+                                const int fz = kz >= iz1 && kz <= iz2, fy = ky >= iy1 && ky <= iy2,
+                                          fx = kx >= ix1 && kx <= ix2;
+                                if (fx && fy && fz) continue;
+                                lst[nt++] = cmm_lin(mm, kx - 1, ky - 1, kz - 1);
+                            }
+                    m->ncm[l][cell] = nt;
+                    m->icm[l][cell] = lst;
+                }
+    }
+    /* nearest cells (DOUB, incel.f90:456-478, m = 1) and the second-nearest
+     * ones for LJ (incel.f90:246-299, idelay > 1) */
+    m->nnr = (int *)calloc(nc1, sizeof(int));
+    m->inr = (int **)calloc(nc1, sizeof(int *));
+    m->nnrv = (int *)calloc(nc1, sizeof(int));
+    m->inrv = (int **)calloc(nc1, sizeof(int *));
+    for (int iz = 1; iz <= nv1; ++iz)
+        for (int iy = 1; iy <= nv1; ++iy)
+            for (int ix = 1; ix <= nv1; ++ix) {
+                const int cell = cmm_lin(nv1, ix - 1, iy - 1, iz - 1);
+                int *lst = (int *)malloc(sizeof(int) * 27), nt = 0;
+                for (int i = (iz - 1 > 1 ? iz - 1 : 1); i <= (iz + 1 < nv1 ? iz + 1 : nv1); ++i)
+                    for (int j = (iy - 1 > 1 ? iy - 1 : 1); j <= (iy + 1 < nv1 ? iy + 1 : nv1); ++j)
+                        for (int k = (ix - 1 > 1 ? ix - 1 : 1); k <= (ix + 1 < nv1 ? ix + 1 : nv1); ++k) {
+                            if (i == iz && j == iy && k == ix) continue;
+                            lst[nt++] = cmm_lin(nv1, k - 1, j - 1, i - 1);
+                        }
+                m->nnr[cell] = nt;
+                m->inr[cell] = lst;
+                if (m->idelay > 1) {
+                    const int d = m->idelay, w = 2 * d + 1;
+                    int *lv = (int *)malloc(sizeof(int) * (size_t)w * w * w), nv2 = 0;
+                    for (int i = (iz - d > 1 ? iz - d : 1); i <= (iz + d < nv1 ? iz + d : nv1); ++i)
+                        for (int j = (iy - d > 1 ? iy - d : 1); j <= (iy + d < nv1 ? iy + d : nv1); ++j)
+                            for (int k = (ix - d > 1 ? ix - d : 1); k <= (ix + d < nv1 ? ix + d : nv1); ++k) {
+                                const int fz = iz - i >= -1 && iz - i <= 1, fy = iy - j >= -1 && iy - j <= 1,
+                                          fx = ix - k >= -1 && ix - k <= 1;
+                                if (fx && fy && fz) continue;
+                                lv[nv2++] = cmm_lin(nv1, k - 1, j - 1, i - 1);
+                            }
+                    m->nnrv[cell] = nv2;
+                    m->inrv[cell] = lv;
+                }
+            }
+    /* table capacities (incel.f90:137-150, PARALV= DOUB doubles them) */
+    const double vol = pow(4.0 * m->siz[0], 3.0);
+    m->ipmax = (int)ceil(m->siz[0] * m->siz[0] * m->siz[0] * 0.27) * 2;
+    m->n15mx = (int)ceil(0.15 * vol) * 2;
+    m->n15mxel = (int)ceil(0.15 * vol) * 2;
+    m->nvdw = (int)ceil(0.027 * vol) * 2;
+    for (int l = 0; l < nlev; ++l) {
+        const int ncl = m->nv[l] * m->nv[l] * m->nv[l];
+        m->npcl[l] = (int *)calloc((size_t)ncl * c->nfrag, sizeof(int));
+        m->pot[l] = (double *)calloc((size_t)ncl * c->nfrag * 13, sizeof(double));
+    }
+    m->gwk = (double *)calloc((size_t)c->nfrg * c->n * 3, sizeof(double));
+    c->cmm = m;
+    return 0;
+}
+
+/* celprp for CMM: returns ier (0, 28, 51, 99). */
+int jo_cmm_celprp(jo_ctx *c, const double *x) {
+    jo_cmm *m = c->cmm;
+    if (!m) return JP_EINVAL;
+    const int n = c->n, nf = c->nfrag, nfrg = c->nfrg, nv1 = m->nv[0], nc1 = nv1 * nv1 * nv1;
+    /* celpar_CB: centres shifted by the coordinate mean (celpar.f90:49-58) */
+    double cord0[3];
+    const double rn = 1.0 / (double)n;
+    for (int k = 0; k < 3; ++k) {
+        double sm = 0.0;
+        for (int i = 0; i < n; ++i) sm = sm + x[3 * i + k];
+        cord0[k] = sm * rn;
+    }
+    for (int l = 0; l < m->nlev; ++l)
+        for (int k = 0; k < 3; ++k)
+            for (int i = 0; i < m->nv[l]; ++i) m->cc[l][k][i] = m->cg[l][i] + cord0[k];
+    const double zz0 = m->siz[0] * 0.5;
+    const double xo = m->cc[0][0][0] - zz0, yo = m->cc[0][1][0] - zz0, zo = m->cc[0][2][0] - zz0;
+    const double rr = 1.0 / m->siz[0];
+    /* binning in atom order (celpar.f90:110-125) */
+    int *cnt = (int *)calloc((size_t)nc1 * nf, sizeof(int));
+    int *cellof = (int *)malloc(sizeof(int) * n);
+    int ier = 0;
+    for (int j = 0; j < n; ++j) {
+        const double gx = x[3 * j] - xo, gy = x[3 * j + 1] - yo, gz = x[3 * j + 2] - zo;
+        /* Fortran int() truncates toward zero, as the C conversion does */
+        const double fx = gx * rr, fy = gy * rr, fz = gz * rr;
+        const int ok = fx > -1.0 && fy > -1.0 && fz > -1.0 && fx < (double)nv1 && fy < (double)nv1 && fz < (double)nv1;
+        const int mx = ok ? (int)fx : 0, my = ok ? (int)fy : 0, mz = ok ? (int)fz : 0;
+        if (!ok) {
+            ier = 28;   /* an atom off the cell field (celpar.f90:184-192) */
+            cellof[j] = -1;
+            continue;
+        }
+        cellof[j] = cmm_lin(nv1, mx, my, mz);
+        cnt[cellof[j] * nf + c->icls[j]]++;
+    }
+    for (int q = 0; q < nc1 * nf && !ier; ++q)
+        if (cnt[q] > m->ipmax) ier = 28;   /* celpar.f90:172-181 */
+    if (ier) { free(cnt); free(cellof); return ier; }
+    /* ipcl per (cell, fragment) in atom order, then EL-only atoms first
+     * (celprp.f90:47-72) */
+    free(m->pstart); free(m->patom); free(m->cnel);
+    m->pstart = (int *)malloc(sizeof(int) * ((size_t)nc1 * nf + 1));
+    m->patom = (int *)malloc(sizeof(int) * n);
+    m->cnel = (int *)calloc((size_t)nc1 * nf, sizeof(int));
+    m->pstart[0] = 0;
+    for (int q = 0; q < nc1 * nf; ++q) m->pstart[q + 1] = m->pstart[q] + cnt[q];
+    {
+        int *fill = (int *)calloc((size_t)nc1 * nf, sizeof(int));
+        int *tmp = (int *)malloc(sizeof(int) * n);
+        for (int j = 0; j < n; ++j) {
+            const int q = cellof[j] * nf + c->icls[j];
+            tmp[m->pstart[q] + fill[q]++] = j;
+        }
+        for (int q = 0; q < nc1 * nf; ++q) {
+            int w = m->pstart[q];
+            for (int t = m->pstart[q]; t < m->pstart[q + 1]; ++t)
+                if (c->zero_vdw[tmp[t]]) m->patom[w++] = tmp[t];
+            m->cnel[q] = w - m->pstart[q];
+            for (int t = m->pstart[q]; t < m->pstart[q + 1]; ++t)
+                if (!c->zero_vdw[tmp[t]]) m->patom[w++] = tmp[t];
+        }
+        free(fill); free(tmp);
+    }
+    /* inum: 2x2x2 blocks of cells, then fragment, then ipcl order
+     * (celprp.f90:75-95) */
+    {
+        int ii = 0;
+        for (int iz = 0; iz < nv1; iz += 2)
+            for (int iy = 0; iy < nv1; iy += 2)
+                for (int ix = 0; ix < nv1; ix += 2)
+                    for (int jz = iz; jz <= iz + 1; ++jz)
+                        for (int jy = iy; jy <= iy + 1; ++jy)
+                            for (int jx = ix; jx <= ix + 1; ++jx)
+                                for (int i = 0; i < nf; ++i) {
+                                    const int q = cmm_lin(nv1, jx, jy, jz) * nf + i;
+                                    for (int t = m->pstart[q]; t < m->pstart[q + 1]; ++t) {
+                                        c->inum[m->patom[t]] = ii;
+                                        c->order[ii] = m->patom[t];
+                                        ++ii;
+                                    }
+                                }
+    }
+    /* celoth: atoms per cell and fragment at every level */
+    for (int q = 0; q < nc1 * nf; ++q) m->npcl[0][q] = cnt[q];
+    for (int l = 1; l < m->nlev; ++l) {
+        const int mm = m->nv[l], mc = m->nv[l - 1];
+        for (int iz = 0; iz < mm; ++iz)
+            for (int iy = 0; iy < mm; ++iy)
+                for (int ix = 0; ix < mm; ++ix)
+                    for (int i = 0; i < nf; ++i) {
+                        int sm = 0;
+                        for (int cz = 2 * iz; cz <= 2 * iz + 1; ++cz)
+                            for (int cy = 2 * iy; cy <= 2 * iy + 1; ++cy)
+                                for (int cx = 2 * ix; cx <= 2 * ix + 1; ++cx)
+                                    sm += m->npcl[l - 1][cmm_lin(mc, cx, cy, cz) * nf + i];
+                        m->npcl[l][cmm_lin(mm, ix, iy, iz) * nf + i] = sm;
+                    }
+    }
+    free(cnt); free(cellof);
+    /* celpol, level 1 from the atoms (celpol.f90:31-79); raw charges */
+    for (int iz = 0; iz < nv1; ++iz)
+        for (int iy = 0; iy < nv1; ++iy)
+            for (int ix = 0; ix < nv1; ++ix)
+                for (int i = 0; i < nf; ++i) {
+                    const int q = cmm_lin(nv1, ix, iy, iz) * nf + i;
+                    double *P = m->pot[0] + (size_t)q * 13;
+                    if (m->npcl[0][q] == 0) { for (int k = 0; k < 13; ++k) P[k] = 0.0; continue; }
+                    double qz = 0.0, rx = 0.0, ry = 0.0, rz = 0.0, rxx = 0.0, ryy = 0.0, rzz = 0.0, rxy = 0.0,
+                           ryz = 0.0, rzx = 0.0;
+                    const double gx = m->cc[0][0][ix], gy = m->cc[0][1][iy], gz = m->cc[0][2][iz];
+                    for (int t = m->pstart[q]; t < m->pstart[q + 1]; ++t) {
+                        const int a = m->patom[t];
+                        const double cq = c->charge[a];
+                        qz = qz + cq;
+                        const double dx = x[3 * a] - gx, dy = x[3 * a + 1] - gy, dz = x[3 * a + 2] - gz;
+                        rx = rx + cq * dx; ry = ry + cq * dy; rz = rz + cq * dz;
+                        const double dxx = dx * dx, dyy = dy * dy, dzz = dz * dz;
+                        rxx = rxx + cq * (2.0 * dxx - dyy - dzz);
+                        ryy = ryy + cq * (2.0 * dyy - dzz - dxx);
+                        rzz = rzz + cq * (2.0 * dzz - dxx - dyy);
+                        rxy = rxy + cq * dx * dy;
+                        ryz = ryz + cq * dy * dz;
+                        rzx = rzx + cq * dz * dx;
+                    }
+                    const double v[13] = {qz, rx, ry, rz, rxx * 0.5, ryy * 0.5, rzz * 0.5, rxx, ryy, rzz,
+                                          rxy * 3.0, ryz * 3.0, rzx * 3.0};
+                    memcpy(P, v, sizeof(v));
+                }
+    /* celpol, level l from its 8 children (celpol.f90:84-153 and the
+     * identical blocks for levels 3-5) */
+    for (int l = 1; l < m->nlev; ++l) {
+        const int mm = m->nv[l], mc = m->nv[l - 1];
+        for (int iz = 0; iz < mm; ++iz)
+            for (int iy = 0; iy < mm; ++iy)
+                for (int ix = 0; ix < mm; ++ix)
+                    for (int i = 0; i < nf; ++i) {
+                        const int q = cmm_lin(mm, ix, iy, iz) * nf + i;
+                        double *P = m->pot[l] + (size_t)q * 13;
+                        if (m->npcl[l][q] == 0) { for (int k = 0; k < 13; ++k) P[k] = 0.0; continue; }
+                        double qz = 0.0, rx = 0.0, ry = 0.0, rz = 0.0, rxx = 0.0, ryy = 0.0, rzz = 0.0, rxy = 0.0,
+                               ryz = 0.0, rzx = 0.0;
+                        for (int jj = 0; jj < 8; ++jj) {
+                            const int mx = 2 * ix + (jj & 1), my = 2 * iy + ((jj >> 1) & 1),
+                                      mz = 2 * iz + ((jj >> 2) & 1);
+                            const int qc = cmm_lin(mc, mx, my, mz) * nf + i;
+                            if (m->npcl[l - 1][qc] == 0) continue;
+                            const double *C = m->pot[l - 1] + (size_t)qc * 13;
+                            const double qq = C[0];
+                            qz = qz + qq;
+                            const double xx = m->cc[l - 1][0][mx] - m->cc[l][0][ix];
+                            const double yy = m->cc[l - 1][1][my] - m->cc[l][1][iy];
+                            const double zz = m->cc[l - 1][2][mz] - m->cc[l][2][iz];
+                            const double ax = C[1], ay = C[2], az = C[3];
+                            rx = rx + ax + qq * xx;
+                            ry = ry + ay + qq * yy;
+                            rz = rz + az + qq * zz;
+                            const double dxx = xx * xx, dyy = yy * yy, dzz = zz * zz;
+                            const double xax = xx * ax, yay = yy * ay, zaz = zz * az;
+                            rxx = rxx + C[4] + 2.0 * xax - yay - zaz + qq * (dxx - 0.5 * (dyy + dzz));
+                            ryy = ryy + C[5] + 2.0 * yay - zaz - xax + qq * (dyy - 0.5 * (dzz + dxx));
+                            rzz = rzz + C[6] + 2.0 * zaz - xax - yay + qq * (dzz - 0.5 * (dxx + dyy));
+                            rxy = rxy + C[10] * 0.5 + (yy * ax + xx * ay + xx * yy * qq) * 1.5;
+                            ryz = ryz + C[11] * 0.5 + (zz * ay + yy * az + yy * zz * qq) * 1.5;
+                            rzx = rzx + C[12] * 0.5 + (xx * az + zz * ax + zz * xx * qq) * 1.5;
+                        }
+                        const double v[13] = {qz, rx, ry, rz, rxx, ryy, rzz, rxx * 2.0, ryy * 2.0, rzz * 2.0,
+                                              rxy * 2.0, ryz * 2.0, rzx * 2.0};
+                        memcpy(P, v, sizeof(v));
+                    }
+    }
+    /* neartb, DOUB tables (neartb.f90:155-320): per atom its EL partners
+     * and EL+vdW partners in its own and the nearest cells */
+    const size_t nl = (size_t)n * nfrg;
+    free(m->nel); free(m->nvd); free(m->nvw); free(m->tel); free(m->tvd); free(m->tvw);
+    m->nel = (int *)calloc(nl, sizeof(int));
+    m->nvd = (int *)calloc(nl, sizeof(int));
+    m->nvw = (int *)calloc(nl, sizeof(int));
+    m->nel_cap = m->n15mxel; m->nvd_cap = m->n15mx; m->nvw_cap = m->nvdw;
+    m->tel = (int *)malloc(sizeof(int) * nl * (m->nel_cap + 1));
+    m->tvd = (int *)malloc(sizeof(int) * nl * (m->nvd_cap + 1));
+    m->tvw = (int *)malloc(sizeof(int) * nl * (m->nvw_cap + 1));
+    int over = 0;
+#define CMM_PUSH(tab, cnt_, cap, a, k, b)                                   \
+    do {                                                                     \
+        const size_t key_ = (size_t)(a) * nfrg + (k);                        \
+        if (m->cnt_[key_] < (cap)) m->tab[key_ * ((cap) + 1) + m->cnt_[key_]] = (b); \
+        else over = 1;                                                       \
+        m->cnt_[key_]++;                                                     \
+    } while (0)
+    for (int iz = 0; iz < nv1; ++iz)
+        for (int iy = 0; iy < nv1; ++iy)
+            for (int ix = 0; ix < nv1; ++ix) {
+                const int cell = cmm_lin(nv1, ix, iy, iz);
+                int any = 0;
+                for (int i = 0; i < nf; ++i) any |= m->npcl[0][cell * nf + i] != 0;
+                if (!any) continue;
+                for (int i = 0; i < nf; ++i) {
+                    const int qi = cell * nf + i, num = m->npcl[0][qi], nel = m->cnel[qi];
+                    if (num == 0) continue;
+                    const int *A = m->patom + m->pstart[qi];
+                    for (int ii = 0; ii < num; ++ii) {
+                        const int a = A[ii];
+                        int k = c->nmat[i * nf + i];
+                        if (ii < nel) {
+                            for (int jj = 0; jj < ii; ++jj) CMM_PUSH(tel, nel, m->nel_cap, a, k, A[jj]);
+                            for (int jj = ii + 1; jj < num; ++jj) CMM_PUSH(tel, nel, m->nel_cap, a, k, A[jj]);
+                        } else {
+                            for (int jj = 0; jj < nel; ++jj) CMM_PUSH(tel, nel, m->nel_cap, a, k, A[jj]);
+                            for (int jj = nel; jj < ii; ++jj) CMM_PUSH(tvd, nvd, m->nvd_cap, a, k, A[jj]);
+                            for (int jj = ii + 1; jj < num; ++jj) CMM_PUSH(tvd, nvd, m->nvd_cap, a, k, A[jj]);
+                        }
+                        for (int pass = 0; pass < 2; ++pass)
+                            for (int j = (pass ? i + 1 : 0); j < (pass ? nf : i); ++j) {
+                                k = c->nmat[j * nf + i];
+                                const int qj = cell * nf + j;
+                                const int border = ii < nel ? m->npcl[0][qj] : m->cnel[qj];
+                                const int *B = m->patom + m->pstart[qj];
+                                for (int jj = 0; jj < border; ++jj) CMM_PUSH(tel, nel, m->nel_cap, a, k, B[jj]);
+                                for (int jj = border; jj < m->npcl[0][qj]; ++jj) CMM_PUSH(tvd, nvd, m->nvd_cap, a, k, B[jj]);
+                            }
+                        for (int ice = 0; ice < m->nnr[cell]; ++ice) {
+                            const int jc = m->inr[cell][ice];
+                            int any2 = 0;
+                            for (int j = 0; j < nf; ++j) any2 |= m->npcl[0][jc * nf + j] != 0;
+                            if (!any2) continue;
+                            for (int j = 0; j < nf; ++j) {
+                                k = c->nmat[j * nf + i];
+                                const int qj = jc * nf + j;
+                                const int border = ii < nel ? m->npcl[0][qj] : m->cnel[qj];
+                                const int *B = m->patom + m->pstart[qj];
+                                for (int jj = 0; jj < border; ++jj) CMM_PUSH(tel, nel, m->nel_cap, a, k, B[jj]);
+                                for (int jj = border; jj < m->npcl[0][qj]; ++jj) CMM_PUSH(tvd, nvd, m->nvd_cap, a, k, B[jj]);
+                            }
+                        }
+                    }
+                }
+            }
+    /* LJ of the second-nearest cells within rlim (neartb.f90:321-366):
+     * coordinates as celprp copied them (Tcord) */
+    if (m->idelay > 1) {
+        for (int iz = 0; iz < nv1; ++iz)
+            for (int iy = 0; iy < nv1; ++iy)
+                for (int ix = 0; ix < nv1; ++ix) {
+                    const int cell = cmm_lin(nv1, ix, iy, iz);
+                    int any = 0;
+                    for (int i = 0; i < nf; ++i) any |= m->npcl[0][cell * nf + i] != 0;
+                    if (!any) continue;
+                    for (int ice = 0; ice < m->nnrv[cell]; ++ice) {
+                        const int jc = m->inrv[cell][ice];
+                        int any2 = 0;
+                        for (int j = 0; j < nf; ++j) any2 |= m->npcl[0][jc * nf + j] != 0;
+                        if (!any2) continue;
+                        for (int i = 0; i < nf; ++i) {
+                            const int qi = cell * nf + i;
+                            const int *A = m->patom + m->pstart[qi];
+                            for (int ii = m->cnel[qi]; ii < m->npcl[0][qi]; ++ii) {
+                                const int a = A[ii];
+                                for (int j = 0; j < nf; ++j) {
+                                    const int k = c->nmat[j * nf + i];
+                                    const int qj = jc * nf + j;
+                                    const int *B = m->patom + m->pstart[qj];
+                                    for (int jj = m->cnel[qj]; jj < m->npcl[0][qj]; ++jj) {
+                                        const int b = B[jj];
+                                        const double d0 = x[3 * a] - x[3 * b], d1 = x[3 * a + 1] - x[3 * b + 1],
+                                                     d2 = x[3 * a + 2] - x[3 * b + 2];
+                                        const double dd = d0 * d0 + d1 * d1 + d2 * d2;
+                                        if (dd < m->rlim2) CMM_PUSH(tvw, nvw, m->nvw_cap, a, k, b);
+                                    }
+                                }
+                            }
+                        }
+                    }
+                }
+    }
+#undef CMM_PUSH
+    if (over) return 51;   /* neartb.f90:368-400 */
+    /* exclusions: swap-delete from the near tables, then (if some are not
+     * found there) from the LJ table (neartb.f90:402-453) */
+    {
+        int64_t nchk = 0;
+        for (int a = 0; a < n; ++a)
+            for (int e = c->x14p[a]; e < c->x14p[a + 1]; ++e) {
+                const int b = c->x14i[e];
+                const int kk = c->nmat[c->icls[a] * nf + c->icls[b]];
+                const size_t key = (size_t)a * nfrg + kk;
+                if (c->zero_vdw[a] || c->zero_vdw[b]) {
+                    int *L = m->tel + key * (m->nel_cap + 1);
+                    for (int j = 0; j < m->nel[key]; ++j)
+                        if (L[j] == b) { L[j] = L[m->nel[key] - 1]; m->nel[key]--; nchk++; break; }
+                } else {
+                    int *L = m->tvd + key * (m->nvd_cap + 1);
+                    for (int j = 0; j < m->nvd[key]; ++j)
+                        if (L[j] == b) { L[j] = L[m->nvd[key] - 1]; m->nvd[key]--; nchk++; break; }
+                }
+            }
+        if (m->idelay > 1 && nchk / 2 != c->nntab14) {
+            for (int a = 0; a < n; ++a)
+                for (int e = c->x14p[a]; e < c->x14p[a + 1]; ++e) {
+                    const int b = c->x14i[e];
+                    const int kk = c->nmat[c->icls[a] * nf + c->icls[b]];
+                    const size_t key = (size_t)a * nfrg + kk;
+                    int *L = m->tvw + key * (m->nvw_cap + 1);
+                    for (int j = 0; j < m->nvw[key]; ++j)
+                        if (L[j] == b) { L[j] = L[m->nvw[key] - 1]; m->nvw[key]--; nchk++; break; }
+                }
+        }
+        if (nchk / 2 != c->nntab14) return 99;
+    }
+    /* far-cell lists (neartb.f90:596-700): for every cell holding atoms of
+     * fragment k, per level the far cells of its ancestor holding atoms of
+     * fragment kk */
+    if (m->mycel) for (int k = 0; k < m->nfrag; ++k) free(m->mycel[k]);
+    free(m->mycel); free(m->nmy); free(m->yptr); free(m->ycel);
+    m->nmy = (int *)calloc(nf, sizeof(int));
+    m->mycel = (int **)calloc(nf, sizeof(int *));
+    m->maxmy = 0;
+    for (int k = 0; k < nf; ++k) {
+        m->mycel[k] = (int *)malloc(sizeof(int) * nc1);
+        for (int cell = 0; cell < nc1; ++cell)
+            if (m->npcl[0][cell * nf + k]) m->mycel[k][m->nmy[k]++] = cell;
+        if (m->nmy[k] > m->maxmy) m->maxmy = m->nmy[k];
+    }
+    {
+        const size_t nkey = (size_t)nf * (m->maxmy > 0 ? m->maxmy : 1) * m->nlev * nf;
+        m->yptr = (int *)calloc(nkey + 1, sizeof(int));
+        size_t tot = 0;
+        for (int pass = 0; pass < 2; ++pass) {
+            tot = 0;
+            for (int k = 0; k < nf; ++k)
+                for (int ii = 0; ii < m->maxmy; ++ii)
+                    for (int l = 0; l < m->nlev; ++l)
+                        for (int kk = 0; kk < nf; ++kk) {
+                            const size_t key = (((size_t)k * m->maxmy + ii) * m->nlev + l) * nf + kk;
+                            if (pass == 1) m->yptr[key] = (int)tot;
+                            if (ii >= m->nmy[k]) continue;
+                            const int cell = m->mycel[k][ii];
+                            int cx = cell % nv1, cy = (cell / nv1) % nv1, cz = cell / (nv1 * nv1);
+                            for (int u = 0; u < l; ++u) { cx = m->lvup[cx]; cy = m->lvup[cy]; cz = m->lvup[cz]; }
+                            const int mm = m->nv[l];
+                            const int anc = cmm_lin(mm, cx, cy, cz);
+                            for (int t = 0; t < m->ncm[l][anc]; ++t) {
+                                const int jc = m->icm[l][anc][t];
+                                if (m->npcl[l][jc * nf + kk] == 0) continue;
+                                if (pass == 1) m->ycel[tot] = jc;
+                                tot++;
+                            }
+                        }
+            if (pass == 0) m->ycel = (int *)malloc(sizeof(int) * (tot ? tot : 1));
+        }
+        m->yptr[nkey] = (int)tot;
+    }
+    m->have_far = 0;
+    c->have_list = 1;
+    return 0;
+}
+
+/* cmmdo for CMM (cmmdo.f90:25-156): the near tables' Coulomb and LJ and the
+ * second-nearest LJ (enear_CMM_doub, atoms in md_run's sorted order, classes
+ * from the last), then, when itwin, the particle-multipole far field of
+ * every level (compute_particle_multipole, cmmdo.f90:164-288) into gwk and
+ * Ecmm, which are reused while itwin is 0.  Accumulates into grad; eelec,
+ * evdw per class. */
+int jo_cmm_nonbond(jo_ctx *c, const double *x, double *grad, double *eelec, double *evdw, int itwin) {
+    jo_cmm *m = c->cmm;
+    if (!m || !c->have_list) return JP_EINVAL;
+    const int n = c->n, nf = c->nfrag, nfrg = c->nfrg, nt = c->ntyp;
+    double *tg = (double *)calloc((size_t)3 * n * nfrg, sizeof(double));
+    for (int k = 0; k < nfrg; ++k) { eelec[k] = 0.0; evdw[k] = 0.0; }
+    for (int cls = nfrg - 1; cls >= 0; --cls) {
+        double ee = 0.0, vv = 0.0;
+        for (int p = 0; p < n; ++p) {
+            const int a = c->order[p];
+            const int ti = c->atype[a];
+            const double xi = x[3 * a], yi = x[3 * a + 1], zi = x[3 * a + 2], qi = c->chgmod[a];
+            double gx = 0.0, gy = 0.0, gz = 0.0;
+            const size_t key = (size_t)a * nfrg + cls;
+            const int *L = m->tel + key * (m->nel_cap + 1);
+            for (int j = 0; j < m->nel[key]; ++j) {
+                const int b = L[j];
+                const double dx = xi - x[3 * b], dy = yi - x[3 * b + 1], dz = zi - x[3 * b + 2];
+                const double d2i = 1.0 / (dx * dx + dy * dy + dz * dz);
+                const double e = qi * c->chgmod[b] * sqrt(d2i);
+                ee = ee + e;
+                const double cc = e * d2i;
+                gx = gx + dx * cc; gy = gy + dy * cc; gz = gz + dz * cc;
+            }
+            L = m->tvd + key * (m->nvd_cap + 1);
+            for (int j = 0; j < m->nvd[key]; ++j) {
+                const int b = L[j];
+                const double dx = xi - x[3 * b], dy = yi - x[3 * b + 1], dz = zi - x[3 * b + 2];
+                const double c6 = c->c6[c->atype[b] * nt + ti], c12 = c->c12[c->atype[b] * nt + ti];
+                const double d2i = 1.0 / (dx * dx + dy * dy + dz * dz);
+                const double d6i = d2i * d2i * d2i;
+                const double e = qi * c->chgmod[b] * sqrt(d2i);
+                ee = ee + e;
+                const double v1 = c6 * d6i, v2 = c12 * d6i * d6i;
+                vv = vv - v1 + v2;
+                const double cc = (e - 6.0 * v1 + 12.0 * v2) * d2i;
+                gx = gx + dx * cc; gy = gy + dy * cc; gz = gz + dz * cc;
+            }
+            L = m->tvw + key * (m->nvw_cap + 1);
+            for (int j = 0; j < m->nvw[key]; ++j) {
+                const int b = L[j];
+                const double c6 = c->c6[c->atype[b] * nt + ti], c12 = c->c12[c->atype[b] * nt + ti];
+                const double dx = xi - x[3 * b], dy = yi - x[3 * b + 1], dz = zi - x[3 * b + 2];
+                const double d2i = 1.0 / (dx * dx + dy * dy + dz * dz);
+                const double d6i = d2i * d2i * d2i;
+                const double v1 = c6 * d6i, v2 = c12 * d6i * d6i;
+                vv = vv - v1 + v2;
+                const double cc = (-6.0 * v1 + 12.0 * v2) * d2i;
+                gx = gx + dx * cc; gy = gy + dy * cc; gz = gz + dz * cc;
+            }
+            /* gwx = 0 + sum(gwxv): the i_vec = 1 vector sums, then Tgrad -= */
+            double *T = tg + ((size_t)cls * n + a) * 3;
+            T[0] = T[0] - (0.0 + gx); T[1] = T[1] - (0.0 + gy); T[2] = T[2] - (0.0 + gz);
+        }
+        eelec[cls] = eelec[cls] + (0.0 + ee);
+        evdw[cls] = evdw[cls] + (0.0 + vv);
+    }
+    for (int k = 0; k < nfrg; ++k)
+        for (int a = 0; a < n; ++a)
+            for (int d = 0; d < 3; ++d) grad[((size_t)k * n + a) * 3 + d] += tg[((size_t)k * n + a) * 3 + d];
+    for (int k = 0; k < nfrg; ++k) { eelec[k] = eelec[k] * 0.5; evdw[k] = evdw[k] * 0.5; }
+    free(tg);
+    if (itwin || !m->have_far) {
+        for (int l = 0; l < CMM_MAXLEV; ++l) for (int k = 0; k < 3; ++k) m->ecmm[l][k] = 0.0;
+        memset(m->gwk, 0, sizeof(double) * 3 * (size_t)n * nfrg);
+        const int nv1 = m->nv[0];
+        for (int k = 0; k < nf; ++k)
+            for (int ii = 0; ii < m->nmy[k]; ++ii) {
+                const int cell = m->mycel[k][ii], qk = cell * nf + k;
+                const int *A = m->patom + m->pstart[qk];
+                const int num = m->npcl[0][qk];
+                (void)nv1;
+                for (int kk = 0; kk < nf; ++kk) {
+                    const int ncls = c->nmat[kk * nf + k];
+                    for (int l = 0; l < m->nlev; ++l) {
+                        const size_t key = (((size_t)k * m->maxmy + ii) * m->nlev + l) * nf + kk;
+                        const int mm = m->nv[l];
+                        for (int t = m->yptr[key]; t < m->yptr[key + 1]; ++t) {
+                            const int jc = m->ycel[t];
+                            const int jx = jc % mm, jy = (jc / mm) % mm, jz = jc / (mm * mm);
+                            const double *q = m->pot[l] + ((size_t)jc * nf + kk) * 13;
+                            for (int u = 0; u < num; ++u) {
+                                const int a = A[u];
+                                const double dx = x[3 * a] - m->cc[l][0][jx], dy = x[3 * a + 1] - m->cc[l][1][jy],
+                                             dz = x[3 * a + 2] - m->cc[l][2][jz];
+                                const double dx2 = dx * dx, dy2 = dy * dy, dz2 = dz * dz;
+                                const double d2i = 1.0 / (dx2 + dy2 + dz2);
+                                const double di = sqrt(d2i), d3i = di * d2i, d5i = d3i * d2i;
+                                const double dxy = dx * dy, dyz = dy * dz, dzx = dz * dx;
+                                const double aa1 = q[1] * dx + q[2] * dy + q[3] * dz;
+                                const double aa2 = q[4] * dx2 + q[5] * dy2 + q[6] * dz2 + q[10] * dxy + q[11] * dyz +
+                                                   q[12] * dzx;
+                                const double e = q[0] * di + aa1 * d3i + aa2 * d5i;
+                                m->ecmm[l][ncls] = m->ecmm[l][ncls] + c->charge[a] * c->ccoef * e;
+                                const double bbx = (q[0] * dx - q[1]) * d3i, bby = (q[0] * dy - q[2]) * d3i,
+                                             bbz = (q[0] * dz - q[3]) * d3i;
+                                const double cc2x = q[7] * dx + q[10] * dy + q[12] * dz;
+                                const double cc2y = q[10] * dx + q[8] * dy + q[11] * dz;
+                                const double cc2z = q[12] * dx + q[11] * dy + q[9] * dz;
+                                const double cc4 = aa1 * 3.0 + aa2 * d2i * 5.0;
+                                double *G = m->gwk + ((size_t)ncls * n + a) * 3;
+                                G[0] = G[0] + (-bbx + (cc2x - cc4 * dx) * d5i);
+                                G[1] = G[1] + (-bby + (cc2y - cc4 * dy) * d5i);
+                                G[2] = G[2] + (-bbz + (cc2z - cc4 * dz) * d5i);
+                            }
+                        }
+                    }
+                }
+            }
+        for (int l = 0; l < m->nlev; ++l) for (int k = 0; k < nfrg; ++k) m->ecmm[l][k] = m->ecmm[l][k] * 0.5;
+        for (int a = 0; a < n; ++a) {
+            const double q2 = c->ccoef * c->charge[a];   /* chgmod2, inpcls.f90:223 */
+            for (int k = 0; k < nfrg; ++k)
+                for (int d = 0; d < 3; ++d) m->gwk[((size_t)k * n + a) * 3 + d] *= q2;
+        }
+        m->have_far = 1;
+    }
+    for (int k = 0; k < nfrg; ++k) {
+        double sm = 0.0;
+        for (int l = 0; l < m->nlev; ++l) sm = sm + m->ecmm[l][k];
+        eelec[k] = eelec[k] + sm;
+    }
+    for (int k = 0; k < nfrg; ++k)
+        for (int a = 0; a < n; ++a)
+            for (int d = 0; d < 3; ++d) grad[((size_t)k * n + a) * 3 + d] += m->gwk[((size_t)k * n + a) * 3 + d];
+    return 0;
+}
+
+/* md_run's METHOD= MICR loop with CAL15M= CMM (md.f90:519-694): per step
+ * dnergy (bonded terms, then cmmdo with the current itwin) and the leap-frog
+ * step; after every update-th step celprp; after every cmmupd-th step the
+ * next step recomputes the far field (md.f90:679-684).  The first step
+ * computes it (incel sets itwin = 1).  Returns ier. */
+int jo_cmm_run_nve(jo_ctx *c, double *x, double *v, int nsteps, int update, int cmmupd) {
+    if (!c->cmm || update <= 0 || cmmupd <= 0 || update % cmmupd) return JP_EINVAL;
+    const int n = c->n, nfrg = c->nfrg;
+    double *grad = (double *)malloc(sizeof(double) * 3 * (size_t)n * nfrg);
+    double *gt = (double *)malloc(sizeof(double) * 3 * (size_t)n);
+    double *xo = (double *)malloc(sizeof(double) * 3 * (size_t)n);
+    double ee[3], ev[3], eterm[18];
+    int ier = c->have_list ? 0 : jo_cmm_celprp(c, x);
+    int itwin = 1;
+    for (int s = 1; s <= nsteps && !ier; ++s) {
+        memset(grad, 0, sizeof(double) * 3 * (size_t)n * nfrg);
+        jo_bonded(c, x, grad, eterm);
+        ier = jo_cmm_nonbond(c, x, grad, ee, ev, itwin);
+        if (ier) break;
+        jo_combine_grad(c, grad, gt);
+        ier = jo_verlet_step(c, x, v, gt, xo);
+        if (!ier && s % update == 0) ier = jo_cmm_celprp(c, x);
+        itwin = s % cmmupd == 0;
+    }
+    free(grad); free(gt); free(xo);
+    return ier;
 }

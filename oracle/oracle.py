@@ -60,6 +60,10 @@ def lib():
         L.jo_alsd_force.restype = C.c_double
         L.jo_run_alsd.argtypes = [C.c_void_p, f64p, f64p, f64p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int,
                                   C.c_double, C.c_int, i32p, f64p, C.POINTER(AlsdParams), f64p, i32p]
+        L.jo_cmm_setup.argtypes = [C.c_void_p, f64p, C.c_double, C.c_int]
+        L.jo_cmm_celprp.argtypes = [C.c_void_p, f64p]
+        L.jo_cmm_nonbond.argtypes = [C.c_void_p, f64p, f64p, f64p, f64p, C.c_int]
+        L.jo_cmm_run_nve.argtypes = [C.c_void_p, f64p, f64p, C.c_int, C.c_int, C.c_int]
         L.jo_lambda.argtypes = [C.c_void_p]
         L.jo_lambda.restype = C.c_double
         L.jo_set_lambda.argtypes = [C.c_void_p, C.c_double]
@@ -272,6 +276,29 @@ class Oracle:
                                 fcstol, len(dof), None if g is None else _p(g, i32p), _p(dof), C.byref(ap),
                                 _p(lam_log), C.byref(nprov))
         return ier, self.lam, lv.value, lam_log[:nsteps], nprov.value
+
+    # ---- CAL15M= CMM (non-periodic cell multipole method) ----------------
+    def cmm_setup(self, x, cminsiz: float = 6.0, nlev: int = 3) -> int:
+        """incel's CMM set-up (CELSIZ, CELLVL; nlev raised to the extent of x)."""
+        return lib().jo_cmm_setup(self.c, _p(np.ascontiguousarray(x, np.float64)), cminsiz, nlev)
+
+    def cmm_celprp(self, x) -> int:
+        return lib().jo_cmm_celprp(self.c, _p(np.ascontiguousarray(x, np.float64)))
+
+    def cmm_nonbond(self, x, grad=None, itwin: bool = True):
+        """cmmdo under CMM: near tables and far field; returns (grad, eelec,
+        evdw) with grad accumulated (nfrg, N, 3)."""
+        if grad is None:
+            grad = np.zeros((self.nfrg, self.n, 3))
+        ee = np.zeros(self.nfrg)
+        ev = np.zeros(self.nfrg)
+        ier = lib().jo_cmm_nonbond(self.c, _p(np.ascontiguousarray(x, np.float64)), _p(grad), _p(ee), _p(ev),
+                                   int(bool(itwin)))
+        assert ier == 0, ier
+        return grad, ee, ev
+
+    def cmm_run_nve(self, x, v, nsteps: int, update: int = 10, cmmupd: int = 5) -> int:
+        return lib().jo_cmm_run_nve(self.c, _p(x), _p(v), nsteps, update, cmmupd)
 
     def list_brute(self, x):
         x = np.ascontiguousarray(x, np.float64)

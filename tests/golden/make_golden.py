@@ -26,11 +26,14 @@ and N_SIMD= 1 (a fixed summation order):
 A fixture stores the generator recipe, the input x and v, and the outputs.
 The reference itself never leaves this container.
 
+  cmm jobs   (--cmm: CMM_RECIPES) CAL15M= CMM, BOUNDA= NO on droplets: the
+             FDEBUG= MAKE dump and a MICR run with NAMETO.
   alsd jobs  (chain_lmd, --alsd-only) RESTAR= YES, METHOD= GE with GEPSME= LMD
              (adaptive lambda square dynamics, ALSD_* below): lambda moves
              under -2 E1 lambda - E2 - dew(lambda) beside the atoms.
 
-Usage: python tests/golden/make_golden.py [--cano-only | --mon-only | --host-only | --ge-only | --alsd-only]
+Usage: python tests/golden/make_golden.py [--cano-only | --mon-only | --host-only | --ge-only | --alsd-only |
+       --cmm]
        [name ...]
 """
 from __future__ import annotations
@@ -84,6 +87,22 @@ RECIPES = {
     # 42 A edge: md_run's periodic cell table (8 cells of 6.75 A for 42 A) never tests
     # pairs between some cells facing each other across the boundary, inside the cutoff
     "water_42": ("water_box", dict(nwat=2469, L=42.0, seed=19), dict(traj=10)),
+}
+
+
+# Non-periodic systems under CAL15M= CMM (BOUNDA= NO): cmm job = force dump
+# plus a MICR run of `traj` steps (UPDATE= 10, CMMUPD= 5 by default: the far
+# field is recomputed every 5th step and reused in between, the multipoles and
+# tables every 10th).
+CMM_RECIPES = {
+    # 481-water droplet, 15 A radius: 16^3 level-1 cells of 6 A, three levels
+    "droplet_cmm": ("water_droplet", dict(R=15.0, seed=21), dict(traj=20)),
+    # compact 120-atom chain with bonds, angles, torsions and 1-4 terms in a droplet
+    "chain_droplet_cmm": ("chain_droplet", dict(n_chain=120, R=14.0, seed=41, bond_fc=300.0, angle_fc=50.0,
+                                                torsion_fc=1.0), dict(traj=20)),
+    # the same under LMD (two fragments, three energy classes)
+    "chain_droplet_lmd": ("chain_droplet", dict(n_chain=120, R=14.0, seed=41, lmd=0.7, bond_fc=300.0,
+                                                angle_fc=50.0, torsion_fc=1.0), dict(traj=10)),
 }
 
 
@@ -353,6 +372,37 @@ def alsd_jobs(s: systems.System, tmp: str) -> dict:
     return out
 
 
+def make_cmm(name: str) -> str:
+    """CAL15M= CMM, BOUNDA= NO fixture: md_run's FDEBUG= MAKE dump at the
+    stored coordinates and a MICR run with its NAMETO series."""
+    kind, args, opt = CMM_RECIPES[name]
+    s = build_system(kind, args)
+    out = dict(recipe=json.dumps(dict(kind=kind, args=args)), fingerprint=fingerprint(s), x=s.x, v=s.v)
+    cmm = {"CAL15M": "CMM", "BOUNDA": "NO"}
+    tmp = tempfile.mkdtemp(prefix="jpgold_")
+    try:
+        refio.write_restart(os.path.join(tmp, "in.rst"), s.x, s.v, lam=s.lam)
+        run_md(tmp, s, dict(cmm, METHOD="CANO", RESTAR="YES", NAMERI="in.rst", NAMERO="out.rst", LOOPLI="1",
+                            FDEBUG="MAKE", SETTEM="300.0"))
+        dump = refio.read_force_dump(os.path.join(tmp, "1stp_force.bin"), s.n)
+        out["force_grad"] = dump["grad"]
+        out["force_epot"] = dump["epot"]
+        t2 = os.path.join(tmp, "traj")
+        os.makedirs(t2)
+        refio.write_restart(os.path.join(t2, "in.rst"), s.x, s.v, lam=s.lam)
+        n = opt["traj"]
+        run_md(t2, s, dict(cmm, METHOD="MICR", RESTAR="YES", NAMERI="in.rst", NAMERO="out.rst", LOOPLI=str(n),
+                           UPDATE="10", TIMEST="2.0", NAMETO="ene.dat", OUTLOG="100000", OUTCLE="100000"))
+        r = refio.read_restart(os.path.join(t2, "out.rst"))
+        out.update(traj_x=r["x"], traj_v=r["v"], traj_nsteps=np.int64(n),
+                   traj_ene=np.loadtxt(os.path.join(t2, "ene.dat"), ndmin=2))
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    path = os.path.join(HERE, f"{name}.npz")
+    np.savez_compressed(path, **out)
+    return path
+
+
 def add_alsd(name: str) -> str:
     """Adds the ALSD runs to an existing fixture."""
     kind, args, opt = RECIPES[name]
@@ -446,8 +496,15 @@ def main(argv):
     host_only = "--host-only" in argv
     ge_only = "--ge-only" in argv
     alsd_only = "--alsd-only" in argv
-    argv = [a for a in argv if a not in ("--cano-only", "--mon-only", "--host-only", "--ge-only", "--alsd-only")]
+    cmm_all = "--cmm" in argv
+    argv = [a for a in argv if a not in ("--cano-only", "--mon-only", "--host-only", "--ge-only", "--alsd-only",
+                                         "--cmm")]
     names = argv or [n for n in RECIPES if not mon_only or RECIPES[n][2].get("mon")]
+    if cmm_all or any(n in CMM_RECIPES for n in names):
+        for n in [n for n in names if n in CMM_RECIPES] or list(CMM_RECIPES):
+            p = make_cmm(n)
+            print(n, "->", os.path.relpath(p, ROOT), os.path.getsize(p), "bytes", flush=True)
+        return
     for n in names:
         if alsd_only:
             p = add_alsd(n)
