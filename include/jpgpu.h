@@ -289,6 +289,29 @@ int jp_set_alsd(jp_ctx *ctx, int32_t on, int32_t nwindow, int32_t ndeg, const do
 int jp_get_lambda(jp_ctx *ctx, double *lambda, double *lambda_v);
 int jp_set_lambda(jp_ctx *ctx, double lambda, double lambda_v);
 
+/* The cell multipole method in place of ZD (md_run CAL15M= CMM with BOUNDA=
+ * NO, cmmdo.f90:79-288, neartb.f90, celpol.f90) for the following
+ * evaluations and runs of a non-periodic system: cminsiz is CELSIZ (A, md_run
+ * default 6), nlev CELLVL (default 3; raised, as incel.f90:42-52 does, until
+ * the level-1 cells cover the extent of the current coordinates), cmmupd
+ * CMMUPD (default 5): the far field is recomputed on the first step after
+ * every cmmupd-th and reused in between (md.f90:679-684), the tables and
+ * multipoles at every rebuild (jp_build_list, jp_run_nve's update).  The
+ * near field is Coulomb over every pair in a cell and its 26 neighbours
+ * without cutoff, LJ where both atoms carry it, plus LJ of the cells out to
+ * ceil(cutoff / cminsiz) for pairs within the cutoff at the rebuild; there is
+ * no ZD self or excess term.  Everything runs in FP64 whatever precision a
+ * call asks for.  jp_build_list returns md_run's ier 28 (an atom off the
+ * cells, too many in one), 51 (a table over its capacity) or 99 (an
+ * excluded pair outside the tables); jp_nonbond computes the far field
+ * afresh; jp_get_list returns the level-1 cell (1-based per axis) of every
+ * atom and no pair list; jp_count_pairs returns JP_EINVAL.  A new jp_set_state
+ * starts with the far field due.  on = 0 returns to ZD.  Returns JP_EINVAL
+ * for a batch or a context of several systems, cminsiz <= 0, nlev outside
+ * 1..5, cmmupd < 1, or coordinates that need more than 5 levels (md_run
+ * stops there). */
+int jp_set_cmm(jp_ctx *ctx, int32_t on, double cminsiz, int32_t nlev, int32_t cmmupd);
+
 /* Mean duration in seconds of the force-step pair kernel (the kernel of every
  * step without energies) over reps launches back to back on the current list
  * and coordinates, measured with two HIP events around the launches.  A

@@ -22,6 +22,7 @@
 #include "jp_cano.hpp"
 #include "jp_ge.hpp"
 #include "jp_alsd.hpp"
+#include "jp_cmm.hpp"
 
 #define CK(x)                                                          \
     do {                                                               \
@@ -169,6 +170,8 @@ static void md_cell_windows(SysP &P, int nv, double siz, double rlim, std::vecto
 struct HostSys {
     int n = 0, ntyp = 0, nfrag = 1, nfrg = 1, lmd = 0;
     double lambda = 0.0, lambda_v = 0.0, dself[3] = {0, 0, 0};
+    double dself_zd[3] = {0, 0, 0};   // the ZD self energy (dself is 0 under CMM)
+    double cutoff = 0.0, ccoef = 0.0;  // CUTLEN; 332.06378 / DIEVAL
     int nexcl = 0;
     int n15mx = 0;
 };
@@ -185,6 +188,7 @@ struct Engine {
     std::vector<int> atom_sys_h, slot_sys_h, cell_sys_h, clu_sys_h;
     std::vector<int> atom_cls_h;    // cluster (icls - 1) of every atom
     std::vector<double> mass_h;
+    std::vector<double> charge_h;   // raw charges (fxchrg)
     hipStream_t st = nullptr;
     hipEvent_t ev[8];               // [0,1] run, [2,3] list build, [4,5] host-launched pair kernels
     bool pe_pending = false;        // ev[4,5] bracket a pair launch not yet added to t_pair
@@ -227,13 +231,14 @@ struct Engine {
     // within rlim asymmetrically in some system (a half list holds a pair
     // either both ways or not at all).
     bool want_half(int prec) const {
-        if (!half_opt || prec != JP_PREC_FP32_FIXED) return false;
+        if (cmm || !half_opt || prec != JP_PREC_FP32_FIXED) return false;
         for (int s = 0; s < nsys; ++s)
             if (cell_table && sp[s].ctab && !sp[s].ctab_sym) return false;
         return true;
     }
     // the list the pair kernel of `prec` runs on
     int ensure_list(int prec) {
+        if (cmm) return have_list ? 0 : build_list();
         if (have_list && list_half == want_half(prec)) return 0;
         list_half = want_half(prec);
         return build_list();
@@ -298,6 +303,27 @@ struct Engine {
     int alsd_force(int prec);
     int alsd_fac();
     DBuf<double> d_lambk;               // lambda's state beside d_xbak (a deferred rebuild's redo)
+    // CAL15M= CMM (jp_set_cmm, jp_cmm.hpp): the cell multipole method in
+    // place of ZD for one non-periodic system, FP64 whatever the precision
+    // asked for.  Slots are atoms (inum = identity, no images).
+    bool cmm = false;
+    int cmm_upd = 5;                    // CMMUPD: far field every cmm_upd steps (md.f90:679-684)
+    bool cmm_itwin = true;              // md_run's itwin: the next step computes the far field
+    bool cmm_far_now = true;            // the far field of the force evaluation being issued
+    CmmArgs cm{};
+    DBuf<double> d_cmm_cg, d_cmm_c0, d_cmm_q, d_cmm_sx0, d_cmm_pot, d_cmm_gnb, d_cmm_enb, d_cmm_gwk, d_cmm_efar,
+        d_cmm_eatom, d_cmm_wb, d_cmm_wb2;
+    DBuf<double4> d_cmm_sx;
+    DBuf<int> d_cmm_key, d_cmm_kcnt, d_cmm_kstart, d_cmm_kfill, d_cmm_tmp, d_cmm_sorig, d_cmm_spos, d_cmm_sflag,
+        d_cmm_smol, d_cmm_npc, d_cmm_err;
+    int set_cmm(int on, double cminsiz, int nlev, int cmmupd);
+    int cmm_celprp();
+    int cmm_forces(bool energy, bool itwin);
+    int cmm_sums();
+    // the far field of the step after `k` steps of a segment that starts now
+    bool cmm_due(int k) const { return k == 0 ? cmm_itwin : (idloop + k) % cmm_upd == 0; }
+    // non-bonded forces of the current method into d_gslot
+    int nb(int prec, bool energy, bool itwin);
     int lam_upload();                   // host lambda and velocity -> device state
     int lam_download();                 // device state -> host lambda and velocity
     // energy sums on the device: per-system atom and cluster ranges, results,
@@ -358,6 +384,11 @@ struct Engine {
         d_gepar.release(); d_gew.release(); d_eslot.release();
         d_alpar.release(); d_lam.release(); d_lamb.release(); d_lamb2.release(); d_lmass.release(); d_dself.release();
         d_lgrp.release(); d_esys.release(); d_ecls.release(); d_lambk.release();
+        d_cmm_cg.release(); d_cmm_c0.release(); d_cmm_q.release(); d_cmm_sx0.release(); d_cmm_pot.release();
+        d_cmm_gnb.release(); d_cmm_enb.release(); d_cmm_gwk.release(); d_cmm_efar.release(); d_cmm_eatom.release();
+        d_cmm_wb.release(); d_cmm_wb2.release(); d_cmm_sx.release(); d_cmm_key.release(); d_cmm_kcnt.release();
+        d_cmm_kstart.release(); d_cmm_kfill.release(); d_cmm_tmp.release(); d_cmm_sorig.release();
+        d_cmm_spos.release(); d_cmm_sflag.release(); d_cmm_smol.release(); d_cmm_npc.release(); d_cmm_err.release();
         if (h_rsum) hipHostFree(h_rsum);
         if (h_rfix) hipHostFree(h_rfix);
         if (h_err) hipHostFree(h_err);
@@ -424,6 +455,7 @@ struct Engine {
     struct StepGraph {
         hipGraphExec_t exec = nullptr;
         int nsteps = 0, prec = -1, ccap = -1, variant = -1;
+        int cmm_key = -1;   // CMM: far-field phase of the segment's first step, 2 * (idloop % CMMUPD) + itwin
         const void *ent = nullptr;
     };
     std::vector<StepGraph> graphs;
@@ -501,6 +533,8 @@ int Engine::init(const jp_sys_desc *descs, int ns) {
         if (n <= 0 || nt <= 0 || nt > 64 || d.nfrag < 1 || d.nfrag > 2 || d.nfrg < 1 || d.nfrg > 3) return JP_EINVAL;
         h.n = n; h.ntyp = nt; h.nfrag = d.nfrag; h.nfrg = d.nfrg; h.lmd = d.lmd; h.lambda = d.lambda;
         h.nexcl = d.nexcl;
+        h.cutoff = d.cutoff;
+        h.ccoef = 332.06378 / d.diel;
         // box (inpmd.f90:578-583, 764-768) and cell grids
         double rc = d.cutoff, rlim = rc + 1.5;
         for (int k = 0; k < 3; ++k) {
@@ -592,6 +626,7 @@ int Engine::init(const jp_sys_desc *descs, int ns) {
         for (int i = 0; i < n; ++i) {
             for (int k = 0; k < 3; ++k) x.push_back(d.x[3 * i + k]);
             chg.push_back(fac * d.charge[i]);
+            charge_h.push_back(d.charge[i]);
             mass.push_back(d.mass[i]);
             imass.push_back(1.0 / d.mass[i]);
             int t = d.atype[i] - 1, c = d.icls[i] - 1;
@@ -640,6 +675,7 @@ int Engine::init(const jp_sys_desc *descs, int ns) {
             double sum = 0.0;
             for (int i = 0; i < n; ++i) sum = sum + d.charge[i] * dwork[(size_t)i * 3 + k];
             h.dself[k] = -P.zcore * sum * 0.5 * ccoef;
+            h.dself_zd[k] = h.dself[k];
         }
         // SHAKE groups
         for (int g = 0; g < d.nshake; ++g) {
@@ -821,7 +857,7 @@ int Engine::init(const jp_sys_desc *descs, int ns) {
     AL(d_part, 8 * (size_t)JP_SEG_NB * ns); AL(d_partf, 8 * (size_t)JP_SEG_NB * ns);
     CK(hipHostMalloc((void **)&h_rsum, sizeof(double) * 10 * (size_t)ns, hipHostMallocDefault));
     CK(hipHostMalloc((void **)&h_rfix, sizeof(long long) * 6 * (size_t)ns, hipHostMallocDefault));
-    CK(hipHostMalloc((void **)&h_err, sizeof(int), hipHostMallocDefault));
+    CK(hipHostMalloc((void **)&h_err, 2 * sizeof(int), hipHostMallocDefault));
     CK(hipHostMalloc((void **)&h_lst, sizeof(int) * (2 + (size_t)ns), hipHostMallocDefault));
     AL(d_xbak, 3 * (size_t)natom); AL(d_vbak, 3 * (size_t)natom); AL(d_errbak, 1);
     std::vector<int> abeg(ns), aend(ns), cbeg(ns), cend(ns);
@@ -956,6 +992,19 @@ int Engine::regrid(int shift) {
 // celprp: bin and sort the atoms, then build the j-lists into segments of
 // ccap entries per i-cluster.  One status read-back per rebuild.
 int Engine::build_list(bool defer) {
+    if (cmm) {   // synchronous: celprp's ier at once
+        CK(hipEventRecord(ev[2], st));
+        const int r = cmm_celprp();
+        CK(hipEventRecord(ev[3], st));
+        CK(hipEventSynchronize(ev[3]));
+        float ms = 0;
+        hipEventElapsedTime(&ms, ev[2], ev[3]);
+        last_build_ms = ms;
+        t_list += ms * 1e-3;
+        n_list++;
+        have_list = r == 0;
+        return r;
+    }
     CK(hipEventRecord(ev[2], st));
     const int nstat = 2 + nsys;
     const int nrs = std::max(std::max(std::max(ncell + 1, nslot), std::max(nsys, nstat)), nmcnt);
@@ -992,6 +1041,7 @@ int Engine::build_list(bool defer) {
 }
 
 int Engine::finish_list(bool retry) {
+    if (cmm) return 0;   // CMM rebuilds are checked when they run
     const int nstat = 2 + nsys;
     CK(hipEventSynchronize(ev[3]));
     for (int attempt = 0;; ++attempt) {
@@ -1066,10 +1116,10 @@ int Engine::time_pair(int prec, int reps, double &sec) {
     if ((r = ensure_list(prec))) return r;
     if ((r = set_factors(-1))) return r;
     if ((r = refresh_sorted())) return r;
-    if ((r = pair(prec, false))) return r;
+    if ((r = nb(prec, false, false))) return r;
     CK(hipEventRecord(ev[0], st));
     for (int k = 0; k < reps; ++k)
-        if ((r = pair(prec, false))) return r;
+        if ((r = nb(prec, false, false))) return r;
     CK(hipEventRecord(ev[1], st));
     if ((r = facc_zero())) return r;   // the timing launches' sums are not used
     CK(hipEventSynchronize(ev[1]));
@@ -1144,12 +1194,17 @@ int Engine::alsd_fac() {
     return 0;
 }
 
+int Engine::nb(int prec, bool energy, bool itwin) {
+    if (cmm) return cmm_forces(energy, itwin);
+    int r = pair(prec, energy);
+    if (!r) r = pair_out(prec);
+    return r;
+}
+
 int Engine::forces(int prec, bool energy) {
     int r = alsd_fac();
     if (r) return r;
-    r = pair(prec, energy || ge || alsd);
-    if (!r) r = pair_out(prec);
-    if (r) return r;
+    if ((r = nb(prec, energy || ge || alsd, cmm_far_now))) return r;
     if (nbat) {
         hipLaunchKernelGGL(k_bonded, grid1(nbat, 128), 128, 0, st, bonded_args(), d_atom_sys.p, d_sp.p, d_x.p,
                            d_efac.p, d_inum.p, d_gslot.p, (double *)nullptr);
@@ -1163,12 +1218,17 @@ int Engine::forces(int prec, bool energy) {
 // ge_weight), then lambda's force and half-step velocity (k_alsd).
 int Engine::alsd_force(int prec) {
     int r;
-    hipLaunchKernelGGL(k_excess_energy, grid1(natom, 256), 256, 0, st, natom, d_atom_sys.p, d_sp.p, d_x.p, d_chg.p,
-                       d_eptr.p, d_eidx.p, d_eexc.p);
-    if ((r = seg_sum<double>(d_abeg.p, d_aend.p, 3, d_eexc.p, d_part.p, d_rsum.p + 6 * (size_t)nsys))) return r;
-    if (prec == JP_PREC_FP64) r = seg_sum<double>(d_cbeg.p, d_cend.p, 6, d_ep64.p, d_part.p, d_rsum.p);
-    else r = seg_sum<long long>(d_cbeg.p, d_cend.p, 6, d_epfix.p, d_partf.p, d_rfix.p);
-    if (r) return r;
+    if (cmm) {
+        if ((r = cmm_sums())) return r;
+        prec = JP_PREC_FP64;
+    } else {
+        hipLaunchKernelGGL(k_excess_energy, grid1(natom, 256), 256, 0, st, natom, d_atom_sys.p, d_sp.p, d_x.p,
+                           d_chg.p, d_eptr.p, d_eidx.p, d_eexc.p);
+        if ((r = seg_sum<double>(d_abeg.p, d_aend.p, 3, d_eexc.p, d_part.p, d_rsum.p + 6 * (size_t)nsys))) return r;
+        if (prec == JP_PREC_FP64) r = seg_sum<double>(d_cbeg.p, d_cend.p, 6, d_ep64.p, d_part.p, d_rsum.p);
+        else r = seg_sum<long long>(d_cbeg.p, d_cend.p, 6, d_epfix.p, d_partf.p, d_rfix.p);
+        if (r) return r;
+    }
     const int ne = nbt + nat + ntt + 2 * n14;
     if (ne) {
         hipLaunchKernelGGL(k_bonded_energy, grid1(ne, 256), 256, 0, st, nbt, nat, ntt, n14, bonded_args(),
@@ -1307,12 +1367,17 @@ int Engine::set_alsd(int on, int nw, int nd, const double *c, const double *low,
 // into d_gew (jp_ge.hpp).  Launches only: capturable in a step graph.
 int Engine::ge_weight(int prec) {
     int r;
-    hipLaunchKernelGGL(k_excess_energy, grid1(natom, 256), 256, 0, st, natom, d_atom_sys.p, d_sp.p, d_x.p, d_chg.p,
-                       d_eptr.p, d_eidx.p, d_eexc.p);
-    if ((r = seg_sum<double>(d_abeg.p, d_aend.p, 3, d_eexc.p, d_part.p, d_rsum.p + 6 * (size_t)nsys))) return r;
-    if (prec == JP_PREC_FP64) r = seg_sum<double>(d_cbeg.p, d_cend.p, 6, d_ep64.p, d_part.p, d_rsum.p);
-    else r = seg_sum<long long>(d_cbeg.p, d_cend.p, 6, d_epfix.p, d_partf.p, d_rfix.p);
-    if (r) return r;
+    if (cmm) {
+        if ((r = cmm_sums())) return r;
+        prec = JP_PREC_FP64;
+    } else {
+        hipLaunchKernelGGL(k_excess_energy, grid1(natom, 256), 256, 0, st, natom, d_atom_sys.p, d_sp.p, d_x.p,
+                           d_chg.p, d_eptr.p, d_eidx.p, d_eexc.p);
+        if ((r = seg_sum<double>(d_abeg.p, d_aend.p, 3, d_eexc.p, d_part.p, d_rsum.p + 6 * (size_t)nsys))) return r;
+        if (prec == JP_PREC_FP64) r = seg_sum<double>(d_cbeg.p, d_cend.p, 6, d_ep64.p, d_part.p, d_rsum.p);
+        else r = seg_sum<long long>(d_cbeg.p, d_cend.p, 6, d_epfix.p, d_partf.p, d_rfix.p);
+        if (r) return r;
+    }
     const int ne = nbt + nat + ntt + 2 * n14;
     if (ne) {
         hipLaunchKernelGGL(k_bonded_energy, grid1(ne, 256), 256, 0, st, nbt, nat, ntt, n14, bonded_args(),
@@ -1610,14 +1675,20 @@ int Engine::energies(int prec, std::vector<double> &eel, std::vector<double> &ev
     evd.assign(3 * (size_t)nsys, 0.0);
     eex.assign(3 * (size_t)nsys, 0.0);
     // per-atom excess energies and per-cluster pair energies, summed per
-    // system on the device in a fixed order; one small read-back
-    hipLaunchKernelGGL(k_excess_energy, grid1(natom, 256), 256, 0, st, natom, d_atom_sys.p, d_sp.p, d_x.p, d_chg.p,
-                       d_eptr.p, d_eidx.p, d_eexc.p);
+    // system on the device in a fixed order; one small read-back.  CMM: the
+    // per-atom near and far energies, no excess or self term.
     int r;
-    if ((r = seg_sum<double>(d_abeg.p, d_aend.p, 3, d_eexc.p, d_part.p, d_rsum.p + 6 * (size_t)nsys))) return r;
-    if (prec == JP_PREC_FP64) r = seg_sum<double>(d_cbeg.p, d_cend.p, 6, d_ep64.p, d_part.p, d_rsum.p);
-    else r = seg_sum<long long>(d_cbeg.p, d_cend.p, 6, d_epfix.p, d_partf.p, d_rfix.p);
-    if (r) return r;
+    if (cmm) {
+        if ((r = cmm_sums())) return r;
+        prec = JP_PREC_FP64;
+    } else {
+        hipLaunchKernelGGL(k_excess_energy, grid1(natom, 256), 256, 0, st, natom, d_atom_sys.p, d_sp.p, d_x.p,
+                           d_chg.p, d_eptr.p, d_eidx.p, d_eexc.p);
+        if ((r = seg_sum<double>(d_abeg.p, d_aend.p, 3, d_eexc.p, d_part.p, d_rsum.p + 6 * (size_t)nsys))) return r;
+        if (prec == JP_PREC_FP64) r = seg_sum<double>(d_cbeg.p, d_cend.p, 6, d_ep64.p, d_part.p, d_rsum.p);
+        else r = seg_sum<long long>(d_cbeg.p, d_cend.p, 6, d_epfix.p, d_partf.p, d_rfix.p);
+        if (r) return r;
+    }
     CK(hipMemcpyAsync(h_rsum, d_rsum.p, sizeof(double) * 9 * (size_t)nsys, hipMemcpyDeviceToHost, st));
     if (prec != JP_PREC_FP64)
         CK(hipMemcpyAsync(h_rfix, d_rfix.p, sizeof(long long) * 6 * (size_t)nsys, hipMemcpyDeviceToHost, st));
@@ -1658,8 +1729,8 @@ int Engine::nonbond(const double *cord, double *grad, double *eelec, double *evd
     std::vector<double> g(3 * (size_t)natom);
     for (int k = 0; k < nfrg; ++k) {
         if ((r = set_factors(nfrg == 1 ? -1 : k))) return r;
-        if ((r = pair(prec, k == 0))) return r;
-        if ((r = pair_out(prec))) return r;
+        // CMM: the far field once, at the first class (md_run's itwin = 1)
+        if ((r = nb(prec, k == 0, k == 0))) return r;
         if (k == 0 && (r = energies(prec, eel, evd, eex))) return r;
         hipLaunchKernelGGL(k_kick, grid1(natom, 256), 256, 0, st, natom, d_atom_sys.p, d_sp.p, d_inum.p, d_gslot.p,
                            d_mass.p, d_x.p, d_v.p, d_xold.p, (double *)nullptr, d_grad.p, 0);
@@ -1854,6 +1925,11 @@ int Engine::save_state(double *x, double *v, int *e) {
     if (alsd)
         CK(hipMemcpyAsync(x == d_xs2.p ? d_lamb2.p : d_lamb.p, d_lam.p, 8 * (size_t)JP_LS * nsys,
                           hipMemcpyDeviceToDevice, st));
+    if (cmm) {   // the far field a redone step may reuse
+        double *w = x == d_xs2.p ? d_cmm_wb2.p : d_cmm_wb.p;
+        CK(hipMemcpyAsync(w, d_cmm_gwk.p, 72 * (size_t)natom, hipMemcpyDeviceToDevice, st));
+        CK(hipMemcpyAsync(w + 9 * (size_t)natom, d_cmm_efar.p, 24 * (size_t)natom, hipMemcpyDeviceToDevice, st));
+    }
     return 0;
 }
 
@@ -1864,6 +1940,11 @@ int Engine::load_state(const double *x, const double *v, const int *e) {
     if (alsd)
         CK(hipMemcpyAsync(d_lam.p, x == d_xs2.p ? d_lamb2.p : d_lamb.p, 8 * (size_t)JP_LS * nsys,
                           hipMemcpyDeviceToDevice, st));
+    if (cmm) {
+        const double *w = x == d_xs2.p ? d_cmm_wb2.p : d_cmm_wb.p;
+        CK(hipMemcpyAsync(d_cmm_gwk.p, w, 72 * (size_t)natom, hipMemcpyDeviceToDevice, st));
+        CK(hipMemcpyAsync(d_cmm_efar.p, w + 9 * (size_t)natom, 24 * (size_t)natom, hipMemcpyDeviceToDevice, st));
+    }
     return refresh_sorted();
 }
 
@@ -1886,6 +1967,7 @@ int Engine::cano_retry_step(int prec) {
     double T = dt0;
     int ishk = 0, jshk = 1;
     ++n_retry;
+    // every substep evaluates the forces with the step's itwin (cmm_far_now)
     for (;;) {
         if ((r = save_state(d_xs2.p, d_vs2.p, nullptr))) break;
         if (sp[0].dt != T) {
@@ -1923,6 +2005,7 @@ int Engine::cano_retry_step(int prec) {
 int Engine::cano_steps_eager(int prec, int n) {
     int r;
     for (int k = 0; k < n; ++k) {
+        cmm_far_now = cmm_due(k);
         if ((r = save_state(d_xs.p, d_vs.p, d_errs.p))) return r;
         if ((r = forces(prec, false)) || (r = integrate(prec, false))) return r;
         bool failed = false;
@@ -1951,7 +2034,8 @@ int Engine::restart_step(int prec, jp_energy_log *log) {
     if (alsd && (r = lam_upload())) return r;
     CK(hipMemsetAsync(d_err.p, 0, 4, st));
     CK(hipMemcpyAsync(d_xbak.p, d_x.p, 24 * (size_t)natom, hipMemcpyDeviceToDevice, st));
-    if ((r = alsd_fac()) || (r = pair(prec, true)) || (r = pair_out(prec))) return r;
+    cmm_far_now = cmm_itwin;
+    if ((r = alsd_fac()) || (r = nb(prec, true, cmm_far_now))) return r;
     if (nbat) {
         hipLaunchKernelGGL(k_bonded, grid1(nbat, 128), 128, 0, st, bonded_args(), d_atom_sys.p, d_sp.p, d_x.p,
                            d_efac.p, d_inum.p, d_gslot.p, (double *)nullptr);
@@ -1996,8 +2080,10 @@ void Engine::drop_graphs() {
 // Graph of n plain steps for the current list segment size and buffers.
 Engine::StepGraph *Engine::step_graph(int n, int prec) {
     const int variant = pair_variant();
+    const int ckey = cmm ? 2 * (int)(idloop % cmm_upd) + (cmm_itwin ? 1 : 0) : -1;
     for (auto &g : graphs)
-        if (g.exec && g.nsteps == n && g.prec == prec && g.ccap == ccap && g.ent == d_ent.p && g.variant == variant)
+        if (g.exec && g.nsteps == n && g.prec == prec && g.ccap == ccap && g.ent == d_ent.p && g.variant == variant &&
+            g.cmm_key == ckey)
             return &g;
     // the list buffers changed: stale graphs go
     for (auto it = graphs.begin(); it != graphs.end();) {
@@ -2009,12 +2095,13 @@ Engine::StepGraph *Engine::step_graph(int n, int prec) {
         }
     }
     StepGraph g;
-    g.nsteps = n; g.prec = prec; g.ccap = ccap; g.ent = d_ent.p; g.variant = variant;
+    g.nsteps = n; g.prec = prec; g.ccap = ccap; g.ent = d_ent.p; g.variant = variant; g.cmm_key = ckey;
     ++n_captures;
     hipGraph_t graph = nullptr;
     if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) return nullptr;
     int r = 0;
     for (int k = 0; k < n && !r; ++k) {
+        cmm_far_now = cmm_due(k);
         r = forces(prec, false);
         if (!r) r = integrate(prec, false);
     }
@@ -2099,7 +2186,7 @@ int Engine::run_nve(int nsteps, int update, int nmon, int prec, jp_energy_log *l
     if ((r = ensure_list(prec))) return r;
     if ((r = set_factors(-1))) return r;
     if ((r = refresh_sorted())) return r;
-    if (autotune && !tuned && (r = tune(prec, update))) return r;
+    if (autotune && !tuned && !cmm && (r = tune(prec, update))) return r;
     if (alsd && (r = lam_upload())) return r;
     CK(hipMemsetAsync(d_err.p, 0, 4, st));
     t_pair = 0; t_step = 0; n_pair = 0; n_pair_steps = nsteps;
@@ -2172,6 +2259,7 @@ int Engine::run_nve(int nsteps, int update, int nmon, int prec, jp_energy_log *l
                 }
                 step += nplain;
                 idloop += nplain;
+                cmm_itwin = idloop % cmm_upd == 0;   // md.f90:679-684
                 if (idloop % update == 0) {
                     if ((r = build_list(true))) return r;
                     pending = true;
@@ -2195,11 +2283,11 @@ int Engine::run_nve(int nsteps, int update, int nmon, int prec, jp_energy_log *l
             pe_pending = false;
         }
         if ((r = alsd_fac())) return r;
+        cmm_far_now = cmm_itwin;
         CK(hipEventRecord(ev[4], st));
-        if ((r = pair(prec, mon || ge || alsd))) return r;
+        if ((r = nb(prec, mon || ge || alsd, cmm_far_now))) return r;
         CK(hipEventRecord(ev[5], st));
         pe_pending = true;
-        if ((r = pair_out(prec))) return r;
         if (nbat) {
             hipLaunchKernelGGL(k_bonded, grid1(nbat, 128), 128, 0, st, bonded_args(), d_atom_sys.p, d_sp.p, d_x.p,
                                d_efac.p, d_inum.p, d_gslot.p, (double *)nullptr);
@@ -2227,6 +2315,7 @@ int Engine::run_nve(int nsteps, int update, int nmon, int prec, jp_energy_log *l
         }
         ++idloop;
         ++step;
+        cmm_itwin = idloop % cmm_upd == 0;
         if (idloop % update == 0) {     // md.f90:655-660, after the step
             if ((r = build_list())) return r;
         }
@@ -2285,8 +2374,198 @@ int Engine::prepare_graphs(int nsteps, int update, int nmon, int prec) {
     return 0;
 }
 
+
+// ------------------------------------------------------------------ CMM
+
+// CAL15M= CMM for the following evaluations and runs (one system): the cell
+// levels from CELSIZ and CELLVL, raised to cover the extent of the current
+// coordinates (incel.f90:29-57, 137-150), the far field every CMMUPD steps.
+int Engine::set_cmm(int on, double cminsiz, int nlev, int cmmupd) {
+    drop_graphs();
+    have_list = false;
+    if (!on) {
+        if (cmm) {
+            cmm = false;
+            for (auto &h : hs)
+                for (int k = 0; k < 3; ++k) h.dself[k] = h.dself_zd[k];
+            if (alsd) {
+                std::vector<double> ds(3 * (size_t)nsys);
+                for (int s = 0; s < nsys; ++s)
+                    for (int k = 0; k < 3; ++k) ds[3 * s + k] = hs[s].dself[k];
+                CK(hipMemcpyAsync(d_dself.p, ds.data(), 8 * ds.size(), hipMemcpyHostToDevice, st));
+                CK(hipStreamSynchronize(st));
+            }
+        }
+        return 0;
+    }
+    if (nsys != 1 || !(cminsiz > 0.0) || !std::isfinite(cminsiz) || nlev < 1 || nlev > JP_CMM_MAXLEV || cmmupd < 1)
+        return JP_EINVAL;
+    std::vector<double> x(3 * (size_t)natom);
+    CK(hipMemcpyAsync(x.data(), d_x.p, 8 * x.size(), hipMemcpyDeviceToHost, st));
+    CK(hipStreamSynchronize(st));
+    double ext = 0.0;
+    for (int k = 0; k < 3; ++k) {
+        double lo = x[k], hi = x[k];
+        for (int i = 1; i < natom; ++i) {
+            lo = std::min(lo, x[3 * (size_t)i + k]);
+            hi = std::max(hi, x[3 * (size_t)i + k]);
+        }
+        ext = k == 0 ? hi - lo : std::max(hi - lo, ext);
+    }
+    if ((double)(1 << (nlev + 1)) * cminsiz < ext) {
+        nlev = (int)std::ceil(std::log(ext / cminsiz) / std::log(2.0) - 1.0);
+        if (nlev > JP_CMM_MAXLEV) return JP_EINVAL;   // md_run stops (incel.f90:46-52)
+    }
+    const HostSys &h = hs[0];
+    CmmArgs A{};
+    A.n = natom; A.nf = h.nfrag; A.nfrg = h.nfrg; A.nlev = nlev; A.ntyp = h.ntyp;
+    for (int i = 0; i < nlev; ++i) {
+        A.nv[nlev - 1 - i] = 4 << i;
+        A.siz[i] = cminsiz * (double)(1 << i);
+    }
+    A.nv1 = A.nv[0];
+    A.nc1 = A.nv1 * A.nv1 * A.nv1;
+    A.rlim2 = h.cutoff * h.cutoff;
+    A.idelay = (int)std::ceil(h.cutoff / A.siz[0]);
+    A.ccoef = h.ccoef;
+    const double vol = std::pow(4.0 * A.siz[0], 3.0);
+    A.ipmax = (int)std::ceil(A.siz[0] * A.siz[0] * A.siz[0] * 0.27) * 2;
+    A.capvd = (int)std::ceil(0.15 * vol) * 2;
+    A.capel = A.capvd;
+    A.capvw = (int)std::ceil(0.027 * vol) * 2;
+    A.nkey = A.nc1 * A.nf * 2;
+    std::vector<double> cg;
+    int npc = 0;
+    for (int l = 0; l < nlev; ++l) {
+        A.cgoff[l] = (int)cg.size();
+        A.pcoff[l] = npc;
+        const double r = -0.5 * (A.nv[l] + 1);   // mk_c: centres siz (i - (nv + 1)/2), 1-based i
+        for (int i = 1; i <= A.nv[l]; ++i) cg.push_back(A.siz[l] * ((double)i + r));
+        npc += A.nv[l] * A.nv[l] * A.nv[l] * A.nf;
+    }
+    const size_t n = (size_t)natom;
+    AL(d_cmm_cg, cg.size()); AL(d_cmm_c0, 3); AL(d_cmm_q, n); AL(d_cmm_sx0, 3 * n); AL(d_cmm_pot, 13 * (size_t)npc);
+    AL(d_cmm_gnb, 9 * n); AL(d_cmm_enb, 6 * n); AL(d_cmm_gwk, 9 * n); AL(d_cmm_efar, 3 * n); AL(d_cmm_eatom, 6 * n);
+    AL(d_cmm_wb, 12 * n); AL(d_cmm_wb2, 12 * n); AL(d_cmm_sx, n);
+    AL(d_cmm_key, n); AL(d_cmm_kcnt, A.nkey); AL(d_cmm_kstart, A.nkey + 1); AL(d_cmm_kfill, A.nkey);
+    AL(d_cmm_tmp, n); AL(d_cmm_sorig, n); AL(d_cmm_spos, n); AL(d_cmm_sflag, n); AL(d_cmm_smol, n);
+    AL(d_cmm_npc, npc); AL(d_cmm_err, 2);
+    CK(hipMemcpyAsync(d_cmm_cg.p, cg.data(), 8 * cg.size(), hipMemcpyHostToDevice, st));
+    CK(hipMemcpyAsync(d_cmm_q.p, charge_h.data(), 8 * n, hipMemcpyHostToDevice, st));
+    CK(hipMemsetAsync(d_cmm_gwk.p, 0, 72 * n, st));
+    CK(hipMemsetAsync(d_cmm_efar.p, 0, 24 * n, st));
+    // slots are atoms, with no periodic images
+    std::vector<int> id(n);
+    std::iota(id.begin(), id.end(), 0);
+    CK(hipMemcpyAsync(d_inum.p, id.data(), 4 * n, hipMemcpyHostToDevice, st));
+    CK(hipMemsetAsync(d_img.p, 0, 12 * n, st));
+    A.cg = d_cmm_cg.p; A.c0 = d_cmm_c0.p; A.x = d_x.p; A.chg = d_chg.p; A.q = d_cmm_q.p; A.tflag = d_tflag.p;
+    A.mol = d_mol.p; A.xptr = d_xptr.p; A.xidx = d_xidx.p; A.lj = d_lj64.p;
+    A.key = d_cmm_key.p; A.kcnt = d_cmm_kcnt.p; A.kstart = d_cmm_kstart.p; A.kfill = d_cmm_kfill.p;
+    A.tmp = d_cmm_tmp.p; A.sorig = d_cmm_sorig.p; A.spos = d_cmm_spos.p; A.sflag = d_cmm_sflag.p;
+    A.smol = d_cmm_smol.p; A.sx = d_cmm_sx.p; A.sx0 = d_cmm_sx0.p; A.npc = d_cmm_npc.p; A.pot = d_cmm_pot.p;
+    A.err = d_cmm_err.p; A.sp = d_sp.p; A.efac = d_efac.p; A.gnb = d_cmm_gnb.p; A.enb = d_cmm_enb.p;
+    A.gwk = d_cmm_gwk.p; A.efar = d_cmm_efar.p; A.gslot = d_gslot.p; A.eatom = d_cmm_eatom.p;
+    cm = A;
+    hs[0].dself[0] = hs[0].dself[1] = hs[0].dself[2] = 0.0;   // no ZD self energy under CMM
+    if (alsd) {
+        const double z[3] = {0.0, 0.0, 0.0};
+        CK(hipMemcpyAsync(d_dself.p, z, 24, hipMemcpyHostToDevice, st));
+    }
+    CK(hipStreamSynchronize(st));
+    cmm = true;
+    cmm_upd = cmmupd;
+    cmm_itwin = true;
+    list_half = false;
+    return 0;
+}
+
+// celprp under CMM: cell centres from the coordinate mean, binning and the
+// sort, cell counts and multipoles of every level, the table checks; one
+// read-back of the status (ier 28, 51, 99).
+int Engine::cmm_celprp() {
+    CmmArgs &A = cm;
+    const unsigned gn = grid1(natom, 256);
+    hipLaunchKernelGGL(k_cmm_mean, 1, 1024, 0, st, A);
+    hipLaunchKernelGGL(k_cmm_reset, grid1(std::max(A.nkey, 2), 256), 256, 0, st, A);
+    hipLaunchKernelGGL(k_cmm_bin, gn, 256, 0, st, A);
+    hipLaunchKernelGGL(k_scan, 1, 1024, 0, st, A.nkey, (const int *)A.kcnt, A.kstart);
+    hipLaunchKernelGGL(k_cmm_place, gn, 256, 0, st, A);
+    hipLaunchKernelGGL(k_cmm_rank, gn, 256, 0, st, A);
+    hipLaunchKernelGGL(k_cmm_fill, gn, 256, 0, st, A);
+    for (int l = 0; l < A.nlev; ++l) {
+        const int nq = A.nv[l] * A.nv[l] * A.nv[l] * A.nf;
+        hipLaunchKernelGGL(k_cmm_npc, grid1(nq, 256), 256, 0, st, A, l);
+    }
+    for (int l = 0; l < A.nlev; ++l) {
+        const int nq = A.nv[l] * A.nv[l] * A.nv[l] * A.nf;
+        hipLaunchKernelGGL(k_cmm_pol, grid1(nq, 128), 128, 0, st, A, l);
+    }
+    hipLaunchKernelGGL(k_cmm_check, gn, 256, 0, st, A);
+    CK(hipGetLastError());
+    CK(hipMemcpyAsync(h_err, d_cmm_err.p, 8, hipMemcpyDeviceToHost, st));
+    CK(hipStreamSynchronize(st));
+    if (h_err[0] & 1) return JP_IER_CELL;    // celpar.f90:172-192
+    if (h_err[0] & 2) return JP_IER_LIST;    // neartb.f90:368-400
+    if (h_err[1] != 2 * hs[0].nexcl) return JP_IER_EXCL;   // neartb.f90:446-453
+    return 0;
+}
+
+template <bool MULTI, bool E>
+static void launch_cmm(Engine &g, bool itwin) {
+    const CmmArgs &A = g.cm;
+    hipLaunchKernelGGL(k_cmm_gather, grid1(A.n, 256), 256, 0, g.st, A);
+    hipLaunchKernelGGL((k_cmm_near<MULTI, E>), A.nc1, 64, 0, g.st, A);
+    if (A.idelay > 1) hipLaunchKernelGGL((k_cmm_lj2<MULTI, E>), A.nc1, 64, 0, g.st, A);
+    if (itwin) hipLaunchKernelGGL(k_cmm_far<MULTI>, A.nc1, 64, 0, g.st, A);
+    hipLaunchKernelGGL(k_cmm_combine<E>, grid1(A.n, 256), 256, 0, g.st, A);
+}
+
+// enear_CMM_doub and the far field (when itwin) into d_gslot; with energy the
+// per-atom energies of the step.
+int Engine::cmm_forces(bool energy, bool itwin) {
+    const bool multi = hs[0].nfrg > 1;
+    if (multi) {
+        if (energy) launch_cmm<true, true>(*this, itwin);
+        else launch_cmm<true, false>(*this, itwin);
+    } else {
+        if (energy) launch_cmm<false, true>(*this, itwin);
+        else launch_cmm<false, false>(*this, itwin);
+    }
+    CK(hipGetLastError());
+    return 0;
+}
+
+__global__ void k_cmm_zero(double *p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0;
+}
+
+// The step's per-atom CMM energies summed per system in a fixed order into
+// the pair-energy slots of d_rsum (halved with them); no excess energy.
+int Engine::cmm_sums() {
+    hipLaunchKernelGGL(k_cmm_zero, 1, 64, 0, st, d_rsum.p + 6 * (size_t)nsys, 3 * nsys);
+    CK(hipGetLastError());
+    return seg_sum<double>(d_abeg.p, d_aend.p, 6, d_cmm_eatom.p, d_part.p, d_rsum.p);
+}
+
 int Engine::get_list(int32_t *cell3, int64_t *npairs, int32_t *ptr, int32_t *idx, int64_t cap) {
     if (!have_list) return JP_EINVAL;
+    if (cmm) {   // CMM keeps no pair list: the level-1 cell (1-based per axis) of every atom
+        if (ptr || idx) return JP_EINVAL;
+        std::vector<int> key(natom);
+        CK(hipMemcpyAsync(key.data(), d_cmm_key.p, 4 * (size_t)natom, hipMemcpyDeviceToHost, st));
+        CK(hipStreamSynchronize(st));
+        const int nv = cm.nv1;
+        for (int a = 0; a < natom && cell3; ++a) {
+            const int c = key[a] / (2 * cm.nf);
+            cell3[3 * a] = c % nv + 1;
+            cell3[3 * a + 1] = (c / nv) % nv + 1;
+            cell3[3 * a + 2] = c / (nv * nv) + 1;
+        }
+        if (npairs) *npairs = 0;
+        return 0;
+    }
     CK(hipStreamSynchronize(st));
     std::vector<int> perm(nslot);
     std::vector<int2> lc(nclu);
@@ -2331,7 +2610,7 @@ int Engine::get_list(int32_t *cell3, int64_t *npairs, int32_t *ptr, int32_t *idx
 // jp_count_pairs without allocation: per-cluster counts into the static
 // scratch, summed by one block in cluster order.
 int Engine::count_pairs_noalloc(int64_t *np, int64_t *nlj, int64_t *nlist) {
-    if (!have_list) return JP_EINVAL;
+    if (!have_list || cmm) return JP_EINVAL;
     if (nclu > JP_NOALLOC_MAX_CLU) return JP_EINVAL;   // beyond the scratch
     unsigned long long *sc = noalloc_scratch();
     if (!sc) return JP_ENODEV;
@@ -2350,7 +2629,7 @@ int Engine::count_pairs_noalloc(int64_t *np, int64_t *nlj, int64_t *nlist) {
 }
 
 int Engine::count_pairs(int64_t *np, int64_t *nlj, int64_t *nlist) {
-    if (!have_list) return JP_EINVAL;
+    if (!have_list || cmm) return JP_EINVAL;
     CK(hipMemsetAsync(d_u64.p, 0, 24, st));
     hipLaunchKernelGGL(k_count_pairs, nclu, 64, 0, st, nclu, d_clu_sys.p, d_sp.p, d_ys.p, ccap, d_lcnt.p,
                        d_ent.p, d_u64.p);
@@ -2397,6 +2676,7 @@ int jp_ctx_destroy(jp_ctx *ctx) {
 int jp_set_state(jp_ctx *ctx, const double *cord, const double *vel) {
     if (!ctx) return JP_EINVAL;
     Engine &E = ctx->e;
+    if (cord) E.cmm_itwin = true;   // CMM: a new start computes the far field first (incel's itwin = 1)
     if (cord) CK(hipMemcpyAsync(E.d_x.p, cord, 24 * (size_t)E.natom, hipMemcpyHostToDevice, E.st));
     if (vel) CK(hipMemcpyAsync(E.d_v.p, vel, 24 * (size_t)E.natom, hipMemcpyHostToDevice, E.st));
     if (cord && E.have_list) {
@@ -2666,6 +2946,11 @@ int jp_set_mcmd(jp_ctx *ctx, int32_t on, int32_t nwindow, int32_t ndeg, const do
                 const double *high, double alphalw, double alphaup, double celw, double ceup, double tempce) {
     if (!ctx) return JP_EINVAL;
     return ctx->e.set_mcmd(on, nwindow, ndeg, c, low, high, alphalw, alphaup, celw, ceup, tempce);
+}
+
+int jp_set_cmm(jp_ctx *ctx, int32_t on, double cminsiz, int32_t nlev, int32_t cmmupd) {
+    if (!ctx) return JP_EINVAL;
+    return ctx->e.set_cmm(on, cminsiz, nlev, cmmupd);
 }
 
 int jp_set_alsd(jp_ctx *ctx, int32_t on, int32_t nwindow, int32_t ndeg, const double *c, const double *low,

@@ -95,6 +95,15 @@ class MdSystem:
             idx.ctypes.data_as(native.i32p), len(idx)))
         return cell, ptr, idx[:n.value]
 
+    def cmm_cells(self):
+        """Under CMM: the level-1 cell (1-based per axis) of every atom at the
+        last rebuild, (N, 3)."""
+        n = C.c_int64()
+        cell = np.zeros((self.n, 3), np.int32)
+        native.check("jp_get_list", self.lib.jp_get_list(self.ctx, cell.ctypes.data_as(native.i32p), C.byref(n),
+                                                          None, None, 0))
+        return cell
+
     def cmmdo(self, cord=None, grad=None, precision: Optional[int] = None):
         """Returns (grad(nfrg,N,3) accumulated, eelecn, evancn, ehydcn, ier)."""
         prec = self.precision if precision is None else precision
@@ -202,6 +211,13 @@ class MdSystem:
         ceup, tempce (the NAMEGE= file), lmass (LWEIGH; <= 0: md_run's
         default) and tctl (LTCNTL); None switches it off."""
         self._al = _alsd_call(self.lib.jp_set_alsd, "jp_set_alsd", self.ctx, al)
+
+    def set_cmm(self, cminsiz: Optional[float] = 6.0, nlev: int = 3, cmmupd: int = 5):
+        """CAL15M= CMM (cell multipole method, BOUNDA= NO) for the following
+        evaluations and runs: CELSIZ, CELLVL and CMMUPD; cminsiz=None returns
+        to ZD.  A rebuild (celprp) follows before the next forces."""
+        on = 0 if cminsiz is None else 1
+        native.check("jp_set_cmm", self.lib.jp_set_cmm(self.ctx, on, float(cminsiz or 0.0), int(nlev), int(cmmupd)))
 
     def get_lambda(self):
         """(lambda, its velocity)."""
