@@ -96,13 +96,13 @@ RECIPES = {
 # tables every 10th).
 CMM_RECIPES = {
     # 481-water droplet, 15 A radius: 16^3 level-1 cells of 6 A, three levels
-    "droplet_cmm": ("water_droplet", dict(R=15.0, seed=21), dict(traj=20)),
+    "droplet_cmm": ("water_droplet", dict(R=15.0, seed=21), dict(traj=20, cano=12)),
     # compact 120-atom chain with bonds, angles, torsions and 1-4 terms in a droplet
     "chain_droplet_cmm": ("chain_droplet", dict(n_chain=120, R=14.0, seed=41, bond_fc=300.0, angle_fc=50.0,
                                                 torsion_fc=1.0), dict(traj=20)),
     # the same under LMD (two fragments, three energy classes)
     "chain_droplet_lmd": ("chain_droplet", dict(n_chain=120, R=14.0, seed=41, lmd=0.7, bond_fc=300.0,
-                                                angle_fc=50.0, torsion_fc=1.0), dict(traj=10)),
+                                                angle_fc=50.0, torsion_fc=1.0), dict(traj=10, cano=12, alsd="a")),
 }
 
 
@@ -374,7 +374,9 @@ def alsd_jobs(s: systems.System, tmp: str) -> dict:
 
 def make_cmm(name: str) -> str:
     """CAL15M= CMM, BOUNDA= NO fixture: md_run's FDEBUG= MAKE dump at the
-    stored coordinates and a MICR run with its NAMETO series."""
+    stored coordinates and a MICR run with its NAMETO series; with cano a
+    METHOD= CANO run at 300 K (TEMPCO= NO) and with alsd an ALSD run with the
+    parameters of ALSD_RUNS[alsd], each from md_run's restart set-up."""
     kind, args, opt = CMM_RECIPES[name]
     s = build_system(kind, args)
     out = dict(recipe=json.dumps(dict(kind=kind, args=args)), fingerprint=fingerprint(s), x=s.x, v=s.v)
@@ -396,6 +398,39 @@ def make_cmm(name: str) -> str:
         r = refio.read_restart(os.path.join(t2, "out.rst"))
         out.update(traj_x=r["x"], traj_v=r["v"], traj_nsteps=np.int64(n),
                    traj_ene=np.loadtxt(os.path.join(t2, "ene.dat"), ndmin=2))
+        if opt.get("cano"):
+            t3 = os.path.join(tmp, "cano")
+            os.makedirs(t3)
+            refio.write_restart(os.path.join(t3, "in.rst"), s.x, s.v, lam=s.lam)
+            nc = opt["cano"]
+            run_md(t3, s, dict(cmm, METHOD="CANO", RESTAR="YES", NAMERI="in.rst", NAMERO="out.rst",
+                               LOOPLI=str(nc), UPDATE="10", TIMEST="2.0", SETTEM="300.0", TEMPCO="NO",
+                               NAMETO="ene.dat", OUTLOG="100000", OUTCLE="100000"))
+            r = refio.read_restart(os.path.join(t3, "out.rst"))
+            out.update(cano_x=r["x"], cano_v=r["v"], cano_nsteps=np.int64(nc),
+                       cano_ene=np.loadtxt(os.path.join(t3, "ene.dat"), ndmin=2))
+        if opt.get("alsd"):
+            tag = opt["alsd"]
+            celw, ceup, tempco, ltc, lw = ALSD_RUNS[tag]
+            t4 = os.path.join(tmp, "alsd")
+            os.makedirs(t4)
+            refio.write_restart(os.path.join(t4, "in.rst"), s.x, s.v, lam=s.lam)
+            write_alsd_params(os.path.join(t4, "ge.par"), celw, ceup)
+            inp = {"LTCNTL": ltc}
+            if lw is not None:
+                inp["LWEIGH"] = repr(float(lw))
+            run_md(t4, s, dict(cmm, METHOD="GE", RESTAR="YES", NAMERI="in.rst", NAMERO="out.rst", UPDATE="10",
+                               TIMEST="2.0", SETTEM="300.0", TEMPCO=tempco, NAMEGE="ge.par", NAMETO="ene.dat",
+                               LOOPLI="10", OUTLOG="100000", OUTCLE="100000"), inp=inp)
+            r = refio.read_restart(os.path.join(t4, "out.rst"))
+            out.update({"alsd_poly": ALSD_POLY, "alsd_low": ALSD_LOW, "alsd_high": ALSD_HIGH,
+                        "alsd_walls": np.array(ALSD_WALLS), "alsd_tempce": np.array(ALSD_TEMPCE),
+                        f"alsd_{tag}_x": r["x"], f"alsd_{tag}_v": r["v"], f"alsd_{tag}_lam": np.float64(r["lam"]),
+                        f"alsd_{tag}_lamv": np.float64(r["lam_v"]), f"alsd_{tag}_loop": np.int64(r["loop"]),
+                        f"alsd_{tag}_range": np.array([celw, ceup]),
+                        f"alsd_{tag}_ene": np.loadtxt(os.path.join(t4, "ene.dat"), ndmin=2),
+                        f"alsd_{tag}_opt": np.array([1 if tempco == "YES" else 0, 1 if ltc == "YES" else 0,
+                                                     0.0 if lw is None else lw])})
     finally:
         shutil.rmtree(tmp, ignore_errors=True)
     path = os.path.join(HERE, f"{name}.npz")

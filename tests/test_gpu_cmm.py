@@ -138,3 +138,47 @@ def test_cmm_ier_matches_oracle(golden, cminsiz):
     md.close()
     print(f"ier {ier} oracle {ref}", flush=True)
     assert ier == ref
+
+
+@pytest.mark.parametrize("name", ["droplet_cmm", "chain_droplet_lmd"])
+def test_cmm_cano_vs_reference(golden, name):
+    """METHOD= CANO (TEMPCO= NO) under CMM: 12 steps across a rebuild and two
+    far-field recomputations."""
+    s, f = golden(name)
+    print(f"case cmm cano {name}", flush=True)
+    md = cmm_md(s)
+    md.set_canonical(300.0)
+    assert md.celprp() == 0
+    _, ier = md.runmd(int(f["cano_nsteps"]), update=10)
+    assert ier == 0
+    x, v = md.get_state()
+    md.close()
+    dx, dv = np.abs(x - f["cano_x"]).max(), np.abs(v - f["cano_v"]).max()
+    print(f"max |dx| {dx:.2e} |dv| {dv:.2e}", flush=True)
+    assert dx < 1e-9 and dv < 1e-9
+
+
+def test_cmm_alsd_vs_reference(golden):
+    """ALSD (METHOD= GE under GEPSME= LMD) under CMM: lambda's force from the
+    near and far class energies of every step."""
+    from conftest import alsd_params
+    s, f = golden("chain_droplet_lmd")
+    print("case cmm alsd", flush=True)
+    al, tempco = alsd_params(s, f, "a")
+    md = cmm_md(s)
+    md.set_canonical(300.0, 3, 1e-3, tempco=tempco)
+    md.set_alsd(al)
+    md.set_lambda(s.lam, 0.0)
+    assert md.celprp() == 0
+    _, ier = md.restart_step()
+    assert ier == 0
+    _, ier = md.runmd(int(f["alsd_a_loop"]), update=10)
+    assert ier == 0
+    x, v = md.get_state()
+    lam, lamv = md.get_lambda()
+    md.close()
+    print(f"max |dx| {np.abs(x - f['alsd_a_x']).max():.2e} dlam {lam - float(f['alsd_a_lam']):.2e}", flush=True)
+    assert np.abs(x - f["alsd_a_x"]).max() < 1e-9
+    assert np.abs(v - f["alsd_a_v"]).max() < 1e-9
+    assert abs(lam - float(f["alsd_a_lam"])) < 1e-11
+    assert abs(lamv - float(f["alsd_a_lamv"])) < 1e-11
