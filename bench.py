@@ -17,6 +17,8 @@ usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
        python bench.py --ensemble           64 uneven systems: one batch vs 64 sequential runs
        python bench.py --drift N            N-step NVE energy drift, FP32/fixed point vs FP64
        python bench.py --method CANO        the same runs with METHOD= CANO (velocity scaling, 300 K)
+       python bench.py --cmm R              the cell multipole method (CAL15M= CMM, BOUNDA= NO) on a
+                                            non-periodic water droplet of radius R A, FP64
 """
 from __future__ import annotations
 
@@ -34,6 +36,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_FP32_TFLOPS = 157.3      # MI355X FP32 vector (packed FMA counted)
+PEAK_FP64_TFLOPS = 78.6       # MI355X FP64 vector
+# CMM near field, algorithmic FP64 flop per ordered pair evaluated (FMA = 2,
+# divide and square root 1 each): Coulomb only, Coulomb + LJ, LJ only
+FLOP_CMM_EL, FLOP_CMM_LJEL, FLOP_CMM_LJ = 20, 31, 26
 PEAK_HBM_GBS = 8000.0
 FLOP_EL, FLOP_LJEL = 42, 54   # algorithmic flop per pair (FMA = 2, rsqrt = 1)
 DT_FS = 2.0
@@ -215,6 +221,151 @@ def drift_fit(log, natom):
             "etot_first": float(e[0]), "etot_last": float(e[-1]), "samples": int(len(e))}
 
 
+def cmm_pair_counts(s, cells, idelay: int, rlim: float):
+    """Ordered pairs the CMM near-field kernels evaluate at the rebuild:
+    Coulomb-only and Coulomb + LJ pairs of the 27-cell neighbourhoods
+    (excluded pairs and i = j left out), LJ pairs of the second-nearest
+    cells within rlim."""
+    import numpy as np
+    from scipy.spatial import cKDTree
+    vdw = np.array([t.eps != 0.0 for t in s.types])[s.atype - 1]
+    nv = int(cells.max()) + 2
+    c = cells - 1
+    lin = (c[:, 2] * nv + c[:, 1]) * nv + c[:, 0]
+    cnt = np.bincount(lin, minlength=nv ** 3).reshape(nv, nv, nv)
+    cntv = np.bincount(lin[vdw], minlength=nv ** 3).reshape(nv, nv, nv)
+
+    def box27(a):
+        p = np.pad(a, 1)
+        out = np.zeros_like(a)
+        for dz in range(3):
+            for dy in range(3):
+                for dx in range(3):
+                    out += p[dz:dz + nv, dy:dy + nv, dx:dx + nv]
+        return out
+    n27, n27v = box27(cnt), box27(cntv)
+    tot = n27[c[:, 2], c[:, 1], c[:, 0]] - 1
+    ljel = np.where(vdw, n27v[c[:, 2], c[:, 1], c[:, 0]] - 1, 0)
+    ex = s.excl14()
+    cheb = np.abs(c[ex[:, 0]] - c[ex[:, 1]]).max(1)
+    near_ex = int(2 * (cheb <= 1).sum())
+    near_ex_v = int(2 * ((cheb <= 1) & vdw[ex[:, 0]] & vdw[ex[:, 1]]).sum())
+    n_all = int(tot.sum()) - near_ex
+    n_ljel = int(ljel.sum()) - near_ex_v
+    iv = np.where(vdw)[0]
+    pairs = cKDTree(s.x[iv]).query_pairs(rlim, output_type="ndarray")
+    a, b = iv[pairs[:, 0]], iv[pairs[:, 1]]
+    d = s.x[a] - s.x[b]
+    ch = np.abs(c[a] - c[b]).max(1)
+    lj2 = ((ch >= 2) & (ch <= idelay) & ((d * d).sum(1) < rlim * rlim))
+    return {"el": n_all - n_ljel, "ljel": n_ljel, "lj2": int(2 * lj2.sum())}
+
+
+def cmm_cpu_reference(system, nsteps: int):
+    """md_run (oracle/_ref/md_run) on the droplet with CAL15M= CMM, BOUNDA= NO
+    on all host cores: a bounded sample of MICR steps."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "md_run")
+    if not os.path.exists(exe):
+        return None
+    from j_presto_amd import refio
+    cores = len(os.sched_getaffinity(0))
+    tmp = tempfile.mkdtemp(prefix="jpcmm_")
+    try:
+        files = refio.write_inputs(tmp, system)
+        refio.write_restart(os.path.join(tmp, "in.rst"), system.x, system.v)
+        refio.write_md_inp(os.path.join(tmp, "md.inp"), files, system, {
+            "METHOD": "MICR", "RESTAR": "YES", "NAMERI": "in.rst", "NAMERO": "out.rst", "CAL15M": "CMM",
+            "BOUNDA": "NO", "LOOPLI": str(nsteps), "UPDATE": str(UPDATE), "TIMEST": str(DT_FS),
+            "OUTLOG": "100000", "OUTCLE": "100000"})
+        env = dict(os.environ, OMP_NUM_THREADS=str(cores), OMP_STACKSIZE="512M")
+        with open(os.path.join(tmp, "md.inp")) as fin, open(os.path.join(tmp, "md.log"), "w") as fout:
+            subprocess.run(["bash", "-c", f"ulimit -s unlimited; exec {exe}"], stdin=fin, stdout=fout,
+                           stderr=subprocess.STDOUT, cwd=tmp, env=env, check=True, timeout=900)
+        log = open(os.path.join(tmp, "md.log")).read()
+        speed = float(re.search(r"SPEED \(ns/day\)\s*:\s*([0-9.Ee+-]+)", log).group(1))
+        loop = float(re.search(r"^\s+MD LOOP\s+:\s+([0-9.Ee+-]+)", log, re.M).group(1))
+        return dict(value=speed, unit="ns/day", cores=cores, kind="reference", cpu_model=cpu_model(),
+                    sample=f"md_run (flang -O2 build of the reference sources), {nsteps} MICR steps of this "
+                           f"droplet with CAL15M=CMM BOUNDA=NO, {cores} OpenMP threads, MD loop {loop:.2f} s")
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
+def run_cmm(R: float, steps: int, warmup: int, equil: int, cpu_steps: int, no_cpu: bool):
+    """CAL15M= CMM (BOUNDA= NO) MICR steps on a water droplet of radius R:
+    relaxed and equilibrated with the CMM forces themselves, then K timed
+    steps (far field every CMMUPD= 5 steps, tables every 10, energies every
+    100), FP64 as md_run."""
+    import numpy as np
+    from j_presto_amd import engine, systems
+    s = systems.water_droplet(R, seed=97)
+    cminsiz, nlev, cmmupd = 6.0, 3, 5
+    state = None
+    for dt, n in ((DT_RELAX, RELAX_STEPS), (DT_FS, equil)):
+        md = engine.MdSystem(s, dt=dt, precision=engine.FP64)
+        if state is not None:
+            md.set_state(*state)
+        md.set_cmm(cminsiz, nlev, cmmupd)
+        assert md.celprp() == 0
+        assert engine.equilibrate(md, n) == 0
+        state = md.get_state()
+        md.close()
+    md = engine.MdSystem(s, dt=DT_FS, precision=engine.FP64)
+    md.set_state(*state)
+    md.set_cmm(cminsiz, nlev, cmmupd)
+    assert md.celprp() == 0
+    if warmup:
+        _, ier = md.runmd(warmup, update=UPDATE)
+        assert ier == 0, ier
+    assert md.celprp() == 0
+    md.set_step(0)
+    md.prepare_graphs(steps, update=UPDATE, nmon=NMON)
+    t0 = time.perf_counter()
+    log, ier = md.runmd(steps, update=UPDATE, nmon=NMON)
+    t1 = time.perf_counter()
+    assert ier == 0, ier
+    tm = md.timing()
+    assert md.celprp() == 0
+    t_near, t_far = md.time_cmm(50)
+    cells = md.cmm_cells()
+    x_end, v_end = md.get_state()
+    md.close()
+    s.x = x_end
+    nv1 = int(cells.max())
+    idelay = int(np.ceil(s.cutoff / cminsiz))
+    pc = cmm_pair_counts(s, cells, idelay, s.cutoff)
+    flop = pc["el"] * FLOP_CMM_EL + pc["ljel"] * FLOP_CMM_LJEL + pc["lj2"] * FLOP_CMM_LJ
+    achieved = flop / t_near / 1e12
+    sps = steps / (t1 - t0)
+    et = log["etot"]
+    out = {
+        "metric": f"ns/day, CMM (CAL15M=CMM, BOUNDA=NO) water droplet R={R:g} A, FP64",
+        "value": round(ns_per_day(sps), 3), "unit": "ns/day", "n_gpus": 1, "steps": steps, "warmup": warmup,
+        "ms_per_step": round(1e3 * (t1 - t0) / steps, 5), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"water droplet R={R:g} A ({s.n} atoms, non-periodic)", "natom": s.n,
+                   "cutoff_A": s.cutoff, "celsiz_A": cminsiz, "cellvl": nlev, "cmmupd": cmmupd,
+                   "cells_per_edge": nv1, "dt_fs": DT_FS, "list_update": UPDATE, "energy_every": NMON,
+                   "method": "MICR"},
+        "pairs_per_step": pc,
+        "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_TFLOPS, 5), "traffic": None,
+                     "kernel": "k_cmm_near + k_cmm_lj2", "avg_launch_us": round(t_near * 1e6, 3),
+                     "flop_per_launch": flop},
+        "far_field_us": round(t_far * 1e6, 3),
+        "step_breakdown_ms": {"near": round(1e3 * t_near, 4), "far_per_step": round(1e3 * t_far / cmmupd, 4),
+                              "rebuild_per_step": round(1e3 * tm["list"] / steps, 4),
+                              "rest": round(1e3 * (t1 - t0) / steps - 1e3 * (t_near + t_far / cmmupd)
+                                            - 1e3 * tm["list"] / steps, 4)},
+        "nve_etot_span_kcal_mol": float(et.max() - et.min()) if len(et) else None,
+        "cpu_baseline": None,
+    }
+    if not no_cpu:
+        s.x, s.v = state
+        out["cpu_baseline"] = cmm_cpu_reference(s, cpu_steps)
+    print(json.dumps(out), flush=True)
+
+
 def run_drift(nsteps: int, equil: int, reps: int = 1):
     """N-step NVE runs in both precisions from the same equilibrated state;
     with reps > 1, further start states 1,000 rescaled steps apart, and the
@@ -320,6 +471,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--workload", type=int, default=2, choices=[0, 1, 2, 4])
     ap.add_argument("--ensemble", action="store_true")
+    ap.add_argument("--cmm", type=float, default=0.0, help="CMM water droplet of this radius (A) instead")
     ap.add_argument("--drift", type=int, default=0, help="run an N-step NVE drift comparison instead")
     ap.add_argument("--drift-reps", type=int, default=1, help="independent start states for --drift")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -345,6 +497,9 @@ def main():
         DT_FS = DT_CHAIN_FS
     if args.drift:
         run_drift(args.drift, args.equil, args.drift_reps)
+        return
+    if args.cmm > 0:
+        run_cmm(args.cmm, args.steps, args.warmup, args.equil, args.cpu_steps, args.no_cpu)
         return
     if args.ensemble:
         run_ensemble(args.steps, args.warmup, args.equil)

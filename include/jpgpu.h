@@ -311,6 +311,13 @@ int jp_set_lambda(jp_ctx *ctx, double lambda, double lambda_v);
  * 1..5, cmmupd < 1, or coordinates that need more than 5 levels (md_run
  * stops there). */
 int jp_set_cmm(jp_ctx *ctx, int32_t on, double cminsiz, int32_t nlev, int32_t cmmupd);
+/* Under CMM: mean durations in seconds of the near-field kernels (the
+ * 27-cell Coulomb/LJ pass and the second-nearest LJ pass) and of the
+ * far-field kernel, over reps launches of each back to back at the device
+ * coordinates, between HIP events.  A measurement helper with no counterpart
+ * in the reference; the positions, velocities and the stored far field are
+ * those of a fresh evaluation afterwards. */
+int jp_time_cmm(jp_ctx *ctx, int32_t reps, double *near_s, double *far_s);
 
 /* Mean duration in seconds of the force-step pair kernel (the kernel of every
  * step without energies) over reps launches back to back on the current list

@@ -320,6 +320,7 @@ struct Engine {
     int cmm_celprp();
     int cmm_forces(bool energy, bool itwin);
     int cmm_sums();
+    int cmm_time(int reps, double &near, double &far);
     // the far field of the step after `k` steps of a segment that starts now
     bool cmm_due(int k) const { return k == 0 ? cmm_itwin : (idloop + k) % cmm_upd == 0; }
     // non-bonded forces of the current method into d_gslot
@@ -2536,6 +2537,42 @@ int Engine::cmm_forces(bool energy, bool itwin) {
     return 0;
 }
 
+// Mean durations of the near-field kernel (k_cmm_near, with k_cmm_lj2 when
+// idelay > 1) and of the far-field kernel over reps launches each, between
+// HIP events on the engine's stream, at the device coordinates.
+int Engine::cmm_time(int reps, double &near, double &far) {
+    int r;
+    if ((r = ensure_list(JP_PREC_FP64))) return r;
+    if ((r = set_factors(-1))) return r;
+    if ((r = cmm_forces(false, true))) return r;
+    const CmmArgs &A = cm;
+    const bool multi = hs[0].nfrg > 1;
+    CK(hipEventRecord(ev[0], st));
+    for (int k = 0; k < reps; ++k) {
+        if (multi) hipLaunchKernelGGL((k_cmm_near<true, false>), A.nc1, 64, 0, st, A);
+        else hipLaunchKernelGGL((k_cmm_near<false, false>), A.nc1, 64, 0, st, A);
+        if (A.idelay > 1) {
+            if (multi) hipLaunchKernelGGL((k_cmm_lj2<true, false>), A.nc1, 64, 0, st, A);
+            else hipLaunchKernelGGL((k_cmm_lj2<false, false>), A.nc1, 64, 0, st, A);
+        }
+    }
+    CK(hipEventRecord(ev[1], st));
+    for (int k = 0; k < reps; ++k) {
+        if (multi) hipLaunchKernelGGL(k_cmm_far<true>, A.nc1, 64, 0, st, A);
+        else hipLaunchKernelGGL(k_cmm_far<false>, A.nc1, 64, 0, st, A);
+    }
+    CK(hipEventRecord(ev[2], st));
+    CK(hipGetLastError());
+    CK(hipEventSynchronize(ev[2]));
+    float m1 = 0, m2 = 0;
+    CK(hipEventElapsedTime(&m1, ev[0], ev[1]));
+    CK(hipEventElapsedTime(&m2, ev[1], ev[2]));
+    near = m1 * 1e-3 / reps;
+    far = m2 * 1e-3 / reps;
+    // the timing launches recomputed the same near and far sums: the state is unchanged
+    return cmm_forces(false, true);
+}
+
 __global__ void k_cmm_zero(double *p, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0.0;
@@ -2946,6 +2983,15 @@ int jp_set_mcmd(jp_ctx *ctx, int32_t on, int32_t nwindow, int32_t ndeg, const do
                 const double *high, double alphalw, double alphaup, double celw, double ceup, double tempce) {
     if (!ctx) return JP_EINVAL;
     return ctx->e.set_mcmd(on, nwindow, ndeg, c, low, high, alphalw, alphaup, celw, ceup, tempce);
+}
+
+int jp_time_cmm(jp_ctx *ctx, int32_t reps, double *near, double *far) {
+    if (!ctx || reps < 1 || !ctx->e.cmm) return JP_EINVAL;
+    double a = 0, b = 0;
+    const int r = ctx->e.cmm_time(reps, a, b);
+    if (near) *near = a;
+    if (far) *far = b;
+    return r;
 }
 
 int jp_set_cmm(jp_ctx *ctx, int32_t on, double cminsiz, int32_t nlev, int32_t cmmupd) {

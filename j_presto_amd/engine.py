@@ -219,6 +219,12 @@ class MdSystem:
         on = 0 if cminsiz is None else 1
         native.check("jp_set_cmm", self.lib.jp_set_cmm(self.ctx, on, float(cminsiz or 0.0), int(nlev), int(cmmupd)))
 
+    def time_cmm(self, reps: int = 50):
+        """Under CMM: (near-field, far-field) kernel seconds per launch."""
+        a, b = C.c_double(), C.c_double()
+        native.check("jp_time_cmm", self.lib.jp_time_cmm(self.ctx, int(reps), C.byref(a), C.byref(b)))
+        return a.value, b.value
+
     def get_lambda(self):
         """(lambda, its velocity)."""
         lam, lv = C.c_double(), C.c_double()

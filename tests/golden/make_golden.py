@@ -96,7 +96,7 @@ RECIPES = {
 # tables every 10th).
 CMM_RECIPES = {
     # 481-water droplet, 15 A radius: 16^3 level-1 cells of 6 A, three levels
-    "droplet_cmm": ("water_droplet", dict(R=15.0, seed=21), dict(traj=20, cano=12)),
+    "droplet_cmm": ("water_droplet", dict(R=15.0, seed=21), dict(traj=20, cano=12, mon=200)),
     # compact 120-atom chain with bonds, angles, torsions and 1-4 terms in a droplet
     "chain_droplet_cmm": ("chain_droplet", dict(n_chain=120, R=14.0, seed=41, bond_fc=300.0, angle_fc=50.0,
                                                 torsion_fc=1.0), dict(traj=20)),
@@ -375,8 +375,10 @@ def alsd_jobs(s: systems.System, tmp: str) -> dict:
 def make_cmm(name: str) -> str:
     """CAL15M= CMM, BOUNDA= NO fixture: md_run's FDEBUG= MAKE dump at the
     stored coordinates and a MICR run with its NAMETO series; with cano a
-    METHOD= CANO run at 300 K (TEMPCO= NO) and with alsd an ALSD run with the
-    parameters of ALSD_RUNS[alsd], each from md_run's restart set-up."""
+    METHOD= CANO run at 300 K (TEMPCO= NO), with alsd an ALSD run with the
+    parameters of ALSD_RUNS[alsd], and with mon a MICR run of mon steps whose
+    monitoring blocks (every 20 steps) are kept, each from md_run's restart
+    set-up."""
     kind, args, opt = CMM_RECIPES[name]
     s = build_system(kind, args)
     out = dict(recipe=json.dumps(dict(kind=kind, args=args)), fingerprint=fingerprint(s), x=s.x, v=s.v)
@@ -398,6 +400,14 @@ def make_cmm(name: str) -> str:
         r = refio.read_restart(os.path.join(t2, "out.rst"))
         out.update(traj_x=r["x"], traj_v=r["v"], traj_nsteps=np.int64(n),
                    traj_ene=np.loadtxt(os.path.join(t2, "ene.dat"), ndmin=2))
+        if opt.get("mon"):
+            t5 = os.path.join(tmp, "mon")
+            os.makedirs(t5)
+            refio.write_restart(os.path.join(t5, "in.rst"), s.x, s.v, lam=s.lam)
+            log = run_md(t5, s, dict(cmm, METHOD="MICR", RESTAR="YES", NAMERI="in.rst", NAMERO="out.rst",
+                                     LOOPLI=str(opt["mon"]), UPDATE="10", TIMEST="2.0", OUTLOG="20",
+                                     OUTCLE="100000"))
+            out["mon_micr"] = parse_monitor(log)
         if opt.get("cano"):
             t3 = os.path.join(tmp, "cano")
             os.makedirs(t3)

@@ -182,3 +182,27 @@ def test_cmm_alsd_vs_reference(golden):
     assert np.abs(v - f["alsd_a_v"]).max() < 1e-9
     assert abs(lam - float(f["alsd_a_lam"])) < 1e-11
     assert abs(lamv - float(f["alsd_a_lamv"])) < 1e-11
+
+
+def test_cmm_energy_log_vs_reference(golden):
+    """200 MICR steps of the droplet under CMM from its generated (strained)
+    state, which heats to about 1000 K: the monitored E_tot, E_pot and K
+    every 20 steps against md_run's printed ones (7 significant digits)
+    within one unit of the last digit.  md_run's E_tot itself rises by
+    about 370 kcal/mol over these steps (far field and tables held between
+    updates); the GPU run follows it."""
+    s, f = golden("droplet_cmm")
+    print("case cmm energy log", flush=True)
+    md = cmm_md(s)
+    assert md.celprp() == 0
+    log, ier = md.runmd(200, update=10, nmon=20)
+    assert ier == 0
+    md.close()
+    mon = f["mon_micr"]
+    mon = mon[(mon[:, 0] <= 200) & (mon[:, 0] % 20 == 0)]
+    assert np.array_equal(log["step"], mon[:, 0].astype(np.int32))
+    for key, col in (("etot", 1), ("epot", 2), ("ekin", 4)):
+        ulp = 10.0 ** (np.floor(np.log10(np.abs(mon[:, col]))) - 6)
+        d = np.abs(log[key] - mon[:, col])
+        print(key, (d / ulp).max(), flush=True)
+        assert (d <= 1.0001 * ulp).all(), key
