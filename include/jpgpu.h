@@ -307,7 +307,8 @@ int jp_set_lambda(jp_ctx *ctx, double lambda, double lambda_v);
  * afresh; jp_get_list returns the level-1 cell (1-based per axis) of every
  * atom and no pair list; jp_count_pairs returns JP_EINVAL.  A new jp_set_state
  * starts with the far field due.  on = 0 returns to ZD.  Returns JP_EINVAL
- * for a batch or a context of several systems, cminsiz <= 0, nlev outside
+ * for a batch or a context of several systems, cminsiz <= 0 or below
+ * cutoff / 15, nlev outside
  * 1..5, cmmupd < 1, or coordinates that need more than 5 levels (md_run
  * stops there). */
 int jp_set_cmm(jp_ctx *ctx, int32_t on, double cminsiz, int32_t nlev, int32_t cmmupd);

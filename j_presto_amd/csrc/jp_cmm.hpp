@@ -13,8 +13,9 @@
 //   sx[p] = (x, y, z, chgmod) of the sorted atom p, refreshed every step;
 //   sx0[p] its coordinates at the rebuild (md_run's Tcord: the second-nearest
 //   LJ table is cut at rlim with them, neartb.f90:321-366);
-//   npc, pot: atoms and the 13 multipole coefficients of every (cell,
-//   fragment) at every level (celoth, celpol).
+//   npc, pot: atoms of every (cell, fragment) at every level (celoth) and
+//   its record of 16 doubles: the shifted cell centre and the 13 multipole
+//   coefficients (celpol).
 // Per step:
 //   k_cmm_gather   sx from x;
 //   k_cmm_near     Coulomb of every pair in a cell and its 26 neighbours,
@@ -40,7 +41,7 @@ struct CmmArgs {
     int n, nf, nfrg, nlev, idelay, nv1, nc1, ntyp;
     int nv[JP_CMM_MAXLEV];
     int cgoff[JP_CMM_MAXLEV];   // offset of level l's centres in cg
-    int pcoff[JP_CMM_MAXLEV];   // offset of level l's (cell, fragment) entries in npc (x 13 in pot)
+    int pcoff[JP_CMM_MAXLEV];   // offset of level l's (cell, fragment) entries in npc (x 16 in pot)
     double siz[JP_CMM_MAXLEV];
     double rlim2, ccoef;
     int ipmax, capel, capvd, capvw;   // ipmax, n15mxel, n15mx, nvdw (incel.f90:137-150, DOUB)
@@ -67,12 +68,28 @@ struct CmmArgs {
     double *efar;               // [a][3] far-field energy per class (kept likewise)
     double *gslot;              // [a][3] combined gradient
     double *eatom;              // [a][6] elec (near + far), vdW per class; summed and halved by the host
+    unsigned long long *xmask;  // [a] excluded partners b with b - a in [-32, 31] (bit b - a + 32; bit 32: a)
+    int *xwide;                 // [a] 1: some excluded partner beyond that window
+    int *l2cnt, *l2ptr, *l2idx; // second-nearest LJ table per sorted atom (fixed between rebuilds)
+    int *vflag, *vpos, *vlist;  // the sorted atoms with such a table; how many at vpos[n]
 };
 
 __device__ __forceinline__ int cmm_lin(int nv, int x, int y, int z) { return (z * nv + y) * nv + x; }
 
 // Energy class of a pair of cluster classes (n_matrix - 1).
 __device__ __forceinline__ int cmm_cls(const SysP &P, int ci, int cj) { return P.cls[ci * 2 + cj]; }
+
+// 1/sqrt(x) for the pair kernels (x a squared distance, far from 0 and
+// infinity): the hardware estimate and two Newton steps, each doubling its
+// correct bits, to within a few units in the last place of FP64.  md_run
+// forms 1/r^2 and sqrt of it; the pair terms here are within 1e-15 of those.
+__device__ __forceinline__ double cmm_rsqrt(double x) {
+    double y = __builtin_amdgcn_rsq(x);
+    const double hx = 0.5 * x;
+    y = y * (1.5 - hx * y * y);
+    y = y * (1.5 - hx * y * y);
+    return y;
+}
 
 // ------------------------------------------------------------------ rebuild
 
@@ -196,13 +213,15 @@ __global__ void k_cmm_pol(CmmArgs A, int l) {
     if (q >= nq || (A.err[0] & 1)) return;
     const int f = q % A.nf, cell = q / A.nf;
     const int ix = cell % mm, iy = (cell / mm) % mm, iz = cell / (mm * mm);
-    double *P = A.pot + (size_t)(A.pcoff[l] + q) * 13;
+    double *P = A.pot + (size_t)(A.pcoff[l] + q) * 16;
+    const double *cgl = A.cg + A.cgoff[l];
+    const double gx = cgl[ix] + A.c0[0], gy = cgl[iy] + A.c0[1], gz = cgl[iz] + A.c0[2];
+    P[0] = gx; P[1] = gy; P[2] = gz;
+    P += 3;
     if (A.npc[A.pcoff[l] + q] == 0) {
         for (int k = 0; k < 13; ++k) P[k] = 0.0;
         return;
     }
-    const double *cgl = A.cg + A.cgoff[l];
-    const double gx = cgl[ix] + A.c0[0], gy = cgl[iy] + A.c0[1], gz = cgl[iz] + A.c0[2];
     double qz = 0.0, rx = 0.0, ry = 0.0, rz = 0.0, rxx = 0.0, ryy = 0.0, rzz = 0.0, rxy = 0.0, ryz = 0.0, rzx = 0.0;
     if (l == 0) {
         const int b = A.kstart[2 * q], e = A.kstart[2 * q + 2];
@@ -231,7 +250,7 @@ __global__ void k_cmm_pol(CmmArgs A, int l) {
         const int mx = 2 * ix + (jj & 1), my = 2 * iy + ((jj >> 1) & 1), mz = 2 * iz + ((jj >> 2) & 1);
         const int qc = A.pcoff[l - 1] + cmm_lin(mc, mx, my, mz) * A.nf + f;
         if (A.npc[qc] == 0) continue;
-        const double *C = A.pot + (size_t)qc * 13;
+        const double *C = A.pot + (size_t)qc * 16 + 3;
         const double qq = C[0];
         qz = qz + qq;
         const double xx = (cgc[mx] + A.c0[0]) - gx;
@@ -255,10 +274,50 @@ __global__ void k_cmm_pol(CmmArgs A, int l) {
     for (int k = 0; k < 13; ++k) P[k] = v[k];
 }
 
+// ------------------------------------------------------------------ per step
+
+__global__ void k_cmm_gather(CmmArgs A) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= A.n) return;
+    const int a = A.sorig[p];
+    A.sx[p] = make_double4(A.x[3 * a], A.x[3 * a + 1], A.x[3 * a + 2], A.chg[a]);
+}
+
+__device__ __forceinline__ bool cmm_excluded(const CmmArgs &A, int ai, int aj) {
+    for (int e = A.xptr[ai]; e < A.xptr[ai + 1]; ++e)
+        if (A.xidx[e] == aj) return true;
+    return false;
+}
+
+// Excluded pair (or i = j): the window mask of atom i (partners b with
+// b - ai in [-32, 31], bit 32 being i itself), else, for atoms with
+// partners beyond the window, the partner list when both lie in one molecule.
+__device__ __forceinline__ bool cmm_excl(const CmmArgs &A, int ai, int aj, unsigned long long mask, bool wide,
+                                         int moli, int molj) {
+    const unsigned d = (unsigned)(aj - ai + 32);
+    if (d < 64u && ((mask >> d) & 1ull)) return true;
+    return wide && molj == moli && cmm_excluded(A, ai, aj);
+}
+
+// Set-up: the exclusion window masks.
+__global__ void k_cmm_xmask(CmmArgs A) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= A.n) return;
+    unsigned long long m = 1ull << 32;
+    int wide = 0;
+    for (int e = A.xptr[a]; e < A.xptr[a + 1]; ++e) {
+        const unsigned d = (unsigned)(A.xidx[e] - a + 32);
+        if (d < 64u) m |= 1ull << d;
+        else wide = 1;
+    }
+    A.xmask[a] = m;
+    A.xwide[a] = wide;
+}
+
 // Table checks of neartb (DOUB): partners per atom and class in the EL-only
-// table (itbEL), the EL+vdW table (itb) and the second-nearest LJ table
-// against their capacities (ier 51, neartb.f90:368-400), and the excluded
-// pairs the tables hold (ier 99 unless all, neartb.f90:402-453).
+// table (itbEL) and the EL+vdW table (itb) against their capacities (ier 51,
+// neartb.f90:368-400), and the excluded pairs the tables hold (ier 99 unless
+// all, neartb.f90:402-453).  The second-nearest LJ table is k_cmm_l2's.
 __global__ void k_cmm_check(CmmArgs A) {
 #pragma clang fp contract(off)
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -269,7 +328,7 @@ __global__ void k_cmm_check(CmmArgs A) {
     const int nf = A.nf, nv = A.nv1;
     const int cell = A.key[a] / (2 * nf);
     const int cx = cell % nv, cy = (cell / nv) % nv, cz = cell / (nv * nv);
-    int nel[3] = {0, 0, 0}, nvd[3] = {0, 0, 0}, nvw[3] = {0, 0, 0};
+    int nel[3] = {0, 0, 0}, nvd[3] = {0, 0, 0};
     for (int z = max(cz - 1, 0); z <= min(cz + 1, nv - 1); ++z)
         for (int y = max(cy - 1, 0); y <= min(cy + 1, nv - 1); ++y)
             for (int x = max(cx - 1, 0); x <= min(cx + 1, nv - 1); ++x) {
@@ -288,26 +347,9 @@ __global__ void k_cmm_check(CmmArgs A) {
                 }
             }
     const double xi = A.sx0[3 * p], yi = A.sx0[3 * p + 1], zi = A.sx0[3 * p + 2];
-    if (vi && A.idelay > 1) {
-        const int d = A.idelay;
-        for (int z = max(cz - d, 0); z <= min(cz + d, nv - 1); ++z)
-            for (int y = max(cy - d, 0); y <= min(cy + d, nv - 1); ++y)
-                for (int x = max(cx - d, 0); x <= min(cx + d, nv - 1); ++x) {
-                    if (abs(z - cz) <= 1 && abs(y - cy) <= 1 && abs(x - cx) <= 1) continue;
-                    const int c2 = cmm_lin(nv, x, y, z);
-                    for (int fj = 0; fj < nf; ++fj) {
-                        const int k = cmm_cls(P, fi, fj);
-                        const int q = (c2 * nf + fj) * 2 + 1;
-                        for (int j = A.kstart[q]; j < A.kstart[q] + A.kcnt[q]; ++j) {
-                            const double d0 = xi - A.sx0[3 * j], d1 = yi - A.sx0[3 * j + 1],
-                                         d2 = zi - A.sx0[3 * j + 2];
-                            if (d0 * d0 + d1 * d1 + d2 * d2 < A.rlim2) nvw[k]++;
-                        }
-                    }
-                }
-    }
+    A.l2cnt[p] = 0;   // k_cmm_l2 sets those of the vdW atoms
     bool over = false;
-    for (int k = 0; k < 3; ++k) over = over || nel[k] > A.capel || nvd[k] > A.capvd || nvw[k] > A.capvw;
+    for (int k = 0; k < 3; ++k) over = over || nel[k] > A.capel || nvd[k] > A.capvd;
     if (over) atomicOr(A.err, 2);
     int found = 0;
     for (int e = A.xptr[a]; e < A.xptr[a + 1]; ++e) {
@@ -326,326 +368,473 @@ __global__ void k_cmm_check(CmmArgs A) {
     if (found) atomicAdd(A.err + 1, found);
 }
 
-// ------------------------------------------------------------------ per step
-
-__global__ void k_cmm_gather(CmmArgs A) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= A.n) return;
-    const int a = A.sorig[p];
-    A.sx[p] = make_double4(A.x[3 * a], A.x[3 * a + 1], A.x[3 * a + 2], A.chg[a]);
-}
-
-__device__ __forceinline__ bool cmm_excluded(const CmmArgs &A, int ai, int aj) {
-    for (int e = A.xptr[ai]; e < A.xptr[ai + 1]; ++e)
-        if (A.xidx[e] == aj) return true;
-    return false;
-}
-
-// Lanes = (replica r, i atom li) with R replicas of nic atoms in one wave.
-struct CmmLanes {
-    int R, nic, r, li;
-    __device__ CmmLanes(int ni, int lane) {
-        nic = min(ni, 64);
-        R = 1;
-        while (2 * R * nic <= 64) R *= 2;
-        r = lane / nic;
-        li = lane - r * nic;
-    }
-    __device__ bool active() const { return r < R; }
-};
-
-// Per-lane sums of one i atom: gradient per class and energies per class.
+// Per-thread sums of one i atom: gradient per class and energies per class.
 template <bool MULTI>
 struct CmmAcc {
-    double g[MULTI ? 3 : 1][3];
-    double ee[MULTI ? 3 : 1], vv[MULTI ? 3 : 1];
+    static constexpr int NC = MULTI ? 3 : 1;
+    static constexpr int NV = 5 * NC;   // g[k][3], ee[k], vv[k]
+    double v[NV];
     __device__ void zero() {
-        for (int k = 0; k < (MULTI ? 3 : 1); ++k) {
-            g[k][0] = g[k][1] = g[k][2] = 0.0;
-            ee[k] = vv[k] = 0.0;
-        }
+#pragma unroll
+        for (int m = 0; m < NV; ++m) v[m] = 0.0;
     }
     // md_run's sums: e, then gx + dx * cc (enear_CMM_doub)
-    __device__ void add(int k, double dx, double dy, double dz, double cc, double e, double v) {
-#pragma clang fp contract(off)
-        if constexpr (MULTI) {
+    __device__ void add(int k, double dx, double dy, double dz, double cc, double e, double vw) {
 #pragma unroll
-            for (int m = 0; m < 3; ++m) {
-                const bool on = m == k;
-                g[m][0] = g[m][0] + (on ? dx * cc : 0.0);
-                g[m][1] = g[m][1] + (on ? dy * cc : 0.0);
-                g[m][2] = g[m][2] + (on ? dz * cc : 0.0);
-                ee[m] = ee[m] + (on ? e : 0.0);
-                vv[m] = vv[m] + (on ? v : 0.0);
-            }
-        } else {
-            g[0][0] = g[0][0] + dx * cc;
-            g[0][1] = g[0][1] + dy * cc;
-            g[0][2] = g[0][2] + dz * cc;
-            ee[0] = ee[0] + e;
-            vv[0] = vv[0] + v;
+        for (int m = 0; m < NC; ++m) {
+            const bool on = !MULTI || m == k;
+            v[3 * m] = v[3 * m] + (on ? dx * cc : 0.0);
+            v[3 * m + 1] = v[3 * m + 1] + (on ? dy * cc : 0.0);
+            v[3 * m + 2] = v[3 * m + 2] + (on ? dz * cc : 0.0);
+            v[3 * NC + m] = v[3 * NC + m] + (on ? e : 0.0);
+            v[4 * NC + m] = v[4 * NC + m] + (on ? vw : 0.0);
         }
     }
 };
 
-// Sum the replicas of every i atom in replica order through LDS and store
-// (or add, for the second-nearest pass) the result of the sorted atom p.
-template <bool MULTI, bool E>
-__device__ void cmm_reduce_store(const CmmArgs &A, const CmmAcc<MULTI> &acc, const CmmLanes &L, int p, bool add,
-                                 double (*s)[15]) {
-#pragma clang fp contract(off)
-    constexpr int NC = MULTI ? 3 : 1;
-    const int lane = threadIdx.x;
-    for (int k = 0; k < NC; ++k) {
-        s[lane][3 * k] = acc.g[k][0];
-        s[lane][3 * k + 1] = acc.g[k][1];
-        s[lane][3 * k + 2] = acc.g[k][2];
-        s[lane][9 + k] = acc.ee[k];
-        s[lane][12 + k] = acc.vv[k];
-    }
-    __syncthreads();
-    if (L.r == 0 && L.li < L.nic && p >= 0) {
-        double t[15];
-        for (int m = 0; m < 15; ++m) t[m] = 0.0;
-        for (int r = 0; r < L.R; ++r) {
-            const int ln = r * L.nic + L.li;
-            for (int k = 0; k < NC; ++k) {
-                for (int d = 0; d < 3; ++d) t[3 * k + d] = t[3 * k + d] + s[ln][3 * k + d];
-                t[9 + k] = t[9 + k] + s[ln][9 + k];
-                t[12 + k] = t[12 + k] + s[ln][12 + k];
-            }
-        }
-        double *G = A.gnb + (size_t)p * 9;
-        double *En = A.enb + (size_t)p * 6;
-        for (int k = 0; k < 3; ++k) {
-            for (int d = 0; d < 3; ++d) {
-                const double v = k < NC ? t[3 * k + d] : 0.0;
-                G[3 * k + d] = add ? G[3 * k + d] + v : v;
-            }
-            if (E) {
-                const double e = k < NC ? t[9 + k] : 0.0, v = k < NC ? t[12 + k] : 0.0;
-                En[k] = add ? En[k] + e : e;
-                En[3 + k] = add ? En[3 + k] + v : v;
-            }
-        }
-    }
-    __syncthreads();
-}
+#define CMM_NT 256     // threads per block of k_cmm_near and k_cmm_far
+#define CMM_JCH 512    // j atoms staged in LDS per pass
 
 // Near field: every pair of the cell and its 26 neighbours (clipped at the
-// field's faces).  One wavefront per level-1 cell.
+// field's faces).  One block per level-1 cell: the j atoms of the 27 cells
+// (nine runs of x-adjacent cells, contiguous in the sorted order) are staged
+// through LDS in chunks; thread t takes i atom t % nic and, as replica
+// t / nic of R = 256 / nic, every R-th staged j; the replicas are summed in
+// replica order.
 template <bool MULTI, bool E>
-__global__ void __launch_bounds__(64) k_cmm_near(CmmArgs A) {
-#pragma clang fp contract(off)
-    __shared__ double s[64][15];
-    const int cell = blockIdx.x, nf = A.nf, nv = A.nv1;
+__global__ void __launch_bounds__(CMM_NT) k_cmm_near(CmmArgs A) {
+    using Acc = CmmAcc<MULTI>;
+    constexpr int NV = Acc::NV, NC = Acc::NC;
+    constexpr int SJ = 4 * CMM_JCH + 3 * CMM_JCH / 2, SR = CMM_NT * NV;
+    __shared__ double s_buf[SJ > SR ? SJ : SR];
+    __shared__ int s_run[9], s_off[10], s_nrun;
+    extern __shared__ double2 s_lj[];
+    double *jx = s_buf, *jy = jx + CMM_JCH, *jz = jy + CMM_JCH, *jq = jz + CMM_JCH;
+    int *jf = (int *)(jq + CMM_JCH), *ja = jf + CMM_JCH, *jm = ja + CMM_JCH;
+    const int cell = blockIdx.x, nf = A.nf, nv = A.nv1, t = threadIdx.x;
     const int cs = A.kstart[cell * nf * 2], ce = A.kstart[(cell + 1) * nf * 2];
     if (ce == cs) return;
     const SysP &P = A.sp[0];
-    const int cx = cell % nv, cy = (cell / nv) % nv, cz = cell / (nv * nv);
     const int nt = A.ntyp;
-    for (int i0 = cs; i0 < ce; i0 += 64) {
-        const CmmLanes L(ce - i0, threadIdx.x);
-        const int p = i0 + L.li;
-        const bool act = L.active();
+    for (int k = t; k < nt * nt; k += CMM_NT) s_lj[k] = A.lj[k];
+    const int cx = cell % nv, cy = (cell / nv) % nv, cz = cell / (nv * nv);
+    if (t == 0) {
+        int n = 0, off = 0;
+        const int x0 = max(cx - 1, 0), x1 = min(cx + 1, nv - 1);
+        for (int z = max(cz - 1, 0); z <= min(cz + 1, nv - 1); ++z)
+            for (int y = max(cy - 1, 0); y <= min(cy + 1, nv - 1); ++y) {
+                const int b = A.kstart[cmm_lin(nv, x0, y, z) * nf * 2];
+                const int e = A.kstart[(cmm_lin(nv, x1, y, z) + 1) * nf * 2];
+                s_run[n] = b;
+                s_off[n] = off;
+                off += e - b;
+                ++n;
+            }
+        s_off[n] = off;
+        s_nrun = n;
+    }
+    __syncthreads();
+    const int J = s_off[s_nrun];
+    for (int i0 = cs; i0 < ce; i0 += CMM_NT) {
+        const int nic = min(CMM_NT, ce - i0);
+        const int R = CMM_NT / nic, r = t / nic, li = t - r * nic;
+        const bool act = r < R;
+        const int p = i0 + li;
         const double4 xi = act ? A.sx[p] : make_double4(0.0, 0.0, 0.0, 0.0);
         const int fi = act ? A.sflag[p] : 0;
         const int ti = fi & 0xfff, vi = (fi >> 12) & 1, ci = (fi >> 13) & 1;
-        const int moli = act ? A.smol[p] : -1;
         const int ai = act ? A.sorig[p] : 0;
-        CmmAcc<MULTI> acc;
+        const int moli = act ? A.smol[p] : -1;
+        const unsigned long long mask = act ? A.xmask[ai] : 0ull;
+        const bool wide = act && A.xwide[ai];
+        Acc acc;
         acc.zero();
-        for (int z = max(cz - 1, 0); z <= min(cz + 1, nv - 1); ++z)
-            for (int y = max(cy - 1, 0); y <= min(cy + 1, nv - 1); ++y)
-                for (int x = max(cx - 1, 0); x <= min(cx + 1, nv - 1); ++x) {
-                    const int c2 = cmm_lin(nv, x, y, z);
-                    const int js = A.kstart[c2 * nf * 2], je = A.kstart[(c2 + 1) * nf * 2];
-                    if (!act) continue;
-                    for (int j = js + L.r; j < je; j += L.R) {
-                        if (j == p) continue;
-                        if (A.smol[j] == moli && cmm_excluded(A, ai, A.sorig[j])) continue;
-                        const double4 xj = A.sx[j];
-                        const int fj = A.sflag[j];
-                        const double dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-                        const double d2i = 1.0 / (dx * dx + dy * dy + dz * dz);
-                        const double e = xi.w * xj.w * sqrt(d2i);
-                        double cc = e, v = 0.0;
-                        if (vi & (fj >> 12)) {
-                            const double2 c = A.lj[(fj & 0xfff) * nt + ti];
-                            const double d6i = d2i * d2i * d2i;
-                            const double v1 = c.x * d6i, v2 = c.y * d6i * d6i;
-                            v = v2 - v1;
-                            cc = e - 6.0 * v1 + 12.0 * v2;
-                        }
-                        cc = cc * d2i;
-                        acc.add(MULTI ? cmm_cls(P, ci, (fj >> 13) & 1) : 0, dx, dy, dz, cc, e, v);
+        for (int q0 = 0; q0 < J; q0 += CMM_JCH) {
+            const int nq = min(CMM_JCH, J - q0);
+            __syncthreads();
+            for (int s = t; s < nq; s += CMM_NT) {
+                const int g = q0 + s;
+                int k = 0;
+                while (g >= s_off[k + 1]) ++k;
+                const int pj = s_run[k] + (g - s_off[k]);
+                const double4 v = A.sx[pj];
+                jx[s] = v.x; jy[s] = v.y; jz[s] = v.z; jq[s] = v.w;
+                jf[s] = A.sflag[pj];
+                ja[s] = A.sorig[pj];
+                jm[s] = A.smol[pj];
+            }
+            __syncthreads();
+            if (act) {
+                for (int s = r; s < nq; s += R) {
+                    // excluded pairs (and i itself) lie in i's molecule
+                    if (jm[s] == moli && cmm_excl(A, ai, ja[s], mask, wide, moli, moli)) continue;
+                    const int fj = jf[s];
+                    const double dx = xi.x - jx[s], dy = xi.y - jy[s], dz = xi.z - jz[s];
+                    const double ri = cmm_rsqrt(dx * dx + dy * dy + dz * dz);
+                    const double d2i = ri * ri;
+                    const double e = xi.w * jq[s] * ri;
+                    double cc = e, vw = 0.0;
+                    if (vi & (fj >> 12)) {
+                        const double2 c = s_lj[(fj & 0xfff) * nt + ti];
+                        const double d6i = d2i * d2i * d2i;
+                        const double v1 = c.x * d6i, v2 = c.y * d6i * d6i;
+                        vw = v2 - v1;
+                        cc = e - 6.0 * v1 + 12.0 * v2;
                     }
+                    cc = cc * d2i;
+                    acc.add(MULTI ? cmm_cls(P, ci, (fj >> 13) & 1) : 0, dx, dy, dz, cc, e, vw);
                 }
-        cmm_reduce_store<MULTI, E>(A, acc, L, act ? p : -1, false, s);
+            }
+        }
+        __syncthreads();
+        double *red = s_buf;
+#pragma unroll
+        for (int m = 0; m < NV; ++m) red[m * CMM_NT + t] = acc.v[m];
+        __syncthreads();
+        if (act && r == 0) {
+            double tot[NV];
+#pragma unroll
+            for (int m = 0; m < NV; ++m) tot[m] = 0.0;
+            for (int rr = 0; rr < R; ++rr)
+#pragma unroll
+                for (int m = 0; m < NV; ++m) tot[m] = tot[m] + red[m * CMM_NT + rr * nic + li];
+            double *G = A.gnb + (size_t)p * 9;
+            for (int k = 0; k < 3; ++k)
+                for (int d = 0; d < 3; ++d) G[3 * k + d] = k < NC ? tot[3 * k + d] : 0.0;
+            if (E) {
+                double *En = A.enb + (size_t)p * 6;
+                for (int k = 0; k < 3; ++k) {
+                    En[k] = k < NC ? tot[3 * NC + k] : 0.0;
+                    En[3 + k] = k < NC ? tot[4 * NC + k] : 0.0;
+                }
+            }
+        }
     }
 }
 
-// LJ of the second-nearest cells (idelay > 1): vdW pairs within rlim at the
-// rebuild, in cells at Chebyshev distance 2 .. idelay (neartb.f90:321-366,
-// enear's third table).  One wavefront per cell over its vdW atoms.
-template <bool MULTI, bool E>
-__global__ void __launch_bounds__(64) k_cmm_lj2(CmmArgs A) {
+#define CMM_L2CH 1024   // candidates staged in LDS per pass of k_cmm_l2
+#define CMM_MAXRUN 1024 // x-runs of a shell (idelay <= 15)
+
+// The cells at Chebyshev distance 2 .. d of cell (cx, cy, cz) as runs of
+// x-adjacent cells (contiguous atom ranges): s_run[k] their first sorted
+// atom, s_off[k] their offset in the concatenation; returns the run count.
+__device__ int cmm_shell_runs(const CmmArgs &A, int cx, int cy, int cz, int d, int *s_run, int *s_off) {
+    const int nv = A.nv1, nf = A.nf;
+    int n = 0, off = 0;
+    auto run = [&](int xa, int xb, int y, int z) {
+        const int b = A.kstart[cmm_lin(nv, xa, y, z) * nf * 2], e = A.kstart[(cmm_lin(nv, xb, y, z) + 1) * nf * 2];
+        s_run[n] = b;
+        s_off[n] = off;
+        off += e - b;
+        ++n;
+    };
+    const int x0 = max(cx - d, 0), x1 = min(cx + d, nv - 1);
+    for (int z = max(cz - d, 0); z <= min(cz + d, nv - 1); ++z)
+        for (int y = max(cy - d, 0); y <= min(cy + d, nv - 1); ++y) {
+            if (abs(z - cz) <= 1 && abs(y - cy) <= 1) {
+                if (x0 <= cx - 2) run(x0, cx - 2, y, z);
+                if (cx + 2 <= x1) run(cx + 2, x1, y, z);
+            } else {
+                run(x0, x1, y, z);
+            }
+        }
+    s_off[n] = off;
+    return n;
+}
+
+// The second-nearest LJ tables (neartb.f90:321-366): for every vdW atom the
+// vdW atoms of the cells at Chebyshev distance 2 .. idelay within rlim at the
+// rebuild (sx0), less its excluded pairs.  One block per cell: the
+// candidates staged through LDS, thread t takes vdW atom t % nic and, as
+// replica r = t / nic of R = 256 / nic, every R-th candidate.  COUNT: the
+// partners per class before the exclusions against nvdw (ier 51), and the
+// table lengths l2cnt.  FILL: each replica counts its entries again, and
+// writes them after those of the replicas before it, in candidate order.
+template <bool FILL>
+__global__ void __launch_bounds__(256) k_cmm_l2(CmmArgs A) {
 #pragma clang fp contract(off)
-    __shared__ double s[64][15];
-    __shared__ int vlist[256];
-    const int cell = blockIdx.x, nf = A.nf, nv = A.nv1;
+    __shared__ int s_run[CMM_MAXRUN], s_off[CMM_MAXRUN + 1], s_n, s_vi[256], s_c[4][256];
+    __shared__ double s_x[CMM_L2CH], s_y[CMM_L2CH], s_z[CMM_L2CH];
+    __shared__ int s_f[CMM_L2CH], s_a[CMM_L2CH], s_m[CMM_L2CH], s_p[CMM_L2CH];
+    const int cell = blockIdx.x, nf = A.nf, nv = A.nv1, t = threadIdx.x;
+    if (A.err[0] & (FILL ? 3 : 1)) return;
     int nvi = 0;
     for (int f = 0; f < nf; ++f) nvi += A.kcnt[(cell * nf + f) * 2 + 1];
     if (nvi == 0) return;
     const SysP &P = A.sp[0];
     const int cx = cell % nv, cy = (cell / nv) % nv, cz = cell / (nv * nv);
-    const int nt = A.ntyp, d = A.idelay;
+    if (t == 0) s_n = cmm_shell_runs(A, cx, cy, cz, A.idelay, s_run, s_off);
+    __syncthreads();
+    const int J = s_off[s_n];
     for (int i0 = 0; i0 < nvi; i0 += 256) {
-        // this block's vdW atoms (sorted positions), fragment by fragment
-        if (threadIdx.x == 0) {
+        if (t == 0) {
             int w = 0, skip = i0;
             for (int f = 0; f < nf && w < 256; ++f) {
                 const int q = (cell * nf + f) * 2 + 1;
-                for (int t = 0; t < A.kcnt[q] && w < 256; ++t) {
+                for (int u = 0; u < A.kcnt[q] && w < 256; ++u) {
                     if (skip > 0) { --skip; continue; }
-                    vlist[w++] = A.kstart[q] + t;
+                    s_vi[w++] = A.kstart[q] + u;
                 }
             }
         }
         __syncthreads();
-        const int nchunk = min(256, nvi - i0);
-        for (int c0 = 0; c0 < nchunk; c0 += 64) {
-            const CmmLanes L(nchunk - c0, threadIdx.x);
-            const bool act = L.active();
-            const int p = act ? vlist[c0 + L.li] : -1;
-            const double4 xi = act ? A.sx[p] : make_double4(0.0, 0.0, 0.0, 0.0);
-            const double x0 = act ? A.sx0[3 * p] : 0.0, y0 = act ? A.sx0[3 * p + 1] : 0.0,
-                         z0 = act ? A.sx0[3 * p + 2] : 0.0;
-            const int fi = act ? A.sflag[p] : 0;
-            const int ti = fi & 0xfff, ci = (fi >> 13) & 1;
-            const int moli = act ? A.smol[p] : -1;
-            const int ai = act ? A.sorig[p] : 0;
-            CmmAcc<MULTI> acc;
-            acc.zero();
-            if (act) {
-                for (int z = max(cz - d, 0); z <= min(cz + d, nv - 1); ++z)
-                    for (int y = max(cy - d, 0); y <= min(cy + d, nv - 1); ++y)
-                        for (int x = max(cx - d, 0); x <= min(cx + d, nv - 1); ++x) {
-                            if (abs(z - cz) <= 1 && abs(y - cy) <= 1 && abs(x - cx) <= 1) continue;
-                            const int c2 = cmm_lin(nv, x, y, z);
-                            for (int fj = 0; fj < nf; ++fj) {
-                                const int q = (c2 * nf + fj) * 2 + 1;
-                                const int js = A.kstart[q], je = js + A.kcnt[q];
-                                for (int j = js + L.r; j < je; j += L.R) {
-                                    const double e0 = x0 - A.sx0[3 * j], e1 = y0 - A.sx0[3 * j + 1],
-                                                 e2 = z0 - A.sx0[3 * j + 2];
-                                    if (!(e0 * e0 + e1 * e1 + e2 * e2 < A.rlim2)) continue;
-                                    if (A.smol[j] == moli && cmm_excluded(A, ai, A.sorig[j])) continue;
-                                    const double4 xj = A.sx[j];
-                                    const int fj2 = A.sflag[j];
-                                    const double dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-                                    const double2 c = A.lj[(fj2 & 0xfff) * nt + ti];
-                                    const double d2i = 1.0 / (dx * dx + dy * dy + dz * dz);
-                                    const double d6i = d2i * d2i * d2i;
-                                    const double v1 = c.x * d6i, v2 = c.y * d6i * d6i;
-                                    const double cc = (-6.0 * v1 + 12.0 * v2) * d2i;
-                                    acc.add(MULTI ? cmm_cls(P, ci, (fj2 >> 13) & 1) : 0, dx, dy, dz, cc, 0.0,
-                                            v2 - v1);
-                                }
-                            }
-                        }
+        const int nic = min(256, nvi - i0);
+        const int R = 256 / nic, r = t / nic, li = t - r * nic;
+        const bool act = r < R;
+        const int p = s_vi[act ? li : 0];
+        const double xi = A.sx0[3 * p], yi = A.sx0[3 * p + 1], zi = A.sx0[3 * p + 2];
+        const int fi = (A.sflag[p] >> 13) & 1, ai = A.sorig[p], moli = A.smol[p];
+        const unsigned long long mask = A.xmask[ai];
+        const bool wide = A.xwide[ai];
+        int nvw[3] = {0, 0, 0}, listed = 0, w = 0;
+        for (int pass = 0; pass < (FILL ? 2 : 1); ++pass) {
+            if (FILL && pass == 1) {   // this replica's entries follow those of replicas 0 .. r-1
+                s_c[0][t] = listed;
+                __syncthreads();
+                w = A.l2ptr[p];
+                for (int rr = 0; rr < r; ++rr) w += s_c[0][rr * nic + li];
             }
-            cmm_reduce_store<MULTI, E>(A, acc, L, p, true, s);
+            for (int q0 = 0; q0 < J; q0 += CMM_L2CH) {
+                const int nq = min(CMM_L2CH, J - q0);
+                __syncthreads();
+                for (int u = t; u < nq; u += 256) {
+                    const int g = q0 + u;
+                    int lo = 0, hi = s_n - 1;   // the run holding g: s_off[lo] <= g < s_off[lo + 1]
+                    while (lo < hi) {
+                        const int mid = (lo + hi + 1) >> 1;
+                        if (s_off[mid] <= g) lo = mid;
+                        else hi = mid - 1;
+                    }
+                    const int pj = s_run[lo] + (g - s_off[lo]);
+                    s_x[u] = A.sx0[3 * pj]; s_y[u] = A.sx0[3 * pj + 1]; s_z[u] = A.sx0[3 * pj + 2];
+                    s_f[u] = A.sflag[pj]; s_a[u] = A.sorig[pj]; s_m[u] = A.smol[pj]; s_p[u] = pj;
+                }
+                __syncthreads();
+                if (act) {
+                    for (int u = r; u < nq; u += R) {
+                        const int fj = s_f[u];
+                        if (!((fj >> 12) & 1)) continue;
+                        const double d0 = xi - s_x[u], d1 = yi - s_y[u], d2 = zi - s_z[u];
+                        if (!(d0 * d0 + d1 * d1 + d2 * d2 < A.rlim2)) continue;
+                        if (!FILL) nvw[cmm_cls(P, fi, (fj >> 13) & 1)]++;
+                        if (cmm_excl(A, ai, s_a[u], mask, wide, moli, s_m[u])) continue;
+                        if (FILL && pass == 1) A.l2idx[w++] = s_p[u];
+                        else ++listed;
+                    }
+                }
+            }
+        }
+        if (!FILL) {
+            __syncthreads();
+            s_c[0][t] = act ? listed : 0;
+            s_c[1][t] = act ? nvw[0] : 0;
+            s_c[2][t] = act ? nvw[1] : 0;
+            s_c[3][t] = act ? nvw[2] : 0;
+            __syncthreads();
+            if (act && r == 0) {
+                int c[4] = {0, 0, 0, 0};
+                for (int rr = 0; rr < R; ++rr)
+                    for (int m = 0; m < 4; ++m) c[m] += s_c[m][rr * nic + li];
+                A.l2cnt[p] = c[0];
+                if (c[1] > A.capvw || c[2] > A.capvw || c[3] > A.capvw) atomicOr(A.err, 2);
+            }
         }
         __syncthreads();
     }
 }
 
+// The sorted positions of the atoms with a second-nearest LJ table, in order.
+__global__ void k_cmm_vflag(CmmArgs A) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < A.n) A.vflag[p] = A.l2cnt[p] > 0 ? 1 : 0;
+}
+
+__global__ void k_cmm_vfill(CmmArgs A) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < A.n && A.l2cnt[p] > 0) A.vlist[A.vpos[p]] = p;
+}
+
+// LJ of the second-nearest tables, per step: one wavefront per atom with a
+// table, lanes over its entries, a fixed butterfly sum, added to the atom's
+// near-field sums (enear's third table).
+template <bool MULTI, bool E>
+__global__ void __launch_bounds__(256) k_cmm_lj2(CmmArgs A) {
+    using Acc = CmmAcc<MULTI>;
+    constexpr int NV = Acc::NV, NC = Acc::NC;
+    const int w = (int)((blockIdx.x * 256 + threadIdx.x) >> 6), lane = threadIdx.x & 63;
+    if (w >= A.vpos[A.n]) return;
+    const SysP &P = A.sp[0];
+    const int p = A.vlist[w];
+    const double4 xi = A.sx[p];
+    const int fi = A.sflag[p];
+    const int ti = fi & 0xfff, ci = (fi >> 13) & 1, nt = A.ntyp;
+    Acc acc;
+    acc.zero();
+    const int b = A.l2ptr[p], e = b + A.l2cnt[p];
+    for (int k = b + lane; k < e; k += 64) {
+        const int j = A.l2idx[k];
+        const double4 xj = A.sx[j];
+        const int fj = A.sflag[j];
+        const double dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+        const double2 c = A.lj[(fj & 0xfff) * nt + ti];
+        const double ri = cmm_rsqrt(dx * dx + dy * dy + dz * dz);
+        const double d2i = ri * ri;
+        const double d6i = d2i * d2i * d2i;
+        const double v1 = c.x * d6i, v2 = c.y * d6i * d6i;
+        const double cc = (-6.0 * v1 + 12.0 * v2) * d2i;
+        acc.add(MULTI ? cmm_cls(P, ci, (fj >> 13) & 1) : 0, dx, dy, dz, cc, 0.0, v2 - v1);
+    }
+#pragma unroll
+    for (int m = 0; m < NV; ++m)
+        for (int o = 32; o > 0; o >>= 1) acc.v[m] = acc.v[m] + __shfl_xor(acc.v[m], o, 64);
+    if (lane == 0) {
+        double *G = A.gnb + (size_t)p * 9;
+        for (int k = 0; k < NC; ++k)
+            for (int d = 0; d < 3; ++d) G[3 * k + d] = G[3 * k + d] + acc.v[3 * k + d];
+        if (E) {
+            double *En = A.enb + (size_t)p * 6;
+            for (int k = 0; k < NC; ++k) En[3 + k] = En[3 + k] + acc.v[4 * NC + k];
+        }
+    }
+}
+
+#define CMM_FCH 128      // far-cell records staged per pass of k_cmm_far
+#define CMM_MAXFRUN 1024
+
 // Far field (compute_particle_multipole, cmmdo.f90:164-288): for every atom,
 // per source fragment and level, the cells of that level in the far list of
 // its ancestor (mk_cm: the children of the ancestor's parent's neighbours
 // less the ancestor's own neighbours; all cells less the neighbours at the
-// top level) that hold atoms of the fragment.  The per-atom sums run in
-// md_run's order; the gradient is scaled by chgmod2 = ccoef q at the end.
+// top level) that hold atoms of the fragment.  One block per level-1 cell:
+// the far list as runs of x-adjacent records, staged through LDS in chunks;
+// thread t takes atom t % nic and, as replica t / nic of R = 256 / nic,
+// every R-th record; replicas are summed in replica order and the gradient
+// is scaled by chgmod2 = ccoef q.
 template <bool MULTI>
-__global__ void __launch_bounds__(64) k_cmm_far(CmmArgs A) {
-#pragma clang fp contract(off)
-    const int cell = blockIdx.x, nf = A.nf, nv1 = A.nv1;
+__global__ void __launch_bounds__(CMM_NT) k_cmm_far(CmmArgs A) {
+    constexpr int NC = MULTI ? 3 : 1, NV = 4 * NC;
+    __shared__ double s_rec[CMM_FCH * 16];
+    __shared__ int s_ok[CMM_FCH];
+    __shared__ int s_base[CMM_MAXFRUN], s_off[CMM_MAXFRUN + 1], s_kk[CMM_MAXFRUN], s_n;
+    __shared__ double s_red[NV * CMM_NT];
+    const int cell = blockIdx.x, nf = A.nf, nv1 = A.nv1, t = threadIdx.x;
     const int cs = A.kstart[cell * nf * 2], ce = A.kstart[(cell + 1) * nf * 2];
     if (ce == cs) return;
     const SysP &P = A.sp[0];
-    constexpr int NC = MULTI ? 3 : 1;
-    for (int i0 = cs; i0 < ce; i0 += 64) {
-        const int p = i0 + (int)threadIdx.x;
-        const bool act = p < ce;
-        const int a = act ? A.sorig[p] : 0;
-        const double4 xi = act ? A.sx[p] : make_double4(0.0, 0.0, 0.0, 0.0);
-        const int fi = act ? A.sflag[p] : 0;
-        const int k = (fi >> 13) & 1;
-        double G[NC][3], Ec[NC];
-        for (int m = 0; m < NC; ++m) { G[m][0] = G[m][1] = G[m][2] = 0.0; Ec[m] = 0.0; }
-        const double qa = act ? A.q[a] : 0.0;
+    if (t == 0) {
+        int n = 0, off = 0;
         for (int kk = 0; kk < nf; ++kk) {
-            const int ncls = MULTI ? cmm_cls(P, kk, k) : 0;
             int ix = cell % nv1, iy = (cell / nv1) % nv1, iz = cell / (nv1 * nv1);
             for (int l = 0; l < A.nlev; ++l) {
                 if (l > 0) { ix >>= 1; iy >>= 1; iz >>= 1; }
                 const int mm = A.nv[l];
                 int lo[3], hi[3];
                 const int ii[3] = {ix, iy, iz};
-                for (int t = 0; t < 3; ++t) {
+                for (int d = 0; d < 3; ++d) {
                     if (l == A.nlev - 1) {
-                        lo[t] = 0; hi[t] = mm - 1;
+                        lo[d] = 0; hi[d] = mm - 1;
                     } else {
-                        const int m1 = A.nv[l + 1], pj = ii[t] >> 1;
-                        lo[t] = max(pj - 1, 0) * 2;
-                        hi[t] = min(pj + 1, m1 - 1) * 2 + 1;
+                        const int m1 = A.nv[l + 1], pj = ii[d] >> 1;
+                        lo[d] = max(pj - 1, 0) * 2;
+                        hi[d] = min(pj + 1, m1 - 1) * 2 + 1;
                     }
                 }
-                const double *cgl = A.cg + A.cgoff[l];
+                auto run = [&](int xa, int xb, int y, int z) {
+                    s_base[n] = A.pcoff[l] + cmm_lin(mm, xa, y, z) * nf + kk;
+                    s_off[n] = off;
+                    s_kk[n] = kk;
+                    off += xb - xa + 1;
+                    ++n;
+                };
                 for (int jz = lo[2]; jz <= hi[2]; ++jz)
-                    for (int jy = lo[1]; jy <= hi[1]; ++jy)
-                        for (int jx = lo[0]; jx <= hi[0]; ++jx) {
-                            if (abs(jz - iz) <= 1 && abs(jy - iy) <= 1 && abs(jx - ix) <= 1) continue;
-                            const int jc = cmm_lin(mm, jx, jy, jz);
-                            if (A.npc[A.pcoff[l] + jc * nf + kk] == 0) continue;
-                            const double *q = A.pot + (size_t)(A.pcoff[l] + jc * nf + kk) * 13;
-                            const double dx = xi.x - (cgl[jx] + A.c0[0]), dy = xi.y - (cgl[jy] + A.c0[1]),
-                                         dz = xi.z - (cgl[jz] + A.c0[2]);
-                            const double dx2 = dx * dx, dy2 = dy * dy, dz2 = dz * dz;
-                            const double d2i = 1.0 / (dx2 + dy2 + dz2);
-                            const double di = sqrt(d2i), d3i = di * d2i, d5i = d3i * d2i;
-                            const double dxy = dx * dy, dyz = dy * dz, dzx = dz * dx;
-                            const double aa1 = q[1] * dx + q[2] * dy + q[3] * dz;
-                            const double aa2 = q[4] * dx2 + q[5] * dy2 + q[6] * dz2 + q[10] * dxy + q[11] * dyz +
-                                               q[12] * dzx;
-                            const double e = q[0] * di + aa1 * d3i + aa2 * d5i;
-                            const double bbx = (q[0] * dx - q[1]) * d3i, bby = (q[0] * dy - q[2]) * d3i,
-                                         bbz = (q[0] * dz - q[3]) * d3i;
-                            const double cc2x = q[7] * dx + q[10] * dy + q[12] * dz;
-                            const double cc2y = q[10] * dx + q[8] * dy + q[11] * dz;
-                            const double cc2z = q[12] * dx + q[11] * dy + q[9] * dz;
-                            const double cc4 = aa1 * 3.0 + aa2 * d2i * 5.0;
-                            const double gx = -bbx + (cc2x - cc4 * dx) * d5i;
-                            const double gy = -bby + (cc2y - cc4 * dy) * d5i;
-                            const double gz = -bbz + (cc2z - cc4 * dz) * d5i;
-                            const double ea = qa * A.ccoef * e;
-#pragma unroll
-                            for (int m = 0; m < NC; ++m) {
-                                const bool on = m == ncls;
-                                G[m][0] = G[m][0] + (on ? gx : 0.0);
-                                G[m][1] = G[m][1] + (on ? gy : 0.0);
-                                G[m][2] = G[m][2] + (on ? gz : 0.0);
-                                Ec[m] = Ec[m] + (on ? ea : 0.0);
-                            }
+                    for (int jy = lo[1]; jy <= hi[1]; ++jy) {
+                        if (abs(jz - iz) <= 1 && abs(jy - iy) <= 1) {
+                            if (lo[0] <= ix - 2) run(lo[0], ix - 2, jy, jz);
+                            if (ix + 2 <= hi[0]) run(ix + 2, hi[0], jy, jz);
+                        } else {
+                            run(lo[0], hi[0], jy, jz);
                         }
+                    }
             }
         }
-        if (act) {
-            const double q2 = A.ccoef * qa;
+        s_off[n] = off;
+        s_n = n;
+    }
+    __syncthreads();
+    const int J = s_off[s_n];
+    for (int i0 = cs; i0 < ce; i0 += CMM_NT) {
+        const int nic = min(CMM_NT, ce - i0);
+        const int R = CMM_NT / nic, r = t / nic, li = t - r * nic;
+        const bool act = r < R;
+        const int p = i0 + li;
+        const double4 xi = act ? A.sx[p] : make_double4(0.0, 0.0, 0.0, 0.0);
+        const int k = act ? (A.sflag[p] >> 13) & 1 : 0;
+        double G[NV];
+#pragma unroll
+        for (int m = 0; m < NV; ++m) G[m] = 0.0;
+        for (int q0 = 0; q0 < J; q0 += CMM_FCH) {
+            const int nq = min(CMM_FCH, J - q0);
+            __syncthreads();
+            for (int u = t; u < nq * 16; u += CMM_NT) {
+                const int slot = u >> 4, m = u & 15, g = q0 + slot;
+                int lo = 0, hi = s_n - 1;
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (s_off[mid] <= g) lo = mid;
+                    else hi = mid - 1;
+                }
+                const int rec = s_base[lo] + (g - s_off[lo]) * nf;
+                s_rec[u] = A.pot[(size_t)rec * 16 + m];
+                if (m == 0) s_ok[slot] = (A.npc[rec] != 0) ? 1 + s_kk[lo] : 0;
+            }
+            __syncthreads();
+            if (act) {
+                for (int u = r; u < nq; u += R) {
+                    const int ok = s_ok[u];
+                    if (!ok) continue;
+                    const double *c = s_rec + u * 16, *q = c + 3;
+                    const double dx = xi.x - c[0], dy = xi.y - c[1], dz = xi.z - c[2];
+                    const double dx2 = dx * dx, dy2 = dy * dy, dz2 = dz * dz;
+                    const double di = cmm_rsqrt(dx2 + dy2 + dz2);
+                    const double d2i = di * di, d3i = di * d2i, d5i = d3i * d2i;
+                    const double dxy = dx * dy, dyz = dy * dz, dzx = dz * dx;
+                    const double aa1 = q[1] * dx + q[2] * dy + q[3] * dz;
+                    const double aa2 = q[4] * dx2 + q[5] * dy2 + q[6] * dz2 + q[10] * dxy + q[11] * dyz + q[12] * dzx;
+                    const double e = q[0] * di + aa1 * d3i + aa2 * d5i;
+                    const double bbx = (q[0] * dx - q[1]) * d3i, bby = (q[0] * dy - q[2]) * d3i,
+                                 bbz = (q[0] * dz - q[3]) * d3i;
+                    const double cc2x = q[7] * dx + q[10] * dy + q[12] * dz;
+                    const double cc2y = q[10] * dx + q[8] * dy + q[11] * dz;
+                    const double cc2z = q[12] * dx + q[11] * dy + q[9] * dz;
+                    const double cc4 = aa1 * 3.0 + aa2 * d2i * 5.0;
+                    const int ncls = MULTI ? cmm_cls(P, ok - 1, k) : 0;
+#pragma unroll
+                    for (int m = 0; m < NC; ++m) {
+                        const bool on = !MULTI || m == ncls;
+                        G[4 * m] = G[4 * m] + (on ? -bbx + (cc2x - cc4 * dx) * d5i : 0.0);
+                        G[4 * m + 1] = G[4 * m + 1] + (on ? -bby + (cc2y - cc4 * dy) * d5i : 0.0);
+                        G[4 * m + 2] = G[4 * m + 2] + (on ? -bbz + (cc2z - cc4 * dz) * d5i : 0.0);
+                        G[4 * m + 3] = G[4 * m + 3] + (on ? e : 0.0);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < NV; ++m) s_red[m * CMM_NT + t] = G[m];
+        __syncthreads();
+        if (act && r == 0) {
+            double tot[NV];
+#pragma unroll
+            for (int m = 0; m < NV; ++m) tot[m] = 0.0;
+            for (int rr = 0; rr < R; ++rr)
+#pragma unroll
+                for (int m = 0; m < NV; ++m) tot[m] = tot[m] + s_red[m * CMM_NT + rr * nic + li];
+            const int a = A.sorig[p];
+            const double qa = A.q[a], q2 = A.ccoef * qa;
             for (int m = 0; m < 3; ++m) {
-                for (int t = 0; t < 3; ++t) A.gwk[((size_t)a * 3 + m) * 3 + t] = m < NC ? G[m][t] * q2 : 0.0;
-                A.efar[(size_t)a * 3 + m] = m < NC ? Ec[m] : 0.0;
+                for (int d = 0; d < 3; ++d) A.gwk[((size_t)a * 3 + m) * 3 + d] = m < NC ? tot[4 * m + d] * q2 : 0.0;
+                A.efar[(size_t)a * 3 + m] = m < NC ? qa * A.ccoef * tot[4 * m + 3] : 0.0;
             }
         }
     }

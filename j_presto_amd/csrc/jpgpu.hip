@@ -315,7 +315,10 @@ struct Engine {
         d_cmm_eatom, d_cmm_wb, d_cmm_wb2;
     DBuf<double4> d_cmm_sx;
     DBuf<int> d_cmm_key, d_cmm_kcnt, d_cmm_kstart, d_cmm_kfill, d_cmm_tmp, d_cmm_sorig, d_cmm_spos, d_cmm_sflag,
-        d_cmm_smol, d_cmm_npc, d_cmm_err;
+        d_cmm_smol, d_cmm_npc, d_cmm_err, d_cmm_xwide, d_cmm_l2cnt, d_cmm_l2ptr, d_cmm_l2idx, d_cmm_vflag,
+        d_cmm_vpos, d_cmm_vlist;
+    DBuf<unsigned long long> d_cmm_xmask;
+    int cmm_vcap = 0;                   // k_cmm_lj2's grid: atoms with a second-nearest table, only grows
     int set_cmm(int on, double cminsiz, int nlev, int cmmupd);
     int cmm_celprp();
     int cmm_forces(bool energy, bool itwin);
@@ -390,6 +393,8 @@ struct Engine {
         d_cmm_wb.release(); d_cmm_wb2.release(); d_cmm_sx.release(); d_cmm_key.release(); d_cmm_kcnt.release();
         d_cmm_kstart.release(); d_cmm_kfill.release(); d_cmm_tmp.release(); d_cmm_sorig.release();
         d_cmm_spos.release(); d_cmm_sflag.release(); d_cmm_smol.release(); d_cmm_npc.release(); d_cmm_err.release();
+        d_cmm_xwide.release(); d_cmm_l2cnt.release(); d_cmm_l2ptr.release(); d_cmm_l2idx.release();
+        d_cmm_vflag.release(); d_cmm_vpos.release(); d_cmm_vlist.release(); d_cmm_xmask.release();
         if (h_rsum) hipHostFree(h_rsum);
         if (h_rfix) hipHostFree(h_rfix);
         if (h_err) hipHostFree(h_err);
@@ -2082,13 +2087,17 @@ void Engine::drop_graphs() {
 Engine::StepGraph *Engine::step_graph(int n, int prec) {
     const int variant = pair_variant();
     const int ckey = cmm ? 2 * (int)(idloop % cmm_upd) + (cmm_itwin ? 1 : 0) : -1;
+    // the list a graph's kernels read: ZD's entries and segment size, CMM's
+    // second-nearest table and its grid
+    const void *lent = cmm ? (const void *)d_cmm_l2idx.p : (const void *)d_ent.p;
+    const int lcap = cmm ? cmm_vcap : ccap;
     for (auto &g : graphs)
-        if (g.exec && g.nsteps == n && g.prec == prec && g.ccap == ccap && g.ent == d_ent.p && g.variant == variant &&
+        if (g.exec && g.nsteps == n && g.prec == prec && g.ccap == lcap && g.ent == lent && g.variant == variant &&
             g.cmm_key == ckey)
             return &g;
     // the list buffers changed: stale graphs go
     for (auto it = graphs.begin(); it != graphs.end();) {
-        if (it->ccap != ccap || it->ent != d_ent.p) {
+        if (it->ccap != lcap || it->ent != lent) {
             if (it->exec) hipGraphExecDestroy(it->exec);
             it = graphs.erase(it);
         } else {
@@ -2096,7 +2105,7 @@ Engine::StepGraph *Engine::step_graph(int n, int prec) {
         }
     }
     StepGraph g;
-    g.nsteps = n; g.prec = prec; g.ccap = ccap; g.ent = d_ent.p; g.variant = variant; g.cmm_key = ckey;
+    g.nsteps = n; g.prec = prec; g.ccap = lcap; g.ent = lent; g.variant = variant; g.cmm_key = ckey;
     ++n_captures;
     hipGraph_t graph = nullptr;
     if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) return nullptr;
@@ -2428,6 +2437,7 @@ int Engine::set_cmm(int on, double cminsiz, int nlev, int cmmupd) {
     A.nc1 = A.nv1 * A.nv1 * A.nv1;
     A.rlim2 = h.cutoff * h.cutoff;
     A.idelay = (int)std::ceil(h.cutoff / A.siz[0]);
+    if (A.idelay > 15) return JP_EINVAL;   // cells below cutoff / 15: beyond k_cmm_l2's run table
     A.ccoef = h.ccoef;
     const double vol = std::pow(4.0 * A.siz[0], 3.0);
     A.ipmax = (int)std::ceil(A.siz[0] * A.siz[0] * A.siz[0] * 0.27) * 2;
@@ -2445,12 +2455,14 @@ int Engine::set_cmm(int on, double cminsiz, int nlev, int cmmupd) {
         npc += A.nv[l] * A.nv[l] * A.nv[l] * A.nf;
     }
     const size_t n = (size_t)natom;
-    AL(d_cmm_cg, cg.size()); AL(d_cmm_c0, 3); AL(d_cmm_q, n); AL(d_cmm_sx0, 3 * n); AL(d_cmm_pot, 13 * (size_t)npc);
+    AL(d_cmm_cg, cg.size()); AL(d_cmm_c0, 3); AL(d_cmm_q, n); AL(d_cmm_sx0, 3 * n); AL(d_cmm_pot, 16 * (size_t)npc);
     AL(d_cmm_gnb, 9 * n); AL(d_cmm_enb, 6 * n); AL(d_cmm_gwk, 9 * n); AL(d_cmm_efar, 3 * n); AL(d_cmm_eatom, 6 * n);
     AL(d_cmm_wb, 12 * n); AL(d_cmm_wb2, 12 * n); AL(d_cmm_sx, n);
     AL(d_cmm_key, n); AL(d_cmm_kcnt, A.nkey); AL(d_cmm_kstart, A.nkey + 1); AL(d_cmm_kfill, A.nkey);
     AL(d_cmm_tmp, n); AL(d_cmm_sorig, n); AL(d_cmm_spos, n); AL(d_cmm_sflag, n); AL(d_cmm_smol, n);
     AL(d_cmm_npc, npc); AL(d_cmm_err, 2);
+    AL(d_cmm_xmask, n); AL(d_cmm_xwide, n); AL(d_cmm_l2cnt, n); AL(d_cmm_l2ptr, n + 1); AL(d_cmm_l2idx, 1);
+    AL(d_cmm_vflag, n); AL(d_cmm_vpos, n + 1); AL(d_cmm_vlist, n);
     CK(hipMemcpyAsync(d_cmm_cg.p, cg.data(), 8 * cg.size(), hipMemcpyHostToDevice, st));
     CK(hipMemcpyAsync(d_cmm_q.p, charge_h.data(), 8 * n, hipMemcpyHostToDevice, st));
     CK(hipMemsetAsync(d_cmm_gwk.p, 0, 72 * n, st));
@@ -2467,7 +2479,12 @@ int Engine::set_cmm(int on, double cminsiz, int nlev, int cmmupd) {
     A.smol = d_cmm_smol.p; A.sx = d_cmm_sx.p; A.sx0 = d_cmm_sx0.p; A.npc = d_cmm_npc.p; A.pot = d_cmm_pot.p;
     A.err = d_cmm_err.p; A.sp = d_sp.p; A.efac = d_efac.p; A.gnb = d_cmm_gnb.p; A.enb = d_cmm_enb.p;
     A.gwk = d_cmm_gwk.p; A.efar = d_cmm_efar.p; A.gslot = d_gslot.p; A.eatom = d_cmm_eatom.p;
+    A.xmask = d_cmm_xmask.p; A.xwide = d_cmm_xwide.p; A.l2cnt = d_cmm_l2cnt.p; A.l2ptr = d_cmm_l2ptr.p;
+    A.l2idx = d_cmm_l2idx.p; A.vflag = d_cmm_vflag.p; A.vpos = d_cmm_vpos.p; A.vlist = d_cmm_vlist.p;
     cm = A;
+    hipLaunchKernelGGL(k_cmm_xmask, grid1(n, 256), 256, 0, st, A);
+    CK(hipGetLastError());
+    cmm_vcap = 0;
     hs[0].dself[0] = hs[0].dself[1] = hs[0].dself[2] = 0.0;   // no ZD self energy under CMM
     if (alsd) {
         const double z[3] = {0.0, 0.0, 0.0};
@@ -2503,12 +2520,32 @@ int Engine::cmm_celprp() {
         hipLaunchKernelGGL(k_cmm_pol, grid1(nq, 128), 128, 0, st, A, l);
     }
     hipLaunchKernelGGL(k_cmm_check, gn, 256, 0, st, A);
+    if (A.idelay > 1) hipLaunchKernelGGL(k_cmm_l2<false>, A.nc1, 256, 0, st, A);
+    // the second-nearest LJ tables: lengths -> offsets, the atoms that have one
+    hipLaunchKernelGGL(k_scan, 1, 1024, 0, st, natom, (const int *)A.l2cnt, A.l2ptr);
+    hipLaunchKernelGGL(k_cmm_vflag, gn, 256, 0, st, A);
+    hipLaunchKernelGGL(k_scan, 1, 1024, 0, st, natom, (const int *)A.vflag, A.vpos);
     CK(hipGetLastError());
-    CK(hipMemcpyAsync(h_err, d_cmm_err.p, 8, hipMemcpyDeviceToHost, st));
+    int st4[4] = {0, 0, 0, 0};
+    CK(hipMemcpyAsync(st4, d_cmm_err.p, 8, hipMemcpyDeviceToHost, st));
+    CK(hipMemcpyAsync(st4 + 2, A.l2ptr + natom, 4, hipMemcpyDeviceToHost, st));
+    CK(hipMemcpyAsync(st4 + 3, A.vpos + natom, 4, hipMemcpyDeviceToHost, st));
     CK(hipStreamSynchronize(st));
-    if (h_err[0] & 1) return JP_IER_CELL;    // celpar.f90:172-192
-    if (h_err[0] & 2) return JP_IER_LIST;    // neartb.f90:368-400
-    if (h_err[1] != 2 * hs[0].nexcl) return JP_IER_EXCL;   // neartb.f90:446-453
+    if (st4[0] & 1) return JP_IER_CELL;    // celpar.f90:172-192
+    if (st4[0] & 2) return JP_IER_LIST;    // neartb.f90:368-400
+    if (st4[1] != 2 * hs[0].nexcl) return JP_IER_EXCL;   // neartb.f90:446-453
+    if ((size_t)st4[2] > d_cmm_l2idx.n) {   // grows only: captured step graphs hold the pointer
+        drop_graphs();
+        AL(d_cmm_l2idx, (size_t)st4[2] + (size_t)st4[2] / 4);
+        cm.l2idx = d_cmm_l2idx.p;
+    }
+    if (st4[3] > cmm_vcap) {
+        drop_graphs();
+        cmm_vcap = st4[3] + st4[3] / 8;
+    }
+    if (A.idelay > 1) hipLaunchKernelGGL(k_cmm_l2<true>, A.nc1, 256, 0, st, cm);
+    hipLaunchKernelGGL(k_cmm_vfill, gn, 256, 0, st, cm);
+    CK(hipGetLastError());
     return 0;
 }
 
@@ -2516,9 +2553,10 @@ template <bool MULTI, bool E>
 static void launch_cmm(Engine &g, bool itwin) {
     const CmmArgs &A = g.cm;
     hipLaunchKernelGGL(k_cmm_gather, grid1(A.n, 256), 256, 0, g.st, A);
-    hipLaunchKernelGGL((k_cmm_near<MULTI, E>), A.nc1, 64, 0, g.st, A);
-    if (A.idelay > 1) hipLaunchKernelGGL((k_cmm_lj2<MULTI, E>), A.nc1, 64, 0, g.st, A);
-    if (itwin) hipLaunchKernelGGL(k_cmm_far<MULTI>, A.nc1, 64, 0, g.st, A);
+    hipLaunchKernelGGL((k_cmm_near<MULTI, E>), A.nc1, CMM_NT, sizeof(double2) * A.ntyp * A.ntyp, g.st, A);
+    if (A.idelay > 1 && g.cmm_vcap > 0)
+        hipLaunchKernelGGL((k_cmm_lj2<MULTI, E>), grid1(g.cmm_vcap, 4), 256, 0, g.st, A);
+    if (itwin) hipLaunchKernelGGL(k_cmm_far<MULTI>, A.nc1, CMM_NT, 0, g.st, A);
     hipLaunchKernelGGL(k_cmm_combine<E>, grid1(A.n, 256), 256, 0, g.st, A);
 }
 
@@ -2548,18 +2586,19 @@ int Engine::cmm_time(int reps, double &near, double &far) {
     const CmmArgs &A = cm;
     const bool multi = hs[0].nfrg > 1;
     CK(hipEventRecord(ev[0], st));
+    const size_t lds = sizeof(double2) * A.ntyp * A.ntyp;
     for (int k = 0; k < reps; ++k) {
-        if (multi) hipLaunchKernelGGL((k_cmm_near<true, false>), A.nc1, 64, 0, st, A);
-        else hipLaunchKernelGGL((k_cmm_near<false, false>), A.nc1, 64, 0, st, A);
-        if (A.idelay > 1) {
-            if (multi) hipLaunchKernelGGL((k_cmm_lj2<true, false>), A.nc1, 64, 0, st, A);
-            else hipLaunchKernelGGL((k_cmm_lj2<false, false>), A.nc1, 64, 0, st, A);
+        if (multi) hipLaunchKernelGGL((k_cmm_near<true, false>), A.nc1, CMM_NT, lds, st, A);
+        else hipLaunchKernelGGL((k_cmm_near<false, false>), A.nc1, CMM_NT, lds, st, A);
+        if (A.idelay > 1 && cmm_vcap > 0) {
+            if (multi) hipLaunchKernelGGL((k_cmm_lj2<true, false>), grid1(cmm_vcap, 4), 256, 0, st, A);
+            else hipLaunchKernelGGL((k_cmm_lj2<false, false>), grid1(cmm_vcap, 4), 256, 0, st, A);
         }
     }
     CK(hipEventRecord(ev[1], st));
     for (int k = 0; k < reps; ++k) {
-        if (multi) hipLaunchKernelGGL(k_cmm_far<true>, A.nc1, 64, 0, st, A);
-        else hipLaunchKernelGGL(k_cmm_far<false>, A.nc1, 64, 0, st, A);
+        if (multi) hipLaunchKernelGGL(k_cmm_far<true>, A.nc1, CMM_NT, 0, st, A);
+        else hipLaunchKernelGGL(k_cmm_far<false>, A.nc1, CMM_NT, 0, st, A);
     }
     CK(hipEventRecord(ev[2], st));
     CK(hipGetLastError());
