@@ -72,6 +72,12 @@ struct CmmArgs {
     int *xwide;                 // [a] 1: some excluded partner beyond that window
     int *l2cnt, *l2ptr, *l2idx; // second-nearest LJ table per sorted atom (fixed between rebuilds)
     int *vflag, *vpos, *vlist;  // the sorted atoms with such a table; how many at vpos[n]
+    // the vdW atoms alone, in the sorted order (second-nearest candidates):
+    // kvstart[key] how many precede the key, then per vdW atom its
+    // coordinates at the rebuild, flags, atom number, molecule, sorted position
+    int *kvcnt, *kvstart;
+    double4 *vx0;
+    int *vf, *va, *vm, *vp;
 };
 
 __device__ __forceinline__ int cmm_lin(int nv, int x, int y, int z) { return (z * nv + y) * nv + x; }
@@ -89,6 +95,45 @@ __device__ __forceinline__ double cmm_rsqrt(double x) {
     y = y * (1.5 - hx * y * y);
     y = y * (1.5 - hx * y * y);
     return y;
+}
+
+// ------------------------------------------------------------------ scans
+
+// Exclusive scan of n ints in three launches: k_scan_tile scans tiles of
+// 1024 (4 per thread) and stores their totals, k_scan (one block) scans the
+// totals in place, k_scan_add adds each tile's offset; out[n] receives the
+// total.  tsum holds ceil(n / 1024) + 1 ints.
+__global__ void __launch_bounds__(256) k_scan_tile(int n, const int *__restrict__ in, int *__restrict__ out,
+                                                   int *__restrict__ tsum) {
+    __shared__ int wsum[4];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int i0 = blockIdx.x * 1024 + 4 * t;
+    int v[4], s = 0;
+    for (int k = 0; k < 4; ++k) {
+        v[k] = i0 + k < n ? in[i0 + k] : 0;
+        s += v[k];
+    }
+    int x = s;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    int base = 0;
+    for (int k = 0; k < w; ++k) base += wsum[k];
+    int e = base + x - s;
+    for (int k = 0; k < 4; ++k) {
+        if (i0 + k < n) out[i0 + k] = e;
+        e += v[k];
+    }
+    if (t == 255) tsum[blockIdx.x] = base + x;
+}
+
+__global__ void k_scan_add(int n, int *__restrict__ out, const int *__restrict__ tsum, int ntile) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] += tsum[i >> 10];
+    if (i == 0) out[n] = tsum[ntile];
 }
 
 // ------------------------------------------------------------------ rebuild
@@ -179,6 +224,26 @@ __global__ void k_cmm_fill(CmmArgs A) {
     A.sx0[3 * p] = x0; A.sx0[3 * p + 1] = x1; A.sx0[3 * p + 2] = x2;
     A.sflag[p] = A.tflag[a];
     A.smol[p] = A.mol[a];
+}
+
+// The vdW atoms alone (second-nearest candidates): their count per key, and,
+// after its scan, their copy in the sorted order.
+__global__ void k_cmm_kv(CmmArgs A) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < A.nkey) A.kvcnt[k] = (k & 1) ? A.kcnt[k] : 0;
+}
+
+__global__ void k_cmm_vcopy(CmmArgs A) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= A.n || (A.err[0] & 1)) return;
+    const int a = A.sorig[p], k = A.key[a];
+    if (!(k & 1)) return;
+    const int q = A.kvstart[k] + (p - A.kstart[k]);
+    A.vx0[q] = make_double4(A.sx0[3 * p], A.sx0[3 * p + 1], A.sx0[3 * p + 2], 0.0);
+    A.vf[q] = A.sflag[p];
+    A.va[q] = a;
+    A.vm[q] = A.smol[p];
+    A.vp[q] = p;
 }
 
 // celoth: atoms per (cell, fragment), level 1 from the keys (with celpar's
@@ -368,6 +433,37 @@ __global__ void k_cmm_check(CmmArgs A) {
     if (found) atomicAdd(A.err + 1, found);
 }
 
+// In-place exclusive scan of a[0, n) in LDS by a block of 256 threads
+// (n <= 2048, 8 entries per thread); a[n] receives the total.  Every thread
+// of the block calls it; it ends with a barrier.
+__device__ void cmm_block_scan(int *a, int n) {
+    __shared__ int ws[4];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6, b = 8 * t;
+    int v[8], s = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        v[k] = b + k < n ? a[b + k] : 0;
+        s += v[k];
+    }
+    int x = s;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) ws[w] = x;
+    __syncthreads();
+    int base = 0;
+    for (int k = 0; k < w; ++k) base += ws[k];
+    int e = base + x - s;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        if (b + k < n) a[b + k] = e;
+        e += v[k];
+    }
+    if (t == 255) a[n] = base + x;
+    __syncthreads();
+}
+
 // Per-thread sums of one i atom: gradient per class and energies per class.
 template <bool MULTI>
 struct CmmAcc {
@@ -407,7 +503,7 @@ __global__ void __launch_bounds__(CMM_NT) k_cmm_near(CmmArgs A) {
     constexpr int NV = Acc::NV, NC = Acc::NC;
     constexpr int SJ = 4 * CMM_JCH + 3 * CMM_JCH / 2, SR = CMM_NT * NV;
     __shared__ double s_buf[SJ > SR ? SJ : SR];
-    __shared__ int s_run[9], s_off[10], s_nrun;
+    __shared__ int s_run[9], s_off[10];
     extern __shared__ double2 s_lj[];
     double *jx = s_buf, *jy = jx + CMM_JCH, *jz = jy + CMM_JCH, *jq = jz + CMM_JCH;
     int *jf = (int *)(jq + CMM_JCH), *ja = jf + CMM_JCH, *jm = ja + CMM_JCH;
@@ -418,23 +514,28 @@ __global__ void __launch_bounds__(CMM_NT) k_cmm_near(CmmArgs A) {
     const int nt = A.ntyp;
     for (int k = t; k < nt * nt; k += CMM_NT) s_lj[k] = A.lj[k];
     const int cx = cell % nv, cy = (cell / nv) % nv, cz = cell / (nv * nv);
-    if (t == 0) {
-        int n = 0, off = 0;
-        const int x0 = max(cx - 1, 0), x1 = min(cx + 1, nv - 1);
-        for (int z = max(cz - 1, 0); z <= min(cz + 1, nv - 1); ++z)
-            for (int y = max(cy - 1, 0); y <= min(cy + 1, nv - 1); ++y) {
-                const int b = A.kstart[cmm_lin(nv, x0, y, z) * nf * 2];
-                const int e = A.kstart[(cmm_lin(nv, x1, y, z) + 1) * nf * 2];
-                s_run[n] = b;
-                s_off[n] = off;
-                off += e - b;
-                ++n;
-            }
-        s_off[n] = off;
-        s_nrun = n;
+    if (t < 9) {   // the nine runs (z, y) of x-adjacent cells, one per thread; empty off the field
+        const int z = cz + t / 3 - 1, y = cy + t % 3 - 1;
+        int b = 0, e = 0;
+        if (z >= 0 && z < nv && y >= 0 && y < nv) {
+            b = A.kstart[cmm_lin(nv, max(cx - 1, 0), y, z) * nf * 2];
+            e = A.kstart[(cmm_lin(nv, min(cx + 1, nv - 1), y, z) + 1) * nf * 2];
+        }
+        s_run[t] = b;
+        s_off[t] = e - b;
     }
     __syncthreads();
-    const int J = s_off[s_nrun];
+    if (t == 0) {
+        int off = 0;
+        for (int k = 0; k < 9; ++k) {
+            const int len = s_off[k];
+            s_off[k] = off;
+            off += len;
+        }
+        s_off[9] = off;
+    }
+    __syncthreads();
+    const int J = s_off[9];
     for (int i0 = cs; i0 < ce; i0 += CMM_NT) {
         const int nic = min(CMM_NT, ce - i0);
         const int R = CMM_NT / nic, r = t / nic, li = t - r * nic;
@@ -512,40 +613,48 @@ __global__ void __launch_bounds__(CMM_NT) k_cmm_near(CmmArgs A) {
     }
 }
 
-#define CMM_L2CH 1024   // candidates staged in LDS per pass of k_cmm_l2
-#define CMM_MAXRUN 1024 // x-runs of a shell (idelay <= 15)
+#define CMM_L2CH 512    // candidates staged in LDS per pass of k_cmm_l2
+#define CMM_MAXRUN 2048 // run slots of a shell, 2 (2 idelay + 1)^2 (idelay <= 15)
 
 // The cells at Chebyshev distance 2 .. d of cell (cx, cy, cz) as runs of
-// x-adjacent cells (contiguous atom ranges): s_run[k] their first sorted
-// atom, s_off[k] their offset in the concatenation; returns the run count.
+// x-adjacent cells (contiguous ranges of the vdW atoms), two slots per row
+// (z, y), built by the whole block: s_run[k] the first vdW atom of slot k,
+// s_off[k] its offset in the concatenation (empty slots have none); returns
+// the slot count, s_off[count] the total.
 __device__ int cmm_shell_runs(const CmmArgs &A, int cx, int cy, int cz, int d, int *s_run, int *s_off) {
-    const int nv = A.nv1, nf = A.nf;
-    int n = 0, off = 0;
-    auto run = [&](int xa, int xb, int y, int z) {
-        const int b = A.kstart[cmm_lin(nv, xa, y, z) * nf * 2], e = A.kstart[(cmm_lin(nv, xb, y, z) + 1) * nf * 2];
-        s_run[n] = b;
-        s_off[n] = off;
-        off += e - b;
-        ++n;
-    };
+    const int nv = A.nv1, nf = A.nf, w = 2 * d + 1, nrow = w * w;
     const int x0 = max(cx - d, 0), x1 = min(cx + d, nv - 1);
-    for (int z = max(cz - d, 0); z <= min(cz + d, nv - 1); ++z)
-        for (int y = max(cy - d, 0); y <= min(cy + d, nv - 1); ++y) {
+    for (int k = threadIdx.x; k < nrow; k += blockDim.x) {
+        const int z = cz - d + k / w, y = cy - d + k % w;
+        int xa[2] = {0, 0}, xb[2] = {-1, -1};
+        if (z >= 0 && z < nv && y >= 0 && y < nv) {
             if (abs(z - cz) <= 1 && abs(y - cy) <= 1) {
-                if (x0 <= cx - 2) run(x0, cx - 2, y, z);
-                if (cx + 2 <= x1) run(cx + 2, x1, y, z);
+                xa[0] = x0; xb[0] = cx - 2;
+                xa[1] = cx + 2; xb[1] = x1;
             } else {
-                run(x0, x1, y, z);
+                xa[0] = x0; xb[0] = x1;
             }
         }
-    s_off[n] = off;
-    return n;
+        for (int h = 0; h < 2; ++h) {
+            int b = 0, len = 0;
+            if (xa[h] <= xb[h]) {
+                b = A.kvstart[cmm_lin(nv, xa[h], y, z) * nf * 2];
+                len = A.kvstart[(cmm_lin(nv, xb[h], y, z) + 1) * nf * 2] - b;
+            }
+            s_run[2 * k + h] = b;
+            s_off[2 * k + h] = len;
+        }
+    }
+    __syncthreads();
+    cmm_block_scan(s_off, 2 * nrow);
+    return 2 * nrow;
 }
 
 // The second-nearest LJ tables (neartb.f90:321-366): for every vdW atom the
 // vdW atoms of the cells at Chebyshev distance 2 .. idelay within rlim at the
 // rebuild (sx0), less its excluded pairs.  One block per cell: the
-// candidates staged through LDS, thread t takes vdW atom t % nic and, as
+// candidates (the vdW atoms of the shell's cells, from the vdW copy) staged
+// through LDS, thread t takes vdW atom t % nic and, as
 // replica r = t / nic of R = 256 / nic, every R-th candidate.  COUNT: the
 // partners per class before the exclusions against nvdw (ier 51), and the
 // table lengths l2cnt.  FILL: each replica counts its entries again, and
@@ -553,7 +662,7 @@ __device__ int cmm_shell_runs(const CmmArgs &A, int cx, int cy, int cz, int d, i
 template <bool FILL>
 __global__ void __launch_bounds__(256) k_cmm_l2(CmmArgs A) {
 #pragma clang fp contract(off)
-    __shared__ int s_run[CMM_MAXRUN], s_off[CMM_MAXRUN + 1], s_n, s_vi[256], s_c[4][256];
+    __shared__ int s_run[CMM_MAXRUN], s_off[CMM_MAXRUN + 1], s_vi[256], s_c[4][256];
     __shared__ double s_x[CMM_L2CH], s_y[CMM_L2CH], s_z[CMM_L2CH];
     __shared__ int s_f[CMM_L2CH], s_a[CMM_L2CH], s_m[CMM_L2CH], s_p[CMM_L2CH];
     const int cell = blockIdx.x, nf = A.nf, nv = A.nv1, t = threadIdx.x;
@@ -563,9 +672,8 @@ __global__ void __launch_bounds__(256) k_cmm_l2(CmmArgs A) {
     if (nvi == 0) return;
     const SysP &P = A.sp[0];
     const int cx = cell % nv, cy = (cell / nv) % nv, cz = cell / (nv * nv);
-    if (t == 0) s_n = cmm_shell_runs(A, cx, cy, cz, A.idelay, s_run, s_off);
-    __syncthreads();
-    const int J = s_off[s_n];
+    const int nslot = cmm_shell_runs(A, cx, cy, cz, A.idelay, s_run, s_off);
+    const int J = s_off[nslot];
     for (int i0 = 0; i0 < nvi; i0 += 256) {
         if (t == 0) {
             int w = 0, skip = i0;
@@ -599,15 +707,16 @@ __global__ void __launch_bounds__(256) k_cmm_l2(CmmArgs A) {
                 __syncthreads();
                 for (int u = t; u < nq; u += 256) {
                     const int g = q0 + u;
-                    int lo = 0, hi = s_n - 1;   // the run holding g: s_off[lo] <= g < s_off[lo + 1]
+                    int lo = 0, hi = nslot - 1;   // the last slot with s_off <= g holds g
                     while (lo < hi) {
                         const int mid = (lo + hi + 1) >> 1;
                         if (s_off[mid] <= g) lo = mid;
                         else hi = mid - 1;
                     }
-                    const int pj = s_run[lo] + (g - s_off[lo]);
-                    s_x[u] = A.sx0[3 * pj]; s_y[u] = A.sx0[3 * pj + 1]; s_z[u] = A.sx0[3 * pj + 2];
-                    s_f[u] = A.sflag[pj]; s_a[u] = A.sorig[pj]; s_m[u] = A.smol[pj]; s_p[u] = pj;
+                    const int vj = s_run[lo] + (g - s_off[lo]);
+                    const double4 c = A.vx0[vj];
+                    s_x[u] = c.x; s_y[u] = c.y; s_z[u] = c.z;
+                    s_f[u] = A.vf[vj]; s_a[u] = A.va[vj]; s_m[u] = A.vm[vj]; s_p[u] = A.vp[vj];
                 }
                 __syncthreads();
                 if (act) {
@@ -715,53 +824,50 @@ __global__ void __launch_bounds__(CMM_NT) k_cmm_far(CmmArgs A) {
     constexpr int NC = MULTI ? 3 : 1, NV = 4 * NC;
     __shared__ double s_rec[CMM_FCH * 16];
     __shared__ int s_ok[CMM_FCH];
-    __shared__ int s_base[CMM_MAXFRUN], s_off[CMM_MAXFRUN + 1], s_kk[CMM_MAXFRUN], s_n;
+    __shared__ int s_base[CMM_MAXFRUN], s_off[CMM_MAXFRUN + 1], s_kk[CMM_MAXFRUN];
     __shared__ double s_red[NV * CMM_NT];
     const int cell = blockIdx.x, nf = A.nf, nv1 = A.nv1, t = threadIdx.x;
     const int cs = A.kstart[cell * nf * 2], ce = A.kstart[(cell + 1) * nf * 2];
     if (ce == cs) return;
     const SysP &P = A.sp[0];
-    if (t == 0) {
-        int n = 0, off = 0;
-        for (int kk = 0; kk < nf; ++kk) {
-            int ix = cell % nv1, iy = (cell / nv1) % nv1, iz = cell / (nv1 * nv1);
-            for (int l = 0; l < A.nlev; ++l) {
-                if (l > 0) { ix >>= 1; iy >>= 1; iz >>= 1; }
-                const int mm = A.nv[l];
-                int lo[3], hi[3];
-                const int ii[3] = {ix, iy, iz};
-                for (int d = 0; d < 3; ++d) {
-                    if (l == A.nlev - 1) {
-                        lo[d] = 0; hi[d] = mm - 1;
-                    } else {
-                        const int m1 = A.nv[l + 1], pj = ii[d] >> 1;
-                        lo[d] = max(pj - 1, 0) * 2;
-                        hi[d] = min(pj + 1, m1 - 1) * 2 + 1;
-                    }
-                }
-                auto run = [&](int xa, int xb, int y, int z) {
-                    s_base[n] = A.pcoff[l] + cmm_lin(mm, xa, y, z) * nf + kk;
-                    s_off[n] = off;
-                    s_kk[n] = kk;
-                    off += xb - xa + 1;
-                    ++n;
-                };
-                for (int jz = lo[2]; jz <= hi[2]; ++jz)
-                    for (int jy = lo[1]; jy <= hi[1]; ++jy) {
-                        if (abs(jz - iz) <= 1 && abs(jy - iy) <= 1) {
-                            if (lo[0] <= ix - 2) run(lo[0], ix - 2, jy, jz);
-                            if (ix + 2 <= hi[0]) run(ix + 2, hi[0], jy, jz);
-                        } else {
-                            run(lo[0], hi[0], jy, jz);
-                        }
-                    }
+    // the far list as slots ((kk, level, row jz, row jy), 2 x-runs each), one
+    // (kk, level, jz, jy) per thread; rows beyond the level's range are empty
+    const int nrow = nf * A.nlev * 36, nslot = 2 * nrow;
+    for (int q = t; q < nrow; q += CMM_NT) {
+        const int kk = q / (A.nlev * 36), l = (q / 36) % A.nlev, jzi = (q % 36) / 6, jyi = q % 6;
+        const int ix = (cell % nv1) >> l, iy = ((cell / nv1) % nv1) >> l, iz = (cell / (nv1 * nv1)) >> l;
+        const int mm = A.nv[l];
+        int lo[3], hi[3];
+        const int ii[3] = {ix, iy, iz};
+        for (int d = 0; d < 3; ++d) {
+            if (l == A.nlev - 1) {
+                lo[d] = 0; hi[d] = mm - 1;
+            } else {
+                const int m1 = A.nv[l + 1], pj = ii[d] >> 1;
+                lo[d] = max(pj - 1, 0) * 2;
+                hi[d] = min(pj + 1, m1 - 1) * 2 + 1;
             }
         }
-        s_off[n] = off;
-        s_n = n;
+        const int jz = lo[2] + jzi, jy = lo[1] + jyi;
+        int xa[2] = {0, 0}, xb[2] = {-1, -1};
+        if (jz <= hi[2] && jy <= hi[1]) {
+            if (abs(jz - iz) <= 1 && abs(jy - iy) <= 1) {
+                xa[0] = lo[0]; xb[0] = ix - 2;
+                xa[1] = ix + 2; xb[1] = hi[0];
+            } else {
+                xa[0] = lo[0]; xb[0] = hi[0];
+            }
+        }
+        for (int h = 0; h < 2; ++h) {
+            const bool on = xa[h] <= xb[h];
+            s_base[2 * q + h] = on ? A.pcoff[l] + cmm_lin(mm, xa[h], jy, jz) * nf + kk : 0;
+            s_off[2 * q + h] = on ? xb[h] - xa[h] + 1 : 0;
+            s_kk[2 * q + h] = kk;
+        }
     }
     __syncthreads();
-    const int J = s_off[s_n];
+    cmm_block_scan(s_off, nslot);
+    const int J = s_off[nslot];
     for (int i0 = cs; i0 < ce; i0 += CMM_NT) {
         const int nic = min(CMM_NT, ce - i0);
         const int R = CMM_NT / nic, r = t / nic, li = t - r * nic;
@@ -777,7 +883,7 @@ __global__ void __launch_bounds__(CMM_NT) k_cmm_far(CmmArgs A) {
             __syncthreads();
             for (int u = t; u < nq * 16; u += CMM_NT) {
                 const int slot = u >> 4, m = u & 15, g = q0 + slot;
-                int lo = 0, hi = s_n - 1;
+                int lo = 0, hi = nslot - 1;   // the last slot with s_off <= g holds g
                 while (lo < hi) {
                     const int mid = (lo + hi + 1) >> 1;
                     if (s_off[mid] <= g) lo = mid;

@@ -316,8 +316,10 @@ struct Engine {
     DBuf<double4> d_cmm_sx;
     DBuf<int> d_cmm_key, d_cmm_kcnt, d_cmm_kstart, d_cmm_kfill, d_cmm_tmp, d_cmm_sorig, d_cmm_spos, d_cmm_sflag,
         d_cmm_smol, d_cmm_npc, d_cmm_err, d_cmm_xwide, d_cmm_l2cnt, d_cmm_l2ptr, d_cmm_l2idx, d_cmm_vflag,
-        d_cmm_vpos, d_cmm_vlist;
+        d_cmm_vpos, d_cmm_vlist, d_cmm_kvcnt, d_cmm_kvstart, d_cmm_vf, d_cmm_va, d_cmm_vm, d_cmm_vp, d_cmm_tsum;
+    DBuf<double4> d_cmm_vx0;
     DBuf<unsigned long long> d_cmm_xmask;
+    int cmm_scan(int n, const int *in, int *out);
     int cmm_vcap = 0;                   // k_cmm_lj2's grid: atoms with a second-nearest table, only grows
     int set_cmm(int on, double cminsiz, int nlev, int cmmupd);
     int cmm_celprp();
@@ -395,6 +397,8 @@ struct Engine {
         d_cmm_spos.release(); d_cmm_sflag.release(); d_cmm_smol.release(); d_cmm_npc.release(); d_cmm_err.release();
         d_cmm_xwide.release(); d_cmm_l2cnt.release(); d_cmm_l2ptr.release(); d_cmm_l2idx.release();
         d_cmm_vflag.release(); d_cmm_vpos.release(); d_cmm_vlist.release(); d_cmm_xmask.release();
+        d_cmm_kvcnt.release(); d_cmm_kvstart.release(); d_cmm_vf.release(); d_cmm_va.release(); d_cmm_vm.release();
+        d_cmm_vp.release(); d_cmm_tsum.release(); d_cmm_vx0.release();
         if (h_rsum) hipHostFree(h_rsum);
         if (h_rfix) hipHostFree(h_rfix);
         if (h_err) hipHostFree(h_err);
@@ -2463,6 +2467,9 @@ int Engine::set_cmm(int on, double cminsiz, int nlev, int cmmupd) {
     AL(d_cmm_npc, npc); AL(d_cmm_err, 2);
     AL(d_cmm_xmask, n); AL(d_cmm_xwide, n); AL(d_cmm_l2cnt, n); AL(d_cmm_l2ptr, n + 1); AL(d_cmm_l2idx, 1);
     AL(d_cmm_vflag, n); AL(d_cmm_vpos, n + 1); AL(d_cmm_vlist, n);
+    AL(d_cmm_kvcnt, A.nkey); AL(d_cmm_kvstart, A.nkey + 1); AL(d_cmm_vx0, n); AL(d_cmm_vf, n); AL(d_cmm_va, n);
+    AL(d_cmm_vm, n); AL(d_cmm_vp, n);
+    AL(d_cmm_tsum, 2 * ((std::max((size_t)A.nkey, n) + 1023) / 1024 + 1));
     CK(hipMemcpyAsync(d_cmm_cg.p, cg.data(), 8 * cg.size(), hipMemcpyHostToDevice, st));
     CK(hipMemcpyAsync(d_cmm_q.p, charge_h.data(), 8 * n, hipMemcpyHostToDevice, st));
     CK(hipMemsetAsync(d_cmm_gwk.p, 0, 72 * n, st));
@@ -2481,6 +2488,8 @@ int Engine::set_cmm(int on, double cminsiz, int nlev, int cmmupd) {
     A.gwk = d_cmm_gwk.p; A.efar = d_cmm_efar.p; A.gslot = d_gslot.p; A.eatom = d_cmm_eatom.p;
     A.xmask = d_cmm_xmask.p; A.xwide = d_cmm_xwide.p; A.l2cnt = d_cmm_l2cnt.p; A.l2ptr = d_cmm_l2ptr.p;
     A.l2idx = d_cmm_l2idx.p; A.vflag = d_cmm_vflag.p; A.vpos = d_cmm_vpos.p; A.vlist = d_cmm_vlist.p;
+    A.kvcnt = d_cmm_kvcnt.p; A.kvstart = d_cmm_kvstart.p; A.vx0 = d_cmm_vx0.p; A.vf = d_cmm_vf.p;
+    A.va = d_cmm_va.p; A.vm = d_cmm_vm.p; A.vp = d_cmm_vp.p;
     cm = A;
     hipLaunchKernelGGL(k_cmm_xmask, grid1(n, 256), 256, 0, st, A);
     CK(hipGetLastError());
@@ -2507,10 +2516,13 @@ int Engine::cmm_celprp() {
     hipLaunchKernelGGL(k_cmm_mean, 1, 1024, 0, st, A);
     hipLaunchKernelGGL(k_cmm_reset, grid1(std::max(A.nkey, 2), 256), 256, 0, st, A);
     hipLaunchKernelGGL(k_cmm_bin, gn, 256, 0, st, A);
-    hipLaunchKernelGGL(k_scan, 1, 1024, 0, st, A.nkey, (const int *)A.kcnt, A.kstart);
+    cmm_scan(A.nkey, A.kcnt, A.kstart);
     hipLaunchKernelGGL(k_cmm_place, gn, 256, 0, st, A);
     hipLaunchKernelGGL(k_cmm_rank, gn, 256, 0, st, A);
     hipLaunchKernelGGL(k_cmm_fill, gn, 256, 0, st, A);
+    hipLaunchKernelGGL(k_cmm_kv, grid1(A.nkey, 256), 256, 0, st, A);
+    cmm_scan(A.nkey, A.kvcnt, A.kvstart);
+    hipLaunchKernelGGL(k_cmm_vcopy, gn, 256, 0, st, A);
     for (int l = 0; l < A.nlev; ++l) {
         const int nq = A.nv[l] * A.nv[l] * A.nv[l] * A.nf;
         hipLaunchKernelGGL(k_cmm_npc, grid1(nq, 256), 256, 0, st, A, l);
@@ -2522,9 +2534,9 @@ int Engine::cmm_celprp() {
     hipLaunchKernelGGL(k_cmm_check, gn, 256, 0, st, A);
     if (A.idelay > 1) hipLaunchKernelGGL(k_cmm_l2<false>, A.nc1, 256, 0, st, A);
     // the second-nearest LJ tables: lengths -> offsets, the atoms that have one
-    hipLaunchKernelGGL(k_scan, 1, 1024, 0, st, natom, (const int *)A.l2cnt, A.l2ptr);
+    cmm_scan(natom, A.l2cnt, A.l2ptr);
     hipLaunchKernelGGL(k_cmm_vflag, gn, 256, 0, st, A);
-    hipLaunchKernelGGL(k_scan, 1, 1024, 0, st, natom, (const int *)A.vflag, A.vpos);
+    cmm_scan(natom, A.vflag, A.vpos);
     CK(hipGetLastError());
     int st4[4] = {0, 0, 0, 0};
     CK(hipMemcpyAsync(st4, d_cmm_err.p, 8, hipMemcpyDeviceToHost, st));
@@ -2545,6 +2557,17 @@ int Engine::cmm_celprp() {
     }
     if (A.idelay > 1) hipLaunchKernelGGL(k_cmm_l2<true>, A.nc1, 256, 0, st, cm);
     hipLaunchKernelGGL(k_cmm_vfill, gn, 256, 0, st, cm);
+    CK(hipGetLastError());
+    return 0;
+}
+
+// Exclusive scan of n ints (out[n] the total) in three launches.
+int Engine::cmm_scan(int n, const int *in, int *out) {
+    const int nt = (n + 1023) / 1024;
+    int *t0 = d_cmm_tsum.p, *t1 = d_cmm_tsum.p + (d_cmm_tsum.n / 2);
+    hipLaunchKernelGGL(k_scan_tile, std::max(nt, 1), 256, 0, st, n, in, out, t0);
+    hipLaunchKernelGGL(k_scan, 1, 1024, 0, st, nt, (const int *)t0, t1);
+    hipLaunchKernelGGL(k_scan_add, grid1((size_t)n + 1, 256), 256, 0, st, n, out, (const int *)t1, nt);
     CK(hipGetLastError());
     return 0;
 }
