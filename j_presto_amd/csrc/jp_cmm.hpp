@@ -506,7 +506,8 @@ __global__ void __launch_bounds__(CMM_NT) k_cmm_near(CmmArgs A) {
     __shared__ int s_run[9], s_off[10];
     extern __shared__ double2 s_lj[];
     double *jx = s_buf, *jy = jx + CMM_JCH, *jz = jy + CMM_JCH, *jq = jz + CMM_JCH;
-    int *jf = (int *)(jq + CMM_JCH), *ja = jf + CMM_JCH, *jm = ja + CMM_JCH;
+    int2 *jfa = (int2 *)(jq + CMM_JCH);   // (flags, atom number)
+    int *jm = (int *)(jfa + CMM_JCH);
     const int cell = blockIdx.x, nf = A.nf, nv = A.nv1, t = threadIdx.x;
     const int cs = A.kstart[cell * nf * 2], ce = A.kstart[(cell + 1) * nf * 2];
     if (ce == cs) return;
@@ -560,16 +561,18 @@ __global__ void __launch_bounds__(CMM_NT) k_cmm_near(CmmArgs A) {
                 const int pj = s_run[k] + (g - s_off[k]);
                 const double4 v = A.sx[pj];
                 jx[s] = v.x; jy[s] = v.y; jz[s] = v.z; jq[s] = v.w;
-                jf[s] = A.sflag[pj];
-                ja[s] = A.sorig[pj];
+                jfa[s] = make_int2(A.sflag[pj], A.sorig[pj]);
                 jm[s] = A.smol[pj];
             }
             __syncthreads();
             if (act) {
                 for (int s = r; s < nq; s += R) {
-                    // excluded pairs (and i itself) lie in i's molecule
-                    if (jm[s] == moli && cmm_excl(A, ai, ja[s], mask, wide, moli, moli)) continue;
-                    const int fj = jf[s];
+                    const int2 fa = jfa[s];
+                    // excluded pairs and i itself: the window mask, else the list
+                    const unsigned dw = (unsigned)(fa.y - ai + 32);
+                    if (dw < 64u && ((mask >> dw) & 1ull)) continue;
+                    if (wide && jm[s] == moli && cmm_excluded(A, ai, fa.y)) continue;
+                    const int fj = fa.x;
                     const double dx = xi.x - jx[s], dy = xi.y - jy[s], dz = xi.z - jz[s];
                     const double ri = cmm_rsqrt(dx * dx + dy * dy + dz * dz);
                     const double d2i = ri * ri;
