@@ -14,7 +14,10 @@
 !                 METHOD (MICR, CANO or GE; required), SETTEM, TEMPCO, OUTLOG,
 !                 RESTAR, NAMERI, NAMERO, NAMETO, NAMEGE, LAMBDA, LMDSQU, CUTLEN,
 !                 DIEVAL, XMINCL..ZMAXCL, COVSHK, LIMSHK (inpmd.f90; defaults of
-!                 main.f90:150-185).  METHOD= GE reads the NAMEGE= parameter
+!                 main.f90:150-185).  CAL15M= ZD runs under BOUNDA= PBC;
+!                 CAL15M= CMM (md_run's default) under BOUNDA= NO (its default)
+!                 with EXE> INPUT's CELSIZ, CELLVL and EXE> MD's CMMUPD
+!                 (jp_set_cmm).  METHOD= GE reads the NAMEGE= parameter
 !                 file (inmcmd.f90): multicanonical MD (jp_set_mcmd), or under
 !                 GEPSME= LMD adaptive lambda square dynamics (jp_set_alsd, with
 !                 EXE> INPUT's LWEIGH and LTCNTL).
@@ -68,6 +71,10 @@ program jp_host
   integer(c_int32_t) :: limshk, cellvl
   real(c_double) :: cputim
   logical :: initzero
+  ! CAL15M= CMM (md_run's default) with BOUNDA= NO: CELSIZ, CELLVL, CMMUPD
+  logical :: cmm
+  real(c_double) :: celsiz
+  integer(c_int32_t) :: cmmupd
   integer(8) :: tstart, trate, tnow          ! wall clock since the program started (md_run's fxcpus)
   integer(c_int32_t), allocatable, target :: grp(:)
   type(top_t) :: top
@@ -187,6 +194,14 @@ program jp_host
         if (fromtpl .and. desc%lambda == 0.0d0) desc%lambda = rlam
         if (gemode) lamv0 = rlamv
      end if
+     if (cmm) then
+        ! CMM has no box: the context's walls only enclose the atoms (their
+        ! cells follow the coordinates, celpar_CB)
+        do k = 1, 3
+           desc%celwal(2 * k - 1) = minval(cord(k, :)) - cutlen - 1.0d0
+           desc%celwal(2 * k) = max(maxval(cord(k, :)) + cutlen + 1.0d0, desc%celwal(2 * k - 1) + 2.0d0 * cutlen + 2.0d0)
+        end do
+     end if
      if (tplsys) then
         call write_system_file()
         stop
@@ -201,6 +216,13 @@ program jp_host
      stop 1
   end if
   ier(1) = jp_set_state(ctx, c_null_ptr, c_loc(vel))
+  if (mdmode .and. cmm) then
+     ier(1) = jp_set_cmm(ctx, 1_c_int32_t, celsiz, cellvl, cmmupd)   ! incel's CMM set-up
+     if (ier(1) /= JP_OK) then
+        write(*,'(a,i0)') 'jp_set_cmm: ier = ', ier(1)
+        stop 1
+     end if
+  end if
   ier(2) = jp_build_list(ctx, c_null_ptr)                 ! celprp
   if (mdmode) then
      call run_mdinp()
@@ -302,7 +324,7 @@ contains
     character(len=*), intent(in) :: path
     character(len=256) :: line, rest, key, val
     integer :: uu, ios, p, q
-    logical :: inmd, inin, have_method, have_zd, have_pbc
+    logical :: inmd, inin, have_method, have_pbc
     integer :: iv
     real(c_double) :: rv
     nsteps = 0; update = 10; nmon = 0; timest = 2.0d0; settem = -300.0d0   ! negative: METHOD= MICR
@@ -310,8 +332,9 @@ contains
     tplf = ' '; crdf = ' '; shkf = ' '; crdfmt = 'PDB'; cla = 0; clb = 0; lmdin = .false.
     cutlen = 12.0d0; dieval = 1.0d0; box = 0.0d0; covshk = 1.0d-6; limshk = 1000
     cellvl = 3; cputim = 99999999.0d0; initzero = .false.
+    cmm = .true.; celsiz = 6.0d0; cmmupd = 5   ! main.f90:170, inpinp.f90:86, inpmd.f90:151
     gemode = .false.; gef = ' '; lweigh = 0.0d0; ltcntl = 1; lamkw = 0.0d0; lsqkw = 0.0d0
-    have_method = .false.; have_zd = .false.; have_pbc = .false.
+    have_method = .false.; have_pbc = .false.
     inmd = .false.; inin = .false.
     open(newunit=uu, file=path, status='old', action='read')
     do
@@ -356,6 +379,9 @@ contains
              case ('CELLVL')
                 read(val, *, iostat=ios) cellvl
                 if (ios == 0 .and. (cellvl < 1 .or. cellvl > 6)) call refuse(key, val, 'cell levels 1-6 only')
+             case ('CELSIZ')   ! the level-1 cell of CMM (ZD sets its own, incel.f90:29-34)
+                read(val, *, iostat=ios) celsiz
+                if (ios == 0 .and. .not. (celsiz > 0.0d0)) call refuse(key, val, 'a positive cell size')
              case ('SCALET')
                 if (trim(val) /= '0030') call refuse(key, val, 'only the default class choice 0030 is implemented')
              case ('LWEIGH')   ! lambda's mass (0: the class-1 atoms' mass), ALSD only
@@ -370,7 +396,7 @@ contains
                 end if
              case ('REFCOO', 'OUTMON')
                 write(*,'(3a)') 'jp_host: ', trim(key), ' (monitoring only) is not written by this host'
-             case ('SETVAR', 'SETBOU', 'POSITI', 'DISTAN', 'DIHEDR', 'SETORI', 'EXTCMM', 'REPULS', 'CELSIZ', &
+             case ('SETVAR', 'SETBOU', 'POSITI', 'DISTAN', 'DIHEDR', 'SETORI', 'EXTCMM', 'REPULS', &
                    'CELRES', 'CLUSTB')
                 call refuse(key, val, 'not implemented by this host')
              case default
@@ -392,11 +418,18 @@ contains
           ! the run differ from md_run's
           select case (trim(key))
           case ('CAL15M')
-             if (trim(val) /= 'ZD') call refuse(key, val, 'only zero-dipole (ZD) electrostatics is implemented')
-             have_zd = .true.
+             if (trim(val) == 'ZD') then
+                cmm = .false.
+             else if (trim(val) /= 'CMM') then
+                call refuse(key, val, 'only zero-dipole (ZD) and the cell multipole method (CMM) are implemented')
+             end if
           case ('BOUNDA')
-             if (trim(val) /= 'PBC') call refuse(key, val, 'only periodic boundaries (PBC) are implemented')
-             have_pbc = .true.
+             if (trim(val) == 'PBC') then
+                have_pbc = .true.
+             else if (trim(val) /= 'NO') then
+                call refuse(key, val, 'only periodic boundaries (PBC, with ZD) and none (NO, with CMM) are implemented')
+             end if
+          case ('CMMUPD'); read(val, *, iostat=ios) cmmupd
           case ('STOPCE')
              if (trim(val) /= 'NO') call refuse(key, val, 'centre-of-mass motion removal is not implemented')
           case ('HEATLO')
@@ -423,12 +456,13 @@ contains
                 call refuse(key, val, 'NAMETO is written as ASCII only')
              end if
           case ('PARALV', 'N_SIMD', 'LOGFOR', 'RELAXA', 'TEMPER', 'NSTPCN', 'OUTATM', 'DUMMYL', 'BESTFI', &
-                'RANDOM', 'STARTT', 'CMMUPD', 'RADCAP', 'FORCAP', 'FUNCAP', 'WETPSR', 'WETDSR', 'WETDHR', &
+                'RANDOM', 'STARTT', 'RADCAP', 'FORCAP', 'FUNCAP', 'WETPSR', 'WETDSR', 'WETDHR', &
                 'CAPSHP', 'FOACAP', 'RADPRO', 'HTPSLO', 'STAPST', 'ELLIPA', 'ELLIPB', 'ELLIPC', 'ELLBUF', &
                 'CELBUF', 'RADIUS')
              ! thread layout and SIMD width, log format, methods and boundaries
-             ! this run does not use (CONS relaxation, random initial
-             ! velocities, CMM, CAP, ellipsoid): no effect on a ZD/PBC run
+             ! this host does not use (CONS relaxation, random initial
+             ! velocities, the CAP and ellipsoid parameters, which act only
+             ! through SETBOU=, refused above): no effect on the run
           case ('NAMECO', 'NAMEVE', 'NAMEEN', 'NAMETR', 'NAMECL', 'OUTCOO', 'OUTVEL', 'OUTENE', &
                 'OUTTRJ', 'OUTCLE', 'MNTRCO', 'MNTRVE', 'MNTREN', 'MNTRTR', 'MNTRCL', 'FDEBUG')
              write(*,'(3a)') 'jp_host: ', trim(key), ' (trajectory/monitoring output) is not written by this host'
@@ -482,13 +516,18 @@ contains
        end if
        call read_namege(trim(gef))
     end if
-    ! md_run's defaults are CMM electrostatics and no boundary (main.f90:149,170)
-    if (.not. have_zd) then
-       write(*,'(a)') 'jp_host: CAL15M= ZD must be given (md_run''s default is CMM, which this host does not run)'
+    ! md_run's defaults are CMM electrostatics and no boundary (main.f90:149,170);
+    ! ZD runs here under periodic boundaries, CMM without
+    if (.not. cmm .and. .not. have_pbc) then
+       write(*,'(a)') 'jp_host: CAL15M= ZD needs BOUNDA= PBC (md_run''s default is no boundary)'
        stop 3
     end if
-    if (.not. have_pbc) then
-       write(*,'(a)') 'jp_host: BOUNDA= PBC must be given (md_run''s default is no boundary)'
+    if (cmm .and. have_pbc) then
+       write(*,'(a)') 'jp_host: CAL15M= CMM is implemented for BOUNDA= NO only'
+       stop 3
+    end if
+    if (cmm .and. (cmmupd < 1 .or. mod(update, cmmupd) /= 0)) then
+       write(*,'(a)') 'jp_host: ERROR> INPMD: MOD(UPDATE,CMMUPD) should be 0'   ! inpmd.f90:284-287
        stop 3
     end if
     if (.not. restar .and. .not. initzero .and. len_trim(fin) == 0) then
@@ -713,7 +752,7 @@ contains
        write(*,'(a)') 'jp_host: GEPSME= LMD and CLUSTA= a<b go together (other cluster methods not supported)'
        stop 3
     end if
-    if (box(2) <= box(1) .or. box(4) <= box(3) .or. box(6) <= box(5)) then
+    if (.not. cmm .and. (box(2) <= box(1) .or. box(4) <= box(3) .or. box(6) <= box(5))) then
        write(*,'(a)') 'jp_host: EXE> MD must give the periodic box XMINCL..ZMAXCL'
        stop 3
     end if

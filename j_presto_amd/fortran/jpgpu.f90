@@ -197,6 +197,13 @@ module jpgpu
        real(c_double), value :: lower, upper, celw, ceup, tempce, lambda_m
      end function jp_set_alsd
 
+     integer(c_int) function jp_set_cmm(ctx, on, cminsiz, nlev, cmmupd) bind(C, name='jp_set_cmm')
+       import :: c_int, c_int32_t, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: on, nlev, cmmupd
+       real(c_double), value :: cminsiz
+     end function jp_set_cmm
+
      integer(c_int) function jp_get_lambda(ctx, lambda, lambda_v) bind(C, name='jp_get_lambda')
        import :: c_int, c_ptr, c_double
        type(c_ptr), value :: ctx

@@ -335,11 +335,16 @@ def test_mdinp_refuses_what_it_does_not_run(tmp_path):
         return text.replace("EXE> MD\n", f"EXE> MD\n      {key}= {val}\n")
 
     cases = {
-        "CAL15M= CMM": inp.replace("CAL15M= ZD", "CAL15M= CMM"),
-        "CAL15M= ZD must be given": "\n".join(l for l in inp.splitlines() if "CAL15M" not in l),
-        "BOUNDA= NO": inp.replace("BOUNDA= PBC", "BOUNDA= NO"),
+        "CAL15M= CMM is implemented for BOUNDA= NO only": inp.replace("CAL15M= ZD", "CAL15M= CMM"),
+        # md_run's default is CMM: without CAL15M the periodic box is refused
+        "CMM is implemented for BOUNDA= NO only": "\n".join(l for l in inp.splitlines() if "CAL15M" not in l),
+        "CAL15M= RESI": inp.replace("CAL15M= ZD", "CAL15M= RESI"),
+        "CAL15M= ZD needs BOUNDA= PBC": inp.replace("BOUNDA= PBC", "BOUNDA= NO"),
         "BOUNDA= ELLI": inp.replace("BOUNDA= PBC", "BOUNDA= ELLI"),
-        "BOUNDA= PBC must be given": "\n".join(l for l in inp.splitlines() if "BOUNDA" not in l),
+        "ZD needs BOUNDA= PBC": "\n".join(l for l in inp.splitlines() if "BOUNDA" not in l),
+        "MOD(UPDATE,CMMUPD) should be 0": md_line(inp.replace("CAL15M= ZD", "CAL15M= CMM")
+                                                  .replace("BOUNDA= PBC", "BOUNDA= NO"), "CMMUPD", "3"),
+        "CELSIZ= -1.0": inp.replace("QUIT\n\nEXE> MD", "      CELSIZ= -1.0\nQUIT\n\nEXE> MD"),
         "STOPCE= YES": inp.replace("STOPCE= NO", "STOPCE= BOTH").replace("BOTH", "YES"),
         "STOPCE= TRAN": inp.replace("STOPCE= NO", "STOPCE= TRAN"),
         "HEATLO= 1000": md_line(inp, "HEATLO", "1000"),
@@ -362,7 +367,7 @@ def test_mdinp_refuses_what_it_does_not_run(tmp_path):
         r = subprocess.run([_host(), "--tplsys", "bad.inp", "got.bin"], capture_output=True, text=True,
                            timeout=60, cwd=d)
         assert r.returncode == 3 and what in r.stdout, (what, r.stdout)
-    # keywords with no effect on a ZD/PBC run are accepted; output-only ones are named
+    # keywords with no effect on the run are accepted; output-only ones are named
     ok = md_line(md_line(md_line(inp, "N_SIMD", "64"), "RADCAP", "20.0"), "OUTCOO", "10")
     ok = md_line(md_line(ok, "CPUTIM", "3600.0"), "INITIA", "SET")
     with open(os.path.join(d, "ok.inp"), "w") as f:
@@ -483,3 +488,60 @@ def test_mdinp_alsd_like_md_run(golden, tmp_path, tag):
     assert ene.shape == ref.shape
     # E15.7: one unit of the last printed digit is at most 1e-6 relative
     assert np.allclose(ene, ref, rtol=1.01e-6, atol=0)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,method", [("droplet_cmm", "MICR"), ("chain_droplet_cmm", "MICR"),
+                                         ("droplet_cmm", "CANO"), ("chain_droplet_lmd", "CANO")])
+def test_mdinp_cmm_like_md_run(golden, tmp_path, name, method):
+    """CAL15M= CMM with BOUNDA= NO (md_run's defaults; CELSIZ 6, CELLVL 3,
+    CMMUPD 5) from md_run's own inputs: the restart the host writes holds
+    md_run's x and v within 1e-9."""
+    s, f = golden(name)
+    print(f"case host CMM {name} {method}", flush=True)
+    d = str(tmp_path)
+    if method == "MICR":
+        md = {"METHOD": "MICR", "LOOPLI": str(int(f["traj_nsteps"]))}
+        want_x, want_v = f["traj_x"], f["traj_v"]
+    else:
+        md = {"METHOD": "CANO", "LOOPLI": str(int(f["cano_nsteps"])), "SETTEM": "300.0", "TEMPCO": "NO"}
+        want_x, want_v = f["cano_x"], f["cano_v"]
+    md.update({"CAL15M": "CMM", "BOUNDA": "NO"})
+    _tpl_case(d, s, md)
+    r = subprocess.run([_host(), "--mdinp", "md.inp"], capture_output=True, text=True, timeout=240, cwd=d)
+    print(r.stdout, flush=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    o = refio.read_restart(os.path.join(d, "out.rst"))
+    assert o["n"] == s.n and o["loop"] == int(md["LOOPLI"])
+    assert np.abs(o["x"] - want_x).max() < 1e-9
+    assert np.abs(o["v"] - want_v).max() < 1e-9
+
+
+@pytest.mark.gpu
+def test_mdinp_cmm_alsd_like_md_run(golden, tmp_path):
+    """ALSD under CMM from md_run's inputs, with CAL15M and BOUNDA left at
+    md_run's defaults: x, v within 1e-9, lambda and its velocity within
+    1e-11 of md_run's restart."""
+    from make_golden import ALSD_RUNS, write_alsd_params
+    s, f = golden("chain_droplet_lmd")
+    print("case host CMM ALSD", flush=True)
+    d = str(tmp_path)
+    celw, ceup, tempco, ltc, lw = ALSD_RUNS["a"]
+    write_alsd_params(os.path.join(d, "ge.par"), celw, ceup)
+    inp = {"LTCNTL": ltc}
+    if lw is not None:
+        inp["LWEIGH"] = repr(float(lw))
+    _tpl_case(d, s, {"METHOD": "GE", "LOOPLI": str(int(f["alsd_a_loop"])), "SETTEM": "300.0", "TEMPCO": tempco,
+                     "NAMEGE": "ge.par"}, inp=inp)
+    text = open(os.path.join(d, "md.inp")).read()
+    text = "\n".join(l for l in text.splitlines() if "CAL15M" not in l and "BOUNDA" not in l)
+    with open(os.path.join(d, "md.inp"), "w") as fo:
+        fo.write(text)
+    r = subprocess.run([_host(), "--mdinp", "md.inp"], capture_output=True, text=True, timeout=240, cwd=d)
+    print(r.stdout, flush=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    o = refio.read_restart(os.path.join(d, "out.rst"))
+    assert np.abs(o["x"] - f["alsd_a_x"]).max() < 1e-9
+    assert np.abs(o["v"] - f["alsd_a_v"]).max() < 1e-9
+    assert abs(o["lam"] - float(f["alsd_a_lam"])) < 1e-11
+    assert abs(o["lam_v"] - float(f["alsd_a_lamv"])) < 1e-11
