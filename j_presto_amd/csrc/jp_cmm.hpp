@@ -65,6 +65,7 @@ struct CmmArgs {
     double *gnb;                // [p][3 classes][3] near-field sums of the sorted atom p
     double *enb;                // [p][6] near-field elec, vdW per class
     double *gwk;                // [a][3 classes][3] far-field gradient (kept between far-field steps)
+    double *gwkp;               // [part][a][12] k_cmm_far's partial sums (x, y, z, energy per class)
     double *efar;               // [a][3] far-field energy per class (kept likewise)
     double *gslot;              // [a][3] combined gradient
     double *eatom;              // [a][6] elec (near + far), vdW per class; summed and halved by the host
@@ -812,36 +813,66 @@ __global__ void __launch_bounds__(256) k_cmm_lj2(CmmArgs A) {
 
 #define CMM_FCH 128      // far-cell records staged per pass of k_cmm_far
 #define CMM_MAXFRUN 1024
+#define CMM_FP 4         // blocks of k_cmm_far per group, each over a quarter of the records
 
 // Far field (compute_particle_multipole, cmmdo.f90:164-288): for every atom,
 // per source fragment and level, the cells of that level in the far list of
 // its ancestor (mk_cm: the children of the ancestor's parent's neighbours
 // less the ancestor's own neighbours; all cells less the neighbours at the
-// top level) that hold atoms of the fragment.  One block per level-1 cell:
-// the far list as runs of x-adjacent records, staged through LDS in chunks;
-// thread t takes atom t % nic and, as replica t / nic of R = 256 / nic,
-// every R-th record; replicas are summed in replica order and the gradient
-// is scaled by chgmod2 = ccoef q.
+// top level) that hold atoms of the fragment.
+//
+// Work is grouped by parent: the 8 level-1 cells of one level-2 cell (one
+// level-1 cell when there is a single level) share the far lists of levels
+// 2 and up, and their level-1 lists are the children of the parent's
+// neighbours, each less its own neighbours (a test per atom).  Block
+// (group, part) stages the part-th quarter of the group's records through
+// LDS; thread t takes atom t % nic of the group and, as replica t / nic of
+// R = 256 / nic, every R-th record; replicas are summed in replica order into
+// the part's partial sums, which k_cmm_farsum adds in part order.
 template <bool MULTI>
 __global__ void __launch_bounds__(CMM_NT) k_cmm_far(CmmArgs A) {
     constexpr int NC = MULTI ? 3 : 1, NV = 4 * NC;
     __shared__ double s_rec[CMM_FCH * 16];
-    __shared__ int s_ok[CMM_FCH];
-    __shared__ int s_base[CMM_MAXFRUN], s_off[CMM_MAXFRUN + 1], s_kk[CMM_MAXFRUN];
+    __shared__ int s_meta[CMM_FCH];
+    __shared__ int s_base[CMM_MAXFRUN], s_off[CMM_MAXFRUN + 1], s_inf[CMM_MAXFRUN];
+    __shared__ int s_irun[4], s_ioff[5];
     __shared__ double s_red[NV * CMM_NT];
-    const int cell = blockIdx.x, nf = A.nf, nv1 = A.nv1, t = threadIdx.x;
-    const int cs = A.kstart[cell * nf * 2], ce = A.kstart[(cell + 1) * nf * 2];
-    if (ce == cs) return;
+    const int nf = A.nf, nv1 = A.nv1, t = threadIdx.x;
+    const int g2 = A.nlev > 1 ? 2 : 1, ng = nv1 / g2;
+    const int grp = blockIdx.x / CMM_FP, part = blockIdx.x % CMM_FP;
+    const int gx = grp % ng, gy = (grp / ng) % ng, gz = grp / (ng * ng);
+    // the group's atoms: g2 x g2 rows of g2 x-adjacent level-1 cells
+    if (t < g2 * g2) {
+        const int z = g2 * gz + t / g2, y = g2 * gy + t % g2;
+        const int b = A.kstart[cmm_lin(nv1, g2 * gx, y, z) * nf * 2];
+        s_irun[t] = b;
+        s_ioff[t] = A.kstart[(cmm_lin(nv1, g2 * gx + g2 - 1, y, z) + 1) * nf * 2] - b;
+    }
+    __syncthreads();
+    if (t == 0) {
+        int off = 0;
+        for (int k = 0; k < g2 * g2; ++k) {
+            const int len = s_ioff[k];
+            s_ioff[k] = off;
+            off += len;
+        }
+        s_ioff[g2 * g2] = off;
+    }
+    __syncthreads();
+    const int NI = s_ioff[g2 * g2];
+    if (NI == 0) return;
     const SysP &P = A.sp[0];
-    // the far list as slots ((kk, level, row jz, row jy), 2 x-runs each), one
-    // (kk, level, jz, jy) per thread; rows beyond the level's range are empty
+    // record slots ((kk, level, row jz, row jy), 2 x-runs each); level 0
+    // takes the whole candidate row, the atoms drop their own neighbours
     const int nrow = nf * A.nlev * 36, nslot = 2 * nrow;
     for (int q = t; q < nrow; q += CMM_NT) {
         const int kk = q / (A.nlev * 36), l = (q / 36) % A.nlev, jzi = (q % 36) / 6, jyi = q % 6;
-        const int ix = (cell % nv1) >> l, iy = ((cell / nv1) % nv1) >> l, iz = (cell / (nv1 * nv1)) >> l;
+        // level 0: the candidates of the group's cells; level l >= 1: the
+        // far list of their common ancestor
+        const int ax = (g2 * gx) >> l, ay = (g2 * gy) >> l, az = (g2 * gz) >> l;
         const int mm = A.nv[l];
         int lo[3], hi[3];
-        const int ii[3] = {ix, iy, iz};
+        const int ii[3] = {ax, ay, az};
         for (int d = 0; d < 3; ++d) {
             if (l == A.nlev - 1) {
                 lo[d] = 0; hi[d] = mm - 1;
@@ -854,9 +885,9 @@ __global__ void __launch_bounds__(CMM_NT) k_cmm_far(CmmArgs A) {
         const int jz = lo[2] + jzi, jy = lo[1] + jyi;
         int xa[2] = {0, 0}, xb[2] = {-1, -1};
         if (jz <= hi[2] && jy <= hi[1]) {
-            if (abs(jz - iz) <= 1 && abs(jy - iy) <= 1) {
-                xa[0] = lo[0]; xb[0] = ix - 2;
-                xa[1] = ix + 2; xb[1] = hi[0];
+            if (l > 0 && abs(jz - az) <= 1 && abs(jy - ay) <= 1) {
+                xa[0] = lo[0]; xb[0] = ax - 2;
+                xa[1] = ax + 2; xb[1] = hi[0];
             } else {
                 xa[0] = lo[0]; xb[0] = hi[0];
             }
@@ -865,24 +896,34 @@ __global__ void __launch_bounds__(CMM_NT) k_cmm_far(CmmArgs A) {
             const bool on = xa[h] <= xb[h];
             s_base[2 * q + h] = on ? A.pcoff[l] + cmm_lin(mm, xa[h], jy, jz) * nf + kk : 0;
             s_off[2 * q + h] = on ? xb[h] - xa[h] + 1 : 0;
-            s_kk[2 * q + h] = kk;
+            // kk, level 0 or not, first x, y, z of the run (7 bits each)
+            s_inf[2 * q + h] = kk | (l == 0 ? 2 : 0) | (xa[h] << 2) | (jy << 9) | (jz << 16);
         }
     }
     __syncthreads();
     cmm_block_scan(s_off, nslot);
     const int J = s_off[nslot];
-    for (int i0 = cs; i0 < ce; i0 += CMM_NT) {
-        const int nic = min(CMM_NT, ce - i0);
+    const int pb = (int)((long long)J * part / CMM_FP), pe = (int)((long long)J * (part + 1) / CMM_FP);
+    for (int i0 = 0; i0 < NI; i0 += CMM_NT) {
+        const int nic = min(CMM_NT, NI - i0);
         const int R = CMM_NT / nic, r = t / nic, li = t - r * nic;
         const bool act = r < R;
-        const int p = i0 + li;
+        int p = 0;
+        if (act) {
+            const int g = i0 + li;
+            int k = 0;
+            while (g >= s_ioff[k + 1]) ++k;
+            p = s_irun[k] + (g - s_ioff[k]);
+        }
         const double4 xi = act ? A.sx[p] : make_double4(0.0, 0.0, 0.0, 0.0);
         const int k = act ? (A.sflag[p] >> 13) & 1 : 0;
+        const int ca = act ? A.key[A.sorig[p]] / (2 * nf) : 0;
+        const int cx = ca % nv1, cy = (ca / nv1) % nv1, cz = ca / (nv1 * nv1);
         double G[NV];
 #pragma unroll
         for (int m = 0; m < NV; ++m) G[m] = 0.0;
-        for (int q0 = 0; q0 < J; q0 += CMM_FCH) {
-            const int nq = min(CMM_FCH, J - q0);
+        for (int q0 = pb; q0 < pe; q0 += CMM_FCH) {
+            const int nq = min(CMM_FCH, pe - q0);
             __syncthreads();
             for (int u = t; u < nq * 16; u += CMM_NT) {
                 const int slot = u >> 4, m = u & 15, g = q0 + slot;
@@ -892,15 +933,20 @@ __global__ void __launch_bounds__(CMM_NT) k_cmm_far(CmmArgs A) {
                     if (s_off[mid] <= g) lo = mid;
                     else hi = mid - 1;
                 }
-                const int rec = s_base[lo] + (g - s_off[lo]) * nf;
+                const int dx = g - s_off[lo];
+                const int rec = s_base[lo] + dx * nf;
                 s_rec[u] = A.pot[(size_t)rec * 16 + m];
-                if (m == 0) s_ok[slot] = (A.npc[rec] != 0) ? 1 + s_kk[lo] : 0;
+                if (m == 0) s_meta[slot] = (A.npc[rec] != 0) ? 1 | (s_inf[lo] + (dx << 2)) << 1 : 0;
             }
             __syncthreads();
             if (act) {
                 for (int u = r; u < nq; u += R) {
-                    const int ok = s_ok[u];
-                    if (!ok) continue;
+                    const int meta = s_meta[u];
+                    if (!meta) continue;
+                    if ((meta >> 2) & 1) {   // level 1: not the atom's own neighbours
+                        const int jx = (meta >> 3) & 127, jy = (meta >> 10) & 127, jz = (meta >> 17) & 127;
+                        if (abs(jx - cx) <= 1 && abs(jy - cy) <= 1 && abs(jz - cz) <= 1) continue;
+                    }
                     const double *c = s_rec + u * 16, *q = c + 3;
                     const double dx = xi.x - c[0], dy = xi.y - c[1], dz = xi.z - c[2];
                     const double dx2 = dx * dx, dy2 = dy * dy, dz2 = dz * dz;
@@ -916,7 +962,7 @@ __global__ void __launch_bounds__(CMM_NT) k_cmm_far(CmmArgs A) {
                     const double cc2y = q[10] * dx + q[8] * dy + q[11] * dz;
                     const double cc2z = q[12] * dx + q[11] * dy + q[9] * dz;
                     const double cc4 = aa1 * 3.0 + aa2 * d2i * 5.0;
-                    const int ncls = MULTI ? cmm_cls(P, ok - 1, k) : 0;
+                    const int ncls = MULTI ? cmm_cls(P, (meta >> 1) & 1, k) : 0;
 #pragma unroll
                     for (int m = 0; m < NC; ++m) {
                         const bool on = !MULTI || m == ncls;
@@ -939,13 +985,27 @@ __global__ void __launch_bounds__(CMM_NT) k_cmm_far(CmmArgs A) {
             for (int rr = 0; rr < R; ++rr)
 #pragma unroll
                 for (int m = 0; m < NV; ++m) tot[m] = tot[m] + s_red[m * CMM_NT + rr * nic + li];
-            const int a = A.sorig[p];
-            const double qa = A.q[a], q2 = A.ccoef * qa;
-            for (int m = 0; m < 3; ++m) {
-                for (int d = 0; d < 3; ++d) A.gwk[((size_t)a * 3 + m) * 3 + d] = m < NC ? tot[4 * m + d] * q2 : 0.0;
-                A.efar[(size_t)a * 3 + m] = m < NC ? qa * A.ccoef * tot[4 * m + 3] : 0.0;
-            }
+            double *W = A.gwkp + ((size_t)part * A.n + A.sorig[p]) * 12;
+            for (int m = 0; m < 12; ++m) W[m] = m < NV ? tot[m] : 0.0;
         }
+    }
+}
+
+// The parts' sums in part order, scaled by chgmod2 = ccoef q (gradient) and
+// ccoef q (energy), into the far field kept between far-field steps.
+__global__ void k_cmm_farsum(CmmArgs A) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= A.n) return;
+    double t[12];
+    for (int m = 0; m < 12; ++m) t[m] = 0.0;
+    for (int part = 0; part < CMM_FP; ++part) {
+        const double *W = A.gwkp + ((size_t)part * A.n + a) * 12;
+        for (int m = 0; m < 12; ++m) t[m] = t[m] + W[m];
+    }
+    const double qa = A.q[a], q2 = A.ccoef * qa;
+    for (int m = 0; m < 3; ++m) {
+        for (int d = 0; d < 3; ++d) A.gwk[((size_t)a * 3 + m) * 3 + d] = t[4 * m + d] * q2;
+        A.efar[(size_t)a * 3 + m] = qa * A.ccoef * t[4 * m + 3];
     }
 }
 

@@ -312,7 +312,7 @@ struct Engine {
     bool cmm_far_now = true;            // the far field of the force evaluation being issued
     CmmArgs cm{};
     DBuf<double> d_cmm_cg, d_cmm_c0, d_cmm_q, d_cmm_sx0, d_cmm_pot, d_cmm_gnb, d_cmm_enb, d_cmm_gwk, d_cmm_efar,
-        d_cmm_eatom, d_cmm_wb, d_cmm_wb2;
+        d_cmm_eatom, d_cmm_wb, d_cmm_wb2, d_cmm_gwkp;
     DBuf<double4> d_cmm_sx;
     DBuf<int> d_cmm_key, d_cmm_kcnt, d_cmm_kstart, d_cmm_kfill, d_cmm_tmp, d_cmm_sorig, d_cmm_spos, d_cmm_sflag,
         d_cmm_smol, d_cmm_npc, d_cmm_err, d_cmm_xwide, d_cmm_l2cnt, d_cmm_l2ptr, d_cmm_l2idx, d_cmm_vflag,
@@ -392,7 +392,7 @@ struct Engine {
         d_lgrp.release(); d_esys.release(); d_ecls.release(); d_lambk.release();
         d_cmm_cg.release(); d_cmm_c0.release(); d_cmm_q.release(); d_cmm_sx0.release(); d_cmm_pot.release();
         d_cmm_gnb.release(); d_cmm_enb.release(); d_cmm_gwk.release(); d_cmm_efar.release(); d_cmm_eatom.release();
-        d_cmm_wb.release(); d_cmm_wb2.release(); d_cmm_sx.release(); d_cmm_key.release(); d_cmm_kcnt.release();
+        d_cmm_wb.release(); d_cmm_wb2.release(); d_cmm_gwkp.release(); d_cmm_sx.release(); d_cmm_key.release(); d_cmm_kcnt.release();
         d_cmm_kstart.release(); d_cmm_kfill.release(); d_cmm_tmp.release(); d_cmm_sorig.release();
         d_cmm_spos.release(); d_cmm_sflag.release(); d_cmm_smol.release(); d_cmm_npc.release(); d_cmm_err.release();
         d_cmm_xwide.release(); d_cmm_l2cnt.release(); d_cmm_l2ptr.release(); d_cmm_l2idx.release();
@@ -2461,6 +2461,7 @@ int Engine::set_cmm(int on, double cminsiz, int nlev, int cmmupd) {
     const size_t n = (size_t)natom;
     AL(d_cmm_cg, cg.size()); AL(d_cmm_c0, 3); AL(d_cmm_q, n); AL(d_cmm_sx0, 3 * n); AL(d_cmm_pot, 16 * (size_t)npc);
     AL(d_cmm_gnb, 9 * n); AL(d_cmm_enb, 6 * n); AL(d_cmm_gwk, 9 * n); AL(d_cmm_efar, 3 * n); AL(d_cmm_eatom, 6 * n);
+    AL(d_cmm_gwkp, (size_t)CMM_FP * 12 * n);
     AL(d_cmm_wb, 12 * n); AL(d_cmm_wb2, 12 * n); AL(d_cmm_sx, n);
     AL(d_cmm_key, n); AL(d_cmm_kcnt, A.nkey); AL(d_cmm_kstart, A.nkey + 1); AL(d_cmm_kfill, A.nkey);
     AL(d_cmm_tmp, n); AL(d_cmm_sorig, n); AL(d_cmm_spos, n); AL(d_cmm_sflag, n); AL(d_cmm_smol, n);
@@ -2486,6 +2487,7 @@ int Engine::set_cmm(int on, double cminsiz, int nlev, int cmmupd) {
     A.smol = d_cmm_smol.p; A.sx = d_cmm_sx.p; A.sx0 = d_cmm_sx0.p; A.npc = d_cmm_npc.p; A.pot = d_cmm_pot.p;
     A.err = d_cmm_err.p; A.sp = d_sp.p; A.efac = d_efac.p; A.gnb = d_cmm_gnb.p; A.enb = d_cmm_enb.p;
     A.gwk = d_cmm_gwk.p; A.efar = d_cmm_efar.p; A.gslot = d_gslot.p; A.eatom = d_cmm_eatom.p;
+    A.gwkp = d_cmm_gwkp.p;
     A.xmask = d_cmm_xmask.p; A.xwide = d_cmm_xwide.p; A.l2cnt = d_cmm_l2cnt.p; A.l2ptr = d_cmm_l2ptr.p;
     A.l2idx = d_cmm_l2idx.p; A.vflag = d_cmm_vflag.p; A.vpos = d_cmm_vpos.p; A.vlist = d_cmm_vlist.p;
     A.kvcnt = d_cmm_kvcnt.p; A.kvstart = d_cmm_kvstart.p; A.vx0 = d_cmm_vx0.p; A.vf = d_cmm_vf.p;
@@ -2561,6 +2563,13 @@ int Engine::cmm_celprp() {
     return 0;
 }
 
+// k_cmm_far's grid: CMM_FP blocks per group of 2 x 2 x 2 level-1 cells (per
+// level-1 cell with one level).
+static unsigned cmm_far_blocks(const CmmArgs &A) {
+    const int ng = A.nlev > 1 ? A.nv1 / 2 : A.nv1;
+    return (unsigned)(ng * ng * ng * CMM_FP);
+}
+
 // Exclusive scan of n ints (out[n] the total) in three launches.
 int Engine::cmm_scan(int n, const int *in, int *out) {
     const int nt = (n + 1023) / 1024;
@@ -2579,7 +2588,10 @@ static void launch_cmm(Engine &g, bool itwin) {
     hipLaunchKernelGGL((k_cmm_near<MULTI, E>), A.nc1, CMM_NT, sizeof(double2) * A.ntyp * A.ntyp, g.st, A);
     if (A.idelay > 1 && g.cmm_vcap > 0)
         hipLaunchKernelGGL((k_cmm_lj2<MULTI, E>), grid1(g.cmm_vcap, 4), 256, 0, g.st, A);
-    if (itwin) hipLaunchKernelGGL(k_cmm_far<MULTI>, A.nc1, CMM_NT, 0, g.st, A);
+    if (itwin) {
+        hipLaunchKernelGGL(k_cmm_far<MULTI>, cmm_far_blocks(A), CMM_NT, 0, g.st, A);
+        hipLaunchKernelGGL(k_cmm_farsum, grid1(A.n, 256), 256, 0, g.st, A);
+    }
     hipLaunchKernelGGL(k_cmm_combine<E>, grid1(A.n, 256), 256, 0, g.st, A);
 }
 
@@ -2620,8 +2632,9 @@ int Engine::cmm_time(int reps, double &near, double &far) {
     }
     CK(hipEventRecord(ev[1], st));
     for (int k = 0; k < reps; ++k) {
-        if (multi) hipLaunchKernelGGL(k_cmm_far<true>, A.nc1, CMM_NT, 0, st, A);
-        else hipLaunchKernelGGL(k_cmm_far<false>, A.nc1, CMM_NT, 0, st, A);
+        if (multi) hipLaunchKernelGGL(k_cmm_far<true>, cmm_far_blocks(A), CMM_NT, 0, st, A);
+        else hipLaunchKernelGGL(k_cmm_far<false>, cmm_far_blocks(A), CMM_NT, 0, st, A);
+        hipLaunchKernelGGL(k_cmm_farsum, grid1(A.n, 256), 256, 0, st, A);
     }
     CK(hipEventRecord(ev[2], st));
     CK(hipGetLastError());
