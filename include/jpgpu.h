@@ -240,6 +240,18 @@ int jp_last_timing(jp_ctx *ctx, double *t_total, double *t_pair, double *t_list,
  * label < 1 or a group without degrees of freedom. */
 int jp_set_canonical(jp_ctx *ctx, int32_t on, double temps, int32_t icslop, double fcstol, const int32_t *group);
 
+/* md_run's heating loop (HEATLO, STARTT; verlethe.f90:419-423, 455-459,
+ * 600-604, 641-646, 671-675) for the canonical runs that follow: while
+ * md_run's step count idloop (jp_set_step, the restart's loop count) is at
+ * most heatlo, the target temperature of the step is
+ * startt + ((temps - startt) / heatlo) idloop in place of temps (the
+ * velocity scaling and the end-of-round test; without SHAKE groups the
+ * current temperature is still taken as temps), the restart set-up step
+ * (jp_restart_step) using the restart's idloop.  Those steps run as host
+ * launches, the later ones as step graphs again.  heatlo = 0 (md_run's
+ * default) switches it off.  Returns JP_EINVAL for heatlo < 0 or startt < 0. */
+int jp_set_heating(jp_ctx *ctx, int32_t heatlo, double startt);
+
 /* Multicanonical MD (md_run METHOD= GE, idfvsl = 4, verlethe.f90:196-233) for
  * the following jp_run_nve calls, on top of jp_set_canonical's temperature
  * control: every step's gradient is multiplied by the weight
@@ -438,6 +450,8 @@ int jp_batch_get_option(jp_batch *b, int32_t option, int32_t *value);
 
 /* jp_set_canonical for every system of a batch (group: labels for all atoms
  * of the batch in order, or NULL). */
+/* jp_set_heating for every system of a batch. */
+int jp_batch_set_heating(jp_batch *b, int32_t heatlo, double startt);
 int jp_batch_set_canonical(jp_batch *b, int32_t on, double temps, int32_t icslop, double fcstol,
                            const int32_t *group);
 /* jp_set_alsd, jp_get_lambda and jp_set_lambda for the systems of a batch

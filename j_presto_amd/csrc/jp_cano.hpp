@@ -50,6 +50,8 @@ struct CanoArgs {
     const int *sfirst;         // first atom of system s (writes the system's flag)
     const int *shk_on;         // system s has SHAKE groups
     const double *gdof, *temps;
+    const double *ttgt;        // per system the step's target temperature: temps, or the heating
+                               // loop's ramp (verlethe.f90:419-423, 600-604)
     const double *mass, *imass, *gslot;
     const double *gew;         // METHOD= GE: per system, the weight every gradient read carries (null: 1)
     // ALSD (jp_alsd.hpp; lam null: off): per system the lambda state
@@ -134,14 +136,15 @@ __device__ __forceinline__ void block_temps(const CanoArgs &A, const double *__r
 // velocity scaled with the temperature T of its group when l_temp_control
 // (else a plain kick), then lambda's drift, and the mean velocity the later
 // rounds' temperatures use.  One thread per system.
-__device__ __forceinline__ void alsd_move(const CanoArgs &A, int s, double T, double ts, double dt) {
+// T: the temperature of lambda's group; tt: the step's target temperature.
+__device__ __forceinline__ void alsd_move(const CanoArgs &A, int s, double T, double tt, double dt) {
 #pragma clang fp contract(off)
     double *L = A.lam + JP_LS * s;
     const double lf = L[2], ilm = 1.0 / A.lmass[s];
     double lv = L[1];
     if (A.ltc && A.lgrp[s] >= 0) {
         const double itc = 1.0 / T;
-        const double chi = sqrt(ts * itc);
+        const double chi = sqrt(tt * itc);
         const double rt = chi * dt * JP_FDCVCV, rt2 = 2.0 * chi - 1.0;
         lv = rt2 * lv + rt * lf * ilm;
     } else {
@@ -241,14 +244,14 @@ __global__ void __launch_bounds__(256) k_cano_drift(CanoArgs A, const SysP *__re
     if (a >= B.y) return;
     const int s = A.atom_sys[a];
     const SysP &P = sp[s];
-    const double dt = P.dt, im = A.imass[a], ts = A.temps[s];
-    const double T = A.shk_on[s] ? sT[A.gid[a] - g0] : ts;   // verlethe.f90:396-404
+    const double dt = P.dt, im = A.imass[a], ts = A.temps[s], tt = A.ttgt[s];
+    const double T = A.shk_on[s] ? sT[A.gid[a] - g0] : ts;   // verlethe.f90:396-415
     if (A.lam && a == A.sfirst[s]) {
         const int lg = A.lgrp[s];
-        alsd_move(A, s, (A.shk_on[s] && lg >= 0) ? sT[lg - g0] : ts, ts, dt);
+        alsd_move(A, s, (A.shk_on[s] && lg >= 0) ? sT[lg - g0] : ts, tt, dt);
     }
     const double itc = 1.0 / T;
-    const double chi = sqrt(ts * itc);
+    const double chi = sqrt(tt * itc);
     const double rt = chi * dt * JP_FDCVCV, rt2 = 2.0 * chi - 1.0;
     const size_t sl = (size_t)A.inum[a];
     double xa[3];
@@ -283,7 +286,7 @@ __global__ void __launch_bounds__(128) k_cano_shake(CanoArgs A, const SysP *__re
     double *pu_out = A.pu + (size_t)(round & 1) * half, *pn_out = A.pn + (size_t)(round & 1) * half;
     const double *pu_in = A.pu + (size_t)((round - 1) & 1) * half;
     const double *pn_in = A.pn + (size_t)((round - 1) & 1) * half;
-    const double ts = A.temps[s];
+    const double ts = A.temps[s], tt = A.ttgt[s];
     // the unit's atom data, loaded before the group sums so that both
     // memory round trips overlap: x, x_o, g, f_c, v_b, and v (round 1) or
     // x_b (later rounds)
@@ -316,7 +319,7 @@ __global__ void __launch_bounds__(128) k_cano_shake(CanoArgs A, const SysP *__re
         block_temps(A, A.pa, A.nab, A.ab0, A.ab1, g0, g1, sT, 3);
         if (A.lam && threadIdx.x == 0 && blockIdx.x == A.ub0[s]) {
             const int lg = A.lgrp[s];
-            alsd_move(A, s, (A.shk_on[s] && lg >= 0) ? sT[lg - g0] : ts, ts, sp[s].dt);
+            alsd_move(A, s, (A.shk_on[s] && lg >= 0) ? sT[lg - g0] : ts, tt, sp[s].dt);
         }
     } else {
         if (threadIdx.x == 0) s_run = !A.done[s];
@@ -330,7 +333,7 @@ __global__ void __launch_bounds__(128) k_cano_shake(CanoArgs A, const SysP *__re
             tmin = (g == g0) ? T : fmin(tmin, T);
             tmax = (g == g0) ? T : fmax(tmax, T);
         }
-        if (fmax(fabs(tmin - ts), fabs(tmax - ts)) < A.fcstol * ts) {
+        if (fmax(fabs(tmin - tt), fabs(tmax - tt)) < A.fcstol * tt) {
             if (threadIdx.x == 0 && blockIdx.x == A.ub0[s]) A.done[s] = 1;
             return;
         }
@@ -355,7 +358,7 @@ __global__ void __launch_bounds__(128) k_cano_shake(CanoArgs A, const SysP *__re
             if (round == 1) {   // verlethe.f90:396-404, 509-516
                 const double T = shk ? sT[agid[q] - g0] : ts;
                 const double itc = 1.0 / T;
-                const double chi = sqrt(ts * itc);
+                const double chi = sqrt(tt * itc);
                 const double rt = chi * dt * JP_FDCVCV, rt2 = 2.0 * chi - 1.0;
                 for (int k = 0; k < 3; ++k) {
                     const double vk = rt2 * avx[q][k] - rt * ag[q][k] * im;
@@ -364,7 +367,7 @@ __global__ void __launch_bounds__(128) k_cano_shake(CanoArgs A, const SysP *__re
                 }
             } else {   // verlethe.f90:611-657: constraint force, then the update
                 const double itc = 1.0 / sT[agid[q] - g0];
-                const double chi = sqrt(ts * itc);
+                const double chi = sqrt(tt * itc);
                 const double rt = chi * dt, rt2 = 2.0 * chi - 1.0, r = 1.0 / (dt * dt);
                 for (int k = 0; k < 3; ++k) {
                     const double f = afc[q][k] + r * (ax[q][k] - avx[q][k]);

@@ -272,6 +272,19 @@ struct Engine {
     DBuf<int2> d_ablk, d_ublk;
     DBuf<double> d_gdof, d_ctemps, d_fcc, d_xcb, d_pa, d_pu, d_pn;
     int set_canonical(int on, double temps, int icslop, double fcstol, const int *group);
+    // md_run's heating loop (jp_set_heating: HEATLO, STARTT): while md_run's
+    // step count idloop is at most heat_lp the canonical target temperature
+    // is heat_t0 + ((temps - heat_t0) / heat_lp) idloop (verlethe.f90:419-423)
+    int heat_lp = 0;
+    double heat_t0 = 0.0, cano_temps = 300.0, ttgt_set = -1.0;
+    DBuf<double> d_ttgt;                // the target of the step being issued, per system
+    int set_heating(int htlp, double tmpsta);
+    double heat_target(long long id) const {
+        if (heat_lp <= 0 || id > heat_lp) return cano_temps;
+        return heat_t0 + ((cano_temps - heat_t0) * (1.0 / (double)heat_lp)) * (double)id;
+    }
+    bool heating(long long id) const { return cano && heat_lp > 0 && id <= heat_lp; }
+    int put_target(double t);
     CanoArgs cano_args();
     int cano_step();
     bool cano_retry() const { return cano && nsys == 1 && (n2 + n3) > 0; }
@@ -383,7 +396,7 @@ struct Engine {
         d_bndp.release(); d_angp.release(); d_eterm.release(); d_unit.release();
         d_gid.release(); d_gsys.release(); d_sg0.release(); d_sg1.release(); d_ab0.release(); d_ab1.release();
         d_ub0.release(); d_ub1.release(); d_ablk.release(); d_ublk.release(); d_sfirst.release(); d_shkon.release(); d_cdone.release();
-        d_gdof.release(); d_ctemps.release(); d_fcc.release(); d_xcb.release(); d_pa.release(); d_pu.release(); d_pn.release();
+        d_gdof.release(); d_ctemps.release(); d_ttgt.release(); d_fcc.release(); d_xcb.release(); d_pa.release(); d_pu.release(); d_pn.release();
         d_torc.release(); d_tor.release(); d_p14.release(); d_p14p.release(); d_torp.release();
         d_abeg.release(); d_aend.release(); d_cbeg.release(); d_cend.release(); d_rsum.release(); d_rfix.release();
         d_part.release(); d_partf.release();
@@ -1854,7 +1867,7 @@ int Engine::set_canonical(int on, double temps, int icslop, double fcstol, const
     std::vector<double> tset(nsys, temps);
     AL(d_gid, natom); AL(d_gsys, ngrp); AL(d_gdof, ngrp);
     AL(d_sg0, nsys); AL(d_sg1, nsys); AL(d_ab0, nsys); AL(d_ab1, nsys); AL(d_ub0, nsys); AL(d_ub1, nsys);
-    AL(d_sfirst, nsys); AL(d_shkon, nsys); AL(d_cdone, nsys); AL(d_ctemps, nsys);
+    AL(d_sfirst, nsys); AL(d_shkon, nsys); AL(d_cdone, nsys); AL(d_ctemps, nsys); AL(d_ttgt, nsys);
     AL(d_fcc, 3 * (size_t)natom); AL(d_xcb, 3 * (size_t)natom);
     AL(d_pa, (size_t)ngrp * nab); AL(d_pu, 2 * (size_t)ngrp * nub); AL(d_pn, 2 * (size_t)ngrp * nub); AL(d_ablk, nab); AL(d_ublk, nub);
     auto H2D = [&](void *dst, const void *src, size_t bytes) -> int {
@@ -1876,8 +1889,11 @@ int Engine::set_canonical(int on, double temps, int icslop, double fcstol, const
     r |= H2D(d_sfirst.p, sfirst.data(), 4 * (size_t)nsys);
     r |= H2D(d_shkon.p, shkon.data(), 4 * (size_t)nsys);
     r |= H2D(d_ctemps.p, tset.data(), 8 * (size_t)nsys);
+    r |= H2D(d_ttgt.p, tset.data(), 8 * (size_t)nsys);
     if (r) return r;
     CK(hipStreamSynchronize(st));   // the host vectors end with this call
+    cano_temps = temps;
+    ttgt_set = temps;
     cano_icslop = icslop;
     cano_fcstol = fcstol;
     cano_grouped = group != nullptr;
@@ -1885,6 +1901,28 @@ int Engine::set_canonical(int on, double temps, int icslop, double fcstol, const
     sg0_h = sg0;
     sg1_h = sg1;
     cano = true;
+    return 0;
+}
+
+__global__ void k_fill_d(double *p, int n, double v) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// The canonical target temperature of the steps that follow, in stream
+// order (a fill kernel: the heating steps are host launches).
+int Engine::put_target(double t) {
+    if (t == ttgt_set) return 0;
+    hipLaunchKernelGGL(k_fill_d, grid1(nsys, 64), 64, 0, st, d_ttgt.p, nsys, t);
+    CK(hipGetLastError());
+    ttgt_set = t;
+    return 0;
+}
+
+int Engine::set_heating(int htlp, double tmpsta) {
+    if (htlp < 0 || !std::isfinite(tmpsta) || tmpsta < 0.0) return JP_EINVAL;
+    heat_lp = htlp;
+    heat_t0 = tmpsta;
     return 0;
 }
 
@@ -1896,7 +1934,7 @@ CanoArgs Engine::cano_args() {
     A.gid = d_gid.p; A.gsys = d_gsys.p; A.sg0 = d_sg0.p; A.sg1 = d_sg1.p;
     A.ab0 = d_ab0.p; A.ab1 = d_ab1.p; A.ub0 = d_ub0.p; A.ub1 = d_ub1.p; A.sfirst = d_sfirst.p;
     A.ablk = d_ablk.p; A.ublk = d_ublk.p;
-    A.shk_on = d_shkon.p; A.gdof = d_gdof.p; A.temps = d_ctemps.p;
+    A.shk_on = d_shkon.p; A.gdof = d_gdof.p; A.temps = d_ctemps.p; A.ttgt = d_ttgt.p;
     A.mass = d_mass.p; A.imass = d_imass.p; A.gslot = d_gslot.p;
     A.gew = ge ? d_gew.p : nullptr;
     A.lam = alsd ? d_lam.p : nullptr;
@@ -2045,6 +2083,7 @@ int Engine::restart_step(int prec, jp_energy_log *log) {
     CK(hipMemsetAsync(d_err.p, 0, 4, st));
     CK(hipMemcpyAsync(d_xbak.p, d_x.p, 24 * (size_t)natom, hipMemcpyDeviceToDevice, st));
     cmm_far_now = cmm_itwin;
+    if (cano && (r = put_target(heat_target(idloop)))) return r;   // inirst: md_run's idloop is the restart's
     if ((r = alsd_fac()) || (r = nb(prec, true, cmm_far_now))) return r;
     if (nbat) {
         hipLaunchKernelGGL(k_bonded, grid1(nbat, 128), 128, 0, st, bonded_args(), d_atom_sys.p, d_sp.p, d_x.p,
@@ -2226,13 +2265,14 @@ int Engine::run_nve(int nsteps, int update, int nmon, int prec, jp_energy_log *l
         const bool mon = nmon > 0 && (idloop + 1) % nmon == 0;
         // plain steps up to the next rebuild, energy step or the end
         int nplain = 0;
-        if (!mon && use_graphs) {
+        if (!mon && use_graphs && !heating(idloop + 1)) {
             const int to_rebuild = (int)(update - idloop % update);
             int lim = std::min(nsteps - step + 1, to_rebuild);
             if (nmon > 0) lim = std::min(lim, (int)(nmon - (idloop + 1) % nmon));   // stop before the energy step
             nplain = lim;
         }
         if (nplain >= 2) {
+            if (cano && (r = put_target(cano_temps))) return r;
             StepGraph *g = step_graph(nplain, prec);
             if (g) {
                 if (retry && (r = save_state(d_xs.p, d_vs.p, d_errs.p))) return r;
@@ -2296,6 +2336,7 @@ int Engine::run_nve(int nsteps, int update, int nmon, int prec, jp_energy_log *l
             t_pair += m2 * 1e-3;
             pe_pending = false;
         }
+        if (cano && (r = put_target(heat_target(idloop + 1)))) return r;
         if ((r = alsd_fac())) return r;
         cmm_far_now = cmm_itwin;
         CK(hipEventRecord(ev[4], st));
@@ -2372,7 +2413,7 @@ int Engine::prepare_graphs(int nsteps, int update, int nmon, int prec) {
     while (step <= nsteps) {
         const bool mon = nmon > 0 && (id + 1) % nmon == 0;
         int nplain = 0;
-        if (!mon) {
+        if (!mon && !heating(id + 1)) {
             nplain = std::min(nsteps - step + 1, (int)(update - id % update));
             if (nmon > 0) nplain = std::min(nplain, (int)(nmon - (id + 1) % nmon));
         }
@@ -3058,6 +3099,16 @@ int jp_set_mcmd(jp_ctx *ctx, int32_t on, int32_t nwindow, int32_t ndeg, const do
                 const double *high, double alphalw, double alphaup, double celw, double ceup, double tempce) {
     if (!ctx) return JP_EINVAL;
     return ctx->e.set_mcmd(on, nwindow, ndeg, c, low, high, alphalw, alphaup, celw, ceup, tempce);
+}
+
+int jp_set_heating(jp_ctx *ctx, int32_t heatlo, double startt) {
+    if (!ctx) return JP_EINVAL;
+    return ctx->e.set_heating(heatlo, startt);
+}
+
+int jp_batch_set_heating(jp_batch *b, int32_t heatlo, double startt) {
+    if (!b) return JP_EINVAL;
+    return b->e.set_heating(heatlo, startt);
 }
 
 int jp_time_cmm(jp_ctx *ctx, int32_t reps, double *near, double *far) {

@@ -212,6 +212,11 @@ class MdSystem:
         default) and tctl (LTCNTL); None switches it off."""
         self._al = _alsd_call(self.lib.jp_set_alsd, "jp_set_alsd", self.ctx, al)
 
+    def set_heating(self, heatlo: int, startt: float = 300.0):
+        """md_run's heating loop (HEATLO steps from STARTT K to SETTEM) for
+        the canonical runs that follow; heatlo = 0 switches it off."""
+        native.check("jp_set_heating", self.lib.jp_set_heating(self.ctx, int(heatlo), float(startt)))
+
     def set_cmm(self, cminsiz: Optional[float] = 6.0, nlev: int = 3, cmmupd: int = 5):
         """CAL15M= CMM (cell multipole method, BOUNDA= NO) for the following
         evaluations and runs: CELSIZ, CELLVL and CMMUPD; cminsiz=None returns

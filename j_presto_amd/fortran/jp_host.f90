@@ -11,7 +11,7 @@
 !                 COORDI with CRDFMT= PDB (default) or BINA, SETSHK, and CLUSTA= a<b (atoms
 !                 a..b in cluster 1, the rest in cluster 2) with GEPSME= LMD,
 !                 read by jp_input.f90.  EXE> MD: LOOPLI, UPDATE, TIMEST,
-!                 METHOD (MICR, CANO or GE; required), SETTEM, TEMPCO, OUTLOG,
+!                 METHOD (MICR, CANO or GE; required), SETTEM, TEMPCO, HEATLO, STARTT, OUTLOG,
 !                 RESTAR, NAMERI, NAMERO, NAMETO, NAMEGE, LAMBDA, LMDSQU, CUTLEN,
 !                 DIEVAL, XMINCL..ZMAXCL, COVSHK, LIMSHK (inpmd.f90; defaults of
 !                 main.f90:150-185).  CAL15M= ZD runs under BOUNDA= PBC;
@@ -75,6 +75,9 @@ program jp_host
   logical :: cmm
   real(c_double) :: celsiz
   integer(c_int32_t) :: cmmupd
+  ! the heating loop of METHOD= CANO / GE (HEATLO steps from STARTT K)
+  integer(c_int32_t) :: heatlo
+  real(c_double) :: startt
   integer(8) :: tstart, trate, tnow          ! wall clock since the program started (md_run's fxcpus)
   integer(c_int32_t), allocatable, target :: grp(:)
   type(top_t) :: top
@@ -333,6 +336,7 @@ contains
     cutlen = 12.0d0; dieval = 1.0d0; box = 0.0d0; covshk = 1.0d-6; limshk = 1000
     cellvl = 3; cputim = 99999999.0d0; initzero = .false.
     cmm = .true.; celsiz = 6.0d0; cmmupd = 5   ! main.f90:170, inpinp.f90:86, inpmd.f90:151
+    heatlo = 0; startt = 300.0d0                 ! inpmd.f90:127,147
     gemode = .false.; gef = ' '; lweigh = 0.0d0; ltcntl = 1; lamkw = 0.0d0; lsqkw = 0.0d0
     have_method = .false.; have_pbc = .false.
     inmd = .false.; inin = .false.
@@ -433,8 +437,9 @@ contains
           case ('STOPCE')
              if (trim(val) /= 'NO') call refuse(key, val, 'centre-of-mass motion removal is not implemented')
           case ('HEATLO')
-             read(val, *, iostat=ios) iv
-             if (ios == 0 .and. iv /= 0) call refuse(key, val, 'the heating loop is not implemented')
+             read(val, *, iostat=ios) heatlo
+             if (ios == 0 .and. heatlo < 0) call refuse(key, val, 'a step count >= 0')
+          case ('STARTT'); read(val, *, iostat=ios) startt
           case ('SHAKEM')
              if (trim(val) /= 'HBON') call refuse(key, val, 'only SHAKEM= HBON (the SHK file''s groups) is implemented')
           case ('PSETMP')
@@ -456,7 +461,7 @@ contains
                 call refuse(key, val, 'NAMETO is written as ASCII only')
              end if
           case ('PARALV', 'N_SIMD', 'LOGFOR', 'RELAXA', 'TEMPER', 'NSTPCN', 'OUTATM', 'DUMMYL', 'BESTFI', &
-                'RANDOM', 'STARTT', 'RADCAP', 'FORCAP', 'FUNCAP', 'WETPSR', 'WETDSR', 'WETDHR', &
+                'RANDOM', 'RADCAP', 'FORCAP', 'FUNCAP', 'WETPSR', 'WETDSR', 'WETDHR', &
                 'CAPSHP', 'FOACAP', 'RADPRO', 'HTPSLO', 'STAPST', 'ELLIPA', 'ELLIPB', 'ELLIPC', 'ELLBUF', &
                 'CELBUF', 'RADIUS')
              ! thread layout and SIMD width, log format, methods and boundaries
@@ -624,6 +629,8 @@ contains
        else
           ier(9) = jp_set_canonical(ctx, 1_c_int32_t, settem, 3_c_int32_t, 1.0d-3, c_null_ptr)
        end if
+       ! the heating loop (verlethe.f90:419-423): the target ramps from STARTT
+       if (ier(9) == 0 .and. heatlo > 0) ier(9) = jp_set_heating(ctx, heatlo, startt)
     end if
     if (gemode .and. ier(9) == 0) then
        if (lmdin) then   ! ALSD (verlethe.f90:235-290)

@@ -197,6 +197,13 @@ module jpgpu
        real(c_double), value :: lower, upper, celw, ceup, tempce, lambda_m
      end function jp_set_alsd
 
+     integer(c_int) function jp_set_heating(ctx, heatlo, startt) bind(C, name='jp_set_heating')
+       import :: c_int, c_int32_t, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: heatlo
+       real(c_double), value :: startt
+     end function jp_set_heating
+
      integer(c_int) function jp_set_cmm(ctx, on, cminsiz, nlev, cmmupd) bind(C, name='jp_set_cmm')
        import :: c_int, c_int32_t, c_ptr, c_double
        type(c_ptr), value :: ctx

@@ -51,6 +51,8 @@ SIGNATURES = {
     "jp_batch_set_alsd": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_int32, f64p, f64p, f64p, C.c_double,
                                     C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int32]),
     "jp_set_cmm": (C.c_int, [vp, C.c_int32, C.c_double, C.c_int32, C.c_int32]),
+    "jp_set_heating": (C.c_int, [vp, C.c_int32, C.c_double]),
+    "jp_batch_set_heating": (C.c_int, [vp, C.c_int32, C.c_double]),
     "jp_time_cmm": (C.c_int, [vp, C.c_int32, f64p, f64p]),
     "jp_get_lambda": (C.c_int, [vp, f64p, f64p]),
     "jp_set_lambda": (C.c_int, [vp, C.c_double, C.c_double]),

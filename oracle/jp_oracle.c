@@ -1140,30 +1140,35 @@ static void group_temps(const jo_ctx *c, const double *w, int ngrp, const int *g
     group_temps_l(c, w, ngrp, grp, dof, T, 0.0, 0.0, 0, -1);
 }
 
-/* verlethe.f90:79-757 for METHOD= CANO (idfvsl = 3) with constant lambda,
- * STOPCE= NO and HEATLO= 0: itmeth 1 (grp == NULL, one temperature) or 2
+/* verlethe.f90:79-757 for METHOD= CANO (idfvsl = 3) with constant lambda
+ * and STOPCE= NO: itmeth 1 (grp == NULL, one temperature) or 2
  * (grp[i] = 0-based temperature group of atom i, md.f90:136-172).  gtot is
  * the lambda-combined gradient; v is v(t-dt/2) on entry, v(t+dt/2) on exit;
  * vback receives v(t-dt/2) and xold x(t).  Velocity scaling toward temps
  * (chi = sqrt(temps/T) on the unconstrained half-step velocities), then up
  * to icslop rounds of SHAKE + temperature (exit when every group is within
  * fcstol*temps).  A SHAKE failure returns its ier (md_run would halve dt
- * and retry, verlethe.f90:546-573).  Expressions keep md_run's operand order. */
+ * and retry, verlethe.f90:546-573).  Expressions keep md_run's operand order.
+ * ttgt is the step's target temperature: temps, or during md_run's heating
+ * loop (HEATLO) the ramp from STARTT (verlethe.f90:419-423, 455-459,
+ * 600-604, 641-646, 671-675); without SHAKE groups the current temperature
+ * is taken as temps (tempc = temps, verlethe.f90:410-415). */
 /* al != NULL: ALSD (verlethe.f90:235-290, 375-445, 465-470, 545-546,
  * 577-595, 742): lambda (c->lambda) and its velocity *lv take the step
  * beside the atoms under the force lf. */
 static int cano_step_l(jo_ctx *c, double *x, double *v, const double *gtot, double *xold, double *vback,
-                       double temps, int icslop, double fcstol, int ngrp, const int *grp, const double *dof,
-                       const jo_alsd *al, double lf, double *lv);
+                       double temps, double ttgt, int icslop, double fcstol, int ngrp, const int *grp,
+                       const double *dof, const jo_alsd *al, double lf, double *lv);
 
 int jo_cano_step(const jo_ctx *c, double *x, double *v, const double *gtot, double *xold, double *vback,
                  double temps, int icslop, double fcstol, int ngrp, const int *grp, const double *dof) {
-    return cano_step_l((jo_ctx *)c, x, v, gtot, xold, vback, temps, icslop, fcstol, ngrp, grp, dof, NULL, 0.0, NULL);
+    return cano_step_l((jo_ctx *)c, x, v, gtot, xold, vback, temps, temps, icslop, fcstol, ngrp, grp, dof, NULL,
+                       0.0, NULL);
 }
 
 static int cano_step_l(jo_ctx *c, double *x, double *v, const double *gtot, double *xold, double *vback,
-                       double temps, int icslop, double fcstol, int ngrp, const int *grp, const double *dof,
-                       const jo_alsd *al, double lf, double *lv) {
+                       double temps, double ttgt, int icslop, double fcstol, int ngrp, const int *grp,
+                       const double *dof, const jo_alsd *al, double lf, double *lv) {
     const int n = c->n;
     const double dt = c->dt, conv = 4.184e-4; /* md.f90:56 fdcvcv */
     double *im = (double *)malloc(sizeof(double) * n);
@@ -1188,14 +1193,14 @@ static int cano_step_l(jo_ctx *c, double *x, double *v, const double *gtot, doub
     else for (int g = 0; g < ngrp; ++g) T[g] = temps;
     for (int i = 0; i < n; ++i) {
         const double itc = 1.0 / T[grp ? grp[i] : 0];
-        const double chi = sqrt(temps * itc);
+        const double chi = sqrt(ttgt * itc);
         const double rt = chi * dt * conv, rt2 = 2.0 * chi - 1.0;
         for (int k = 0; k < 3; ++k) v[3 * i + k] = rt2 * v[3 * i + k] - rt * gtot[3 * i + k] * im[i];
     }
     if (al) {
         const int g = grp ? lgrp : 0;
         if (tctl && g >= 0) {
-            const double chi = sqrt(temps * (1.0 / T[g]));
+            const double chi = sqrt(ttgt * (1.0 / T[g]));
             const double rt = chi * dt * conv, rt2 = 2.0 * chi - 1.0;
             *lv = rt2 * *lv + rt * lf * ilm;
         } else {
@@ -1218,7 +1223,7 @@ static int cano_step_l(jo_ctx *c, double *x, double *v, const double *gtot, doub
             group_temps_l(c, w, ngrp, grp, dof, T, lm, lvt, tctl, lgrp);
             double tmin = T[0], tmax = T[0];
             for (int g = 1; g < ngrp; ++g) { tmin = fmin(tmin, T[g]); tmax = fmax(tmax, T[g]); }
-            if (fmax(fabs(tmin - temps), fabs(tmax - temps)) < fcstol * temps) break;
+            if (fmax(fabs(tmin - ttgt), fabs(tmax - ttgt)) < fcstol * ttgt) break;
             if (it == icslop) break;
             r = 1.0 / (dt * dt);
             const double r2 = 0.5 * dt;
@@ -1231,7 +1236,7 @@ static int cano_step_l(jo_ctx *c, double *x, double *v, const double *gtot, doub
             group_temps_l(c, w, ngrp, grp, dof, T, lm, lvt, tctl, lgrp);
             for (int i = 0; i < n; ++i) {
                 const double itc = 1.0 / T[grp ? grp[i] : 0];
-                const double chi = sqrt(temps * itc);
+                const double chi = sqrt(ttgt * itc);
                 const double rt = chi * dt, rt2 = 2.0 * chi - 1.0;
                 for (int k = 0; k < 3; ++k) {
                     const int q = 3 * i + k;
@@ -1404,8 +1409,9 @@ double jo_alsd_force(const jo_alsd *a, double lambda, double e1, double e2) {
  * ishk == jshk, and two successes at T double it back (745-752).  vback
  * receives the velocities the last substep started from. */
 static int cano_verlethe(jo_ctx *c, double *x, double *v, double *vback, double *grad, double *gt, double *xo,
-                         double *xs, double *vs, double temps, int icslop, double fcstol, int ngrp, const int *grp,
-                         const double *dof, const jo_ge *ge, const jo_alsd *al, double *lv, int *nprov) {
+                         double *xs, double *vs, double temps, double ttgt, int icslop, double fcstol, int ngrp,
+                         const int *grp, const double *dof, const jo_ge *ge, const jo_alsd *al, double *lv,
+                         int *nprov) {
     const int n = c->n, nfrg = c->nfrg;
     const double dt0 = c->dt;
     double T = dt0, eterm[18], ee[3], ev[3];
@@ -1425,7 +1431,7 @@ static int cano_verlethe(jo_ctx *c, double *x, double *v, double *vback, double 
         }
         double lf = 0.0;
         if (al) lf = jo_alsd_force(al, c->lambda, class_epot(eterm, ee, ev, 0), class_epot(eterm, ee, ev, 1));
-        ier = cano_step_l(c, x, v, gt, xo, vback, temps, icslop, fcstol, ngrp, grp, dof, al, lf, lv);
+        ier = cano_step_l(c, x, v, gt, xo, vback, temps, ttgt, icslop, fcstol, ngrp, grp, dof, al, lf, lv);
         if (ier) {
             memcpy(x, xs, sizeof(double) * 3 * n);
             memcpy(v, vs, sizeof(double) * 3 * n);
@@ -1457,14 +1463,33 @@ static int cano_verlethe(jo_ctx *c, double *x, double *v, double *vback, double 
  * from), then nsteps verlethe calls with a list rebuild after every step
  * that is a multiple of update (md.f90:519-694).  nprov counts the SHAKE
  * provisions (failed substeps) md_run prints. */
+/* The target temperature of md_run's step idloop (verlethe.f90:419-423):
+ * temps, or inside the heating loop of htlp steps the ramp from tmpsta. */
+static double heat_target(double temps, int idloop, int htlp, double tmpsta) {
+    if (htlp <= 0 || idloop > htlp) return temps;
+    return tmpsta + ((temps - tmpsta) * (1.0 / (double)htlp)) * (double)idloop;
+}
+
 static int run_cano_l(jo_ctx *c, double *x, double *v, int nsteps, int update, int inirst, double temps,
                       int icslop, double fcstol, int ngrp, const int *grp, const double *dof, const jo_ge *ge,
-                      const jo_alsd *al, double *lv, double *lam_log, int *nprov);
+                      const jo_alsd *al, double *lv, double *lam_log, int loop0, int htlp, double tmpsta,
+                      int *nprov);
 
 int jo_run_cano(jo_ctx *c, double *x, double *v, int nsteps, int update, int inirst, double temps, int icslop,
                 double fcstol, int ngrp, const int *grp, const double *dof, const jo_ge *ge, int *nprov) {
     return run_cano_l(c, x, v, nsteps, update, inirst, temps, icslop, fcstol, ngrp, grp, dof, ge, NULL, NULL, NULL,
-                      nprov);
+                      0, 0, 0.0, nprov);
+}
+
+/* jo_run_cano from a restart at md_run's loop count loop0 with the heating
+ * loop: while md_run's idloop is at most htlp the target temperature is
+ * tmpsta + ((temps - tmpsta) / htlp) idloop (HEATLO, STARTT; verlethe.f90:
+ * 419-423, 600-604), the restart set-up step taking idloop = loop0. */
+int jo_run_cano_heat(jo_ctx *c, double *x, double *v, int nsteps, int update, int inirst, double temps, int icslop,
+                     double fcstol, int ngrp, const int *grp, const double *dof, int loop0, int htlp, double tmpsta,
+                     int *nprov) {
+    return run_cano_l(c, x, v, nsteps, update, inirst, temps, icslop, fcstol, ngrp, grp, dof, NULL, NULL, NULL, NULL,
+                      loop0, htlp, tmpsta, nprov);
 }
 
 /* md_run's METHOD= GE loop under GEPSME= LMD (ALSD) from a restart: as
@@ -1479,12 +1504,13 @@ int jo_run_alsd(jo_ctx *c, double *x, double *v, double *lv, int nsteps, int upd
                 double *lam_log, int *nprov) {
     if (!al || c->nfrg != 3 || al->nwindow < 1 || al->ndeg < 0 || !(al->lmass > 0.0)) return JP_EINVAL;
     return run_cano_l(c, x, v, nsteps, update, inirst, temps, icslop, fcstol, ngrp, grp, dof, NULL, al, lv, lam_log,
-                      nprov);
+                      0, 0, 0.0, nprov);
 }
 
 static int run_cano_l(jo_ctx *c, double *x, double *v, int nsteps, int update, int inirst, double temps,
                       int icslop, double fcstol, int ngrp, const int *grp, const double *dof, const jo_ge *ge,
-                      const jo_alsd *al, double *lv, double *lam_log, int *nprov) {
+                      const jo_alsd *al, double *lv, double *lam_log, int loop0, int htlp, double tmpsta,
+                      int *nprov) {
     const size_t n3 = 3 * (size_t)c->n;
     double *grad = (double *)malloc(sizeof(double) * n3 * c->nfrg);
     double *gt = (double *)malloc(sizeof(double) * n3);
@@ -1502,18 +1528,19 @@ static int run_cano_l(jo_ctx *c, double *x, double *v, int nsteps, int update, i
     if (!ier && inirst) {
         const double l0 = c->lambda, lv0 = al ? *lv : 0.0;
         memcpy(x0, x, sizeof(double) * n3);
-        ier = cano_verlethe(c, x, v, vb, grad, gt, xo, xs, vs, temps, icslop, fcstol, ngrp, grp, dof, ge, al, lv,
-                            nprov);
+        ier = cano_verlethe(c, x, v, vb, grad, gt, xo, xs, vs, temps, heat_target(temps, loop0, htlp, tmpsta),
+                            icslop, fcstol, ngrp, grp, dof, ge, al, lv, nprov);
         memcpy(x, x0, sizeof(double) * n3);
         memcpy(v, vb, sizeof(double) * n3);
         c->lambda = l0;
         if (al) *lv = lv0;
     }
     for (int s = 1; s <= nsteps && !ier; ++s) {
-        ier = cano_verlethe(c, x, v, vb, grad, gt, xo, xs, vs, temps, icslop, fcstol, ngrp, grp, dof, ge, al, lv,
-                            nprov);
+        ier = cano_verlethe(c, x, v, vb, grad, gt, xo, xs, vs, temps, heat_target(temps, loop0 + s, htlp, tmpsta),
+                            icslop, fcstol, ngrp, grp, dof, ge, al, lv, nprov);
         if (lam_log) lam_log[s - 1] = c->lambda;
-        if (!ier && update > 0 && s % update == 0) ier = jo_celprp(c, x);
+        /* celprp after md_run's step count reaches a multiple of UPDATE (md.f90:655-656) */
+        if (!ier && update > 0 && (loop0 + s) % update == 0) ier = jo_celprp(c, x);
     }
     free(grad); free(gt); free(xo); free(vb); free(xs); free(vs); free(x0);
     return ier;

@@ -52,6 +52,8 @@ def lib():
         L.jo_run_nve.argtypes = [C.c_void_p, f64p, f64p, C.c_int, C.c_int]
         L.jo_run_cano.argtypes = [C.c_void_p, f64p, f64p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int,
                                   C.c_double, C.c_int, i32p, f64p, C.POINTER(GeParams), i32p]
+        L.jo_run_cano_heat.argtypes = [C.c_void_p, f64p, f64p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int,
+                                       C.c_double, C.c_int, i32p, f64p, C.c_int, C.c_int, C.c_double, i32p]
         L.jo_ge_weight.argtypes = [C.POINTER(GeParams), C.c_double]
         L.jo_ge_weight.restype = C.c_double
         L.jo_alsd_dew.argtypes = [C.POINTER(AlsdParams), C.c_double]
@@ -225,13 +227,21 @@ class Oracle:
         return lib().jo_run_nve(self.c, _p(x), _p(v), nsteps, update)
 
     def run_cano(self, x, v, nsteps: int, update: int = 10, temps: float = 300.0, icslop: int = 3,
-                 fcstol: float = 1e-3, grp=None, inirst: bool = True, ge=None):
+                 fcstol: float = 1e-3, grp=None, inirst: bool = True, ge=None, heat=None):
         """md_run's METHOD= CANO loop with verlethe's SHAKE retry (METHOD= GE
         with the multicanonical parameters ge), in place, after the restart
-        set-up step when inirst; returns (ier, provisions)."""
+        set-up step when inirst; heat = (restart loop, HEATLO, STARTT) runs
+        md_run's heating loop.  Returns (ier, provisions)."""
         dof = self.group_dof(grp)
         g = None if grp is None else np.ascontiguousarray(grp, np.int32)
         nprov = C.c_int32()
+        if heat is not None:
+            assert ge is None
+            loop0, htlp, tmpsta = heat
+            ier = lib().jo_run_cano_heat(self.c, _p(x), _p(v), nsteps, update, int(inirst), temps, icslop, fcstol,
+                                         len(dof), None if g is None else _p(g, i32p), _p(dof), int(loop0),
+                                         int(htlp), float(tmpsta), C.byref(nprov))
+            return ier, nprov.value
         gp, keep = ge_params(ge) if ge is not None else (None, None)
         ier = lib().jo_run_cano(self.c, _p(x), _p(v), nsteps, update, int(inirst), temps, icslop, fcstol, len(dof),
                                 None if g is None else _p(g, i32p), _p(dof),

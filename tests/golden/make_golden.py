@@ -26,6 +26,8 @@ and N_SIMD= 1 (a fixed summation order):
 A fixture stores the generator recipe, the input x and v, and the outputs.
 The reference itself never leaves this container.
 
+  heat jobs  (--heat-only: HEAT_RUNS) METHOD= CANO through md_run's heating loop
+             (HEATLO= 10, STARTT= 100), from restarts at loop 0 and 4.
   cmm jobs   (--cmm: CMM_RECIPES) CAL15M= CMM, BOUNDA= NO on droplets: the
              FDEBUG= MAKE dump and a MICR run with NAMETO.
   alsd jobs  (chain_lmd, --alsd-only) RESTAR= YES, METHOD= GE with GEPSME= LMD
@@ -33,7 +35,7 @@ The reference itself never leaves this container.
              under -2 E1 lambda - E2 - dew(lambda) beside the atoms.
 
 Usage: python tests/golden/make_golden.py [--cano-only | --mon-only | --host-only | --ge-only | --alsd-only |
-       --cmm]
+       --heat-only | --cmm]
        [name ...]
 """
 from __future__ import annotations
@@ -448,6 +450,46 @@ def make_cmm(name: str) -> str:
     return path
 
 
+# md_run's heating loop (METHOD= CANO, HEATLO= 10, STARTT= 100 K, SETTEM=
+# 300 K): tag -> (restart loop count, steps, TEMPCO)
+HEAT_RUNS = {"a": (0, 15, "NO"), "b": (4, 12, "YES")}
+
+
+def heat_jobs(s: systems.System, tmp: str) -> dict:
+    """METHOD= CANO runs through md_run's heating loop: the target
+    temperature ramps from STARTT to SETTEM over HEATLO steps of md_run's
+    step count, which continues from the restart's loop count."""
+    out = {}
+    for tag, (loop0, n, tempco) in HEAT_RUNS.items():
+        d = os.path.join(tmp, "heat_" + tag)
+        os.makedirs(d)
+        refio.write_restart(os.path.join(d, "in.rst"), s.x, s.v, loop=loop0, lam=s.lam)
+        run_md(d, s, {"METHOD": "CANO", "RESTAR": "YES", "NAMERI": "in.rst", "NAMERO": "out.rst", "LOOPLI": str(n),
+                      "UPDATE": "10", "TIMEST": "2.0", "SETTEM": "300.0", "STARTT": "100.0", "HEATLO": "10",
+                      "TEMPCO": tempco, "OUTLOG": "100000", "OUTCLE": "100000"})
+        r = refio.read_restart(os.path.join(d, "out.rst"))
+        out.update({f"heat_{tag}_x": r["x"], f"heat_{tag}_v": r["v"], f"heat_{tag}_loop": np.int64(r["loop"]),
+                    f"heat_{tag}_opt": np.array([loop0, n, 1 if tempco == "YES" else 0])})
+    return out
+
+
+def add_heat(name: str) -> str:
+    """Adds the heating-loop runs to an existing fixture."""
+    kind, args, opt = RECIPES[name]
+    s = build_system(kind, args)
+    path = os.path.join(HERE, f"{name}.npz")
+    with np.load(path, allow_pickle=False) as z:
+        out = {k: z[k] for k in z.files}
+    assert str(out["fingerprint"]) == fingerprint(s)
+    tmp = tempfile.mkdtemp(prefix="jpgold_")
+    try:
+        out.update(heat_jobs(s, tmp))
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    np.savez_compressed(path, **out)
+    return path
+
+
 def add_alsd(name: str) -> str:
     """Adds the ALSD runs to an existing fixture."""
     kind, args, opt = RECIPES[name]
@@ -541,9 +583,10 @@ def main(argv):
     host_only = "--host-only" in argv
     ge_only = "--ge-only" in argv
     alsd_only = "--alsd-only" in argv
+    heat_only = "--heat-only" in argv
     cmm_all = "--cmm" in argv
     argv = [a for a in argv if a not in ("--cano-only", "--mon-only", "--host-only", "--ge-only", "--alsd-only",
-                                         "--cmm")]
+                                         "--cmm", "--heat-only")]
     names = argv or [n for n in RECIPES if not mon_only or RECIPES[n][2].get("mon")]
     if cmm_all or any(n in CMM_RECIPES for n in names):
         for n in [n for n in names if n in CMM_RECIPES] or list(CMM_RECIPES):
@@ -551,7 +594,9 @@ def main(argv):
             print(n, "->", os.path.relpath(p, ROOT), os.path.getsize(p), "bytes", flush=True)
         return
     for n in names:
-        if alsd_only:
+        if heat_only:
+            p = add_heat(n)
+        elif alsd_only:
             p = add_alsd(n)
         elif ge_only:
             p = add_ge(n)

@@ -347,7 +347,7 @@ def test_mdinp_refuses_what_it_does_not_run(tmp_path):
         "CELSIZ= -1.0": inp.replace("QUIT\n\nEXE> MD", "      CELSIZ= -1.0\nQUIT\n\nEXE> MD"),
         "STOPCE= YES": inp.replace("STOPCE= NO", "STOPCE= BOTH").replace("BOTH", "YES"),
         "STOPCE= TRAN": inp.replace("STOPCE= NO", "STOPCE= TRAN"),
-        "HEATLO= 1000": md_line(inp, "HEATLO", "1000"),
+        "HEATLO= -5": md_line(inp, "HEATLO", "-5"),
         "SHAKEM= ALLB": md_line(inp, "SHAKEM", "ALLB"),
         "PSETMP= 300.0": md_line(inp, "PSETMP", "300.0"),
         "cannot read the value of LAMBDA": md_line(inp, "LAMBDA", "lam.dat"),
@@ -545,3 +545,25 @@ def test_mdinp_cmm_alsd_like_md_run(golden, tmp_path):
     assert np.abs(o["v"] - f["alsd_a_v"]).max() < 1e-9
     assert abs(o["lam"] - float(f["alsd_a_lam"])) < 1e-11
     assert abs(o["lam_v"] - float(f["alsd_a_lamv"])) < 1e-11
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("tag", ["a", "b"])
+def test_mdinp_heating_like_md_run(golden, tmp_path, tag):
+    """METHOD= CANO with HEATLO= 10 and STARTT= 100 from md_run's inputs
+    (restart at loop 0 or 4): the restart the host writes holds md_run's x
+    and v within 1e-9."""
+    s, f = golden("water_1500")
+    loop0, n, tempco = (int(q) for q in f[f"heat_{tag}_opt"])
+    print(f"case host heating {tag}", flush=True)
+    d = str(tmp_path)
+    _tpl_case(d, s, {"METHOD": "CANO", "LOOPLI": str(n), "SETTEM": "300.0", "STARTT": "100.0", "HEATLO": "10",
+                     "TEMPCO": "YES" if tempco else "NO"})
+    refio.write_restart(os.path.join(d, "in.rst"), s.x, s.v, loop=loop0, lam=s.lam)
+    r = subprocess.run([_host(), "--mdinp", "md.inp"], capture_output=True, text=True, timeout=240, cwd=d)
+    print(r.stdout, flush=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    o = refio.read_restart(os.path.join(d, "out.rst"))
+    assert o["loop"] == loop0 + n
+    assert np.abs(o["x"] - f[f"heat_{tag}_x"]).max() < 1e-9
+    assert np.abs(o["v"] - f[f"heat_{tag}_v"]).max() < 1e-9

@@ -288,3 +288,30 @@ def test_ge_requires_one_class_and_canonical(golden):
     with pytest.raises(native.JpError):
         md.set_mcmd(ge)
     md.close()
+
+
+@pytest.mark.parametrize("name", ["water_1500", "chain_lmd"])
+@pytest.mark.parametrize("tag", ["a", "b"])
+def test_heating_loop_vs_reference(golden, name, tag):
+    """md_run's heating loop (HEATLO= 10, STARTT= 100 K to SETTEM= 300 K)
+    from restarts at loop 0 and 4: the ramp's steps as host launches, the
+    rest as graphs; x and v within 1e-9 of md_run's."""
+    s, f = golden(name)
+    loop0, n, tempco = (int(q) for q in f[f"heat_{tag}_opt"])
+    print(f"case heating {name} {tag}", flush=True)
+    md = engine.MdSystem(s, precision=engine.FP64)
+    md.set_state(s.x, s.v)
+    md.set_canonical(300.0, tempco=bool(tempco))
+    md.set_heating(10, 100.0)
+    md.set_step(loop0)
+    assert md.celprp() == 0
+    _, ier = md.restart_step()
+    assert ier == 0
+    _, ier = md.runmd(n, update=10)
+    assert ier == 0
+    assert md.get_step() == loop0 + n
+    x, v = md.get_state()
+    md.close()
+    dx, dv = np.abs(x - f[f"heat_{tag}_x"]).max(), np.abs(v - f[f"heat_{tag}_v"]).max()
+    print(f"max |dx| {dx:.2e} |dv| {dv:.2e}", flush=True)
+    assert dx < 1e-9 and dv < 1e-9

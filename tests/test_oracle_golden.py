@@ -289,3 +289,24 @@ def test_alsd_monitored_temperature(golden):
     assert abs(temp(mw2 + lke, dof + 1.0) - row[3]) <= 6e-8
     assert abs(temp(mw2, dof + 1.0) - row[3]) > 1e-6
     assert abs(temp(mw2 + lke, dof) - row[3]) > 1e-3
+
+
+@pytest.mark.parametrize("name", ["water_1500", "chain_lmd"])
+@pytest.mark.parametrize("tag", ["a", "b"])
+def test_heating_loop_bit_exact(golden, name, tag):
+    """md_run's heating loop (HEATLO= 10, STARTT= 100 K, SETTEM= 300 K): the
+    target temperature of step idloop <= 10 is 100 + (200/10) idloop, the
+    restart set-up step taking the restart's loop count (0 or 4); the final
+    x and v bit-exact to md_run's, and a run without the ramp differs."""
+    s, f = golden(name)
+    loop0, n, tempco = (int(q) for q in f[f"heat_{tag}_opt"])
+    grp = s.temperature_groups() if tempco else None
+    x, v = s.x.copy(), s.v.copy()
+    ier, _ = Oracle(s).run_cano(x, v, n, 10, 300.0, grp=grp, heat=(loop0, 10, 100.0))
+    assert ier == 0
+    assert int(f[f"heat_{tag}_loop"]) == loop0 + n
+    assert np.array_equal(x, f[f"heat_{tag}_x"]), np.abs(x - f[f"heat_{tag}_x"]).max()
+    assert np.array_equal(v, f[f"heat_{tag}_v"])
+    x2, v2 = s.x.copy(), s.v.copy()
+    assert Oracle(s).run_cano(x2, v2, n, 10, 300.0, grp=grp)[0] == 0
+    assert np.abs(x2 - x).max() > 1e-6
