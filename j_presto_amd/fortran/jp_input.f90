@@ -32,6 +32,7 @@ module jp_input
      integer :: n = 0, ntyp = 0, nexcl = 0, nexcess = 0, nshake = 0, nbond = 0, nangle = 0, ntors = 0, n14 = 0
      real(8), allocatable :: mass(:), chg(:), rmin2(:), eps(:), s14e(:), s14v(:)
      integer(c_int32_t), allocatable :: atype(:), kind(:)
+     integer(c_int32_t), allocatable :: cend(:)   ! last atom of every chain (molecule copy), md_run's ixcend
      integer(c_int32_t), allocatable :: excl(:,:), excess(:,:)
      integer(c_int32_t), allocatable :: snat(:), satm(:,:)
      real(8), allocatable :: sdist(:,:)
@@ -343,7 +344,7 @@ contains
     type(top_t), intent(inout) :: top
     real(8), intent(in) :: diel, tr2(:), te(:), ts(:), tv(:)
     integer, intent(inout) :: ier
-    integer :: im, c, off, a, k, n, nb, na, nt, ns, nx, ne, g, sz, q, i1, i4
+    integer :: im, c, off, a, k, n, nb, na, nt, ns, nx, ne, g, sz, q, i1, i4, nch
     integer, allocatable :: xp(:,:), gorder(:), nbf(:)
     logical, allocatable :: offb(:), offa(:)
     integer :: np14
@@ -361,6 +362,9 @@ contains
     end do
     top%n = n
     allocate(top%mass(n), top%chg(n), top%atype(n), top%kind(n))
+    allocate(top%cend(max(sum(mols(:)%ncopy), 1)))
+    top%cend = 0
+    nch = 0
     allocate(xp(2, max(nx, 1)), bt(2, max(nb, 1)), bfc(max(nb, 1)), br0(max(nb, 1)))
     allocate(at(3, max(na, 1)), afc(max(na, 1)), ath(max(na, 1)))
     allocate(top%tatm(4, max(nt, 1)), top%tfc(max(nt, 1)), top%trot(max(nt, 1)), top%tph(max(nt, 1)), &
@@ -402,6 +406,8 @@ contains
                top%sdist(:, ns) = m%sd(:, k) * m%sd(:, k)
             end do
             off = off + m%na
+            nch = nch + 1                 ! inptpl.f90:279, 680: ixcend(chain) = atoms so far
+            top%cend(nch) = off
          end do
        end associate
     end do

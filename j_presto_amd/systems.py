@@ -147,6 +147,16 @@ class System:
         g = (kind - 1) * self.nfrag + self.cluster
         return np.unique(g, return_inverse=True)[1].astype(np.int32)
 
+    def chain_end(self, nstpcn: int = 1) -> int:
+        """Atoms of the first nstpcn chains (molecule copies): md_run's
+        ixcend(nstpcn) (inptpl.f90:279, 680), the atoms STOPCE acts on."""
+        ends = np.cumsum([m.natom for m in self.mols for _ in range(m.ncopy)])
+        return int(ends[nstpcn - 1])
+
+    def first_kind_atoms(self) -> int:
+        """Atoms from atom 1 of atom 1's molecule kind (md.f90:187-193)."""
+        return int(self.mols[0].natom * self.mols[0].ncopy)
+
     @property
     def celwal(self) -> np.ndarray:
         return np.array([0.0, self.box[0], 0.0, self.box[1], 0.0, self.box[2]])

@@ -92,6 +92,9 @@ typedef struct {
     int n14, *p14, *p14cls;
     double *p14v, *p14e;
     struct jo_cmm *cmm;   /* CAL15M= CMM state (jo_cmm_setup), else NULL */
+    /* STOPCE (idfstp 1 TRAN, 2 ROTA, 3 BOTH; 0 none) over atoms [0, stop_n):
+     * md_run's first nstpcn chains, stop_n = ixcend(nstpcn) */
+    int stop_mode, stop_n;
 } jo_ctx;
 
 static void jo_cmm_free(struct jo_cmm *m);
@@ -1078,6 +1081,83 @@ int jo_shake(const jo_ctx *c, double *r1, const double *r2) {
 
 /* ------------------------------------------------------------ integrator */
 
+#define JO_STOP_MASS (-1)
+#define JO_STOP_SING (-3)
+
+/* STOPCE= TRAN/ROTA/BOTH (inpmd.f90:124, idfstp 1/2/3) over the first
+ * stop_n atoms (md_run's ixcend(NSTPCN), NSTPCN = 1 by default).  mode 0
+ * turns it off. */
+int jo_set_stopce(jo_ctx *c, int mode, int natom) {
+    if (mode < 0 || mode > 3 || (mode && (natom < 1 || natom > c->n))) return JP_EINVAL;
+    c->stop_mode = mode;
+    c->stop_n = mode ? natom : 0;
+    return 0;
+}
+
+/* remotv.f90:2-131: the centre-of-mass translation (mode 1, 3) and rotation
+ * (mode 2, 3) of atoms [0, stop_n) taken out of v, both about the centre of
+ * mass of the coordinates xp.  Sums in atom order as md_run's loops; the
+ * inertia tensor's inverse by cofactors.  Returns JO_STOP_MASS for a total
+ * mass <= 0 (md_run's ier -1) and JO_STOP_SING for a singular inertia
+ * tensor (md_run's -2, which its SHAKE uses too: -3 here), v untouched. */
+int jo_remotv(const jo_ctx *c, const double *xp, double *v) {
+    const int n = c->stop_n, mode = c->stop_mode;
+    const double *m = c->mass;
+    double tot = 0.0, cen[3] = {0.0, 0.0, 0.0}, tv[3] = {0.0, 0.0, 0.0}, av[3] = {0.0, 0.0, 0.0};
+    for (int i = 0; i < n; ++i) tot = tot + m[i];
+    if (tot <= 0.0) return JO_STOP_MASS;
+    tot = 1.0 / tot;
+    for (int i = 0; i < n; ++i)
+        for (int k = 0; k < 3; ++k) {
+            cen[k] = cen[k] + m[i] * xp[3 * i + k];
+            if (mode == 1 || mode == 3) tv[k] = tv[k] + m[i] * v[3 * i + k];
+        }
+    for (int k = 0; k < 3; ++k) {
+        cen[k] = cen[k] * tot;
+        tv[k] = tv[k] * tot;
+    }
+    if (mode == 2 || mode == 3) {
+        double ixx = 0.0, iyy = 0.0, izz = 0.0, ixy = 0.0, iyz = 0.0, izx = 0.0;
+        for (int i = 0; i < n; ++i) {
+            const double dx = xp[3 * i] - cen[0], dy = xp[3 * i + 1] - cen[1], dz = xp[3 * i + 2] - cen[2];
+            ixx = ixx + m[i] * (dy * dy + dz * dz);
+            iyy = iyy + m[i] * (dz * dz + dx * dx);
+            izz = izz + m[i] * (dx * dx + dy * dy);
+            ixy = ixy - m[i] * dx * dy;
+            iyz = iyz - m[i] * dy * dz;
+            izx = izx - m[i] * dz * dx;
+        }
+        const double ixy2 = ixy * ixy, iyz2 = iyz * iyz, izx2 = izx * izx;
+        double det = ixx * iyy * izz + 2.0 * ixy * iyz * izx - iyz2 * ixx - izx2 * iyy - ixy2 * izz;
+        if (det == 0.0) return JO_STOP_SING;
+        det = 1.0 / det;
+        double im[3][3];
+        im[0][0] = (iyy * izz - iyz2) * det;
+        im[0][1] = (iyz * izx - izz * ixy) * det;
+        im[0][2] = (ixy * iyz - izx * iyy) * det;
+        im[1][0] = im[0][1];
+        im[1][1] = (izz * ixx - izx2) * det;
+        im[1][2] = (izx * ixy - ixx * iyz) * det;
+        im[2][0] = im[0][2];
+        im[2][1] = im[1][2];
+        im[2][2] = (ixx * iyy - ixy2) * det;
+        for (int i = 0; i < n; ++i) {
+            const double dx = xp[3 * i] - cen[0], dy = xp[3 * i + 1] - cen[1], dz = xp[3 * i + 2] - cen[2];
+            const double lx = dy * v[3 * i + 2] - dz * v[3 * i + 1];
+            const double ly = dz * v[3 * i] - dx * v[3 * i + 2];
+            const double lz = dx * v[3 * i + 1] - dy * v[3 * i];
+            for (int k = 0; k < 3; ++k) av[k] = av[k] + m[i] * (im[k][0] * lx + im[k][1] * ly + im[k][2] * lz);
+        }
+    }
+    for (int i = 0; i < n; ++i) {
+        const double dx = xp[3 * i] - cen[0], dy = xp[3 * i + 1] - cen[1], dz = xp[3 * i + 2] - cen[2];
+        v[3 * i] = v[3 * i] - av[1] * dz + av[2] * dy - tv[0];
+        v[3 * i + 1] = v[3 * i + 1] - av[2] * dx + av[0] * dz - tv[1];
+        v[3 * i + 2] = v[3 * i + 2] - av[0] * dy + av[1] * dx - tv[2];
+    }
+    return 0;
+}
+
 /* verlet.f90:144-215 (METHOD= MICR): gtot is the total gradient (already
  * lambda-combined); v is v(t-dt/2) on entry, v(t+dt/2) on exit. */
 int jo_verlet_step(const jo_ctx *c, double *x, double *v, const double *gtot, double *xold) {
@@ -1086,6 +1166,10 @@ int jo_verlet_step(const jo_ctx *c, double *x, double *v, const double *gtot, do
     memcpy(xold, x, sizeof(double) * 3 * n);
     for (int i = 0; i < n; ++i)
         for (int k = 0; k < 3; ++k) v[3 * i + k] = v[3 * i + k] + rtmp * gtot[3 * i + k] / c->mass[i];
+    if (c->stop_mode) { /* verlet.f90:156-169: STOPCE about x(t) */
+        const int ier = jo_remotv(c, xold, v);
+        if (ier) return ier;
+    }
     for (int i = 0; i < n; ++i)
         for (int k = 0; k < 3; ++k) x[3 * i + k] = x[3 * i + k] + c->dt * v[3 * i + k];
     if (c->nshk) {
@@ -1141,7 +1225,7 @@ static void group_temps(const jo_ctx *c, const double *w, int ngrp, const int *g
 }
 
 /* verlethe.f90:79-757 for METHOD= CANO (idfvsl = 3) with constant lambda
- * and STOPCE= NO: itmeth 1 (grp == NULL, one temperature) or 2
+ * (STOPCE as jo_set_stopce gave it): itmeth 1 (grp == NULL, one temperature) or 2
  * (grp[i] = 0-based temperature group of atom i, md.f90:136-172).  gtot is
  * the lambda-combined gradient; v is v(t-dt/2) on entry, v(t+dt/2) on exit;
  * vback receives v(t-dt/2) and xold x(t).  Velocity scaling toward temps
@@ -1207,6 +1291,10 @@ static int cano_step_l(jo_ctx *c, double *x, double *v, const double *gtot, doub
             *lv = *lv + dt * conv * lf * ilm;
         }
     }
+    if (c->stop_mode && (ier = jo_remotv(c, xold, v))) { /* verlethe.f90:489-502: STOPCE about x(t) */
+        free(im); free(w); free(fc); free(xb);
+        return ier;
+    }
     for (int i = 0; i < 3 * n; ++i) x[i] = x[i] + dt * v[i];
     if (al) c->lambda = c->lambda + dt * *lv;
     if (c->nshk) {
@@ -1243,6 +1331,7 @@ static int cano_step_l(jo_ctx *c, double *x, double *v, const double *gtot, doub
                     v[q] = (-conv * gtot[q] * im[i] + fc[q]) * rt + vback[q] * rt2;
                 }
             }
+            if (c->stop_mode && (ier = jo_remotv(c, xold, v))) break; /* verlethe.f90:705-718 */
             for (int i = 0; i < 3 * n; ++i) x[i] = xold[i] + dt * v[i];
             if (al) c->lambda = lb + dt * *lv;   /* lambda's velocity is not touched by the rounds */
         }
@@ -1261,6 +1350,28 @@ void jo_group_dof(const jo_ctx *c, int ngrp, const int *grp, double *dof) {
         const int h = c->shk_n[g] - 1;
         const int a = c->shk_a[4 * g];
         dof[grp ? grp[a] : 0] -= (double)((h + 1) * h / 2);
+    }
+}
+
+/* STOPCE's degrees of freedom (md.f90:114-121 for one group, 183-215 per
+ * group): 3 (TRAN, ROTA) or 6 (BOTH) come off group 0, or, when atom 1's
+ * molecule kind (its first nkind atoms, md_run's ixamol(1)) spans groups 0 to
+ * ifrg - 1 with ifrg >= 2, off each of those in proportion to its degrees of
+ * freedom. */
+void jo_stop_dof(const jo_ctx *c, int ngrp, const int *grp, int nkind, double *dof) {
+    if (!c->stop_mode) return;
+    const double k = c->stop_mode == 3 ? 6.0 : 3.0;
+    int ifrg = 1;
+    if (grp)
+        for (int i = 0; i < nkind && i < c->n; ++i)
+            if (grp[i] + 1 > ifrg) ifrg = grp[i] + 1;
+    if (ifrg >= 2 && ifrg <= ngrp && ifrg <= 64) {
+        double ratio[64], sum = 0.0;
+        for (int g = 0; g < ifrg; ++g) sum = sum + dof[g];
+        for (int g = 0; g < ifrg; ++g) ratio[g] = dof[g] / sum;
+        for (int g = 0; g < ifrg; ++g) dof[g] = dof[g] - k * ratio[g];
+    } else {
+        dof[0] = dof[0] - k;
     }
 }
 
@@ -1432,6 +1543,7 @@ static int cano_verlethe(jo_ctx *c, double *x, double *v, double *vback, double 
         double lf = 0.0;
         if (al) lf = jo_alsd_force(al, c->lambda, class_epot(eterm, ee, ev, 0), class_epot(eterm, ee, ev, 1));
         ier = cano_step_l(c, x, v, gt, xo, vback, temps, ttgt, icslop, fcstol, ngrp, grp, dof, al, lf, lv);
+        if (ier == JO_STOP_MASS || ier == JO_STOP_SING) break;   /* remotv's errors end the run (verlethe.f90:493-501) */
         if (ier) {
             memcpy(x, xs, sizeof(double) * 3 * n);
             memcpy(v, vs, sizeof(double) * 3 * n);
@@ -2269,6 +2381,42 @@ int jo_cmm_nonbond(jo_ctx *c, const double *x, double *grad, double *eelec, doub
  * step; after every update-th step celprp; after every cmmupd-th step the
  * next step recomputes the far field (md.f90:679-684).  The first step
  * computes it (incel sets itwin = 1).  Returns ier. */
+/* md_run's initial step of a run without a restart (RESTAR= NO: inista,
+ * md.f90:1056-1065, staver.f90:27-207): the list of x, the forces at x
+ * (bonded, then ZD or, under CAL15M= CMM, the cell multipole method with a
+ * fresh far field), the lambda-combined gradient (no METHOD= GE weight, no
+ * lambda step), a half kick v + (-dt c / 2) g (1/m), STOPCE about x, the
+ * drift and SHAKE of the coordinates; v keeps the unconstrained half-kick
+ * velocity (staver does not form (x - x_old)/dt).  Returns ier. */
+int jo_start_step(jo_ctx *c, double *x, double *v) {
+    const int n = c->n, nfrg = c->nfrg;
+    double *grad = (double *)calloc(3 * (size_t)n * nfrg, sizeof(double));
+    double *gt = (double *)malloc(sizeof(double) * 3 * (size_t)n);
+    double *xo = (double *)malloc(sizeof(double) * 3 * (size_t)n);
+    double ee[3], ev[3], eterm[18];
+    int ier = c->have_list ? 0 : (c->cmm ? jo_cmm_celprp(c, x) : jo_celprp(c, x));
+    if (!ier) {
+        memcpy(xo, x, sizeof(double) * 3 * (size_t)n);
+        jo_bonded(c, x, grad, eterm);
+        ier = c->cmm ? jo_cmm_nonbond(c, x, grad, ee, ev, 1) : jo_nonbond(c, x, grad, ee, ev);
+    }
+    if (!ier) {
+        jo_combine_grad(c, grad, gt);
+        const double rtmp = -0.5 * c->dt * 4.184e-4; /* md.f90:56 fdcvcv */
+        for (int i = 0; i < n; ++i) {
+            const double im = 1.0 / c->mass[i]; /* md.f90:68 ifxmass */
+            for (int k = 0; k < 3; ++k) v[3 * i + k] = v[3 * i + k] + rtmp * gt[3 * i + k] * im;
+        }
+        if (c->stop_mode) ier = jo_remotv(c, xo, v); /* staver.f90:165-178 */
+    }
+    if (!ier) {
+        for (int q = 0; q < 3 * n; ++q) x[q] = x[q] + c->dt * v[q];
+        if (c->nshk) ier = jo_shake(c, x, xo);
+    }
+    free(grad); free(gt); free(xo);
+    return ier;
+}
+
 int jo_cmm_run_nve(jo_ctx *c, double *x, double *v, int nsteps, int update, int cmmupd) {
     if (!c->cmm || update <= 0 || cmmupd <= 0 || update % cmmupd) return JP_EINVAL;
     const int n = c->n, nfrg = c->nfrg;

@@ -66,6 +66,10 @@ def lib():
         L.jo_cmm_celprp.argtypes = [C.c_void_p, f64p]
         L.jo_cmm_nonbond.argtypes = [C.c_void_p, f64p, f64p, f64p, f64p, C.c_int]
         L.jo_cmm_run_nve.argtypes = [C.c_void_p, f64p, f64p, C.c_int, C.c_int, C.c_int]
+        L.jo_set_stopce.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        L.jo_start_step.argtypes = [C.c_void_p, f64p, f64p]
+        L.jo_remotv.argtypes = [C.c_void_p, f64p, f64p]
+        L.jo_stop_dof.argtypes = [C.c_void_p, C.c_int, i32p, C.c_int, f64p]
         L.jo_lambda.argtypes = [C.c_void_p]
         L.jo_lambda.restype = C.c_double
         L.jo_set_lambda.argtypes = [C.c_void_p, C.c_double]
@@ -142,6 +146,7 @@ class Oracle:
         self.nfrg = self.h.nfrg
         self.ntyp = len(system.types)
         self.c = lib().jo_create(C.byref(self.h.desc))
+        self.stop_kind = 0
 
     def __del__(self):
         try:
@@ -208,6 +213,7 @@ class Oracle:
         dof = np.zeros(ng)
         g = None if grp is None else np.ascontiguousarray(grp, np.int32)
         lib().jo_group_dof(self.c, ng, None if g is None else _p(g, i32p), _p(dof))
+        lib().jo_stop_dof(self.c, ng, None if g is None else _p(g, i32p), self.stop_kind, _p(dof))
         return dof
 
     def cano_step(self, x, v, gtot, temps: float, icslop: int = 3, fcstol: float = 1e-3, grp=None):
@@ -309,6 +315,24 @@ class Oracle:
 
     def cmm_run_nve(self, x, v, nsteps: int, update: int = 10, cmmupd: int = 5) -> int:
         return lib().jo_cmm_run_nve(self.c, _p(x), _p(v), nsteps, update, cmmupd)
+
+    STOPCE = {"NO": 0, "TRAN": 1, "ROTA": 2, "BOTH": 3}
+
+    def set_stopce(self, mode: str, natom: int, nkind: int = 0) -> int:
+        """STOPCE= TRAN/ROTA/BOTH/NO over the first natom atoms (md_run's
+        ixcend(NSTPCN)) for every later step; nkind: atoms from atom 1 of its
+        molecule kind (the degrees of freedom under TEMPCO= YES)."""
+        self.stop_kind = int(nkind)
+        return lib().jo_set_stopce(self.c, self.STOPCE[mode], int(natom))
+
+    def remotv(self, xp, v) -> int:
+        """remotv alone: the stopped motions taken out of v in place."""
+        return lib().jo_remotv(self.c, _p(np.ascontiguousarray(xp, np.float64)), _p(v))
+
+    def start_step(self, x, v) -> int:
+        """md_run's initial step of a run without a restart (staver), in
+        place."""
+        return lib().jo_start_step(self.c, _p(x), _p(v))
 
     def list_brute(self, x):
         x = np.ascontiguousarray(x, np.float64)

@@ -28,6 +28,9 @@ The reference itself never leaves this container.
 
   heat jobs  (--heat-only: HEAT_RUNS) METHOD= CANO through md_run's heating loop
              (HEATLO= 10, STARTT= 100), from restarts at loop 0 and 4.
+  stop jobs  (--stop-only name ...: STOP_RUNS) STOPCE= TRAN, ROTA and BOTH under
+             METHOD= MICR and CANO, and runs without a restart (RESTAR= NO,
+             INITIA= ZERO: md_run's initial half step, staver).
   cmm jobs   (--cmm: CMM_RECIPES) CAL15M= CMM, BOUNDA= NO on droplets: the
              FDEBUG= MAKE dump and a MICR run with NAMETO.
   alsd jobs  (chain_lmd, --alsd-only) RESTAR= YES, METHOD= GE with GEPSME= LMD
@@ -35,7 +38,7 @@ The reference itself never leaves this container.
              under -2 E1 lambda - E2 - dew(lambda) beside the atoms.
 
 Usage: python tests/golden/make_golden.py [--cano-only | --mon-only | --host-only | --ge-only | --alsd-only |
-       --heat-only | --cmm]
+       --heat-only | --stop-only | --cmm]
        [name ...]
 """
 from __future__ import annotations
@@ -473,6 +476,59 @@ def heat_jobs(s: systems.System, tmp: str) -> dict:
     return out
 
 
+# Centre-of-mass stopping (STOPCE= TRAN/ROTA/BOTH of the first chain,
+# NSTPCN= 1, remotv.f90) and md_run's initial step of a run without a
+# restart (RESTAR= NO, INITIA= ZERO: v = 0, then staver's half kick):
+# tag -> (METHOD, STOPCE, RESTAR, steps, TEMPCO).  md_run's remotv leaves
+# the angular velocity unset under STOPCE= TRAN (remotv.f90:54-69, 122-127
+# read avelcn, set only for ROTA and BOTH at 73-116), so TRAN runs of md_run
+# are not fixtures.
+STOP_RUNS = {"mb": ("MICR", "BOTH", "YES", 12, "NO"), "mr": ("MICR", "ROTA", "YES", 12, "NO"),
+             "cb": ("CANO", "BOTH", "YES", 12, "NO"), "cy": ("CANO", "ROTA", "YES", 12, "YES"),
+             "s0": ("MICR", "NO", "NO", 12, "NO"), "sc": ("CANO", "BOTH", "NO", 12, "NO")}
+
+
+def stop_jobs(s: systems.System, tmp: str, extra: dict = None) -> dict:
+    """md_run runs of STOP_RUNS (extra: more EXE> MD keywords, e.g. CMM)."""
+    out = {}
+    for tag, (method, stopce, restar, n, tempco) in STOP_RUNS.items():
+        d = os.path.join(tmp, "stop_" + tag)
+        os.makedirs(d)
+        md = dict(extra or {}, METHOD=method, RESTAR=restar, NAMERO="out.rst", LOOPLI=str(n), UPDATE="10",
+                  TIMEST="2.0", STOPCE=stopce, OUTLOG="100000", OUTCLE="100000")
+        if restar == "YES":
+            refio.write_restart(os.path.join(d, "in.rst"), s.x, s.v, lam=s.lam)
+            md["NAMERI"] = "in.rst"
+        else:
+            md["INITIA"] = "ZERO"
+            if s.lam:   # no restart to take lambda from (inpmd.f90:290-297)
+                md["LAMBDA"] = repr(float(s.lam))
+        if method == "CANO":
+            md.update(SETTEM="300.0", TEMPCO=tempco)
+        run_md(d, s, md)
+        r = refio.read_restart(os.path.join(d, "out.rst"))
+        out.update({f"stop_{tag}_x": r["x"], f"stop_{tag}_v": r["v"], f"stop_{tag}_loop": np.int64(r["loop"])})
+    return out
+
+
+def add_stop(name: str) -> str:
+    """Adds the STOPCE and RESTAR= NO runs to an existing fixture."""
+    cmm = name in CMM_RECIPES
+    kind, args, opt = (CMM_RECIPES if cmm else RECIPES)[name]
+    s = build_system(kind, args)
+    path = os.path.join(HERE, f"{name}.npz")
+    with np.load(path, allow_pickle=False) as z:
+        out = {k: z[k] for k in z.files}
+    assert str(out["fingerprint"]) == fingerprint(s)
+    tmp = tempfile.mkdtemp(prefix="jpgold_")
+    try:
+        out.update(stop_jobs(s, tmp, {"CAL15M": "CMM", "BOUNDA": "NO"} if cmm else None))
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    np.savez_compressed(path, **out)
+    return path
+
+
 def add_heat(name: str) -> str:
     """Adds the heating-loop runs to an existing fixture."""
     kind, args, opt = RECIPES[name]
@@ -584,9 +640,15 @@ def main(argv):
     ge_only = "--ge-only" in argv
     alsd_only = "--alsd-only" in argv
     heat_only = "--heat-only" in argv
+    stop_only = "--stop-only" in argv
     cmm_all = "--cmm" in argv
     argv = [a for a in argv if a not in ("--cano-only", "--mon-only", "--host-only", "--ge-only", "--alsd-only",
-                                         "--cmm", "--heat-only")]
+                                         "--cmm", "--heat-only", "--stop-only")]
+    if stop_only:
+        for n in argv:
+            p = add_stop(n)
+            print(n, "->", os.path.relpath(p, ROOT), os.path.getsize(p), "bytes", flush=True)
+        return
     names = argv or [n for n in RECIPES if not mon_only or RECIPES[n][2].get("mon")]
     if cmm_all or any(n in CMM_RECIPES for n in names):
         for n in [n for n in names if n in CMM_RECIPES] or list(CMM_RECIPES):
