@@ -32,6 +32,8 @@ extern "C" {
 #define JP_IER_EXCL 99        /* cutoff.f90:90-96 */
 #define JP_IER_SHAKE_CONV 1   /* shake.f90:286-319 */
 #define JP_IER_SHAKE_SING (-2)/* shake.f90:266-284 */
+#define JP_IER_STOP_MASS (-1)  /* remotv.f90:47-50: the stopped atoms have no mass */
+#define JP_IER_STOP_SING (-3)  /* remotv.f90:86-88: md_run's ier is -2, as SHAKE's singular case */
 #define JP_EINVAL (-101)
 #define JP_ENOMEM (-102)
 #define JP_ENODEV (-103)
@@ -252,6 +254,42 @@ int jp_set_canonical(jp_ctx *ctx, int32_t on, double temps, int32_t icslop, doub
  * default) switches it off.  Returns JP_EINVAL for heatlo < 0 or startt < 0. */
 int jp_set_heating(jp_ctx *ctx, int32_t heatlo, double startt);
 
+/* Centre-of-mass stopping (md_run STOPCE= TRAN, ROTA, BOTH; remotv.f90:2-131)
+ * for the steps that follow: mode JP_STOP_TRAN takes the first natom atoms'
+ * centre-of-mass velocity out of their velocities, JP_STOP_ROTA their
+ * rotation about the centre of mass, JP_STOP_BOTH both, JP_STOP_NO nothing;
+ * natom is md_run's ixcend(NSTPCN), the atoms of the first NSTPCN chains
+ * (molecule copies).  It acts where md_run calls remotv: in METHOD= MICR
+ * between the kick and the drift (verlet.f90:156-169), in the canonical
+ * step after the velocity update of every round (verlethe.f90:489-502,
+ * 705-718), and in jp_start_step, always about the coordinates at the start
+ * of the step.  jp_set_canonical, called after it, takes md_run's 3 (TRAN,
+ * ROTA) or 6 (BOTH) degrees of freedom off (md.f90:114-121; under
+ * temperature groups, md.f90:183-215, off the groups whose labels the first
+ * nkind atoms carry, nkind being the atoms from atom 1 of atom 1's molecule
+ * kind, in proportion to their degrees of freedom when there are several).
+ * A run whose stopped atoms have no mass ends with JP_IER_STOP_MASS, one
+ * whose inertia tensor is singular with JP_IER_STOP_SING.  md_run's own
+ * STOPCE= TRAN reads an unset angular velocity (remotv.f90:122-127; the
+ * angular velocity is set only for ROTA and BOTH at 106-116); here it is
+ * zero.  Returns JP_EINVAL for another mode, natom outside 1..N, nkind
+ * outside 0..N, or a call after jp_set_canonical. */
+#define JP_STOP_NO 0
+#define JP_STOP_TRAN 1
+#define JP_STOP_ROTA 2
+#define JP_STOP_BOTH 3
+int jp_set_stopce(jp_ctx *ctx, int32_t mode, int32_t natom, int32_t nkind);
+
+/* md_run's initial step of a run without a restart (RESTAR= NO; inista,
+ * md.f90:1056-1065, staver.f90:27-207), from the context's coordinates and
+ * velocities (zero under INITIA= ZERO): the pair list of x, the forces at x
+ * (the lambda-combined gradient, without the METHOD= GE weight; lambda does
+ * not move), v += (-dt c / 2) g / m, STOPCE (jp_set_stopce) about x,
+ * x += dt v and SHAKE of x; v keeps the unconstrained half-kick velocity, as
+ * md_run's does.  The step count stays (0 for md_run's fresh start), and
+ * the first jp_run_nve step follows.  Returns ier (SHAKE's, or remotv's). */
+int jp_start_step(jp_ctx *ctx, int32_t precision);
+
 /* Multicanonical MD (md_run METHOD= GE, idfvsl = 4, verlethe.f90:196-233) for
  * the following jp_run_nve calls, on top of jp_set_canonical's temperature
  * control: every step's gradient is multiplied by the weight
@@ -450,10 +488,15 @@ int jp_batch_get_option(jp_batch *b, int32_t option, int32_t *value);
 
 /* jp_set_canonical for every system of a batch (group: labels for all atoms
  * of the batch in order, or NULL). */
-/* jp_set_heating for every system of a batch. */
-int jp_batch_set_heating(jp_batch *b, int32_t heatlo, double startt);
 int jp_batch_set_canonical(jp_batch *b, int32_t on, double temps, int32_t icslop, double fcstol,
                            const int32_t *group);
+/* jp_set_heating for every system of a batch. */
+int jp_batch_set_heating(jp_batch *b, int32_t heatlo, double startt);
+/* jp_set_stopce for every system of a batch: natom[s] and nkind[s] (nkind
+ * may be NULL) per system in batch order. */
+int jp_batch_set_stopce(jp_batch *b, int32_t mode, const int32_t *natom, const int32_t *nkind);
+/* jp_start_step for every system of a batch. */
+int jp_batch_start_step(jp_batch *b, int32_t precision);
 /* jp_set_alsd, jp_get_lambda and jp_set_lambda for the systems of a batch
  * (one parameter set; lambda, its velocity and, with lambda_m <= 0, its
  * mass per system). */

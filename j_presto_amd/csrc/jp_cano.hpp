@@ -1,5 +1,4 @@
-// Canonical MD (md_run METHOD= CANO, verlethe.f90:79-757 with idfvsl = 3,
-// constant lambda, STOPCE= NO, HEATLO= 0).
+// Canonical MD (md_run METHOD= CANO, verlethe.f90:79-757 with idfvsl = 3).
 //
 // One step after the forces:
 //   begin   v_b = v, x_o = x, f_c = 0, w = v - (dt/2) c g / m  -> m|w|^2
@@ -15,7 +14,10 @@
 //           -> m|w'|^2, so that the next round needs no kernel of its own
 //           to reduce them
 // Each round is one launch (k_cano_shake); a system without SHAKE groups
-// takes only round 1's drift.  The sorted coordinates follow x.
+// takes only round 1's drift.  The sorted coordinates follow x.  Under
+// STOPCE each round is two launches with the centre-of-mass motion taken
+// out of the updated velocities in between (k_remotv, verlethe.f90:489-502,
+// 705-718).
 // T_g = (sum over group g of m|w|^2) 1e7 / (dof_g gascst) (caltmp.f90), with
 // one group per system (TEMPCO= NO) or per molecule kind and cluster
 // (TEMPCO= YES, caltmp2).  c = 4.184e-4 (md.f90:56), 1/m is md_run's ifxmass.
@@ -233,11 +235,27 @@ __global__ void __launch_bounds__(256) k_cano_begin(CanoArgs A, const SysP *__re
     block_partials(val, gq, g0, g1, A.pa, A.nab);
 }
 
-__global__ void __launch_bounds__(256) k_cano_drift(CanoArgs A, const SysP *__restrict__ sp, JAtom *__restrict__ ys) {
+// phase 0: the velocity update and the drift; 1: the update alone (v
+// written); 2: the drift alone from v (the centre-of-mass motion taken out
+// in between, k_remotv).
+__global__ void __launch_bounds__(256) k_cano_drift(CanoArgs A, const SysP *__restrict__ sp, JAtom *__restrict__ ys,
+                                                    int phase) {
 #pragma clang fp contract(off)
     __shared__ double sT[JP_CANO_MAXG];
     const int2 B = A.ablk[blockIdx.x];
     const int a = B.x + threadIdx.x;
+    if (phase == 2) {
+        if (a >= B.y) return;
+        const int s = A.atom_sys[a];
+        const SysP &P = sp[s];
+        double xa[3];
+        for (int k = 0; k < 3; ++k) {
+            xa[k] = A.x[3 * a + k] + P.dt * A.v[3 * a + k];
+            A.x[3 * a + k] = xa[k];
+        }
+        if (!A.shk_on[s]) cano_sorted(P, A, a, xa, ys);
+        return;
+    }
     int g0, g1;
     block_groups(A, B.x, g0, g1);
     block_temps(A, A.pa, A.nab, A.ab0, A.ab1, g0, g1, sT, 3);
@@ -258,10 +276,11 @@ __global__ void __launch_bounds__(256) k_cano_drift(CanoArgs A, const SysP *__re
     for (int k = 0; k < 3; ++k) {
         const double vk = rt2 * A.v[3 * a + k] - rt * cano_grad(A, s, sl, k) * im;
         A.v[3 * a + k] = vk;
+        if (phase == 1) continue;
         xa[k] = A.x[3 * a + k] + dt * vk;
         A.x[3 * a + k] = xa[k];
     }
-    if (!A.shk_on[s]) cano_sorted(P, A, a, xa, ys);
+    if (phase == 0 && !A.shk_on[s]) cano_sorted(P, A, a, xa, ys);
 }
 
 // One round per unit (SHAKE group or free atom): round 1 first applies the
@@ -269,7 +288,10 @@ __global__ void __launch_bounds__(256) k_cano_drift(CanoArgs A, const SysP *__re
 // velocity update (verlethe.f90:396-404, 601-611, 646-657); then SHAKE as in
 // k_step.  A block holds units of one system, so the decisions below are
 // uniform over the block, and every block of a system reaches the same one
-// from the same fixed-order sums.
+// from the same fixed-order sums.  PH 0: the whole round; PH 1: up to the
+// velocity update (v and the constraint force written); PH 2: the rest from
+// v, the centre-of-mass motion taken out in between (k_remotv).
+template <int PH>
 __global__ void __launch_bounds__(128) k_cano_shake(CanoArgs A, const SysP *__restrict__ sp,
                                                     const int4 *__restrict__ unit, const double *__restrict__ s2d,
                                                     const double *__restrict__ s3d, double tol, int maxit,
@@ -312,10 +334,16 @@ __global__ void __launch_bounds__(128) k_cano_shake(CanoArgs A, const SysP *__re
             ag[q][k] = cano_grad(A, s, sl, k);
             afc[q][k] = A.fcc[qq];
             avb[q][k] = A.vback[qq];
-            avx[q][k] = round == 1 ? A.v[qq] : A.xcb[qq];
+            avx[q][k] = (round == 1 || PH == 2) ? A.v[qq] : A.xcb[qq];
         }
     }
-    if (round == 1) {
+    if (PH == 2) {
+        if (round > 1) {
+            if (threadIdx.x == 0) s_run = !A.done[s];
+            __syncthreads();
+            if (!s_run) return;   // uniform over the block
+        }
+    } else if (round == 1) {
         block_temps(A, A.pa, A.nab, A.ab0, A.ab1, g0, g1, sT, 3);
         if (A.lam && threadIdx.x == 0 && blockIdx.x == A.ub0[s]) {
             const int lg = A.lgrp[s];
@@ -355,7 +383,12 @@ __global__ void __launch_bounds__(128) k_cano_shake(CanoArgs A, const SysP *__re
         for (int q = 0; q < 3; ++q) {
             if (q >= kind) break;
             const double im = aim[q];
-            if (round == 1) {   // verlethe.f90:396-404, 509-516
+            if (PH == 2) {   // the updated velocity from PH 1 (and k_remotv)
+                for (int k = 0; k < 3; ++k) {
+                    vd[q][k] = avx[q][k];
+                    xn[q][k] = (round == 1 ? ax[q][k] : axo[q][k]) + dt * avx[q][k];
+                }
+            } else if (round == 1) {   // verlethe.f90:396-404, 509-516
                 const double T = shk ? sT[agid[q] - g0] : ts;
                 const double itc = 1.0 / T;
                 const double chi = sqrt(tt * itc);
@@ -378,13 +411,17 @@ __global__ void __launch_bounds__(128) k_cano_shake(CanoArgs A, const SysP *__re
                     xn[q][k] = axo[q][k] + dt * vk;
                 }
             }
+            if (PH == 1) {
+                for (int k = 0; k < 3; ++k) A.v[3 * a[q] + k] = vd[q][k];
+                continue;
+            }
             for (int k = 0; k < 3; ++k) {
                 xb[q][k] = xn[q][k];
                 xo[q][k] = axo[q][k];
                 if (shk) A.xcb[3 * a[q] + k] = xn[q][k];
             }
         }
-        if (shk) {
+        if (PH != 1 && shk) {
             if (kind == 3) {
                 const int e = shake3_regs(P, aim, T3, tol, maxit, xn, xo);
                 if (e) atomicOr(err, e);
@@ -395,7 +432,7 @@ __global__ void __launch_bounds__(128) k_cano_shake(CanoArgs A, const SysP *__re
         const double r = 1.0 / dt, r2 = 1.0 / (dt * dt), r3 = 0.5 * dt;
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
-            if (q >= kind) break;
+            if (PH == 1 || q >= kind) break;
             const int b = a[q];
             if (shk) {
                 double w[3], wn[3];
@@ -423,7 +460,7 @@ __global__ void __launch_bounds__(128) k_cano_shake(CanoArgs A, const SysP *__re
             j->z = xn[q][2] - P.L[2] * (double)aimg[q][2];
         }
     }
-    if (shk && round < A.icslop) {
+    if (PH != 1 && shk && round < A.icslop) {
         block_partials(val, gq, g0, g1, pu_out, A.nub);
         block_partials(valn, gq, g0, g1, pn_out, A.nub);
     }

@@ -551,6 +551,10 @@ __global__ void __launch_bounds__(128) k_shake3(int ng, const int *__restrict__ 
 // has constraints (verlet.f90:196-215), and the refreshed sorted coordinates.
 // The same operations in the same order as k_kick -> k_shake* -> k_post.
 // unit = {a0, a1, a2, kind | param << 2}, kind 1/2/3 = atoms in the unit.
+// KICK false: v already holds the kicked velocity (a step that stops the
+// centre-of-mass motion between kick and drift); VUPD false: v keeps it
+// after SHAKE (md_run's initial step, staver.f90:196-204).
+template <bool KICK, bool VUPD>
 __global__ void __launch_bounds__(128) k_step(int nunit, const int4 *__restrict__ unit, const int *__restrict__ atom_sys,
                        const SysP *__restrict__ sp, const int *__restrict__ inum, const int *__restrict__ img,
                        const double *__restrict__ gslot, const double *__restrict__ mass,
@@ -591,10 +595,12 @@ __global__ void __launch_bounds__(128) k_step(int nunit, const int4 *__restrict_
         const size_t sl = sl3[q];
         const double m = m3[q];
         for (int k = 0; k < 3; ++k) {
-            const double g = gslot[3 * sl + k];
             double vk = v[3 * b + k];
-            if (vback) vback[3 * b + k] = vk;
-            vk = vk + rt * g / m;
+            if (KICK) {
+                const double g = gslot[3 * sl + k];
+                if (vback) vback[3 * b + k] = vk;
+                vk = vk + rt * g / m;
+            }
             const double p = x[3 * b + k];
             xo[q][k] = p;
             xn[q][k] = p + P.dt * vk;
@@ -615,7 +621,7 @@ __global__ void __launch_bounds__(128) k_step(int nunit, const int4 *__restrict_
         JAtom *j = ys + sl3[q];
         for (int k = 0; k < 3; ++k) {
             x[3 * b + k] = xn[q][k];
-            v[3 * b + k] = shake_on ? (xn[q][k] - xo[q][k]) * r : vv[q][k];
+            if (VUPD) v[3 * b + k] = shake_on ? (xn[q][k] - xo[q][k]) * r : vv[q][k];
         }
         j->x = xn[q][0] - P.L[0] * (double)im3[q][0];
         j->y = xn[q][1] - P.L[1] * (double)im3[q][1];

@@ -23,6 +23,7 @@
 #include "jp_ge.hpp"
 #include "jp_alsd.hpp"
 #include "jp_cmm.hpp"
+#include "jp_stop.hpp"
 
 #define CK(x)                                                          \
     do {                                                               \
@@ -275,6 +276,14 @@ struct Engine {
     // md_run's heating loop (jp_set_heating: HEATLO, STARTT): while md_run's
     // step count idloop is at most heat_lp the canonical target temperature
     // is heat_t0 + ((temps - heat_t0) / heat_lp) idloop (verlethe.f90:419-423)
+    // STOPCE (jp_set_stopce): per system {mode, first atom, atoms stopped,
+    // atoms of atom 1's molecule kind}; mode 1 TRAN, 2 ROTA, 3 BOTH
+    int stop_mode = 0;
+    std::vector<int4> stop_h;
+    DBuf<int4> d_stop;
+    int set_stopce(int mode, const int *natom, const int *nkind);
+    void stop_dof(std::vector<double> &dof, const std::vector<long long> &keys, const int *group) const;
+    int start_step(int prec);
     int heat_lp = 0;
     double heat_t0 = 0.0, cano_temps = 300.0, ttgt_set = -1.0;
     DBuf<double> d_ttgt;                // the target of the step being issued, per system
@@ -395,7 +404,7 @@ struct Engine {
         d_bat.release(); d_bptr.release(); d_bref.release(); d_bnd.release(); d_ang.release();
         d_bndp.release(); d_angp.release(); d_eterm.release(); d_unit.release();
         d_gid.release(); d_gsys.release(); d_sg0.release(); d_sg1.release(); d_ab0.release(); d_ab1.release();
-        d_ub0.release(); d_ub1.release(); d_ablk.release(); d_ublk.release(); d_sfirst.release(); d_shkon.release(); d_cdone.release();
+        d_ub0.release(); d_ub1.release(); d_ablk.release(); d_ublk.release(); d_sfirst.release(); d_shkon.release(); d_cdone.release(); d_stop.release();
         d_gdof.release(); d_ctemps.release(); d_ttgt.release(); d_fcc.release(); d_xcb.release(); d_pa.release(); d_pu.release(); d_pn.release();
         d_torc.release(); d_tor.release(); d_p14.release(); d_p14p.release(); d_torp.release();
         d_abeg.release(); d_aend.release(); d_cbeg.release(); d_cend.release(); d_rsum.release(); d_rfix.release();
@@ -501,6 +510,15 @@ struct Engine {
 };
 
 static void launch_list8(Engine &g, int hiwater);
+
+// The ier of the device status word of a step: SHAKE's, then remotv's.
+static int step_ier(int err) {
+    if (err & 8) return JP_IER_SHAKE_SING;
+    if (err & 4) return JP_IER_SHAKE_CONV;
+    if (err & JP_ERR_STOP_MASS) return JP_IER_STOP_MASS;
+    if (err & JP_ERR_STOP_SING) return JP_IER_STOP_SING;
+    return 0;
+}
 
 // Box grid of one system: 2^hbits cells per edge, box-aligned, binned in
 // Hilbert order.  shift 0 gives cells of 4.5-9 A; each unit of shift halves
@@ -1787,11 +1805,115 @@ int Engine::shake(int &ier_out) {
 int Engine::integrate(int prec, bool mon) {
     (void)prec;
     if (cano) return cano_step();
-    hipLaunchKernelGGL(k_step, grid1(nunit, 128), 128, 0, st, nunit, d_unit.p, d_atom_sys.p, d_sp.p, d_inum.p,
-                       d_img.p, d_gslot.p, d_mass.p, d_imass.p, d_s2d.p, d_s3d.p, shake_tol, shake_maxit,
-                       (n2 + n3) > 0 ? 1 : 0, d_x.p, d_v.p, mon ? d_vback.p : (double *)nullptr, d_ys.p, d_err.p);
+    double *vb = mon ? d_vback.p : (double *)nullptr;
+    if (stop_mode) {   // verlet.f90:144-190: kick, STOPCE about x(t), drift and SHAKE
+        hipLaunchKernelGGL(k_kick, grid1(natom, 256), 256, 0, st, natom, d_atom_sys.p, d_sp.p, d_inum.p, d_gslot.p,
+                           d_mass.p, d_imass.p, 0, d_v.p, vb);
+        hipLaunchKernelGGL(k_remotv, (unsigned)nsys, JP_STOP_NT, 0, st, d_stop.p, d_mass.p, d_x.p, d_v.p,
+                           (const int *)nullptr, 1, d_err.p);
+        hipLaunchKernelGGL((k_step<false, true>), grid1(nunit, 128), 128, 0, st, nunit, d_unit.p, d_atom_sys.p,
+                           d_sp.p, d_inum.p, d_img.p, d_gslot.p, d_mass.p, d_imass.p, d_s2d.p, d_s3d.p, shake_tol,
+                           shake_maxit, (n2 + n3) > 0 ? 1 : 0, d_x.p, d_v.p, vb, d_ys.p, d_err.p);
+    } else {
+        hipLaunchKernelGGL((k_step<true, true>), grid1(nunit, 128), 128, 0, st, nunit, d_unit.p, d_atom_sys.p,
+                           d_sp.p, d_inum.p, d_img.p, d_gslot.p, d_mass.p, d_imass.p, d_s2d.p, d_s3d.p, shake_tol,
+                           shake_maxit, (n2 + n3) > 0 ? 1 : 0, d_x.p, d_v.p, vb, d_ys.p, d_err.p);
+    }
     CK(hipGetLastError());
     return 0;
+}
+
+// STOPCE= TRAN (1), ROTA (2), BOTH (3) or NO (0) for every system (remotv,
+// verlet.f90:156-169, verlethe.f90:489-502, 705-718, staver.f90:165-178):
+// the first natom[s] atoms of system s are stopped (md_run's
+// ixcend(NSTPCN)); nkind[s] atoms from its first share atom 1's molecule
+// kind (md.f90:187-193, the degrees of freedom under TEMPCO= YES).  Before
+// jp_set_canonical, which takes the degrees of freedom off.
+int Engine::set_stopce(int mode, const int *natom, const int *nkind) {
+    if (mode < 0 || mode > 3 || cano) return JP_EINVAL;
+    std::vector<int4> h(nsys, make_int4(0, 0, 0, 0));
+    if (mode) {
+        if (!natom) return JP_EINVAL;
+        for (int s = 0; s < nsys; ++s) {
+            const int k = nkind ? nkind[s] : 0;
+            if (natom[s] < 1 || natom[s] > hs[s].n || k < 0 || k > hs[s].n) return JP_EINVAL;
+            h[s] = make_int4(mode, sp[s].atom0, natom[s], k);
+        }
+    }
+    drop_graphs();
+    AL(d_stop, nsys);
+    CK(hipMemcpyAsync(d_stop.p, h.data(), sizeof(int4) * nsys, hipMemcpyHostToDevice, st));
+    CK(hipStreamSynchronize(st));
+    stop_h = h;
+    stop_mode = mode;
+    return 0;
+}
+
+// STOPCE's degrees of freedom (md.f90:114-121, 183-215): 3 (TRAN, ROTA) or
+// 6 (BOTH) off the system's group labelled 1 (its only group without
+// labels), or, when the atoms of atom 1's molecule kind carry labels up to
+// ifrg >= 2, off the groups labelled 1 to ifrg in proportion to their
+// degrees of freedom.  keys: the sorted (system, label) of every group.
+void Engine::stop_dof(std::vector<double> &dof, const std::vector<long long> &keys, const int *group) const {
+    if (!stop_mode) return;
+    for (int s = 0; s < nsys; ++s) {
+        const int4 S = stop_h[s];
+        const double k = S.x == 3 ? 6.0 : 3.0;
+        int ifrg = 1;
+        if (group)
+            for (int a = S.y; a < S.y + S.w; ++a) ifrg = std::max(ifrg, group[a]);
+        std::vector<int> gs;   // the system's groups labelled 1..ifrg, by label
+        for (int g = 0; g < (int)keys.size(); ++g)
+            if ((int)(keys[g] >> 32) == s && (int)(keys[g] & 0xffffffff) <= ifrg) gs.push_back(g);
+        if (gs.empty())
+            for (int g = 0; g < (int)keys.size(); ++g)
+                if ((int)(keys[g] >> 32) == s) { gs.push_back(g); break; }
+        if (gs.empty()) continue;
+        if (ifrg >= 2) {
+            double sum = 0.0;
+            for (int g : gs) sum = sum + dof[g];
+            std::vector<double> ratio;
+            for (int g : gs) ratio.push_back(dof[g] / sum);
+            for (size_t i = 0; i < gs.size(); ++i) dof[gs[i]] = dof[gs[i]] - k * ratio[i];
+        } else {
+            dof[gs[0]] = dof[gs[0]] - k;
+        }
+    }
+}
+
+// md_run's initial step of a run without a restart (inista, md.f90:1056-1065;
+// staver.f90:27-207): the list of x, the forces at x (the lambda-combined
+// gradient; no METHOD= GE weight, lambda stays), the half kick
+// v + (-dt c / 2) g (1/m), STOPCE about x, the drift and SHAKE of the
+// coordinates; v keeps the half-kick velocity.  idloop stays (0 for
+// md_run's fresh start); under CMM the far field is computed.
+int Engine::start_step(int prec) {
+    int r;
+    if ((r = ensure_list(prec))) return r;
+    if ((r = set_factors(-1))) return r;
+    if ((r = refresh_sorted())) return r;
+    if (alsd && (r = lam_upload())) return r;
+    CK(hipMemsetAsync(d_err.p, 0, 4, st));
+    cmm_far_now = true;
+    if ((r = alsd_fac()) || (r = nb(prec, false, cmm_far_now))) return r;
+    if (nbat) {
+        hipLaunchKernelGGL(k_bonded, grid1(nbat, 128), 128, 0, st, bonded_args(), d_atom_sys.p, d_sp.p, d_x.p,
+                           d_efac.p, d_inum.p, d_gslot.p, (double *)nullptr);
+    }
+    hipLaunchKernelGGL(k_kick, grid1(natom, 256), 256, 0, st, natom, d_atom_sys.p, d_sp.p, d_inum.p, d_gslot.p,
+                       d_mass.p, d_imass.p, 1, d_v.p, (double *)nullptr);
+    if (stop_mode)
+        hipLaunchKernelGGL(k_remotv, (unsigned)nsys, JP_STOP_NT, 0, st, d_stop.p, d_mass.p, d_x.p, d_v.p,
+                           (const int *)nullptr, 1, d_err.p);
+    hipLaunchKernelGGL((k_step<false, false>), grid1(nunit, 128), 128, 0, st, nunit, d_unit.p, d_atom_sys.p, d_sp.p,
+                       d_inum.p, d_img.p, d_gslot.p, d_mass.p, d_imass.p, d_s2d.p, d_s3d.p, shake_tol, shake_maxit,
+                       (n2 + n3) > 0 ? 1 : 0, d_x.p, d_v.p, (double *)nullptr, d_ys.p, d_err.p);
+    CK(hipGetLastError());
+    int err = 0;
+    CK(hipMemcpyAsync(&err, d_err.p, 4, hipMemcpyDeviceToHost, st));
+    CK(hipStreamSynchronize(st));
+    if (alsd && (r = lam_download())) return r;
+    return step_ier(err);
 }
 
 // METHOD= CANO: temperature groups (one per system, or per (system, label)
@@ -1854,6 +1976,7 @@ int Engine::set_canonical(int on, double temps, int icslop, double fcstol, const
         dof[gid[U.x]] -= (double)(kind * (kind - 1) / 2);
         shkon[atom_sys_h[U.x]] = 1;
     }
+    stop_dof(dof, keys, group);
     for (int g = 0; g < ngrp; ++g) {
         if (!(dof[g] > 0.0)) return JP_EINVAL;   // md.f90:124-129
         const int s = (int)(keys[g] >> 32);
@@ -1954,11 +2077,26 @@ int Engine::cano_step() {
     const unsigned ga = (unsigned)nab;
     hipLaunchKernelGGL(k_cano_begin, ga, 256, 0, st, A, d_sp.p);
     if (n2 + n3 > 0) {
-        for (int round = 1; round <= cano_icslop; ++round)
-            hipLaunchKernelGGL(k_cano_shake, (unsigned)nub, 128, 0, st, A, d_sp.p, d_unit.p, d_s2d.p, d_s3d.p,
+        for (int round = 1; round <= cano_icslop; ++round) {
+            if (!stop_mode) {
+                hipLaunchKernelGGL(k_cano_shake<0>, (unsigned)nub, 128, 0, st, A, d_sp.p, d_unit.p, d_s2d.p, d_s3d.p,
+                                   shake_tol, shake_maxit, round, d_ys.p, d_err.p);
+                continue;
+            }
+            hipLaunchKernelGGL(k_cano_shake<1>, (unsigned)nub, 128, 0, st, A, d_sp.p, d_unit.p, d_s2d.p, d_s3d.p,
                                shake_tol, shake_maxit, round, d_ys.p, d_err.p);
+            hipLaunchKernelGGL(k_remotv, (unsigned)nsys, JP_STOP_NT, 0, st, d_stop.p, d_mass.p, d_xold.p, d_v.p,
+                               (const int *)d_cdone.p, round, d_err.p);
+            hipLaunchKernelGGL(k_cano_shake<2>, (unsigned)nub, 128, 0, st, A, d_sp.p, d_unit.p, d_s2d.p, d_s3d.p,
+                               shake_tol, shake_maxit, round, d_ys.p, d_err.p);
+        }
+    } else if (!stop_mode) {
+        hipLaunchKernelGGL(k_cano_drift, ga, 256, 0, st, A, d_sp.p, d_ys.p, 0);
     } else {
-        hipLaunchKernelGGL(k_cano_drift, ga, 256, 0, st, A, d_sp.p, d_ys.p);
+        hipLaunchKernelGGL(k_cano_drift, ga, 256, 0, st, A, d_sp.p, d_ys.p, 1);
+        hipLaunchKernelGGL(k_remotv, (unsigned)nsys, JP_STOP_NT, 0, st, d_stop.p, d_mass.p, d_xold.p, d_v.p,
+                           (const int *)nullptr, 1, d_err.p);
+        hipLaunchKernelGGL(k_cano_drift, ga, 256, 0, st, A, d_sp.p, d_ys.p, 2);
     }
     CK(hipGetLastError());
     return 0;
@@ -2115,9 +2253,7 @@ int Engine::restart_step(int prec, jp_energy_log *log) {
     if ((r = refresh_sorted())) return r;
     if (alsd && (r = lam_upload())) return r;   // lambda and its velocity back too (md.f90:1255-1258)
     if (rl) return rl;
-    if (err & 8) return JP_IER_SHAKE_SING;
-    if (err & 4) return JP_IER_SHAKE_CONV;
-    return 0;
+    return step_ier(err);
 }
 
 void Engine::drop_graphs() {
@@ -2188,8 +2324,7 @@ int Engine::mon_record(int step, int nsteps, int nmon, int nlog, const std::vect
     }
     CK(hipStreamSynchronize(st));
     const int err = *h_err;
-    if (err & 8) return JP_IER_SHAKE_SING;
-    if (err & 4) return JP_IER_SHAKE_CONV;
+    if (int e = step_ier(err)) return e;
     for (int s = 0; s < nsys; ++s) {
         double e = h_rsum[9 * (size_t)nsys + s];
         // ALSD: caltmp starts with lambda's term, lambda_vt the mean of its
@@ -2394,9 +2529,7 @@ int Engine::run_nve(int nsteps, int update, int nmon, int prec, jp_energy_log *l
     t_list = tl;
     captures_last_run = n_captures - cap0;
     if (alsd && (r = lam_download())) return r;
-    if (err & 8) return JP_IER_SHAKE_SING;
-    if (err & 4) return JP_IER_SHAKE_CONV;
-    return 0;
+    return step_ier(err);
 }
 
 // Captures the step graphs a following run_nve(nsteps, update, nmon, prec)
@@ -2886,9 +3019,7 @@ int jp_shake(jp_ctx *ctx, double *cord, const double *cord_old) {
     CK(hipMemcpyAsync(cord, E.d_x.p, 24 * (size_t)E.natom, hipMemcpyDeviceToHost, E.st));
     CK(hipMemcpyAsync(&err, E.d_err.p, 4, hipMemcpyDeviceToHost, E.st));
     CK(hipStreamSynchronize(E.st));
-    if (err & 8) return JP_IER_SHAKE_SING;
-    if (err & 4) return JP_IER_SHAKE_CONV;
-    return 0;
+    return step_ier(err);
 }
 
 int jp_run_nve(jp_ctx *ctx, int32_t nsteps, int32_t update, int32_t nmon, int32_t precision, jp_energy_log *log) {
@@ -3109,6 +3240,26 @@ int jp_set_heating(jp_ctx *ctx, int32_t heatlo, double startt) {
 int jp_batch_set_heating(jp_batch *b, int32_t heatlo, double startt) {
     if (!b) return JP_EINVAL;
     return b->e.set_heating(heatlo, startt);
+}
+
+int jp_set_stopce(jp_ctx *ctx, int32_t mode, int32_t natom, int32_t nkind) {
+    if (!ctx) return JP_EINVAL;
+    return ctx->e.set_stopce(mode, &natom, &nkind);
+}
+
+int jp_batch_set_stopce(jp_batch *b, int32_t mode, const int32_t *natom, const int32_t *nkind) {
+    if (!b) return JP_EINVAL;
+    return b->e.set_stopce(mode, natom, nkind);
+}
+
+int jp_start_step(jp_ctx *ctx, int32_t precision) {
+    if (!ctx || (precision != JP_PREC_FP64 && precision != JP_PREC_FP32_FIXED)) return JP_EINVAL;
+    return ctx->e.start_step(precision);
+}
+
+int jp_batch_start_step(jp_batch *b, int32_t precision) {
+    if (!b || (precision != JP_PREC_FP64 && precision != JP_PREC_FP32_FIXED)) return JP_EINVAL;
+    return b->e.start_step(precision);
 }
 
 int jp_time_cmm(jp_ctx *ctx, int32_t reps, double *near, double *far) {

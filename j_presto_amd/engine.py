@@ -39,6 +39,10 @@ def _i32(a):
     return a.ctypes.data_as(native.i32p) if a is not None else None
 
 
+
+# md_run's STOPCE values (inpmd.f90:124 idfstp; NO is off)
+STOPCE = {"NO": 0, "TRAN": 1, "ROTA": 2, "BOTH": 3}
+
 class MdSystem:
     """One system on the GPU (a jp_ctx)."""
 
@@ -216,6 +220,19 @@ class MdSystem:
         """md_run's heating loop (HEATLO steps from STARTT K to SETTEM) for
         the canonical runs that follow; heatlo = 0 switches it off."""
         native.check("jp_set_heating", self.lib.jp_set_heating(self.ctx, int(heatlo), float(startt)))
+
+    def set_stopce(self, mode: str, nstpcn: int = 1):
+        """STOPCE= TRAN, ROTA, BOTH or NO over the first nstpcn chains
+        (molecule copies) for the steps that follow; before set_canonical."""
+        s = self.system
+        native.check("jp_set_stopce", self.lib.jp_set_stopce(
+            self.ctx, STOPCE[mode], s.chain_end(nstpcn) if mode != "NO" else 0, s.first_kind_atoms()))
+
+    def start_step(self, precision: Optional[int] = None) -> int:
+        """md_run's initial step of a run without a restart (staver) from the
+        context's x and v; returns ier."""
+        prec = self.precision if precision is None else precision
+        return native.check("jp_start_step", self.lib.jp_start_step(self.ctx, prec))
 
     def set_cmm(self, cminsiz: Optional[float] = 6.0, nlev: int = 3, cmmupd: int = 5):
         """CAL15M= CMM (cell multipole method, BOUNDA= NO) for the following
@@ -406,6 +423,19 @@ class Ensemble:
 
     def set_lambda(self, i: int, lam: float, lam_v: float = 0.0):
         native.check("jp_batch_set_lambda", self.lib.jp_batch_set_lambda(self.b, i, float(lam), float(lam_v)))
+
+    def set_stopce(self, mode: str, nstpcn: int = 1):
+        """MdSystem.set_stopce for every system of the batch."""
+        on = mode != "NO"
+        self._stop = (np.array([s.chain_end(nstpcn) if on else 0 for s in self.systems], np.int32),
+                      np.array([s.first_kind_atoms() for s in self.systems], np.int32))
+        native.check("jp_batch_set_stopce", self.lib.jp_batch_set_stopce(
+            self.b, STOPCE[mode], _i32(self._stop[0]), _i32(self._stop[1])))
+
+    def start_step(self, precision: Optional[int] = None) -> int:
+        """MdSystem.start_step for every system of the batch."""
+        prec = self.precision if precision is None else precision
+        return native.check("jp_batch_start_step", self.lib.jp_batch_start_step(self.b, prec))
 
     def set_canonical(self, temps: Optional[float] = 300.0, icslop: int = 3, fcstol: float = 1e-3,
                       tempco: bool = False):

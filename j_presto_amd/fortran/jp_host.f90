@@ -12,7 +12,9 @@
 !                 a..b in cluster 1, the rest in cluster 2) with GEPSME= LMD,
 !                 read by jp_input.f90.  EXE> MD: LOOPLI, UPDATE, TIMEST,
 !                 METHOD (MICR, CANO or GE; required), SETTEM, TEMPCO, HEATLO, STARTT, OUTLOG,
-!                 RESTAR, NAMERI, NAMERO, NAMETO, NAMEGE, LAMBDA, LMDSQU, CUTLEN,
+!                 STOPCE, NSTPCN (jp_set_stopce, the chains of the TPL),
+!                 RESTAR (NO: md_run's initial half step, jp_start_step, from
+!                 INITIA= ZERO), NAMERI, NAMERO, NAMETO, NAMEGE, LAMBDA, LMDSQU, CUTLEN,
 !                 DIEVAL, XMINCL..ZMAXCL, COVSHK, LIMSHK (inpmd.f90; defaults of
 !                 main.f90:150-185).  CAL15M= ZD runs under BOUNDA= PBC;
 !                 CAL15M= CMM (md_run's default) under BOUNDA= NO (its default)
@@ -78,6 +80,8 @@ program jp_host
   ! the heating loop of METHOD= CANO / GE (HEATLO steps from STARTT K)
   integer(c_int32_t) :: heatlo
   real(c_double) :: startt
+  ! STOPCE= TRAN/ROTA/BOTH (1/2/3; 0 NO) of the first NSTPCN chains
+  integer(c_int32_t) :: stopce, nstpcn
   integer(8) :: tstart, trate, tnow          ! wall clock since the program started (md_run's fxcpus)
   integer(c_int32_t), allocatable, target :: grp(:)
   type(top_t) :: top
@@ -337,6 +341,7 @@ contains
     cellvl = 3; cputim = 99999999.0d0; initzero = .false.
     cmm = .true.; celsiz = 6.0d0; cmmupd = 5   ! main.f90:170, inpinp.f90:86, inpmd.f90:151
     heatlo = 0; startt = 300.0d0                 ! inpmd.f90:127,147
+    stopce = 0; nstpcn = 1                       ! inpmd.f90:124, intele(84)
     gemode = .false.; gef = ' '; lweigh = 0.0d0; ltcntl = 1; lamkw = 0.0d0; lsqkw = 0.0d0
     have_method = .false.; have_pbc = .false.
     inmd = .false.; inin = .false.
@@ -435,7 +440,16 @@ contains
              end if
           case ('CMMUPD'); read(val, *, iostat=ios) cmmupd
           case ('STOPCE')
-             if (trim(val) /= 'NO') call refuse(key, val, 'centre-of-mass motion removal is not implemented')
+             select case (trim(val))
+             case ('NO'); stopce = 0
+             case ('TRAN'); stopce = 1
+             case ('ROTA'); stopce = 2
+             case ('BOTH'); stopce = 3
+             case default; call refuse(key, val, 'TRAN, ROTA, BOTH or NO')
+             end select
+          case ('NSTPCN')
+             read(val, *, iostat=ios) nstpcn
+             if (ios == 0 .and. nstpcn < 1) call refuse(key, val, 'a chain count >= 1')
           case ('HEATLO')
              read(val, *, iostat=ios) heatlo
              if (ios == 0 .and. heatlo < 0) call refuse(key, val, 'a step count >= 0')
@@ -460,7 +474,7 @@ contains
              else if (trim(val) /= 'ASCI') then
                 call refuse(key, val, 'NAMETO is written as ASCII only')
              end if
-          case ('PARALV', 'N_SIMD', 'LOGFOR', 'RELAXA', 'TEMPER', 'NSTPCN', 'OUTATM', 'DUMMYL', 'BESTFI', &
+          case ('PARALV', 'N_SIMD', 'LOGFOR', 'RELAXA', 'TEMPER', 'OUTATM', 'DUMMYL', 'BESTFI', &
                 'RANDOM', 'RADCAP', 'FORCAP', 'FUNCAP', 'WETPSR', 'WETDSR', 'WETDHR', &
                 'CAPSHP', 'FOACAP', 'RADPRO', 'HTPSLO', 'STAPST', 'ELLIPA', 'ELLIPB', 'ELLIPC', 'ELLBUF', &
                 'CELBUF', 'RADIUS')
@@ -540,6 +554,10 @@ contains
        stop 3
     end if
     if (nmon <= 0 .or. nmon > nsteps) nmon = 0
+    if (stopce > 0 .and. len_trim(fin) > 0) then
+       write(*,'(a)') 'jp_host: STOPCE needs the chains of the TPL (a system file has none)'
+       stop 3
+    end if
     if (settem > 0.0d0 .and. tempco .and. len_trim(fin) > 0) then
        write(*,'(a)') 'jp_host: TEMPCO= YES needs the molecule kinds of the TPL (a system file has none)'
        stop 3
@@ -622,6 +640,23 @@ contains
     if (.not. restar) then
        rloop = 0; rtime = 0.0d0
     end if
+    if (stopce > 0) then
+       ! remotv over ixcend(NSTPCN) atoms; the degrees of freedom of atom 1's
+       ! molecule kind (md.f90:114-121, 183-215) before the canonical set-up
+       if (nstpcn > size(top%cend) .or. top%cend(min(nstpcn, size(top%cend))) <= 0) then
+          write(*,'(a,i0)') 'jp_host: NSTPCN beyond the chains of the TPL: ', nstpcn
+          stop 3
+       end if
+       kk = 1
+       do while (kk < n)
+          if (top%kind(kk + 1) /= top%kind(1)) exit
+          kk = kk + 1
+       end do
+       if (jp_set_stopce(ctx, stopce, top%cend(nstpcn), int(kk, c_int32_t)) /= JP_OK) then
+          write(*,'(a)') 'jp_host: jp_set_stopce failed'
+          stop 1
+       end if
+    end if
     if (settem > 0.0d0) then
        if (tempco .and. fromtpl) then
           call temperature_groups()
@@ -659,6 +694,9 @@ contains
     ! coordinates and leaves the velocities its last verlethe substep started
     ! from (the restart's own unless a SHAKE retry split the step)
     if (restar) ier(7) = jp_restart_step(ctx, prec, c_null_ptr)
+    ! without a restart md_run starts with staver's half step (inista,
+    ! md.f90:1056-1065) from the initial velocities
+    if (.not. restar) ier(7) = jp_start_step(ctx, prec)
     ndone = nsteps
     if (cputim < 99999999.0d0) then
        ! CPUTIM: md_run checks the elapsed time after every list update and,

@@ -204,6 +204,18 @@ module jpgpu
        real(c_double), value :: startt
      end function jp_set_heating
 
+     integer(c_int) function jp_set_stopce(ctx, mode, natom, nkind) bind(C, name='jp_set_stopce')
+       import :: c_int, c_int32_t, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: mode, natom, nkind
+     end function jp_set_stopce
+
+     integer(c_int) function jp_start_step(ctx, precision) bind(C, name='jp_start_step')
+       import :: c_int, c_int32_t, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: precision
+     end function jp_start_step
+
      integer(c_int) function jp_set_cmm(ctx, on, cminsiz, nlev, cmmupd) bind(C, name='jp_set_cmm')
        import :: c_int, c_int32_t, c_ptr, c_double
        type(c_ptr), value :: ctx

@@ -52,6 +52,10 @@ SIGNATURES = {
                                     C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int32]),
     "jp_set_cmm": (C.c_int, [vp, C.c_int32, C.c_double, C.c_int32, C.c_int32]),
     "jp_set_heating": (C.c_int, [vp, C.c_int32, C.c_double]),
+    "jp_set_stopce": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_int32]),
+    "jp_batch_set_stopce": (C.c_int, [vp, C.c_int32, i32p, i32p]),
+    "jp_start_step": (C.c_int, [vp, C.c_int32]),
+    "jp_batch_start_step": (C.c_int, [vp, C.c_int32]),
     "jp_batch_set_heating": (C.c_int, [vp, C.c_int32, C.c_double]),
     "jp_time_cmm": (C.c_int, [vp, C.c_int32, f64p, f64p]),
     "jp_get_lambda": (C.c_int, [vp, f64p, f64p]),

@@ -346,7 +346,7 @@ def test_mdinp_refuses_what_it_does_not_run(tmp_path):
                                                   .replace("BOUNDA= PBC", "BOUNDA= NO"), "CMMUPD", "3"),
         "CELSIZ= -1.0": inp.replace("QUIT\n\nEXE> MD", "      CELSIZ= -1.0\nQUIT\n\nEXE> MD"),
         "STOPCE= YES": inp.replace("STOPCE= NO", "STOPCE= BOTH").replace("BOTH", "YES"),
-        "STOPCE= TRAN": inp.replace("STOPCE= NO", "STOPCE= TRAN"),
+        "NSTPCN= 0": md_line(inp, "NSTPCN", "0"),
         "HEATLO= -5": md_line(inp, "HEATLO", "-5"),
         "SHAKEM= ALLB": md_line(inp, "SHAKEM", "ALLB"),
         "PSETMP= 300.0": md_line(inp, "PSETMP", "300.0"),
@@ -567,3 +567,40 @@ def test_mdinp_heating_like_md_run(golden, tmp_path, tag):
     assert o["loop"] == loop0 + n
     assert np.abs(o["x"] - f[f"heat_{tag}_x"]).max() < 1e-9
     assert np.abs(o["v"] - f[f"heat_{tag}_v"]).max() < 1e-9
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,tag", [("chain_tors", "mb"), ("chain_tors", "cy"), ("chain_tors", "s0"),
+                                      ("chain_lmd_tors", "sc"), ("chain_droplet_cmm", "mr"),
+                                      ("chain_droplet_cmm", "sc")])
+def test_mdinp_stopce_like_md_run(golden, tmp_path, name, tag):
+    """STOPCE= BOTH/ROTA (NSTPCN= 1) under METHOD= MICR and CANO, and runs
+    without a restart (RESTAR= NO, INITIA= ZERO: md_run's initial half step)
+    from md_run's own inputs: the restart the host writes holds md_run's x
+    and v within 1e-9."""
+    s, f = golden(name)
+    method, stopce, restar, n, tempco = {"mb": ("MICR", "BOTH", "YES", 12, "NO"),
+                                         "mr": ("MICR", "ROTA", "YES", 12, "NO"),
+                                         "cy": ("CANO", "ROTA", "YES", 12, "YES"),
+                                         "s0": ("MICR", "NO", "NO", 12, "NO"),
+                                         "sc": ("CANO", "BOTH", "NO", 12, "NO")}[tag]
+    print(f"case host stop {name} {tag}", flush=True)
+    d = str(tmp_path)
+    md = {"METHOD": method, "LOOPLI": str(n), "STOPCE": stopce, "NSTPCN": "1", "RESTAR": restar}
+    if restar == "NO":
+        md["INITIA"] = "ZERO"
+        if s.lam:
+            md["LAMBDA"] = repr(float(s.lam))
+    if method == "CANO":
+        md.update({"SETTEM": "300.0", "TEMPCO": tempco})
+    if name == "chain_droplet_cmm":
+        md.update({"CAL15M": "CMM", "BOUNDA": "NO"})
+    _tpl_case(d, s, md)
+    r = subprocess.run([_host(), "--mdinp", "md.inp"], capture_output=True, text=True, timeout=240, cwd=d)
+    print(r.stdout, flush=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    o = refio.read_restart(os.path.join(d, "out.rst"))
+    assert o["n"] == s.n and o["loop"] == n
+    dx, dv = np.abs(o["x"] - f[f"stop_{tag}_x"]).max(), np.abs(o["v"] - f[f"stop_{tag}_v"]).max()
+    print(f"max |dx| {dx:.2e} |dv| {dv:.2e}", flush=True)
+    assert dx < 1e-9 and dv < 1e-9
