@@ -66,9 +66,9 @@ typedef struct jp_sys_desc {
     int32_t nexcess;           /* ZD excess pairs: bonds, angle 1-3, 1-4 (iytnbf=1) */
     const int32_t *excess;     /* [nexcess][2] 1-based, TPL order (incel.f90:665-688) */
     int32_t nshake;            /* SHAKE groups (iugshk) */
-    const int32_t *shake_natom;/* [nshake] 2 or 3 atoms */
+    const int32_t *shake_natom;/* [nshake] 2, 3 or 4 atoms (shak2e, shake3, shake4) */
     const int32_t *shake_atoms;/* [nshake][4] 1-based, unused slots 0 */
-    const double *shake_dist2; /* [nshake][6] fudshk: squared targets, pair order 1-2,2-3,1-3,... */
+    const double *shake_dist2; /* [nshake][6] fudshk: squared targets, pair order 1-2,2-3,1-3,1-4,3-4,2-4 */
     double shake_tol;          /* fustol (COVSHK), md_run default 1e-6 */
     int32_t shake_maxit;       /* iuslop (LIMSHK), md_run default 1000 */
     int32_t cell_level;        /* CELLVL (nlev); 0 -> md_run default 3 */

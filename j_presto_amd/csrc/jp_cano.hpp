@@ -291,10 +291,11 @@ __global__ void __launch_bounds__(256) k_cano_drift(CanoArgs A, const SysP *__re
 // from the same fixed-order sums.  PH 0: the whole round; PH 1: up to the
 // velocity update (v and the constraint force written); PH 2: the rest from
 // v, the centre-of-mass motion taken out in between (k_remotv).
-template <int PH>
+template <int PH, int NQ>
 __global__ void __launch_bounds__(128) k_cano_shake(CanoArgs A, const SysP *__restrict__ sp,
                                                     const int4 *__restrict__ unit, const double *__restrict__ s2d,
-                                                    const double *__restrict__ s3d, double tol, int maxit,
+                                                    const double *__restrict__ s3d, const int4 *__restrict__ s4a,
+                                                    const double *__restrict__ s4d, double tol, int maxit,
                                                     int round, JAtom *__restrict__ ys, int *__restrict__ err) {
 #pragma clang fp contract(off)
     __shared__ double sT[JP_CANO_MAXG], sU[JP_CANO_MAXG];
@@ -313,13 +314,15 @@ __global__ void __launch_bounds__(128) k_cano_shake(CanoArgs A, const SysP *__re
     // memory round trips overlap: x, x_o, g, f_c, v_b, and v (round 1) or
     // x_b (later rounds)
     const int4 U = unit[min(u, B.y - 1)];
-    const int kind = U.w & 3, prm = U.w >> 2;
-    const int a[3] = {U.x, U.y, U.z};
+    const int kind = U.w & 7, prm = U.w >> 3;
+    int a[NQ];
+    a[0] = U.x; a[1] = U.y; a[2] = U.z;
+    if constexpr (NQ == 4) a[3] = kind == 4 ? s4a[prm].w : U.x;
     const bool live = u < B.y;
-    double ax[3][3], axo[3][3], ag[3][3], afc[3][3], avb[3][3], avx[3][3], aim[3], am[3];
-    int agid[3], asl[3], aimg[3][3];
+    double ax[NQ][3], axo[NQ][3], ag[NQ][3], afc[NQ][3], avb[NQ][3], avx[NQ][3], aim[NQ], am[NQ];
+    int agid[NQ], asl[NQ], aimg[NQ][3];
 #pragma unroll
-    for (int q = 0; q < 3; ++q) {
+    for (int q = 0; q < NQ; ++q) {
         const int b = (live && q < kind) ? a[q] : a[0];
         const size_t sl = (size_t)A.inum[b];
         asl[q] = (int)sl;
@@ -366,21 +369,25 @@ __global__ void __launch_bounds__(128) k_cano_shake(CanoArgs A, const SysP *__re
             return;
         }
     }
-    double T3[3] = {0.0, 0.0, 0.0};
-    if (kind == 3) {
+    double T3[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (NQ == 4 && kind == 4) {
+        for (int q = 0; q < 6; ++q) T3[q] = s4d[6 * prm + q];
+    } else if (kind == 3) {
         T3[0] = s3d[6 * prm]; T3[1] = s3d[6 * prm + 1]; T3[2] = s3d[6 * prm + 2];
     } else if (kind == 2) {
         T3[0] = s2d[6 * prm];
     }
-    double val[3] = {0.0, 0.0, 0.0}, valn[3] = {0.0, 0.0, 0.0};
-    int gq[3] = {-1, -1, -1};
+    double val[NQ], valn[NQ];
+    int gq[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) { val[q] = 0.0; valn[q] = 0.0; gq[q] = -1; }
     const bool shk = A.shk_on[s] != 0;
     if (live) {
         const SysP &P = sp[s];
         const double dt = P.dt;
-        double xn[3][3], xo[3][3], xb[3][3], vd[3][3];
+        double xn[NQ][3], xo[NQ][3], xb[NQ][3], vd[NQ][3];
 #pragma unroll
-        for (int q = 0; q < 3; ++q) {
+        for (int q = 0; q < NQ; ++q) {
             if (q >= kind) break;
             const double im = aim[q];
             if (PH == 2) {   // the updated velocity from PH 1 (and k_remotv)
@@ -421,6 +428,12 @@ __global__ void __launch_bounds__(128) k_cano_shake(CanoArgs A, const SysP *__re
                 if (shk) A.xcb[3 * a[q] + k] = xn[q][k];
             }
         }
+        if constexpr (NQ == 4) {
+            if (PH != 1 && shk && kind == 4) {
+                const int e = shake4_regs(P, aim, T3, tol, maxit, xn, xo);
+                if (e) atomicOr(err, e);
+            }
+        }
         if (PH != 1 && shk) {
             if (kind == 3) {
                 const int e = shake3_regs(P, aim, T3, tol, maxit, xn, xo);
@@ -431,7 +444,7 @@ __global__ void __launch_bounds__(128) k_cano_shake(CanoArgs A, const SysP *__re
         }
         const double r = 1.0 / dt, r2 = 1.0 / (dt * dt), r3 = 0.5 * dt;
 #pragma unroll
-        for (int q = 0; q < 3; ++q) {
+        for (int q = 0; q < NQ; ++q) {
             if (PH == 1 || q >= kind) break;
             const int b = a[q];
             if (shk) {

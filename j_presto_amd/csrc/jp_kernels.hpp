@@ -511,6 +511,168 @@ __device__ __forceinline__ int shake3_regs(const SysP &P, const double *im, cons
     return 4;
 }
 
+// shake4_PB (shake.f90:333-562) on one 4-atom group held in registers: six
+// constraints, pairs (1,2) (2,3) (3,1) (4,1) (3,4) (2,4) (parlis), Newton
+// iterations whose 6 x 6 systems are solved as simulu does (simulu.f90:
+// ludeco's partial pivoting by row indices, pivots below eps^3 = 1e-30
+// singular, then the two triangular solves; row-major a[i][j] = A(i, j)).
+// Returns 0, 4 (not converged: r1 unchanged) or 8 (singular).
+__device__ __forceinline__ int shake4_regs(const SysP &P, const double *im, const double *T, double tol, int maxit,
+                                           double (*r1)[3], const double (*r2)[3]) {
+#pragma clang fp contract(off)
+    const double co[6][6] = {{im[0] + im[1], -im[1], -im[0], -im[0], 0.0, -im[1]},
+                             {-im[1], im[1] + im[2], -im[2], 0.0, -im[2], im[1]},
+                             {-im[0], -im[2], im[2] + im[0], im[0], im[2], 0.0},
+                             {-im[0], 0.0, im[0], im[3] + im[0], -im[3], -im[3]},
+                             {0.0, -im[2], im[2], -im[3], im[2] + im[3], im[3]},
+                             {-im[1], im[1], 0.0, -im[3], im[3], im[1] + im[3]}};
+    const int p0[6] = {0, 1, 2, 3, 2, 1}, p1[6] = {1, 2, 0, 0, 3, 3};
+    double d1[6][3], d2[6][3], w[6][3], G[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        G[q] = 0.0;
+        for (int k = 0; k < 3; ++k) {
+            d1[q][k] = r1[p0[q]][k] - r1[p1[q]][k];
+            d2[q][k] = r2[p0[q]][k] - r2[p1[q]][k];
+        }
+        mimg3(P, d1[q]);
+        mimg3(P, d2[q]);
+        for (int k = 0; k < 3; ++k) w[q][k] = d1[q][k];
+    }
+    for (int it = 0; it < maxit; ++it) {
+        double B[6];
+        bool ok = true;
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            B[q] = T[q] - (w[q][0] * w[q][0] + w[q][1] * w[q][1] + w[q][2] * w[q][2]);
+            if (fabs(B[q]) >= tol * T[q]) ok = false;
+        }
+        if (ok) {
+            for (int k = 0; k < 3; ++k) {
+                r1[0][k] = r1[0][k] + im[0] * (G[0] * d2[0][k] - G[2] * d2[2][k] - G[3] * d2[3][k]);
+                r1[1][k] = r1[1][k] + im[1] * (-G[0] * d2[0][k] + G[1] * d2[1][k] + G[5] * d2[5][k]);
+                r1[2][k] = r1[2][k] + im[2] * (-G[1] * d2[1][k] + G[2] * d2[2][k] + G[4] * d2[4][k]);
+                r1[3][k] = r1[3][k] + im[3] * (G[3] * d2[3][k] - G[4] * d2[4][k] - G[5] * d2[5][k]);
+            }
+            return 0;
+        }
+        double a[6][6];
+#pragma unroll
+        for (int ig = 0; ig < 6; ++ig)
+#pragma unroll
+            for (int q = 0; q < 6; ++q)
+                a[q][ig] = 2.0 * co[q][ig] * (w[q][0] * d2[ig][0] + w[q][1] * d2[ig][1] + w[q][2] * d2[ig][2]);
+#pragma unroll
+        for (int ig = 0; ig < 6; ++ig)
+#pragma unroll
+            for (int q = 0; q < 6; ++q) B[q] = B[q] + a[q][ig] * G[ig];
+        // ludeco: the permutation as row indices; rows are addressed through
+        // ip, kept in registers by selecting with unrolled compares
+        int ip[6] = {0, 1, 2, 3, 4, 5};
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            int l = k;
+            double al = 0.0;
+#pragma unroll
+            for (int r = 0; r < 6; ++r)
+                if (r == ip[k]) al = fabs(a[r][k]);
+#pragma unroll
+            for (int i = k + 1; i < 6; ++i) {
+                double v = 0.0;
+#pragma unroll
+                for (int r = 0; r < 6; ++r)
+                    if (r == ip[i]) v = fabs(a[r][k]);
+                if (v > al) { l = i; al = v; }
+            }
+            if (l != k) { const int t = ip[k]; ip[k] = ip[l]; ip[l] = t; }
+            if (al < 1.0e-30) return 8;   // PHYCNS.f90:34 eps = 1e-10, cubed
+            double piv[6];
+#pragma unroll
+            for (int j = 0; j < 6; ++j) {
+                piv[j] = 0.0;
+#pragma unroll
+                for (int r = 0; r < 6; ++r)
+                    if (r == ip[k]) piv[j] = a[r][j];
+            }
+            piv[k] = 1.0 / piv[k];
+#pragma unroll
+            for (int r = 0; r < 6; ++r)
+                if (r == ip[k]) a[r][k] = piv[k];
+#pragma unroll
+            for (int i = k + 1; i < 6; ++i) {
+                const int ri = ip[i];
+#pragma unroll
+                for (int r = 0; r < 6; ++r) {
+                    if (r != ri) continue;
+                    a[r][k] = a[r][k] * piv[k];
+#pragma unroll
+                    for (int j = k + 1; j < 6; ++j) a[r][j] = a[r][j] - a[r][k] * piv[j];
+                }
+            }
+        }
+        double x[6];
+#pragma unroll
+        for (int i = 0; i < 6; ++i) {
+            double t = 0.0, row[6];
+#pragma unroll
+            for (int r = 0; r < 6; ++r)
+                if (r == ip[i]) {
+                    t = B[r];
+#pragma unroll
+                    for (int j = 0; j < 6; ++j) row[j] = a[r][j];
+                }
+#pragma unroll
+            for (int j = 0; j < i; ++j) t = t - row[j] * x[j];
+            x[i] = t;
+        }
+#pragma unroll
+        for (int i = 5; i >= 0; --i) {
+            double row[6];
+#pragma unroll
+            for (int r = 0; r < 6; ++r)
+                if (r == ip[i])
+#pragma unroll
+                    for (int j = 0; j < 6; ++j) row[j] = a[r][j];
+            double t = x[i];
+#pragma unroll
+            for (int j = i + 1; j < 6; ++j) t = t - row[j] * x[j];
+            x[i] = t * row[i];
+        }
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            G[q] = x[q];
+            for (int k = 0; k < 3; ++k) w[q][k] = d1[q][k];
+        }
+#pragma unroll
+        for (int ig = 0; ig < 6; ++ig)
+#pragma unroll
+            for (int q = 0; q < 6; ++q)
+                for (int k = 0; k < 3; ++k) w[q][k] = w[q][k] + co[q][ig] * d2[ig][k] * G[ig];
+    }
+    return 4;
+}
+
+__global__ void __launch_bounds__(128) k_shake4(int ng, const int *__restrict__ gsys, const SysP *__restrict__ sp,
+                                                const int4 *__restrict__ ga, const double *__restrict__ gd,
+                                                const double *__restrict__ imass, double *__restrict__ r1,
+                                                const double *__restrict__ r2, double tol, int maxit,
+                                                int *__restrict__ err) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ng) return;
+    const SysP &P = sp[gsys[g]];
+    const int a[4] = {ga[g].x, ga[g].y, ga[g].z, ga[g].w};
+    double im[4], T[6], x1[4][3], x2[4][3];
+    for (int q = 0; q < 4; ++q) {
+        im[q] = imass[a[q]];
+        for (int k = 0; k < 3; ++k) { x1[q][k] = r1[3 * a[q] + k]; x2[q][k] = r2[3 * a[q] + k]; }
+    }
+    for (int q = 0; q < 6; ++q) T[q] = gd[6 * g + q];
+    const int e = shake4_regs(P, im, T, tol, maxit, x1, x2);
+    if (e) { atomicOr(err, e); return; }
+    for (int q = 0; q < 4; ++q)
+        for (int k = 0; k < 3; ++k) r1[3 * a[q] + k] = x1[q][k];
+}
+
 __global__ void __launch_bounds__(128) k_shake2(int ng, const int *__restrict__ gsys, const SysP *__restrict__ sp,
                          const int4 *__restrict__ ga, const double *__restrict__ gd, const double *__restrict__ mass,
                          double *__restrict__ r1, const double *__restrict__ r2) {
@@ -550,46 +712,57 @@ __global__ void __launch_bounds__(128) k_shake3(int ng, const int *__restrict__ 
 // SHAKE on the unit's atoms in registers, v = (x - x_old)/dt when the run
 // has constraints (verlet.f90:196-215), and the refreshed sorted coordinates.
 // The same operations in the same order as k_kick -> k_shake* -> k_post.
-// unit = {a0, a1, a2, kind | param << 2}, kind 1/2/3 = atoms in the unit.
+// unit = {a0, a1, a2, kind | param << 3}, kind 1-4 = atoms in the unit (the
+// fourth atom of a 4-atom group in s4a[param].w).  NQ: 3, or 4 when the
+// run has 4-atom groups.
 // KICK false: v already holds the kicked velocity (a step that stops the
 // centre-of-mass motion between kick and drift); VUPD false: v keeps it
 // after SHAKE (md_run's initial step, staver.f90:196-204).
-template <bool KICK, bool VUPD>
+template <bool KICK, bool VUPD, int NQ>
 __global__ void __launch_bounds__(128) k_step(int nunit, const int4 *__restrict__ unit, const int *__restrict__ atom_sys,
                        const SysP *__restrict__ sp, const int *__restrict__ inum, const int *__restrict__ img,
                        const double *__restrict__ gslot, const double *__restrict__ mass,
                        const double *__restrict__ imass, const double *__restrict__ s2d,
-                       const double *__restrict__ s3d, double tol, int maxit, int shake_on, double *__restrict__ x,
+                       const double *__restrict__ s3d, const int4 *__restrict__ s4a, const double *__restrict__ s4d,
+                       double tol, int maxit, int shake_on, double *__restrict__ x,
                        double *__restrict__ v, double *__restrict__ vback, JAtom *__restrict__ ys,
                        int *__restrict__ err) {
 #pragma clang fp contract(off)
     const int u = blockIdx.x * blockDim.x + threadIdx.x;
     if (u >= nunit) return;
     const int4 U = unit[u];
-    const int kind = U.w & 3, prm = U.w >> 2;
-    const int a[3] = {U.x, U.y, U.z};
+    const int kind = U.w & 7, prm = U.w >> 3;
+    int a[NQ];
+    a[0] = U.x; a[1] = U.y; a[2] = U.z;
+    if constexpr (NQ == 4) a[3] = kind == 4 ? s4a[prm].w : U.x;
     const SysP &P = sp[atom_sys[a[0]]];
     const double rt = -P.dt * 4.184e-4;   // md.f90:56 fdcvcv
-    double xn[3][3], xo[3][3], vv[3][3];
+    double xn[NQ][3], xo[NQ][3], vv[NQ][3];
     // what SHAKE and the sorted copy need, loaded up front so that these
     // loads overlap the first ones instead of following the SHAKE loop
-    int sl3[3] = {0, 0, 0}, im3[3][3];
-    double m3[3] = {1.0, 1.0, 1.0}, iw[3] = {0.0, 0.0, 0.0}, T3[3] = {0.0, 0.0, 0.0};
+    int sl3[NQ], im3[NQ][3];
+    double m3[NQ], iw[NQ], T3[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int q = 0; q < 3; ++q) {
+    for (int q = 0; q < NQ; ++q) {
         const int b = q < kind ? a[q] : a[0];
         sl3[q] = inum[b];
         m3[q] = mass[b];
+        iw[q] = 0.0;
         for (int k = 0; k < 3; ++k) im3[q][k] = img[3 * b + k];
     }
-    if (kind == 3) {
+    if (NQ == 4 && kind == 4) {
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) iw[q] = imass[a[q]];
+#pragma unroll
+        for (int q = 0; q < 6; ++q) T3[q] = s4d[6 * prm + q];
+    } else if (kind == 3) {
         iw[0] = imass[a[0]]; iw[1] = imass[a[1]]; iw[2] = imass[a[2]];
         T3[0] = s3d[6 * prm]; T3[1] = s3d[6 * prm + 1]; T3[2] = s3d[6 * prm + 2];
     } else if (kind == 2) {
         T3[0] = s2d[6 * prm];
     }
 #pragma unroll
-    for (int q = 0; q < 3; ++q) {   // fixed trip count: the arrays stay in registers
+    for (int q = 0; q < NQ; ++q) {   // fixed trip count: the arrays stay in registers
         if (q >= kind) break;
         const int b = a[q];
         const size_t sl = sl3[q];
@@ -607,6 +780,12 @@ __global__ void __launch_bounds__(128) k_step(int nunit, const int4 *__restrict_
             vv[q][k] = vk;
         }
     }
+    if constexpr (NQ == 4) {
+        if (kind == 4) {
+            const int e = shake4_regs(P, iw, T3, tol, maxit, xn, xo);
+            if (e) atomicOr(err, e);
+        }
+    }
     if (kind == 3) {
         const int e = shake3_regs(P, iw, T3, tol, maxit, xn, xo);
         if (e) atomicOr(err, e);
@@ -615,7 +794,7 @@ __global__ void __launch_bounds__(128) k_step(int nunit, const int4 *__restrict_
     }
     const double r = 1.0 / P.dt;
 #pragma unroll
-    for (int q = 0; q < 3; ++q) {
+    for (int q = 0; q < NQ; ++q) {
         if (q >= kind) break;
         const int b = a[q];
         JAtom *j = ys + sl3[q];

@@ -179,7 +179,7 @@ struct HostSys {
 
 struct Engine {
     int nsys = 0, natom = 0, nslot = 0, nclu = 0, ncell = 0;
-    int n2 = 0, n3 = 0;  // SHAKE groups
+    int n2 = 0, n3 = 0, n4 = 0;  // SHAKE groups of 2, 3 and 4 atoms
     int prec_built = -1;
     bool have_list = false;
     double shake_tol = 1e-6;
@@ -255,8 +255,9 @@ struct Engine {
     DBuf<double> d_gslot, d_ep64;
     DBuf<long long> d_epfix;
     DBuf<int> d_s2sys, d_s3sys;
-    DBuf<int4> d_s2a, d_s3a;
-    DBuf<double> d_s2d, d_s3d;
+    DBuf<int4> d_s2a, d_s3a, d_s4a;
+    DBuf<double> d_s2d, d_s3d, d_s4d;
+    DBuf<int> d_s4sys;
     DBuf<unsigned long long> d_u64;
     // bonded terms (jp_bonded.hpp): bonds, angles, torsions (+ impropers),
     // 1-4 pairs, per-atom gather lists
@@ -296,7 +297,7 @@ struct Engine {
     int put_target(double t);
     CanoArgs cano_args();
     int cano_step();
-    bool cano_retry() const { return cano && nsys == 1 && (n2 + n3) > 0; }
+    bool cano_retry() const { return cano && nsys == 1 && (n2 + n3 + n4) > 0; }
     int save_state(double *x, double *v, int *e);
     int load_state(const double *x, const double *v, const int *e);
     int shake_failed(bool &failed);
@@ -400,6 +401,7 @@ struct Engine {
         d_cellext.release(); d_sysext.release(); d_lcnt.release(); d_crec.release();
         d_ent.release(); d_mb.release(); d_gslot.release(); d_ep64.release(); d_epfix.release();
         d_s2sys.release(); d_s3sys.release(); d_s2a.release(); d_s3a.release(); d_s2d.release(); d_s3d.release();
+        d_s4sys.release(); d_s4a.release(); d_s4d.release();
         d_u64.release();
         d_bat.release(); d_bptr.release(); d_bref.release(); d_bnd.release(); d_ang.release();
         d_bndp.release(); d_angp.release(); d_eterm.release(); d_unit.release();
@@ -552,9 +554,9 @@ int Engine::init(const jp_sys_desc *descs, int ns) {
     std::vector<int2> eidx;
     std::vector<double2> lj64;
     std::vector<float2> lj32;
-    std::vector<int> s2sys, s3sys;
-    std::vector<int4> s2a, s3a;
-    std::vector<double> s2d, s3d;
+    std::vector<int> s2sys, s3sys, s4sys;
+    std::vector<int4> s2a, s3a, s4a;
+    std::vector<double> s2d, s3d, s4d;
     std::vector<int4> bnd, ang, tor, p14;
     std::vector<double2> bndp, angp, p14p;
     std::vector<double4> torp;
@@ -731,8 +733,12 @@ int Engine::init(const jp_sys_desc *descs, int ns) {
             } else if (na == 3) {
                 s3a.push_back(q); s3sys.push_back(s);
                 for (int k = 0; k < 6; ++k) s3d.push_back(d.shake_dist2[6 * g + k]);
+            } else if (na == 4) {   // shake4_PB (shake.f90:333-562)
+                q.w = aa[3] - 1 + atom0;
+                s4a.push_back(q); s4sys.push_back(s);
+                for (int k = 0; k < 6; ++k) s4d.push_back(d.shake_dist2[6 * g + k]);
             } else {
-                return JP_EINVAL;  // 4-atom groups (shake4) are not supported
+                return JP_EINVAL;
             }
         }
         // bonded terms (dnergv.f90:62-193); class default max(n_matrix) (incel.f90:952-969)
@@ -843,11 +849,15 @@ int Engine::init(const jp_sys_desc *descs, int ns) {
     {
         std::vector<int> owner(natom, -1);   // unit code per group atom
         std::vector<int4> gu;
-        for (int g = 0; g < (int)s2a.size(); ++g) gu.push_back(make_int4(s2a[g].x, s2a[g].y, -1, 2 | (g << 2)));
-        for (int g = 0; g < (int)s3a.size(); ++g) gu.push_back(make_int4(s3a[g].x, s3a[g].y, s3a[g].z, 3 | (g << 2)));
+        for (int g = 0; g < (int)s2a.size(); ++g) gu.push_back(make_int4(s2a[g].x, s2a[g].y, -1, 2 | (g << 3)));
+        for (int g = 0; g < (int)s3a.size(); ++g) gu.push_back(make_int4(s3a[g].x, s3a[g].y, s3a[g].z, 3 | (g << 3)));
+        for (int g = 0; g < (int)s4a.size(); ++g) gu.push_back(make_int4(s4a[g].x, s4a[g].y, s4a[g].z, 4 | (g << 3)));
+        auto unit_atom = [&](const int4 &q, int k) {
+            return k == 0 ? q.x : k == 1 ? q.y : k == 2 ? q.z : s4a[q.w >> 3].w;
+        };
         for (int q = 0; q < (int)gu.size(); ++q) {
-            const int m[3] = {gu[q].x, gu[q].y, gu[q].z};
-            for (int k = 0; k < (gu[q].w & 3); ++k) {
+            const int m[4] = {gu[q].x, gu[q].y, gu[q].z, (gu[q].w & 7) == 4 ? unit_atom(gu[q], 3) : -1};
+            for (int k = 0; k < (gu[q].w & 7); ++k) {
                 if (owner[m[k]] >= 0) return JP_EINVAL;   // an atom in two SHAKE groups (shake.f90:71-73)
                 owner[m[k]] = q;
             }
@@ -857,13 +867,15 @@ int Engine::init(const jp_sys_desc *descs, int ns) {
                 units.push_back(make_int4(a, -1, -1, 1));
             } else {
                 const int4 q = gu[owner[a]];
-                if (std::min(q.x, std::min(q.y, (q.w & 3) == 3 ? q.z : q.y)) == a) units.push_back(q);
+                int first = q.x;
+                for (int k = 1; k < (q.w & 7); ++k) first = std::min(first, unit_atom(q, k));
+                if (first == a) units.push_back(q);
             }
         }
         nunit = (int)units.size();
         units_h = units;
     }
-    n2 = (int)s2a.size(); n3 = (int)s3a.size();
+    n2 = (int)s2a.size(); n3 = (int)s3a.size(); n4 = (int)s4a.size();
     mass_h = mass;
     set_factors(-1);
     // upload
@@ -887,6 +899,7 @@ int Engine::init(const jp_sys_desc *descs, int ns) {
     AL(d_gslot, 3 * (size_t)nslot);
     AL(d_ep64, 6 * (size_t)nclu); AL(d_epfix, 6 * (size_t)nclu);
     AL(d_s2sys, n2); AL(d_s3sys, n3); AL(d_s2a, n2); AL(d_s3a, n3); AL(d_s2d, 6 * (size_t)n2); AL(d_s3d, 6 * (size_t)n3);
+    AL(d_s4sys, n4); AL(d_s4a, n4); AL(d_s4d, 6 * (size_t)n4);
     AL(d_u64, 3);
     AL(d_bat, bat.size()); AL(d_bptr, bptr.size()); AL(d_bref, bref.size());
     AL(d_bnd, bnd.size()); AL(d_bndp, bndp.size()); AL(d_ang, ang.size()); AL(d_angp, angp.size());
@@ -934,6 +947,9 @@ int Engine::init(const jp_sys_desc *descs, int ns) {
     r |= H2D(d_s3a.p, s3a.data(), 16 * s3a.size());
     r |= H2D(d_s2d.p, s2d.data(), 8 * s2d.size());
     r |= H2D(d_s3d.p, s3d.data(), 8 * s3d.size());
+    r |= H2D(d_s4sys.p, s4sys.data(), 4 * s4sys.size());
+    r |= H2D(d_s4a.p, s4a.data(), 16 * s4a.size());
+    r |= H2D(d_s4d.p, s4d.data(), 8 * s4d.size());
     r |= H2D(d_bat.p, bat.data(), 4 * bat.size());
     r |= H2D(d_bptr.p, bptr.data(), 4 * bptr.size());
     r |= H2D(d_bref.p, bref.data(), 4 * bref.size());
@@ -1798,8 +1814,26 @@ int Engine::shake(int &ier_out) {
                                d_x.p, d_xold.p);
     if (n3) hipLaunchKernelGGL(k_shake3, grid1(n3, 128), 128, 0, st, n3, d_s3sys.p, d_sp.p, d_s3a.p, d_s3d.p, d_imass.p,
                                d_x.p, d_xold.p, shake_tol, shake_maxit, d_err.p);
+    if (n4) hipLaunchKernelGGL(k_shake4, grid1(n4, 128), 128, 0, st, n4, d_s4sys.p, d_sp.p, d_s4a.p, d_s4d.p,
+                               d_imass.p, d_x.p, d_xold.p, shake_tol, shake_maxit, d_err.p);
     CK(hipGetLastError());
     return 0;
+}
+
+// k_step over every unit: with 4-atom SHAKE groups in the run the variant
+// that holds four atoms per unit (shake4_PB), else three.
+template <bool KICK, bool VUPD, int NQ>
+static void launch_step_nq(Engine &g, double *vb) {
+    hipLaunchKernelGGL((k_step<KICK, VUPD, NQ>), grid1(g.nunit, 128), 128, 0, g.st, g.nunit, g.d_unit.p,
+                       g.d_atom_sys.p, g.d_sp.p, g.d_inum.p, g.d_img.p, g.d_gslot.p, g.d_mass.p, g.d_imass.p, g.d_s2d.p,
+                       g.d_s3d.p, g.d_s4a.p, g.d_s4d.p, g.shake_tol, g.shake_maxit, (g.n2 + g.n3 + g.n4) > 0 ? 1 : 0,
+                       g.d_x.p, g.d_v.p, vb, g.d_ys.p, g.d_err.p);
+}
+
+template <bool KICK, bool VUPD>
+static void launch_step(Engine &g, double *vb) {
+    if (g.n4) launch_step_nq<KICK, VUPD, 4>(g, vb);
+    else launch_step_nq<KICK, VUPD, 3>(g, vb);
 }
 
 int Engine::integrate(int prec, bool mon) {
@@ -1811,13 +1845,9 @@ int Engine::integrate(int prec, bool mon) {
                            d_mass.p, d_imass.p, 0, d_v.p, vb);
         hipLaunchKernelGGL(k_remotv, (unsigned)nsys, JP_STOP_NT, 0, st, d_stop.p, d_mass.p, d_x.p, d_v.p,
                            (const int *)nullptr, 1, d_err.p);
-        hipLaunchKernelGGL((k_step<false, true>), grid1(nunit, 128), 128, 0, st, nunit, d_unit.p, d_atom_sys.p,
-                           d_sp.p, d_inum.p, d_img.p, d_gslot.p, d_mass.p, d_imass.p, d_s2d.p, d_s3d.p, shake_tol,
-                           shake_maxit, (n2 + n3) > 0 ? 1 : 0, d_x.p, d_v.p, vb, d_ys.p, d_err.p);
+        launch_step<false, true>(*this, vb);
     } else {
-        hipLaunchKernelGGL((k_step<true, true>), grid1(nunit, 128), 128, 0, st, nunit, d_unit.p, d_atom_sys.p,
-                           d_sp.p, d_inum.p, d_img.p, d_gslot.p, d_mass.p, d_imass.p, d_s2d.p, d_s3d.p, shake_tol,
-                           shake_maxit, (n2 + n3) > 0 ? 1 : 0, d_x.p, d_v.p, vb, d_ys.p, d_err.p);
+        launch_step<true, true>(*this, vb);
     }
     CK(hipGetLastError());
     return 0;
@@ -1905,9 +1935,7 @@ int Engine::start_step(int prec) {
     if (stop_mode)
         hipLaunchKernelGGL(k_remotv, (unsigned)nsys, JP_STOP_NT, 0, st, d_stop.p, d_mass.p, d_x.p, d_v.p,
                            (const int *)nullptr, 1, d_err.p);
-    hipLaunchKernelGGL((k_step<false, false>), grid1(nunit, 128), 128, 0, st, nunit, d_unit.p, d_atom_sys.p, d_sp.p,
-                       d_inum.p, d_img.p, d_gslot.p, d_mass.p, d_imass.p, d_s2d.p, d_s3d.p, shake_tol, shake_maxit,
-                       (n2 + n3) > 0 ? 1 : 0, d_x.p, d_v.p, (double *)nullptr, d_ys.p, d_err.p);
+    launch_step<false, false>(*this, (double *)nullptr);
     CK(hipGetLastError());
     int err = 0;
     CK(hipMemcpyAsync(&err, d_err.p, 4, hipMemcpyDeviceToHost, st));
@@ -1971,7 +1999,7 @@ int Engine::set_canonical(int on, double temps, int icslop, double fcstol, const
     nab = (int)ablk.size();
     nub = (int)ublk.size();
     for (const int4 &U : units_h) {
-        const int kind = U.w & 3;
+        const int kind = U.w & 7;
         if (kind < 2) continue;
         dof[gid[U.x]] -= (double)(kind * (kind - 1) / 2);
         shkon[atom_sys_h[U.x]] = 1;
@@ -2072,23 +2100,32 @@ CanoArgs Engine::cano_args() {
 // One METHOD= CANO step after the forces (jp_cano.hpp): 1 + icslop launches
 // (2 without SHAKE groups); the rounds of a system that has ended are skipped
 // on the device, and there is none after the last.
+// One launch of a SHAKE/temperature round (phase PH of k_cano_shake), with
+// four atoms per unit when the run has 4-atom SHAKE groups.
+template <int PH>
+static void launch_cano_round(Engine &g, const CanoArgs &A, int round) {
+    if (g.n4)
+        hipLaunchKernelGGL((k_cano_shake<PH, 4>), (unsigned)g.nub, 128, 0, g.st, A, g.d_sp.p, g.d_unit.p, g.d_s2d.p,
+                           g.d_s3d.p, g.d_s4a.p, g.d_s4d.p, g.shake_tol, g.shake_maxit, round, g.d_ys.p, g.d_err.p);
+    else
+        hipLaunchKernelGGL((k_cano_shake<PH, 3>), (unsigned)g.nub, 128, 0, g.st, A, g.d_sp.p, g.d_unit.p, g.d_s2d.p,
+                           g.d_s3d.p, g.d_s4a.p, g.d_s4d.p, g.shake_tol, g.shake_maxit, round, g.d_ys.p, g.d_err.p);
+}
+
 int Engine::cano_step() {
     const CanoArgs A = cano_args();
     const unsigned ga = (unsigned)nab;
     hipLaunchKernelGGL(k_cano_begin, ga, 256, 0, st, A, d_sp.p);
-    if (n2 + n3 > 0) {
+    if (n2 + n3 + n4 > 0) {
         for (int round = 1; round <= cano_icslop; ++round) {
             if (!stop_mode) {
-                hipLaunchKernelGGL(k_cano_shake<0>, (unsigned)nub, 128, 0, st, A, d_sp.p, d_unit.p, d_s2d.p, d_s3d.p,
-                                   shake_tol, shake_maxit, round, d_ys.p, d_err.p);
+                launch_cano_round<0>(*this, A, round);
                 continue;
             }
-            hipLaunchKernelGGL(k_cano_shake<1>, (unsigned)nub, 128, 0, st, A, d_sp.p, d_unit.p, d_s2d.p, d_s3d.p,
-                               shake_tol, shake_maxit, round, d_ys.p, d_err.p);
+            launch_cano_round<1>(*this, A, round);
             hipLaunchKernelGGL(k_remotv, (unsigned)nsys, JP_STOP_NT, 0, st, d_stop.p, d_mass.p, d_xold.p, d_v.p,
                                (const int *)d_cdone.p, round, d_err.p);
-            hipLaunchKernelGGL(k_cano_shake<2>, (unsigned)nub, 128, 0, st, A, d_sp.p, d_unit.p, d_s2d.p, d_s3d.p,
-                               shake_tol, shake_maxit, round, d_ys.p, d_err.p);
+            launch_cano_round<2>(*this, A, round);
         }
     } else if (!stop_mode) {
         hipLaunchKernelGGL(k_cano_drift, ga, 256, 0, st, A, d_sp.p, d_ys.p, 0);

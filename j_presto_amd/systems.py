@@ -425,6 +425,52 @@ def water_box(nwat: int, L: float, seed: int, n_ion_pairs: int = 0, temp: float 
     return s
 
 
+R_CH = 1.09
+MASS_C = 12.011
+
+
+def methyl_template(ncopy: int, type_c: int, type_h: int) -> MolTemplate:
+    """A rigid CH3 (C + 3 H, charges -0.3 / +0.1): one 4-atom SHAKE group
+    (md_run's shake4: pairs 1-2, 2-3, 1-3, 1-4, 3-4, 2-4) with tetrahedral
+    H-C-H angles."""
+    r_hh = R_CH * math.sqrt(8.0 / 3.0)
+    return MolTemplate(
+        name="MET", ncopy=ncopy, aname=["C", "H1", "H2", "H3"], atname=["CT", "HC", "HC", "HC"],
+        itype=np.array([type_c, type_h, type_h, type_h]), rname=["MET"] * 4, resnum=np.array([1, 1, 1, 1]),
+        mass=np.array([MASS_C, MASS_H, MASS_H, MASS_H]), charge=np.array([-0.3, 0.1, 0.1, 0.1]),
+        n1x=np.array([[3, 0, 0], [0, 2, 0], [0, 1, 0], [0, 0, 0]]), partners=[[1, 2, 3], [1, 2], [1], []],
+        bonds=np.array([[1, 2], [1, 3], [1, 4]]), bond_fc=np.full(3, 340.0), bond_r0=np.full(3, R_CH),
+        angles=np.array([[2, 1, 3], [2, 1, 4], [3, 1, 4]]), angle_fc=np.full(3, 35.0),
+        angle_deg=np.full(3, math.degrees(math.acos(-1.0 / 3.0))),
+        shake=[((1, 2, 3, 4), (R_CH, r_hh, R_CH, R_CH, r_hh, r_hh))])
+
+
+def methyl_water(nmet: int, nwat: int, L: float, seed: int, temp: float = 300.0,
+                 name: str = "methyl_water") -> System:
+    """nmet rigid CH3 groups (4-atom SHAKE groups) and nwat waters in a
+    periodic box: a water_box of nmet + nwat waters whose first nmet waters
+    become CH3 groups on their oxygen sites, randomly oriented."""
+    w = water_box(nmet + nwat, L, seed, temp=temp)
+    rng = np.random.Generator(np.random.PCG64(seed + 15485863))
+    tet = np.array([[1.0, 1.0, 1.0], [1.0, -1.0, -1.0], [-1.0, 1.0, -1.0]]) / math.sqrt(3.0)
+    xm = []
+    for i in range(nmet):
+        q = rng.normal(size=4)
+        q /= np.linalg.norm(q)
+        a, b, c, d = q
+        rot = np.array([[a * a + b * b - c * c - d * d, 2 * (b * c - a * d), 2 * (b * d + a * c)],
+                        [2 * (b * c + a * d), a * a - b * b + c * c - d * d, 2 * (c * d - a * b)],
+                        [2 * (b * d - a * c), 2 * (c * d + a * b), a * a - b * b - c * c + d * d]])
+        o = w.x[3 * i]
+        xm.append(np.vstack([o, o + R_CH * tet @ rot.T]))
+    types = water_types() + [AtomType("CT", 1.908, 0.1094), AtomType("HC", 1.487, 0.0157)]
+    x = np.mod(np.concatenate(xm + [w.x[3 * nmet:]]), L)
+    s = System(name=name, mols=[methyl_template(nmet, 3, 4), water_template(nwat)], types=types,
+               box=w.box.copy(), x=x, v=np.zeros_like(x))
+    s.v = thermal_velocities(rng, s, temp)
+    return s
+
+
 def water_droplet(R: float, seed: int, density_L: float = 31.04, temp: float = 300.0,
                   name: str = "droplet") -> System:
     """A non-periodic water droplet for the cell multipole (CMM) path: the

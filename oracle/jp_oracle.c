@@ -1063,7 +1063,102 @@ static int shake3(const jo_ctx *c, int g, double *r1, const double *r2) {
     return 1;
 }
 
-/* shake_PB (shake.f90:2-59): 2-atom groups, then 3-atom groups. */
+/* simulu.f90:2-67 (iflag 1): LU decomposition with partial pivoting by row
+ * indices (ludeco), then the two triangular solves, on the leading n x n of
+ * A (row-major here, md_run's A(i, j) = a[i][j]); A keeps the factors with
+ * the reciprocal pivots on the diagonal.  Returns -1 for a pivot below epsl. */
+static int simulu6(int n, double a[6][6], const double *b, double *x, double epsl) {
+    int ip[6];
+    for (int k = 0; k < n; ++k) ip[k] = k;
+    for (int k = 0; k < n; ++k) {
+        int l = k;
+        double al = fabs(a[ip[l]][k]);
+        for (int i = k + 1; i < n; ++i)
+            if (fabs(a[ip[i]][k]) > al) { l = i; al = fabs(a[ip[l]][k]); }
+        if (l != k) { const int t = ip[k]; ip[k] = ip[l]; ip[l] = t; }
+        if (fabs(a[ip[k]][k]) < epsl) return -1;
+        a[ip[k]][k] = 1.0 / a[ip[k]][k];
+        for (int i = k + 1; i < n; ++i) {
+            a[ip[i]][k] = a[ip[i]][k] * a[ip[k]][k];
+            for (int j = k + 1; j < n; ++j) a[ip[i]][j] = a[ip[i]][j] - a[ip[i]][k] * a[ip[k]][j];
+        }
+    }
+    for (int i = 0; i < n; ++i) {
+        double t = b[ip[i]];
+        for (int j = 0; j < i; ++j) t = t - a[ip[i]][j] * x[j];
+        x[i] = t;
+    }
+    for (int i = n - 1; i >= 0; --i) {
+        double t = x[i];
+        for (int j = i + 1; j < n; ++j) t = t - a[ip[i]][j] * x[j];
+        x[i] = t * a[ip[i]][i];
+    }
+    return 0;
+}
+
+/* shake4_PB, shake.f90:333-562: six constraints of four atoms, pairs
+ * (1,2) (2,3) (3,1) (4,1) (3,4) (2,4) (parlis), Newton iterations whose
+ * linear systems simulu solves (pivots below eps^3 = 1e-30: singular).
+ * Returns 0, 1 (not converged: r1 unchanged), -2 (singular). */
+static int shake4(const jo_ctx *c, int g, double *r1, const double *r2) {
+    static const int pl[6][2] = {{0, 1}, {1, 2}, {2, 0}, {3, 0}, {2, 3}, {1, 3}};
+    const int *a = c->shk_a + 4 * g;
+    double im[4];
+    for (int q = 0; q < 4; ++q) im[q] = 1.0 / c->mass[a[q]];
+    double co[6][6] = {
+        {im[0] + im[1], -im[1], -im[0], -im[0], 0.0, -im[1]},
+        {-im[1], im[1] + im[2], -im[2], 0.0, -im[2], im[1]},
+        {-im[0], -im[2], im[2] + im[0], im[0], im[2], 0.0},
+        {-im[0], 0.0, im[0], im[3] + im[0], -im[3], -im[3]},
+        {0.0, -im[2], im[2], -im[3], im[2] + im[3], im[3]},
+        {-im[1], im[1], 0.0, -im[3], im[3], im[1] + im[3]}};
+    const double *T = c->shk_d2 + 6 * g;
+    double TT[6], G[6] = {0, 0, 0, 0, 0, 0}, d1[6][3], d2[6][3], w[6][3];
+    for (int q = 0; q < 6; ++q) {
+        TT[q] = c->shk_tol * T[q];
+        for (int k = 0; k < 3; ++k) {
+            d1[q][k] = r1[3 * a[pl[q][0]] + k] - r1[3 * a[pl[q][1]] + k];
+            d2[q][k] = r2[3 * a[pl[q][0]] + k] - r2[3 * a[pl[q][1]] + k];
+        }
+        mimg(c, d1[q]);
+        mimg(c, d2[q]);
+        for (int k = 0; k < 3; ++k) w[q][k] = d1[q][k];
+    }
+    for (int it = 0; it < c->shk_maxit; ++it) {
+        double B[6];
+        int ok = 1;
+        for (int q = 0; q < 6; ++q) {
+            B[q] = T[q] - (w[q][0] * w[q][0] + w[q][1] * w[q][1] + w[q][2] * w[q][2]);
+            if (fabs(B[q]) >= TT[q]) ok = 0;
+        }
+        if (ok) {
+            for (int k = 0; k < 3; ++k) {
+                r1[3 * a[0] + k] = r1[3 * a[0] + k] + im[0] * (G[0] * d2[0][k] - G[2] * d2[2][k] - G[3] * d2[3][k]);
+                r1[3 * a[1] + k] = r1[3 * a[1] + k] + im[1] * (-G[0] * d2[0][k] + G[1] * d2[1][k] + G[5] * d2[5][k]);
+                r1[3 * a[2] + k] = r1[3 * a[2] + k] + im[2] * (-G[1] * d2[1][k] + G[2] * d2[2][k] + G[4] * d2[4][k]);
+                r1[3 * a[3] + k] = r1[3 * a[3] + k] + im[3] * (G[3] * d2[3][k] - G[4] * d2[4][k] - G[5] * d2[5][k]);
+            }
+            return 0;
+        }
+        double gf[6][6];
+        for (int ig = 0; ig < 6; ++ig)
+            for (int q = 0; q < 6; ++q)
+                gf[q][ig] = 2.0 * co[q][ig] * (w[q][0] * d2[ig][0] + w[q][1] * d2[ig][1] + w[q][2] * d2[ig][2]);
+        for (int ig = 0; ig < 6; ++ig)
+            for (int q = 0; q < 6; ++q) B[q] = B[q] + gf[q][ig] * G[ig];
+        if (simulu6(6, gf, B, G, 1.0e-30)) return -2;   /* PHYCNS.f90:34 eps = 1e-10, cubed */
+        for (int q = 0; q < 6; ++q)
+            for (int k = 0; k < 3; ++k) w[q][k] = d1[q][k];
+        for (int ig = 0; ig < 6; ++ig)
+            for (int q = 0; q < 6; ++q)
+                for (int k = 0; k < 3; ++k) w[q][k] = w[q][k] + co[q][ig] * d2[ig][k] * G[ig];
+    }
+    return 1;
+}
+
+/* shake_PB (shake.f90:2-59): 2-atom groups, then 3-atom groups, then 4-atom
+ * groups; a singular 4-atom system makes the call's ier -2 (shake4_PB ends
+ * its loop first: the other groups are still applied). */
 int jo_shake(const jo_ctx *c, double *r1, const double *r2) {
     int ier = 0;
     for (int g = 0; g < c->nshk; ++g)
@@ -1074,8 +1169,18 @@ int jo_shake(const jo_ctx *c, double *r1, const double *r2) {
             if (e == -2) return -2;
             if (e == 1) ier = 1;
         }
+    if (ier) return ier;   /* shake.f90:38-44: the 4-atom groups only after the 3-atom ones succeeded */
+    int sing = 0, nc = 0;
     for (int g = 0; g < c->nshk; ++g)
-        if (c->shk_n[g] > 3) return JP_EINVAL;
+        if (c->shk_n[g] == 4) {
+            const int e = shake4(c, g, r1, r2);
+            if (e == -2) sing = 1;
+            if (e == 1) nc = 1;
+        } else if (c->shk_n[g] > 4) {
+            return JP_EINVAL;
+        }
+    if (sing) return -2;
+    if (nc) return 1;
     return ier;
 }
 

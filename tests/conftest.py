@@ -12,7 +12,7 @@ for p in (ROOT, os.path.join(ROOT, "oracle"), os.path.join(ROOT, "tests", "golde
 
 GOLDEN = os.path.join(ROOT, "tests", "golden")
 GOLDEN_NAMES = ["water_1500", "water_6k", "chain_3k", "chain_lmd", "ions_3k", "water_54", "chain_bonded",
-                "chain_lmd_bonded", "chain_tors", "chain_lmd_tors", "water_42"]
+                "chain_lmd_bonded", "chain_tors", "chain_lmd_tors", "water_42", "methyl_water"]
 
 
 def pytest_configure(config):

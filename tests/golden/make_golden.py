@@ -89,6 +89,8 @@ RECIPES = {
     "ions_3k": ("water_box", dict(nwat=900, L=30.0, seed=13, n_ion_pairs=10), dict(traj=0)),
     # box edge a multiple of rlim = 13.5 A (md_run's phantom-cell case)
     "water_54": ("water_box", dict(nwat=5200, L=54.0, seed=14), dict(traj=0)),
+    # rigid CH3 groups in water: 4-atom SHAKE groups (shake4_PB)
+    "methyl_water": ("methyl_water", dict(nmet=40, nwat=460, L=30.0, seed=31), dict(traj=20, cano=("NO", 12))),
     # 42 A edge: md_run's periodic cell table (8 cells of 6.75 A for 42 A) never tests
     # pairs between some cells facing each other across the boundary, inside the cutoff
     "water_42": ("water_box", dict(nwat=2469, L=42.0, seed=19), dict(traj=10)),

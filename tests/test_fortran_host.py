@@ -199,7 +199,7 @@ def _tpl_case(d, s, md, inp=None):
 
 
 @pytest.mark.parametrize("name", ["water_1500", "chain_3k", "chain_lmd", "chain_lmd_bonded", "chain_tors",
-                                  "chain_lmd_tors", "ions_3k"])
+                                  "chain_lmd_tors", "ions_3k", "methyl_water"])
 def test_tplsys_equals_system_file(golden, tmp_path, name):
     """jp_host --tplsys reads md.inp's TPL, SHK and BINA files and the
     restart (jp_input.f90) and writes the system file: every record equals,

@@ -14,7 +14,7 @@ from oracle import Oracle
 
 pytestmark = pytest.mark.gpu
 
-CANO_NAMES = ["water_1500", "chain_3k", "chain_lmd"]
+CANO_NAMES = ["water_1500", "chain_3k", "chain_lmd", "methyl_water"]
 
 
 @pytest.mark.parametrize("name", CANO_NAMES)

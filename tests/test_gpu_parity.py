@@ -97,8 +97,11 @@ def test_bonded_per_class_vs_oracle(golden, name):
     md.close()
 
 
-def test_shake_bit_exact_vs_oracle():
-    s = systems.chain_in_water(40, 25.0, 5)
+@pytest.mark.parametrize("kind", ["chain", "methyl"])
+def test_shake_bit_exact_vs_oracle(kind):
+    """jp_shake (2- and 3-atom groups of a chain in water; 4-atom CH3
+    groups, shake4) equals the oracle's SHAKE bit for bit."""
+    s = systems.chain_in_water(40, 25.0, 5) if kind == "chain" else systems.methyl_water(20, 300, 24.0, 6)
     rng = np.random.default_rng(1)
     x0 = s.x.copy()
     x1 = np.ascontiguousarray(x0 + rng.normal(scale=0.02, size=x0.shape))
@@ -112,7 +115,7 @@ def test_shake_bit_exact_vs_oracle():
 
 
 @pytest.mark.parametrize("name", ["water_1500", "water_6k", "chain_3k", "chain_lmd", "chain_bonded",
-                                  "chain_lmd_bonded", "chain_tors", "chain_lmd_tors", "water_42"])
+                                  "chain_lmd_bonded", "chain_tors", "chain_lmd_tors", "water_42", "methyl_water"])
 def test_nve_trajectory_vs_reference(golden, name):
     s, f = golden(name)
     print(f"case {name}", flush=True)

@@ -94,7 +94,7 @@ def test_potential_energy_series(golden):
     assert [float(f"{e:.6e}") for e in ep] == list(ref)
 
 
-@pytest.mark.parametrize("name", ["water_1500", "chain_3k", "chain_lmd"])
+@pytest.mark.parametrize("name", ["water_1500", "chain_3k", "chain_lmd", "methyl_water"])
 def test_cano_trajectory_bit_exact(golden, name):
     """md_run METHOD= CANO (verlethe.f90) from a restart, TEMPCO= NO (one
     temperature) or YES (one per molecule kind and cluster): velocity scaling
